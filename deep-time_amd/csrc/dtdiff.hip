@@ -1,0 +1,914 @@
+// dtdiff: the C ABI of include/dtdiff.h over the HIP kernels of kernels.hpp (gfx950).
+//
+// Reference files (relative to the deep-time repository root):
+//   D     crates/deeptime-core/src/surface/diffusion.rs
+//   MG    crates/deeptime-core/src/surface/diffusion/multigrid.rs
+//   MGGPU crates/deeptime-core/src/surface/diffusion/multigrid/gpu.rs
+//
+// Design: the whole solve stays on the device.  The operator, the cycle's levels, the first
+// guess, the f64 conjugate-gradient vectors and their scalars (CgState) are resident in HBM, so
+// an iteration needs no host transfer (the reference's device cycle moved r and z across the bus
+// every iteration, MGGPU:669-700).  One iteration (direction pass, p.Ap, update pass, r.r, the
+// V-cycle, r.z) is captured once per grid shape as a hipGraph, in two copies whose direction
+// buffers alternate; the host launches them in turn and reads the device's stop flag one
+// iteration behind, so the loop runs ahead of the flag by at most one (skipped) iteration.
+#include "../../include/dtdiff.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace dtd {
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw ::dtd::Error{DTD_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+#define LAUNCH_CHECK() HIPCHK(hipGetLastError())
+
+static inline int nblocks(long n, int bs = kBlock) { return (int)((n + bs - 1) / bs); }
+
+template <typename T>
+static T *dalloc(size_t n) {
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, (n ? n : 1) * sizeof(T)));
+    return (T *)p;
+}
+
+// The level shapes `Level::coarsen` produces (MG:117-123; `shapes`, MGGPU:336-347): halve,
+// rounding up, until a level holds at most 64 cells or stops shrinking; a coarse level wraps
+// when the finer one does and it is more than one cell wide.
+struct Shape {
+    int w, h, wrap;
+};
+static std::vector<Shape> shapes(int w, int h, int wrap) {
+    std::vector<Shape> out{{w, h, wrap}};
+    for (;;) {
+        Shape s = out.back();
+        int wc = (s.w + 1) / 2, hc = (s.h + 1) / 2;
+        if ((long)s.w * s.h <= 64 || (wc == s.w && hc == s.h)) return out;
+        out.push_back({wc, hc, s.wrap && wc > 1});
+    }
+}
+
+// Device memory of one grid shape (the reference's Workspace, D:213-260, and the device cycle's
+// levels, MGGPU:350-558, in one place), kept while the shape stays.
+struct Workspace {
+    int w = 0, h = 0, wrap = 0, prec = -1;
+    long n = 0;
+    double *b = nullptr, *x = nullptr, *east = nullptr, *south = nullptr, *r = nullptr;
+    double *p0 = nullptr, *p1 = nullptr, *ap = nullptr, *kd = nullptr;
+    double *parts = nullptr, *pair = nullptr;
+    CgState *st = nullptr;
+    std::vector<Lvl<float>> lf;
+    std::vector<Lvl<double>> ld;
+    std::vector<void *> owned;  // every allocation of the levels
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int zfinal = 1;  // which z buffer of level 0 holds the cycle's result
+};
+
+}  // namespace dtd
+
+struct dtd_ctx {
+    dtd_opts opts;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    hipEvent_t flag_ev[2] = {};
+    dtd::CgState *host_st = nullptr;  // pinned
+    int *host_flag = nullptr;         // pinned, two slots
+    double *host_pair = nullptr;      // pinned, two doubles
+    dtd::Workspace ws;
+    std::string err;
+    size_t cycle_levels = 0;  // levels built by dtd_cycle_new
+    bool use_graph = true;
+};
+
+namespace dtd {
+
+static void free_ws(Workspace &ws) {
+    for (int k = 0; k < 2; k++)
+        if (ws.graph[k]) {
+            (void)hipGraphExecDestroy(ws.graph[k]);
+            ws.graph[k] = nullptr;
+        }
+    double *bufs[] = {ws.b, ws.x, ws.east, ws.south, ws.r, ws.p0, ws.p1, ws.ap, ws.kd,
+                      ws.parts, ws.pair};
+    for (double *p : bufs)
+        if (p) (void)hipFree(p);
+    if (ws.st) (void)hipFree(ws.st);
+    for (void *p : ws.owned) (void)hipFree(p);
+    ws = Workspace();
+}
+
+// Sizes the workspace for a w x h grid and the cycle's precision; the levels' shapes are
+// allocated here and filled by build_levels.
+static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
+    Workspace &ws = ctx->ws;
+    const int prec = ctx->opts.cycle_precision;
+    if (ws.w == w && ws.h == h && ws.wrap == wrap && ws.prec == prec) return;
+    free_ws(ws);
+    const long n = (long)w * h;
+    ws.w = w;
+    ws.h = h;
+    ws.wrap = wrap;
+    ws.prec = prec;
+    ws.n = n;
+    ws.b = dalloc<double>(n);
+    ws.x = dalloc<double>(n);
+    ws.east = dalloc<double>(n);
+    ws.south = dalloc<double>(n);
+    ws.r = dalloc<double>(n);
+    ws.p0 = dalloc<double>(n);
+    ws.p1 = dalloc<double>(n);
+    ws.ap = dalloc<double>(n);
+    ws.kd = dalloc<double>(n);
+    ws.parts = dalloc<double>(2 * (size_t)nblocks(n) + 2);
+    ws.pair = dalloc<double>(2);
+    ws.st = dalloc<CgState>(1);
+    auto sh = shapes(w, h, wrap);
+    auto own = [&](size_t cnt, size_t elt) {
+        void *p = nullptr;
+        HIPCHK(hipMalloc(&p, (cnt ? cnt : 1) * elt));
+        ws.owned.push_back(p);
+        return p;
+    };
+    for (size_t l = 0; l < sh.size(); l++) {
+        const long nl = (long)sh[l].w * sh[l].h;
+        if (prec == DTD_CYCLE_F64) {
+            Lvl<double> L{};
+            L.w = sh[l].w, L.h = sh[l].h, L.wrap = sh[l].wrap, L.n = (int)nl;
+            if (l == 0) {
+                L.east = ws.east;
+                L.south = ws.south;
+                L.r = ws.r;
+                L.mass = nullptr;
+            } else {
+                L.east = (double *)own(nl, 8);
+                L.south = (double *)own(nl, 8);
+                L.r = (double *)own(nl, 8);
+                L.mass = (double *)own(nl, 8);
+            }
+            L.diag = (double *)own(nl, 8);
+            L.wd = (double *)own(nl, 8);
+            L.z[0] = (double *)own(nl, 8);
+            L.z[1] = (double *)own(nl, 8);
+            ws.ld.push_back(L);
+        } else {
+            Lvl<float> L{};
+            L.w = sh[l].w, L.h = sh[l].h, L.wrap = sh[l].wrap, L.n = (int)nl;
+            L.east = (float *)own(nl, 4);
+            L.south = (float *)own(nl, 4);
+            L.diag = (float *)own(nl, 4);
+            L.wd = (float *)own(nl, 4);
+            L.r = (float *)own(nl, 4);
+            L.mass = l == 0 ? nullptr : (float *)own(nl, 4);
+            L.z[0] = (float *)own(nl, 4);
+            L.z[1] = (float *)own(nl, 4);
+            ws.lf.push_back(L);
+        }
+    }
+}
+
+static Grid grid_of(const Workspace &ws) { return Grid{ws.w, ws.h, ws.wrap, (int)ws.n}; }
+
+// f64 diagonal and omega / diagonal of the finest level of an f64 cycle.
+__global__ void k_finest64(Grid g, const double *__restrict__ east,
+                           const double *__restrict__ south, double *__restrict__ d,
+                           double *__restrict__ wd, double omega) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= g.n) return;
+    const int j = c / g.w, i = c - j * g.w;
+    const double dv = diag64(g, east, south, c, i, j);
+    d[c] = dv;
+    wd[c] = omega / dv;
+}
+
+// `Level::finest` (MG:96-114) from a given f64 operator: f32 copies and omega / diag.
+__global__ void k_finest32(int n, const double *__restrict__ east,
+                           const double *__restrict__ south, const double *__restrict__ diag,
+                           float *__restrict__ e32, float *__restrict__ s32,
+                           float *__restrict__ d32, float *__restrict__ wd32, float omega) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    e32[c] = (float)east[c];
+    s32[c] = (float)south[c];
+    const float d = (float)diag[c];
+    d32[c] = d;
+    wd32[c] = omega / d;
+}
+
+__global__ void k_copy2(long n, const double *__restrict__ src, double *__restrict__ a,
+                        double *__restrict__ b) {
+    const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const double v = src[c];
+    a[c] = v;
+    b[c] = v;
+}
+
+__global__ void k_wd64(int n, const double *__restrict__ d, double *__restrict__ wd,
+                       double omega) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c < n) wd[c] = omega / d[c];
+}
+
+__global__ void k_sumsq(int n, const double *__restrict__ v, double *__restrict__ parts) {
+    __shared__ double lds[kBlock / 64];
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    const double a = c < n ? v[c] * v[c] : 0.0;
+    const double s = block_sum<kBlock>(a, lds);
+    if (threadIdx.x == 0) parts[blockIdx.x] = s;
+}
+
+__global__ void k_sum1(const double *__restrict__ parts, int nparts, double *out) {
+    __shared__ double lds[kReduceBlock / 64];
+    const double v = reduce_parts(parts, nparts, 1, 0, lds);
+    if (threadIdx.x == 0) out[0] = v;
+}
+
+__global__ void k_init_state(CgState *st, double tol2, int max_iter) {
+    if (threadIdx.x != 0) return;
+    CgState s{};
+    s.tol2 = tol2;
+    s.max_iter = max_iter;
+    *st = s;
+}
+
+// Coarse levels from the finest (`Hierarchy::new`, MG:481-487; device `COARSEN`/`CDIAG`,
+// MGGPU:62-140).
+template <typename T>
+static void build_coarse(std::vector<Lvl<T>> &lv, T omega, T scale, hipStream_t s) {
+    for (size_t l = 1; l < lv.size(); l++) {
+        k_coarsen<T><<<nblocks(lv[l].n), kBlock, 0, s>>>(lv[l - 1], lv[l], scale);
+        LAUNCH_CHECK();
+        k_cdiag<T><<<nblocks(lv[l].n), kBlock, 0, s>>>(lv[l], omega);
+        LAUNCH_CHECK();
+    }
+}
+
+// `Operator::new` on the device from kd (already on the device) and the cycle's levels.
+static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt) {
+    Workspace &ws = ctx->ws;
+    const Grid g = grid_of(ws);
+    const double s2 = dx * dx;
+    hipStream_t s = ctx->stream;
+    if (ws.prec == DTD_CYCLE_F64) {
+        k_operator<<<nblocks(ws.n), kBlock, 0, s>>>(g, kd, dt, s2, ws.east, ws.south, nullptr,
+                                                    nullptr, nullptr, nullptr, 0.0f);
+        LAUNCH_CHECK();
+        k_finest64<<<nblocks(ws.n), kBlock, 0, s>>>(g, ws.east, ws.south, ws.ld[0].diag,
+                                                    ws.ld[0].wd, (double)ctx->opts.omega);
+        LAUNCH_CHECK();
+        build_coarse<double>(ws.ld, (double)ctx->opts.omega, (double)ctx->opts.scale, s);
+    } else {
+        Lvl<float> &L = ws.lf[0];
+        k_operator<<<nblocks(ws.n), kBlock, 0, s>>>(g, kd, dt, s2, ws.east, ws.south, L.east,
+                                                    L.south, L.diag, L.wd, ctx->opts.omega);
+        LAUNCH_CHECK();
+        build_coarse<float>(ws.lf, ctx->opts.omega, ctx->opts.scale, s);
+    }
+}
+
+// `vcycle` (MG:500-523) from level l, enqueued on s.  The first sweep writes z[0] and the
+// sweeps alternate after it; with 2 nu - 1 sweeps after the first on every level but the
+// coarsest (and COARSE_SWEEPS - 1 = 15 there), each level's result lands in z[1].  With dot,
+// the last sweep of level l leaves its per-block sums dot * z in parts (block count returned).
+template <typename T>
+static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, double *parts,
+                  const CgState *st, hipStream_t s) {
+    Lvl<T> &L = lv[l];
+    if (l + 1 == lv.size()) {
+        if (L.n > 64) throw Error{DTD_ESTATE, "the coarsest level exceeds one wave"};
+        k_coarsest<T><<<1, 64, 0, s>>>(L, L.z[1], 16, dot, parts, st);
+        LAUNCH_CHECK();
+        return 1;
+    }
+    const int nb = nblocks(L.n);
+    int cur = 0;
+    k_first<T><<<nb, kBlock, 0, s>>>(L, L.z[0], st);
+    LAUNCH_CHECK();
+    for (int k = 1; k < nu; k++) {
+        k_sweep<T><<<nb, kBlock, 0, s>>>(L, L.z[cur], L.z[cur ^ 1], nullptr, 0, nullptr, nullptr,
+                                         st);
+        LAUNCH_CHECK();
+        cur ^= 1;
+    }
+    Lvl<T> &C = lv[l + 1];
+    k_restrict<T><<<nblocks(C.n), kBlock, 0, s>>>(L, L.z[cur], C, st);
+    LAUNCH_CHECK();
+    vcycle<T>(lv, l + 1, nu, nullptr, parts, st, s);
+    {
+        const double *d = nu == 1 ? dot : nullptr;
+        k_sweep<T><<<nb, kBlock, 0, s>>>(L, L.z[cur], L.z[cur ^ 1], C.z[1], C.w, d, parts, st);
+        LAUNCH_CHECK();
+        cur ^= 1;
+    }
+    for (int k = 1; k < nu; k++) {
+        const double *d = k + 1 == nu ? dot : nullptr;
+        k_sweep<T><<<nb, kBlock, 0, s>>>(L, L.z[cur], L.z[cur ^ 1], nullptr, 0, d, parts, st);
+        LAUNCH_CHECK();
+        cur ^= 1;
+    }
+    if (cur != 1) throw Error{DTD_ESTATE, "the cycle's result is not in z[1]"};
+    return nb;
+}
+
+static int run_vcycle(dtd_ctx *ctx, const double *dot, const CgState *st) {
+    Workspace &ws = ctx->ws;
+    const int nu = (int)ctx->opts.nu;
+    if (ws.prec == DTD_CYCLE_F64)
+        return vcycle<double>(ws.ld, 0, nu, dot, ws.parts, st, ctx->stream);
+    return vcycle<float>(ws.lf, 0, nu, dot, ws.parts, st, ctx->stream);
+}
+
+// One conjugate-gradient iteration (MG:630-733): parity 0 reads the direction from p1 and writes
+// p0 (odd iterations), parity 1 the other way.
+static void enqueue_iteration(dtd_ctx *ctx, int parity) {
+    Workspace &ws = ctx->ws;
+    const Grid g = grid_of(ws);
+    hipStream_t s = ctx->stream;
+    const int nb = nblocks(ws.n);
+    double *pold = parity == 0 ? ws.p1 : ws.p0;
+    double *pnew = parity == 0 ? ws.p0 : ws.p1;
+    if (ws.prec == DTD_CYCLE_F64) {
+        k_direction<double><<<nb, kBlock, 0, s>>>(g, ws.east, ws.south, ws.ld[0].z[1], pold, pnew,
+                                                  ws.ap, ws.x, ws.parts, ws.st);
+    } else {
+        k_direction<float><<<nb, kBlock, 0, s>>>(g, ws.east, ws.south, ws.lf[0].z[1], pold, pnew,
+                                                 ws.ap, ws.x, ws.parts, ws.st);
+    }
+    LAUNCH_CHECK();
+    k_cg_step<kStepPAP><<<1, kReduceBlock, 0, s>>>(ws.parts, nb, ws.st);
+    LAUNCH_CHECK();
+    if (ws.prec == DTD_CYCLE_F64)
+        k_update<double><<<nb, kBlock, 0, s>>>((int)ws.n, ws.ap, ws.r, nullptr, ws.parts, ws.st);
+    else
+        k_update<float><<<nb, kBlock, 0, s>>>((int)ws.n, ws.ap, ws.r, ws.lf[0].r, ws.parts, ws.st);
+    LAUNCH_CHECK();
+    k_cg_step<kStepRR><<<1, kReduceBlock, 0, s>>>(ws.parts, nb, ws.st);
+    LAUNCH_CHECK();
+    const int nbz = run_vcycle(ctx, ws.r, ws.st);
+    k_cg_step<kStepRZ><<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.st);
+    LAUNCH_CHECK();
+}
+
+static void capture_graphs(dtd_ctx *ctx) {
+    Workspace &ws = ctx->ws;
+    for (int k = 0; k < 2; k++) {
+        if (ws.graph[k]) continue;
+        hipGraph_t gr = nullptr;
+        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue_iteration(ctx, k);
+        } catch (...) {
+            (void)hipStreamEndCapture(ctx->stream, &gr);
+            if (gr) (void)hipGraphDestroy(gr);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(ctx->stream, &gr));
+        hipError_t e = hipGraphInstantiate(&ws.graph[k], gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        HIPCHK(e);
+    }
+}
+
+static void lod(dtd_ctx *ctx, double *x, const double *kd, double dx, double dt, int wrap_x) {
+    Workspace &ws = ctx->ws;
+    const Grid g = grid_of(ws);
+    const double s2 = dx * dx;
+    k_lod_rows<<<nblocks(ws.h), kBlock, 0, ctx->stream>>>(g, wrap_x, kd, dt, s2, x, ws.p0, ws.p1);
+    LAUNCH_CHECK();
+    k_lod_cols<<<nblocks(ws.w), kBlock, 0, ctx->stream>>>(g, kd, dt, s2, x, ws.p0);
+    LAUNCH_CHECK();
+}
+
+static float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    return ms;
+}
+
+// The solve on the workspace: ws.b and ws.x hold h0, kd is on the device.  Leaves h1 in ws.x.
+// Returns the iterations.  `MG:562-763` after `lod` (D:101-116).
+static size_t solve_core(dtd_ctx *ctx, const double *kd, double dx, double dt, int wrap_x,
+                         double tol_m, bool setup, bool guess, dtd_stats *stats) {
+    Workspace &ws = ctx->ws;
+    hipStream_t s = ctx->stream;
+    const Grid g = grid_of(ws);
+    const int nb = nblocks(ws.n);
+    HIPCHK(hipEventRecord(ctx->ev[0], s));
+    if (setup) build_operator(ctx, kd, dx, dt);
+    HIPCHK(hipEventRecord(ctx->ev[1], s));
+    if (guess && ctx->opts.guess == DTD_GUESS_LOD) lod(ctx, ws.x, kd, dx, dt, wrap_x);
+    HIPCHK(hipEventRecord(ctx->ev[2], s));
+    const double tol2 = tol_m * tol_m * (double)ws.n;
+    k_init_state<<<1, 64, 0, s>>>(ws.st, tol2, (int)ctx->opts.max_iter);
+    LAUNCH_CHECK();
+    if (ws.prec == DTD_CYCLE_F64)
+        k_resid_init<double><<<nb, kBlock, 0, s>>>(g, ws.east, ws.south, ws.b, ws.x, ws.r,
+                                                   (double *)nullptr, ws.parts);
+    else
+        k_resid_init<float><<<nb, kBlock, 0, s>>>(g, ws.east, ws.south, ws.b, ws.x, ws.r,
+                                                  ws.lf[0].r, ws.parts);
+    LAUNCH_CHECK();
+    k_cg_step<kStepRR0><<<1, kReduceBlock, 0, s>>>(ws.parts, nb, ws.st);
+    LAUNCH_CHECK();
+    const int nbz = run_vcycle(ctx, ws.r, ws.st);
+    k_cg_step<kStepRZ0><<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.st);
+    LAUNCH_CHECK();
+    if (ctx->use_graph) capture_graphs(ctx);
+    // The loop: iteration k (1-based) runs with parity (k - 1) % 2; the stop flag of iteration
+    // k is read while iteration k + 1 runs.
+    const int max_iter = (int)ctx->opts.max_iter;
+    for (int k = 0; k < max_iter; k++) {
+        const int par = k & 1;
+        if (ctx->use_graph)
+            HIPCHK(hipGraphLaunch(ws.graph[par], s));
+        else
+            enqueue_iteration(ctx, par);
+        HIPCHK(hipMemcpyAsync(&ctx->host_flag[par], &ws.st->done, sizeof(int),
+                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipEventRecord(ctx->flag_ev[par], s));
+        if (k >= 1) {
+            HIPCHK(hipEventSynchronize(ctx->flag_ev[par ^ 1]));
+            if (ctx->host_flag[par ^ 1]) break;
+        }
+    }
+    k_final_x<<<nb, kBlock, 0, s>>>((int)ws.n, ws.p0, ws.p1, ws.x, ws.st);
+    LAUNCH_CHECK();
+    // Mass put-back (MG:737-761).
+    k_putback_sums<<<nb, kBlock, 0, s>>>(g, ws.east, ws.south, ws.b, ws.x, ws.parts);
+    LAUNCH_CHECK();
+    k_putback_step<<<1, kReduceBlock, 0, s>>>(ws.parts, nb, ws.st);
+    LAUNCH_CHECK();
+    k_putback_apply<<<nb, kBlock, 0, s>>>(g, ws.east, ws.south, ws.x, ws.st);
+    LAUNCH_CHECK();
+    HIPCHK(hipEventRecord(ctx->ev[3], s));
+    HIPCHK(hipMemcpyAsync(ctx->host_st, ws.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (stats) {
+        stats->iterations += (size_t)ctx->host_st->its;
+        stats->ms_setup += elapsed(ctx->ev[0], ctx->ev[1]);
+        stats->ms_guess += elapsed(ctx->ev[1], ctx->ev[2]);
+        stats->ms_cg += elapsed(ctx->ev[2], ctx->ev[3]);
+        stats->ms_total += elapsed(ctx->ev[0], ctx->ev[3]);
+    }
+    return (size_t)ctx->host_st->its;
+}
+
+// ||b - A x|| / ||b|| on the workspace (b, x, east, south built).
+static double rel_residual(dtd_ctx *ctx) {
+    Workspace &ws = ctx->ws;
+    const int nb = nblocks(ws.n);
+    k_true_resid<<<nb, kBlock, 0, ctx->stream>>>(grid_of(ws), ws.east, ws.south, ws.b, ws.x,
+                                                 ws.parts);
+    LAUNCH_CHECK();
+    k_pair_sum<<<1, kReduceBlock, 0, ctx->stream>>>(ws.parts, nb, ws.pair);
+    LAUNCH_CHECK();
+    HIPCHK(hipMemcpyAsync(ctx->host_pair, ws.pair, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const double rr = ctx->host_pair[0], bb = ctx->host_pair[1];
+    return bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+}
+
+static void check_grid(size_t w, size_t h) {
+    // Cell indices are 32-bit; keep a margin for the last block's threads.
+    const size_t cap = 0x7fff0000ull;
+    if (w > cap || h > cap || (w && h > cap / w))
+        throw Error{DTD_ETOOBIG, "the grid exceeds 2^31 - 2^16 cells"};
+}
+
+static void begin(dtd_ctx *ctx) {
+    if (!ctx) throw Error{DTD_EINVAL, "null context"};
+    ctx->err.clear();
+    HIPCHK(hipSetDevice(ctx->opts.device));
+}
+
+// Device copy of a host array on the context's stream (synchronous).
+static void upload(dtd_ctx *ctx, double *dst, const double *src, size_t n) {
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+}
+static void download(dtd_ctx *ctx, double *dst, const double *src, size_t n) {
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+}
+
+template <class F>
+static int guard(dtd_ctx *ctx, F f) {
+    try {
+        f();
+        return DTD_OK;
+    } catch (const Error &e) {
+        if (ctx) ctx->err = e.msg;
+        if (e.code == DTD_EDEVICE && ctx) {
+            // A failed shape is rebuilt on the next call.
+            (void)hipStreamSynchronize(ctx->stream);
+            free_ws(ctx->ws);
+            ctx->cycle_levels = 0;
+        }
+        return e.code;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->err = e.what();
+        return DTD_EDEVICE;
+    }
+}
+
+}  // namespace dtd
+
+using namespace dtd;
+
+extern "C" {
+
+void dtd_default_opts(dtd_opts *o) {
+    if (!o) return;
+    o->device = 0;
+    o->omega = 0.8f;
+    o->nu = 2;
+    o->scale = 0.5f;
+    o->max_iter = 400;
+    o->cycle_precision = DTD_CYCLE_F32;
+    o->guess = DTD_GUESS_LOD;
+}
+
+int dtd_create(const dtd_opts *opts, dtd_ctx **out) {
+    if (!out) return DTD_EINVAL;
+    *out = nullptr;
+    dtd_ctx *ctx = new dtd_ctx();
+    if (opts)
+        ctx->opts = *opts;
+    else
+        dtd_default_opts(&ctx->opts);
+    if (ctx->opts.nu < 1 || ctx->opts.max_iter < 1 ||
+        (ctx->opts.cycle_precision != DTD_CYCLE_F32 && ctx->opts.cycle_precision != DTD_CYCLE_F64) ||
+        (ctx->opts.guess != DTD_GUESS_LOD && ctx->opts.guess != DTD_GUESS_RHS)) {
+        delete ctx;
+        return DTD_EINVAL;
+    }
+    const char *ng = getenv("DTD_NO_GRAPH");
+    ctx->use_graph = !(ng && *ng && *ng != '0');
+    int rc = guard(ctx, [&] {
+        HIPCHK(hipSetDevice(ctx->opts.device));
+        HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        for (auto &e : ctx->ev) HIPCHK(hipEventCreate(&e));
+        for (auto &e : ctx->flag_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HIPCHK(hipHostMalloc((void **)&ctx->host_st, sizeof(CgState), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void **)&ctx->host_flag, 2 * sizeof(int), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void **)&ctx->host_pair, 2 * sizeof(double), hipHostMallocDefault));
+    });
+    if (rc != DTD_OK) {
+        dtd_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return DTD_OK;
+}
+
+void dtd_destroy(dtd_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->opts.device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_ws(ctx->ws);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : ctx->flag_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->host_st) (void)hipHostFree(ctx->host_st);
+    if (ctx->host_flag) (void)hipHostFree(ctx->host_flag);
+    if (ctx->host_pair) (void)hipHostFree(ctx->host_pair);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *dtd_last_error(const dtd_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int dtd_solve(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h_inout,
+              const double *kd, double dt, int solver, double tol_m, size_t *iterations) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (iterations) *iterations = 0;
+        if (solver != DTD_SOLVER_MULTIGRID)
+            throw Error{DTD_EINVAL, "only the multigrid solver is implemented"};
+        if (!(std::isfinite(tol_m) && tol_m > 0.0))
+            throw Error{DTD_EINVAL, "tol_m must be finite and positive"};
+        check_grid(w, h);
+        const size_t n = w * h;
+        if (n == 0) return;
+        if (!h_inout || !kd) throw Error{DTD_EINVAL, "null array"};
+        fit(ctx, (int)w, (int)h, wrap_x && w > 1);
+        Workspace &ws = ctx->ws;
+        upload(ctx, ws.kd, kd, n);
+        upload(ctx, ws.b, h_inout, n);
+        HIPCHK(hipMemcpyAsync(ws.x, ws.b, n * sizeof(double), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+        const size_t its = solve_core(ctx, ws.kd, dx, dt, wrap_x, tol_m, true, true, nullptr);
+        download(ctx, h_inout, ws.x, n);
+        if (iterations) *iterations = its;
+    });
+}
+
+int dtd_solve_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h_dev,
+                     const double *kd_dev, double dt, double tol_m, dtd_stats *stats) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (stats) *stats = dtd_stats{};
+        if (!(std::isfinite(tol_m) && tol_m > 0.0))
+            throw Error{DTD_EINVAL, "tol_m must be finite and positive"};
+        check_grid(w, h);
+        const size_t n = w * h;
+        if (n == 0) return;
+        if (!h_dev || !kd_dev) throw Error{DTD_EINVAL, "null array"};
+        fit(ctx, (int)w, (int)h, wrap_x && w > 1);
+        Workspace &ws = ctx->ws;
+        k_copy2<<<nblocks(n), kBlock, 0, ctx->stream>>>((long)n, h_dev, ws.b, ws.x);
+        LAUNCH_CHECK();
+        solve_core(ctx, kd_dev, dx, dt, wrap_x, tol_m, true, true, stats);
+        if (stats) stats->rel_residual = rel_residual(ctx);
+        HIPCHK(hipMemcpyAsync(h_dev, ws.x, n * sizeof(double), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int dtd_solve_rel_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h_dev,
+                         const double *kd_dev, double dt, double rel_tol, dtd_stats *stats) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        dtd_stats local{};
+        dtd_stats *st = stats ? stats : &local;
+        *st = dtd_stats{};
+        if (!(std::isfinite(rel_tol) && rel_tol > 0.0))
+            throw Error{DTD_EINVAL, "rel_tol must be finite and positive"};
+        check_grid(w, h);
+        const size_t n = w * h;
+        if (n == 0) return;
+        if (!h_dev || !kd_dev) throw Error{DTD_EINVAL, "null array"};
+        fit(ctx, (int)w, (int)h, wrap_x && w > 1);
+        Workspace &ws = ctx->ws;
+        k_copy2<<<nblocks(n), kBlock, 0, ctx->stream>>>((long)n, h_dev, ws.b, ws.x);
+        LAUNCH_CHECK();
+        // ||b||, for the first solve's tolerance.
+        k_sumsq<<<nblocks(n), kBlock, 0, ctx->stream>>>((int)n, ws.b, ws.parts);
+        LAUNCH_CHECK();
+        k_sum1<<<1, kReduceBlock, 0, ctx->stream>>>(ws.parts, nblocks(n), ws.pair);
+        LAUNCH_CHECK();
+        HIPCHK(hipMemcpyAsync(ctx->host_pair, ws.pair, sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        const double bnorm = std::sqrt(ctx->host_pair[0]);
+        if (bnorm == 0.0) {
+            HIPCHK(hipMemcpyAsync(h_dev, ws.b, n * sizeof(double), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            return;
+        }
+        double tol_m = rel_tol * bnorm / std::sqrt((double)n);
+        solve_core(ctx, kd_dev, dx, dt, wrap_x, tol_m, true, true, st);
+        double rel = rel_residual(ctx);
+        while (rel > rel_tol && st->restarts < 4) {
+            // Solve again from the last answer (setup kept), aiming below the target by the
+            // factor the last solve missed it by.
+            tol_m *= 0.5 * rel_tol / rel;
+            st->restarts++;
+            solve_core(ctx, kd_dev, dx, dt, wrap_x, tol_m, false, false, st);
+            rel = rel_residual(ctx);
+        }
+        st->rel_residual = rel;
+        HIPCHK(hipMemcpyAsync(h_dev, ws.x, n * sizeof(double), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int dtd_operator(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, const double *kd,
+                 double dt, double *east, double *south, double *diag) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        check_grid(w, h);
+        const size_t n = w * h;
+        if (n == 0) return;
+        if (!kd || !east || !south || !diag) throw Error{DTD_EINVAL, "null array"};
+        fit(ctx, (int)w, (int)h, wrap_x && w > 1);
+        Workspace &ws = ctx->ws;
+        upload(ctx, ws.kd, kd, n);
+        const Grid g = grid_of(ws);
+        k_operator<<<nblocks(n), kBlock, 0, ctx->stream>>>(g, ws.kd, dt, dx * dx, ws.east,
+                                                           ws.south, nullptr, nullptr, nullptr,
+                                                           nullptr, 0.0f);
+        LAUNCH_CHECK();
+        k_finest64<<<nblocks(n), kBlock, 0, ctx->stream>>>(g, ws.east, ws.south, ws.ap, ws.r, 1.0);
+        LAUNCH_CHECK();
+        download(ctx, east, ws.east, n);
+        download(ctx, south, ws.south, n);
+        download(ctx, diag, ws.ap, n);
+    });
+}
+
+int dtd_apply(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, const double *kd,
+              double dt, const double *x, double *y) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        check_grid(w, h);
+        const size_t n = w * h;
+        if (n == 0) return;
+        if (!kd || !x || !y) throw Error{DTD_EINVAL, "null array"};
+        fit(ctx, (int)w, (int)h, wrap_x && w > 1);
+        Workspace &ws = ctx->ws;
+        upload(ctx, ws.kd, kd, n);
+        upload(ctx, ws.x, x, n);
+        const Grid g = grid_of(ws);
+        k_operator<<<nblocks(n), kBlock, 0, ctx->stream>>>(g, ws.kd, dt, dx * dx, ws.east,
+                                                           ws.south, nullptr, nullptr, nullptr,
+                                                           nullptr, 0.0f);
+        LAUNCH_CHECK();
+        k_apply<<<nblocks(n), kBlock, 0, ctx->stream>>>(g, ws.east, ws.south, ws.x, ws.ap);
+        LAUNCH_CHECK();
+        download(ctx, y, ws.ap, n);
+    });
+}
+
+int dtd_lod(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h_inout,
+            const double *kd, double dt) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        check_grid(w, h);
+        const size_t n = w * h;
+        if (n == 0) return;
+        if (!kd || !h_inout) throw Error{DTD_EINVAL, "null array"};
+        fit(ctx, (int)w, (int)h, wrap_x && w > 1);
+        Workspace &ws = ctx->ws;
+        upload(ctx, ws.kd, kd, n);
+        upload(ctx, ws.x, h_inout, n);
+        lod(ctx, ws.x, ws.kd, dx, dt, wrap_x);
+        download(ctx, h_inout, ws.x, n);
+    });
+}
+
+int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *bytes) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        Workspace &ws = ctx->ws;
+        if (ws.n == 0 || ws.prec != DTD_CYCLE_F32 || ws.lf.size() < 2)
+            throw Error{DTD_ESTATE, "time a kernel after an f32-cycle solve of a multi-level grid"};
+        if (reps < 1) throw Error{DTD_EINVAL, "reps must be positive"};
+        hipStream_t s = ctx->stream;
+        const Grid g = grid_of(ws);
+        Lvl<float> &L = ws.lf[0];
+        const int nb = nblocks(ws.n);
+        const double n = (double)ws.n;
+        double b = 0.0;
+        // A private state that never stops, so the loop kernels run in full.
+        k_init_state<<<1, 64, 0, s>>>(ws.st, 0.0, 1 << 30);
+        LAUNCH_CHECK();
+        auto one = [&] {
+            switch (kernel) {
+            case DTD_KERNEL_SWEEP:  // r, wd, diag, east, south, z in; z out (f32)
+                k_sweep<float><<<nb, kBlock, 0, s>>>(L, L.z[1], L.z[0], nullptr, 0, nullptr,
+                                                     nullptr, ws.st);
+                b = 28.0 * n;
+                break;
+            case DTD_KERNEL_DIRECTION:  // z (f32), p_old, x, east, south in; p, A p, x out
+                k_direction<float><<<nb, kBlock, 0, s>>>(g, ws.east, ws.south, L.z[1], ws.p1,
+                                                         ws.p0, ws.ap, ws.x, ws.parts, ws.st);
+                b = (4.0 + 8.0 * 4 + 8.0 * 3) * n;
+                break;
+            case DTD_KERNEL_UPDATE:  // A p, r in; r, r32 out
+                k_update<float><<<nb, kBlock, 0, s>>>((int)ws.n, ws.ap, ws.r, L.r, ws.parts,
+                                                      ws.st);
+                b = (8.0 * 3 + 4.0) * n;
+                break;
+            case DTD_KERNEL_RESTRICT: {  // fine r, z, diag, east, south in; coarse r out
+                Lvl<float> &C = ws.lf[1];
+                k_restrict<float><<<nblocks(C.n), kBlock, 0, s>>>(L, L.z[1], C, ws.st);
+                b = 20.0 * n + 4.0 * C.n;
+                break;
+            }
+            default:
+                throw Error{DTD_EINVAL, "unknown kernel"};
+            }
+            LAUNCH_CHECK();
+        };
+        one();  // warm
+        HIPCHK(hipEventRecord(ctx->ev[4], s));
+        for (int k = 0; k < reps; k++) one();
+        HIPCHK(hipEventRecord(ctx->ev[5], s));
+        HIPCHK(hipEventSynchronize(ctx->ev[5]));
+        if (ms_avg) *ms_avg = elapsed(ctx->ev[4], ctx->ev[5]) / reps;
+        if (bytes) *bytes = b;
+    });
+}
+
+int dtd_cycle_new(dtd_ctx *ctx, size_t w, size_t h, int wrap_x, const double *east,
+                  const double *south, const double *diag) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        ctx->cycle_levels = 0;
+        check_grid(w, h);
+        const size_t n = w * h;
+        if (n == 0) throw Error{DTD_EINVAL, "an empty grid has no cycle"};
+        if (!east || !south || !diag) throw Error{DTD_EINVAL, "null array"};
+        fit(ctx, (int)w, (int)h, wrap_x && w > 1);
+        Workspace &ws = ctx->ws;
+        upload(ctx, ws.east, east, n);
+        upload(ctx, ws.south, south, n);
+        upload(ctx, ws.ap, diag, n);
+        hipStream_t s = ctx->stream;
+        if (ws.prec == DTD_CYCLE_F64) {
+            HIPCHK(hipMemcpyAsync(ws.ld[0].diag, ws.ap, n * 8, hipMemcpyDeviceToDevice, s));
+            k_wd64<<<nblocks(n), kBlock, 0, s>>>((int)n, ws.ld[0].diag, ws.ld[0].wd,
+                                                 (double)ctx->opts.omega);
+            LAUNCH_CHECK();
+            build_coarse<double>(ws.ld, (double)ctx->opts.omega, (double)ctx->opts.scale, s);
+        } else {
+            Lvl<float> &L = ws.lf[0];
+            k_finest32<<<nblocks(n), kBlock, 0, s>>>((int)n, ws.east, ws.south, ws.ap, L.east,
+                                                     L.south, L.diag, L.wd, ctx->opts.omega);
+            LAUNCH_CHECK();
+            build_coarse<float>(ws.lf, ctx->opts.omega, ctx->opts.scale, s);
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        ctx->cycle_levels = ws.prec == DTD_CYCLE_F64 ? ws.ld.size() : ws.lf.size();
+    });
+}
+
+size_t dtd_cycle_levels(const dtd_ctx *ctx) { return ctx ? ctx->cycle_levels : 0; }
+
+int dtd_cycle_level(dtd_ctx *ctx, size_t l, size_t *w, size_t *h, int *wrap, float *east,
+                    float *south, float *diag, float *wd, float *mass) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (l >= ctx->cycle_levels) throw Error{DTD_ESTATE, "no such level"};
+        Workspace &ws = ctx->ws;
+        if (ws.prec != DTD_CYCLE_F32) throw Error{DTD_ESTATE, "the levels are f64"};
+        Lvl<float> &L = ws.lf[l];
+        if (w) *w = (size_t)L.w;
+        if (h) *h = (size_t)L.h;
+        if (wrap) *wrap = L.wrap;
+        const size_t n = (size_t)L.n;
+        auto get = [&](float *dst, const float *src) {
+            if (dst) HIPCHK(hipMemcpyAsync(dst, src, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+        };
+        get(east, L.east);
+        get(south, L.south);
+        get(diag, L.diag);
+        get(wd, L.wd);
+        if (mass) {
+            if (L.mass)
+                get(mass, L.mass);
+            else
+                for (size_t c = 0; c < n; c++) mass[c] = 1.0f;
+        }
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int dtd_cycle_precondition(dtd_ctx *ctx, const float *r32, const double *r64, float *z32,
+                           double *rz) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (ctx->cycle_levels == 0) throw Error{DTD_ESTATE, "dtd_cycle_new first"};
+        if (!r32 || !r64 || !z32) throw Error{DTD_EINVAL, "null array"};
+        Workspace &ws = ctx->ws;
+        const size_t n = (size_t)ws.n;
+        hipStream_t s = ctx->stream;
+        upload(ctx, ws.r, r64, n);
+        int nbz;
+        if (ws.prec == DTD_CYCLE_F64) {
+            // An f64 cycle reads the f64 residual; r32 is its value rounded, kept for the ABI.
+            nbz = vcycle<double>(ws.ld, 0, (int)ctx->opts.nu, ws.r, ws.parts, nullptr, s);
+        } else {
+            HIPCHK(hipMemcpyAsync(ws.lf[0].r, r32, n * 4, hipMemcpyHostToDevice, s));
+            nbz = vcycle<float>(ws.lf, 0, (int)ctx->opts.nu, ws.r, ws.parts, nullptr, s);
+        }
+        k_sum1<<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.pair);
+        LAUNCH_CHECK();
+        if (ws.prec == DTD_CYCLE_F64) {
+            std::vector<double> tmp(n);
+            HIPCHK(hipMemcpyAsync(tmp.data(), ws.ld[0].z[1], n * 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            for (size_t c = 0; c < n; c++) z32[c] = (float)tmp[c];
+        } else {
+            HIPCHK(hipMemcpyAsync(z32, ws.lf[0].z[1], n * 4, hipMemcpyDeviceToHost, s));
+        }
+        HIPCHK(hipMemcpyAsync(ctx->host_pair, ws.pair, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (rz) *rz = ctx->host_pair[0];
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,662 @@
+// Device kernels of the diffusion solve (gfx950, wave64).  Included once, by dtdiff.hip.
+//
+// Reference files (relative to the deep-time repository root):
+//   D  crates/deeptime-core/src/surface/diffusion.rs
+//   MG crates/deeptime-core/src/surface/diffusion/multigrid.rs
+//
+// Every kernel does each cell's arithmetic in the reference's order (the build passes
+// -ffp-contract=off, so no multiply-add is fused and every operation rounds once, as on the
+// CPU), so a kernel's per-cell results equal the reference's bit for bit.  Sums are taken per
+// block in a fixed shuffle/LDS order and the block partials are added in ascending block order by
+// one block: run to run they are identical, though not in the reference's strip order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dtd {
+
+constexpr int kBlock = 256;       // threads per block of the cell-parallel kernels
+constexpr int kReduceBlock = 1024;  // threads of the single-block reductions
+
+// Conjugate-gradient scalars, resident on the device so an iteration needs no host round trip.
+struct CgState {
+    double rr;      // r.r of the current residual
+    double rz;      // r.z of the current residual and its correction
+    double pap;     // p.A p of the current direction
+    double alpha;   // step of the last update (the pending x += alpha p)
+    double beta;    // rz_new / rz
+    double tol2;    // tol_m^2 * n
+    double rr0;     // r.r of the first residual
+    double lost;    // put-back: sum (b - A x)
+    double weight;  // put-back: sum (diag - 1)
+    int have_beta;  // an iteration has formed beta (the first direction is z itself)
+    int have_pending;  // an update has set alpha
+    int done;       // the loop has stopped (1 converged, 2 p.A p, 3 r.z, 4 cap)
+    int its;        // iterations started
+    int max_iter;
+    int pad[3];
+};
+
+// ---- reductions ----------------------------------------------------------------------------
+
+// Sum of v over a block of BS threads in a fixed order (shuffles within each wave, then the
+// wave sums in ascending order); the result is valid in thread 0.
+template <int BS>
+__device__ inline double block_sum(double v, double *lds) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        s = lds[0];
+        for (int k = 1; k < BS / 64; k++) s += lds[k];
+    }
+    __syncthreads();
+    return s;
+}
+
+// Sum of parts[k * stride + slot] over k < n by one block, in a fixed order.
+__device__ inline double reduce_parts(const double *parts, int n, int stride, int slot,
+                                      double *lds) {
+    double v = 0.0;
+    for (int k = threadIdx.x; k < n; k += kReduceBlock) v += parts[(size_t)k * stride + slot];
+    return block_sum<kReduceBlock>(v, lds);
+}
+
+// ---- the f64 operator (D:532-663) ----------------------------------------------------------
+
+struct Grid {
+    int w, h, wrap;  // wrap: wrap_x && w > 1
+    int n;
+};
+
+// Face between cells a and b: 0.5 * (K_a + K_b) * dt / dx^2 (D:556-572).
+__device__ inline double face(double ka, double kb, double dt, double s2) {
+    return 0.5 * (ka + kb) * dt / s2;
+}
+
+// `Operator::new` (D:532-603) and `Level::finest` (MG:96-114) in one pass: the f64 east and
+// south couplings, and the finest f32 level's east, south, diag and omega / diag.  The diagonal
+// is ((((1 + e) + w) + s) + n) with the west and north faces formed by the same expression as
+// the neighbour's own east and south faces.
+__global__ void k_operator(Grid g, const double *__restrict__ kd, double dt, double s2,
+                           double *__restrict__ east, double *__restrict__ south,
+                           float *__restrict__ e32, float *__restrict__ s32,
+                           float *__restrict__ d32, float *__restrict__ wd32, float omega) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= g.n) return;
+    const int j = c / g.w, i = c - j * g.w, c0 = j * g.w;
+    const double k = kd[c];
+    double e;
+    if (i + 1 < g.w)
+        e = face(k, kd[c + 1], dt, s2);
+    else if (g.wrap)
+        e = face(k, kd[c0], dt, s2);
+    else
+        e = 0.0;
+    const double s = j + 1 < g.h ? face(k, kd[c + g.w], dt, s2) : 0.0;
+    double wv;
+    if (i > 0)
+        wv = face(kd[c - 1], k, dt, s2);
+    else if (g.wrap)
+        wv = face(kd[c0 + g.w - 1], kd[c0], dt, s2);
+    else
+        wv = 0.0;
+    const double nv = j > 0 ? face(kd[c - g.w], k, dt, s2) : 0.0;
+    const double d = 1.0 + e + wv + s + nv;
+    east[c] = e;
+    south[c] = s;
+    if (e32) {
+        e32[c] = (float)e;
+        s32[c] = (float)s;
+        const float df = (float)d;
+        d32[c] = df;
+        wd32[c] = omega / df;
+    }
+}
+
+// The f64 diagonal of cell c from the stored faces (the same sum Operator::new takes).
+__device__ inline double diag64(const Grid &g, const double *east, const double *south, int c,
+                                int i, int j) {
+    const int c0 = j * g.w;
+    const double wv = i > 0 ? east[c - 1] : (g.wrap ? east[c0 + g.w - 1] : 0.0);
+    const double nv = j > 0 ? south[c - g.w] : 0.0;
+    return 1.0 + east[c] + wv + south[c] + nv;
+}
+
+// One cell of y = A x in the order of `Operator::apply_row` (D:609-649): diagonal, east (the
+// wrapped one for the last cell), west (the wrapped one for the first), south, north.  `X` is a
+// functor giving x at a cell, so a direction can be formed where it is used.
+template <class X>
+__device__ inline double apply64(const Grid &g, const double *east, const double *south, int c,
+                                 int i, int j, X x) {
+    const int c0 = j * g.w;
+    const double e = east[c];
+    const double wv = i > 0 ? east[c - 1] : (g.wrap ? east[c0 + g.w - 1] : 0.0);
+    const double s = south[c];
+    const double nv = j > 0 ? south[c - g.w] : 0.0;
+    const double d = 1.0 + e + wv + s + nv;
+    double o = d * x(c);
+    if (i + 1 < g.w)
+        o -= e * x(c + 1);
+    else if (g.wrap)
+        o -= e * x(c0);
+    if (i > 0)
+        o -= wv * x(c - 1);
+    else if (g.wrap)
+        o -= wv * x(c0 + g.w - 1);
+    if (j + 1 < g.h) o -= s * x(c + g.w);
+    if (j > 0) o -= nv * x(c - g.w);
+    return o;
+}
+
+// y = A x (`Operator::apply`, D:652-663).
+__global__ void k_apply(Grid g, const double *__restrict__ east, const double *__restrict__ south,
+                        const double *__restrict__ x, double *__restrict__ y) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= g.n) return;
+    const int j = c / g.w, i = c - j * g.w;
+    y[c] = apply64(g, east, south, c, i, j, [&](int k) { return x[k]; });
+}
+
+// r = b - A x, its copy in the cycle's precision, and per block sum r*r (MG:596-619).
+template <typename T>
+__global__ void k_resid_init(Grid g, const double *__restrict__ east,
+                             const double *__restrict__ south, const double *__restrict__ b,
+                             const double *__restrict__ x, double *__restrict__ r,
+                             T *__restrict__ rc, double *__restrict__ parts) {
+    __shared__ double lds[kBlock / 64];
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    double acc = 0.0;
+    if (c < g.n) {
+        const int j = c / g.w, i = c - j * g.w;
+        const double rv = b[c] - apply64(g, east, south, c, i, j, [&](int k) { return x[k]; });
+        r[c] = rv;
+        if (rc) rc[c] = (T)rv;
+        acc = rv * rv;
+    }
+    const double s = block_sum<kBlock>(acc, lds);
+    if (threadIdx.x == 0) parts[blockIdx.x] = s;
+}
+
+// First pass of an iteration (MG:633-695): the pending x += alpha p_old, the direction
+// p = z + beta p_old (z itself on the first iteration) formed at the cell and its neighbours,
+// A p, and per block sum p * A p.
+template <typename T>
+__global__ void k_direction(Grid g, const double *__restrict__ east,
+                            const double *__restrict__ south, const T *__restrict__ z,
+                            const double *__restrict__ pold, double *__restrict__ pnew,
+                            double *__restrict__ ap, double *__restrict__ x,
+                            double *__restrict__ parts, const CgState *__restrict__ st) {
+    __shared__ double lds[kBlock / 64];
+    if (st->done) return;
+    const int have_beta = st->have_beta, pending = st->have_pending;
+    const double beta = st->beta, alpha = st->alpha;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    double acc = 0.0;
+    if (c < g.n) {
+        const int j = c / g.w, i = c - j * g.w;
+        auto p = [&](int k) {
+            return have_beta ? (double)z[k] + beta * pold[k] : (double)z[k];
+        };
+        if (pending) x[c] += alpha * pold[c];
+        const double pc = p(c);
+        const double a = apply64(g, east, south, c, i, j, p);
+        pnew[c] = pc;
+        ap[c] = a;
+        acc = pc * a;
+    }
+    const double s = block_sum<kBlock>(acc, lds);
+    if (threadIdx.x == 0) parts[blockIdx.x] = s;
+}
+
+// Second pass (MG:703-723): r -= alpha A p, its copy in the cycle's precision, per block r*r.
+template <typename T>
+__global__ void k_update(int n, const double *__restrict__ ap, double *__restrict__ r,
+                         T *__restrict__ rc, double *__restrict__ parts,
+                         const CgState *__restrict__ st) {
+    __shared__ double lds[kBlock / 64];
+    if (st->done) return;
+    const double alpha = st->alpha;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    double acc = 0.0;
+    if (c < n) {
+        const double rv = r[c] - alpha * ap[c];
+        r[c] = rv;
+        if (rc) rc[c] = (T)rv;
+        acc = rv * rv;
+    }
+    const double s = block_sum<kBlock>(acc, lds);
+    if (threadIdx.x == 0) parts[blockIdx.x] = s;
+}
+
+// The scalar steps between the passes, one block each (MG:624-730).
+enum Step { kStepRR0 = 0, kStepPAP = 1, kStepRR = 2, kStepRZ = 3, kStepRZ0 = 4 };
+
+template <int STEP>
+__global__ void k_cg_step(const double *__restrict__ parts, int nparts, CgState *st) {
+    __shared__ double lds[kReduceBlock / 64];
+    if (STEP != kStepRR0 && STEP != kStepRZ0 && st->done) return;
+    if (STEP == kStepRZ0 && st->done) return;
+    const double v = reduce_parts(parts, nparts, 1, 0, lds);
+    if (threadIdx.x != 0) return;
+    if (STEP == kStepRR0) {
+        // Stop before the loop when the first guess already meets the tolerance (MG:624-626).
+        st->rr0 = v;
+        st->rr = v;
+        st->have_beta = 0;
+        st->have_pending = 0;
+        st->its = 0;
+        st->done = v > st->tol2 ? 0 : 1;
+    } else if (STEP == kStepRZ0) {
+        st->rz = v;
+    } else if (STEP == kStepPAP) {
+        // iterations += 1 and the p.A p check (MG:631, 697-699); alpha = rz / pAp (MG:700-702).
+        st->its += 1;
+        st->pap = v;
+        if (v <= 0.0 || !isfinite(v)) {
+            st->done = 2;
+            return;
+        }
+        st->alpha = st->rz / v;
+        st->have_pending = 1;
+    } else if (STEP == kStepRR) {
+        st->rr = v;
+        if (v <= st->tol2) st->done = 1;  // MG:724-726
+    } else {
+        // rz_new, the break when it or the old rz is unusable, beta (MG:727-733); the cap of
+        // the loop (MG:630).
+        if (!(isfinite(v) && st->rz > 0.0)) {
+            st->done = 3;
+            return;
+        }
+        st->beta = v / st->rz;
+        st->have_beta = 1;
+        st->rz = v;
+        if (st->its >= st->max_iter) st->done = 4;
+    }
+}
+
+// After the loop: x += alpha p for the last update (MG:734-736).  p is the direction of the
+// last iteration, in buffer its % 2 (iteration k writes buffer k % 2... see the host's ping-pong).
+__global__ void k_final_x(int n, const double *__restrict__ p0, const double *__restrict__ p1,
+                          double *__restrict__ x, const CgState *__restrict__ st) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n || !st->have_pending) return;
+    const double *p = (st->its & 1) ? p0 : p1;
+    x[c] += st->alpha * p[c];
+}
+
+// Put-back sums (MG:737-755): per block sum (b - A x) and sum (diag - 1).
+__global__ void k_putback_sums(Grid g, const double *__restrict__ east,
+                               const double *__restrict__ south, const double *__restrict__ b,
+                               const double *__restrict__ x, double *__restrict__ parts) {
+    __shared__ double lds[kBlock / 64];
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    double l = 0.0, wg = 0.0;
+    if (c < g.n) {
+        const int j = c / g.w, i = c - j * g.w;
+        l = b[c] - apply64(g, east, south, c, i, j, [&](int k) { return x[k]; });
+        wg = diag64(g, east, south, c, i, j) - 1.0;
+    }
+    const double sl = block_sum<kBlock>(l, lds);
+    const double sw = block_sum<kBlock>(wg, lds);
+    if (threadIdx.x == 0) {
+        parts[2 * blockIdx.x] = sl;
+        parts[2 * blockIdx.x + 1] = sw;
+    }
+}
+
+__global__ void k_putback_step(const double *__restrict__ parts, int nparts, CgState *st) {
+    __shared__ double lds[kReduceBlock / 64];
+    const double l = reduce_parts(parts, nparts, 2, 0, lds);
+    const double wg = reduce_parts(parts, nparts, 2, 1, lds);
+    if (threadIdx.x == 0) {
+        st->lost = l;
+        st->weight = wg;
+    }
+}
+
+// x += (lost / weight) (diag - 1) when weight > 0 and lost != 0 (MG:756-761).
+__global__ void k_putback_apply(Grid g, const double *__restrict__ east,
+                                const double *__restrict__ south, double *__restrict__ x,
+                                const CgState *__restrict__ st) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= g.n) return;
+    const double lost = st->lost, weight = st->weight;
+    if (!(weight > 0.0 && lost != 0.0)) return;
+    const double k = lost / weight;
+    const int j = c / g.w, i = c - j * g.w;
+    x[c] += k * (diag64(g, east, south, c, i, j) - 1.0);
+}
+
+// Per block sum (b - A x)^2 and b^2: the true relative residual.
+__global__ void k_true_resid(Grid g, const double *__restrict__ east,
+                             const double *__restrict__ south, const double *__restrict__ b,
+                             const double *__restrict__ x, double *__restrict__ parts) {
+    __shared__ double lds[kBlock / 64];
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    double rr = 0.0, bb = 0.0;
+    if (c < g.n) {
+        const int j = c / g.w, i = c - j * g.w;
+        const double rv = b[c] - apply64(g, east, south, c, i, j, [&](int k) { return x[k]; });
+        rr = rv * rv;
+        bb = b[c] * b[c];
+    }
+    const double s1 = block_sum<kBlock>(rr, lds);
+    const double s2 = block_sum<kBlock>(bb, lds);
+    if (threadIdx.x == 0) {
+        parts[2 * blockIdx.x] = s1;
+        parts[2 * blockIdx.x + 1] = s2;
+    }
+}
+
+__global__ void k_pair_sum(const double *__restrict__ parts, int nparts, double *out) {
+    __shared__ double lds[kReduceBlock / 64];
+    const double a = reduce_parts(parts, nparts, 2, 0, lds);
+    const double b = reduce_parts(parts, nparts, 2, 1, lds);
+    if (threadIdx.x == 0) {
+        out[0] = a;
+        out[1] = b;
+    }
+}
+
+// ---- the V-cycle's levels (MG:73-523) -------------------------------------------------------
+
+template <typename T>
+struct Lvl {
+    int w, h, wrap, n;
+    T *east, *south, *diag, *wd, *mass;  // mass: nullptr on the finest level (1 a cell)
+    T *r, *z[2];
+};
+
+// `Level::coarsen` (MG:117-188): one thread per coarse cell.
+template <typename T>
+__global__ void k_coarsen(Lvl<T> f, Lvl<T> c, T scale) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= c.n) return;
+    const int jc = k / c.w, ic = k - jc * c.w;
+    const int r0 = 2 * jc, r1 = min(2 * jc + 2, f.h), c0 = 2 * ic, c1 = min(2 * ic + 2, f.w);
+    const int last_row = r1 - 1, last_col = c1 - 1;
+    T m = (T)0;
+    for (int j = r0; j < r1; j++)
+        for (int i = c0; i < c1; i++) m += f.mass ? f.mass[j * f.w + i] : (T)1;
+    c.mass[k] = m;
+    T e = (T)0;
+    if (ic + 1 < c.w || c.wrap) {
+        T s = (T)0;
+        for (int j = r0; j < r1; j++) s += f.east[j * f.w + last_col];
+        e = scale * s;
+    }
+    c.east[k] = e;
+    T so = (T)0;
+    if (jc + 1 < c.h) {
+        T s = (T)0;
+        for (int i = c0; i < c1; i++) s += f.south[last_row * f.w + i];
+        so = scale * s;
+    }
+    c.south[k] = so;
+}
+
+// The coarse diagonal mass + e + w + s + n and omega / diag (MG:189-198).
+template <typename T>
+__global__ void k_cdiag(Lvl<T> c, T omega) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= c.n) return;
+    const int j = k / c.w, i = k - j * c.w, c0 = j * c.w;
+    const T west = i > 0 ? c.east[k - 1] : (c.wrap ? c.east[c0 + c.w - 1] : (T)0);
+    const T north = j > 0 ? c.south[k - c.w] : (T)0;
+    const T d = c.mass[k] + c.east[k] + west + c.south[k] + north;
+    c.diag[k] = d;
+    c.wd[k] = omega / d;
+}
+
+// One cell of A x on a level in the order of `Level::apply_row` (MG:203-240): diagonal, east
+// and west (non-wrapped), the wrapped east term of the last cell, the wrapped west term of the
+// first, south, north.
+template <typename T, class X>
+__device__ inline T lvl_apply(const Lvl<T> &L, int c, int i, int j, X x) {
+    const int c0 = j * L.w;
+    T o = L.diag[c] * x(c);
+    if (i + 1 < L.w) o -= L.east[c] * x(c + 1);
+    if (i > 0) o -= L.east[c - 1] * x(c - 1);
+    if (L.wrap) {
+        if (i == L.w - 1) o -= L.east[c0 + L.w - 1] * x(c0);
+        if (i == 0) o -= L.east[c0 + L.w - 1] * x(c0 + L.w - 1);
+    }
+    if (j + 1 < L.h) o -= L.south[c] * x(c + L.w);
+    if (j > 0) o -= L.south[c - L.w] * x(c - L.w);
+    return o;
+}
+
+// `Level::first_sweep` (MG:243-256): z = wd * r.
+template <typename T>
+__global__ void k_first(Lvl<T> L, T *__restrict__ zout, const CgState *__restrict__ st) {
+    if (st && st->done) return;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= L.n) return;
+    zout[c] = L.wd[c] * L.r[c];
+}
+
+// `Level::sweep` (MG:260-329): zout = z' + wd (r - A z') with z' = zin + zc[j/2][i/2] when zc is
+// given; with dot, per block sum dot * zout.
+template <typename T>
+__global__ void k_sweep(Lvl<T> L, const T *__restrict__ zin, T *__restrict__ zout,
+                        const T *__restrict__ zc, int wc, const double *__restrict__ dot,
+                        double *__restrict__ parts, const CgState *__restrict__ st) {
+    __shared__ double lds[kBlock / 64];
+    if (st && st->done) return;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    double acc = 0.0;
+    if (c < L.n) {
+        const int j = c / L.w, i = c - j * L.w;
+        auto zp = [&](int k) -> T {
+            if (!zc) return zin[k];
+            const int jj = k / L.w, ii = k - jj * L.w;
+            return zin[k] + zc[(jj >> 1) * wc + (ii >> 1)];
+        };
+        const T ax = lvl_apply(L, c, i, j, zp);
+        const T v = zp(c) + L.wd[c] * (L.r[c] - ax);
+        zout[c] = v;
+        if (dot) acc = dot[c] * (double)v;
+    }
+    if (dot) {
+        const double s = block_sum<kBlock>(acc, lds);
+        if (threadIdx.x == 0) parts[blockIdx.x] = s;
+    }
+}
+
+// `Level::restrict_into` (MG:332-356): the coarse r = sum over the block of (r - A z), fine row
+// j0 then j0 + 1, columns ascending, from +0.
+template <typename T>
+__global__ void k_restrict(Lvl<T> f, const T *__restrict__ z, Lvl<T> c,
+                           const CgState *__restrict__ st) {
+    if (st && st->done) return;
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= c.n) return;
+    const int jc = k / c.w, ic = k - jc * c.w;
+    const int r1 = min(2 * jc + 2, f.h), c1 = min(2 * ic + 2, f.w);
+    T acc = (T)0;
+    for (int j = 2 * jc; j < r1; j++)
+        for (int i = 2 * ic; i < c1; i++) {
+            const int cf = j * f.w + i;
+            acc += f.r[cf] - lvl_apply(f, cf, i, j, [&](int q) { return z[q]; });
+        }
+    c.r[k] = acc;
+}
+
+// The coarsest level (MG:502-510): z = wd r, then COARSE_SWEEPS - 1 sweeps, in one wave (the
+// level has at most 64 cells).  With dot (the coarsest level is the finest), sum dot * z in the
+// reference's order: per strip of 8 rows, then the strips in order.
+template <typename T>
+__global__ void k_coarsest(Lvl<T> L, T *__restrict__ zout, int sweeps,
+                           const double *__restrict__ dot, double *__restrict__ parts,
+                           const CgState *__restrict__ st) {
+    __shared__ T zs[2][64];
+    if (st && st->done) return;
+    const int c = threadIdx.x;
+    const bool live = c < L.n;
+    int j = 0, i = 0;
+    if (live) {
+        j = c / L.w;
+        i = c - j * L.w;
+        zs[0][c] = L.wd[c] * L.r[c];
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int k = 1; k < sweeps; k++) {
+        if (live) {
+            const T *zz = zs[cur];
+            const T ax = lvl_apply(L, c, i, j, [&](int q) { return zz[q]; });
+            zs[cur ^ 1][c] = zz[c] + L.wd[c] * (L.r[c] - ax);
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    if (live) zout[c] = zs[cur][c];
+    if (dot && c == 0) {
+        double total = -0.0;
+        for (int s0 = 0; s0 < L.n; s0 += 8 * L.w) {
+            double a = -0.0;
+            const int s1 = min(s0 + 8 * L.w, L.n);
+            for (int q = s0; q < s1; q++) a += dot[q] * (double)zs[cur][q];
+            total += a;
+        }
+        parts[0] = total;
+    }
+}
+
+// ---- the first guess (D:370-489) -----------------------------------------------------------
+
+// The x sweep, one thread per row: `cyclic` (D:1050-1071) on rows of a wrapping grid wider than
+// two cells, `thomas` (D:1029-1046) otherwise, with the tridiagonal coefficients formed from kd
+// as they are used.  Both Thomas solves of `cyclic` share their elimination (they factor the
+// same matrix), so they run in one pass; each value is the one the separate solves form.
+// cbuf and zbuf are w*h scratch arrays.
+__global__ void k_lod_rows(Grid g, int wrap_x, const double *__restrict__ kd, double dt, double s2,
+                           double *__restrict__ x, double *__restrict__ cbuf,
+                           double *__restrict__ zbuf) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= g.h) return;
+    const int w = g.w;
+    const double *k = kd + (size_t)j * w;
+    double *b = x + (size_t)j * w;
+    double *cc = cbuf + (size_t)j * w;
+    double *zz = zbuf + (size_t)j * w;
+    const bool ring = wrap_x && w > 1;
+    auto fc = [&](int i) -> double {
+        if (i + 1 < w) return face(k[i], k[i + 1], dt, s2);
+        return ring ? face(k[w - 1], k[0], dt, s2) : 0.0;
+    };
+    if (wrap_x && w > 2) {
+        const double flast = fc(w - 1);
+        // lo[i] = -west(i), up[i] = -east(i), di[i] = 1 + west + east; corners alpha = up[w-1],
+        // beta = lo[0] (D:386-399).
+        const double alpha = -flast, beta = -flast;
+        double f0 = fc(0);
+        const double di0 = 1.0 + flast + f0;
+        const double gamma = -di0;
+        // bb[0] = di[0] - gamma, bb[n-1] = di[n-1] - alpha beta / gamma; l[0] = 0, u[n-1] = 0.
+        double bt = di0 - gamma;
+        cc[0] = 0.0;
+        b[0] /= bt;
+        double zprev = gamma / bt;
+        zz[0] = zprev;
+        double xprev = b[0];
+        double fprev = f0;  // face(i - 1)
+        for (int i = 1; i < w; i++) {
+            const double fi = (i + 1 < w) ? fc(i) : flast;
+            const double lo = -fprev, up_prev = -fprev;  // up[i-1] = -face(i-1)
+            double di = 1.0 + fprev + fi;
+            if (i == w - 1) di = di - alpha * beta / gamma;
+            const double ci = up_prev / bt;
+            cc[i] = ci;
+            bt = di - lo * ci;
+            const double xi = (b[i] - lo * xprev) / bt;
+            b[i] = xi;
+            xprev = xi;
+            const double z0 = (i == w - 1) ? alpha : 0.0;
+            const double zi = (z0 - lo * zprev) / bt;
+            zz[i] = zi;
+            zprev = zi;
+            fprev = fi;
+        }
+        for (int i = w - 2; i >= 0; i--) {
+            b[i] -= cc[i + 1] * b[i + 1];
+            zz[i] -= cc[i + 1] * zz[i + 1];
+        }
+        const double fact = (b[0] + beta * b[w - 1] / gamma) / (1.0 + zz[0] + beta * zz[w - 1] / gamma);
+        for (int i = 0; i < w; i++) b[i] = b[i] - fact * zz[i];
+        return;
+    }
+    // Plain Thomas: lo[0] = 0, up[w-1] = 0; two wrapping cells fold their two faces into one;
+    // one cell has diagonal 1 (D:400-419).
+    double f2 = 0.0;
+    if (w == 2 && wrap_x) f2 = fc(0) + fc(1);
+    auto west = [&](int i) -> double { return i == 0 ? fc(w - 1) : fc(i - 1); };
+    auto lo_at = [&](int i) -> double {
+        if (i == 0) return 0.0;
+        if (w == 2 && wrap_x) return -f2;
+        return -west(i);
+    };
+    auto up_at = [&](int i) -> double {
+        if (i == w - 1) return 0.0;
+        if (w == 2 && wrap_x) return -f2;
+        return -fc(i);
+    };
+    auto di_at = [&](int i) -> double {
+        if (w == 1) return 1.0;
+        if (w == 2 && wrap_x) return 1.0 + f2;
+        return 1.0 + west(i) + fc(i);
+    };
+    double bt = di_at(0);
+    cc[0] = 0.0;
+    b[0] /= bt;
+    for (int i = 1; i < w; i++) {
+        const double ci = up_at(i - 1) / bt;
+        cc[i] = ci;
+        const double lo = lo_at(i);
+        bt = di_at(i) - lo * ci;
+        b[i] = (b[i] - lo * b[i - 1]) / bt;
+    }
+    for (int i = w - 2; i >= 0; i--) b[i] -= cc[i + 1] * b[i + 1];
+}
+
+// The z sweep, one thread per column (coalesced across the wave): `thomas` down the column with
+// north and south faces (D:436-488).
+__global__ void k_lod_cols(Grid g, const double *__restrict__ kd, double dt, double s2,
+                           double *__restrict__ x, double *__restrict__ cbuf) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= g.w) return;
+    const int w = g.w, h = g.h;
+    auto fz = [&](int j) -> double {
+        return j + 1 < h ? face(kd[(size_t)j * w + i], kd[(size_t)(j + 1) * w + i], dt, s2) : 0.0;
+    };
+    double fn = 0.0;  // face(j - 1), the north face
+    double fs = fz(0);
+    double bt = 1.0 + fn + fs;
+    cbuf[i] = 0.0;
+    double prev = x[i] / bt;
+    x[i] = prev;
+    for (int j = 1; j < h; j++) {
+        const double up_prev = -fs;  // up[j-1] = -south(j-1)
+        fn = fs;
+        fs = fz(j);
+        const double lo = -fn;
+        const double di = 1.0 + fn + fs;
+        const double cj = up_prev / bt;
+        const size_t cidx = (size_t)j * w + i;
+        cbuf[cidx] = cj;
+        bt = di - lo * cj;
+        prev = (x[cidx] - lo * prev) / bt;
+        x[cidx] = prev;
+    }
+    for (int j = h - 2; j >= 0; j--) {
+        const size_t cidx = (size_t)j * w + i;
+        x[cidx] -= cbuf[cidx + w] * x[cidx + w];
+    }
+}
+
+}  // namespace dtd

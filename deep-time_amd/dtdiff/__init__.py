@@ -1,0 +1,295 @@
+"""deep-time's implicit diffusion solve on MI355X, behind the C ABI of include/dtdiff.h.
+
+The Python surface mirrors the reference's interface for this path
+(crates/deeptime-core/src/surface/diffusion.rs:77-97, grid.rs:11-37, params.rs:154):
+
+    ws = Workspace()                      # the session's device slot + the solve's scratch
+    d = Dims(w, h, dx, wrap_x=True)
+    its = solve(ws, d, h, kd, dt, DiffusionSolver.MULTIGRID, tol_m)   # h: float64, in place
+
+and adds the device-resident entry points the benchmark uses (solve_device, solve_rel_device),
+plus the pieces the parity tests compare one by one (operator, apply, lod, Cycle).
+
+There is no CPU fallback: if the HIP library is missing or a call fails, an exception says so.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdtdiff.so")
+
+DTD_OK = 0
+DTD_EINVAL = -1
+DTD_ETOOBIG = -2
+DTD_EDEVICE = -3
+DTD_ESTATE = -4
+CYCLE_F32 = 0
+CYCLE_F64 = 1
+GUESS_LOD = 0
+GUESS_RHS = 1
+
+# Every symbol include/dtdiff.h declares (the CPU tests check the library exports them all).
+SYMBOLS = (
+    "dtd_default_opts", "dtd_create", "dtd_destroy", "dtd_last_error", "dtd_solve",
+    "dtd_solve_device", "dtd_solve_rel_device", "dtd_operator", "dtd_apply", "dtd_lod",
+    "dtd_cycle_new", "dtd_cycle_levels", "dtd_cycle_level", "dtd_cycle_precondition",
+    "dtd_time_kernel",
+)
+KERNEL_SWEEP, KERNEL_DIRECTION, KERNEL_UPDATE, KERNEL_RESTRICT = 0, 1, 2, 3
+
+
+class DiffusionSolver(enum.IntEnum):
+    """`DiffusionSolver`, crates/deeptime-core/src/surface/params.rs:154."""
+    MULTIGRID = 0
+    JACOBI = 1
+
+
+@dataclass(frozen=True)
+class Dims:
+    """`Dims`, crates/deeptime-core/src/surface/grid.rs:11-37: w x h cells of side dx metres,
+    wrapping in x when wrap_x; row-major, row 0 the north row."""
+    w: int
+    h: int
+    dx: float
+    wrap_x: bool
+
+    def cells(self) -> int:
+        return self.w * self.h
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("omega", ctypes.c_float), ("nu", ctypes.c_uint),
+                ("scale", ctypes.c_float), ("max_iter", ctypes.c_uint),
+                ("cycle_precision", ctypes.c_int), ("guess", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_size_t), ("restarts", ctypes.c_size_t),
+                ("rel_residual", ctypes.c_double), ("ms_setup", ctypes.c_double),
+                ("ms_guess", ctypes.c_double), ("ms_cg", ctypes.c_double),
+                ("ms_total", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class DtdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dtdiff error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_sz = ctypes.c_size_t
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def load(path: str = LIB_PATH):
+    """The HIP library (raises if it has not been built: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python deep-time_amd/build.py`")
+    L = ctypes.CDLL(path)
+    L.dtd_default_opts.argtypes = [ctypes.POINTER(Opts)]
+    L.dtd_create.argtypes = [ctypes.POINTER(Opts), ctypes.POINTER(_vp)]
+    L.dtd_create.restype = ctypes.c_int
+    L.dtd_destroy.argtypes = [_vp]
+    L.dtd_last_error.argtypes = [_vp]
+    L.dtd_last_error.restype = ctypes.c_char_p
+    L.dtd_solve.argtypes = [_vp, _sz, _sz, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                            ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                            ctypes.POINTER(_sz)]
+    L.dtd_solve_device.argtypes = [_vp, _sz, _sz, ctypes.c_double, ctypes.c_int, _vp, _vp,
+                                   ctypes.c_double, ctypes.c_double, ctypes.POINTER(Stats)]
+    L.dtd_solve_rel_device.argtypes = [_vp, _sz, _sz, ctypes.c_double, ctypes.c_int, _vp, _vp,
+                                       ctypes.c_double, ctypes.c_double,
+                                       ctypes.POINTER(Stats)]
+    L.dtd_operator.argtypes = [_vp, _sz, _sz, ctypes.c_double, ctypes.c_int, _dp,
+                               ctypes.c_double, _dp, _dp, _dp]
+    L.dtd_apply.argtypes = [_vp, _sz, _sz, ctypes.c_double, ctypes.c_int, _dp, ctypes.c_double,
+                            _dp, _dp]
+    L.dtd_lod.argtypes = [_vp, _sz, _sz, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                          ctypes.c_double]
+    L.dtd_cycle_new.argtypes = [_vp, _sz, _sz, ctypes.c_int, _dp, _dp, _dp]
+    L.dtd_cycle_levels.argtypes = [_vp]
+    L.dtd_cycle_levels.restype = _sz
+    L.dtd_cycle_level.argtypes = [_vp, _sz, ctypes.POINTER(_sz), ctypes.POINTER(_sz),
+                                  ctypes.POINTER(ctypes.c_int), _fp, _fp, _fp, _fp, _fp]
+    L.dtd_cycle_precondition.argtypes = [_vp, _fp, _dp, _fp, _dp]
+    L.dtd_time_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+    for name in ("dtd_solve", "dtd_solve_device", "dtd_solve_rel_device", "dtd_operator",
+                 "dtd_apply", "dtd_lod", "dtd_cycle_new", "dtd_cycle_level",
+                 "dtd_cycle_precondition", "dtd_time_kernel"):
+        getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def default_opts() -> Opts:
+    o = Opts()
+    load().dtd_default_opts(ctypes.byref(o))
+    return o
+
+
+def _f64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(_fp)
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (or an int address)."""
+    return ctypes.c_void_p(t if isinstance(t, int) else t.data_ptr())
+
+
+class Workspace:
+    """The solve's kept state (`Workspace`, diffusion.rs:213-260) together with the session's
+    device slot (`Ambient::kept`, compute.rs:110-122): a dtdiff context whose device buffers and
+    levels stay resident between solves of one grid shape."""
+
+    def __init__(self, device: int = 0, opts: Opts | None = None, **kw):
+        L = load()
+        o = opts if opts is not None else default_opts()
+        o.device = device
+        for k, v in kw.items():
+            setattr(o, k, v)
+        self.opts = o
+        self._ctx = _vp()
+        rc = L.dtd_create(ctypes.byref(o), ctypes.byref(self._ctx))
+        if rc != DTD_OK:
+            raise DtdError(rc, "dtd_create failed")
+
+    def close(self):
+        if getattr(self, "_ctx", None) and self._ctx.value:
+            load().dtd_destroy(self._ctx)
+            self._ctx = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc != DTD_OK:
+            raise DtdError(rc, load().dtd_last_error(self._ctx).decode())
+
+
+def solve(ws: Workspace, d: Dims, h: np.ndarray, kd: np.ndarray, dt: float,
+          solver: DiffusionSolver, tol_m: float) -> int:
+    """One implicit step (`diffusion::solve`, diffusion.rs:77-97): h (float64, w*h) holds h0 on
+    entry and h1 on return; returns the conjugate-gradient iterations."""
+    if not (isinstance(h, np.ndarray) and h.dtype == np.float64 and h.flags.c_contiguous):
+        raise TypeError("h must be a C-contiguous float64 array (it is solved in place)")
+    if h.size != d.cells() or np.size(kd) != d.cells():
+        raise ValueError("h and kd must have w*h cells")
+    kd, kdp = _f64(kd)
+    its = _sz(0)
+    ws.check(load().dtd_solve(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x), h.ctypes.data_as(_dp),
+                              kdp, dt, int(solver), tol_m, ctypes.byref(its)))
+    return its.value
+
+
+def solve_device(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float, tol_m: float) -> Stats:
+    """The solve on device arrays (torch tensors or addresses of w*h doubles), h in place."""
+    st = Stats()
+    ws.check(load().dtd_solve_device(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x), _ptr(h_dev),
+                                     _ptr(kd_dev), dt, tol_m, ctypes.byref(st)))
+    return st
+
+
+def solve_rel_device(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float, rel_tol: float) -> Stats:
+    """The solve to a true f64 relative residual ||h0 - A h1|| / ||h0|| <= rel_tol."""
+    st = Stats()
+    ws.check(load().dtd_solve_rel_device(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x), _ptr(h_dev),
+                                         _ptr(kd_dev), dt, rel_tol, ctypes.byref(st)))
+    return st
+
+
+def operator(ws: Workspace, d: Dims, kd, dt):
+    """`Operator::new` (diffusion.rs:532-603) on the device -> (east, south, diag)."""
+    n = d.cells()
+    kd, kdp = _f64(kd)
+    e, s, g = np.zeros(n), np.zeros(n), np.zeros(n)
+    ws.check(load().dtd_operator(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x), kdp, dt,
+                                 e.ctypes.data_as(_dp), s.ctypes.data_as(_dp),
+                                 g.ctypes.data_as(_dp)))
+    return e, s, g
+
+
+def apply(ws: Workspace, d: Dims, kd, dt, x):
+    """(I + dt L) x on the device (`Operator::apply`, diffusion.rs:652-663)."""
+    kd, kdp = _f64(kd)
+    x, xp = _f64(x)
+    y = np.zeros(d.cells())
+    ws.check(load().dtd_apply(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x), kdp, dt, xp,
+                              y.ctypes.data_as(_dp)))
+    return y
+
+
+def lod(ws: Workspace, d: Dims, h, kd, dt):
+    """The splitting's first guess (`lod`, diffusion.rs:370-489) on the device."""
+    kd, kdp = _f64(kd)
+    x = np.array(h, dtype=np.float64, copy=True)
+    ws.check(load().dtd_lod(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x), x.ctypes.data_as(_dp), kdp,
+                            dt))
+    return x
+
+
+class Cycle:
+    """The device V-cycle of one operator (`DeviceCycle`, multigrid/gpu.rs:562-714)."""
+
+    def __init__(self, ws: Workspace, d: Dims, east, south, diag):
+        self.ws, self.d = ws, d
+        e, ep = _f64(east)
+        s, sp = _f64(south)
+        g, gp = _f64(diag)
+        ws.check(load().dtd_cycle_new(ws._ctx, d.w, d.h, int(d.wrap_x), ep, sp, gp))
+
+    def levels(self) -> int:
+        return load().dtd_cycle_levels(self.ws._ctx)
+
+    def level(self, l: int):
+        w, h, wr = _sz(), _sz(), ctypes.c_int()
+        L = load()
+        self.ws.check(L.dtd_cycle_level(self.ws._ctx, l, ctypes.byref(w), ctypes.byref(h),
+                                        ctypes.byref(wr), None, None, None, None, None))
+        n = w.value * h.value
+        out = {k: np.zeros(n, np.float32) for k in ("east", "south", "diag", "wd", "mass")}
+        self.ws.check(L.dtd_cycle_level(self.ws._ctx, l, None, None, None,
+                                        *[out[k].ctypes.data_as(_fp)
+                                          for k in ("east", "south", "diag", "wd", "mass")]))
+        out.update(w=w.value, h=h.value, wrap=bool(wr.value))
+        return out
+
+    def precondition(self, r32, r64):
+        """z = M^-1 r by one V-cycle from zero; returns (z, sum r64 * z)."""
+        r32, r32p = _f32(r32)
+        r64, r64p = _f64(r64)
+        z = np.zeros(self.d.cells(), np.float32)
+        rz = ctypes.c_double()
+        self.ws.check(load().dtd_cycle_precondition(self.ws._ctx, r32p, r64p,
+                                                    z.ctypes.data_as(_fp), ctypes.byref(rz)))
+        return z, rz.value
+
+
+def time_kernel(ws: Workspace, kernel: int, reps: int = 20):
+    """Average duration (ms, HIP events on the context's stream) and compulsory bytes per launch
+    of one of the solve's kernels over the resident data of the last solve."""
+    ms, b = ctypes.c_double(), ctypes.c_double()
+    ws.check(load().dtd_time_kernel(ws._ctx, kernel, reps, ctypes.byref(ms), ctypes.byref(b)))
+    return ms.value, b.value

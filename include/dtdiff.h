@@ -1,0 +1,153 @@
+/*
+ * dtdiff: the implicit hillslope/marine diffusion solve of deep-time on AMD Instinct MI355X
+ * (gfx950), behind a C ABI.
+ *
+ * The system is (I + dt * div K grad) h1 = h0 on a w x h raster of square cells of side dx,
+ * periodic in x when wrap_x (and w > 1), with no flux through the first and last rows.  Face
+ * couplings are 0.5 * (K_i + K_j) * dt / dx^2.  It is solved by f64 conjugate gradients
+ * preconditioned by one multigrid V-cycle (2x2 aggregation, coarse faces scaled by 1/2, damped
+ * Jacobi with omega = 0.8, two sweeps before and after, 16 sweeps on the coarsest level of at
+ * most 64 cells), from the locally one-dimensional splitting's first guess, to an RMS residual
+ * of tol_m metres, after which the residual's sum is returned to the cells in proportion to
+ * their coupling, so sum(h1) == sum(h0) to round-off.
+ *
+ * Reference interfaces replaced (paths relative to the deep-time repository root):
+ *   dtd_solve / dtd_solve_device      crates/deeptime-core/src/surface/diffusion.rs:77-116
+ *                                     (`solve` with DiffusionSolver::Multigrid -> `multigrid_in`
+ *                                      -> `lod` + `multigrid::cg` -> `cg_on`, multigrid.rs:562-763)
+ *   dtd_create / dtd_destroy          the session's device slot, compute.rs:110-122
+ *                                     (`Ambient::kept`), whose levels stay resident between solves
+ *   dtd_cycle_new / dtd_cycle_precondition
+ *                                     `DeviceCycle::new` / `DeviceCycle::precondition`,
+ *                                     multigrid/gpu.rs:587 and :661 (the device seam of `Pre`,
+ *                                     multigrid.rs:385-458)
+ *   dtd_cycle_level                   the levels `Level::finest`/`Level::coarsen` build,
+ *                                     multigrid.rs:96-199
+ *   dtd_operator / dtd_apply          `Operator::new` / `Operator::apply`, diffusion.rs:532-663
+ *   dtd_lod                           `lod`, diffusion.rs:370-489 (every row visited)
+ *
+ * Errors: every entry point returns DTD_OK (0) or a negative DTD_E* code; the message is kept
+ * per context (dtd_last_error).  On error the caller's output arrays are left as they were.
+ * No exception crosses the ABI.  A context is not thread-safe; distinct contexts are
+ * independent.  Results do not depend on timing: every device reduction sums in a fixed order.
+ */
+#ifndef DTDIFF_H
+#define DTDIFF_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DTD_OK 0
+#define DTD_EINVAL (-1)   /* a bad argument (NULL pointer, non-finite or non-positive tol) */
+#define DTD_ETOOBIG (-2)  /* the grid exceeds 2^31 - 1 cells or the device memory */
+#define DTD_EDEVICE (-3)  /* a HIP runtime error */
+#define DTD_ESTATE (-4)   /* a call out of order (dtd_cycle_precondition before dtd_cycle_new) */
+
+/* `DiffusionSolver`, crates/deeptime-core/src/surface/params.rs:154. */
+#define DTD_SOLVER_MULTIGRID 0
+#define DTD_SOLVER_JACOBI 1
+
+/* Precision of the V-cycle that preconditions the f64 conjugate gradients. */
+#define DTD_CYCLE_F32 0  /* the reference's: every level in f32 (multigrid.rs:24-27) */
+#define DTD_CYCLE_F64 1  /* every level in f64 (the plain FP64 multigrid) */
+
+/* First guess of a solve. */
+#define DTD_GUESS_LOD 0  /* the splitting, as `multigrid_in` (diffusion.rs:110-114) */
+#define DTD_GUESS_RHS 1  /* h0 itself */
+
+typedef struct dtd_ctx dtd_ctx;
+
+/* Settings of a context; dtd_default_opts gives the reference's (multigrid.rs:45-56,
+ * diffusion.rs:497). */
+typedef struct dtd_opts {
+    int device;           /* HIP device ordinal */
+    float omega;          /* Jacobi damping, 0.8 */
+    unsigned nu;          /* sweeps before and after the coarse correction, 2 */
+    float scale;          /* coarse coupling share, 0.5 */
+    unsigned max_iter;    /* conjugate-gradient iteration cap, 400 */
+    int cycle_precision;  /* DTD_CYCLE_F32 */
+    int guess;            /* DTD_GUESS_LOD */
+} dtd_opts;
+
+/* What a solve did (all times in milliseconds, from HIP events on the context's stream). */
+typedef struct dtd_stats {
+    size_t iterations;    /* conjugate-gradient iterations, summed over restarts */
+    size_t restarts;      /* extra solves dtd_solve_rel_device took to meet its target */
+    double rel_residual;  /* ||h0 - A h1||_2 / ||h0||_2 in f64 after the mass put-back */
+    double ms_setup;      /* operator, levels */
+    double ms_guess;      /* first guess */
+    double ms_cg;         /* conjugate gradients */
+    double ms_total;      /* the whole solve on the device */
+} dtd_stats;
+
+void dtd_default_opts(dtd_opts *opts);
+
+/* A context on opts->device (NULL opts: the defaults on device 0). */
+int dtd_create(const dtd_opts *opts, dtd_ctx **out);
+void dtd_destroy(dtd_ctx *ctx);
+/* The last error's message ("" when none); valid until the next call on ctx. */
+const char *dtd_last_error(const dtd_ctx *ctx);
+
+/* `diffusion::solve` (diffusion.rs:77-97) on host arrays: h_inout holds h0 on entry and h1 on
+ * return; kd (m^2/yr) has w*h cells; dt in years; tol_m the RMS residual in metres.  Only
+ * DTD_SOLVER_MULTIGRID is implemented (DTD_SOLVER_JACOBI returns DTD_EINVAL).  *iterations
+ * (may be NULL) receives the conjugate-gradient iterations.  w*h == 0 returns 0 iterations. */
+int dtd_solve(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h_inout,
+              const double *kd, double dt, int solver, double tol_m, size_t *iterations);
+
+/* The same solve on device-resident arrays (h_dev in/out and kd_dev: device pointers of w*h
+ * doubles on the context's device).  stats may be NULL. */
+int dtd_solve_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h_dev,
+                     const double *kd_dev, double dt, double tol_m, dtd_stats *stats);
+
+/* The solve to a true relative residual: ||h0 - A h1||_2 / ||h0||_2 <= rel_tol in f64, measured
+ * after the mass put-back.  Runs the solve at tol_m = rel_tol * ||h0|| / sqrt(n) and, while the
+ * measured residual misses the target, solves again from the last answer at a tighter tol_m
+ * (at most 4 restarts). */
+int dtd_solve_rel_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h_dev,
+                         const double *kd_dev, double dt, double rel_tol, dtd_stats *stats);
+
+/* `Operator::new` (host arrays): east, south, diag of w*h doubles each. */
+int dtd_operator(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, const double *kd,
+                 double dt, double *east, double *south, double *diag);
+
+/* `Operator::apply` with the operator built from kd: y = (I + dt L) x (host arrays). */
+int dtd_apply(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, const double *kd,
+              double dt, const double *x, double *y);
+
+/* `lod` on host arrays: h_inout becomes the splitting's first guess. */
+int dtd_lod(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h_inout,
+            const double *kd, double dt);
+
+/* `DeviceCycle::new`: the cycle's levels for the f64 operator (host arrays of w*h doubles). */
+int dtd_cycle_new(dtd_ctx *ctx, size_t w, size_t h, int wrap_x, const double *east,
+                  const double *south, const double *diag);
+/* Number of levels of the cycle dtd_cycle_new built (0 before). */
+size_t dtd_cycle_levels(const dtd_ctx *ctx);
+/* Level l's shape and f32 operator (NULL outputs are skipped; mass is 1 on level 0). */
+int dtd_cycle_level(dtd_ctx *ctx, size_t l, size_t *w, size_t *h, int *wrap, float *east,
+                    float *south, float *diag, float *wd, float *mass);
+/* `DeviceCycle::precondition`: z32 = M^-1 r32 by one V-cycle from zero; *rz = sum r64 * z32. */
+int dtd_cycle_precondition(dtd_ctx *ctx, const float *r32, const double *r64, float *z32,
+                           double *rz);
+
+/* Kernels dtd_time_kernel can time on the workspace of the last f32-cycle solve. */
+#define DTD_KERNEL_SWEEP 0      /* one damped-Jacobi sweep of the finest level (f32) */
+#define DTD_KERNEL_DIRECTION 1  /* the conjugate gradients' direction pass (f64) */
+#define DTD_KERNEL_UPDATE 2     /* the residual update pass (f64) */
+#define DTD_KERNEL_RESTRICT 3   /* the finest level's residual restriction (f32) */
+
+/* Measurement hook, no reference counterpart: launches `kernel` reps times back to back on the
+ * context's stream over the resident data of the last solve and returns the average duration
+ * between two HIP events (ms) and the kernel's compulsory bytes per launch (every array it
+ * touches read or written once, at its storage precision).  The workspace's vectors are
+ * overwritten: solve again before reading results. */
+int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

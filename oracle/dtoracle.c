@@ -1,0 +1,974 @@
+/*
+ * CPU oracle for the implicit diffusion solve: TEST INFRASTRUCTURE ONLY (see dtoracle.h).
+ *
+ * Reference files (relative to the reference repository root):
+ *   D   crates/deeptime-core/src/surface/diffusion.rs
+ *   MG  crates/deeptime-core/src/surface/diffusion/multigrid.rs
+ *   MGT crates/deeptime-core/src/surface/diffusion/multigrid/tests.rs
+ *   PAR crates/deeptime-core/src/surface/diffusion/parity.rs
+ *   SM  crates/deeptime-core/src/sphere/mod.rs
+ *   MAR crates/deeptime-core/src/surface/marine.rs
+ *
+ * Build: gcc -O2 -fopenmp -ffp-contract=off -fno-fast-math (oracle/Makefile).  With SSE2
+ * arithmetic (x86-64) float and double operations round exactly once each, so the cell-by-cell
+ * arithmetic is the reference's bit for bit; the reductions keep the reference's order.
+ */
+#include "dtoracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Rows per strip of the reference's fused passes (D:502); every strip sum is taken over its
+ * rows in row-major order and the strip sums are then added in ascending strip order. */
+#define STRIP_ROWS 8
+/* CG iteration cap (D:497). */
+#define CG_MAX_ITER 400
+/* Cycle constants (MG:45-56). */
+#define OMEGA 0.8f
+#define NU 2
+#define SCALE 0.5f
+#define COARSEST 64
+#define COARSE_SWEEPS 16
+
+int or_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* The value an empty f64 sum starts from in the reference (D:507-510): -0.0. */
+static inline double sum_zero(void) { return -0.0; }
+
+static size_t div_ceil(size_t a, size_t b) { return (a + b - 1) / b; }
+
+/* ---- fixtures ------------------------------------------------------------------------------ */
+
+/* SplitMix64::next_u64, SM:52-58 (Steele, Lea & Flood 2014). */
+uint64_t or_splitmix_next(uint64_t *state) {
+    *state += 0x9E3779B97F4A7C15ull;
+    uint64_t z = *state;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+/* SplitMix64::next_f64, SM:61-63: 53 random bits scaled into [0, 1). */
+double or_splitmix_f64(uint64_t *state) {
+    return (double)(or_splitmix_next(state) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/* Rust's f64::fract: x - trunc(x). */
+static inline double fract(double x) { return x - trunc(x); }
+
+/* `terrain`, MGT:77-98: six octaves of bilinear value noise, metres. */
+void or_terrain(size_t w, size_t h, uint64_t seed, double *z) {
+    uint64_t st = seed;
+    memset(z, 0, w * h * sizeof(double));
+    for (int oct = 0; oct < 6; oct++) {
+        size_t g = (size_t)2 << oct;
+        double *grid = (double *)malloc(g * g * sizeof(double));
+        for (size_t k = 0; k < g * g; k++) grid[k] = or_splitmix_f64(&st) - 0.5;
+        double amp = 3000.0 / (double)(1 << oct);
+        size_t hmax = h > 1 ? h : 1;
+        for (size_t j = 0; j < h; j++) {
+            double fy = (double)j / (double)hmax * (double)(g - 1);
+            size_t j0 = (size_t)floor(fy);
+            double ty = fract(fy);
+            for (size_t i = 0; i < w; i++) {
+                double fx = (double)i / (double)w * (double)g;
+                size_t i0 = (size_t)floor(fx);
+                double tx = fract(fx);
+#define AT(a, b) grid[((b) < g - 1 ? (b) : g - 1) * g + (a) % g]
+                double top = AT(i0, j0) * (1.0 - tx) + AT(i0 + 1, j0) * tx;
+                double bot = AT(i0, j0 + 1) * (1.0 - tx) + AT(i0 + 1, j0 + 1) * tx;
+#undef AT
+                z[j * w + i] += amp * (top * (1.0 - ty) + bot * ty);
+            }
+        }
+        free(grid);
+    }
+}
+
+/* `fractal`, PAR:340-359: eight octaves of value noise on a 2:1 lattice. */
+void or_fractal(size_t w, size_t h, uint64_t seed, double *z) {
+    uint64_t st = seed;
+    memset(z, 0, w * h * sizeof(double));
+    for (int oct = 0; oct < 8; oct++) {
+        size_t gw = (size_t)4 << oct, gh = (size_t)2 << oct;
+        double *g = (double *)malloc(gw * gh * sizeof(double));
+        for (size_t k = 0; k < gw * gh; k++) g[k] = or_splitmix_f64(&st) - 0.5;
+        double amp = 4000.0 / pow((double)(1 << oct), 0.9);
+        for (size_t j = 0; j < h; j++) {
+            double fy = (double)j / (double)h * (double)gh;
+            size_t j0 = (size_t)floor(fy);
+            double ty = fract(fy);
+            for (size_t i = 0; i < w; i++) {
+                double fx = (double)i / (double)w * (double)gw;
+                size_t i0 = (size_t)floor(fx);
+                double tx = fract(fx);
+#define AT(a, b) g[((b) < gh - 1 ? (b) : gh - 1) * gw + (a) % gw]
+                double top = AT(i0, j0) * (1.0 - tx) + AT(i0 + 1, j0) * tx;
+                double bot = AT(i0, j0 + 1) * (1.0 - tx) + AT(i0 + 1, j0 + 1) * tx;
+#undef AT
+                z[j * w + i] += amp * (top * (1.0 - ty) + bot * ty);
+            }
+        }
+        free(g);
+    }
+}
+
+/* `submarine_kd`, MAR:44-59: depth-dependent marine diffusivity blended to `land` ashore. */
+double or_submarine_kd(double z, double d0, double wave_base, double tidal_range, double land) {
+    double f1;
+    if (tidal_range > 0.0)
+        f1 = (tanh(z / tidal_range) + 1.0) / 2.0;
+    else
+        f1 = z >= 0.0 ? 1.0 : 0.0;
+    double f2 = (z > wave_base && wave_base > 0.0) ? exp(-(z - wave_base) / wave_base) : 1.0;
+    return d0 * f1 * f2 + land * (1.0 - f1);
+}
+
+/* `shelf_kd`, MGT:102-110. */
+void or_shelf_kd(size_t n, const double *z, uint64_t seed, double *kd) {
+    uint64_t st = seed ^ 0x5eedull;
+    for (size_t c = 0; c < n; c++) {
+        double land = 2.0 * (0.25 + 4.0 * or_splitmix_f64(&st));
+        kd[c] = or_submarine_kd(-z[c], 100.0, 60.0, 2.0, land);
+    }
+}
+
+/* ---- operator ------------------------------------------------------------------------------ */
+
+/* `Operator::new`, D:532-603 (every row visited): east/south face couplings
+ * 0.5*(K_i + K_j)*dt/dx^2 and diag = ((((1 + e) + w) + s) + n). */
+void or_operator(size_t w, size_t h, double dx, int wrap_x, const double *kd, double dt,
+                 double *east, double *south, double *diag) {
+    double s2 = dx * dx;
+    int wrap = wrap_x && w > 1;
+#pragma omp parallel for schedule(static)
+    for (long jl = 0; jl < (long)h; jl++) {
+        size_t j = (size_t)jl;
+        const double *krow = kd + j * w;
+        double *erow = east + j * w, *srow = south + j * w;
+        for (size_t i = 0; i < w; i++) {
+            if (i + 1 < w)
+                erow[i] = 0.5 * (krow[i] + krow[i + 1]) * dt / s2;
+            else if (wrap)
+                erow[i] = 0.5 * (krow[i] + krow[0]) * dt / s2;
+            else
+                erow[i] = 0.0;
+        }
+        if (j + 1 < h) {
+            const double *below = kd + (j + 1) * w;
+            for (size_t i = 0; i < w; i++) srow[i] = 0.5 * (krow[i] + below[i]) * dt / s2;
+        } else {
+            for (size_t i = 0; i < w; i++) srow[i] = 0.0;
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (long jl = 0; jl < (long)h; jl++) {
+        size_t j = (size_t)jl, c0 = j * w;
+        for (size_t i = 0; i < w; i++) {
+            double west = i > 0 ? east[c0 + i - 1] : (wrap ? east[c0 + w - 1] : 0.0);
+            double north = j > 0 ? south[c0 - w + i] : 0.0;
+            diag[c0 + i] = 1.0 + east[c0 + i] + west + south[c0 + i] + north;
+        }
+    }
+}
+
+/* One cell of y = A x in the order of `Operator::apply_row`, D:609-649: diagonal, then the east
+ * neighbour (the wrapped one for the last cell), then the west neighbour (the wrapped one for the
+ * first cell), then south, then north, each product subtracted from the running value. */
+static inline double op_cell(size_t w, size_t h, int wrap, const double *east,
+                             const double *south, const double *diag, const double *x, size_t j,
+                             size_t i) {
+    size_t c = j * w + i, c0 = j * w;
+    double o = diag[c] * x[c];
+    if (i + 1 < w)
+        o -= east[c] * x[c + 1];
+    else if (wrap)
+        o -= east[c] * x[c0];
+    if (i > 0)
+        o -= east[c - 1] * x[c - 1];
+    else if (wrap)
+        o -= east[c0 + w - 1] * x[c0 + w - 1];
+    if (j + 1 < h) o -= south[c] * x[c + w];
+    if (j > 0) o -= south[c - w] * x[c - w];
+    return o;
+}
+
+/* `Operator::apply`, D:652-663. */
+void or_apply(size_t w, size_t h, int wrap_x, const double *east, const double *south,
+              const double *diag, const double *x, double *y) {
+    int wrap = wrap_x && w > 1;
+#pragma omp parallel for schedule(static)
+    for (long jl = 0; jl < (long)h; jl++)
+        for (size_t i = 0; i < w; i++)
+            y[(size_t)jl * w + i] = op_cell(w, h, wrap, east, south, diag, x, (size_t)jl, i);
+}
+
+/* `apply_reference`, MGT:17-43: (I + dt L) x from the definition, cell by cell. */
+void or_apply_reference(size_t w, size_t h, double dx, int wrap_x, const double *kd, double dt,
+                        const double *x, double *y) {
+    double s2 = dx * dx;
+    int wrap = wrap_x && w > 1;
+    for (size_t j = 0; j < h; j++) {
+        for (size_t i = 0; i < w; i++) {
+            size_t c = j * w + i;
+            long nb[4];
+            nb[0] = i + 1 < w ? (long)(c + 1) : (wrap ? (long)(j * w) : -1);
+            nb[1] = i > 0 ? (long)(c - 1) : (wrap ? (long)(j * w + w - 1) : -1);
+            nb[2] = j + 1 < h ? (long)(c + w) : -1;
+            nb[3] = j > 0 ? (long)(c - w) : -1;
+            double v = x[c];
+            for (int k = 0; k < 4; k++) {
+                if (nb[k] < 0) continue;
+                size_t m = (size_t)nb[k];
+                v += 0.5 * (kd[c] + kd[m]) * dt / s2 * (x[c] - x[m]);
+            }
+            y[c] = v;
+        }
+    }
+}
+
+/* ---- first guess --------------------------------------------------------------------------- */
+
+/* `thomas`, D:1029-1046. */
+void or_thomas(size_t n, const double *lo, const double *di, const double *up, double *b) {
+    if (n == 0) return;
+    double *c = (double *)calloc(n, sizeof(double));
+    double beta = di[0];
+    b[0] /= beta;
+    for (size_t i = 1; i < n; i++) {
+        c[i] = up[i - 1] / beta;
+        beta = di[i] - lo[i] * c[i];
+        b[i] = (b[i] - lo[i] * b[i - 1]) / beta;
+    }
+    for (size_t i = n - 1; i-- > 0;) {
+        double next = b[i + 1];
+        b[i] -= c[i + 1] * next;
+    }
+    free(c);
+}
+
+/* `cyclic`, D:1050-1071: Sherman-Morrison over two Thomas solves; `alpha` couples row n-1 to
+ * column 0, `beta` row 0 to column n-1. */
+void or_cyclic(size_t n, const double *lo, const double *di, const double *up, double alpha,
+               double beta, double *b) {
+    double gamma = -di[0];
+    double *bb = (double *)malloc(n * sizeof(double));
+    double *l = (double *)malloc(n * sizeof(double));
+    double *u = (double *)malloc(n * sizeof(double));
+    double *x = (double *)malloc(n * sizeof(double));
+    double *z = (double *)calloc(n, sizeof(double));
+    memcpy(bb, di, n * sizeof(double));
+    bb[0] = di[0] - gamma;
+    bb[n - 1] = di[n - 1] - alpha * beta / gamma;
+    memcpy(l, lo, n * sizeof(double));
+    memcpy(u, up, n * sizeof(double));
+    l[0] = 0.0;
+    u[n - 1] = 0.0;
+    memcpy(x, b, n * sizeof(double));
+    or_thomas(n, l, bb, u, x);
+    z[0] = gamma;
+    z[n - 1] = alpha;
+    or_thomas(n, l, bb, u, z);
+    double fact = (x[0] + beta * x[n - 1] / gamma) / (1.0 + z[0] + beta * z[n - 1] / gamma);
+    for (size_t i = 0; i < n; i++) b[i] = x[i] - fact * z[i];
+    free(bb);
+    free(l);
+    free(u);
+    free(x);
+    free(z);
+}
+
+/* `lod`, D:370-489 with every row visited: an implicit sweep along x (a cyclic tridiagonal
+ * system per row when the grid wraps and is wider than two cells), then along the columns. */
+void or_lod(size_t w, size_t h, double dx, int wrap_x, double *hgt, const double *kd, double dt) {
+    double s2 = dx * dx;
+    if (w == 0 || h == 0) return;
+#pragma omp parallel
+    {
+        double *lo = (double *)malloc(w * sizeof(double));
+        double *di = (double *)malloc(w * sizeof(double));
+        double *up = (double *)malloc(w * sizeof(double));
+#pragma omp for schedule(static)
+        for (long jl = 0; jl < (long)h; jl++) {
+            size_t j = (size_t)jl;
+            const double *krow = kd + j * w;
+            double *row = hgt + j * w;
+#define FACE(i)                                                                                \
+    ((i) + 1 < w ? 0.5 * (krow[(i)] + krow[(i) + 1]) * dt / s2                                  \
+                 : ((wrap_x && w > 1) ? 0.5 * (krow[w - 1] + krow[0]) * dt / s2 : 0.0))
+            for (size_t i = 0; i < w; i++) {
+                double west = i == 0 ? FACE(w - 1) : FACE(i - 1);
+                double east = FACE(i);
+                lo[i] = -west;
+                up[i] = -east;
+                di[i] = 1.0 + west + east;
+            }
+            if (wrap_x && w > 2) {
+                double corner_lo = lo[0], corner_up = up[w - 1];
+                or_cyclic(w, lo, di, up, corner_up, corner_lo, row);
+            } else {
+                if (w == 2 && wrap_x) {
+                    double f = FACE(0) + FACE(1);
+                    lo[1] = -f;
+                    up[0] = -f;
+                    di[0] = 1.0 + f;
+                    di[1] = 1.0 + f;
+                }
+                lo[0] = 0.0;
+                up[w - 1] = 0.0;
+                if (w == 1) di[0] = 1.0;
+                or_thomas(w, lo, di, up, row);
+            }
+#undef FACE
+        }
+        free(lo);
+        free(di);
+        free(up);
+    }
+#pragma omp parallel
+    {
+        double *col = (double *)malloc(h * sizeof(double));
+        double *lo = (double *)malloc(h * sizeof(double));
+        double *di = (double *)malloc(h * sizeof(double));
+        double *up = (double *)malloc(h * sizeof(double));
+#pragma omp for schedule(static)
+        for (long il = 0; il < (long)w; il++) {
+            size_t i = (size_t)il;
+#define FACEZ(j) ((j) + 1 < h ? 0.5 * (kd[(j) * w + i] + kd[((j) + 1) * w + i]) * dt / s2 : 0.0)
+            for (size_t j = 0; j < h; j++) {
+                col[j] = hgt[j * w + i];
+                double north = j == 0 ? 0.0 : FACEZ(j - 1);
+                double south = FACEZ(j);
+                lo[j] = -north;
+                up[j] = -south;
+                di[j] = 1.0 + north + south;
+            }
+#undef FACEZ
+            or_thomas(h, lo, di, up, col);
+            for (size_t j = 0; j < h; j++) hgt[j * w + i] = col[j];
+        }
+        free(col);
+        free(lo);
+        free(di);
+        free(up);
+    }
+}
+
+/* ---- multigrid cycle (MG:73-523) ----------------------------------------------------------- */
+
+typedef struct {
+    size_t w, h;
+    int wrap;
+    float *east, *south, *diag, *wd, *mass; /* mass == NULL on the finest level (1 a cell) */
+    float *r, *z, *t, *zp;
+} lvl;
+
+struct or_hier {
+    size_t n;
+    lvl *lv;
+    float omega;
+    size_t nu;
+    float scale;
+};
+
+static float *fzalloc(size_t n) { return (float *)calloc(n ? n : 1, sizeof(float)); }
+
+/* `Level::finest`, MG:96-114: f32 copies of the operator, wd = omega / diag. */
+static void level_finest(lvl *L, size_t w, size_t h, int wrap, const double *east,
+                         const double *south, const double *diag, float omega) {
+    size_t n = w * h;
+    L->w = w;
+    L->h = h;
+    L->wrap = wrap;
+    L->east = fzalloc(n);
+    L->south = fzalloc(n);
+    L->diag = fzalloc(n);
+    L->wd = fzalloc(n);
+    L->mass = NULL;
+    L->r = fzalloc(n);
+    L->z = fzalloc(n);
+    L->t = fzalloc(n);
+    L->zp = fzalloc(n);
+    for (size_t c = 0; c < n; c++) {
+        L->east[c] = (float)east[c];
+        L->south[c] = (float)south[c];
+        L->diag[c] = (float)diag[c];
+        L->wd[c] = omega / L->diag[c];
+    }
+}
+
+/* `Level::coarsen`, MG:117-199: 2x2 aggregation (a lone last row/column when odd); mass summed,
+ * coarse faces = scale * the fine faces across them; returns 0 when this level is the coarsest. */
+static int level_coarsen(const lvl *F, lvl *C, float omega, float scale) {
+    size_t w = F->w, h = F->h;
+    size_t wc = div_ceil(w, 2), hc = div_ceil(h, 2);
+    if (w * h <= COARSEST || (wc == w && hc == h)) return 0;
+    size_t nc = wc * hc;
+    int wrap = F->wrap && wc > 1;
+    C->w = wc;
+    C->h = hc;
+    C->wrap = wrap;
+    C->east = fzalloc(nc);
+    C->south = fzalloc(nc);
+    C->diag = fzalloc(nc);
+    C->wd = fzalloc(nc);
+    C->mass = fzalloc(nc);
+    C->r = fzalloc(nc);
+    C->z = fzalloc(nc);
+    C->t = fzalloc(nc);
+    C->zp = fzalloc(nc);
+    for (size_t jc = 0; jc < hc; jc++) {
+        size_t r0 = 2 * jc, r1 = 2 * jc + 2 < h ? 2 * jc + 2 : h;
+        size_t last_row = r1 - 1;
+        for (size_t ic = 0; ic < wc; ic++) {
+            size_t c0 = 2 * ic, c1 = 2 * ic + 2 < w ? 2 * ic + 2 : w;
+            size_t last_col = c1 - 1;
+            float m = 0.0f;
+            for (size_t j = r0; j < r1; j++)
+                for (size_t i = c0; i < c1; i++) m += F->mass ? F->mass[j * w + i] : 1.0f;
+            C->mass[jc * wc + ic] = m;
+            if (ic + 1 < wc || wrap) {
+                float e = 0.0f;
+                for (size_t j = r0; j < r1; j++) e += F->east[j * w + last_col];
+                C->east[jc * wc + ic] = scale * e;
+            }
+            if (jc + 1 < hc) {
+                float s = 0.0f;
+                for (size_t i = c0; i < c1; i++) s += F->south[last_row * w + i];
+                C->south[jc * wc + ic] = scale * s;
+            }
+        }
+    }
+    for (size_t j = 0; j < hc; j++) {
+        size_t c0 = j * wc;
+        for (size_t i = 0; i < wc; i++) {
+            float west = i > 0 ? C->east[c0 + i - 1] : (wrap ? C->east[c0 + wc - 1] : 0.0f);
+            float north = j > 0 ? C->south[c0 - wc + i] : 0.0f;
+            C->diag[c0 + i] = C->mass[c0 + i] + C->east[c0 + i] + west + C->south[c0 + i] + north;
+        }
+    }
+    for (size_t c = 0; c < nc; c++) C->wd[c] = omega / C->diag[c];
+    return 1;
+}
+
+/* One cell of A x on a level, in the order of `Level::apply_row`, MG:203-240: diagonal, east
+ * (non-wrapped), west (non-wrapped), then the wrapped east term of the last cell and the wrapped
+ * west term of the first, then south, then north. */
+static inline float lv_cell(const lvl *L, const float *x, size_t j, size_t i) {
+    size_t w = L->w, c = j * w + i, c0 = j * w;
+    float o = L->diag[c] * x[c];
+    if (i + 1 < w) o -= L->east[c] * x[c + 1];
+    if (i > 0) o -= L->east[c - 1] * x[c - 1];
+    if (L->wrap) {
+        if (i == w - 1) o -= L->east[c0 + w - 1] * x[c0];
+        if (i == 0) o -= L->east[c0 + w - 1] * x[c0 + w - 1];
+    }
+    if (j + 1 < L->h) o -= L->south[c] * x[c + w];
+    if (j > 0) o -= L->south[c - w] * x[c - w];
+    return o;
+}
+
+/* `Level::first_sweep`, MG:243-256: z = wd * r. */
+static void first_sweep(lvl *L) {
+    size_t n = L->w * L->h;
+#pragma omp parallel for schedule(static) if (n > 65536)
+    for (long c = 0; c < (long)n; c++) L->z[c] = L->wd[c] * L->r[c];
+}
+
+/* `Level::sweep`, MG:260-329: z <- z' + wd (r - A z') with z' = z + P zc when `zc` is given
+ * (piecewise-constant prolongation); with `dot`, returns sum dot * z_new, summed per strip of
+ * STRIP_ROWS rows in row-major order and the strips added in order. */
+static double sweep(lvl *L, const float *zc, size_t wc, const double *dot) {
+    size_t w = L->w, h = L->h, n = w * h;
+    const float *zp = L->z;
+    if (zc) {
+        for (size_t j = 0; j < h; j++)
+            for (size_t i = 0; i < w; i++)
+                L->zp[j * w + i] = L->z[j * w + i] + zc[(j / 2) * wc + i / 2];
+        zp = L->zp;
+    }
+    size_t ns = div_ceil(h, STRIP_ROWS);
+    double *acc = (double *)malloc((ns ? ns : 1) * sizeof(double));
+#pragma omp parallel for schedule(static) if (n > 65536)
+    for (long sl = 0; sl < (long)ns; sl++) {
+        size_t s = (size_t)sl, j0 = s * STRIP_ROWS;
+        size_t j1 = j0 + STRIP_ROWS < h ? j0 + STRIP_ROWS : h;
+        double a = sum_zero();
+        for (size_t j = j0; j < j1; j++) {
+            for (size_t i = 0; i < w; i++) {
+                size_t c = j * w + i;
+                float ax = lv_cell(L, zp, j, i);
+                L->t[c] = zp[c] + L->wd[c] * (L->r[c] - ax);
+            }
+            if (dot)
+                for (size_t i = 0; i < w; i++) a += dot[j * w + i] * (double)L->t[j * w + i];
+        }
+        acc[s] = a;
+    }
+    double total = sum_zero();
+    for (size_t s = 0; s < ns; s++) total += acc[s];
+    free(acc);
+    float *tmp = L->z;
+    L->z = L->t;
+    L->t = tmp;
+    return total;
+}
+
+/* `Level::restrict_into`, MG:332-356: rc = sum over the 2x2 block of (r - A z), fine row j0 then
+ * j0 + 1, columns ascending, from +0. */
+static void restrict_into(const lvl *F, lvl *C) {
+    size_t w = F->w, h = F->h, wc = C->w, hc = C->h;
+#pragma omp parallel for schedule(static) if (w * h > 65536)
+    for (long jcl = 0; jcl < (long)hc; jcl++) {
+        size_t jc = (size_t)jcl;
+        float *rc = C->r + jc * wc;
+        for (size_t ic = 0; ic < wc; ic++) rc[ic] = 0.0f;
+        size_t j1 = 2 * jc + 2 < h ? 2 * jc + 2 : h;
+        for (size_t j = 2 * jc; j < j1; j++)
+            for (size_t i = 0; i < w; i++) rc[i / 2] += F->r[j * w + i] - lv_cell(F, F->z, j, i);
+    }
+}
+
+/* `vcycle`, MG:500-523: from z = 0; on the coarsest level a first sweep and COARSE_SWEEPS - 1
+ * sweeps; otherwise nu sweeps, restriction, the coarse cycle, the corrected sweep and nu - 1
+ * sweeps.  With `dot`, returns sum dot * z of the last sweep. */
+static double vcycle(or_hier *hy, size_t l, const double *dot) {
+    lvl *L = &hy->lv[l];
+    if (l + 1 == hy->n) {
+        first_sweep(L);
+        double acc = 0.0;
+        size_t ks = COARSE_SWEEPS > 2 ? COARSE_SWEEPS : 2;
+        for (size_t k = 1; k < ks; k++) acc = sweep(L, NULL, 0, k + 1 == ks ? dot : NULL);
+        return acc;
+    }
+    first_sweep(L);
+    for (size_t k = 1; k < hy->nu; k++) sweep(L, NULL, 0, NULL);
+    restrict_into(L, &hy->lv[l + 1]);
+    vcycle(hy, l + 1, NULL);
+    size_t wc = hy->lv[l + 1].w;
+    double acc = sweep(L, hy->lv[l + 1].z, wc, hy->nu == 1 ? dot : NULL);
+    for (size_t k = 1; k < hy->nu; k++) acc = sweep(L, NULL, 0, k + 1 == hy->nu ? dot : NULL);
+    return acc;
+}
+
+/* `Hierarchy::new`, MG:481-487. */
+or_hier *or_hier_new(size_t w, size_t h, int wrap_x, const double *east, const double *south,
+                     const double *diag, float omega, size_t nu, float scale) {
+    or_hier *hy = (or_hier *)calloc(1, sizeof(or_hier));
+    size_t cap = 4;
+    hy->lv = (lvl *)calloc(cap, sizeof(lvl));
+    hy->omega = omega;
+    hy->nu = nu;
+    hy->scale = scale;
+    level_finest(&hy->lv[0], w, h, wrap_x && w > 1, east, south, diag, omega);
+    hy->n = 1;
+    for (;;) {
+        if (hy->n == cap) {
+            cap *= 2;
+            hy->lv = (lvl *)realloc(hy->lv, cap * sizeof(lvl));
+        }
+        lvl next;
+        memset(&next, 0, sizeof next);
+        if (!level_coarsen(&hy->lv[hy->n - 1], &next, omega, scale)) break;
+        hy->lv[hy->n++] = next;
+    }
+    return hy;
+}
+
+void or_hier_free(or_hier *hy) {
+    if (!hy) return;
+    for (size_t l = 0; l < hy->n; l++) {
+        lvl *L = &hy->lv[l];
+        free(L->east);
+        free(L->south);
+        free(L->diag);
+        free(L->wd);
+        free(L->mass);
+        free(L->r);
+        free(L->z);
+        free(L->t);
+        free(L->zp);
+    }
+    free(hy->lv);
+    free(hy);
+}
+
+size_t or_hier_levels(const or_hier *hy) { return hy->n; }
+
+void or_hier_level(const or_hier *hy, size_t l, size_t *w, size_t *h, int *wrap, float *east,
+                   float *south, float *diag, float *wd, float *mass) {
+    const lvl *L = &hy->lv[l];
+    size_t n = L->w * L->h;
+    if (w) *w = L->w;
+    if (h) *h = L->h;
+    if (wrap) *wrap = L->wrap;
+    if (east) memcpy(east, L->east, n * sizeof(float));
+    if (south) memcpy(south, L->south, n * sizeof(float));
+    if (diag) memcpy(diag, L->diag, n * sizeof(float));
+    if (wd) memcpy(wd, L->wd, n * sizeof(float));
+    if (mass) {
+        if (L->mass)
+            memcpy(mass, L->mass, n * sizeof(float));
+        else
+            for (size_t c = 0; c < n; c++) mass[c] = 1.0f;
+    }
+}
+
+/* `Hierarchy::precondition`, MG:489-492. */
+double or_hier_precondition(or_hier *hy, const float *r32, const double *r64, float *z) {
+    size_t n = hy->lv[0].w * hy->lv[0].h;
+    memcpy(hy->lv[0].r, r32, n * sizeof(float));
+    double rz = vcycle(hy, 0, r64);
+    if (z) memcpy(z, hy->lv[0].z, n * sizeof(float));
+    return rz;
+}
+
+/* ---- solves -------------------------------------------------------------------------------- */
+
+/* The residual's sum put back in proportion to each cell's coupling, MG:737-761: per row the
+ * sums of (b - A x) and (diag - 1) from -0.0 in column order, rows folded in order. */
+static void put_back(size_t w, size_t h, int wrap, const double *east, const double *south,
+                     const double *diag, const double *b, double *x, double *ap) {
+    or_apply(w, h, wrap, east, south, diag, x, ap);
+    double lost = sum_zero(), weight = sum_zero();
+    for (size_t j = 0; j < h; j++) {
+        double l = sum_zero(), g = sum_zero();
+        for (size_t i = 0; i < w; i++) {
+            size_t c = j * w + i;
+            l += b[c] - ap[c];
+            g += diag[c] - 1.0;
+        }
+        lost = lost + l;
+        weight = weight + g;
+    }
+    if (weight > 0.0 && lost != 0.0) {
+        double k = lost / weight;
+        for (size_t c = 0; c < w * h; c++) x[c] += k * (diag[c] - 1.0);
+    }
+}
+
+/* `cg_on`, MG:562-763: f64 conjugate gradients preconditioned by one f32 V-cycle to an RMS
+ * residual of tol_m, then the mass put-back.  Returns the iterations.  Reproduces the reference's
+ * loop exactly, including that the pending x += alpha p is not cleared by the direction pass
+ * (MG:650-657), so a break on p.A.p <= 0 applies it a second time after the loop (MG:731-735). */
+size_t or_cg_on(size_t w, size_t h, double dx, int wrap_x, double *x, const double *b,
+                const double *kd, double dt, double tol_m, float omega, size_t nu, float scale) {
+    size_t n = w * h;
+    if (n == 0) return 0;
+    int wrap = wrap_x && w > 1;
+    double *east = (double *)malloc(n * sizeof(double));
+    double *south = (double *)malloc(n * sizeof(double));
+    double *diag = (double *)malloc(n * sizeof(double));
+    double *r = (double *)malloc(n * sizeof(double));
+    double *ap = (double *)malloc(n * sizeof(double));
+    double *pold = (double *)calloc(n, sizeof(double));
+    double *pnew = (double *)calloc(n, sizeof(double));
+    or_operator(w, h, dx, wrap_x, kd, dt, east, south, diag);
+    or_hier *hy = or_hier_new(w, h, wrap_x, east, south, diag, omega, nu, scale);
+    float *r32 = hy->lv[0].r;
+    size_t ns = div_ceil(h, STRIP_ROWS);
+    double *acc = (double *)malloc(ns * sizeof(double));
+
+    /* r = b - A x, its f32 copy, r.r (MG:596-619). */
+#pragma omp parallel for schedule(static)
+    for (long sl = 0; sl < (long)ns; sl++) {
+        size_t j0 = (size_t)sl * STRIP_ROWS, j1 = j0 + STRIP_ROWS < h ? j0 + STRIP_ROWS : h;
+        double a = sum_zero();
+        for (size_t j = j0; j < j1; j++)
+            for (size_t i = 0; i < w; i++) {
+                size_t c = j * w + i;
+                double rv = b[c] - op_cell(w, h, wrap, east, south, diag, x, j, i);
+                r[c] = rv;
+                r32[c] = (float)rv;
+                a += rv * rv;
+            }
+        acc[sl] = a;
+    }
+    double rr0 = sum_zero();
+    for (size_t s = 0; s < ns; s++) rr0 += acc[s];
+    double tol2 = tol_m * tol_m * (double)n;
+    size_t iterations = 0;
+    if (rr0 > tol2) {
+        double rz = vcycle(hy, 0, r);
+        int have_beta = 0, have_pending = 0;
+        double beta = 0.0, alpha = 0.0;
+        while (iterations < CG_MAX_ITER) {
+            iterations++;
+            const float *z = hy->lv[0].z;
+            /* p = z + beta p_old, A p, p.A p, and the pending x += alpha p_old (MG:633-695). */
+#pragma omp parallel for schedule(static)
+            for (long c = 0; c < (long)n; c++) {
+                if (have_pending) x[c] += alpha * pold[c];
+                pnew[c] = have_beta ? (double)z[c] + beta * pold[c] : (double)z[c];
+            }
+#pragma omp parallel for schedule(static)
+            for (long sl = 0; sl < (long)ns; sl++) {
+                size_t j0 = (size_t)sl * STRIP_ROWS;
+                size_t j1 = j0 + STRIP_ROWS < h ? j0 + STRIP_ROWS : h;
+                double a = sum_zero();
+                for (size_t j = j0; j < j1; j++)
+                    for (size_t i = 0; i < w; i++) {
+                        size_t c = j * w + i;
+                        double av = op_cell(w, h, wrap, east, south, diag, pnew, j, i);
+                        ap[c] = av;
+                        a += pnew[c] * av;
+                    }
+                acc[sl] = a;
+            }
+            double pap = sum_zero();
+            for (size_t s = 0; s < ns; s++) pap += acc[s];
+            double *tp = pold;
+            pold = pnew;
+            pnew = tp;
+            if (pap <= 0.0 || !isfinite(pap)) break;
+            alpha = rz / pap;
+            have_pending = 1;
+            /* r -= alpha A p, its f32 copy, r.r (MG:703-723). */
+#pragma omp parallel for schedule(static)
+            for (long sl = 0; sl < (long)ns; sl++) {
+                size_t c0 = (size_t)sl * STRIP_ROWS * w;
+                size_t c1 = c0 + STRIP_ROWS * w < n ? c0 + STRIP_ROWS * w : n;
+                double a = sum_zero();
+                for (size_t c = c0; c < c1; c++) {
+                    r[c] -= alpha * ap[c];
+                    r32[c] = (float)r[c];
+                    a += r[c] * r[c];
+                }
+                acc[sl] = a;
+            }
+            double rr = sum_zero();
+            for (size_t s = 0; s < ns; s++) rr += acc[s];
+            if (rr <= tol2) break;
+            double rz_new = vcycle(hy, 0, r);
+            if (!(isfinite(rz_new) && rz > 0.0)) break;
+            beta = rz_new / rz;
+            have_beta = 1;
+            rz = rz_new;
+        }
+        if (have_pending)
+            for (size_t c = 0; c < n; c++) x[c] += alpha * pold[c];
+    }
+    put_back(w, h, wrap, east, south, diag, b, x, ap);
+    or_hier_free(hy);
+    free(acc);
+    free(east);
+    free(south);
+    free(diag);
+    free(r);
+    free(ap);
+    free(pold);
+    free(pnew);
+    return iterations;
+}
+
+/* `solve` with the multigrid solver, D:77-116 (`multigrid_in`): the right-hand side is h, the
+ * first guess the splitting, then the preconditioned conjugate gradients. */
+size_t or_solve(size_t w, size_t h, double dx, int wrap_x, double *hgt, const double *kd,
+                double dt, double tol_m) {
+    size_t n = w * h;
+    if (n == 0) return 0;
+    double *b = (double *)malloc(n * sizeof(double));
+    memcpy(b, hgt, n * sizeof(double));
+    or_lod(w, h, dx, wrap_x, hgt, kd, dt);
+    size_t its = or_cg_on(w, h, dx, wrap_x, hgt, b, kd, dt, tol_m, OMEGA, NU, SCALE);
+    free(b);
+    return its;
+}
+
+/* The Jacobi-preconditioned conjugate gradients `cg`, D:927-1026, every row visited: per-row
+ * sums (ascending columns from -0.0), rows added in order; z = r / diag formed where used;
+ * x += alpha p deferred to the next direction pass (taken once there). */
+size_t or_jacobi_cg(size_t w, size_t h, double dx, int wrap_x, double *x, const double *b,
+                    const double *kd, double dt, double tol_m) {
+    size_t n = w * h;
+    if (n == 0) return 0;
+    int wrap = wrap_x && w > 1;
+    double *east = (double *)malloc(n * sizeof(double));
+    double *south = (double *)malloc(n * sizeof(double));
+    double *diag = (double *)malloc(n * sizeof(double));
+    double *r = (double *)malloc(n * sizeof(double));
+    double *ap = (double *)malloc(n * sizeof(double));
+    double *pold = (double *)calloc(n, sizeof(double));
+    double *pnew = (double *)calloc(n, sizeof(double));
+    double *s0 = (double *)malloc(h * sizeof(double));
+    double *s1 = (double *)malloc(h * sizeof(double));
+    or_operator(w, h, dx, wrap_x, kd, dt, east, south, diag);
+    for (size_t j = 0; j < h; j++) {
+        double a = sum_zero(), bsum = sum_zero();
+        for (size_t i = 0; i < w; i++) {
+            size_t c = j * w + i;
+            double rv = b[c] - op_cell(w, h, wrap, east, south, diag, x, j, i);
+            r[c] = rv;
+            a += rv * rv;
+            bsum += rv * (rv / diag[c]);
+        }
+        s0[j] = a;
+        s1[j] = bsum;
+    }
+    double tol2 = tol_m * tol_m * (double)n;
+    size_t iterations = 0;
+    double rr0 = sum_zero();
+    for (size_t j = 0; j < h; j++) rr0 += s0[j];
+    if (rr0 > tol2) {
+        double rz = sum_zero();
+        for (size_t j = 0; j < h; j++) rz += s1[j];
+        int have_beta = 0, pending = 0;
+        double beta = 0.0, alpha = 0.0;
+        while (iterations < CG_MAX_ITER) {
+            iterations++;
+            for (size_t c = 0; c < n; c++) {
+                if (pending) x[c] += alpha * pold[c];
+                pnew[c] = have_beta ? r[c] / diag[c] + beta * pold[c] : r[c] / diag[c];
+            }
+            pending = 0;
+            double pap = sum_zero();
+            for (size_t j = 0; j < h; j++) {
+                double a = sum_zero();
+                for (size_t i = 0; i < w; i++) {
+                    size_t c = j * w + i;
+                    ap[c] = op_cell(w, h, wrap, east, south, diag, pnew, j, i);
+                    a += pnew[c] * ap[c];
+                }
+                pap += a;
+            }
+            double *tp = pold;
+            pold = pnew;
+            pnew = tp;
+            if (pap <= 0.0) break;
+            alpha = rz / pap;
+            for (size_t j = 0; j < h; j++) {
+                double a = sum_zero(), bsum = sum_zero();
+                for (size_t i = 0; i < w; i++) {
+                    size_t c = j * w + i;
+                    r[c] -= alpha * ap[c];
+                    a += r[c] * r[c];
+                    bsum += r[c] * (r[c] / diag[c]);
+                }
+                s0[j] = a;
+                s1[j] = bsum;
+            }
+            pending = 1;
+            double rr = sum_zero();
+            for (size_t j = 0; j < h; j++) rr += s0[j];
+            if (rr <= tol2) break;
+            double rz_new = sum_zero();
+            for (size_t j = 0; j < h; j++) rz_new += s1[j];
+            beta = rz_new / rz;
+            have_beta = 1;
+            rz = rz_new;
+        }
+        if (pending)
+            for (size_t c = 0; c < n; c++) x[c] += alpha * pold[c];
+    }
+    /* Put-back, D:1000-1023: one running sum over the cells in order. */
+    or_apply(w, h, wrap_x, east, south, diag, x, ap);
+    double lost = sum_zero(), weight = sum_zero();
+    for (size_t c = 0; c < n; c++) lost += b[c] - ap[c];
+    for (size_t c = 0; c < n; c++) weight += diag[c] - 1.0;
+    if (weight > 0.0 && lost != 0.0) {
+        double k = lost / weight;
+        for (size_t c = 0; c < n; c++) x[c] += k * (diag[c] - 1.0);
+    }
+    free(east);
+    free(south);
+    free(diag);
+    free(r);
+    free(ap);
+    free(pold);
+    free(pnew);
+    free(s0);
+    free(s1);
+    return iterations;
+}
+
+/* `reference`, MGT:46-74: unpreconditioned CG on the definitional operator to a relative
+ * residual of 1e-13. */
+size_t or_reference_cg(size_t w, size_t h, double dx, int wrap_x, const double *b,
+                       const double *kd, double dt, double *x) {
+    size_t n = w * h, its = 0;
+    double *ax = (double *)malloc((n ? n : 1) * sizeof(double));
+    double *r = (double *)malloc((n ? n : 1) * sizeof(double));
+    double *p = (double *)malloc((n ? n : 1) * sizeof(double));
+    memcpy(x, b, n * sizeof(double));
+    or_apply_reference(w, h, dx, wrap_x, kd, dt, x, ax);
+    double rr = sum_zero(), bb = sum_zero();
+    for (size_t k = 0; k < n; k++) {
+        r[k] = b[k] - ax[k];
+        p[k] = r[k];
+        rr += r[k] * r[k];
+        bb += b[k] * b[k];
+    }
+    if (bb < 1.0) bb = 1.0;
+    for (size_t it = 0; it < 20 * n + 100; it++) {
+        if (rr <= 1e-26 * bb) break;
+        its++;
+        or_apply_reference(w, h, dx, wrap_x, kd, dt, p, ax);
+        double pap = sum_zero();
+        for (size_t k = 0; k < n; k++) pap += p[k] * ax[k];
+        double alpha = rr / pap;
+        for (size_t k = 0; k < n; k++) {
+            x[k] += alpha * p[k];
+            r[k] -= alpha * ax[k];
+        }
+        double rr_new = sum_zero();
+        for (size_t k = 0; k < n; k++) rr_new += r[k] * r[k];
+        double beta = rr_new / rr;
+        rr = rr_new;
+        for (size_t k = 0; k < n; k++) p[k] = r[k] + beta * p[k];
+    }
+    free(ax);
+    free(r);
+    free(p);
+    return its;
+}
+
+/* `bound`, MGT:157-164: the RMS error a solve stopped at RMS residual `tol` may have. */
+double or_bound(size_t w, size_t h, double dx, int wrap_x, const double *kd, double dt,
+                double tol) {
+    size_t n = w * h;
+    double *e = (double *)malloc(n * sizeof(double));
+    double *s = (double *)malloc(n * sizeof(double));
+    double *g = (double *)malloc(n * sizeof(double));
+    or_operator(w, h, dx, wrap_x, kd, dt, e, s, g);
+    double m = sum_zero(), q = sum_zero();
+    for (size_t c = 0; c < n; c++) m += g[c] - 1.0;
+    for (size_t c = 0; c < n; c++) q += (g[c] - 1.0) * (g[c] - 1.0);
+    double mean = m / (double)n, rms = sqrt(q / (double)n);
+    free(e);
+    free(s);
+    free(g);
+    return tol * (1.0 + (mean > 0.0 ? rms / mean : 0.0)) * 1.01;
+}
+
+/* ||b - A x||_2 / ||b||_2 in f64 (the north-star convergence measure; not a reference function). */
+double or_rel_residual(size_t w, size_t h, double dx, int wrap_x, const double *kd, double dt,
+                       const double *b, const double *x) {
+    size_t n = w * h;
+    double *e = (double *)malloc(n * sizeof(double));
+    double *s = (double *)malloc(n * sizeof(double));
+    double *g = (double *)malloc(n * sizeof(double));
+    double *y = (double *)malloc(n * sizeof(double));
+    or_operator(w, h, dx, wrap_x, kd, dt, e, s, g);
+    or_apply(w, h, wrap_x, e, s, g, x, y);
+    double rr = 0.0, bb = 0.0;
+    for (size_t c = 0; c < n; c++) {
+        double d = b[c] - y[c];
+        rr += d * d;
+        bb += b[c] * b[c];
+    }
+    free(e);
+    free(s);
+    free(g);
+    free(y);
+    return bb > 0.0 ? sqrt(rr / bb) : sqrt(rr);
+}

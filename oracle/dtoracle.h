@@ -1,0 +1,85 @@
+/*
+ * CPU oracle for the implicit diffusion solve: TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of deep-time's CPU solve of (I + dt*L(K)) h1 = h0 on the x-periodic,
+ * row-closed raster: the five-point operator, the locally one-dimensional first guess (Thomas
+ * and Sherman-Morrison cyclic solves), the f32 multigrid V-cycle (aggregation levels, damped
+ * Jacobi) and the f64 conjugate gradients it preconditions, with the residual's sum put back.
+ * Each function cites the reference file:line it restates (paths relative to the reference
+ * repository root).  Arithmetic follows the reference cell by cell and operation by operation
+ * (built with -ffp-contract=off, no fast-math), and every reduction keeps the reference order:
+ * a sequential sum per strip of 8 rows (or per row where the reference sums per row), then the
+ * partial sums added in ascending order from -0.0.  OpenMP only schedules strips; no result
+ * depends on the thread count.
+ *
+ * Only the test suite, the smoke check and the benchmark's CPU-baseline leg may load this
+ * library.  The product (the HIP library) never calls it.
+ *
+ * Parity pins (tests/test_oracle.py): the closed-form eigen-solution for constant K, dense
+ * Cholesky on small grids, the reference's own definitional oracles (apply_reference, plain CG,
+ * bound), its known-answer ring test, the SplitMix64 published vectors, and invariants (A*1 = 1,
+ * symmetry, mass, an SPD cycle).  Functions without such a pin are marked "parity unpinned".
+ */
+#ifndef DTORACLE_H
+#define DTORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- fixtures ------------------------------------------------------------------------------ */
+uint64_t or_splitmix_next(uint64_t *state);
+double or_splitmix_f64(uint64_t *state);
+void or_terrain(size_t w, size_t h, uint64_t seed, double *z);
+double or_submarine_kd(double z, double d0, double wave_base, double tidal_range, double land);
+void or_shelf_kd(size_t n, const double *z, uint64_t seed, double *kd);
+void or_fractal(size_t w, size_t h, uint64_t seed, double *z);
+
+/* ---- operator ------------------------------------------------------------------------------ */
+void or_operator(size_t w, size_t h, double dx, int wrap_x, const double *kd, double dt,
+                 double *east, double *south, double *diag);
+void or_apply(size_t w, size_t h, int wrap_x, const double *east, const double *south,
+              const double *diag, const double *x, double *y);
+void or_apply_reference(size_t w, size_t h, double dx, int wrap_x, const double *kd, double dt,
+                        const double *x, double *y);
+
+/* ---- first guess --------------------------------------------------------------------------- */
+void or_thomas(size_t n, const double *lo, const double *di, const double *up, double *b);
+void or_cyclic(size_t n, const double *lo, const double *di, const double *up, double alpha,
+               double beta, double *b);
+void or_lod(size_t w, size_t h, double dx, int wrap_x, double *hgt, const double *kd, double dt);
+
+/* ---- multigrid cycle ----------------------------------------------------------------------- */
+typedef struct or_hier or_hier;
+or_hier *or_hier_new(size_t w, size_t h, int wrap, const double *east, const double *south,
+                     const double *diag, float omega, size_t nu, float scale);
+void or_hier_free(or_hier *hy);
+size_t or_hier_levels(const or_hier *hy);
+/* Level l's shape and its operator arrays (NULL pointers are skipped; mass is 1 on level 0). */
+void or_hier_level(const or_hier *hy, size_t l, size_t *w, size_t *h, int *wrap, float *east,
+                   float *south, float *diag, float *wd, float *mass);
+/* z = M^-1 r for r32 on the finest level; returns sum r64 * z (reference order). */
+double or_hier_precondition(or_hier *hy, const float *r32, const double *r64, float *z);
+
+/* ---- solves -------------------------------------------------------------------------------- */
+size_t or_cg_on(size_t w, size_t h, double dx, int wrap_x, double *x, const double *b,
+                const double *kd, double dt, double tol_m, float omega, size_t nu, float scale);
+size_t or_solve(size_t w, size_t h, double dx, int wrap_x, double *hgt, const double *kd,
+                double dt, double tol_m);
+size_t or_jacobi_cg(size_t w, size_t h, double dx, int wrap_x, double *x, const double *b,
+                    const double *kd, double dt, double tol_m);
+size_t or_reference_cg(size_t w, size_t h, double dx, int wrap_x, const double *b,
+                       const double *kd, double dt, double *x);
+double or_bound(size_t w, size_t h, double dx, int wrap_x, const double *kd, double dt,
+                double tol);
+double or_rel_residual(size_t w, size_t h, double dx, int wrap_x, const double *kd, double dt,
+                       const double *b, const double *x);
+int or_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

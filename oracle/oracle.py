@@ -1,0 +1,264 @@
+"""ctypes wrapper of the CPU oracle (oracle/dtoracle.c). TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py`` load this
+module, and only as the checker. The product (the HIP library behind include/dtdiff.h) never
+calls it. Each wrapper names the reference function it restates; dtoracle.c cites file:line.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "libdtoracle.so")
+
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_sz = ctypes.c_size_t
+_lib = None
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = ctypes.CDLL(LIB_PATH)
+    L.or_splitmix_next.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
+    L.or_splitmix_next.restype = ctypes.c_uint64
+    L.or_splitmix_f64.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
+    L.or_splitmix_f64.restype = ctypes.c_double
+    L.or_terrain.argtypes = [_sz, _sz, ctypes.c_uint64, _f64p]
+    L.or_fractal.argtypes = [_sz, _sz, ctypes.c_uint64, _f64p]
+    L.or_submarine_kd.argtypes = [ctypes.c_double] * 5
+    L.or_submarine_kd.restype = ctypes.c_double
+    L.or_shelf_kd.argtypes = [_sz, _f64p, ctypes.c_uint64, _f64p]
+    L.or_operator.argtypes = [_sz, _sz, ctypes.c_double, ctypes.c_int, _f64p, ctypes.c_double,
+                              _f64p, _f64p, _f64p]
+    L.or_apply.argtypes = [_sz, _sz, ctypes.c_int, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.or_apply_reference.argtypes = [_sz, _sz, ctypes.c_double, ctypes.c_int, _f64p,
+                                     ctypes.c_double, _f64p, _f64p]
+    L.or_thomas.argtypes = [_sz, _f64p, _f64p, _f64p, _f64p]
+    L.or_cyclic.argtypes = [_sz, _f64p, _f64p, _f64p, ctypes.c_double, ctypes.c_double, _f64p]
+    L.or_lod.argtypes = [_sz, _sz, ctypes.c_double, ctypes.c_int, _f64p, _f64p, ctypes.c_double]
+    L.or_hier_new.argtypes = [_sz, _sz, ctypes.c_int, _f64p, _f64p, _f64p, ctypes.c_float, _sz,
+                              ctypes.c_float]
+    L.or_hier_new.restype = ctypes.c_void_p
+    L.or_hier_free.argtypes = [ctypes.c_void_p]
+    L.or_hier_levels.argtypes = [ctypes.c_void_p]
+    L.or_hier_levels.restype = _sz
+    L.or_hier_level.argtypes = [ctypes.c_void_p, _sz, ctypes.POINTER(_sz), ctypes.POINTER(_sz),
+                                ctypes.POINTER(ctypes.c_int)] + [ctypes.c_void_p] * 5
+    L.or_hier_precondition.argtypes = [ctypes.c_void_p, _f32p, _f64p, _f32p]
+    L.or_hier_precondition.restype = ctypes.c_double
+    L.or_cg_on.argtypes = [_sz, _sz, ctypes.c_double, ctypes.c_int, _f64p, _f64p, _f64p,
+                           ctypes.c_double, ctypes.c_double, ctypes.c_float, _sz, ctypes.c_float]
+    L.or_cg_on.restype = _sz
+    L.or_solve.argtypes = [_sz, _sz, ctypes.c_double, ctypes.c_int, _f64p, _f64p,
+                           ctypes.c_double, ctypes.c_double]
+    L.or_solve.restype = _sz
+    L.or_jacobi_cg.argtypes = [_sz, _sz, ctypes.c_double, ctypes.c_int, _f64p, _f64p, _f64p,
+                               ctypes.c_double, ctypes.c_double]
+    L.or_jacobi_cg.restype = _sz
+    L.or_reference_cg.argtypes = [_sz, _sz, ctypes.c_double, ctypes.c_int, _f64p, _f64p,
+                                  ctypes.c_double, _f64p]
+    L.or_reference_cg.restype = _sz
+    L.or_bound.argtypes = [_sz, _sz, ctypes.c_double, ctypes.c_int, _f64p, ctypes.c_double,
+                           ctypes.c_double]
+    L.or_bound.restype = ctypes.c_double
+    L.or_rel_residual.argtypes = [_sz, _sz, ctypes.c_double, ctypes.c_int, _f64p,
+                                  ctypes.c_double, _f64p, _f64p]
+    L.or_rel_residual.restype = ctypes.c_double
+    L.or_threads.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _c(a, dt=np.float64):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# ---- fixtures ------------------------------------------------------------------------------
+
+class SplitMix64:
+    """SplitMix64 (crates/deeptime-core/src/sphere/mod.rs:43-63) through the oracle."""
+
+    def __init__(self, seed: int):
+        self.state = ctypes.c_uint64(seed & 0xFFFFFFFFFFFFFFFF)
+
+    def next_u64(self) -> int:
+        return lib().or_splitmix_next(ctypes.byref(self.state))
+
+    def next_f64(self) -> float:
+        return lib().or_splitmix_f64(ctypes.byref(self.state))
+
+
+def terrain(w, h, seed):
+    """`terrain`, multigrid/tests.rs:77-98."""
+    z = np.zeros(w * h)
+    lib().or_terrain(w, h, seed, z)
+    return z
+
+
+def fractal(w, h, seed):
+    """`fractal`, diffusion/parity.rs:340-359."""
+    z = np.zeros(w * h)
+    lib().or_fractal(w, h, seed, z)
+    return z
+
+
+def submarine_kd(z, d0, wave_base, tidal_range, land):
+    """`submarine_kd`, surface/marine.rs:44-59."""
+    return lib().or_submarine_kd(z, d0, wave_base, tidal_range, land)
+
+
+def shelf_kd(z, seed):
+    """`shelf_kd`, multigrid/tests.rs:102-110."""
+    z = _c(z)
+    kd = np.zeros_like(z)
+    lib().or_shelf_kd(z.size, z, seed, kd)
+    return kd
+
+
+# ---- operator ------------------------------------------------------------------------------
+
+def operator(w, h, dx, wrap, kd, dt):
+    """`Operator::new`, diffusion.rs:532-603 -> (east, south, diag)."""
+    e, s, g = np.zeros(w * h), np.zeros(w * h), np.zeros(w * h)
+    lib().or_operator(w, h, dx, int(wrap), _c(kd), dt, e, s, g)
+    return e, s, g
+
+
+def apply(w, h, wrap, east, south, diag, x):
+    """`Operator::apply`, diffusion.rs:609-663."""
+    y = np.zeros(w * h)
+    lib().or_apply(w, h, int(wrap), _c(east), _c(south), _c(diag), _c(x), y)
+    return y
+
+
+def apply_reference(w, h, dx, wrap, kd, dt, x):
+    """`apply_reference`, multigrid/tests.rs:17-43."""
+    y = np.zeros(w * h)
+    lib().or_apply_reference(w, h, dx, int(wrap), _c(kd), dt, _c(x), y)
+    return y
+
+
+# ---- first guess ---------------------------------------------------------------------------
+
+def thomas(lo, di, up, b):
+    """`thomas`, diffusion.rs:1029-1046."""
+    b = _c(b).copy()
+    lib().or_thomas(b.size, _c(lo), _c(di), _c(up), b)
+    return b
+
+
+def cyclic(lo, di, up, alpha, beta, b):
+    """`cyclic`, diffusion.rs:1050-1071."""
+    b = _c(b).copy()
+    lib().or_cyclic(b.size, _c(lo), _c(di), _c(up), alpha, beta, b)
+    return b
+
+
+def lod(w, h, dx, wrap, hgt, kd, dt):
+    """`lod`, diffusion.rs:370-489 (every row visited)."""
+    x = _c(hgt).copy()
+    lib().or_lod(w, h, dx, int(wrap), x, _c(kd), dt)
+    return x
+
+
+# ---- multigrid cycle -----------------------------------------------------------------------
+
+class Hierarchy:
+    """`Hierarchy`, multigrid.rs:475-497 (f32 levels of the V-cycle)."""
+
+    def __init__(self, w, h, wrap, east, south, diag, omega=0.8, nu=2, scale=0.5):
+        self.w, self.h = w, h
+        self._keep = (_c(east), _c(south), _c(diag))
+        self.ptr = lib().or_hier_new(w, h, int(wrap), *self._keep, omega, nu, scale)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().or_hier_free(self.ptr)
+            self.ptr = None
+
+    def levels(self):
+        return lib().or_hier_levels(self.ptr)
+
+    def level(self, l):
+        """Level l: dict of w, h, wrap and f32 east, south, diag, wd, mass."""
+        w, h, wr = _sz(), _sz(), ctypes.c_int()
+        lib().or_hier_level(self.ptr, l, ctypes.byref(w), ctypes.byref(h), ctypes.byref(wr),
+                            None, None, None, None, None)
+        n = w.value * h.value
+        arrs = {k: np.zeros(n, np.float32) for k in ("east", "south", "diag", "wd", "mass")}
+        lib().or_hier_level(self.ptr, l, None, None, None,
+                            *[arrs[k].ctypes.data_as(ctypes.c_void_p)
+                              for k in ("east", "south", "diag", "wd", "mass")])
+        arrs.update(w=w.value, h=h.value, wrap=bool(wr.value))
+        return arrs
+
+    def precondition(self, r32, r64):
+        """z = M^-1 r (one V-cycle from zero); returns (z, sum r64*z)."""
+        z = np.zeros(self.w * self.h, np.float32)
+        rz = lib().or_hier_precondition(self.ptr, _c(r32, np.float32), _c(r64), z)
+        return z, rz
+
+
+# ---- solves --------------------------------------------------------------------------------
+
+def cg_on(w, h, dx, wrap, x, b, kd, dt, tol_m, omega=0.8, nu=2, scale=0.5):
+    """`cg_on`, multigrid.rs:562-763. Returns (x, iterations)."""
+    x = _c(x).copy()
+    its = lib().or_cg_on(w, h, dx, int(wrap), x, _c(b), _c(kd), dt, tol_m, omega, nu, scale)
+    return x, its
+
+
+def solve(w, h, dx, wrap, hgt, kd, dt, tol_m):
+    """`solve` (multigrid), diffusion.rs:77-116. Returns (h1, iterations)."""
+    x = _c(hgt).copy()
+    its = lib().or_solve(w, h, dx, int(wrap), x, _c(kd), dt, tol_m)
+    return x, its
+
+
+def jacobi_cg(w, h, dx, wrap, x, b, kd, dt, tol_m=1e-6):
+    """Jacobi-preconditioned `cg`, diffusion.rs:927-1026 (every row)."""
+    x = _c(x).copy()
+    its = lib().or_jacobi_cg(w, h, dx, int(wrap), x, _c(b), _c(kd), dt, tol_m)
+    return x, its
+
+
+def jacobi_step(w, h, dx, wrap, hgt, kd, dt):
+    """`step` (Jacobi solve with the splitting's guess), diffusion.rs:204-281, every row."""
+    b = _c(hgt)
+    x = lod(w, h, dx, wrap, b, kd, dt)
+    return jacobi_cg(w, h, dx, wrap, x, b, kd, dt, 1e-6)
+
+
+def reference_cg(w, h, dx, wrap, b, kd, dt):
+    """`reference`, multigrid/tests.rs:46-74: plain CG to relative residual 1e-13."""
+    x = np.zeros(w * h)
+    lib().or_reference_cg(w, h, dx, int(wrap), _c(b), _c(kd), dt, x)
+    return x
+
+
+def bound(w, h, dx, wrap, kd, dt, tol):
+    """`bound`, multigrid/tests.rs:157-164."""
+    return lib().or_bound(w, h, dx, int(wrap), _c(kd), dt, tol)
+
+
+def rel_residual(w, h, dx, wrap, kd, dt, b, x):
+    """||b - A x|| / ||b|| in f64."""
+    return lib().or_rel_residual(w, h, dx, int(wrap), _c(kd), dt, _c(b), _c(x))
+
+
+def threads() -> int:
+    return lib().or_threads()

@@ -1,0 +1,205 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle, on the GPU.
+
+Per kernel, the device does each cell's arithmetic in the reference's order, so the operator,
+the first guess, the cycle's levels and the cycle's correction are compared bit for bit.  The
+solve's sums are taken in another (fixed) order than the reference's strips, so whole solves are
+compared as the reference compares its device cycle (multigrid/tests.rs:445-493): within the
+tolerance bound of the exact solution, within twice the bound of the oracle's solve, in at most
+two more iterations; and, at a tight tolerance, to a relative L2 difference of 1e-10.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import dtdiff
+import oracle as O
+from dtdiff import fields as F
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(1, 1), (1, 7), (7, 1), (2, 2), (2, 5), (3, 5), (5, 3), (17, 9), (33, 33), (64, 40),
+         (129, 67), (200, 96)]
+SHAPES = [(w, h, wrap) for (w, h) in SIZES for wrap in (True, False)]
+
+
+@pytest.fixture(scope="module")
+def ws():
+    w = dtdiff.Workspace()
+    yield w
+    w.close()
+
+
+def rms(a, b):
+    return float(np.sqrt(np.mean((np.asarray(a) - np.asarray(b)) ** 2)))
+
+
+def case(w, h, seed=3):
+    b = O.terrain(w, h, seed * 7 + w)
+    return b, O.shelf_kd(b, seed)
+
+
+def same_bits(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+@pytest.mark.parametrize("w,h,wrap", SHAPES)
+def test_operator_and_apply_bit_for_bit(ws, w, h, wrap):
+    print(f"operator {w}x{h} wrap={wrap}", flush=True)
+    b, kd = case(w, h)
+    d = dtdiff.Dims(w, h, 2400.0, wrap)
+    e, s, g = dtdiff.operator(ws, d, kd, 2e6)
+    oe, os_, og = O.operator(w, h, 2400.0, wrap, kd, 2e6)
+    assert same_bits(e, oe) and same_bits(s, os_) and same_bits(g, og)
+    y = dtdiff.apply(ws, d, kd, 2e6, b)
+    assert same_bits(y, O.apply(w, h, wrap, oe, os_, og, b))
+
+
+@pytest.mark.parametrize("w,h,wrap", SHAPES + [(300, 129, True), (1024, 512, True)])
+def test_first_guess_bit_for_bit(ws, w, h, wrap):
+    print(f"lod {w}x{h} wrap={wrap}", flush=True)
+    b, kd = case(w, h)
+    d = dtdiff.Dims(w, h, 2400.0, wrap)
+    assert same_bits(dtdiff.lod(ws, d, b, kd, 2e6), O.lod(w, h, 2400.0, wrap, b, kd, 2e6))
+
+
+@pytest.mark.parametrize("w,h,wrap", SHAPES + [(300, 129, True), (513, 257, False)])
+def test_cycle_levels_and_correction_bit_for_bit(ws, w, h, wrap):
+    """The device cycle against the CPU cycle (multigrid/tests.rs:381-440), here bit for bit."""
+    print(f"cycle {w}x{h} wrap={wrap}", flush=True)
+    b, kd = case(w, h, 13)
+    d = dtdiff.Dims(w, h, 2400.0, wrap)
+    e, s, g = O.operator(w, h, 2400.0, wrap, kd, 2e6)
+    hy = O.Hierarchy(w, h, wrap, e, s, g)
+    cy = dtdiff.Cycle(ws, d, e, s, g)
+    assert cy.levels() == hy.levels()
+    for l in range(hy.levels()):
+        a, o = cy.level(l), hy.level(l)
+        assert (a["w"], a["h"], a["wrap"]) == (o["w"], o["h"], o["wrap"])
+        for k in ("east", "south", "diag", "wd", "mass"):
+            assert same_bits(a[k], o[k]), (l, k)
+    rng = O.SplitMix64(17 + w)
+    for _ in range(3):
+        r = np.array([rng.next_f64() - 0.5 for _ in range(w * h)])
+        z, rz = cy.precondition(r.astype(np.float32), r)
+        zo, rzo = hy.precondition(r.astype(np.float32), r)
+        assert same_bits(z, zo)
+        assert abs(rz - rzo) <= 1e-12 * abs(rzo)
+        own = float(r @ z.astype(np.float64))
+        assert abs(rz - own) <= 1e-9 * max(abs(own), 1e-30)
+
+
+def test_the_gpu_solve_meets_its_tolerance_and_keeps_the_mass(ws):
+    """multigrid/tests.rs:445-493 over the reference's SIZES, seeds and (dt, tol) pairs."""
+    for (w, h) in SIZES:
+        for wrap in (True, False):
+            for seed in (1, 2):
+                b = O.terrain(w, h, seed * 7 + w)
+                kd = O.shelf_kd(b, seed)
+                d = dtdiff.Dims(w, h, 2400.0, wrap)
+                for dt, tol in [(2.0e6, 1.0e-6), (2.0e6, 1.0e-3), (2.0e4, 1.0e-6)]:
+                    label = f"{w}x{h} wrap {wrap} seed {seed} dt {dt} tol {tol}"
+                    exact = O.reference_cg(w, h, 2400.0, wrap, b, kd, dt)
+                    x = b.copy()
+                    its = dtdiff.solve(ws, d, x, kd, dt, dtdiff.DiffusionSolver.MULTIGRID, tol)
+                    assert its < 400, label
+                    lim = O.bound(w, h, 2400.0, wrap, kd, dt, tol)
+                    assert rms(x, exact) <= lim, label
+                    scale = max(np.abs(b).sum(), 1.0)
+                    assert abs(b.sum() - x.sum()) <= 1e-12 * scale, label
+                    xo, its_o = O.solve(w, h, 2400.0, wrap, b, kd, dt, tol)
+                    assert rms(x, xo) <= 2.0 * lim, label
+                    assert its <= its_o + 2, label
+        print(f"solve {w}x{h} ok", flush=True)
+
+
+@pytest.mark.parametrize("w,h,gen", [(512, 256, "c1"), (1024, 512, "c2"), (300, 129, "c2")])
+def test_tight_solve_matches_the_oracle_to_1e10(ws, w, h, gen):
+    """The north-star bar: relative L2 between the GPU FP64 answer and the oracle's <= 1e-10 on
+    identical seeded inputs, both solved to a relative residual of 1e-12."""
+    print(f"tight {gen} {w}x{h}", flush=True)
+    d, h0, kd, dt = (F.c1 if gen == "c1" else F.c2)(w=w, h=h)
+    tol = 1e-12 * np.linalg.norm(h0) / np.sqrt(h0.size)
+    x = h0.copy()
+    its = dtdiff.solve(ws, d, x, kd, dt, dtdiff.DiffusionSolver.MULTIGRID, tol)
+    xo, its_o = O.solve(d.w, d.h, d.dx, d.wrap_x, h0, kd, dt, tol)
+    assert np.linalg.norm(x - xo) <= 1e-10 * np.linalg.norm(xo)
+    assert abs(its - its_o) <= 2
+    assert O.rel_residual(d.w, d.h, d.dx, d.wrap_x, kd, dt, h0, x) <= 2e-12
+
+
+def test_solve_rel_device_meets_the_true_relative_residual(ws):
+    import torch
+    d, h0, kd, dt = F.c2(w=2048, h=1024)
+    hd = torch.tensor(h0, device="cuda:0")
+    kdd = torch.tensor(kd, device="cuda:0")
+    torch.cuda.synchronize()
+    st = dtdiff.solve_rel_device(ws, d, hd, kdd, dt, 1e-10)
+    x = hd.cpu().numpy()
+    rel = O.rel_residual(d.w, d.h, d.dx, d.wrap_x, kd, dt, h0, x)
+    assert rel <= 1e-10 and abs(st.rel_residual - rel) <= 1e-3 * rel + 1e-16
+    assert abs(h0.sum() - x.sum()) <= 1e-12 * np.abs(h0).sum()
+    # No new extremes (the scheme is monotone for this M-matrix).
+    assert x.max() <= h0.max() + 1e-6 and x.min() >= h0.min() - 1e-6
+
+
+def test_empty_grid_and_bad_arguments(ws):
+    d = dtdiff.Dims(0, 5, 1.0, True)
+    x = np.zeros(0)
+    assert dtdiff.solve(ws, d, x, np.zeros(0), 1.0, dtdiff.DiffusionSolver.MULTIGRID, 1e-6) == 0
+    d = dtdiff.Dims(4, 4, 1.0, True)
+    x = np.ones(16)
+    for tol in (0.0, -1.0, float("nan"), float("inf")):
+        with pytest.raises(dtdiff.DtdError) as e:
+            dtdiff.solve(ws, d, x, np.ones(16), 1.0, dtdiff.DiffusionSolver.MULTIGRID, tol)
+        assert e.value.code == dtdiff.DTD_EINVAL
+    with pytest.raises(dtdiff.DtdError) as e:
+        dtdiff.solve(ws, d, x, np.ones(16), 1.0, dtdiff.DiffusionSolver.JACOBI, 1e-6)
+    assert e.value.code == dtdiff.DTD_EINVAL
+    assert (x == 1).all()  # untouched on error
+    # A constant field is already the answer (A 1 = 1): no iteration, nothing moves.
+    its = dtdiff.solve(ws, d, x, np.ones(16), 1.0, dtdiff.DiffusionSolver.MULTIGRID, 1e-6)
+    assert its == 0 and np.allclose(x, 1.0, rtol=0, atol=1e-14)
+
+
+def test_results_are_repeatable_and_independent_of_workspace_history(ws):
+    out = []
+    for (w, h) in [(200, 96), (64, 40), (200, 96)]:
+        b, kd = case(w, h, 5)
+        d = dtdiff.Dims(w, h, 2400.0, True)
+        x = b.copy()
+        its = dtdiff.solve(ws, d, x, kd, 2e6, dtdiff.DiffusionSolver.MULTIGRID, 1e-7)
+        out.append((x, its))
+    assert same_bits(out[0][0], out[2][0]) and out[0][1] == out[2][1]
+    fresh = dtdiff.Workspace()
+    b, kd = case(200, 96, 5)
+    x = b.copy()
+    dtdiff.solve(fresh, dtdiff.Dims(200, 96, 2400.0, True), x, kd, 2e6,
+                 dtdiff.DiffusionSolver.MULTIGRID, 1e-7)
+    assert same_bits(x, out[0][0])
+
+
+def test_graph_and_direct_launches_agree(ws, monkeypatch):
+    b, kd = case(129, 67, 4)
+    d = dtdiff.Dims(129, 67, 2400.0, True)
+    x1 = b.copy()
+    dtdiff.solve(ws, d, x1, kd, 2e6, dtdiff.DiffusionSolver.MULTIGRID, 1e-8)
+    monkeypatch.setenv("DTD_NO_GRAPH", "1")
+    direct = dtdiff.Workspace()
+    x2 = b.copy()
+    dtdiff.solve(direct, d, x2, kd, 2e6, dtdiff.DiffusionSolver.MULTIGRID, 1e-8)
+    assert same_bits(x1, x2)
+
+
+@pytest.mark.parametrize("prec,guess", [(dtdiff.CYCLE_F64, dtdiff.GUESS_LOD),
+                                        (dtdiff.CYCLE_F32, dtdiff.GUESS_RHS)])
+def test_cycle_variants_reach_the_same_answer(prec, guess):
+    wsv = dtdiff.Workspace(cycle_precision=prec, guess=guess)
+    d, h0, kd, dt = F.c2(w=512, h=256)
+    tol = 1e-11 * np.linalg.norm(h0) / np.sqrt(h0.size)
+    x = h0.copy()
+    its = dtdiff.solve(wsv, d, x, kd, dt, dtdiff.DiffusionSolver.MULTIGRID, tol)
+    xo, _ = O.solve(d.w, d.h, d.dx, d.wrap_x, h0, kd, dt, tol)
+    assert its < 400
+    assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo)
