@@ -48,10 +48,10 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
 
     import dtdiff
     from dtdiff import fields as F
+    from dtdiff.hipmem import DeviceArray, synchronize
 
     w, h = (int(v) for v in args.size.split("x"))
     t = time.time()
@@ -60,36 +60,33 @@ def main():
     n = d.cells()
 
     t = time.time()
-    h0d = torch.from_numpy(h0).to("cuda:0")
-    kdd = torch.from_numpy(kd).to("cuda:0")
-    torch.cuda.synchronize()
+    h0d = DeviceArray.from_numpy(h0)
+    kdd = DeviceArray.from_numpy(kd)
+    synchronize()
     upload_s = time.time() - t
-    hd = torch.empty_like(h0d)
+    hd = DeviceArray(n)
 
     ws = dtdiff.Workspace(device=0,
                           cycle_precision=dtdiff.CYCLE_F32 if args.cycle == "f32"
                           else dtdiff.CYCLE_F64,
                           guess=dtdiff.GUESS_LOD if args.guess == "lod" else dtdiff.GUESS_RHS)
-    stream = torch.cuda.current_stream()
-
     def step():
-        hd.copy_(h0d)
-        stream.synchronize()
+        hd.copy_from(h0d)  # synchronous device-to-device copy
         return dtdiff.solve_rel_device(ws, d, hd, kdd, dt, args.rel)
 
     for k in range(args.warmup):
         st = step()
         log(f"warmup {k}: {st.iterations} its, rel {st.rel_residual:.2e}, {st.ms_total:.2f} ms")
-    torch.cuda.synchronize()
+    synchronize()
     t0 = time.perf_counter()
     stats = []
     for _ in range(args.steps):
         stats.append(step())
-    torch.cuda.synchronize()
+    synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
     last = stats[-1]
-    x_gpu = hd.cpu().numpy()
+    x_gpu = hd.numpy()
     log(f"timed {args.steps} steps: {elapsed * 1e3 / args.steps:.3f} ms/step, "
         f"{last.iterations} its, {last.restarts} restarts, rel {last.rel_residual:.3e}; "
         f"setup {last.ms_setup:.2f} guess {last.ms_guess:.2f} cg {last.ms_cg:.2f} ms")

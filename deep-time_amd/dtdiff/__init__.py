@@ -152,7 +152,7 @@ def _f32(a):
 
 
 def _ptr(t):
-    """Device pointer of a torch tensor (or an int address)."""
+    """Device address of a DeviceArray (anything with data_ptr()) or an int address."""
     return ctypes.c_void_p(t if isinstance(t, int) else t.data_ptr())
 
 
@@ -205,7 +205,7 @@ def solve(ws: Workspace, d: Dims, h: np.ndarray, kd: np.ndarray, dt: float,
 
 
 def solve_device(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float, tol_m: float) -> Stats:
-    """The solve on device arrays (torch tensors or addresses of w*h doubles), h in place."""
+    """The solve on device arrays (DeviceArray or addresses of w*h doubles), h in place."""
     st = Stats()
     ws.check(load().dtd_solve_device(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x), _ptr(h_dev),
                                      _ptr(kd_dev), dt, tol_m, ctypes.byref(st)))
