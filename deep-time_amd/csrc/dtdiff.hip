@@ -284,6 +284,16 @@ static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt)
     }
 }
 
+// The fused smoothing kernels serve levels whose width is a multiple of 64 (DTD_NO_FUSE=1 turns
+// them off, for comparison).
+static bool fused_enabled() {
+    static const bool on = [] {
+        const char *v = getenv("DTD_NO_FUSE");
+        return !(v && *v && *v != '0');
+    }();
+    return on;
+}
+
 // `vcycle` (MG:500-523) from level l, enqueued on s.  The first sweep writes z[0] and the
 // sweeps alternate after it; with 2 nu - 1 sweeps after the first on every level but the
 // coarsest (and COARSE_SWEEPS - 1 = 15 there), each level's result lands in z[1].  With dot,
@@ -299,6 +309,20 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
         return 1;
     }
     const int nb = nblocks(L.n);
+    Lvl<T> &C0 = lv[l + 1];
+    if (nu == 2 && L.w % kTW == 0 && fused_enabled()) {
+        // Fused, LDS-tiled smoothing (kernels.hpp): pre-smoothing and restriction in one
+        // launch, the coarse cycle, then both post-smoothing sweeps in one launch.
+        constexpr int TH = TileRows<T>::TH;
+        const int tiles_x = L.w / kTW, tiles = tiles_x * ((L.h + TH - 1) / TH);
+        k_pre2<T><<<tiles, kBlock, 0, s>>>(L, L.z[0], C0, tiles_x, tiles, st);
+        LAUNCH_CHECK();
+        vcycle<T>(lv, l + 1, nu, nullptr, parts, st, s);
+        k_post2<T><<<tiles, kBlock, 0, s>>>(L, L.z[0], C0.z[1], C0.w, L.z[1], tiles_x, tiles, dot,
+                                            parts, st);
+        LAUNCH_CHECK();
+        return tiles;
+    }
     int cur = 0;
     k_first<T><<<nb, kBlock, 0, s>>>(L, L.z[0], st);
     LAUNCH_CHECK();
@@ -342,7 +366,7 @@ static void enqueue_iteration(dtd_ctx *ctx, int parity) {
     Workspace &ws = ctx->ws;
     const Grid g = grid_of(ws);
     hipStream_t s = ctx->stream;
-    const int nb = nblocks(ws.n);
+    const int nb = nblocks(ws.n, kCgCells);
     double *pold = parity == 0 ? ws.p1 : ws.p0;
     double *pnew = parity == 0 ? ws.p0 : ws.p1;
     if (ws.prec == DTD_CYCLE_F64) {
@@ -419,14 +443,15 @@ static size_t solve_core(dtd_ctx *ctx, const double *kd, double dx, double dt, i
     const double tol2 = tol_m * tol_m * (double)ws.n;
     k_init_state<<<1, 64, 0, s>>>(ws.st, tol2, (int)ctx->opts.max_iter);
     LAUNCH_CHECK();
+    const int nbc = nblocks(ws.n, kCgCells);
     if (ws.prec == DTD_CYCLE_F64)
-        k_resid_init<double><<<nb, kBlock, 0, s>>>(g, ws.east, ws.south, ws.b, ws.x, ws.r,
-                                                   (double *)nullptr, ws.parts);
+        k_resid_init<double><<<nbc, kBlock, 0, s>>>(g, ws.east, ws.south, ws.b, ws.x, ws.r,
+                                                    (double *)nullptr, ws.parts);
     else
-        k_resid_init<float><<<nb, kBlock, 0, s>>>(g, ws.east, ws.south, ws.b, ws.x, ws.r,
-                                                  ws.lf[0].r, ws.parts);
+        k_resid_init<float><<<nbc, kBlock, 0, s>>>(g, ws.east, ws.south, ws.b, ws.x, ws.r,
+                                                   ws.lf[0].r, ws.parts);
     LAUNCH_CHECK();
-    k_cg_step<kStepRR0><<<1, kReduceBlock, 0, s>>>(ws.parts, nb, ws.st);
+    k_cg_step<kStepRR0><<<1, kReduceBlock, 0, s>>>(ws.parts, nbc, ws.st);
     LAUNCH_CHECK();
     const int nbz = run_vcycle(ctx, ws.r, ws.st);
     k_cg_step<kStepRZ0><<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.st);
@@ -770,6 +795,7 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
         const Grid g = grid_of(ws);
         Lvl<float> &L = ws.lf[0];
         const int nb = nblocks(ws.n);
+        const int nbc = nblocks(ws.n, kCgCells);
         const double n = (double)ws.n;
         double b = 0.0;
         // A private state that never stops, so the loop kernels run in full.
@@ -783,12 +809,12 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
                 b = 28.0 * n;
                 break;
             case DTD_KERNEL_DIRECTION:  // z (f32), p_old, x, east, south in; p, A p, x out
-                k_direction<float><<<nb, kBlock, 0, s>>>(g, ws.east, ws.south, L.z[1], ws.p1,
+                k_direction<float><<<nbc, kBlock, 0, s>>>(g, ws.east, ws.south, L.z[1], ws.p1,
                                                          ws.p0, ws.ap, ws.x, ws.parts, ws.st);
                 b = (4.0 + 8.0 * 4 + 8.0 * 3) * n;
                 break;
             case DTD_KERNEL_UPDATE:  // A p, r in; r, r32 out
-                k_update<float><<<nb, kBlock, 0, s>>>((int)ws.n, ws.ap, ws.r, L.r, ws.parts,
+                k_update<float><<<nbc, kBlock, 0, s>>>((int)ws.n, ws.ap, ws.r, L.r, ws.parts,
                                                       ws.st);
                 b = (8.0 * 3 + 4.0) * n;
                 break;

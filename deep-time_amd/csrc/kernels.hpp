@@ -57,13 +57,30 @@ __device__ inline double block_sum(double v, double *lds) {
     return s;
 }
 
-// Sum of parts[k * stride + slot] over k < n by one block, in a fixed order.
+// Sum of parts[k * stride + slot] over k < n by one block, in a fixed order: thread t adds
+// k = t, t + 1024, ... in turn (the loads unrolled so several are in flight), then block_sum.
 __device__ inline double reduce_parts(const double *parts, int n, int stride, int slot,
                                       double *lds) {
     double v = 0.0;
-    for (int k = threadIdx.x; k < n; k += kReduceBlock) v += parts[(size_t)k * stride + slot];
+    int k = threadIdx.x;
+    for (; k + 3 * kReduceBlock < n; k += 4 * kReduceBlock) {
+        const double a = parts[(size_t)k * stride + slot];
+        const double b = parts[(size_t)(k + kReduceBlock) * stride + slot];
+        const double c = parts[(size_t)(k + 2 * kReduceBlock) * stride + slot];
+        const double d = parts[(size_t)(k + 3 * kReduceBlock) * stride + slot];
+        v += a;
+        v += b;
+        v += c;
+        v += d;
+    }
+    for (; k < n; k += kReduceBlock) v += parts[(size_t)k * stride + slot];
     return block_sum<kReduceBlock>(v, lds);
 }
+
+// Cells per thread of the conjugate-gradient passes: a block covers kCgCells consecutive cells,
+// thread t taking t, t + 256, t + 512 and t + 768.
+constexpr int kCgPer = 4;
+constexpr int kCgCells = kBlock * kCgPer;
 
 // ---- the f64 operator (D:532-663) ----------------------------------------------------------
 
@@ -168,14 +185,18 @@ __global__ void k_resid_init(Grid g, const double *__restrict__ east,
                              const double *__restrict__ x, double *__restrict__ r,
                              T *__restrict__ rc, double *__restrict__ parts) {
     __shared__ double lds[kBlock / 64];
-    const int c = blockIdx.x * kBlock + threadIdx.x;
     double acc = 0.0;
-    if (c < g.n) {
-        const int j = c / g.w, i = c - j * g.w;
-        const double rv = b[c] - apply64(g, east, south, c, i, j, [&](int k) { return x[k]; });
-        r[c] = rv;
-        if (rc) rc[c] = (T)rv;
-        acc = rv * rv;
+#pragma unroll
+    for (int q = 0; q < kCgPer; q++) {
+        const int c = blockIdx.x * kCgCells + q * kBlock + threadIdx.x;
+        if (c < g.n) {
+            const int j = c / g.w, i = c - j * g.w;
+            const double rv =
+                b[c] - apply64(g, east, south, c, i, j, [&](int k) { return x[k]; });
+            r[c] = rv;
+            if (rc) rc[c] = (T)rv;
+            acc += rv * rv;
+        }
     }
     const double s = block_sum<kBlock>(acc, lds);
     if (threadIdx.x == 0) parts[blockIdx.x] = s;
@@ -194,19 +215,20 @@ __global__ void k_direction(Grid g, const double *__restrict__ east,
     if (st->done) return;
     const int have_beta = st->have_beta, pending = st->have_pending;
     const double beta = st->beta, alpha = st->alpha;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
+    auto p = [&](int k) { return have_beta ? (double)z[k] + beta * pold[k] : (double)z[k]; };
     double acc = 0.0;
-    if (c < g.n) {
-        const int j = c / g.w, i = c - j * g.w;
-        auto p = [&](int k) {
-            return have_beta ? (double)z[k] + beta * pold[k] : (double)z[k];
-        };
-        if (pending) x[c] += alpha * pold[c];
-        const double pc = p(c);
-        const double a = apply64(g, east, south, c, i, j, p);
-        pnew[c] = pc;
-        ap[c] = a;
-        acc = pc * a;
+#pragma unroll
+    for (int q = 0; q < kCgPer; q++) {
+        const int c = blockIdx.x * kCgCells + q * kBlock + threadIdx.x;
+        if (c < g.n) {
+            const int j = c / g.w, i = c - j * g.w;
+            if (pending) x[c] += alpha * pold[c];
+            const double pc = p(c);
+            const double a = apply64(g, east, south, c, i, j, p);
+            pnew[c] = pc;
+            ap[c] = a;
+            acc += pc * a;
+        }
     }
     const double s = block_sum<kBlock>(acc, lds);
     if (threadIdx.x == 0) parts[blockIdx.x] = s;
@@ -220,13 +242,16 @@ __global__ void k_update(int n, const double *__restrict__ ap, double *__restric
     __shared__ double lds[kBlock / 64];
     if (st->done) return;
     const double alpha = st->alpha;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
     double acc = 0.0;
-    if (c < n) {
-        const double rv = r[c] - alpha * ap[c];
-        r[c] = rv;
-        if (rc) rc[c] = (T)rv;
-        acc = rv * rv;
+#pragma unroll
+    for (int q = 0; q < kCgPer; q++) {
+        const int c = blockIdx.x * kCgCells + q * kBlock + threadIdx.x;
+        if (c < n) {
+            const double rv = r[c] - alpha * ap[c];
+            r[c] = rv;
+            if (rc) rc[c] = (T)rv;
+            acc += rv * rv;
+        }
     }
     const double s = block_sum<kBlock>(acc, lds);
     if (threadIdx.x == 0) parts[blockIdx.x] = s;
@@ -525,6 +550,234 @@ __global__ void k_coarsest(Lvl<T> L, T *__restrict__ zout, int sweeps,
             total += a;
         }
         parts[0] = total;
+    }
+}
+
+// ---- fused, LDS-tiled smoothing of a level (nu = 2) ----------------------------------------
+//
+// A block takes a tile of TH x 64 cells of a level whose width is a multiple of 64 and stages
+// the level's arrays over the tile and a halo of two rows and four columns (float4-aligned) in
+// LDS.  The pre-smoothing kernel forms the first sweep, the second sweep and the residual
+// restriction in one pass (`vcycle`, MG:511-514); the post-smoothing kernel forms the corrected
+// sweep and the last sweep with its dot (MG:516-521).  Halo cells are recomputed by the
+// neighbouring tiles, so each cell's arithmetic is the unfused kernels', operation for
+// operation, and the results are identical.  Tiles are numbered so that each XCD (workgroups
+// are dealt to the eight XCDs in turn) takes a contiguous band of them, keeping the halo reads
+// in its own L2.
+
+constexpr int kTW = 64;            // tile width (cells)
+constexpr int kRW = kTW + 8;       // staged width: 4 halo columns each side
+template <typename T> struct TileRows { static constexpr int TH = 16; };
+template <> struct TileRows<double> { static constexpr int TH = 12; };
+
+__device__ inline int tile_of_block(int b, int tiles) {
+    if (tiles % 8) return b;
+    return (b % 8) * (tiles / 8) + b / 8;
+}
+
+// Four consecutive elements from global memory (16-byte loads).
+__device__ inline void load4(const float *p, float *d) {
+    const float4 v = *reinterpret_cast<const float4 *>(p);
+    d[0] = v.x, d[1] = v.y, d[2] = v.z, d[3] = v.w;
+}
+__device__ inline void load4(const double *p, double *d) {
+    const double2 a = reinterpret_cast<const double2 *>(p)[0];
+    const double2 b = reinterpret_cast<const double2 *>(p)[1];
+    d[0] = a.x, d[1] = a.y, d[2] = b.x, d[3] = b.y;
+}
+
+// One cell of A x on LDS arrays in the order of `Level::apply_row` (MG:203-240); k is the LDS
+// index, (gi, gj) the cell's place on the level.  For a wrapping level the staged halo holds
+// the wrapped columns, so the wrapped neighbours sit at k - 1 and k + 1 like any other.
+template <typename T>
+__device__ inline T tile_apply(const T *X, const T *E, const T *S, const T *D, int k, int gi,
+                               int gj, int w, int h, int wrap) {
+    T o = D[k] * X[k];
+    if (gi + 1 < w) o -= E[k] * X[k + 1];
+    if (gi > 0) o -= E[k - 1] * X[k - 1];
+    if (wrap) {
+        if (gi == w - 1) o -= E[k] * X[k + 1];
+        if (gi == 0) o -= E[k - 1] * X[k - 1];
+    }
+    if (gj + 1 < h) o -= S[k] * X[k + kRW];
+    if (gj > 0) o -= S[k - kRW] * X[k - kRW];
+    return o;
+}
+
+// Stages `na` arrays of a level over the tile at (tx0, ty0) and its halo into LDS (rows
+// ty0 - 2 .. ty0 + TH + 1, columns tx0 - 4 .. tx0 + 67, wrapped when the level wraps); cells off
+// the level are zero.
+template <typename T, int TH, int NA>
+__device__ inline void stage(const T *const (&src)[NA], T *const (&dst)[NA], int tx0, int ty0,
+                             int w, int h, int wrap) {
+    constexpr int RH = TH + 4, Q = kRW / 4;
+    for (int k = threadIdx.x; k < RH * Q; k += kBlock) {
+        const int rr = k / Q, q = k - rr * Q;
+        const int gj = ty0 - 2 + rr;
+        int gi = tx0 - 4 + 4 * q;
+        bool ok = gj >= 0 && gj < h;
+        if (wrap) {
+            if (gi < 0) gi += w;
+            else if (gi >= w) gi -= w;
+        } else {
+            ok = ok && gi >= 0 && gi < w;
+        }
+        const int o = rr * kRW + 4 * q;
+        if (ok) {
+            const size_t g = (size_t)gj * w + gi;
+#pragma unroll
+            for (int a = 0; a < NA; a++) load4(src[a] + g, dst[a] + o);
+        } else {
+#pragma unroll
+            for (int a = 0; a < NA; a++)
+                dst[a][o] = dst[a][o + 1] = dst[a][o + 2] = dst[a][o + 3] = (T)0;
+        }
+    }
+}
+
+// Maps LDS position (lr, lc) to the level's cell (gi, gj), wrapped; false if off the level.
+__device__ inline bool cell_at(int lr, int lc, int tx0, int ty0, int w, int h, int wrap, int &gi,
+                               int &gj) {
+    gj = ty0 - 2 + lr;
+    gi = tx0 - 4 + lc;
+    if (gj < 0 || gj >= h) return false;
+    if (gi < 0 || gi >= w) {
+        if (!wrap) return false;
+        gi = gi < 0 ? gi + w : gi - w;
+    }
+    return true;
+}
+
+// Pre-smoothing of level L (nu = 2): z1 = wd r, z2 = z1 + wd (r - A z1) -> zout, and the
+// coarse residual C.r = sum over each 2x2 block of (r - A z2), fine row j0 then j0 + 1,
+// columns ascending, from +0 (MG:332-356).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pre2(Lvl<T> L, T *__restrict__ zout, Lvl<T> C,
+                                                 int tiles_x, int tiles,
+                                                 const CgState *__restrict__ st) {
+    constexpr int TH = TileRows<T>::TH, RH = TH + 4, N = RH * kRW;
+    __shared__ T sr[N], swd[N], sd[N], se[N], ss[N], z1[N], z2[N];
+    if (st && st->done) return;
+    const int t = tile_of_block(blockIdx.x, tiles);
+    const int tx0 = (t % tiles_x) * kTW, ty0 = (t / tiles_x) * TH;
+    const int w = L.w, h = L.h, wrap = L.wrap;
+    {
+        const T *src[5] = {L.r, L.wd, L.diag, L.east, L.south};
+        T *dst[5] = {sr, swd, sd, se, ss};
+        stage<T, TH, 5>(src, dst, tx0, ty0, w, h, wrap);
+    }
+    __syncthreads();
+    // z1 on rows [0, RH), columns [2, kRW - 2).
+    for (int k = threadIdx.x; k < RH * (kRW - 4); k += kBlock) {
+        const int lr = k / (kRW - 4), lc = 2 + k % (kRW - 4);
+        int gi, gj;
+        const int o = lr * kRW + lc;
+        z1[o] = cell_at(lr, lc, tx0, ty0, w, h, wrap, gi, gj) ? swd[o] * sr[o] : (T)0;
+    }
+    __syncthreads();
+    // z2 on rows [1, RH - 1), columns [3, kRW - 3).
+    for (int k = threadIdx.x; k < (RH - 2) * (kRW - 6); k += kBlock) {
+        const int lr = 1 + k / (kRW - 6), lc = 3 + k % (kRW - 6);
+        int gi, gj;
+        const int o = lr * kRW + lc;
+        T v = (T)0;
+        if (cell_at(lr, lc, tx0, ty0, w, h, wrap, gi, gj)) {
+            const T ax = tile_apply(z1, se, ss, sd, o, gi, gj, w, h, wrap);
+            v = z1[o] + swd[o] * (sr[o] - ax);
+        }
+        z2[o] = v;
+    }
+    __syncthreads();
+    // The tile's z2 out, and its residual r - A z2 into z1.
+    for (int k = threadIdx.x; k < TH * kTW; k += kBlock) {
+        const int lr = 2 + k / kTW, lc = 4 + k % kTW;
+        int gi, gj;
+        const int o = lr * kRW + lc;
+        T res = (T)0;
+        if (cell_at(lr, lc, tx0, ty0, w, h, wrap, gi, gj)) {
+            zout[(size_t)gj * w + gi] = z2[o];
+            res = sr[o] - tile_apply(z2, se, ss, sd, o, gi, gj, w, h, wrap);
+        }
+        z1[o] = res;
+    }
+    __syncthreads();
+    // Restriction: one coarse cell per thread.
+    for (int k = threadIdx.x; k < (TH / 2) * (kTW / 2); k += kBlock) {
+        const int cr = k / (kTW / 2), cc = k % (kTW / 2);
+        const int jc = ty0 / 2 + cr, ic = tx0 / 2 + cc;
+        if (jc >= C.h) continue;
+        const int lr = 2 + 2 * cr, lc = 4 + 2 * cc;
+        T acc = (T)0;
+        acc += z1[lr * kRW + lc];
+        acc += z1[lr * kRW + lc + 1];
+        if (2 * jc + 1 < h) {
+            acc += z1[(lr + 1) * kRW + lc];
+            acc += z1[(lr + 1) * kRW + lc + 1];
+        }
+        C.r[(size_t)jc * C.w + ic] = acc;
+    }
+}
+
+// Post-smoothing of level L (nu = 2): z' = zin + zc[j/2][i/2], z3 = z' + wd (r - A z'),
+// z4 = z3 + wd (r - A z3) -> zout; with dot, per tile sum dot * z4 into parts[tile].
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_post2(Lvl<T> L, const T *__restrict__ zin,
+                                                  const T *__restrict__ zc, int wc,
+                                                  T *__restrict__ zout, int tiles_x, int tiles,
+                                                  const double *__restrict__ dot,
+                                                  double *__restrict__ parts,
+                                                  const CgState *__restrict__ st) {
+    constexpr int TH = TileRows<T>::TH, RH = TH + 4, N = RH * kRW;
+    __shared__ T sr[N], swd[N], sd[N], se[N], ss[N], za[N], zb[N];
+    __shared__ double lds[kBlock / 64];
+    if (st && st->done) return;
+    const int t = tile_of_block(blockIdx.x, tiles);
+    const int tx0 = (t % tiles_x) * kTW, ty0 = (t / tiles_x) * TH;
+    const int w = L.w, h = L.h, wrap = L.wrap;
+    {
+        const T *src[6] = {L.r, L.wd, L.diag, L.east, L.south, zin};
+        T *dst[6] = {sr, swd, sd, se, ss, za};
+        stage<T, TH, 6>(src, dst, tx0, ty0, w, h, wrap);
+    }
+    __syncthreads();
+    // z' = z + P zc on rows [0, RH), columns [2, kRW - 2), in place.
+    for (int k = threadIdx.x; k < RH * (kRW - 4); k += kBlock) {
+        const int lr = k / (kRW - 4), lc = 2 + k % (kRW - 4);
+        int gi, gj;
+        if (cell_at(lr, lc, tx0, ty0, w, h, wrap, gi, gj)) {
+            const int o = lr * kRW + lc;
+            za[o] = za[o] + zc[(size_t)(gj >> 1) * wc + (gi >> 1)];
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < (RH - 2) * (kRW - 6); k += kBlock) {
+        const int lr = 1 + k / (kRW - 6), lc = 3 + k % (kRW - 6);
+        int gi, gj;
+        const int o = lr * kRW + lc;
+        T v = (T)0;
+        if (cell_at(lr, lc, tx0, ty0, w, h, wrap, gi, gj)) {
+            const T ax = tile_apply(za, se, ss, sd, o, gi, gj, w, h, wrap);
+            v = za[o] + swd[o] * (sr[o] - ax);
+        }
+        zb[o] = v;
+    }
+    __syncthreads();
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < TH * kTW; k += kBlock) {
+        const int lr = 2 + k / kTW, lc = 4 + k % kTW;
+        int gi, gj;
+        const int o = lr * kRW + lc;
+        if (cell_at(lr, lc, tx0, ty0, w, h, wrap, gi, gj)) {
+            const T ax = tile_apply(zb, se, ss, sd, o, gi, gj, w, h, wrap);
+            const T v = zb[o] + swd[o] * (sr[o] - ax);
+            const size_t c = (size_t)gj * w + gi;
+            zout[c] = v;
+            if (dot) acc += dot[c] * (double)v;
+        }
+    }
+    if (dot) {
+        const double sum = block_sum<kBlock>(acc, lds);
+        if (threadIdx.x == 0) parts[t] = sum;
     }
 }
 

@@ -64,7 +64,12 @@ def test_first_guess_bit_for_bit(ws, w, h, wrap):
     assert same_bits(dtdiff.lod(ws, d, b, kd, 2e6), O.lod(w, h, 2400.0, wrap, b, kd, 2e6))
 
 
-@pytest.mark.parametrize("w,h,wrap", SHAPES + [(300, 129, True), (513, 257, False)])
+# Widths that are multiples of 64 take the fused, LDS-tiled smoothing kernels on those levels.
+FUSED = [(64, 40, False), (128, 67, True), (256, 33, False), (192, 130, True), (1024, 512, True),
+         (2048, 1023, False)]
+
+
+@pytest.mark.parametrize("w,h,wrap", SHAPES + [(300, 129, True), (513, 257, False)] + FUSED)
 def test_cycle_levels_and_correction_bit_for_bit(ws, w, h, wrap):
     """The device cycle against the CPU cycle (multigrid/tests.rs:381-440), here bit for bit."""
     print(f"cycle {w}x{h} wrap={wrap}", flush=True)
@@ -79,9 +84,8 @@ def test_cycle_levels_and_correction_bit_for_bit(ws, w, h, wrap):
         assert (a["w"], a["h"], a["wrap"]) == (o["w"], o["h"], o["wrap"])
         for k in ("east", "south", "diag", "wd", "mass"):
             assert same_bits(a[k], o[k]), (l, k)
-    rng = O.SplitMix64(17 + w)
-    for _ in range(3):
-        r = np.array([rng.next_f64() - 0.5 for _ in range(w * h)])
+    for rnd in range(3):
+        r = F.splitmix_f64(17 + w, w * h, start=rnd * w * h) - 0.5
         z, rz = cy.precondition(r.astype(np.float32), r)
         zo, rzo = hy.precondition(r.astype(np.float32), r)
         assert same_bits(z, zo)
