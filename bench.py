@@ -97,7 +97,9 @@ def main():
         for name, kid in (("sweep_f32", dtdiff.KERNEL_SWEEP),
                           ("direction_f64", dtdiff.KERNEL_DIRECTION),
                           ("update_f64", dtdiff.KERNEL_UPDATE),
-                          ("restrict_f32", dtdiff.KERNEL_RESTRICT)):
+                          ("restrict_f32", dtdiff.KERNEL_RESTRICT),
+                          ("pre_smooth_f32", dtdiff.KERNEL_PRE),
+                          ("post_smooth_f32", dtdiff.KERNEL_POST)):
             ms, b = dtdiff.time_kernel(ws, kid, 20)
             kernels[name] = {"ms": ms, "bytes": b, "GB/s": b / (ms * 1e-3) / 1e9}
             log(f"kernel {name}: {ms * 1e3:.1f} us, {b / 1e6:.1f} MB, "

@@ -284,8 +284,8 @@ static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt)
     }
 }
 
-// The fused smoothing kernels serve levels whose width is a multiple of 64 (DTD_NO_FUSE=1 turns
-// them off, for comparison).
+// The fused smoothing kernels serve every level but the coarsest when nu = 2 (DTD_NO_FUSE=1
+// turns them off, for comparison).
 static bool fused_enabled() {
     static const bool on = [] {
         const char *v = getenv("DTD_NO_FUSE");
@@ -310,18 +310,16 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
     }
     const int nb = nblocks(L.n);
     Lvl<T> &C0 = lv[l + 1];
-    if (nu == 2 && L.w % kTW == 0 && fused_enabled()) {
-        // Fused, LDS-tiled smoothing (kernels.hpp): pre-smoothing and restriction in one
-        // launch, the coarse cycle, then both post-smoothing sweeps in one launch.
-        constexpr int TH = TileRows<T>::TH;
-        const int tiles_x = L.w / kTW, tiles = tiles_x * ((L.h + TH - 1) / TH);
-        k_pre2<T><<<tiles, kBlock, 0, s>>>(L, L.z[0], C0, tiles_x, tiles, st);
+    if (nu == 2 && fused_enabled()) {
+        // Fused smoothing (kernels.hpp): pre-smoothing and restriction in one launch, the coarse
+        // cycle, then both post-smoothing sweeps in one launch.
+        const int nbs = stream_blocks(L.w, L.h);
+        k_spre<T><<<nbs, kBlock, 0, s>>>(L, L.z[0], C0, st);
         LAUNCH_CHECK();
         vcycle<T>(lv, l + 1, nu, nullptr, parts, st, s);
-        k_post2<T><<<tiles, kBlock, 0, s>>>(L, L.z[0], C0.z[1], C0.w, L.z[1], tiles_x, tiles, dot,
-                                            parts, st);
+        k_spost<T><<<nbs, kBlock, 0, s>>>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts, st);
         LAUNCH_CHECK();
-        return tiles;
+        return nbs;
     }
     int cur = 0;
     k_first<T><<<nb, kBlock, 0, s>>>(L, L.z[0], st);
@@ -822,6 +820,20 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
                 Lvl<float> &C = ws.lf[1];
                 k_restrict<float><<<nblocks(C.n), kBlock, 0, s>>>(L, L.z[1], C, ws.st);
                 b = 20.0 * n + 4.0 * C.n;
+                break;
+            }
+            case DTD_KERNEL_PRE:  // r, wd, diag, east, south in; z and the coarse r out
+            case DTD_KERNEL_POST: {  // z, r, wd, diag, east, south, r64, coarse z in; z out
+                Lvl<float> &C = ws.lf[1];
+                const int nbs = stream_blocks(L.w, L.h);
+                if (kernel == DTD_KERNEL_PRE) {
+                    k_spre<float><<<nbs, kBlock, 0, s>>>(L, L.z[0], C, ws.st);
+                    b = 24.0 * n + 4.0 * C.n;
+                } else {
+                    k_spost<float><<<nbs, kBlock, 0, s>>>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r,
+                                                          ws.parts, ws.st);
+                    b = 36.0 * n + 4.0 * C.n;
+                }
                 break;
             }
             default:

@@ -553,231 +553,202 @@ __global__ void k_coarsest(Lvl<T> L, T *__restrict__ zout, int sweeps,
     }
 }
 
-// ---- fused, LDS-tiled smoothing of a level (nu = 2) ----------------------------------------
+// ---- fused, register-streaming smoothing of a level (nu = 2) -------------------------------
 //
-// A block takes a tile of TH x 64 cells of a level whose width is a multiple of 64 and stages
-// the level's arrays over the tile and a halo of two rows and four columns (float4-aligned) in
-// LDS.  The pre-smoothing kernel forms the first sweep, the second sweep and the residual
-// restriction in one pass (`vcycle`, MG:511-514); the post-smoothing kernel forms the corrected
-// sweep and the last sweep with its dot (MG:516-521).  Halo cells are recomputed by the
-// neighbouring tiles, so each cell's arithmetic is the unfused kernels', operation for
-// operation, and the results are identical.  Tiles are numbered so that each XCD (workgroups
-// are dealt to the eight XCDs in turn) takes a contiguous band of them, keeping the halo reads
-// in its own L2.
+// Each wave owns a band of 60 columns (lanes 2..61; lanes 0, 1, 62, 63 hold the halo columns,
+// wrapped when the level wraps) and 64 rows, and walks down the band one row per step keeping a
+// few rows of every quantity in registers; the east and west neighbours come from the adjacent
+// lanes by shuffles.  The pre-smoothing kernel forms, per step, the first sweep of the newest
+// row, the second sweep of the row above it and the residual of the row above that, restricted
+// to the coarse level in 2 x 2 blocks (`vcycle`, MG:511-514); the post-smoothing kernel forms the
+// corrected sweep and the last sweep with its dot the same way (MG:516-521).  Halo lanes and the
+// two rows above the band are recomputed by the neighbouring bands, so each cell's arithmetic is
+// the unfused kernels', operation for operation, and the results are identical.
 
-constexpr int kTW = 64;            // tile width (cells)
-constexpr int kRW = kTW + 8;       // staged width: 4 halo columns each side
-template <typename T> struct TileRows { static constexpr int TH = 16; };
-template <> struct TileRows<double> { static constexpr int TH = 12; };
+constexpr int kOwn = 60;       // owned columns per wave
+constexpr int kBandRows = 64;  // rows per wave band (even, so 2 x 2 blocks never straddle bands)
 
-__device__ inline int tile_of_block(int b, int tiles) {
-    if (tiles % 8) return b;
-    return (b % 8) * (tiles / 8) + b / 8;
-}
-
-// Four consecutive elements from global memory (16-byte loads).
-__device__ inline void load4(const float *p, float *d) {
-    const float4 v = *reinterpret_cast<const float4 *>(p);
-    d[0] = v.x, d[1] = v.y, d[2] = v.z, d[3] = v.w;
-}
-__device__ inline void load4(const double *p, double *d) {
-    const double2 a = reinterpret_cast<const double2 *>(p)[0];
-    const double2 b = reinterpret_cast<const double2 *>(p)[1];
-    d[0] = a.x, d[1] = a.y, d[2] = b.x, d[3] = b.y;
-}
-
-// One cell of A x on LDS arrays in the order of `Level::apply_row` (MG:203-240); k is the LDS
-// index, (gi, gj) the cell's place on the level.  For a wrapping level the staged halo holds
-// the wrapped columns, so the wrapped neighbours sit at k - 1 and k + 1 like any other.
 template <typename T>
-__device__ inline T tile_apply(const T *X, const T *E, const T *S, const T *D, int k, int gi,
+__device__ inline T from_west(T v) { return __shfl_up(v, 1, 64); }    // lane - 1's value
+template <typename T>
+__device__ inline T from_east(T v) { return __shfl_down(v, 1, 64); }  // lane + 1's value
+
+// A x at one cell from its own and its four neighbours' values, in the order of
+// `Level::apply_row` (MG:203-240); ew is the west neighbour's east coupling, sn the north
+// neighbour's south coupling.
+template <typename T>
+__device__ inline T lane_apply(T xc, T xe, T xw, T xs, T xn, T d, T e, T ew, T s, T sn, int gi,
                                int gj, int w, int h, int wrap) {
-    T o = D[k] * X[k];
-    if (gi + 1 < w) o -= E[k] * X[k + 1];
-    if (gi > 0) o -= E[k - 1] * X[k - 1];
+    T o = d * xc;
+    if (gi + 1 < w) o -= e * xe;
+    if (gi > 0) o -= ew * xw;
     if (wrap) {
-        if (gi == w - 1) o -= E[k] * X[k + 1];
-        if (gi == 0) o -= E[k - 1] * X[k - 1];
+        if (gi == w - 1) o -= e * xe;
+        if (gi == 0) o -= ew * xw;
     }
-    if (gj + 1 < h) o -= S[k] * X[k + kRW];
-    if (gj > 0) o -= S[k - kRW] * X[k - kRW];
+    if (gj + 1 < h) o -= s * xs;
+    if (gj > 0) o -= sn * xn;
     return o;
 }
 
-// Stages `na` arrays of a level over the tile at (tx0, ty0) and its halo into LDS (rows
-// ty0 - 2 .. ty0 + TH + 1, columns tx0 - 4 .. tx0 + 67, wrapped when the level wraps); cells off
-// the level are zero.
-template <typename T, int TH, int NA>
-__device__ inline void stage(const T *const (&src)[NA], T *const (&dst)[NA], int tx0, int ty0,
-                             int w, int h, int wrap) {
-    constexpr int RH = TH + 4, Q = kRW / 4;
-    for (int k = threadIdx.x; k < RH * Q; k += kBlock) {
-        const int rr = k / Q, q = k - rr * Q;
-        const int gj = ty0 - 2 + rr;
-        int gi = tx0 - 4 + 4 * q;
-        bool ok = gj >= 0 && gj < h;
-        if (wrap) {
-            if (gi < 0) gi += w;
-            else if (gi >= w) gi -= w;
-        } else {
-            ok = ok && gi >= 0 && gi < w;
-        }
-        const int o = rr * kRW + 4 * q;
-        if (ok) {
-            const size_t g = (size_t)gj * w + gi;
-#pragma unroll
-            for (int a = 0; a < NA; a++) load4(src[a] + g, dst[a] + o);
-        } else {
-#pragma unroll
-            for (int a = 0; a < NA; a++)
-                dst[a][o] = dst[a][o + 1] = dst[a][o + 2] = dst[a][o + 3] = (T)0;
-        }
+struct Band {
+    int gi;      // the lane's column on the level (wrapped)
+    int col_ok;  // the column is on the level
+    int own;     // the lane owns its column (writes it)
+    int row0;    // first row of the band
+    int active;  // the wave has a band
+};
+
+__device__ inline Band band_of(int w, int h, int wrap) {
+    const int lane = threadIdx.x & 63;
+    const int bands_x = (w + kOwn - 1) / kOwn, bands_y = (h + kBandRows - 1) / kBandRows;
+    const int wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    Band b;
+    b.active = wave < bands_x * bands_y;
+    const int bx = b.active ? wave % bands_x : 0, by = b.active ? wave / bands_x : 0;
+    const int raw = bx * kOwn - 2 + lane;
+    b.row0 = by * kBandRows;
+    if (wrap) {
+        int g = raw % w;
+        b.gi = g < 0 ? g + w : g;
+        b.col_ok = 1;
+    } else {
+        b.gi = raw;
+        b.col_ok = raw >= 0 && raw < w;
     }
+    b.own = b.active && lane >= 2 && lane < 2 + kOwn && raw >= 0 && raw < w;
+    return b;
 }
 
-// Maps LDS position (lr, lc) to the level's cell (gi, gj), wrapped; false if off the level.
-__device__ inline bool cell_at(int lr, int lc, int tx0, int ty0, int w, int h, int wrap, int &gi,
-                               int &gj) {
-    gj = ty0 - 2 + lr;
-    gi = tx0 - 4 + lc;
-    if (gj < 0 || gj >= h) return false;
-    if (gi < 0 || gi >= w) {
-        if (!wrap) return false;
-        gi = gi < 0 ? gi + w : gi - w;
-    }
-    return true;
+__host__ __device__ inline int stream_blocks(int w, int h) {
+    const int waves = ((w + kOwn - 1) / kOwn) * ((h + kBandRows - 1) / kBandRows);
+    return (waves + kBlock / 64 - 1) / (kBlock / 64);
 }
 
-// Pre-smoothing of level L (nu = 2): z1 = wd r, z2 = z1 + wd (r - A z1) -> zout, and the
-// coarse residual C.r = sum over each 2x2 block of (r - A z2), fine row j0 then j0 + 1,
-// columns ascending, from +0 (MG:332-356).
+// Pre-smoothing of level L (nu = 2): z1 = wd r, z2 = z1 + wd (r - A z1) -> zout, and the coarse
+// residual C.r = sum over each 2x2 block of (r - A z2): fine row j0 then j0 + 1, columns
+// ascending, from +0 (MG:332-356).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pre2(Lvl<T> L, T *__restrict__ zout, Lvl<T> C,
-                                                 int tiles_x, int tiles,
+__global__ __launch_bounds__(kBlock) void k_spre(Lvl<T> L, T *__restrict__ zout, Lvl<T> C,
                                                  const CgState *__restrict__ st) {
-    constexpr int TH = TileRows<T>::TH, RH = TH + 4, N = RH * kRW;
-    __shared__ T sr[N], swd[N], sd[N], se[N], ss[N], z1[N], z2[N];
     if (st && st->done) return;
-    const int t = tile_of_block(blockIdx.x, tiles);
-    const int tx0 = (t % tiles_x) * kTW, ty0 = (t / tiles_x) * TH;
     const int w = L.w, h = L.h, wrap = L.wrap;
-    {
-        const T *src[5] = {L.r, L.wd, L.diag, L.east, L.south};
-        T *dst[5] = {sr, swd, sd, se, ss};
-        stage<T, TH, 5>(src, dst, tx0, ty0, w, h, wrap);
-    }
-    __syncthreads();
-    // z1 on rows [0, RH), columns [2, kRW - 2).
-    for (int k = threadIdx.x; k < RH * (kRW - 4); k += kBlock) {
-        const int lr = k / (kRW - 4), lc = 2 + k % (kRW - 4);
-        int gi, gj;
-        const int o = lr * kRW + lc;
-        z1[o] = cell_at(lr, lc, tx0, ty0, w, h, wrap, gi, gj) ? swd[o] * sr[o] : (T)0;
-    }
-    __syncthreads();
-    // z2 on rows [1, RH - 1), columns [3, kRW - 3).
-    for (int k = threadIdx.x; k < (RH - 2) * (kRW - 6); k += kBlock) {
-        const int lr = 1 + k / (kRW - 6), lc = 3 + k % (kRW - 6);
-        int gi, gj;
-        const int o = lr * kRW + lc;
-        T v = (T)0;
-        if (cell_at(lr, lc, tx0, ty0, w, h, wrap, gi, gj)) {
-            const T ax = tile_apply(z1, se, ss, sd, o, gi, gj, w, h, wrap);
-            v = z1[o] + swd[o] * (sr[o] - ax);
+    const Band bd = band_of(w, h, wrap);
+    const int gi = bd.gi;
+    // Rows by age: a = newest (jk), b = jk - 1, c = jk - 2, d = jk - 3.
+    T z1a = 0, z1b = 0, z1c = 0, z2b = 0, z2c = 0, z2d = 0;
+    T ra = 0, rb = 0, rc = 0, wda = 0, wdb = 0, dga = 0, dgb = 0, dgc = 0;
+    T ea = 0, eb = 0, ec = 0, sa = 0, sb = 0, sc = 0, sd = 0;
+    T acc = 0;
+    const int steps = bd.active ? kBandRows + 4 : 0;
+    for (int k = 0; k < steps; k++) {
+        const int ja = bd.row0 - 2 + k, jb = ja - 1, jc = ja - 2;
+        // Shift the window, then load the newest row.
+        z1c = z1b, z1b = z1a;
+        z2d = z2c, z2c = z2b;
+        rc = rb, rb = ra, wdb = wda, dgc = dgb, dgb = dga, ec = eb, eb = ea;
+        sd = sc, sc = sb, sb = sa;
+        const bool oka = ja >= 0 && ja < h && bd.col_ok;
+        if (oka) {
+            const size_t g = (size_t)ja * w + gi;
+            ra = L.r[g], wda = L.wd[g], dga = L.diag[g], ea = L.east[g], sa = L.south[g];
+            z1a = wda * ra;
+        } else {
+            ra = wda = dga = ea = sa = z1a = (T)0;
         }
-        z2[o] = v;
-    }
-    __syncthreads();
-    // The tile's z2 out, and its residual r - A z2 into z1.
-    for (int k = threadIdx.x; k < TH * kTW; k += kBlock) {
-        const int lr = 2 + k / kTW, lc = 4 + k % kTW;
-        int gi, gj;
-        const int o = lr * kRW + lc;
-        T res = (T)0;
-        if (cell_at(lr, lc, tx0, ty0, w, h, wrap, gi, gj)) {
-            zout[(size_t)gj * w + gi] = z2[o];
-            res = sr[o] - tile_apply(z2, se, ss, sd, o, gi, gj, w, h, wrap);
+        // Second sweep of row b.
+        {
+            const T xe = from_east(z1b), xw = from_west(z1b), ew = from_west(eb);
+            T v = (T)0;
+            if (jb >= 0 && jb < h && bd.col_ok) {
+                const T ax = lane_apply(z1b, xe, xw, z1a, z1c, dgb, eb, ew, sb, sc, gi, jb, w, h,
+                                        wrap);
+                v = z1b + wdb * (rb - ax);
+            }
+            z2b = v;
+            if (bd.own && jb >= bd.row0 && jb < bd.row0 + kBandRows && jb < h)
+                zout[(size_t)jb * w + gi] = v;
         }
-        z1[o] = res;
-    }
-    __syncthreads();
-    // Restriction: one coarse cell per thread.
-    for (int k = threadIdx.x; k < (TH / 2) * (kTW / 2); k += kBlock) {
-        const int cr = k / (kTW / 2), cc = k % (kTW / 2);
-        const int jc = ty0 / 2 + cr, ic = tx0 / 2 + cc;
-        if (jc >= C.h) continue;
-        const int lr = 2 + 2 * cr, lc = 4 + 2 * cc;
-        T acc = (T)0;
-        acc += z1[lr * kRW + lc];
-        acc += z1[lr * kRW + lc + 1];
-        if (2 * jc + 1 < h) {
-            acc += z1[(lr + 1) * kRW + lc];
-            acc += z1[(lr + 1) * kRW + lc + 1];
+        // Residual of row c, restricted in 2 x 2 blocks.
+        {
+            const T xe = from_east(z2c), xw = from_west(z2c), ew = from_west(ec);
+            T res = (T)0;
+            const bool okc = jc >= 0 && jc < h && bd.col_ok;
+            if (okc)
+                res = rc - lane_apply(z2c, xe, xw, z2b, z2d, dgc, ec, ew, sc, sd, gi, jc, w, h,
+                                      wrap);
+            const T rese = from_east(res);
+            if (bd.own && okc && jc >= bd.row0 && (gi & 1) == 0) {
+                if ((jc & 1) == 0) acc = (T)0;
+                acc += res;
+                if (gi + 1 < w) acc += rese;
+                if ((jc & 1) == 1 || jc == h - 1)
+                    C.r[(size_t)(jc >> 1) * C.w + (gi >> 1)] = acc;
+            }
         }
-        C.r[(size_t)jc * C.w + ic] = acc;
     }
 }
 
 // Post-smoothing of level L (nu = 2): z' = zin + zc[j/2][i/2], z3 = z' + wd (r - A z'),
-// z4 = z3 + wd (r - A z3) -> zout; with dot, per tile sum dot * z4 into parts[tile].
+// z4 = z3 + wd (r - A z3) -> zout; with dot, per block sum dot * z4 into parts[block].
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_post2(Lvl<T> L, const T *__restrict__ zin,
+__global__ __launch_bounds__(kBlock) void k_spost(Lvl<T> L, const T *__restrict__ zin,
                                                   const T *__restrict__ zc, int wc,
-                                                  T *__restrict__ zout, int tiles_x, int tiles,
+                                                  T *__restrict__ zout,
                                                   const double *__restrict__ dot,
                                                   double *__restrict__ parts,
                                                   const CgState *__restrict__ st) {
-    constexpr int TH = TileRows<T>::TH, RH = TH + 4, N = RH * kRW;
-    __shared__ T sr[N], swd[N], sd[N], se[N], ss[N], za[N], zb[N];
     __shared__ double lds[kBlock / 64];
     if (st && st->done) return;
-    const int t = tile_of_block(blockIdx.x, tiles);
-    const int tx0 = (t % tiles_x) * kTW, ty0 = (t / tiles_x) * TH;
     const int w = L.w, h = L.h, wrap = L.wrap;
-    {
-        const T *src[6] = {L.r, L.wd, L.diag, L.east, L.south, zin};
-        T *dst[6] = {sr, swd, sd, se, ss, za};
-        stage<T, TH, 6>(src, dst, tx0, ty0, w, h, wrap);
-    }
-    __syncthreads();
-    // z' = z + P zc on rows [0, RH), columns [2, kRW - 2), in place.
-    for (int k = threadIdx.x; k < RH * (kRW - 4); k += kBlock) {
-        const int lr = k / (kRW - 4), lc = 2 + k % (kRW - 4);
-        int gi, gj;
-        if (cell_at(lr, lc, tx0, ty0, w, h, wrap, gi, gj)) {
-            const int o = lr * kRW + lc;
-            za[o] = za[o] + zc[(size_t)(gj >> 1) * wc + (gi >> 1)];
-        }
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < (RH - 2) * (kRW - 6); k += kBlock) {
-        const int lr = 1 + k / (kRW - 6), lc = 3 + k % (kRW - 6);
-        int gi, gj;
-        const int o = lr * kRW + lc;
-        T v = (T)0;
-        if (cell_at(lr, lc, tx0, ty0, w, h, wrap, gi, gj)) {
-            const T ax = tile_apply(za, se, ss, sd, o, gi, gj, w, h, wrap);
-            v = za[o] + swd[o] * (sr[o] - ax);
-        }
-        zb[o] = v;
-    }
-    __syncthreads();
+    const Band bd = band_of(w, h, wrap);
+    const int gi = bd.gi;
+    T zpa = 0, zpb = 0, zpc = 0, z3b = 0, z3c = 0, z3d = 0;
+    T ra = 0, rb = 0, rc = 0, wda = 0, wdb = 0, wdc = 0, dga = 0, dgb = 0, dgc = 0;
+    T ea = 0, eb = 0, ec = 0, sa = 0, sb = 0, sc = 0, sd = 0;
     double acc = 0.0;
-    for (int k = threadIdx.x; k < TH * kTW; k += kBlock) {
-        const int lr = 2 + k / kTW, lc = 4 + k % kTW;
-        int gi, gj;
-        const int o = lr * kRW + lc;
-        if (cell_at(lr, lc, tx0, ty0, w, h, wrap, gi, gj)) {
-            const T ax = tile_apply(zb, se, ss, sd, o, gi, gj, w, h, wrap);
-            const T v = zb[o] + swd[o] * (sr[o] - ax);
-            const size_t c = (size_t)gj * w + gi;
-            zout[c] = v;
-            if (dot) acc += dot[c] * (double)v;
+    const int steps = bd.active ? kBandRows + 4 : 0;
+    for (int k = 0; k < steps; k++) {
+        const int ja = bd.row0 - 2 + k, jb = ja - 1, jc = ja - 2;
+        zpc = zpb, zpb = zpa;
+        z3d = z3c, z3c = z3b;
+        rc = rb, rb = ra, wdc = wdb, wdb = wda, dgc = dgb, dgb = dga, ec = eb, eb = ea;
+        sd = sc, sc = sb, sb = sa;
+        const bool oka = ja >= 0 && ja < h && bd.col_ok;
+        if (oka) {
+            const size_t g = (size_t)ja * w + gi;
+            ra = L.r[g], wda = L.wd[g], dga = L.diag[g], ea = L.east[g], sa = L.south[g];
+            zpa = zin[g] + zc[(size_t)(ja >> 1) * wc + (gi >> 1)];
+        } else {
+            ra = wda = dga = ea = sa = zpa = (T)0;
+        }
+        {
+            const T xe = from_east(zpb), xw = from_west(zpb), ew = from_west(eb);
+            T v = (T)0;
+            if (jb >= 0 && jb < h && bd.col_ok) {
+                const T ax = lane_apply(zpb, xe, xw, zpa, zpc, dgb, eb, ew, sb, sc, gi, jb, w, h,
+                                        wrap);
+                v = zpb + wdb * (rb - ax);
+            }
+            z3b = v;
+        }
+        {
+            const T xe = from_east(z3c), xw = from_west(z3c), ew = from_west(ec);
+            if (jc >= 0 && jc < h && bd.col_ok) {
+                const T ax = lane_apply(z3c, xe, xw, z3b, z3d, dgc, ec, ew, sc, sd, gi, jc, w, h,
+                                        wrap);
+                const T v = z3c + wdc * (rc - ax);
+                if (bd.own && jc >= bd.row0) {
+                    const size_t g = (size_t)jc * w + gi;
+                    zout[g] = v;
+                    if (dot) acc += dot[g] * (double)v;
+                }
+            }
         }
     }
     if (dot) {
         const double sum = block_sum<kBlock>(acc, lds);
-        if (threadIdx.x == 0) parts[t] = sum;
+        if (threadIdx.x == 0) parts[blockIdx.x] = sum;
     }
 }
 

@@ -206,3 +206,17 @@ def test_cycle_variants_reach_the_same_answer(prec, guess):
     xo, _ = O.solve(d.w, d.h, d.dx, d.wrap_x, h0, kd, dt, tol)
     assert its < 400
     assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("w,h,wrap", [(200, 96, True), (129, 67, False), (64, 40, True)])
+def test_unfused_cycle_with_three_sweeps_bit_for_bit(w, h, wrap):
+    """nu = 3 takes the one-sweep-per-launch kernels; the cycle still equals the oracle's."""
+    wsn = dtdiff.Workspace(nu=3)
+    b, kd = case(w, h, 21)
+    e, s, g = O.operator(w, h, 2400.0, wrap, kd, 2e6)
+    hy = O.Hierarchy(w, h, wrap, e, s, g, 0.8, 3, 0.5)
+    cy = dtdiff.Cycle(wsn, dtdiff.Dims(w, h, 2400.0, wrap), e, s, g)
+    r = F.splitmix_f64(5 + w, w * h) - 0.5
+    z, rz = cy.precondition(r.astype(np.float32), r)
+    zo, rzo = hy.precondition(r.astype(np.float32), r)
+    assert same_bits(z, zo) and abs(rz - rzo) <= 1e-12 * abs(rzo)
