@@ -41,6 +41,8 @@ def main():
     ap.add_argument("--cycle", choices=["f32", "f64"], default="f32")
     ap.add_argument("--guess", choices=["lod", "rhs"], default="lod")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="one solve, then only the per-kernel timings (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -74,6 +76,18 @@ def main():
         hd.copy_from(h0d)  # synchronous device-to-device copy
         return dtdiff.solve_rel_device(ws, d, hd, kdd, dt, args.rel)
 
+    if args.kernels_only:
+        st = step()
+        log(f"solve: {st.iterations} its, rel {st.rel_residual:.2e}, {st.ms_total:.2f} ms")
+        for name, kid in (("copy_f32", dtdiff.KERNEL_COPY), ("stream5_f32", dtdiff.KERNEL_STREAM5),
+                          ("pre_smooth_f32", dtdiff.KERNEL_PRE),
+                          ("post_smooth_f32", dtdiff.KERNEL_POST),
+                          ("direction_f64", dtdiff.KERNEL_DIRECTION),
+                          ("update_f64", dtdiff.KERNEL_UPDATE)):
+            ms, b = dtdiff.time_kernel(ws, kid, 5)
+            log(f"kernel {name}: {ms * 1e3:.1f} us, {b / 1e6:.1f} MB, {b / ms / 1e6:.0f} GB/s")
+        return
+
     for k in range(args.warmup):
         st = step()
         log(f"warmup {k}: {st.iterations} its, rel {st.rel_residual:.2e}, {st.ms_total:.2f} ms")
@@ -99,7 +113,9 @@ def main():
                           ("update_f64", dtdiff.KERNEL_UPDATE),
                           ("restrict_f32", dtdiff.KERNEL_RESTRICT),
                           ("pre_smooth_f32", dtdiff.KERNEL_PRE),
-                          ("post_smooth_f32", dtdiff.KERNEL_POST)):
+                          ("post_smooth_f32", dtdiff.KERNEL_POST),
+                          ("copy_f32", dtdiff.KERNEL_COPY),
+                          ("stream5_f32", dtdiff.KERNEL_STREAM5)):
             ms, b = dtdiff.time_kernel(ws, kid, 20)
             kernels[name] = {"ms": ms, "bytes": b, "GB/s": b / (ms * 1e-3) / 1e9}
             log(f"kernel {name}: {ms * 1e3:.1f} us, {b / 1e6:.1f} MB, "

@@ -16,6 +16,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -167,6 +168,7 @@ static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
             L.wd = (double *)own(nl, 8);
             L.z[0] = (double *)own(nl, 8);
             L.z[1] = (double *)own(nl, 8);
+            L.coef = (double *)own(4 * nl, 8);
             ws.ld.push_back(L);
         } else {
             Lvl<float> L{};
@@ -179,6 +181,7 @@ static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
             L.mass = l == 0 ? nullptr : (float *)own(nl, 4);
             L.z[0] = (float *)own(nl, 4);
             L.z[1] = (float *)own(nl, 4);
+            L.coef = (float *)own(4 * nl, 4);
             ws.lf.push_back(L);
         }
     }
@@ -259,6 +262,10 @@ static void build_coarse(std::vector<Lvl<T>> &lv, T omega, T scale, hipStream_t 
         k_cdiag<T><<<nblocks(lv[l].n), kBlock, 0, s>>>(lv[l], omega);
         LAUNCH_CHECK();
     }
+    for (size_t l = 0; l < lv.size(); l++) {
+        k_pack<T><<<nblocks(lv[l].n), kBlock, 0, s>>>(lv[l]);
+        LAUNCH_CHECK();
+    }
 }
 
 // `Operator::new` on the device from kd (already on the device) and the cycle's levels.
@@ -282,6 +289,44 @@ static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt)
         LAUNCH_CHECK();
         build_coarse<float>(ws.lf, ctx->opts.omega, ctx->opts.scale, s);
     }
+}
+
+// Band height for a streaming kernel on a w x h level: the smallest multiple of 4 rows for which
+// the level's bands fit in one round of the kernel's resident waves (all bands then run at
+// once, none waits for a second round), at least 4.
+template <int V>
+static int band_height(const void *kernel, int w, int h) {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0));
+    const long resident = (long)std::max(per_cu, 1) * cus * (kBlock / 64);
+    const long nby = std::max(1L, resident / bands_x<V>(w));
+    long th = (h + nby - 1) / nby;
+    th = std::max(4L, (th + 3) / 4 * 4);
+    return (int)th;
+}
+
+template <typename T, int V>
+static void launch_pre(Lvl<T> &L, Lvl<T> &C, T *zout, const CgState *st, hipStream_t s) {
+    const int th = band_height<V>((const void *)k_spre<T, V>, L.w, L.h);
+    const int nb = (stream_waves<V>(L.w, L.h, th) + kBlock / 64 - 1) / (kBlock / 64);
+    k_spre<T, V><<<nb, kBlock, 0, s>>>(L, zout, C, th, st);
+    LAUNCH_CHECK();
+}
+
+template <typename T, int V>
+static int launch_post(Lvl<T> &L, const T *zin, const T *zc, int wc, T *zout, const double *dot,
+                       double *parts, const CgState *st, hipStream_t s) {
+    const int th = band_height<V>((const void *)k_spost<T, V>, L.w, L.h);
+    const int nb = (stream_waves<V>(L.w, L.h, th) + kBlock / 64 - 1) / (kBlock / 64);
+    k_spost<T, V><<<nb, kBlock, 0, s>>>(L, zin, zc, wc, zout, th, dot, parts, st);
+    LAUNCH_CHECK();
+    return nb;
 }
 
 // The fused smoothing kernels serve every level but the coarsest when nu = 2 (DTD_NO_FUSE=1
@@ -312,13 +357,17 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
     Lvl<T> &C0 = lv[l + 1];
     if (nu == 2 && fused_enabled()) {
         // Fused smoothing (kernels.hpp): pre-smoothing and restriction in one launch, the coarse
-        // cycle, then both post-smoothing sweeps in one launch.
-        const int nbs = stream_blocks(L.w, L.h);
-        k_spre<T><<<nbs, kBlock, 0, s>>>(L, L.z[0], C0, st);
-        LAUNCH_CHECK();
+        // cycle, then both post-smoothing sweeps in one launch; 8-byte lanes when the width
+        // allows.
+        constexpr int VW = 8 / sizeof(T);
+        const bool wide = L.w % VW == 0;
+        if (wide)
+            launch_pre<T, VW>(L, C0, L.z[0], st, s);
+        else
+            launch_pre<T, 1>(L, C0, L.z[0], st, s);
         vcycle<T>(lv, l + 1, nu, nullptr, parts, st, s);
-        k_spost<T><<<nbs, kBlock, 0, s>>>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts, st);
-        LAUNCH_CHECK();
+        const int nbs = wide ? launch_post<T, VW>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts, st, s)
+                             : launch_post<T, 1>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts, st, s);
         return nbs;
     }
     int cur = 0;
@@ -782,6 +831,22 @@ int dtd_lod(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h_i
     });
 }
 
+// Calibration kernels of dtd_time_kernel: a float4 copy, and five float4 input streams combined
+// into one output (the smoother's stream count without its stencil).
+__global__ void k_copy4(long n4, const float4 *__restrict__ a, float4 *__restrict__ o) {
+    const long k = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (k < n4) o[k] = a[k];
+}
+__global__ void k_stream5(long n4, const float4 *__restrict__ a, const float4 *__restrict__ b,
+                          const float4 *__restrict__ c, const float4 *__restrict__ d,
+                          const float4 *__restrict__ e, float4 *__restrict__ o) {
+    const long k = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n4) return;
+    const float4 p = a[k], q = b[k], r = c[k], t = d[k], u = e[k];
+    o[k] = make_float4(p.x + q.x * (r.x - t.x * u.x), p.y + q.y * (r.y - t.y * u.y),
+                       p.z + q.z * (r.z - t.z * u.z), p.w + q.w * (r.w - t.w * u.w));
+}
+
 int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *bytes) {
     return guard(ctx, [&] {
         begin(ctx);
@@ -824,15 +889,30 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
             }
             case DTD_KERNEL_PRE:  // r, wd, diag, east, south in; z and the coarse r out
             case DTD_KERNEL_POST: {  // z, r, wd, diag, east, south, r64, coarse z in; z out
+                if (L.w % 2) throw Error{DTD_ESTATE, "the finest width is odd"};
                 Lvl<float> &C = ws.lf[1];
-                const int nbs = stream_blocks(L.w, L.h);
                 if (kernel == DTD_KERNEL_PRE) {
-                    k_spre<float><<<nbs, kBlock, 0, s>>>(L, L.z[0], C, ws.st);
+                    launch_pre<float, 2>(L, C, L.z[0], ws.st, s);
                     b = 24.0 * n + 4.0 * C.n;
                 } else {
-                    k_spost<float><<<nbs, kBlock, 0, s>>>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r,
-                                                          ws.parts, ws.st);
+                    launch_post<float, 2>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts, ws.st, s);
                     b = 36.0 * n + 4.0 * C.n;
+                }
+                break;
+            }
+            case DTD_KERNEL_COPY:  // r32 in, z out
+            case DTD_KERNEL_STREAM5: {  // r, wd, diag, east, south in; z out
+                const long n4 = ws.n / 4;
+                const int nb4 = (int)((n4 + kBlock - 1) / kBlock);
+                if (kernel == DTD_KERNEL_COPY) {
+                    k_copy4<<<nb4, kBlock, 0, s>>>(n4, (const float4 *)L.r, (float4 *)L.z[0]);
+                    b = 8.0 * 4 * n4;
+                } else {
+                    k_stream5<<<nb4, kBlock, 0, s>>>(n4, (const float4 *)L.r, (const float4 *)L.wd,
+                                                     (const float4 *)L.diag,
+                                                     (const float4 *)L.east,
+                                                     (const float4 *)L.south, (float4 *)L.z[0]);
+                    b = 24.0 * 4 * n4;
                 }
                 break;
             }

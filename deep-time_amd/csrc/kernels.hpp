@@ -395,7 +395,17 @@ struct Lvl {
     int w, h, wrap, n;
     T *east, *south, *diag, *wd, *mass;  // mass: nullptr on the finest level (1 a cell)
     T *r, *z[2];
+    T *coef;  // east, south, diag, wd of each cell side by side (4 values a cell)
 };
+
+// Packs a level's static operator into one record a cell for the streaming kernels.
+template <typename T>
+__global__ void k_pack(Lvl<T> L) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= L.n) return;
+    T *q = L.coef + 4 * (size_t)c;
+    q[0] = L.east[c], q[1] = L.south[c], q[2] = L.diag[c], q[3] = L.wd[c];
+}
 
 // `Level::coarsen` (MG:117-188): one thread per coarse cell.
 template <typename T>
@@ -565,13 +575,68 @@ __global__ void k_coarsest(Lvl<T> L, T *__restrict__ zout, int sweeps,
 // two rows above the band are recomputed by the neighbouring bands, so each cell's arithmetic is
 // the unfused kernels', operation for operation, and the results are identical.
 
-constexpr int kOwn = 60;       // owned columns per wave
-constexpr int kBandRows = 64;  // rows per wave band (even, so 2 x 2 blocks never straddle bands)
+// Cells per lane: V consecutive columns (8 bytes a lane: two f32 or one f64 cell).  Halo lanes
+// on each side: enough for two columns.
+template <int V> struct LaneGeom {
+    static constexpr int halo = V >= 2 ? 1 : 2;
+    static constexpr int own = (64 - 2 * halo) * V;  // owned columns per wave
+};
 
 template <typename T>
 __device__ inline T from_west(T v) { return __shfl_up(v, 1, 64); }    // lane - 1's value
 template <typename T>
 __device__ inline T from_east(T v) { return __shfl_down(v, 1, 64); }  // lane + 1's value
+
+template <typename T, int V>
+__device__ inline void loadv(const T *p, T (&d)[V]) {
+    if constexpr (V * sizeof(T) == 8 && V == 2) {
+        const float2 q = *reinterpret_cast<const float2 *>(p);
+        d[0] = q.x, d[1] = q.y;
+    } else if constexpr (V * sizeof(T) == 16) {
+        if constexpr (sizeof(T) == 4) {
+            const float4 q = *reinterpret_cast<const float4 *>(p);
+            d[0] = q.x, d[1] = q.y, d[2] = q.z, d[3] = q.w;
+        } else {
+            const double2 q = *reinterpret_cast<const double2 *>(p);
+            d[0] = q.x, d[1] = q.y;
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < V; v++) d[v] = p[v];
+    }
+}
+// The packed operator of V consecutive cells (16- or 32-byte records).
+template <typename T, int V>
+__device__ inline void load_coef(const T *coef, size_t g, T (&e)[V], T (&s)[V], T (&d)[V],
+                                 T (&wd)[V]) {
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        const T *q = coef + 4 * (g + v);
+        if constexpr (sizeof(T) == 4) {
+            const float4 a = *reinterpret_cast<const float4 *>(q);
+            e[v] = a.x, s[v] = a.y, d[v] = a.z, wd[v] = a.w;
+        } else {
+            const double2 a = reinterpret_cast<const double2 *>(q)[0];
+            const double2 b = reinterpret_cast<const double2 *>(q)[1];
+            e[v] = a.x, s[v] = a.y, d[v] = b.x, wd[v] = b.y;
+        }
+    }
+}
+
+template <typename T, int V>
+__device__ inline void storev(T *p, const T (&d)[V]) {
+    if constexpr (V * sizeof(T) == 8 && V == 2) {
+        *reinterpret_cast<float2 *>(p) = make_float2(d[0], d[1]);
+    } else if constexpr (V * sizeof(T) == 16) {
+        if constexpr (sizeof(T) == 4)
+            *reinterpret_cast<float4 *>(p) = make_float4(d[0], d[1], d[2], d[3]);
+        else
+            *reinterpret_cast<double2 *>(p) = make_double2(d[0], d[1]);
+    } else {
+#pragma unroll
+        for (int v = 0; v < V; v++) p[v] = d[v];
+    }
+}
 
 // A x at one cell from its own and its four neighbours' values, in the order of
 // `Level::apply_row` (MG:203-240); ew is the west neighbour's east coupling, sn the north
@@ -591,161 +656,261 @@ __device__ inline T lane_apply(T xc, T xe, T xw, T xs, T xn, T d, T e, T ew, T s
     return o;
 }
 
+// A x for the V cells of a lane.  `inner`: the row has rows above and below and the lane's
+// cells have non-wrapped east and west neighbours, so every term is present and the five are
+// subtracted in `apply_row`'s order without tests (most cells take this path).
+template <typename T, int V>
+__device__ inline void lane_ax(const T (&x)[V], const T (&xs)[V], const T (&xn)[V],
+                               const T (&d)[V], const T (&e)[V], const T (&s)[V],
+                               const T (&sn)[V], int gi0, int gj, int w, int h, int wrap,
+                               bool inner, T (&ax)[V]) {
+    const T xe0 = from_east(x[0]), xwl = from_west(x[V - 1]), ewl = from_west(e[V - 1]);
+    if (inner) {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const T xe = v + 1 < V ? x[v + 1] : xe0;
+            const T xw = v > 0 ? x[v - 1] : xwl;
+            const T ew = v > 0 ? e[v - 1] : ewl;
+            T o = d[v] * x[v];
+            o -= e[v] * xe;
+            o -= ew * xw;
+            o -= s[v] * xs[v];
+            o -= sn[v] * xn[v];
+            ax[v] = o;
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const T xe = v + 1 < V ? x[v + 1] : xe0;
+            const T xw = v > 0 ? x[v - 1] : xwl;
+            const T ew = v > 0 ? e[v - 1] : ewl;
+            ax[v] = lane_apply(x[v], xe, xw, xs[v], xn[v], d[v], e[v], ew, s[v], sn[v], gi0 + v,
+                               gj, w, h, wrap);
+        }
+    }
+}
+
+// One sweep of a row of V cells per lane: out = x + wd (r - A x) for the cells of row gj (xs,
+// xn: the rows below and above; the row's own and the north row's couplings); 0 where !ok.
+template <typename T, int V>
+__device__ inline void lane_sweep(const T (&x)[V], const T (&xs)[V], const T (&xn)[V],
+                                  const T (&r)[V], const T (&wd)[V], const T (&d)[V],
+                                  const T (&e)[V], const T (&s)[V], const T (&sn)[V], int gi0,
+                                  int gj, int w, int h, int wrap, bool ok, bool inner,
+                                  T (&out)[V]) {
+    T ax[V];
+    lane_ax<T, V>(x, xs, xn, d, e, s, sn, gi0, gj, w, h, wrap, inner, ax);
+#pragma unroll
+    for (int v = 0; v < V; v++) out[v] = ok ? x[v] + wd[v] * (r[v] - ax[v]) : (T)0;
+}
+
+// A level's bands: columns of LaneGeom<V>::own, rows of `th` (a multiple of 4, chosen by the
+// host so that all bands run at once in one round of resident waves).
+template <int V>
+__host__ __device__ inline int bands_x(int w) {
+    return (w + LaneGeom<V>::own - 1) / LaneGeom<V>::own;
+}
+template <int V>
+__host__ __device__ inline int stream_waves(int w, int h, int th) {
+    return bands_x<V>(w) * ((h + th - 1) / th);
+}
+
 struct Band {
-    int gi;      // the lane's column on the level (wrapped)
-    int col_ok;  // the column is on the level
-    int own;     // the lane owns its column (writes it)
+    int inner;   // the lane's columns all have non-wrapped east and west neighbours
+    int gi0;     // the lane's first column on the level (wrapped)
+    int col_ok;  // the lane's columns are on the level
+    int own;     // the lane owns its columns (writes them)
     int row0;    // first row of the band
+    int rows;    // band height
     int active;  // the wave has a band
 };
 
-__device__ inline Band band_of(int w, int h, int wrap) {
+template <int V>
+__device__ inline Band band_of(int w, int h, int wrap, int th) {
     const int lane = threadIdx.x & 63;
-    const int bands_x = (w + kOwn - 1) / kOwn, bands_y = (h + kBandRows - 1) / kBandRows;
+    constexpr int H = LaneGeom<V>::halo, OWN = LaneGeom<V>::own;
+    const int nbx = bands_x<V>(w), nby = (h + th - 1) / th;
     const int wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     Band b;
-    b.active = wave < bands_x * bands_y;
-    const int bx = b.active ? wave % bands_x : 0, by = b.active ? wave / bands_x : 0;
-    const int raw = bx * kOwn - 2 + lane;
-    b.row0 = by * kBandRows;
+    b.rows = th;
+    b.active = wave < nbx * nby;
+    const int bx = b.active ? wave % nbx : 0, by = b.active ? wave / nbx : 0;
+    const int raw = bx * OWN + (lane - H) * V;
+    b.row0 = by * th;
     if (wrap) {
         int g = raw % w;
-        b.gi = g < 0 ? g + w : g;
+        b.gi0 = g < 0 ? g + w : g;
         b.col_ok = 1;
     } else {
-        b.gi = raw;
+        b.gi0 = raw;
         b.col_ok = raw >= 0 && raw < w;
     }
-    b.own = b.active && lane >= 2 && lane < 2 + kOwn && raw >= 0 && raw < w;
+    b.own = b.active && lane >= H && lane < 64 - H && raw >= 0 && raw < w;
+    b.inner = b.col_ok && b.gi0 >= 1 && b.gi0 + V <= w - 1;
     return b;
 }
 
-__host__ __device__ inline int stream_blocks(int w, int h) {
-    const int waves = ((w + kOwn - 1) / kOwn) * ((h + kBandRows - 1) / kBandRows);
-    return (waves + kBlock / 64 - 1) / (kBlock / 64);
+template <typename T, int V>
+__device__ inline void zero(T (&a)[V]) {
+#pragma unroll
+    for (int v = 0; v < V; v++) a[v] = (T)0;
+}
+template <typename T, int V>
+__device__ inline void copy(T (&d)[V], const T (&s)[V]) {
+#pragma unroll
+    for (int v = 0; v < V; v++) d[v] = s[v];
 }
 
 // Pre-smoothing of level L (nu = 2): z1 = wd r, z2 = z1 + wd (r - A z1) -> zout, and the coarse
 // residual C.r = sum over each 2x2 block of (r - A z2): fine row j0 then j0 + 1, columns
-// ascending, from +0 (MG:332-356).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_spre(Lvl<T> L, T *__restrict__ zout, Lvl<T> C,
+// ascending, from +0 (MG:332-356).  Rows are loaded kAhead steps before they are consumed.
+template <typename T, int V>
+__global__ __launch_bounds__(kBlock) void k_spre(Lvl<T> L, T *__restrict__ zout, Lvl<T> C, int th,
                                                  const CgState *__restrict__ st) {
+    constexpr int A = 4;  // rows in flight
     if (st && st->done) return;
     const int w = L.w, h = L.h, wrap = L.wrap;
-    const Band bd = band_of(w, h, wrap);
-    const int gi = bd.gi;
-    // Rows by age: a = newest (jk), b = jk - 1, c = jk - 2, d = jk - 3.
-    T z1a = 0, z1b = 0, z1c = 0, z2b = 0, z2c = 0, z2d = 0;
-    T ra = 0, rb = 0, rc = 0, wda = 0, wdb = 0, dga = 0, dgb = 0, dgc = 0;
-    T ea = 0, eb = 0, ec = 0, sa = 0, sb = 0, sc = 0, sd = 0;
-    T acc = 0;
-    const int steps = bd.active ? kBandRows + 4 : 0;
-    for (int k = 0; k < steps; k++) {
-        const int ja = bd.row0 - 2 + k, jb = ja - 1, jc = ja - 2;
-        // Shift the window, then load the newest row.
-        z1c = z1b, z1b = z1a;
-        z2d = z2c, z2c = z2b;
-        rc = rb, rb = ra, wdb = wda, dgc = dgb, dgb = dga, ec = eb, eb = ea;
-        sd = sc, sc = sb, sb = sa;
-        const bool oka = ja >= 0 && ja < h && bd.col_ok;
-        if (oka) {
-            const size_t g = (size_t)ja * w + gi;
-            ra = L.r[g], wda = L.wd[g], dga = L.diag[g], ea = L.east[g], sa = L.south[g];
-            z1a = wda * ra;
-        } else {
-            ra = wda = dga = ea = sa = z1a = (T)0;
-        }
-        // Second sweep of row b.
-        {
-            const T xe = from_east(z1b), xw = from_west(z1b), ew = from_west(eb);
-            T v = (T)0;
-            if (jb >= 0 && jb < h && bd.col_ok) {
-                const T ax = lane_apply(z1b, xe, xw, z1a, z1c, dgb, eb, ew, sb, sc, gi, jb, w, h,
-                                        wrap);
-                v = z1b + wdb * (rb - ax);
-            }
-            z2b = v;
-            if (bd.own && jb >= bd.row0 && jb < bd.row0 + kBandRows && jb < h)
-                zout[(size_t)jb * w + gi] = v;
-        }
-        // Residual of row c, restricted in 2 x 2 blocks.
-        {
-            const T xe = from_east(z2c), xw = from_west(z2c), ew = from_west(ec);
-            T res = (T)0;
+    const Band bd = band_of<V>(w, h, wrap, th);
+    const int gi0 = bd.gi0;
+    T qr[A][V], qwd[A][V], qdg[A][V], qe[A][V], qs[A][V];
+#define DTD_FETCH_PRE(j, u)                                                                   \
+    do {                                                                                      \
+        const int j_ = (j);                                                                   \
+        if (j_ >= 0 && j_ < h && bd.col_ok) {                                                 \
+            const size_t g_ = (size_t)j_ * w + gi0;                                           \
+            loadv<T, V>(L.r + g_, qr[u]);                                                     \
+            load_coef<T, V>(L.coef, g_, qe[u], qs[u], qdg[u], qwd[u]);                        \
+        } else {                                                                              \
+            zero(qr[u]), zero(qwd[u]), zero(qdg[u]), zero(qe[u]), zero(qs[u]);                \
+        }                                                                                     \
+    } while (0)
+    // Rows by age: a = newest (ja), b = ja - 1, c = ja - 2, d = ja - 3.
+    T z1a[V], z1b[V], z1c[V], z2b[V], z2c[V], z2d[V];
+    T ra[V], rb[V], rc[V], wda[V], wdb[V], dga[V], dgb[V], dgc[V];
+    T ea[V], eb[V], ec[V], sa[V], sb[V], sc[V], sd[V];
+    zero(z1a), zero(z1b), zero(z2b), zero(z2c), zero(ra), zero(rb), zero(wda), zero(dga);
+    zero(dgb), zero(ea), zero(eb), zero(sa), zero(sb), zero(sc);
+    T acc[V];
+    zero(acc);
+    const int steps = bd.active ? bd.rows + 4 : 0;  // a multiple of A
+#pragma unroll
+    for (int u = 0; u < A; u++) DTD_FETCH_PRE(bd.row0 - 2 + u, u);
+    for (int k0 = 0; k0 < steps; k0 += A) {
+#pragma unroll
+        for (int u = 0; u < A; u++) {
+            const int ja = bd.row0 - 2 + k0 + u, jb = ja - 1, jc = ja - 2;
+            copy(z1c, z1b), copy(z1b, z1a), copy(z2d, z2c), copy(z2c, z2b);
+            copy(rc, rb), copy(rb, ra), copy(wdb, wda), copy(dgc, dgb), copy(dgb, dga);
+            copy(ec, eb), copy(eb, ea), copy(sd, sc), copy(sc, sb), copy(sb, sa);
+            copy(ra, qr[u]), copy(wda, qwd[u]), copy(dga, qdg[u]), copy(ea, qe[u]);
+            copy(sa, qs[u]);
+            if (k0 + u + A < steps) DTD_FETCH_PRE(ja + A, u);
+            const bool oka = ja >= 0 && ja < h && bd.col_ok;
+#pragma unroll
+            for (int v = 0; v < V; v++) z1a[v] = oka ? wda[v] * ra[v] : (T)0;
+            // Second sweep of row b.
+            lane_sweep<T, V>(z1b, z1a, z1c, rb, wdb, dgb, eb, sb, sc, gi0, jb, w, h, wrap,
+                             jb >= 0 && jb < h && bd.col_ok, bd.inner && jb > 0 && jb + 1 < h,
+                             z2b);
+            if (bd.own && jb >= bd.row0 && jb < bd.row0 + bd.rows && jb < h)
+                storev<T, V>(zout + (size_t)jb * w + gi0, z2b);
+            // Residual of row c, restricted in 2 x 2 blocks.
             const bool okc = jc >= 0 && jc < h && bd.col_ok;
-            if (okc)
-                res = rc - lane_apply(z2c, xe, xw, z2b, z2d, dgc, ec, ew, sc, sd, gi, jc, w, h,
-                                      wrap);
-            const T rese = from_east(res);
-            if (bd.own && okc && jc >= bd.row0 && (gi & 1) == 0) {
-                if ((jc & 1) == 0) acc = (T)0;
-                acc += res;
-                if (gi + 1 < w) acc += rese;
-                if ((jc & 1) == 1 || jc == h - 1)
-                    C.r[(size_t)(jc >> 1) * C.w + (gi >> 1)] = acc;
-            }
-        }
-    }
-}
-
-// Post-smoothing of level L (nu = 2): z' = zin + zc[j/2][i/2], z3 = z' + wd (r - A z'),
-// z4 = z3 + wd (r - A z3) -> zout; with dot, per block sum dot * z4 into parts[block].
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_spost(Lvl<T> L, const T *__restrict__ zin,
-                                                  const T *__restrict__ zc, int wc,
-                                                  T *__restrict__ zout,
-                                                  const double *__restrict__ dot,
-                                                  double *__restrict__ parts,
-                                                  const CgState *__restrict__ st) {
-    __shared__ double lds[kBlock / 64];
-    if (st && st->done) return;
-    const int w = L.w, h = L.h, wrap = L.wrap;
-    const Band bd = band_of(w, h, wrap);
-    const int gi = bd.gi;
-    T zpa = 0, zpb = 0, zpc = 0, z3b = 0, z3c = 0, z3d = 0;
-    T ra = 0, rb = 0, rc = 0, wda = 0, wdb = 0, wdc = 0, dga = 0, dgb = 0, dgc = 0;
-    T ea = 0, eb = 0, ec = 0, sa = 0, sb = 0, sc = 0, sd = 0;
-    double acc = 0.0;
-    const int steps = bd.active ? kBandRows + 4 : 0;
-    for (int k = 0; k < steps; k++) {
-        const int ja = bd.row0 - 2 + k, jb = ja - 1, jc = ja - 2;
-        zpc = zpb, zpb = zpa;
-        z3d = z3c, z3c = z3b;
-        rc = rb, rb = ra, wdc = wdb, wdb = wda, dgc = dgb, dgb = dga, ec = eb, eb = ea;
-        sd = sc, sc = sb, sb = sa;
-        const bool oka = ja >= 0 && ja < h && bd.col_ok;
-        if (oka) {
-            const size_t g = (size_t)ja * w + gi;
-            ra = L.r[g], wda = L.wd[g], dga = L.diag[g], ea = L.east[g], sa = L.south[g];
-            zpa = zin[g] + zc[(size_t)(ja >> 1) * wc + (gi >> 1)];
-        } else {
-            ra = wda = dga = ea = sa = zpa = (T)0;
-        }
-        {
-            const T xe = from_east(zpb), xw = from_west(zpb), ew = from_west(eb);
-            T v = (T)0;
-            if (jb >= 0 && jb < h && bd.col_ok) {
-                const T ax = lane_apply(zpb, xe, xw, zpa, zpc, dgb, eb, ew, sb, sc, gi, jb, w, h,
-                                        wrap);
-                v = zpb + wdb * (rb - ax);
-            }
-            z3b = v;
-        }
-        {
-            const T xe = from_east(z3c), xw = from_west(z3c), ew = from_west(ec);
-            if (jc >= 0 && jc < h && bd.col_ok) {
-                const T ax = lane_apply(z3c, xe, xw, z3b, z3d, dgc, ec, ew, sc, sd, gi, jc, w, h,
-                                        wrap);
-                const T v = z3c + wdc * (rc - ax);
-                if (bd.own && jc >= bd.row0) {
-                    const size_t g = (size_t)jc * w + gi;
-                    zout[g] = v;
-                    if (dot) acc += dot[g] * (double)v;
+            T res[V];
+            lane_ax<T, V>(z2c, z2b, z2d, dgc, ec, sc, sd, gi0, jc, w, h, wrap,
+                          bd.inner && jc > 0 && jc + 1 < h, res);
+#pragma unroll
+            for (int v = 0; v < V; v++) res[v] = okc ? rc[v] - res[v] : (T)0;
+            const T rese = from_east(res[0]);
+            if (bd.own && okc && jc >= bd.row0) {
+#pragma unroll
+                for (int v = 0; v < V; v += 2) {
+                    const int gi = gi0 + v;
+                    if (V == 1 && (gi & 1)) continue;
+                    const T partner = v + 1 < V ? res[v + 1] : rese;
+                    if ((jc & 1) == 0) acc[v] = (T)0;
+                    acc[v] += res[v];
+                    if (gi + 1 < w) acc[v] += partner;
+                    if ((jc & 1) == 1 || jc == h - 1)
+                        C.r[(size_t)(jc >> 1) * C.w + (gi >> 1)] = acc[v];
                 }
             }
         }
     }
+#undef DTD_FETCH_PRE
+}
+
+// Post-smoothing of level L (nu = 2): z' = zin + zc[j/2][i/2], z3 = z' + wd (r - A z'),
+// z4 = z3 + wd (r - A z3) -> zout; with dot, per block sum dot * z4 into parts[block].
+template <typename T, int V>
+__global__ __launch_bounds__(kBlock) void k_spost(Lvl<T> L, const T *__restrict__ zin,
+                                                  const T *__restrict__ zc, int wc,
+                                                  T *__restrict__ zout, int th,
+                                                  const double *__restrict__ dot,
+                                                  double *__restrict__ parts,
+                                                  const CgState *__restrict__ st) {
+    constexpr int A = 4;
+    __shared__ double lds[kBlock / 64];
+    if (st && st->done) return;
+    const int w = L.w, h = L.h, wrap = L.wrap;
+    const Band bd = band_of<V>(w, h, wrap, th);
+    const int gi0 = bd.gi0;
+    T qr[A][V], qwd[A][V], qdg[A][V], qe[A][V], qs[A][V], qz[A][V], qc[A][V];
+    // z' = z + zc is formed when the row is consumed.
+#define DTD_FETCH_POST(j, u)                                                                  \
+    do {                                                                                      \
+        const int j_ = (j);                                                                   \
+        if (j_ >= 0 && j_ < h && bd.col_ok) {                                                 \
+            const size_t g_ = (size_t)j_ * w + gi0;                                           \
+            loadv<T, V>(L.r + g_, qr[u]), loadv<T, V>(zin + g_, qz[u]);                       \
+            load_coef<T, V>(L.coef, g_, qe[u], qs[u], qdg[u], qwd[u]);                        \
+            const T *zcr_ = zc + (size_t)(j_ >> 1) * wc;                                      \
+            _Pragma("unroll") for (int v = 0; v < V; v++) qc[u][v] = zcr_[(gi0 + v) >> 1];    \
+        } else {                                                                              \
+            zero(qr[u]), zero(qwd[u]), zero(qdg[u]), zero(qe[u]), zero(qs[u]), zero(qz[u]);   \
+            zero(qc[u]);                                                                      \
+        }                                                                                     \
+    } while (0)
+    T zpa[V], zpb[V], zpc[V], z3b[V], z3c[V], z3d[V];
+    T ra[V], rb[V], rc[V], wda[V], wdb[V], wdc[V], dga[V], dgb[V], dgc[V];
+    T ea[V], eb[V], ec[V], sa[V], sb[V], sc[V], sd[V];
+    zero(zpa), zero(zpb), zero(z3b), zero(z3c), zero(ra), zero(rb), zero(wda), zero(wdb);
+    zero(dga), zero(dgb), zero(ea), zero(eb), zero(sa), zero(sb), zero(sc);
+    double acc = 0.0;
+    const int steps = bd.active ? bd.rows + 4 : 0;
+#pragma unroll
+    for (int u = 0; u < A; u++) DTD_FETCH_POST(bd.row0 - 2 + u, u);
+    for (int k0 = 0; k0 < steps; k0 += A) {
+#pragma unroll
+        for (int u = 0; u < A; u++) {
+            const int ja = bd.row0 - 2 + k0 + u, jb = ja - 1, jc = ja - 2;
+            copy(zpc, zpb), copy(zpb, zpa), copy(z3d, z3c), copy(z3c, z3b);
+            copy(rc, rb), copy(rb, ra), copy(wdc, wdb), copy(wdb, wda), copy(dgc, dgb);
+            copy(dgb, dga), copy(ec, eb), copy(eb, ea), copy(sd, sc), copy(sc, sb), copy(sb, sa);
+            copy(ra, qr[u]), copy(wda, qwd[u]), copy(dga, qdg[u]), copy(ea, qe[u]);
+            copy(sa, qs[u]);
+            const bool oka = ja >= 0 && ja < h && bd.col_ok;
+#pragma unroll
+            for (int v = 0; v < V; v++) zpa[v] = oka ? qz[u][v] + qc[u][v] : (T)0;
+            if (k0 + u + A < steps) DTD_FETCH_POST(ja + A, u);
+            lane_sweep<T, V>(zpb, zpa, zpc, rb, wdb, dgb, eb, sb, sc, gi0, jb, w, h, wrap,
+                             jb >= 0 && jb < h && bd.col_ok, bd.inner && jb > 0 && jb + 1 < h,
+                             z3b);
+            T z4[V];
+            const bool okc = jc >= 0 && jc < h && bd.col_ok;
+            lane_sweep<T, V>(z3c, z3b, z3d, rc, wdc, dgc, ec, sc, sd, gi0, jc, w, h, wrap, okc,
+                             bd.inner && jc > 0 && jc + 1 < h, z4);
+            if (bd.own && okc && jc >= bd.row0) {
+                const size_t g = (size_t)jc * w + gi0;
+                storev<T, V>(zout + g, z4);
+                if (dot) {
+#pragma unroll
+                    for (int v = 0; v < V; v++) acc += dot[g + v] * (double)z4[v];
+                }
+            }
+        }
+    }
+#undef DTD_FETCH_POST
     if (dot) {
         const double sum = block_sum<kBlock>(acc, lds);
         if (threadIdx.x == 0) parts[blockIdx.x] = sum;

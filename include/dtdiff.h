@@ -141,6 +141,8 @@ int dtd_cycle_precondition(dtd_ctx *ctx, const float *r32, const double *r64, fl
 #define DTD_KERNEL_RESTRICT 3   /* the finest level's residual restriction (f32) */
 #define DTD_KERNEL_PRE 4        /* fused pre-smoothing + restriction of the finest level (f32) */
 #define DTD_KERNEL_POST 5       /* fused post-smoothing (two sweeps + dot) of the finest level */
+#define DTD_KERNEL_COPY 6       /* calibration: a float4 copy of one finest-level array */
+#define DTD_KERNEL_STREAM5 7    /* calibration: five finest-level arrays combined into one */
 
 /* Measurement hook, no reference counterpart: launches `kernel` reps times back to back on the
  * context's stream over the resident data of the last solve and returns the average duration
