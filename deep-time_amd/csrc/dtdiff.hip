@@ -75,6 +75,7 @@ struct Workspace {
     double *b = nullptr, *x = nullptr, *east = nullptr, *south = nullptr, *r = nullptr;
     double *p0 = nullptr, *p1 = nullptr, *ap = nullptr, *kd = nullptr;
     double *parts = nullptr, *pair = nullptr;
+    double2 *es = nullptr;  // (east, south) of each cell, for the streamed direction pass
     CgState *st = nullptr;
     std::vector<Lvl<float>> lf;
     std::vector<Lvl<double>> ld;
@@ -112,6 +113,7 @@ static void free_ws(Workspace &ws) {
     for (double *p : bufs)
         if (p) (void)hipFree(p);
     if (ws.st) (void)hipFree(ws.st);
+    if (ws.es) (void)hipFree(ws.es);
     for (void *p : ws.owned) (void)hipFree(p);
     ws = Workspace();
 }
@@ -141,6 +143,7 @@ static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
     ws.parts = dalloc<double>(2 * (size_t)nblocks(n) + 2);
     ws.pair = dalloc<double>(2);
     ws.st = dalloc<CgState>(1);
+    ws.es = dalloc<double2>(n);
     auto sh = shapes(w, h, wrap);
     auto own = [&](size_t cnt, size_t elt) {
         void *p = nullptr;
@@ -244,6 +247,15 @@ __global__ void k_sum1(const double *__restrict__ parts, int nparts, double *out
     if (threadIdx.x == 0) out[0] = v;
 }
 
+__global__ void k_time_state(CgState *st) {
+    if (threadIdx.x != 0) return;
+    CgState s{};
+    s.max_iter = 1 << 30;
+    s.have_beta = 1;
+    s.have_pending = 1;
+    *st = s;
+}
+
 __global__ void k_init_state(CgState *st, double tol2, int max_iter) {
     if (threadIdx.x != 0) return;
     CgState s{};
@@ -289,12 +301,13 @@ static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt)
         LAUNCH_CHECK();
         build_coarse<float>(ws.lf, ctx->opts.omega, ctx->opts.scale, s);
     }
+    k_pack_es<<<nblocks(ws.n), kBlock, 0, s>>>((int)ws.n, ws.east, ws.south, ws.es);
+    LAUNCH_CHECK();
 }
 
-// Band height for a streaming kernel on a w x h level: the smallest multiple of 4 rows for which
-// the level's bands fit in one round of the kernel's resident waves (all bands then run at
-// once, none waits for a second round), at least 4.
-template <int V>
+// Band height for a streaming kernel on a w x h level (a multiple of 4 rows; DTD_BAND_ROWS
+// overrides it).
+template <int V, int H = LaneGeom<V>::halo>
 static int band_height(const void *kernel, int w, int h) {
     static int cus = 0;
     if (!cus) {
@@ -302,13 +315,19 @@ static int band_height(const void *kernel, int w, int h) {
         HIPCHK(hipGetDevice(&dev));
         HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     }
+    static const int forced = [] {
+        const char *v = getenv("DTD_BAND_ROWS");
+        return v ? atoi(v) : 0;
+    }();
+    if (forced > 0) return std::max(4, (forced + 3) / 4 * 4);
     int per_cu = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0));
     const long resident = (long)std::max(per_cu, 1) * cus * (kBlock / 64);
-    const long nby = std::max(1L, resident / bands_x<V>(w));
-    long th = (h + nby - 1) / nby;
-    th = std::max(4L, (th + 3) / 4 * 4);
-    return (int)th;
+    // Short bands in many rounds keep the waves in flight on a compact stretch of rows (the best
+    // measured on the finest levels); levels too small for two rounds take 4 rows.
+    for (int th = 16; th >= 8; th >>= 1)
+        if ((long)stream_waves<V, H>(w, h, th) >= 2 * resident) return th;
+    return 4;
 }
 
 template <typename T, int V>
@@ -325,6 +344,18 @@ static int launch_post(Lvl<T> &L, const T *zin, const T *zc, int wc, T *zout, co
     const int th = band_height<V>((const void *)k_spost<T, V>, L.w, L.h);
     const int nb = (stream_waves<V>(L.w, L.h, th) + kBlock / 64 - 1) / (kBlock / 64);
     k_spost<T, V><<<nb, kBlock, 0, s>>>(L, zin, zc, wc, zout, th, dot, parts, st);
+    LAUNCH_CHECK();
+    return nb;
+}
+
+// The streamed direction pass; returns its block count (the p.Ap partial sums).
+template <typename TZ>
+static int launch_dir(Workspace &ws, const TZ *z, const double *pold, double *pnew,
+                      hipStream_t s) {
+    const int th = band_height<1, 1>((const void *)k_sdir<TZ>, ws.w, ws.h);
+    const int nb = (stream_waves<1, 1>(ws.w, ws.h, th) + kBlock / 64 - 1) / (kBlock / 64);
+    k_sdir<TZ><<<nb, kBlock, 0, s>>>(grid_of(ws), ws.es, z, pold, pnew, ws.ap, ws.x, th, ws.parts,
+                                     ws.st);
     LAUNCH_CHECK();
     return nb;
 }
@@ -416,15 +447,9 @@ static void enqueue_iteration(dtd_ctx *ctx, int parity) {
     const int nb = nblocks(ws.n, kCgCells);
     double *pold = parity == 0 ? ws.p1 : ws.p0;
     double *pnew = parity == 0 ? ws.p0 : ws.p1;
-    if (ws.prec == DTD_CYCLE_F64) {
-        k_direction<double><<<nb, kBlock, 0, s>>>(g, ws.east, ws.south, ws.ld[0].z[1], pold, pnew,
-                                                  ws.ap, ws.x, ws.parts, ws.st);
-    } else {
-        k_direction<float><<<nb, kBlock, 0, s>>>(g, ws.east, ws.south, ws.lf[0].z[1], pold, pnew,
-                                                 ws.ap, ws.x, ws.parts, ws.st);
-    }
-    LAUNCH_CHECK();
-    k_cg_step<kStepPAP><<<1, kReduceBlock, 0, s>>>(ws.parts, nb, ws.st);
+    const int nbd = ws.prec == DTD_CYCLE_F64 ? launch_dir<double>(ws, ws.ld[0].z[1], pold, pnew, s)
+                                             : launch_dir<float>(ws, ws.lf[0].z[1], pold, pnew, s);
+    k_cg_step<kStepPAP><<<1, kReduceBlock, 0, s>>>(ws.parts, nbd, ws.st);
     LAUNCH_CHECK();
     if (ws.prec == DTD_CYCLE_F64)
         k_update<double><<<nb, kBlock, 0, s>>>((int)ws.n, ws.ap, ws.r, nullptr, ws.parts, ws.st);
@@ -458,10 +483,47 @@ static void capture_graphs(dtd_ctx *ctx) {
     }
 }
 
+// Whether the parallel line solver takes lines of n cells: 16 * 2^k cells, 3 to 1024 segments.
+static bool parallel_line(int n) {
+    if (n % kSeg) return false;
+    const int p = n / kSeg;
+    return p >= 4 && p <= 1024 && (p & (p - 1)) == 0;
+}
+
+static bool parallel_lod_enabled() {
+    static const bool on = [] {
+        const char *v = getenv("DTD_SEQ_LOD");
+        return !(v && *v && *v != '0');
+    }();
+    return on;
+}
+
 static void lod(dtd_ctx *ctx, double *x, const double *kd, double dx, double dt, int wrap_x) {
     Workspace &ws = ctx->ws;
     const Grid g = grid_of(ws);
     const double s2 = dx * dx;
+    if (parallel_lod_enabled() && parallel_line(ws.w) && parallel_line(ws.h)) {
+        // Rows in place; then the columns as rows of the transposed grid (kd^T in p0, x^T in
+        // p1), transposed back into x.
+        hipStream_t s = ctx->stream;
+        auto lines = [&](int nlines, int n, int ring, const double *k, double *xx) {
+            if (n / kSeg <= 512)
+                k_lod_lines<512><<<nlines, n / kSeg, 0, s>>>(n, ring, k, xx, dt, s2);
+            else
+                k_lod_lines<1024><<<nlines, n / kSeg, 0, s>>>(n, ring, k, xx, dt, s2);
+            LAUNCH_CHECK();
+        };
+        lines(ws.h, ws.w, wrap_x && ws.w > 2, kd, x);
+        const dim3 tg((ws.w + 31) / 32, (ws.h + 31) / 32), tgt((ws.h + 31) / 32, (ws.w + 31) / 32);
+        k_transpose<<<tg, 256, 0, s>>>(kd, ws.p0, ws.h, ws.w);
+        LAUNCH_CHECK();
+        k_transpose<<<tg, 256, 0, s>>>(x, ws.p1, ws.h, ws.w);
+        LAUNCH_CHECK();
+        lines(ws.w, ws.h, 0, ws.p0, ws.p1);
+        k_transpose<<<tgt, 256, 0, s>>>(ws.p1, x, ws.w, ws.h);
+        LAUNCH_CHECK();
+        return;
+    }
     k_lod_rows<<<nblocks(ws.h), kBlock, 0, ctx->stream>>>(g, wrap_x, kd, dt, s2, x, ws.p0, ws.p1);
     LAUNCH_CHECK();
     k_lod_cols<<<nblocks(ws.w), kBlock, 0, ctx->stream>>>(g, kd, dt, s2, x, ws.p0);
@@ -861,8 +923,9 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
         const int nbc = nblocks(ws.n, kCgCells);
         const double n = (double)ws.n;
         double b = 0.0;
-        // A private state that never stops, so the loop kernels run in full.
-        k_init_state<<<1, 64, 0, s>>>(ws.st, 0.0, 1 << 30);
+        // A private state that never stops and takes every branch of the passes (beta and the
+        // pending update on, both zero), so the loop kernels run in full and change nothing.
+        k_time_state<<<1, 64, 0, s>>>(ws.st);
         LAUNCH_CHECK();
         auto one = [&] {
             switch (kernel) {
@@ -871,9 +934,8 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
                                                      nullptr, ws.st);
                 b = 28.0 * n;
                 break;
-            case DTD_KERNEL_DIRECTION:  // z (f32), p_old, x, east, south in; p, A p, x out
-                k_direction<float><<<nbc, kBlock, 0, s>>>(g, ws.east, ws.south, L.z[1], ws.p1,
-                                                         ws.p0, ws.ap, ws.x, ws.parts, ws.st);
+            case DTD_KERNEL_DIRECTION:  // z (f32), p_old, x, (east, south) in; p, A p, x out
+                launch_dir<float>(ws, L.z[1], ws.p1, ws.p0, s);
                 b = (4.0 + 8.0 * 4 + 8.0 * 3) * n;
                 break;
             case DTD_KERNEL_UPDATE:  // A p, r in; r, r32 out

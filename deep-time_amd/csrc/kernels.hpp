@@ -577,8 +577,8 @@ __global__ void k_coarsest(Lvl<T> L, T *__restrict__ zout, int sweeps,
 
 // Cells per lane: V consecutive columns (8 bytes a lane: two f32 or one f64 cell).  Halo lanes
 // on each side: enough for two columns.
-template <int V> struct LaneGeom {
-    static constexpr int halo = V >= 2 ? 1 : 2;
+template <int V, int H = (V >= 2 ? 1 : 2)> struct LaneGeom {
+    static constexpr int halo = H;                   // halo lanes on each side
     static constexpr int own = (64 - 2 * halo) * V;  // owned columns per wave
 };
 
@@ -706,13 +706,13 @@ __device__ inline void lane_sweep(const T (&x)[V], const T (&xs)[V], const T (&x
 
 // A level's bands: columns of LaneGeom<V>::own, rows of `th` (a multiple of 4, chosen by the
 // host so that all bands run at once in one round of resident waves).
-template <int V>
+template <int V, int H = LaneGeom<V>::halo>
 __host__ __device__ inline int bands_x(int w) {
-    return (w + LaneGeom<V>::own - 1) / LaneGeom<V>::own;
+    return (w + LaneGeom<V, H>::own - 1) / LaneGeom<V, H>::own;
 }
-template <int V>
+template <int V, int H = LaneGeom<V>::halo>
 __host__ __device__ inline int stream_waves(int w, int h, int th) {
-    return bands_x<V>(w) * ((h + th - 1) / th);
+    return bands_x<V, H>(w) * ((h + th - 1) / th);
 }
 
 struct Band {
@@ -725,11 +725,11 @@ struct Band {
     int active;  // the wave has a band
 };
 
-template <int V>
+template <int V, int H = LaneGeom<V>::halo>
 __device__ inline Band band_of(int w, int h, int wrap, int th) {
     const int lane = threadIdx.x & 63;
-    constexpr int H = LaneGeom<V>::halo, OWN = LaneGeom<V>::own;
-    const int nbx = bands_x<V>(w), nby = (h + th - 1) / th;
+    constexpr int OWN = LaneGeom<V, H>::own;
+    const int nbx = bands_x<V, H>(w), nby = (h + th - 1) / th;
     const int wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     Band b;
     b.rows = th;
@@ -917,6 +917,107 @@ __global__ __launch_bounds__(kBlock) void k_spost(Lvl<T> L, const T *__restrict_
     }
 }
 
+// ---- the conjugate gradients' direction pass, streamed (MG:633-695) ------------------------
+//
+// One f64 cell a lane, a band of 62 columns (one halo lane each side) walked row by row with
+// four rows of loads in flight.  Per row: the direction p = z + beta p_old (z on the first
+// iteration) of the newest row, stored for the owned cells; the pending x += alpha p_old of the
+// owned cells; and A p of the row above with its p . A p.  The f64 couplings come packed as
+// (east, south) pairs.
+
+// (east, south) of each cell side by side, from the operator's arrays.
+__global__ void k_pack_es(int n, const double *__restrict__ east,
+                          const double *__restrict__ south, double2 *__restrict__ es) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c < n) es[c] = make_double2(east[c], south[c]);
+}
+
+template <typename TZ>
+__global__ __launch_bounds__(kBlock) void k_sdir(Grid g, const double2 *__restrict__ es,
+                                                 const TZ *__restrict__ z,
+                                                 const double *__restrict__ pold,
+                                                 double *__restrict__ pnew, double *__restrict__ ap,
+                                                 double *__restrict__ x, int th,
+                                                 double *__restrict__ parts,
+                                                 const CgState *__restrict__ st) {
+    constexpr int A = 4;
+    __shared__ double lds[kBlock / 64];
+    if (st->done) return;
+    const int have_beta = st->have_beta, pending = st->have_pending;
+    const double beta = st->beta, alpha = st->alpha;
+    const int w = g.w, h = g.h, wrap = g.wrap;
+    const Band bd = band_of<1, 1>(w, h, wrap, th);
+    const int gi = bd.gi0;
+    const bool inner_col = bd.col_ok && gi >= 1 && gi + 1 <= w - 1;
+    // Raw loads only: p, and the owned cells' x update, are formed when the row is consumed, so
+    // the loads of a row stay in flight for A steps.
+    double qe[A], qs[A], qpo[A], qx[A];
+    TZ qz[A];
+    const bool need_old = have_beta || pending;
+#define DTD_FETCH_DIR(j, u)                                                                   \
+    do {                                                                                      \
+        const int j_ = (j);                                                                   \
+        if (j_ >= 0 && j_ < h && bd.col_ok && j_ <= bd.row0 + bd.rows) {                      \
+            const size_t g_ = (size_t)j_ * w + gi;                                            \
+            const double2 c_ = es[g_];                                                        \
+            qe[u] = c_.x, qs[u] = c_.y, qz[u] = z[g_];                                        \
+            qpo[u] = need_old ? pold[g_] : 0.0;                                               \
+            qx[u] = pending && bd.own ? x[g_] : 0.0;                                          \
+        } else {                                                                              \
+            qe[u] = qs[u] = qpo[u] = qx[u] = 0.0;                                             \
+            qz[u] = (TZ)0;                                                                    \
+        }                                                                                     \
+    } while (0)
+    double pa = 0, pb = 0, pc = 0, eb = 0, ea = 0, sa = 0, sb = 0, sc = 0;
+    double acc = 0.0;
+    const int steps = bd.active ? bd.rows + 4 : 0;  // rows row0 - 1 .. row0 + rows (+2 spare)
+#pragma unroll
+    for (int u = 0; u < A; u++) DTD_FETCH_DIR(bd.row0 - 1 + u, u);
+    for (int k0 = 0; k0 < steps; k0 += A) {
+#pragma unroll
+        for (int u = 0; u < A; u++) {
+            const int ja = bd.row0 - 1 + k0 + u, jb = ja - 1;
+            pc = pb, pb = pa, eb = ea, sc = sb, sb = sa;
+            ea = qe[u], sa = qs[u];
+            const double po = qpo[u], xo = qx[u];
+            pa = have_beta ? (double)qz[u] + beta * po : (double)qz[u];
+            if (k0 + u + A < steps) DTD_FETCH_DIR(ja + A, u);
+            if (bd.own && ja >= bd.row0 && ja < bd.row0 + bd.rows && ja < h) {
+                const size_t ga = (size_t)ja * w + gi;
+                pnew[ga] = pa;
+                if (pending) x[ga] = xo + alpha * po;
+            }
+            // A p of row b.
+            const double pe = from_east(pb), pw = from_west(pb), ew = from_west(eb);
+            if (bd.own && jb >= bd.row0 && jb < bd.row0 + bd.rows && jb < h) {
+                double o;
+                if (inner_col && jb > 0 && jb + 1 < h) {
+                    const double d = 1.0 + eb + ew + sb + sc;
+                    o = d * pb;
+                    o -= eb * pe;
+                    o -= ew * pw;
+                    o -= sb * pa;
+                    o -= sc * pc;
+                } else {
+                    const double wv = (gi > 0 || wrap) ? ew : 0.0;
+                    const double nv = jb > 0 ? sc : 0.0;
+                    const double d = 1.0 + eb + wv + sb + nv;
+                    o = d * pb;
+                    if (gi + 1 < w || wrap) o -= eb * pe;
+                    if (gi > 0 || wrap) o -= wv * pw;
+                    if (jb + 1 < h) o -= sb * pa;
+                    if (jb > 0) o -= nv * pc;
+                }
+                ap[(size_t)jb * w + gi] = o;
+                acc += pb * o;
+            }
+        }
+    }
+#undef DTD_FETCH_DIR
+    const double sum = block_sum<kBlock>(acc, lds);
+    if (threadIdx.x == 0) parts[blockIdx.x] = sum;
+}
+
 // ---- the first guess (D:370-489) -----------------------------------------------------------
 
 // The x sweep, one thread per row: `cyclic` (D:1050-1071) on rows of a wrapping grid wider than
@@ -1046,6 +1147,165 @@ __global__ void k_lod_cols(Grid g, const double *__restrict__ kd, double dt, dou
         const size_t cidx = (size_t)j * w + i;
         x[cidx] -= cbuf[cidx + w] * x[cidx + w];
     }
+}
+
+// ---- the first guess, parallel (lines of 16 * 2^k cells) ----------------------------------
+//
+// The same two sweeps as `lod` (D:370-489): every row's tridiagonal system (cyclic when the grid
+// wraps), then every column's, solved by a partition method instead of one Thomas recurrence per
+// line (Wang 1981; the Thomas/PCR hybrid of Lászlo, Giles and Appleyard 2016).  A block takes
+// one line of n cells, a thread one segment of 16: it eliminates the segment's interior rows
+// against the segment's first and last unknowns (forward, then backward, keeping the 14 interior
+// rows' coefficients in registers), which leaves for each segment two equations coupling only the
+// first and last unknowns of neighbouring segments: a tridiagonal system of 2 n / 16 unknowns
+// (cyclic for a wrapping row), solved in LDS by parallel cyclic reduction; the interior unknowns
+// follow from it.  The result is the solution of the same systems, equal to the sequential
+// recurrences' to rounding (the systems are diagonally dominant).  Columns are solved as the rows
+// of the transposed grid.
+
+constexpr int kSeg = 16;
+
+// dst[c * rows + r] = src[r * cols + c], through a 32 x 32 tile in LDS.
+__global__ void k_transpose(const double *__restrict__ src, double *__restrict__ dst, int rows,
+                            int cols) {
+    __shared__ double t[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols) t[k][tx] = src[(size_t)r * cols + c];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int c = c0 + k, r = r0 + tx;
+        if (r < rows && c < cols) dst[(size_t)c * rows + r] = t[tx][k];
+    }
+}
+
+// One line per block (n / 16 threads, a power of two), in place on x; faces from kd along the
+// line, the last one wrapping when `ring`.
+template <int MAXT>
+__global__ __launch_bounds__(MAXT) void k_lod_lines(int n, int ring,
+                                                    const double *__restrict__ kd,
+                                                    double *__restrict__ x, double dt, double s2) {
+    __shared__ double A[2 * MAXT], B[2 * MAXT], Cc[2 * MAXT], R[2 * MAXT];
+    const int P = n / kSeg, N = 2 * P, t = threadIdx.x;
+    const size_t base = (size_t)blockIdx.x * n;
+    const double *k = kd + base;
+    double *xl = x + base;
+    const int a = t * kSeg, b = a + kSeg - 1;
+    // kd[a - 1 .. b + 1] (wrapped for a ring; a missing neighbour gives no face).
+    double kk[kSeg + 2];
+#pragma unroll
+    for (int q = 0; q < kSeg; q += 2) {
+        const double2 v = *reinterpret_cast<const double2 *>(k + a + q);
+        kk[1 + q] = v.x, kk[2 + q] = v.y;
+    }
+    const bool has_w = a > 0 || ring, has_e = b + 1 < n || ring;
+    kk[0] = has_w ? k[a > 0 ? a - 1 : n - 1] : 0.0;
+    kk[kSeg + 1] = has_e ? k[b + 1 < n ? b + 1 : 0] : 0.0;
+    // face(i) for i = a - 1 .. b: f[q] = face(a - 1 + q), the reference's expression.
+    double f[kSeg + 1];
+#pragma unroll
+    for (int q = 0; q <= kSeg; q++) f[q] = 0.5 * (kk[q] + kk[q + 1]) * dt / s2;
+    if (!has_w) f[0] = 0.0;
+    if (!has_e) f[kSeg] = 0.0;
+    double d[kSeg];
+#pragma unroll
+    for (int q = 0; q < kSeg; q += 2) {
+        const double2 v = *reinterpret_cast<const double2 *>(xl + a + q);
+        d[q] = v.x, d[q + 1] = v.y;
+    }
+    // Row q of the segment: lo = -f[q], up = -f[q + 1], di = 1 + f[q] + f[q + 1].
+    // Forward over the interior rows 1 .. 14, against x_a.
+    double al[kSeg], ga[kSeg], de[kSeg];
+    {
+        const double lo = -f[1], up = -f[2], di = 1.0 + f[1] + f[2];
+        al[1] = lo / di, ga[1] = up / di, de[1] = d[1] / di;
+    }
+#pragma unroll
+    for (int q = 2; q <= kSeg - 2; q++) {
+        const double lo = -f[q], up = -f[q + 1], di = 1.0 + f[q] + f[q + 1];
+        const double r = di - lo * ga[q - 1];
+        al[q] = -lo * al[q - 1] / r;
+        ga[q] = up / r;
+        de[q] = (d[q] - lo * de[q - 1]) / r;
+    }
+    // Backward: x_q = de[q] - al[q] x_a - ga[q] x_b.
+#pragma unroll
+    for (int q = kSeg - 3; q >= 1; q--) {
+        de[q] = de[q] - ga[q] * de[q + 1];
+        al[q] = al[q] - ga[q] * al[q + 1];
+        ga[q] = -ga[q] * ga[q + 1];
+    }
+    // The segment's two reduced equations: F (row 0) couples L of the segment before, L (row
+    // 15) couples F of the segment after.
+    {
+        const double lo0 = -f[0], up0 = -f[1], di0 = 1.0 + f[0] + f[1];
+        A[2 * t] = lo0;
+        B[2 * t] = di0 - up0 * al[1];
+        Cc[2 * t] = -up0 * ga[1];
+        R[2 * t] = d[0] - up0 * de[1];
+        const double lo15 = -f[kSeg - 1], up15 = -f[kSeg], di15 = 1.0 + f[kSeg - 1] + f[kSeg];
+        A[2 * t + 1] = -lo15 * al[kSeg - 2];
+        B[2 * t + 1] = di15 - lo15 * ga[kSeg - 2];
+        Cc[2 * t + 1] = up15;
+        R[2 * t + 1] = d[kSeg - 1] - lo15 * de[kSeg - 2];
+    }
+    __syncthreads();
+    // Parallel cyclic reduction: after the step of stride s every equation couples the unknowns
+    // 2s away.  A ring stops at stride N / 2, where both couplings reach the same unknown.
+    const int last = ring ? N / 2 : N;
+    for (int s = 1; s < last; s <<= 1) {
+        double na[2], nb[2], nc[2], nr[2];
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const int i = 2 * t + e;
+            int im = i - s, ip = i + s;
+            double a_m = 0, b_m = 1, c_m = 0, r_m = 0, a_p = 0, b_p = 1, c_p = 0, r_p = 0;
+            if (ring) {
+                im = (im + N) % N, ip = ip % N;
+            }
+            if (im >= 0) a_m = A[im], b_m = B[im], c_m = Cc[im], r_m = R[im];
+            if (ip < N) a_p = A[ip], b_p = B[ip], c_p = Cc[ip], r_p = R[ip];
+            const double k1 = A[i] / b_m, k2 = Cc[i] / b_p;
+            na[e] = -a_m * k1;
+            nc[e] = -c_p * k2;
+            nb[e] = B[i] - c_m * k1 - a_p * k2;
+            nr[e] = R[i] - r_m * k1 - r_p * k2;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const int i = 2 * t + e;
+            A[i] = na[e], B[i] = nb[e], Cc[i] = nc[e], R[i] = nr[e];
+        }
+        __syncthreads();
+    }
+    double xf, xl_;
+    {
+        double sol[2];
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const int i = 2 * t + e;
+            if (ring && N > 1) {
+                // b_i x_i + (a_i + c_i) x_j = r_i and its partner equation j = i + N/2.
+                const int j = (i + N / 2) % N;
+                const double g1 = A[i] + Cc[i], g2 = A[j] + Cc[j];
+                sol[e] = (R[i] * B[j] - g1 * R[j]) / (B[i] * B[j] - g1 * g2);
+            } else {
+                sol[e] = R[i] / B[i];
+            }
+        }
+        xf = sol[0], xl_ = sol[1];
+    }
+    d[0] = xf;
+    d[kSeg - 1] = xl_;
+#pragma unroll
+    for (int q = 1; q <= kSeg - 2; q++) d[q] = de[q] - al[q] * xf - ga[q] * xl_;
+#pragma unroll
+    for (int q = 0; q < kSeg; q += 2)
+        *reinterpret_cast<double2 *>(xl + a + q) = make_double2(d[q], d[q + 1]);
 }
 
 }  // namespace dtd

@@ -56,12 +56,41 @@ def test_operator_and_apply_bit_for_bit(ws, w, h, wrap):
     assert same_bits(y, O.apply(w, h, wrap, oe, os_, og, b))
 
 
-@pytest.mark.parametrize("w,h,wrap", SHAPES + [(300, 129, True), (1024, 512, True)])
+def parallel_lines(w, h):
+    """Whether the device solves the first guess's lines by the partition method (widths and
+    heights of 16 * 2^k cells, 64 to 16384); other shapes take one recurrence per line."""
+    ok = lambda n: n % 16 == 0 and 4 <= n // 16 <= 1024 and (n // 16) & (n // 16 - 1) == 0
+    return ok(w) and ok(h)
+
+
+@pytest.mark.parametrize("w,h,wrap", SHAPES + [(300, 129, True), (1000, 24, False)])
 def test_first_guess_bit_for_bit(ws, w, h, wrap):
+    """One Thomas (or Sherman-Morrison) recurrence per line: the reference's, bit for bit."""
     print(f"lod {w}x{h} wrap={wrap}", flush=True)
+    assert not parallel_lines(w, h)
     b, kd = case(w, h)
     d = dtdiff.Dims(w, h, 2400.0, wrap)
     assert same_bits(dtdiff.lod(ws, d, b, kd, 2e6), O.lod(w, h, 2400.0, wrap, b, kd, 2e6))
+
+
+@pytest.mark.parametrize("w,h,wrap,gen", [(64, 64, True, "shelf"), (128, 64, False, "shelf"),
+                                          (1024, 512, True, "shelf"), (256, 2048, False, "shelf"),
+                                          (2048, 1024, True, "c2"), (8192, 64, True, "c2")])
+def test_parallel_first_guess_matches_the_recurrences(ws, w, h, wrap, gen):
+    """The partition + cyclic-reduction line solver solves the same systems: equal to the
+    oracle's recurrences to 1e-12 of the field's largest value."""
+    print(f"parallel lod {w}x{h} wrap={wrap} {gen}", flush=True)
+    assert parallel_lines(w, h)
+    if gen == "shelf":
+        b, kd = case(w, h)
+        dx, dt = 2400.0, 2e6
+    else:
+        _, b, kd, dt = F.c2(w=w, h=h)
+        dx = 1.0
+    d = dtdiff.Dims(w, h, dx, wrap)
+    got = dtdiff.lod(ws, d, b, kd, dt)
+    want = O.lod(w, h, dx, wrap, b, kd, dt)
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
 
 
 # Widths that are multiples of 64 take the fused, LDS-tiled smoothing kernels on those levels.
