@@ -31,6 +31,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def traffic_of(kernel, w, h):
+    """HBM bytes per launch of `kernel` on the finest level, from the committed counter summary
+    (profiles/traffic_<w>x<h>.json: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors from a
+    float4 copy of known size), or None when no summary covers this grid."""
+    path = os.path.join(REPO, "profiles", f"traffic_{w}x{h}.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)["kernels"][kernel]
+        return t["bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,10 +133,13 @@ def main():
             kernels[name] = {"ms": ms, "bytes": b, "GB/s": b / (ms * 1e-3) / 1e9}
             log(f"kernel {name}: {ms * 1e3:.1f} us, {b / 1e6:.1f} MB, "
                 f"{kernels[name]['GB/s']:.0f} GB/s")
-        sw = kernels["sweep_f32"]
+        # The smoother: the finest level's post-smoothing kernel (corrected sweep + last sweep +
+        # r.z), the larger of the fused pair; compulsory bytes / HIP-event duration.
+        sw = kernels["post_smooth_f32"]
         roof = {"bound": "hbm", "achieved": sw["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": sw["GB/s"] / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "k_sweep<float> (finest level, 28 B/cell)"}
+                "frac": sw["GB/s"] / HBM_PEAK_GBS, "traffic": traffic_of("k_spost<float, 2>", w, h),
+                "kernel": "k_spost<float, 2> (finest level: two damped-Jacobi sweeps + r.z, "
+                          "36 B/cell compulsory)"}
 
     cpu = None
     parity = None
