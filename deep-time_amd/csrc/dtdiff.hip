@@ -37,7 +37,8 @@ struct Error {
     do {                                                                                 \
         hipError_t e_ = (expr);                                                          \
         if (e_ != hipSuccess)                                                            \
-            throw ::dtd::Error{DTD_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+            throw ::dtd::Error{e_ == hipErrorOutOfMemory ? DTD_ETOOBIG : DTD_EDEVICE,    \
+                               std::string(#expr) + ": " + hipGetErrorString(e_)};       \
     } while (0)
 
 #define LAUNCH_CHECK() HIPCHK(hipGetLastError())
@@ -72,16 +73,14 @@ static std::vector<Shape> shapes(int w, int h, int wrap) {
 struct Workspace {
     int w = 0, h = 0, wrap = 0, prec = -1;
     long n = 0;
-    double *b = nullptr, *x = nullptr, *east = nullptr, *south = nullptr, *r = nullptr;
-    double *p0 = nullptr, *p1 = nullptr, *ap = nullptr, *kd = nullptr;
-    double *parts = nullptr, *pair = nullptr;
-    double2 *es = nullptr;  // (east, south) of each cell, for the streamed direction pass
+    double *b = nullptr, *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr;
+    double *ap = nullptr, *kd = nullptr, *parts = nullptr, *pair = nullptr;
+    double2 *es = nullptr;  // the f64 operator: (east, south) of each cell
     CgState *st = nullptr;
     std::vector<Lvl<float>> lf;
     std::vector<Lvl<double>> ld;
     std::vector<void *> owned;  // every allocation of the levels
     hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int zfinal = 1;  // which z buffer of level 0 holds the cycle's result
 };
 
 }  // namespace dtd
@@ -108,18 +107,17 @@ static void free_ws(Workspace &ws) {
             (void)hipGraphExecDestroy(ws.graph[k]);
             ws.graph[k] = nullptr;
         }
-    double *bufs[] = {ws.b, ws.x, ws.east, ws.south, ws.r, ws.p0, ws.p1, ws.ap, ws.kd,
-                      ws.parts, ws.pair};
+    double *bufs[] = {ws.b, ws.x, ws.r, ws.p0, ws.p1, ws.ap, ws.kd, ws.parts, ws.pair};
     for (double *p : bufs)
         if (p) (void)hipFree(p);
-    if (ws.st) (void)hipFree(ws.st);
     if (ws.es) (void)hipFree(ws.es);
+    if (ws.st) (void)hipFree(ws.st);
     for (void *p : ws.owned) (void)hipFree(p);
     ws = Workspace();
 }
 
 // Sizes the workspace for a w x h grid and the cycle's precision; the levels' shapes are
-// allocated here and filled by build_levels.
+// allocated here and filled by build_operator.
 static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
     Workspace &ws = ctx->ws;
     const int prec = ctx->opts.cycle_precision;
@@ -133,8 +131,6 @@ static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
     ws.n = n;
     ws.b = dalloc<double>(n);
     ws.x = dalloc<double>(n);
-    ws.east = dalloc<double>(n);
-    ws.south = dalloc<double>(n);
     ws.r = dalloc<double>(n);
     ws.p0 = dalloc<double>(n);
     ws.p1 = dalloc<double>(n);
@@ -156,35 +152,20 @@ static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
         if (prec == DTD_CYCLE_F64) {
             Lvl<double> L{};
             L.w = sh[l].w, L.h = sh[l].h, L.wrap = sh[l].wrap, L.n = (int)nl;
-            if (l == 0) {
-                L.east = ws.east;
-                L.south = ws.south;
-                L.r = ws.r;
-                L.mass = nullptr;
-            } else {
-                L.east = (double *)own(nl, 8);
-                L.south = (double *)own(nl, 8);
-                L.r = (double *)own(nl, 8);
-                L.mass = (double *)own(nl, 8);
-            }
-            L.diag = (double *)own(nl, 8);
-            L.wd = (double *)own(nl, 8);
+            L.coef = (double *)own(4 * nl, 8);
+            L.mass = l == 0 ? nullptr : (double *)own(nl, 8);
+            L.r = l == 0 ? ws.r : (double *)own(nl, 8);  // the f64 cycle reads the CG residual
             L.z[0] = (double *)own(nl, 8);
             L.z[1] = (double *)own(nl, 8);
-            L.coef = (double *)own(4 * nl, 8);
             ws.ld.push_back(L);
         } else {
             Lvl<float> L{};
             L.w = sh[l].w, L.h = sh[l].h, L.wrap = sh[l].wrap, L.n = (int)nl;
-            L.east = (float *)own(nl, 4);
-            L.south = (float *)own(nl, 4);
-            L.diag = (float *)own(nl, 4);
-            L.wd = (float *)own(nl, 4);
-            L.r = (float *)own(nl, 4);
+            L.coef = (float *)own(4 * nl, 4);
             L.mass = l == 0 ? nullptr : (float *)own(nl, 4);
+            L.r = (float *)own(nl, 4);
             L.z[0] = (float *)own(nl, 4);
             L.z[1] = (float *)own(nl, 4);
-            L.coef = (float *)own(4 * nl, 4);
             ws.lf.push_back(L);
         }
     }
@@ -192,61 +173,25 @@ static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
 
 static Grid grid_of(const Workspace &ws) { return Grid{ws.w, ws.h, ws.wrap, (int)ws.n}; }
 
-// f64 diagonal and omega / diagonal of the finest level of an f64 cycle.
-__global__ void k_finest64(Grid g, const double *__restrict__ east,
-                           const double *__restrict__ south, double *__restrict__ d,
-                           double *__restrict__ wd, double omega) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= g.n) return;
-    const int j = c / g.w, i = c - j * g.w;
-    const double dv = diag64(g, east, south, c, i, j);
-    d[c] = dv;
-    wd[c] = omega / dv;
-}
-
-// `Level::finest` (MG:96-114) from a given f64 operator: f32 copies and omega / diag.
-__global__ void k_finest32(int n, const double *__restrict__ east,
-                           const double *__restrict__ south, const double *__restrict__ diag,
-                           float *__restrict__ e32, float *__restrict__ s32,
-                           float *__restrict__ d32, float *__restrict__ wd32, float omega) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= n) return;
-    e32[c] = (float)east[c];
-    s32[c] = (float)south[c];
-    const float d = (float)diag[c];
-    d32[c] = d;
-    wd32[c] = omega / d;
-}
-
-__global__ void k_copy2(long n, const double *__restrict__ src, double *__restrict__ a,
-                        double *__restrict__ b) {
-    const long c = (long)blockIdx.x * kBlock + threadIdx.x;
-    if (c >= n) return;
-    const double v = src[c];
-    a[c] = v;
-    b[c] = v;
-}
-
-__global__ void k_wd64(int n, const double *__restrict__ d, double *__restrict__ wd,
-                       double omega) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c < n) wd[c] = omega / d[c];
-}
-
-__global__ void k_sumsq(int n, const double *__restrict__ v, double *__restrict__ parts) {
-    __shared__ double lds[kBlock / 64];
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    const double a = c < n ? v[c] * v[c] : 0.0;
-    const double s = block_sum<kBlock>(a, lds);
-    if (threadIdx.x == 0) parts[blockIdx.x] = s;
-}
-
 __global__ void k_sum1(const double *__restrict__ parts, int nparts, double *out) {
     __shared__ double lds[kReduceBlock / 64];
     const double v = reduce_parts(parts, nparts, 1, 0, lds);
     if (threadIdx.x == 0) out[0] = v;
 }
 
+// A fresh state for a solve: tol2 (or, with rel2 > 0, tol2 = rel2 * b.b from the first residual)
+// and the iteration cap.
+__global__ void k_init_state(CgState *st, double tol2, double rel2, int max_iter) {
+    if (threadIdx.x != 0) return;
+    CgState s{};
+    s.tol2 = tol2;
+    s.rel2 = rel2;
+    s.max_iter = max_iter;
+    *st = s;
+}
+
+// A state that never stops and takes every branch of the passes (beta and the pending update on,
+// both zero), for timing the loop's kernels on resident data without changing it.
 __global__ void k_time_state(CgState *st) {
     if (threadIdx.x != 0) return;
     CgState s{};
@@ -256,53 +201,29 @@ __global__ void k_time_state(CgState *st) {
     *st = s;
 }
 
-__global__ void k_init_state(CgState *st, double tol2, int max_iter) {
-    if (threadIdx.x != 0) return;
-    CgState s{};
-    s.tol2 = tol2;
-    s.max_iter = max_iter;
-    *st = s;
-}
-
-// Coarse levels from the finest (`Hierarchy::new`, MG:481-487; device `COARSEN`/`CDIAG`,
-// MGGPU:62-140).
+// The coarse levels from the finest (`Hierarchy::new`, MG:481-487; device `COARSEN`/`CDIAG`,
+// MGGPU:62-140): one launch a level.
 template <typename T>
 static void build_coarse(std::vector<Lvl<T>> &lv, T omega, T scale, hipStream_t s) {
     for (size_t l = 1; l < lv.size(); l++) {
-        k_coarsen<T><<<nblocks(lv[l].n), kBlock, 0, s>>>(lv[l - 1], lv[l], scale);
-        LAUNCH_CHECK();
-        k_cdiag<T><<<nblocks(lv[l].n), kBlock, 0, s>>>(lv[l], omega);
-        LAUNCH_CHECK();
-    }
-    for (size_t l = 0; l < lv.size(); l++) {
-        k_pack<T><<<nblocks(lv[l].n), kBlock, 0, s>>>(lv[l]);
+        k_coarsen<T><<<nblocks(lv[l].n), kBlock, 0, s>>>(lv[l - 1], lv[l], scale, omega);
         LAUNCH_CHECK();
     }
 }
 
-// `Operator::new` on the device from kd (already on the device) and the cycle's levels.
+// `Operator::new` on the device from kd (already on the device), with the cycle's levels.
 static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt) {
     Workspace &ws = ctx->ws;
-    const Grid g = grid_of(ws);
-    const double s2 = dx * dx;
     hipStream_t s = ctx->stream;
-    if (ws.prec == DTD_CYCLE_F64) {
-        k_operator<<<nblocks(ws.n), kBlock, 0, s>>>(g, kd, dt, s2, ws.east, ws.south, nullptr,
-                                                    nullptr, nullptr, nullptr, 0.0f);
-        LAUNCH_CHECK();
-        k_finest64<<<nblocks(ws.n), kBlock, 0, s>>>(g, ws.east, ws.south, ws.ld[0].diag,
-                                                    ws.ld[0].wd, (double)ctx->opts.omega);
-        LAUNCH_CHECK();
-        build_coarse<double>(ws.ld, (double)ctx->opts.omega, (double)ctx->opts.scale, s);
-    } else {
-        Lvl<float> &L = ws.lf[0];
-        k_operator<<<nblocks(ws.n), kBlock, 0, s>>>(g, kd, dt, s2, ws.east, ws.south, L.east,
-                                                    L.south, L.diag, L.wd, ctx->opts.omega);
-        LAUNCH_CHECK();
-        build_coarse<float>(ws.lf, ctx->opts.omega, ctx->opts.scale, s);
-    }
-    k_pack_es<<<nblocks(ws.n), kBlock, 0, s>>>((int)ws.n, ws.east, ws.south, ws.es);
+    const bool f64 = ws.prec == DTD_CYCLE_F64;
+    k_operator<<<nblocks(ws.n), kBlock, 0, s>>>(grid_of(ws), kd, dt, dx * dx, ws.es,
+                                                f64 ? nullptr : ws.lf[0].coef,
+                                                f64 ? ws.ld[0].coef : nullptr, ctx->opts.omega);
     LAUNCH_CHECK();
+    if (f64)
+        build_coarse<double>(ws.ld, (double)ctx->opts.omega, (double)ctx->opts.scale, s);
+    else
+        build_coarse<float>(ws.lf, ctx->opts.omega, ctx->opts.scale, s);
 }
 
 // Band height for a streaming kernel on a w x h level (a multiple of 4 rows; DTD_BAND_ROWS
@@ -330,11 +251,16 @@ static int band_height(const void *kernel, int w, int h) {
     return 4;
 }
 
+template <int V, int H>
+static int stream_grid(int w, int h, int th) {
+    return (stream_waves<V, H>(w, h, th) + kBlock / 64 - 1) / (kBlock / 64);
+}
+
 template <typename T, int V>
 static void launch_pre(Lvl<T> &L, Lvl<T> &C, T *zout, const CgState *st, hipStream_t s) {
     const int th = band_height<V>((const void *)k_spre<T, V>, L.w, L.h);
-    const int nb = (stream_waves<V>(L.w, L.h, th) + kBlock / 64 - 1) / (kBlock / 64);
-    k_spre<T, V><<<nb, kBlock, 0, s>>>(L, zout, C, th, st);
+    k_spre<T, V><<<stream_grid<V, LaneGeom<V>::halo>(L.w, L.h, th), kBlock, 0, s>>>(L, zout, C,
+                                                                                     th, st);
     LAUNCH_CHECK();
 }
 
@@ -342,7 +268,7 @@ template <typename T, int V>
 static int launch_post(Lvl<T> &L, const T *zin, const T *zc, int wc, T *zout, const double *dot,
                        double *parts, const CgState *st, hipStream_t s) {
     const int th = band_height<V>((const void *)k_spost<T, V>, L.w, L.h);
-    const int nb = (stream_waves<V>(L.w, L.h, th) + kBlock / 64 - 1) / (kBlock / 64);
+    const int nb = stream_grid<V, LaneGeom<V>::halo>(L.w, L.h, th);
     k_spost<T, V><<<nb, kBlock, 0, s>>>(L, zin, zc, wc, zout, th, dot, parts, st);
     LAUNCH_CHECK();
     return nb;
@@ -353,9 +279,29 @@ template <typename TZ>
 static int launch_dir(Workspace &ws, const TZ *z, const double *pold, double *pnew,
                       hipStream_t s) {
     const int th = band_height<1, 1>((const void *)k_sdir<TZ>, ws.w, ws.h);
-    const int nb = (stream_waves<1, 1>(ws.w, ws.h, th) + kBlock / 64 - 1) / (kBlock / 64);
+    const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
     k_sdir<TZ><<<nb, kBlock, 0, s>>>(grid_of(ws), ws.es, z, pold, pnew, ws.ap, ws.x, th, ws.parts,
                                      ws.st);
+    LAUNCH_CHECK();
+    return nb;
+}
+
+// The first residual r = b - A xin (xout = xin's copy when they differ); returns the block count
+// (the (r.r, b.b) partial pairs).
+static int launch_resid(Workspace &ws, const double *b, const double *xin, hipStream_t s) {
+    const Grid g = grid_of(ws);
+    if (ws.prec == DTD_CYCLE_F64) {
+        const int th = band_height<1, 1>((const void *)k_sresid<double>, ws.w, ws.h);
+        const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
+        k_sresid<double><<<nb, kBlock, 0, s>>>(g, ws.es, b, xin, ws.x, ws.r, (double *)nullptr,
+                                               th, ws.parts);
+        LAUNCH_CHECK();
+        return nb;
+    }
+    const int th = band_height<1, 1>((const void *)k_sresid<float>, ws.w, ws.h);
+    const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
+    k_sresid<float><<<nb, kBlock, 0, s>>>(g, ws.es, b, xin, ws.x, ws.r, ws.lf[0].r, th,
+                                          ws.parts);
     LAUNCH_CHECK();
     return nb;
 }
@@ -397,9 +343,8 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
         else
             launch_pre<T, 1>(L, C0, L.z[0], st, s);
         vcycle<T>(lv, l + 1, nu, nullptr, parts, st, s);
-        const int nbs = wide ? launch_post<T, VW>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts, st, s)
-                             : launch_post<T, 1>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts, st, s);
-        return nbs;
+        return wide ? launch_post<T, VW>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts, st, s)
+                    : launch_post<T, 1>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts, st, s);
     }
     int cur = 0;
     k_first<T><<<nb, kBlock, 0, s>>>(L, L.z[0], st);
@@ -410,13 +355,12 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
         LAUNCH_CHECK();
         cur ^= 1;
     }
-    Lvl<T> &C = lv[l + 1];
-    k_restrict<T><<<nblocks(C.n), kBlock, 0, s>>>(L, L.z[cur], C, st);
+    k_restrict<T><<<nblocks(C0.n), kBlock, 0, s>>>(L, L.z[cur], C0, st);
     LAUNCH_CHECK();
     vcycle<T>(lv, l + 1, nu, nullptr, parts, st, s);
     {
         const double *d = nu == 1 ? dot : nullptr;
-        k_sweep<T><<<nb, kBlock, 0, s>>>(L, L.z[cur], L.z[cur ^ 1], C.z[1], C.w, d, parts, st);
+        k_sweep<T><<<nb, kBlock, 0, s>>>(L, L.z[cur], L.z[cur ^ 1], C0.z[1], C0.w, d, parts, st);
         LAUNCH_CHECK();
         cur ^= 1;
     }
@@ -442,21 +386,21 @@ static int run_vcycle(dtd_ctx *ctx, const double *dot, const CgState *st) {
 // p0 (odd iterations), parity 1 the other way.
 static void enqueue_iteration(dtd_ctx *ctx, int parity) {
     Workspace &ws = ctx->ws;
-    const Grid g = grid_of(ws);
     hipStream_t s = ctx->stream;
-    const int nb = nblocks(ws.n, kCgCells);
     double *pold = parity == 0 ? ws.p1 : ws.p0;
     double *pnew = parity == 0 ? ws.p0 : ws.p1;
     const int nbd = ws.prec == DTD_CYCLE_F64 ? launch_dir<double>(ws, ws.ld[0].z[1], pold, pnew, s)
                                              : launch_dir<float>(ws, ws.lf[0].z[1], pold, pnew, s);
     k_cg_step<kStepPAP><<<1, kReduceBlock, 0, s>>>(ws.parts, nbd, ws.st);
     LAUNCH_CHECK();
+    const int nbu = nblocks(ws.n, kCgCells);
     if (ws.prec == DTD_CYCLE_F64)
-        k_update<double><<<nb, kBlock, 0, s>>>((int)ws.n, ws.ap, ws.r, nullptr, ws.parts, ws.st);
+        k_update<double><<<nbu, kBlock, 0, s>>>((int)ws.n, ws.ap, ws.r, nullptr, ws.parts, ws.st);
     else
-        k_update<float><<<nb, kBlock, 0, s>>>((int)ws.n, ws.ap, ws.r, ws.lf[0].r, ws.parts, ws.st);
+        k_update<float><<<nbu, kBlock, 0, s>>>((int)ws.n, ws.ap, ws.r, ws.lf[0].r, ws.parts,
+                                               ws.st);
     LAUNCH_CHECK();
-    k_cg_step<kStepRR><<<1, kReduceBlock, 0, s>>>(ws.parts, nb, ws.st);
+    k_cg_step<kStepRR><<<1, kReduceBlock, 0, s>>>(ws.parts, nbu, ws.st);
     LAUNCH_CHECK();
     const int nbz = run_vcycle(ctx, ws.r, ws.st);
     k_cg_step<kStepRZ><<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.st);
@@ -483,7 +427,7 @@ static void capture_graphs(dtd_ctx *ctx) {
     }
 }
 
-// Whether the parallel line solver takes lines of n cells: 16 * 2^k cells, 3 to 1024 segments.
+// Whether the parallel line solver takes lines of n cells: 16 * 2^k cells, 4 to 1024 segments.
 static bool parallel_line(int n) {
     if (n % kSeg) return false;
     const int p = n / kSeg;
@@ -498,35 +442,40 @@ static bool parallel_lod_enabled() {
     return on;
 }
 
-static void lod(dtd_ctx *ctx, double *x, const double *kd, double dx, double dt, int wrap_x) {
+// `lod` (D:370-489): the splitting's first guess of b into x (x may be b).
+static void lod(dtd_ctx *ctx, const double *b, double *x, const double *kd, double dx, double dt,
+                int wrap_x) {
     Workspace &ws = ctx->ws;
     const Grid g = grid_of(ws);
     const double s2 = dx * dx;
+    hipStream_t s = ctx->stream;
     if (parallel_lod_enabled() && parallel_line(ws.w) && parallel_line(ws.h)) {
-        // Rows in place; then the columns as rows of the transposed grid (kd^T in p0, x^T in
-        // p1), transposed back into x.
-        hipStream_t s = ctx->stream;
-        auto lines = [&](int nlines, int n, int ring, const double *k, double *xx) {
+        // Rows from b into x; then the columns as rows of the transposed grid (kd^T in p0, x^T
+        // in p1), transposed back into x.
+        auto lines = [&](int nlines, int n, int ring, const double *k, const double *src,
+                         double *dst) {
             if (n / kSeg <= 512)
-                k_lod_lines<512><<<nlines, n / kSeg, 0, s>>>(n, ring, k, xx, dt, s2);
+                k_lod_lines<512><<<nlines, n / kSeg, 0, s>>>(n, ring, k, src, dst, dt, s2);
             else
-                k_lod_lines<1024><<<nlines, n / kSeg, 0, s>>>(n, ring, k, xx, dt, s2);
+                k_lod_lines<1024><<<nlines, n / kSeg, 0, s>>>(n, ring, k, src, dst, dt, s2);
             LAUNCH_CHECK();
         };
-        lines(ws.h, ws.w, wrap_x && ws.w > 2, kd, x);
+        lines(ws.h, ws.w, wrap_x && ws.w > 2, kd, b, x);
         const dim3 tg((ws.w + 31) / 32, (ws.h + 31) / 32), tgt((ws.h + 31) / 32, (ws.w + 31) / 32);
         k_transpose<<<tg, 256, 0, s>>>(kd, ws.p0, ws.h, ws.w);
         LAUNCH_CHECK();
         k_transpose<<<tg, 256, 0, s>>>(x, ws.p1, ws.h, ws.w);
         LAUNCH_CHECK();
-        lines(ws.w, ws.h, 0, ws.p0, ws.p1);
+        lines(ws.w, ws.h, 0, ws.p0, ws.p1, ws.p1);
         k_transpose<<<tgt, 256, 0, s>>>(ws.p1, x, ws.w, ws.h);
         LAUNCH_CHECK();
         return;
     }
-    k_lod_rows<<<nblocks(ws.h), kBlock, 0, ctx->stream>>>(g, wrap_x, kd, dt, s2, x, ws.p0, ws.p1);
+    if (x != b)
+        HIPCHK(hipMemcpyAsync(x, b, ws.n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    k_lod_rows<<<nblocks(ws.h), kBlock, 0, s>>>(g, wrap_x, kd, dt, s2, x, ws.p0, ws.p1);
     LAUNCH_CHECK();
-    k_lod_cols<<<nblocks(ws.w), kBlock, 0, ctx->stream>>>(g, kd, dt, s2, x, ws.p0);
+    k_lod_cols<<<nblocks(ws.w), kBlock, 0, s>>>(g, kd, dt, s2, x, ws.p0);
     LAUNCH_CHECK();
 }
 
@@ -536,31 +485,34 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-// The solve on the workspace: ws.b and ws.x hold h0, kd is on the device.  Leaves h1 in ws.x.
-// Returns the iterations.  `MG:562-763` after `lod` (D:101-116).
-static size_t solve_core(dtd_ctx *ctx, const double *kd, double dx, double dt, int wrap_x,
-                         double tol_m, bool setup, bool guess, dtd_stats *stats) {
+enum Guess { kGuessLod, kGuessRhs, kGuessKeep };
+
+// One solve on the workspace (`multigrid_in` + `cg_on`, D:101-116, MG:562-763).  b is the
+// right-hand side on the device (the caller's array or ws.b; only read); kd likewise.  The first
+// guess is the splitting of b (or b itself when the context says so), or (kGuessKeep) the x
+// already in ws.x.  With rel2 > 0 the tolerance is tol2 = rel2 * b.b, else tol_m^2 * n.  Leaves
+// h1 in ws.x and returns the iterations; *rel gets ||b - A h1|| / ||b|| and *bnorm ||b||.
+static size_t solve_core(dtd_ctx *ctx, const double *b, const double *kd, double dx, double dt,
+                         int wrap_x, double tol_m, double rel2, bool setup, Guess guess,
+                         dtd_stats *stats, double *rel, double *bnorm) {
     Workspace &ws = ctx->ws;
     hipStream_t s = ctx->stream;
     const Grid g = grid_of(ws);
-    const int nb = nblocks(ws.n);
     HIPCHK(hipEventRecord(ctx->ev[0], s));
     if (setup) build_operator(ctx, kd, dx, dt);
     HIPCHK(hipEventRecord(ctx->ev[1], s));
-    if (guess && ctx->opts.guess == DTD_GUESS_LOD) lod(ctx, ws.x, kd, dx, dt, wrap_x);
+    const double *xin = ws.x;
+    if (guess == kGuessLod && ctx->opts.guess == DTD_GUESS_LOD) {
+        lod(ctx, b, ws.x, kd, dx, dt, wrap_x);
+    } else if (guess != kGuessKeep) {
+        xin = b;  // the first residual copies b into x
+    }
     HIPCHK(hipEventRecord(ctx->ev[2], s));
-    const double tol2 = tol_m * tol_m * (double)ws.n;
-    k_init_state<<<1, 64, 0, s>>>(ws.st, tol2, (int)ctx->opts.max_iter);
+    k_init_state<<<1, 64, 0, s>>>(ws.st, tol_m * tol_m * (double)ws.n, rel2,
+                                  (int)ctx->opts.max_iter);
     LAUNCH_CHECK();
-    const int nbc = nblocks(ws.n, kCgCells);
-    if (ws.prec == DTD_CYCLE_F64)
-        k_resid_init<double><<<nbc, kBlock, 0, s>>>(g, ws.east, ws.south, ws.b, ws.x, ws.r,
-                                                    (double *)nullptr, ws.parts);
-    else
-        k_resid_init<float><<<nbc, kBlock, 0, s>>>(g, ws.east, ws.south, ws.b, ws.x, ws.r,
-                                                   ws.lf[0].r, ws.parts);
-    LAUNCH_CHECK();
-    k_cg_step<kStepRR0><<<1, kReduceBlock, 0, s>>>(ws.parts, nbc, ws.st);
+    const int nbr = launch_resid(ws, b, xin, s);
+    k_cg_step<kStepRR0><<<1, kReduceBlock, 0, s>>>(ws.parts, nbr, ws.st);
     LAUNCH_CHECK();
     const int nbz = run_vcycle(ctx, ws.r, ws.st);
     k_cg_step<kStepRZ0><<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.st);
@@ -583,18 +535,30 @@ static size_t solve_core(dtd_ctx *ctx, const double *kd, double dx, double dt, i
             if (ctx->host_flag[par ^ 1]) break;
         }
     }
-    k_final_x<<<nb, kBlock, 0, s>>>((int)ws.n, ws.p0, ws.p1, ws.x, ws.st);
-    LAUNCH_CHECK();
-    // Mass put-back (MG:737-761).
-    k_putback_sums<<<nb, kBlock, 0, s>>>(g, ws.east, ws.south, ws.b, ws.x, ws.parts);
-    LAUNCH_CHECK();
-    k_putback_step<<<1, kReduceBlock, 0, s>>>(ws.parts, nb, ws.st);
-    LAUNCH_CHECK();
-    k_putback_apply<<<nb, kBlock, 0, s>>>(g, ws.east, ws.south, ws.x, ws.st);
-    LAUNCH_CHECK();
+    // The last update and the put-back's sums (x' into ap), the put-back and the true residual
+    // (x'' into x).
+    {
+        const int th = band_height<1, 1>((const void *)k_finish1, ws.w, ws.h);
+        const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
+        k_finish1<<<nb, kBlock, 0, s>>>(g, ws.es, b, ws.x, ws.p0, ws.p1, ws.ap, th, ws.parts,
+                                        ws.st);
+        LAUNCH_CHECK();
+        k_putback_step<<<1, kReduceBlock, 0, s>>>(ws.parts, nb, ws.st);
+        LAUNCH_CHECK();
+        const int th2 = band_height<1, 1>((const void *)k_finish2, ws.w, ws.h);
+        const int nb2 = stream_grid<1, 1>(ws.w, ws.h, th2);
+        k_finish2<<<nb2, kBlock, 0, s>>>(g, ws.es, b, ws.ap, ws.x, th2, ws.parts, ws.st);
+        LAUNCH_CHECK();
+        k_pair_sum<<<1, kReduceBlock, 0, s>>>(ws.parts, nb2, ws.pair);
+        LAUNCH_CHECK();
+    }
     HIPCHK(hipEventRecord(ctx->ev[3], s));
     HIPCHK(hipMemcpyAsync(ctx->host_st, ws.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(ctx->host_pair, ws.pair, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    const double rr = ctx->host_pair[0], bb = ctx->host_pair[1];
+    if (rel) *rel = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    if (bnorm) *bnorm = std::sqrt(bb);
     if (stats) {
         stats->iterations += (size_t)ctx->host_st->its;
         stats->ms_setup += elapsed(ctx->ev[0], ctx->ev[1]);
@@ -603,22 +567,6 @@ static size_t solve_core(dtd_ctx *ctx, const double *kd, double dx, double dt, i
         stats->ms_total += elapsed(ctx->ev[0], ctx->ev[3]);
     }
     return (size_t)ctx->host_st->its;
-}
-
-// ||b - A x|| / ||b|| on the workspace (b, x, east, south built).
-static double rel_residual(dtd_ctx *ctx) {
-    Workspace &ws = ctx->ws;
-    const int nb = nblocks(ws.n);
-    k_true_resid<<<nb, kBlock, 0, ctx->stream>>>(grid_of(ws), ws.east, ws.south, ws.b, ws.x,
-                                                 ws.parts);
-    LAUNCH_CHECK();
-    k_pair_sum<<<1, kReduceBlock, 0, ctx->stream>>>(ws.parts, nb, ws.pair);
-    LAUNCH_CHECK();
-    HIPCHK(hipMemcpyAsync(ctx->host_pair, ws.pair, 2 * sizeof(double), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    const double rr = ctx->host_pair[0], bb = ctx->host_pair[1];
-    return bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
 }
 
 static void check_grid(size_t w, size_t h) {
@@ -634,12 +582,12 @@ static void begin(dtd_ctx *ctx) {
     HIPCHK(hipSetDevice(ctx->opts.device));
 }
 
-// Device copy of a host array on the context's stream (synchronous).
-static void upload(dtd_ctx *ctx, double *dst, const double *src, size_t n) {
-    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+// Device copy of a host array on the context's stream.
+static void upload(dtd_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
 }
-static void download(dtd_ctx *ctx, double *dst, const double *src, size_t n) {
-    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+static void download(dtd_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
 }
 
@@ -650,7 +598,7 @@ static int guard(dtd_ctx *ctx, F f) {
         return DTD_OK;
     } catch (const Error &e) {
         if (ctx) ctx->err = e.msg;
-        if (e.code == DTD_EDEVICE && ctx) {
+        if ((e.code == DTD_EDEVICE || e.code == DTD_ETOOBIG) && ctx) {
             // A failed shape is rebuilt on the next call.
             (void)hipStreamSynchronize(ctx->stream);
             free_ws(ctx->ws);
@@ -661,6 +609,22 @@ static int guard(dtd_ctx *ctx, F f) {
         if (ctx) ctx->err = e.what();
         return DTD_EDEVICE;
     }
+}
+
+// Calibration kernels of dtd_time_kernel: a float4 copy, and five float4 input streams combined
+// into one output (the smoother's stream count without its stencil).
+__global__ void k_copy4(long n4, const float4 *__restrict__ a, float4 *__restrict__ o) {
+    const long k = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (k < n4) o[k] = a[k];
+}
+__global__ void k_stream5(long n4, const float4 *__restrict__ a, const float4 *__restrict__ b,
+                          const float4 *__restrict__ c, const float4 *__restrict__ d,
+                          const float4 *__restrict__ e, float4 *__restrict__ o) {
+    const long k = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n4) return;
+    const float4 p = a[k], q = b[k], r = c[k], t = d[k], u = e[k];
+    o[k] = make_float4(p.x + q.x * (r.x - t.x * u.x), p.y + q.y * (r.y - t.y * u.y),
+                       p.z + q.z * (r.z - t.z * u.z), p.w + q.w * (r.w - t.w * u.w));
 }
 
 }  // namespace dtd
@@ -746,12 +710,11 @@ int dtd_solve(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h
         if (!h_inout || !kd) throw Error{DTD_EINVAL, "null array"};
         fit(ctx, (int)w, (int)h, wrap_x && w > 1);
         Workspace &ws = ctx->ws;
-        upload(ctx, ws.kd, kd, n);
-        upload(ctx, ws.b, h_inout, n);
-        HIPCHK(hipMemcpyAsync(ws.x, ws.b, n * sizeof(double), hipMemcpyDeviceToDevice,
-                              ctx->stream));
-        const size_t its = solve_core(ctx, ws.kd, dx, dt, wrap_x, tol_m, true, true, nullptr);
-        download(ctx, h_inout, ws.x, n);
+        upload(ctx, ws.kd, kd, n * sizeof(double));
+        upload(ctx, ws.b, h_inout, n * sizeof(double));
+        const size_t its = solve_core(ctx, ws.b, ws.kd, dx, dt, wrap_x, tol_m, 0.0, true,
+                                      kGuessLod, nullptr, nullptr, nullptr);
+        download(ctx, h_inout, ws.x, n * sizeof(double));
         if (iterations) *iterations = its;
     });
 }
@@ -769,10 +732,10 @@ int dtd_solve_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, do
         if (!h_dev || !kd_dev) throw Error{DTD_EINVAL, "null array"};
         fit(ctx, (int)w, (int)h, wrap_x && w > 1);
         Workspace &ws = ctx->ws;
-        k_copy2<<<nblocks(n), kBlock, 0, ctx->stream>>>((long)n, h_dev, ws.b, ws.x);
-        LAUNCH_CHECK();
-        solve_core(ctx, kd_dev, dx, dt, wrap_x, tol_m, true, true, stats);
-        if (stats) stats->rel_residual = rel_residual(ctx);
+        double rel = 0.0;
+        solve_core(ctx, h_dev, kd_dev, dx, dt, wrap_x, tol_m, 0.0, true, kGuessLod, stats, &rel,
+                   nullptr);
+        if (stats) stats->rel_residual = rel;
         HIPCHK(hipMemcpyAsync(h_dev, ws.x, n * sizeof(double), hipMemcpyDeviceToDevice,
                               ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -794,33 +757,19 @@ int dtd_solve_rel_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x
         if (!h_dev || !kd_dev) throw Error{DTD_EINVAL, "null array"};
         fit(ctx, (int)w, (int)h, wrap_x && w > 1);
         Workspace &ws = ctx->ws;
-        k_copy2<<<nblocks(n), kBlock, 0, ctx->stream>>>((long)n, h_dev, ws.b, ws.x);
-        LAUNCH_CHECK();
-        // ||b||, for the first solve's tolerance.
-        k_sumsq<<<nblocks(n), kBlock, 0, ctx->stream>>>((int)n, ws.b, ws.parts);
-        LAUNCH_CHECK();
-        k_sum1<<<1, kReduceBlock, 0, ctx->stream>>>(ws.parts, nblocks(n), ws.pair);
-        LAUNCH_CHECK();
-        HIPCHK(hipMemcpyAsync(ctx->host_pair, ws.pair, sizeof(double), hipMemcpyDeviceToHost,
-                              ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        const double bnorm = std::sqrt(ctx->host_pair[0]);
-        if (bnorm == 0.0) {
-            HIPCHK(hipMemcpyAsync(h_dev, ws.b, n * sizeof(double), hipMemcpyDeviceToDevice,
-                                  ctx->stream));
-            HIPCHK(hipStreamSynchronize(ctx->stream));
-            return;
-        }
+        // The first solve stops at r.r <= rel^2 b.b (tol_m = rel ||b|| / sqrt(n)), b.b formed on
+        // the device by the first residual's pass.
+        double rel = 0.0, bnorm = 0.0;
+        solve_core(ctx, h_dev, kd_dev, dx, dt, wrap_x, 0.0, rel_tol * rel_tol, true, kGuessLod,
+                   st, &rel, &bnorm);
         double tol_m = rel_tol * bnorm / std::sqrt((double)n);
-        solve_core(ctx, kd_dev, dx, dt, wrap_x, tol_m, true, true, st);
-        double rel = rel_residual(ctx);
-        while (rel > rel_tol && st->restarts < 4) {
+        while (rel > rel_tol && st->restarts < 4 && bnorm > 0.0) {
             // Solve again from the last answer (setup kept), aiming below the target by the
             // factor the last solve missed it by.
             tol_m *= 0.5 * rel_tol / rel;
             st->restarts++;
-            solve_core(ctx, kd_dev, dx, dt, wrap_x, tol_m, false, false, st);
-            rel = rel_residual(ctx);
+            solve_core(ctx, h_dev, kd_dev, dx, dt, wrap_x, tol_m, 0.0, false, kGuessKeep, st,
+                       &rel, &bnorm);
         }
         st->rel_residual = rel;
         HIPCHK(hipMemcpyAsync(h_dev, ws.x, n * sizeof(double), hipMemcpyDeviceToDevice,
@@ -839,17 +788,16 @@ int dtd_operator(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, const 
         if (!kd || !east || !south || !diag) throw Error{DTD_EINVAL, "null array"};
         fit(ctx, (int)w, (int)h, wrap_x && w > 1);
         Workspace &ws = ctx->ws;
-        upload(ctx, ws.kd, kd, n);
+        upload(ctx, ws.kd, kd, n * sizeof(double));
         const Grid g = grid_of(ws);
-        k_operator<<<nblocks(n), kBlock, 0, ctx->stream>>>(g, ws.kd, dt, dx * dx, ws.east,
-                                                           ws.south, nullptr, nullptr, nullptr,
+        k_operator<<<nblocks(n), kBlock, 0, ctx->stream>>>(g, ws.kd, dt, dx * dx, ws.es, nullptr,
                                                            nullptr, 0.0f);
         LAUNCH_CHECK();
-        k_finest64<<<nblocks(n), kBlock, 0, ctx->stream>>>(g, ws.east, ws.south, ws.ap, ws.r, 1.0);
+        k_unpack_es<<<nblocks(n), kBlock, 0, ctx->stream>>>(g, ws.es, ws.p0, ws.p1, ws.ap);
         LAUNCH_CHECK();
-        download(ctx, east, ws.east, n);
-        download(ctx, south, ws.south, n);
-        download(ctx, diag, ws.ap, n);
+        download(ctx, east, ws.p0, n * sizeof(double));
+        download(ctx, south, ws.p1, n * sizeof(double));
+        download(ctx, diag, ws.ap, n * sizeof(double));
     });
 }
 
@@ -863,16 +811,15 @@ int dtd_apply(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, const dou
         if (!kd || !x || !y) throw Error{DTD_EINVAL, "null array"};
         fit(ctx, (int)w, (int)h, wrap_x && w > 1);
         Workspace &ws = ctx->ws;
-        upload(ctx, ws.kd, kd, n);
-        upload(ctx, ws.x, x, n);
+        upload(ctx, ws.kd, kd, n * sizeof(double));
+        upload(ctx, ws.x, x, n * sizeof(double));
         const Grid g = grid_of(ws);
-        k_operator<<<nblocks(n), kBlock, 0, ctx->stream>>>(g, ws.kd, dt, dx * dx, ws.east,
-                                                           ws.south, nullptr, nullptr, nullptr,
+        k_operator<<<nblocks(n), kBlock, 0, ctx->stream>>>(g, ws.kd, dt, dx * dx, ws.es, nullptr,
                                                            nullptr, 0.0f);
         LAUNCH_CHECK();
-        k_apply<<<nblocks(n), kBlock, 0, ctx->stream>>>(g, ws.east, ws.south, ws.x, ws.ap);
+        k_apply<<<nblocks(n), kBlock, 0, ctx->stream>>>(g, ws.es, ws.x, ws.ap);
         LAUNCH_CHECK();
-        download(ctx, y, ws.ap, n);
+        download(ctx, y, ws.ap, n * sizeof(double));
     });
 }
 
@@ -886,27 +833,11 @@ int dtd_lod(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h_i
         if (!kd || !h_inout) throw Error{DTD_EINVAL, "null array"};
         fit(ctx, (int)w, (int)h, wrap_x && w > 1);
         Workspace &ws = ctx->ws;
-        upload(ctx, ws.kd, kd, n);
-        upload(ctx, ws.x, h_inout, n);
-        lod(ctx, ws.x, ws.kd, dx, dt, wrap_x);
-        download(ctx, h_inout, ws.x, n);
+        upload(ctx, ws.kd, kd, n * sizeof(double));
+        upload(ctx, ws.b, h_inout, n * sizeof(double));
+        lod(ctx, ws.b, ws.x, ws.kd, dx, dt, wrap_x);
+        download(ctx, h_inout, ws.x, n * sizeof(double));
     });
-}
-
-// Calibration kernels of dtd_time_kernel: a float4 copy, and five float4 input streams combined
-// into one output (the smoother's stream count without its stencil).
-__global__ void k_copy4(long n4, const float4 *__restrict__ a, float4 *__restrict__ o) {
-    const long k = (long)blockIdx.x * kBlock + threadIdx.x;
-    if (k < n4) o[k] = a[k];
-}
-__global__ void k_stream5(long n4, const float4 *__restrict__ a, const float4 *__restrict__ b,
-                          const float4 *__restrict__ c, const float4 *__restrict__ d,
-                          const float4 *__restrict__ e, float4 *__restrict__ o) {
-    const long k = (long)blockIdx.x * kBlock + threadIdx.x;
-    if (k >= n4) return;
-    const float4 p = a[k], q = b[k], r = c[k], t = d[k], u = e[k];
-    o[k] = make_float4(p.x + q.x * (r.x - t.x * u.x), p.y + q.y * (r.y - t.y * u.y),
-                       p.z + q.z * (r.z - t.z * u.z), p.w + q.w * (r.w - t.w * u.w));
 }
 
 int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *bytes) {
@@ -917,19 +848,17 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
             throw Error{DTD_ESTATE, "time a kernel after an f32-cycle solve of a multi-level grid"};
         if (reps < 1) throw Error{DTD_EINVAL, "reps must be positive"};
         hipStream_t s = ctx->stream;
-        const Grid g = grid_of(ws);
         Lvl<float> &L = ws.lf[0];
+        Lvl<float> &C = ws.lf[1];
         const int nb = nblocks(ws.n);
         const int nbc = nblocks(ws.n, kCgCells);
         const double n = (double)ws.n;
         double b = 0.0;
-        // A private state that never stops and takes every branch of the passes (beta and the
-        // pending update on, both zero), so the loop kernels run in full and change nothing.
         k_time_state<<<1, 64, 0, s>>>(ws.st);
         LAUNCH_CHECK();
         auto one = [&] {
             switch (kernel) {
-            case DTD_KERNEL_SWEEP:  // r, wd, diag, east, south, z in; z out (f32)
+            case DTD_KERNEL_SWEEP:  // r, record, z in; z out (f32)
                 k_sweep<float><<<nb, kBlock, 0, s>>>(L, L.z[1], L.z[0], nullptr, 0, nullptr,
                                                      nullptr, ws.st);
                 b = 28.0 * n;
@@ -940,40 +869,35 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
                 break;
             case DTD_KERNEL_UPDATE:  // A p, r in; r, r32 out
                 k_update<float><<<nbc, kBlock, 0, s>>>((int)ws.n, ws.ap, ws.r, L.r, ws.parts,
-                                                      ws.st);
+                                                       ws.st);
                 b = (8.0 * 3 + 4.0) * n;
                 break;
-            case DTD_KERNEL_RESTRICT: {  // fine r, z, diag, east, south in; coarse r out
-                Lvl<float> &C = ws.lf[1];
+            case DTD_KERNEL_RESTRICT:  // fine r, z, record in; coarse r out
                 k_restrict<float><<<nblocks(C.n), kBlock, 0, s>>>(L, L.z[1], C, ws.st);
-                b = 20.0 * n + 4.0 * C.n;
+                b = 24.0 * n + 4.0 * C.n;
                 break;
-            }
-            case DTD_KERNEL_PRE:  // r, wd, diag, east, south in; z and the coarse r out
-            case DTD_KERNEL_POST: {  // z, r, wd, diag, east, south, r64, coarse z in; z out
+            case DTD_KERNEL_PRE:  // r, record in; z and the coarse r out
                 if (L.w % 2) throw Error{DTD_ESTATE, "the finest width is odd"};
-                Lvl<float> &C = ws.lf[1];
-                if (kernel == DTD_KERNEL_PRE) {
-                    launch_pre<float, 2>(L, C, L.z[0], ws.st, s);
-                    b = 24.0 * n + 4.0 * C.n;
-                } else {
-                    launch_post<float, 2>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts, ws.st, s);
-                    b = 36.0 * n + 4.0 * C.n;
-                }
+                launch_pre<float, 2>(L, C, L.z[0], ws.st, s);
+                b = 24.0 * n + 4.0 * C.n;
                 break;
-            }
+            case DTD_KERNEL_POST:  // z, r, record, r64, coarse z in; z out
+                if (L.w % 2) throw Error{DTD_ESTATE, "the finest width is odd"};
+                launch_post<float, 2>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts, ws.st, s);
+                b = 36.0 * n + 4.0 * C.n;
+                break;
             case DTD_KERNEL_COPY:  // r32 in, z out
-            case DTD_KERNEL_STREAM5: {  // r, wd, diag, east, south in; z out
+            case DTD_KERNEL_STREAM5: {  // five arrays of n floats in; one out
                 const long n4 = ws.n / 4;
                 const int nb4 = (int)((n4 + kBlock - 1) / kBlock);
                 if (kernel == DTD_KERNEL_COPY) {
                     k_copy4<<<nb4, kBlock, 0, s>>>(n4, (const float4 *)L.r, (float4 *)L.z[0]);
                     b = 8.0 * 4 * n4;
                 } else {
-                    k_stream5<<<nb4, kBlock, 0, s>>>(n4, (const float4 *)L.r, (const float4 *)L.wd,
-                                                     (const float4 *)L.diag,
-                                                     (const float4 *)L.east,
-                                                     (const float4 *)L.south, (float4 *)L.z[0]);
+                    k_stream5<<<nb4, kBlock, 0, s>>>(n4, (const float4 *)L.r,
+                                                     (const float4 *)L.z[1],
+                                                     (const float4 *)ws.p0, (const float4 *)ws.p1,
+                                                     (const float4 *)ws.ap, (float4 *)L.z[0]);
                     b = 24.0 * 4 * n4;
                 }
                 break;
@@ -1004,20 +928,18 @@ int dtd_cycle_new(dtd_ctx *ctx, size_t w, size_t h, int wrap_x, const double *ea
         if (!east || !south || !diag) throw Error{DTD_EINVAL, "null array"};
         fit(ctx, (int)w, (int)h, wrap_x && w > 1);
         Workspace &ws = ctx->ws;
-        upload(ctx, ws.east, east, n);
-        upload(ctx, ws.south, south, n);
-        upload(ctx, ws.ap, diag, n);
+        upload(ctx, ws.p0, east, n * sizeof(double));
+        upload(ctx, ws.p1, south, n * sizeof(double));
+        upload(ctx, ws.ap, diag, n * sizeof(double));
         hipStream_t s = ctx->stream;
         if (ws.prec == DTD_CYCLE_F64) {
-            HIPCHK(hipMemcpyAsync(ws.ld[0].diag, ws.ap, n * 8, hipMemcpyDeviceToDevice, s));
-            k_wd64<<<nblocks(n), kBlock, 0, s>>>((int)n, ws.ld[0].diag, ws.ld[0].wd,
-                                                 (double)ctx->opts.omega);
+            k_finest64<<<nblocks(n), kBlock, 0, s>>>((int)n, ws.p0, ws.p1, ws.ap, ws.ld[0].coef,
+                                                     (double)ctx->opts.omega);
             LAUNCH_CHECK();
             build_coarse<double>(ws.ld, (double)ctx->opts.omega, (double)ctx->opts.scale, s);
         } else {
-            Lvl<float> &L = ws.lf[0];
-            k_finest32<<<nblocks(n), kBlock, 0, s>>>((int)n, ws.east, ws.south, ws.ap, L.east,
-                                                     L.south, L.diag, L.wd, ctx->opts.omega);
+            k_finest32<<<nblocks(n), kBlock, 0, s>>>((int)n, ws.p0, ws.p1, ws.ap, ws.lf[0].coef,
+                                                     ctx->opts.omega);
             LAUNCH_CHECK();
             build_coarse<float>(ws.lf, ctx->opts.omega, ctx->opts.scale, s);
         }
@@ -1040,20 +962,18 @@ int dtd_cycle_level(dtd_ctx *ctx, size_t l, size_t *w, size_t *h, int *wrap, flo
         if (h) *h = (size_t)L.h;
         if (wrap) *wrap = L.wrap;
         const size_t n = (size_t)L.n;
-        auto get = [&](float *dst, const float *src) {
-            if (dst) HIPCHK(hipMemcpyAsync(dst, src, n * 4, hipMemcpyDeviceToHost, ctx->stream));
-        };
-        get(east, L.east);
-        get(south, L.south);
-        get(diag, L.diag);
-        get(wd, L.wd);
+        std::vector<float> rec(4 * n);
+        download(ctx, rec.data(), L.coef, 4 * n * sizeof(float));
+        float *outs[4] = {east, south, diag, wd};
+        for (int q = 0; q < 4; q++)
+            if (outs[q])
+                for (size_t c = 0; c < n; c++) outs[q][c] = rec[4 * c + q];
         if (mass) {
             if (L.mass)
-                get(mass, L.mass);
+                download(ctx, mass, L.mass, n * sizeof(float));
             else
                 for (size_t c = 0; c < n; c++) mass[c] = 1.0f;
         }
-        HIPCHK(hipStreamSynchronize(ctx->stream));
     });
 }
 
@@ -1066,27 +986,25 @@ int dtd_cycle_precondition(dtd_ctx *ctx, const float *r32, const double *r64, fl
         Workspace &ws = ctx->ws;
         const size_t n = (size_t)ws.n;
         hipStream_t s = ctx->stream;
-        upload(ctx, ws.r, r64, n);
+        upload(ctx, ws.r, r64, n * sizeof(double));
         int nbz;
         if (ws.prec == DTD_CYCLE_F64) {
             // An f64 cycle reads the f64 residual; r32 is its value rounded, kept for the ABI.
             nbz = vcycle<double>(ws.ld, 0, (int)ctx->opts.nu, ws.r, ws.parts, nullptr, s);
         } else {
-            HIPCHK(hipMemcpyAsync(ws.lf[0].r, r32, n * 4, hipMemcpyHostToDevice, s));
+            upload(ctx, ws.lf[0].r, r32, n * sizeof(float));
             nbz = vcycle<float>(ws.lf, 0, (int)ctx->opts.nu, ws.r, ws.parts, nullptr, s);
         }
         k_sum1<<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.pair);
         LAUNCH_CHECK();
         if (ws.prec == DTD_CYCLE_F64) {
             std::vector<double> tmp(n);
-            HIPCHK(hipMemcpyAsync(tmp.data(), ws.ld[0].z[1], n * 8, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
+            download(ctx, tmp.data(), ws.ld[0].z[1], n * sizeof(double));
             for (size_t c = 0; c < n; c++) z32[c] = (float)tmp[c];
         } else {
-            HIPCHK(hipMemcpyAsync(z32, ws.lf[0].z[1], n * 4, hipMemcpyDeviceToHost, s));
+            download(ctx, z32, ws.lf[0].z[1], n * sizeof(float));
         }
-        HIPCHK(hipMemcpyAsync(ctx->host_pair, ws.pair, sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        download(ctx, ctx->host_pair, ws.pair, sizeof(double));
         if (rz) *rz = ctx->host_pair[0];
     });
 }

@@ -30,6 +30,8 @@ struct CgState {
     double rr0;     // r.r of the first residual
     double lost;    // put-back: sum (b - A x)
     double weight;  // put-back: sum (diag - 1)
+    double bb;      // b.b of the right-hand side
+    double rel2;    // > 0: a relative target, tol2 = rel2 * b.b (set by the first residual)
     int have_beta;  // an iteration has formed beta (the first direction is z itself)
     int have_pending;  // an update has set alpha
     int done;       // the loop has stopped (1 converged, 2 p.A p, 3 r.z, 4 cap)
@@ -94,14 +96,15 @@ __device__ inline double face(double ka, double kb, double dt, double s2) {
     return 0.5 * (ka + kb) * dt / s2;
 }
 
-// `Operator::new` (D:532-603) and `Level::finest` (MG:96-114) in one pass: the f64 east and
-// south couplings, and the finest f32 level's east, south, diag and omega / diag.  The diagonal
-// is ((((1 + e) + w) + s) + n) with the west and north faces formed by the same expression as
-// the neighbour's own east and south faces.
+// `Operator::new` (D:532-603) with `Level::finest` (MG:96-114) in one pass.  The f64 couplings
+// go out packed, es[c] = (east, south); the diagonal ((((1 + e) + w) + s) + n) takes the west and
+// north faces from the same expression as the neighbours' own east and south, and is never
+// stored (it is re-formed the same way wherever it is used).  The finest level of the cycle goes
+// out as one record a cell, (east, south, diag, omega / diag): in f32 (coef32, the reference's
+// cycle) or in f64 (coef64, the f64 cycle).
 __global__ void k_operator(Grid g, const double *__restrict__ kd, double dt, double s2,
-                           double *__restrict__ east, double *__restrict__ south,
-                           float *__restrict__ e32, float *__restrict__ s32,
-                           float *__restrict__ d32, float *__restrict__ wd32, float omega) {
+                           double2 *__restrict__ es, float *__restrict__ coef32,
+                           double *__restrict__ coef64, float omega) {
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= g.n) return;
     const int j = c / g.w, i = c - j * g.w, c0 = j * g.w;
@@ -123,37 +126,37 @@ __global__ void k_operator(Grid g, const double *__restrict__ kd, double dt, dou
         wv = 0.0;
     const double nv = j > 0 ? face(kd[c - g.w], k, dt, s2) : 0.0;
     const double d = 1.0 + e + wv + s + nv;
-    east[c] = e;
-    south[c] = s;
-    if (e32) {
-        e32[c] = (float)e;
-        s32[c] = (float)s;
+    es[c] = make_double2(e, s);
+    if (coef32) {
         const float df = (float)d;
-        d32[c] = df;
-        wd32[c] = omega / df;
+        *reinterpret_cast<float4 *>(coef32 + 4 * (size_t)c) =
+            make_float4((float)e, (float)s, df, omega / df);
+    }
+    if (coef64) {
+        double2 *q = reinterpret_cast<double2 *>(coef64 + 4 * (size_t)c);
+        q[0] = make_double2(e, s);
+        q[1] = make_double2(d, (double)omega / d);
     }
 }
 
-// The f64 diagonal of cell c from the stored faces (the same sum Operator::new takes).
-__device__ inline double diag64(const Grid &g, const double *east, const double *south, int c,
-                                int i, int j) {
+// The f64 diagonal of cell c from the packed faces (the sum Operator::new takes).
+__device__ inline double diag_es(const Grid &g, const double2 *es, int c, int i, int j) {
     const int c0 = j * g.w;
-    const double wv = i > 0 ? east[c - 1] : (g.wrap ? east[c0 + g.w - 1] : 0.0);
-    const double nv = j > 0 ? south[c - g.w] : 0.0;
-    return 1.0 + east[c] + wv + south[c] + nv;
+    const double2 me = es[c];
+    const double wv = i > 0 ? es[c - 1].x : (g.wrap ? es[c0 + g.w - 1].x : 0.0);
+    const double nv = j > 0 ? es[c - g.w].y : 0.0;
+    return 1.0 + me.x + wv + me.y + nv;
 }
 
 // One cell of y = A x in the order of `Operator::apply_row` (D:609-649): diagonal, east (the
-// wrapped one for the last cell), west (the wrapped one for the first), south, north.  `X` is a
-// functor giving x at a cell, so a direction can be formed where it is used.
+// wrapped one for the last cell), west (the wrapped one for the first), south, north.
 template <class X>
-__device__ inline double apply64(const Grid &g, const double *east, const double *south, int c,
-                                 int i, int j, X x) {
+__device__ inline double apply_es(const Grid &g, const double2 *es, int c, int i, int j, X x) {
     const int c0 = j * g.w;
-    const double e = east[c];
-    const double wv = i > 0 ? east[c - 1] : (g.wrap ? east[c0 + g.w - 1] : 0.0);
-    const double s = south[c];
-    const double nv = j > 0 ? south[c - g.w] : 0.0;
+    const double2 me = es[c];
+    const double e = me.x, s = me.y;
+    const double wv = i > 0 ? es[c - 1].x : (g.wrap ? es[c0 + g.w - 1].x : 0.0);
+    const double nv = j > 0 ? es[c - g.w].y : 0.0;
     const double d = 1.0 + e + wv + s + nv;
     double o = d * x(c);
     if (i + 1 < g.w)
@@ -170,71 +173,27 @@ __device__ inline double apply64(const Grid &g, const double *east, const double
 }
 
 // y = A x (`Operator::apply`, D:652-663).
-__global__ void k_apply(Grid g, const double *__restrict__ east, const double *__restrict__ south,
-                        const double *__restrict__ x, double *__restrict__ y) {
+__global__ void k_apply(Grid g, const double2 *__restrict__ es, const double *__restrict__ x,
+                        double *__restrict__ y) {
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= g.n) return;
     const int j = c / g.w, i = c - j * g.w;
-    y[c] = apply64(g, east, south, c, i, j, [&](int k) { return x[k]; });
+    y[c] = apply_es(g, es, c, i, j, [&](int k) { return x[k]; });
 }
 
-// r = b - A x, its copy in the cycle's precision, and per block sum r*r (MG:596-619).
-template <typename T>
-__global__ void k_resid_init(Grid g, const double *__restrict__ east,
-                             const double *__restrict__ south, const double *__restrict__ b,
-                             const double *__restrict__ x, double *__restrict__ r,
-                             T *__restrict__ rc, double *__restrict__ parts) {
-    __shared__ double lds[kBlock / 64];
-    double acc = 0.0;
-#pragma unroll
-    for (int q = 0; q < kCgPer; q++) {
-        const int c = blockIdx.x * kCgCells + q * kBlock + threadIdx.x;
-        if (c < g.n) {
-            const int j = c / g.w, i = c - j * g.w;
-            const double rv =
-                b[c] - apply64(g, east, south, c, i, j, [&](int k) { return x[k]; });
-            r[c] = rv;
-            if (rc) rc[c] = (T)rv;
-            acc += rv * rv;
-        }
-    }
-    const double s = block_sum<kBlock>(acc, lds);
-    if (threadIdx.x == 0) parts[blockIdx.x] = s;
+// east, south and diag of the f64 operator, unpacked (for dtd_operator).
+__global__ void k_unpack_es(Grid g, const double2 *__restrict__ es, double *__restrict__ east,
+                            double *__restrict__ south, double *__restrict__ diag) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= g.n) return;
+    const int j = c / g.w, i = c - j * g.w;
+    east[c] = es[c].x;
+    south[c] = es[c].y;
+    diag[c] = diag_es(g, es, c, i, j);
 }
 
-// First pass of an iteration (MG:633-695): the pending x += alpha p_old, the direction
-// p = z + beta p_old (z itself on the first iteration) formed at the cell and its neighbours,
-// A p, and per block sum p * A p.
-template <typename T>
-__global__ void k_direction(Grid g, const double *__restrict__ east,
-                            const double *__restrict__ south, const T *__restrict__ z,
-                            const double *__restrict__ pold, double *__restrict__ pnew,
-                            double *__restrict__ ap, double *__restrict__ x,
-                            double *__restrict__ parts, const CgState *__restrict__ st) {
-    __shared__ double lds[kBlock / 64];
-    if (st->done) return;
-    const int have_beta = st->have_beta, pending = st->have_pending;
-    const double beta = st->beta, alpha = st->alpha;
-    auto p = [&](int k) { return have_beta ? (double)z[k] + beta * pold[k] : (double)z[k]; };
-    double acc = 0.0;
-#pragma unroll
-    for (int q = 0; q < kCgPer; q++) {
-        const int c = blockIdx.x * kCgCells + q * kBlock + threadIdx.x;
-        if (c < g.n) {
-            const int j = c / g.w, i = c - j * g.w;
-            if (pending) x[c] += alpha * pold[c];
-            const double pc = p(c);
-            const double a = apply64(g, east, south, c, i, j, p);
-            pnew[c] = pc;
-            ap[c] = a;
-            acc += pc * a;
-        }
-    }
-    const double s = block_sum<kBlock>(acc, lds);
-    if (threadIdx.x == 0) parts[blockIdx.x] = s;
-}
-
-// Second pass (MG:703-723): r -= alpha A p, its copy in the cycle's precision, per block r*r.
+// Second pass of an iteration (MG:703-723): r -= alpha A p, its copy in the cycle's precision,
+// per block r*r.
 template <typename T>
 __global__ void k_update(int n, const double *__restrict__ ap, double *__restrict__ r,
                          T *__restrict__ rc, double *__restrict__ parts,
@@ -263,12 +222,18 @@ enum Step { kStepRR0 = 0, kStepPAP = 1, kStepRR = 2, kStepRZ = 3, kStepRZ0 = 4 }
 template <int STEP>
 __global__ void k_cg_step(const double *__restrict__ parts, int nparts, CgState *st) {
     __shared__ double lds[kReduceBlock / 64];
-    if (STEP != kStepRR0 && STEP != kStepRZ0 && st->done) return;
-    if (STEP == kStepRZ0 && st->done) return;
-    const double v = reduce_parts(parts, nparts, 1, 0, lds);
+    if (STEP != kStepRR0 && st->done) return;
+    // The first residual's pass leaves (r.r, b.b) per block; the others one sum.
+    const int stride = STEP == kStepRR0 ? 2 : 1;
+    const double v = reduce_parts(parts, nparts, stride, 0, lds);
+    double bb = 0.0;
+    if (STEP == kStepRR0) bb = reduce_parts(parts, nparts, 2, 1, lds);
     if (threadIdx.x != 0) return;
     if (STEP == kStepRR0) {
-        // Stop before the loop when the first guess already meets the tolerance (MG:624-626).
+        // A relative target sets tol2 = rel^2 b.b here; then stop before the loop when the
+        // first guess already meets the tolerance (MG:624-626).
+        st->bb = bb;
+        if (st->rel2 > 0.0) st->tol2 = st->rel2 * bb;
         st->rr0 = v;
         st->rr = v;
         st->have_beta = 0;
@@ -304,36 +269,7 @@ __global__ void k_cg_step(const double *__restrict__ parts, int nparts, CgState 
     }
 }
 
-// After the loop: x += alpha p for the last update (MG:734-736).  p is the direction of the
-// last iteration, in buffer its % 2 (iteration k writes buffer k % 2... see the host's ping-pong).
-__global__ void k_final_x(int n, const double *__restrict__ p0, const double *__restrict__ p1,
-                          double *__restrict__ x, const CgState *__restrict__ st) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= n || !st->have_pending) return;
-    const double *p = (st->its & 1) ? p0 : p1;
-    x[c] += st->alpha * p[c];
-}
-
-// Put-back sums (MG:737-755): per block sum (b - A x) and sum (diag - 1).
-__global__ void k_putback_sums(Grid g, const double *__restrict__ east,
-                               const double *__restrict__ south, const double *__restrict__ b,
-                               const double *__restrict__ x, double *__restrict__ parts) {
-    __shared__ double lds[kBlock / 64];
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    double l = 0.0, wg = 0.0;
-    if (c < g.n) {
-        const int j = c / g.w, i = c - j * g.w;
-        l = b[c] - apply64(g, east, south, c, i, j, [&](int k) { return x[k]; });
-        wg = diag64(g, east, south, c, i, j) - 1.0;
-    }
-    const double sl = block_sum<kBlock>(l, lds);
-    const double sw = block_sum<kBlock>(wg, lds);
-    if (threadIdx.x == 0) {
-        parts[2 * blockIdx.x] = sl;
-        parts[2 * blockIdx.x + 1] = sw;
-    }
-}
-
+// The put-back's two sums (MG:737-755), then its step k = lost / weight.
 __global__ void k_putback_step(const double *__restrict__ parts, int nparts, CgState *st) {
     __shared__ double lds[kReduceBlock / 64];
     const double l = reduce_parts(parts, nparts, 2, 0, lds);
@@ -344,40 +280,7 @@ __global__ void k_putback_step(const double *__restrict__ parts, int nparts, CgS
     }
 }
 
-// x += (lost / weight) (diag - 1) when weight > 0 and lost != 0 (MG:756-761).
-__global__ void k_putback_apply(Grid g, const double *__restrict__ east,
-                                const double *__restrict__ south, double *__restrict__ x,
-                                const CgState *__restrict__ st) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= g.n) return;
-    const double lost = st->lost, weight = st->weight;
-    if (!(weight > 0.0 && lost != 0.0)) return;
-    const double k = lost / weight;
-    const int j = c / g.w, i = c - j * g.w;
-    x[c] += k * (diag64(g, east, south, c, i, j) - 1.0);
-}
-
-// Per block sum (b - A x)^2 and b^2: the true relative residual.
-__global__ void k_true_resid(Grid g, const double *__restrict__ east,
-                             const double *__restrict__ south, const double *__restrict__ b,
-                             const double *__restrict__ x, double *__restrict__ parts) {
-    __shared__ double lds[kBlock / 64];
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    double rr = 0.0, bb = 0.0;
-    if (c < g.n) {
-        const int j = c / g.w, i = c - j * g.w;
-        const double rv = b[c] - apply64(g, east, south, c, i, j, [&](int k) { return x[k]; });
-        rr = rv * rv;
-        bb = b[c] * b[c];
-    }
-    const double s1 = block_sum<kBlock>(rr, lds);
-    const double s2 = block_sum<kBlock>(bb, lds);
-    if (threadIdx.x == 0) {
-        parts[2 * blockIdx.x] = s1;
-        parts[2 * blockIdx.x + 1] = s2;
-    }
-}
-
+// (sum of slot 0, sum of slot 1) of pairs of partial sums.
 __global__ void k_pair_sum(const double *__restrict__ parts, int nparts, double *out) {
     __shared__ double lds[kReduceBlock / 64];
     const double a = reduce_parts(parts, nparts, 2, 0, lds);
@@ -393,59 +296,85 @@ __global__ void k_pair_sum(const double *__restrict__ parts, int nparts, double 
 template <typename T>
 struct Lvl {
     int w, h, wrap, n;
-    T *east, *south, *diag, *wd, *mass;  // mass: nullptr on the finest level (1 a cell)
+    T *coef;  // (east, south, diag, omega / diag) of each cell side by side
+    T *mass;  // the identity's share of the diagonal; nullptr on the finest level (1 a cell)
     T *r, *z[2];
-    T *coef;  // east, south, diag, wd of each cell side by side (4 values a cell)
 };
 
-// Packs a level's static operator into one record a cell for the streaming kernels.
-template <typename T>
-__global__ void k_pack(Lvl<T> L) {
+template <typename T> __device__ inline T c_east(const Lvl<T> &L, int c) { return L.coef[4 * (size_t)c]; }
+template <typename T> __device__ inline T c_south(const Lvl<T> &L, int c) { return L.coef[4 * (size_t)c + 1]; }
+template <typename T> __device__ inline T c_diag(const Lvl<T> &L, int c) { return L.coef[4 * (size_t)c + 2]; }
+template <typename T> __device__ inline T c_wd(const Lvl<T> &L, int c) { return L.coef[4 * (size_t)c + 3]; }
+
+// `Level::finest` (MG:96-114) from a given f64 operator: the f32 record and omega / diag.
+__global__ void k_finest32(int n, const double *__restrict__ east,
+                           const double *__restrict__ south, const double *__restrict__ diag,
+                           float *__restrict__ coef, float omega) {
     const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= L.n) return;
-    T *q = L.coef + 4 * (size_t)c;
-    q[0] = L.east[c], q[1] = L.south[c], q[2] = L.diag[c], q[3] = L.wd[c];
+    if (c >= n) return;
+    const float d = (float)diag[c];
+    *reinterpret_cast<float4 *>(coef + 4 * (size_t)c) =
+        make_float4((float)east[c], (float)south[c], d, omega / d);
 }
 
-// `Level::coarsen` (MG:117-188): one thread per coarse cell.
+// The f64 cycle's finest level from a given f64 operator.
+__global__ void k_finest64(int n, const double *__restrict__ east,
+                           const double *__restrict__ south, const double *__restrict__ diag,
+                           double *__restrict__ coef, double omega) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    double2 *q = reinterpret_cast<double2 *>(coef + 4 * (size_t)c);
+    q[0] = make_double2(east[c], south[c]);
+    q[1] = make_double2(diag[c], omega / diag[c]);
+}
+
+// `Level::coarsen` (MG:117-199) for coarse cell (ic, jc): the mass of the 2x2 block (rows then
+// columns), the coarse east face = scale * the fine east faces leaving the block's last column
+// (rows in order; none on a ring of one block), the coarse south face likewise over the block's
+// last row's columns.
 template <typename T>
-__global__ void k_coarsen(Lvl<T> f, Lvl<T> c, T scale) {
+__device__ inline T coarse_mass(const Lvl<T> &f, int ic, int jc) {
+    const int r1 = min(2 * jc + 2, f.h), c1 = min(2 * ic + 2, f.w);
+    T m = (T)0;
+    for (int j = 2 * jc; j < r1; j++)
+        for (int i = 2 * ic; i < c1; i++) m += f.mass ? f.mass[j * f.w + i] : (T)1;
+    return m;
+}
+template <typename T>
+__device__ inline T coarse_east(const Lvl<T> &f, int wc, int cwrap, T scale, int ic, int jc) {
+    if (!(ic + 1 < wc || cwrap)) return (T)0;
+    const int r1 = min(2 * jc + 2, f.h), last_col = min(2 * ic + 2, f.w) - 1;
+    T e = (T)0;
+    for (int j = 2 * jc; j < r1; j++) e += c_east(f, j * f.w + last_col);
+    return scale * e;
+}
+template <typename T>
+__device__ inline T coarse_south(const Lvl<T> &f, int hc, T scale, int ic, int jc) {
+    if (!(jc + 1 < hc)) return (T)0;
+    const int last_row = min(2 * jc + 2, f.h) - 1, c1 = min(2 * ic + 2, f.w);
+    T s = (T)0;
+    for (int i = 2 * ic; i < c1; i++) s += c_south(f, last_row * f.w + i);
+    return scale * s;
+}
+
+// One coarse level from the finer one: mass, faces, and the diagonal mass + e + w + s + n with
+// omega / diag (MG:189-198); the west and north faces are formed by the same sums as the
+// neighbours' own, so every value equals the reference's.
+template <typename T>
+__global__ void k_coarsen(Lvl<T> f, Lvl<T> c, T scale, T omega) {
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= c.n) return;
     const int jc = k / c.w, ic = k - jc * c.w;
-    const int r0 = 2 * jc, r1 = min(2 * jc + 2, f.h), c0 = 2 * ic, c1 = min(2 * ic + 2, f.w);
-    const int last_row = r1 - 1, last_col = c1 - 1;
-    T m = (T)0;
-    for (int j = r0; j < r1; j++)
-        for (int i = c0; i < c1; i++) m += f.mass ? f.mass[j * f.w + i] : (T)1;
+    const T m = coarse_mass(f, ic, jc);
+    const T e = coarse_east(f, c.w, c.wrap, scale, ic, jc);
+    const T s = coarse_south(f, c.h, scale, ic, jc);
+    const T west = ic > 0 ? coarse_east(f, c.w, c.wrap, scale, ic - 1, jc)
+                          : (c.wrap ? coarse_east(f, c.w, c.wrap, scale, c.w - 1, jc) : (T)0);
+    const T north = jc > 0 ? coarse_south(f, c.h, scale, ic, jc - 1) : (T)0;
+    const T d = m + e + west + s + north;
     c.mass[k] = m;
-    T e = (T)0;
-    if (ic + 1 < c.w || c.wrap) {
-        T s = (T)0;
-        for (int j = r0; j < r1; j++) s += f.east[j * f.w + last_col];
-        e = scale * s;
-    }
-    c.east[k] = e;
-    T so = (T)0;
-    if (jc + 1 < c.h) {
-        T s = (T)0;
-        for (int i = c0; i < c1; i++) s += f.south[last_row * f.w + i];
-        so = scale * s;
-    }
-    c.south[k] = so;
-}
-
-// The coarse diagonal mass + e + w + s + n and omega / diag (MG:189-198).
-template <typename T>
-__global__ void k_cdiag(Lvl<T> c, T omega) {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= c.n) return;
-    const int j = k / c.w, i = k - j * c.w, c0 = j * c.w;
-    const T west = i > 0 ? c.east[k - 1] : (c.wrap ? c.east[c0 + c.w - 1] : (T)0);
-    const T north = j > 0 ? c.south[k - c.w] : (T)0;
-    const T d = c.mass[k] + c.east[k] + west + c.south[k] + north;
-    c.diag[k] = d;
-    c.wd[k] = omega / d;
+    T *q = c.coef + 4 * (size_t)k;
+    q[0] = e, q[1] = s, q[2] = d, q[3] = omega / d;
 }
 
 // One cell of A x on a level in the order of `Level::apply_row` (MG:203-240): diagonal, east
@@ -454,15 +383,15 @@ __global__ void k_cdiag(Lvl<T> c, T omega) {
 template <typename T, class X>
 __device__ inline T lvl_apply(const Lvl<T> &L, int c, int i, int j, X x) {
     const int c0 = j * L.w;
-    T o = L.diag[c] * x(c);
-    if (i + 1 < L.w) o -= L.east[c] * x(c + 1);
-    if (i > 0) o -= L.east[c - 1] * x(c - 1);
+    T o = c_diag(L, c) * x(c);
+    if (i + 1 < L.w) o -= c_east(L, c) * x(c + 1);
+    if (i > 0) o -= c_east(L, c - 1) * x(c - 1);
     if (L.wrap) {
-        if (i == L.w - 1) o -= L.east[c0 + L.w - 1] * x(c0);
-        if (i == 0) o -= L.east[c0 + L.w - 1] * x(c0 + L.w - 1);
+        if (i == L.w - 1) o -= c_east(L, c0 + L.w - 1) * x(c0);
+        if (i == 0) o -= c_east(L, c0 + L.w - 1) * x(c0 + L.w - 1);
     }
-    if (j + 1 < L.h) o -= L.south[c] * x(c + L.w);
-    if (j > 0) o -= L.south[c - L.w] * x(c - L.w);
+    if (j + 1 < L.h) o -= c_south(L, c) * x(c + L.w);
+    if (j > 0) o -= c_south(L, c - L.w) * x(c - L.w);
     return o;
 }
 
@@ -472,7 +401,7 @@ __global__ void k_first(Lvl<T> L, T *__restrict__ zout, const CgState *__restric
     if (st && st->done) return;
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= L.n) return;
-    zout[c] = L.wd[c] * L.r[c];
+    zout[c] = c_wd(L, c) * L.r[c];
 }
 
 // `Level::sweep` (MG:260-329): zout = z' + wd (r - A z') with z' = zin + zc[j/2][i/2] when zc is
@@ -493,7 +422,7 @@ __global__ void k_sweep(Lvl<T> L, const T *__restrict__ zin, T *__restrict__ zou
             return zin[k] + zc[(jj >> 1) * wc + (ii >> 1)];
         };
         const T ax = lvl_apply(L, c, i, j, zp);
-        const T v = zp(c) + L.wd[c] * (L.r[c] - ax);
+        const T v = zp(c) + c_wd(L, c) * (L.r[c] - ax);
         zout[c] = v;
         if (dot) acc = dot[c] * (double)v;
     }
@@ -537,7 +466,7 @@ __global__ void k_coarsest(Lvl<T> L, T *__restrict__ zout, int sweeps,
     if (live) {
         j = c / L.w;
         i = c - j * L.w;
-        zs[0][c] = L.wd[c] * L.r[c];
+        zs[0][c] = c_wd(L, c) * L.r[c];
     }
     __syncthreads();
     int cur = 0;
@@ -545,7 +474,7 @@ __global__ void k_coarsest(Lvl<T> L, T *__restrict__ zout, int sweeps,
         if (live) {
             const T *zz = zs[cur];
             const T ax = lvl_apply(L, c, i, j, [&](int q) { return zz[q]; });
-            zs[cur ^ 1][c] = zz[c] + L.wd[c] * (L.r[c] - ax);
+            zs[cur ^ 1][c] = zz[c] + c_wd(L, c) * (L.r[c] - ax);
         }
         __syncthreads();
         cur ^= 1;
@@ -925,13 +854,6 @@ __global__ __launch_bounds__(kBlock) void k_spost(Lvl<T> L, const T *__restrict_
 // owned cells; and A p of the row above with its p . A p.  The f64 couplings come packed as
 // (east, south) pairs.
 
-// (east, south) of each cell side by side, from the operator's arrays.
-__global__ void k_pack_es(int n, const double *__restrict__ east,
-                          const double *__restrict__ south, double2 *__restrict__ es) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c < n) es[c] = make_double2(east[c], south[c]);
-}
-
 template <typename TZ>
 __global__ __launch_bounds__(kBlock) void k_sdir(Grid g, const double2 *__restrict__ es,
                                                  const TZ *__restrict__ z,
@@ -1016,6 +938,232 @@ __global__ __launch_bounds__(kBlock) void k_sdir(Grid g, const double2 *__restri
 #undef DTD_FETCH_DIR
     const double sum = block_sum<kBlock>(acc, lds);
     if (threadIdx.x == 0) parts[blockIdx.x] = sum;
+}
+
+// ---- the other f64 passes, streamed the same way -------------------------------------------
+
+// A x at one cell of an f64 lane in the order of `Operator::apply_row` (D:609-649), from its own
+// and its neighbours' values; ew is the west neighbour's east face, sn the north neighbour's
+// south face.  `inner`: every term present, no wrap (most cells).
+__device__ inline double lane_apply64(double xc, double xe, double xw, double xs, double xn,
+                                      double e, double ew, double s, double sn, int gi, int gj,
+                                      int w, int h, int wrap, bool inner) {
+    if (inner) {
+        const double d = 1.0 + e + ew + s + sn;
+        double o = d * xc;
+        o -= e * xe;
+        o -= ew * xw;
+        o -= s * xs;
+        o -= sn * xn;
+        return o;
+    }
+    const double wv = (gi > 0 || wrap) ? ew : 0.0;
+    const double nv = gj > 0 ? sn : 0.0;
+    const double d = 1.0 + e + wv + s + nv;
+    double o = d * xc;
+    if (gi + 1 < w || wrap) o -= e * xe;
+    if (gi > 0 || wrap) o -= wv * xw;
+    if (gj + 1 < h) o -= s * xs;
+    if (gj > 0) o -= nv * xn;
+    return o;
+}
+
+// The diagonal of one cell of an f64 lane (the sum Operator::new takes).
+__device__ inline double lane_diag64(double e, double ew, double s, double sn, int gi, int gj,
+                                     int wrap) {
+    const double wv = (gi > 0 || wrap) ? ew : 0.0;
+    const double nv = gj > 0 ? sn : 0.0;
+    return 1.0 + e + wv + s + nv;
+}
+
+// The first residual (MG:596-619): r = b - A x, its copy in the cycle's precision, per block
+// (r.r, b.b).  x is read from xin; when xout differs (the first guess is b itself) the owned
+// cells of x are written there.
+template <typename TR>
+__global__ __launch_bounds__(kBlock) void k_sresid(Grid g, const double2 *__restrict__ es,
+                                                   const double *__restrict__ b,
+                                                   const double *__restrict__ xin,
+                                                   double *__restrict__ xout,
+                                                   double *__restrict__ r, TR *__restrict__ rc,
+                                                   int th, double *__restrict__ parts) {
+    constexpr int A = 4;
+    __shared__ double lds[kBlock / 64];
+    const int w = g.w, h = g.h, wrap = g.wrap;
+    const Band bd = band_of<1, 1>(w, h, wrap, th);
+    const int gi = bd.gi0;
+    const bool inner_col = bd.col_ok && gi >= 1 && gi + 1 <= w - 1;
+    const bool copy = xout != xin;
+    double qx[A], qe[A], qs[A], qb[A];
+#define DTD_FETCH_RES(j, u)                                                                   \
+    do {                                                                                      \
+        const int j_ = (j);                                                                   \
+        if (j_ >= 0 && j_ < h && bd.col_ok && j_ <= bd.row0 + bd.rows) {                      \
+            const size_t g_ = (size_t)j_ * w + gi;                                            \
+            const double2 c_ = es[g_];                                                        \
+            qe[u] = c_.x, qs[u] = c_.y, qx[u] = xin[g_], qb[u] = b[g_];                       \
+        } else {                                                                              \
+            qe[u] = qs[u] = qx[u] = qb[u] = 0.0;                                              \
+        }                                                                                     \
+    } while (0)
+    double xa = 0, xb = 0, xc = 0, ea = 0, eb = 0, sa = 0, sb = 0, sc = 0, ba = 0, bb = 0;
+    double acc_r = 0.0, acc_b = 0.0;
+    const int steps = bd.active ? bd.rows + 4 : 0;
+#pragma unroll
+    for (int u = 0; u < A; u++) DTD_FETCH_RES(bd.row0 - 1 + u, u);
+    for (int k0 = 0; k0 < steps; k0 += A) {
+#pragma unroll
+        for (int u = 0; u < A; u++) {
+            const int ja = bd.row0 - 1 + k0 + u, jb = ja - 1;
+            xc = xb, xb = xa, eb = ea, sc = sb, sb = sa, bb = ba;
+            xa = qx[u], ea = qe[u], sa = qs[u], ba = qb[u];
+            if (k0 + u + A < steps) DTD_FETCH_RES(ja + A, u);
+            const bool rows_a = ja >= bd.row0 && ja < bd.row0 + bd.rows && ja < h;
+            if (copy && bd.own && rows_a) xout[(size_t)ja * w + gi] = xa;
+            const double xe = from_east(xb), xw = from_west(xb), ew = from_west(eb);
+            if (bd.own && jb >= bd.row0 && jb < bd.row0 + bd.rows && jb < h) {
+                const double o = lane_apply64(xb, xe, xw, xa, xc, eb, ew, sb, sc, gi, jb, w, h,
+                                              wrap, inner_col && jb > 0 && jb + 1 < h);
+                const double rv = bb - o;
+                const size_t gb = (size_t)jb * w + gi;
+                r[gb] = rv;
+                if (rc) rc[gb] = (TR)rv;
+                acc_r += rv * rv;
+                acc_b += bb * bb;
+            }
+        }
+    }
+#undef DTD_FETCH_RES
+    const double s1 = block_sum<kBlock>(acc_r, lds);
+    const double s2 = block_sum<kBlock>(acc_b, lds);
+    if (threadIdx.x == 0) parts[2 * blockIdx.x] = s1, parts[2 * blockIdx.x + 1] = s2;
+}
+
+// After the loop: the last update x' = x + alpha p (MG:734-736; p of the last iteration, in
+// buffer its % 2 ... p0 for odd its), written to xout, and the put-back's sums over x'
+// (MG:737-755): per block (sum (b - A x'), sum (diag - 1)).
+__global__ __launch_bounds__(kBlock) void k_finish1(Grid g, const double2 *__restrict__ es,
+                                                    const double *__restrict__ b,
+                                                    const double *__restrict__ x,
+                                                    const double *__restrict__ p0,
+                                                    const double *__restrict__ p1,
+                                                    double *__restrict__ xout, int th,
+                                                    double *__restrict__ parts,
+                                                    const CgState *__restrict__ st) {
+    constexpr int A = 4;
+    __shared__ double lds[kBlock / 64];
+    const int pending = st->have_pending;
+    const double alpha = st->alpha;
+    const double *p = (st->its & 1) ? p0 : p1;
+    const int w = g.w, h = g.h, wrap = g.wrap;
+    const Band bd = band_of<1, 1>(w, h, wrap, th);
+    const int gi = bd.gi0;
+    const bool inner_col = bd.col_ok && gi >= 1 && gi + 1 <= w - 1;
+    double qx[A], qp[A], qe[A], qs[A], qb[A];
+#define DTD_FETCH_FIN(j, u)                                                                   \
+    do {                                                                                      \
+        const int j_ = (j);                                                                   \
+        if (j_ >= 0 && j_ < h && bd.col_ok && j_ <= bd.row0 + bd.rows) {                      \
+            const size_t g_ = (size_t)j_ * w + gi;                                            \
+            const double2 c_ = es[g_];                                                        \
+            qe[u] = c_.x, qs[u] = c_.y, qx[u] = x[g_], qb[u] = b[g_];                         \
+            qp[u] = pending ? p[g_] : 0.0;                                                    \
+        } else {                                                                              \
+            qe[u] = qs[u] = qx[u] = qb[u] = qp[u] = 0.0;                                      \
+        }                                                                                     \
+    } while (0)
+    double xa = 0, xb = 0, xc = 0, ea = 0, eb = 0, sa = 0, sb = 0, sc = 0, ba = 0, bb = 0;
+    double acc_l = 0.0, acc_w = 0.0;
+    const int steps = bd.active ? bd.rows + 4 : 0;
+#pragma unroll
+    for (int u = 0; u < A; u++) DTD_FETCH_FIN(bd.row0 - 1 + u, u);
+    for (int k0 = 0; k0 < steps; k0 += A) {
+#pragma unroll
+        for (int u = 0; u < A; u++) {
+            const int ja = bd.row0 - 1 + k0 + u, jb = ja - 1;
+            xc = xb, xb = xa, eb = ea, sc = sb, sb = sa, bb = ba;
+            xa = pending ? qx[u] + alpha * qp[u] : qx[u];
+            ea = qe[u], sa = qs[u], ba = qb[u];
+            if (k0 + u + A < steps) DTD_FETCH_FIN(ja + A, u);
+            if (bd.own && ja >= bd.row0 && ja < bd.row0 + bd.rows && ja < h)
+                xout[(size_t)ja * w + gi] = xa;
+            const double xe = from_east(xb), xw = from_west(xb), ew = from_west(eb);
+            if (bd.own && jb >= bd.row0 && jb < bd.row0 + bd.rows && jb < h) {
+                const double o = lane_apply64(xb, xe, xw, xa, xc, eb, ew, sb, sc, gi, jb, w, h,
+                                              wrap, inner_col && jb > 0 && jb + 1 < h);
+                acc_l += bb - o;
+                acc_w += lane_diag64(eb, ew, sb, sc, gi, jb, wrap) - 1.0;
+            }
+        }
+    }
+#undef DTD_FETCH_FIN
+    const double s1 = block_sum<kBlock>(acc_l, lds);
+    const double s2 = block_sum<kBlock>(acc_w, lds);
+    if (threadIdx.x == 0) parts[2 * blockIdx.x] = s1, parts[2 * blockIdx.x + 1] = s2;
+}
+
+// The put-back x'' = x' + (lost / weight) (diag - 1) when weight > 0 and lost != 0 (MG:756-761),
+// written to xout, and the true residual of x'': per block (sum (b - A x'')^2, sum b^2).
+__global__ __launch_bounds__(kBlock) void k_finish2(Grid g, const double2 *__restrict__ es,
+                                                    const double *__restrict__ b,
+                                                    const double *__restrict__ xp,
+                                                    double *__restrict__ xout, int th,
+                                                    double *__restrict__ parts,
+                                                    const CgState *__restrict__ st) {
+    constexpr int A = 4;
+    __shared__ double lds[kBlock / 64];
+    const double lost = st->lost, weight = st->weight;
+    const bool apply = weight > 0.0 && lost != 0.0;
+    const double kk = apply ? lost / weight : 0.0;
+    const int w = g.w, h = g.h, wrap = g.wrap;
+    const Band bd = band_of<1, 1>(w, h, wrap, th);
+    const int gi = bd.gi0;
+    const bool inner_col = bd.col_ok && gi >= 1 && gi + 1 <= w - 1;
+    double qx[A], qe[A], qs[A], qb[A];
+#define DTD_FETCH_FIN2(j, u)                                                                  \
+    do {                                                                                      \
+        const int j_ = (j);                                                                   \
+        if (j_ >= 0 && j_ < h && bd.col_ok && j_ <= bd.row0 + bd.rows) {                      \
+            const size_t g_ = (size_t)j_ * w + gi;                                            \
+            const double2 c_ = es[g_];                                                        \
+            qe[u] = c_.x, qs[u] = c_.y, qx[u] = xp[g_], qb[u] = b[g_];                        \
+        } else {                                                                              \
+            qe[u] = qs[u] = qx[u] = qb[u] = 0.0;                                              \
+        }                                                                                     \
+    } while (0)
+    // The window starts two rows above the band: the diagonal of row a needs row a - 1's south.
+    double xa = 0, xb = 0, xc = 0, ea = 0, eb = 0, sa = 0, sb = 0, sc = 0, ba = 0, bb = 0;
+    double acc_r = 0.0, acc_b = 0.0;
+    const int steps = bd.active ? bd.rows + 4 : 0;
+#pragma unroll
+    for (int u = 0; u < A; u++) DTD_FETCH_FIN2(bd.row0 - 2 + u, u);
+    for (int k0 = 0; k0 < steps; k0 += A) {
+#pragma unroll
+        for (int u = 0; u < A; u++) {
+            const int ja = bd.row0 - 2 + k0 + u, jb = ja - 1;
+            xc = xb, xb = xa, eb = ea, sc = sb, sb = sa, bb = ba;
+            ea = qe[u], sa = qs[u], ba = qb[u];
+            const double ewa = from_west(ea);
+            double v = qx[u];
+            if (apply && ja >= 0 && ja < h && bd.col_ok)
+                v = v + kk * (lane_diag64(ea, ewa, sa, sb, gi, ja, wrap) - 1.0);
+            xa = v;
+            if (k0 + u + A < steps) DTD_FETCH_FIN2(ja + A, u);
+            if (bd.own && ja >= bd.row0 && ja < bd.row0 + bd.rows && ja < h)
+                xout[(size_t)ja * w + gi] = xa;
+            const double xe = from_east(xb), xw = from_west(xb), ew = from_west(eb);
+            if (bd.own && jb >= bd.row0 && jb < bd.row0 + bd.rows && jb < h) {
+                const double o = lane_apply64(xb, xe, xw, xa, xc, eb, ew, sb, sc, gi, jb, w, h,
+                                              wrap, inner_col && jb > 0 && jb + 1 < h);
+                const double rv = bb - o;
+                acc_r += rv * rv;
+                acc_b += bb * bb;
+            }
+        }
+    }
+#undef DTD_FETCH_FIN2
+    const double s1 = block_sum<kBlock>(acc_r, lds);
+    const double s2 = block_sum<kBlock>(acc_b, lds);
+    if (threadIdx.x == 0) parts[2 * blockIdx.x] = s1, parts[2 * blockIdx.x + 1] = s2;
 }
 
 // ---- the first guess (D:370-489) -----------------------------------------------------------
@@ -1182,17 +1330,20 @@ __global__ void k_transpose(const double *__restrict__ src, double *__restrict__
     }
 }
 
-// One line per block (n / 16 threads, a power of two), in place on x; faces from kd along the
-// line, the last one wrapping when `ring`.
+// One line per block (n / 16 threads, a power of two), from src to dst (may be the same array);
+// faces from kd along the line, the last one wrapping when `ring`.
 template <int MAXT>
 __global__ __launch_bounds__(MAXT) void k_lod_lines(int n, int ring,
                                                     const double *__restrict__ kd,
-                                                    double *__restrict__ x, double dt, double s2) {
+                                                    const double *__restrict__ src,
+                                                    double *__restrict__ dst, double dt,
+                                                    double s2) {
     __shared__ double A[2 * MAXT], B[2 * MAXT], Cc[2 * MAXT], R[2 * MAXT];
     const int P = n / kSeg, N = 2 * P, t = threadIdx.x;
     const size_t base = (size_t)blockIdx.x * n;
     const double *k = kd + base;
-    double *xl = x + base;
+    const double *xs = src + base;
+    double *xl = dst + base;
     const int a = t * kSeg, b = a + kSeg - 1;
     // kd[a - 1 .. b + 1] (wrapped for a ring; a missing neighbour gives no face).
     double kk[kSeg + 2];
@@ -1213,7 +1364,7 @@ __global__ __launch_bounds__(MAXT) void k_lod_lines(int n, int ring,
     double d[kSeg];
 #pragma unroll
     for (int q = 0; q < kSeg; q += 2) {
-        const double2 v = *reinterpret_cast<const double2 *>(xl + a + q);
+        const double2 v = *reinterpret_cast<const double2 *>(xs + a + q);
         d[q] = v.x, d[q + 1] = v.y;
     }
     // Row q of the segment: lo = -f[q], up = -f[q + 1], di = 1 + f[q] + f[q + 1].
