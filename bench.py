@@ -93,6 +93,7 @@ def main():
         st = step()
         log(f"solve: {st.iterations} its, rel {st.rel_residual:.2e}, {st.ms_total:.2f} ms")
         for name, kid in (("copy_f32", dtdiff.KERNEL_COPY), ("stream5_f32", dtdiff.KERNEL_STREAM5),
+                          ("bandcopy_f32", dtdiff.KERNEL_BANDCOPY),
                           ("pre_smooth_f32", dtdiff.KERNEL_PRE),
                           ("post_smooth_f32", dtdiff.KERNEL_POST),
                           ("direction_f64", dtdiff.KERNEL_DIRECTION),

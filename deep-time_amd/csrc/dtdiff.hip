@@ -886,6 +886,12 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
                 launch_post<float, 2>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts, ws.st, s);
                 b = 36.0 * n + 4.0 * C.n;
                 break;
+            case DTD_KERNEL_BANDCOPY: {  // r, record in; z out, the pre-smoother's walk
+                const int th = band_height<2>((const void *)k_bandcopy<2>, L.w, L.h);
+                k_bandcopy<2><<<stream_grid<2, 1>(L.w, L.h, th), kBlock, 0, s>>>(L, L.z[0], th);
+                b = 24.0 * n;
+                break;
+            }
             case DTD_KERNEL_COPY:  // r32 in, z out
             case DTD_KERNEL_STREAM5: {  // five arrays of n floats in; one out
                 const long n4 = ws.n / 4;

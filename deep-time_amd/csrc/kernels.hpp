@@ -846,6 +846,45 @@ __global__ __launch_bounds__(kBlock) void k_spost(Lvl<T> L, const T *__restrict_
     }
 }
 
+// Calibration: the pre-smoothing kernel's band walk, loads and store without its stencil work
+// (z = r + east), to separate the access pattern's cost from the arithmetic's.
+template <int V>
+__global__ __launch_bounds__(kBlock) void k_bandcopy(Lvl<float> L, float *__restrict__ zout,
+                                                     int th) {
+    constexpr int A = 4;
+    const int w = L.w, h = L.h;
+    const Band bd = band_of<V>(w, h, L.wrap, th);
+    const int gi0 = bd.gi0;
+    float qr[A][V], qwd[A][V], qdg[A][V], qe[A][V], qs[A][V];
+    const int steps = bd.active ? bd.rows + 4 : 0;
+#define DTD_FETCH_BC(j, u)                                                                    \
+    do {                                                                                      \
+        const int j_ = (j);                                                                   \
+        if (j_ >= 0 && j_ < h && bd.col_ok) {                                                 \
+            const size_t g_ = (size_t)j_ * w + gi0;                                           \
+            loadv<float, V>(L.r + g_, qr[u]);                                                 \
+            load_coef<float, V>(L.coef, g_, qe[u], qs[u], qdg[u], qwd[u]);                    \
+        } else {                                                                              \
+            zero(qr[u]), zero(qwd[u]), zero(qdg[u]), zero(qe[u]), zero(qs[u]);                \
+        }                                                                                     \
+    } while (0)
+#pragma unroll
+    for (int u = 0; u < A; u++) DTD_FETCH_BC(bd.row0 - 2 + u, u);
+    for (int k0 = 0; k0 < steps; k0 += A) {
+#pragma unroll
+        for (int u = 0; u < A; u++) {
+            const int ja = bd.row0 - 2 + k0 + u;
+            float out[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) out[v] = qr[u][v] + qe[u][v] + qs[u][v] + qdg[u][v] + qwd[u][v];
+            if (k0 + u + A < steps) DTD_FETCH_BC(ja + A, u);
+            if (bd.own && ja >= bd.row0 && ja < bd.row0 + bd.rows && ja < h)
+                storev<float, V>(zout + (size_t)ja * w + gi0, out);
+        }
+    }
+#undef DTD_FETCH_BC
+}
+
 // ---- the conjugate gradients' direction pass, streamed (MG:633-695) ------------------------
 //
 // One f64 cell a lane, a band of 62 columns (one halo lane each side) walked row by row with

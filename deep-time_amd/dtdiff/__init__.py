@@ -42,7 +42,7 @@ SYMBOLS = (
     "dtd_time_kernel",
 )
 KERNEL_SWEEP, KERNEL_DIRECTION, KERNEL_UPDATE, KERNEL_RESTRICT = 0, 1, 2, 3
-KERNEL_PRE, KERNEL_POST, KERNEL_COPY, KERNEL_STREAM5 = 4, 5, 6, 7
+KERNEL_PRE, KERNEL_POST, KERNEL_COPY, KERNEL_STREAM5, KERNEL_BANDCOPY = 4, 5, 6, 7, 8
 
 
 class DiffusionSolver(enum.IntEnum):

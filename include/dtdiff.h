@@ -143,6 +143,7 @@ int dtd_cycle_precondition(dtd_ctx *ctx, const float *r32, const double *r64, fl
 #define DTD_KERNEL_POST 5       /* fused post-smoothing (two sweeps + dot) of the finest level */
 #define DTD_KERNEL_COPY 6       /* calibration: a float4 copy of one finest-level array */
 #define DTD_KERNEL_STREAM5 7    /* calibration: five finest-level arrays combined into one */
+#define DTD_KERNEL_BANDCOPY 8   /* calibration: the pre-smoother's band walk without its stencil */
 
 /* Measurement hook, no reference counterpart: launches `kernel` reps times back to back on the
  * context's stream over the resident data of the last solve and returns the average duration
