@@ -1,7 +1,7 @@
 """Benchmark: unknowns/s of the implicit diffusion solve to an FP64 relative residual of 1e-10.
 
-Workload (one step = one whole solve): 8192 x 4096 cells, periodic in x, closed pole rows,
-K log-normal (ln K ~ N(0,1) smoothed by a Gaussian of 8 cells, renormalised), dt mean(K)/dx^2 =
+Workload (one step = one whole solve; --workload picks another): 8192 x 4096 cells, periodic in
+x, closed pole rows, K log-normal (ln K ~ N(0,1) smoothed by a Gaussian of 8 cells, renormalised), dt mean(K)/dx^2 =
 50, h0 isotropic k^-2 spectral noise (mean 0, std 1000 m); seeded, synthetic.  Each step copies
 h0 into the working array and solves it on the device (operator and levels rebuilt, the
 splitting's first guess, f64 CG preconditioned by the f32 V-cycle, mass put-back, the true f64
@@ -49,7 +49,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--size", default="8192x4096")
+    ap.add_argument("--workload", default="lognormal-8192x4096",
+                    help="constant-512x256 | lognormal-8192x4096 | constant-16384x8192")
+    ap.add_argument("--size", default=None, help="WxH: the same workload at another size")
     ap.add_argument("--rel", type=float, default=1e-10)
     ap.add_argument("--cycle", choices=["f32", "f64"], default="f32")
     ap.add_argument("--guess", choices=["lod", "rhs"], default="lod")
@@ -68,10 +70,14 @@ def main():
     from dtdiff import fields as F
     from dtdiff.hipmem import DeviceArray, synchronize
 
-    w, h = (int(v) for v in args.size.split("x"))
+    gen, kw = F.WORKLOADS[args.workload]
+    kw = dict(kw)
+    if args.size:
+        kw["w"], kw["h"] = (int(v) for v in args.size.split("x"))
+    w, h = kw["w"], kw["h"]
     t = time.time()
-    d, h0, kd, dt = F.c2(w=w, h=h)
-    log(f"inputs {w}x{h} generated in {time.time() - t:.1f} s")
+    d, h0, kd, dt = gen(**kw)
+    log(f"inputs {args.workload} at {w}x{h} generated in {time.time() - t:.1f} s")
     n = d.cells()
 
     t = time.time()
@@ -176,8 +182,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 (CG, residual, update) + f32 (V-cycle)" if args.cycle == "f32" else "f64",
         "data": "synthetic",
-        "config": {"workload": f"diffusion solve {w}x{h}, log-normal K (sigma 8 cells), "
-                               f"dt*mean(K)/dx^2=50, k^-2 h0, to FP64 rel. residual {args.rel:g}",
+        "config": {"workload": f"diffusion solve {w}x{h}, " + (
+                       "log-normal K (sigma 8 cells), dt*mean(K)/dx^2=50"
+                       if gen is F.case_lognormal else "constant K, dt*K/dx^2=10")
+                   + f", k^-2 h0, to FP64 rel. residual {args.rel:g}",
                    "grid": [w, h], "cycle": args.cycle, "guess": args.guess},
         "roofline": roof,
         "cpu_baseline": cpu,

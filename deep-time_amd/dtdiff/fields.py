@@ -79,19 +79,23 @@ def lognormal_k(w: int, h: int, seed: int, sigma: float = 8.0) -> np.ndarray:
     return k.reshape(-1)
 
 
-def c1(seed: int = 1, w: int = 512, h: int = 256):
-    """Constant K = 1, dt K / dx^2 = 10, k^-2 h0 (512 x 256 by default)."""
+def case_constant(seed: int = 1, w: int = 512, h: int = 256):
+    """Constant K = 1, dt K / dx^2 = 10, k^-2 h0 (512 x 256 by default; the large case is the same
+    at 16384 x 8192, seed 3)."""
     d = Dims(w, h, 1.0, True)
     return d, spectral_noise(w, h, seed), np.ones(w * h), 10.0
 
 
-def c2(seed: int = 2, w: int = 8192, h: int = 4096, ratio: float = 50.0):
+def case_lognormal(seed: int = 2, w: int = 8192, h: int = 4096, ratio: float = 50.0):
     """Log-normal K (sigma 8 cells), dt mean(K) / dx^2 = 50, k^-2 h0 (8192 x 4096 by default)."""
     d = Dims(w, h, 1.0, True)
     kd = lognormal_k(w, h, seed)
     return d, spectral_noise(w, h, seed), kd, ratio / float(kd.mean())
 
 
-def c3(seed: int = 3, w: int = 16384, h: int = 8192):
-    """The C1 distributions at 16384 x 8192."""
-    return c1(seed, w, h)
+# The benchmark's workloads by name: generator and its arguments.
+WORKLOADS = {
+    "constant-512x256": (case_constant, dict(seed=1, w=512, h=256)),
+    "lognormal-8192x4096": (case_lognormal, dict(seed=2, w=8192, h=4096)),
+    "constant-16384x8192": (case_constant, dict(seed=3, w=16384, h=8192)),
+}

@@ -75,7 +75,7 @@ def test_first_guess_bit_for_bit(ws, w, h, wrap):
 
 @pytest.mark.parametrize("w,h,wrap,gen", [(64, 64, True, "shelf"), (128, 64, False, "shelf"),
                                           (1024, 512, True, "shelf"), (256, 2048, False, "shelf"),
-                                          (2048, 1024, True, "c2"), (8192, 64, True, "c2")])
+                                          (2048, 1024, True, "lognormal"), (8192, 64, True, "lognormal")])
 def test_parallel_first_guess_matches_the_recurrences(ws, w, h, wrap, gen):
     """The partition + cyclic-reduction line solver solves the same systems: equal to the
     oracle's recurrences to 1e-12 of the field's largest value."""
@@ -85,7 +85,7 @@ def test_parallel_first_guess_matches_the_recurrences(ws, w, h, wrap, gen):
         b, kd = case(w, h)
         dx, dt = 2400.0, 2e6
     else:
-        _, b, kd, dt = F.c2(w=w, h=h)
+        _, b, kd, dt = F.case_lognormal(w=w, h=h)
         dx = 1.0
     d = dtdiff.Dims(w, h, dx, wrap)
     got = dtdiff.lod(ws, d, b, kd, dt)
@@ -147,12 +147,12 @@ def test_the_gpu_solve_meets_its_tolerance_and_keeps_the_mass(ws):
         print(f"solve {w}x{h} ok", flush=True)
 
 
-@pytest.mark.parametrize("w,h,gen", [(512, 256, "c1"), (1024, 512, "c2"), (300, 129, "c2")])
+@pytest.mark.parametrize("w,h,gen", [(512, 256, "constant"), (1024, 512, "lognormal"), (300, 129, "lognormal")])
 def test_tight_solve_matches_the_oracle_to_1e10(ws, w, h, gen):
     """The north-star bar: relative L2 between the GPU FP64 answer and the oracle's <= 1e-10 on
     identical seeded inputs, both solved to a relative residual of 1e-12."""
     print(f"tight {gen} {w}x{h}", flush=True)
-    d, h0, kd, dt = (F.c1 if gen == "c1" else F.c2)(w=w, h=h)
+    d, h0, kd, dt = (F.case_constant if gen == "constant" else F.case_lognormal)(w=w, h=h)
     tol = 1e-12 * np.linalg.norm(h0) / np.sqrt(h0.size)
     x = h0.copy()
     its = dtdiff.solve(ws, d, x, kd, dt, dtdiff.DiffusionSolver.MULTIGRID, tol)
@@ -164,7 +164,7 @@ def test_tight_solve_matches_the_oracle_to_1e10(ws, w, h, gen):
 
 def test_solve_rel_device_meets_the_true_relative_residual(ws):
     from dtdiff.hipmem import DeviceArray
-    d, h0, kd, dt = F.c2(w=2048, h=1024)
+    d, h0, kd, dt = F.case_lognormal(w=2048, h=1024)
     hd = DeviceArray.from_numpy(h0)
     kdd = DeviceArray.from_numpy(kd)
     st = dtdiff.solve_rel_device(ws, d, hd, kdd, dt, 1e-10)
@@ -228,7 +228,7 @@ def test_graph_and_direct_launches_agree(ws, monkeypatch):
                                         (dtdiff.CYCLE_F32, dtdiff.GUESS_RHS)])
 def test_cycle_variants_reach_the_same_answer(prec, guess):
     wsv = dtdiff.Workspace(cycle_precision=prec, guess=guess)
-    d, h0, kd, dt = F.c2(w=512, h=256)
+    d, h0, kd, dt = F.case_lognormal(w=512, h=256)
     tol = 1e-11 * np.linalg.norm(h0) / np.sqrt(h0.size)
     x = h0.copy()
     its = dtdiff.solve(wsv, d, x, kd, dt, dtdiff.DiffusionSolver.MULTIGRID, tol)

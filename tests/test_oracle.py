@@ -404,7 +404,7 @@ def test_results_do_not_depend_on_the_thread_count():
 
 def test_the_config_fields():
     from dtdiff import fields as F
-    d, h0, kd, dt = F.c1(w=128, h=64)
+    d, h0, kd, dt = F.case_constant(w=128, h=64)
     assert abs(h0.mean()) < 1e-9 and abs(h0.std() - 1000.0) < 1e-9
     assert (kd == 1).all() and dt == 10.0
     # k^-2 power: the radially averaged spectrum falls by ~4x per doubling of |k|.
@@ -414,7 +414,7 @@ def test_the_config_fields():
     lo = f[(k > 0.05) & (k < 0.07)].mean()
     hi = f[(k > 0.1) & (k < 0.14)].mean()
     assert 2.5 < lo / hi < 6.5
-    d, h0, kd, dt = F.c2(w=256, h=128)
+    d, h0, kd, dt = F.case_lognormal(w=256, h=128)
     assert abs(kd.mean() - 1.0) < 1e-12 and dt == pytest.approx(50.0)
     lk = np.log(kd)
     assert abs(lk.std() - 1.0) < 1e-9
