@@ -1,0 +1,773 @@
+// The batched solve (dtd_solve_batch): many independent grids of different shapes, each solved as
+// dtd_solve_rel_device solves one (the operator and its levels, the splitting's first guess, f64
+// conjugate gradients preconditioned by the f32 V-cycle, the put-back, the true residual), with one
+// launch per pass and level over all of them.  Included once, by dtdiff.hip.
+//
+// Reference files (relative to the deep-time repository root):
+//   D  crates/deeptime-core/src/surface/diffusion.rs
+//   MG crates/deeptime-core/src/surface/diffusion/multigrid.rs
+//
+// The reference solves one grid a call (D:77-116); a batch here is that call on every grid.
+// Layout: the grids' f64 vectors lie back to back, each starting on a multiple of kCgCells cells;
+// each level of the cycle has one pool holding every grid's copy of that level.  A launch takes a
+// table of (grid, block of that grid) pairs, so its blocks never straddle two grids, and writes one
+// partial sum per block; a step kernel with one block per grid adds each grid's partials in block
+// order into that grid's CgState.  Band heights depend on the grid's shape alone, so a grid's
+// answer is the same bits whatever else is in the batch.  A grid whose loop has stopped skips
+// every later pass (its state says done) while the others go on; the host loop ends when all
+// have stopped.
+#pragma once
+
+#include "context.hpp"
+#include "kernels.hpp"
+
+namespace dtd {
+
+// ---- what one launch knows of each grid ------------------------------------------------------
+
+struct BGrid {  // one grid's f64 side
+    Grid g;
+    int wrap_x;
+    int th;  // band height of the streamed f64 passes
+    double dt, s2;
+    const double *kd, *b;  // the caller's arrays (b: h on entry, read until the end)
+    double *h;             // the caller's array, written last
+    double *x, *r, *p0, *p1, *ap;
+    double2 *es;
+    float *r32, *z32, *coef32;  // the finest f32 level
+    CgState *st;
+};
+
+struct BLevel {      // one grid's level l of the f32 cycle
+    Lvl<float> L, C;  // C: level l + 1 (unused on the grid's coarsest level)
+    int th;          // band height of the fused smoothing kernels on this level
+    int pad;
+    const double *dot;  // the f64 residual on the finest level (the r.z sum), else null
+    const CgState *st;
+};
+
+struct BInit {  // a solve's start for one grid
+    double tol2, rel2;
+    int active, pad;
+};
+
+// Band height of a streamed kernel on an h-row level of a batch: a function of the shape alone.
+inline int batch_band(int h) { return h >= 64 ? 16 : (h >= 16 ? 8 : 4); }
+
+// ---- kernels: a table maps each block to (grid, block of the grid) ---------------------------
+
+__global__ __launch_bounds__(kBlock) void kb_operator(const BGrid *__restrict__ t,
+                                                      const int2 *__restrict__ blk, float omega) {
+    const int2 m = blk[blockIdx.x];
+    const BGrid &q = t[m.x];
+    const int c = m.y * kBlock + threadIdx.x;
+    if (c < q.g.n) operator_cell(q.g, q.kd, q.dt, q.s2, q.es, q.coef32, nullptr, omega, c);
+}
+
+// Level l + 1 from level l, one coarse cell a thread (t: the grids' level l).
+__global__ __launch_bounds__(kBlock) void kb_coarsen(const BLevel *__restrict__ t,
+                                                     const int2 *__restrict__ blk, float scale,
+                                                     float omega) {
+    const int2 m = blk[blockIdx.x];
+    const BLevel &q = t[m.x];
+    const int k = m.y * kBlock + threadIdx.x;
+    if (k < q.C.n) coarsen_cell(q.L, q.C, scale, omega, k);
+}
+
+// x = b (out = 0) or the caller's h = x (out = 1).
+__global__ __launch_bounds__(kBlock) void kb_copy(const BGrid *__restrict__ t,
+                                                  const int2 *__restrict__ blk, int out) {
+    const int2 m = blk[blockIdx.x];
+    const BGrid &q = t[m.x];
+    const int c = m.y * kBlock + threadIdx.x;
+    if (c >= q.g.n) return;
+    if (out)
+        q.h[c] = q.x[c];
+    else
+        q.x[c] = q.b[c];
+}
+
+// The first guess's sweeps (`lod`, D:370-489), in place on x: one thread a row, then one a
+// column, the reference's recurrences (p0 and p1 are the scratch lines).
+__global__ __launch_bounds__(kBlock) void kb_lod_rows(const BGrid *__restrict__ t,
+                                                      const int2 *__restrict__ blk) {
+    const int2 m = blk[blockIdx.x];
+    const BGrid &q = t[m.x];
+    const int j = m.y * kBlock + threadIdx.x;
+    if (j < q.g.h) lod_row(q.g, q.wrap_x, q.kd, q.dt, q.s2, q.x, q.p0, q.p1, j);
+}
+__global__ __launch_bounds__(kBlock) void kb_lod_cols(const BGrid *__restrict__ t,
+                                                      const int2 *__restrict__ blk) {
+    const int2 m = blk[blockIdx.x];
+    const BGrid &q = t[m.x];
+    const int i = m.y * kBlock + threadIdx.x;
+    if (i < q.g.w) lod_col(q.g, q.kd, q.dt, q.s2, q.x, q.p0, i);
+}
+
+// A fresh state for each grid (inactive grids are marked done and skipped by every pass).
+__global__ void kb_init(CgState *st, const BInit *__restrict__ in, int n, int max_iter) {
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= n) return;
+    CgState s{};
+    s.tol2 = in[g].tol2;
+    s.rel2 = in[g].rel2;
+    s.max_iter = max_iter;
+    s.skip = !in[g].active;
+    s.done = s.skip ? 5 : 0;
+    st[g] = s;
+}
+
+// The first residual r = b - A x (x = b first when from_b); (r.r, b.b) per block.
+__global__ __launch_bounds__(kBlock) void kb_sresid(const BGrid *__restrict__ t,
+                                                    const int2 *__restrict__ blk, int from_b,
+                                                    double *__restrict__ parts) {
+    __shared__ double lds[kBlock / 64];
+    const int2 m = blk[blockIdx.x];
+    const BGrid &q = t[m.x];
+    if (q.st->skip) return;
+    block_pair(sresid_band<float>(q.g, q.es, q.b, from_b ? q.b : q.x, q.x, q.r, q.r32, q.th,
+                                  block_wave(m.y)),
+               parts, blockIdx.x, lds);
+}
+
+// rng[g] = (first block, block count) of grid g in the launch that left the partials.
+template <int STEP>
+__global__ void kb_step(const int2 *__restrict__ rng, CgState *st,
+                        const double *__restrict__ parts) {
+    __shared__ double lds[kReduceBlock / 64];
+    CgState *s = st + blockIdx.x;
+    if (s->skip) return;
+    const int2 r = rng[blockIdx.x];
+    cg_step_block<STEP>(parts + (size_t)(STEP == kStepRR0 ? 2 : 1) * r.x, r.y, s, lds);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_sdir(const BGrid *__restrict__ t,
+                                                  const int2 *__restrict__ blk, int parity,
+                                                  double *__restrict__ parts) {
+    __shared__ double lds[kBlock / 64];
+    const int2 m = blk[blockIdx.x];
+    const BGrid &q = t[m.x];
+    if (q.st->done) return;
+    const double *pold = parity == 0 ? q.p1 : q.p0;
+    double *pnew = parity == 0 ? q.p0 : q.p1;
+    const double acc =
+        sdir_band<float>(q.g, q.es, q.z32, pold, pnew, q.ap, q.x, q.th, q.st, block_wave(m.y));
+    const double sum = block_sum<kBlock>(acc, lds);
+    if (threadIdx.x == 0) parts[blockIdx.x] = sum;
+}
+
+__global__ __launch_bounds__(kBlock) void kb_update(const BGrid *__restrict__ t,
+                                                    const int2 *__restrict__ blk,
+                                                    double *__restrict__ parts) {
+    __shared__ double lds[kBlock / 64];
+    const int2 m = blk[blockIdx.x];
+    const BGrid &q = t[m.x];
+    if (q.st->done) return;
+    const double acc = update_block<float>(q.g.n, q.ap, q.r, q.r32, q.st->alpha, m.y);
+    const double sum = block_sum<kBlock>(acc, lds);
+    if (threadIdx.x == 0) parts[blockIdx.x] = sum;
+}
+
+template <int V>
+__global__ __launch_bounds__(kBlock) void kb_spre(const BLevel *__restrict__ t,
+                                                  const int2 *__restrict__ blk) {
+    const int2 m = blk[blockIdx.x];
+    const BLevel &q = t[m.x];
+    if (q.st->done) return;
+    spre_band<float, V>(q.L, q.L.z[0], q.C, q.th, block_wave(m.y));
+}
+
+template <int V>
+__global__ __launch_bounds__(kBlock) void kb_spost(const BLevel *__restrict__ t,
+                                                   const int2 *__restrict__ blk,
+                                                   double *__restrict__ parts) {
+    __shared__ double lds[kBlock / 64];
+    const int2 m = blk[blockIdx.x];
+    const BLevel &q = t[m.x];
+    if (q.st->done) return;
+    const double acc = spost_band<float, V>(q.L, q.L.z[0], q.C.z[1], q.C.w, q.L.z[1], q.th,
+                                            q.dot, block_wave(m.y));
+    if (q.dot) {
+        const double sum = block_sum<kBlock>(acc, lds);
+        if (threadIdx.x == 0) parts[blockIdx.x] = sum;
+    }
+}
+
+// The coarsest level of each grid whose cycle ends here: one wave a grid.
+__global__ __launch_bounds__(64) void kb_coarsest(const BLevel *__restrict__ t,
+                                                  const int2 *__restrict__ blk,
+                                                  double *__restrict__ parts) {
+    __shared__ float zs[2][64];
+    const BLevel &q = t[blk[blockIdx.x].x];
+    if (q.st->done) return;
+    coarsest_wave<float>(q.L, q.L.z[1], 16, q.dot, parts + blockIdx.x, zs);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_finish1(const BGrid *__restrict__ t,
+                                                     const int2 *__restrict__ blk,
+                                                     double *__restrict__ parts) {
+    __shared__ double lds[kBlock / 64];
+    const int2 m = blk[blockIdx.x];
+    const BGrid &q = t[m.x];
+    if (q.st->skip) return;
+    block_pair(finish1_band(q.g, q.es, q.b, q.x, q.p0, q.p1, q.ap, q.th, q.st, block_wave(m.y)),
+               parts, blockIdx.x, lds);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_finish2(const BGrid *__restrict__ t,
+                                                     const int2 *__restrict__ blk,
+                                                     double *__restrict__ parts) {
+    __shared__ double lds[kBlock / 64];
+    const int2 m = blk[blockIdx.x];
+    const BGrid &q = t[m.x];
+    if (q.st->skip) return;
+    block_pair(finish2_band(q.g, q.es, q.b, q.ap, q.x, q.th, q.st, block_wave(m.y)), parts,
+               blockIdx.x, lds);
+}
+
+// The put-back's sums into each grid's state; then each grid's (r.r, b.b) of the true residual.
+__global__ void kb_putback(const int2 *__restrict__ rng, CgState *st,
+                           const double *__restrict__ parts) {
+    __shared__ double lds[kReduceBlock / 64];
+    CgState *s = st + blockIdx.x;
+    if (s->skip) return;
+    const int2 r = rng[blockIdx.x];
+    const double l = reduce_parts(parts + 2 * (size_t)r.x, r.y, 2, 0, lds);
+    const double wg = reduce_parts(parts + 2 * (size_t)r.x, r.y, 2, 1, lds);
+    if (threadIdx.x == 0) s->lost = l, s->weight = wg;
+}
+__global__ void kb_pair(const int2 *__restrict__ rng, const CgState *__restrict__ st,
+                        const double *__restrict__ parts, double *__restrict__ out) {
+    __shared__ double lds[kReduceBlock / 64];
+    if (st[blockIdx.x].skip) return;
+    const int2 r = rng[blockIdx.x];
+    const double a = reduce_parts(parts + 2 * (size_t)r.x, r.y, 2, 0, lds);
+    const double b = reduce_parts(parts + 2 * (size_t)r.x, r.y, 2, 1, lds);
+    if (threadIdx.x == 0) out[2 * blockIdx.x] = a, out[2 * blockIdx.x + 1] = b;
+}
+
+// flag = 1 when every grid's loop has stopped.
+__global__ __launch_bounds__(kBlock) void kb_all_done(const CgState *__restrict__ st, int n,
+                                                      int *flag) {
+    int open = 0;
+    for (int g = threadIdx.x; g < n; g += kBlock) open |= st[g].done == 0;
+    open = __syncthreads_or(open);
+    if (threadIdx.x == 0) flag[0] = !open;
+}
+
+// ---- the plan of one batch shape -------------------------------------------------------------
+
+struct BLaunch {  // one launch: its block table (device) and block count
+    const int2 *blk = nullptr;
+    int nblk = 0;
+};
+
+struct BatchWs {
+    std::vector<long> key;  // (w, h, wrap) of each grid, omega and scale bits
+    int ngrids = 0;
+    int nlevels = 0;  // the deepest grid's level count
+    size_t cells = 0;
+    std::vector<int> nlev;       // levels of each grid
+    std::vector<BGrid> grids;    // host copy of the f64-side table
+    BGrid *d_grids = nullptr;
+    std::vector<BLevel *> d_lv;  // per level: the table of every grid that has the level
+    CgState *st = nullptr;
+    BInit *init = nullptr;
+    double *parts = nullptr, *pairs = nullptr;
+    int *flag = nullptr;
+    int2 *tables = nullptr;  // every block and range table, one allocation
+    // f64 side
+    BLaunch cells_l, update_l, rows_l, cols_l, band_l;
+    const int2 *rng_band = nullptr, *rng_update = nullptr, *rng_rz = nullptr;
+    // cycle, per level
+    std::vector<BLaunch> coarsen_l, pre2, pre1, post2, post1, coarsest;
+    int rz_base_post1 = 0, rz_base_coarsest = 0;
+    std::vector<void *> owned;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // pinned host
+    CgState *h_st = nullptr;
+    double *h_pairs = nullptr;
+    BInit *h_init = nullptr;
+    int *h_flag = nullptr;
+};
+
+void free_batch(dtd_ctx *ctx) {
+    BatchWs *bw = ctx ? ctx->batch : nullptr;
+    if (!bw) return;
+    for (auto &g : bw->graph)
+        if (g) (void)hipGraphExecDestroy(g);
+    for (void *p : bw->owned) (void)hipFree(p);
+    if (bw->h_st) (void)hipHostFree(bw->h_st);
+    if (bw->h_pairs) (void)hipHostFree(bw->h_pairs);
+    if (bw->h_init) (void)hipHostFree(bw->h_init);
+    if (bw->h_flag) (void)hipHostFree(bw->h_flag);
+    delete bw;
+    ctx->batch = nullptr;
+}
+
+static std::vector<long> batch_key(const dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
+    std::vector<long> key;
+    key.reserve(3 * n + 2);
+    for (size_t k = 0; k < n; k++) {
+        key.push_back((long)gs[k].w);
+        key.push_back((long)gs[k].h);
+        key.push_back(gs[k].wrap_x && gs[k].w > 1);
+    }
+    long om = 0, sc = 0;
+    std::memcpy(&om, &ctx->opts.omega, sizeof(float));
+    std::memcpy(&sc, &ctx->opts.scale, sizeof(float));
+    key.push_back(om);
+    key.push_back(sc);
+    return key;
+}
+
+static inline size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+// Builds the device buffers and launch tables for grids gs (all non-empty).
+static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
+    auto key = batch_key(ctx, n, gs);
+    if (ctx->batch && ctx->batch->key == key) return;
+    free_batch(ctx);
+    BatchWs *bw = new BatchWs();
+    ctx->batch = bw;
+    bw->key = key;
+    bw->ngrids = (int)n;
+    auto own = [&](size_t bytes) {
+        void *p = nullptr;
+        HIPCHK(hipMalloc(&p, bytes ? bytes : 1));
+        bw->owned.push_back(p);
+        return p;
+    };
+    // Shapes, and the offsets of each grid's f64 cells and of its levels in the level pools.
+    std::vector<std::vector<Shape>> sh(n);
+    std::vector<size_t> off(n);
+    size_t total = 0;
+    for (size_t g = 0; g < n; g++) {
+        sh[g] = shapes((int)gs[g].w, (int)gs[g].h, gs[g].wrap_x && gs[g].w > 1);
+        bw->nlev.push_back((int)sh[g].size());
+        bw->nlevels = std::max(bw->nlevels, (int)sh[g].size());
+        off[g] = total;
+        total += round_up(gs[g].w * gs[g].h, kCgCells);
+        bw->cells += gs[g].w * gs[g].h;
+    }
+    const int L = bw->nlevels;
+    std::vector<std::vector<size_t>> loff(L, std::vector<size_t>(n, 0));
+    std::vector<size_t> ltotal(L, 0);
+    for (int l = 0; l < L; l++)
+        for (size_t g = 0; g < n; g++)
+            if (l < bw->nlev[g]) {
+                loff[l][g] = ltotal[l];
+                ltotal[l] += round_up((size_t)sh[g][l].w * sh[g][l].h, 64);
+            }
+    double *x = (double *)own(total * 8), *r = (double *)own(total * 8);
+    double *p0 = (double *)own(total * 8), *p1 = (double *)own(total * 8);
+    double *ap = (double *)own(total * 8);
+    double2 *es = (double2 *)own(total * 16);
+    bw->st = (CgState *)own(n * sizeof(CgState));
+    bw->init = (BInit *)own(n * sizeof(BInit));
+    bw->pairs = (double *)own(2 * n * sizeof(double));
+    bw->flag = (int *)own(sizeof(int));
+    struct Pool {
+        float *coef, *mass, *r, *z0, *z1;
+    };
+    std::vector<Pool> pool(L);
+    for (int l = 0; l < L; l++) {
+        pool[l].coef = (float *)own(4 * ltotal[l] * 4);
+        pool[l].mass = l == 0 ? nullptr : (float *)own(ltotal[l] * 4);
+        pool[l].r = (float *)own(ltotal[l] * 4);
+        pool[l].z0 = (float *)own(ltotal[l] * 4);
+        pool[l].z1 = (float *)own(ltotal[l] * 4);
+    }
+    auto level = [&](size_t g, int l) {
+        Lvl<float> v{};
+        if (l >= bw->nlev[g]) return v;
+        const size_t o = loff[l][g];
+        v.w = sh[g][l].w, v.h = sh[g][l].h, v.wrap = sh[g][l].wrap, v.n = v.w * v.h;
+        v.coef = pool[l].coef + 4 * o;
+        v.mass = l == 0 ? nullptr : pool[l].mass + o;
+        v.r = pool[l].r + o;
+        v.z[0] = pool[l].z0 + o;
+        v.z[1] = pool[l].z1 + o;
+        return v;
+    };
+    // The f64 side's table (the caller's arrays and dt are filled per call).
+    bw->grids.resize(n);
+    for (size_t g = 0; g < n; g++) {
+        BGrid &q = bw->grids[g];
+        q = BGrid{};
+        const int w = (int)gs[g].w, h = (int)gs[g].h;
+        q.g = Grid{w, h, (int)(gs[g].wrap_x && w > 1), w * h};
+        q.th = batch_band(h);
+        const size_t o = off[g];
+        q.x = x + o, q.r = r + o, q.p0 = p0 + o, q.p1 = p1 + o, q.ap = ap + o, q.es = es + o;
+        const Lvl<float> L0 = level(g, 0);
+        q.r32 = L0.r, q.z32 = L0.z[1], q.coef32 = L0.coef;
+        q.st = bw->st + g;
+    }
+    bw->d_grids = (BGrid *)own(n * sizeof(BGrid));
+    // Level tables: every grid that has level l, in grid order (task index = position).
+    std::vector<std::vector<int>> lv_grid(L);  // grid of each task of level l
+    std::vector<std::vector<BLevel>> lv(L);
+    for (int l = 0; l < L; l++) {
+        for (size_t g = 0; g < n; g++) {
+            if (l >= bw->nlev[g]) continue;
+            BLevel t{};
+            t.L = level(g, l);
+            t.C = level(g, l + 1);
+            t.th = batch_band(t.L.h);
+            t.dot = l == 0 ? bw->grids[g].r : nullptr;
+            t.st = bw->st + g;
+            lv[l].push_back(t);
+            lv_grid[l].push_back((int)g);
+        }
+        BLevel *d = (BLevel *)own(lv[l].size() * sizeof(BLevel));
+        HIPCHK(hipMemcpy(d, lv[l].data(), lv[l].size() * sizeof(BLevel), hipMemcpyHostToDevice));
+        bw->d_lv.push_back(d);
+    }
+    // Block and range tables, in one host vector; launches keep their offsets until upload.
+    std::vector<int2> tab;
+    struct Pending {
+        BLaunch *l;
+        size_t at;
+    };
+    std::vector<Pending> pend;
+    auto add_launch = [&](BLaunch &bl, const std::vector<int> &blocks_of, std::vector<int2> *rng) {
+        // blocks_of[task] blocks for each task, in task order.
+        const size_t at = tab.size();
+        for (size_t t = 0; t < blocks_of.size(); t++) {
+            if (rng) rng->push_back(make_int2((int)(tab.size() - at), blocks_of[t]));
+            for (int b = 0; b < blocks_of[t]; b++) tab.push_back(make_int2((int)t, b));
+        }
+        bl.nblk = (int)(tab.size() - at);
+        pend.push_back({&bl, at});
+    };
+    std::vector<int> cnt(n);
+    for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].w * gs[g].h);
+    add_launch(bw->cells_l, cnt, nullptr);
+    for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].w * gs[g].h, kCgCells);
+    std::vector<int2> rng_update, rng_band, rng_rz(n);
+    add_launch(bw->update_l, cnt, &rng_update);
+    for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].h);
+    add_launch(bw->rows_l, cnt, nullptr);
+    for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].w);
+    add_launch(bw->cols_l, cnt, nullptr);
+    for (size_t g = 0; g < n; g++)
+        cnt[g] = stream_grid<1, 1>((int)gs[g].w, (int)gs[g].h, bw->grids[g].th);
+    add_launch(bw->band_l, cnt, &rng_band);
+    // The cycle: coarsening of level l + 1 from level l's tasks; the smoothing launches of the
+    // grids for which level l is not the coarsest, split by lane width; the coarsest launches.
+    bw->coarsen_l.resize(L);
+    bw->pre2.resize(L), bw->pre1.resize(L), bw->post2.resize(L), bw->post1.resize(L);
+    bw->coarsest.resize(L);
+    for (int l = 0; l < L; l++) {
+        const size_t nt = lv[l].size();
+        std::vector<int> cz(nt, 0), c2(nt, 0), c1(nt, 0), cc(nt, 0);
+        for (size_t t = 0; t < nt; t++) {
+            const BLevel &q = lv[l][t];
+            const int g = lv_grid[l][t];
+            const bool last = l + 1 == bw->nlev[g];
+            if (!last) {
+                cz[t] = nblocks(q.C.n);
+                if (q.L.w % 2 == 0)
+                    c2[t] = stream_grid<2, 1>(q.L.w, q.L.h, q.th);
+                else
+                    c1[t] = stream_grid<1, 2>(q.L.w, q.L.h, q.th);
+            } else {
+                if (q.L.n > 64) throw Error{DTD_ESTATE, "the coarsest level exceeds one wave"};
+                cc[t] = 1;
+            }
+        }
+        add_launch(bw->coarsen_l[l], cz, nullptr);
+        std::vector<int2> r2, r1, rc;
+        add_launch(bw->pre2[l], c2, nullptr);
+        add_launch(bw->pre1[l], c1, nullptr);
+        add_launch(bw->post2[l], c2, l == 0 ? &r2 : nullptr);
+        add_launch(bw->post1[l], c1, l == 0 ? &r1 : nullptr);
+        add_launch(bw->coarsest[l], cc, l == 0 ? &rc : nullptr);
+        if (l == 0) {
+            // The r.z partials: post2's blocks, then post1's, then the coarsest launch's.
+            bw->rz_base_post1 = bw->post2[0].nblk;
+            bw->rz_base_coarsest = bw->post2[0].nblk + bw->post1[0].nblk;
+            for (size_t t = 0; t < nt; t++) {
+                const int g = lv_grid[0][t];
+                if (c2[t])
+                    rng_rz[g] = r2[t];
+                else if (c1[t])
+                    rng_rz[g] = make_int2(bw->rz_base_post1 + r1[t].x, r1[t].y);
+                else
+                    rng_rz[g] = make_int2(bw->rz_base_coarsest + rc[t].x, 1);
+            }
+        }
+    }
+    const size_t at_rng = tab.size();
+    tab.insert(tab.end(), rng_band.begin(), rng_band.end());
+    tab.insert(tab.end(), rng_update.begin(), rng_update.end());
+    tab.insert(tab.end(), rng_rz.begin(), rng_rz.end());
+    bw->tables = (int2 *)own(tab.size() * sizeof(int2));
+    HIPCHK(hipMemcpy(bw->tables, tab.data(), tab.size() * sizeof(int2), hipMemcpyHostToDevice));
+    for (auto &p : pend) p.l->blk = bw->tables + p.at;
+    bw->rng_band = bw->tables + at_rng;
+    bw->rng_update = bw->rng_band + n;
+    bw->rng_rz = bw->rng_update + n;
+    // Partial sums: pairs for the widest launch (and the r.z launches side by side).
+    size_t most = (size_t)std::max({bw->band_l.nblk, bw->update_l.nblk,
+                                    bw->post2[0].nblk + bw->post1[0].nblk + bw->coarsest[0].nblk});
+    bw->parts = (double *)own(2 * most * sizeof(double) + 16);
+    HIPCHK(hipHostMalloc((void **)&bw->h_st, n * sizeof(CgState), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&bw->h_pairs, 2 * n * sizeof(double), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&bw->h_init, n * sizeof(BInit), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&bw->h_flag, 2 * sizeof(int), hipHostMallocDefault));
+}
+
+// ---- enqueueing ------------------------------------------------------------------------------
+
+// The V-cycle of every grid (`vcycle`, MG:500-523): down the levels (pre-smoothing and
+// restriction, or the coarsest solve of the grids that end there), then up (post-smoothing; the
+// finest level's leaves the r.z partials).
+static void batch_vcycle(BatchWs &bw, hipStream_t s) {
+    const int L = bw.nlevels;
+    for (int l = 0; l < L; l++) {
+        if (bw.pre2[l].nblk) {
+            kb_spre<2><<<bw.pre2[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.pre2[l].blk);
+            LAUNCH_CHECK();
+        }
+        if (bw.pre1[l].nblk) {
+            kb_spre<1><<<bw.pre1[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.pre1[l].blk);
+            LAUNCH_CHECK();
+        }
+        if (bw.coarsest[l].nblk) {
+            kb_coarsest<<<bw.coarsest[l].nblk, 64, 0, s>>>(
+                bw.d_lv[l], bw.coarsest[l].blk, bw.parts + (l == 0 ? bw.rz_base_coarsest : 0));
+            LAUNCH_CHECK();
+        }
+    }
+    for (int l = L - 2; l >= 0; l--) {
+        if (bw.post2[l].nblk) {
+            kb_spost<2><<<bw.post2[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.post2[l].blk, bw.parts);
+            LAUNCH_CHECK();
+        }
+        if (bw.post1[l].nblk) {
+            kb_spost<1><<<bw.post1[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.post1[l].blk,
+                                                            bw.parts + bw.rz_base_post1);
+            LAUNCH_CHECK();
+        }
+    }
+}
+
+// One iteration of every grid still running (MG:630-733), then the all-stopped flag.
+static void batch_iteration(BatchWs &bw, int parity, hipStream_t s) {
+    const int n = bw.ngrids;
+    kb_sdir<<<bw.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.band_l.blk, parity, bw.parts);
+    LAUNCH_CHECK();
+    kb_step<kStepPAP><<<n, kReduceBlock, 0, s>>>(bw.rng_band, bw.st, bw.parts);
+    LAUNCH_CHECK();
+    kb_update<<<bw.update_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.update_l.blk, bw.parts);
+    LAUNCH_CHECK();
+    kb_step<kStepRR><<<n, kReduceBlock, 0, s>>>(bw.rng_update, bw.st, bw.parts);
+    LAUNCH_CHECK();
+    batch_vcycle(bw, s);
+    kb_step<kStepRZ><<<n, kReduceBlock, 0, s>>>(bw.rng_rz, bw.st, bw.parts);
+    LAUNCH_CHECK();
+    kb_all_done<<<1, kBlock, 0, s>>>(bw.st, n, bw.flag);
+    LAUNCH_CHECK();
+}
+
+static void batch_graphs(dtd_ctx *ctx, BatchWs &bw) {
+    for (int k = 0; k < 2; k++) {
+        if (bw.graph[k]) continue;
+        hipGraph_t gr = nullptr;
+        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            batch_iteration(bw, k, ctx->stream);
+        } catch (...) {
+            (void)hipStreamEndCapture(ctx->stream, &gr);
+            if (gr) (void)hipGraphDestroy(gr);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(ctx->stream, &gr));
+        hipError_t e = hipGraphInstantiate(&bw.graph[k], gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        HIPCHK(e);
+    }
+}
+
+// One pass of the loop over the active grids (h_init says which, and their tolerances): the
+// first residual (from b when from_b, else from x), the cycle, the iterations, the last update,
+// the put-back and the true residual.  Leaves h_st and h_pairs filled.
+static void batch_pass(dtd_ctx *ctx, BatchWs &bw, int from_b) {
+    hipStream_t s = ctx->stream;
+    const int n = bw.ngrids;
+    HIPCHK(hipMemcpyAsync(bw.init, bw.h_init, n * sizeof(BInit), hipMemcpyHostToDevice, s));
+    kb_init<<<nblocks(n), kBlock, 0, s>>>(bw.st, bw.init, n, (int)ctx->opts.max_iter);
+    LAUNCH_CHECK();
+    kb_sresid<<<bw.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.band_l.blk, from_b, bw.parts);
+    LAUNCH_CHECK();
+    kb_step<kStepRR0><<<n, kReduceBlock, 0, s>>>(bw.rng_band, bw.st, bw.parts);
+    LAUNCH_CHECK();
+    batch_vcycle(bw, s);
+    kb_step<kStepRZ0><<<n, kReduceBlock, 0, s>>>(bw.rng_rz, bw.st, bw.parts);
+    LAUNCH_CHECK();
+    kb_all_done<<<1, kBlock, 0, s>>>(bw.st, n, bw.flag);
+    LAUNCH_CHECK();
+    if (ctx->use_graph) batch_graphs(ctx, bw);
+    // The loop: a pass ends when every grid has stopped.  Iteration k + 1 is launched before the
+    // flag after iteration k is read (the flag before the loop counts as iteration 0's), so the
+    // device runs at most one iteration that every grid skips; the state's cap stops every grid by
+    // max_iter.
+    HIPCHK(hipMemcpyAsync(&bw.h_flag[1], bw.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(ctx->flag_ev[1], s));
+    const int max_iter = (int)ctx->opts.max_iter;
+    for (int k = 0; k <= max_iter + 1; k++) {
+        const int par = k & 1;
+        if (ctx->use_graph)
+            HIPCHK(hipGraphLaunch(bw.graph[par], s));
+        else
+            batch_iteration(bw, par, s);
+        HIPCHK(hipMemcpyAsync(&bw.h_flag[par], bw.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipEventRecord(ctx->flag_ev[par], s));
+        HIPCHK(hipEventSynchronize(ctx->flag_ev[par ^ 1]));
+        if (bw.h_flag[par ^ 1]) break;
+    }
+    kb_finish1<<<bw.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.band_l.blk, bw.parts);
+    LAUNCH_CHECK();
+    kb_putback<<<n, kReduceBlock, 0, s>>>(bw.rng_band, bw.st, bw.parts);
+    LAUNCH_CHECK();
+    kb_finish2<<<bw.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.band_l.blk, bw.parts);
+    LAUNCH_CHECK();
+    kb_pair<<<n, kReduceBlock, 0, s>>>(bw.rng_band, bw.st, bw.parts, bw.pairs);
+    LAUNCH_CHECK();
+    HIPCHK(hipMemcpyAsync(bw.h_st, bw.st, n * sizeof(CgState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(bw.h_pairs, bw.pairs, 2 * n * sizeof(double), hipMemcpyDeviceToHost,
+                          s));
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+// dtd_solve_batch.
+static void solve_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs_in, double rel_tol,
+                        size_t *iterations, double *rels, dtd_batch_stats *stats) {
+    if (ctx->opts.cycle_precision != DTD_CYCLE_F32 || ctx->opts.nu != 2)
+        throw Error{DTD_EINVAL, "the batched solve runs the f32 cycle with nu = 2"};
+    const bool relative = rel_tol > 0.0;
+    if (!std::isfinite(rel_tol)) throw Error{DTD_EINVAL, "rel_tol must be finite"};
+    // The non-empty grids take part; the others get 0 iterations and residual 0.
+    std::vector<dtd_grid> gs;
+    std::vector<size_t> idx;
+    for (size_t k = 0; k < count; k++) {
+        const dtd_grid &g = gs_in[k];
+        check_grid(g.w, g.h);
+        if (iterations) iterations[k] = 0;
+        if (rels) rels[k] = 0.0;
+        if (g.w * g.h == 0) continue;
+        if (!g.h_dev || !g.kd_dev) throw Error{DTD_EINVAL, "null array in grid " + std::to_string(k)};
+        if (!(std::isfinite(g.dx) && g.dx > 0.0 && std::isfinite(g.dt)))
+            throw Error{DTD_EINVAL, "dx and dt must be finite (grid " + std::to_string(k) + ")"};
+        if (!relative && !(std::isfinite(g.tol_m) && g.tol_m > 0.0))
+            throw Error{DTD_EINVAL, "tol_m must be finite and positive (grid " +
+                                        std::to_string(k) + ")"};
+        gs.push_back(g);
+        idx.push_back(k);
+    }
+    const size_t n = gs.size();
+    if (n == 0) return;
+    if (n > (size_t)INT32_MAX / 2) throw Error{DTD_ETOOBIG, "too many grids"};
+    plan_batch(ctx, n, gs.data());
+    BatchWs &bw = *ctx->batch;
+    hipStream_t s = ctx->stream;
+    for (size_t g = 0; g < n; g++) {
+        BGrid &q = bw.grids[g];
+        q.wrap_x = gs[g].wrap_x;
+        q.dt = gs[g].dt;
+        q.s2 = gs[g].dx * gs[g].dx;
+        q.kd = gs[g].kd_dev;
+        q.b = gs[g].h_dev;
+        q.h = gs[g].h_dev;
+    }
+    HIPCHK(hipMemcpyAsync(bw.d_grids, bw.grids.data(), n * sizeof(BGrid), hipMemcpyHostToDevice,
+                          s));
+    HIPCHK(hipEventRecord(ctx->ev[0], s));
+    kb_operator<<<bw.cells_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.cells_l.blk, ctx->opts.omega);
+    LAUNCH_CHECK();
+    for (int l = 0; l + 1 < bw.nlevels; l++) {
+        if (!bw.coarsen_l[l].nblk) continue;
+        kb_coarsen<<<bw.coarsen_l[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.coarsen_l[l].blk,
+                                                           ctx->opts.scale, ctx->opts.omega);
+        LAUNCH_CHECK();
+    }
+    HIPCHK(hipEventRecord(ctx->ev[1], s));
+    // Every grid active for the first pass.
+    for (size_t g = 0; g < n; g++) {
+        const double nn = (double)(gs[g].w * gs[g].h);
+        bw.h_init[g] = relative ? BInit{0.0, rel_tol * rel_tol, 1, 0}
+                                : BInit{gs[g].tol_m * gs[g].tol_m * nn, 0.0, 1, 0};
+    }
+    const bool lod_guess = ctx->opts.guess == DTD_GUESS_LOD;
+    if (lod_guess) {
+        kb_copy<<<bw.cells_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.cells_l.blk, 0);
+        LAUNCH_CHECK();
+        kb_lod_rows<<<bw.rows_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.rows_l.blk);
+        LAUNCH_CHECK();
+        kb_lod_cols<<<bw.cols_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.cols_l.blk);
+        LAUNCH_CHECK();
+    }
+    HIPCHK(hipEventRecord(ctx->ev[2], s));
+    batch_pass(ctx, bw, lod_guess ? 0 : 1);
+    // Per grid: iterations, the true relative residual; then the grids that missed the target
+    // solve again from their last answer, aiming below it by the factor they missed it by (at most
+    // four times), the others left alone.
+    std::vector<size_t> its(n, 0);
+    std::vector<double> rel(n, 0.0), tolm(n, 0.0);
+    size_t restarts = 0;
+    auto collect = [&](bool first) {
+        for (size_t g = 0; g < n; g++) {
+            if (!first && !bw.h_init[g].active) continue;
+            its[g] += (size_t)bw.h_st[g].its;
+            const double rr = bw.h_pairs[2 * g], bb = bw.h_pairs[2 * g + 1];
+            rel[g] = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+            if (first)
+                tolm[g] = relative ? rel_tol * std::sqrt(bb) / std::sqrt((double)(gs[g].w * gs[g].h))
+                                   : gs[g].tol_m;
+        }
+    };
+    collect(true);
+    for (int round = 0; relative && round < 4; round++) {
+        bool any = false;
+        for (size_t g = 0; g < n; g++) {
+            const double bb = bw.h_pairs[2 * g + 1];
+            const bool again = rel[g] > rel_tol && bb > 0.0;
+            if (again) {
+                tolm[g] *= 0.5 * rel_tol / rel[g];
+                restarts++;
+                any = true;
+            }
+            const double nn = (double)(gs[g].w * gs[g].h);
+            bw.h_init[g] = BInit{tolm[g] * tolm[g] * nn, 0.0, again ? 1 : 0, 0};
+        }
+        if (!any) break;
+        batch_pass(ctx, bw, 0);
+        collect(false);
+    }
+    HIPCHK(hipEventRecord(ctx->ev[3], s));
+    // Every grid's answer into the caller's h.
+    kb_copy<<<bw.cells_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.cells_l.blk, 1);
+    LAUNCH_CHECK();
+    HIPCHK(hipEventRecord(ctx->ev[4], s));
+    HIPCHK(hipStreamSynchronize(s));
+    dtd_batch_stats out{};
+    out.grids = n;
+    out.cells = bw.cells;
+    out.restarts = restarts;
+    for (size_t g = 0; g < n; g++) {
+        if (iterations) iterations[idx[g]] = its[g];
+        if (rels) rels[idx[g]] = rel[g];
+        out.iterations_max = std::max(out.iterations_max, its[g]);
+        out.iterations_sum += its[g];
+        out.rel_residual_max = std::max(out.rel_residual_max, rel[g]);
+    }
+    out.ms_setup = elapsed(ctx->ev[0], ctx->ev[1]);
+    out.ms_guess = elapsed(ctx->ev[1], ctx->ev[2]);
+    out.ms_cg = elapsed(ctx->ev[2], ctx->ev[3]);
+    out.ms_total = elapsed(ctx->ev[0], ctx->ev[4]);
+    if (stats) *stats = out;
+}
+
+}  // namespace dtd

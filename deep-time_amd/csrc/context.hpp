@@ -1,0 +1,177 @@
+// Host-side state shared by the library's translation units: errors, the context and its
+// single-grid workspace, and the launch helpers of the streamed kernels.
+#pragma once
+
+#include "../../include/dtdiff.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace dtd {
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw ::dtd::Error{e_ == hipErrorOutOfMemory ? DTD_ETOOBIG : DTD_EDEVICE,    \
+                               std::string(#expr) + ": " + hipGetErrorString(e_)};       \
+    } while (0)
+
+#define LAUNCH_CHECK() HIPCHK(hipGetLastError())
+
+inline int nblocks(long n, int bs = kBlock) { return (int)((n + bs - 1) / bs); }
+
+template <typename T>
+T *dalloc(size_t n) {
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, (n ? n : 1) * sizeof(T)));
+    return (T *)p;
+}
+
+// The level shapes `Level::coarsen` produces (MG:117-123; `shapes`, MGGPU:336-347): halve,
+// rounding up, until a level holds at most 64 cells or stops shrinking; a coarse level wraps
+// when the finer one does and it is more than one cell wide.
+struct Shape {
+    int w, h, wrap;
+};
+inline std::vector<Shape> shapes(int w, int h, int wrap) {
+    std::vector<Shape> out{{w, h, wrap}};
+    for (;;) {
+        Shape s = out.back();
+        int wc = (s.w + 1) / 2, hc = (s.h + 1) / 2;
+        if ((long)s.w * s.h <= 64 || (wc == s.w && hc == s.h)) return out;
+        out.push_back({wc, hc, s.wrap && wc > 1});
+    }
+}
+
+// Device memory of one grid shape (the reference's Workspace, D:213-260, and the device cycle's
+// levels, MGGPU:350-558, in one place), kept while the shape stays.
+struct Workspace {
+    int w = 0, h = 0, wrap = 0, prec = -1;
+    long n = 0;
+    double *b = nullptr, *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr;
+    double *ap = nullptr, *kd = nullptr, *parts = nullptr, *pair = nullptr;
+    double2 *es = nullptr;  // the f64 operator: (east, south) of each cell
+    CgState *st = nullptr;
+    std::vector<Lvl<float>> lf;
+    std::vector<Lvl<double>> ld;
+    std::vector<void *> owned;  // every allocation of the levels
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+};
+
+struct BatchWs;
+
+}  // namespace dtd
+
+struct dtd_ctx {
+    dtd_opts opts;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    hipEvent_t flag_ev[2] = {};
+    dtd::CgState *host_st = nullptr;  // pinned
+    int *host_flag = nullptr;         // pinned, two slots
+    double *host_pair = nullptr;      // pinned, two doubles
+    dtd::Workspace ws;
+    std::string err;
+    size_t cycle_levels = 0;  // levels built by dtd_cycle_new
+    bool use_graph = true;
+    dtd::BatchWs *batch = nullptr;  // the batched solve's buffers (batch.hip)
+};
+
+namespace dtd {
+
+void free_ws(Workspace &ws);
+void free_batch(dtd_ctx *ctx);  // drops the batched solve's buffers (batch.hip)
+
+// Band height for a streaming kernel on a w x h level (a multiple of 4 rows; DTD_BAND_ROWS
+// overrides it).
+template <int V, int H = LaneGeom<V>::halo>
+int band_height(const void *kernel, int w, int h) {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    static const int forced = [] {
+        const char *v = getenv("DTD_BAND_ROWS");
+        return v ? atoi(v) : 0;
+    }();
+    if (forced > 0) return std::max(4, (forced + 3) / 4 * 4);
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0));
+    const long resident = (long)std::max(per_cu, 1) * cus * (kBlock / 64);
+    // Short bands in many rounds keep the waves in flight on a compact stretch of rows (the best
+    // measured on the finest levels); levels too small for two rounds take 4 rows.
+    for (int th = 16; th >= 8; th >>= 1)
+        if ((long)stream_waves<V, H>(w, h, th) >= 2 * resident) return th;
+    return 4;
+}
+
+template <int V, int H>
+int stream_grid(int w, int h, int th) {
+    return (stream_waves<V, H>(w, h, th) + kBlock / 64 - 1) / (kBlock / 64);
+}
+
+inline float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    return ms;
+}
+
+inline void check_grid(size_t w, size_t h) {
+    // Cell indices are 32-bit; keep a margin for the last block's threads.
+    const size_t cap = 0x7fff0000ull;
+    if (w > cap || h > cap || (w && h > cap / w))
+        throw Error{DTD_ETOOBIG, "the grid exceeds 2^31 - 2^16 cells"};
+}
+
+inline void begin(dtd_ctx *ctx) {
+    if (!ctx) throw Error{DTD_EINVAL, "null context"};
+    ctx->err.clear();
+    HIPCHK(hipSetDevice(ctx->opts.device));
+}
+
+// Device copy of a host array on the context's stream.
+inline void upload(dtd_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+}
+inline void download(dtd_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+}
+
+template <class F>
+int guard(dtd_ctx *ctx, F f) {
+    try {
+        f();
+        return DTD_OK;
+    } catch (const Error &e) {
+        if (ctx) ctx->err = e.msg;
+        if ((e.code == DTD_EDEVICE || e.code == DTD_ETOOBIG) && ctx) {
+            // A failed shape is rebuilt on the next call.
+            (void)hipStreamSynchronize(ctx->stream);
+            free_ws(ctx->ws);
+            free_batch(ctx);
+            ctx->cycle_levels = 0;
+        }
+        return e.code;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->err = e.what();
+        return DTD_EDEVICE;
+    }
+}
+
+}  // namespace dtd

@@ -24,84 +24,12 @@
 #include <string>
 #include <vector>
 
+#include "context.hpp"
 #include "kernels.hpp"
 
 namespace dtd {
 
-struct Error {
-    int code;
-    std::string msg;
-};
-
-#define HIPCHK(expr)                                                                     \
-    do {                                                                                 \
-        hipError_t e_ = (expr);                                                          \
-        if (e_ != hipSuccess)                                                            \
-            throw ::dtd::Error{e_ == hipErrorOutOfMemory ? DTD_ETOOBIG : DTD_EDEVICE,    \
-                               std::string(#expr) + ": " + hipGetErrorString(e_)};       \
-    } while (0)
-
-#define LAUNCH_CHECK() HIPCHK(hipGetLastError())
-
-static inline int nblocks(long n, int bs = kBlock) { return (int)((n + bs - 1) / bs); }
-
-template <typename T>
-static T *dalloc(size_t n) {
-    void *p = nullptr;
-    HIPCHK(hipMalloc(&p, (n ? n : 1) * sizeof(T)));
-    return (T *)p;
-}
-
-// The level shapes `Level::coarsen` produces (MG:117-123; `shapes`, MGGPU:336-347): halve,
-// rounding up, until a level holds at most 64 cells or stops shrinking; a coarse level wraps
-// when the finer one does and it is more than one cell wide.
-struct Shape {
-    int w, h, wrap;
-};
-static std::vector<Shape> shapes(int w, int h, int wrap) {
-    std::vector<Shape> out{{w, h, wrap}};
-    for (;;) {
-        Shape s = out.back();
-        int wc = (s.w + 1) / 2, hc = (s.h + 1) / 2;
-        if ((long)s.w * s.h <= 64 || (wc == s.w && hc == s.h)) return out;
-        out.push_back({wc, hc, s.wrap && wc > 1});
-    }
-}
-
-// Device memory of one grid shape (the reference's Workspace, D:213-260, and the device cycle's
-// levels, MGGPU:350-558, in one place), kept while the shape stays.
-struct Workspace {
-    int w = 0, h = 0, wrap = 0, prec = -1;
-    long n = 0;
-    double *b = nullptr, *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr;
-    double *ap = nullptr, *kd = nullptr, *parts = nullptr, *pair = nullptr;
-    double2 *es = nullptr;  // the f64 operator: (east, south) of each cell
-    CgState *st = nullptr;
-    std::vector<Lvl<float>> lf;
-    std::vector<Lvl<double>> ld;
-    std::vector<void *> owned;  // every allocation of the levels
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-};
-
-}  // namespace dtd
-
-struct dtd_ctx {
-    dtd_opts opts;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[6] = {};
-    hipEvent_t flag_ev[2] = {};
-    dtd::CgState *host_st = nullptr;  // pinned
-    int *host_flag = nullptr;         // pinned, two slots
-    double *host_pair = nullptr;      // pinned, two doubles
-    dtd::Workspace ws;
-    std::string err;
-    size_t cycle_levels = 0;  // levels built by dtd_cycle_new
-    bool use_graph = true;
-};
-
-namespace dtd {
-
-static void free_ws(Workspace &ws) {
+void free_ws(Workspace &ws) {
     for (int k = 0; k < 2; k++)
         if (ws.graph[k]) {
             (void)hipGraphExecDestroy(ws.graph[k]);
@@ -224,36 +152,6 @@ static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt)
         build_coarse<double>(ws.ld, (double)ctx->opts.omega, (double)ctx->opts.scale, s);
     else
         build_coarse<float>(ws.lf, ctx->opts.omega, ctx->opts.scale, s);
-}
-
-// Band height for a streaming kernel on a w x h level (a multiple of 4 rows; DTD_BAND_ROWS
-// overrides it).
-template <int V, int H = LaneGeom<V>::halo>
-static int band_height(const void *kernel, int w, int h) {
-    static int cus = 0;
-    if (!cus) {
-        int dev = 0;
-        HIPCHK(hipGetDevice(&dev));
-        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    }
-    static const int forced = [] {
-        const char *v = getenv("DTD_BAND_ROWS");
-        return v ? atoi(v) : 0;
-    }();
-    if (forced > 0) return std::max(4, (forced + 3) / 4 * 4);
-    int per_cu = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0));
-    const long resident = (long)std::max(per_cu, 1) * cus * (kBlock / 64);
-    // Short bands in many rounds keep the waves in flight on a compact stretch of rows (the best
-    // measured on the finest levels); levels too small for two rounds take 4 rows.
-    for (int th = 16; th >= 8; th >>= 1)
-        if ((long)stream_waves<V, H>(w, h, th) >= 2 * resident) return th;
-    return 4;
-}
-
-template <int V, int H>
-static int stream_grid(int w, int h, int th) {
-    return (stream_waves<V, H>(w, h, th) + kBlock / 64 - 1) / (kBlock / 64);
 }
 
 template <typename T, int V>
@@ -479,12 +377,6 @@ static void lod(dtd_ctx *ctx, const double *b, double *x, const double *kd, doub
     LAUNCH_CHECK();
 }
 
-static float elapsed(hipEvent_t a, hipEvent_t b) {
-    float ms = 0.0f;
-    HIPCHK(hipEventElapsedTime(&ms, a, b));
-    return ms;
-}
-
 enum Guess { kGuessLod, kGuessRhs, kGuessKeep };
 
 // One solve on the workspace (`multigrid_in` + `cg_on`, D:101-116, MG:562-763).  b is the
@@ -569,48 +461,6 @@ static size_t solve_core(dtd_ctx *ctx, const double *b, const double *kd, double
     return (size_t)ctx->host_st->its;
 }
 
-static void check_grid(size_t w, size_t h) {
-    // Cell indices are 32-bit; keep a margin for the last block's threads.
-    const size_t cap = 0x7fff0000ull;
-    if (w > cap || h > cap || (w && h > cap / w))
-        throw Error{DTD_ETOOBIG, "the grid exceeds 2^31 - 2^16 cells"};
-}
-
-static void begin(dtd_ctx *ctx) {
-    if (!ctx) throw Error{DTD_EINVAL, "null context"};
-    ctx->err.clear();
-    HIPCHK(hipSetDevice(ctx->opts.device));
-}
-
-// Device copy of a host array on the context's stream.
-static void upload(dtd_ctx *ctx, void *dst, const void *src, size_t bytes) {
-    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
-}
-static void download(dtd_ctx *ctx, void *dst, const void *src, size_t bytes) {
-    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-}
-
-template <class F>
-static int guard(dtd_ctx *ctx, F f) {
-    try {
-        f();
-        return DTD_OK;
-    } catch (const Error &e) {
-        if (ctx) ctx->err = e.msg;
-        if ((e.code == DTD_EDEVICE || e.code == DTD_ETOOBIG) && ctx) {
-            // A failed shape is rebuilt on the next call.
-            (void)hipStreamSynchronize(ctx->stream);
-            free_ws(ctx->ws);
-            ctx->cycle_levels = 0;
-        }
-        return e.code;
-    } catch (const std::exception &e) {
-        if (ctx) ctx->err = e.what();
-        return DTD_EDEVICE;
-    }
-}
-
 // Calibration kernels of dtd_time_kernel: a float4 copy, and five float4 input streams combined
 // into one output (the smoother's stream count without its stencil).
 __global__ void k_copy4(long n4, const float4 *__restrict__ a, float4 *__restrict__ o) {
@@ -628,6 +478,8 @@ __global__ void k_stream5(long n4, const float4 *__restrict__ a, const float4 *_
 }
 
 }  // namespace dtd
+
+#include "batch.hpp"
 
 using namespace dtd;
 
@@ -682,6 +534,7 @@ void dtd_destroy(dtd_ctx *ctx) {
     (void)hipSetDevice(ctx->opts.device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_ws(ctx->ws);
+    free_batch(ctx);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->flag_ev)
@@ -775,6 +628,17 @@ int dtd_solve_rel_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x
         HIPCHK(hipMemcpyAsync(h_dev, ws.x, n * sizeof(double), hipMemcpyDeviceToDevice,
                               ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int dtd_solve_batch(dtd_ctx *ctx, size_t n, const dtd_grid *grids, double rel_tol,
+                    size_t *iterations, double *rel_residuals, dtd_batch_stats *stats) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (stats) *stats = dtd_batch_stats{};
+        if (n == 0) return;
+        if (!grids) throw Error{DTD_EINVAL, "null grid array"};
+        solve_batch(ctx, n, grids, rel_tol, iterations, rel_residuals, stats);
     });
 }
 

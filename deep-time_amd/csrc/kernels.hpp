@@ -1,4 +1,5 @@
-// Device kernels of the diffusion solve (gfx950, wave64).  Included once, by dtdiff.hip.
+// Device kernels of the diffusion solve (gfx950, wave64).  Included once, by dtdiff.hip (through
+// context.hpp and batch.hpp as well).
 //
 // Reference files (relative to the deep-time repository root):
 //   D  crates/deeptime-core/src/surface/diffusion.rs
@@ -37,7 +38,8 @@ struct CgState {
     int done;       // the loop has stopped (1 converged, 2 p.A p, 3 r.z, 4 cap)
     int its;        // iterations started
     int max_iter;
-    int pad[3];
+    int skip;       // a batched solve leaves this grid alone (its answer is final)
+    int pad[2];
 };
 
 // ---- reductions ----------------------------------------------------------------------------
@@ -102,11 +104,10 @@ __device__ inline double face(double ka, double kb, double dt, double s2) {
 // stored (it is re-formed the same way wherever it is used).  The finest level of the cycle goes
 // out as one record a cell, (east, south, diag, omega / diag): in f32 (coef32, the reference's
 // cycle) or in f64 (coef64, the f64 cycle).
-__global__ void k_operator(Grid g, const double *__restrict__ kd, double dt, double s2,
-                           double2 *__restrict__ es, float *__restrict__ coef32,
-                           double *__restrict__ coef64, float omega) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= g.n) return;
+__device__ __forceinline__ void operator_cell(const Grid &g, const double *__restrict__ kd,
+                                              double dt, double s2, double2 *__restrict__ es,
+                                              float *__restrict__ coef32,
+                                              double *__restrict__ coef64, float omega, int c) {
     const int j = c / g.w, i = c - j * g.w, c0 = j * g.w;
     const double k = kd[c];
     double e;
@@ -137,6 +138,13 @@ __global__ void k_operator(Grid g, const double *__restrict__ kd, double dt, dou
         q[0] = make_double2(e, s);
         q[1] = make_double2(d, (double)omega / d);
     }
+}
+
+__global__ void k_operator(Grid g, const double *__restrict__ kd, double dt, double s2,
+                           double2 *__restrict__ es, float *__restrict__ coef32,
+                           double *__restrict__ coef64, float omega) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c < g.n) operator_cell(g, kd, dt, s2, es, coef32, coef64, omega, c);
 }
 
 // The f64 diagonal of cell c from the packed faces (the sum Operator::new takes).
@@ -195,16 +203,13 @@ __global__ void k_unpack_es(Grid g, const double2 *__restrict__ es, double *__re
 // Second pass of an iteration (MG:703-723): r -= alpha A p, its copy in the cycle's precision,
 // per block r*r.
 template <typename T>
-__global__ void k_update(int n, const double *__restrict__ ap, double *__restrict__ r,
-                         T *__restrict__ rc, double *__restrict__ parts,
-                         const CgState *__restrict__ st) {
-    __shared__ double lds[kBlock / 64];
-    if (st->done) return;
-    const double alpha = st->alpha;
+__device__ __forceinline__ double update_block(int n, const double *__restrict__ ap,
+                                               double *__restrict__ r, T *__restrict__ rc,
+                                               double alpha, int blk) {
     double acc = 0.0;
 #pragma unroll
     for (int q = 0; q < kCgPer; q++) {
-        const int c = blockIdx.x * kCgCells + q * kBlock + threadIdx.x;
+        const int c = blk * kCgCells + q * kBlock + threadIdx.x;
         if (c < n) {
             const double rv = r[c] - alpha * ap[c];
             r[c] = rv;
@@ -212,6 +217,16 @@ __global__ void k_update(int n, const double *__restrict__ ap, double *__restric
             acc += rv * rv;
         }
     }
+    return acc;
+}
+
+template <typename T>
+__global__ void k_update(int n, const double *__restrict__ ap, double *__restrict__ r,
+                         T *__restrict__ rc, double *__restrict__ parts,
+                         const CgState *__restrict__ st) {
+    __shared__ double lds[kBlock / 64];
+    if (st->done) return;
+    const double acc = update_block<T>(n, ap, r, rc, st->alpha, blockIdx.x);
     const double s = block_sum<kBlock>(acc, lds);
     if (threadIdx.x == 0) parts[blockIdx.x] = s;
 }
@@ -219,16 +234,9 @@ __global__ void k_update(int n, const double *__restrict__ ap, double *__restric
 // The scalar steps between the passes, one block each (MG:624-730).
 enum Step { kStepRR0 = 0, kStepPAP = 1, kStepRR = 2, kStepRZ = 3, kStepRZ0 = 4 };
 
+// The step itself, from the pass's sum v (and b.b for the first residual); one thread.
 template <int STEP>
-__global__ void k_cg_step(const double *__restrict__ parts, int nparts, CgState *st) {
-    __shared__ double lds[kReduceBlock / 64];
-    if (STEP != kStepRR0 && st->done) return;
-    // The first residual's pass leaves (r.r, b.b) per block; the others one sum.
-    const int stride = STEP == kStepRR0 ? 2 : 1;
-    const double v = reduce_parts(parts, nparts, stride, 0, lds);
-    double bb = 0.0;
-    if (STEP == kStepRR0) bb = reduce_parts(parts, nparts, 2, 1, lds);
-    if (threadIdx.x != 0) return;
+__device__ inline void cg_step_apply(double v, double bb, CgState *st) {
     if (STEP == kStepRR0) {
         // A relative target sets tol2 = rel^2 b.b here; then stop before the loop when the
         // first guess already meets the tolerance (MG:624-626).
@@ -267,6 +275,25 @@ __global__ void k_cg_step(const double *__restrict__ parts, int nparts, CgState 
         st->rz = v;
         if (st->its >= st->max_iter) st->done = 4;
     }
+}
+
+// The step's reduction over one block: the first residual's pass leaves (r.r, b.b) per block,
+// the others one sum.
+template <int STEP>
+__device__ inline void cg_step_block(const double *__restrict__ parts, int nparts, CgState *st,
+                                     double *lds) {
+    if (STEP != kStepRR0 && st->done) return;
+    const int stride = STEP == kStepRR0 ? 2 : 1;
+    const double v = reduce_parts(parts, nparts, stride, 0, lds);
+    double bb = 0.0;
+    if (STEP == kStepRR0) bb = reduce_parts(parts, nparts, 2, 1, lds);
+    if (threadIdx.x == 0) cg_step_apply<STEP>(v, bb, st);
+}
+
+template <int STEP>
+__global__ void k_cg_step(const double *__restrict__ parts, int nparts, CgState *st) {
+    __shared__ double lds[kReduceBlock / 64];
+    cg_step_block<STEP>(parts, nparts, st, lds);
 }
 
 // The put-back's two sums (MG:737-755), then its step k = lost / weight.
@@ -361,9 +388,8 @@ __device__ inline T coarse_south(const Lvl<T> &f, int hc, T scale, int ic, int j
 // omega / diag (MG:189-198); the west and north faces are formed by the same sums as the
 // neighbours' own, so every value equals the reference's.
 template <typename T>
-__global__ void k_coarsen(Lvl<T> f, Lvl<T> c, T scale, T omega) {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= c.n) return;
+__device__ __forceinline__ void coarsen_cell(const Lvl<T> &f, const Lvl<T> &c, T scale, T omega,
+                                             int k) {
     const int jc = k / c.w, ic = k - jc * c.w;
     const T m = coarse_mass(f, ic, jc);
     const T e = coarse_east(f, c.w, c.wrap, scale, ic, jc);
@@ -375,6 +401,12 @@ __global__ void k_coarsen(Lvl<T> f, Lvl<T> c, T scale, T omega) {
     c.mass[k] = m;
     T *q = c.coef + 4 * (size_t)k;
     q[0] = e, q[1] = s, q[2] = d, q[3] = omega / d;
+}
+
+template <typename T>
+__global__ void k_coarsen(Lvl<T> f, Lvl<T> c, T scale, T omega) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k < c.n) coarsen_cell(f, c, scale, omega, k);
 }
 
 // One cell of A x on a level in the order of `Level::apply_row` (MG:203-240): diagonal, east
@@ -454,12 +486,11 @@ __global__ void k_restrict(Lvl<T> f, const T *__restrict__ z, Lvl<T> c,
 // The coarsest level (MG:502-510): z = wd r, then COARSE_SWEEPS - 1 sweeps, in one wave (the
 // level has at most 64 cells).  With dot (the coarsest level is the finest), sum dot * z in the
 // reference's order: per strip of 8 rows, then the strips in order.
+// One wave of 64 threads; zs is 2 x 64 values of LDS; with dot, *part gets the sum.
 template <typename T>
-__global__ void k_coarsest(Lvl<T> L, T *__restrict__ zout, int sweeps,
-                           const double *__restrict__ dot, double *__restrict__ parts,
-                           const CgState *__restrict__ st) {
-    __shared__ T zs[2][64];
-    if (st && st->done) return;
+__device__ __forceinline__ void coarsest_wave(const Lvl<T> &L, T *__restrict__ zout, int sweeps,
+                                              const double *__restrict__ dot, double *part,
+                                              T (*zs)[64]) {
     const int c = threadIdx.x;
     const bool live = c < L.n;
     int j = 0, i = 0;
@@ -488,8 +519,17 @@ __global__ void k_coarsest(Lvl<T> L, T *__restrict__ zout, int sweeps,
             for (int q = s0; q < s1; q++) a += dot[q] * (double)zs[cur][q];
             total += a;
         }
-        parts[0] = total;
+        *part = total;
     }
+}
+
+template <typename T>
+__global__ void k_coarsest(Lvl<T> L, T *__restrict__ zout, int sweeps,
+                           const double *__restrict__ dot, double *__restrict__ parts,
+                           const CgState *__restrict__ st) {
+    __shared__ T zs[2][64];
+    if (st && st->done) return;
+    coarsest_wave<T>(L, zout, sweeps, dot, parts, zs);
 }
 
 // ---- fused, register-streaming smoothing of a level (nu = 2) -------------------------------
@@ -654,12 +694,12 @@ struct Band {
     int active;  // the wave has a band
 };
 
+// The band of wave `wave` of a level (waves numbered across the launch's blocks for one grid).
 template <int V, int H = LaneGeom<V>::halo>
-__device__ inline Band band_of(int w, int h, int wrap, int th) {
+__device__ inline Band band_at(int w, int h, int wrap, int th, int wave) {
     const int lane = threadIdx.x & 63;
     constexpr int OWN = LaneGeom<V, H>::own;
     const int nbx = bands_x<V, H>(w), nby = (h + th - 1) / th;
-    const int wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     Band b;
     b.rows = th;
     b.active = wave < nbx * nby;
@@ -679,6 +719,14 @@ __device__ inline Band band_of(int w, int h, int wrap, int th) {
     return b;
 }
 
+// The wave number of this thread's wave when a grid's waves start at block `block`.
+__device__ inline int block_wave(int block) { return block * (kBlock / 64) + (threadIdx.x >> 6); }
+
+template <int V, int H = LaneGeom<V>::halo>
+__device__ inline Band band_of(int w, int h, int wrap, int th) {
+    return band_at<V, H>(w, h, wrap, th, block_wave(blockIdx.x));
+}
+
 template <typename T, int V>
 __device__ inline void zero(T (&a)[V]) {
 #pragma unroll
@@ -694,12 +742,11 @@ __device__ inline void copy(T (&d)[V], const T (&s)[V]) {
 // residual C.r = sum over each 2x2 block of (r - A z2): fine row j0 then j0 + 1, columns
 // ascending, from +0 (MG:332-356).  Rows are loaded kAhead steps before they are consumed.
 template <typename T, int V>
-__global__ __launch_bounds__(kBlock) void k_spre(Lvl<T> L, T *__restrict__ zout, Lvl<T> C, int th,
-                                                 const CgState *__restrict__ st) {
+__device__ __forceinline__ void spre_band(const Lvl<T> &L, T *__restrict__ zout, const Lvl<T> &C,
+                                          int th, int wave) {
     constexpr int A = 4;  // rows in flight
-    if (st && st->done) return;
     const int w = L.w, h = L.h, wrap = L.wrap;
-    const Band bd = band_of<V>(w, h, wrap, th);
+    const Band bd = band_at<V>(w, h, wrap, th, wave);
     const int gi0 = bd.gi0;
     T qr[A][V], qwd[A][V], qdg[A][V], qe[A][V], qs[A][V];
 #define DTD_FETCH_PRE(j, u)                                                                   \
@@ -769,20 +816,24 @@ __global__ __launch_bounds__(kBlock) void k_spre(Lvl<T> L, T *__restrict__ zout,
 #undef DTD_FETCH_PRE
 }
 
+template <typename T, int V>
+__global__ __launch_bounds__(kBlock) void k_spre(Lvl<T> L, T *__restrict__ zout, Lvl<T> C, int th,
+                                                 const CgState *__restrict__ st) {
+    if (st && st->done) return;
+    spre_band<T, V>(L, zout, C, th, block_wave(blockIdx.x));
+}
+
 // Post-smoothing of level L (nu = 2): z' = zin + zc[j/2][i/2], z3 = z' + wd (r - A z'),
 // z4 = z3 + wd (r - A z3) -> zout; with dot, per block sum dot * z4 into parts[block].
+// Returns the lane's share of sum dot * z4 (0 without dot).
 template <typename T, int V>
-__global__ __launch_bounds__(kBlock) void k_spost(Lvl<T> L, const T *__restrict__ zin,
-                                                  const T *__restrict__ zc, int wc,
-                                                  T *__restrict__ zout, int th,
-                                                  const double *__restrict__ dot,
-                                                  double *__restrict__ parts,
-                                                  const CgState *__restrict__ st) {
+__device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restrict__ zin,
+                                             const T *__restrict__ zc, int wc,
+                                             T *__restrict__ zout, int th,
+                                             const double *__restrict__ dot, int wave) {
     constexpr int A = 4;
-    __shared__ double lds[kBlock / 64];
-    if (st && st->done) return;
     const int w = L.w, h = L.h, wrap = L.wrap;
-    const Band bd = band_of<V>(w, h, wrap, th);
+    const Band bd = band_at<V>(w, h, wrap, th, wave);
     const int gi0 = bd.gi0;
     T qr[A][V], qwd[A][V], qdg[A][V], qe[A][V], qs[A][V], qz[A][V], qc[A][V];
     // z' = z + zc is formed when the row is consumed.
@@ -840,6 +891,19 @@ __global__ __launch_bounds__(kBlock) void k_spost(Lvl<T> L, const T *__restrict_
         }
     }
 #undef DTD_FETCH_POST
+    return acc;
+}
+
+template <typename T, int V>
+__global__ __launch_bounds__(kBlock) void k_spost(Lvl<T> L, const T *__restrict__ zin,
+                                                  const T *__restrict__ zc, int wc,
+                                                  T *__restrict__ zout, int th,
+                                                  const double *__restrict__ dot,
+                                                  double *__restrict__ parts,
+                                                  const CgState *__restrict__ st) {
+    __shared__ double lds[kBlock / 64];
+    if (st && st->done) return;
+    const double acc = spost_band<T, V>(L, zin, zc, wc, zout, th, dot, block_wave(blockIdx.x));
     if (dot) {
         const double sum = block_sum<kBlock>(acc, lds);
         if (threadIdx.x == 0) parts[blockIdx.x] = sum;
@@ -893,21 +957,19 @@ __global__ __launch_bounds__(kBlock) void k_bandcopy(Lvl<float> L, float *__rest
 // owned cells; and A p of the row above with its p . A p.  The f64 couplings come packed as
 // (east, south) pairs.
 
+// Returns the lane's share of p . A p.
 template <typename TZ>
-__global__ __launch_bounds__(kBlock) void k_sdir(Grid g, const double2 *__restrict__ es,
-                                                 const TZ *__restrict__ z,
-                                                 const double *__restrict__ pold,
-                                                 double *__restrict__ pnew, double *__restrict__ ap,
-                                                 double *__restrict__ x, int th,
-                                                 double *__restrict__ parts,
-                                                 const CgState *__restrict__ st) {
+__device__ __forceinline__ double sdir_band(const Grid &g, const double2 *__restrict__ es,
+                                            const TZ *__restrict__ z,
+                                            const double *__restrict__ pold,
+                                            double *__restrict__ pnew, double *__restrict__ ap,
+                                            double *__restrict__ x, int th,
+                                            const CgState *__restrict__ st, int wave) {
     constexpr int A = 4;
-    __shared__ double lds[kBlock / 64];
-    if (st->done) return;
     const int have_beta = st->have_beta, pending = st->have_pending;
     const double beta = st->beta, alpha = st->alpha;
     const int w = g.w, h = g.h, wrap = g.wrap;
-    const Band bd = band_of<1, 1>(w, h, wrap, th);
+    const Band bd = band_at<1, 1>(w, h, wrap, th, wave);
     const int gi = bd.gi0;
     const bool inner_col = bd.col_ok && gi >= 1 && gi + 1 <= w - 1;
     // Raw loads only: p, and the owned cells' x update, are formed when the row is consumed, so
@@ -975,6 +1037,20 @@ __global__ __launch_bounds__(kBlock) void k_sdir(Grid g, const double2 *__restri
         }
     }
 #undef DTD_FETCH_DIR
+    return acc;
+}
+
+template <typename TZ>
+__global__ __launch_bounds__(kBlock) void k_sdir(Grid g, const double2 *__restrict__ es,
+                                                 const TZ *__restrict__ z,
+                                                 const double *__restrict__ pold,
+                                                 double *__restrict__ pnew, double *__restrict__ ap,
+                                                 double *__restrict__ x, int th,
+                                                 double *__restrict__ parts,
+                                                 const CgState *__restrict__ st) {
+    __shared__ double lds[kBlock / 64];
+    if (st->done) return;
+    const double acc = sdir_band<TZ>(g, es, z, pold, pnew, ap, x, th, st, block_wave(blockIdx.x));
     const double sum = block_sum<kBlock>(acc, lds);
     if (threadIdx.x == 0) parts[blockIdx.x] = sum;
 }
@@ -1018,17 +1094,16 @@ __device__ inline double lane_diag64(double e, double ew, double s, double sn, i
 // The first residual (MG:596-619): r = b - A x, its copy in the cycle's precision, per block
 // (r.r, b.b).  x is read from xin; when xout differs (the first guess is b itself) the owned
 // cells of x are written there.
+// Returns the lane's shares of (r.r, b.b).
 template <typename TR>
-__global__ __launch_bounds__(kBlock) void k_sresid(Grid g, const double2 *__restrict__ es,
-                                                   const double *__restrict__ b,
-                                                   const double *__restrict__ xin,
-                                                   double *__restrict__ xout,
-                                                   double *__restrict__ r, TR *__restrict__ rc,
-                                                   int th, double *__restrict__ parts) {
+__device__ __forceinline__ double2 sresid_band(const Grid &g, const double2 *__restrict__ es,
+                                               const double *__restrict__ b,
+                                               const double *__restrict__ xin,
+                                               double *__restrict__ xout, double *__restrict__ r,
+                                               TR *__restrict__ rc, int th, int wave) {
     constexpr int A = 4;
-    __shared__ double lds[kBlock / 64];
     const int w = g.w, h = g.h, wrap = g.wrap;
-    const Band bd = band_of<1, 1>(w, h, wrap, th);
+    const Band bd = band_at<1, 1>(w, h, wrap, th, wave);
     const int gi = bd.gi0;
     const bool inner_col = bd.col_ok && gi >= 1 && gi + 1 <= w - 1;
     const bool copy = xout != xin;
@@ -1072,29 +1147,44 @@ __global__ __launch_bounds__(kBlock) void k_sresid(Grid g, const double2 *__rest
         }
     }
 #undef DTD_FETCH_RES
-    const double s1 = block_sum<kBlock>(acc_r, lds);
-    const double s2 = block_sum<kBlock>(acc_b, lds);
-    if (threadIdx.x == 0) parts[2 * blockIdx.x] = s1, parts[2 * blockIdx.x + 1] = s2;
+    return make_double2(acc_r, acc_b);
+}
+
+// Per block, the pair (sum of v.x, sum of v.y) into parts[2 * slot], parts[2 * slot + 1].
+__device__ inline void block_pair(double2 v, double *parts, int slot, double *lds) {
+    const double s1 = block_sum<kBlock>(v.x, lds);
+    const double s2 = block_sum<kBlock>(v.y, lds);
+    if (threadIdx.x == 0) parts[2 * slot] = s1, parts[2 * slot + 1] = s2;
+}
+
+template <typename TR>
+__global__ __launch_bounds__(kBlock) void k_sresid(Grid g, const double2 *__restrict__ es,
+                                                   const double *__restrict__ b,
+                                                   const double *__restrict__ xin,
+                                                   double *__restrict__ xout,
+                                                   double *__restrict__ r, TR *__restrict__ rc,
+                                                   int th, double *__restrict__ parts) {
+    __shared__ double lds[kBlock / 64];
+    block_pair(sresid_band<TR>(g, es, b, xin, xout, r, rc, th, block_wave(blockIdx.x)), parts,
+               blockIdx.x, lds);
 }
 
 // After the loop: the last update x' = x + alpha p (MG:734-736; p of the last iteration, in
 // buffer its % 2 ... p0 for odd its), written to xout, and the put-back's sums over x'
 // (MG:737-755): per block (sum (b - A x'), sum (diag - 1)).
-__global__ __launch_bounds__(kBlock) void k_finish1(Grid g, const double2 *__restrict__ es,
-                                                    const double *__restrict__ b,
-                                                    const double *__restrict__ x,
-                                                    const double *__restrict__ p0,
-                                                    const double *__restrict__ p1,
-                                                    double *__restrict__ xout, int th,
-                                                    double *__restrict__ parts,
-                                                    const CgState *__restrict__ st) {
+__device__ __forceinline__ double2 finish1_band(const Grid &g, const double2 *__restrict__ es,
+                                                const double *__restrict__ b,
+                                                const double *__restrict__ x,
+                                                const double *__restrict__ p0,
+                                                const double *__restrict__ p1,
+                                                double *__restrict__ xout, int th,
+                                                const CgState *__restrict__ st, int wave) {
     constexpr int A = 4;
-    __shared__ double lds[kBlock / 64];
     const int pending = st->have_pending;
     const double alpha = st->alpha;
     const double *p = (st->its & 1) ? p0 : p1;
     const int w = g.w, h = g.h, wrap = g.wrap;
-    const Band bd = band_of<1, 1>(w, h, wrap, th);
+    const Band bd = band_at<1, 1>(w, h, wrap, th, wave);
     const int gi = bd.gi0;
     const bool inner_col = bd.col_ok && gi >= 1 && gi + 1 <= w - 1;
     double qx[A], qp[A], qe[A], qs[A], qb[A];
@@ -1135,26 +1225,35 @@ __global__ __launch_bounds__(kBlock) void k_finish1(Grid g, const double2 *__res
         }
     }
 #undef DTD_FETCH_FIN
-    const double s1 = block_sum<kBlock>(acc_l, lds);
-    const double s2 = block_sum<kBlock>(acc_w, lds);
-    if (threadIdx.x == 0) parts[2 * blockIdx.x] = s1, parts[2 * blockIdx.x + 1] = s2;
+    return make_double2(acc_l, acc_w);
+}
+
+__global__ __launch_bounds__(kBlock) void k_finish1(Grid g, const double2 *__restrict__ es,
+                                                    const double *__restrict__ b,
+                                                    const double *__restrict__ x,
+                                                    const double *__restrict__ p0,
+                                                    const double *__restrict__ p1,
+                                                    double *__restrict__ xout, int th,
+                                                    double *__restrict__ parts,
+                                                    const CgState *__restrict__ st) {
+    __shared__ double lds[kBlock / 64];
+    block_pair(finish1_band(g, es, b, x, p0, p1, xout, th, st, block_wave(blockIdx.x)), parts,
+               blockIdx.x, lds);
 }
 
 // The put-back x'' = x' + (lost / weight) (diag - 1) when weight > 0 and lost != 0 (MG:756-761),
 // written to xout, and the true residual of x'': per block (sum (b - A x'')^2, sum b^2).
-__global__ __launch_bounds__(kBlock) void k_finish2(Grid g, const double2 *__restrict__ es,
-                                                    const double *__restrict__ b,
-                                                    const double *__restrict__ xp,
-                                                    double *__restrict__ xout, int th,
-                                                    double *__restrict__ parts,
-                                                    const CgState *__restrict__ st) {
+__device__ __forceinline__ double2 finish2_band(const Grid &g, const double2 *__restrict__ es,
+                                                const double *__restrict__ b,
+                                                const double *__restrict__ xp,
+                                                double *__restrict__ xout, int th,
+                                                const CgState *__restrict__ st, int wave) {
     constexpr int A = 4;
-    __shared__ double lds[kBlock / 64];
     const double lost = st->lost, weight = st->weight;
     const bool apply = weight > 0.0 && lost != 0.0;
     const double kk = apply ? lost / weight : 0.0;
     const int w = g.w, h = g.h, wrap = g.wrap;
-    const Band bd = band_of<1, 1>(w, h, wrap, th);
+    const Band bd = band_at<1, 1>(w, h, wrap, th, wave);
     const int gi = bd.gi0;
     const bool inner_col = bd.col_ok && gi >= 1 && gi + 1 <= w - 1;
     double qx[A], qe[A], qs[A], qb[A];
@@ -1200,9 +1299,18 @@ __global__ __launch_bounds__(kBlock) void k_finish2(Grid g, const double2 *__res
         }
     }
 #undef DTD_FETCH_FIN2
-    const double s1 = block_sum<kBlock>(acc_r, lds);
-    const double s2 = block_sum<kBlock>(acc_b, lds);
-    if (threadIdx.x == 0) parts[2 * blockIdx.x] = s1, parts[2 * blockIdx.x + 1] = s2;
+    return make_double2(acc_r, acc_b);
+}
+
+__global__ __launch_bounds__(kBlock) void k_finish2(Grid g, const double2 *__restrict__ es,
+                                                    const double *__restrict__ b,
+                                                    const double *__restrict__ xp,
+                                                    double *__restrict__ xout, int th,
+                                                    double *__restrict__ parts,
+                                                    const CgState *__restrict__ st) {
+    __shared__ double lds[kBlock / 64];
+    block_pair(finish2_band(g, es, b, xp, xout, th, st, block_wave(blockIdx.x)), parts, blockIdx.x,
+               lds);
 }
 
 // ---- the first guess (D:370-489) -----------------------------------------------------------
@@ -1212,11 +1320,10 @@ __global__ __launch_bounds__(kBlock) void k_finish2(Grid g, const double2 *__res
 // as they are used.  Both Thomas solves of `cyclic` share their elimination (they factor the
 // same matrix), so they run in one pass; each value is the one the separate solves form.
 // cbuf and zbuf are w*h scratch arrays.
-__global__ void k_lod_rows(Grid g, int wrap_x, const double *__restrict__ kd, double dt, double s2,
-                           double *__restrict__ x, double *__restrict__ cbuf,
-                           double *__restrict__ zbuf) {
-    const int j = blockIdx.x * kBlock + threadIdx.x;
-    if (j >= g.h) return;
+__device__ __forceinline__ void lod_row(const Grid &g, int wrap_x, const double *__restrict__ kd,
+                                        double dt, double s2, double *__restrict__ x,
+                                        double *__restrict__ cbuf, double *__restrict__ zbuf,
+                                        int j) {
     const int w = g.w;
     const double *k = kd + (size_t)j * w;
     double *b = x + (size_t)j * w;
@@ -1301,12 +1408,18 @@ __global__ void k_lod_rows(Grid g, int wrap_x, const double *__restrict__ kd, do
     for (int i = w - 2; i >= 0; i--) b[i] -= cc[i + 1] * b[i + 1];
 }
 
+__global__ void k_lod_rows(Grid g, int wrap_x, const double *__restrict__ kd, double dt, double s2,
+                           double *__restrict__ x, double *__restrict__ cbuf,
+                           double *__restrict__ zbuf) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j < g.h) lod_row(g, wrap_x, kd, dt, s2, x, cbuf, zbuf, j);
+}
+
 // The z sweep, one thread per column (coalesced across the wave): `thomas` down the column with
 // north and south faces (D:436-488).
-__global__ void k_lod_cols(Grid g, const double *__restrict__ kd, double dt, double s2,
-                           double *__restrict__ x, double *__restrict__ cbuf) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= g.w) return;
+__device__ __forceinline__ void lod_col(const Grid &g, const double *__restrict__ kd, double dt,
+                                        double s2, double *__restrict__ x,
+                                        double *__restrict__ cbuf, int i) {
     const int w = g.w, h = g.h;
     auto fz = [&](int j) -> double {
         return j + 1 < h ? face(kd[(size_t)j * w + i], kd[(size_t)(j + 1) * w + i], dt, s2) : 0.0;
@@ -1334,6 +1447,12 @@ __global__ void k_lod_cols(Grid g, const double *__restrict__ kd, double dt, dou
         const size_t cidx = (size_t)j * w + i;
         x[cidx] -= cbuf[cidx + w] * x[cidx + w];
     }
+}
+
+__global__ void k_lod_cols(Grid g, const double *__restrict__ kd, double dt, double s2,
+                           double *__restrict__ x, double *__restrict__ cbuf) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < g.w) lod_col(g, kd, dt, s2, x, cbuf, i);
 }
 
 // ---- the first guess, parallel (lines of 16 * 2^k cells) ----------------------------------

@@ -39,7 +39,7 @@ SYMBOLS = (
     "dtd_default_opts", "dtd_create", "dtd_destroy", "dtd_last_error", "dtd_solve",
     "dtd_solve_device", "dtd_solve_rel_device", "dtd_operator", "dtd_apply", "dtd_lod",
     "dtd_cycle_new", "dtd_cycle_levels", "dtd_cycle_level", "dtd_cycle_precondition",
-    "dtd_time_kernel",
+    "dtd_time_kernel", "dtd_solve_batch",
 )
 KERNEL_SWEEP, KERNEL_DIRECTION, KERNEL_UPDATE, KERNEL_RESTRICT = 0, 1, 2, 3
 KERNEL_PRE, KERNEL_POST, KERNEL_COPY, KERNEL_STREAM5, KERNEL_BANDCOPY = 4, 5, 6, 7, 8
@@ -75,6 +75,24 @@ class Stats(ctypes.Structure):
                 ("rel_residual", ctypes.c_double), ("ms_setup", ctypes.c_double),
                 ("ms_guess", ctypes.c_double), ("ms_cg", ctypes.c_double),
                 ("ms_total", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Grid(ctypes.Structure):
+    """One grid of a batched solve (dtd_grid): dtd_solve_device's arguments."""
+    _fields_ = [("w", ctypes.c_size_t), ("h", ctypes.c_size_t), ("dx", ctypes.c_double),
+                ("wrap_x", ctypes.c_int), ("dt", ctypes.c_double), ("tol_m", ctypes.c_double),
+                ("h_dev", ctypes.c_void_p), ("kd_dev", ctypes.c_void_p)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("grids", ctypes.c_size_t), ("cells", ctypes.c_size_t),
+                ("iterations_max", ctypes.c_size_t), ("iterations_sum", ctypes.c_size_t),
+                ("restarts", ctypes.c_size_t), ("rel_residual_max", ctypes.c_double),
+                ("ms_setup", ctypes.c_double), ("ms_guess", ctypes.c_double),
+                ("ms_cg", ctypes.c_double), ("ms_total", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -128,6 +146,8 @@ def load(path: str = LIB_PATH):
                                   ctypes.POINTER(ctypes.c_int), _fp, _fp, _fp, _fp, _fp]
     L.dtd_cycle_precondition.argtypes = [_vp, _fp, _dp, _fp, _dp]
     L.dtd_time_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+    L.dtd_solve_batch.argtypes = [_vp, _sz, ctypes.POINTER(Grid), ctypes.c_double,
+                                  ctypes.POINTER(_sz), _dp, ctypes.POINTER(BatchStats)]
     for name in ("dtd_solve", "dtd_solve_device", "dtd_solve_rel_device", "dtd_operator",
                  "dtd_apply", "dtd_lod", "dtd_cycle_new", "dtd_cycle_level",
                  "dtd_cycle_precondition", "dtd_time_kernel"):
@@ -219,6 +239,22 @@ def solve_rel_device(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float, rel_tol: 
     ws.check(load().dtd_solve_rel_device(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x), _ptr(h_dev),
                                          _ptr(kd_dev), dt, rel_tol, ctypes.byref(st)))
     return st
+
+
+def solve_batch(ws: Workspace, items, rel_tol: float = 0.0):
+    """Many grids at once (dtd_solve_batch).  items: (Dims, h_dev, kd_dev, dt, tol_m) per grid,
+    device arrays solved in place.  rel_tol > 0 takes every grid to that true relative residual
+    (tol_m ignored).  Returns (iterations per grid, relative residual per grid, BatchStats)."""
+    n = len(items)
+    arr = (Grid * max(n, 1))()
+    for k, (d, h_dev, kd_dev, dt, tol_m) in enumerate(items):
+        arr[k] = Grid(d.w, d.h, d.dx, int(d.wrap_x), dt, tol_m, _ptr(h_dev).value,
+                      _ptr(kd_dev).value)
+    its = (ctypes.c_size_t * max(n, 1))()
+    rels = (ctypes.c_double * max(n, 1))()
+    st = BatchStats()
+    ws.check(load().dtd_solve_batch(ws._ctx, n, arr, rel_tol, its, rels, ctypes.byref(st)))
+    return list(its)[:n], list(rels)[:n], st
 
 
 def operator(ws: Workspace, d: Dims, kd, dt):

@@ -93,9 +93,24 @@ def case_lognormal(seed: int = 2, w: int = 8192, h: int = 4096, ratio: float = 5
     return d, spectral_noise(w, h, seed), kd, ratio / float(kd.mean())
 
 
+def case_regional(seed: int = 4, count: int = 512):
+    """Independent regional grids: w uniform in {64, 96, ..., 1024}, h in {64, 96, ..., 512}, each
+    periodic in x with its own log-normal K (sigma 8 cells) and k^-2 h0, dt mean(K) / dx^2 uniform
+    in [5, 200].  Returns a list of (Dims, h0, kd, dt)."""
+    u = splitmix_u64(seed, 3 * count)
+    out = []
+    for k in range(count):
+        w = 64 + 32 * int(u[3 * k] % np.uint64(31))
+        h = 64 + 32 * int(u[3 * k + 1] % np.uint64(15))
+        ratio = 5.0 + 195.0 * float(u[3 * k + 2] >> np.uint64(11)) * 2.0 ** -53
+        out.append(case_lognormal(seed=1000 * seed + k, w=w, h=h, ratio=ratio))
+    return out
+
+
 # The benchmark's workloads by name: generator and its arguments.
 WORKLOADS = {
     "constant-512x256": (case_constant, dict(seed=1, w=512, h=256)),
     "lognormal-8192x4096": (case_lognormal, dict(seed=2, w=8192, h=4096)),
     "constant-16384x8192": (case_constant, dict(seed=3, w=16384, h=8192)),
+    "regional-512": (case_regional, dict(seed=4, count=512)),
 }

@@ -134,6 +134,39 @@ int dtd_cycle_level(dtd_ctx *ctx, size_t l, size_t *w, size_t *h, int *wrap, flo
 int dtd_cycle_precondition(dtd_ctx *ctx, const float *r32, const double *r64, float *z32,
                            double *rz);
 
+/* One grid of a batched solve: a dtd_solve_device call's arguments. */
+typedef struct dtd_grid {
+    size_t w, h;        /* cells */
+    double dx;          /* metres */
+    int wrap_x;         /* periodic in x */
+    double dt;          /* years */
+    double tol_m;       /* RMS residual target in metres (used when rel_tol <= 0) */
+    double *h_dev;      /* device: h0 on entry, h1 on return */
+    const double *kd_dev; /* device: diffusivity, m^2/yr */
+} dtd_grid;
+
+typedef struct dtd_batch_stats {
+    size_t grids;           /* non-empty grids solved */
+    size_t cells;           /* their cells */
+    size_t iterations_max;  /* most CG iterations of one grid (restarts included) */
+    size_t iterations_sum;
+    size_t restarts;        /* grid restarts of the relative target */
+    double rel_residual_max; /* largest true relative residual after the put-back */
+    double ms_setup, ms_guess, ms_cg, ms_total; /* device time */
+} dtd_batch_stats;
+
+/* Many independent grids solved together, each as dtd_solve_device (rel_tol <= 0: grid k stops
+ * at its tol_m) or dtd_solve_rel_device (rel_tol > 0: every grid to a true relative residual of
+ * rel_tol, with the same restarts) would solve it alone, with one launch per pass and level over
+ * all the grids.  No reference counterpart: the reference solves one grid per call of
+ * `surface::diffusion::solve` (crates/deeptime-core/src/surface/diffusion.rs:77-116); this is
+ * that call over many regional grids at once.  The context must use the f32 cycle with nu = 2.
+ * Empty grids are skipped (0 iterations).  iterations[k] and rel_residuals[k] (either may be
+ * NULL) get grid k's iteration count and true relative residual.  A grid's answer does not depend
+ * on the other grids of the batch.  The device buffers of the last batch shape stay resident. */
+int dtd_solve_batch(dtd_ctx *ctx, size_t n, const dtd_grid *grids, double rel_tol,
+                    size_t *iterations, double *rel_residuals, dtd_batch_stats *stats);
+
 /* Kernels dtd_time_kernel can time on the workspace of the last f32-cycle solve. */
 #define DTD_KERNEL_SWEEP 0      /* one damped-Jacobi sweep of the finest level (f32) */
 #define DTD_KERNEL_DIRECTION 1  /* the conjugate gradients' direction pass (f64) */
