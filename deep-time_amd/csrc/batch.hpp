@@ -74,27 +74,40 @@ __global__ __launch_bounds__(kBlock) void kb_coarsen(const BLevel *__restrict__ 
     if (k < q.C.n) coarsen_cell(q.L, q.C, scale, omega, k);
 }
 
-// x = b (out = 0) or the caller's h = x (out = 1).
-__global__ __launch_bounds__(kBlock) void kb_copy(const BGrid *__restrict__ t,
-                                                  const int2 *__restrict__ blk, int out) {
+// The caller's h = x.
+__global__ __launch_bounds__(kBlock) void kb_copy_out(const BGrid *__restrict__ t,
+                                                      const int2 *__restrict__ blk) {
     const int2 m = blk[blockIdx.x];
     const BGrid &q = t[m.x];
     const int c = m.y * kBlock + threadIdx.x;
-    if (c >= q.g.n) return;
-    if (out)
-        q.h[c] = q.x[c];
-    else
-        q.x[c] = q.b[c];
+    if (c < q.g.n) q.h[c] = q.x[c];
 }
 
-// The first guess's sweeps (`lod`, D:370-489), in place on x: one thread a row, then one a
-// column, the reference's recurrences (p0 and p1 are the scratch lines).
+// The first guess (`lod`, D:370-489) with the reference's recurrences, one thread a line: the
+// rows on the transposed grid, where a wave's 64 rows lie side by side in memory, then the columns
+// in place.  Transposes through 32 x 32 tiles: kd into r (mode 0) and b into ap (mode 1), w x h;
+// then ap back into x (mode 2).
+__global__ __launch_bounds__(kBlock) void kb_transpose(const BGrid *__restrict__ t,
+                                                       const int2 *__restrict__ blk, int mode) {
+    __shared__ double tile[32][33];
+    const int2 m = blk[blockIdx.x];
+    const BGrid &q = t[m.x];
+    const double *src = mode == 0 ? q.kd : (mode == 1 ? q.b : q.ap);
+    double *dst = mode == 0 ? q.r : (mode == 1 ? q.ap : q.x);
+    const int rows = mode == 2 ? q.g.w : q.g.h, cols = mode == 2 ? q.g.h : q.g.w;
+    const int ntc = (cols + 31) / 32;
+    transpose_tile(src, dst, rows, cols, (m.y / ntc) * 32, (m.y % ntc) * 32, tile);
+}
 __global__ __launch_bounds__(kBlock) void kb_lod_rows(const BGrid *__restrict__ t,
                                                       const int2 *__restrict__ blk) {
     const int2 m = blk[blockIdx.x];
     const BGrid &q = t[m.x];
     const int j = m.y * kBlock + threadIdx.x;
-    if (j < q.g.h) lod_row(q.g, q.wrap_x, q.kd, q.dt, q.s2, q.x, q.p0, q.p1, j);
+    const int h = q.g.h;
+    if (j < h)
+        lod_line_x(Line<const double>{q.r + j, h}, Line<double>{q.ap + j, h},
+                   Line<double>{q.p0 + j, h}, Line<double>{q.p1 + j, h}, q.g.w, q.wrap_x, q.dt,
+                   q.s2);
 }
 __global__ __launch_bounds__(kBlock) void kb_lod_cols(const BGrid *__restrict__ t,
                                                       const int2 *__restrict__ blk) {
@@ -277,7 +290,7 @@ struct BatchWs {
     int *flag = nullptr;
     int2 *tables = nullptr;  // every block and range table, one allocation
     // f64 side
-    BLaunch cells_l, update_l, rows_l, cols_l, band_l;
+    BLaunch cells_l, update_l, rows_l, cols_l, tiles_l, band_l;
     const int2 *rng_band = nullptr, *rng_update = nullptr, *rng_rz = nullptr;
     // cycle, per level
     std::vector<BLaunch> coarsen_l, pre2, pre1, post2, post1, coarsest;
@@ -451,6 +464,8 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
     add_launch(bw->rows_l, cnt, nullptr);
     for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].w);
     add_launch(bw->cols_l, cnt, nullptr);
+    for (size_t g = 0; g < n; g++) cnt[g] = (int)(((gs[g].w + 31) / 32) * ((gs[g].h + 31) / 32));
+    add_launch(bw->tiles_l, cnt, nullptr);
     for (size_t g = 0; g < n; g++)
         cnt[g] = stream_grid<1, 1>((int)gs[g].w, (int)gs[g].h, bw->grids[g].th);
     add_launch(bw->band_l, cnt, &rng_band);
@@ -702,9 +717,13 @@ static void solve_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs_in, doubl
     }
     const bool lod_guess = ctx->opts.guess == DTD_GUESS_LOD;
     if (lod_guess) {
-        kb_copy<<<bw.cells_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.cells_l.blk, 0);
-        LAUNCH_CHECK();
+        for (int mode = 0; mode < 2; mode++) {
+            kb_transpose<<<bw.tiles_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.tiles_l.blk, mode);
+            LAUNCH_CHECK();
+        }
         kb_lod_rows<<<bw.rows_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.rows_l.blk);
+        LAUNCH_CHECK();
+        kb_transpose<<<bw.tiles_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.tiles_l.blk, 2);
         LAUNCH_CHECK();
         kb_lod_cols<<<bw.cols_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.cols_l.blk);
         LAUNCH_CHECK();
@@ -748,7 +767,7 @@ static void solve_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs_in, doubl
     }
     HIPCHK(hipEventRecord(ctx->ev[3], s));
     // Every grid's answer into the caller's h.
-    kb_copy<<<bw.cells_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.cells_l.blk, 1);
+    kb_copy_out<<<bw.cells_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.cells_l.blk);
     LAUNCH_CHECK();
     HIPCHK(hipEventRecord(ctx->ev[4], s));
     HIPCHK(hipStreamSynchronize(s));

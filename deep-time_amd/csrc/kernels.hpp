@@ -1320,15 +1320,19 @@ __global__ __launch_bounds__(kBlock) void k_finish2(Grid g, const double2 *__res
 // as they are used.  Both Thomas solves of `cyclic` share their elimination (they factor the
 // same matrix), so they run in one pass; each value is the one the separate solves form.
 // cbuf and zbuf are w*h scratch arrays.
-__device__ __forceinline__ void lod_row(const Grid &g, int wrap_x, const double *__restrict__ kd,
-                                        double dt, double s2, double *__restrict__ x,
-                                        double *__restrict__ cbuf, double *__restrict__ zbuf,
-                                        int j) {
-    const int w = g.w;
-    const double *k = kd + (size_t)j * w;
-    double *b = x + (size_t)j * w;
-    double *cc = cbuf + (size_t)j * w;
-    double *zz = zbuf + (size_t)j * w;
+// One line of a grid array: element i at p[i * s] (s = 1 along a row of the grid, s = h along a
+// row of the transposed grid).
+template <typename T>
+struct Line {
+    T *p;
+    int s;
+    __device__ T &operator[](int i) const { return p[(size_t)i * s]; }
+};
+
+// One row of w cells: kd in k, the right-hand side in b (solved in place), cc and zz scratch.
+__device__ __forceinline__ void lod_line_x(Line<const double> k, Line<double> b, Line<double> cc,
+                                           Line<double> zz, int w, int wrap_x, double dt,
+                                           double s2) {
     const bool ring = wrap_x && w > 1;
     auto fc = [&](int i) -> double {
         if (i + 1 < w) return face(k[i], k[i + 1], dt, s2);
@@ -1408,6 +1412,15 @@ __device__ __forceinline__ void lod_row(const Grid &g, int wrap_x, const double 
     for (int i = w - 2; i >= 0; i--) b[i] -= cc[i + 1] * b[i + 1];
 }
 
+__device__ __forceinline__ void lod_row(const Grid &g, int wrap_x, const double *__restrict__ kd,
+                                        double dt, double s2, double *__restrict__ x,
+                                        double *__restrict__ cbuf, double *__restrict__ zbuf,
+                                        int j) {
+    const size_t o = (size_t)j * g.w;
+    lod_line_x(Line<const double>{kd + o, 1}, Line<double>{x + o, 1}, Line<double>{cbuf + o, 1},
+               Line<double>{zbuf + o, 1}, g.w, wrap_x, dt, s2);
+}
+
 __global__ void k_lod_rows(Grid g, int wrap_x, const double *__restrict__ kd, double dt, double s2,
                            double *__restrict__ x, double *__restrict__ cbuf,
                            double *__restrict__ zbuf) {
@@ -1472,10 +1485,9 @@ __global__ void k_lod_cols(Grid g, const double *__restrict__ kd, double dt, dou
 constexpr int kSeg = 16;
 
 // dst[c * rows + r] = src[r * cols + c], through a 32 x 32 tile in LDS.
-__global__ void k_transpose(const double *__restrict__ src, double *__restrict__ dst, int rows,
-                            int cols) {
-    __shared__ double t[32][33];
-    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+__device__ __forceinline__ void transpose_tile(const double *__restrict__ src,
+                                               double *__restrict__ dst, int rows, int cols,
+                                               int r0, int c0, double (*t)[33]) {
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
     for (int k = ty; k < 32; k += 8) {
         const int r = r0 + k, c = c0 + tx;
@@ -1486,6 +1498,12 @@ __global__ void k_transpose(const double *__restrict__ src, double *__restrict__
         const int c = c0 + k, r = r0 + tx;
         if (r < rows && c < cols) dst[(size_t)c * rows + r] = t[tx][k];
     }
+}
+
+__global__ void k_transpose(const double *__restrict__ src, double *__restrict__ dst, int rows,
+                            int cols) {
+    __shared__ double t[32][33];
+    transpose_tile(src, dst, rows, cols, blockIdx.y * 32, blockIdx.x * 32, t);
 }
 
 // One line per block (n / 16 threads, a power of two), from src to dst (may be the same array);

@@ -102,6 +102,20 @@ def test_batch_tolerance_mode_stops_like_the_oracle(ws):
         assert np.sqrt(np.mean((x - xo) ** 2)) <= 2 * bound, d
 
 
+def test_batch_first_guess_is_the_splitting(ws):
+    """With a tolerance the first guess already meets, no iteration runs and the answer is the
+    splitting's guess plus the put-back, as in the oracle's solve (its put-back sums are taken in
+    strips, so the two differ by rounding only)."""
+    print("batch first guess", flush=True)
+    grids = make(MIXED, seed=61)
+    xs, its, _, _ = run(ws, grids, 0.0, [1e30] * len(grids))
+    assert its == [0] * len(grids)
+    for (d, b, kd, dt), x in zip(grids, xs):
+        xo, its_o = O.solve(d.w, d.h, d.dx, d.wrap_x, b, kd, dt, 1e30)
+        assert its_o == 0
+        assert np.abs(x - xo).max() <= 1e-13 * np.abs(xo).max(), d
+
+
 def test_a_grid_answer_does_not_depend_on_the_batch(ws):
     print("batch independence", flush=True)
     grids = make(MIXED, seed=21)
