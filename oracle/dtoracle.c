@@ -27,6 +27,8 @@
 #define STRIP_ROWS 8
 /* CG iteration cap (D:497). */
 #define CG_MAX_ITER 400
+/* The Jacobi solve's RMS tolerance, metres (D:495). */
+#define CG_TOL_M 1.0e-6
 /* Cycle constants (MG:45-56). */
 #define OMEGA 0.8f
 #define NU 2
@@ -288,9 +290,12 @@ void or_cyclic(size_t n, const double *lo, const double *di, const double *up, d
     free(z);
 }
 
-/* `lod`, D:370-489 with every row visited: an implicit sweep along x (a cyclic tridiagonal
- * system per row when the grid wraps and is wider than two cells), then along the columns. */
-void or_lod(size_t w, size_t h, double dx, int wrap_x, double *hgt, const double *kd, double dt) {
+/* `lod`, D:370-489: an implicit sweep along x (a cyclic tridiagonal system per row when the grid
+ * wraps and is wider than two cells), then along the columns.  row_on (NULL: every row) marks the
+ * visited rows (`Live`): a skipped row is left as it is by the x sweep, and each column is solved
+ * one run of visited rows at a time (D:430-488). */
+static void lod_live(size_t w, size_t h, double dx, int wrap_x, double *hgt, const double *kd,
+                     double dt, const unsigned char *row_on) {
     double s2 = dx * dx;
     if (w == 0 || h == 0) return;
 #pragma omp parallel
@@ -301,6 +306,7 @@ void or_lod(size_t w, size_t h, double dx, int wrap_x, double *hgt, const double
 #pragma omp for schedule(static)
         for (long jl = 0; jl < (long)h; jl++) {
             size_t j = (size_t)jl;
+            if (row_on && !row_on[j]) continue;
             const double *krow = kd + j * w;
             double *row = hgt + j * w;
 #define FACE(i)                                                                                \
@@ -345,23 +351,39 @@ void or_lod(size_t w, size_t h, double dx, int wrap_x, double *hgt, const double
         for (long il = 0; il < (long)w; il++) {
             size_t i = (size_t)il;
 #define FACEZ(j) ((j) + 1 < h ? 0.5 * (kd[(j) * w + i] + kd[((j) + 1) * w + i]) * dt / s2 : 0.0)
-            for (size_t j = 0; j < h; j++) {
-                col[j] = hgt[j * w + i];
-                double north = j == 0 ? 0.0 : FACEZ(j - 1);
-                double south = FACEZ(j);
-                lo[j] = -north;
-                up[j] = -south;
-                di[j] = 1.0 + north + south;
+            /* The runs of visited rows, [a, e), in order. */
+            for (size_t a = 0; a < h;) {
+                if (row_on && !row_on[a]) {
+                    a++;
+                    continue;
+                }
+                size_t e = a + 1;
+                while (e < h && (!row_on || row_on[e])) e++;
+                size_t len = e - a;
+                for (size_t jj = 0; jj < len; jj++) {
+                    size_t j = a + jj;
+                    col[jj] = hgt[j * w + i];
+                    double north = j == 0 ? 0.0 : FACEZ(j - 1);
+                    double south = FACEZ(j);
+                    lo[jj] = -north;
+                    up[jj] = -south;
+                    di[jj] = 1.0 + north + south;
+                }
+                or_thomas(len, lo, di, up, col);
+                for (size_t jj = 0; jj < len; jj++) hgt[(a + jj) * w + i] = col[jj];
+                a = e;
             }
 #undef FACEZ
-            or_thomas(h, lo, di, up, col);
-            for (size_t j = 0; j < h; j++) hgt[j * w + i] = col[j];
         }
         free(col);
         free(lo);
         free(di);
         free(up);
     }
+}
+
+void or_lod(size_t w, size_t h, double dx, int wrap_x, double *hgt, const double *kd, double dt) {
+    lod_live(w, h, dx, wrap_x, hgt, kd, dt, NULL);
 }
 
 /* ---- multigrid cycle (MG:73-523) ----------------------------------------------------------- */
@@ -785,14 +807,20 @@ size_t or_solve(size_t w, size_t h, double dx, int wrap_x, double *hgt, const do
     return its;
 }
 
-/* The Jacobi-preconditioned conjugate gradients `cg`, D:927-1026, every row visited: per-row
- * sums (ascending columns from -0.0), rows added in order; z = r / diag formed where used;
- * x += alpha p deferred to the next direction pass (taken once there). */
-size_t or_jacobi_cg(size_t w, size_t h, double dx, int wrap_x, double *x, const double *b,
-                    const double *kd, double dt, double tol_m) {
+/* The Jacobi-preconditioned conjugate gradients `cg`, D:927-1026, to an RMS residual of tol_m:
+ * per-row sums (ascending columns from -0.0), rows added in order; z = r / diag formed where
+ * used; x += alpha p deferred to the next direction pass (taken once there).  strip_on (NULL:
+ * every strip) marks the visited strips of STRIP_ROWS rows (`Live`): on a skipped strip the
+ * operator keeps +0 faces and diagonal 1 (D:585-600), r, p, A p and the sums are +0 (D:757-772,
+ * 799-807) and x is left alone; the put-back adds one +0 per skipped row (D:1000-1012). */
+static size_t jacobi_cg_live(size_t w, size_t h, double dx, int wrap_x, double *x,
+                             const double *b, const double *kd, double dt, double tol_m,
+                             const unsigned char *strip_on) {
     size_t n = w * h;
     if (n == 0) return 0;
     int wrap = wrap_x && w > 1;
+    size_t nstrips = div_ceil(h, STRIP_ROWS);
+#define ROW_ON(j) (!strip_on || strip_on[(j) / STRIP_ROWS])
     double *east = (double *)malloc(n * sizeof(double));
     double *south = (double *)malloc(n * sizeof(double));
     double *diag = (double *)malloc(n * sizeof(double));
@@ -802,8 +830,23 @@ size_t or_jacobi_cg(size_t w, size_t h, double dx, int wrap_x, double *x, const 
     double *pnew = (double *)calloc(n, sizeof(double));
     double *s0 = (double *)malloc(h * sizeof(double));
     double *s1 = (double *)malloc(h * sizeof(double));
+    double *pr = (double *)malloc(h * sizeof(double));
     or_operator(w, h, dx, wrap_x, kd, dt, east, south, diag);
+    for (size_t j = 0; j < h; j++)
+        if (!ROW_ON(j))
+            for (size_t i = 0; i < w; i++) {
+                east[j * w + i] = 0.0;
+                south[j * w + i] = 0.0;
+                diag[j * w + i] = 1.0;
+            }
+    /* pass_init, D:747-780. */
     for (size_t j = 0; j < h; j++) {
+        if (!ROW_ON(j)) {
+            for (size_t i = 0; i < w; i++) r[j * w + i] = 0.0;
+            s0[j] = 0.0;
+            s1[j] = 0.0;
+            continue;
+        }
         double a = sum_zero(), bsum = sum_zero();
         for (size_t i = 0; i < w; i++) {
             size_t c = j * w + i;
@@ -826,27 +869,44 @@ size_t or_jacobi_cg(size_t w, size_t h, double dx, int wrap_x, double *x, const 
         double beta = 0.0, alpha = 0.0;
         while (iterations < CG_MAX_ITER) {
             iterations++;
-            for (size_t c = 0; c < n; c++) {
-                if (pending) x[c] += alpha * pold[c];
-                pnew[c] = have_beta ? r[c] / diag[c] + beta * pold[c] : r[c] / diag[c];
+            /* pass_direction, D:782-880: the pending x update and p on the visited strips,
+             * +0 on the others; then A p and p.A p per row. */
+            for (size_t s = 0; s < nstrips; s++) {
+                size_t j0 = s * STRIP_ROWS, j1 = j0 + STRIP_ROWS < h ? j0 + STRIP_ROWS : h;
+                for (size_t c = j0 * w; c < j1 * w; c++) {
+                    if (strip_on && !strip_on[s]) {
+                        pnew[c] = 0.0;
+                        continue;
+                    }
+                    if (pending) x[c] += alpha * pold[c];
+                    pnew[c] = have_beta ? r[c] / diag[c] + beta * pold[c] : r[c] / diag[c];
+                }
             }
             pending = 0;
             double pap = sum_zero();
             for (size_t j = 0; j < h; j++) {
-                double a = sum_zero();
-                for (size_t i = 0; i < w; i++) {
-                    size_t c = j * w + i;
-                    ap[c] = op_cell(w, h, wrap, east, south, diag, pnew, j, i);
-                    a += pnew[c] * ap[c];
+                if (!ROW_ON(j)) {
+                    for (size_t i = 0; i < w; i++) ap[j * w + i] = 0.0;
+                    pr[j] = 0.0;
+                } else {
+                    double a = sum_zero();
+                    for (size_t i = 0; i < w; i++) {
+                        size_t c = j * w + i;
+                        ap[c] = op_cell(w, h, wrap, east, south, diag, pnew, j, i);
+                        a += pnew[c] * ap[c];
+                    }
+                    pr[j] = a;
                 }
-                pap += a;
+                pap += pr[j];
             }
             double *tp = pold;
             pold = pnew;
             pnew = tp;
             if (pap <= 0.0) break;
             alpha = rz / pap;
+            /* pass_update, D:883-905: skipped strips keep r and their sums. */
             for (size_t j = 0; j < h; j++) {
+                if (!ROW_ON(j)) continue;
                 double a = sum_zero(), bsum = sum_zero();
                 for (size_t i = 0; i < w; i++) {
                     size_t c = j * w + i;
@@ -867,18 +927,32 @@ size_t or_jacobi_cg(size_t w, size_t h, double dx, int wrap_x, double *x, const 
             have_beta = 1;
             rz = rz_new;
         }
+        /* pass_x, D:907-919. */
         if (pending)
-            for (size_t c = 0; c < n; c++) x[c] += alpha * pold[c];
+            for (size_t c = 0; c < n; c++)
+                if (ROW_ON(c / w)) x[c] += alpha * pold[c];
     }
-    /* Put-back, D:1000-1023: one running sum over the cells in order. */
-    or_apply(w, h, wrap_x, east, south, diag, x, ap);
+    /* Put-back, D:1000-1023: one running sum over the cells in order (a skipped row adds +0). */
     double lost = sum_zero(), weight = sum_zero();
-    for (size_t c = 0; c < n; c++) lost += b[c] - ap[c];
-    for (size_t c = 0; c < n; c++) weight += diag[c] - 1.0;
+    for (size_t j = 0; j < h; j++) {
+        if (!ROW_ON(j)) {
+            lost += 0.0;
+            weight += 0.0;
+            continue;
+        }
+        for (size_t i = 0; i < w; i++) {
+            size_t c = j * w + i;
+            ap[c] = op_cell(w, h, wrap, east, south, diag, x, j, i);
+        }
+        for (size_t i = 0; i < w; i++) lost += b[j * w + i] - ap[j * w + i];
+        for (size_t i = 0; i < w; i++) weight += diag[j * w + i] - 1.0;
+    }
     if (weight > 0.0 && lost != 0.0) {
         double k = lost / weight;
-        for (size_t c = 0; c < n; c++) x[c] += k * (diag[c] - 1.0);
+        for (size_t c = 0; c < n; c++)
+            if (ROW_ON(c / w)) x[c] += k * (diag[c] - 1.0);
     }
+#undef ROW_ON
     free(east);
     free(south);
     free(diag);
@@ -888,7 +962,102 @@ size_t or_jacobi_cg(size_t w, size_t h, double dx, int wrap_x, double *x, const 
     free(pnew);
     free(s0);
     free(s1);
+    free(pr);
     return iterations;
+}
+
+size_t or_jacobi_cg(size_t w, size_t h, double dx, int wrap_x, double *x, const double *b,
+                    const double *kd, double dt, double tol_m) {
+    return jacobi_cg_live(w, h, dx, wrap_x, x, b, kd, dt, tol_m, NULL);
+}
+
+/* `Live::of`, D:308-355: whether a step may skip rows, and which strips it visits.  A row is a
+ * zero row when its kd is +0 throughout (bit pattern 0); a row is skipped when it and its
+ * neighbours are zero rows; a strip is visited when it holds a row that is not skipped.  The
+ * skipping is off (returns 0) unless some strip is skipped, and unless every zero row's elevations
+ * v have 0.5 v normal. */
+int or_live_of(size_t w, size_t h, double dx, const double *hgt, const double *kd, double dt,
+               unsigned char *strip_on) {
+    double s2 = dx * dx;
+    if (w == 0 || h == 0 || !(isfinite(dt) && dt > 0.0 && isfinite(s2) && s2 > 0.0)) return 0;
+    unsigned char *zero = (unsigned char *)malloc(h);
+    int any = 0;
+    for (size_t j = 0; j < h; j++) {
+        int z = 1;
+        for (size_t i = 0; i < w && z; i++) {
+            uint64_t bits;
+            memcpy(&bits, &kd[j * w + i], 8);
+            z = bits == 0;
+        }
+        zero[j] = (unsigned char)z;
+        any |= z;
+    }
+    if (!any) {
+        free(zero);
+        return 0;
+    }
+    size_t nstrips = div_ceil(h, STRIP_ROWS);
+    int all = 1;
+    for (size_t s = 0; s < nstrips; s++) {
+        size_t j1 = (s + 1) * STRIP_ROWS < h ? (s + 1) * STRIP_ROWS : h;
+        int on = 0;
+        for (size_t j = s * STRIP_ROWS; j < j1; j++) {
+            int skip = zero[j] && (j == 0 || zero[j - 1]) && (j + 1 == h || zero[j + 1]);
+            on |= !skip;
+        }
+        strip_on[s] = (unsigned char)on;
+        all &= on;
+    }
+    int normal = 1;
+    for (size_t j = 0; j < h && normal; j++)
+        if (zero[j])
+            for (size_t i = 0; i < w && normal; i++)
+                normal = fpclassify(0.5 * hgt[j * w + i]) == FP_NORMAL;
+    free(zero);
+    return !all && normal;
+}
+
+/* `step_in`, D:262-283: the right-hand side is h, the splitting's first guess and the Jacobi
+ * conjugate gradients to CG_TOL_M over the strips `Live` visits; when rows were skipped and a
+ * visited value ends up not finite, the step is solved again on every row. */
+size_t or_step_in(size_t w, size_t h, double dx, int wrap_x, double *hgt, const double *kd,
+                  double dt) {
+    size_t n = w * h;
+    if (n == 0) return 0;
+    size_t nstrips = div_ceil(h, STRIP_ROWS);
+    unsigned char *strip_on = (unsigned char *)malloc(nstrips);
+    unsigned char *row_on = (unsigned char *)malloc(h);
+    int live = or_live_of(w, h, dx, hgt, kd, dt, strip_on);
+    for (size_t j = 0; j < h; j++) row_on[j] = !live || strip_on[j / STRIP_ROWS];
+    double *b = (double *)malloc(n * sizeof(double));
+    memcpy(b, hgt, n * sizeof(double));
+    lod_live(w, h, dx, wrap_x, hgt, kd, dt, live ? row_on : NULL);
+    size_t its = jacobi_cg_live(w, h, dx, wrap_x, hgt, b, kd, dt, CG_TOL_M, live ? strip_on : NULL);
+    if (live) {
+        int finite = 1;
+        for (size_t c = 0; c < n && finite; c++)
+            if (row_on[c / w]) finite = isfinite(hgt[c]);
+        if (!finite) {
+            memcpy(hgt, b, n * sizeof(double));
+            lod_live(w, h, dx, wrap_x, hgt, kd, dt, NULL);
+            its = jacobi_cg_live(w, h, dx, wrap_x, hgt, b, kd, dt, CG_TOL_M, NULL);
+        }
+    }
+    free(b);
+    free(strip_on);
+    free(row_on);
+    return its;
+}
+
+/* `spread::diffusivity`, spread.rs:76-84: zero up to the strength height, d0 x^3 above it with x
+ * the excess over it (relative, at most MAX_EXCESS = 2, spread.rs:60). */
+double or_spread_diffusivity(double z, double strength, double d0) {
+    if (z > strength && strength > 0.0) {
+        double x = z / strength - 1.0;
+        if (x > 2.0) x = 2.0;
+        return d0 * x * x * x;
+    }
+    return 0.0;
 }
 
 /* `reference`, MGT:46-74: unpreconditioned CG on the definitional operator to a relative

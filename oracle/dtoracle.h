@@ -71,6 +71,11 @@ size_t or_solve(size_t w, size_t h, double dx, int wrap_x, double *hgt, const do
                 double dt, double tol_m);
 size_t or_jacobi_cg(size_t w, size_t h, double dx, int wrap_x, double *x, const double *b,
                     const double *kd, double dt, double tol_m);
+int or_live_of(size_t w, size_t h, double dx, const double *hgt, const double *kd, double dt,
+               unsigned char *strip_on);
+size_t or_step_in(size_t w, size_t h, double dx, int wrap_x, double *hgt, const double *kd,
+                  double dt);
+double or_spread_diffusivity(double z, double strength, double d0);
 size_t or_reference_cg(size_t w, size_t h, double dx, int wrap_x, const double *b,
                        const double *kd, double dt, double *x);
 double or_bound(size_t w, size_t h, double dx, int wrap_x, const double *kd, double dt,

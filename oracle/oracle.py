@@ -79,6 +79,14 @@ def lib():
                                   ctypes.c_double, _f64p, _f64p]
     L.or_rel_residual.restype = ctypes.c_double
     L.or_threads.restype = ctypes.c_int
+    L.or_live_of.argtypes = [_sz, _sz, ctypes.c_double, _f64p, _f64p, ctypes.c_double,
+                             ctypes.POINTER(ctypes.c_ubyte)]
+    L.or_live_of.restype = ctypes.c_int
+    L.or_step_in.argtypes = [_sz, _sz, ctypes.c_double, ctypes.c_int, _f64p, _f64p,
+                             ctypes.c_double]
+    L.or_step_in.restype = _sz
+    L.or_spread_diffusivity.argtypes = [ctypes.c_double] * 3
+    L.or_spread_diffusivity.restype = ctypes.c_double
     _lib = L
     return L
 
@@ -241,6 +249,28 @@ def jacobi_step(w, h, dx, wrap, hgt, kd, dt):
     b = _c(hgt)
     x = lod(w, h, dx, wrap, b, kd, dt)
     return jacobi_cg(w, h, dx, wrap, x, b, kd, dt, 1e-6)
+
+
+def live_of(w, h, dx, hgt, kd, dt):
+    """`Live::of`, diffusion.rs:308-355: the visited strips of 8 rows (a bool array), or None when
+    every row is visited."""
+    ns = (h + 7) // 8
+    out = (ctypes.c_ubyte * max(ns, 1))()
+    on = lib().or_live_of(w, h, dx, _c(hgt), _c(kd), dt, out)
+    return np.array(list(out)[:ns], dtype=bool) if on else None
+
+
+def step_in(w, h, dx, wrap, hgt, kd, dt):
+    """`step_in` (the Jacobi solve), diffusion.rs:262-283, skipping the rows `Live` skips.
+    Returns (h1, iterations)."""
+    x = _c(hgt).copy()
+    its = lib().or_step_in(w, h, dx, int(wrap), x, _c(kd), dt)
+    return x, its
+
+
+def spread_diffusivity(z, strength, d0):
+    """`spread::diffusivity`, spread.rs:76-84."""
+    return lib().or_spread_diffusivity(z, strength, d0)
 
 
 def reference_cg(w, h, dx, wrap, b, kd, dt):
