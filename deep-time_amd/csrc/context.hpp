@@ -69,6 +69,10 @@ struct Workspace {
     std::vector<Lvl<double>> ld;
     std::vector<void *> owned;  // every allocation of the levels
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // The Jacobi solve's extra buffers (diag, two row sums a row, its scalars) and graphs.
+    double *jdiag = nullptr, *jrows = nullptr;
+    void *jst = nullptr;
+    hipGraphExec_t jgraph[2] = {nullptr, nullptr};
 };
 
 struct BatchWs;

@@ -30,11 +30,13 @@
 namespace dtd {
 
 void free_ws(Workspace &ws) {
-    for (int k = 0; k < 2; k++)
-        if (ws.graph[k]) {
-            (void)hipGraphExecDestroy(ws.graph[k]);
-            ws.graph[k] = nullptr;
-        }
+    for (int k = 0; k < 2; k++) {
+        if (ws.graph[k]) (void)hipGraphExecDestroy(ws.graph[k]);
+        if (ws.jgraph[k]) (void)hipGraphExecDestroy(ws.jgraph[k]);
+    }
+    if (ws.jdiag) (void)hipFree(ws.jdiag);
+    if (ws.jrows) (void)hipFree(ws.jrows);
+    if (ws.jst) (void)hipFree(ws.jst);
     double *bufs[] = {ws.b, ws.x, ws.r, ws.p0, ws.p1, ws.ap, ws.kd, ws.parts, ws.pair};
     for (double *p : bufs)
         if (p) (void)hipFree(p);
@@ -480,6 +482,7 @@ __global__ void k_stream5(long n4, const float4 *__restrict__ a, const float4 *_
 }  // namespace dtd
 
 #include "batch.hpp"
+#include "jacobi.hpp"
 
 using namespace dtd;
 
@@ -553,9 +556,10 @@ int dtd_solve(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h
     return guard(ctx, [&] {
         begin(ctx);
         if (iterations) *iterations = 0;
-        if (solver != DTD_SOLVER_MULTIGRID)
-            throw Error{DTD_EINVAL, "only the multigrid solver is implemented"};
-        if (!(std::isfinite(tol_m) && tol_m > 0.0))
+        if (solver != DTD_SOLVER_MULTIGRID && solver != DTD_SOLVER_JACOBI)
+            throw Error{DTD_EINVAL, "unknown solver"};
+        // The Jacobi solve keeps its own tolerance (D:72-74), as the reference's does.
+        if (solver == DTD_SOLVER_MULTIGRID && !(std::isfinite(tol_m) && tol_m > 0.0))
             throw Error{DTD_EINVAL, "tol_m must be finite and positive"};
         check_grid(w, h);
         const size_t n = w * h;
@@ -565,8 +569,11 @@ int dtd_solve(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h
         Workspace &ws = ctx->ws;
         upload(ctx, ws.kd, kd, n * sizeof(double));
         upload(ctx, ws.b, h_inout, n * sizeof(double));
-        const size_t its = solve_core(ctx, ws.b, ws.kd, dx, dt, wrap_x, tol_m, 0.0, true,
-                                      kGuessLod, nullptr, nullptr, nullptr);
+        const size_t its =
+            solver == DTD_SOLVER_JACOBI
+                ? jacobi_solve(ctx, ws.b, ws.kd, dx, dt, wrap_x, nullptr)
+                : solve_core(ctx, ws.b, ws.kd, dx, dt, wrap_x, tol_m, 0.0, true, kGuessLod,
+                             nullptr, nullptr, nullptr);
         download(ctx, h_inout, ws.x, n * sizeof(double));
         if (iterations) *iterations = its;
     });
@@ -625,6 +632,24 @@ int dtd_solve_rel_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x
                        &rel, &bnorm);
         }
         st->rel_residual = rel;
+        HIPCHK(hipMemcpyAsync(h_dev, ws.x, n * sizeof(double), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int dtd_step_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h_dev,
+                    const double *kd_dev, double dt, dtd_stats *stats) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (stats) *stats = dtd_stats{};
+        check_grid(w, h);
+        const size_t n = w * h;
+        if (n == 0) return;
+        if (!h_dev || !kd_dev) throw Error{DTD_EINVAL, "null array"};
+        fit(ctx, (int)w, (int)h, wrap_x && w > 1);
+        Workspace &ws = ctx->ws;
+        jacobi_solve(ctx, h_dev, kd_dev, dx, dt, wrap_x, stats);
         HIPCHK(hipMemcpyAsync(h_dev, ws.x, n * sizeof(double), hipMemcpyDeviceToDevice,
                               ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));

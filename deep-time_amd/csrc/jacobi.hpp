@@ -1,0 +1,401 @@
+// The Jacobi-preconditioned solve (`step_in`, D:262-283; `cg`, D:927-1026) on the device.
+// Included once, by dtdiff.hip.
+//
+// Reference files (relative to the deep-time repository root):
+//   D  crates/deeptime-core/src/surface/diffusion.rs
+//
+// The reference takes this solve's sums in a fixed order so that a run reproduces bit for bit at
+// any thread count: per row in ascending column order from -0.0, then the row sums in row order.
+// The device keeps that order.  Each row's sum is one chain of additions by one thread, over
+// tiles of the row staged through LDS by the whole block (so the loads stay coalesced), and the
+// row sums are added by one thread; every per-cell value is the reference's expression.  The
+// iterations and every iterate are therefore the reference's.  The one place the order differs is
+// the put-back (D:1000-1023), whose two sums run over all n cells in one chain in the reference:
+// here they are block sums added in block order, so the put-back's step differs from the
+// reference's by rounding only.
+//
+// Rows with nothing to diffuse (`Live`, D:285-367) are skipped by the reference only when doing so
+// gives the bits of visiting them (their faces are +0 and their elevations normal, so the passes
+// carry +0 through them and leave them as they are), and when a visited value ends up not finite
+// it solves again on every row.  The device visits every row, which is that same result.
+#pragma once
+
+#include "context.hpp"
+#include "kernels.hpp"
+
+namespace dtd {
+
+constexpr int kRowTile = 16;    // rows per block of the row sums
+constexpr int kColTile = 256;   // columns per tile (one per thread of the block)
+constexpr double kCgTolM = 1.0e-6;  // the Jacobi solve's RMS tolerance, metres (D:495)
+
+// The Jacobi loop's scalars (D:941-998).
+struct JState {
+    double rz, alpha, beta, tol2, lost, weight;
+    int have_beta, pending, done, its, max_iter, pad;
+};
+
+// diag (D:585-600), stored once a solve: every pass divides by it.
+__global__ __launch_bounds__(kBlock) void kj_diag(Grid g, const double2 *__restrict__ es,
+                                                  double *__restrict__ diag) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= g.n) return;
+    const int j = c / g.w, i = c - j * g.w;
+    diag[c] = diag_es(g, es, c, i, j);
+}
+
+// r = b - A x (pass_init, D:747-780).
+__global__ __launch_bounds__(kBlock) void kj_resid(Grid g, const double2 *__restrict__ es,
+                                                   const double *__restrict__ b,
+                                                   const double *__restrict__ x,
+                                                   double *__restrict__ r) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= g.n) return;
+    const int j = c / g.w, i = c - j * g.w;
+    r[c] = b[c] - apply_es(g, es, c, i, j, [&](int k) { return x[k]; });
+}
+
+// Per row, in ascending column order from -0.0 (`row_sums`/`row_dots`, D:666-745):
+//   kRowsRZ:  s0[j] = sum r r,  s1[j] = sum r (r / g)    (a = r, b = diag)
+//   kRowsPAP: s0[j] = sum p Ap                            (a = p, b = A p)
+// The block takes kRowTile rows; for each tile of kColTile columns its threads form the products
+// (one column each) into LDS, then thread k adds row k's in column order.
+enum { kRowsRZ = 0, kRowsPAP = 1 };
+
+template <int MODE>
+__global__ __launch_bounds__(kColTile) void kj_rows(int w, int h, const double *__restrict__ a,
+                                                    const double *__restrict__ b,
+                                                    double *__restrict__ s0,
+                                                    double *__restrict__ s1,
+                                                    const JState *__restrict__ st) {
+    __shared__ double t0[kRowTile][kColTile + 1];
+    __shared__ double t1[MODE == kRowsRZ ? kRowTile : 1][kColTile + 1];
+    if (st->done) return;
+    const int j0 = blockIdx.x * kRowTile;
+    const int tid = threadIdx.x;
+    double acc0 = -0.0, acc1 = -0.0;
+    for (int i0 = 0; i0 < w; i0 += kColTile) {
+        const int i = i0 + tid;
+#pragma unroll 4
+        for (int k = 0; k < kRowTile; k++) {
+            const int j = j0 + k;
+            if (j < h && i < w) {
+                const size_t c = (size_t)j * w + i;
+                const double av = a[c], bv = b[c];
+                if (MODE == kRowsRZ) {
+                    t0[k][tid] = av * av;
+                    t1[k][tid] = av * (av / bv);
+                } else {
+                    t0[k][tid] = av * bv;
+                }
+            }
+        }
+        __syncthreads();
+        const int cols = min(kColTile, w - i0);
+        if (tid < kRowTile && j0 + tid < h) {
+            for (int q = 0; q < cols; q++) {
+                acc0 += t0[tid][q];
+                if (MODE == kRowsRZ) acc1 += t1[tid][q];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < kRowTile && j0 + tid < h) {
+        s0[j0 + tid] = acc0;
+        if (MODE == kRowsRZ) s1[j0 + tid] = acc1;
+    }
+}
+
+// The scalar steps (D:940-998), one thread, the row sums added in row order from -0.0.
+enum { kJInit = 0, kJPap = 1, kJUpdate = 2 };
+
+__device__ inline double rows_total(const double *s, int h) {
+    double v = -0.0;
+    for (int j = 0; j < h; j++) v += s[j];
+    return v;
+}
+
+template <int STEP>
+__global__ void kj_step(const double *__restrict__ s0, const double *__restrict__ s1, int h,
+                        JState *st) {
+    if (threadIdx.x != 0 || st->done) return;
+    if (STEP == kJInit) {
+        // rr0 > tol2 starts the loop with rz = sum r z (D:947-953).
+        const double rr0 = rows_total(s0, h);
+        if (rr0 > st->tol2) {
+            st->rz = rows_total(s1, h);
+        } else {
+            st->done = 1;
+        }
+    } else if (STEP == kJPap) {
+        // iterations += 1; the direction pass took the pending update; the p.A p break
+        // (D:955-977).
+        st->its += 1;
+        st->pending = 0;
+        const double pap = rows_total(s0, h);
+        if (pap <= 0.0) {
+            st->done = 2;
+            return;
+        }
+        st->alpha = st->rz / pap;
+    } else {
+        // After r -= alpha A p: the pending update, the r.r stop, beta (D:978-992); the cap.
+        st->pending = 1;
+        const double rr = rows_total(s0, h);
+        if (rr <= st->tol2) {
+            st->done = 1;
+            return;
+        }
+        const double rz_new = rows_total(s1, h);
+        st->beta = rz_new / st->rz;
+        st->have_beta = 1;
+        st->rz = rz_new;
+        if (st->its >= st->max_iter) st->done = 4;
+    }
+}
+
+// The direction pass (D:782-880) cell by cell: the pending x += alpha p_old, then
+// p = r / g (+ beta p_old).
+__global__ __launch_bounds__(kBlock) void kj_dir(int n, const double *__restrict__ r,
+                                                 const double *__restrict__ diag,
+                                                 const double *__restrict__ pold,
+                                                 double *__restrict__ pnew,
+                                                 double *__restrict__ x,
+                                                 const JState *__restrict__ st) {
+    if (st->done) return;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const double po = (st->have_beta || st->pending) ? pold[c] : 0.0;
+    if (st->pending) x[c] += st->alpha * po;
+    pnew[c] = st->have_beta ? r[c] / diag[c] + st->beta * po : r[c] / diag[c];
+}
+
+// A p (`Operator::apply_row`, D:609-649).
+__global__ __launch_bounds__(kBlock) void kj_apply(Grid g, const double2 *__restrict__ es,
+                                                   const double *__restrict__ p,
+                                                   double *__restrict__ ap,
+                                                   const JState *__restrict__ st) {
+    if (st && st->done) return;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= g.n) return;
+    const int j = c / g.w, i = c - j * g.w;
+    ap[c] = apply_es(g, es, c, i, j, [&](int k) { return p[k]; });
+}
+
+// r -= alpha A p (pass_update, D:883-905).
+__global__ __launch_bounds__(kBlock) void kj_update(int n, const double *__restrict__ ap,
+                                                    double *__restrict__ r,
+                                                    const JState *__restrict__ st) {
+    if (st->done) return;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c < n) r[c] -= st->alpha * ap[c];
+}
+
+// After the loop: the pending x += alpha p (pass_x, D:907-919) with p the last direction
+// (iteration k wrote p0 when k is odd, p1 when even).
+__global__ __launch_bounds__(kBlock) void kj_last(int n, const double *__restrict__ p0,
+                                                  const double *__restrict__ p1,
+                                                  double *__restrict__ x,
+                                                  const JState *__restrict__ st) {
+    if (!st->pending) return;
+    const double *p = (st->its & 1) ? p0 : p1;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c < n) x[c] += st->alpha * p[c];
+}
+
+// The put-back's sums per block (sum (b - A x), sum (g - 1)), A x already in ax.
+__global__ __launch_bounds__(kBlock) void kj_putback_sums(int n, const double *__restrict__ b,
+                                                          const double *__restrict__ ax,
+                                                          const double *__restrict__ diag,
+                                                          double *__restrict__ parts) {
+    __shared__ double lds[kBlock / 64];
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    double l = 0.0, wg = 0.0;
+    if (c < n) {
+        l = b[c] - ax[c];
+        wg = diag[c] - 1.0;
+    }
+    block_pair(make_double2(l, wg), parts, blockIdx.x, lds);
+}
+
+__global__ void kj_putback_step(const double *__restrict__ parts, int nparts, JState *st) {
+    __shared__ double lds[kReduceBlock / 64];
+    const double l = reduce_parts(parts, nparts, 2, 0, lds);
+    const double wg = reduce_parts(parts, nparts, 2, 1, lds);
+    if (threadIdx.x == 0) st->lost = l, st->weight = wg;
+}
+
+// x += (lost / weight) (g - 1) when weight > 0 and lost != 0 (D:1013-1023).
+__global__ __launch_bounds__(kBlock) void kj_putback(int n, const double *__restrict__ diag,
+                                                     double *__restrict__ x,
+                                                     const JState *__restrict__ st) {
+    const double lost = st->lost, weight = st->weight;
+    if (!(weight > 0.0 && lost != 0.0)) return;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c < n) x[c] += (lost / weight) * (diag[c] - 1.0);
+}
+
+__global__ void kj_init(JState *st, double tol2, int max_iter) {
+    if (threadIdx.x != 0) return;
+    JState s{};
+    s.tol2 = tol2;
+    s.max_iter = max_iter;
+    *st = s;
+}
+
+// The first guess with the reference's recurrences (bit for bit), the rows on the transposed
+// grid (a wave's rows side by side in memory): kd^T in kdt, b^T in xt (solved in place), cbuf
+// and zbuf scratch, all w x h.
+__global__ __launch_bounds__(kBlock) void kj_lod_rows_t(Grid g, int wrap_x,
+                                                        const double *__restrict__ kdt,
+                                                        double *__restrict__ xt,
+                                                        double *__restrict__ cbuf,
+                                                        double *__restrict__ zbuf, double dt,
+                                                        double s2) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    const int h = g.h;
+    if (j < h)
+        lod_line_x(Line<const double>{kdt + j, h}, Line<double>{xt + j, h},
+                   Line<double>{cbuf + j, h}, Line<double>{zbuf + j, h}, g.w, wrap_x, dt, s2);
+}
+
+}  // namespace dtd
+
+namespace dtd {
+
+// Device buffers of the Jacobi solve beside the workspace's vectors (kept with the shape).
+static void jacobi_fit(Workspace &ws) {
+    if (ws.jdiag) return;
+    ws.jdiag = dalloc<double>(ws.n);
+    ws.jrows = dalloc<double>(2 * (size_t)ws.h);
+    ws.jst = dalloc<JState>(1);
+}
+
+// One iteration (D:955-992): parity 0 reads the direction from p1 and writes p0, parity 1 the
+// other way.
+static void jacobi_iteration(dtd_ctx *ctx, int parity) {
+    Workspace &ws = ctx->ws;
+    hipStream_t s = ctx->stream;
+    JState *st = (JState *)ws.jst;
+    const Grid g = grid_of(ws);
+    const int nb = nblocks(ws.n), nr = (ws.h + kRowTile - 1) / kRowTile;
+    double *pold = parity == 0 ? ws.p1 : ws.p0, *pnew = parity == 0 ? ws.p0 : ws.p1;
+    double *s0 = ws.jrows, *s1 = ws.jrows + ws.h;
+    kj_dir<<<nb, kBlock, 0, s>>>((int)ws.n, ws.r, ws.jdiag, pold, pnew, ws.x, st);
+    LAUNCH_CHECK();
+    kj_apply<<<nb, kBlock, 0, s>>>(g, ws.es, pnew, ws.ap, st);
+    LAUNCH_CHECK();
+    kj_rows<kRowsPAP><<<nr, kColTile, 0, s>>>(ws.w, ws.h, pnew, ws.ap, s0, nullptr, st);
+    LAUNCH_CHECK();
+    kj_step<kJPap><<<1, 64, 0, s>>>(s0, s1, ws.h, st);
+    LAUNCH_CHECK();
+    kj_update<<<nb, kBlock, 0, s>>>((int)ws.n, ws.ap, ws.r, st);
+    LAUNCH_CHECK();
+    kj_rows<kRowsRZ><<<nr, kColTile, 0, s>>>(ws.w, ws.h, ws.r, ws.jdiag, s0, s1, st);
+    LAUNCH_CHECK();
+    kj_step<kJUpdate><<<1, 64, 0, s>>>(s0, s1, ws.h, st);
+    LAUNCH_CHECK();
+}
+
+// `step_in` (D:262-283) of b (device; only read) into ws.x: the operator, the splitting's first
+// guess by the reference's recurrences, the loop to CG_TOL_M, the last update and the put-back.
+static size_t jacobi_solve(dtd_ctx *ctx, const double *b, const double *kd, double dx, double dt,
+                           int wrap_x, dtd_stats *stats) {
+    Workspace &ws = ctx->ws;
+    hipStream_t s = ctx->stream;
+    jacobi_fit(ws);
+    JState *st = (JState *)ws.jst;
+    const Grid g = grid_of(ws);
+    const int nb = nblocks(ws.n), nr = (ws.h + kRowTile - 1) / kRowTile;
+    const double s2 = dx * dx;
+    double *s0 = ws.jrows, *s1 = ws.jrows + ws.h;
+    HIPCHK(hipEventRecord(ctx->ev[0], s));
+    k_operator<<<nb, kBlock, 0, s>>>(g, kd, dt, s2, ws.es, nullptr, nullptr, 0.0f);
+    LAUNCH_CHECK();
+    kj_diag<<<nb, kBlock, 0, s>>>(g, ws.es, ws.jdiag);
+    LAUNCH_CHECK();
+    HIPCHK(hipEventRecord(ctx->ev[1], s));
+    {
+        // kd^T into r, b^T into ap; the rows; ap^T back into x; the columns in place.
+        const dim3 tg((ws.w + 31) / 32, (ws.h + 31) / 32), tgt((ws.h + 31) / 32, (ws.w + 31) / 32);
+        k_transpose<<<tg, 256, 0, s>>>(kd, ws.r, ws.h, ws.w);
+        LAUNCH_CHECK();
+        k_transpose<<<tg, 256, 0, s>>>(b, ws.ap, ws.h, ws.w);
+        LAUNCH_CHECK();
+        kj_lod_rows_t<<<nblocks(ws.h), kBlock, 0, s>>>(g, wrap_x, ws.r, ws.ap, ws.p0, ws.p1, dt,
+                                                        s2);
+        LAUNCH_CHECK();
+        k_transpose<<<tgt, 256, 0, s>>>(ws.ap, ws.x, ws.w, ws.h);
+        LAUNCH_CHECK();
+        k_lod_cols<<<nblocks(ws.w), kBlock, 0, s>>>(g, kd, dt, s2, ws.x, ws.p0);
+        LAUNCH_CHECK();
+    }
+    HIPCHK(hipEventRecord(ctx->ev[2], s));
+    kj_init<<<1, 64, 0, s>>>(st, kCgTolM * kCgTolM * (double)ws.n, (int)ctx->opts.max_iter);
+    LAUNCH_CHECK();
+    kj_resid<<<nb, kBlock, 0, s>>>(g, ws.es, b, ws.x, ws.r);
+    LAUNCH_CHECK();
+    kj_rows<kRowsRZ><<<nr, kColTile, 0, s>>>(ws.w, ws.h, ws.r, ws.jdiag, s0, s1, st);
+    LAUNCH_CHECK();
+    kj_step<kJInit><<<1, 64, 0, s>>>(s0, s1, ws.h, st);
+    LAUNCH_CHECK();
+    if (ctx->use_graph)
+        for (int k = 0; k < 2; k++) {
+            if (ws.jgraph[k]) continue;
+            hipGraph_t gr = nullptr;
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                jacobi_iteration(ctx, k);
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &gr);
+                if (gr) (void)hipGraphDestroy(gr);
+                throw;
+            }
+            HIPCHK(hipStreamEndCapture(s, &gr));
+            hipError_t e = hipGraphInstantiate(&ws.jgraph[k], gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            HIPCHK(e);
+        }
+    // The loop, reading the stop flag one iteration behind (the flag before the loop counts as
+    // iteration 0's); the state's cap stops it by max_iter.
+    HIPCHK(hipMemcpyAsync(&ctx->host_flag[1], &st->done, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(ctx->flag_ev[1], s));
+    const int max_iter = (int)ctx->opts.max_iter;
+    for (int k = 0; k <= max_iter + 1; k++) {
+        const int par = k & 1;
+        if (ctx->use_graph)
+            HIPCHK(hipGraphLaunch(ws.jgraph[par], s));
+        else
+            jacobi_iteration(ctx, par);
+        HIPCHK(hipMemcpyAsync(&ctx->host_flag[par], &st->done, sizeof(int), hipMemcpyDeviceToHost,
+                              s));
+        HIPCHK(hipEventRecord(ctx->flag_ev[par], s));
+        HIPCHK(hipEventSynchronize(ctx->flag_ev[par ^ 1]));
+        if (ctx->host_flag[par ^ 1]) break;
+    }
+    kj_last<<<nb, kBlock, 0, s>>>((int)ws.n, ws.p0, ws.p1, ws.x, st);
+    LAUNCH_CHECK();
+    kj_apply<<<nb, kBlock, 0, s>>>(g, ws.es, ws.x, ws.ap, nullptr);
+    LAUNCH_CHECK();
+    kj_putback_sums<<<nb, kBlock, 0, s>>>((int)ws.n, b, ws.ap, ws.jdiag, ws.parts);
+    LAUNCH_CHECK();
+    kj_putback_step<<<1, kReduceBlock, 0, s>>>(ws.parts, nb, st);
+    LAUNCH_CHECK();
+    kj_putback<<<nb, kBlock, 0, s>>>((int)ws.n, ws.jdiag, ws.x, st);
+    LAUNCH_CHECK();
+    HIPCHK(hipEventRecord(ctx->ev[3], s));
+    JState hs{};
+    HIPCHK(hipMemcpyAsync(ctx->host_st, st, sizeof(JState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::memcpy(&hs, ctx->host_st, sizeof(JState));
+    if (stats) {
+        stats->iterations += (size_t)hs.its;
+        stats->ms_setup += elapsed(ctx->ev[0], ctx->ev[1]);
+        stats->ms_guess += elapsed(ctx->ev[1], ctx->ev[2]);
+        stats->ms_cg += elapsed(ctx->ev[2], ctx->ev[3]);
+        stats->ms_total += elapsed(ctx->ev[0], ctx->ev[3]);
+    }
+    return (size_t)hs.its;
+}
+
+}  // namespace dtd

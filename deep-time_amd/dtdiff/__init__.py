@@ -39,7 +39,7 @@ SYMBOLS = (
     "dtd_default_opts", "dtd_create", "dtd_destroy", "dtd_last_error", "dtd_solve",
     "dtd_solve_device", "dtd_solve_rel_device", "dtd_operator", "dtd_apply", "dtd_lod",
     "dtd_cycle_new", "dtd_cycle_levels", "dtd_cycle_level", "dtd_cycle_precondition",
-    "dtd_time_kernel", "dtd_solve_batch",
+    "dtd_time_kernel", "dtd_solve_batch", "dtd_step_device",
 )
 KERNEL_SWEEP, KERNEL_DIRECTION, KERNEL_UPDATE, KERNEL_RESTRICT = 0, 1, 2, 3
 KERNEL_PRE, KERNEL_POST, KERNEL_COPY, KERNEL_STREAM5, KERNEL_BANDCOPY = 4, 5, 6, 7, 8
@@ -146,6 +146,8 @@ def load(path: str = LIB_PATH):
                                   ctypes.POINTER(ctypes.c_int), _fp, _fp, _fp, _fp, _fp]
     L.dtd_cycle_precondition.argtypes = [_vp, _fp, _dp, _fp, _dp]
     L.dtd_time_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+    L.dtd_step_device.argtypes = [_vp, _sz, _sz, ctypes.c_double, ctypes.c_int, _vp, _vp,
+                                  ctypes.c_double, ctypes.POINTER(Stats)]
     L.dtd_solve_batch.argtypes = [_vp, _sz, ctypes.POINTER(Grid), ctypes.c_double,
                                   ctypes.POINTER(_sz), _dp, ctypes.POINTER(BatchStats)]
     for name in ("dtd_solve", "dtd_solve_device", "dtd_solve_rel_device", "dtd_operator",
@@ -238,6 +240,14 @@ def solve_rel_device(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float, rel_tol: 
     st = Stats()
     ws.check(load().dtd_solve_rel_device(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x), _ptr(h_dev),
                                          _ptr(kd_dev), dt, rel_tol, ctypes.byref(st)))
+    return st
+
+
+def step_device(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float) -> Stats:
+    """`step_in` (the Jacobi-preconditioned solve, diffusion.rs:262-283) on device arrays."""
+    st = Stats()
+    ws.check(load().dtd_step_device(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x), _ptr(h_dev),
+                                    _ptr(kd_dev), dt, ctypes.byref(st)))
     return st
 
 

@@ -92,11 +92,20 @@ void dtd_destroy(dtd_ctx *ctx);
 const char *dtd_last_error(const dtd_ctx *ctx);
 
 /* `diffusion::solve` (diffusion.rs:77-97) on host arrays: h_inout holds h0 on entry and h1 on
- * return; kd (m^2/yr) has w*h cells; dt in years; tol_m the RMS residual in metres.  Only
- * DTD_SOLVER_MULTIGRID is implemented (DTD_SOLVER_JACOBI returns DTD_EINVAL).  *iterations
- * (may be NULL) receives the conjugate-gradient iterations.  w*h == 0 returns 0 iterations. */
+ * return; kd (m^2/yr) has w*h cells; dt in years.  DTD_SOLVER_MULTIGRID stops at the RMS
+ * residual tol_m (metres); DTD_SOLVER_JACOBI is `step_in` (diffusion.rs:262-283), which keeps the
+ * reference's own 1e-6 m (tol_m is not used).  *iterations (may be NULL) receives the
+ * conjugate-gradient iterations.  w*h == 0 returns 0 iterations. */
 int dtd_solve(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h_inout,
               const double *kd, double dt, int solver, double tol_m, size_t *iterations);
+
+/* `step_in` (diffusion.rs:262-283, the Jacobi-preconditioned solve) on device-resident arrays:
+ * h_dev holds h0 on entry and h1 on return.  Its conjugate-gradient sums are the reference's, in
+ * its order (per row in column order, then rows in order), so its iterations and iterates are the
+ * reference's; the put-back's sums are taken in another order (its step differs by rounding).
+ * stats (may be NULL) gets the iterations and device times. */
+int dtd_step_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h_dev,
+                    const double *kd_dev, double dt, dtd_stats *stats);
 
 /* The same solve on device-resident arrays (h_dev in/out and kd_dev: device pointers of w*h
  * doubles on the context's device).  stats may be NULL. */

@@ -187,7 +187,7 @@ def test_empty_grid_and_bad_arguments(ws):
             dtdiff.solve(ws, d, x, np.ones(16), 1.0, dtdiff.DiffusionSolver.MULTIGRID, tol)
         assert e.value.code == dtdiff.DTD_EINVAL
     with pytest.raises(dtdiff.DtdError) as e:
-        dtdiff.solve(ws, d, x, np.ones(16), 1.0, dtdiff.DiffusionSolver.JACOBI, 1e-6)
+        dtdiff.solve(ws, d, x, np.ones(16), 1.0, 7, 1e-6)  # no such solver
     assert e.value.code == dtdiff.DTD_EINVAL
     assert (x == 1).all()  # untouched on error
     # A constant field is already the answer (A 1 = 1): no iteration, nothing moves.
