@@ -53,6 +53,8 @@ def main():
                     help="constant-512x256 | lognormal-8192x4096 | constant-16384x8192 | "
                          "regional-512 (many grids, one batched solve)")
     ap.add_argument("--size", default=None, help="WxH: the same workload at another size")
+    ap.add_argument("--solver", choices=["multigrid", "jacobi"], default="multigrid",
+                    help="jacobi: the Jacobi-preconditioned step (its own 1e-6 m tolerance)")
     ap.add_argument("--rel", type=float, default=1e-10)
     ap.add_argument("--cycle", choices=["f32", "f64"], default="f32")
     ap.add_argument("--guess", choices=["lod", "rhs"], default="lod")
@@ -74,6 +76,8 @@ def main():
     gen, kw = F.WORKLOADS[args.workload]
     if gen is F.case_regional:
         return bench_batch(args, gen, dict(kw))
+    if args.solver == "jacobi":
+        return bench_jacobi(args, gen, dict(kw))
     kw = dict(kw)
     if args.size:
         kw["w"], kw["h"] = (int(v) for v in args.size.split("x"))
@@ -200,6 +204,75 @@ def main():
                          "device_total": last.ms_total},
         "upload_s_pcie": upload_s,
         "kernels": kernels,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_jacobi(args, gen, kw):
+    """The Jacobi-preconditioned step (`step_in`) on the device at the workload's size: its own
+    tolerance of 1e-6 m RMS, its sums in the reference's order.  The CPU leg is the oracle's
+    (serial) restatement on the same generator at 1/16 of the cells."""
+    import numpy as np
+
+    import dtdiff
+    from dtdiff.hipmem import DeviceArray, synchronize
+
+    if args.size:
+        kw["w"], kw["h"] = (int(v) for v in args.size.split("x"))
+    w, h = kw["w"], kw["h"]
+    d, h0, kd, dt = gen(**kw)
+    n = d.cells()
+    h0d, kdd, hd = DeviceArray.from_numpy(h0), DeviceArray.from_numpy(kd), DeviceArray(n)
+    ws = dtdiff.Workspace(device=0)
+
+    def step():
+        hd.copy_from(h0d)
+        return dtdiff.step_device(ws, d, hd, kdd, dt)
+
+    for k in range(args.warmup):
+        st = step()
+        log(f"warmup {k}: {st.iterations} its, {st.ms_total:.2f} ms")
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = step()
+    synchronize()
+    elapsed = time.perf_counter() - t0
+    log(f"timed {args.steps} steps: {elapsed * 1e3 / args.steps:.3f} ms/step, {st.iterations} "
+        f"its; setup {st.ms_setup:.2f} guess {st.ms_guess:.2f} cg {st.ms_cg:.2f} ms")
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O
+
+        ks = dict(kw, w=max(w // 4, 1), h=max(h // 4, 1))
+        ds, h0s, kds, dts = gen(**ks)
+        t = time.perf_counter()
+        _, its_o = O.step_in(ds.w, ds.h, ds.dx, ds.wrap_x, h0s, kds, dts)
+        cpu_s = time.perf_counter() - t
+        cpu = {"value": ds.cells() / cpu_s, "unit": "unknowns/s", "cores": 1, "kind": "port",
+               "sample": f"the same generator at {ds.w}x{ds.h} (1/16 of the cells), one step by "
+                         f"the oracle's serial restatement: {cpu_s:.2f} s, {its_o} iterations"}
+        log(f"cpu oracle sample {ds.w}x{ds.h}: {cpu_s:.2f} s, {its_o} its")
+    out = {
+        "metric": f"unknowns/s of the Jacobi-preconditioned step (1e-6 m RMS) at {w}x{h}",
+        "value": n * args.steps / elapsed,
+        "unit": "unknowns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"Jacobi step {w}x{h} ({args.workload} generator)"},
+        "roofline": None,
+        "cpu_baseline": cpu,
+        "iterations": int(st.iterations),
+        "ms_breakdown": {"setup": st.ms_setup, "guess": st.ms_guess, "cg": st.ms_cg,
+                         "device_total": st.ms_total},
     }
     print(json.dumps(out), flush=True)
 

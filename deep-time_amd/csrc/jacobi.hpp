@@ -73,26 +73,36 @@ __global__ __launch_bounds__(kColTile) void kj_rows(int w, int h, const double *
     if (st->done) return;
     const int j0 = blockIdx.x * kRowTile;
     const int tid = threadIdx.x;
+    // The next tile's values are loaded into registers while the current tile is being added.
+    double qa[kRowTile], qb[kRowTile];
+#define DTD_FETCH_ROWS(i0_)                                                                    \
+    do {                                                                                       \
+        const int i_ = (i0_) + tid;                                                            \
+        _Pragma("unroll") for (int k = 0; k < kRowTile; k++) {                                 \
+            const int j_ = j0 + k;                                                             \
+            if (j_ < h && i_ < w) {                                                            \
+                qa[k] = a[(size_t)j_ * w + i_];                                                \
+                qb[k] = b[(size_t)j_ * w + i_];                                                \
+            }                                                                                  \
+        }                                                                                      \
+    } while (0)
     double acc0 = -0.0, acc1 = -0.0;
+    DTD_FETCH_ROWS(0);
     for (int i0 = 0; i0 < w; i0 += kColTile) {
-        const int i = i0 + tid;
-#pragma unroll 4
+#pragma unroll
         for (int k = 0; k < kRowTile; k++) {
-            const int j = j0 + k;
-            if (j < h && i < w) {
-                const size_t c = (size_t)j * w + i;
-                const double av = a[c], bv = b[c];
-                if (MODE == kRowsRZ) {
-                    t0[k][tid] = av * av;
-                    t1[k][tid] = av * (av / bv);
-                } else {
-                    t0[k][tid] = av * bv;
-                }
+            if (MODE == kRowsRZ) {
+                t0[k][tid] = qa[k] * qa[k];
+                t1[k][tid] = qa[k] * (qa[k] / qb[k]);
+            } else {
+                t0[k][tid] = qa[k] * qb[k];
             }
         }
         __syncthreads();
+        if (i0 + kColTile < w) DTD_FETCH_ROWS(i0 + kColTile);
         const int cols = min(kColTile, w - i0);
         if (tid < kRowTile && j0 + tid < h) {
+#pragma unroll 8
             for (int q = 0; q < cols; q++) {
                 acc0 += t0[tid][q];
                 if (MODE == kRowsRZ) acc1 += t1[tid][q];
@@ -100,6 +110,7 @@ __global__ __launch_bounds__(kColTile) void kj_rows(int w, int h, const double *
         }
         __syncthreads();
     }
+#undef DTD_FETCH_ROWS
     if (tid < kRowTile && j0 + tid < h) {
         s0[j0 + tid] = acc0;
         if (MODE == kRowsRZ) s1[j0 + tid] = acc1;
@@ -109,47 +120,60 @@ __global__ __launch_bounds__(kColTile) void kj_rows(int w, int h, const double *
 // The scalar steps (D:940-998), one thread, the row sums added in row order from -0.0.
 enum { kJInit = 0, kJPap = 1, kJUpdate = 2 };
 
-__device__ inline double rows_total(const double *s, int h) {
-    double v = -0.0;
-    for (int j = 0; j < h; j++) v += s[j];
-    return v;
-}
+// The row sums are staged through LDS by the whole block, then added by thread 0 in row order
+// from -0.0.
+constexpr int kStepBlock = 1024, kStepChunk = 4096;
 
 template <int STEP>
-__global__ void kj_step(const double *__restrict__ s0, const double *__restrict__ s1, int h,
-                        JState *st) {
-    if (threadIdx.x != 0 || st->done) return;
+__global__ __launch_bounds__(kStepBlock) void kj_step(const double *__restrict__ s0,
+                                                      const double *__restrict__ s1, int h,
+                                                      JState *st) {
+    __shared__ double l0[kStepChunk], l1[STEP == kJPap ? 1 : kStepChunk];
+    if (st->done) return;
+    double v0 = -0.0, v1 = -0.0;
+    for (int c0 = 0; c0 < h; c0 += kStepChunk) {
+        const int cnt = min(kStepChunk, h - c0);
+        for (int k = threadIdx.x; k < cnt; k += kStepBlock) {
+            l0[k] = s0[c0 + k];
+            if (STEP != kJPap) l1[k] = s1[c0 + k];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+#pragma unroll 8
+            for (int k = 0; k < cnt; k++) {
+                v0 += l0[k];
+                if (STEP != kJPap) v1 += l1[k];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
     if (STEP == kJInit) {
         // rr0 > tol2 starts the loop with rz = sum r z (D:947-953).
-        const double rr0 = rows_total(s0, h);
-        if (rr0 > st->tol2) {
-            st->rz = rows_total(s1, h);
-        } else {
+        if (v0 > st->tol2)
+            st->rz = v1;
+        else
             st->done = 1;
-        }
     } else if (STEP == kJPap) {
         // iterations += 1; the direction pass took the pending update; the p.A p break
         // (D:955-977).
         st->its += 1;
         st->pending = 0;
-        const double pap = rows_total(s0, h);
-        if (pap <= 0.0) {
+        if (v0 <= 0.0) {
             st->done = 2;
             return;
         }
-        st->alpha = st->rz / pap;
+        st->alpha = st->rz / v0;
     } else {
         // After r -= alpha A p: the pending update, the r.r stop, beta (D:978-992); the cap.
         st->pending = 1;
-        const double rr = rows_total(s0, h);
-        if (rr <= st->tol2) {
+        if (v0 <= st->tol2) {
             st->done = 1;
             return;
         }
-        const double rz_new = rows_total(s1, h);
-        st->beta = rz_new / st->rz;
+        st->beta = v1 / st->rz;
         st->have_beta = 1;
-        st->rz = rz_new;
+        st->rz = v1;
         if (st->its >= st->max_iter) st->done = 4;
     }
 }
@@ -287,13 +311,13 @@ static void jacobi_iteration(dtd_ctx *ctx, int parity) {
     LAUNCH_CHECK();
     kj_rows<kRowsPAP><<<nr, kColTile, 0, s>>>(ws.w, ws.h, pnew, ws.ap, s0, nullptr, st);
     LAUNCH_CHECK();
-    kj_step<kJPap><<<1, 64, 0, s>>>(s0, s1, ws.h, st);
+    kj_step<kJPap><<<1, kStepBlock, 0, s>>>(s0, s1, ws.h, st);
     LAUNCH_CHECK();
     kj_update<<<nb, kBlock, 0, s>>>((int)ws.n, ws.ap, ws.r, st);
     LAUNCH_CHECK();
     kj_rows<kRowsRZ><<<nr, kColTile, 0, s>>>(ws.w, ws.h, ws.r, ws.jdiag, s0, s1, st);
     LAUNCH_CHECK();
-    kj_step<kJUpdate><<<1, 64, 0, s>>>(s0, s1, ws.h, st);
+    kj_step<kJUpdate><<<1, kStepBlock, 0, s>>>(s0, s1, ws.h, st);
     LAUNCH_CHECK();
 }
 
@@ -337,7 +361,7 @@ static size_t jacobi_solve(dtd_ctx *ctx, const double *b, const double *kd, doub
     LAUNCH_CHECK();
     kj_rows<kRowsRZ><<<nr, kColTile, 0, s>>>(ws.w, ws.h, ws.r, ws.jdiag, s0, s1, st);
     LAUNCH_CHECK();
-    kj_step<kJInit><<<1, 64, 0, s>>>(s0, s1, ws.h, st);
+    kj_step<kJInit><<<1, kStepBlock, 0, s>>>(s0, s1, ws.h, st);
     LAUNCH_CHECK();
     if (ctx->use_graph)
         for (int k = 0; k < 2; k++) {
