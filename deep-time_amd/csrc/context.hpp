@@ -62,6 +62,7 @@ struct Workspace {
     int w = 0, h = 0, wrap = 0, prec = -1;
     long n = 0;
     double *b = nullptr, *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr;
+    double *r2 = nullptr;  // the residual's second buffer (the f32 cycle's fused update)
     double *ap = nullptr, *kd = nullptr, *parts = nullptr, *pair = nullptr;
     double2 *es = nullptr;  // the f64 operator: (east, south) of each cell
     CgState *st = nullptr;

@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "context.hpp"
@@ -37,7 +38,7 @@ void free_ws(Workspace &ws) {
     if (ws.jdiag) (void)hipFree(ws.jdiag);
     if (ws.jrows) (void)hipFree(ws.jrows);
     if (ws.jst) (void)hipFree(ws.jst);
-    double *bufs[] = {ws.b, ws.x, ws.r, ws.p0, ws.p1, ws.ap, ws.kd, ws.parts, ws.pair};
+    double *bufs[] = {ws.b, ws.x, ws.r, ws.r2, ws.p0, ws.p1, ws.ap, ws.kd, ws.parts, ws.pair};
     for (double *p : bufs)
         if (p) (void)hipFree(p);
     if (ws.es) (void)hipFree(ws.es);
@@ -62,6 +63,7 @@ static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
     ws.b = dalloc<double>(n);
     ws.x = dalloc<double>(n);
     ws.r = dalloc<double>(n);
+    if (prec == DTD_CYCLE_F32) ws.r2 = dalloc<double>(n);
     ws.p0 = dalloc<double>(n);
     ws.p1 = dalloc<double>(n);
     ws.ap = dalloc<double>(n);
@@ -156,20 +158,23 @@ static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt)
         build_coarse<float>(ws.lf, ctx->opts.omega, ctx->opts.scale, s);
 }
 
-template <typename T, int V>
-static void launch_pre(Lvl<T> &L, Lvl<T> &C, T *zout, const CgState *st, hipStream_t s) {
-    const int th = band_height<V>((const void *)k_spre<T, V>, L.w, L.h);
-    k_spre<T, V><<<stream_grid<V, LaneGeom<V>::halo>(L.w, L.h, th), kBlock, 0, s>>>(L, zout, C,
-                                                                                     th, st);
+// Returns the block count (with RM = kRUpdate, the r.r partial sums left in parts).
+template <typename T, int V, int RM = kRLevel>
+static int launch_pre(Lvl<T> &L, Lvl<T> &C, T *zout, const CgState *st, hipStream_t s,
+                      R64 rio = R64{}, double *parts = nullptr) {
+    const int th = band_height<V>((const void *)k_spre<T, V, RM>, L.w, L.h);
+    const int nb = stream_grid<V, LaneGeom<V>::halo>(L.w, L.h, th);
+    k_spre<T, V, RM><<<nb, kBlock, 0, s>>>(L, zout, C, th, st, rio, parts);
     LAUNCH_CHECK();
+    return nb;
 }
 
-template <typename T, int V>
+template <typename T, int V, int RM = kRLevel>
 static int launch_post(Lvl<T> &L, const T *zin, const T *zc, int wc, T *zout, const double *dot,
                        double *parts, const CgState *st, hipStream_t s) {
-    const int th = band_height<V>((const void *)k_spost<T, V>, L.w, L.h);
+    const int th = band_height<V>((const void *)k_spost<T, V, RM>, L.w, L.h);
     const int nb = stream_grid<V, LaneGeom<V>::halo>(L.w, L.h, th);
-    k_spost<T, V><<<nb, kBlock, 0, s>>>(L, zin, zc, wc, zout, th, dot, parts, st);
+    k_spost<T, V, RM><<<nb, kBlock, 0, s>>>(L, zin, zc, wc, zout, th, dot, parts, st);
     LAUNCH_CHECK();
     return nb;
 }
@@ -188,7 +193,8 @@ static int launch_dir(Workspace &ws, const TZ *z, const double *pold, double *pn
 
 // The first residual r = b - A xin (xout = xin's copy when they differ); returns the block count
 // (the (r.r, b.b) partial pairs).
-static int launch_resid(Workspace &ws, const double *b, const double *xin, hipStream_t s) {
+static int launch_resid(Workspace &ws, const double *b, const double *xin, hipStream_t s,
+                        bool r32 = true) {
     const Grid g = grid_of(ws);
     if (ws.prec == DTD_CYCLE_F64) {
         const int th = band_height<1, 1>((const void *)k_sresid<double>, ws.w, ws.h);
@@ -200,8 +206,8 @@ static int launch_resid(Workspace &ws, const double *b, const double *xin, hipSt
     }
     const int th = band_height<1, 1>((const void *)k_sresid<float>, ws.w, ws.h);
     const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
-    k_sresid<float><<<nb, kBlock, 0, s>>>(g, ws.es, b, xin, ws.x, ws.r, ws.lf[0].r, th,
-                                          ws.parts);
+    k_sresid<float><<<nb, kBlock, 0, s>>>(g, ws.es, b, xin, ws.x, ws.r, r32 ? ws.lf[0].r : nullptr,
+                                          th, ws.parts);
     LAUNCH_CHECK();
     return nb;
 }
@@ -220,9 +226,12 @@ static bool fused_enabled() {
 // sweeps alternate after it; with 2 nu - 1 sweeps after the first on every level but the
 // coarsest (and COARSE_SWEEPS - 1 = 15 there), each level's result lands in z[1].  With dot,
 // the last sweep of level l leaves its per-block sums dot * z in parts (block count returned).
+// rm (f32 levels, fused smoothing, finest level only): where the finest level's r comes from
+// (kRLevel: its own r; kR64: the f64 residual dot; kRUpdate: rio.in - alpha A p, written to
+// rio.out = dot, with the r.r step of the conjugate gradients launched after the pre-smoothing).
 template <typename T>
 static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, double *parts,
-                  const CgState *st, hipStream_t s) {
+                  CgState *st, hipStream_t s, int rm = kRLevel, R64 rio = R64{}) {
     Lvl<T> &L = lv[l];
     if (l + 1 == lv.size()) {
         if (L.n > 64) throw Error{DTD_ESTATE, "the coarsest level exceeds one wave"};
@@ -238,6 +247,26 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
         // allows.
         constexpr int VW = 8 / sizeof(T);
         const bool wide = L.w % VW == 0;
+        if constexpr (std::is_same<T, float>::value) {
+            if (rm != kRLevel) {
+                if (l != 0) throw Error{DTD_ESTATE, "the f64 residual feeds the finest level only"};
+                if (rm == kRUpdate) {
+                    const int nbp = wide ? launch_pre<T, VW, kRUpdate>(L, C0, L.z[0], st, s, rio, parts)
+                                         : launch_pre<T, 1, kRUpdate>(L, C0, L.z[0], st, s, rio, parts);
+                    k_cg_step<kStepRR><<<1, kReduceBlock, 0, s>>>(parts, nbp, st);
+                    LAUNCH_CHECK();
+                } else if (wide) {
+                    launch_pre<T, VW, kR64>(L, C0, L.z[0], st, s, rio);
+                } else {
+                    launch_pre<T, 1, kR64>(L, C0, L.z[0], st, s, rio);
+                }
+                vcycle<T>(lv, l + 1, nu, nullptr, parts, st, s);
+                return wide ? launch_post<T, VW, kR64>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts,
+                                                        st, s)
+                            : launch_post<T, 1, kR64>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts,
+                                                       st, s);
+            }
+        }
         if (wide)
             launch_pre<T, VW>(L, C0, L.z[0], st, s);
         else
@@ -274,12 +303,19 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
     return nb;
 }
 
-static int run_vcycle(dtd_ctx *ctx, const double *dot, const CgState *st) {
+static int run_vcycle(dtd_ctx *ctx, const double *dot, CgState *st, int rm = kRLevel,
+                      R64 rio = R64{}) {
     Workspace &ws = ctx->ws;
     const int nu = (int)ctx->opts.nu;
     if (ws.prec == DTD_CYCLE_F64)
         return vcycle<double>(ws.ld, 0, nu, dot, ws.parts, st, ctx->stream);
-    return vcycle<float>(ws.lf, 0, nu, dot, ws.parts, st, ctx->stream);
+    return vcycle<float>(ws.lf, 0, nu, dot, ws.parts, st, ctx->stream, rm, rio);
+}
+
+// Whether the f32 cycle's finest level takes the f64 residual straight from the conjugate
+// gradients (fused smoothing; the residual's update rides on the pre-smoothing pass).
+static bool fused_update(const Workspace &ws, const dtd_opts &o) {
+    return ws.prec == DTD_CYCLE_F32 && o.nu == 2 && fused_enabled() && ws.lf.size() > 1;
 }
 
 // One conjugate-gradient iteration (MG:630-733): parity 0 reads the direction from p1 and writes
@@ -293,6 +329,15 @@ static void enqueue_iteration(dtd_ctx *ctx, int parity) {
                                              : launch_dir<float>(ws, ws.lf[0].z[1], pold, pnew, s);
     k_cg_step<kStepPAP><<<1, kReduceBlock, 0, s>>>(ws.parts, nbd, ws.st);
     LAUNCH_CHECK();
+    if (fused_update(ws, ctx->opts)) {
+        // r_new = r_old - alpha A p inside the finest pre-smoothing (r.r and the stop test after
+        // it); the residual's buffers alternate with the parity: parity 0 reads r, writes r2.
+        double *rin = parity == 0 ? ws.r : ws.r2, *rout = parity == 0 ? ws.r2 : ws.r;
+        const int nbz = run_vcycle(ctx, rout, ws.st, kRUpdate, R64{rin, ws.ap, rout, 0.0});
+        k_cg_step<kStepRZ><<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.st);
+        LAUNCH_CHECK();
+        return;
+    }
     const int nbu = nblocks(ws.n, kCgCells);
     if (ws.prec == DTD_CYCLE_F64)
         k_update<double><<<nbu, kBlock, 0, s>>>((int)ws.n, ws.ap, ws.r, nullptr, ws.parts, ws.st);
@@ -405,10 +450,12 @@ static size_t solve_core(dtd_ctx *ctx, const double *b, const double *kd, double
     k_init_state<<<1, 64, 0, s>>>(ws.st, tol_m * tol_m * (double)ws.n, rel2,
                                   (int)ctx->opts.max_iter);
     LAUNCH_CHECK();
-    const int nbr = launch_resid(ws, b, xin, s);
+    const bool fused = fused_update(ws, ctx->opts);
+    const int nbr = launch_resid(ws, b, xin, s, !fused);
     k_cg_step<kStepRR0><<<1, kReduceBlock, 0, s>>>(ws.parts, nbr, ws.st);
     LAUNCH_CHECK();
-    const int nbz = run_vcycle(ctx, ws.r, ws.st);
+    const int nbz = fused ? run_vcycle(ctx, ws.r, ws.st, kR64, R64{ws.r, nullptr, nullptr, 0.0})
+                          : run_vcycle(ctx, ws.r, ws.st);
     k_cg_step<kStepRZ0><<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.st);
     LAUNCH_CHECK();
     if (ctx->use_graph) capture_graphs(ctx);
@@ -765,15 +812,17 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
                 k_restrict<float><<<nblocks(C.n), kBlock, 0, s>>>(L, L.z[1], C, ws.st);
                 b = 24.0 * n + 4.0 * C.n;
                 break;
-            case DTD_KERNEL_PRE:  // r, record in; z and the coarse r out
-                if (L.w % 2) throw Error{DTD_ESTATE, "the finest width is odd"};
-                launch_pre<float, 2>(L, C, L.z[0], ws.st, s);
-                b = 24.0 * n + 4.0 * C.n;
+            case DTD_KERNEL_PRE:  // r64, A p, record in; r64, z and the coarse r out
+                if (L.w % 2 || !ws.r2) throw Error{DTD_ESTATE, "the finest width is odd"};
+                launch_pre<float, 2, kRUpdate>(L, C, L.z[0], ws.st, s,
+                                               R64{ws.r, ws.ap, ws.r2, 0.0}, ws.parts);
+                b = 44.0 * n + 4.0 * C.n;
                 break;
-            case DTD_KERNEL_POST:  // z, r, record, r64, coarse z in; z out
+            case DTD_KERNEL_POST:  // z, r64, record, coarse z in; z out
                 if (L.w % 2) throw Error{DTD_ESTATE, "the finest width is odd"};
-                launch_post<float, 2>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts, ws.st, s);
-                b = 36.0 * n + 4.0 * C.n;
+                launch_post<float, 2, kR64>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts, ws.st,
+                                            s);
+                b = 32.0 * n + 4.0 * C.n;
                 break;
             case DTD_KERNEL_BANDCOPY: {  // r, record in; z out, the pre-smoother's walk
                 const int th = band_height<2>((const void *)k_bandcopy<2>, L.w, L.h);

@@ -738,26 +738,48 @@ __device__ inline void copy(T (&d)[V], const T (&s)[V]) {
     for (int v = 0; v < V; v++) d[v] = s[v];
 }
 
+// Where the finest level's smoothers take r from (the f32 cycle on the conjugate gradients'
+// residual): kRLevel, the level's own r; kR64, the f64 residual, rounded to T as it is read;
+// kRUpdate (pre-smoothing only), the f64 residual after this iteration's r -= alpha A p, formed
+// from r_in and A p as the rows are read (the reference's update pass, MG:703-723, with the same
+// expression), written to r_out for the owned cells, and summed as r.r.  The f64 residual
+// alternates between two buffers, so a band's halo rows are read before any update.
+enum { kRLevel = 0, kR64 = 1, kRUpdate = 2 };
+
+struct R64 {
+    const double *in;  // f64 residual (kR64, kRUpdate)
+    const double *ap;  // A p (kRUpdate)
+    double *out;       // updated residual (kRUpdate)
+    double alpha;
+};
+
 // Pre-smoothing of level L (nu = 2): z1 = wd r, z2 = z1 + wd (r - A z1) -> zout, and the coarse
 // residual C.r = sum over each 2x2 block of (r - A z2): fine row j0 then j0 + 1, columns
 // ascending, from +0 (MG:332-356).  Rows are loaded kAhead steps before they are consumed.
-template <typename T, int V>
-__device__ __forceinline__ void spre_band(const Lvl<T> &L, T *__restrict__ zout, const Lvl<T> &C,
-                                          int th, int wave) {
+// Returns the lane's share of r.r (kRUpdate; else 0).
+template <typename T, int V, int RM = kRLevel>
+__device__ __forceinline__ double spre_band(const Lvl<T> &L, T *__restrict__ zout,
+                                            const Lvl<T> &C, int th, int wave, R64 rio = R64{}) {
     constexpr int A = 4;  // rows in flight
     const int w = L.w, h = L.h, wrap = L.wrap;
     const Band bd = band_at<V>(w, h, wrap, th, wave);
     const int gi0 = bd.gi0;
     T qr[A][V], qwd[A][V], qdg[A][V], qe[A][V], qs[A][V];
+    double qr64[RM == kRLevel ? 1 : A][V], qap[RM == kRUpdate ? A : 1][V];
+    double acc_rr = 0.0;
 #define DTD_FETCH_PRE(j, u)                                                                   \
     do {                                                                                      \
         const int j_ = (j);                                                                   \
         if (j_ >= 0 && j_ < h && bd.col_ok) {                                                 \
             const size_t g_ = (size_t)j_ * w + gi0;                                           \
-            loadv<T, V>(L.r + g_, qr[u]);                                                     \
+            if (RM == kRLevel) loadv<T, V>(L.r + g_, qr[u]);                                  \
+            if (RM != kRLevel) loadv<double, V>(rio.in + g_, qr64[RM == kRLevel ? 0 : u]);    \
+            if (RM == kRUpdate) loadv<double, V>(rio.ap + g_, qap[RM == kRUpdate ? u : 0]);   \
             load_coef<T, V>(L.coef, g_, qe[u], qs[u], qdg[u], qwd[u]);                        \
         } else {                                                                              \
             zero(qr[u]), zero(qwd[u]), zero(qdg[u]), zero(qe[u]), zero(qs[u]);                \
+            if (RM != kRLevel) zero(qr64[RM == kRLevel ? 0 : u]);                             \
+            if (RM == kRUpdate) zero(qap[RM == kRUpdate ? u : 0]);                            \
         }                                                                                     \
     } while (0)
     // Rows by age: a = newest (ja), b = ja - 1, c = ja - 2, d = ja - 3.
@@ -778,8 +800,26 @@ __device__ __forceinline__ void spre_band(const Lvl<T> &L, T *__restrict__ zout,
             copy(z1c, z1b), copy(z1b, z1a), copy(z2d, z2c), copy(z2c, z2b);
             copy(rc, rb), copy(rb, ra), copy(wdb, wda), copy(dgc, dgb), copy(dgb, dga);
             copy(ec, eb), copy(eb, ea), copy(sd, sc), copy(sc, sb), copy(sb, sa);
-            copy(ra, qr[u]), copy(wda, qwd[u]), copy(dga, qdg[u]), copy(ea, qe[u]);
+            copy(wda, qwd[u]), copy(dga, qdg[u]), copy(ea, qe[u]);
             copy(sa, qs[u]);
+            if (RM == kRLevel) {
+                copy(ra, qr[u]);
+            } else {
+                // The f64 residual of row a (updated when asked), rounded for the cycle.
+                const bool own_a = bd.own && ja >= bd.row0 && ja < bd.row0 + bd.rows && ja < h;
+                double rv[V];
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    const double r0 = qr64[RM == kRLevel ? 0 : u][v];
+                    rv[v] = RM == kRUpdate ? r0 - rio.alpha * qap[RM == kRUpdate ? u : 0][v] : r0;
+                    ra[v] = (T)rv[v];
+                }
+                if (RM == kRUpdate && own_a) {
+                    storev<double, V>(rio.out + (size_t)ja * w + gi0, rv);
+#pragma unroll
+                    for (int v = 0; v < V; v++) acc_rr += rv[v] * rv[v];
+                }
+            }
             if (k0 + u + A < steps) DTD_FETCH_PRE(ja + A, u);
             const bool oka = ja >= 0 && ja < h && bd.col_ok;
 #pragma unroll
@@ -814,19 +854,30 @@ __device__ __forceinline__ void spre_band(const Lvl<T> &L, T *__restrict__ zout,
         }
     }
 #undef DTD_FETCH_PRE
+    return acc_rr;
 }
 
-template <typename T, int V>
+// With RM = kRUpdate the residual's update of this iteration rides on the pass (alpha from st)
+// and parts[block] gets the block's r.r.
+template <typename T, int V, int RM = kRLevel>
 __global__ __launch_bounds__(kBlock) void k_spre(Lvl<T> L, T *__restrict__ zout, Lvl<T> C, int th,
-                                                 const CgState *__restrict__ st) {
+                                                 const CgState *__restrict__ st, R64 rio,
+                                                 double *__restrict__ parts) {
+    __shared__ double lds[kBlock / 64];
     if (st && st->done) return;
-    spre_band<T, V>(L, zout, C, th, block_wave(blockIdx.x));
+    if (RM == kRUpdate) rio.alpha = st->alpha;
+    const double acc = spre_band<T, V, RM>(L, zout, C, th, block_wave(blockIdx.x), rio);
+    if (RM == kRUpdate) {
+        const double sum = block_sum<kBlock>(acc, lds);
+        if (threadIdx.x == 0) parts[blockIdx.x] = sum;
+    }
 }
 
 // Post-smoothing of level L (nu = 2): z' = zin + zc[j/2][i/2], z3 = z' + wd (r - A z'),
 // z4 = z3 + wd (r - A z3) -> zout; with dot, per block sum dot * z4 into parts[block].
-// Returns the lane's share of sum dot * z4 (0 without dot).
-template <typename T, int V>
+// Returns the lane's share of sum dot * z4 (0 without dot).  With RM = kR64, r is the f64
+// residual dot itself, rounded to T as it is read.
+template <typename T, int V, int RM = kRLevel>
 __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restrict__ zin,
                                              const T *__restrict__ zc, int wc,
                                              T *__restrict__ zout, int th,
@@ -836,19 +887,25 @@ __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restric
     const Band bd = band_at<V>(w, h, wrap, th, wave);
     const int gi0 = bd.gi0;
     T qr[A][V], qwd[A][V], qdg[A][V], qe[A][V], qs[A][V], qz[A][V], qc[A][V];
+    double qr64[RM == kR64 ? A : 1][V];
     // z' = z + zc is formed when the row is consumed.
 #define DTD_FETCH_POST(j, u)                                                                  \
     do {                                                                                      \
         const int j_ = (j);                                                                   \
         if (j_ >= 0 && j_ < h && bd.col_ok) {                                                 \
             const size_t g_ = (size_t)j_ * w + gi0;                                           \
-            loadv<T, V>(L.r + g_, qr[u]), loadv<T, V>(zin + g_, qz[u]);                       \
+            if (RM == kR64)                                                                   \
+                loadv<double, V>(dot + g_, qr64[RM == kR64 ? u : 0]);                         \
+            else                                                                              \
+                loadv<T, V>(L.r + g_, qr[u]);                                                 \
+            loadv<T, V>(zin + g_, qz[u]);                                                     \
             load_coef<T, V>(L.coef, g_, qe[u], qs[u], qdg[u], qwd[u]);                        \
             const T *zcr_ = zc + (size_t)(j_ >> 1) * wc;                                      \
             _Pragma("unroll") for (int v = 0; v < V; v++) qc[u][v] = zcr_[(gi0 + v) >> 1];    \
         } else {                                                                              \
             zero(qr[u]), zero(qwd[u]), zero(qdg[u]), zero(qe[u]), zero(qs[u]), zero(qz[u]);   \
             zero(qc[u]);                                                                      \
+            if (RM == kR64) zero(qr64[RM == kR64 ? u : 0]);                                   \
         }                                                                                     \
     } while (0)
     T zpa[V], zpb[V], zpc[V], z3b[V], z3c[V], z3d[V];
@@ -867,8 +924,14 @@ __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restric
             copy(zpc, zpb), copy(zpb, zpa), copy(z3d, z3c), copy(z3c, z3b);
             copy(rc, rb), copy(rb, ra), copy(wdc, wdb), copy(wdb, wda), copy(dgc, dgb);
             copy(dgb, dga), copy(ec, eb), copy(eb, ea), copy(sd, sc), copy(sc, sb), copy(sb, sa);
-            copy(ra, qr[u]), copy(wda, qwd[u]), copy(dga, qdg[u]), copy(ea, qe[u]);
+            copy(wda, qwd[u]), copy(dga, qdg[u]), copy(ea, qe[u]);
             copy(sa, qs[u]);
+            if (RM == kR64) {
+#pragma unroll
+                for (int v = 0; v < V; v++) ra[v] = (T)qr64[RM == kR64 ? u : 0][v];
+            } else {
+                copy(ra, qr[u]);
+            }
             const bool oka = ja >= 0 && ja < h && bd.col_ok;
 #pragma unroll
             for (int v = 0; v < V; v++) zpa[v] = oka ? qz[u][v] + qc[u][v] : (T)0;
@@ -894,7 +957,7 @@ __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restric
     return acc;
 }
 
-template <typename T, int V>
+template <typename T, int V, int RM = kRLevel>
 __global__ __launch_bounds__(kBlock) void k_spost(Lvl<T> L, const T *__restrict__ zin,
                                                   const T *__restrict__ zc, int wc,
                                                   T *__restrict__ zout, int th,
@@ -903,7 +966,8 @@ __global__ __launch_bounds__(kBlock) void k_spost(Lvl<T> L, const T *__restrict_
                                                   const CgState *__restrict__ st) {
     __shared__ double lds[kBlock / 64];
     if (st && st->done) return;
-    const double acc = spost_band<T, V>(L, zin, zc, wc, zout, th, dot, block_wave(blockIdx.x));
+    const double acc =
+        spost_band<T, V, RM>(L, zin, zc, wc, zout, th, dot, block_wave(blockIdx.x));
     if (dot) {
         const double sum = block_sum<kBlock>(acc, lds);
         if (threadIdx.x == 0) parts[blockIdx.x] = sum;
