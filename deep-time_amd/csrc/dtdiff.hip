@@ -526,6 +526,30 @@ __global__ void k_stream5(long n4, const float4 *__restrict__ a, const float4 *_
                        p.z + q.z * (r.z - t.z * u.z), p.w + q.w * (r.w - t.w * u.w));
 }
 
+// The surface stage's diffusivity before its solve (surface/mod.rs:1982-1992): per cell,
+// `submarine_kd` (marine.rs:44-59) of the water depth sea - h, with the land value
+// kd_land * diffu[class] ashore.  The device's tanh and exp are not glibc's: values agree to
+// ~1e-14 relative where they are used (an ulp of tanh near -1 grows in 1 + tanh), exactly
+// elsewhere.
+__global__ __launch_bounds__(kBlock) void k_fill_kd(long n, const double *__restrict__ h,
+                                                    const unsigned char *__restrict__ cls,
+                                                    const double *__restrict__ diffu,
+                                                    double kd_land, double sea, double d0,
+                                                    double wave_base, double tidal_range,
+                                                    double *__restrict__ kd) {
+    const long c = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const double z = sea - h[c];
+    const double land = kd_land * diffu[cls[c]];
+    double f1;
+    if (tidal_range > 0.0)
+        f1 = (tanh(z / tidal_range) + 1.0) / 2.0;
+    else
+        f1 = z >= 0.0 ? 1.0 : 0.0;
+    const double f2 = (z > wave_base && wave_base > 0.0) ? exp(-(z - wave_base) / wave_base) : 1.0;
+    kd[c] = d0 * f1 * f2 + land * (1.0 - f1);
+}
+
 }  // namespace dtd
 
 #include "batch.hpp"
@@ -681,6 +705,21 @@ int dtd_solve_rel_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x
         st->rel_residual = rel;
         HIPCHK(hipMemcpyAsync(h_dev, ws.x, n * sizeof(double), hipMemcpyDeviceToDevice,
                               ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int dtd_fill_kd(dtd_ctx *ctx, size_t n, const double *h_dev, const unsigned char *class_dev,
+                const double *diffu_dev, double kd_land, double sea, double d0, double wave_base,
+                double tidal_range, double *kd_dev) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (n == 0) return;
+        if (!h_dev || !class_dev || !diffu_dev || !kd_dev) throw Error{DTD_EINVAL, "null array"};
+        if (n > (size_t)INT32_MAX * kBlock) throw Error{DTD_ETOOBIG, "too many cells"};
+        k_fill_kd<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(
+            (long)n, h_dev, class_dev, diffu_dev, kd_land, sea, d0, wave_base, tidal_range, kd_dev);
+        LAUNCH_CHECK();
         HIPCHK(hipStreamSynchronize(ctx->stream));
     });
 }

@@ -39,7 +39,7 @@ SYMBOLS = (
     "dtd_default_opts", "dtd_create", "dtd_destroy", "dtd_last_error", "dtd_solve",
     "dtd_solve_device", "dtd_solve_rel_device", "dtd_operator", "dtd_apply", "dtd_lod",
     "dtd_cycle_new", "dtd_cycle_levels", "dtd_cycle_level", "dtd_cycle_precondition",
-    "dtd_time_kernel", "dtd_solve_batch", "dtd_step_device",
+    "dtd_time_kernel", "dtd_solve_batch", "dtd_step_device", "dtd_fill_kd",
 )
 KERNEL_SWEEP, KERNEL_DIRECTION, KERNEL_UPDATE, KERNEL_RESTRICT = 0, 1, 2, 3
 KERNEL_PRE, KERNEL_POST, KERNEL_COPY, KERNEL_STREAM5, KERNEL_BANDCOPY = 4, 5, 6, 7, 8
@@ -146,6 +146,7 @@ def load(path: str = LIB_PATH):
                                   ctypes.POINTER(ctypes.c_int), _fp, _fp, _fp, _fp, _fp]
     L.dtd_cycle_precondition.argtypes = [_vp, _fp, _dp, _fp, _dp]
     L.dtd_time_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+    L.dtd_fill_kd.argtypes = [_vp, _sz, _vp, _vp, _vp] + [ctypes.c_double] * 5 + [_vp]
     L.dtd_step_device.argtypes = [_vp, _sz, _sz, ctypes.c_double, ctypes.c_int, _vp, _vp,
                                   ctypes.c_double, ctypes.POINTER(Stats)]
     L.dtd_solve_batch.argtypes = [_vp, _sz, ctypes.POINTER(Grid), ctypes.c_double,
@@ -241,6 +242,13 @@ def solve_rel_device(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float, rel_tol: 
     ws.check(load().dtd_solve_rel_device(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x), _ptr(h_dev),
                                          _ptr(kd_dev), dt, rel_tol, ctypes.byref(st)))
     return st
+
+
+def fill_kd(ws: Workspace, n: int, h_dev, class_dev, diffu_dev, kd_land: float, sea: float,
+            d0: float, wave_base: float, tidal_range: float, kd_dev) -> None:
+    """The surface stage's kd fill (surface/mod.rs:1982-1992, `submarine_kd`) on device arrays."""
+    ws.check(load().dtd_fill_kd(ws._ctx, n, _ptr(h_dev), _ptr(class_dev), _ptr(diffu_dev),
+                                kd_land, sea, d0, wave_base, tidal_range, _ptr(kd_dev)))
 
 
 def step_device(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float) -> Stats:

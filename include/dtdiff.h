@@ -143,6 +143,15 @@ int dtd_cycle_level(dtd_ctx *ctx, size_t l, size_t *w, size_t *h, int *wrap, flo
 int dtd_cycle_precondition(dtd_ctx *ctx, const float *r32, const double *r64, float *z32,
                            double *rz);
 
+/* The surface stage's diffusivity fill before its solve (crates/deeptime-core/src/surface/
+ * mod.rs:1982-1992) on device arrays of n cells: kd[c] = `submarine_kd` (surface/marine.rs:44-59)
+ * of the water depth sea - h[c], blending to kd_land * diffu[class[c]] ashore.  diffu is the rock
+ * classes' multiplier table (device, indexed by class).  The device's tanh and exp are not glibc's:
+ * values agree with the CPU's to ~1e-14 relative where they are used, exactly elsewhere. */
+int dtd_fill_kd(dtd_ctx *ctx, size_t n, const double *h_dev, const unsigned char *class_dev,
+                const double *diffu_dev, double kd_land, double sea, double d0, double wave_base,
+                double tidal_range, double *kd_dev);
+
 /* One grid of a batched solve: a dtd_solve_device call's arguments. */
 typedef struct dtd_grid {
     size_t w, h;        /* cells */
