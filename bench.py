@@ -56,7 +56,8 @@ def main():
     ap.add_argument("--solver", choices=["multigrid", "jacobi"], default="multigrid",
                     help="jacobi: the Jacobi-preconditioned step (its own 1e-6 m tolerance)")
     ap.add_argument("--rel", type=float, default=1e-10)
-    ap.add_argument("--cycle", choices=["f32", "f64"], default="f32")
+    ap.add_argument("--cycle", choices=["f32", "f64", "bf16"], default="f32",
+                    help="bf16: the f32 cycle with its two finest levels' records stored in bf16")
     ap.add_argument("--guess", choices=["lod", "rhs"], default="lod")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--kernels-only", action="store_true",
@@ -95,8 +96,8 @@ def main():
     hd = DeviceArray(n)
 
     ws = dtdiff.Workspace(device=0,
-                          cycle_precision=dtdiff.CYCLE_F32 if args.cycle == "f32"
-                          else dtdiff.CYCLE_F64,
+                          cycle_precision={"f32": dtdiff.CYCLE_F32, "f64": dtdiff.CYCLE_F64,
+                                           "bf16": dtdiff.CYCLE_BF16}[args.cycle],
                           guess=dtdiff.GUESS_LOD if args.guess == "lod" else dtdiff.GUESS_RHS)
     def step():
         hd.copy_from(h0d)  # synchronous device-to-device copy
@@ -134,7 +135,7 @@ def main():
 
     kernels = {}
     roof = None
-    if args.cycle == "f32":
+    if args.cycle in ("f32", "bf16"):
         for name, kid in (("sweep_f32", dtdiff.KERNEL_SWEEP),
                           ("direction_f64", dtdiff.KERNEL_DIRECTION),
                           ("update_f64", dtdiff.KERNEL_UPDATE),
@@ -189,7 +190,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (CG, residual, update) + f32 (V-cycle)" if args.cycle == "f32" else "f64",
+        "dtype": {"f32": "f64 (CG, residual, update) + f32 (V-cycle)",
+                  "bf16": "f64 (CG, residual, update) + f32 (V-cycle; bf16 records on the two "
+                          "finest levels)",
+                  "f64": "f64"}[args.cycle],
         "data": "synthetic",
         "config": {"workload": f"diffusion solve {w}x{h}, " + (
                        "log-normal K (sigma 8 cells), dt*mean(K)/dx^2=50"

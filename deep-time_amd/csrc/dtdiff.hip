@@ -63,7 +63,7 @@ static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
     ws.b = dalloc<double>(n);
     ws.x = dalloc<double>(n);
     ws.r = dalloc<double>(n);
-    if (prec == DTD_CYCLE_F32) ws.r2 = dalloc<double>(n);
+    if (prec != DTD_CYCLE_F64) ws.r2 = dalloc<double>(n);
     ws.p0 = dalloc<double>(n);
     ws.p1 = dalloc<double>(n);
     ws.ap = dalloc<double>(n);
@@ -98,6 +98,9 @@ static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
             L.r = (float *)own(nl, 4);
             L.z[0] = (float *)own(nl, 4);
             L.z[1] = (float *)own(nl, 4);
+            // The bf16 variant: the two finest levels (not a coarsest) keep bf16 records too.
+            if (prec == DTD_CYCLE_BF16 && l < 2 && l + 1 < sh.size())
+                L.coefh = (unsigned short *)own(4 * nl, 2);
             ws.lf.push_back(L);
         }
     }
@@ -143,6 +146,16 @@ static void build_coarse(std::vector<Lvl<T>> &lv, T omega, T scale, hipStream_t 
     }
 }
 
+// The bf16 copies of the records of the levels that keep them (after coarsening, which reads the
+// f32 records).
+static void pack_bf16(Workspace &ws, hipStream_t s) {
+    for (auto &L : ws.lf)
+        if (L.coefh) {
+            k_pack_bf16<<<nblocks(L.n), 256, 0, s>>>(L.n, L.coef, L.coefh);
+            LAUNCH_CHECK();
+        }
+}
+
 // `Operator::new` on the device from kd (already on the device), with the cycle's levels.
 static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt) {
     Workspace &ws = ctx->ws;
@@ -156,27 +169,43 @@ static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt)
         build_coarse<double>(ws.ld, (double)ctx->opts.omega, (double)ctx->opts.scale, s);
     else
         build_coarse<float>(ws.lf, ctx->opts.omega, ctx->opts.scale, s);
+    pack_bf16(ws, s);
 }
 
-// Returns the block count (with RM = kRUpdate, the r.r partial sums left in parts).
+// Returns the block count (with RM = kRUpdate, the r.r partial sums left in parts).  A level
+// with bf16 records is smoothed from them (CB = 1).
+template <typename T, int V, int RM, int CB>
+static int launch_pre_cb(Lvl<T> &L, Lvl<T> &C, T *zout, const CgState *st, hipStream_t s,
+                         R64 rio, double *parts) {
+    const int th = band_height<V>((const void *)k_spre<T, V, RM, CB>, L.w, L.h);
+    const int nb = stream_grid<V, LaneGeom<V>::halo>(L.w, L.h, th);
+    k_spre<T, V, RM, CB><<<nb, kBlock, 0, s>>>(L, zout, C, th, st, rio, parts);
+    LAUNCH_CHECK();
+    return nb;
+}
 template <typename T, int V, int RM = kRLevel>
 static int launch_pre(Lvl<T> &L, Lvl<T> &C, T *zout, const CgState *st, hipStream_t s,
                       R64 rio = R64{}, double *parts = nullptr) {
-    const int th = band_height<V>((const void *)k_spre<T, V, RM>, L.w, L.h);
+    if constexpr (std::is_same<T, float>::value)
+        if (L.coefh) return launch_pre_cb<T, V, RM, 1>(L, C, zout, st, s, rio, parts);
+    return launch_pre_cb<T, V, RM, 0>(L, C, zout, st, s, rio, parts);
+}
+
+template <typename T, int V, int RM, int CB>
+static int launch_post_cb(Lvl<T> &L, const T *zin, const T *zc, int wc, T *zout,
+                          const double *dot, double *parts, const CgState *st, hipStream_t s) {
+    const int th = band_height<V>((const void *)k_spost<T, V, RM, CB>, L.w, L.h);
     const int nb = stream_grid<V, LaneGeom<V>::halo>(L.w, L.h, th);
-    k_spre<T, V, RM><<<nb, kBlock, 0, s>>>(L, zout, C, th, st, rio, parts);
+    k_spost<T, V, RM, CB><<<nb, kBlock, 0, s>>>(L, zin, zc, wc, zout, th, dot, parts, st);
     LAUNCH_CHECK();
     return nb;
 }
-
 template <typename T, int V, int RM = kRLevel>
 static int launch_post(Lvl<T> &L, const T *zin, const T *zc, int wc, T *zout, const double *dot,
                        double *parts, const CgState *st, hipStream_t s) {
-    const int th = band_height<V>((const void *)k_spost<T, V, RM>, L.w, L.h);
-    const int nb = stream_grid<V, LaneGeom<V>::halo>(L.w, L.h, th);
-    k_spost<T, V, RM><<<nb, kBlock, 0, s>>>(L, zin, zc, wc, zout, th, dot, parts, st);
-    LAUNCH_CHECK();
-    return nb;
+    if constexpr (std::is_same<T, float>::value)
+        if (L.coefh) return launch_post_cb<T, V, RM, 1>(L, zin, zc, wc, zout, dot, parts, st, s);
+    return launch_post_cb<T, V, RM, 0>(L, zin, zc, wc, zout, dot, parts, st, s);
 }
 
 // The streamed direction pass; returns its block count (the p.Ap partial sums).
@@ -315,7 +344,7 @@ static int run_vcycle(dtd_ctx *ctx, const double *dot, CgState *st, int rm = kRL
 // Whether the f32 cycle's finest level takes the f64 residual straight from the conjugate
 // gradients (fused smoothing; the residual's update rides on the pre-smoothing pass).
 static bool fused_update(const Workspace &ws, const dtd_opts &o) {
-    return ws.prec == DTD_CYCLE_F32 && o.nu == 2 && fused_enabled() && ws.lf.size() > 1;
+    return ws.prec != DTD_CYCLE_F64 && o.nu == 2 && fused_enabled() && ws.lf.size() > 1;
 }
 
 // One conjugate-gradient iteration (MG:630-733): parity 0 reads the direction from p1 and writes
@@ -579,7 +608,8 @@ int dtd_create(const dtd_opts *opts, dtd_ctx **out) {
     else
         dtd_default_opts(&ctx->opts);
     if (ctx->opts.nu < 1 || ctx->opts.max_iter < 1 ||
-        (ctx->opts.cycle_precision != DTD_CYCLE_F32 && ctx->opts.cycle_precision != DTD_CYCLE_F64) ||
+        (ctx->opts.cycle_precision != DTD_CYCLE_F32 && ctx->opts.cycle_precision != DTD_CYCLE_F64 &&
+         ctx->opts.cycle_precision != DTD_CYCLE_BF16) ||
         (ctx->opts.guess != DTD_GUESS_LOD && ctx->opts.guess != DTD_GUESS_RHS)) {
         delete ctx;
         return DTD_EINVAL;
@@ -819,7 +849,7 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
     return guard(ctx, [&] {
         begin(ctx);
         Workspace &ws = ctx->ws;
-        if (ws.n == 0 || ws.prec != DTD_CYCLE_F32 || ws.lf.size() < 2)
+        if (ws.n == 0 || ws.prec == DTD_CYCLE_F64 || ws.lf.size() < 2)
             throw Error{DTD_ESTATE, "time a kernel after an f32-cycle solve of a multi-level grid"};
         if (reps < 1) throw Error{DTD_EINVAL, "reps must be positive"};
         hipStream_t s = ctx->stream;
@@ -855,13 +885,13 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
                 if (L.w % 2 || !ws.r2) throw Error{DTD_ESTATE, "the finest width is odd"};
                 launch_pre<float, 2, kRUpdate>(L, C, L.z[0], ws.st, s,
                                                R64{ws.r, ws.ap, ws.r2, 0.0}, ws.parts);
-                b = 44.0 * n + 4.0 * C.n;
+                b = (L.coefh ? 36.0 : 44.0) * n + 4.0 * C.n;  // bf16 records: 8 B, not 16
                 break;
             case DTD_KERNEL_POST:  // z, r64, record, coarse z in; z out
                 if (L.w % 2) throw Error{DTD_ESTATE, "the finest width is odd"};
                 launch_post<float, 2, kR64>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts, ws.st,
                                             s);
-                b = 32.0 * n + 4.0 * C.n;
+                b = (L.coefh ? 24.0 : 32.0) * n + 4.0 * C.n;
                 break;
             case DTD_KERNEL_BANDCOPY: {  // r, record in; z out, the pre-smoother's walk
                 const int th = band_height<2>((const void *)k_bandcopy<2>, L.w, L.h);
@@ -925,6 +955,7 @@ int dtd_cycle_new(dtd_ctx *ctx, size_t w, size_t h, int wrap_x, const double *ea
                                                      ctx->opts.omega);
             LAUNCH_CHECK();
             build_coarse<float>(ws.lf, ctx->opts.omega, ctx->opts.scale, s);
+            pack_bf16(ws, s);
         }
         HIPCHK(hipStreamSynchronize(s));
         ctx->cycle_levels = ws.prec == DTD_CYCLE_F64 ? ws.ld.size() : ws.lf.size();
@@ -939,7 +970,7 @@ int dtd_cycle_level(dtd_ctx *ctx, size_t l, size_t *w, size_t *h, int *wrap, flo
         begin(ctx);
         if (l >= ctx->cycle_levels) throw Error{DTD_ESTATE, "no such level"};
         Workspace &ws = ctx->ws;
-        if (ws.prec != DTD_CYCLE_F32) throw Error{DTD_ESTATE, "the levels are f64"};
+        if (ws.prec == DTD_CYCLE_F64) throw Error{DTD_ESTATE, "the levels are f64"};
         Lvl<float> &L = ws.lf[l];
         if (w) *w = (size_t)L.w;
         if (h) *h = (size_t)L.h;

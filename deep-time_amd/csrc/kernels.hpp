@@ -326,7 +326,31 @@ struct Lvl {
     T *coef;  // (east, south, diag, omega / diag) of each cell side by side
     T *mass;  // the identity's share of the diagonal; nullptr on the finest level (1 a cell)
     T *r, *z[2];
+    // The same records rounded to bf16 (8 bytes a cell), which the fused smoothing kernels read
+    // instead when given (the bf16-storage variant of the cycle); nullptr otherwise.
+    unsigned short *coefh;
 };
+
+// f32 -> bf16 rounded to nearest, ties to even (a NaN stays a quiet NaN); bf16 -> f32 is exact.
+__host__ __device__ inline unsigned short bf16_of(float f) {
+    unsigned u;
+    __builtin_memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ inline float float_of_bf16(unsigned b) { return __uint_as_float(b << 16); }
+
+// coefh = the level's records rounded to bf16, 4 per cell.
+__global__ void k_pack_bf16(int n, const float *__restrict__ coef,
+                            unsigned short *__restrict__ coefh) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const float4 a = *reinterpret_cast<const float4 *>(coef + 4 * (size_t)c);
+    const unsigned lo = bf16_of(a.x) | ((unsigned)bf16_of(a.y) << 16);
+    const unsigned hi = bf16_of(a.z) | ((unsigned)bf16_of(a.w) << 16);
+    *reinterpret_cast<uint2 *>(coefh + 4 * (size_t)c) = make_uint2(lo, hi);
+}
 
 template <typename T> __device__ inline T c_east(const Lvl<T> &L, int c) { return L.coef[4 * (size_t)c]; }
 template <typename T> __device__ inline T c_south(const Lvl<T> &L, int c) { return L.coef[4 * (size_t)c + 1]; }
@@ -592,6 +616,38 @@ __device__ inline void load_coef(const T *coef, size_t g, T (&e)[V], T (&s)[V], 
     }
 }
 
+// The bf16 records of V consecutive cells (8 or 16 bytes), widened to f32.
+template <int V>
+__device__ inline void load_coef_bf16(const unsigned short *coefh, size_t g, float (&e)[V],
+                                      float (&s)[V], float (&d)[V], float (&wd)[V]) {
+    if constexpr (V == 2) {
+        const uint4 q = *reinterpret_cast<const uint4 *>(coefh + 4 * g);
+        const unsigned a[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int v = 0; v < 2; v++) {
+            e[v] = float_of_bf16(a[2 * v] & 0xffffu), s[v] = float_of_bf16(a[2 * v] >> 16);
+            d[v] = float_of_bf16(a[2 * v + 1] & 0xffffu), wd[v] = float_of_bf16(a[2 * v + 1] >> 16);
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const uint2 q = *reinterpret_cast<const uint2 *>(coefh + 4 * (g + v));
+            e[v] = float_of_bf16(q.x & 0xffffu), s[v] = float_of_bf16(q.x >> 16);
+            d[v] = float_of_bf16(q.y & 0xffffu), wd[v] = float_of_bf16(q.y >> 16);
+        }
+    }
+}
+
+// The records of V cells from the level: its bf16 copy when CB, else its own.
+template <typename T, int V, int CB>
+__device__ inline void load_rec(const Lvl<T> &L, size_t g, T (&e)[V], T (&s)[V], T (&d)[V],
+                                T (&wd)[V]) {
+    if constexpr (CB && sizeof(T) == 4)
+        load_coef_bf16<V>(L.coefh, g, e, s, d, wd);
+    else
+        load_coef<T, V>(L.coef, g, e, s, d, wd);
+}
+
 template <typename T, int V>
 __device__ inline void storev(T *p, const T (&d)[V]) {
     if constexpr (V * sizeof(T) == 8 && V == 2) {
@@ -757,7 +813,7 @@ struct R64 {
 // residual C.r = sum over each 2x2 block of (r - A z2): fine row j0 then j0 + 1, columns
 // ascending, from +0 (MG:332-356).  Rows are loaded kAhead steps before they are consumed.
 // Returns the lane's share of r.r (kRUpdate; else 0).
-template <typename T, int V, int RM = kRLevel>
+template <typename T, int V, int RM = kRLevel, int CB = 0>
 __device__ __forceinline__ double spre_band(const Lvl<T> &L, T *__restrict__ zout,
                                             const Lvl<T> &C, int th, int wave, R64 rio = R64{}) {
     constexpr int A = 4;  // rows in flight
@@ -775,7 +831,7 @@ __device__ __forceinline__ double spre_band(const Lvl<T> &L, T *__restrict__ zou
             if (RM == kRLevel) loadv<T, V>(L.r + g_, qr[u]);                                  \
             if (RM != kRLevel) loadv<double, V>(rio.in + g_, qr64[RM == kRLevel ? 0 : u]);    \
             if (RM == kRUpdate) loadv<double, V>(rio.ap + g_, qap[RM == kRUpdate ? u : 0]);   \
-            load_coef<T, V>(L.coef, g_, qe[u], qs[u], qdg[u], qwd[u]);                        \
+            load_rec<T, V, CB>(L, g_, qe[u], qs[u], qdg[u], qwd[u]);                          \
         } else {                                                                              \
             zero(qr[u]), zero(qwd[u]), zero(qdg[u]), zero(qe[u]), zero(qs[u]);                \
             if (RM != kRLevel) zero(qr64[RM == kRLevel ? 0 : u]);                             \
@@ -859,14 +915,14 @@ __device__ __forceinline__ double spre_band(const Lvl<T> &L, T *__restrict__ zou
 
 // With RM = kRUpdate the residual's update of this iteration rides on the pass (alpha from st)
 // and parts[block] gets the block's r.r.
-template <typename T, int V, int RM = kRLevel>
+template <typename T, int V, int RM = kRLevel, int CB = 0>
 __global__ __launch_bounds__(kBlock) void k_spre(Lvl<T> L, T *__restrict__ zout, Lvl<T> C, int th,
                                                  const CgState *__restrict__ st, R64 rio,
                                                  double *__restrict__ parts) {
     __shared__ double lds[kBlock / 64];
     if (st && st->done) return;
     if (RM == kRUpdate) rio.alpha = st->alpha;
-    const double acc = spre_band<T, V, RM>(L, zout, C, th, block_wave(blockIdx.x), rio);
+    const double acc = spre_band<T, V, RM, CB>(L, zout, C, th, block_wave(blockIdx.x), rio);
     if (RM == kRUpdate) {
         const double sum = block_sum<kBlock>(acc, lds);
         if (threadIdx.x == 0) parts[blockIdx.x] = sum;
@@ -877,7 +933,7 @@ __global__ __launch_bounds__(kBlock) void k_spre(Lvl<T> L, T *__restrict__ zout,
 // z4 = z3 + wd (r - A z3) -> zout; with dot, per block sum dot * z4 into parts[block].
 // Returns the lane's share of sum dot * z4 (0 without dot).  With RM = kR64, r is the f64
 // residual dot itself, rounded to T as it is read.
-template <typename T, int V, int RM = kRLevel>
+template <typename T, int V, int RM = kRLevel, int CB = 0>
 __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restrict__ zin,
                                              const T *__restrict__ zc, int wc,
                                              T *__restrict__ zout, int th,
@@ -899,7 +955,7 @@ __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restric
             else                                                                              \
                 loadv<T, V>(L.r + g_, qr[u]);                                                 \
             loadv<T, V>(zin + g_, qz[u]);                                                     \
-            load_coef<T, V>(L.coef, g_, qe[u], qs[u], qdg[u], qwd[u]);                        \
+            load_rec<T, V, CB>(L, g_, qe[u], qs[u], qdg[u], qwd[u]);                              \
             const T *zcr_ = zc + (size_t)(j_ >> 1) * wc;                                      \
             _Pragma("unroll") for (int v = 0; v < V; v++) qc[u][v] = zcr_[(gi0 + v) >> 1];    \
         } else {                                                                              \
@@ -957,7 +1013,7 @@ __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restric
     return acc;
 }
 
-template <typename T, int V, int RM = kRLevel>
+template <typename T, int V, int RM = kRLevel, int CB = 0>
 __global__ __launch_bounds__(kBlock) void k_spost(Lvl<T> L, const T *__restrict__ zin,
                                                   const T *__restrict__ zc, int wc,
                                                   T *__restrict__ zout, int th,
@@ -967,7 +1023,7 @@ __global__ __launch_bounds__(kBlock) void k_spost(Lvl<T> L, const T *__restrict_
     __shared__ double lds[kBlock / 64];
     if (st && st->done) return;
     const double acc =
-        spost_band<T, V, RM>(L, zin, zc, wc, zout, th, dot, block_wave(blockIdx.x));
+        spost_band<T, V, RM, CB>(L, zin, zc, wc, zout, th, dot, block_wave(blockIdx.x));
     if (dot) {
         const double sum = block_sum<kBlock>(acc, lds);
         if (threadIdx.x == 0) parts[blockIdx.x] = sum;

@@ -31,6 +31,7 @@ DTD_EDEVICE = -3
 DTD_ESTATE = -4
 CYCLE_F32 = 0
 CYCLE_F64 = 1
+CYCLE_BF16 = 2  # the f32 cycle with its two finest levels' records stored in bf16
 GUESS_LOD = 0
 GUESS_RHS = 1
 

@@ -53,6 +53,8 @@ extern "C" {
 /* Precision of the V-cycle that preconditions the f64 conjugate gradients. */
 #define DTD_CYCLE_F32 0  /* the reference's: every level in f32 (multigrid.rs:24-27) */
 #define DTD_CYCLE_F64 1  /* every level in f64 (the plain FP64 multigrid) */
+#define DTD_CYCLE_BF16 2 /* the f32 cycle, its two finest levels' smoothers reading their
+                            operator records stored in bf16 (a reduced-storage variant) */
 
 /* First guess of a solve. */
 #define DTD_GUESS_LOD 0  /* the splitting, as `multigrid_in` (diffusion.rs:110-114) */
