@@ -68,6 +68,8 @@ struct Workspace {
     CgState *st = nullptr;
     std::vector<Lvl<float>> lf;
     std::vector<Lvl<double>> ld;
+    Lvl<float> *d_lf = nullptr;  // device copies of the level tables (the bottom-of-cycle kernel)
+    Lvl<double> *d_ld = nullptr;
     std::vector<void *> owned;  // every allocation of the levels
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     // The Jacobi solve's extra buffers (diag, two row sums a row, its scalars) and graphs.

@@ -104,6 +104,16 @@ static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
             ws.lf.push_back(L);
         }
     }
+    if (!ws.lf.empty()) {
+        ws.d_lf = (Lvl<float> *)own(ws.lf.size() * sizeof(Lvl<float>), 1);
+        HIPCHK(hipMemcpy(ws.d_lf, ws.lf.data(), ws.lf.size() * sizeof(Lvl<float>),
+                         hipMemcpyHostToDevice));
+    }
+    if (!ws.ld.empty()) {
+        ws.d_ld = (Lvl<double> *)own(ws.ld.size() * sizeof(Lvl<double>), 1);
+        HIPCHK(hipMemcpy(ws.d_ld, ws.ld.data(), ws.ld.size() * sizeof(Lvl<double>),
+                         hipMemcpyHostToDevice));
+    }
 }
 
 static Grid grid_of(const Workspace &ws) { return Grid{ws.w, ws.h, ws.wrap, (int)ws.n}; }
@@ -241,6 +251,16 @@ static int launch_resid(Workspace &ws, const double *b, const double *xin, hipSt
     return nb;
 }
 
+// Levels of at most this many cells (below the finest) are cycled by one block, k_bottom
+// (DTD_BOTTOM_CELLS overrides it; 0 turns it off).
+static int bottom_cells() {
+    static const int v = [] {
+        const char *e = getenv("DTD_BOTTOM_CELLS");
+        return e ? atoi(e) : 1024;
+    }();
+    return v;
+}
+
 // The fused smoothing kernels serve every level but the coarsest when nu = 2 (DTD_NO_FUSE=1
 // turns them off, for comparison).
 static bool fused_enabled() {
@@ -260,8 +280,15 @@ static bool fused_enabled() {
 // rio.out = dot, with the r.r step of the conjugate gradients launched after the pre-smoothing).
 template <typename T>
 static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, double *parts,
-                  CgState *st, hipStream_t s, int rm = kRLevel, R64 rio = R64{}) {
+                  CgState *st, hipStream_t s, int rm = kRLevel, R64 rio = R64{},
+                  const Lvl<T> *dlv = nullptr) {
     Lvl<T> &L = lv[l];
+    if (dlv && l >= 1 && nu == 2 && fused_enabled() && L.n <= bottom_cells() && !L.coefh) {
+        // This level and every coarser one in one block.
+        k_bottom<T><<<1, kBottomBlock, 0, s>>>(dlv, (int)l, (int)lv.size(), st);
+        LAUNCH_CHECK();
+        return 1;
+    }
     if (l + 1 == lv.size()) {
         if (L.n > 64) throw Error{DTD_ESTATE, "the coarsest level exceeds one wave"};
         k_coarsest<T><<<1, 64, 0, s>>>(L, L.z[1], 16, dot, parts, st);
@@ -289,7 +316,7 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
                 } else {
                     launch_pre<T, 1, kR64>(L, C0, L.z[0], st, s, rio);
                 }
-                vcycle<T>(lv, l + 1, nu, nullptr, parts, st, s);
+                vcycle<T>(lv, l + 1, nu, nullptr, parts, st, s, kRLevel, R64{}, dlv);
                 return wide ? launch_post<T, VW, kR64>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts,
                                                         st, s)
                             : launch_post<T, 1, kR64>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts,
@@ -300,7 +327,7 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
             launch_pre<T, VW>(L, C0, L.z[0], st, s);
         else
             launch_pre<T, 1>(L, C0, L.z[0], st, s);
-        vcycle<T>(lv, l + 1, nu, nullptr, parts, st, s);
+        vcycle<T>(lv, l + 1, nu, nullptr, parts, st, s, kRLevel, R64{}, dlv);
         return wide ? launch_post<T, VW>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts, st, s)
                     : launch_post<T, 1>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts, st, s);
     }
@@ -315,7 +342,7 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
     }
     k_restrict<T><<<nblocks(C0.n), kBlock, 0, s>>>(L, L.z[cur], C0, st);
     LAUNCH_CHECK();
-    vcycle<T>(lv, l + 1, nu, nullptr, parts, st, s);
+    vcycle<T>(lv, l + 1, nu, nullptr, parts, st, s, kRLevel, R64{}, dlv);
     {
         const double *d = nu == 1 ? dot : nullptr;
         k_sweep<T><<<nb, kBlock, 0, s>>>(L, L.z[cur], L.z[cur ^ 1], C0.z[1], C0.w, d, parts, st);
@@ -337,8 +364,9 @@ static int run_vcycle(dtd_ctx *ctx, const double *dot, CgState *st, int rm = kRL
     Workspace &ws = ctx->ws;
     const int nu = (int)ctx->opts.nu;
     if (ws.prec == DTD_CYCLE_F64)
-        return vcycle<double>(ws.ld, 0, nu, dot, ws.parts, st, ctx->stream);
-    return vcycle<float>(ws.lf, 0, nu, dot, ws.parts, st, ctx->stream, rm, rio);
+        return vcycle<double>(ws.ld, 0, nu, dot, ws.parts, st, ctx->stream, kRLevel, R64{},
+                              ws.d_ld);
+    return vcycle<float>(ws.lf, 0, nu, dot, ws.parts, st, ctx->stream, rm, rio, ws.d_lf);
 }
 
 // Whether the f32 cycle's finest level takes the f64 residual straight from the conjugate
@@ -1004,10 +1032,12 @@ int dtd_cycle_precondition(dtd_ctx *ctx, const float *r32, const double *r64, fl
         int nbz;
         if (ws.prec == DTD_CYCLE_F64) {
             // An f64 cycle reads the f64 residual; r32 is its value rounded, kept for the ABI.
-            nbz = vcycle<double>(ws.ld, 0, (int)ctx->opts.nu, ws.r, ws.parts, nullptr, s);
+            nbz = vcycle<double>(ws.ld, 0, (int)ctx->opts.nu, ws.r, ws.parts, nullptr, s, kRLevel,
+                                 R64{}, ws.d_ld);
         } else {
             upload(ctx, ws.lf[0].r, r32, n * sizeof(float));
-            nbz = vcycle<float>(ws.lf, 0, (int)ctx->opts.nu, ws.r, ws.parts, nullptr, s);
+            nbz = vcycle<float>(ws.lf, 0, (int)ctx->opts.nu, ws.r, ws.parts, nullptr, s, kRLevel,
+                                R64{}, ws.d_lf);
         }
         k_sum1<<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.pair);
         LAUNCH_CHECK();

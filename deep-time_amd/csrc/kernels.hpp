@@ -460,6 +460,37 @@ __global__ void k_first(Lvl<T> L, T *__restrict__ zout, const CgState *__restric
     zout[c] = c_wd(L, c) * L.r[c];
 }
 
+// One cell of `Level::sweep` (MG:260-329): z' + wd (r - A z') with z' = zin + zc[j/2][i/2] when
+// zc is given.
+template <typename T>
+__device__ inline T sweep_cell(const Lvl<T> &L, const T *__restrict__ zin,
+                               const T *__restrict__ zc, int wc, int c) {
+    const int j = c / L.w, i = c - j * L.w;
+    auto zp = [&](int k) -> T {
+        if (!zc) return zin[k];
+        const int jj = k / L.w, ii = k - jj * L.w;
+        return zin[k] + zc[(jj >> 1) * wc + (ii >> 1)];
+    };
+    const T ax = lvl_apply(L, c, i, j, zp);
+    return zp(c) + c_wd(L, c) * (L.r[c] - ax);
+}
+
+// One coarse cell of `Level::restrict_into` (MG:332-356): the sum over its 2 x 2 fine block of
+// (r - A z), fine row j0 then j0 + 1, columns ascending, from +0.
+template <typename T>
+__device__ inline T restrict_cell(const Lvl<T> &f, const T *__restrict__ z, const Lvl<T> &c,
+                                  int k) {
+    const int jc = k / c.w, ic = k - jc * c.w;
+    const int r1 = min(2 * jc + 2, f.h), c1 = min(2 * ic + 2, f.w);
+    T acc = (T)0;
+    for (int j = 2 * jc; j < r1; j++)
+        for (int i = 2 * ic; i < c1; i++) {
+            const int cf = j * f.w + i;
+            acc += f.r[cf] - lvl_apply(f, cf, i, j, [&](int q) { return z[q]; });
+        }
+    return acc;
+}
+
 // `Level::sweep` (MG:260-329): zout = z' + wd (r - A z') with z' = zin + zc[j/2][i/2] when zc is
 // given; with dot, per block sum dot * zout.
 template <typename T>
@@ -554,6 +585,43 @@ __global__ void k_coarsest(Lvl<T> L, T *__restrict__ zout, int sweeps,
     __shared__ T zs[2][64];
     if (st && st->done) return;
     coarsest_wave<T>(L, zout, sweeps, dot, parts, zs);
+}
+
+// The bottom of the V-cycle (nu = 2) in one block: from level b0 down, each level's first sweep,
+// second sweep and restriction; the coarsest level's 16 sweeps; then back up, each level's
+// corrected sweep and last sweep (MG:500-523).  Every phase is one pass of the block over the
+// level's cells with the one-sweep kernels' arithmetic, so the result is theirs bit for bit; a
+// barrier separates the phases.  Each level's result lands in z[1], as the launch-per-sweep cycle
+// leaves it.  lv holds the hierarchy's levels (device copy); nl is their count.
+constexpr int kBottomBlock = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(kBottomBlock) void k_bottom(const Lvl<T> *__restrict__ lv, int b0,
+                                                         int nl, const CgState *__restrict__ st) {
+    __shared__ T zs[2][64];
+    if (st && st->done) return;
+    const int t = threadIdx.x;
+    for (int l = b0; l + 1 < nl; l++) {
+        const Lvl<T> L = lv[l], C = lv[l + 1];
+        for (int c = t; c < L.n; c += kBottomBlock) L.z[0][c] = c_wd(L, c) * L.r[c];
+        __syncthreads();
+        for (int c = t; c < L.n; c += kBottomBlock) L.z[1][c] = sweep_cell(L, L.z[0], (const T *)nullptr, 0, c);
+        __syncthreads();
+        for (int k = t; k < C.n; k += kBottomBlock) C.r[k] = restrict_cell(L, L.z[1], C, k);
+        __syncthreads();
+    }
+    {
+        const Lvl<T> L = lv[nl - 1];
+        coarsest_wave<T>(L, L.z[1], 16, nullptr, nullptr, zs);
+        __syncthreads();
+    }
+    for (int l = nl - 2; l >= b0; l--) {
+        const Lvl<T> L = lv[l], C = lv[l + 1];
+        for (int c = t; c < L.n; c += kBottomBlock) L.z[0][c] = sweep_cell(L, L.z[1], C.z[1], C.w, c);
+        __syncthreads();
+        for (int c = t; c < L.n; c += kBottomBlock) L.z[1][c] = sweep_cell(L, L.z[0], (const T *)nullptr, 0, c);
+        __syncthreads();
+    }
 }
 
 // ---- fused, register-streaming smoothing of a level (nu = 2) -------------------------------
@@ -1634,7 +1702,7 @@ __global__ __launch_bounds__(MAXT) void k_lod_lines(int n, int ring,
                                                     const double *__restrict__ src,
                                                     double *__restrict__ dst, double dt,
                                                     double s2) {
-    __shared__ double A[2 * MAXT], B[2 * MAXT], Cc[2 * MAXT], R[2 * MAXT];
+    __shared__ double A[2 * MAXT], B[2 * MAXT], Cc[2 * MAXT], R[2 * MAXT], Bi[2 * MAXT];
     const int P = n / kSeg, N = 2 * P, t = threadIdx.x;
     const size_t base = (size_t)blockIdx.x * n;
     const double *k = kd + base;
@@ -1651,10 +1719,12 @@ __global__ __launch_bounds__(MAXT) void k_lod_lines(int n, int ring,
     const bool has_w = a > 0 || ring, has_e = b + 1 < n || ring;
     kk[0] = has_w ? k[a > 0 ? a - 1 : n - 1] : 0.0;
     kk[kSeg + 1] = has_e ? k[b + 1 < n ? b + 1 : 0] : 0.0;
-    // face(i) for i = a - 1 .. b: f[q] = face(a - 1 + q), the reference's expression.
+    // face(i) for i = a - 1 .. b: f[q] = face(a - 1 + q), with dt / dx^2 divided once (this
+    // solver is a first guess: it need not repeat the reference's rounding, only its system).
+    const double dts2 = dt / s2;
     double f[kSeg + 1];
 #pragma unroll
-    for (int q = 0; q <= kSeg; q++) f[q] = 0.5 * (kk[q] + kk[q + 1]) * dt / s2;
+    for (int q = 0; q <= kSeg; q++) f[q] = 0.5 * (kk[q] + kk[q + 1]) * dts2;
     if (!has_w) f[0] = 0.0;
     if (!has_e) f[kSeg] = 0.0;
     double d[kSeg];
@@ -1667,16 +1737,16 @@ __global__ __launch_bounds__(MAXT) void k_lod_lines(int n, int ring,
     // Forward over the interior rows 1 .. 14, against x_a.
     double al[kSeg], ga[kSeg], de[kSeg];
     {
-        const double lo = -f[1], up = -f[2], di = 1.0 + f[1] + f[2];
-        al[1] = lo / di, ga[1] = up / di, de[1] = d[1] / di;
+        const double lo = -f[1], up = -f[2], ri = 1.0 / (1.0 + f[1] + f[2]);
+        al[1] = lo * ri, ga[1] = up * ri, de[1] = d[1] * ri;
     }
 #pragma unroll
     for (int q = 2; q <= kSeg - 2; q++) {
         const double lo = -f[q], up = -f[q + 1], di = 1.0 + f[q] + f[q + 1];
-        const double r = di - lo * ga[q - 1];
-        al[q] = -lo * al[q - 1] / r;
-        ga[q] = up / r;
-        de[q] = (d[q] - lo * de[q - 1]) / r;
+        const double ri = 1.0 / (di - lo * ga[q - 1]);
+        al[q] = -lo * al[q - 1] * ri;
+        ga[q] = up * ri;
+        de[q] = (d[q] - lo * de[q - 1]) * ri;
     }
     // Backward: x_q = de[q] - al[q] x_a - ga[q] x_b.
 #pragma unroll
@@ -1698,6 +1768,8 @@ __global__ __launch_bounds__(MAXT) void k_lod_lines(int n, int ring,
         B[2 * t + 1] = di15 - lo15 * ga[kSeg - 2];
         Cc[2 * t + 1] = up15;
         R[2 * t + 1] = d[kSeg - 1] - lo15 * de[kSeg - 2];
+        Bi[2 * t] = 1.0 / B[2 * t];
+        Bi[2 * t + 1] = 1.0 / B[2 * t + 1];
     }
     __syncthreads();
     // Parallel cyclic reduction: after the step of stride s every equation couples the unknowns
@@ -1709,13 +1781,13 @@ __global__ __launch_bounds__(MAXT) void k_lod_lines(int n, int ring,
         for (int e = 0; e < 2; e++) {
             const int i = 2 * t + e;
             int im = i - s, ip = i + s;
-            double a_m = 0, b_m = 1, c_m = 0, r_m = 0, a_p = 0, b_p = 1, c_p = 0, r_p = 0;
+            double a_m = 0, bi_m = 1, c_m = 0, r_m = 0, a_p = 0, bi_p = 1, c_p = 0, r_p = 0;
             if (ring) {
                 im = (im + N) % N, ip = ip % N;
             }
-            if (im >= 0) a_m = A[im], b_m = B[im], c_m = Cc[im], r_m = R[im];
-            if (ip < N) a_p = A[ip], b_p = B[ip], c_p = Cc[ip], r_p = R[ip];
-            const double k1 = A[i] / b_m, k2 = Cc[i] / b_p;
+            if (im >= 0) a_m = A[im], bi_m = Bi[im], c_m = Cc[im], r_m = R[im];
+            if (ip < N) a_p = A[ip], bi_p = Bi[ip], c_p = Cc[ip], r_p = R[ip];
+            const double k1 = A[i] * bi_m, k2 = Cc[i] * bi_p;
             na[e] = -a_m * k1;
             nc[e] = -c_p * k2;
             nb[e] = B[i] - c_m * k1 - a_p * k2;
@@ -1725,7 +1797,7 @@ __global__ __launch_bounds__(MAXT) void k_lod_lines(int n, int ring,
 #pragma unroll
         for (int e = 0; e < 2; e++) {
             const int i = 2 * t + e;
-            A[i] = na[e], B[i] = nb[e], Cc[i] = nc[e], R[i] = nr[e];
+            A[i] = na[e], B[i] = nb[e], Cc[i] = nc[e], R[i] = nr[e], Bi[i] = 1.0 / nb[e];
         }
         __syncthreads();
     }
