@@ -154,8 +154,8 @@ def main():
         sw = kernels["post_smooth_f32"]
         roof = {"bound": "hbm", "achieved": sw["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": sw["GB/s"] / HBM_PEAK_GBS,
-                "traffic": traffic_of("k_spost<float, 2, 1>", w, h),
-                "kernel": "k_spost<float, 2, 1> (finest level: two damped-Jacobi sweeps + r.z; "
+                "traffic": traffic_of("k_spost<float, 2, 1, 0>", w, h),
+                "kernel": "k_spost<float, 2, 1, 0> (finest level: two damped-Jacobi sweeps + r.z; "
                           "r64, the level record, z and z_coarse in, z out: 33 B/cell)"}
 
     cpu = None

@@ -1037,6 +1037,9 @@ __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restric
     T ea[V], eb[V], ec[V], sa[V], sb[V], sc[V], sd[V];
     zero(zpa), zero(zpb), zero(z3b), zero(z3c), zero(ra), zero(rb), zero(wda), zero(wdb);
     zero(dga), zero(dgb), zero(ea), zero(eb), zero(sa), zero(sb), zero(sc);
+    // With RM = kR64 the f64 residual of rows a, b, c stays in registers for the dot of row c.
+    double r64a[RM == kR64 ? V : 1], r64b[RM == kR64 ? V : 1], r64c[RM == kR64 ? V : 1];
+    zero(r64a), zero(r64b), zero(r64c);
     double acc = 0.0;
     const int steps = bd.active ? bd.rows + 4 : 0;
 #pragma unroll
@@ -1051,8 +1054,12 @@ __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restric
             copy(wda, qwd[u]), copy(dga, qdg[u]), copy(ea, qe[u]);
             copy(sa, qs[u]);
             if (RM == kR64) {
+                copy(r64c, r64b), copy(r64b, r64a);
 #pragma unroll
-                for (int v = 0; v < V; v++) ra[v] = (T)qr64[RM == kR64 ? u : 0][v];
+                for (int v = 0; v < V; v++) {
+                    r64a[v] = qr64[RM == kR64 ? u : 0][v];
+                    ra[v] = (T)r64a[v];
+                }
             } else {
                 copy(ra, qr[u]);
             }
@@ -1070,7 +1077,10 @@ __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restric
             if (bd.own && okc && jc >= bd.row0) {
                 const size_t g = (size_t)jc * w + gi0;
                 storev<T, V>(zout + g, z4);
-                if (dot) {
+                if (RM == kR64) {
+#pragma unroll
+                    for (int v = 0; v < V; v++) acc += r64c[v] * (double)z4[v];
+                } else if (dot) {
 #pragma unroll
                     for (int v = 0; v < V; v++) acc += dot[g + v] * (double)z4[v];
                 }
