@@ -164,7 +164,8 @@ __global__ __launch_bounds__(kBlock) void kb_sdir(const BGrid *__restrict__ t,
     const double *pold = parity == 0 ? q.p1 : q.p0;
     double *pnew = parity == 0 ? q.p0 : q.p1;
     const double acc =
-        sdir_band<float>(q.g, q.es, q.z32, pold, pnew, q.ap, q.x, q.th, q.st, block_wave(m.y));
+        sdir_band<float>(q.g, q.kd, q.dt, q.s2, q.z32, pold, pnew, q.ap, q.x, q.th, q.st,
+                         block_wave(m.y));
     const double sum = block_sum<kBlock>(acc, lds);
     if (threadIdx.x == 0) parts[blockIdx.x] = sum;
 }

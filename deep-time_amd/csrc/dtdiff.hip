@@ -124,25 +124,33 @@ __global__ void k_sum1(const double *__restrict__ parts, int nparts, double *out
     if (threadIdx.x == 0) out[0] = v;
 }
 
-// A fresh state for a solve: tol2 (or, with rel2 > 0, tol2 = rel2 * b.b from the first residual)
-// and the iteration cap.
-__global__ void k_init_state(CgState *st, double tol2, double rel2, int max_iter) {
+// A fresh state for a solve: tol2 (or, with rel2 > 0, tol2 = rel2 * b.b from the first residual),
+// the iteration cap, and the fields the operator's faces come from.
+__global__ void k_init_state(CgState *st, double tol2, double rel2, int max_iter, const double *kd,
+                             double dt, double s2) {
     if (threadIdx.x != 0) return;
     CgState s{};
     s.tol2 = tol2;
     s.rel2 = rel2;
     s.max_iter = max_iter;
+    s.kd = kd;
+    s.dt = dt;
+    s.s2 = s2;
     *st = s;
 }
 
 // A state that never stops and takes every branch of the passes (beta and the pending update on,
-// both zero), for timing the loop's kernels on resident data without changing it.
-__global__ void k_time_state(CgState *st) {
+// both zero), for timing the loop's kernels on resident data without changing it (the faces from
+// the workspace's own kd buffer, whatever it holds).
+__global__ void k_time_state(CgState *st, const double *kd) {
     if (threadIdx.x != 0) return;
     CgState s{};
     s.max_iter = 1 << 30;
     s.have_beta = 1;
     s.have_pending = 1;
+    s.kd = kd;
+    s.dt = 1.0;
+    s.s2 = 1.0;
     *st = s;
 }
 
@@ -224,8 +232,7 @@ static int launch_dir(Workspace &ws, const TZ *z, const double *pold, double *pn
                       hipStream_t s) {
     const int th = band_height<1, 1>((const void *)k_sdir<TZ>, ws.w, ws.h);
     const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
-    k_sdir<TZ><<<nb, kBlock, 0, s>>>(grid_of(ws), ws.es, z, pold, pnew, ws.ap, ws.x, th, ws.parts,
-                                     ws.st);
+    k_sdir<TZ><<<nb, kBlock, 0, s>>>(grid_of(ws), z, pold, pnew, ws.ap, ws.x, th, ws.parts, ws.st);
     LAUNCH_CHECK();
     return nb;
 }
@@ -505,7 +512,7 @@ static size_t solve_core(dtd_ctx *ctx, const double *b, const double *kd, double
     }
     HIPCHK(hipEventRecord(ctx->ev[2], s));
     k_init_state<<<1, 64, 0, s>>>(ws.st, tol_m * tol_m * (double)ws.n, rel2,
-                                  (int)ctx->opts.max_iter);
+                                  (int)ctx->opts.max_iter, kd, dt, dx * dx);
     LAUNCH_CHECK();
     const bool fused = fused_update(ws, ctx->opts);
     const int nbr = launch_resid(ws, b, xin, s, !fused);
@@ -887,7 +894,7 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
         const int nbc = nblocks(ws.n, kCgCells);
         const double n = (double)ws.n;
         double b = 0.0;
-        k_time_state<<<1, 64, 0, s>>>(ws.st);
+        k_time_state<<<1, 64, 0, s>>>(ws.st, ws.kd);
         LAUNCH_CHECK();
         auto one = [&] {
             switch (kernel) {
@@ -896,9 +903,9 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
                                                      nullptr, ws.st);
                 b = 28.0 * n;
                 break;
-            case DTD_KERNEL_DIRECTION:  // z (f32), p_old, x, (east, south) in; p, A p, x out
+            case DTD_KERNEL_DIRECTION:  // z (f32), p_old, x, kd in; p, A p, x out
                 launch_dir<float>(ws, L.z[1], ws.p1, ws.p0, s);
-                b = (4.0 + 8.0 * 4 + 8.0 * 3) * n;
+                b = (4.0 + 8.0 * 3 + 8.0 * 3) * n;
                 break;
             case DTD_KERNEL_UPDATE:  // A p, r in; r, r32 out
                 k_update<float><<<nbc, kBlock, 0, s>>>((int)ws.n, ws.ap, ws.r, L.r, ws.parts,

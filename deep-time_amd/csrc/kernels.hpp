@@ -40,6 +40,10 @@ struct CgState {
     int max_iter;
     int skip;       // a batched solve leaves this grid alone (its answer is final)
     int pad[2];
+    // This solve's kd (device) and dt, dx^2: the direction pass forms the operator's faces from
+    // them (set per solve, so a captured iteration stays valid for new fields of the shape).
+    const double *kd;
+    double dt, s2;
 };
 
 // ---- reductions ----------------------------------------------------------------------------
@@ -1157,8 +1161,8 @@ __global__ __launch_bounds__(kBlock) void k_bandcopy(Lvl<float> L, float *__rest
 
 // Returns the lane's share of p . A p.
 template <typename TZ>
-__device__ __forceinline__ double sdir_band(const Grid &g, const double2 *__restrict__ es,
-                                            const TZ *__restrict__ z,
+__device__ __forceinline__ double sdir_band(const Grid &g, const double *__restrict__ kd,
+                                            double dt, double s2, const TZ *__restrict__ z,
                                             const double *__restrict__ pold,
                                             double *__restrict__ pnew, double *__restrict__ ap,
                                             double *__restrict__ x, int th,
@@ -1171,8 +1175,10 @@ __device__ __forceinline__ double sdir_band(const Grid &g, const double2 *__rest
     const int gi = bd.gi0;
     const bool inner_col = bd.col_ok && gi >= 1 && gi + 1 <= w - 1;
     // Raw loads only: p, and the owned cells' x update, are formed when the row is consumed, so
-    // the loads of a row stay in flight for A steps.
-    double qe[A], qs[A], qpo[A], qx[A];
+    // the loads of a row stay in flight for A steps.  The operator's faces are formed from kd
+    // with the reference's expression (`face`), the east one from the east lane's kd and the
+    // south one of row b when row a (below it) arrives: the same bits as Operator::new's.
+    double qk[A], qpo[A], qx[A];
     TZ qz[A];
     const bool need_old = have_beta || pending;
 #define DTD_FETCH_DIR(j, u)                                                                   \
@@ -1180,16 +1186,16 @@ __device__ __forceinline__ double sdir_band(const Grid &g, const double2 *__rest
         const int j_ = (j);                                                                   \
         if (j_ >= 0 && j_ < h && bd.col_ok && j_ <= bd.row0 + bd.rows) {                      \
             const size_t g_ = (size_t)j_ * w + gi;                                            \
-            const double2 c_ = es[g_];                                                        \
-            qe[u] = c_.x, qs[u] = c_.y, qz[u] = z[g_];                                        \
+            qk[u] = kd[g_], qz[u] = z[g_];                                                    \
             qpo[u] = need_old ? pold[g_] : 0.0;                                               \
             qx[u] = pending && bd.own ? x[g_] : 0.0;                                          \
         } else {                                                                              \
-            qe[u] = qs[u] = qpo[u] = qx[u] = 0.0;                                             \
+            qk[u] = qpo[u] = qx[u] = 0.0;                                                     \
             qz[u] = (TZ)0;                                                                    \
         }                                                                                     \
     } while (0)
-    double pa = 0, pb = 0, pc = 0, eb = 0, ea = 0, sa = 0, sb = 0, sc = 0;
+    const bool has_e = gi + 1 < w || wrap;
+    double pa = 0, pb = 0, pc = 0, eb = 0, ea = 0, sb = 0, sc = 0, ka = 0;
     double acc = 0.0;
     const int steps = bd.active ? bd.rows + 4 : 0;  // rows row0 - 1 .. row0 + rows (+2 spare)
 #pragma unroll
@@ -1198,8 +1204,12 @@ __device__ __forceinline__ double sdir_band(const Grid &g, const double2 *__rest
 #pragma unroll
         for (int u = 0; u < A; u++) {
             const int ja = bd.row0 - 1 + k0 + u, jb = ja - 1;
-            pc = pb, pb = pa, eb = ea, sc = sb, sb = sa;
-            ea = qe[u], sa = qs[u];
+            pc = pb, pb = pa, eb = ea, sc = sb;
+            const double kb = ka;
+            ka = qk[u];
+            const double ke = from_east(ka);
+            ea = has_e ? face(ka, ke, dt, s2) : 0.0;
+            sb = ja < h ? face(kb, ka, dt, s2) : 0.0;  // row b's south face
             const double po = qpo[u], xo = qx[u];
             pa = have_beta ? (double)qz[u] + beta * po : (double)qz[u];
             if (k0 + u + A < steps) DTD_FETCH_DIR(ja + A, u);
@@ -1239,8 +1249,7 @@ __device__ __forceinline__ double sdir_band(const Grid &g, const double2 *__rest
 }
 
 template <typename TZ>
-__global__ __launch_bounds__(kBlock) void k_sdir(Grid g, const double2 *__restrict__ es,
-                                                 const TZ *__restrict__ z,
+__global__ __launch_bounds__(kBlock) void k_sdir(Grid g, const TZ *__restrict__ z,
                                                  const double *__restrict__ pold,
                                                  double *__restrict__ pnew, double *__restrict__ ap,
                                                  double *__restrict__ x, int th,
@@ -1248,7 +1257,8 @@ __global__ __launch_bounds__(kBlock) void k_sdir(Grid g, const double2 *__restri
                                                  const CgState *__restrict__ st) {
     __shared__ double lds[kBlock / 64];
     if (st->done) return;
-    const double acc = sdir_band<TZ>(g, es, z, pold, pnew, ap, x, th, st, block_wave(blockIdx.x));
+    const double acc = sdir_band<TZ>(g, st->kd, st->dt, st->s2, z, pold, pnew, ap, x, th, st,
+                                     block_wave(blockIdx.x));
     const double sum = block_sum<kBlock>(acc, lds);
     if (threadIdx.x == 0) parts[blockIdx.x] = sum;
 }
