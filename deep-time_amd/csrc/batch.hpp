@@ -33,7 +33,6 @@ struct BGrid {  // one grid's f64 side
     const double *kd, *b;  // the caller's arrays (b: h on entry, read until the end)
     double *h;             // the caller's array, written last
     double *x, *r, *p0, *p1, *ap;
-    double2 *es;
     float *r32, *z32, *coef32;  // the finest f32 level
     CgState *st;
 };
@@ -61,7 +60,7 @@ __global__ __launch_bounds__(kBlock) void kb_operator(const BGrid *__restrict__ 
     const int2 m = blk[blockIdx.x];
     const BGrid &q = t[m.x];
     const int c = m.y * kBlock + threadIdx.x;
-    if (c < q.g.n) operator_cell(q.g, q.kd, q.dt, q.s2, q.es, q.coef32, nullptr, omega, c);
+    if (c < q.g.n) operator_cell(q.g, q.kd, q.dt, q.s2, nullptr, q.coef32, nullptr, omega, c);
 }
 
 // Level l + 1 from level l, one coarse cell a thread (t: the grids' level l).
@@ -138,8 +137,8 @@ __global__ __launch_bounds__(kBlock) void kb_sresid(const BGrid *__restrict__ t,
     const int2 m = blk[blockIdx.x];
     const BGrid &q = t[m.x];
     if (q.st->skip) return;
-    block_pair(sresid_band<float>(q.g, q.es, q.b, from_b ? q.b : q.x, q.x, q.r, q.r32, q.th,
-                                  block_wave(m.y)),
+    block_pair(sresid_band<float>(q.g, q.kd, q.dt, q.s2, q.b, from_b ? q.b : q.x, q.x, q.r, q.r32,
+                                  q.th, block_wave(m.y)),
                parts, blockIdx.x, lds);
 }
 
@@ -224,7 +223,8 @@ __global__ __launch_bounds__(kBlock) void kb_finish1(const BGrid *__restrict__ t
     const int2 m = blk[blockIdx.x];
     const BGrid &q = t[m.x];
     if (q.st->skip) return;
-    block_pair(finish1_band(q.g, q.es, q.b, q.x, q.p0, q.p1, q.ap, q.th, q.st, block_wave(m.y)),
+    block_pair(finish1_band(q.g, q.kd, q.dt, q.s2, q.b, q.x, q.p0, q.p1, q.ap, q.th, q.st,
+                            block_wave(m.y)),
                parts, blockIdx.x, lds);
 }
 
@@ -235,8 +235,8 @@ __global__ __launch_bounds__(kBlock) void kb_finish2(const BGrid *__restrict__ t
     const int2 m = blk[blockIdx.x];
     const BGrid &q = t[m.x];
     if (q.st->skip) return;
-    block_pair(finish2_band(q.g, q.es, q.b, q.ap, q.x, q.th, q.st, block_wave(m.y)), parts,
-               blockIdx.x, lds);
+    block_pair(finish2_band(q.g, q.kd, q.dt, q.s2, q.b, q.ap, q.x, q.th, q.st, block_wave(m.y)),
+               parts, blockIdx.x, lds);
 }
 
 // The put-back's sums into each grid's state; then each grid's (r.r, b.b) of the true residual.
@@ -376,7 +376,6 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
     double *x = (double *)own(total * 8), *r = (double *)own(total * 8);
     double *p0 = (double *)own(total * 8), *p1 = (double *)own(total * 8);
     double *ap = (double *)own(total * 8);
-    double2 *es = (double2 *)own(total * 16);
     bw->st = (CgState *)own(n * sizeof(CgState));
     bw->init = (BInit *)own(n * sizeof(BInit));
     bw->pairs = (double *)own(2 * n * sizeof(double));
@@ -413,7 +412,7 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
         q.g = Grid{w, h, (int)(gs[g].wrap_x && w > 1), w * h};
         q.th = batch_band(h);
         const size_t o = off[g];
-        q.x = x + o, q.r = r + o, q.p0 = p0 + o, q.p1 = p1 + o, q.ap = ap + o, q.es = es + o;
+        q.x = x + o, q.r = r + o, q.p0 = p0 + o, q.p1 = p1 + o, q.ap = ap + o;
         const Lvl<float> L0 = level(g, 0);
         q.r32 = L0.r, q.z32 = L0.z[1], q.coef32 = L0.coef;
         q.st = bw->st + g;

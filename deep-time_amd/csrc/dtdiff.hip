@@ -179,7 +179,8 @@ static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt)
     Workspace &ws = ctx->ws;
     hipStream_t s = ctx->stream;
     const bool f64 = ws.prec == DTD_CYCLE_F64;
-    k_operator<<<nblocks(ws.n), kBlock, 0, s>>>(grid_of(ws), kd, dt, dx * dx, ws.es,
+    // The f64 passes form the faces from kd themselves; only the cycle's finest records are kept.
+    k_operator<<<nblocks(ws.n), kBlock, 0, s>>>(grid_of(ws), kd, dt, dx * dx, nullptr,
                                                 f64 ? nullptr : ws.lf[0].coef,
                                                 f64 ? ws.ld[0].coef : nullptr, ctx->opts.omega);
     LAUNCH_CHECK();
@@ -239,20 +240,20 @@ static int launch_dir(Workspace &ws, const TZ *z, const double *pold, double *pn
 
 // The first residual r = b - A xin (xout = xin's copy when they differ); returns the block count
 // (the (r.r, b.b) partial pairs).
-static int launch_resid(Workspace &ws, const double *b, const double *xin, hipStream_t s,
-                        bool r32 = true) {
+static int launch_resid(Workspace &ws, const double *kd, double dt, double s2, const double *b,
+                        const double *xin, hipStream_t s, bool r32 = true) {
     const Grid g = grid_of(ws);
     if (ws.prec == DTD_CYCLE_F64) {
         const int th = band_height<1, 1>((const void *)k_sresid<double>, ws.w, ws.h);
         const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
-        k_sresid<double><<<nb, kBlock, 0, s>>>(g, ws.es, b, xin, ws.x, ws.r, (double *)nullptr,
+        k_sresid<double><<<nb, kBlock, 0, s>>>(g, kd, dt, s2, b, xin, ws.x, ws.r, (double *)nullptr,
                                                th, ws.parts);
         LAUNCH_CHECK();
         return nb;
     }
     const int th = band_height<1, 1>((const void *)k_sresid<float>, ws.w, ws.h);
     const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
-    k_sresid<float><<<nb, kBlock, 0, s>>>(g, ws.es, b, xin, ws.x, ws.r, r32 ? ws.lf[0].r : nullptr,
+    k_sresid<float><<<nb, kBlock, 0, s>>>(g, kd, dt, s2, b, xin, ws.x, ws.r, r32 ? ws.lf[0].r : nullptr,
                                           th, ws.parts);
     LAUNCH_CHECK();
     return nb;
@@ -515,7 +516,7 @@ static size_t solve_core(dtd_ctx *ctx, const double *b, const double *kd, double
                                   (int)ctx->opts.max_iter, kd, dt, dx * dx);
     LAUNCH_CHECK();
     const bool fused = fused_update(ws, ctx->opts);
-    const int nbr = launch_resid(ws, b, xin, s, !fused);
+    const int nbr = launch_resid(ws, kd, dt, dx * dx, b, xin, s, !fused);
     k_cg_step<kStepRR0><<<1, kReduceBlock, 0, s>>>(ws.parts, nbr, ws.st);
     LAUNCH_CHECK();
     const int nbz = fused ? run_vcycle(ctx, ws.r, ws.st, kR64, R64{ws.r, nullptr, nullptr, 0.0})
@@ -545,14 +546,14 @@ static size_t solve_core(dtd_ctx *ctx, const double *b, const double *kd, double
     {
         const int th = band_height<1, 1>((const void *)k_finish1, ws.w, ws.h);
         const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
-        k_finish1<<<nb, kBlock, 0, s>>>(g, ws.es, b, ws.x, ws.p0, ws.p1, ws.ap, th, ws.parts,
+        k_finish1<<<nb, kBlock, 0, s>>>(g, kd, dt, dx * dx, b, ws.x, ws.p0, ws.p1, ws.ap, th, ws.parts,
                                         ws.st);
         LAUNCH_CHECK();
         k_putback_step<<<1, kReduceBlock, 0, s>>>(ws.parts, nb, ws.st);
         LAUNCH_CHECK();
         const int th2 = band_height<1, 1>((const void *)k_finish2, ws.w, ws.h);
         const int nb2 = stream_grid<1, 1>(ws.w, ws.h, th2);
-        k_finish2<<<nb2, kBlock, 0, s>>>(g, ws.es, b, ws.ap, ws.x, th2, ws.parts, ws.st);
+        k_finish2<<<nb2, kBlock, 0, s>>>(g, kd, dt, dx * dx, b, ws.ap, ws.x, th2, ws.parts, ws.st);
         LAUNCH_CHECK();
         k_pair_sum<<<1, kReduceBlock, 0, s>>>(ws.parts, nb2, ws.pair);
         LAUNCH_CHECK();

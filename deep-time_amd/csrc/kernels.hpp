@@ -131,7 +131,7 @@ __device__ __forceinline__ void operator_cell(const Grid &g, const double *__res
         wv = 0.0;
     const double nv = j > 0 ? face(kd[c - g.w], k, dt, s2) : 0.0;
     const double d = 1.0 + e + wv + s + nv;
-    es[c] = make_double2(e, s);
+    if (es) es[c] = make_double2(e, s);
     if (coef32) {
         const float df = (float)d;
         *reinterpret_cast<float4 *>(coef32 + 4 * (size_t)c) =
@@ -1304,7 +1304,8 @@ __device__ inline double lane_diag64(double e, double ew, double s, double sn, i
 // cells of x are written there.
 // Returns the lane's shares of (r.r, b.b).
 template <typename TR>
-__device__ __forceinline__ double2 sresid_band(const Grid &g, const double2 *__restrict__ es,
+__device__ __forceinline__ double2 sresid_band(const Grid &g, const double *__restrict__ kd,
+                                               double dt, double s2,
                                                const double *__restrict__ b,
                                                const double *__restrict__ xin,
                                                double *__restrict__ xout, double *__restrict__ r,
@@ -1315,19 +1316,19 @@ __device__ __forceinline__ double2 sresid_band(const Grid &g, const double2 *__r
     const int gi = bd.gi0;
     const bool inner_col = bd.col_ok && gi >= 1 && gi + 1 <= w - 1;
     const bool copy = xout != xin;
-    double qx[A], qe[A], qs[A], qb[A];
+    const bool has_e = gi + 1 < w || wrap;
+    double qx[A], qk[A], qb[A];
 #define DTD_FETCH_RES(j, u)                                                                   \
     do {                                                                                      \
         const int j_ = (j);                                                                   \
         if (j_ >= 0 && j_ < h && bd.col_ok && j_ <= bd.row0 + bd.rows) {                      \
             const size_t g_ = (size_t)j_ * w + gi;                                            \
-            const double2 c_ = es[g_];                                                        \
-            qe[u] = c_.x, qs[u] = c_.y, qx[u] = xin[g_], qb[u] = b[g_];                       \
+            qk[u] = kd[g_], qx[u] = xin[g_], qb[u] = b[g_];                                   \
         } else {                                                                              \
-            qe[u] = qs[u] = qx[u] = qb[u] = 0.0;                                              \
+            qk[u] = qx[u] = qb[u] = 0.0;                                                      \
         }                                                                                     \
     } while (0)
-    double xa = 0, xb = 0, xc = 0, ea = 0, eb = 0, sa = 0, sb = 0, sc = 0, ba = 0, bb = 0;
+    double xa = 0, xb = 0, xc = 0, ea = 0, eb = 0, sb = 0, sc = 0, ba = 0, bb = 0, ka = 0;
     double acc_r = 0.0, acc_b = 0.0;
     const int steps = bd.active ? bd.rows + 4 : 0;
 #pragma unroll
@@ -1336,8 +1337,13 @@ __device__ __forceinline__ double2 sresid_band(const Grid &g, const double2 *__r
 #pragma unroll
         for (int u = 0; u < A; u++) {
             const int ja = bd.row0 - 1 + k0 + u, jb = ja - 1;
-            xc = xb, xb = xa, eb = ea, sc = sb, sb = sa, bb = ba;
-            xa = qx[u], ea = qe[u], sa = qs[u], ba = qb[u];
+            xc = xb, xb = xa, eb = ea, sc = sb, bb = ba;
+            const double kb = ka;
+            ka = qk[u];
+            const double ke = from_east(ka);
+            xa = qx[u], ba = qb[u];
+            ea = has_e ? face(ka, ke, dt, s2) : 0.0;
+            sb = ja < h ? face(kb, ka, dt, s2) : 0.0;  // row b's south face
             if (k0 + u + A < steps) DTD_FETCH_RES(ja + A, u);
             const bool rows_a = ja >= bd.row0 && ja < bd.row0 + bd.rows && ja < h;
             if (copy && bd.own && rows_a) xout[(size_t)ja * w + gi] = xa;
@@ -1366,21 +1372,23 @@ __device__ inline void block_pair(double2 v, double *parts, int slot, double *ld
 }
 
 template <typename TR>
-__global__ __launch_bounds__(kBlock) void k_sresid(Grid g, const double2 *__restrict__ es,
+__global__ __launch_bounds__(kBlock) void k_sresid(Grid g, const double *__restrict__ kd,
+                                                   double dt, double s2,
                                                    const double *__restrict__ b,
                                                    const double *__restrict__ xin,
                                                    double *__restrict__ xout,
                                                    double *__restrict__ r, TR *__restrict__ rc,
                                                    int th, double *__restrict__ parts) {
     __shared__ double lds[kBlock / 64];
-    block_pair(sresid_band<TR>(g, es, b, xin, xout, r, rc, th, block_wave(blockIdx.x)), parts,
-               blockIdx.x, lds);
+    block_pair(sresid_band<TR>(g, kd, dt, s2, b, xin, xout, r, rc, th, block_wave(blockIdx.x)),
+               parts, blockIdx.x, lds);
 }
 
 // After the loop: the last update x' = x + alpha p (MG:734-736; p of the last iteration, in
 // buffer its % 2 ... p0 for odd its), written to xout, and the put-back's sums over x'
 // (MG:737-755): per block (sum (b - A x'), sum (diag - 1)).
-__device__ __forceinline__ double2 finish1_band(const Grid &g, const double2 *__restrict__ es,
+__device__ __forceinline__ double2 finish1_band(const Grid &g, const double *__restrict__ kd,
+                                                double dt, double s2,
                                                 const double *__restrict__ b,
                                                 const double *__restrict__ x,
                                                 const double *__restrict__ p0,
@@ -1395,20 +1403,20 @@ __device__ __forceinline__ double2 finish1_band(const Grid &g, const double2 *__
     const Band bd = band_at<1, 1>(w, h, wrap, th, wave);
     const int gi = bd.gi0;
     const bool inner_col = bd.col_ok && gi >= 1 && gi + 1 <= w - 1;
-    double qx[A], qp[A], qe[A], qs[A], qb[A];
+    const bool has_e = gi + 1 < w || wrap;
+    double qx[A], qp[A], qk[A], qb[A];
 #define DTD_FETCH_FIN(j, u)                                                                   \
     do {                                                                                      \
         const int j_ = (j);                                                                   \
         if (j_ >= 0 && j_ < h && bd.col_ok && j_ <= bd.row0 + bd.rows) {                      \
             const size_t g_ = (size_t)j_ * w + gi;                                            \
-            const double2 c_ = es[g_];                                                        \
-            qe[u] = c_.x, qs[u] = c_.y, qx[u] = x[g_], qb[u] = b[g_];                         \
+            qk[u] = kd[g_], qx[u] = x[g_], qb[u] = b[g_];                                     \
             qp[u] = pending ? p[g_] : 0.0;                                                    \
         } else {                                                                              \
-            qe[u] = qs[u] = qx[u] = qb[u] = qp[u] = 0.0;                                      \
+            qk[u] = qx[u] = qb[u] = qp[u] = 0.0;                                              \
         }                                                                                     \
     } while (0)
-    double xa = 0, xb = 0, xc = 0, ea = 0, eb = 0, sa = 0, sb = 0, sc = 0, ba = 0, bb = 0;
+    double xa = 0, xb = 0, xc = 0, ea = 0, eb = 0, sb = 0, sc = 0, ba = 0, bb = 0, ka = 0;
     double acc_l = 0.0, acc_w = 0.0;
     const int steps = bd.active ? bd.rows + 4 : 0;
 #pragma unroll
@@ -1417,9 +1425,14 @@ __device__ __forceinline__ double2 finish1_band(const Grid &g, const double2 *__
 #pragma unroll
         for (int u = 0; u < A; u++) {
             const int ja = bd.row0 - 1 + k0 + u, jb = ja - 1;
-            xc = xb, xb = xa, eb = ea, sc = sb, sb = sa, bb = ba;
+            xc = xb, xb = xa, eb = ea, sc = sb, bb = ba;
+            const double kb = ka;
+            ka = qk[u];
+            const double ke = from_east(ka);
             xa = pending ? qx[u] + alpha * qp[u] : qx[u];
-            ea = qe[u], sa = qs[u], ba = qb[u];
+            ba = qb[u];
+            ea = has_e ? face(ka, ke, dt, s2) : 0.0;
+            sb = ja < h ? face(kb, ka, dt, s2) : 0.0;  // row b's south face
             if (k0 + u + A < steps) DTD_FETCH_FIN(ja + A, u);
             if (bd.own && ja >= bd.row0 && ja < bd.row0 + bd.rows && ja < h)
                 xout[(size_t)ja * w + gi] = xa;
@@ -1436,7 +1449,8 @@ __device__ __forceinline__ double2 finish1_band(const Grid &g, const double2 *__
     return make_double2(acc_l, acc_w);
 }
 
-__global__ __launch_bounds__(kBlock) void k_finish1(Grid g, const double2 *__restrict__ es,
+__global__ __launch_bounds__(kBlock) void k_finish1(Grid g, const double *__restrict__ kd,
+                                                    double dt, double s2,
                                                     const double *__restrict__ b,
                                                     const double *__restrict__ x,
                                                     const double *__restrict__ p0,
@@ -1445,13 +1459,14 @@ __global__ __launch_bounds__(kBlock) void k_finish1(Grid g, const double2 *__res
                                                     double *__restrict__ parts,
                                                     const CgState *__restrict__ st) {
     __shared__ double lds[kBlock / 64];
-    block_pair(finish1_band(g, es, b, x, p0, p1, xout, th, st, block_wave(blockIdx.x)), parts,
-               blockIdx.x, lds);
+    block_pair(finish1_band(g, kd, dt, s2, b, x, p0, p1, xout, th, st, block_wave(blockIdx.x)),
+               parts, blockIdx.x, lds);
 }
 
 // The put-back x'' = x' + (lost / weight) (diag - 1) when weight > 0 and lost != 0 (MG:756-761),
 // written to xout, and the true residual of x'': per block (sum (b - A x'')^2, sum b^2).
-__device__ __forceinline__ double2 finish2_band(const Grid &g, const double2 *__restrict__ es,
+__device__ __forceinline__ double2 finish2_band(const Grid &g, const double *__restrict__ kd,
+                                                double dt, double s2,
                                                 const double *__restrict__ b,
                                                 const double *__restrict__ xp,
                                                 double *__restrict__ xout, int th,
@@ -1464,20 +1479,23 @@ __device__ __forceinline__ double2 finish2_band(const Grid &g, const double2 *__
     const Band bd = band_at<1, 1>(w, h, wrap, th, wave);
     const int gi = bd.gi0;
     const bool inner_col = bd.col_ok && gi >= 1 && gi + 1 <= w - 1;
-    double qx[A], qe[A], qs[A], qb[A];
+    const bool has_e = gi + 1 < w || wrap;
+    double qx[A], qk[A], qb[A];
 #define DTD_FETCH_FIN2(j, u)                                                                  \
     do {                                                                                      \
         const int j_ = (j);                                                                   \
-        if (j_ >= 0 && j_ < h && bd.col_ok && j_ <= bd.row0 + bd.rows) {                      \
+        if (j_ >= 0 && j_ < h && bd.col_ok && j_ <= bd.row0 + bd.rows + 1) {                  \
             const size_t g_ = (size_t)j_ * w + gi;                                            \
-            const double2 c_ = es[g_];                                                        \
-            qe[u] = c_.x, qs[u] = c_.y, qx[u] = xp[g_], qb[u] = b[g_];                        \
+            qk[u] = kd[g_], qx[u] = xp[g_], qb[u] = b[g_];                                    \
         } else {                                                                              \
-            qe[u] = qs[u] = qx[u] = qb[u] = 0.0;                                              \
+            qk[u] = qx[u] = qb[u] = 0.0;                                                      \
         }                                                                                     \
     } while (0)
-    // The window starts two rows above the band: the diagonal of row a needs row a - 1's south.
-    double xa = 0, xb = 0, xc = 0, ea = 0, eb = 0, sa = 0, sb = 0, sc = 0, ba = 0, bb = 0;
+    // Rows by age: a (just loaded: x' and kd), b (its faces complete now that a is here: x'' of
+    // b is formed), c (the residual's row), d.  The window starts two rows above the band, so the
+    // x'' of the row above the band has its north face.
+    double xpa = 0, xb = 0, xc = 0, xd = 0, ea = 0, eb = 0, ec = 0, sb = 0, sc = 0, sd = 0;
+    double ba = 0, bb = 0, bc = 0, ka = 0;
     double acc_r = 0.0, acc_b = 0.0;
     const int steps = bd.active ? bd.rows + 4 : 0;
 #pragma unroll
@@ -1485,24 +1503,31 @@ __device__ __forceinline__ double2 finish2_band(const Grid &g, const double2 *__
     for (int k0 = 0; k0 < steps; k0 += A) {
 #pragma unroll
         for (int u = 0; u < A; u++) {
-            const int ja = bd.row0 - 2 + k0 + u, jb = ja - 1;
-            xc = xb, xb = xa, eb = ea, sc = sb, sb = sa, bb = ba;
-            ea = qe[u], sa = qs[u], ba = qb[u];
-            const double ewa = from_west(ea);
-            double v = qx[u];
-            if (apply && ja >= 0 && ja < h && bd.col_ok)
-                v = v + kk * (lane_diag64(ea, ewa, sa, sb, gi, ja, wrap) - 1.0);
-            xa = v;
+            const int ja = bd.row0 - 2 + k0 + u, jb = ja - 1, jc = ja - 2;
+            xd = xc, xc = xb, ec = eb, eb = ea, sd = sc, sc = sb, bc = bb, bb = ba;
+            const double xpb = xpa, kb = ka;
+            ka = qk[u];
+            const double ke = from_east(ka);
+            xpa = qx[u], ba = qb[u];
+            ea = has_e ? face(ka, ke, dt, s2) : 0.0;
+            sb = ja < h ? face(kb, ka, dt, s2) : 0.0;  // row b's south face
             if (k0 + u + A < steps) DTD_FETCH_FIN2(ja + A, u);
-            if (bd.own && ja >= bd.row0 && ja < bd.row0 + bd.rows && ja < h)
-                xout[(size_t)ja * w + gi] = xa;
-            const double xe = from_east(xb), xw = from_west(xb), ew = from_west(eb);
-            if (bd.own && jb >= bd.row0 && jb < bd.row0 + bd.rows && jb < h) {
-                const double o = lane_apply64(xb, xe, xw, xa, xc, eb, ew, sb, sc, gi, jb, w, h,
-                                              wrap, inner_col && jb > 0 && jb + 1 < h);
-                const double rv = bb - o;
+            // The put-back on row b: x'' = x' + (lost / weight) (diag - 1) (MG:756-761).
+            const double ewb = from_west(eb);
+            double v = xpb;
+            if (apply && jb >= 0 && jb < h && bd.col_ok)
+                v = v + kk * (lane_diag64(eb, ewb, sb, sc, gi, jb, wrap) - 1.0);
+            xb = v;
+            if (bd.own && jb >= bd.row0 && jb < bd.row0 + bd.rows && jb < h)
+                xout[(size_t)jb * w + gi] = xb;
+            // The true residual of row c.
+            const double xe = from_east(xc), xw = from_west(xc), ewc = from_west(ec);
+            if (bd.own && jc >= bd.row0 && jc < bd.row0 + bd.rows && jc < h) {
+                const double o = lane_apply64(xc, xe, xw, xb, xd, ec, ewc, sc, sd, gi, jc, w, h,
+                                              wrap, inner_col && jc > 0 && jc + 1 < h);
+                const double rv = bc - o;
                 acc_r += rv * rv;
-                acc_b += bb * bb;
+                acc_b += bc * bc;
             }
         }
     }
@@ -1510,15 +1535,16 @@ __device__ __forceinline__ double2 finish2_band(const Grid &g, const double2 *__
     return make_double2(acc_r, acc_b);
 }
 
-__global__ __launch_bounds__(kBlock) void k_finish2(Grid g, const double2 *__restrict__ es,
+__global__ __launch_bounds__(kBlock) void k_finish2(Grid g, const double *__restrict__ kd,
+                                                    double dt, double s2,
                                                     const double *__restrict__ b,
                                                     const double *__restrict__ xp,
                                                     double *__restrict__ xout, int th,
                                                     double *__restrict__ parts,
                                                     const CgState *__restrict__ st) {
     __shared__ double lds[kBlock / 64];
-    block_pair(finish2_band(g, es, b, xp, xout, th, st, block_wave(blockIdx.x)), parts, blockIdx.x,
-               lds);
+    block_pair(finish2_band(g, kd, dt, s2, b, xp, xout, th, st, block_wave(blockIdx.x)), parts,
+               blockIdx.x, lds);
 }
 
 // ---- the first guess (D:370-489) -----------------------------------------------------------
