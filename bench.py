@@ -152,11 +152,14 @@ def main():
         # sweep + last sweep + r.z, r read as the f64 residual); compulsory bytes / HIP-event
         # duration.
         sw = kernels["post_smooth_f32"]
+        cb = int(args.cycle == "bf16")
+        key = f"k_spost<float, 2, 1, {cb}>"
         roof = {"bound": "hbm", "achieved": sw["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": sw["GB/s"] / HBM_PEAK_GBS,
-                "traffic": traffic_of("k_spost<float, 2, 1, 0>", w, h),
-                "kernel": "k_spost<float, 2, 1, 0> (finest level: two damped-Jacobi sweeps + r.z; "
-                          "r64, the level record, z and z_coarse in, z out: 33 B/cell)"}
+                "traffic": traffic_of(key, w, h),
+                "kernel": f"{key} (finest level: two damped-Jacobi sweeps + r.z; r64 8, the level "
+                          f"record {8 if cb else 12}, z 4 and z_coarse 1 in, z 4 out: "
+                          f"{25 if cb else 29} B/cell)"}
 
     cpu = None
     parity = None

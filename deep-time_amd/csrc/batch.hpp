@@ -56,21 +56,20 @@ inline int batch_band(int h) { return h >= 64 ? 16 : (h >= 16 ? 8 : 4); }
 // ---- kernels: a table maps each block to (grid, block of the grid) ---------------------------
 
 __global__ __launch_bounds__(kBlock) void kb_operator(const BGrid *__restrict__ t,
-                                                      const int2 *__restrict__ blk, float omega) {
+                                                      const int2 *__restrict__ blk) {
     const int2 m = blk[blockIdx.x];
     const BGrid &q = t[m.x];
     const int c = m.y * kBlock + threadIdx.x;
-    if (c < q.g.n) operator_cell(q.g, q.kd, q.dt, q.s2, nullptr, q.coef32, nullptr, omega, c);
+    if (c < q.g.n) operator_cell(q.g, q.kd, q.dt, q.s2, nullptr, q.coef32, nullptr, c);
 }
 
 // Level l + 1 from level l, one coarse cell a thread (t: the grids' level l).
 __global__ __launch_bounds__(kBlock) void kb_coarsen(const BLevel *__restrict__ t,
-                                                     const int2 *__restrict__ blk, float scale,
-                                                     float omega) {
+                                                     const int2 *__restrict__ blk, float scale) {
     const int2 m = blk[blockIdx.x];
     const BLevel &q = t[m.x];
     const int k = m.y * kBlock + threadIdx.x;
-    if (k < q.C.n) coarsen_cell(q.L, q.C, scale, omega, k);
+    if (k < q.C.n) coarsen_cell(q.L, q.C, scale, k);
 }
 
 // The caller's h = x.
@@ -385,7 +384,7 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
     };
     std::vector<Pool> pool(L);
     for (int l = 0; l < L; l++) {
-        pool[l].coef = (float *)own(4 * ltotal[l] * 4);
+        pool[l].coef = (float *)own(kRec * ltotal[l] * 4);
         pool[l].mass = l == 0 ? nullptr : (float *)own(ltotal[l] * 4);
         pool[l].r = (float *)own(ltotal[l] * 4);
         pool[l].z0 = (float *)own(ltotal[l] * 4);
@@ -396,7 +395,8 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
         if (l >= bw->nlev[g]) return v;
         const size_t o = loff[l][g];
         v.w = sh[g][l].w, v.h = sh[g][l].h, v.wrap = sh[g][l].wrap, v.n = v.w * v.h;
-        v.coef = pool[l].coef + 4 * o;
+        v.omega = ctx->opts.omega;
+        v.coef = pool[l].coef + kRec * o;
         v.mass = l == 0 ? nullptr : pool[l].mass + o;
         v.r = pool[l].r + o;
         v.z[0] = pool[l].z0 + o;
@@ -700,12 +700,12 @@ static void solve_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs_in, doubl
     HIPCHK(hipMemcpyAsync(bw.d_grids, bw.grids.data(), n * sizeof(BGrid), hipMemcpyHostToDevice,
                           s));
     HIPCHK(hipEventRecord(ctx->ev[0], s));
-    kb_operator<<<bw.cells_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.cells_l.blk, ctx->opts.omega);
+    kb_operator<<<bw.cells_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.cells_l.blk);
     LAUNCH_CHECK();
     for (int l = 0; l + 1 < bw.nlevels; l++) {
         if (!bw.coarsen_l[l].nblk) continue;
         kb_coarsen<<<bw.coarsen_l[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.coarsen_l[l].blk,
-                                                           ctx->opts.scale, ctx->opts.omega);
+                                                           ctx->opts.scale);
         LAUNCH_CHECK();
     }
     HIPCHK(hipEventRecord(ctx->ev[1], s));

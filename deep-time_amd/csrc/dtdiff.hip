@@ -84,7 +84,8 @@ static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
         if (prec == DTD_CYCLE_F64) {
             Lvl<double> L{};
             L.w = sh[l].w, L.h = sh[l].h, L.wrap = sh[l].wrap, L.n = (int)nl;
-            L.coef = (double *)own(4 * nl, 8);
+            L.omega = (double)ctx->opts.omega;
+            L.coef = (double *)own(kRec * nl, 8);
             L.mass = l == 0 ? nullptr : (double *)own(nl, 8);
             L.r = l == 0 ? ws.r : (double *)own(nl, 8);  // the f64 cycle reads the CG residual
             L.z[0] = (double *)own(nl, 8);
@@ -93,7 +94,8 @@ static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
         } else {
             Lvl<float> L{};
             L.w = sh[l].w, L.h = sh[l].h, L.wrap = sh[l].wrap, L.n = (int)nl;
-            L.coef = (float *)own(4 * nl, 4);
+            L.omega = ctx->opts.omega;
+            L.coef = (float *)own(kRec * nl, 4);
             L.mass = l == 0 ? nullptr : (float *)own(nl, 4);
             L.r = (float *)own(nl, 4);
             L.z[0] = (float *)own(nl, 4);
@@ -157,9 +159,9 @@ __global__ void k_time_state(CgState *st, const double *kd) {
 // The coarse levels from the finest (`Hierarchy::new`, MG:481-487; device `COARSEN`/`CDIAG`,
 // MGGPU:62-140): one launch a level.
 template <typename T>
-static void build_coarse(std::vector<Lvl<T>> &lv, T omega, T scale, hipStream_t s) {
+static void build_coarse(std::vector<Lvl<T>> &lv, T scale, hipStream_t s) {
     for (size_t l = 1; l < lv.size(); l++) {
-        k_coarsen<T><<<nblocks(lv[l].n), kBlock, 0, s>>>(lv[l - 1], lv[l], scale, omega);
+        k_coarsen<T><<<nblocks(lv[l].n), kBlock, 0, s>>>(lv[l - 1], lv[l], scale);
         LAUNCH_CHECK();
     }
 }
@@ -169,7 +171,7 @@ static void build_coarse(std::vector<Lvl<T>> &lv, T omega, T scale, hipStream_t 
 static void pack_bf16(Workspace &ws, hipStream_t s) {
     for (auto &L : ws.lf)
         if (L.coefh) {
-            k_pack_bf16<<<nblocks(L.n), 256, 0, s>>>(L.n, L.coef, L.coefh);
+            k_pack_bf16<<<nblocks(L.n), 256, 0, s>>>(L.n, L.coef, L.omega, L.coefh);
             LAUNCH_CHECK();
         }
 }
@@ -182,12 +184,12 @@ static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt)
     // The f64 passes form the faces from kd themselves; only the cycle's finest records are kept.
     k_operator<<<nblocks(ws.n), kBlock, 0, s>>>(grid_of(ws), kd, dt, dx * dx, nullptr,
                                                 f64 ? nullptr : ws.lf[0].coef,
-                                                f64 ? ws.ld[0].coef : nullptr, ctx->opts.omega);
+                                                f64 ? ws.ld[0].coef : nullptr);
     LAUNCH_CHECK();
     if (f64)
-        build_coarse<double>(ws.ld, (double)ctx->opts.omega, (double)ctx->opts.scale, s);
+        build_coarse<double>(ws.ld, (double)ctx->opts.scale, s);
     else
-        build_coarse<float>(ws.lf, ctx->opts.omega, ctx->opts.scale, s);
+        build_coarse<float>(ws.lf, ctx->opts.scale, s);
     pack_bf16(ws, s);
 }
 
@@ -832,7 +834,7 @@ int dtd_operator(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, const 
         upload(ctx, ws.kd, kd, n * sizeof(double));
         const Grid g = grid_of(ws);
         k_operator<<<nblocks(n), kBlock, 0, ctx->stream>>>(g, ws.kd, dt, dx * dx, ws.es, nullptr,
-                                                           nullptr, 0.0f);
+                                                           nullptr);
         LAUNCH_CHECK();
         k_unpack_es<<<nblocks(n), kBlock, 0, ctx->stream>>>(g, ws.es, ws.p0, ws.p1, ws.ap);
         LAUNCH_CHECK();
@@ -856,7 +858,7 @@ int dtd_apply(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, const dou
         upload(ctx, ws.x, x, n * sizeof(double));
         const Grid g = grid_of(ws);
         k_operator<<<nblocks(n), kBlock, 0, ctx->stream>>>(g, ws.kd, dt, dx * dx, ws.es, nullptr,
-                                                           nullptr, 0.0f);
+                                                           nullptr);
         LAUNCH_CHECK();
         k_apply<<<nblocks(n), kBlock, 0, ctx->stream>>>(g, ws.es, ws.x, ws.ap);
         LAUNCH_CHECK();
@@ -902,7 +904,7 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
             case DTD_KERNEL_SWEEP:  // r, record, z in; z out (f32)
                 k_sweep<float><<<nb, kBlock, 0, s>>>(L, L.z[1], L.z[0], nullptr, 0, nullptr,
                                                      nullptr, ws.st);
-                b = 28.0 * n;
+                b = 24.0 * n;
                 break;
             case DTD_KERNEL_DIRECTION:  // z (f32), p_old, x, kd in; p, A p, x out
                 launch_dir<float>(ws, L.z[1], ws.p1, ws.p0, s);
@@ -915,24 +917,24 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
                 break;
             case DTD_KERNEL_RESTRICT:  // fine r, z, record in; coarse r out
                 k_restrict<float><<<nblocks(C.n), kBlock, 0, s>>>(L, L.z[1], C, ws.st);
-                b = 24.0 * n + 4.0 * C.n;
+                b = 20.0 * n + 4.0 * C.n;
                 break;
             case DTD_KERNEL_PRE:  // r64, A p, record in; r64, z and the coarse r out
                 if (L.w % 2 || !ws.r2) throw Error{DTD_ESTATE, "the finest width is odd"};
                 launch_pre<float, 2, kRUpdate>(L, C, L.z[0], ws.st, s,
                                                R64{ws.r, ws.ap, ws.r2, 0.0}, ws.parts);
-                b = (L.coefh ? 36.0 : 44.0) * n + 4.0 * C.n;  // bf16 records: 8 B, not 16
+                b = (L.coefh ? 36.0 : 40.0) * n + 4.0 * C.n;  // bf16 records: 8 B, not 12
                 break;
             case DTD_KERNEL_POST:  // z, r64, record, coarse z in; z out
                 if (L.w % 2) throw Error{DTD_ESTATE, "the finest width is odd"};
                 launch_post<float, 2, kR64>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts, ws.st,
                                             s);
-                b = (L.coefh ? 24.0 : 32.0) * n + 4.0 * C.n;
+                b = (L.coefh ? 24.0 : 28.0) * n + 4.0 * C.n;
                 break;
             case DTD_KERNEL_BANDCOPY: {  // r, record in; z out, the pre-smoother's walk
                 const int th = band_height<2>((const void *)k_bandcopy<2>, L.w, L.h);
                 k_bandcopy<2><<<stream_grid<2, 1>(L.w, L.h, th), kBlock, 0, s>>>(L, L.z[0], th);
-                b = 24.0 * n;
+                b = 20.0 * n;
                 break;
             }
             case DTD_KERNEL_COPY:  // r32 in, z out
@@ -982,15 +984,13 @@ int dtd_cycle_new(dtd_ctx *ctx, size_t w, size_t h, int wrap_x, const double *ea
         upload(ctx, ws.ap, diag, n * sizeof(double));
         hipStream_t s = ctx->stream;
         if (ws.prec == DTD_CYCLE_F64) {
-            k_finest64<<<nblocks(n), kBlock, 0, s>>>((int)n, ws.p0, ws.p1, ws.ap, ws.ld[0].coef,
-                                                     (double)ctx->opts.omega);
+            k_finest64<<<nblocks(n), kBlock, 0, s>>>((int)n, ws.p0, ws.p1, ws.ap, ws.ld[0].coef);
             LAUNCH_CHECK();
-            build_coarse<double>(ws.ld, (double)ctx->opts.omega, (double)ctx->opts.scale, s);
+            build_coarse<double>(ws.ld, (double)ctx->opts.scale, s);
         } else {
-            k_finest32<<<nblocks(n), kBlock, 0, s>>>((int)n, ws.p0, ws.p1, ws.ap, ws.lf[0].coef,
-                                                     ctx->opts.omega);
+            k_finest32<<<nblocks(n), kBlock, 0, s>>>((int)n, ws.p0, ws.p1, ws.ap, ws.lf[0].coef);
             LAUNCH_CHECK();
-            build_coarse<float>(ws.lf, ctx->opts.omega, ctx->opts.scale, s);
+            build_coarse<float>(ws.lf, ctx->opts.scale, s);
             pack_bf16(ws, s);
         }
         HIPCHK(hipStreamSynchronize(s));
@@ -1012,12 +1012,18 @@ int dtd_cycle_level(dtd_ctx *ctx, size_t l, size_t *w, size_t *h, int *wrap, flo
         if (h) *h = (size_t)L.h;
         if (wrap) *wrap = L.wrap;
         const size_t n = (size_t)L.n;
-        std::vector<float> rec(4 * n);
-        download(ctx, rec.data(), L.coef, 4 * n * sizeof(float));
-        float *outs[4] = {east, south, diag, wd};
-        for (int q = 0; q < 4; q++)
+        std::vector<float> rec(kRec * n);
+        download(ctx, rec.data(), L.coef, kRec * n * sizeof(float));
+        float *outs[3] = {east, south, diag};
+        for (int q = 0; q < kRec; q++)
             if (outs[q])
-                for (size_t c = 0; c < n; c++) outs[q][c] = rec[4 * c + q];
+                for (size_t c = 0; c < n; c++) outs[q][c] = rec[kRec * c + q];
+        // The smoother's weight omega / diag is formed where it is used, in f32 as here.
+        if (wd)
+            for (size_t c = 0; c < n; c++) {
+                const float d = rec[kRec * c + 2];
+                wd[c] = d != 0.0f ? L.omega / d : 0.0f;
+            }
         if (mass) {
             if (L.mass)
                 download(ctx, mass, L.mass, n * sizeof(float));

@@ -334,7 +334,7 @@ static size_t jacobi_solve(dtd_ctx *ctx, const double *b, const double *kd, doub
     const double s2 = dx * dx;
     double *s0 = ws.jrows, *s1 = ws.jrows + ws.h;
     HIPCHK(hipEventRecord(ctx->ev[0], s));
-    k_operator<<<nb, kBlock, 0, s>>>(g, kd, dt, s2, ws.es, nullptr, nullptr, 0.0f);
+    k_operator<<<nb, kBlock, 0, s>>>(g, kd, dt, s2, ws.es, nullptr, nullptr);
     LAUNCH_CHECK();
     kj_diag<<<nb, kBlock, 0, s>>>(g, ws.es, ws.jdiag);
     LAUNCH_CHECK();

@@ -102,16 +102,21 @@ __device__ inline double face(double ka, double kb, double dt, double s2) {
     return 0.5 * (ka + kb) * dt / s2;
 }
 
+// A cycle level's operator record: (east, south, diag) of a cell side by side.  omega / diag, the
+// smoother's weight, is one correctly rounded division of diag, formed where it is used: the same
+// bits as a stored copy, without its 4 bytes a cell of traffic.
+constexpr int kRec = 3;
+
 // `Operator::new` (D:532-603) with `Level::finest` (MG:96-114) in one pass.  The f64 couplings
 // go out packed, es[c] = (east, south); the diagonal ((((1 + e) + w) + s) + n) takes the west and
 // north faces from the same expression as the neighbours' own east and south, and is never
 // stored (it is re-formed the same way wherever it is used).  The finest level of the cycle goes
-// out as one record a cell, (east, south, diag, omega / diag): in f32 (coef32, the reference's
-// cycle) or in f64 (coef64, the f64 cycle).
+// out as one record a cell, (east, south, diag): in f32 (coef32, the reference's cycle) or in f64
+// (coef64, the f64 cycle).
 __device__ __forceinline__ void operator_cell(const Grid &g, const double *__restrict__ kd,
                                               double dt, double s2, double2 *__restrict__ es,
                                               float *__restrict__ coef32,
-                                              double *__restrict__ coef64, float omega, int c) {
+                                              double *__restrict__ coef64, int c) {
     const int j = c / g.w, i = c - j * g.w, c0 = j * g.w;
     const double k = kd[c];
     double e;
@@ -133,22 +138,20 @@ __device__ __forceinline__ void operator_cell(const Grid &g, const double *__res
     const double d = 1.0 + e + wv + s + nv;
     if (es) es[c] = make_double2(e, s);
     if (coef32) {
-        const float df = (float)d;
-        *reinterpret_cast<float4 *>(coef32 + 4 * (size_t)c) =
-            make_float4((float)e, (float)s, df, omega / df);
+        float *q = coef32 + kRec * (size_t)c;
+        q[0] = (float)e, q[1] = (float)s, q[2] = (float)d;
     }
     if (coef64) {
-        double2 *q = reinterpret_cast<double2 *>(coef64 + 4 * (size_t)c);
-        q[0] = make_double2(e, s);
-        q[1] = make_double2(d, (double)omega / d);
+        double *q = coef64 + kRec * (size_t)c;
+        q[0] = e, q[1] = s, q[2] = d;
     }
 }
 
 __global__ void k_operator(Grid g, const double *__restrict__ kd, double dt, double s2,
                            double2 *__restrict__ es, float *__restrict__ coef32,
-                           double *__restrict__ coef64, float omega) {
+                           double *__restrict__ coef64) {
     const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c < g.n) operator_cell(g, kd, dt, s2, es, coef32, coef64, omega, c);
+    if (c < g.n) operator_cell(g, kd, dt, s2, es, coef32, coef64, c);
 }
 
 // The f64 diagonal of cell c from the packed faces (the sum Operator::new takes).
@@ -324,10 +327,12 @@ __global__ void k_pair_sum(const double *__restrict__ parts, int nparts, double 
 
 // ---- the V-cycle's levels (MG:73-523) -------------------------------------------------------
 
+// A level's operator record is kRec values a cell, (east, south, diag), side by side.
 template <typename T>
 struct Lvl {
     int w, h, wrap, n;
-    T *coef;  // (east, south, diag, omega / diag) of each cell side by side
+    T omega;  // the smoother's damping
+    T *coef;  // kRec values a cell
     T *mass;  // the identity's share of the diagonal; nullptr on the finest level (1 a cell)
     T *r, *z[2];
     // The same records rounded to bf16 (8 bytes a cell), which the fused smoothing kernels read
@@ -345,42 +350,40 @@ __host__ __device__ inline unsigned short bf16_of(float f) {
 }
 __device__ inline float float_of_bf16(unsigned b) { return __uint_as_float(b << 16); }
 
-// coefh = the level's records rounded to bf16, 4 per cell.
-__global__ void k_pack_bf16(int n, const float *__restrict__ coef,
+// coefh = the level's records with omega / diag, rounded to bf16: 4 per cell.
+__global__ void k_pack_bf16(int n, const float *__restrict__ coef, float omega,
                             unsigned short *__restrict__ coefh) {
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= n) return;
-    const float4 a = *reinterpret_cast<const float4 *>(coef + 4 * (size_t)c);
-    const unsigned lo = bf16_of(a.x) | ((unsigned)bf16_of(a.y) << 16);
-    const unsigned hi = bf16_of(a.z) | ((unsigned)bf16_of(a.w) << 16);
+    const float *q = coef + kRec * (size_t)c;
+    const unsigned lo = bf16_of(q[0]) | ((unsigned)bf16_of(q[1]) << 16);
+    const unsigned hi = bf16_of(q[2]) | ((unsigned)bf16_of(omega / q[2]) << 16);
     *reinterpret_cast<uint2 *>(coefh + 4 * (size_t)c) = make_uint2(lo, hi);
 }
 
-template <typename T> __device__ inline T c_east(const Lvl<T> &L, int c) { return L.coef[4 * (size_t)c]; }
-template <typename T> __device__ inline T c_south(const Lvl<T> &L, int c) { return L.coef[4 * (size_t)c + 1]; }
-template <typename T> __device__ inline T c_diag(const Lvl<T> &L, int c) { return L.coef[4 * (size_t)c + 2]; }
-template <typename T> __device__ inline T c_wd(const Lvl<T> &L, int c) { return L.coef[4 * (size_t)c + 3]; }
+template <typename T> __device__ inline T c_east(const Lvl<T> &L, int c) { return L.coef[kRec * (size_t)c]; }
+template <typename T> __device__ inline T c_south(const Lvl<T> &L, int c) { return L.coef[kRec * (size_t)c + 1]; }
+template <typename T> __device__ inline T c_diag(const Lvl<T> &L, int c) { return L.coef[kRec * (size_t)c + 2]; }
+template <typename T> __device__ inline T c_wd(const Lvl<T> &L, int c) { return L.omega / c_diag(L, c); }
 
-// `Level::finest` (MG:96-114) from a given f64 operator: the f32 record and omega / diag.
+// `Level::finest` (MG:96-114) from a given f64 operator: the f32 record.
 __global__ void k_finest32(int n, const double *__restrict__ east,
                            const double *__restrict__ south, const double *__restrict__ diag,
-                           float *__restrict__ coef, float omega) {
+                           float *__restrict__ coef) {
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= n) return;
-    const float d = (float)diag[c];
-    *reinterpret_cast<float4 *>(coef + 4 * (size_t)c) =
-        make_float4((float)east[c], (float)south[c], d, omega / d);
+    float *q = coef + kRec * (size_t)c;
+    q[0] = (float)east[c], q[1] = (float)south[c], q[2] = (float)diag[c];
 }
 
 // The f64 cycle's finest level from a given f64 operator.
 __global__ void k_finest64(int n, const double *__restrict__ east,
                            const double *__restrict__ south, const double *__restrict__ diag,
-                           double *__restrict__ coef, double omega) {
+                           double *__restrict__ coef) {
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= n) return;
-    double2 *q = reinterpret_cast<double2 *>(coef + 4 * (size_t)c);
-    q[0] = make_double2(east[c], south[c]);
-    q[1] = make_double2(diag[c], omega / diag[c]);
+    double *q = coef + kRec * (size_t)c;
+    q[0] = east[c], q[1] = south[c], q[2] = diag[c];
 }
 
 // `Level::coarsen` (MG:117-199) for coarse cell (ic, jc): the mass of the 2x2 block (rows then
@@ -412,12 +415,11 @@ __device__ inline T coarse_south(const Lvl<T> &f, int hc, T scale, int ic, int j
     return scale * s;
 }
 
-// One coarse level from the finer one: mass, faces, and the diagonal mass + e + w + s + n with
-// omega / diag (MG:189-198); the west and north faces are formed by the same sums as the
+// One coarse level from the finer one: mass, faces, and the diagonal mass + e + w + s + n
+// (MG:189-198); the west and north faces are formed by the same sums as the
 // neighbours' own, so every value equals the reference's.
 template <typename T>
-__device__ __forceinline__ void coarsen_cell(const Lvl<T> &f, const Lvl<T> &c, T scale, T omega,
-                                             int k) {
+__device__ __forceinline__ void coarsen_cell(const Lvl<T> &f, const Lvl<T> &c, T scale, int k) {
     const int jc = k / c.w, ic = k - jc * c.w;
     const T m = coarse_mass(f, ic, jc);
     const T e = coarse_east(f, c.w, c.wrap, scale, ic, jc);
@@ -427,14 +429,14 @@ __device__ __forceinline__ void coarsen_cell(const Lvl<T> &f, const Lvl<T> &c, T
     const T north = jc > 0 ? coarse_south(f, c.h, scale, ic, jc - 1) : (T)0;
     const T d = m + e + west + s + north;
     c.mass[k] = m;
-    T *q = c.coef + 4 * (size_t)k;
-    q[0] = e, q[1] = s, q[2] = d, q[3] = omega / d;
+    T *q = c.coef + kRec * (size_t)k;
+    q[0] = e, q[1] = s, q[2] = d;
 }
 
 template <typename T>
-__global__ void k_coarsen(Lvl<T> f, Lvl<T> c, T scale, T omega) {
+__global__ void k_coarsen(Lvl<T> f, Lvl<T> c, T scale) {
     const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k < c.n) coarsen_cell(f, c, scale, omega, k);
+    if (k < c.n) coarsen_cell(f, c, scale, k);
 }
 
 // One cell of A x on a level in the order of `Level::apply_row` (MG:203-240): diagonal, east
@@ -670,21 +672,19 @@ __device__ inline void loadv(const T *p, T (&d)[V]) {
         for (int v = 0; v < V; v++) d[v] = p[v];
     }
 }
-// The packed operator of V consecutive cells (16- or 32-byte records).
+// The operator records of V consecutive cells (kRec values each): e, s, diag.
 template <typename T, int V>
-__device__ inline void load_coef(const T *coef, size_t g, T (&e)[V], T (&s)[V], T (&d)[V],
-                                 T (&wd)[V]) {
+__device__ inline void load_coef(const T *coef, size_t g, T (&e)[V], T (&s)[V], T (&d)[V]) {
+    const T *q = coef + kRec * g;
+    if constexpr (V == 2 && sizeof(T) == 4) {
+        // 6 floats from an 8-byte boundary (g is even): three float2 loads.
+        const float2 a = reinterpret_cast<const float2 *>(q)[0];
+        const float2 b = reinterpret_cast<const float2 *>(q)[1];
+        const float2 c = reinterpret_cast<const float2 *>(q)[2];
+        e[0] = a.x, s[0] = a.y, d[0] = b.x, e[1] = b.y, s[1] = c.x, d[1] = c.y;
+    } else {
 #pragma unroll
-    for (int v = 0; v < V; v++) {
-        const T *q = coef + 4 * (g + v);
-        if constexpr (sizeof(T) == 4) {
-            const float4 a = *reinterpret_cast<const float4 *>(q);
-            e[v] = a.x, s[v] = a.y, d[v] = a.z, wd[v] = a.w;
-        } else {
-            const double2 a = reinterpret_cast<const double2 *>(q)[0];
-            const double2 b = reinterpret_cast<const double2 *>(q)[1];
-            e[v] = a.x, s[v] = a.y, d[v] = b.x, wd[v] = b.y;
-        }
+        for (int v = 0; v < V; v++) e[v] = q[kRec * v], s[v] = q[kRec * v + 1], d[v] = q[kRec * v + 2];
     }
 }
 
@@ -710,14 +710,28 @@ __device__ inline void load_coef_bf16(const unsigned short *coefh, size_t g, flo
     }
 }
 
-// The records of V cells from the level: its bf16 copy when CB, else its own.
+// The records of V cells from the level: its bf16 copy when CB (with omega / diag), else its own
+// (e, s, diag; the weight is formed by weights() when the row is consumed).
 template <typename T, int V, int CB>
 __device__ inline void load_rec(const Lvl<T> &L, size_t g, T (&e)[V], T (&s)[V], T (&d)[V],
                                 T (&wd)[V]) {
     if constexpr (CB && sizeof(T) == 4)
         load_coef_bf16<V>(L.coefh, g, e, s, d, wd);
     else
-        load_coef<T, V>(L.coef, g, e, s, d, wd);
+        load_coef<T, V>(L.coef, g, e, s, d);
+}
+
+// The smoother's weights omega / diag of a row of V cells (the bf16 copy's own when CB; 0 on a row
+// outside the level, whose diag reads 0).
+template <typename T, int V, int CB>
+__device__ inline void weights(const Lvl<T> &L, const T (&d)[V], const T (&wdq)[V], T (&wd)[V]) {
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        if constexpr (CB && sizeof(T) == 4)
+            wd[v] = wdq[v];
+        else
+            wd[v] = d[v] != (T)0 ? L.omega / d[v] : (T)0;
+    }
 }
 
 template <typename T, int V>
@@ -928,7 +942,7 @@ __device__ __forceinline__ double spre_band(const Lvl<T> &L, T *__restrict__ zou
             copy(z1c, z1b), copy(z1b, z1a), copy(z2d, z2c), copy(z2c, z2b);
             copy(rc, rb), copy(rb, ra), copy(wdb, wda), copy(dgc, dgb), copy(dgb, dga);
             copy(ec, eb), copy(eb, ea), copy(sd, sc), copy(sc, sb), copy(sb, sa);
-            copy(wda, qwd[u]), copy(dga, qdg[u]), copy(ea, qe[u]);
+            weights<T, V, CB>(L, qdg[u], qwd[u], wda), copy(dga, qdg[u]), copy(ea, qe[u]);
             copy(sa, qs[u]);
             if (RM == kRLevel) {
                 copy(ra, qr[u]);
@@ -1055,7 +1069,7 @@ __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restric
             copy(zpc, zpb), copy(zpb, zpa), copy(z3d, z3c), copy(z3c, z3b);
             copy(rc, rb), copy(rb, ra), copy(wdc, wdb), copy(wdb, wda), copy(dgc, dgb);
             copy(dgb, dga), copy(ec, eb), copy(eb, ea), copy(sd, sc), copy(sc, sb), copy(sb, sa);
-            copy(wda, qwd[u]), copy(dga, qdg[u]), copy(ea, qe[u]);
+            weights<T, V, CB>(L, qdg[u], qwd[u], wda), copy(dga, qdg[u]), copy(ea, qe[u]);
             copy(sa, qs[u]);
             if (RM == kR64) {
                 copy(r64c, r64b), copy(r64b, r64a);
@@ -1129,7 +1143,7 @@ __global__ __launch_bounds__(kBlock) void k_bandcopy(Lvl<float> L, float *__rest
         if (j_ >= 0 && j_ < h && bd.col_ok) {                                                 \
             const size_t g_ = (size_t)j_ * w + gi0;                                           \
             loadv<float, V>(L.r + g_, qr[u]);                                                 \
-            load_coef<float, V>(L.coef, g_, qe[u], qs[u], qdg[u], qwd[u]);                    \
+            load_coef<float, V>(L.coef, g_, qe[u], qs[u], qdg[u]);                            \
         } else {                                                                              \
             zero(qr[u]), zero(qwd[u]), zero(qdg[u]), zero(qe[u]), zero(qs[u]);                \
         }                                                                                     \
@@ -1142,7 +1156,7 @@ __global__ __launch_bounds__(kBlock) void k_bandcopy(Lvl<float> L, float *__rest
             const int ja = bd.row0 - 2 + k0 + u;
             float out[V];
 #pragma unroll
-            for (int v = 0; v < V; v++) out[v] = qr[u][v] + qe[u][v] + qs[u][v] + qdg[u][v] + qwd[u][v];
+            for (int v = 0; v < V; v++) out[v] = qr[u][v] + qe[u][v] + qs[u][v] + qdg[u][v];
             if (k0 + u + A < steps) DTD_FETCH_BC(ja + A, u);
             if (bd.own && ja >= bd.row0 && ja < bd.row0 + bd.rows && ja < h)
                 storev<float, V>(zout + (size_t)ja * w + gi0, out);
