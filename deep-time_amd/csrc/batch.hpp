@@ -260,12 +260,16 @@ __global__ void kb_pair(const int2 *__restrict__ rng, const CgState *__restrict_
 }
 
 // flag = 1 when every grid's loop has stopped.
+// flag[0] = every grid has stopped; also into host (host memory the device can write) if given.
 __global__ __launch_bounds__(kBlock) void kb_all_done(const CgState *__restrict__ st, int n,
-                                                      int *flag) {
+                                                      int *flag, int *host = nullptr) {
     int open = 0;
     for (int g = threadIdx.x; g < n; g += kBlock) open |= st[g].done == 0;
     open = __syncthreads_or(open);
-    if (threadIdx.x == 0) flag[0] = !open;
+    if (threadIdx.x == 0) {
+        flag[0] = !open;
+        if (host) *reinterpret_cast<volatile int *>(host) = !open;
+    }
 }
 
 // ---- the plan of one batch shape -------------------------------------------------------------
@@ -301,7 +305,6 @@ struct BatchWs {
     CgState *h_st = nullptr;
     double *h_pairs = nullptr;
     BInit *h_init = nullptr;
-    int *h_flag = nullptr;
 };
 
 void free_batch(dtd_ctx *ctx) {
@@ -313,7 +316,6 @@ void free_batch(dtd_ctx *ctx) {
     if (bw->h_st) (void)hipHostFree(bw->h_st);
     if (bw->h_pairs) (void)hipHostFree(bw->h_pairs);
     if (bw->h_init) (void)hipHostFree(bw->h_init);
-    if (bw->h_flag) (void)hipHostFree(bw->h_flag);
     delete bw;
     ctx->batch = nullptr;
 }
@@ -531,7 +533,6 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
     HIPCHK(hipHostMalloc((void **)&bw->h_st, n * sizeof(CgState), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&bw->h_pairs, 2 * n * sizeof(double), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&bw->h_init, n * sizeof(BInit), hipHostMallocDefault));
-    HIPCHK(hipHostMalloc((void **)&bw->h_flag, 2 * sizeof(int), hipHostMallocDefault));
 }
 
 // ---- enqueueing ------------------------------------------------------------------------------
@@ -570,7 +571,7 @@ static void batch_vcycle(BatchWs &bw, hipStream_t s) {
 }
 
 // One iteration of every grid still running (MG:630-733), then the all-stopped flag.
-static void batch_iteration(BatchWs &bw, int parity, hipStream_t s) {
+static void batch_iteration(BatchWs &bw, int parity, hipStream_t s, int *host_flag) {
     const int n = bw.ngrids;
     kb_sdir<<<bw.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.band_l.blk, parity, bw.parts);
     LAUNCH_CHECK();
@@ -583,7 +584,7 @@ static void batch_iteration(BatchWs &bw, int parity, hipStream_t s) {
     batch_vcycle(bw, s);
     kb_step<kStepRZ><<<n, kReduceBlock, 0, s>>>(bw.rng_rz, bw.st, bw.parts);
     LAUNCH_CHECK();
-    kb_all_done<<<1, kBlock, 0, s>>>(bw.st, n, bw.flag);
+    kb_all_done<<<1, kBlock, 0, s>>>(bw.st, n, bw.flag, host_flag);
     LAUNCH_CHECK();
 }
 
@@ -593,7 +594,7 @@ static void batch_graphs(dtd_ctx *ctx, BatchWs &bw) {
         hipGraph_t gr = nullptr;
         HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         try {
-            batch_iteration(bw, k, ctx->stream);
+            batch_iteration(bw, k, ctx->stream, ctx->dev_flag + k);
         } catch (...) {
             (void)hipStreamEndCapture(ctx->stream, &gr);
             if (gr) (void)hipGraphDestroy(gr);
@@ -629,7 +630,7 @@ static void batch_pass(dtd_ctx *ctx, BatchWs &bw, int from_b) {
     // flag after iteration k is read (the flag before the loop counts as iteration 0's), so the
     // device runs at most one iteration that every grid skips; the state's cap stops every grid by
     // max_iter.
-    HIPCHK(hipMemcpyAsync(&bw.h_flag[1], bw.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&ctx->host_flag[1], bw.flag, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(ctx->flag_ev[1], s));
     const int max_iter = (int)ctx->opts.max_iter;
     for (int k = 0; k <= max_iter + 1; k++) {
@@ -637,11 +638,10 @@ static void batch_pass(dtd_ctx *ctx, BatchWs &bw, int from_b) {
         if (ctx->use_graph)
             HIPCHK(hipGraphLaunch(bw.graph[par], s));
         else
-            batch_iteration(bw, par, s);
-        HIPCHK(hipMemcpyAsync(&bw.h_flag[par], bw.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+            batch_iteration(bw, par, s, ctx->dev_flag + par);
         HIPCHK(hipEventRecord(ctx->flag_ev[par], s));
         HIPCHK(hipEventSynchronize(ctx->flag_ev[par ^ 1]));
-        if (bw.h_flag[par ^ 1]) break;
+        if (ctx->host_flag[par ^ 1]) break;
     }
     kb_finish1<<<bw.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.band_l.blk, bw.parts);
     LAUNCH_CHECK();

@@ -88,7 +88,8 @@ struct dtd_ctx {
     hipEvent_t ev[6] = {};
     hipEvent_t flag_ev[2] = {};
     dtd::CgState *host_st = nullptr;  // pinned
-    int *host_flag = nullptr;         // pinned, two slots
+    int *host_flag = nullptr;         // pinned and coherent, two slots the device writes
+    int *dev_flag = nullptr;          // host_flag's device address
     double *host_pair = nullptr;      // pinned, two doubles
     dtd::Workspace ws;
     std::string err;

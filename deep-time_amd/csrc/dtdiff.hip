@@ -401,7 +401,8 @@ static void enqueue_iteration(dtd_ctx *ctx, int parity) {
         // it); the residual's buffers alternate with the parity: parity 0 reads r, writes r2.
         double *rin = parity == 0 ? ws.r : ws.r2, *rout = parity == 0 ? ws.r2 : ws.r;
         const int nbz = run_vcycle(ctx, rout, ws.st, kRUpdate, R64{rin, ws.ap, rout, 0.0});
-        k_cg_step<kStepRZ><<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.st);
+        k_cg_step<kStepRZ><<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.st,
+                                                       ctx->dev_flag + parity);
         LAUNCH_CHECK();
         return;
     }
@@ -415,7 +416,7 @@ static void enqueue_iteration(dtd_ctx *ctx, int parity) {
     k_cg_step<kStepRR><<<1, kReduceBlock, 0, s>>>(ws.parts, nbu, ws.st);
     LAUNCH_CHECK();
     const int nbz = run_vcycle(ctx, ws.r, ws.st);
-    k_cg_step<kStepRZ><<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.st);
+    k_cg_step<kStepRZ><<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.st, ctx->dev_flag + parity);
     LAUNCH_CHECK();
 }
 
@@ -535,8 +536,6 @@ static size_t solve_core(dtd_ctx *ctx, const double *b, const double *kd, double
             HIPCHK(hipGraphLaunch(ws.graph[par], s));
         else
             enqueue_iteration(ctx, par);
-        HIPCHK(hipMemcpyAsync(&ctx->host_flag[par], &ws.st->done, sizeof(int),
-                              hipMemcpyDeviceToHost, s));
         HIPCHK(hipEventRecord(ctx->flag_ev[par], s));
         if (k >= 1) {
             HIPCHK(hipEventSynchronize(ctx->flag_ev[par ^ 1]));
@@ -660,7 +659,9 @@ int dtd_create(const dtd_opts *opts, dtd_ctx **out) {
         for (auto &e : ctx->ev) HIPCHK(hipEventCreate(&e));
         for (auto &e : ctx->flag_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         HIPCHK(hipHostMalloc((void **)&ctx->host_st, sizeof(CgState), hipHostMallocDefault));
-        HIPCHK(hipHostMalloc((void **)&ctx->host_flag, 2 * sizeof(int), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void **)&ctx->host_flag, 2 * sizeof(int),
+                             hipHostMallocMapped | hipHostMallocCoherent));
+        HIPCHK(hipHostGetDevicePointer((void **)&ctx->dev_flag, ctx->host_flag, 0));
         HIPCHK(hipHostMalloc((void **)&ctx->host_pair, 2 * sizeof(double), hipHostMallocDefault));
     });
     if (rc != DTD_OK) {

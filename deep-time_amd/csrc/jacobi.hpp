@@ -125,10 +125,8 @@ enum { kJInit = 0, kJPap = 1, kJUpdate = 2 };
 constexpr int kStepBlock = 1024, kStepChunk = 4096;
 
 template <int STEP>
-__global__ __launch_bounds__(kStepBlock) void kj_step(const double *__restrict__ s0,
-                                                      const double *__restrict__ s1, int h,
-                                                      JState *st) {
-    __shared__ double l0[kStepChunk], l1[STEP == kJPap ? 1 : kStepChunk];
+__device__ inline void kj_step_body(const double *__restrict__ s0, const double *__restrict__ s1,
+                                    int h, JState *st, double *l0, double *l1) {
     if (st->done) return;
     double v0 = -0.0, v1 = -0.0;
     for (int c0 = 0; c0 < h; c0 += kStepChunk) {
@@ -176,6 +174,16 @@ __global__ __launch_bounds__(kStepBlock) void kj_step(const double *__restrict__
         st->rz = v1;
         if (st->its >= st->max_iter) st->done = 4;
     }
+}
+
+// With flag given (host memory the device can write), the step leaves the stop flag there too.
+template <int STEP>
+__global__ __launch_bounds__(kStepBlock) void kj_step(const double *__restrict__ s0,
+                                                      const double *__restrict__ s1, int h,
+                                                      JState *st, int *flag = nullptr) {
+    __shared__ double l0[kStepChunk], l1[STEP == kJPap ? 1 : kStepChunk];
+    kj_step_body<STEP>(s0, s1, h, st, l0, l1);
+    if (flag && threadIdx.x == 0) *reinterpret_cast<volatile int *>(flag) = st->done;
 }
 
 // The direction pass (D:782-880) cell by cell: the pending x += alpha p_old, then
@@ -317,7 +325,7 @@ static void jacobi_iteration(dtd_ctx *ctx, int parity) {
     LAUNCH_CHECK();
     kj_rows<kRowsRZ><<<nr, kColTile, 0, s>>>(ws.w, ws.h, ws.r, ws.jdiag, s0, s1, st);
     LAUNCH_CHECK();
-    kj_step<kJUpdate><<<1, kStepBlock, 0, s>>>(s0, s1, ws.h, st);
+    kj_step<kJUpdate><<<1, kStepBlock, 0, s>>>(s0, s1, ws.h, st, ctx->dev_flag + parity);
     LAUNCH_CHECK();
 }
 
@@ -391,8 +399,6 @@ static size_t jacobi_solve(dtd_ctx *ctx, const double *b, const double *kd, doub
             HIPCHK(hipGraphLaunch(ws.jgraph[par], s));
         else
             jacobi_iteration(ctx, par);
-        HIPCHK(hipMemcpyAsync(&ctx->host_flag[par], &st->done, sizeof(int), hipMemcpyDeviceToHost,
-                              s));
         HIPCHK(hipEventRecord(ctx->flag_ev[par], s));
         HIPCHK(hipEventSynchronize(ctx->flag_ev[par ^ 1]));
         if (ctx->host_flag[par ^ 1]) break;

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace dtd {
 
 constexpr int kBlock = 256;       // threads per block of the cell-parallel kernels
@@ -113,20 +115,18 @@ constexpr int kRec = 3;
 // stored (it is re-formed the same way wherever it is used).  The finest level of the cycle goes
 // out as one record a cell, (east, south, diag): in f32 (coef32, the reference's cycle) or in f64
 // (coef64, the f64 cycle).
-__device__ __forceinline__ void operator_cell(const Grid &g, const double *__restrict__ kd,
-                                              double dt, double s2, double2 *__restrict__ es,
-                                              float *__restrict__ coef32,
-                                              double *__restrict__ coef64, int c) {
+__device__ __forceinline__ void operator_faces(const Grid &g, const double *__restrict__ kd,
+                                               double dt, double s2, int c, double &e, double &s,
+                                               double &d) {
     const int j = c / g.w, i = c - j * g.w, c0 = j * g.w;
     const double k = kd[c];
-    double e;
     if (i + 1 < g.w)
         e = face(k, kd[c + 1], dt, s2);
     else if (g.wrap)
         e = face(k, kd[c0], dt, s2);
     else
         e = 0.0;
-    const double s = j + 1 < g.h ? face(k, kd[c + g.w], dt, s2) : 0.0;
+    s = j + 1 < g.h ? face(k, kd[c + g.w], dt, s2) : 0.0;
     double wv;
     if (i > 0)
         wv = face(kd[c - 1], k, dt, s2);
@@ -135,7 +135,15 @@ __device__ __forceinline__ void operator_cell(const Grid &g, const double *__res
     else
         wv = 0.0;
     const double nv = j > 0 ? face(kd[c - g.w], k, dt, s2) : 0.0;
-    const double d = 1.0 + e + wv + s + nv;
+    d = 1.0 + e + wv + s + nv;
+}
+
+__device__ __forceinline__ void operator_cell(const Grid &g, const double *__restrict__ kd,
+                                              double dt, double s2, double2 *__restrict__ es,
+                                              float *__restrict__ coef32,
+                                              double *__restrict__ coef64, int c) {
+    double e, s, d;
+    operator_faces(g, kd, dt, s2, c, e, s, d);
     if (es) es[c] = make_double2(e, s);
     if (coef32) {
         float *q = coef32 + kRec * (size_t)c;
@@ -147,11 +155,41 @@ __device__ __forceinline__ void operator_cell(const Grid &g, const double *__res
     }
 }
 
-__global__ void k_operator(Grid g, const double *__restrict__ kd, double dt, double s2,
-                           double2 *__restrict__ es, float *__restrict__ coef32,
-                           double *__restrict__ coef64) {
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c < g.n) operator_cell(g, kd, dt, s2, es, coef32, coef64, c);
+// The block's kRec * kBlock record values gathered in LDS, then written out as whole 16-byte
+// words (the records of a full block start 16-byte aligned).
+template <typename T>
+__device__ __forceinline__ void store_records(T *__restrict__ out, const T *buf, int cells) {
+    constexpr int per = 16 / sizeof(T);
+    if (cells == kBlock) {
+        using W = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+        for (int q = threadIdx.x; q < kRec * kBlock / per; q += kBlock)
+            reinterpret_cast<W *>(out)[q] = reinterpret_cast<const W *>(buf)[q];
+    } else {
+        for (int q = threadIdx.x; q < kRec * cells; q += kBlock) out[q] = buf[q];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_operator(Grid g, const double *__restrict__ kd,
+                                                     double dt, double s2,
+                                                     double2 *__restrict__ es,
+                                                     float *__restrict__ coef32,
+                                                     double *__restrict__ coef64) {
+    __shared__ double buf64[kRec * kBlock];
+    float *buf32 = reinterpret_cast<float *>(buf64);
+    const int c0 = blockIdx.x * kBlock, t = threadIdx.x, c = c0 + t;
+    if (c < g.n) {
+        double e, s, d;
+        operator_faces(g, kd, dt, s2, c, e, s, d);
+        if (es) es[c] = make_double2(e, s);
+        if (coef32) buf32[kRec * t] = (float)e, buf32[kRec * t + 1] = (float)s,
+                    buf32[kRec * t + 2] = (float)d;
+        if (coef64) buf64[kRec * t] = e, buf64[kRec * t + 1] = s, buf64[kRec * t + 2] = d;
+    }
+    if (!coef32 && !coef64) return;
+    __syncthreads();
+    const int cells = min(kBlock, g.n - c0);
+    if (coef32) store_records<float>(coef32 + kRec * (size_t)c0, buf32, cells);
+    if (coef64) store_records<double>(coef64 + kRec * (size_t)c0, buf64, cells);
 }
 
 // The f64 diagonal of cell c from the packed faces (the sum Operator::new takes).
@@ -297,10 +335,14 @@ __device__ inline void cg_step_block(const double *__restrict__ parts, int npart
     if (threadIdx.x == 0) cg_step_apply<STEP>(v, bb, st);
 }
 
+// With flag given (host memory the device can write), the step leaves the stop flag there too:
+// the host reads it once the iteration's event has completed, with no copy in the stream.
 template <int STEP>
-__global__ void k_cg_step(const double *__restrict__ parts, int nparts, CgState *st) {
+__global__ void k_cg_step(const double *__restrict__ parts, int nparts, CgState *st,
+                          int *flag = nullptr) {
     __shared__ double lds[kReduceBlock / 64];
     cg_step_block<STEP>(parts, nparts, st, lds);
+    if (flag && threadIdx.x == 0) *reinterpret_cast<volatile int *>(flag) = st->done;
 }
 
 // The put-back's two sums (MG:737-755), then its step k = lost / weight.
@@ -1757,7 +1799,7 @@ __global__ void k_transpose(const double *__restrict__ src, double *__restrict__
 // One line per block (n / 16 threads, a power of two), from src to dst (may be the same array);
 // faces from kd along the line, the last one wrapping when `ring`.
 template <int MAXT>
-__global__ __launch_bounds__(MAXT) void k_lod_lines(int n, int ring,
+__global__ __launch_bounds__(MAXT, 2048 / MAXT) void k_lod_lines(int n, int ring,
                                                     const double *__restrict__ kd,
                                                     const double *__restrict__ src,
                                                     double *__restrict__ dst, double dt,
