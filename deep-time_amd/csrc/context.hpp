@@ -79,6 +79,7 @@ struct Workspace {
 };
 
 struct BatchWs;
+struct FailWs;
 
 }  // namespace dtd
 
@@ -95,13 +96,15 @@ struct dtd_ctx {
     std::string err;
     size_t cycle_levels = 0;  // levels built by dtd_cycle_new
     bool use_graph = true;
-    dtd::BatchWs *batch = nullptr;  // the batched solve's buffers (batch.hip)
+    dtd::BatchWs *batch = nullptr;  // the batched solve's buffers (batch.hpp)
+    dtd::FailWs *fail = nullptr;    // the slope failure's buffers (failure.hpp)
 };
 
 namespace dtd {
 
 void free_ws(Workspace &ws);
-void free_batch(dtd_ctx *ctx);  // drops the batched solve's buffers (batch.hip)
+void free_batch(dtd_ctx *ctx);  // drops the batched solve's buffers (batch.hpp)
+void free_fail(dtd_ctx *ctx);   // drops the slope failure's buffers (failure.hpp)
 
 // Band height for a streaming kernel on a w x h level (a multiple of 4 rows; DTD_BAND_ROWS
 // overrides it).
@@ -173,6 +176,7 @@ int guard(dtd_ctx *ctx, F f) {
             (void)hipStreamSynchronize(ctx->stream);
             free_ws(ctx->ws);
             free_batch(ctx);
+            free_fail(ctx);
             ctx->cycle_levels = 0;
         }
         return e.code;

@@ -266,7 +266,7 @@ static int launch_resid(Workspace &ws, const double *kd, double dt, double s2, c
 static int bottom_cells() {
     static const int v = [] {
         const char *e = getenv("DTD_BOTTOM_CELLS");
-        return e ? atoi(e) : 1024;
+        return e ? atoi(e) : 2048;
     }();
     return v;
 }
@@ -293,9 +293,12 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
                   CgState *st, hipStream_t s, int rm = kRLevel, R64 rio = R64{},
                   const Lvl<T> *dlv = nullptr) {
     Lvl<T> &L = lv[l];
-    if (dlv && l >= 1 && nu == 2 && fused_enabled() && L.n <= bottom_cells() && !L.coefh) {
-        // This level and every coarser one in one block.
-        k_bottom<T><<<1, kBottomBlock, 0, s>>>(dlv, (int)l, (int)lv.size(), st);
+    size_t lds = 0;
+    for (size_t k = l; k < lv.size(); k++) lds += (kRec + 3) * (size_t)lv[k].n * sizeof(T);
+    if (dlv && l >= 1 && nu == 2 && fused_enabled() && L.n <= bottom_cells() && !L.coefh &&
+        lds <= kBottomLds && lv.size() - l <= (size_t)kBottomLevels) {
+        // This level and every coarser one in one block, in LDS.
+        k_bottom<T><<<1, kBottomBlock, lds, s>>>(dlv, (int)l, (int)lv.size(), st);
         LAUNCH_CHECK();
         return 1;
     }
@@ -620,6 +623,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_kd(long n, const double *__rest
 
 #include "batch.hpp"
 #include "jacobi.hpp"
+#include "failure.hpp"
 
 using namespace dtd;
 
@@ -678,6 +682,7 @@ void dtd_destroy(dtd_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_ws(ctx->ws);
     free_batch(ctx);
+    free_fail(ctx);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->flag_ev)
@@ -790,6 +795,56 @@ int dtd_fill_kd(dtd_ctx *ctx, size_t n, const double *h_dev, const unsigned char
             (long)n, h_dev, class_dev, diffu_dev, kd_land, sea, d0, wave_base, tidal_range, kd_dev);
         LAUNCH_CHECK();
         HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+static void check_failure(size_t w, size_t h, const dtd_failure *p, const void *sea,
+                          const void *hgt, const void *avail) {
+    check_grid(w, h);
+    if (!p) throw Error{DTD_EINVAL, "null parameters"};
+    if (w * h && (!sea || !hgt || !avail)) throw Error{DTD_EINVAL, "null array"};
+    if (!(p->rate >= 0.0) || !std::isfinite(p->rate) || !std::isfinite(p->critical_slope))
+        throw Error{DTD_EINVAL, "rate and critical slope must be finite, rate >= 0"};
+}
+
+int dtd_slope_failure_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x,
+                             const unsigned char *sea_dev, double sea_level, const dtd_failure *p,
+                             double *h_dev, double *avail_dev, double *moved,
+                             unsigned *iterations) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (moved) *moved = 0.0;
+        if (iterations) *iterations = 0;
+        check_failure(w, h, p, sea_dev, h_dev, avail_dev);
+        if (w * h == 0 || p->iterations == 0) return;
+        FailWs &f = fit_fail(ctx, w * h, false);
+        const double m = slope_failure(ctx, f, w, h, dx, wrap_x, sea_dev, sea_level,
+                                       p->critical_slope, p->rate, p->iterations, h_dev,
+                                       avail_dev, iterations);
+        if (moved) *moved = m;
+    });
+}
+
+int dtd_slope_failure(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x,
+                      const unsigned char *sea, double sea_level, const dtd_failure *p,
+                      double *h_inout, double *avail_inout, double *moved, unsigned *iterations) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (moved) *moved = 0.0;
+        if (iterations) *iterations = 0;
+        check_failure(w, h, p, sea, h_inout, avail_inout);
+        const size_t n = w * h;
+        if (n == 0 || p->iterations == 0) return;
+        FailWs &f = fit_fail(ctx, n, true);
+        upload(ctx, f.sd, sea, n);
+        upload(ctx, f.hd, h_inout, n * sizeof(double));
+        upload(ctx, f.ad, avail_inout, n * sizeof(double));
+        const double m = slope_failure(ctx, f, w, h, dx, wrap_x, f.sd, sea_level,
+                                       p->critical_slope, p->rate, p->iterations, f.hd, f.ad,
+                                       iterations);
+        download(ctx, h_inout, f.hd, n * sizeof(double));
+        download(ctx, avail_inout, f.ad, n * sizeof(double));
+        if (moved) *moved = m;
     });
 }
 

@@ -639,35 +639,147 @@ __global__ void k_coarsest(Lvl<T> L, T *__restrict__ zout, int sweeps,
 // second sweep and restriction; the coarsest level's 16 sweeps; then back up, each level's
 // corrected sweep and last sweep (MG:500-523).  Every phase is one pass of the block over the
 // level's cells with the one-sweep kernels' arithmetic, so the result is theirs bit for bit; a
-// barrier separates the phases.  Each level's result lands in z[1], as the launch-per-sweep cycle
-// leaves it.  lv holds the hierarchy's levels (device copy); nl is their count.
+// barrier separates the phases.  The levels live in LDS for the launch (records, r, z[0], z[1]:
+// (kRec + 3) values a cell, kBottomLds bytes at most, dynamic shared memory): they are loaded once,
+// and only level b0's result goes back, into its z[1], where the launch-per-sweep cycle leaves
+// it.  lv holds the hierarchy's levels (device copy); nl is their count.
 constexpr int kBottomBlock = 1024;
+constexpr int kBottomLevels = 16;
+constexpr size_t kBottomLds = 65536;
+
+// One sweep of cell (i, j) of level L from z (z + the prolonged correction, already added, on the
+// way up): z + wd (r - A z).
+template <typename T>
+__device__ inline T bottom_sweep(const Lvl<T> &L, const T *z, int c, int i, int j) {
+    const T ax = lvl_apply(L, c, i, j, [&](int k) { return z[k]; });
+    return z[c] + c_wd(L, c) * (L.r[c] - ax);
+}
 
 template <typename T>
 __global__ __launch_bounds__(kBottomBlock) void k_bottom(const Lvl<T> *__restrict__ lv, int b0,
                                                          int nl, const CgState *__restrict__ st) {
+    extern __shared__ double bottom_mem[];
+    __shared__ Lvl<T> loc[kBottomLevels];
+    __shared__ int cells[kBottomLevels];
     __shared__ T zs[2][64];
     if (st && st->done) return;
-    const int t = threadIdx.x;
-    for (int l = b0; l + 1 < nl; l++) {
-        const Lvl<T> L = lv[l], C = lv[l + 1];
+    const int t = threadIdx.x, nb = nl - b0;
+    // The level table: one thread a level reads it, then each level's LDS place follows the
+    // sizes of the levels before it.
+    Lvl<T> mine{};
+    if (t < nb) {
+        mine = lv[b0 + t];
+        cells[t] = mine.n;
+    }
+    __syncthreads();
+    if (t < nb) {
+        size_t off = 0;
+        for (int q = 0; q < t; q++) off += (size_t)(kRec + 3) * cells[q];
+        T *p = reinterpret_cast<T *>(bottom_mem) + off;
+        Lvl<T> L = mine;
+        L.coef = p, p += kRec * L.n;
+        L.r = p, p += L.n;
+        L.z[0] = p, p += L.n;
+        L.z[1] = p;
+        L.mass = nullptr, L.coefh = nullptr;
+        loc[t] = L;
+    }
+    __syncthreads();
+    for (int q = 0; q < nb; q++) {
+        const T *gc = lv[b0 + q].coef;
+        T *lc = loc[q].coef;
+        for (int k = t; k < kRec * cells[q]; k += kBottomBlock) lc[k] = gc[k];
+    }
+    {
+        const T *gr = lv[b0].r;
+        T *lr = loc[0].r;
+        for (int c = t; c < cells[0]; c += kBottomBlock) lr[c] = gr[c];
+    }
+    __syncthreads();
+    for (int q = 0; q + 1 < nb; q++) {
+        const Lvl<T> L = loc[q], C = loc[q + 1];
         for (int c = t; c < L.n; c += kBottomBlock) L.z[0][c] = c_wd(L, c) * L.r[c];
         __syncthreads();
-        for (int c = t; c < L.n; c += kBottomBlock) L.z[1][c] = sweep_cell(L, L.z[0], (const T *)nullptr, 0, c);
+        for (int c = t; c < L.n; c += kBottomBlock) {
+            const int j = c / L.w;
+            L.z[1][c] = bottom_sweep(L, L.z[0], c, c - j * L.w, j);
+        }
         __syncthreads();
         for (int k = t; k < C.n; k += kBottomBlock) C.r[k] = restrict_cell(L, L.z[1], C, k);
         __syncthreads();
     }
     {
-        const Lvl<T> L = lv[nl - 1];
-        coarsest_wave<T>(L, L.z[1], 16, nullptr, nullptr, zs);
+        // The coarsest level (at most 64 cells): z = wd r, then 15 sweeps, by the first wave with
+        // its stencil's records in registers.
+        const Lvl<T> L = loc[nb - 1];
+        if (t < 64) {
+            const int c = t;
+            const bool live = c < L.n;
+            int j = 0, i = 0;
+            T d = 0, wd = 0, rr = 0, ce = 0, cw = 0, cs = 0, cn = 0;
+            int ke = c, kw = c, ks = c, kn = c;
+            bool he = false, hw = false, hs = false, hn = false, hew = false, hww = false;
+            if (live) {
+                j = c / L.w, i = c - j * L.w;
+                const int c0 = j * L.w;
+                d = c_diag(L, c), wd = c_wd(L, c), rr = L.r[c];
+                he = i + 1 < L.w, hw = i > 0, hs = j + 1 < L.h, hn = j > 0;
+                hew = L.wrap && i == L.w - 1, hww = L.wrap && i == 0;
+                if (he) ce = c_east(L, c), ke = c + 1;
+                if (hw) cw = c_east(L, c - 1), kw = c - 1;
+                if (hew) ce = c_east(L, c0 + L.w - 1), ke = c0;
+                if (hww) cw = c_east(L, c0 + L.w - 1), kw = c0 + L.w - 1;
+                if (hs) cs = c_south(L, c), ks = c + L.w;
+                if (hn) cn = c_south(L, c - L.w), kn = c - L.w;
+                zs[0][c] = wd * rr;
+            }
+            // One wave: its LDS accesses take effect in program order; the fence keeps the
+            // compiler from moving them across the sweep boundary.
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            int cur = 0;
+            for (int k = 1; k < 16; k++) {
+                if (live) {
+                    const T *zz = zs[cur];
+                    // lvl_apply's terms in its order: diag, east, west, the wrapped east / west,
+                    // south, north.
+                    T o = d * zz[c];
+                    if (he) o -= ce * zz[ke];
+                    if (hw) o -= cw * zz[kw];
+                    if (hew) o -= ce * zz[ke];
+                    if (hww) o -= cw * zz[kw];
+                    if (hs) o -= cs * zz[ks];
+                    if (hn) o -= cn * zz[kn];
+                    zs[cur ^ 1][c] = zz[c] + wd * (rr - o);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                cur ^= 1;
+            }
+            if (live) (nb == 1 ? lv[b0].z[1] : L.z[0])[c] = zs[cur][c];
+        }
         __syncthreads();
     }
-    for (int l = nl - 2; l >= b0; l--) {
-        const Lvl<T> L = lv[l], C = lv[l + 1];
-        for (int c = t; c < L.n; c += kBottomBlock) L.z[0][c] = sweep_cell(L, L.z[1], C.z[1], C.w, c);
+    // Back up: each level's result in its z[0] (the coarsest's too).
+    for (int q = nb - 2; q >= 0; q--) {
+        const Lvl<T> L = loc[q], C = loc[q + 1];
+        // z' = z + the coarse correction, once a cell (the sum the corrected sweep takes for
+        // every point of its stencil); then the corrected sweep and the last.
+        for (int c = t; c < L.n; c += kBottomBlock) {
+            const int j = c / L.w, i = c - j * L.w;
+            L.z[0][c] = L.z[1][c] + C.z[0][(j >> 1) * C.w + (i >> 1)];
+        }
         __syncthreads();
-        for (int c = t; c < L.n; c += kBottomBlock) L.z[1][c] = sweep_cell(L, L.z[0], (const T *)nullptr, 0, c);
+        for (int c = t; c < L.n; c += kBottomBlock) {
+            const int j = c / L.w;
+            L.z[1][c] = bottom_sweep(L, L.z[0], c, c - j * L.w, j);
+        }
+        __syncthreads();
+        T *out = q > 0 ? L.z[0] : lv[b0].z[1];
+        for (int c = t; c < L.n; c += kBottomBlock) {
+            const int j = c / L.w;
+            out[c] = bottom_sweep(L, L.z[1], c, c - j * L.w, j);
+        }
         __syncthreads();
     }
 }

@@ -41,7 +41,11 @@ SYMBOLS = (
     "dtd_solve_device", "dtd_solve_rel_device", "dtd_operator", "dtd_apply", "dtd_lod",
     "dtd_cycle_new", "dtd_cycle_levels", "dtd_cycle_level", "dtd_cycle_precondition",
     "dtd_time_kernel", "dtd_solve_batch", "dtd_step_device", "dtd_fill_kd",
+    "dtd_slope_failure_device", "dtd_slope_failure",
 )
+# The stage's slope failure constants (crates/deeptime-core/src/surface/marine.rs:231-232).
+FAILURE_RATE = 0.05
+FAILURE_ITERATIONS = 40
 KERNEL_SWEEP, KERNEL_DIRECTION, KERNEL_UPDATE, KERNEL_RESTRICT = 0, 1, 2, 3
 KERNEL_PRE, KERNEL_POST, KERNEL_COPY, KERNEL_STREAM5, KERNEL_BANDCOPY = 4, 5, 6, 7, 8
 
@@ -86,6 +90,12 @@ class Grid(ctypes.Structure):
     _fields_ = [("w", ctypes.c_size_t), ("h", ctypes.c_size_t), ("dx", ctypes.c_double),
                 ("wrap_x", ctypes.c_int), ("dt", ctypes.c_double), ("tol_m", ctypes.c_double),
                 ("h_dev", ctypes.c_void_p), ("kd_dev", ctypes.c_void_p)]
+
+
+class Failure(ctypes.Structure):
+    """`FailureParams` (surface/marine.rs:254-262) as dtd_failure."""
+    _fields_ = [("critical_slope", ctypes.c_double), ("rate", ctypes.c_double),
+                ("iterations", ctypes.c_uint)]
 
 
 class BatchStats(ctypes.Structure):
@@ -152,6 +162,15 @@ def load(path: str = LIB_PATH):
                                   ctypes.c_double, ctypes.POINTER(Stats)]
     L.dtd_solve_batch.argtypes = [_vp, _sz, ctypes.POINTER(Grid), ctypes.c_double,
                                   ctypes.POINTER(_sz), _dp, ctypes.POINTER(BatchStats)]
+    L.dtd_slope_failure_device.argtypes = [_vp, _sz, _sz, ctypes.c_double, ctypes.c_int, _vp,
+                                           ctypes.c_double, ctypes.POINTER(Failure), _vp, _vp,
+                                           ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_uint)]
+    _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+    L.dtd_slope_failure.argtypes = [_vp, _sz, _sz, ctypes.c_double, ctypes.c_int, _u8p,
+                                    ctypes.c_double, ctypes.POINTER(Failure), _dp, _dp,
+                                    ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_uint)]
     for name in ("dtd_solve", "dtd_solve_device", "dtd_solve_rel_device", "dtd_operator",
                  "dtd_apply", "dtd_lod", "dtd_cycle_new", "dtd_cycle_level",
                  "dtd_cycle_precondition", "dtd_time_kernel"):
@@ -250,6 +269,40 @@ def fill_kd(ws: Workspace, n: int, h_dev, class_dev, diffu_dev, kd_land: float, 
     """The surface stage's kd fill (surface/mod.rs:1982-1992, `submarine_kd`) on device arrays."""
     ws.check(load().dtd_fill_kd(ws._ctx, n, _ptr(h_dev), _ptr(class_dev), _ptr(diffu_dev),
                                 kd_land, sea, d0, wave_base, tidal_range, _ptr(kd_dev)))
+
+
+def slope_failure(ws: Workspace, d: Dims, sea, sea_level: float, h, avail,
+                  critical_slope: float, rate: float = FAILURE_RATE,
+                  iterations: int = FAILURE_ITERATIONS):
+    """`marine::slope_failure` (surface/marine.rs:292-580) on host arrays: h and avail (f64,
+    C-contiguous) are updated in place; sea is one 0/1 byte a cell.  Returns (metres moved,
+    iterations run)."""
+    for a in (h, avail):
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+            raise TypeError("h and avail must be C-contiguous float64 arrays (changed in place)")
+        if a.size != d.cells() or np.size(sea) != d.cells():
+            raise ValueError("sea, h and avail must have w*h cells")
+    p = Failure(critical_slope, rate, iterations)
+    moved, its = ctypes.c_double(), ctypes.c_uint()
+    ws.check(load().dtd_slope_failure(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x),
+                                      np.ascontiguousarray(sea, dtype=np.uint8), sea_level,
+                                      ctypes.byref(p), h.ctypes.data_as(_dp),
+                                      avail.ctypes.data_as(_dp), ctypes.byref(moved),
+                                      ctypes.byref(its)))
+    return moved.value, its.value
+
+
+def slope_failure_device(ws: Workspace, d: Dims, sea_dev, sea_level: float, h_dev, avail_dev,
+                         critical_slope: float, rate: float = FAILURE_RATE,
+                         iterations: int = FAILURE_ITERATIONS):
+    """The slope failure on device arrays (sea: one byte a cell).  Returns (moved, iterations)."""
+    p = Failure(critical_slope, rate, iterations)
+    moved, its = ctypes.c_double(), ctypes.c_uint()
+    ws.check(load().dtd_slope_failure_device(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x),
+                                             _ptr(sea_dev), sea_level, ctypes.byref(p),
+                                             _ptr(h_dev), _ptr(avail_dev), ctypes.byref(moved),
+                                             ctypes.byref(its)))
+    return moved.value, its.value
 
 
 def step_device(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float) -> Stats:

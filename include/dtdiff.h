@@ -154,6 +154,31 @@ int dtd_fill_kd(dtd_ctx *ctx, size_t n, const double *h_dev, const unsigned char
                 const double *diffu_dev, double kd_land, double sea, double d0, double wave_base,
                 double tidal_range, double *kd_dev);
 
+/* The submarine slope failure's parameters (`FailureParams`, crates/deeptime-core/src/surface/
+ * marine.rs:254-262; the stage uses rate 0.05 and 40 iterations, marine.rs:231-232). */
+typedef struct dtd_failure {
+    double critical_slope; /* the steepest gradient sediment holds under the sea (tan) */
+    double rate;           /* share of a pair's excess over the critical drop moved per iteration */
+    unsigned iterations;
+} dtd_failure;
+
+/* `marine::slope_failure` (crates/deeptime-core/src/surface/marine.rs:292-580), the stage's step
+ * after the solve (surface/mod.rs:2012-2023), on device arrays of w*h cells: sea (0/1 a cell),
+ * h and avail (in/out).  Sediment on the sea floor steeper than the critical slope slides to its
+ * wet D8 neighbours, a cell giving at most its avail; an iteration that moves nothing ends the
+ * loop.  The heights and avail are the reference's bit for bit; *moved (may be NULL) gets the
+ * metres moved (sum |dz| / 2 over the movers, equal to rounding: the blocks' sums are added in
+ * order, the reference's cell by cell), *iterations (may be NULL) the iterations run.  The
+ * reference's own device path (marine/gpu.rs) rounds in f32; this one keeps f64. */
+int dtd_slope_failure_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x,
+                             const unsigned char *sea_dev, double sea_level, const dtd_failure *p,
+                             double *h_dev, double *avail_dev, double *moved,
+                             unsigned *iterations);
+/* The same on host arrays (copied to the device and back). */
+int dtd_slope_failure(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x,
+                      const unsigned char *sea, double sea_level, const dtd_failure *p,
+                      double *h_inout, double *avail_inout, double *moved, unsigned *iterations);
+
 /* One grid of a batched solve: a dtd_solve_device call's arguments. */
 typedef struct dtd_grid {
     size_t w, h;        /* cells */

@@ -8,6 +8,8 @@
  *   PAR crates/deeptime-core/src/surface/diffusion/parity.rs
  *   SM  crates/deeptime-core/src/sphere/mod.rs
  *   MAR crates/deeptime-core/src/surface/marine.rs
+ *   GRID crates/deeptime-core/src/surface/grid.rs
+ *   PROJ crates/deeptime-format/src/projection.rs
  *
  * Build: gcc -O2 -fopenmp -ffp-contract=off -fno-fast-math (oracle/Makefile).  With SSE2
  * arithmetic (x86-64) float and double operations round exactly once each, so the cell-by-cell
@@ -1140,4 +1142,95 @@ double or_rel_residual(size_t w, size_t h, double dx, int wrap_x, const double *
     free(g);
     free(y);
     return bb > 0.0 ? sqrt(rr / bb) : sqrt(rr);
+}
+
+/* ---- submarine slope failure (MAR:254-580) ---------------------------------------------------- */
+
+/* D8 offsets in the grid's fixed order E, NE, N, NW, W, SW, S, SE ("north" is row j - 1) and
+ * whether each is diagonal (GRID:21-31). */
+static const int D8_DI[8] = {1, 1, 0, -1, -1, -1, 0, 1};
+static const int D8_DJ[8] = {0, -1, -1, -1, 0, 1, 1, 1};
+static const int D8_DIAG[8] = {0, 1, 0, 1, 0, 1, 0, 1};
+/* The double nearest sqrt 2 (0x3FF6A09E667F3BCD, Rust's std::f64::consts::SQRT_2). */
+static const double SQRT_2 = 1.4142135623730951;
+
+/* `Dims::neighbour` (GRID:46-60) by the projection's rule (PROJ:350-368): rows clamp, columns wrap
+ * when wrap_x (rem_euclid), and the cell itself is no neighbour.  -1 when there is none. */
+long or_d8_neighbour(size_t w, size_t h, int wrap_x, size_t c, int k) {
+    long i = (long)(c % w), j = (long)(c / w);
+    long nj = j + D8_DJ[k], ni = i + D8_DI[k];
+    if (nj < 0 || nj >= (long)h) return -1;
+    if (ni < 0 || ni >= (long)w) {
+        if (!wrap_x) return -1;
+        ni = ((ni % (long)w) + (long)w) % (long)w;
+    }
+    long m = nj * (long)w + ni;
+    return m == (long)c ? -1 : m;
+}
+
+/* The slope failure as a full sweep of every cell every iteration (the test's
+ * `slope_failure_reference`, MAR:595-655, which the active-set path of MAR:292-580 equals bit for
+ * bit).  Per iteration: each open-sea cell's share min(avail / give, 1) of what it would give,
+ * give = sum over wet neighbours of rate * max(h_c - h_m - S_c d_k, 0); then each cell's change
+ * sum over wet neighbours of rate * (in * share_m - out * share_c), neighbours in D8 order; the
+ * movers (change != 0) in ascending cell order take h += dz, avail = max(avail + dz, 0), and
+ * moved += |dz| / 2.  The loop stops after an iteration that moves nothing.  S_c d_k is
+ * critical_slope * dx, or critical_slope * (dx * sqrt 2) for a diagonal (`step_len`,
+ * GRID:65-71).  Returns the metres moved. */
+double or_slope_failure(size_t w, size_t h, double dx, int wrap_x, const unsigned char *sea,
+                        double sea_level, double critical_slope, double rate, unsigned iterations,
+                        double *hgt, double *avail) {
+    size_t n = w * h;
+    if (n == 0) return 0.0;
+    double crit[8];
+    for (int k = 0; k < 8; k++) crit[k] = critical_slope * (D8_DIAG[k] ? dx * SQRT_2 : dx);
+    double *scale = (double *)malloc(n * sizeof(double));
+    double *dz = (double *)malloc(n * sizeof(double));
+    unsigned char *wet = (unsigned char *)malloc(n);
+    double moved = 0.0;
+    for (unsigned it = 0; it < iterations; it++) {
+#pragma omp parallel for schedule(static)
+        for (long cl = 0; cl < (long)n; cl++) wet[cl] = sea[cl] && hgt[cl] < sea_level;
+#pragma omp parallel for schedule(static)
+        for (long cl = 0; cl < (long)n; cl++) {
+            size_t c = (size_t)cl;
+            double sc = 0.0;
+            if (wet[c] && avail[c] > 0.0) {
+                double give = 0.0;
+                for (int k = 0; k < 8; k++) {
+                    long m = or_d8_neighbour(w, h, wrap_x, c, k);
+                    if (m >= 0 && wet[m]) give += rate * fmax(hgt[c] - hgt[m] - crit[k], 0.0);
+                }
+                if (give > 0.0) sc = fmin(avail[c] / give, 1.0);
+            }
+            scale[c] = sc;
+        }
+#pragma omp parallel for schedule(static)
+        for (long cl = 0; cl < (long)n; cl++) {
+            size_t c = (size_t)cl;
+            double d = 0.0;
+            if (wet[c])
+                for (int k = 0; k < 8; k++) {
+                    long m = or_d8_neighbour(w, h, wrap_x, c, k);
+                    if (m < 0 || !wet[m]) continue;
+                    double out = fmax(hgt[c] - hgt[m] - crit[k], 0.0);
+                    double inn = fmax(hgt[m] - hgt[c] - crit[k], 0.0);
+                    d += rate * (inn * scale[m] - out * scale[c]);
+                }
+            dz[c] = d;
+        }
+        int any = 0;
+        for (size_t c = 0; c < n; c++)
+            if (dz[c] != 0.0) {
+                any = 1;
+                hgt[c] += dz[c];
+                avail[c] = fmax(avail[c] + dz[c], 0.0);
+                moved += fabs(dz[c]) * 0.5;
+            }
+        if (!any) break;
+    }
+    free(scale);
+    free(dz);
+    free(wet);
+    return moved;
 }

@@ -84,6 +84,12 @@ double or_rel_residual(size_t w, size_t h, double dx, int wrap_x, const double *
                        const double *b, const double *x);
 int or_threads(void);
 
+/* ---- submarine slope failure (MAR:254-655) ---------------------------------------------------- */
+long or_d8_neighbour(size_t w, size_t h, int wrap_x, size_t c, int k);
+double or_slope_failure(size_t w, size_t h, double dx, int wrap_x, const unsigned char *sea,
+                        double sea_level, double critical_slope, double rate, unsigned iterations,
+                        double *hgt, double *avail);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,7 @@ calls it. Each wrapper names the reference function it restates; dtoracle.c cite
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import subprocess
 
@@ -87,6 +88,13 @@ def lib():
     L.or_step_in.restype = _sz
     L.or_spread_diffusivity.argtypes = [ctypes.c_double] * 3
     L.or_spread_diffusivity.restype = ctypes.c_double
+    _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+    L.or_d8_neighbour.argtypes = [_sz, _sz, ctypes.c_int, _sz, ctypes.c_int]
+    L.or_d8_neighbour.restype = ctypes.c_long
+    L.or_slope_failure.argtypes = [_sz, _sz, ctypes.c_double, ctypes.c_int, _u8p,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_uint, _f64p, _f64p]
+    L.or_slope_failure.restype = ctypes.c_double
     _lib = L
     return L
 
@@ -292,3 +300,78 @@ def rel_residual(w, h, dx, wrap, kd, dt, b, x):
 
 def threads() -> int:
     return lib().or_threads()
+
+
+# ---- submarine slope failure -----------------------------------------------------------------
+
+FAILURE_RATE = 0.05  # surface/marine.rs:231
+FAILURE_ITERATIONS = 40  # surface/marine.rs:232
+# The reference tests' S_c, `2.5f64.to_radians().tan()` (marine.rs:721): libm's tan, as math's.
+CRITICAL_SLOPE = math.tan(2.5 * (math.pi / 180.0))
+
+
+def d8_neighbour(w, h, wrap, c, k):
+    """`Dims::neighbour`, surface/grid.rs:46-60 (-1 for none)."""
+    return lib().or_d8_neighbour(w, h, int(wrap), c, k)
+
+
+def slope_failure(w, h, dx, wrap, sea, sea_level, critical_slope, rate, iterations, hgt, avail):
+    """The full-sweep slope failure (`slope_failure_reference`, surface/marine.rs:595-655; the
+    active-set `slope_failure`, marine.rs:292-580, equals it bit for bit).  Returns (h, avail,
+    metres moved); the inputs are not changed."""
+    x, a = _c(hgt).copy(), _c(avail).copy()
+    m = lib().or_slope_failure(w, h, dx, int(wrap), _c(sea, np.uint8), sea_level,
+                               critical_slope, rate, iterations, x, a)
+    return x, a, m
+
+
+def splitmix_below(rng, n):
+    """`SplitMix64::below` (sphere/mod.rs:66-69): Lemire's multiply-shift into [0, n)."""
+    return (rng.next_u64() * n) >> 64
+
+
+_GAMMA = 0x9E3779B97F4A7C15
+
+
+def _splitmix_f64_block(seed, start, n):
+    """Draws start+1 .. start+n of SplitMix64(seed).next_f64 (sphere/mod.rs:52-63), vectorized:
+    the state after k draws is seed + k * gamma."""
+    k = np.arange(start + 1, start + n + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) + k * np.uint64(_GAMMA)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 2.0**53)
+
+
+def failure_world(w, hh, seed, mounds):
+    """`failure_world` (surface/marine.rs:661-716): a coast, a sea floor, and either sediment
+    everywhere (mounds = 0) or `mounds` sediment mounds on a crystalline floor.  The draws are the
+    reference's in its order (one a cell for the heights, then the sediment).  Returns
+    (h, avail, sea as uint8)."""
+    n = w * hh
+    f = _splitmix_f64_block(seed, 0, n)
+    i = np.arange(n) % w
+    h = np.where(i < w // 6, 5.0 + 50.0 * f,
+                 np.where(i < w // 6 + 2, (f - 0.5) * 0.02, -3000.0 + 400.0 * f))
+    avail = np.zeros(n)
+    if mounds == 0:
+        avail = 2000.0 * _splitmix_f64_block(seed, n, n)
+        neg = np.flatnonzero(h < 0.0)
+        h[neg] += 1500.0 * _splitmix_f64_block(seed, 2 * n, neg.size)
+    else:
+        rng = SplitMix64((seed + n * _GAMMA) & 0xFFFFFFFFFFFFFFFF)
+        for _ in range(mounds):
+            ci, cj = splitmix_below(rng, w), splitmix_below(rng, hh)
+            r = 1 + splitmix_below(rng, 4)
+            for dj in range(2 * r + 1):
+                for di in range(2 * r + 1):
+                    ii, j = (ci + di + r * (w - 1)) % w, cj + dj
+                    if j < r or j - r >= hh:
+                        continue
+                    c = (j - r) * w + ii
+                    if h[c] < -10.0:
+                        h[c] = -100.0 - 30.0 * rng.next_f64()
+                        avail[c] = 3000.0 * rng.next_f64()
+    return h, avail, (h < 0.0).astype(np.uint8)
