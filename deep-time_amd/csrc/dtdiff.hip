@@ -817,7 +817,7 @@ int dtd_slope_failure_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wr
         if (iterations) *iterations = 0;
         check_failure(w, h, p, sea_dev, h_dev, avail_dev);
         if (w * h == 0 || p->iterations == 0) return;
-        FailWs &f = fit_fail(ctx, w * h, false);
+        FailWs &f = fit_fail(ctx, w, h, false);
         const double m = slope_failure(ctx, f, w, h, dx, wrap_x, sea_dev, sea_level,
                                        p->critical_slope, p->rate, p->iterations, h_dev,
                                        avail_dev, iterations);
@@ -835,7 +835,7 @@ int dtd_slope_failure(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x,
         check_failure(w, h, p, sea, h_inout, avail_inout);
         const size_t n = w * h;
         if (n == 0 || p->iterations == 0) return;
-        FailWs &f = fit_fail(ctx, n, true);
+        FailWs &f = fit_fail(ctx, w, h, true);
         upload(ctx, f.sd, sea, n);
         upload(ctx, f.hd, h_inout, n * sizeof(double));
         upload(ctx, f.ad, avail_inout, n * sizeof(double));

@@ -8,14 +8,14 @@
 //
 // The reference's CPU path (MAR:292-580) keeps an active set of the cells near the last
 // iteration's movers; it equals the full sweep over every cell (MAR:595-655) bit for bit.  The
-// device runs the full sweep: each iteration is a share pass and a change pass over every cell,
-// each cell's value formed by the reference's expression in its order (D8 neighbours in the fixed
-// order, dry or missing ones skipped), in f64, so the heights and the sediment are the
-// reference's bit for bit.  Only the metres moved, a diagnostic sum over the movers, is added in
-// another order (per block, then the blocks in order; the reference adds cell by cell), which
-// makes it equal to rounding.  An iteration that moves nothing ends the loop, as in the
-// reference: its step marks the state done and every later launch returns at once, so the host
-// launches all the iterations without waiting for any of them.
+// device runs the full sweep, one tiled pass an iteration (k_fail_sweep), each cell's value formed
+// by the reference's expression in its order (D8 neighbours in the fixed order, dry or missing
+// ones skipped), in f64, so the heights and the sediment are the reference's bit for bit.  Only
+// the metres moved, a diagnostic sum over the movers, is added in another order (per block, then
+// the blocks in order; the reference adds cell by cell), which makes it equal to rounding.  An
+// iteration that moves nothing ends the loop, as in the reference: its step marks the state done
+// and every later launch returns at once, so the host launches all the iterations without waiting
+// for any of them.
 #pragma once
 
 #include "context.hpp"
@@ -37,101 +37,135 @@ struct FGrid {
     double rate, sea_level;
 };
 
-// The neighbour of (i, j) at D8 offset k (E, NE, N, NW, W, SW, S, SE; "north" is row j - 1,
-// GRID:21-31) by the projection's rule (PROJ:350-368): rows clamp, columns wrap when the grid
-// does, and the cell itself is no neighbour (GRID:46-60).  -1 for none.
-__device__ __forceinline__ int d8_neighbour(const FGrid &g, int c, int i, int j, int k) {
-    constexpr int di[8] = {1, 1, 0, -1, -1, -1, 0, 1};
-    constexpr int dj[8] = {0, -1, -1, -1, 0, 1, 1, 1};
-    const int nj = j + dj[k];
-    if (nj < 0 || nj >= g.h) return -1;
-    int ni = i + di[k];
-    if (ni < 0 || ni >= g.w) {
-        if (!g.wrap) return -1;
-        ni = ni < 0 ? ni + g.w : ni - g.w;  // |di| = 1: one turn (a 1-wide grid: the cell)
-    }
-    const int m = nj * g.w + ni;
-    return m == c ? -1 : m;
-}
-
-__device__ __forceinline__ bool fail_wet(const FGrid &g, const unsigned char *sea,
-                                         const double *h, int c) {
-    return sea[c] && h[c] < g.sea_level;
-}
-
 __device__ __forceinline__ double fail_crit(const FGrid &g, int k) {
     return (k & 1) ? g.crit_diag : g.crit_side;
 }
 
-// Each cell's share of what it would give (the `share` closure, MAR:346-366):
-// min(avail / give, 1) with give = sum over wet neighbours of rate * max(h_c - h_m - S_c d, 0);
-// 0 for a dry cell, one without sediment, or one that gives nothing.
-__global__ __launch_bounds__(kBlock) void k_fail_share(FGrid g, const unsigned char *__restrict__ sea,
-                                                      const double *__restrict__ h,
-                                                      const double *__restrict__ avail,
-                                                      double *__restrict__ scale,
-                                                      const FailState *__restrict__ st) {
-    if (st->done) return;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= g.n) return;
-    const int j = c / g.w, i = c - j * g.w;
-    double s = 0.0;
-    const double hc = h[c];
-    if (sea[c] && hc < g.sea_level && avail[c] > 0.0) {
-        double give = 0.0;
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const int m = d8_neighbour(g, c, i, j, k);
-            if (m >= 0 && fail_wet(g, sea, h, m))
-                give += g.rate * fmax(hc - h[m] - fail_crit(g, k), 0.0);
-        }
-        if (give > 0.0) s = fmin(avail[c] / give, 1.0);
-    }
-    scale[c] = s;
-}
+// One iteration in one pass over 64 x 16 tiles.  A block loads the heights and the wet flags of
+// its tile with a 2-cell halo into LDS (each position with its cell's index, -1 where the grid has
+// none, wrapped columns where it wraps), forms the shares of the tile and a 1-cell ring (the
+// `share` closure, MAR:346-366: min(avail / give, 1), give = sum over wet neighbours of
+// rate * max(h_c - h_m - S_c d, 0)), then each tile cell's change (the `change` closure,
+// MAR:367-383: sum over wet neighbours of rate * (in * share_m - out * share_c)).  A ring cell's
+// share is formed again by every block that needs it, from the same inputs, so it has the same
+// bits.  Heights and sediment go from (hin, ain) to (hout, aout), every cell (the buffers
+// alternate, so no block reads a value another block of the same pass wrote); a mover
+// (dz != 0) takes h + dz and max(avail + dz, 0) (MAR:640-648).  parts gets the block's
+// (sum |dz| / 2, movers).
+constexpr int kFailTX = 64, kFailTY = 16;
+constexpr int kFailRX = kFailTX + 4, kFailRY = kFailTY + 4;  // heights: 2-cell halo
+constexpr int kFailSX = kFailTX + 2, kFailSY = kFailTY + 2;  // shares: 1-cell ring
 
-// Each cell's change (the `change` closure, MAR:367-383): sum over wet neighbours of
-// rate * (in * share_m - out * share_c); a mover (dz != 0) takes h += dz and
-// avail = max(avail + dz, 0) (MAR:640-648).  The heights go from hin to hout (every cell, so the
-// buffers alternate); parts gets the block's (sum |dz| / 2, movers).
-__global__ __launch_bounds__(kBlock) void k_fail_change(FGrid g,
-                                                       const unsigned char *__restrict__ sea,
-                                                       const double *__restrict__ hin,
-                                                       double *__restrict__ hout,
-                                                       double *__restrict__ avail,
-                                                       const double *__restrict__ scale,
-                                                       double *__restrict__ parts,
-                                                       const FailState *__restrict__ st) {
+__global__ __launch_bounds__(kBlock) void k_fail_sweep(FGrid g,
+                                                      const unsigned char *__restrict__ sea,
+                                                      const double *__restrict__ hin,
+                                                      double *__restrict__ hout,
+                                                      const double *__restrict__ ain,
+                                                      double *__restrict__ aout,
+                                                      double *__restrict__ parts,
+                                                      const FailState *__restrict__ st) {
+    __shared__ double hs[kFailRY][kFailRX];
+    __shared__ int gid[kFailRY][kFailRX];
+    __shared__ unsigned char wet[kFailRY][kFailRX];
+    __shared__ double sc[kFailSY][kFailSX];
+    __shared__ double as[kFailSY][kFailSX];  // sediment of the share region
     __shared__ double lds[kBlock / 64];
+    constexpr int di[8] = {1, 1, 0, -1, -1, -1, 0, 1};
+    constexpr int dj[8] = {0, -1, -1, -1, 0, 1, 1, 1};
     if (st->done) return;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    double half = 0.0, mover = 0.0;
-    if (c < g.n) {
-        const int j = c / g.w, i = c - j * g.w;
-        const double hc = hin[c];
-        double dz = 0.0;
-        if (sea[c] && hc < g.sea_level) {
-            const double sc = scale[c];
+    const int ntx = (g.w + kFailTX - 1) / kFailTX;
+    const int i0 = (blockIdx.x % ntx) * kFailTX, j0 = (blockIdx.x / ntx) * kFailTY;
+    // The region: rows clamp, columns wrap when the grid does (PROJ:350-368).  Every position's
+    // loads are issued before any is used.
+    constexpr int kPer = (kFailRY * kFailRX + kBlock - 1) / kBlock;
+    int ms[kPer];
+    double hv[kPer], av[kPer];
+    unsigned char sv[kPer];
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const int m = d8_neighbour(g, c, i, j, k);
-                if (m < 0 || !fail_wet(g, sea, hin, m)) continue;
-                const double hm = hin[m], ck = fail_crit(g, k);
-                const double out = fmax(hc - hm - ck, 0.0);
-                const double inn = fmax(hm - hc - ck, 0.0);
-                dz += g.rate * (inn * scale[m] - out * sc);
+    for (int u = 0; u < kPer; u++) {
+        const int q = threadIdx.x + u * kBlock;
+        const int ly = q / kFailRX, lx = q - ly * kFailRX;
+        const int gj = j0 + ly - 2;
+        int gi = i0 + lx - 2, m = -1;
+        if (q < kFailRY * kFailRX && gj >= 0 && gj < g.h) {
+            if (gi >= 0 && gi < g.w) {
+                m = gj * g.w + gi;
+            } else if (g.wrap) {
+                gi %= g.w;
+                if (gi < 0) gi += g.w;
+                m = gj * g.w + gi;
             }
         }
+        ms[u] = m;
+        const bool ring = ly >= 1 && ly <= kFailSY && lx >= 1 && lx <= kFailSX;
+        hv[u] = m >= 0 ? hin[m] : 0.0;
+        sv[u] = m >= 0 ? sea[m] : 0;
+        av[u] = m >= 0 && ring ? ain[m] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+        const int q = threadIdx.x + u * kBlock;
+        if (q >= kFailRY * kFailRX) break;
+        const int ly = q / kFailRX, lx = q - ly * kFailRX;
+        gid[ly][lx] = ms[u], hs[ly][lx] = hv[u];
+        wet[ly][lx] = ms[u] >= 0 && sv[u] && hv[u] < g.sea_level;
+        if (ly >= 1 && ly <= kFailSY && lx >= 1 && lx <= kFailSX) as[ly - 1][lx - 1] = av[u];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < kFailSY * kFailSX; q += kBlock) {
+        const int sy = q / kFailSX, sx = q - sy * kFailSX;
+        const int ly = sy + 1, lx = sx + 1, m = gid[ly][lx];
+        double s = 0.0;
+        if (m >= 0 && wet[ly][lx]) {
+            const double av = as[sy][sx];
+            if (av > 0.0) {
+                const double hc = hs[ly][lx];
+                double give = 0.0;
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int ny = ly + dj[k], nx = lx + di[k], mm = gid[ny][nx];
+                    // A neighbour exists when the grid has it and it is not the cell itself
+                    // (GRID:46-60).
+                    if (mm >= 0 && mm != m && wet[ny][nx])
+                        give += g.rate * fmax(hc - hs[ny][nx] - fail_crit(g, k), 0.0);
+                }
+                if (give > 0.0) s = fmin(av / give, 1.0);
+            }
+        }
+        sc[sy][sx] = s;
+    }
+    __syncthreads();
+    double half = 0.0, movers = 0.0;
+    for (int q = threadIdx.x; q < kFailTX * kFailTY; q += kBlock) {
+        const int ty = q / kFailTX, tx = q - ty * kFailTX;
+        if (i0 + tx >= g.w || j0 + ty >= g.h) continue;
+        const int ly = ty + 2, lx = tx + 2, c = gid[ly][lx];
+        const double hc = hs[ly][lx];
+        double dz = 0.0;
+        if (wet[ly][lx]) {
+            const double s0 = sc[ly - 1][lx - 1];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int ny = ly + dj[k], nx = lx + di[k], mm = gid[ny][nx];
+                if (mm < 0 || mm == c || !wet[ny][nx]) continue;
+                const double hm = hs[ny][nx], ck = fail_crit(g, k);
+                const double out = fmax(hc - hm - ck, 0.0);
+                const double inn = fmax(hm - hc - ck, 0.0);
+                dz += g.rate * (inn * sc[ny - 1][nx - 1] - out * s0);
+            }
+        }
+        const double av = as[ly - 1][lx - 1];
         if (dz != 0.0) {
             hout[c] = hc + dz;
-            avail[c] = fmax(avail[c] + dz, 0.0);
-            half = fabs(dz) * 0.5;
-            mover = 1.0;
+            aout[c] = fmax(av + dz, 0.0);
+            half += fabs(dz) * 0.5;
+            movers += 1.0;
         } else {
             hout[c] = hc;
+            aout[c] = av;
         }
     }
-    block_pair(make_double2(half, mover), parts, blockIdx.x, lds);
+    block_pair(make_double2(half, movers), parts, blockIdx.x, lds);
 }
 
 // The iteration's sums (the blocks in order): the metres moved, and the stop when nothing moved
@@ -153,20 +187,24 @@ __global__ void k_fail_init(FailState *st) {
     if (threadIdx.x == 0) *st = FailState{0.0, 0, 0, 0, 0};
 }
 
-// The heights back into the caller's array when the last iteration left them in the scratch.
-__global__ __launch_bounds__(kBlock) void k_fail_final(int n, const double *__restrict__ scratch,
+// The heights and the sediment back into the caller's arrays when the last iteration left them
+// in the scratch copies.
+__global__ __launch_bounds__(kBlock) void k_fail_final(int n, const double *__restrict__ h2,
+                                                      const double *__restrict__ a2,
                                                       double *__restrict__ h,
+                                                      double *__restrict__ avail,
                                                       const FailState *__restrict__ st) {
     if (st->result == 0) return;
     const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c < n) h[c] = scratch[c];
+    if (c < n) h[c] = h2[c], avail[c] = a2[c];
 }
 
 // ---- host side -------------------------------------------------------------------------------
 
 struct FailWs {
     size_t cap = 0;  // cells
-    double *h2 = nullptr, *scale = nullptr, *parts = nullptr;
+    long tiles = 0;  // partial sums
+    double *h2 = nullptr, *a2 = nullptr, *parts = nullptr;
     FailState *st = nullptr;
     FailState *h_st = nullptr;  // pinned
     // Device copies of a host call's arrays (dtd_slope_failure), made on its first use.
@@ -178,7 +216,7 @@ void free_fail(dtd_ctx *ctx) {
     FailWs *f = ctx ? ctx->fail : nullptr;
     if (!f) return;
     (void)hipFree(f->h2);
-    (void)hipFree(f->scale);
+    (void)hipFree(f->a2);
     (void)hipFree(f->parts);
     (void)hipFree(f->st);
     (void)hipFree(f->hd);
@@ -189,15 +227,23 @@ void free_fail(dtd_ctx *ctx) {
     ctx->fail = nullptr;
 }
 
-static FailWs &fit_fail(dtd_ctx *ctx, size_t n, bool host) {
-    if (ctx->fail && ctx->fail->cap >= n && (!host || ctx->fail->hd)) return *ctx->fail;
+static long fail_tiles(size_t w, size_t h) {
+    return (long)((w + kFailTX - 1) / kFailTX) * (long)((h + kFailTY - 1) / kFailTY);
+}
+
+static FailWs &fit_fail(dtd_ctx *ctx, size_t w, size_t h, bool host) {
+    const size_t n = w * h;
+    if (ctx->fail && ctx->fail->cap >= n && ctx->fail->tiles >= fail_tiles(w, h) &&
+        (!host || ctx->fail->hd))
+        return *ctx->fail;
     free_fail(ctx);
     FailWs *f = new FailWs();
     ctx->fail = f;
     f->cap = n;
+    f->tiles = fail_tiles(w, h);
     HIPCHK(hipMalloc(&f->h2, n * sizeof(double)));
-    HIPCHK(hipMalloc(&f->scale, n * sizeof(double)));
-    HIPCHK(hipMalloc(&f->parts, 2 * (size_t)nblocks((long)n) * sizeof(double)));
+    HIPCHK(hipMalloc(&f->a2, n * sizeof(double)));
+    HIPCHK(hipMalloc(&f->parts, 2 * (size_t)fail_tiles(w, h) * sizeof(double)));
     HIPCHK(hipMalloc(&f->st, sizeof(FailState)));
     HIPCHK(hipHostMalloc((void **)&f->h_st, sizeof(FailState), hipHostMallocDefault));
     if (host) {
@@ -220,19 +266,18 @@ static double slope_failure(dtd_ctx *ctx, FailWs &f, size_t w, size_t h, double 
     const double sqrt2 = 1.4142135623730951;
     const FGrid g{(int)w, (int)h, wrap_x ? 1 : 0, (int)n, critical_slope * dx,
                   critical_slope * (dx * sqrt2), rate, sea_level};
-    const int nb = nblocks((long)n);
+    const int nt = (int)fail_tiles(w, h);
     k_fail_init<<<1, 64, 0, s>>>(f.st);
     LAUNCH_CHECK();
     for (unsigned it = 0; it < iterations; it++) {
-        double *hin = it % 2 == 0 ? hgt : f.h2, *hout = it % 2 == 0 ? f.h2 : hgt;
-        k_fail_share<<<nb, kBlock, 0, s>>>(g, sea, hin, avail, f.scale, f.st);
+        const bool even = it % 2 == 0;
+        k_fail_sweep<<<nt, kBlock, 0, s>>>(g, sea, even ? hgt : f.h2, even ? f.h2 : hgt,
+                                           even ? avail : f.a2, even ? f.a2 : avail, f.parts, f.st);
         LAUNCH_CHECK();
-        k_fail_change<<<nb, kBlock, 0, s>>>(g, sea, hin, hout, avail, f.scale, f.parts, f.st);
-        LAUNCH_CHECK();
-        k_fail_step<<<1, kReduceBlock, 0, s>>>(f.parts, nb, f.st);
+        k_fail_step<<<1, kReduceBlock, 0, s>>>(f.parts, nt, f.st);
         LAUNCH_CHECK();
     }
-    k_fail_final<<<nb, kBlock, 0, s>>>((int)n, f.h2, hgt, f.st);
+    k_fail_final<<<nblocks((long)n), kBlock, 0, s>>>((int)n, f.h2, f.a2, hgt, avail, f.st);
     LAUNCH_CHECK();
     HIPCHK(hipMemcpyAsync(f.h_st, f.st, sizeof(FailState), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
