@@ -60,6 +60,9 @@ def main():
                     help="bf16: the f32 cycle with its two finest levels' records stored in bf16")
     ap.add_argument("--guess", choices=["lod", "rhs"], default="lod")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
+    ap.add_argument("--stage", choices=["solve", "failure"], default="solve",
+                    help="failure: the submarine slope failure after the solve (8192x4096 world "
+                         "with sediment everywhere, 40 iterations)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="one solve, then only the per-kernel timings (for counter runs)")
     args = ap.parse_args()
@@ -74,6 +77,8 @@ def main():
     from dtdiff import fields as F
     from dtdiff.hipmem import DeviceArray, synchronize
 
+    if args.stage == "failure":
+        return bench_failure(args)
     gen, kw = F.WORKLOADS[args.workload]
     if gen is F.case_regional:
         return bench_batch(args, gen, dict(kw))
@@ -282,6 +287,83 @@ def bench_jacobi(args, gen, kw):
         "iterations": int(st.iterations),
         "ms_breakdown": {"setup": st.ms_setup, "guess": st.ms_guess, "cg": st.ms_cg,
                          "device_total": st.ms_total},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_failure(args):
+    """The submarine slope failure (`marine::slope_failure`) on resident arrays: the reference
+    tests' world generator (`failure_world`, sediment everywhere, so every iteration moves) at
+    8192x4096 (or --size), the stage's 40 iterations.  A step restores the inputs (a device copy,
+    outside the timed region) and runs the call, which returns once the device is done (host clock
+    around it).  Roofline: k_fail_sweep's compulsory bytes, h and avail in and out (32 B) and the
+    sea flag (1 B) a cell an iteration, over the whole call's time (its step kernels and launches
+    included, so a lower bound on the sweep kernel's own rate)."""
+    import numpy as np
+
+    import dtdiff
+    from dtdiff.hipmem import DeviceArray, synchronize
+
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+
+    w, h = (8192, 4096) if not args.size else (int(v) for v in args.size.split("x"))
+    t = time.time()
+    h0, a0, sea = O.failure_world(w, h, 7, 0)
+    log(f"failure world {w}x{h} generated in {time.time() - t:.1f} s")
+    d = dtdiff.Dims(w, h, 4800.0, True)
+    h0d, a0d, sd = (DeviceArray.from_numpy(v) for v in (h0, a0, sea))
+    hd, ad = DeviceArray(w * h), DeviceArray(w * h)
+    ws = dtdiff.Workspace(device=0)
+
+    def step():
+        hd.copy_from(h0d)
+        ad.copy_from(a0d)
+        synchronize()
+        t0 = time.perf_counter()
+        r = dtdiff.slope_failure_device(ws, d, sd, 0.0, hd, ad, O.CRITICAL_SLOPE)
+        return r, (time.perf_counter() - t0) * 1e3
+
+    for _ in range(args.warmup):
+        step()
+    total_ms, its = 0.0, 0
+    for _ in range(args.steps):
+        (moved, n), ms = step()
+        total_ms += ms
+        its += n
+    ms_step = total_ms / args.steps
+    sweep_bytes = 33.0 * w * h
+    achieved = sweep_bytes * its / (total_ms * 1e-3) / 1e9
+    log(f"timed {args.steps} calls: {ms_step:.3f} ms/call, {n} iterations, moved {moved:.6e}, "
+        f"{achieved:.0f} GB/s over the call")
+    cpu = None
+    if not args.no_cpu:
+        ws_, hs_ = 1024, 512
+        hc, ac, sc = O.failure_world(ws_, hs_, 7, 0)
+        t = time.perf_counter()
+        _, _, mo = O.slope_failure(ws_, hs_, 4800.0, True, sc, 0.0, O.CRITICAL_SLOPE,
+                                   O.FAILURE_RATE, O.FAILURE_ITERATIONS, hc, ac)
+        cpu_s = time.perf_counter() - t
+        cpu = {"value": ws_ * hs_ * O.FAILURE_ITERATIONS / cpu_s, "unit": "cell-iterations/s",
+               "cores": O.threads(), "kind": "port",
+               "sample": f"the same generator at {ws_}x{hs_}, 40 iterations of the oracle's full "
+                         f"sweep (OpenMP over cells): {cpu_s:.2f} s"}
+        log(f"cpu oracle sample {ws_}x{hs_}: {cpu_s:.2f} s")
+    out = {
+        "metric": f"cell-iterations/s of the submarine slope failure at {w}x{h}",
+        "value": w * h * its / (total_ms * 1e-3),
+        "unit": "cell-iterations/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"slope failure {w}x{h}, failure_world seed 7 with sediment "
+                               f"everywhere, S_c = tan 2.5 deg, rate 0.05, 40 iterations"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_fail_sweep (+ its one-block step, over the whole call): h and "
+                               "avail in and out, sea flag in: 33 B/cell an iteration"},
+        "cpu_baseline": cpu,
+        "iterations": int(n),
     }
     print(json.dumps(out), flush=True)
 

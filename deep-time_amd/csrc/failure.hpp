@@ -42,8 +42,8 @@ __device__ __forceinline__ double fail_crit(const FGrid &g, int k) {
 }
 
 // One iteration in one pass over 64 x 16 tiles.  A block loads the heights and the wet flags of
-// its tile with a 2-cell halo into LDS (each position with its cell's index, -1 where the grid has
-// none, wrapped columns where it wraps), forms the shares of the tile and a 1-cell ring (the
+// its tile with a 2-cell halo into LDS (a position where the grid has no cell is dry; columns wrap
+// where the grid wraps), forms the shares of the tile and a 1-cell ring (the
 // `share` closure, MAR:346-366: min(avail / give, 1), give = sum over wet neighbours of
 // rate * max(h_c - h_m - S_c d, 0)), then each tile cell's change (the `change` closure,
 // MAR:367-383: sum over wet neighbours of rate * (in * share_m - out * share_c)).  A ring cell's
@@ -65,8 +65,7 @@ __global__ __launch_bounds__(kBlock) void k_fail_sweep(FGrid g,
                                                       double *__restrict__ parts,
                                                       const FailState *__restrict__ st) {
     __shared__ double hs[kFailRY][kFailRX];
-    __shared__ int gid[kFailRY][kFailRX];
-    __shared__ unsigned char wet[kFailRY][kFailRX];
+    __shared__ unsigned char wet[kFailRY][kFailRX];  // 0 also where the grid has no cell
     __shared__ double sc[kFailSY][kFailSX];
     __shared__ double as[kFailSY][kFailSX];  // sediment of the share region
     __shared__ double lds[kBlock / 64];
@@ -107,26 +106,28 @@ __global__ __launch_bounds__(kBlock) void k_fail_sweep(FGrid g,
         const int q = threadIdx.x + u * kBlock;
         if (q >= kFailRY * kFailRX) break;
         const int ly = q / kFailRX, lx = q - ly * kFailRX;
-        gid[ly][lx] = ms[u], hs[ly][lx] = hv[u];
+        hs[ly][lx] = hv[u];
         wet[ly][lx] = ms[u] >= 0 && sv[u] && hv[u] < g.sea_level;
         if (ly >= 1 && ly <= kFailSY && lx >= 1 && lx <= kFailSX) as[ly - 1][lx - 1] = av[u];
     }
     __syncthreads();
+    // A neighbour exists when the grid has it and it is not the cell itself (GRID:46-60): only on
+    // a wrapping grid one cell wide are the east and west neighbours the cell itself.
+    const bool self_ew = g.wrap && g.w == 1;
     for (int q = threadIdx.x; q < kFailSY * kFailSX; q += kBlock) {
         const int sy = q / kFailSX, sx = q - sy * kFailSX;
-        const int ly = sy + 1, lx = sx + 1, m = gid[ly][lx];
+        const int ly = sy + 1, lx = sx + 1;
         double s = 0.0;
-        if (m >= 0 && wet[ly][lx]) {
+        if (wet[ly][lx]) {
             const double av = as[sy][sx];
             if (av > 0.0) {
                 const double hc = hs[ly][lx];
                 double give = 0.0;
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
-                    const int ny = ly + dj[k], nx = lx + di[k], mm = gid[ny][nx];
-                    // A neighbour exists when the grid has it and it is not the cell itself
-                    // (GRID:46-60).
-                    if (mm >= 0 && mm != m && wet[ny][nx])
+                    const int ny = ly + dj[k], nx = lx + di[k];
+                    if ((k & 3) == 0 && self_ew) continue;
+                    if (wet[ny][nx])
                         give += g.rate * fmax(hc - hs[ny][nx] - fail_crit(g, k), 0.0);
                 }
                 if (give > 0.0) s = fmin(av / give, 1.0);
@@ -139,15 +140,15 @@ __global__ __launch_bounds__(kBlock) void k_fail_sweep(FGrid g,
     for (int q = threadIdx.x; q < kFailTX * kFailTY; q += kBlock) {
         const int ty = q / kFailTX, tx = q - ty * kFailTX;
         if (i0 + tx >= g.w || j0 + ty >= g.h) continue;
-        const int ly = ty + 2, lx = tx + 2, c = gid[ly][lx];
+        const int ly = ty + 2, lx = tx + 2, c = (j0 + ty) * g.w + i0 + tx;
         const double hc = hs[ly][lx];
         double dz = 0.0;
         if (wet[ly][lx]) {
             const double s0 = sc[ly - 1][lx - 1];
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                const int ny = ly + dj[k], nx = lx + di[k], mm = gid[ny][nx];
-                if (mm < 0 || mm == c || !wet[ny][nx]) continue;
+                const int ny = ly + dj[k], nx = lx + di[k];
+                if (((k & 3) == 0 && self_ew) || !wet[ny][nx]) continue;
                 const double hm = hs[ny][nx], ck = fail_crit(g, k);
                 const double out = fmax(hc - hm - ck, 0.0);
                 const double inn = fmax(hm - hc - ck, 0.0);
