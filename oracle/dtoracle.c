@@ -9,6 +9,9 @@
  *   SM  crates/deeptime-core/src/sphere/mod.rs
  *   MAR crates/deeptime-core/src/surface/marine.rs
  *   GRID crates/deeptime-core/src/surface/grid.rs
+ *   CD  crates/deeptime-core/src/surface/cube/diffusion.rs
+ *   CM  crates/deeptime-core/src/surface/cube/multigrid.rs
+ *   CUBE crates/deeptime-core/src/surface/cube.rs
  *   PROJ crates/deeptime-format/src/projection.rs
  *
  * Build: gcc -O2 -fopenmp -ffp-contract=off -fno-fast-math (oracle/Makefile).  With SSE2
@@ -1233,4 +1236,507 @@ double or_slope_failure(size_t w, size_t h, double dx, int wrap_x, const unsigne
     free(dz);
     free(wet);
     return moved;
+}
+
+/* ---- cube-grid diffusion (CD, CM) ------------------------------------------------------------ */
+
+#define CUBE_CHUNK 4096          /* CUBE:48 */
+#define CUBE_NO_CELL 0xFFFFFFFFu /* sphere/cube.rs:65 */
+
+typedef struct {
+    size_t l;           /* cells taking part */
+    int all;            /* every cell takes part (cells[i] == i) */
+    uint32_t *cells;    /* ascending */
+    uint32_t *cols;     /* l x 8: each slot's neighbour as a position, CUBE_NO_CELL for none */
+    double *k, *mu, *s, *diag; /* l x 8 couplings; mass; sum of couplings; mu + s */
+} cube_op;
+
+static void cube_op_free(cube_op *op) {
+    free(op->cells), free(op->cols), free(op->k), free(op->mu), free(op->s), free(op->diag);
+}
+
+/* `couplings`, CD:94-119: a side pair's mean kd, a diagonal's least of its quad's four side
+ * means, times scale = dt / mean area and the grid's coupling. */
+static void cube_couplings(const uint32_t *slots, const double *lap, const double *kd,
+                           double scale, size_t c, double *k) {
+    const uint32_t *sl = slots + 8 * c;
+    for (int q = 0; q < 8; q++) {
+        k[q] = 0.0;
+        if (sl[q] == CUBE_NO_CELL) continue;
+        size_t m = sl[q];
+        double d;
+        if (q % 2 == 0) {
+            d = 0.5 * (kd[c] + kd[m]);
+        } else {
+            size_t a = sl[q - 1], b = sl[(q + 1) % 8];
+            d = fmin(fmin(fmin(0.5 * (kd[c] + kd[a]), 0.5 * (kd[a] + kd[m])), 0.5 * (kd[m] + kd[b])),
+                     0.5 * (kd[b] + kd[c]));
+        }
+        k[q] = scale * d * lap[8 * c + q];
+    }
+}
+
+/* `Workspace::build`, CD:148-224: every cell (all, or when every cell has a coupling) or the
+ * cells with a coupling, ascending, their neighbours as positions among them. */
+static void cube_build(size_t n6, const uint32_t *slots, const double *lap, const double *area,
+                       double mean_area, const double *kd, double dt, int all, cube_op *op) {
+    double scale = dt / mean_area;
+    double *full = (double *)malloc(8 * n6 * sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (long c = 0; c < (long)n6; c++) cube_couplings(slots, lap, kd, scale, (size_t)c, full + 8 * c);
+    int every = all;
+    if (!every) {
+        every = 1;
+        for (size_t c = 0; c < n6 && every; c++) {
+            int any = 0;
+            for (int q = 0; q < 8; q++) any |= full[8 * c + q] != 0.0;
+            every = any;
+        }
+    }
+    op->all = every;
+    uint32_t *pos = (uint32_t *)malloc(n6 * sizeof(uint32_t));
+    size_t l = 0;
+    for (size_t c = 0; c < n6; c++) {
+        int any = every;
+        for (int q = 0; q < 8 && !any; q++) any = full[8 * c + q] != 0.0;
+        pos[c] = any ? (uint32_t)l++ : CUBE_NO_CELL;
+    }
+    op->l = l;
+    op->cells = (uint32_t *)malloc((l ? l : 1) * sizeof(uint32_t));
+    op->cols = (uint32_t *)malloc((l ? l : 1) * 8 * sizeof(uint32_t));
+    op->k = (double *)malloc((l ? l : 1) * 8 * sizeof(double));
+    op->mu = (double *)malloc((l ? l : 1) * sizeof(double));
+    op->s = (double *)malloc((l ? l : 1) * sizeof(double));
+    op->diag = (double *)malloc((l ? l : 1) * sizeof(double));
+    for (size_t c = 0; c < n6; c++) {
+        if (pos[c] == CUBE_NO_CELL) continue;
+        size_t i = pos[c];
+        op->cells[i] = (uint32_t)c;
+        for (int q = 0; q < 8; q++) {
+            uint32_t m = slots[8 * c + q];
+            op->cols[8 * i + q] = m == CUBE_NO_CELL ? CUBE_NO_CELL : pos[m];
+            op->k[8 * i + q] = full[8 * c + q];
+        }
+        op->mu[i] = area[c] / mean_area;
+        double sum = -0.0; /* `k.iter().sum()` */
+        for (int q = 0; q < 8; q++) sum += op->k[8 * i + q];
+        op->s[i] = sum;
+        op->diag[i] = op->mu[i] + sum;
+    }
+    free(pos);
+    free(full);
+}
+
+/* `Op::row`, CD:79-91. */
+static double cube_row(const cube_op *op, size_t i, const double *p) {
+    double acc = op->diag[i] * p[i];
+    for (int q = 0; q < 8; q++) {
+        uint32_t j = op->cols[8 * i + q];
+        if (j != CUBE_NO_CELL) acc -= op->k[8 * i + q] * p[j];
+    }
+    return acc;
+}
+
+/* The cube's LOD first guess, CD:316-378: rows of each face (slots E and W), then columns (N, S),
+ * each (mu + K) y = mu b divided through by mu, the couplings across a face's edge taken at the
+ * old value of the cell beyond it. */
+static void cube_lod(size_t n, const uint32_t *slots, const cube_op *op, const double *b, double *x,
+                     double *tmp) {
+    size_t nn = n * n;
+#pragma omp parallel
+    {
+        double *lo = (double *)malloc(n * sizeof(double)), *di = (double *)malloc(n * sizeof(double));
+        double *up = (double *)malloc(n * sizeof(double));
+#pragma omp for schedule(static)
+        for (long row = 0; row < (long)(6 * n); row++) {
+            size_t c0 = (size_t)row * n;
+            double *xr = x + c0;
+            for (size_t i = 0; i < n; i++) {
+                size_t c = c0 + i;
+                double ke = op->k[8 * c + 0], kw = op->k[8 * c + 4], mu = op->mu[c];
+                di[i] = 1.0 + (ke + kw) / mu;
+                double v = b[c];
+                lo[i] = 0.0, up[i] = 0.0;
+                if (i > 0) lo[i] = -kw / mu;
+                else v += kw * b[slots[8 * c + 4]] / mu;
+                if (i + 1 < n) up[i] = -ke / mu;
+                else v += ke * b[slots[8 * c + 0]] / mu;
+                xr[i] = v;
+            }
+            or_thomas(n, lo, di, up, xr);
+        }
+#pragma omp for schedule(static)
+        for (long col = 0; col < (long)(6 * n); col++) {
+            size_t f = (size_t)col / n, i = (size_t)col % n;
+            double *tc = tmp + (size_t)col * n;
+            for (size_t j = 0; j < n; j++) {
+                size_t c = f * nn + j * n + i;
+                double kn = op->k[8 * c + 2], ks = op->k[8 * c + 6], mu = op->mu[c];
+                di[j] = 1.0 + (kn + ks) / mu;
+                double v = x[c];
+                lo[j] = 0.0, up[j] = 0.0;
+                if (j > 0) lo[j] = -kn / mu;
+                else v += kn * x[slots[8 * c + 2]] / mu;
+                if (j + 1 < n) up[j] = -ks / mu;
+                else v += ks * x[slots[8 * c + 6]] / mu;
+                tc[j] = v;
+            }
+            or_thomas(n, lo, di, up, tc);
+        }
+        free(lo), free(di), free(up);
+    }
+#pragma omp parallel for schedule(static)
+    for (long row = 0; row < (long)(6 * n); row++) {
+        size_t f = (size_t)row / n, j = (size_t)row % n;
+        for (size_t i = 0; i < n; i++) x[(size_t)row * n + i] = tmp[f * nn + i * n + j];
+    }
+}
+
+/* -- the cube's V-cycle (CM) -- */
+
+typedef struct {
+    size_t side, cells;
+    uint32_t *ptr, *col;         /* neighbour lists (CSR) */
+    uint32_t *src_ptr, *src;     /* per entry, the finer entries summed into it (level >= 1) */
+    uint32_t *kid_ptr, *kids;    /* per cell, its block of finer cells, ascending (level >= 1) */
+    uint32_t *parent;            /* per finer cell, its block's cell here (level >= 1) */
+    float *val, *mass, *diag, *r, *z, *t;
+} cube_level;
+
+static void cube_level_free(cube_level *L) {
+    free(L->ptr), free(L->col), free(L->src_ptr), free(L->src), free(L->kid_ptr), free(L->kids);
+    free(L->parent), free(L->val), free(L->mass), free(L->diag), free(L->r), free(L->z), free(L->t);
+}
+
+/* `finest`, CM:65-90: the slots in order, none skipped. */
+static void cube_finest(size_t n, const uint32_t *slots, cube_level *L) {
+    size_t cells = 6 * n * n;
+    memset(L, 0, sizeof *L);
+    L->side = n, L->cells = cells;
+    L->ptr = (uint32_t *)malloc((cells + 1) * sizeof(uint32_t));
+    L->col = (uint32_t *)malloc(8 * cells * sizeof(uint32_t));
+    uint32_t at = 0;
+    L->ptr[0] = 0;
+    for (size_t c = 0; c < cells; c++) {
+        for (int q = 0; q < 8; q++)
+            if (slots[8 * c + q] != CUBE_NO_CELL) L->col[at++] = slots[8 * c + q];
+        L->ptr[c + 1] = at;
+    }
+}
+
+/* `coarsen`, CM:93-165: 2 x 2 blocks of each face; a coarse cell's neighbours ascending, each
+ * with the finer entries between the two blocks in (kid, entry) order. */
+static void cube_coarsen(const cube_level *f, cube_level *C) {
+    size_t nf = f->side, m = (nf + 1) / 2, cells = 6 * m * m;
+    memset(C, 0, sizeof *C);
+    C->side = m, C->cells = cells;
+    C->parent = (uint32_t *)malloc(f->cells * sizeof(uint32_t));
+    for (size_t c = 0; c < f->cells; c++) {
+        size_t face = c / (nf * nf), rest = c % (nf * nf), i = rest % nf, j = rest / nf;
+        C->parent[c] = (uint32_t)(face * m * m + (j / 2) * m + i / 2);
+    }
+    C->kid_ptr = (uint32_t *)calloc(cells + 1, sizeof(uint32_t));
+    for (size_t c = 0; c < f->cells; c++) C->kid_ptr[C->parent[c] + 1]++;
+    for (size_t q = 0; q < cells; q++) C->kid_ptr[q + 1] += C->kid_ptr[q];
+    C->kids = (uint32_t *)malloc(f->cells * sizeof(uint32_t));
+    uint32_t *fill = (uint32_t *)malloc(cells * sizeof(uint32_t));
+    memcpy(fill, C->kid_ptr, cells * sizeof(uint32_t));
+    for (size_t c = 0; c < f->cells; c++) C->kids[fill[C->parent[c]]++] = (uint32_t)c;
+    free(fill);
+    size_t ne = f->ptr[f->cells];
+    C->ptr = (uint32_t *)malloc((cells + 1) * sizeof(uint32_t));
+    C->col = (uint32_t *)malloc((ne ? ne : 1) * sizeof(uint32_t));
+    C->src_ptr = (uint32_t *)malloc((ne + 1) * sizeof(uint32_t));
+    C->src = (uint32_t *)malloc((ne ? ne : 1) * sizeof(uint32_t));
+    uint32_t *pp = (uint32_t *)malloc((ne ? ne : 1) * sizeof(uint32_t));
+    uint32_t *pe = (uint32_t *)malloc((ne ? ne : 1) * sizeof(uint32_t));
+    size_t ncol = 0, nsrc = 0;
+    C->ptr[0] = 0, C->src_ptr[0] = 0;
+    for (size_t r = 0; r < cells; r++) {
+        size_t np = 0;
+        for (uint32_t q = C->kid_ptr[r]; q < C->kid_ptr[r + 1]; q++) {
+            uint32_t c = C->kids[q];
+            for (uint32_t e = f->ptr[c]; e < f->ptr[c + 1]; e++) {
+                uint32_t p = C->parent[f->col[e]];
+                if (p != r) pp[np] = p, pe[np] = e, np++;
+            }
+        }
+        /* Stable insertion sort by the coarse neighbour. */
+        for (size_t a = 1; a < np; a++) {
+            uint32_t kp = pp[a], ke = pe[a];
+            size_t b = a;
+            while (b > 0 && pp[b - 1] > kp) pp[b] = pp[b - 1], pe[b] = pe[b - 1], b--;
+            pp[b] = kp, pe[b] = ke;
+        }
+        for (size_t a = 0; a < np; a++) {
+            if (a == 0 || pp[a] != pp[a - 1]) {
+                if (a > 0) C->src_ptr[ncol] = (uint32_t)nsrc;
+                C->col[ncol++] = pp[a];
+            }
+            C->src[nsrc++] = pe[a];
+        }
+        if (np) C->src_ptr[ncol] = (uint32_t)nsrc;
+        C->ptr[r + 1] = (uint32_t)ncol;
+    }
+    free(pp), free(pe);
+}
+
+typedef struct {
+    size_t nl;
+    cube_level *lv;
+} cube_hier;
+
+/* `Hierarchy::new`, CM:170-186: coarsen until a level has at most COARSEST cells or side 1. */
+static void cube_hier_new(size_t n, const uint32_t *slots, cube_hier *h) {
+    h->lv = (cube_level *)malloc(40 * sizeof(cube_level));
+    cube_finest(n, slots, &h->lv[0]);
+    h->nl = 1;
+    while (!(h->lv[h->nl - 1].cells <= COARSEST || h->lv[h->nl - 1].side <= 1)) {
+        cube_coarsen(&h->lv[h->nl - 1], &h->lv[h->nl]);
+        h->nl++;
+    }
+}
+
+static void cube_hier_free(cube_hier *h) {
+    for (size_t l = 0; l < h->nl; l++) cube_level_free(&h->lv[l]);
+    free(h->lv);
+}
+
+/* `prepare`, CM:197-260: the finest level is the operator in f32 (couplings in slot order, mass,
+ * diagonal); a coarse entry is SCALE times the sum of its finer entries, a coarse mass the sum of
+ * its block's, its diagonal mass + its couplings. */
+static void cube_prepare(cube_hier *h, const uint32_t *slots, const cube_op *op) {
+    for (size_t l = 0; l < h->nl; l++) {
+        cube_level *L = &h->lv[l];
+        size_t ne = L->ptr[L->cells];
+        L->val = (float *)malloc((ne ? ne : 1) * sizeof(float));
+        L->mass = (float *)malloc(L->cells * sizeof(float));
+        L->diag = (float *)malloc(L->cells * sizeof(float));
+        L->r = (float *)calloc(L->cells, sizeof(float));
+        L->z = (float *)calloc(L->cells, sizeof(float));
+        L->t = (float *)calloc(L->cells, sizeof(float));
+    }
+    cube_level *L0 = &h->lv[0];
+    size_t at = 0;
+    for (size_t c = 0; c < L0->cells; c++)
+        for (int q = 0; q < 8; q++)
+            if (slots[8 * c + q] != CUBE_NO_CELL) L0->val[at++] = (float)op->k[8 * c + q];
+    for (size_t c = 0; c < L0->cells; c++) L0->mass[c] = (float)op->mu[c], L0->diag[c] = (float)op->diag[c];
+    for (size_t l = 1; l < h->nl; l++) {
+        const cube_level *F = &h->lv[l - 1];
+        cube_level *C = &h->lv[l];
+        for (size_t e = 0; e < C->ptr[C->cells]; e++) {
+            float sum = 0.0f;
+            for (uint32_t q = C->src_ptr[e]; q < C->src_ptr[e + 1]; q++) sum += F->val[C->src[q]];
+            C->val[e] = SCALE * sum;
+        }
+        for (size_t r = 0; r < C->cells; r++) {
+            float sm = 0.0f, k = 0.0f;
+            for (uint32_t q = C->kid_ptr[r]; q < C->kid_ptr[r + 1]; q++) sm += F->mass[C->kids[q]];
+            for (uint32_t e = C->ptr[r]; e < C->ptr[r + 1]; e++) k += C->val[e];
+            C->mass[r] = sm;
+            C->diag[r] = sm + k;
+        }
+    }
+}
+
+/* `sweep`, CM:285-310: z = OMEGA r / diag from zero, else z + OMEGA (r - A z) / diag. */
+static void cube_sweep(cube_level *L, int first) {
+    if (first) {
+        for (size_t i = 0; i < L->cells; i++) L->z[i] = OMEGA * L->r[i] / L->diag[i];
+        return;
+    }
+    for (size_t i = 0; i < L->cells; i++) {
+        float az = L->diag[i] * L->z[i];
+        for (uint32_t e = L->ptr[i]; e < L->ptr[i + 1]; e++) az -= L->val[e] * L->z[L->col[e]];
+        L->t[i] = L->z[i] + OMEGA * (L->r[i] - az) / L->diag[i];
+    }
+    float *tmp = L->z;
+    L->z = L->t, L->t = tmp;
+}
+
+/* `vcycle`, CM:312-363. */
+static void cube_vcycle(cube_level *lv, size_t nl) {
+    cube_level *L = &lv[0];
+    if (nl == 1) {
+        for (int k = 0; k < COARSE_SWEEPS; k++) cube_sweep(L, k == 0);
+        return;
+    }
+    for (int k = 0; k < NU; k++) cube_sweep(L, k == 0);
+    for (size_t i = 0; i < L->cells; i++) {
+        float az = L->diag[i] * L->z[i];
+        for (uint32_t e = L->ptr[i]; e < L->ptr[i + 1]; e++) az -= L->val[e] * L->z[L->col[e]];
+        L->t[i] = L->r[i] - az;
+    }
+    cube_level *C = &lv[1];
+    for (size_t r = 0; r < C->cells; r++) {
+        float sum = 0.0f;
+        for (uint32_t q = C->kid_ptr[r]; q < C->kid_ptr[r + 1]; q++) sum += L->t[C->kids[q]];
+        C->r[r] = sum;
+    }
+    cube_vcycle(lv + 1, nl - 1);
+    for (size_t c = 0; c < L->cells; c++) L->z[c] += C->z[C->parent[c]];
+    for (int k = 0; k < NU; k++) cube_sweep(L, 0);
+}
+
+/* `precondition`, CM:262-273. */
+static void cube_precondition(cube_hier *h, const double *r, double *z) {
+    cube_level *L = &h->lv[0];
+    for (size_t i = 0; i < L->cells; i++) L->r[i] = (float)r[i];
+    cube_vcycle(h->lv, h->nl);
+    for (size_t i = 0; i < L->cells; i++) z[i] = (double)L->z[i];
+}
+
+/* `ordered_sum`, CUBE:50-58: each chunk of CUBE_CHUNK summed in order, the chunks in order. */
+static double cube_dot(size_t l, const double *a, const double *b, const double *div) {
+    double total = -0.0;
+    for (size_t c0 = 0; c0 < l; c0 += CUBE_CHUNK) {
+        size_t c1 = c0 + CUBE_CHUNK < l ? c0 + CUBE_CHUNK : l;
+        double part = -0.0;
+        for (size_t i = c0; i < c1; i++) part += div ? a[i] * b[i] / div[i] : a[i] * b[i];
+        total += part;
+    }
+    return total;
+}
+
+/* `cg`, CD:380-500: CG from x, preconditioned by the diagonal or the V-cycle, to an RMS residual
+ * of tol_m over `cells` cells; two passes an iteration. */
+static size_t cube_cg(const cube_op *op, cube_hier *h, double *x, const double *rhs, size_t cells,
+                      double tol_m, int mg) {
+    size_t l = op->l;
+    double tol2 = tol_m * tol_m * (double)cells;
+    double *r = (double *)malloc(l * sizeof(double)), *po = (double *)calloc(l, sizeof(double));
+    double *pn = (double *)calloc(l, sizeof(double)), *ap = (double *)calloc(l, sizeof(double));
+    double *z = (double *)calloc(l, sizeof(double));
+    size_t nparts = (l + CUBE_CHUNK - 1) / CUBE_CHUNK;
+    double *parts = (double *)malloc(2 * (nparts ? nparts : 1) * sizeof(double));
+    for (size_t i = 0; i < l; i++) r[i] = rhs[i] - cube_row(op, i, x);
+    size_t its = 0;
+    double rr0 = cube_dot(l, r, r, NULL);
+    if (!(rr0 <= tol2)) {
+        double rz;
+        if (mg) {
+            cube_precondition(h, r, z);
+            rz = cube_dot(l, r, z, NULL);
+        } else {
+            rz = cube_dot(l, r, r, op->diag);
+        }
+        int have_beta = 0;
+        double beta = 0.0;
+        for (int it = 0; it < CG_MAX_ITER; it++) {
+#pragma omp parallel for schedule(static)
+            for (long q = 0; q < (long)nparts; q++) {
+                size_t c0 = (size_t)q * CUBE_CHUNK, c1 = c0 + CUBE_CHUNK < l ? c0 + CUBE_CHUNK : l;
+                double sum = 0.0;
+                for (size_t i = c0; i < c1; i++) {
+#define DIR(j) ((have_beta ? (mg ? z[j] : r[j] / op->diag[j]) + beta * po[j] \
+                           : (mg ? z[j] : r[j] / op->diag[j])))
+                    double pi = DIR(i);
+                    double acc = op->diag[i] * pi;
+                    for (int s = 0; s < 8; s++) {
+                        uint32_t j = op->cols[8 * i + s];
+                        if (j != CUBE_NO_CELL) acc -= op->k[8 * i + s] * DIR(j);
+                    }
+#undef DIR
+                    pn[i] = pi;
+                    ap[i] = acc;
+                    sum += pi * acc;
+                }
+                parts[q] = sum;
+            }
+            double pap = -0.0;
+            for (size_t q = 0; q < nparts; q++) pap += parts[q];
+            double *sw = po;
+            po = pn, pn = sw;
+            if (isnan(pap) || pap <= 0.0) break;
+            double alpha = rz / pap;
+#pragma omp parallel for schedule(static)
+            for (long q = 0; q < (long)nparts; q++) {
+                size_t c0 = (size_t)q * CUBE_CHUNK, c1 = c0 + CUBE_CHUNK < l ? c0 + CUBE_CHUNK : l;
+                double s0 = 0.0, s1 = 0.0;
+                for (size_t i = c0; i < c1; i++) {
+                    x[i] += alpha * po[i];
+                    double rv = r[i] - alpha * ap[i];
+                    r[i] = rv;
+                    s0 += rv * rv;
+                    s1 += rv * rv / op->diag[i];
+                }
+                parts[2 * q] = s0, parts[2 * q + 1] = s1;
+            }
+            double rr = -0.0, rzj = -0.0;
+            for (size_t q = 0; q < nparts; q++) rr += parts[2 * q];
+            for (size_t q = 0; q < nparts; q++) rzj += parts[2 * q + 1];
+            its++;
+            if (rr <= tol2) break;
+            double rz_new;
+            if (mg) {
+                cube_precondition(h, r, z);
+                rz_new = cube_dot(l, r, z, NULL);
+            } else {
+                rz_new = rzj;
+            }
+            beta = rz_new / rz;
+            have_beta = 1;
+            rz = rz_new;
+        }
+    }
+    free(r), free(po), free(pn), free(ap), free(z), free(parts);
+    return its;
+}
+
+/* `solve` / `step_in` / `run`, CD:227-314, on the grid's slots (6 n^2 x 8, CUBE_NO_CELL for none),
+ * couplings (6 n^2 x 8), areas and their mean: the operator, the first guess (every cell taking
+ * part), CG by the V-cycle (solver 0, tol_m) or the diagonal (solver 1, `step_in`: the cells with
+ * a coupling, CG_TOL_M), and the residual's sum put back in proportion to each cell's coupling. */
+size_t or_cube_solve(size_t n, const uint32_t *slots, const double *lap, const double *area,
+                     double mean_area, double *hgt, const double *kd, double dt, int solver,
+                     double tol_m) {
+    if (isnan(dt) || dt <= 0.0) return 0;
+    size_t n6 = 6 * n * n;
+    int mg = solver == 0;
+    cube_op op;
+    cube_build(n6, slots, lap, area, mean_area, kd, dt, mg, &op);
+    size_t l = op.l;
+    if (l == 0) {
+        cube_op_free(&op);
+        return 0;
+    }
+    double *rhs = (double *)malloc(l * sizeof(double)), *x = (double *)malloc(l * sizeof(double));
+    for (size_t i = 0; i < l; i++) {
+        double v = hgt[op.cells[i]];
+        rhs[i] = op.mu[i] * v;
+        x[i] = v;
+    }
+    if (op.all) {
+        double *b = (double *)malloc(l * sizeof(double)), *tmp = (double *)malloc(l * sizeof(double));
+        memcpy(b, x, l * sizeof(double));
+        cube_lod(n, slots, &op, b, x, tmp);
+        free(b), free(tmp);
+    }
+    cube_hier h = {0, NULL};
+    if (mg) {
+        cube_hier_new(n, slots, &h);
+        cube_prepare(&h, slots, &op);
+    }
+    size_t its = cube_cg(&op, &h, x, rhs, n6, mg ? tol_m : CG_TOL_M, mg);
+    /* The put-back, CD:286-301. */
+    double *ap = (double *)malloc(l * sizeof(double));
+    for (size_t i = 0; i < l; i++) ap[i] = cube_row(&op, i, x);
+    double lost = -0.0, weight = -0.0;
+    for (size_t c0 = 0; c0 < l; c0 += CUBE_CHUNK) {
+        size_t c1 = c0 + CUBE_CHUNK < l ? c0 + CUBE_CHUNK : l;
+        double a = -0.0, w = -0.0;
+        for (size_t i = c0; i < c1; i++) a += rhs[i] - ap[i];
+        for (size_t i = c0; i < c1; i++) w += op.s[i];
+        lost += a, weight += w;
+    }
+    if (weight > 0.0 && lost != 0.0) {
+        double kk = lost / weight;
+        for (size_t i = 0; i < l; i++) x[i] += kk * op.s[i] / op.mu[i];
+    }
+    for (size_t i = 0; i < l; i++) hgt[op.cells[i]] = x[i];
+    free(ap), free(rhs), free(x);
+    if (mg) cube_hier_free(&h);
+    cube_op_free(&op);
+    return its;
 }

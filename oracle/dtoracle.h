@@ -86,6 +86,9 @@ int or_threads(void);
 
 /* ---- submarine slope failure (MAR:254-655) ---------------------------------------------------- */
 long or_d8_neighbour(size_t w, size_t h, int wrap_x, size_t c, int k);
+size_t or_cube_solve(size_t n, const uint32_t *slots, const double *lap, const double *area,
+                     double mean_area, double *hgt, const double *kd, double dt, int solver,
+                     double tol_m);
 double or_slope_failure(size_t w, size_t h, double dx, int wrap_x, const unsigned char *sea,
                         double sea_level, double critical_slope, double rate, unsigned iterations,
                         double *hgt, double *avail);

@@ -95,6 +95,10 @@ def lib():
                                    ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_uint, _f64p, _f64p]
     L.or_slope_failure.restype = ctypes.c_double
+    _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+    L.or_cube_solve.argtypes = [_sz, _u32p, _f64p, _f64p, ctypes.c_double, _f64p, _f64p,
+                                ctypes.c_double, ctypes.c_int, ctypes.c_double]
+    L.or_cube_solve.restype = _sz
     _lib = L
     return L
 
@@ -375,3 +379,15 @@ def failure_world(w, hh, seed, mounds):
                         h[c] = -100.0 - 30.0 * rng.next_f64()
                         avail[c] = 3000.0 * rng.next_f64()
     return h, avail, (h < 0.0).astype(np.uint8)
+
+
+# ---- cube-grid diffusion ---------------------------------------------------------------------
+
+def cube_solve(g, hgt, kd, dt, solver, tol_m):
+    """`surface::cube::diffusion::solve` (cube/diffusion.rs:227-314) on a grid's slots, couplings
+    and areas (a `dtdiff.cubegrid.CubeGrid`): solver 0 multigrid to tol_m, 1 `step_in` (Jacobi,
+    the cells with a coupling, CG_TOL_M).  Returns (h1, iterations); hgt is not changed."""
+    x = _c(hgt).copy()
+    its = lib().or_cube_solve(g.n, np.ascontiguousarray(g.slots, dtype=np.uint32), _c(g.lap),
+                              _c(g.area), g.mean_area, x, _c(kd), dt, int(solver), tol_m)
+    return x, its
