@@ -1,0 +1,855 @@
+// The cube-grid diffusion solve (`surface::cube::diffusion::solve`) on the device: the operator on
+// the cube's cells, the face-wise first guess, conjugate gradients preconditioned by the diagonal
+// (`step_in`) or by the cube's V-cycle, and the residual's sum put back.  Included once, by
+// dtdiff.hip.
+//
+// Reference files (relative to the deep-time repository root):
+//   CD   crates/deeptime-core/src/surface/cube/diffusion.rs
+//   CM   crates/deeptime-core/src/surface/cube/multigrid.rs
+//   CUBE crates/deeptime-core/src/surface/cube.rs
+//
+// The grid comes from the caller as the reference's `CubeGrid` hands it to the solve: each cell's
+// eight neighbour slots (D8 order in the cell's face frame, NO_CELL across a cube corner), the
+// couplings of its Laplacian and the cells' areas with their mean.  dtd_cube_new uploads them and
+// builds the V-cycle's levels (their neighbour lists and the lists each coarse value sums) once;
+// a solve fills in the values.
+//
+// Every value is the reference's expression in its order, with FP contraction off, so each pass
+// gives the reference's bits.  The sums are the reference's too: `ordered_sum` and the CG passes
+// add each chunk of 4096 cells in cell order and then the chunks in order (CUBE:48-58, CD:441-470),
+// and here one block takes one chunk, its threads form the terms into LDS and one thread adds them
+// in order.  The level sums (a coarse value over its finer entries, a block's residuals) run over
+// fixed lists in the reference's order.  So the iterations and the answer are the reference's.
+#pragma once
+
+#include "context.hpp"
+#include "kernels.hpp"
+
+namespace dtd {
+
+constexpr uint32_t kNoCell = 0xFFFFFFFFu;  // sphere/cube.rs:65
+constexpr int kCubeChunk = 4096;           // CUBE:48
+// The raster cycle's constants, which the cube's cycle takes (CM:24;
+// surface/diffusion/multigrid.rs:45-56).
+constexpr int kCubeCoarsest = 64, kCubeNu = 2, kCubeCoarseSweeps = 16;
+
+// The loop's scalars on the device.
+struct CubeState {
+    double rz, pap, alpha, beta, tol2, lost, weight;
+    int have_beta, done, its, max_iter;
+};
+
+// ---- the operator (CD:94-224) ------------------------------------------------------------------
+
+// `couplings`, CD:94-119: a side pair's mean kd, a diagonal's least of its quad's four side means
+// (the two side cells exist: only diagonals go missing), times dt / mean area and the coupling.
+__global__ __launch_bounds__(kBlock) void kc_couplings(int n6, const uint32_t *__restrict__ slots,
+                                                      const double *__restrict__ lap,
+                                                      const double *__restrict__ kd, double scale,
+                                                      double *__restrict__ k,
+                                                      unsigned char *__restrict__ any) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n6) return;
+    const uint32_t *sl = slots + 8 * (size_t)c;
+    const double kc = kd[c];
+    bool nz = false;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        double v = 0.0;
+        const uint32_t m = sl[q];
+        if (m != kNoCell) {
+            double d;
+            if (q % 2 == 0) {
+                d = 0.5 * (kc + kd[m]);
+            } else {
+                const uint32_t a = sl[q - 1], b = sl[(q + 1) % 8];
+                const double ka = kd[a], km = kd[m], kb = kd[b];
+                d = fmin(fmin(fmin(0.5 * (kc + ka), 0.5 * (ka + km)), 0.5 * (km + kb)),
+                         0.5 * (kb + kc));
+            }
+            v = scale * d * lap[8 * (size_t)c + q];
+        }
+        k[8 * (size_t)c + q] = v;
+        nz |= v != 0.0;
+    }
+    any[c] = nz;
+}
+
+// The operator at each position i of the cells taking part (cells[i]; the identity when every cell
+// does): its couplings (gathered when a subset takes part), mu = a / mean area, s = the couplings'
+// sum in slot order from -0.0, diag = mu + s (CD:206-215); rhs = mu h and x = h (CD:268-279).
+__global__ __launch_bounds__(kBlock) void kc_opinit(int l, const uint32_t *__restrict__ cells,
+                                                   const double *__restrict__ kfull,
+                                                   double *__restrict__ kpos,
+                                                   const double *__restrict__ area, double mean,
+                                                   const double *__restrict__ h,
+                                                   double *__restrict__ mu, double *__restrict__ s,
+                                                   double *__restrict__ diag,
+                                                   double *__restrict__ rhs,
+                                                   double *__restrict__ x) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= l) return;
+    const uint32_t c = cells ? cells[i] : (uint32_t)i;
+    double sum = -0.0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const double v = kfull[8 * (size_t)c + q];
+        if (kpos != kfull) kpos[8 * (size_t)i + q] = v;
+        sum += v;
+    }
+    const double m = area[c] / mean;
+    mu[i] = m;
+    s[i] = sum;
+    diag[i] = m + sum;
+    const double v = h[c];
+    rhs[i] = m * v;
+    x[i] = v;
+}
+
+// `Op::row`, CD:79-91.
+__device__ __forceinline__ double cube_row(int i, const double *__restrict__ diag,
+                                           const double *__restrict__ k,
+                                           const uint32_t *__restrict__ cols,
+                                           const double *__restrict__ p) {
+    double acc = diag[i] * p[i];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const uint32_t j = cols[8 * (size_t)i + q];
+        if (j != kNoCell) acc -= k[8 * (size_t)i + q] * p[j];
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(kBlock) void kc_apply(int l, const double *__restrict__ diag,
+                                                  const double *__restrict__ k,
+                                                  const uint32_t *__restrict__ cols,
+                                                  const double *__restrict__ p,
+                                                  double *__restrict__ out,
+                                                  const double *__restrict__ sub) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= l) return;
+    const double a = cube_row(i, diag, k, cols, p);
+    out[i] = sub ? sub[i] - a : a;  // sub: r = b - A x (CD:386-390)
+}
+
+// ---- the face-wise first guess (CD:316-378) ----------------------------------------------------
+
+// `thomas` (D:1029-1046) on one line of n values at v[t * stride] (solved in place), the
+// tridiagonal coefficients formed by coef(t, lo, di, up) as the reference's arrays hold them; c is
+// the line's scratch of n values.
+template <class F>
+__device__ __forceinline__ void thomas_line(int n, double *v, int stride, double *c, F coef) {
+    double lo, di, up, up_prev;
+    coef(0, lo, di, up_prev);
+    double beta = di;
+    v[0] /= beta;
+    for (int t = 1; t < n; t++) {
+        coef(t, lo, di, up);
+        c[t] = up_prev / beta;
+        beta = di - lo * c[t];
+        v[(size_t)t * stride] = (v[(size_t)t * stride] - lo * v[(size_t)(t - 1) * stride]) / beta;
+        up_prev = up;
+    }
+    for (int t = n - 2; t >= 0; t--) {
+        const double next = v[(size_t)(t + 1) * stride];
+        v[(size_t)t * stride] -= c[t + 1] * next;
+    }
+}
+
+// The rows of each face (slots E = 0 and W = 4), one thread a row: x = the solved row of b, the
+// couplings across a face's edge taken at b's value beyond it.
+__global__ __launch_bounds__(kBlock) void kc_lod_rows(int n, const double *__restrict__ k,
+                                                     const double *__restrict__ mu,
+                                                     const uint32_t *__restrict__ slots,
+                                                     const double *__restrict__ b,
+                                                     double *__restrict__ x,
+                                                     double *__restrict__ scratch) {
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    if (row >= 6 * n) return;
+    const size_t c0 = (size_t)row * n;
+    for (int i = 0; i < n; i++) {
+        const size_t c = c0 + i;
+        const double ke = k[8 * c], kw = k[8 * c + 4], m = mu[c];
+        double v = b[c];
+        if (i == 0) v += kw * b[slots[8 * c + 4]] / m;
+        if (i + 1 == n) v += ke * b[slots[8 * c]] / m;
+        x[c] = v;
+    }
+    thomas_line(n, x + c0, 1, scratch + c0, [&](int i, double &lo, double &di, double &up) {
+        const size_t c = c0 + i;
+        const double ke = k[8 * c], kw = k[8 * c + 4], m = mu[c];
+        di = 1.0 + (ke + kw) / m;
+        lo = i > 0 ? -kw / m : 0.0;
+        up = i + 1 < n ? -ke / m : 0.0;
+    });
+}
+
+// The columns of each face (slots N = 2 and S = 6), one thread a column, from y (the rows' result)
+// into tmp column by column.
+__global__ __launch_bounds__(kBlock) void kc_lod_cols(int n, const double *__restrict__ k,
+                                                     const double *__restrict__ mu,
+                                                     const uint32_t *__restrict__ slots,
+                                                     const double *__restrict__ y,
+                                                     double *__restrict__ tmp,
+                                                     double *__restrict__ scratch) {
+    const int col = blockIdx.x * kBlock + threadIdx.x;
+    if (col >= 6 * n) return;
+    const size_t nn = (size_t)n * n, f = col / n, i = col % n;
+    double *tc = tmp + (size_t)col * n;
+    for (int j = 0; j < n; j++) {
+        const size_t c = f * nn + (size_t)j * n + i;
+        const double kn = k[8 * c + 2], ks = k[8 * c + 6], m = mu[c];
+        double v = y[c];
+        if (j == 0) v += kn * y[slots[8 * c + 2]] / m;
+        if (j + 1 == n) v += ks * y[slots[8 * c + 6]] / m;
+        tc[j] = v;
+    }
+    thomas_line(n, tc, 1, scratch + (size_t)col * n, [&](int j, double &lo, double &di, double &up) {
+        const size_t c = f * nn + (size_t)j * n + i;
+        const double kn = k[8 * c + 2], ks = k[8 * c + 6], m = mu[c];
+        di = 1.0 + (kn + ks) / m;
+        lo = j > 0 ? -kn / m : 0.0;
+        up = j + 1 < n ? -ks / m : 0.0;
+    });
+}
+
+// x[f, j, i] = tmp[f, i, j].
+__global__ __launch_bounds__(kBlock) void kc_lod_back(int n, const double *__restrict__ tmp,
+                                                     double *__restrict__ x) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    const int nn = n * n;
+    if (c >= 6 * nn) return;
+    const int f = c / nn, rest = c - f * nn, j = rest / n, i = rest - j * n;
+    x[c] = tmp[(size_t)f * nn + (size_t)i * n + j];
+}
+
+// ---- ordered sums: one block a chunk of kCubeChunk cells ---------------------------------------
+
+// Thread 0 adds the chunk's terms t[0 .. cnt) in order from `init` (the reference's chunk sums
+// start from +0.0 in the CG passes and from -0.0 in `ordered_sum`).
+__device__ __forceinline__ double chunk_sum(const double *t, int cnt, double init) {
+    double s = init;
+    for (int q = 0; q < cnt; q++) s += t[q];
+    return s;
+}
+
+// parts[chunk] = sum of a[i] * b[i] (/ div[i]) over the chunk, from -0.0 (`ordered_sum`, with
+// b32 read as f64 when b is null).
+__global__ __launch_bounds__(kBlock) void kc_dot(int l, const double *__restrict__ a,
+                                                const double *__restrict__ b,
+                                                const float *__restrict__ b32,
+                                                const double *__restrict__ div,
+                                                double *__restrict__ parts,
+                                                const CubeState *__restrict__ st) {
+    __shared__ double t[kCubeChunk];
+    if (st && st->done) return;
+    const int c0 = blockIdx.x * kCubeChunk, cnt = min(kCubeChunk, l - c0);
+    for (int q = threadIdx.x; q < cnt; q += kBlock) {
+        const int i = c0 + q;
+        const double bv = b ? b[i] : (double)b32[i];
+        t[q] = div ? a[i] * bv / div[i] : a[i] * bv;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) parts[blockIdx.x] = chunk_sum(t, cnt, -0.0);
+}
+
+// The direction pass (CD:411-440): p = z + beta p_old (z = r / diag for the diagonal), A p, and
+// the chunk's sum of p A p from +0.0.  Every neighbour's p is formed again by the same expression.
+__global__ __launch_bounds__(kBlock) void kc_dir(int l, int mg, const double *__restrict__ r,
+                                                const float *__restrict__ z32,
+                                                const double *__restrict__ diag,
+                                                const double *__restrict__ k,
+                                                const uint32_t *__restrict__ cols,
+                                                const double *__restrict__ pold,
+                                                double *__restrict__ pnew, double *__restrict__ ap,
+                                                double *__restrict__ parts,
+                                                const CubeState *__restrict__ st) {
+    __shared__ double t[kCubeChunk];
+    if (st->done) return;
+    const bool hb = st->have_beta;
+    const double beta = st->beta;
+    auto dir = [&](uint32_t j) {
+        const double zj = mg ? (double)z32[j] : r[j] / diag[j];
+        return hb ? zj + beta * pold[j] : zj;
+    };
+    const int c0 = blockIdx.x * kCubeChunk, cnt = min(kCubeChunk, l - c0);
+    for (int q = threadIdx.x; q < cnt; q += kBlock) {
+        const int i = c0 + q;
+        const double pi = dir(i);
+        double acc = diag[i] * pi;
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const uint32_t j = cols[8 * (size_t)i + s];
+            if (j != kNoCell) acc -= k[8 * (size_t)i + s] * dir(j);
+        }
+        pnew[i] = pi;
+        ap[i] = acc;
+        t[q] = pi * acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) parts[blockIdx.x] = chunk_sum(t, cnt, 0.0);
+}
+
+// The update pass (CD:449-470): x += alpha p, r -= alpha A p, and the chunk's sums of r r and
+// r r / diag from +0.0 (into parts[2 chunk], parts[2 chunk + 1]).
+__global__ __launch_bounds__(kBlock) void kc_update(int l, const double *__restrict__ p,
+                                                   const double *__restrict__ ap,
+                                                   const double *__restrict__ diag,
+                                                   double *__restrict__ x, double *__restrict__ r,
+                                                   double *__restrict__ parts,
+                                                   const CubeState *__restrict__ st) {
+    __shared__ double t[kCubeChunk];
+    if (st->done) return;
+    const double alpha = st->alpha;
+    const int c0 = blockIdx.x * kCubeChunk, cnt = min(kCubeChunk, l - c0);
+    constexpr int kPer = kCubeChunk / kBlock;
+    double rv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+        const int q = threadIdx.x + u * kBlock;
+        if (q < cnt) {
+            const int i = c0 + q;
+            x[i] += alpha * p[i];
+            rv[u] = r[i] - alpha * ap[i];
+            r[i] = rv[u];
+            t[q] = rv[u] * rv[u];
+        }
+    }
+    __syncthreads();
+    double s0 = 0.0;
+    if (threadIdx.x == 0) s0 = chunk_sum(t, cnt, 0.0);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+        const int q = threadIdx.x + u * kBlock;
+        if (q < cnt) t[q] = rv[u] * rv[u] / diag[c0 + q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) parts[2 * blockIdx.x] = s0, parts[2 * blockIdx.x + 1] = chunk_sum(t, cnt, 0.0);
+}
+
+// The scalar steps, one thread: the parts added in order from -0.0 (`iter().sum()`).
+enum { kCubeRR0 = 0, kCubeRZ0 = 1, kCubePAP = 2, kCubeRR = 3, kCubeRZ = 4, kCubePutback = 5 };
+
+template <int STEP>
+__global__ void kc_step(const double *__restrict__ parts, int np, CubeState *st, int mg,
+                        int *flag) {
+    if (threadIdx.x != 0) return;
+    if (STEP != kCubeRR0 && STEP != kCubePutback && st->done) {
+        if (flag) *reinterpret_cast<volatile int *>(flag) = st->done;
+        return;
+    }
+    const int stride = (STEP == kCubeRR || STEP == kCubePutback) ? 2 : 1;
+    double a = -0.0, b = -0.0;
+    for (int q = 0; q < np; q++) a += parts[stride * q];
+    if (stride == 2)
+        for (int q = 0; q < np; q++) b += parts[2 * q + 1];
+    if (STEP == kCubeRR0) {
+        // rr0 <= tol2 ends the solve before the loop (CD:392-394).
+        st->have_beta = 0, st->its = 0, st->done = a <= st->tol2 ? 1 : 0;
+    } else if (STEP == kCubeRZ0) {
+        st->rz = a;
+    } else if (STEP == kCubePAP) {
+        // The p.A p break (CD:443-446); alpha = rz / pap.
+        st->pap = a;
+        if (isnan(a) || a <= 0.0) {
+            st->done = 2;
+        } else {
+            st->alpha = st->rz / a;
+        }
+    } else if (STEP == kCubeRR) {
+        // iterations += 1; the r.r stop (CD:476-479); the diagonal's rz (CD:480-486).
+        st->its += 1;
+        if (a <= st->tol2) {
+            st->done = 1;
+        } else if (st->its >= st->max_iter) {
+            st->done = 4;
+        } else if (!mg) {
+            st->beta = b / st->rz, st->have_beta = 1, st->rz = b;
+        }
+    } else if (STEP == kCubeRZ) {
+        st->beta = a / st->rz, st->have_beta = 1, st->rz = a;
+    } else {
+        st->lost = a, st->weight = b;
+    }
+    if (flag) *reinterpret_cast<volatile int *>(flag) = st->done;
+}
+
+// The put-back's sums (CD:289-292): per chunk, sum (rhs - A x) and sum s, each from -0.0.
+__global__ __launch_bounds__(kBlock) void kc_putback_sums(int l, const double *__restrict__ rhs,
+                                                         const double *__restrict__ ax,
+                                                         const double *__restrict__ s,
+                                                         double *__restrict__ parts) {
+    __shared__ double t[kCubeChunk];
+    const int c0 = blockIdx.x * kCubeChunk, cnt = min(kCubeChunk, l - c0);
+    for (int q = threadIdx.x; q < cnt; q += kBlock) t[q] = rhs[c0 + q] - ax[c0 + q];
+    __syncthreads();
+    double a = 0.0;
+    if (threadIdx.x == 0) a = chunk_sum(t, cnt, -0.0);
+    __syncthreads();
+    for (int q = threadIdx.x; q < cnt; q += kBlock) t[q] = s[c0 + q];
+    __syncthreads();
+    if (threadIdx.x == 0) parts[2 * blockIdx.x] = a, parts[2 * blockIdx.x + 1] = chunk_sum(t, cnt, -0.0);
+}
+
+// x += (lost / weight) s / mu when weight > 0 and lost != 0 (CD:293-298), then h[cells[i]] = x.
+__global__ __launch_bounds__(kBlock) void kc_putback(int l, const uint32_t *__restrict__ cells,
+                                                    const double *__restrict__ s,
+                                                    const double *__restrict__ mu,
+                                                    const double *__restrict__ x,
+                                                    double *__restrict__ h,
+                                                    const CubeState *__restrict__ st) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= l) return;
+    double v = x[i];
+    if (st->weight > 0.0 && st->lost != 0.0) {
+        const double kk = st->lost / st->weight;
+        v += kk * s[i] / mu[i];
+    }
+    h[cells ? cells[i] : (uint32_t)i] = v;
+}
+
+__global__ void kc_init(CubeState *st, double tol2, int max_iter) {
+    if (threadIdx.x == 0) *st = CubeState{0, 0, 0, 0, tol2, 0, 0, 0, 0, 0, max_iter};
+}
+
+// ---- the cube's V-cycle (CM), f32 ------------------------------------------------------------
+
+struct CLvl {
+    int cells, ne;
+    const uint32_t *ptr, *col;           // neighbour lists
+    const uint32_t *src_ptr, *src;       // per entry, the finer entries (level >= 1)
+    const uint32_t *kid_ptr, *kids;      // per cell, its finer block, ascending (level >= 1)
+    const uint32_t *parent;              // per finer cell, its cell here (level >= 1)
+    float *val, *mass, *diag, *r, *z, *t;
+};
+
+// The finest level's values (CM:205-230): the couplings in slot order (the CSR order), the mass
+// and the f64 diagonal, in f32.
+__global__ __launch_bounds__(kBlock) void kc_mg_finest(CLvl L, const uint32_t *__restrict__ slots,
+                                                      const double *__restrict__ k,
+                                                      const double *__restrict__ mu,
+                                                      const double *__restrict__ diag) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= L.cells) return;
+    uint32_t at = L.ptr[c];
+    for (int q = 0; q < 8; q++)
+        if (slots[8 * (size_t)c + q] != kNoCell) L.val[at++] = (float)k[8 * (size_t)c + q];
+    L.mass[c] = (float)mu[c];
+    L.diag[c] = (float)diag[c];
+}
+
+// A coarse entry: scale times the sum of its finer entries (CM:236-243).
+__global__ __launch_bounds__(kBlock) void kc_mg_vals(CLvl F, CLvl C, float scale) {
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= C.ne) return;
+    float s = 0.0f;
+    for (uint32_t q = C.src_ptr[e]; q < C.src_ptr[e + 1]; q++) s += F.val[C.src[q]];
+    C.val[e] = scale * s;
+}
+
+// A coarse cell's mass (its block's, summed) and diagonal, mass + its couplings (CM:244-258).
+__global__ __launch_bounds__(kBlock) void kc_mg_diag(CLvl F, CLvl C) {
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= C.cells) return;
+    float s = 0.0f, k = 0.0f;
+    for (uint32_t q = C.kid_ptr[r]; q < C.kid_ptr[r + 1]; q++) s += F.mass[C.kids[q]];
+    for (uint32_t e = C.ptr[r]; e < C.ptr[r + 1]; e++) k += C.val[e];
+    C.mass[r] = s;
+    C.diag[r] = s + k;
+}
+
+// r32 = r (f64 -> f32, CM:263-268).
+__global__ __launch_bounds__(kBlock) void kc_mg_r32(int n, const double *__restrict__ r,
+                                                   float *__restrict__ r32,
+                                                   const CubeState *__restrict__ st) {
+    if (st->done) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) r32[i] = (float)r[i];
+}
+
+__device__ __forceinline__ float cube_az(const CLvl &L, const float *z, int i) {
+    float az = L.diag[i] * z[i];
+    for (uint32_t e = L.ptr[i]; e < L.ptr[i + 1]; e++) az -= L.val[e] * z[L.col[e]];
+    return az;
+}
+
+// `sweep`, CM:285-310: z = omega r / diag from zero (first), else zout = z + omega (r - A z) / diag.
+__global__ __launch_bounds__(kBlock) void kc_mg_sweep(CLvl L, const float *__restrict__ zin,
+                                                     float *__restrict__ zout, float omega,
+                                                     int first, const CubeState *__restrict__ st) {
+    if (st->done) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= L.cells) return;
+    if (first)
+        zout[i] = omega * L.r[i] / L.diag[i];
+    else
+        zout[i] = zin[i] + omega * (L.r[i] - cube_az(L, zin, i)) / L.diag[i];
+}
+
+// The residual r - A z of each cell (CM:328-341).
+__global__ __launch_bounds__(kBlock) void kc_mg_resid(CLvl L, const float *__restrict__ z,
+                                                     float *__restrict__ t,
+                                                     const CubeState *__restrict__ st) {
+    if (st->done) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < L.cells) t[i] = L.r[i] - cube_az(L, z, i);
+}
+
+// The coarse r: each block's residuals summed, its cells ascending (CM:343-351).
+__global__ __launch_bounds__(kBlock) void kc_mg_restrict(CLvl C, const float *__restrict__ t,
+                                                        const CubeState *__restrict__ st) {
+    if (st->done) return;
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= C.cells) return;
+    float s = 0.0f;
+    for (uint32_t q = C.kid_ptr[r]; q < C.kid_ptr[r + 1]; q++) s += t[C.kids[q]];
+    C.r[r] = s;
+}
+
+// z += the coarse correction of the cell's block (CM:353-358).
+__global__ __launch_bounds__(kBlock) void kc_mg_prolong(int cells, float *__restrict__ z,
+                                                       const float *__restrict__ zc,
+                                                       const uint32_t *__restrict__ parent,
+                                                       const CubeState *__restrict__ st) {
+    if (st->done) return;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c < cells) z[c] += zc[parent[c]];
+}
+
+// ---- host side -----------------------------------------------------------------------------
+
+}  // namespace dtd
+
+struct dtd_cube {
+    int n = 0, n6 = 0;
+    double mean_area = 0.0;
+    std::vector<uint32_t> slots_h;  // host copy (the subset's neighbour positions)
+    std::vector<void *> owned;
+    uint32_t *slots = nullptr;
+    double *lap = nullptr, *area = nullptr;
+    // The V-cycle's levels (patterns on the device; values filled per solve).
+    std::vector<dtd::CLvl> lv;
+    // Per-solve buffers (n6 cells).
+    double *kfull = nullptr, *kpos = nullptr, *mu = nullptr, *s = nullptr, *diag = nullptr;
+    double *rhs = nullptr, *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr;
+    double *ap = nullptr, *b = nullptr, *tmp = nullptr, *scratch = nullptr, *parts = nullptr;
+    uint32_t *cells = nullptr, *cols = nullptr;
+    unsigned char *any = nullptr;
+    dtd::CubeState *st = nullptr;
+    dtd::CubeState *h_st = nullptr;  // pinned
+    double *hd = nullptr, *kdd = nullptr;  // device copies for the host entry (first use)
+};
+
+namespace dtd {
+
+// One level's pattern on the host (CM:29-45, 65-165).
+struct CPat {
+    size_t side = 0, cells = 0;
+    std::vector<uint32_t> ptr, col, src_ptr, src, kid_ptr, kids, parent;
+};
+
+// `finest`, CM:65-90: the slots in order, the missing ones skipped.
+static CPat cube_finest(int n, const uint32_t *slots) {
+    CPat p;
+    p.side = n, p.cells = 6 * (size_t)n * n;
+    p.ptr.reserve(p.cells + 1);
+    p.ptr.push_back(0);
+    for (size_t c = 0; c < p.cells; c++) {
+        for (int q = 0; q < 8; q++)
+            if (slots[8 * c + q] != kNoCell) p.col.push_back(slots[8 * c + q]);
+        p.ptr.push_back((uint32_t)p.col.size());
+    }
+    return p;
+}
+
+// `coarsen`, CM:93-165: 2 x 2 blocks of each face; each coarse cell's neighbours ascending, each
+// with its finer entries in (kid, entry) order.
+static CPat cube_coarsen(const CPat &f) {
+    CPat c;
+    const size_t nf = f.side, m = (nf + 1) / 2;
+    c.side = m, c.cells = 6 * m * m;
+    c.parent.resize(f.cells);
+    for (size_t q = 0; q < f.cells; q++) {
+        const size_t face = q / (nf * nf), rest = q % (nf * nf), i = rest % nf, j = rest / nf;
+        c.parent[q] = (uint32_t)(face * m * m + (j / 2) * m + i / 2);
+    }
+    c.kid_ptr.assign(c.cells + 1, 0);
+    for (uint32_t p : c.parent) c.kid_ptr[p + 1]++;
+    for (size_t q = 0; q < c.cells; q++) c.kid_ptr[q + 1] += c.kid_ptr[q];
+    c.kids.resize(f.cells);
+    std::vector<uint32_t> fill(c.kid_ptr.begin(), c.kid_ptr.end() - 1);
+    for (size_t q = 0; q < f.cells; q++) c.kids[fill[c.parent[q]]++] = (uint32_t)q;
+    c.ptr.push_back(0);
+    c.src_ptr.push_back(0);
+    std::vector<std::pair<uint32_t, uint32_t>> pairs;
+    for (size_t r = 0; r < c.cells; r++) {
+        pairs.clear();
+        for (uint32_t q = c.kid_ptr[r]; q < c.kid_ptr[r + 1]; q++) {
+            const uint32_t k = c.kids[q];
+            for (uint32_t e = f.ptr[k]; e < f.ptr[k + 1]; e++) {
+                const uint32_t p = c.parent[f.col[e]];
+                if (p != r) pairs.emplace_back(p, e);
+            }
+        }
+        std::stable_sort(pairs.begin(), pairs.end(),
+                         [](const std::pair<uint32_t, uint32_t> &a,
+                            const std::pair<uint32_t, uint32_t> &b) { return a.first < b.first; });
+        for (size_t a = 0; a < pairs.size(); a++) {
+            if (a == 0 || pairs[a].first != pairs[a - 1].first) {
+                if (a > 0) c.src_ptr.push_back((uint32_t)c.src.size());
+                c.col.push_back(pairs[a].first);
+            }
+            c.src.push_back(pairs[a].second);
+        }
+        if (!pairs.empty()) c.src_ptr.push_back((uint32_t)c.src.size());
+        c.ptr.push_back((uint32_t)c.col.size());
+    }
+    return c;
+}
+
+static void free_cube(dtd_cube *g) {
+    if (!g) return;
+    for (void *p : g->owned) (void)hipFree(p);
+    if (g->h_st) (void)hipHostFree(g->h_st);
+    delete g;
+}
+
+// The grid on the device and its levels' patterns (`Hierarchy::new`, CM:170-186).
+static dtd_cube *cube_new(dtd_ctx *ctx, int n, const uint32_t *slots, const double *lap,
+                          const double *area, double mean_area) {
+    dtd_cube *g = new dtd_cube();
+    try {
+        g->n = n, g->n6 = 6 * n * n, g->mean_area = mean_area;
+        const size_t n6 = (size_t)g->n6;
+        g->slots_h.assign(slots, slots + 8 * n6);
+        auto own = [&](size_t bytes) {
+            void *p = nullptr;
+            HIPCHK(hipMalloc(&p, bytes ? bytes : 1));
+            g->owned.push_back(p);
+            return p;
+        };
+        auto put = [&](const void *src, size_t bytes) {
+            void *p = own(bytes);
+            HIPCHK(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+            return p;
+        };
+        g->slots = (uint32_t *)put(slots, 8 * n6 * 4);
+        g->lap = (double *)put(lap, 8 * n6 * 8);
+        g->area = (double *)put(area, n6 * 8);
+        std::vector<CPat> pats;
+        pats.push_back(cube_finest(n, slots));
+        while (!(pats.back().cells <= (size_t)kCubeCoarsest || pats.back().side <= 1))
+            pats.push_back(cube_coarsen(pats.back()));
+        for (const CPat &p : pats) {
+            CLvl L{};
+            L.cells = (int)p.cells, L.ne = (int)p.col.size();
+            L.ptr = (const uint32_t *)put(p.ptr.data(), p.ptr.size() * 4);
+            L.col = (const uint32_t *)put(p.col.data(), p.col.size() * 4);
+            if (!p.parent.empty()) {
+                L.src_ptr = (const uint32_t *)put(p.src_ptr.data(), p.src_ptr.size() * 4);
+                L.src = (const uint32_t *)put(p.src.data(), p.src.size() * 4);
+                L.kid_ptr = (const uint32_t *)put(p.kid_ptr.data(), p.kid_ptr.size() * 4);
+                L.kids = (const uint32_t *)put(p.kids.data(), p.kids.size() * 4);
+                L.parent = (const uint32_t *)put(p.parent.data(), p.parent.size() * 4);
+            }
+            L.val = (float *)own(p.col.size() * 4);
+            L.mass = (float *)own(p.cells * 4);
+            L.diag = (float *)own(p.cells * 4);
+            L.r = (float *)own(p.cells * 4);
+            L.z = (float *)own(p.cells * 4);
+            L.t = (float *)own(p.cells * 4);
+            g->lv.push_back(L);
+        }
+        for (double **v : {&g->kfull, &g->kpos})
+            *v = (double *)own(8 * n6 * 8);
+        for (double **v : {&g->mu, &g->s, &g->diag, &g->rhs, &g->x, &g->r, &g->p0, &g->p1, &g->ap,
+                           &g->b, &g->tmp, &g->scratch})
+            *v = (double *)own(n6 * 8);
+        g->parts = (double *)own(2 * ((n6 + kCubeChunk - 1) / kCubeChunk + 1) * 8);
+        g->cells = (uint32_t *)own(n6 * 4);
+        g->cols = (uint32_t *)own(8 * n6 * 4);
+        g->any = (unsigned char *)own(n6);
+        g->st = (CubeState *)own(sizeof(CubeState));
+        HIPCHK(hipHostMalloc((void **)&g->h_st, sizeof(CubeState), hipHostMallocDefault));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return g;
+    } catch (...) {
+        (void)hipStreamSynchronize(ctx->stream);
+        free_cube(g);
+        throw;
+    }
+}
+
+// One V-cycle of the cube's levels from level l (`vcycle`, CM:312-363); the levels' z buffers
+// alternate with their t buffers as the reference swaps them, so zs[l] tracks which holds z.
+static void cube_vcycle(dtd_ctx *ctx, dtd_cube *g, size_t l, std::vector<float *> &zs,
+                        std::vector<float *> &ts) {
+    hipStream_t s = ctx->stream;
+    const float omega = ctx->opts.omega;
+    CLvl L = g->lv[l];
+    L.z = zs[l], L.t = ts[l];
+    const int nb = nblocks(L.cells);
+    auto sweep = [&](bool first) {
+        kc_mg_sweep<<<nb, kBlock, 0, s>>>(L, zs[l], first ? zs[l] : ts[l], omega, first, g->st);
+        LAUNCH_CHECK();
+        if (!first) std::swap(zs[l], ts[l]);
+    };
+    if (l + 1 == g->lv.size()) {
+        for (int k = 0; k < kCubeCoarseSweeps; k++) sweep(k == 0);
+        return;
+    }
+    for (int k = 0; k < kCubeNu; k++) sweep(k == 0);
+    kc_mg_resid<<<nb, kBlock, 0, s>>>(L, zs[l], ts[l], g->st);
+    LAUNCH_CHECK();
+    const CLvl &C = g->lv[l + 1];
+    kc_mg_restrict<<<nblocks(C.cells), kBlock, 0, s>>>(C, ts[l], g->st);
+    LAUNCH_CHECK();
+    cube_vcycle(ctx, g, l + 1, zs, ts);
+    kc_mg_prolong<<<nb, kBlock, 0, s>>>(L.cells, zs[l], zs[l + 1], C.parent, g->st);
+    LAUNCH_CHECK();
+    for (int k = 0; k < kCubeNu; k++) sweep(false);
+}
+
+// z = V-cycle(r) into the finest level's z (zs[0] after the call).
+static const float *cube_precondition(dtd_ctx *ctx, dtd_cube *g, const double *r) {
+    std::vector<float *> zs, ts;
+    for (const CLvl &L : g->lv) zs.push_back(L.z), ts.push_back(L.t);
+    kc_mg_r32<<<nblocks(g->lv[0].cells), kBlock, 0, ctx->stream>>>(g->lv[0].cells, r, g->lv[0].r,
+                                                                  g->st);
+    LAUNCH_CHECK();
+    cube_vcycle(ctx, g, 0, zs, ts);
+    // The next cycle starts from the same buffers: hand the final z back as level 0's z.
+    return zs[0];
+}
+
+// `solve` / `step_in` / `run`, CD:227-314, on device arrays h (in/out) and kd.
+static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd, double dt,
+                         int solver, double tol_m) {
+    if (std::isnan(dt) || dt <= 0.0) return 0;
+    hipStream_t s = ctx->stream;
+    const bool mg = solver == DTD_SOLVER_MULTIGRID;
+    const int n = g->n, n6 = g->n6;
+    kc_couplings<<<nblocks(n6), kBlock, 0, s>>>(n6, g->slots, g->lap, kd, dt / g->mean_area,
+                                                g->kfull, g->any);
+    LAUNCH_CHECK();
+    // Every cell taking part (the multigrid solve, or every cell coupled) is the grid's own
+    // operator; else the cells with a coupling, ascending, with their neighbours as positions.
+    bool all = mg;
+    int l = n6;
+    if (!mg) {
+        std::vector<unsigned char> any(n6);
+        download(ctx, any.data(), g->any, n6);
+        all = std::all_of(any.begin(), any.end(), [](unsigned char v) { return v != 0; });
+        if (!all) {
+            std::vector<uint32_t> pos(n6, kNoCell), cells;
+            for (int c = 0; c < n6; c++)
+                if (any[c]) pos[c] = (uint32_t)cells.size(), cells.push_back((uint32_t)c);
+            l = (int)cells.size();
+            if (l == 0) return 0;
+            std::vector<uint32_t> cols(8 * (size_t)l);
+            for (int i = 0; i < l; i++)
+                for (int q = 0; q < 8; q++) {
+                    const uint32_t m = g->slots_h[8 * (size_t)cells[i] + q];
+                    cols[8 * (size_t)i + q] = m == kNoCell ? kNoCell : pos[m];
+                }
+            upload(ctx, g->cells, cells.data(), (size_t)l * 4);
+            upload(ctx, g->cols, cols.data(), cols.size() * 4);
+            HIPCHK(hipStreamSynchronize(s));  // the host lists end with this block
+        }
+    }
+    const uint32_t *cells = all ? nullptr : g->cells;
+    const uint32_t *cols = all ? g->slots : g->cols;
+    double *k = all ? g->kfull : g->kpos;
+    const int nb = nblocks(l), nc = (l + kCubeChunk - 1) / kCubeChunk;
+    kc_opinit<<<nb, kBlock, 0, s>>>(l, cells, g->kfull, k, g->area, g->mean_area, h, g->mu, g->s,
+                                    g->diag, g->rhs, g->x);
+    LAUNCH_CHECK();
+    if (all) {
+        // The first guess from b = h (CD:280-284).
+        HIPCHK(hipMemcpyAsync(g->b, g->x, (size_t)n6 * 8, hipMemcpyDeviceToDevice, s));
+        const int nl = nblocks(6 * n);
+        kc_lod_rows<<<nl, kBlock, 0, s>>>(n, k, g->mu, g->slots, g->b, g->x, g->scratch);
+        LAUNCH_CHECK();
+        kc_lod_cols<<<nl, kBlock, 0, s>>>(n, k, g->mu, g->slots, g->x, g->tmp, g->scratch);
+        LAUNCH_CHECK();
+        kc_lod_back<<<nblocks(n6), kBlock, 0, s>>>(n, g->tmp, g->x);
+        LAUNCH_CHECK();
+    }
+    if (mg) {
+        kc_mg_finest<<<nblocks(n6), kBlock, 0, s>>>(g->lv[0], g->slots, k, g->mu, g->diag);
+        LAUNCH_CHECK();
+        for (size_t q = 1; q < g->lv.size(); q++) {
+            kc_mg_vals<<<nblocks(g->lv[q].ne), kBlock, 0, s>>>(g->lv[q - 1], g->lv[q],
+                                                                ctx->opts.scale);
+            LAUNCH_CHECK();
+            kc_mg_diag<<<nblocks(g->lv[q].cells), kBlock, 0, s>>>(g->lv[q - 1], g->lv[q]);
+            LAUNCH_CHECK();
+        }
+    }
+    // The loop (CD:380-500).
+    const double tol = mg ? tol_m : kCgTolM;
+    kc_init<<<1, 64, 0, s>>>(g->st, tol * tol * (double)n6, (int)ctx->opts.max_iter);
+    LAUNCH_CHECK();
+    kc_apply<<<nb, kBlock, 0, s>>>(l, g->diag, k, cols, g->x, g->r, g->rhs);
+    LAUNCH_CHECK();
+    kc_dot<<<nc, kBlock, 0, s>>>(l, g->r, g->r, nullptr, nullptr, g->parts, nullptr);
+    LAUNCH_CHECK();
+    kc_step<kCubeRR0><<<1, 64, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
+    LAUNCH_CHECK();
+    const float *z32 = nullptr;
+    if (mg) {
+        z32 = cube_precondition(ctx, g, g->r);
+        kc_dot<<<nc, kBlock, 0, s>>>(l, g->r, nullptr, z32, nullptr, g->parts, g->st);
+    } else {
+        kc_dot<<<nc, kBlock, 0, s>>>(l, g->r, g->r, nullptr, g->diag, g->parts, g->st);
+    }
+    LAUNCH_CHECK();
+    kc_step<kCubeRZ0><<<1, 64, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
+    LAUNCH_CHECK();
+    // Each iteration's last step leaves the stop flag in host memory; the host reads iteration
+    // k's flag while iteration k + 1 is queued.
+    const int max_iter = (int)ctx->opts.max_iter;
+    double *po = g->p0, *pn = g->p1;
+    for (int it = 0; it < max_iter; it++) {
+        const int par = it & 1;
+        kc_dir<<<nc, kBlock, 0, s>>>(l, mg, g->r, z32, g->diag, k, cols, po, pn, g->ap, g->parts,
+                                     g->st);
+        LAUNCH_CHECK();
+        std::swap(po, pn);
+        kc_step<kCubePAP><<<1, 64, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
+        LAUNCH_CHECK();
+        kc_update<<<nc, kBlock, 0, s>>>(l, po, g->ap, g->diag, g->x, g->r, g->parts, g->st);
+        LAUNCH_CHECK();
+        if (mg) {
+            kc_step<kCubeRR><<<1, 64, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
+            LAUNCH_CHECK();
+            z32 = cube_precondition(ctx, g, g->r);
+            kc_dot<<<nc, kBlock, 0, s>>>(l, g->r, nullptr, z32, nullptr, g->parts, g->st);
+            LAUNCH_CHECK();
+            kc_step<kCubeRZ><<<1, 64, 0, s>>>(g->parts, nc, g->st, mg, ctx->dev_flag + par);
+        } else {
+            kc_step<kCubeRR><<<1, 64, 0, s>>>(g->parts, nc, g->st, mg, ctx->dev_flag + par);
+        }
+        LAUNCH_CHECK();
+        HIPCHK(hipEventRecord(ctx->flag_ev[par], s));
+        if (it >= 1) {
+            HIPCHK(hipEventSynchronize(ctx->flag_ev[par ^ 1]));
+            if (ctx->host_flag[par ^ 1]) break;
+        }
+    }
+    // The put-back (CD:286-301) and the answer into h.
+    kc_apply<<<nb, kBlock, 0, s>>>(l, g->diag, k, cols, g->x, g->ap, nullptr);
+    LAUNCH_CHECK();
+    kc_putback_sums<<<nc, kBlock, 0, s>>>(l, g->rhs, g->ap, g->s, g->parts);
+    LAUNCH_CHECK();
+    kc_step<kCubePutback><<<1, 64, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
+    LAUNCH_CHECK();
+    kc_putback<<<nb, kBlock, 0, s>>>(l, cells, g->s, g->mu, g->x, h, g->st);
+    LAUNCH_CHECK();
+    HIPCHK(hipMemcpyAsync(g->h_st, g->st, sizeof(CubeState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return (size_t)g->h_st->its;
+}
+
+}  // namespace dtd

@@ -624,6 +624,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_kd(long n, const double *__rest
 #include "batch.hpp"
 #include "jacobi.hpp"
 #include "failure.hpp"
+#include "cube.hpp"
 
 using namespace dtd;
 
@@ -822,6 +823,76 @@ int dtd_slope_failure_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wr
                                        p->critical_slope, p->rate, p->iterations, h_dev,
                                        avail_dev, iterations);
         if (moved) *moved = m;
+    });
+}
+
+int dtd_cube_new(dtd_ctx *ctx, size_t n, const uint32_t *slots, const double *lap,
+                 const double *area, double mean_area, dtd_cube **out) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (!out) throw Error{DTD_EINVAL, "null output"};
+        *out = nullptr;
+        if (n < 2) throw Error{DTD_EINVAL, "a cube face needs at least 2 x 2 cells"};
+        if (n > 18000) throw Error{DTD_ETOOBIG, "the cube exceeds 2^31 cells"};
+        if (!slots || !lap || !area) throw Error{DTD_EINVAL, "null array"};
+        if (!(mean_area > 0.0) || !std::isfinite(mean_area))
+            throw Error{DTD_EINVAL, "mean_area must be finite and > 0"};
+        const size_t n6 = 6 * n * n;
+        for (size_t c = 0; c < n6; c++)
+            for (int q = 0; q < 8; q++) {
+                const uint32_t m = slots[8 * c + q];
+                if (m != DTD_NO_CELL && m >= n6)
+                    throw Error{DTD_EINVAL, "a neighbour slot is outside the grid"};
+                // Every cell has its four side neighbours; only a diagonal across a cube corner
+                // goes missing (sphere/cube.rs:12-17).
+                if (q % 2 == 0 && m == DTD_NO_CELL)
+                    throw Error{DTD_EINVAL, "a side neighbour is missing"};
+            }
+        *out = cube_new(ctx, (int)n, slots, lap, area, mean_area);
+    });
+}
+
+void dtd_cube_free(dtd_cube *g) { free_cube(g); }
+
+int dtd_cube_solve_device(dtd_ctx *ctx, dtd_cube *g, double *h_dev, const double *kd_dev,
+                          double dt, int solver, double tol_m, size_t *iterations) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (iterations) *iterations = 0;
+        if (!g) throw Error{DTD_EINVAL, "null grid"};
+        if (!h_dev || !kd_dev) throw Error{DTD_EINVAL, "null array"};
+        if (solver != DTD_SOLVER_MULTIGRID && solver != DTD_SOLVER_JACOBI)
+            throw Error{DTD_EINVAL, "unknown solver"};
+        if (solver == DTD_SOLVER_MULTIGRID && !(tol_m > 0.0))
+            throw Error{DTD_EINVAL, "tol_m must be > 0"};
+        const size_t its = cube_solve(ctx, g, h_dev, kd_dev, dt, solver, tol_m);
+        if (iterations) *iterations = its;
+    });
+}
+
+int dtd_cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h_inout, const double *kd, double dt,
+                   int solver, double tol_m, size_t *iterations) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (iterations) *iterations = 0;
+        if (!g) throw Error{DTD_EINVAL, "null grid"};
+        if (!h_inout || !kd) throw Error{DTD_EINVAL, "null array"};
+        if (solver != DTD_SOLVER_MULTIGRID && solver != DTD_SOLVER_JACOBI)
+            throw Error{DTD_EINVAL, "unknown solver"};
+        if (solver == DTD_SOLVER_MULTIGRID && !(tol_m > 0.0))
+            throw Error{DTD_EINVAL, "tol_m must be > 0"};
+        const size_t bytes = (size_t)g->n6 * sizeof(double);
+        if (!g->hd) {
+            HIPCHK(hipMalloc(&g->hd, bytes));
+            g->owned.push_back(g->hd);
+            HIPCHK(hipMalloc(&g->kdd, bytes));
+            g->owned.push_back(g->kdd);
+        }
+        upload(ctx, g->hd, h_inout, bytes);
+        upload(ctx, g->kdd, kd, bytes);
+        const size_t its = cube_solve(ctx, g, g->hd, g->kdd, dt, solver, tol_m);
+        download(ctx, h_inout, g->hd, bytes);
+        if (iterations) *iterations = its;
     });
 }
 

@@ -41,7 +41,8 @@ SYMBOLS = (
     "dtd_solve_device", "dtd_solve_rel_device", "dtd_operator", "dtd_apply", "dtd_lod",
     "dtd_cycle_new", "dtd_cycle_levels", "dtd_cycle_level", "dtd_cycle_precondition",
     "dtd_time_kernel", "dtd_solve_batch", "dtd_step_device", "dtd_fill_kd",
-    "dtd_slope_failure_device", "dtd_slope_failure",
+    "dtd_slope_failure_device", "dtd_slope_failure", "dtd_cube_new", "dtd_cube_free",
+    "dtd_cube_solve", "dtd_cube_solve_device",
 )
 # The stage's slope failure constants (crates/deeptime-core/src/surface/marine.rs:231-232).
 FAILURE_RATE = 0.05
@@ -167,6 +168,16 @@ def load(path: str = LIB_PATH):
                                            ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_uint)]
     _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+    _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+    _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    L.dtd_cube_new.argtypes = [_vp, _sz, _u32p, _f64p, _f64p, ctypes.c_double,
+                               ctypes.POINTER(_vp)]
+    L.dtd_cube_free.argtypes = [_vp]
+    L.dtd_cube_free.restype = None
+    L.dtd_cube_solve.argtypes = [_vp, _vp, _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                 ctypes.c_double, ctypes.POINTER(_sz)]
+    L.dtd_cube_solve_device.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
+                                        ctypes.c_double, ctypes.POINTER(_sz)]
     L.dtd_slope_failure.argtypes = [_vp, _sz, _sz, ctypes.c_double, ctypes.c_int, _u8p,
                                     ctypes.c_double, ctypes.POINTER(Failure), _dp, _dp,
                                     ctypes.POINTER(ctypes.c_double),
@@ -303,6 +314,46 @@ def slope_failure_device(ws: Workspace, d: Dims, sea_dev, sea_level: float, h_de
                                              _ptr(h_dev), _ptr(avail_dev), ctypes.byref(moved),
                                              ctypes.byref(its)))
     return moved.value, its.value
+
+
+class Cube:
+    """A cube grid on the device (dtd_cube_new): the slots, couplings and areas `CubeGrid` hands
+    the cube's diffusion solve (crates/deeptime-core/src/surface/cube/diffusion.rs), e.g. a
+    `dtdiff.cubegrid.CubeGrid`.  solve() is `cube::diffusion::solve` on host arrays (h in place),
+    solve_device() on device arrays."""
+
+    def __init__(self, ws: Workspace, grid):
+        self.ws, self.n = ws, grid.n
+        self._g = _vp()
+        ws.check(load().dtd_cube_new(ws._ctx, grid.n,
+                                     np.ascontiguousarray(grid.slots, dtype=np.uint32),
+                                     np.ascontiguousarray(grid.lap, dtype=np.float64),
+                                     np.ascontiguousarray(grid.area, dtype=np.float64),
+                                     grid.mean_area, ctypes.byref(self._g)))
+
+    def solve(self, h: np.ndarray, kd, dt: float, solver: DiffusionSolver, tol_m: float) -> int:
+        if not (isinstance(h, np.ndarray) and h.dtype == np.float64 and h.flags.c_contiguous):
+            raise TypeError("h must be a C-contiguous float64 array (it is solved in place)")
+        if h.size != 6 * self.n * self.n or np.size(kd) != h.size:
+            raise ValueError("h and kd must have 6 n^2 cells")
+        kd, kdp = _f64(kd)
+        its = _sz(0)
+        self.ws.check(load().dtd_cube_solve(self.ws._ctx, self._g, h.ctypes.data_as(_dp), kdp,
+                                            dt, int(solver), tol_m, ctypes.byref(its)))
+        return its.value
+
+    def solve_device(self, h_dev, kd_dev, dt: float, solver: DiffusionSolver,
+                     tol_m: float) -> int:
+        its = _sz(0)
+        self.ws.check(load().dtd_cube_solve_device(self.ws._ctx, self._g, _ptr(h_dev),
+                                                   _ptr(kd_dev), dt, int(solver), tol_m,
+                                                   ctypes.byref(its)))
+        return its.value
+
+    def close(self):
+        if self._g:
+            load().dtd_cube_free(self._g)
+            self._g = _vp()
 
 
 def step_device(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float) -> Stats:

@@ -35,6 +35,7 @@
 #define DTDIFF_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -178,6 +179,33 @@ int dtd_slope_failure_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wr
 int dtd_slope_failure(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x,
                       const unsigned char *sea, double sea_level, const dtd_failure *p,
                       double *h_inout, double *avail_inout, double *moved, unsigned *iterations);
+
+/* A cube grid on the device: what `CubeGrid` (crates/deeptime-core/src/sphere/cube.rs) hands the
+ * cube's diffusion solve.  n x n cells on each of 6 faces, cell (face, i, j) at face n^2 + j n + i;
+ * slots: each cell's 8 neighbours in D8 order in its face frame (E, NE, N, NW, W, SW, S, SE),
+ * DTD_NO_CELL across a cube corner (`CubeTopology::slots`); lap: the Laplacian's 8 couplings a
+ * cell (`CubeGrid::lap`, symmetric); area: the cells' areas and mean_area their mean
+ * (`CubeGrid::areas`, `mean_area`).  The arrays are copied; the V-cycle's levels (their neighbour
+ * lists and the finer entries each coarse value sums, cube/multigrid.rs:65-186) are built here,
+ * once.  Free with dtd_cube_free (before the context). */
+#define DTD_NO_CELL 0xFFFFFFFFu
+typedef struct dtd_cube dtd_cube;
+int dtd_cube_new(dtd_ctx *ctx, size_t n, const uint32_t *slots, const double *lap,
+                 const double *area, double mean_area, dtd_cube **out);
+void dtd_cube_free(dtd_cube *g);
+
+/* `surface::cube::diffusion::solve` (crates/deeptime-core/src/surface/cube/diffusion.rs:227-314)
+ * on host arrays of 6 n^2 cells: h (in/out), kd (m^2/yr), dt (yr).  DTD_SOLVER_MULTIGRID: every
+ * cell, the face-wise first guess, CG preconditioned by the cube's f32 V-cycle to an RMS residual
+ * of tol_m; DTD_SOLVER_JACOBI: `step_in` (CD:249-259), the cells with a coupling, the diagonal
+ * preconditioner, its own 1e-6 m.  Then the residual's sum put back (Σ a h kept).  Every sum is
+ * taken in the reference's order, so the iterations and h are the reference's.  dt NaN or <= 0
+ * does nothing.  *iterations (may be NULL) gets the CG iterations. */
+int dtd_cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h_inout, const double *kd, double dt,
+                   int solver, double tol_m, size_t *iterations);
+/* The same on device arrays. */
+int dtd_cube_solve_device(dtd_ctx *ctx, dtd_cube *g, double *h_dev, const double *kd_dev,
+                          double dt, int solver, double tol_m, size_t *iterations);
 
 /* One grid of a batched solve: a dtd_solve_device call's arguments. */
 typedef struct dtd_grid {
