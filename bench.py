@@ -60,9 +60,11 @@ def main():
                     help="bf16: the f32 cycle with its two finest levels' records stored in bf16")
     ap.add_argument("--guess", choices=["lod", "rhs"], default="lod")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
-    ap.add_argument("--stage", choices=["solve", "failure"], default="solve",
+    ap.add_argument("--stage", choices=["solve", "failure", "cube"], default="solve",
                     help="failure: the submarine slope failure after the solve (8192x4096 world "
-                         "with sediment everywhere, 40 iterations)")
+                         "with sediment everywhere, 40 iterations); cube: the cube-grid solve "
+                         "(--cube-n cells a face side)")
+    ap.add_argument("--cube-n", type=int, default=512)
     ap.add_argument("--kernels-only", action="store_true",
                     help="one solve, then only the per-kernel timings (for counter runs)")
     args = ap.parse_args()
@@ -79,6 +81,8 @@ def main():
 
     if args.stage == "failure":
         return bench_failure(args)
+    if args.stage == "cube":
+        return bench_cube(args)
     gen, kw = F.WORKLOADS[args.workload]
     if gen is F.case_regional:
         return bench_batch(args, gen, dict(kw))
@@ -364,6 +368,84 @@ def bench_failure(args):
                                "avail in and out, sea flag in: 33 B/cell an iteration"},
         "cpu_baseline": cpu,
         "iterations": int(n),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_cube(args):
+    """The cube-grid solve (`surface::cube::diffusion::solve`, multigrid) on resident arrays: a
+    cube of --cube-n cells a face side (6 n^2 cells) from `dtdiff.cubegrid`, a log-normal kd, k^-2
+    spectral heights mapped face by face, dt mean(kd) / mean area = 50, to an RMS residual of
+    1e-6 m.  A step restores h (a device copy, outside the timed region) and runs the call, which
+    returns when the device is done (host clock around it).  The CPU leg is the oracle's solve at
+    n / 4 on the same generator."""
+    import numpy as np
+
+    import dtdiff
+    from dtdiff import cubegrid as CG
+    from dtdiff import fields as F
+    from dtdiff.hipmem import DeviceArray, synchronize
+
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+
+    def inputs(n):
+        g = CG.cube_grid(n)
+        h = np.concatenate([F.spectral_noise(n, n, 11 + f) for f in range(6)])
+        lnk = F.splitmix_normal(23, g.cells)
+        kd = np.exp(0.5 * lnk)
+        kd *= 1.0e3 / kd.mean()
+        dt = 50.0 * g.mean_area / 1.0e3
+        return g, h, kd, dt
+
+    n = args.cube_n
+    t = time.time()
+    g, h0, kd, dt = inputs(n)
+    log(f"cube n {n} ({g.cells} cells) generated in {time.time() - t:.1f} s")
+    ws = dtdiff.Workspace(device=0)
+    cube = dtdiff.Cube(ws, g)
+    h0d, kdd, hd = DeviceArray.from_numpy(h0), DeviceArray.from_numpy(kd), DeviceArray(g.cells)
+
+    def step():
+        hd.copy_from(h0d)
+        synchronize()
+        t0 = time.perf_counter()
+        its = cube.solve_device(hd, kdd, dt, dtdiff.DiffusionSolver.MULTIGRID, 1e-6)
+        return its, time.perf_counter() - t0
+
+    for _ in range(args.warmup):
+        step()
+    total, its = 0.0, 0
+    for _ in range(args.steps):
+        its, el = step()
+        total += el
+    ms = total * 1e3 / args.steps
+    log(f"timed {args.steps} solves: {ms:.3f} ms/solve, {its} iterations")
+    cpu = None
+    if not args.no_cpu:
+        ns = max(n // 4, 2)
+        gs, hs, kds, dts = inputs(ns)
+        t = time.perf_counter()
+        _, its_o = O.cube_solve(gs, hs, kds, dts, 0, 1e-6)
+        cpu_s = time.perf_counter() - t
+        cpu = {"value": gs.cells / cpu_s, "unit": "unknowns/s", "cores": O.threads(),
+               "kind": "port",
+               "sample": f"the same generator at n {ns} ({gs.cells} cells), one multigrid solve "
+                         f"by the oracle (OpenMP over cells): {cpu_s:.2f} s, {its_o} iterations"}
+        log(f"cpu oracle sample n {ns}: {cpu_s:.2f} s")
+    cube.close()
+    out = {
+        "metric": f"unknowns/s of the cube-grid multigrid solve at n {n} (6 n^2 cells)",
+        "value": g.cells * args.steps / total,
+        "unit": "unknowns/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"cube grid n {n}: {g.cells} cells, log-normal kd, "
+                               f"dt mean(kd)/mean area = 50, RMS residual 1e-6 m"},
+        "roofline": None,
+        "cpu_baseline": cpu,
+        "iterations": int(its),
     }
     print(json.dumps(out), flush=True)
 

@@ -331,19 +331,34 @@ __global__ __launch_bounds__(kBlock) void kc_update(int l, const double *__restr
 // The scalar steps, one thread: the parts added in order from -0.0 (`iter().sum()`).
 enum { kCubeRR0 = 0, kCubeRZ0 = 1, kCubePAP = 2, kCubeRR = 3, kCubeRZ = 4, kCubePutback = 5 };
 
+// The block stages the parts in LDS, a tile at a time, and thread 0 adds them in order.
+constexpr int kStepTile = 2048;
+
 template <int STEP>
-__global__ void kc_step(const double *__restrict__ parts, int np, CubeState *st, int mg,
-                        int *flag) {
-    if (threadIdx.x != 0) return;
+__global__ __launch_bounds__(kBlock) void kc_step(const double *__restrict__ parts, int np,
+                                                 CubeState *st, int mg, int *flag) {
+    __shared__ double ta[kStepTile], tb[kStepTile];
     if (STEP != kCubeRR0 && STEP != kCubePutback && st->done) {
-        if (flag) *reinterpret_cast<volatile int *>(flag) = st->done;
+        if (flag && threadIdx.x == 0) *reinterpret_cast<volatile int *>(flag) = st->done;
         return;
     }
     const int stride = (STEP == kCubeRR || STEP == kCubePutback) ? 2 : 1;
     double a = -0.0, b = -0.0;
-    for (int q = 0; q < np; q++) a += parts[stride * q];
-    if (stride == 2)
-        for (int q = 0; q < np; q++) b += parts[2 * q + 1];
+    for (int q0 = 0; q0 < np; q0 += kStepTile) {
+        const int cnt = min(kStepTile, np - q0);
+        for (int q = threadIdx.x; q < cnt; q += kBlock) {
+            ta[q] = parts[stride * (q0 + q)];
+            if (stride == 2) tb[q] = parts[2 * (q0 + q) + 1];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int q = 0; q < cnt; q++) a += ta[q];
+            if (stride == 2)
+                for (int q = 0; q < cnt; q++) b += tb[q];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
     if (STEP == kCubeRR0) {
         // rr0 <= tol2 ends the solve before the loop (CD:392-394).
         st->have_beta = 0, st->its = 0, st->done = a <= st->tol2 ? 1 : 0;
@@ -795,7 +810,7 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
     LAUNCH_CHECK();
     kc_dot<<<nc, kBlock, 0, s>>>(l, g->r, g->r, nullptr, nullptr, g->parts, nullptr);
     LAUNCH_CHECK();
-    kc_step<kCubeRR0><<<1, 64, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
+    kc_step<kCubeRR0><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
     LAUNCH_CHECK();
     const float *z32 = nullptr;
     if (mg) {
@@ -805,7 +820,7 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
         kc_dot<<<nc, kBlock, 0, s>>>(l, g->r, g->r, nullptr, g->diag, g->parts, g->st);
     }
     LAUNCH_CHECK();
-    kc_step<kCubeRZ0><<<1, 64, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
+    kc_step<kCubeRZ0><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
     LAUNCH_CHECK();
     // Each iteration's last step leaves the stop flag in host memory; the host reads iteration
     // k's flag while iteration k + 1 is queued.
@@ -817,19 +832,19 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
                                      g->st);
         LAUNCH_CHECK();
         std::swap(po, pn);
-        kc_step<kCubePAP><<<1, 64, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
+        kc_step<kCubePAP><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
         LAUNCH_CHECK();
         kc_update<<<nc, kBlock, 0, s>>>(l, po, g->ap, g->diag, g->x, g->r, g->parts, g->st);
         LAUNCH_CHECK();
         if (mg) {
-            kc_step<kCubeRR><<<1, 64, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
+            kc_step<kCubeRR><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
             LAUNCH_CHECK();
             z32 = cube_precondition(ctx, g, g->r);
             kc_dot<<<nc, kBlock, 0, s>>>(l, g->r, nullptr, z32, nullptr, g->parts, g->st);
             LAUNCH_CHECK();
-            kc_step<kCubeRZ><<<1, 64, 0, s>>>(g->parts, nc, g->st, mg, ctx->dev_flag + par);
+            kc_step<kCubeRZ><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, ctx->dev_flag + par);
         } else {
-            kc_step<kCubeRR><<<1, 64, 0, s>>>(g->parts, nc, g->st, mg, ctx->dev_flag + par);
+            kc_step<kCubeRR><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, ctx->dev_flag + par);
         }
         LAUNCH_CHECK();
         HIPCHK(hipEventRecord(ctx->flag_ev[par], s));
@@ -843,7 +858,7 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
     LAUNCH_CHECK();
     kc_putback_sums<<<nc, kBlock, 0, s>>>(l, g->rhs, g->ap, g->s, g->parts);
     LAUNCH_CHECK();
-    kc_step<kCubePutback><<<1, 64, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
+    kc_step<kCubePutback><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
     LAUNCH_CHECK();
     kc_putback<<<nb, kBlock, 0, s>>>(l, cells, g->s, g->mu, g->x, h, g->st);
     LAUNCH_CHECK();
