@@ -137,22 +137,28 @@ __global__ __launch_bounds__(kBlock) void kc_apply(int l, const double *__restri
 // `thomas` (D:1029-1046) on one line of n values at v[t * stride] (solved in place), the
 // tridiagonal coefficients formed by coef(t, lo, di, up) as the reference's arrays hold them; c is
 // the line's scratch of n values.
+// The running values (the last b, beta and up) stay in registers, so a step waits on no store of
+// the step before.
 template <class F>
 __device__ __forceinline__ void thomas_line(int n, double *v, int stride, double *c, F coef) {
     double lo, di, up, up_prev;
     coef(0, lo, di, up_prev);
     double beta = di;
-    v[0] /= beta;
+    double prev = v[0] / beta;
+    v[0] = prev;
     for (int t = 1; t < n; t++) {
         coef(t, lo, di, up);
-        c[t] = up_prev / beta;
-        beta = di - lo * c[t];
-        v[(size_t)t * stride] = (v[(size_t)t * stride] - lo * v[(size_t)(t - 1) * stride]) / beta;
+        const double ct = up_prev / beta;
+        c[t] = ct;
+        beta = di - lo * ct;
+        prev = (v[(size_t)t * stride] - lo * prev) / beta;
+        v[(size_t)t * stride] = prev;
         up_prev = up;
     }
+    double next = prev;
     for (int t = n - 2; t >= 0; t--) {
-        const double next = v[(size_t)(t + 1) * stride];
-        v[(size_t)t * stride] -= c[t + 1] * next;
+        next = v[(size_t)t * stride] - c[t + 1] * next;
+        v[(size_t)t * stride] = next;
     }
 }
 
