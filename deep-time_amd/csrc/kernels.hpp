@@ -1018,9 +1018,19 @@ __device__ inline Band band_at(int w, int h, int wrap, int th, int wave) {
 // The wave number of this thread's wave when a grid's waves start at block `block`.
 __device__ inline int block_wave(int block) { return block * (kBlock / 64) + (threadIdx.x >> 6); }
 
+// The launch's blocks go to the 8 XCDs in turn (block b to XCD b mod 8).  Numbering the bands so
+// that each XCD takes one contiguous run of them puts a band and the band below it (which reads
+// the first's last rows as its halo) on the same XCD at about the same time, so the halo rows come
+// from that XCD's L2.
+__device__ inline int xcd_block(int b, int nb) {
+    const int per = nb / 8, rem = nb % 8, x = b % 8, k = b / 8;
+    return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+}
+__device__ inline int launch_wave() { return block_wave(xcd_block(blockIdx.x, gridDim.x)); }
+
 template <int V, int H = LaneGeom<V>::halo>
 __device__ inline Band band_of(int w, int h, int wrap, int th) {
-    return band_at<V, H>(w, h, wrap, th, block_wave(blockIdx.x));
+    return band_at<V, H>(w, h, wrap, th, launch_wave());
 }
 
 template <typename T, int V>
@@ -1162,7 +1172,7 @@ __global__ __launch_bounds__(kBlock) void k_spre(Lvl<T> L, T *__restrict__ zout,
     __shared__ double lds[kBlock / 64];
     if (st && st->done) return;
     if (RM == kRUpdate) rio.alpha = st->alpha;
-    const double acc = spre_band<T, V, RM, CB>(L, zout, C, th, block_wave(blockIdx.x), rio);
+    const double acc = spre_band<T, V, RM, CB>(L, zout, C, th, launch_wave(), rio);
     if (RM == kRUpdate) {
         const double sum = block_sum<kBlock>(acc, lds);
         if (threadIdx.x == 0) parts[blockIdx.x] = sum;
@@ -1273,7 +1283,7 @@ __global__ __launch_bounds__(kBlock) void k_spost(Lvl<T> L, const T *__restrict_
     __shared__ double lds[kBlock / 64];
     if (st && st->done) return;
     const double acc =
-        spost_band<T, V, RM, CB>(L, zin, zc, wc, zout, th, dot, block_wave(blockIdx.x));
+        spost_band<T, V, RM, CB>(L, zin, zc, wc, zout, th, dot, launch_wave());
     if (dot) {
         const double sum = block_sum<kBlock>(acc, lds);
         if (threadIdx.x == 0) parts[blockIdx.x] = sum;
@@ -1426,7 +1436,7 @@ __global__ __launch_bounds__(kBlock) void k_sdir(Grid g, const TZ *__restrict__ 
     __shared__ double lds[kBlock / 64];
     if (st->done) return;
     const double acc = sdir_band<TZ>(g, st->kd, st->dt, st->s2, z, pold, pnew, ap, x, th, st,
-                                     block_wave(blockIdx.x));
+                                     launch_wave());
     const double sum = block_sum<kBlock>(acc, lds);
     if (threadIdx.x == 0) parts[blockIdx.x] = sum;
 }
@@ -1548,7 +1558,7 @@ __global__ __launch_bounds__(kBlock) void k_sresid(Grid g, const double *__restr
                                                    double *__restrict__ r, TR *__restrict__ rc,
                                                    int th, double *__restrict__ parts) {
     __shared__ double lds[kBlock / 64];
-    block_pair(sresid_band<TR>(g, kd, dt, s2, b, xin, xout, r, rc, th, block_wave(blockIdx.x)),
+    block_pair(sresid_band<TR>(g, kd, dt, s2, b, xin, xout, r, rc, th, launch_wave()),
                parts, blockIdx.x, lds);
 }
 
@@ -1627,7 +1637,7 @@ __global__ __launch_bounds__(kBlock) void k_finish1(Grid g, const double *__rest
                                                     double *__restrict__ parts,
                                                     const CgState *__restrict__ st) {
     __shared__ double lds[kBlock / 64];
-    block_pair(finish1_band(g, kd, dt, s2, b, x, p0, p1, xout, th, st, block_wave(blockIdx.x)),
+    block_pair(finish1_band(g, kd, dt, s2, b, x, p0, p1, xout, th, st, launch_wave()),
                parts, blockIdx.x, lds);
 }
 
@@ -1711,7 +1721,7 @@ __global__ __launch_bounds__(kBlock) void k_finish2(Grid g, const double *__rest
                                                     double *__restrict__ parts,
                                                     const CgState *__restrict__ st) {
     __shared__ double lds[kBlock / 64];
-    block_pair(finish2_band(g, kd, dt, s2, b, xp, xout, th, st, block_wave(blockIdx.x)), parts,
+    block_pair(finish2_band(g, kd, dt, s2, b, xp, xout, th, st, launch_wave()), parts,
                blockIdx.x, lds);
 }
 
