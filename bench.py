@@ -69,8 +69,8 @@ def main():
                     help="one solve, then only the per-kernel timings (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
-              "described in DESIGN.md and is not built.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); sharding over several "
+              "GPUs is a design here, not built.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

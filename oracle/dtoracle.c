@@ -1124,7 +1124,7 @@ double or_bound(size_t w, size_t h, double dx, int wrap_x, const double *kd, dou
     return tol * (1.0 + (mean > 0.0 ? rms / mean : 0.0)) * 1.01;
 }
 
-/* ||b - A x||_2 / ||b||_2 in f64 (the north-star convergence measure; not a reference function). */
+/* ||b - A x||_2 / ||b||_2 in f64 (the benchmark's convergence measure; not a reference function). */
 double or_rel_residual(size_t w, size_t h, double dx, int wrap_x, const double *kd, double dt,
                        const double *b, const double *x) {
     size_t n = w * h;

@@ -149,7 +149,7 @@ def test_the_gpu_solve_meets_its_tolerance_and_keeps_the_mass(ws):
 
 @pytest.mark.parametrize("w,h,gen", [(512, 256, "constant"), (1024, 512, "lognormal"), (300, 129, "lognormal")])
 def test_tight_solve_matches_the_oracle_to_1e10(ws, w, h, gen):
-    """The north-star bar: relative L2 between the GPU FP64 answer and the oracle's <= 1e-10 on
+    """The parity bar: relative L2 between the GPU FP64 answer and the oracle's <= 1e-10 on
     identical seeded inputs, both solved to a relative residual of 1e-12."""
     print(f"tight {gen} {w}x{h}", flush=True)
     d, h0, kd, dt = (F.case_constant if gen == "constant" else F.case_lognormal)(w=w, h=h)
