@@ -50,8 +50,10 @@ __global__ __launch_bounds__(kBlock) void kc_couplings(int n6, const uint32_t *_
                                                       unsigned char *__restrict__ any) {
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= n6) return;
-    const uint32_t *sl = slots + 8 * (size_t)c;
     const double kc = kd[c];
+    uint32_t sl[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) sl[q] = slots[(size_t)q * n6 + c];
     bool nz = false;
 #pragma unroll
     for (int q = 0; q < 8; q++) {
@@ -67,9 +69,9 @@ __global__ __launch_bounds__(kBlock) void kc_couplings(int n6, const uint32_t *_
                 d = fmin(fmin(fmin(0.5 * (kc + ka), 0.5 * (ka + km)), 0.5 * (km + kb)),
                          0.5 * (kb + kc));
             }
-            v = scale * d * lap[8 * (size_t)c + q];
+            v = scale * d * lap[(size_t)q * n6 + c];
         }
-        k[8 * (size_t)c + q] = v;
+        k[(size_t)q * n6 + c] = v;
         nz |= v != 0.0;
     }
     any[c] = nz;
@@ -78,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void kc_couplings(int n6, const uint32_t *_
 // The operator at each position i of the cells taking part (cells[i]; the identity when every cell
 // does): its couplings (gathered when a subset takes part), mu = a / mean area, s = the couplings'
 // sum in slot order from -0.0, diag = mu + s (CD:206-215); rhs = mu h and x = h (CD:268-279).
-__global__ __launch_bounds__(kBlock) void kc_opinit(int l, const uint32_t *__restrict__ cells,
+__global__ __launch_bounds__(kBlock) void kc_opinit(int l, int n6, const uint32_t *__restrict__ cells,
                                                    const double *__restrict__ kfull,
                                                    double *__restrict__ kpos,
                                                    const double *__restrict__ area, double mean,
@@ -93,8 +95,8 @@ __global__ __launch_bounds__(kBlock) void kc_opinit(int l, const uint32_t *__res
     double sum = -0.0;
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-        const double v = kfull[8 * (size_t)c + q];
-        if (kpos != kfull) kpos[8 * (size_t)i + q] = v;
+        const double v = kfull[(size_t)q * n6 + c];
+        if (kpos != kfull) kpos[(size_t)q * l + i] = v;
         sum += v;
     }
     const double m = area[c] / mean;
@@ -107,15 +109,15 @@ __global__ __launch_bounds__(kBlock) void kc_opinit(int l, const uint32_t *__res
 }
 
 // `Op::row`, CD:79-91.
-__device__ __forceinline__ double cube_row(int i, const double *__restrict__ diag,
+__device__ __forceinline__ double cube_row(int i, int l, const double *__restrict__ diag,
                                            const double *__restrict__ k,
                                            const uint32_t *__restrict__ cols,
                                            const double *__restrict__ p) {
     double acc = diag[i] * p[i];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-        const uint32_t j = cols[8 * (size_t)i + q];
-        if (j != kNoCell) acc -= k[8 * (size_t)i + q] * p[j];
+        const uint32_t j = cols[(size_t)q * l + i];
+        if (j != kNoCell) acc -= k[(size_t)q * l + i] * p[j];
     }
     return acc;
 }
@@ -128,95 +130,77 @@ __global__ __launch_bounds__(kBlock) void kc_apply(int l, const double *__restri
                                                   const double *__restrict__ sub) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= l) return;
-    const double a = cube_row(i, diag, k, cols, p);
+    const double a = cube_row(i, l, diag, k, cols, p);
     out[i] = sub ? sub[i] - a : a;  // sub: r = b - A x (CD:386-390)
 }
 
 // ---- the face-wise first guess (CD:316-378) ----------------------------------------------------
 
-// `thomas` (D:1029-1046) on one line of n values at v[t * stride] (solved in place), the
-// tridiagonal coefficients formed by coef(t, lo, di, up) as the reference's arrays hold them; c is
-// the line's scratch of n values.
-// The running values (the last b, beta and up) stay in registers, so a step waits on no store of
-// the step before.
-template <class F>
-__device__ __forceinline__ void thomas_line(int n, double *v, int stride, double *c, F coef) {
-    double lo, di, up, up_prev;
-    coef(0, lo, di, up_prev);
-    double beta = di;
-    double prev = v[0] / beta;
-    v[0] = prev;
-    for (int t = 1; t < n; t++) {
-        coef(t, lo, di, up);
-        const double ct = up_prev / beta;
-        c[t] = ct;
-        beta = di - lo * ct;
-        prev = (v[(size_t)t * stride] - lo * prev) / beta;
-        v[(size_t)t * stride] = prev;
-        up_prev = up;
-    }
-    double next = prev;
-    for (int t = n - 2; t >= 0; t--) {
-        next = v[(size_t)t * stride] - c[t + 1] * next;
-        v[(size_t)t * stride] = next;
-    }
-}
-
-// The rows of each face (slots E = 0 and W = 4), one thread a row: x = the solved row of b, the
-// couplings across a face's edge taken at b's value beyond it.
-__global__ __launch_bounds__(kBlock) void kc_lod_rows(int n, const double *__restrict__ k,
+// The lines' tridiagonal systems, formed first by one thread a cell: row line of cell (face, j, i)
+// at position i of line face n + j (row-major, as the grid), column line at position j of line
+// face n + i (column-major).  Each position gets the reference's lo, di, up and right-hand side:
+// di = 1 + (K_a + K_b) / mu, lo = -K_a / mu (0 first), up = -K_b / mu (0 last), v = b plus, at a
+// face's edge, K b_beyond / mu (CD:326-345, 350-368).
+__global__ __launch_bounds__(kBlock) void kc_lod_coef(int n, int cols, const double *__restrict__ k,
                                                      const double *__restrict__ mu,
                                                      const uint32_t *__restrict__ slots,
                                                      const double *__restrict__ b,
-                                                     double *__restrict__ x,
-                                                     double *__restrict__ scratch) {
-    const int row = blockIdx.x * kBlock + threadIdx.x;
-    if (row >= 6 * n) return;
-    const size_t c0 = (size_t)row * n;
-    for (int i = 0; i < n; i++) {
-        const size_t c = c0 + i;
-        const double ke = k[8 * c], kw = k[8 * c + 4], m = mu[c];
-        double v = b[c];
-        if (i == 0) v += kw * b[slots[8 * c + 4]] / m;
-        if (i + 1 == n) v += ke * b[slots[8 * c]] / m;
-        x[c] = v;
-    }
-    thomas_line(n, x + c0, 1, scratch + c0, [&](int i, double &lo, double &di, double &up) {
-        const size_t c = c0 + i;
-        const double ke = k[8 * c], kw = k[8 * c + 4], m = mu[c];
-        di = 1.0 + (ke + kw) / m;
-        lo = i > 0 ? -kw / m : 0.0;
-        up = i + 1 < n ? -ke / m : 0.0;
-    });
+                                                     double *__restrict__ lo,
+                                                     double *__restrict__ di,
+                                                     double *__restrict__ up,
+                                                     double *__restrict__ v) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    const int nn = n * n;
+    if (c >= 6 * nn) return;
+    const int f = c / nn, rest = c - f * nn, j = rest / n, i = rest - j * n;
+    // Row lines: slots E (0) and W (4) along i; column lines: N (2) and S (6) along j.
+    const int sa = cols ? 2 : 4, sb = cols ? 6 : 0, t = cols ? j : i;
+    const size_t at = cols ? ((size_t)f * n + i) * n + j : (size_t)c;
+    const size_t n6 = 6 * (size_t)nn;
+    const double ka = k[sa * n6 + c], kb = k[sb * n6 + c], m = mu[c];
+    di[at] = 1.0 + (kb + ka) / m;
+    double rhs = b[c];
+    double l = 0.0, u = 0.0;
+    if (t > 0)
+        l = -ka / m;
+    else
+        rhs += ka * b[slots[sa * n6 + c]] / m;
+    if (t + 1 < n)
+        u = -kb / m;
+    else
+        rhs += kb * b[slots[sb * n6 + c]] / m;
+    lo[at] = l, up[at] = u, v[at] = rhs;
 }
 
-// The columns of each face (slots N = 2 and S = 6), one thread a column, from y (the rows' result)
-// into tmp column by column.
-__global__ __launch_bounds__(kBlock) void kc_lod_cols(int n, const double *__restrict__ k,
-                                                     const double *__restrict__ mu,
-                                                     const uint32_t *__restrict__ slots,
-                                                     const double *__restrict__ y,
-                                                     double *__restrict__ tmp,
-                                                     double *__restrict__ scratch) {
-    const int col = blockIdx.x * kBlock + threadIdx.x;
-    if (col >= 6 * n) return;
-    const size_t nn = (size_t)n * n, f = col / n, i = col % n;
-    double *tc = tmp + (size_t)col * n;
-    for (int j = 0; j < n; j++) {
-        const size_t c = f * nn + (size_t)j * n + i;
-        const double kn = k[8 * c + 2], ks = k[8 * c + 6], m = mu[c];
-        double v = y[c];
-        if (j == 0) v += kn * y[slots[8 * c + 2]] / m;
-        if (j + 1 == n) v += ks * y[slots[8 * c + 6]] / m;
-        tc[j] = v;
+// `thomas` (D:1029-1046) on each line of n values (line L at [L n, L n + n)), one thread a line:
+// v solved in place, cp the line's scratch.  The coefficients stream in independently of the
+// recurrence, so their loads run ahead of it.
+__global__ __launch_bounds__(kBlock) void kc_thomas(int nlines, int n, const double *__restrict__ lo,
+                                                   const double *__restrict__ di,
+                                                   const double *__restrict__ up,
+                                                   double *__restrict__ v,
+                                                   double *__restrict__ cp) {
+    const int line = blockIdx.x * kBlock + threadIdx.x;
+    if (line >= nlines) return;
+    const size_t o = (size_t)line * n;
+    double beta = di[o];
+    double prev = v[o] / beta;
+    v[o] = prev;
+#pragma unroll 4
+    for (int t = 1; t < n; t++) {
+        const double ct = up[o + t - 1] / beta;
+        cp[o + t] = ct;
+        const double lt = lo[o + t];
+        beta = di[o + t] - lt * ct;
+        prev = (v[o + t] - lt * prev) / beta;
+        v[o + t] = prev;
     }
-    thomas_line(n, tc, 1, scratch + (size_t)col * n, [&](int j, double &lo, double &di, double &up) {
-        const size_t c = f * nn + (size_t)j * n + i;
-        const double kn = k[8 * c + 2], ks = k[8 * c + 6], m = mu[c];
-        di = 1.0 + (kn + ks) / m;
-        lo = j > 0 ? -kn / m : 0.0;
-        up = j + 1 < n ? -ks / m : 0.0;
-    });
+    double next = prev;
+#pragma unroll 4
+    for (int t = n - 2; t >= 0; t--) {
+        next = v[o + t] - cp[o + t + 1] * next;
+        v[o + t] = next;
+    }
 }
 
 // x[f, j, i] = tmp[f, i, j].
@@ -235,7 +219,16 @@ __global__ __launch_bounds__(kBlock) void kc_lod_back(int n, const double *__res
 // start from +0.0 in the CG passes and from -0.0 in `ordered_sum`).
 __device__ __forceinline__ double chunk_sum(const double *t, int cnt, double init) {
     double s = init;
-    for (int q = 0; q < cnt; q++) s += t[q];
+    int q = 0;
+    // Eight terms read at a time (the reads do not wait on the sum), added one by one in order.
+    for (; q + 8 <= cnt; q += 8) {
+        double a[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) a[u] = t[q + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += a[u];
+    }
+    for (; q < cnt; q++) s += t[q];
     return s;
 }
 
@@ -259,8 +252,9 @@ __global__ __launch_bounds__(kBlock) void kc_dot(int l, const double *__restrict
     if (threadIdx.x == 0) parts[blockIdx.x] = chunk_sum(t, cnt, -0.0);
 }
 
-// The direction pass (CD:411-440): p = z + beta p_old (z = r / diag for the diagonal), A p, and
-// the chunk's sum of p A p from +0.0.  Every neighbour's p is formed again by the same expression.
+// The direction pass (CD:411-440), one thread a cell: p = z + beta p_old (z = r / diag for the
+// diagonal), A p, and the term p A p into t (added per chunk by kc_chunks).  Every neighbour's p
+// is formed again by the same expression.
 __global__ __launch_bounds__(kBlock) void kc_dir(int l, int mg, const double *__restrict__ r,
                                                 const float *__restrict__ z32,
                                                 const double *__restrict__ diag,
@@ -268,70 +262,73 @@ __global__ __launch_bounds__(kBlock) void kc_dir(int l, int mg, const double *__
                                                 const uint32_t *__restrict__ cols,
                                                 const double *__restrict__ pold,
                                                 double *__restrict__ pnew, double *__restrict__ ap,
-                                                double *__restrict__ parts,
+                                                double *__restrict__ t,
                                                 const CubeState *__restrict__ st) {
-    __shared__ double t[kCubeChunk];
     if (st->done) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= l) return;
     const bool hb = st->have_beta;
     const double beta = st->beta;
     auto dir = [&](uint32_t j) {
         const double zj = mg ? (double)z32[j] : r[j] / diag[j];
         return hb ? zj + beta * pold[j] : zj;
     };
-    const int c0 = blockIdx.x * kCubeChunk, cnt = min(kCubeChunk, l - c0);
-    for (int q = threadIdx.x; q < cnt; q += kBlock) {
-        const int i = c0 + q;
-        const double pi = dir(i);
-        double acc = diag[i] * pi;
+    const double pi = dir(i);
+    double acc = diag[i] * pi;
 #pragma unroll
-        for (int s = 0; s < 8; s++) {
-            const uint32_t j = cols[8 * (size_t)i + s];
-            if (j != kNoCell) acc -= k[8 * (size_t)i + s] * dir(j);
-        }
-        pnew[i] = pi;
-        ap[i] = acc;
-        t[q] = pi * acc;
+    for (int q = 0; q < 8; q++) {
+        const uint32_t j = cols[(size_t)q * l + i];
+        if (j != kNoCell) acc -= k[(size_t)q * l + i] * dir(j);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) parts[blockIdx.x] = chunk_sum(t, cnt, 0.0);
+    pnew[i] = pi;
+    ap[i] = acc;
+    t[i] = pi * acc;
 }
 
-// The update pass (CD:449-470): x += alpha p, r -= alpha A p, and the chunk's sums of r r and
-// r r / diag from +0.0 (into parts[2 chunk], parts[2 chunk + 1]).
+// parts[chunk] (and parts[2 chunk], parts[2 chunk + 1] with two arrays) = the chunk's terms
+// added in order from `init` by one thread after the block stages them in LDS.
+__global__ __launch_bounds__(kBlock) void kc_chunks(int l, const double *__restrict__ t0,
+                                                   const double *__restrict__ t1, double init,
+                                                   double *__restrict__ parts,
+                                                   const CubeState *__restrict__ st) {
+    __shared__ double t[2][kCubeChunk];
+    if (st && st->done) return;
+    const int c0 = blockIdx.x * kCubeChunk, cnt = min(kCubeChunk, l - c0);
+    for (int q = threadIdx.x; q < cnt; q += kBlock) {
+        t[0][q] = t0[c0 + q];
+        if (t1) t[1][q] = t1[c0 + q];
+    }
+    __syncthreads();
+    // The two sums run side by side, each on one thread of its own wave.
+    if (threadIdx.x == 0) {
+        const double a = chunk_sum(t[0], cnt, init);
+        if (t1)
+            parts[2 * blockIdx.x] = a;
+        else
+            parts[blockIdx.x] = a;
+    } else if (t1 && threadIdx.x == 64) {
+        parts[2 * blockIdx.x + 1] = chunk_sum(t[1], cnt, init);
+    }
+}
+
+// The update pass (CD:449-470), one thread a cell: x += alpha p, r -= alpha A p, and the terms
+// r r and r r / diag into t0 and t1 (added per chunk from +0.0 by kc_chunks).
 __global__ __launch_bounds__(kBlock) void kc_update(int l, const double *__restrict__ p,
                                                    const double *__restrict__ ap,
                                                    const double *__restrict__ diag,
                                                    double *__restrict__ x, double *__restrict__ r,
-                                                   double *__restrict__ parts,
+                                                   double *__restrict__ t0,
+                                                   double *__restrict__ t1,
                                                    const CubeState *__restrict__ st) {
-    __shared__ double t[kCubeChunk];
     if (st->done) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= l) return;
     const double alpha = st->alpha;
-    const int c0 = blockIdx.x * kCubeChunk, cnt = min(kCubeChunk, l - c0);
-    constexpr int kPer = kCubeChunk / kBlock;
-    double rv[kPer];
-#pragma unroll
-    for (int u = 0; u < kPer; u++) {
-        const int q = threadIdx.x + u * kBlock;
-        if (q < cnt) {
-            const int i = c0 + q;
-            x[i] += alpha * p[i];
-            rv[u] = r[i] - alpha * ap[i];
-            r[i] = rv[u];
-            t[q] = rv[u] * rv[u];
-        }
-    }
-    __syncthreads();
-    double s0 = 0.0;
-    if (threadIdx.x == 0) s0 = chunk_sum(t, cnt, 0.0);
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < kPer; u++) {
-        const int q = threadIdx.x + u * kBlock;
-        if (q < cnt) t[q] = rv[u] * rv[u] / diag[c0 + q];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) parts[2 * blockIdx.x] = s0, parts[2 * blockIdx.x + 1] = chunk_sum(t, cnt, 0.0);
+    x[i] += alpha * p[i];
+    const double rv = r[i] - alpha * ap[i];
+    r[i] = rv;
+    t0[i] = rv * rv;
+    t1[i] = rv * rv / diag[i];
 }
 
 // The scalar steps, one thread: the parts added in order from -0.0 (`iter().sum()`).
@@ -401,16 +398,17 @@ __global__ __launch_bounds__(kBlock) void kc_putback_sums(int l, const double *_
                                                          const double *__restrict__ ax,
                                                          const double *__restrict__ s,
                                                          double *__restrict__ parts) {
-    __shared__ double t[kCubeChunk];
+    __shared__ double t[2][kCubeChunk];
     const int c0 = blockIdx.x * kCubeChunk, cnt = min(kCubeChunk, l - c0);
-    for (int q = threadIdx.x; q < cnt; q += kBlock) t[q] = rhs[c0 + q] - ax[c0 + q];
+    for (int q = threadIdx.x; q < cnt; q += kBlock) {
+        t[0][q] = rhs[c0 + q] - ax[c0 + q];
+        t[1][q] = s[c0 + q];
+    }
     __syncthreads();
-    double a = 0.0;
-    if (threadIdx.x == 0) a = chunk_sum(t, cnt, -0.0);
-    __syncthreads();
-    for (int q = threadIdx.x; q < cnt; q += kBlock) t[q] = s[c0 + q];
-    __syncthreads();
-    if (threadIdx.x == 0) parts[2 * blockIdx.x] = a, parts[2 * blockIdx.x + 1] = chunk_sum(t, cnt, -0.0);
+    if (threadIdx.x == 0)
+        parts[2 * blockIdx.x] = chunk_sum(t[0], cnt, -0.0);
+    else if (threadIdx.x == 64)
+        parts[2 * blockIdx.x + 1] = chunk_sum(t[1], cnt, -0.0);
 }
 
 // x += (lost / weight) s / mu when weight > 0 and lost != 0 (CD:293-298), then h[cells[i]] = x.
@@ -455,7 +453,8 @@ __global__ __launch_bounds__(kBlock) void kc_mg_finest(CLvl L, const uint32_t *_
     if (c >= L.cells) return;
     uint32_t at = L.ptr[c];
     for (int q = 0; q < 8; q++)
-        if (slots[8 * (size_t)c + q] != kNoCell) L.val[at++] = (float)k[8 * (size_t)c + q];
+        if (slots[(size_t)q * L.cells + c] != kNoCell)
+            L.val[at++] = (float)k[(size_t)q * L.cells + c];
     L.mass[c] = (float)mu[c];
     L.diag[c] = (float)diag[c];
 }
@@ -554,7 +553,7 @@ struct dtd_cube {
     // Per-solve buffers (n6 cells).
     double *kfull = nullptr, *kpos = nullptr, *mu = nullptr, *s = nullptr, *diag = nullptr;
     double *rhs = nullptr, *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr;
-    double *ap = nullptr, *b = nullptr, *tmp = nullptr, *scratch = nullptr, *parts = nullptr;
+    double *ap = nullptr, *b = nullptr, *tmp = nullptr, *parts = nullptr;
     uint32_t *cells = nullptr, *cols = nullptr;
     unsigned char *any = nullptr;
     dtd::CubeState *st = nullptr;
@@ -655,8 +654,16 @@ static dtd_cube *cube_new(dtd_ctx *ctx, int n, const uint32_t *slots, const doub
             HIPCHK(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
             return p;
         };
-        g->slots = (uint32_t *)put(slots, 8 * n6 * 4);
-        g->lap = (double *)put(lap, 8 * n6 * 8);
+        // Slot-major on the device (slot q of cell c at q n6 + c), so a wave's loads of one slot
+        // are contiguous.
+        std::vector<uint32_t> slots_t(8 * n6);
+        std::vector<double> lap_t(8 * n6);
+        for (size_t c = 0; c < n6; c++)
+            for (int q = 0; q < 8; q++)
+                slots_t[q * n6 + c] = slots[8 * c + q], lap_t[q * n6 + c] = lap[8 * c + q];
+        g->slots = (uint32_t *)put(slots_t.data(), 8 * n6 * 4);
+        g->lap = (double *)put(lap_t.data(), 8 * n6 * 8);
+        HIPCHK(hipStreamSynchronize(ctx->stream));  // the host copies end with this block
         g->area = (double *)put(area, n6 * 8);
         std::vector<CPat> pats;
         pats.push_back(cube_finest(n, slots));
@@ -685,7 +692,7 @@ static dtd_cube *cube_new(dtd_ctx *ctx, int n, const uint32_t *slots, const doub
         for (double **v : {&g->kfull, &g->kpos})
             *v = (double *)own(8 * n6 * 8);
         for (double **v : {&g->mu, &g->s, &g->diag, &g->rhs, &g->x, &g->r, &g->p0, &g->p1, &g->ap,
-                           &g->b, &g->tmp, &g->scratch})
+                           &g->b, &g->tmp})
             *v = (double *)own(n6 * 8);
         g->parts = (double *)own(2 * ((n6 + kCubeChunk - 1) / kCubeChunk + 1) * 8);
         g->cells = (uint32_t *)own(n6 * 4);
@@ -772,7 +779,7 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
             for (int i = 0; i < l; i++)
                 for (int q = 0; q < 8; q++) {
                     const uint32_t m = g->slots_h[8 * (size_t)cells[i] + q];
-                    cols[8 * (size_t)i + q] = m == kNoCell ? kNoCell : pos[m];
+                    cols[(size_t)q * l + i] = m == kNoCell ? kNoCell : pos[m];
                 }
             upload(ctx, g->cells, cells.data(), (size_t)l * 4);
             upload(ctx, g->cols, cols.data(), cols.size() * 4);
@@ -783,16 +790,24 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
     const uint32_t *cols = all ? g->slots : g->cols;
     double *k = all ? g->kfull : g->kpos;
     const int nb = nblocks(l), nc = (l + kCubeChunk - 1) / kCubeChunk;
-    kc_opinit<<<nb, kBlock, 0, s>>>(l, cells, g->kfull, k, g->area, g->mean_area, h, g->mu, g->s,
-                                    g->diag, g->rhs, g->x);
+    kc_opinit<<<nb, kBlock, 0, s>>>(l, n6, cells, g->kfull, k, g->area, g->mean_area, h, g->mu,
+                                    g->s, g->diag, g->rhs, g->x);
     LAUNCH_CHECK();
     if (all) {
         // The first guess from b = h (CD:280-284).
         HIPCHK(hipMemcpyAsync(g->b, g->x, (size_t)n6 * 8, hipMemcpyDeviceToDevice, s));
         const int nl = nblocks(6 * n);
-        kc_lod_rows<<<nl, kBlock, 0, s>>>(n, k, g->mu, g->slots, g->b, g->x, g->scratch);
+        // kpos is free when every cell takes part: it holds the lines' coefficients.
+        double *lo = g->kpos, *di = lo + n6, *up = di + n6, *cp = up + n6;
+        kc_lod_coef<<<nblocks(n6), kBlock, 0, s>>>(n, 0, k, g->mu, g->slots, g->b, lo, di, up,
+                                                   g->x);
         LAUNCH_CHECK();
-        kc_lod_cols<<<nl, kBlock, 0, s>>>(n, k, g->mu, g->slots, g->x, g->tmp, g->scratch);
+        kc_thomas<<<nl, kBlock, 0, s>>>(6 * n, n, lo, di, up, g->x, cp);
+        LAUNCH_CHECK();
+        kc_lod_coef<<<nblocks(n6), kBlock, 0, s>>>(n, 1, k, g->mu, g->slots, g->x, lo, di, up,
+                                                   g->tmp);
+        LAUNCH_CHECK();
+        kc_thomas<<<nl, kBlock, 0, s>>>(6 * n, n, lo, di, up, g->tmp, cp);
         LAUNCH_CHECK();
         kc_lod_back<<<nblocks(n6), kBlock, 0, s>>>(n, g->tmp, g->x);
         LAUNCH_CHECK();
@@ -834,13 +849,17 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
     double *po = g->p0, *pn = g->p1;
     for (int it = 0; it < max_iter; it++) {
         const int par = it & 1;
-        kc_dir<<<nc, kBlock, 0, s>>>(l, mg, g->r, z32, g->diag, k, cols, po, pn, g->ap, g->parts,
+        kc_dir<<<nb, kBlock, 0, s>>>(l, mg, g->r, z32, g->diag, k, cols, po, pn, g->ap, g->tmp,
                                      g->st);
+        LAUNCH_CHECK();
+        kc_chunks<<<nc, kBlock, 0, s>>>(l, g->tmp, nullptr, 0.0, g->parts, g->st);
         LAUNCH_CHECK();
         std::swap(po, pn);
         kc_step<kCubePAP><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
         LAUNCH_CHECK();
-        kc_update<<<nc, kBlock, 0, s>>>(l, po, g->ap, g->diag, g->x, g->r, g->parts, g->st);
+        kc_update<<<nb, kBlock, 0, s>>>(l, po, g->ap, g->diag, g->x, g->r, g->tmp, g->b, g->st);
+        LAUNCH_CHECK();
+        kc_chunks<<<nc, kBlock, 0, s>>>(l, g->tmp, g->b, 0.0, g->parts, g->st);
         LAUNCH_CHECK();
         if (mg) {
             kc_step<kCubeRR><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
