@@ -136,12 +136,20 @@ __global__ __launch_bounds__(kBlock) void kc_apply(int l, const double *__restri
 
 // ---- the face-wise first guess (CD:316-378) ----------------------------------------------------
 
-// The lines' tridiagonal systems, formed first by one thread a cell: row line of cell (face, j, i)
-// at position i of line face n + j (row-major, as the grid), column line at position j of line
-// face n + i (column-major).  Each position gets the reference's lo, di, up and right-hand side:
-// di = 1 + (K_a + K_b) / mu, lo = -K_a / mu (0 first), up = -K_b / mu (0 last), v = b plus, at a
-// face's edge, K b_beyond / mu (CD:326-345, 350-368).
-__global__ __launch_bounds__(kBlock) void kc_lod_coef(int n, int cols, const double *__restrict__ k,
+// The lines' tridiagonal systems, formed first by one thread a cell.  Row line of cell
+// (face, j, i): line face n + j, position i; column line: line face n + i, position j.  The
+// systems are stored line-interleaved, position t of line L at t 6n + L, so that a wave of
+// consecutive lines reads each position's values in one contiguous run.  Each position gets the
+// reference's lo, di, up and right-hand side: di = 1 + (K_a + K_b) / mu, lo = -K_a / mu (0 first),
+// up = -K_b / mu (0 last), v = b plus, at a face's edge, K b_beyond / mu (CD:326-345, 350-368).
+// b is in cell order, or (b_rows) in the row lines' interleaved order the row pass left it in.
+__device__ __forceinline__ size_t lod_row_at(uint32_t c, int n) {
+    const uint32_t nn = (uint32_t)n * n, f = c / nn, rest = c - f * nn, j = rest / n;
+    return (size_t)(rest - j * n) * (6 * (size_t)n) + (size_t)f * n + j;
+}
+
+__global__ __launch_bounds__(kBlock) void kc_lod_coef(int n, int cols, int b_rows,
+                                                     const double *__restrict__ k,
                                                      const double *__restrict__ mu,
                                                      const uint32_t *__restrict__ slots,
                                                      const double *__restrict__ b,
@@ -155,62 +163,129 @@ __global__ __launch_bounds__(kBlock) void kc_lod_coef(int n, int cols, const dou
     const int f = c / nn, rest = c - f * nn, j = rest / n, i = rest - j * n;
     // Row lines: slots E (0) and W (4) along i; column lines: N (2) and S (6) along j.
     const int sa = cols ? 2 : 4, sb = cols ? 6 : 0, t = cols ? j : i;
-    const size_t at = cols ? ((size_t)f * n + i) * n + j : (size_t)c;
+    const size_t at = (size_t)t * (6 * (size_t)n) + (size_t)f * n + (cols ? i : j);
     const size_t n6 = 6 * (size_t)nn;
+    auto bv = [&](uint32_t m) { return b[b_rows ? lod_row_at(m, n) : (size_t)m]; };
     const double ka = k[sa * n6 + c], kb = k[sb * n6 + c], m = mu[c];
     di[at] = 1.0 + (kb + ka) / m;
-    double rhs = b[c];
+    double rhs = bv(c);
     double l = 0.0, u = 0.0;
     if (t > 0)
         l = -ka / m;
     else
-        rhs += ka * b[slots[sa * n6 + c]] / m;
+        rhs += ka * bv(slots[sa * n6 + c]) / m;
     if (t + 1 < n)
         u = -kb / m;
     else
-        rhs += kb * b[slots[sb * n6 + c]] / m;
+        rhs += kb * bv(slots[sb * n6 + c]) / m;
     lo[at] = l, up[at] = u, v[at] = rhs;
 }
 
-// `thomas` (D:1029-1046) on each line of n values (line L at [L n, L n + n)), one thread a line:
-// v solved in place, cp the line's scratch.  The coefficients stream in independently of the
-// recurrence, so their loads run ahead of it.
-__global__ __launch_bounds__(kBlock) void kc_thomas(int nlines, int n, const double *__restrict__ lo,
-                                                   const double *__restrict__ di,
-                                                   const double *__restrict__ up,
-                                                   double *__restrict__ v,
-                                                   double *__restrict__ cp) {
-    const int line = blockIdx.x * kBlock + threadIdx.x;
-    if (line >= nlines) return;
-    const size_t o = (size_t)line * n;
-    double beta = di[o];
-    double prev = v[o] / beta;
-    v[o] = prev;
-#pragma unroll 4
-    for (int t = 1; t < n; t++) {
-        const double ct = up[o + t - 1] / beta;
-        cp[o + t] = ct;
-        const double lt = lo[o + t];
-        beta = di[o + t] - lt * ct;
-        prev = (v[o + t] - lt * prev) / beta;
-        v[o + t] = prev;
+// `thomas` (D:1029-1046) on each of nl line-interleaved lines of n values (position t of line L at
+// t nl + L), one thread a line: v solved in place, cp the lines' scratch.  The recurrence is one
+// long dependent chain per line and only nl / 64 waves run, so the loads are software-pipelined:
+// the next kThomasP positions' values are read while the current ones are used.
+constexpr int kThomasP = 16;
+
+__global__ __launch_bounds__(64) void kc_thomas(int nl, int n, const double *__restrict__ lo,
+                                               const double *__restrict__ di,
+                                               const double *__restrict__ up,
+                                               double *__restrict__ v, double *__restrict__ cp) {
+    constexpr int P = kThomasP;
+    const int line = blockIdx.x * 64 + threadIdx.x;
+    if (line >= nl) return;
+    const size_t stride = (size_t)nl;
+    const double *lo_ = lo + line, *di_ = di + line, *up_ = up + line;
+    double *v_ = v + line, *cp_ = cp + line;
+    double beta = di_[0];
+    double prev = v_[0] / beta;
+    v_[0] = prev;
+    {
+        // Forward: positions t = 1 .. n - 1 use up[t - 1], lo[t], di[t], v[t].
+        double ua[P], la[P], da[P], ra[P], ub[P], lb[P], db[P], rb[P];
+        auto load = [&](double *u_, double *l_, double *d_, double *r_, int t0) {
+#pragma unroll
+            for (int q = 0; q < P; q++) {
+                const int t = t0 + q;
+                if (t < n) {
+                    u_[q] = up_[(t - 1) * stride];
+                    l_[q] = lo_[t * stride];
+                    d_[q] = di_[t * stride];
+                    r_[q] = v_[t * stride];
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        };
+        auto step = [&](const double *u_, const double *l_, const double *d_, const double *r_,
+                        int t0) {
+#pragma unroll
+            for (int q = 0; q < P; q++) {
+                const int t = t0 + q;
+                if (t < n) {
+                    const double ct = u_[q] / beta;
+                    cp_[t * stride] = ct;
+                    beta = d_[q] - l_[q] * ct;
+                    prev = (r_[q] - l_[q] * prev) / beta;
+                    v_[t * stride] = prev;
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        };
+        int t0 = 1;
+        load(ua, la, da, ra, t0);
+        for (; t0 + P < n; t0 += 2 * P) {
+            load(ub, lb, db, rb, t0 + P);
+            step(ua, la, da, ra, t0);
+            load(ua, la, da, ra, t0 + 2 * P);
+            step(ub, lb, db, rb, t0 + P);
+        }
+        step(ua, la, da, ra, t0);
     }
-    double next = prev;
-#pragma unroll 4
-    for (int t = n - 2; t >= 0; t--) {
-        next = v[o + t] - cp[o + t + 1] * next;
-        v[o + t] = next;
+    {
+        // Backward: positions t = n - 2 .. 0 use v[t] and cp[t + 1].
+        double va[P], ca[P], vb[P], cb[P];
+        auto load = [&](double *v2, double *c2, int t0) {
+#pragma unroll
+            for (int q = 0; q < P; q++) {
+                const int t = t0 - q;
+                if (t >= 0) {
+                    v2[q] = v_[t * stride];
+                    c2[q] = cp_[(t + 1) * stride];
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        };
+        auto step = [&](const double *v2, const double *c2, int t0) {
+#pragma unroll
+            for (int q = 0; q < P; q++) {
+                const int t = t0 - q;
+                if (t >= 0) {
+                    prev = v2[q] - c2[q] * prev;
+                    v_[t * stride] = prev;
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        };
+        int t0 = n - 2;
+        load(va, ca, t0);
+        for (; t0 - P >= 0; t0 -= 2 * P) {
+            load(vb, cb, t0 - P);
+            step(va, ca, t0);
+            load(va, ca, t0 - 2 * P);
+            step(vb, cb, t0 - P);
+        }
+        step(va, ca, t0);
     }
 }
 
-// x[f, j, i] = tmp[f, i, j].
+// x[f, j, i] = the column lines' value at position j of line f n + i.
 __global__ __launch_bounds__(kBlock) void kc_lod_back(int n, const double *__restrict__ tmp,
                                                      double *__restrict__ x) {
     const int c = blockIdx.x * kBlock + threadIdx.x;
     const int nn = n * n;
     if (c >= 6 * nn) return;
     const int f = c / nn, rest = c - f * nn, j = rest / n, i = rest - j * n;
-    x[c] = tmp[(size_t)f * nn + (size_t)i * n + j];
+    x[c] = tmp[(size_t)j * (6 * (size_t)n) + (size_t)f * n + i];
 }
 
 // ---- ordered sums: one block a chunk of kCubeChunk cells ---------------------------------------
@@ -218,15 +293,39 @@ __global__ __launch_bounds__(kBlock) void kc_lod_back(int n, const double *__res
 // Thread 0 adds the chunk's terms t[0 .. cnt) in order from `init` (the reference's chunk sums
 // start from +0.0 in the CG passes and from -0.0 in `ordered_sum`).
 __device__ __forceinline__ double chunk_sum(const double *t, int cnt, double init) {
-    double s = init;
+    constexpr int D = 16;
+    double s = init, a[D], b[D];
     int q = 0;
-    // Eight terms read at a time (the reads do not wait on the sum), added one by one in order.
-    for (; q + 8 <= cnt; q += 8) {
-        double a[8];
+    // Software-pipelined over two register sets: the next D terms are read from LDS while the
+    // current D are added, one by one in order, so the reads' latency hides behind the dependent
+    // adds.  The scheduling barriers keep the compiler from sinking the reads below the adds.
+    auto load = [&](double *v, int at) {
 #pragma unroll
-        for (int u = 0; u < 8; u++) a[u] = t[q + u];
+        for (int u = 0; u < D; u++) v[u] = t[at + u];
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    auto add = [&](const double *v) {
 #pragma unroll
-        for (int u = 0; u < 8; u++) s += a[u];
+        for (int u = 0; u < D; u++) s += v[u];
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    if (cnt >= D) {
+        load(a, 0);
+        for (q = D; q + 2 * D <= cnt; q += 2 * D) {
+            load(b, q);
+            add(a);
+            load(a, q + D);
+            add(b);
+        }
+        // a holds terms [q - D, q), not yet added.
+        if (q + D <= cnt) {
+            load(b, q);
+            add(a);
+            add(b);
+            q += D;
+        } else {
+            add(a);
+        }
     }
     for (; q < cnt; q++) s += t[q];
     return s;
@@ -355,9 +454,8 @@ __global__ __launch_bounds__(kBlock) void kc_step(const double *__restrict__ par
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            for (int q = 0; q < cnt; q++) a += ta[q];
-            if (stride == 2)
-                for (int q = 0; q < cnt; q++) b += tb[q];
+            a = chunk_sum(ta, cnt, a);
+            if (stride == 2) b = chunk_sum(tb, cnt, b);
         }
         __syncthreads();
     }
@@ -434,10 +532,14 @@ __global__ void kc_init(CubeState *st, double tol2, int max_iter) {
 
 // ---- the cube's V-cycle (CM), f32 ------------------------------------------------------------
 
+// A level's couplings in ELL form, slot-major: entry q of cell c at q cells + c, the row's
+// entries first in the reference's order (its CSR order) and kNoCell after them, so a wave's
+// loads of one entry are contiguous.
 struct CLvl {
-    int cells, ne;
-    const uint32_t *ptr, *col;           // neighbour lists
-    const uint32_t *src_ptr, *src;       // per entry, the finer entries (level >= 1)
+    int cells, ne, w;                    // cells, CSR entries, ELL width
+    const uint32_t *col;                 // ELL neighbours
+    const uint32_t *ell_of;              // per CSR entry, its ELL position (level >= 1)
+    const uint32_t *src_ptr, *src;       // per CSR entry, the finer ELL positions (level >= 1)
     const uint32_t *kid_ptr, *kids;      // per cell, its finer block, ascending (level >= 1)
     const uint32_t *parent;              // per finer cell, its cell here (level >= 1)
     float *val, *mass, *diag, *r, *z, *t;
@@ -451,10 +553,12 @@ __global__ __launch_bounds__(kBlock) void kc_mg_finest(CLvl L, const uint32_t *_
                                                       const double *__restrict__ diag) {
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= L.cells) return;
-    uint32_t at = L.ptr[c];
+    size_t at = c;
     for (int q = 0; q < 8; q++)
-        if (slots[(size_t)q * L.cells + c] != kNoCell)
-            L.val[at++] = (float)k[(size_t)q * L.cells + c];
+        if (slots[(size_t)q * L.cells + c] != kNoCell) {
+            L.val[at] = (float)k[(size_t)q * L.cells + c];
+            at += L.cells;
+        }
     L.mass[c] = (float)mu[c];
     L.diag[c] = (float)diag[c];
 }
@@ -465,7 +569,7 @@ __global__ __launch_bounds__(kBlock) void kc_mg_vals(CLvl F, CLvl C, float scale
     if (e >= C.ne) return;
     float s = 0.0f;
     for (uint32_t q = C.src_ptr[e]; q < C.src_ptr[e + 1]; q++) s += F.val[C.src[q]];
-    C.val[e] = scale * s;
+    C.val[C.ell_of[e]] = scale * s;
 }
 
 // A coarse cell's mass (its block's, summed) and diagonal, mass + its couplings (CM:244-258).
@@ -474,7 +578,11 @@ __global__ __launch_bounds__(kBlock) void kc_mg_diag(CLvl F, CLvl C) {
     if (r >= C.cells) return;
     float s = 0.0f, k = 0.0f;
     for (uint32_t q = C.kid_ptr[r]; q < C.kid_ptr[r + 1]; q++) s += F.mass[C.kids[q]];
-    for (uint32_t e = C.ptr[r]; e < C.ptr[r + 1]; e++) k += C.val[e];
+    for (int q = 0; q < C.w; q++) {
+        const size_t e = (size_t)q * C.cells + r;
+        if (C.col[e] == kNoCell) break;
+        k += C.val[e];
+    }
     C.mass[r] = s;
     C.diag[r] = s + k;
 }
@@ -488,13 +596,30 @@ __global__ __launch_bounds__(kBlock) void kc_mg_r32(int n, const double *__restr
     if (i < n) r32[i] = (float)r[i];
 }
 
+// diag z_i - sum of the row's couplings times z, in the row's order.  W = 8 reads all eight
+// entries' columns at once (the rows are at most eight long); W = 0 walks a wider row.
+template <int W>
 __device__ __forceinline__ float cube_az(const CLvl &L, const float *z, int i) {
     float az = L.diag[i] * z[i];
-    for (uint32_t e = L.ptr[i]; e < L.ptr[i + 1]; e++) az -= L.val[e] * z[L.col[e]];
+    if constexpr (W > 0) {
+        uint32_t j[W];
+#pragma unroll
+        for (int q = 0; q < W; q++) j[q] = L.col[(size_t)q * L.cells + i];
+#pragma unroll
+        for (int q = 0; q < W; q++)
+            if (j[q] != kNoCell) az -= L.val[(size_t)q * L.cells + i] * z[j[q]];
+    } else {
+        for (int q = 0; q < L.w; q++) {
+            const uint32_t jq = L.col[(size_t)q * L.cells + i];
+            if (jq == kNoCell) break;
+            az -= L.val[(size_t)q * L.cells + i] * z[jq];
+        }
+    }
     return az;
 }
 
 // `sweep`, CM:285-310: z = omega r / diag from zero (first), else zout = z + omega (r - A z) / diag.
+template <int W>
 __global__ __launch_bounds__(kBlock) void kc_mg_sweep(CLvl L, const float *__restrict__ zin,
                                                      float *__restrict__ zout, float omega,
                                                      int first, const CubeState *__restrict__ st) {
@@ -504,16 +629,17 @@ __global__ __launch_bounds__(kBlock) void kc_mg_sweep(CLvl L, const float *__res
     if (first)
         zout[i] = omega * L.r[i] / L.diag[i];
     else
-        zout[i] = zin[i] + omega * (L.r[i] - cube_az(L, zin, i)) / L.diag[i];
+        zout[i] = zin[i] + omega * (L.r[i] - cube_az<W>(L, zin, i)) / L.diag[i];
 }
 
 // The residual r - A z of each cell (CM:328-341).
+template <int W>
 __global__ __launch_bounds__(kBlock) void kc_mg_resid(CLvl L, const float *__restrict__ z,
                                                      float *__restrict__ t,
                                                      const CubeState *__restrict__ st) {
     if (st->done) return;
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < L.cells) t[i] = L.r[i] - cube_az(L, z, i);
+    if (i < L.cells) t[i] = L.r[i] - cube_az<W>(L, z, i);
 }
 
 // The coarse r: each block's residuals summed, its cells ascending (CM:343-351).
@@ -669,19 +795,41 @@ static dtd_cube *cube_new(dtd_ctx *ctx, int n, const uint32_t *slots, const doub
         pats.push_back(cube_finest(n, slots));
         while (!(pats.back().cells <= (size_t)kCubeCoarsest || pats.back().side <= 1))
             pats.push_back(cube_coarsen(pats.back()));
-        for (const CPat &p : pats) {
+        // Each level's ELL form; a coarse entry's finer entries are named by their ELL position.
+        std::vector<std::vector<uint32_t>> ell_of(pats.size()), ecol(pats.size());
+        std::vector<int> width(pats.size());
+        for (size_t l = 0; l < pats.size(); l++) {
+            const CPat &p = pats[l];
+            int w = 1;
+            for (size_t c = 0; c < p.cells; c++) w = std::max(w, (int)(p.ptr[c + 1] - p.ptr[c]));
+            width[l] = w;
+            ecol[l].assign((size_t)w * p.cells, kNoCell);
+            ell_of[l].resize(p.col.size());
+            for (size_t c = 0; c < p.cells; c++)
+                for (uint32_t e = p.ptr[c]; e < p.ptr[c + 1]; e++) {
+                    const size_t at = (size_t)(e - p.ptr[c]) * p.cells + c;
+                    if (at > 0xFFFFFFFEu) throw Error{DTD_ETOOBIG, "the cube's levels exceed 2^32 entries"};
+                    ecol[l][at] = p.col[e];
+                    ell_of[l][e] = (uint32_t)at;
+                }
+        }
+        for (size_t l = 0; l < pats.size(); l++) {
+            const CPat &p = pats[l];
             CLvl L{};
-            L.cells = (int)p.cells, L.ne = (int)p.col.size();
-            L.ptr = (const uint32_t *)put(p.ptr.data(), p.ptr.size() * 4);
-            L.col = (const uint32_t *)put(p.col.data(), p.col.size() * 4);
+            L.cells = (int)p.cells, L.ne = (int)p.col.size(), L.w = width[l];
+            L.col = (const uint32_t *)put(ecol[l].data(), ecol[l].size() * 4);
             if (!p.parent.empty()) {
+                std::vector<uint32_t> src(p.src.size());
+                for (size_t q = 0; q < src.size(); q++) src[q] = ell_of[l - 1][p.src[q]];
+                L.ell_of = (const uint32_t *)put(ell_of[l].data(), ell_of[l].size() * 4);
                 L.src_ptr = (const uint32_t *)put(p.src_ptr.data(), p.src_ptr.size() * 4);
-                L.src = (const uint32_t *)put(p.src.data(), p.src.size() * 4);
+                L.src = (const uint32_t *)put(src.data(), src.size() * 4);
                 L.kid_ptr = (const uint32_t *)put(p.kid_ptr.data(), p.kid_ptr.size() * 4);
                 L.kids = (const uint32_t *)put(p.kids.data(), p.kids.size() * 4);
                 L.parent = (const uint32_t *)put(p.parent.data(), p.parent.size() * 4);
+                HIPCHK(hipStreamSynchronize(ctx->stream));  // src ends with this block
             }
-            L.val = (float *)own(p.col.size() * 4);
+            L.val = (float *)own(ecol[l].size() * 4);
             L.mass = (float *)own(p.cells * 4);
             L.diag = (float *)own(p.cells * 4);
             L.r = (float *)own(p.cells * 4);
@@ -718,8 +866,13 @@ static void cube_vcycle(dtd_ctx *ctx, dtd_cube *g, size_t l, std::vector<float *
     CLvl L = g->lv[l];
     L.z = zs[l], L.t = ts[l];
     const int nb = nblocks(L.cells);
+    const bool w8 = L.w <= 8;
     auto sweep = [&](bool first) {
-        kc_mg_sweep<<<nb, kBlock, 0, s>>>(L, zs[l], first ? zs[l] : ts[l], omega, first, g->st);
+        float *zout = first ? zs[l] : ts[l];
+        if (w8)
+            kc_mg_sweep<8><<<nb, kBlock, 0, s>>>(L, zs[l], zout, omega, first, g->st);
+        else
+            kc_mg_sweep<0><<<nb, kBlock, 0, s>>>(L, zs[l], zout, omega, first, g->st);
         LAUNCH_CHECK();
         if (!first) std::swap(zs[l], ts[l]);
     };
@@ -728,7 +881,10 @@ static void cube_vcycle(dtd_ctx *ctx, dtd_cube *g, size_t l, std::vector<float *
         return;
     }
     for (int k = 0; k < kCubeNu; k++) sweep(k == 0);
-    kc_mg_resid<<<nb, kBlock, 0, s>>>(L, zs[l], ts[l], g->st);
+    if (w8)
+        kc_mg_resid<8><<<nb, kBlock, 0, s>>>(L, zs[l], ts[l], g->st);
+    else
+        kc_mg_resid<0><<<nb, kBlock, 0, s>>>(L, zs[l], ts[l], g->st);
     LAUNCH_CHECK();
     const CLvl &C = g->lv[l + 1];
     kc_mg_restrict<<<nblocks(C.cells), kBlock, 0, s>>>(C, ts[l], g->st);
@@ -796,18 +952,18 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
     if (all) {
         // The first guess from b = h (CD:280-284).
         HIPCHK(hipMemcpyAsync(g->b, g->x, (size_t)n6 * 8, hipMemcpyDeviceToDevice, s));
-        const int nl = nblocks(6 * n);
+        const int nl = (6 * n + 63) / 64;
         // kpos is free when every cell takes part: it holds the lines' coefficients.
         double *lo = g->kpos, *di = lo + n6, *up = di + n6, *cp = up + n6;
-        kc_lod_coef<<<nblocks(n6), kBlock, 0, s>>>(n, 0, k, g->mu, g->slots, g->b, lo, di, up,
+        kc_lod_coef<<<nblocks(n6), kBlock, 0, s>>>(n, 0, 0, k, g->mu, g->slots, g->b, lo, di, up,
                                                    g->x);
         LAUNCH_CHECK();
-        kc_thomas<<<nl, kBlock, 0, s>>>(6 * n, n, lo, di, up, g->x, cp);
+        kc_thomas<<<nl, 64, 0, s>>>(6 * n, n, lo, di, up, g->x, cp);
         LAUNCH_CHECK();
-        kc_lod_coef<<<nblocks(n6), kBlock, 0, s>>>(n, 1, k, g->mu, g->slots, g->x, lo, di, up,
+        kc_lod_coef<<<nblocks(n6), kBlock, 0, s>>>(n, 1, 1, k, g->mu, g->slots, g->x, lo, di, up,
                                                    g->tmp);
         LAUNCH_CHECK();
-        kc_thomas<<<nl, kBlock, 0, s>>>(6 * n, n, lo, di, up, g->tmp, cp);
+        kc_thomas<<<nl, 64, 0, s>>>(6 * n, n, lo, di, up, g->tmp, cp);
         LAUNCH_CHECK();
         kc_lod_back<<<nblocks(n6), kBlock, 0, s>>>(n, g->tmp, g->x);
         LAUNCH_CHECK();

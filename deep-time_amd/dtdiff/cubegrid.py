@@ -17,8 +17,9 @@ The map from a face's local coordinates to the sphere is this module's own, the 
 gnomonic map p = normalize(n_f + tan(pi a / 4) e_f + tan(pi b / 4) s_f); the reference uses
 Moser's equal-area map, which the solve does not see.  Neighbours across a face's edge are found
 geometrically: a step across an edge lands on the next face's cell at the same place along the
-edge (the map's coordinate along an edge is the same from both faces).  The tests check the topology (mutual neighbours, seven neighbours at the
-24 corner cells, eight elsewhere) and the metric (areas summing to 4 pi R^2, symmetric couplings).
+edge (the map's coordinate along an edge is the same from both faces).  The tests check the
+topology (mutual neighbours, seven neighbours at the 24 corner cells, eight elsewhere) and the
+metric (areas summing to 4 pi R^2, symmetric couplings).
 """
 from __future__ import annotations
 
