@@ -17,10 +17,13 @@
 // Every value is the reference's expression in its order, with FP contraction off, so each pass
 // gives the reference's bits.  The sums are the reference's too: `ordered_sum` and the CG passes
 // add each chunk of 4096 cells in cell order and then the chunks in order (CUBE:48-58, CD:441-470),
-// and here one block takes one chunk, its threads form the terms into LDS and one thread adds them
-// in order.  The level sums (a coarse value over its finer entries, a block's residuals) run over
-// fixed lists in the reference's order.  So the iterations and the answer are the reference's.
+// and here each chunk's terms are added one by one in order by one 16-lane row of a wave, and the
+// chunk sums in order by one thread.  The level sums (a coarse value over its finer entries, a
+// block's residuals) run over fixed lists in the reference's order.  So the iterations and the
+// answer are the reference's.
 #pragma once
+
+#include <utility>
 
 #include "context.hpp"
 #include "kernels.hpp"
@@ -288,39 +291,44 @@ __global__ __launch_bounds__(kBlock) void kc_lod_back(int n, const double *__res
     x[c] = tmp[(size_t)j * (6 * (size_t)n) + (size_t)f * n + i];
 }
 
-// ---- ordered sums: one block a chunk of kCubeChunk cells ---------------------------------------
+// ---- ordered sums -------------------------------------------------------------------------
 
-// Thread 0 adds the chunk's terms t[0 .. cnt) in order from `init` (the reference's chunk sums
-// start from +0.0 in the CG passes and from -0.0 in `ordered_sum`).
+// One thread adds t[0 .. cnt) from LDS in order from `init` (the scalar steps' sums of the
+// chunk sums).
 __device__ __forceinline__ double chunk_sum(const double *t, int cnt, double init) {
     constexpr int D = 16;
     double s = init, a[D], b[D];
     int q = 0;
-    // Software-pipelined over two register sets: the next D terms are read from LDS while the
-    // current D are added, one by one in order, so the reads' latency hides behind the dependent
-    // adds.  The scheduling barriers keep the compiler from sinking the reads below the adds.
-    auto load = [&](double *v, int at) {
+    // Software-pipelined over two register sets: while the current D terms are added, one by one
+    // in order, the next D are read from LDS, the reads placed between the dependent adds (one
+    // 16-byte read after each pair of adds) so that they issue in the adds' latency.
+    auto stage = [&](double *next, int at, const double *cur) {
 #pragma unroll
-        for (int u = 0; u < D; u++) v[u] = t[at + u];
+        for (int u = 0; u < D; u++) next[u] = t[at + u];
+#pragma unroll
+        for (int u = 0; u < D; u++) s += cur[u];
+#pragma unroll
+        for (int u = 0; u < D / 2; u++) {
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
+            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // two VALU
+        }
         __builtin_amdgcn_sched_barrier(0);
     };
-    auto add = [&](const double *v) {
+    auto add = [&](const double *cur) {
 #pragma unroll
-        for (int u = 0; u < D; u++) s += v[u];
-        __builtin_amdgcn_sched_barrier(0);
+        for (int u = 0; u < D; u++) s += cur[u];
     };
     if (cnt >= D) {
-        load(a, 0);
+#pragma unroll
+        for (int u = 0; u < D; u++) a[u] = t[u];
+        __builtin_amdgcn_sched_barrier(0);
         for (q = D; q + 2 * D <= cnt; q += 2 * D) {
-            load(b, q);
-            add(a);
-            load(a, q + D);
-            add(b);
+            stage(b, q, a);
+            stage(a, q + D, b);
         }
         // a holds terms [q - D, q), not yet added.
         if (q + D <= cnt) {
-            load(b, q);
-            add(a);
+            stage(b, q, a);
             add(b);
             q += D;
         } else {
@@ -331,24 +339,145 @@ __device__ __forceinline__ double chunk_sum(const double *t, int cnt, double ini
     return s;
 }
 
-// parts[chunk] = sum of a[i] * b[i] (/ div[i]) over the chunk, from -0.0 (`ordered_sum`, with
-// b32 read as f64 when b is null).
-__global__ __launch_bounds__(kBlock) void kc_dot(int l, const double *__restrict__ a,
-                                                const double *__restrict__ b,
-                                                const float *__restrict__ b32,
-                                                const double *__restrict__ div,
-                                                double *__restrict__ parts,
-                                                const CubeState *__restrict__ st) {
-    __shared__ double t[kCubeChunk];
-    if (st && st->done) return;
-    const int c0 = blockIdx.x * kCubeChunk, cnt = min(kCubeChunk, l - c0);
-    for (int q = threadIdx.x; q < cnt; q += kBlock) {
-        const int i = c0 + q;
-        const double bv = b ? b[i] : (double)b32[i];
-        t[q] = div ? a[i] * bv / div[i] : a[i] * bv;
+// Each chunk's terms are added one by one in order from `init` (the reference's chunk sums start
+// from +0.0 in the CG passes and from -0.0 in `ordered_sum`): one dependent chain of f64 adds a
+// chunk.  A wave runs four chains, one per 16-lane row.  A row reads its chunk straight from
+// global memory, 32 terms a batch (lane k of the row holding terms 2k and 2k + 1), kRowP batches
+// ahead of the adds, and each term reaches the whole row by a DPP row broadcast
+// (row_newbcast:k), so that the chain costs a broadcast and an add a term and no LDS.
+
+// bc<K>(old, v): lane K's v across its row.  Every lane has a source lane, so `old` is never
+// read; it only orders the move after the instruction that produced `old`.
+template <int K>
+__device__ __forceinline__ double row_bcast(double old, double v) {
+    return __builtin_amdgcn_update_dpp(old, v, 0x150 + K, 0xF, 0xF, false);
+}
+
+struct RowChain {
+    double s1, s2;  // the running sum and its value one term earlier
+};
+
+template <int J>
+__device__ __forceinline__ double row_term(double old, double a, double b) {
+    return row_bcast<(J >> 1)>(old, (J & 1) ? b : a);
+}
+
+// Term J's add.  Before it: s1 = the sum through term J - 1, s2 = through J - 2 (no longer read),
+// x = term J.  The broadcast of term J + 1 is written before the add, into s2's register, so it
+// issues while the previous add completes.
+template <int J>
+__device__ __forceinline__ void row_step(RowChain &c, double &x, double a, double b) {
+    double xn = 0.0;
+    if constexpr (J < 31) xn = row_term<J + 1>(c.s2, a, b);
+    c.s2 = c.s1;
+    c.s1 = c.s1 + x;
+    x = xn;
+}
+
+template <int... J>
+__device__ __forceinline__ void row_add32(RowChain &c, double a, double b,
+                                          std::integer_sequence<int, J...>) {
+    double x = row_term<0>(c.s2, a, b);
+    (row_step<J>(c, x, a, b), ...);
+}
+
+// The chain of this lane's row over terms [0, cnt) of a chunk (cnt <= kCubeChunk, the same for
+// the row's 16 lanes), from `init`; the sum is in every lane of the row.  Src gives
+// `Raw load(i)` (the raw values behind terms i and i + 1, i even; only those below cnt are read)
+// and `double2 terms(const Raw &, i)`; terms at or past cnt are taken as -0.0, which leaves
+// every sum unchanged.
+template <int P, typename Src>
+__device__ __forceinline__ double row_chain(const Src &src, int cnt, double init) {
+    constexpr int NB = kCubeChunk / 32;
+    const int k2 = 2 * (threadIdx.x & 15);
+    typename Src::Raw buf[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) buf[p] = src.load(32 * p + k2, cnt);
+    RowChain c{init, init};
+    for (int b0 = 0; b0 < NB; b0 += P) {
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const typename Src::Raw raw = buf[p];
+            if (b0 + P + p < NB) buf[p] = src.load(32 * (b0 + P + p) + k2, cnt);
+            const int i = 32 * (b0 + p) + k2;
+            double2 x = src.terms(raw, i);
+            if (i >= cnt) x.x = -0.0;
+            if (i + 1 >= cnt) x.y = -0.0;
+            row_add32(c, x.x, x.y, std::make_integer_sequence<int, 32>{});
+        }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) parts[blockIdx.x] = chunk_sum(t, cnt, -0.0);
+    return c.s1;
+}
+
+// Two consecutive f64 values of p from i (i even), only those below cnt read.
+__device__ __forceinline__ double2 load_pair(const double *p, int i, int cnt) {
+    if (i + 1 < cnt) return *reinterpret_cast<const double2 *>(p + i);
+    return make_double2(i < cnt ? p[i] : 0.0, 0.0);
+}
+
+__device__ __forceinline__ float2 load_pair(const float *p, int i, int cnt) {
+    if (i + 1 < cnt) return *reinterpret_cast<const float2 *>(p + i);
+    return make_float2(i < cnt ? p[i] : 0.0f, 0.0f);
+}
+
+// The chunk-sum kernels: one wave a block, chain c of a kernel on row c % 4 of wave c / 4.
+constexpr int kRowBlock = 64, kRowChains = 4;
+
+struct RowChunk {
+    int chain, c0, cnt;  // this row's chain, its chunk's first cell and its term count (0: none)
+};
+
+__device__ __forceinline__ RowChunk row_chunk(int l, int per_chunk) {
+    RowChunk r;
+    r.chain = blockIdx.x * kRowChains + (threadIdx.x >> 4);
+    r.c0 = (r.chain / per_chunk) * kCubeChunk;
+    r.cnt = max(0, min(kCubeChunk, l - r.c0));
+    return r;
+}
+
+// parts[chunk] = sum of the chunk's terms from -0.0 (`ordered_sum`): a[i] * b[i] (kDotF64),
+// a[i] * (f64) b32[i] (kDotF32) or a[i] * b[i] / div[i] (kDotDiv).
+enum { kDotF64 = 0, kDotF32 = 1, kDotDiv = 2 };
+
+template <int MODE>
+struct DotSrc {
+    const double *a, *b, *div;
+    const float *b32;
+    struct Raw {
+        double2 a, b, d;
+        float2 b32;
+    };
+    __device__ __forceinline__ Raw load(int i, int cnt) const {
+        Raw r;
+        r.a = load_pair(a, i, cnt);
+        if constexpr (MODE == kDotF32) r.b32 = load_pair(b32, i, cnt);
+        else r.b = load_pair(b, i, cnt);
+        if constexpr (MODE == kDotDiv) r.d = load_pair(div, i, cnt);
+        return r;
+    }
+    __device__ __forceinline__ double2 terms(const Raw &r, int) const {
+        if constexpr (MODE == kDotF32)
+            return make_double2(r.a.x * (double)r.b32.x, r.a.y * (double)r.b32.y);
+        else if constexpr (MODE == kDotDiv)
+            return make_double2(r.a.x * r.b.x / r.d.x, r.a.y * r.b.y / r.d.y);
+        else
+            return make_double2(r.a.x * r.b.x, r.a.y * r.b.y);
+    }
+};
+
+template <int MODE>
+__global__ __launch_bounds__(kRowBlock) void kc_dot(int l, const double *__restrict__ a,
+                                                   const double *__restrict__ b,
+                                                   const float *__restrict__ b32,
+                                                   const double *__restrict__ div,
+                                                   double *__restrict__ parts,
+                                                   const CubeState *__restrict__ st) {
+    if (st && st->done) return;
+    const RowChunk r = row_chunk(l, 1);
+    const DotSrc<MODE> src{a + r.c0, b ? b + r.c0 : nullptr, div ? div + r.c0 : nullptr,
+                           b32 ? b32 + r.c0 : nullptr};
+    const double s = row_chain<MODE == kDotF32 ? 16 : 8>(src, r.cnt, -0.0);
+    if ((threadIdx.x & 15) == 0 && r.cnt > 0) parts[r.chain] = s;
 }
 
 // The direction pass (CD:411-440), one thread a cell: p = z + beta p_old (z = r / diag for the
@@ -385,29 +514,23 @@ __global__ __launch_bounds__(kBlock) void kc_dir(int l, int mg, const double *__
 }
 
 // parts[chunk] (and parts[2 chunk], parts[2 chunk + 1] with two arrays) = the chunk's terms
-// added in order from `init` by one thread after the block stages them in LDS.
-__global__ __launch_bounds__(kBlock) void kc_chunks(int l, const double *__restrict__ t0,
-                                                   const double *__restrict__ t1, double init,
-                                                   double *__restrict__ parts,
-                                                   const CubeState *__restrict__ st) {
-    __shared__ double t[2][kCubeChunk];
+// added in order from `init`.
+struct ArraySrc {
+    const double *p;
+    using Raw = double2;
+    __device__ __forceinline__ Raw load(int i, int cnt) const { return load_pair(p, i, cnt); }
+    __device__ __forceinline__ double2 terms(const Raw &r, int) const { return r; }
+};
+
+__global__ __launch_bounds__(kRowBlock) void kc_chunks(int l, const double *__restrict__ t0,
+                                                      const double *__restrict__ t1, double init,
+                                                      double *__restrict__ parts,
+                                                      const CubeState *__restrict__ st) {
     if (st && st->done) return;
-    const int c0 = blockIdx.x * kCubeChunk, cnt = min(kCubeChunk, l - c0);
-    for (int q = threadIdx.x; q < cnt; q += kBlock) {
-        t[0][q] = t0[c0 + q];
-        if (t1) t[1][q] = t1[c0 + q];
-    }
-    __syncthreads();
-    // The two sums run side by side, each on one thread of its own wave.
-    if (threadIdx.x == 0) {
-        const double a = chunk_sum(t[0], cnt, init);
-        if (t1)
-            parts[2 * blockIdx.x] = a;
-        else
-            parts[blockIdx.x] = a;
-    } else if (t1 && threadIdx.x == 64) {
-        parts[2 * blockIdx.x + 1] = chunk_sum(t[1], cnt, init);
-    }
+    const RowChunk r = row_chunk(l, t1 ? 2 : 1);
+    const double *src = (t1 && (r.chain & 1)) ? t1 : t0;
+    const double s = row_chain<16>(ArraySrc{src + r.c0}, r.cnt, init);
+    if ((threadIdx.x & 15) == 0 && r.cnt > 0) parts[r.chain] = s;
 }
 
 // The update pass (CD:449-470), one thread a cell: x += alpha p, r -= alpha A p, and the terms
@@ -448,9 +571,23 @@ __global__ __launch_bounds__(kBlock) void kc_step(const double *__restrict__ par
     double a = -0.0, b = -0.0;
     for (int q0 = 0; q0 < np; q0 += kStepTile) {
         const int cnt = min(kStepTile, np - q0);
-        for (int q = threadIdx.x; q < cnt; q += kBlock) {
-            ta[q] = parts[stride * (q0 + q)];
-            if (stride == 2) tb[q] = parts[2 * (q0 + q) + 1];
+        constexpr int per = kStepTile / kBlock;
+        double va[per], vb[per];
+#pragma unroll
+        for (int u = 0; u < per; u++) {
+            const int q = (int)threadIdx.x + u * kBlock;
+            if (q < cnt) {
+                va[u] = parts[stride * (q0 + q)];
+                if (stride == 2) vb[u] = parts[2 * (q0 + q) + 1];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < per; u++) {
+            const int q = (int)threadIdx.x + u * kBlock;
+            if (q < cnt) {
+                ta[q] = va[u];
+                if (stride == 2) tb[q] = vb[u];
+            }
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -491,22 +628,33 @@ __global__ __launch_bounds__(kBlock) void kc_step(const double *__restrict__ par
     if (flag) *reinterpret_cast<volatile int *>(flag) = st->done;
 }
 
-// The put-back's sums (CD:289-292): per chunk, sum (rhs - A x) and sum s, each from -0.0.
-__global__ __launch_bounds__(kBlock) void kc_putback_sums(int l, const double *__restrict__ rhs,
-                                                         const double *__restrict__ ax,
-                                                         const double *__restrict__ s,
-                                                         double *__restrict__ parts) {
-    __shared__ double t[2][kCubeChunk];
-    const int c0 = blockIdx.x * kCubeChunk, cnt = min(kCubeChunk, l - c0);
-    for (int q = threadIdx.x; q < cnt; q += kBlock) {
-        t[0][q] = rhs[c0 + q] - ax[c0 + q];
-        t[1][q] = s[c0 + q];
+// The put-back's sums (CD:289-292): per chunk, sum (rhs - A x) into parts[2 chunk] and sum s
+// into parts[2 chunk + 1], each from -0.0.
+struct DiffSrc {
+    const double *u, *v;  // terms u - v (v null: u)
+    struct Raw {
+        double2 u, v;
+    };
+    __device__ __forceinline__ Raw load(int i, int cnt) const {
+        Raw r;
+        r.u = load_pair(u, i, cnt);
+        if (v) r.v = load_pair(v, i, cnt);
+        return r;
     }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        parts[2 * blockIdx.x] = chunk_sum(t[0], cnt, -0.0);
-    else if (threadIdx.x == 64)
-        parts[2 * blockIdx.x + 1] = chunk_sum(t[1], cnt, -0.0);
+    __device__ __forceinline__ double2 terms(const Raw &r, int) const {
+        return v ? make_double2(r.u.x - r.v.x, r.u.y - r.v.y) : r.u;
+    }
+};
+
+__global__ __launch_bounds__(kRowBlock) void kc_putback_sums(int l, const double *__restrict__ rhs,
+                                                            const double *__restrict__ ax,
+                                                            const double *__restrict__ s,
+                                                            double *__restrict__ parts) {
+    const RowChunk r = row_chunk(l, 2);
+    const DiffSrc src = (r.chain & 1) ? DiffSrc{s + r.c0, nullptr}
+                                      : DiffSrc{rhs + r.c0, ax + r.c0};
+    const double sum = row_chain<8>(src, r.cnt, -0.0);
+    if ((threadIdx.x & 15) == 0 && r.cnt > 0) parts[r.chain] = sum;
 }
 
 // x += (lost / weight) s / mu when weight > 0 and lost != 0 (CD:293-298), then h[cells[i]] = x.
@@ -663,6 +811,77 @@ __global__ __launch_bounds__(kBlock) void kc_mg_prolong(int cells, float *__rest
     if (c < cells) z[c] += zc[parent[c]];
 }
 
+// ---- the cube's coarse levels in one block ----------------------------------------------------
+
+// The levels of at most kCubeBottomCells cells in all are cycled by one block of
+// kCubeBottomThreads, with each level's z, its second buffer and r in LDS: the same per-cell
+// arithmetic as the per-level kernels, with block barriers in place of launches.
+constexpr int kCubeBottomLevels = 8, kCubeBottomThreads = 1024;
+
+struct CBottom {
+    int nlev;
+    int off[kCubeBottomLevels];  // each level's first cell in the LDS arrays
+    CLvl lv[kCubeBottomLevels];
+};
+
+template <int W>
+__global__ __launch_bounds__(kCubeBottomThreads) void kc_mg_bottom(
+    CBottom B, float *__restrict__ ztop, float omega, const CubeState *__restrict__ st) {
+    extern __shared__ float cb_mem[];
+    if (st->done) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    unsigned cur = 0;  // bit l: which of level l's two buffers holds z
+    auto zbuf = [&](int l, unsigned which) {
+        return cb_mem + 3 * B.off[l] + (which ? B.lv[l].cells : 0);
+    };
+    auto rbuf = [&](int l) { return cb_mem + 3 * B.off[l] + 2 * B.lv[l].cells; };
+    auto bit = [&](int l) { return (cur >> l) & 1u; };
+    for (int i = tid; i < B.lv[0].cells; i += nt) rbuf(0)[i] = B.lv[0].r[i];
+    __syncthreads();
+    // `sweep`, CM:285-310.
+    auto sweep = [&](int l, bool first) {
+        const CLvl &L = B.lv[l];
+        const float *zi = zbuf(l, bit(l)), *r = rbuf(l);
+        float *zo = zbuf(l, first ? bit(l) : bit(l) ^ 1u);
+        for (int i = tid; i < L.cells; i += nt) {
+            if (first)
+                zo[i] = omega * r[i] / L.diag[i];
+            else
+                zo[i] = zi[i] + omega * (r[i] - cube_az<W>(L, zi, i)) / L.diag[i];
+        }
+        __syncthreads();
+        if (!first) cur ^= 1u << l;
+    };
+    for (int l = 0; l + 1 < B.nlev; l++) {
+        const CLvl &L = B.lv[l], &C = B.lv[l + 1];
+        for (int k = 0; k < kCubeNu; k++) sweep(l, k == 0);
+        // The residual into the second buffer (CM:328-341), then the coarse r (CM:343-351).
+        const float *z = zbuf(l, bit(l)), *r = rbuf(l);
+        float *t = zbuf(l, bit(l) ^ 1u);
+        for (int i = tid; i < L.cells; i += nt) t[i] = r[i] - cube_az<W>(L, z, i);
+        __syncthreads();
+        float *rc = rbuf(l + 1);
+        for (int c = tid; c < C.cells; c += nt) {
+            float s = 0.0f;
+            for (uint32_t q = C.kid_ptr[c]; q < C.kid_ptr[c + 1]; q++) s += t[C.kids[q]];
+            rc[c] = s;
+        }
+        __syncthreads();
+    }
+    for (int k = 0; k < kCubeCoarseSweeps; k++) sweep(B.nlev - 1, k == 0);
+    for (int l = B.nlev - 2; l >= 0; l--) {
+        // z += the coarse correction of the cell's block (CM:353-358), then the post-sweeps.
+        float *z = zbuf(l, bit(l));
+        const float *zc = zbuf(l + 1, bit(l + 1));
+        const uint32_t *parent = B.lv[l + 1].parent;
+        for (int i = tid; i < B.lv[l].cells; i += nt) z[i] += zc[parent[i]];
+        __syncthreads();
+        for (int k = 0; k < kCubeNu; k++) sweep(l, false);
+    }
+    const float *z = zbuf(0, bit(0));
+    for (int i = tid; i < B.lv[0].cells; i += nt) ztop[i] = z[i];
+}
+
 // ---- host side -----------------------------------------------------------------------------
 
 }  // namespace dtd
@@ -676,6 +895,11 @@ struct dtd_cube {
     double *lap = nullptr, *area = nullptr;
     // The V-cycle's levels (patterns on the device; values filled per solve).
     std::vector<dtd::CLvl> lv;
+    // Levels bottom .. end are cycled by kc_mg_bottom (bottom == lv.size(): none).
+    size_t bottom = 0;
+    int bottom_w8 = 1;
+    size_t bottom_lds = 0;
+    dtd::CBottom cb{};
     // Per-solve buffers (n6 cells).
     double *kfull = nullptr, *kpos = nullptr, *mu = nullptr, *s = nullptr, *diag = nullptr;
     double *rhs = nullptr, *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr;
@@ -688,6 +912,16 @@ struct dtd_cube {
 };
 
 namespace dtd {
+
+// The cube's levels of at most this many cells in all are cycled by one block, kc_mg_bottom
+// (DTD_CUBE_BOTTOM_CELLS overrides it; 0 turns it off).
+static int cube_bottom_cells() {
+    static const int v = [] {
+        const char *e = getenv("DTD_CUBE_BOTTOM_CELLS");
+        return e ? atoi(e) : 2048;
+    }();
+    return v;
+}
 
 // One level's pattern on the host (CM:29-45, 65-165).
 struct CPat {
@@ -808,7 +1042,8 @@ static dtd_cube *cube_new(dtd_ctx *ctx, int n, const uint32_t *slots, const doub
             for (size_t c = 0; c < p.cells; c++)
                 for (uint32_t e = p.ptr[c]; e < p.ptr[c + 1]; e++) {
                     const size_t at = (size_t)(e - p.ptr[c]) * p.cells + c;
-                    if (at > 0xFFFFFFFEu) throw Error{DTD_ETOOBIG, "the cube's levels exceed 2^32 entries"};
+                    if (at > 0xFFFFFFFEu)
+                        throw Error{DTD_ETOOBIG, "the cube's levels exceed 2^32 entries"};
                     ecol[l][at] = p.col[e];
                     ell_of[l][e] = (uint32_t)at;
                 }
@@ -836,6 +1071,28 @@ static dtd_cube *cube_new(dtd_ctx *ctx, int n, const uint32_t *slots, const doub
             L.z = (float *)own(p.cells * 4);
             L.t = (float *)own(p.cells * 4);
             g->lv.push_back(L);
+        }
+        // The bottom levels: from the coarsest up while they hold at most cube_bottom_cells().
+        g->bottom = g->lv.size();
+        size_t tot = 0;
+        for (size_t l = g->lv.size(); l-- > 0;) {
+            tot += (size_t)g->lv[l].cells;
+            if (tot > (size_t)cube_bottom_cells() || 3 * tot * sizeof(float) > 65536 ||
+                g->lv.size() - l > (size_t)kCubeBottomLevels)
+                break;
+            g->bottom = l;
+        }
+        if (g->bottom < g->lv.size()) {
+            CBottom &B = g->cb;
+            B.nlev = (int)(g->lv.size() - g->bottom);
+            int off = 0;
+            for (int q = 0; q < B.nlev; q++) {
+                B.lv[q] = g->lv[g->bottom + q];
+                B.off[q] = off;
+                off += B.lv[q].cells;
+                if (B.lv[q].w > 8) g->bottom_w8 = 0;
+            }
+            g->bottom_lds = 3 * (size_t)off * sizeof(float);
         }
         for (double **v : {&g->kfull, &g->kpos})
             *v = (double *)own(8 * n6 * 8);
@@ -866,6 +1123,15 @@ static void cube_vcycle(dtd_ctx *ctx, dtd_cube *g, size_t l, std::vector<float *
     CLvl L = g->lv[l];
     L.z = zs[l], L.t = ts[l];
     const int nb = nblocks(L.cells);
+    if (l == g->bottom) {
+        const size_t lds = g->bottom_lds;
+        if (g->bottom_w8)
+            kc_mg_bottom<8><<<1, kCubeBottomThreads, lds, s>>>(g->cb, zs[l], omega, g->st);
+        else
+            kc_mg_bottom<0><<<1, kCubeBottomThreads, lds, s>>>(g->cb, zs[l], omega, g->st);
+        LAUNCH_CHECK();
+        return;
+    }
     const bool w8 = L.w <= 8;
     auto sweep = [&](bool first) {
         float *zout = first ? zs[l] : ts[l];
@@ -946,6 +1212,9 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
     const uint32_t *cols = all ? g->slots : g->cols;
     double *k = all ? g->kfull : g->kpos;
     const int nb = nblocks(l), nc = (l + kCubeChunk - 1) / kCubeChunk;
+    // The chunk-sum kernels' grids: four chains a wave (one or two chains a chunk).
+    const int bc = (nc + kRowChains - 1) / kRowChains;
+    const int bc2 = (2 * nc + kRowChains - 1) / kRowChains;
     kc_opinit<<<nb, kBlock, 0, s>>>(l, n6, cells, g->kfull, k, g->area, g->mean_area, h, g->mu,
                                     g->s, g->diag, g->rhs, g->x);
     LAUNCH_CHECK();
@@ -985,16 +1254,19 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
     LAUNCH_CHECK();
     kc_apply<<<nb, kBlock, 0, s>>>(l, g->diag, k, cols, g->x, g->r, g->rhs);
     LAUNCH_CHECK();
-    kc_dot<<<nc, kBlock, 0, s>>>(l, g->r, g->r, nullptr, nullptr, g->parts, nullptr);
+    kc_dot<kDotF64><<<bc, kRowBlock, 0, s>>>(l, g->r, g->r, nullptr, nullptr, g->parts,
+                                             nullptr);
     LAUNCH_CHECK();
     kc_step<kCubeRR0><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
     LAUNCH_CHECK();
     const float *z32 = nullptr;
     if (mg) {
         z32 = cube_precondition(ctx, g, g->r);
-        kc_dot<<<nc, kBlock, 0, s>>>(l, g->r, nullptr, z32, nullptr, g->parts, g->st);
+        kc_dot<kDotF32><<<bc, kRowBlock, 0, s>>>(l, g->r, nullptr, z32, nullptr, g->parts,
+                                                 g->st);
     } else {
-        kc_dot<<<nc, kBlock, 0, s>>>(l, g->r, g->r, nullptr, g->diag, g->parts, g->st);
+        kc_dot<kDotDiv><<<bc, kRowBlock, 0, s>>>(l, g->r, g->r, nullptr, g->diag, g->parts,
+                                                 g->st);
     }
     LAUNCH_CHECK();
     kc_step<kCubeRZ0><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
@@ -1008,20 +1280,21 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
         kc_dir<<<nb, kBlock, 0, s>>>(l, mg, g->r, z32, g->diag, k, cols, po, pn, g->ap, g->tmp,
                                      g->st);
         LAUNCH_CHECK();
-        kc_chunks<<<nc, kBlock, 0, s>>>(l, g->tmp, nullptr, 0.0, g->parts, g->st);
+        kc_chunks<<<bc, kRowBlock, 0, s>>>(l, g->tmp, nullptr, 0.0, g->parts, g->st);
         LAUNCH_CHECK();
         std::swap(po, pn);
         kc_step<kCubePAP><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
         LAUNCH_CHECK();
         kc_update<<<nb, kBlock, 0, s>>>(l, po, g->ap, g->diag, g->x, g->r, g->tmp, g->b, g->st);
         LAUNCH_CHECK();
-        kc_chunks<<<nc, kBlock, 0, s>>>(l, g->tmp, g->b, 0.0, g->parts, g->st);
+        kc_chunks<<<bc2, kRowBlock, 0, s>>>(l, g->tmp, g->b, 0.0, g->parts, g->st);
         LAUNCH_CHECK();
         if (mg) {
             kc_step<kCubeRR><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
             LAUNCH_CHECK();
             z32 = cube_precondition(ctx, g, g->r);
-            kc_dot<<<nc, kBlock, 0, s>>>(l, g->r, nullptr, z32, nullptr, g->parts, g->st);
+            kc_dot<kDotF32><<<bc, kRowBlock, 0, s>>>(l, g->r, nullptr, z32, nullptr, g->parts,
+                                                     g->st);
             LAUNCH_CHECK();
             kc_step<kCubeRZ><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, ctx->dev_flag + par);
         } else {
@@ -1037,7 +1310,7 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
     // The put-back (CD:286-301) and the answer into h.
     kc_apply<<<nb, kBlock, 0, s>>>(l, g->diag, k, cols, g->x, g->ap, nullptr);
     LAUNCH_CHECK();
-    kc_putback_sums<<<nc, kBlock, 0, s>>>(l, g->rhs, g->ap, g->s, g->parts);
+    kc_putback_sums<<<bc2, kRowBlock, 0, s>>>(l, g->rhs, g->ap, g->s, g->parts);
     LAUNCH_CHECK();
     kc_step<kCubePutback><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
     LAUNCH_CHECK();
