@@ -625,6 +625,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_kd(long n, const double *__rest
 #include "jacobi.hpp"
 #include "failure.hpp"
 #include "cube.hpp"
+#include "nn.hpp"
 
 using namespace dtd;
 
@@ -1191,6 +1192,241 @@ int dtd_cycle_precondition(dtd_ctx *ctx, const float *r32, const double *r64, fl
         }
         download(ctx, ctx->host_pair, ws.pair, sizeof(double));
         if (rz) *rz = ctx->host_pair[0];
+    });
+}
+
+// ---- the learned first guess (no reference counterpart) ------------------------------------
+
+int dtd_nn_conv3x3(dtd_ctx *ctx, int n, int h, int w, int cin, int cout, const uint16_t *x,
+                   const uint16_t *wt, const float *bias, int relu, uint16_t *y, float *y32,
+                   int y32c) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        nn::check_shape(n, h, w), nn::check_channels(cin), nn::check_channels(cout);
+        if (!x || !wt || !y) throw Error{DTD_EINVAL, "null array"};
+        if (y32 && (y32c < 1 || y32c > cout)) throw Error{DTD_EINVAL, "y32c out of range"};
+        nn::conv(ctx->stream, nn::Shape{n, h, w}, cin, cout, x, wt, bias, relu, y, y32, y32c);
+    });
+}
+
+int dtd_nn_conv3x3_bwd_input(dtd_ctx *ctx, int n, int h, int w, int cin, int cout,
+                             const uint16_t *dz, const uint16_t *wt, uint16_t *wt_scratch,
+                             uint16_t *dx) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        nn::check_shape(n, h, w), nn::check_channels(cin), nn::check_channels(cout);
+        if (!dz || !wt || !wt_scratch || !dx) throw Error{DTD_EINVAL, "null array"};
+        const long nw = (long)cin * 9 * cout;
+        nn::k_conv_flip<<<nn::blocks(nw), 256, 0, ctx->stream>>>(cin, cout, wt, wt_scratch);
+        LAUNCH_CHECK();
+        nn::conv(ctx->stream, nn::Shape{n, h, w}, cout, cin, dz, wt_scratch, nullptr, 0, dx,
+                nullptr, 0);
+    });
+}
+
+size_t dtd_nn_bwd_weight_scratch(int n, int h, int w, int cin, int cout, size_t *f32_elems) {
+    const long np = (long)n * h * w;
+    const long chunks = (np + nn::kChunk - 1) / nn::kChunk;
+    if (f32_elems) *f32_elems = (size_t)chunks * cout * 9 * cin;
+    return (size_t)(cin + cout) * np;
+}
+
+int dtd_nn_conv3x3_bwd_weight(dtd_ctx *ctx, int n, int h, int w, int cin, int cout,
+                              const uint16_t *x, const uint16_t *dz, uint16_t *planes,
+                              size_t planes_elems, float *part, size_t part_elems, float *dw,
+                              float *db) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        nn::check_shape(n, h, w), nn::check_channels(cin), nn::check_channels(cout);
+        if (!x || !dz || !planes || !part || !dw) throw Error{DTD_EINVAL, "null array"};
+        size_t need_part = 0;
+        const size_t need_planes = dtd_nn_bwd_weight_scratch(n, h, w, cin, cout, &need_part);
+        if (planes_elems < need_planes || part_elems < need_part)
+            throw Error{DTD_EINVAL, "scratch too small"};
+        hipStream_t s = ctx->stream;
+        const long np = (long)n * h * w;
+        uint16_t *xt = planes, *dzt = planes + (size_t)cin * np;
+        nn::k_to_planes<<<dim3((unsigned)((np + 63) / 64), (unsigned)(cin / 16)), 256, 0, s>>>(
+            np, cin, x, xt);
+        LAUNCH_CHECK();
+        nn::k_to_planes<<<dim3((unsigned)((np + 63) / 64), (unsigned)(cout / 16)), 256, 0, s>>>(
+            np, cout, dz, dzt);
+        LAUNCH_CHECK();
+        const int chunks = (int)((np + nn::kChunk - 1) / nn::kChunk);
+        const int combos = (cout / 16) * (cin / 16) * 9;
+        nn::k_conv_dw<<<dim3((unsigned)chunks, (unsigned)((combos + 3) / 4)), 256, 0, s>>>(
+            nn::Shape{n, h, w}, cin, cout, xt, dzt, nn::kChunk, part);
+        LAUNCH_CHECK();
+        const long nw = (long)cout * 9 * cin;
+        nn::k_dw_reduce<<<nn::blocks(nw), 256, 0, s>>>(nw, chunks, part, dw);
+        LAUNCH_CHECK();
+        if (db) {
+            nn::k_db<<<(unsigned)cout, 256, 0, s>>>(np, dzt, db);
+            LAUNCH_CHECK();
+        }
+    });
+}
+
+int dtd_nn_relu_bwd(dtd_ctx *ctx, size_t n, const uint16_t *y, const uint16_t *dy, uint16_t *dz) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (!y || !dy || !dz) throw Error{DTD_EINVAL, "null array"};
+        if (n == 0) return;
+        nn::k_relu_bwd<<<nn::blocks((long)n), 256, 0, ctx->stream>>>((long)n, y, dy, dz);
+        LAUNCH_CHECK();
+    });
+}
+
+int dtd_nn_pool(dtd_ctx *ctx, int n, int h, int w, int c, const uint16_t *x, uint16_t *y) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        nn::check_shape(n, h, w), nn::check_channels(c);
+        if (h % 2 || w % 2) throw Error{DTD_EINVAL, "pooling needs even sizes"};
+        if (!x || !y) throw Error{DTD_EINVAL, "null array"};
+        nn::k_pool<<<nn::blocks((long)n * (h / 2) * (w / 2) * c), 256, 0, ctx->stream>>>(
+            nn::Shape{n, h, w}, c, x, y);
+        LAUNCH_CHECK();
+    });
+}
+
+int dtd_nn_up_cat(dtd_ctx *ctx, int n, int h, int w, int cu, int cs, const uint16_t *u,
+                  const uint16_t *skip, uint16_t *cat) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        nn::check_shape(n, h, w), nn::check_channels(cu), nn::check_channels(cs);
+        if (h % 2 || w % 2) throw Error{DTD_EINVAL, "upsampling needs even sizes"};
+        if (!u || !skip || !cat) throw Error{DTD_EINVAL, "null array"};
+        nn::k_up_cat<<<nn::blocks((long)n * h * w * (cu + cs)), 256, 0, ctx->stream>>>(
+            nn::Shape{n, h, w}, cu, cs, u, skip, cat);
+        LAUNCH_CHECK();
+    });
+}
+
+int dtd_nn_up_cat_bwd(dtd_ctx *ctx, int n, int h, int w, int cu, int cs, const uint16_t *dcat,
+                      const uint16_t *dpool, uint16_t *du, uint16_t *dskip) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        nn::check_shape(n, h, w), nn::check_channels(cu), nn::check_channels(cs);
+        if (h % 2 || w % 2) throw Error{DTD_EINVAL, "upsampling needs even sizes"};
+        if (!dcat) throw Error{DTD_EINVAL, "null array"};
+        const nn::Shape sh{n, h, w};
+        if (du) {
+            nn::k_up_cat_bwd_u<<<nn::blocks((long)n * (h / 2) * (w / 2) * cu), 256, 0,
+                                 ctx->stream>>>(sh, cu, cs, dcat, du);
+            LAUNCH_CHECK();
+        }
+        if (dskip) {
+            nn::k_skip_bwd<<<nn::blocks((long)n * h * w * cs), 256, 0, ctx->stream>>>(
+                sh, cu, cs, dcat, dpool, dskip);
+            LAUNCH_CHECK();
+        }
+    });
+}
+
+int dtd_nn_input(dtd_ctx *ctx, size_t np, const double *h, const double *kd, double dt,
+                 double dx, double scale, uint16_t *x) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (!h || !kd || !x) throw Error{DTD_EINVAL, "null array"};
+        if (!(scale > 0.0) || !(dt > 0.0) || !(dx > 0.0))
+            throw Error{DTD_EINVAL, "scale, dt and dx must be positive"};
+        if (np == 0) return;
+        nn::k_nn_input<<<nn::blocks((long)np), 256, 0, ctx->stream>>>((long)np, h, kd, dt, dx * dx,
+                                                                     scale, x);
+        LAUNCH_CHECK();
+    });
+}
+
+int dtd_nn_loss(dtd_ctx *ctx, size_t np, const uint16_t *x, const float *out, int outc,
+                const double *t, double scale, float *parts, uint16_t *dy, double *loss) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (!x || !out || !t || !parts || !dy || !loss) throw Error{DTD_EINVAL, "null array"};
+        if (np == 0 || outc < 1) throw Error{DTD_EINVAL, "empty loss"};
+        const unsigned nb = nn::blocks((long)np);
+        nn::k_nn_loss<<<nb, 256, 0, ctx->stream>>>((long)np, x, out, outc, t, scale, parts, dy);
+        LAUNCH_CHECK();
+        std::vector<float> h(nb);
+        download(ctx, h.data(), parts, nb * sizeof(float));
+        double s = 0.0;
+        for (float v : h) s += v;
+        *loss = s / (double)np;
+    });
+}
+
+int dtd_nn_guess(dtd_ctx *ctx, size_t np, const double *h, const float *out, int outc,
+                 double scale, double *g) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (!h || !out || !g) throw Error{DTD_EINVAL, "null array"};
+        if (np == 0) return;
+        nn::k_nn_guess<<<nn::blocks((long)np), 256, 0, ctx->stream>>>((long)np, h, out, outc,
+                                                                     scale, g);
+        LAUNCH_CHECK();
+    });
+}
+
+int dtd_nn_adam(dtd_ctx *ctx, size_t n, float *w, float *m, float *v, const float *g, float lr,
+                float beta1, float beta2, float eps, int step, uint16_t *wb) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (!w || !m || !v || !g) throw Error{DTD_EINVAL, "null array"};
+        if (step < 1) throw Error{DTD_EINVAL, "step counts from 1"};
+        if (n == 0) return;
+        const float bc1 = 1.0f - std::pow(beta1, (float)step);
+        const float bc2 = 1.0f - std::pow(beta2, (float)step);
+        nn::k_adam<<<nn::blocks((long)n), 256, 0, ctx->stream>>>((long)n, w, m, v, g, lr, beta1,
+                                                                beta2, eps, bc1, bc2, wb);
+        LAUNCH_CHECK();
+    });
+}
+
+int dtd_nn_to_bf16(dtd_ctx *ctx, size_t n, const float *a, uint16_t *b) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (!a || !b) throw Error{DTD_EINVAL, "null array"};
+        if (n == 0) return;
+        nn::k_to_bf16<<<nn::blocks((long)n), 256, 0, ctx->stream>>>((long)n, a, b);
+        LAUNCH_CHECK();
+    });
+}
+
+int dtd_sync(dtd_ctx *ctx) {
+    return guard(ctx, [&] { HIPCHK(hipStreamSynchronize(ctx->stream)); });
+}
+
+int dtd_solve_rel_guess_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x,
+                               double *h_dev, const double *kd_dev, double dt, double rel_tol,
+                               const double *x0_dev, dtd_stats *stats) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        dtd_stats local{};
+        dtd_stats *st = stats ? stats : &local;
+        *st = dtd_stats{};
+        if (!(std::isfinite(rel_tol) && rel_tol > 0.0))
+            throw Error{DTD_EINVAL, "rel_tol must be finite and positive"};
+        check_grid(w, h);
+        const size_t n = w * h;
+        if (n == 0) return;
+        if (!h_dev || !kd_dev || !x0_dev) throw Error{DTD_EINVAL, "null array"};
+        fit(ctx, (int)w, (int)h, wrap_x && w > 1);
+        Workspace &ws = ctx->ws;
+        HIPCHK(hipMemcpyAsync(ws.x, x0_dev, n * sizeof(double), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+        double rel = 0.0, bnorm = 0.0;
+        solve_core(ctx, h_dev, kd_dev, dx, dt, wrap_x, 0.0, rel_tol * rel_tol, true, kGuessKeep,
+                   st, &rel, &bnorm);
+        double tol_m = rel_tol * bnorm / std::sqrt((double)n);
+        while (rel > rel_tol && st->restarts < 4 && bnorm > 0.0) {
+            tol_m *= 0.5 * rel_tol / rel;
+            st->restarts++;
+            solve_core(ctx, h_dev, kd_dev, dx, dt, wrap_x, tol_m, 0.0, false, kGuessKeep, st,
+                       &rel, &bnorm);
+        }
+        st->rel_residual = rel;
+        HIPCHK(hipMemcpyAsync(h_dev, ws.x, n * sizeof(double), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
     });
 }
 

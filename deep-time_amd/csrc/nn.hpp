@@ -1,0 +1,408 @@
+// The learned first guess's network on the device: a 4-level U-Net of 3x3 convolutions over NHWC
+// bf16 tensors (periodic in x, zero-padded in y, as the planet grid), its backward passes and its
+// optimiser.  Included once, by dtdiff.hip.  The solve never depends on it: a guess it gives only
+// changes where conjugate gradients start, and the same stopping test ends the solve.
+//
+// Tensors: n images of h x w pixels with c channels, element (img, y, x, ch) at
+// ((img h + y) w + x) c + ch, bf16 (the upper half of an f32).  Channel counts are multiples of 16
+// and w of 16, so that a wave's 16-pixel tiles stay in one row.  Weights of a convolution with cin
+// inputs and cout outputs: wt[co][tap][ci] (tap = 3 (dy + 1) + dx + 1), bf16, with f32 masters.
+//
+// The convolution is an implicit GEMM on the matrix cores: pixels x (9 cin) times (9 cin) x cout,
+// v_mfma_f32_16x16x16_bf16 with f32 accumulation, one wave a tile of 16 pixels and up to 64
+// outputs.  Its backward pass for the input is the same kernel with the taps mirrored and the
+// weights' channel roles swapped (the adjoint of "read x at p + d, wrapping in x, zero past the
+// rows" is "read at p - d" under the same rules).  The weights' gradient is a GEMM over pixels:
+// per chunk of pixels one wave a (16 outputs, 16 inputs, tap) tile on channel-major copies of the
+// layer's input and output gradient, then the chunks added in order, so every sum has a fixed
+// order and the training is deterministic.
+#pragma once
+
+#include "context.hpp"
+
+namespace dtd {
+namespace nn {
+
+typedef short s4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// f32 -> bf16, round to nearest even (NaN kept a NaN).
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+struct Shape {
+    int n, h, w;  // images, rows, columns
+};
+
+__device__ __forceinline__ long pixels(const Shape &s) { return (long)s.n * s.h * s.w; }
+
+// y[p][co] = act(bias[co] + sum over taps and ci of x[p + d(tap)][ci] wt[co][tap][ci]); act is
+// max(0, .) when relu.  y32 (optional) takes the first y32c outputs in f32 as well.  One wave a
+// tile of 16 pixels and NT * 16 outputs (blockIdx.y picks the outputs' group).
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3x3(Shape s, int cin, int cout,
+                                                 const uint16_t *__restrict__ x,
+                                                 const uint16_t *__restrict__ wt,
+                                                 const float *__restrict__ bias, int relu,
+                                                 uint16_t *__restrict__ y, float *__restrict__ y32,
+                                                 int y32c) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long tile = (long)blockIdx.x * 4 + wave;
+    if (tile * 16 >= pixels(s)) return;
+    const int co0 = blockIdx.y * 16 * NT;
+    const long p0 = tile * 16;
+    const int col0 = (int)(p0 % s.w);
+    const long rowi = p0 / s.w;  // image row index over all images
+    const int row = (int)(rowi % s.h);
+    const long img = rowi / s.h;
+    const int m = lane & 15, kq = (lane >> 4) * 4;
+    f4_t acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = f4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int tap = 0; tap < 9; tap++) {
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        const int yy = row + dy;
+        const bool inside = yy >= 0 && yy < s.h;
+        int xx = col0 + m + dx;
+        xx = xx < 0 ? xx + s.w : (xx >= s.w ? xx - s.w : xx);
+        const uint16_t *xp = inside ? x + ((img * s.h + yy) * s.w + xx) * (long)cin : nullptr;
+        const uint16_t *wp = wt + ((size_t)(co0 + m) * 9 + tap) * cin + kq;
+        for (int c0 = 0; c0 < cin; c0 += 16) {
+            s4_t a = s4_t{0, 0, 0, 0};
+            if (inside) a = *reinterpret_cast<const s4_t *>(xp + c0 + kq);
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const s4_t b = *reinterpret_cast<const s4_t *>(wp + (size_t)16 * t * 9 * cin + c0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const int co = co0 + 16 * t + m;
+        const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const long p = p0 + kq + q;
+            float v = acc[t][q] + bv;
+            if (relu) v = fmaxf(v, 0.0f);
+            y[p * cout + co] = f2bf(v);
+            if (y32 && co < y32c) y32[p * y32c + co] = v;
+        }
+    }
+}
+
+// wt_t[ci][tap][co] = wt[co][8 - tap][ci]: the weights of the input-gradient convolution.
+__global__ __launch_bounds__(256) void k_conv_flip(int cin, int cout, const uint16_t *__restrict__ wt,
+                                                   uint16_t *__restrict__ wt_t) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)cin * 9 * cout) return;
+    const int co = (int)(i % cout), tap = (int)((i / cout) % 9), ci = (int)(i / (9L * cout));
+    wt_t[i] = wt[((size_t)co * 9 + (8 - tap)) * cin + ci];
+}
+
+// dz = dy where y > 0, else 0 (the ReLU's backward pass on its output).
+__global__ __launch_bounds__(256) void k_relu_bwd(long n, const uint16_t *__restrict__ y,
+                                                  const uint16_t *__restrict__ dy,
+                                                  uint16_t *__restrict__ dz) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dz[i] = bf2f(y[i]) > 0.0f ? dy[i] : (uint16_t)0;
+}
+
+// xt[ch][p] = x[p][ch] for all pixels p (c channels, tiles of 64 pixels x 16 channels).
+__global__ __launch_bounds__(256) void k_to_planes(long np, int c, const uint16_t *__restrict__ x,
+                                                   uint16_t *__restrict__ xt) {
+    __shared__ uint16_t t[16][65];
+    const long p0 = (long)blockIdx.x * 64;
+    const int c0 = blockIdx.y * 16;
+    for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+        const int pp = e / 16, cc = e % 16;
+        t[cc][pp] = p0 + pp < np ? x[(p0 + pp) * c + c0 + cc] : (uint16_t)0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+        const int cc = e / 64, pp = e % 64;
+        if (p0 + pp < np) xt[(long)(c0 + cc) * np + p0 + pp] = t[cc][pp];
+    }
+}
+
+// Per chunk of `chunk` pixels, part[k][co][tap][ci] = sum over the chunk's pixels p of
+// dz[p][co] x[p + d(tap)][ci], from channel-major copies dzt[co][p], xt[ci][p].  One wave a
+// (16 outputs, 16 inputs, tap) tile; blockIdx.x the chunk.
+__global__ __launch_bounds__(256) void k_conv_dw(Shape s, int cin, int cout,
+                                                 const uint16_t *__restrict__ xt,
+                                                 const uint16_t *__restrict__ dzt, int chunk,
+                                                 float *__restrict__ part) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int combo = blockIdx.y * 4 + wave;
+    const int ncit = cin / 16, ncot = cout / 16;
+    if (combo >= ncot * ncit * 9) return;
+    const int tap = combo % 9, cit = (combo / 9) % ncit, cot = combo / (9 * ncit);
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const long np = pixels(s);
+    const long pa = (long)blockIdx.x * chunk, pb = min(np, pa + chunk);
+    const int m = lane & 15, kq = (lane >> 4) * 4;
+    const uint16_t *dzp = dzt + (long)(cot * 16 + m) * np;
+    const uint16_t *xp = xt + (long)(cit * 16 + m) * np;
+    f4_t acc = f4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (long p = pa; p < pb; p += 16) {
+        const long q0 = p + kq;  // this lane's 4 pixels: q0 .. q0 + 3, in one row
+        const s4_t a = *reinterpret_cast<const s4_t *>(dzp + q0);
+        const int col = (int)(q0 % s.w);
+        const long rowi = q0 / s.w;
+        const int row = (int)(rowi % s.h), yy = row + dy;
+        s4_t b = s4_t{0, 0, 0, 0};
+        if (yy >= 0 && yy < s.h) {
+            const uint16_t *r = xp + (rowi + dy) * s.w;
+            if (dx == 0) {
+                b = *reinterpret_cast<const s4_t *>(r + col);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    int xx = col + q + dx;
+                    xx = xx < 0 ? xx + s.w : (xx >= s.w ? xx - s.w : xx);
+                    b[q] = (short)r[xx];
+                }
+            }
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    // D[co][ci]: lane holds rows kq + q (outputs), column m (inputs).
+    float *out = part + (size_t)blockIdx.x * cout * 9 * cin;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        out[((size_t)(cot * 16 + kq + q) * 9 + tap) * cin + cit * 16 + m] = acc[q];
+}
+
+// dw[i] = the chunks' parts added in chunk order; db[co] = sum over pixels of dzt[co][.]
+// (a fixed tree over the block's threads).
+__global__ __launch_bounds__(256) void k_dw_reduce(long nw, int chunks,
+                                                   const float *__restrict__ part,
+                                                   float *__restrict__ dw) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nw) return;
+    float s = 0.0f;
+    for (int k = 0; k < chunks; k++) s += part[(size_t)k * nw + i];
+    dw[i] = s;
+}
+
+__global__ __launch_bounds__(256) void k_db(long np, const uint16_t *__restrict__ dzt,
+                                            float *__restrict__ db) {
+    __shared__ float t[256];
+    const uint16_t *r = dzt + (long)blockIdx.x * np;
+    float s = 0.0f;
+    for (long p = threadIdx.x; p < np; p += 256) s += bf2f(r[p]);
+    t[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) t[threadIdx.x] += t[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[blockIdx.x] = t[0];
+}
+
+// 2 x 2 mean: y (h/2 x w/2) from x (h x w), c channels.
+__global__ __launch_bounds__(256) void k_pool(Shape s, int c, const uint16_t *__restrict__ x,
+                                              uint16_t *__restrict__ y) {
+    const int hh = s.h / 2, ww = s.w / 2;
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)s.n * hh * ww * c) return;
+    const int ch = (int)(i % c);
+    const long pc = i / c;
+    const int xc = (int)(pc % ww);
+    const long r = pc / ww;
+    const int yc = (int)(r % hh);
+    const long img = r / hh;
+    const long base = ((img * s.h + 2 * yc) * s.w + 2 * xc) * c + ch;
+    const float v = (bf2f(x[base]) + bf2f(x[base + c])) +
+                    (bf2f(x[base + (long)s.w * c]) + bf2f(x[base + (long)s.w * c + c]));
+    y[i] = f2bf(0.25f * v);
+}
+
+// cat[p][0 .. cu) = u[parent(p)] (nearest, from the half-size grid), cat[p][cu .. cu + cs) =
+// skip[p]; s is the full-size shape.
+__global__ __launch_bounds__(256) void k_up_cat(Shape s, int cu, int cs, const uint16_t *__restrict__ u,
+                                                const uint16_t *__restrict__ skip,
+                                                uint16_t *__restrict__ cat) {
+    const int c = cu + cs;
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= pixels(s) * c) return;
+    const int ch = (int)(i % c);
+    const long p = i / c;
+    if (ch >= cu) {
+        cat[i] = skip[p * cs + (ch - cu)];
+        return;
+    }
+    const int col = (int)(p % s.w);
+    const long r = p / s.w;
+    const int row = (int)(r % s.h);
+    const long img = r / s.h;
+    const long pc = (img * (s.h / 2) + row / 2) * (s.w / 2) + col / 2;
+    cat[i] = u[pc * cu + ch];
+}
+
+// The concat's backward pass: du[q][ch] = sum of dcat[p][ch] over q's four children (ch < cu),
+// and dskip[p][ch] = dcat[p][cu + ch] + 0.25 dpool[parent(p)][ch] (the skip's tensor also fed the
+// next level's pooling; dpool null: none).  s is the full-size shape.
+__global__ __launch_bounds__(256) void k_up_cat_bwd_u(Shape s, int cu, int cs,
+                                                      const uint16_t *__restrict__ dcat,
+                                                      uint16_t *__restrict__ du) {
+    const int c = cu + cs, hh = s.h / 2, ww = s.w / 2;
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)s.n * hh * ww * cu) return;
+    const int ch = (int)(i % cu);
+    const long pc = i / cu;
+    const int xc = (int)(pc % ww);
+    const long r = pc / ww;
+    const int yc = (int)(r % hh);
+    const long img = r / hh;
+    const long base = ((img * s.h + 2 * yc) * s.w + 2 * xc) * c + ch;
+    const float v = (bf2f(dcat[base]) + bf2f(dcat[base + c])) +
+                    (bf2f(dcat[base + (long)s.w * c]) + bf2f(dcat[base + (long)s.w * c + c]));
+    du[i] = f2bf(v);
+}
+
+__global__ __launch_bounds__(256) void k_skip_bwd(Shape s, int cu, int cs,
+                                                  const uint16_t *__restrict__ dcat,
+                                                  const uint16_t *__restrict__ dpool,
+                                                  uint16_t *__restrict__ dskip) {
+    const int c = cu + cs;
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= pixels(s) * cs) return;
+    const int ch = (int)(i % cs);
+    const long p = i / cs;
+    float v = bf2f(dcat[p * c + cu + ch]);
+    if (dpool) {
+        const int col = (int)(p % s.w);
+        const long r = p / s.w;
+        const int row = (int)(r % s.h);
+        const long img = r / s.h;
+        const long pc = (img * (s.h / 2) + row / 2) * (s.w / 2) + col / 2;
+        v += 0.25f * bf2f(dpool[pc * cs + ch]);
+    }
+    dskip[i] = f2bf(v);
+}
+
+// The network's input (16 channels, three used): h / scale, ln kd, ln(dt kd / dx^2) / 4.
+__global__ __launch_bounds__(256) void k_nn_input(long np, const double *__restrict__ h,
+                                                  const double *__restrict__ kd, double dt,
+                                                  double dx2, double scale,
+                                                  uint16_t *__restrict__ x) {
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= np) return;
+    const double k = fmax(kd[p], 1e-30);
+    uint16_t *o = x + p * 16;
+    o[0] = f2bf((float)(h[p] / scale));
+    o[1] = f2bf((float)log(k));
+    o[2] = f2bf((float)(0.25 * log(dt * k / dx2)));
+#pragma unroll
+    for (int c = 3; c < 16; c++) o[c] = 0;
+}
+
+// The guess: g = scale (bf16(h / scale) + out[p][0]), the prediction the loss trains (the
+// network predicts the change from its own input channel).
+__global__ __launch_bounds__(256) void k_nn_guess(long np, const double *__restrict__ h,
+                                                  const float *__restrict__ out, int outc,
+                                                  double scale, double *__restrict__ g) {
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    if (p < np)
+        g[p] = scale * (double)(bf2f(f2bf((float)(h[p] / scale))) + out[p * outc]);
+}
+
+// Loss against the target t (in units of scale): pred = bf16(h / scale) + out[p][0];
+// parts[block] = sum of (pred - t)^2 over the block (a fixed tree), dy[p][0] = 2 (pred - t) / np,
+// the other 15 channels 0.
+__global__ __launch_bounds__(256) void k_nn_loss(long np, const uint16_t *__restrict__ x,
+                                                 const float *__restrict__ out, int outc,
+                                                 const double *__restrict__ t, double scale,
+                                                 float *__restrict__ parts,
+                                                 uint16_t *__restrict__ dy) {
+    __shared__ float red[256];
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    float e = 0.0f;
+    if (p < np) {
+        const float pred = bf2f(x[p * 16]) + out[p * outc];
+        const float d = pred - (float)(t[p] / scale);
+        e = d * d;
+        uint16_t *o = dy + p * 16;
+        o[0] = f2bf(2.0f * d / (float)np);
+#pragma unroll
+        for (int c = 1; c < 16; c++) o[c] = 0;
+    }
+    red[threadIdx.x] = e;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) parts[blockIdx.x] = red[0];
+}
+
+// Adam (Kingma & Ba 2015) on f32 masters w with moments m, v; wb gets bf16(w).  bc1, bc2 are the
+// bias corrections 1 - beta^t.
+__global__ __launch_bounds__(256) void k_adam(long n, float *__restrict__ w, float *__restrict__ m,
+                                              float *__restrict__ v, const float *__restrict__ g,
+                                              float lr, float b1, float b2, float eps, float bc1,
+                                              float bc2, uint16_t *__restrict__ wb) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi, v[i] = vi;
+    const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = f2bf(wi);
+}
+
+__global__ __launch_bounds__(256) void k_to_bf16(long n, const float *__restrict__ a,
+                                                 uint16_t *__restrict__ b) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) b[i] = f2bf(a[i]);
+}
+
+// ---- host side ---------------------------------------------------------------------------
+
+constexpr int kChunk = 4096;  // pixels a chunk of the weights' gradient
+
+static void check_shape(int n, int h, int w) {
+    if (n < 1 || h < 1 || w < 16 || w % 16)
+        throw Error{DTD_EINVAL, "a network tensor needs n, h >= 1 and w a multiple of 16"};
+    if ((long)n * h * w > (long)INT32_MAX * 16) throw Error{DTD_ETOOBIG, "too many pixels"};
+}
+
+static void check_channels(int c) {
+    if (c < 16 || c % 16) throw Error{DTD_EINVAL, "channel counts must be multiples of 16"};
+}
+
+static unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
+
+template <int NT>
+static void conv_launch(hipStream_t s, Shape sh, int cin, int cout, const uint16_t *x,
+                           const uint16_t *wt, const float *bias, int relu, uint16_t *y,
+                           float *y32, int y32c) {
+    const long tiles = (long)sh.n * sh.h * sh.w / 16;
+    dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)(cout / (16 * NT)));
+    k_conv3x3<NT><<<grid, 256, 0, s>>>(sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+    LAUNCH_CHECK();
+}
+
+static void conv(hipStream_t s, Shape sh, int cin, int cout, const uint16_t *x,
+                    const uint16_t *wt, const float *bias, int relu, uint16_t *y, float *y32,
+                    int y32c) {
+    if (cout % 64 == 0)
+        conv_launch<4>(s, sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+    else if (cout % 32 == 0)
+        conv_launch<2>(s, sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+    else
+        conv_launch<1>(s, sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+}
+
+}  // namespace nn
+}  // namespace dtd

@@ -42,7 +42,10 @@ SYMBOLS = (
     "dtd_cycle_new", "dtd_cycle_levels", "dtd_cycle_level", "dtd_cycle_precondition",
     "dtd_time_kernel", "dtd_solve_batch", "dtd_step_device", "dtd_fill_kd",
     "dtd_slope_failure_device", "dtd_slope_failure", "dtd_cube_new", "dtd_cube_free",
-    "dtd_cube_solve", "dtd_cube_solve_device",
+    "dtd_cube_solve", "dtd_cube_solve_device", "dtd_nn_conv3x3", "dtd_nn_conv3x3_bwd_input",
+    "dtd_nn_bwd_weight_scratch", "dtd_nn_conv3x3_bwd_weight", "dtd_nn_relu_bwd", "dtd_nn_pool",
+    "dtd_nn_up_cat", "dtd_nn_up_cat_bwd", "dtd_nn_input", "dtd_nn_loss", "dtd_nn_guess",
+    "dtd_nn_adam", "dtd_nn_to_bf16", "dtd_sync", "dtd_solve_rel_guess_device",
 )
 # The stage's slope failure constants (crates/deeptime-core/src/surface/marine.rs:231-232).
 FAILURE_RATE = 0.05
@@ -182,6 +185,27 @@ def load(path: str = LIB_PATH):
                                     ctypes.c_double, ctypes.POINTER(Failure), _dp, _dp,
                                     ctypes.POINTER(ctypes.c_double),
                                     ctypes.POINTER(ctypes.c_uint)]
+    _i, _d, _f = ctypes.c_int, ctypes.c_double, ctypes.c_float
+    L.dtd_nn_conv3x3.argtypes = [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _i]
+    L.dtd_nn_conv3x3_bwd_input.argtypes = [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]
+    L.dtd_nn_bwd_weight_scratch.argtypes = [_i, _i, _i, _i, _i, ctypes.POINTER(_sz)]
+    L.dtd_nn_bwd_weight_scratch.restype = _sz
+    L.dtd_nn_conv3x3_bwd_weight.argtypes = [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _sz, _vp,
+                                            _sz, _vp, _vp]
+    L.dtd_nn_relu_bwd.argtypes = [_vp, _sz, _vp, _vp, _vp]
+    L.dtd_nn_pool.argtypes = [_vp, _i, _i, _i, _i, _vp, _vp]
+    L.dtd_nn_up_cat.argtypes = [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp]
+    L.dtd_nn_up_cat_bwd.argtypes = [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]
+    L.dtd_nn_input.argtypes = [_vp, _sz, _vp, _vp, _d, _d, _d, _vp]
+    L.dtd_nn_loss.argtypes = [_vp, _sz, _vp, _vp, _i, _vp, _d, _vp, _vp,
+                              ctypes.POINTER(ctypes.c_double)]
+    L.dtd_nn_guess.argtypes = [_vp, _sz, _vp, _vp, _i, _d, _vp]
+    L.dtd_nn_adam.argtypes = [_vp, _sz, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _i, _vp]
+    L.dtd_nn_to_bf16.argtypes = [_vp, _sz, _vp, _vp]
+    L.dtd_sync.argtypes = [_vp]
+    L.dtd_solve_rel_guess_device.argtypes = [_vp, _sz, _sz, ctypes.c_double, ctypes.c_int, _vp,
+                                             _vp, ctypes.c_double, ctypes.c_double, _vp,
+                                             ctypes.POINTER(Stats)]
     for name in ("dtd_solve", "dtd_solve_device", "dtd_solve_rel_device", "dtd_operator",
                  "dtd_apply", "dtd_lod", "dtd_cycle_new", "dtd_cycle_level",
                  "dtd_cycle_precondition", "dtd_time_kernel"):
@@ -272,6 +296,16 @@ def solve_rel_device(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float, rel_tol: 
     st = Stats()
     ws.check(load().dtd_solve_rel_device(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x), _ptr(h_dev),
                                          _ptr(kd_dev), dt, rel_tol, ctypes.byref(st)))
+    return st
+
+
+def solve_rel_guess_device(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float, rel_tol: float,
+                           x0_dev) -> Stats:
+    """solve_rel_device started from the guess x0_dev instead of the splitting."""
+    st = Stats()
+    ws.check(load().dtd_solve_rel_guess_device(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x),
+                                               _ptr(h_dev), _ptr(kd_dev), dt, rel_tol,
+                                               _ptr(x0_dev), ctypes.byref(st)))
     return st
 
 

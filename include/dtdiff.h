@@ -258,6 +258,90 @@ int dtd_solve_batch(dtd_ctx *ctx, size_t n, const dtd_grid *grids, double rel_to
  * overwritten: solve again before reading results. */
 int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *bytes);
 
+/* ---- The learned first guess ---------------------------------------------------------------
+ * No reference counterpart: the reference starts conjugate gradients from its splitting
+ * (diffusion.rs:110-114).  These entry points run a small convolutional network that predicts h1
+ * from (h0, ln kd, dt) and a solve that starts from a given guess; the same stopping test ends
+ * the solve, so a guess can only change how many iterations it takes.
+ *
+ * Network tensors are device arrays of bf16 (uint16_t, the upper half of an f32) in NHWC order:
+ * n images of h x w pixels with c channels, element (img, y, x, ch) at ((img h + y) w + x) c + ch;
+ * w and every c must be multiples of 16.  A 3x3 convolution wraps in x and reads zeros past the
+ * first and last rows, as the planet grid; its weights are wt[cout][9][cin] (tap 3 (dy + 1) +
+ * dx + 1), bf16, its bias f32.  All calls are asynchronous on the context's stream except
+ * dtd_nn_loss (which returns the loss) and dtd_sync. */
+
+/* y = relu?(bias + conv3x3(x, wt)) (bias may be NULL); y32 (optional) also gets the first y32c
+ * output channels in f32, y32[p y32c + ch]. */
+int dtd_nn_conv3x3(dtd_ctx *ctx, int n, int h, int w, int cin, int cout, const uint16_t *x,
+                   const uint16_t *wt, const float *bias, int relu, uint16_t *y, float *y32,
+                   int y32c);
+
+/* The input's gradient dx (cin channels) of a convolution from its output's gradient dz (cout
+ * channels); wt_scratch holds cin * 9 * cout bf16. */
+int dtd_nn_conv3x3_bwd_input(dtd_ctx *ctx, int n, int h, int w, int cin, int cout,
+                             const uint16_t *dz, const uint16_t *wt, uint16_t *wt_scratch,
+                             uint16_t *dx);
+
+/* The scratch dtd_nn_conv3x3_bwd_weight needs: returns the bf16 elements of `planes`, and
+ * *f32_elems gets the f32 elements of `part`. */
+size_t dtd_nn_bwd_weight_scratch(int n, int h, int w, int cin, int cout, size_t *f32_elems);
+
+/* The weights' gradient dw[cout][9][cin] (f32) and the bias's db[cout] (optional) from the
+ * layer's input x and output gradient dz; sums in a fixed order. */
+int dtd_nn_conv3x3_bwd_weight(dtd_ctx *ctx, int n, int h, int w, int cin, int cout,
+                              const uint16_t *x, const uint16_t *dz, uint16_t *planes,
+                              size_t planes_elems, float *part, size_t part_elems, float *dw,
+                              float *db);
+
+/* dz = dy where y > 0, else 0 (n elements). */
+int dtd_nn_relu_bwd(dtd_ctx *ctx, size_t n, const uint16_t *y, const uint16_t *dy, uint16_t *dz);
+
+/* y = 2x2 mean of x (h x w, even) into h/2 x w/2. */
+int dtd_nn_pool(dtd_ctx *ctx, int n, int h, int w, int c, const uint16_t *x, uint16_t *y);
+
+/* cat (h x w, cu + cs channels) = [nearest upsampling of u (h/2 x w/2, cu), skip (cs)]. */
+int dtd_nn_up_cat(dtd_ctx *ctx, int n, int h, int w, int cu, int cs, const uint16_t *u,
+                  const uint16_t *skip, uint16_t *cat);
+
+/* Its backward pass: du = the upsampling's adjoint of dcat's first cu channels; dskip = dcat's
+ * last cs channels plus 0.25 x the pooling's gradient dpool (h/2 x w/2, cs; NULL: none) at each
+ * pixel's parent.  du or dskip may be NULL (that part is skipped). */
+int dtd_nn_up_cat_bwd(dtd_ctx *ctx, int n, int h, int w, int cu, int cs, const uint16_t *dcat,
+                      const uint16_t *dpool, uint16_t *du, uint16_t *dskip);
+
+/* The network's input x (16 channels) from f64 h and kd: h / scale, ln kd, ln(dt kd / dx^2) / 4,
+ * zeros. */
+int dtd_nn_input(dtd_ctx *ctx, size_t np, const double *h, const double *kd, double dt,
+                 double dx, double scale, uint16_t *x);
+
+/* Mean squared error of the prediction (x's channel 0 plus out[p outc]) against t / scale into
+ * *loss, and its gradient dy (16 channels, channel 0 only); parts holds np / 256 + 1 floats. */
+int dtd_nn_loss(dtd_ctx *ctx, size_t np, const uint16_t *x, const float *out, int outc,
+                const double *t, double scale, float *parts, uint16_t *dy, double *loss);
+
+/* The guess g = scale (bf16(h / scale) + out[p outc]) in f64: the prediction dtd_nn_loss
+ * trains. */
+int dtd_nn_guess(dtd_ctx *ctx, size_t np, const double *h, const float *out, int outc,
+                 double scale, double *g);
+
+/* One Adam step (Kingma & Ba 2015, step counted from 1) on f32 masters w with moments m, v and
+ * gradient g; wb (optional) gets bf16(w). */
+int dtd_nn_adam(dtd_ctx *ctx, size_t n, float *w, float *m, float *v, const float *g, float lr,
+                float beta1, float beta2, float eps, int step, uint16_t *wb);
+
+/* b = bf16(a). */
+int dtd_nn_to_bf16(dtd_ctx *ctx, size_t n, const float *a, uint16_t *b);
+
+/* Waits for the context's stream. */
+int dtd_sync(dtd_ctx *ctx);
+
+/* dtd_solve_rel_device started from the guess x0_dev (device, w h f64) instead of the
+ * splitting. */
+int dtd_solve_rel_guess_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x,
+                               double *h_dev, const double *kd_dev, double dt, double rel_tol,
+                               const double *x0_dev, dtd_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif
