@@ -25,6 +25,7 @@ sys.path.insert(0, os.path.join(REPO, "deep-time_amd"))
 
 METRIC = "unknowns/s to FP64 rel. residual 1e-10 at 8192×4096; smoother HBM GB/s vs peak"
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 matrix-core peak
 
 
 def log(*a):
@@ -60,10 +61,11 @@ def main():
                     help="bf16: the f32 cycle with its two finest levels' records stored in bf16")
     ap.add_argument("--guess", choices=["lod", "rhs"], default="lod")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
-    ap.add_argument("--stage", choices=["solve", "failure", "cube"], default="solve",
+    ap.add_argument("--stage", choices=["solve", "failure", "cube", "learned"], default="solve",
                     help="failure: the submarine slope failure after the solve (8192x4096 world "
                          "with sediment everywhere, 40 iterations); cube: the cube-grid solve "
-                         "(--cube-n cells a face side)")
+                         "(--cube-n cells a face side); learned: the solve from the learned "
+                         "first guess beside the plain solve")
     ap.add_argument("--cube-n", type=int, default=512)
     ap.add_argument("--kernels-only", action="store_true",
                     help="one solve, then only the per-kernel timings (for counter runs)")
@@ -84,6 +86,8 @@ def main():
     if args.stage == "cube":
         return bench_cube(args)
     gen, kw = F.WORKLOADS[args.workload]
+    if args.stage == "learned":
+        return bench_learned(args, gen, kw)
     if gen is F.case_regional:
         return bench_batch(args, gen, dict(kw))
     if args.solver == "jacobi":
@@ -446,6 +450,131 @@ def bench_cube(args):
         "roofline": None,
         "cpu_baseline": cpu,
         "iterations": int(its),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_learned(args, gen, kw):
+    """The learned first guess at the workload's size (8192x4096 log-normal by default): the plain
+    solve (the splitting's guess) and the solve started from the network's guess, each to the
+    same true FP64 relative residual, on resident arrays.  A learned step is the guess (splitting +
+    network) plus the solve, host clock around both with the device synchronised.  The network's
+    weights are the committed ones (trained by `python -m dtdiff.learned train` on 1024x512 solves
+    of the same generators)."""
+    import numpy as np
+
+    import dtdiff
+    from dtdiff import learned as LN
+    from dtdiff.hipmem import DeviceArray, synchronize
+
+    kw = dict(kw)
+    if args.size:
+        kw["w"], kw["h"] = (int(v) for v in args.size.split("x"))
+    w, h = kw["w"], kw["h"]
+    d, h0, kd, dt = gen(**kw)
+    n = d.cells()
+    h0d, kdd, hd, gd = (DeviceArray.from_numpy(h0), DeviceArray.from_numpy(kd), DeviceArray(n),
+                        DeviceArray(n))
+    ws = dtdiff.Workspace(device=0)
+    net = LN.UNet(ws, LN.load_params())
+    L = dtdiff.load()
+
+    def plain():
+        hd.copy_from(h0d)
+        synchronize()
+        t0 = time.perf_counter()
+        st = dtdiff.solve_rel_device(ws, d, hd, kdd, dt, args.rel)
+        return st, time.perf_counter() - t0
+
+    def learned():
+        hd.copy_from(h0d)
+        synchronize()
+        t0 = time.perf_counter()
+        net.guess(d, hd, kdd, dt, gd)
+        ws.check(L.dtd_sync(ws._ctx))
+        t1 = time.perf_counter()
+        st = dtdiff.solve_rel_guess_device(ws, d, hd, kdd, dt, args.rel, gd)
+        return st, time.perf_counter() - t0, t1 - t0
+
+    for _ in range(args.warmup):
+        plain()
+        learned()
+    tp, tl, tg = 0.0, 0.0, 0.0
+    for _ in range(args.steps):
+        sp, e = plain()
+        tp += e
+        sl, e, g = learned()
+        tl += e
+        tg += g
+    x_ref = None
+    # Accuracy of the two guesses against the converged answer.
+    plain()
+    synchronize()
+    x_ref = hd.numpy()
+    dtdiff.lod_device(ws, d, h0d, kdd, dt, gd)
+    ws.check(L.dtd_sync(ws._ctx))
+    err_lod = float(np.sqrt(np.mean((gd.numpy() - x_ref) ** 2)))
+    net.guess(d, h0d, kdd, dt, gd)
+    ws.check(L.dtd_sync(ws._ctx))
+    err_net = float(np.sqrt(np.mean((gd.numpy() - x_ref) ** 2)))
+    # The network's forward alone (the convolutions on the matrix cores), and its flops.
+    net.set_input(d, h0d, kdd, gd, dt)
+    ws.check(L.dtd_sync(ws._ctx))
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        net.forward()
+    ws.check(L.dtd_sync(ws._ctx))
+    fwd_ms = (time.perf_counter() - t0) * 1e3 / reps
+    flops = 0.0
+    for name, ci, co, _ in LN.LAYERS:
+        lvl = 0 if name == "out" else int(name[3])
+        flops += 2.0 * (n >> (2 * lvl)) * co * 9 * ci
+    tflops = flops / (fwd_ms * 1e-3) / 1e12
+    log(f"plain {tp * 1e3 / args.steps:.2f} ms ({sp.iterations} its), learned "
+        f"{tl * 1e3 / args.steps:.2f} ms ({sl.iterations} its, guess {tg * 1e3 / args.steps:.2f} "
+        f"ms), network forward {fwd_ms:.1f} ms = {tflops:.0f} TFLOP/s; guess RMS error: "
+        f"splitting {err_lod:.3f} m, network {err_net:.3f} m")
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O
+
+        ds, hs, kds, dts = gen(**dict(kw, w=1024, h=512))
+        tol_m = args.rel * float(np.linalg.norm(hs)) / np.sqrt(ds.cells())
+        t = time.perf_counter()
+        _, its_o = O.solve(ds.w, ds.h, ds.dx, ds.wrap_x, hs, kds, dts, tol_m)
+        cpu_s = time.perf_counter() - t
+        cpu = {"value": ds.cells() / cpu_s, "unit": "unknowns/s", "cores": O.threads(),
+               "kind": "port",
+               "sample": f"the plain solve of the same generator at 1024x512 by the oracle "
+                         f"(reference algorithm restated in C, OpenMP): {cpu_s:.2f} s, "
+                         f"{its_o} iterations"}
+    out = {
+        "metric": f"unknowns/s to FP64 rel. residual {args.rel:g} at {w}x{h} with the learned "
+                  f"first guess",
+        "value": n * args.steps / tl,
+        "unit": "unknowns/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": tl * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 network (f32 accumulation) + f64 CG with the f32 V-cycle",
+        "data": "synthetic",
+        "config": {"workload": f"diffusion solve {w}x{h}, log-normal K (sigma 8 cells), "
+                               f"dt*mean(K)/dx^2=50, k^-2 h0, first guess = splitting + U-Net "
+                               f"correction (32/64/128/256 channels)"},
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": tflops / MFMA_BF16_PEAK_TFLOPS, "traffic": None,
+                     "kernel": f"the network's forward (15 3x3 convolutions, {flops / 1e12:.2f} "
+                               f"TFLOP, bf16 MFMA with f32 accumulation), {fwd_ms:.1f} ms"},
+        "cpu_baseline": cpu,
+        "plain_unknowns_per_s": n * args.steps / tp,
+        "plain_ms": tp * 1e3 / args.steps,
+        "iterations_plain": int(sp.iterations),
+        "iterations_learned": int(sl.iterations),
+        "guess_ms": tg * 1e3 / args.steps,
+        "network_forward_ms": fwd_ms,
+        "guess_rms_error_m": {"splitting": err_lod, "network": err_net},
+        "rel_residual": {"plain": sp.rel_residual, "learned": sl.rel_residual},
     }
     print(json.dumps(out), flush=True)
 

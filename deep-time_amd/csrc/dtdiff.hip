@@ -1323,46 +1323,63 @@ int dtd_nn_up_cat_bwd(dtd_ctx *ctx, int n, int h, int w, int cu, int cs, const u
     });
 }
 
-int dtd_nn_input(dtd_ctx *ctx, size_t np, const double *h, const double *kd, double dt,
-                 double dx, double scale, uint16_t *x) {
+int dtd_nn_input(dtd_ctx *ctx, size_t np, const double *h, const double *kd, const double *base,
+                 double dt, double dx, double scale, uint16_t *x) {
     return guard(ctx, [&] {
         begin(ctx);
         if (!h || !kd || !x) throw Error{DTD_EINVAL, "null array"};
         if (!(scale > 0.0) || !(dt > 0.0) || !(dx > 0.0))
             throw Error{DTD_EINVAL, "scale, dt and dx must be positive"};
         if (np == 0) return;
-        nn::k_nn_input<<<nn::blocks((long)np), 256, 0, ctx->stream>>>((long)np, h, kd, dt, dx * dx,
-                                                                     scale, x);
+        nn::k_nn_input<<<nn::blocks((long)np), 256, 0, ctx->stream>>>((long)np, h, kd, base, dt,
+                                                                      dx * dx, scale, x);
         LAUNCH_CHECK();
     });
 }
 
-int dtd_nn_loss(dtd_ctx *ctx, size_t np, const uint16_t *x, const float *out, int outc,
-                const double *t, double scale, float *parts, uint16_t *dy, double *loss) {
+int dtd_nn_loss(dtd_ctx *ctx, size_t np, const float *out, int outc, const double *t,
+                const double *base, double scale, float *parts, uint16_t *dy, double *loss) {
     return guard(ctx, [&] {
         begin(ctx);
-        if (!x || !out || !t || !parts || !dy || !loss) throw Error{DTD_EINVAL, "null array"};
+        if (!out || !t || !base || !parts || !dy || !loss) throw Error{DTD_EINVAL, "null array"};
         if (np == 0 || outc < 1) throw Error{DTD_EINVAL, "empty loss"};
         const unsigned nb = nn::blocks((long)np);
-        nn::k_nn_loss<<<nb, 256, 0, ctx->stream>>>((long)np, x, out, outc, t, scale, parts, dy);
+        nn::k_nn_loss<<<nb, 256, 0, ctx->stream>>>((long)np, out, outc, t, base, scale, parts,
+                                                   dy);
         LAUNCH_CHECK();
-        std::vector<float> h(nb);
-        download(ctx, h.data(), parts, nb * sizeof(float));
+        std::vector<float> hp(nb);
+        download(ctx, hp.data(), parts, nb * sizeof(float));
         double s = 0.0;
-        for (float v : h) s += v;
+        for (float v : hp) s += v;
         *loss = s / (double)np;
     });
 }
 
-int dtd_nn_guess(dtd_ctx *ctx, size_t np, const double *h, const float *out, int outc,
+int dtd_nn_guess(dtd_ctx *ctx, size_t np, const double *base, const float *out, int outc,
                  double scale, double *g) {
     return guard(ctx, [&] {
         begin(ctx);
-        if (!h || !out || !g) throw Error{DTD_EINVAL, "null array"};
+        if (!base || !out || !g) throw Error{DTD_EINVAL, "null array"};
         if (np == 0) return;
-        nn::k_nn_guess<<<nn::blocks((long)np), 256, 0, ctx->stream>>>((long)np, h, out, outc,
-                                                                     scale, g);
+        nn::k_nn_guess<<<nn::blocks((long)np), 256, 0, ctx->stream>>>((long)np, base, out, outc,
+                                                                      scale, g);
         LAUNCH_CHECK();
+    });
+}
+
+int dtd_lod_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, const double *h_dev,
+                   const double *kd_dev, double dt, double *out_dev) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        check_grid(w, h);
+        const size_t n = w * h;
+        if (n == 0) return;
+        if (!h_dev || !kd_dev || !out_dev) throw Error{DTD_EINVAL, "null array"};
+        fit(ctx, (int)w, (int)h, wrap_x && w > 1);
+        Workspace &ws = ctx->ws;
+        lod(ctx, h_dev, ws.x, kd_dev, dx, dt, wrap_x);
+        HIPCHK(hipMemcpyAsync(out_dev, ws.x, n * sizeof(double), hipMemcpyDeviceToDevice,
+                              ctx->stream));
     });
 }
 

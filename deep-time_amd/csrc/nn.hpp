@@ -44,9 +44,11 @@ __device__ __forceinline__ long pixels(const Shape &s) { return (long)s.n * s.h 
 
 // y[p][co] = act(bias[co] + sum over taps and ci of x[p + d(tap)][ci] wt[co][tap][ci]); act is
 // max(0, .) when relu.  y32 (optional) takes the first y32c outputs in f32 as well.  One wave a
-// tile of 16 pixels and NT * 16 outputs (blockIdx.y picks the outputs' group).
-template <int NT>
-__global__ __launch_bounds__(256) void k_conv3x3(Shape s, int cin, int cout,
+// tile of 16 pixels and NT * 16 outputs (blockIdx.y picks the outputs' group).  CIN > 0 fixes the
+// input channels at compile time, so that the tap and channel loops unroll and their loads issue
+// ahead of the matrix instructions; CIN = 0 reads cin.
+template <int NT, int CIN>
+__global__ __launch_bounds__(256) void k_conv3x3(Shape s, int cin_, int cout,
                                                  const uint16_t *__restrict__ x,
                                                  const uint16_t *__restrict__ wt,
                                                  const float *__restrict__ bias, int relu,
@@ -62,9 +64,12 @@ __global__ __launch_bounds__(256) void k_conv3x3(Shape s, int cin, int cout,
     const int row = (int)(rowi % s.h);
     const long img = rowi / s.h;
     const int m = lane & 15, kq = (lane >> 4) * 4;
+    const int cin = CIN > 0 ? CIN : cin_;
+    constexpr int U = CIN > 0 && CIN <= 64 ? (CIN > 0 ? CIN / 16 : 1) : 4;
     f4_t acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[t] = f4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
     for (int tap = 0; tap < 9; tap++) {
         const int dy = tap / 3 - 1, dx = tap % 3 - 1;
         const int yy = row + dy;
@@ -73,6 +78,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(Shape s, int cin, int cout,
         xx = xx < 0 ? xx + s.w : (xx >= s.w ? xx - s.w : xx);
         const uint16_t *xp = inside ? x + ((img * s.h + yy) * s.w + xx) * (long)cin : nullptr;
         const uint16_t *wp = wt + ((size_t)(co0 + m) * 9 + tap) * cin + kq;
+#pragma unroll U
         for (int c0 = 0; c0 < cin; c0 += 16) {
             s4_t a = s4_t{0, 0, 0, 0};
             if (inside) a = *reinterpret_cast<const s4_t *>(xp + c0 + kq);
@@ -151,6 +157,8 @@ __global__ __launch_bounds__(256) void k_conv_dw(Shape s, int cin, int cout,
     const uint16_t *dzp = dzt + (long)(cot * 16 + m) * np;
     const uint16_t *xp = xt + (long)(cit * 16 + m) * np;
     f4_t acc = f4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    // Unrolled so that several steps' loads are in flight before their matrix instructions.
+#pragma unroll 8
     for (long p = pa; p < pb; p += 16) {
         const long q0 = p + kq;  // this lane's 4 pixels: q0 .. q0 + 3, in one row
         const s4_t a = *reinterpret_cast<const s4_t *>(dzp + q0);
@@ -289,9 +297,11 @@ __global__ __launch_bounds__(256) void k_skip_bwd(Shape s, int cu, int cs,
     dskip[i] = f2bf(v);
 }
 
-// The network's input (16 channels, three used): h / scale, ln kd, ln(dt kd / dx^2) / 4.
+// The network's input (16 channels, four used): h / scale, ln kd, ln(dt kd / dx^2) / 4 and the
+// base guess / scale (0 without one).
 __global__ __launch_bounds__(256) void k_nn_input(long np, const double *__restrict__ h,
-                                                  const double *__restrict__ kd, double dt,
+                                                  const double *__restrict__ kd,
+                                                  const double *__restrict__ base, double dt,
                                                   double dx2, double scale,
                                                   uint16_t *__restrict__ x) {
     const long p = (long)blockIdx.x * 256 + threadIdx.x;
@@ -301,34 +311,32 @@ __global__ __launch_bounds__(256) void k_nn_input(long np, const double *__restr
     o[0] = f2bf((float)(h[p] / scale));
     o[1] = f2bf((float)log(k));
     o[2] = f2bf((float)(0.25 * log(dt * k / dx2)));
+    o[3] = base ? f2bf((float)(base[p] / scale)) : (uint16_t)0;
 #pragma unroll
-    for (int c = 3; c < 16; c++) o[c] = 0;
+    for (int c = 4; c < 16; c++) o[c] = 0;
 }
 
-// The guess: g = scale (bf16(h / scale) + out[p][0]), the prediction the loss trains (the
-// network predicts the change from its own input channel).
-__global__ __launch_bounds__(256) void k_nn_guess(long np, const double *__restrict__ h,
+// The guess: g = base + scale out[p][0] (the network predicts the change from the base guess).
+__global__ __launch_bounds__(256) void k_nn_guess(long np, const double *__restrict__ base,
                                                   const float *__restrict__ out, int outc,
                                                   double scale, double *__restrict__ g) {
     const long p = (long)blockIdx.x * 256 + threadIdx.x;
-    if (p < np)
-        g[p] = scale * (double)(bf2f(f2bf((float)(h[p] / scale))) + out[p * outc]);
+    if (p < np) g[p] = base[p] + scale * (double)out[p * outc];
 }
 
-// Loss against the target t (in units of scale): pred = bf16(h / scale) + out[p][0];
-// parts[block] = sum of (pred - t)^2 over the block (a fixed tree), dy[p][0] = 2 (pred - t) / np,
-// the other 15 channels 0.
-__global__ __launch_bounds__(256) void k_nn_loss(long np, const uint16_t *__restrict__ x,
-                                                 const float *__restrict__ out, int outc,
-                                                 const double *__restrict__ t, double scale,
+// Loss against the target t: the network's out[p][0] against (t - base) / scale (formed in f64);
+// parts[block] = sum of the squared errors over the block (a fixed tree), dy[p][0] =
+// 2 (out - target) / np, the other 15 channels 0.
+__global__ __launch_bounds__(256) void k_nn_loss(long np, const float *__restrict__ out, int outc,
+                                                 const double *__restrict__ t,
+                                                 const double *__restrict__ base, double scale,
                                                  float *__restrict__ parts,
                                                  uint16_t *__restrict__ dy) {
     __shared__ float red[256];
     const long p = (long)blockIdx.x * 256 + threadIdx.x;
     float e = 0.0f;
     if (p < np) {
-        const float pred = bf2f(x[p * 16]) + out[p * outc];
-        const float d = pred - (float)(t[p] / scale);
+        const float d = out[p * outc] - (float)((t[p] - base[p]) / scale);
         e = d * d;
         uint16_t *o = dy + p * 16;
         o[0] = f2bf(2.0f * d / (float)np);
@@ -383,14 +391,32 @@ static void check_channels(int c) {
 
 static unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
 
-template <int NT>
-static void conv_launch(hipStream_t s, Shape sh, int cin, int cout, const uint16_t *x,
-                           const uint16_t *wt, const float *bias, int relu, uint16_t *y,
-                           float *y32, int y32c) {
+template <int NT, int CIN>
+static void conv_launch_c(hipStream_t s, Shape sh, int cin, int cout, const uint16_t *x,
+                          const uint16_t *wt, const float *bias, int relu, uint16_t *y,
+                          float *y32, int y32c) {
     const long tiles = (long)sh.n * sh.h * sh.w / 16;
     dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)(cout / (16 * NT)));
-    k_conv3x3<NT><<<grid, 256, 0, s>>>(sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+    k_conv3x3<NT, CIN><<<grid, 256, 0, s>>>(sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
     LAUNCH_CHECK();
+}
+
+// The network's input channel counts get their own unrolled kernels; others take the general one.
+template <int NT>
+static void conv_launch(hipStream_t s, Shape sh, int cin, int cout, const uint16_t *x,
+                        const uint16_t *wt, const float *bias, int relu, uint16_t *y, float *y32,
+                        int y32c) {
+    switch (cin) {
+    case 16: return conv_launch_c<NT, 16>(s, sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+    case 32: return conv_launch_c<NT, 32>(s, sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+    case 64: return conv_launch_c<NT, 64>(s, sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+    case 96: return conv_launch_c<NT, 96>(s, sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+    case 128: return conv_launch_c<NT, 128>(s, sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+    case 192: return conv_launch_c<NT, 192>(s, sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+    case 256: return conv_launch_c<NT, 256>(s, sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+    case 384: return conv_launch_c<NT, 384>(s, sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+    default: return conv_launch_c<NT, 0>(s, sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+    }
 }
 
 static void conv(hipStream_t s, Shape sh, int cin, int cout, const uint16_t *x,

@@ -45,7 +45,7 @@ SYMBOLS = (
     "dtd_cube_solve", "dtd_cube_solve_device", "dtd_nn_conv3x3", "dtd_nn_conv3x3_bwd_input",
     "dtd_nn_bwd_weight_scratch", "dtd_nn_conv3x3_bwd_weight", "dtd_nn_relu_bwd", "dtd_nn_pool",
     "dtd_nn_up_cat", "dtd_nn_up_cat_bwd", "dtd_nn_input", "dtd_nn_loss", "dtd_nn_guess",
-    "dtd_nn_adam", "dtd_nn_to_bf16", "dtd_sync", "dtd_solve_rel_guess_device",
+    "dtd_nn_adam", "dtd_nn_to_bf16", "dtd_sync", "dtd_solve_rel_guess_device", "dtd_lod_device",
 )
 # The stage's slope failure constants (crates/deeptime-core/src/surface/marine.rs:231-232).
 FAILURE_RATE = 0.05
@@ -196,10 +196,11 @@ def load(path: str = LIB_PATH):
     L.dtd_nn_pool.argtypes = [_vp, _i, _i, _i, _i, _vp, _vp]
     L.dtd_nn_up_cat.argtypes = [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp]
     L.dtd_nn_up_cat_bwd.argtypes = [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]
-    L.dtd_nn_input.argtypes = [_vp, _sz, _vp, _vp, _d, _d, _d, _vp]
-    L.dtd_nn_loss.argtypes = [_vp, _sz, _vp, _vp, _i, _vp, _d, _vp, _vp,
+    L.dtd_nn_input.argtypes = [_vp, _sz, _vp, _vp, _vp, _d, _d, _d, _vp]
+    L.dtd_nn_loss.argtypes = [_vp, _sz, _vp, _i, _vp, _vp, _d, _vp, _vp,
                               ctypes.POINTER(ctypes.c_double)]
     L.dtd_nn_guess.argtypes = [_vp, _sz, _vp, _vp, _i, _d, _vp]
+    L.dtd_lod_device.argtypes = [_vp, _sz, _sz, _d, _i, _vp, _vp, _d, _vp]
     L.dtd_nn_adam.argtypes = [_vp, _sz, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _i, _vp]
     L.dtd_nn_to_bf16.argtypes = [_vp, _sz, _vp, _vp]
     L.dtd_sync.argtypes = [_vp]
@@ -307,6 +308,13 @@ def solve_rel_guess_device(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float, rel
                                                _ptr(h_dev), _ptr(kd_dev), dt, rel_tol,
                                                _ptr(x0_dev), ctypes.byref(st)))
     return st
+
+
+def lod_device(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float, out_dev) -> None:
+    """The splitting's first guess of h_dev into out_dev (device arrays), on the context's
+    stream."""
+    ws.check(load().dtd_lod_device(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x), _ptr(h_dev),
+                                   _ptr(kd_dev), dt, _ptr(out_dev)))
 
 
 def fill_kd(ws: Workspace, n: int, h_dev, class_dev, diffu_dev, kd_land: float, sea: float,

@@ -1,14 +1,16 @@
 """The learned first guess: a 4-level U-Net that predicts h1 from (h0, ln kd, dt), trained on the
 multigrid's own solves, run on the device by the library's network kernels (include/dtdiff.h,
 `dtd_nn_*`).  No reference counterpart: the reference starts its conjugate gradients from the
-splitting (crates/deeptime-core/src/surface/diffusion.rs:110-114).  A guess only changes where the
-solve starts; `solve_with_guess` ends on the same true f64 residual test as the plain solve.
+splitting (crates/deeptime-core/src/surface/diffusion.rs:110-114).  The network refines that
+splitting: it sees it as a fourth input and predicts the correction h1 - splitting, which is added
+to the splitting in f64.  A guess only changes where the solve starts; `solve_with_guess` ends on
+the same true f64 residual test as the plain solve.
 
 Network: 3x3 convolutions (periodic in x, zero-padded past the pole rows, as the grid), 32 / 64 /
 128 / 256 channels over four levels, 2x2 mean pooling down, nearest upsampling up with skip
 concatenation, ReLU; bf16 weights and activations with f32 accumulation on the matrix cores, f32
-master weights and Adam.  Input channels: h0 / SCALE, ln kd, ln(dt kd / dx^2) / 4; output channel
-0 is the change (h1 - h0) / SCALE, channel 1 a spare trained towards nothing.
+master weights and Adam.  Input channels: h0 / SCALE, ln kd, ln(dt kd / dx^2) / 4, splitting /
+SCALE; output channel 0 is the correction (h1 - splitting) / SCALE, channel 1 a spare.
 
     python -m dtdiff.learned train --steps 20000 --out weights.npz   (on a GPU)
 """
@@ -21,7 +23,8 @@ import time
 
 import numpy as np
 
-from . import Dims, Stats, Workspace, load, solve_rel_device, solve_rel_guess_device
+from . import (Dims, Stats, Workspace, load, lod_device, solve_rel_device,
+               solve_rel_guess_device)
 from .hipmem import DeviceArray, synchronize
 
 SCALE = 1000.0  # metres: the k^-2 fields' standard deviation
@@ -144,11 +147,11 @@ class UNet:
         self._chk(self.L.dtd_nn_up_cat(self.ws._ctx, n, h, w, cu, cs, self._p(u),
                                        self._p(skip), self._p(cat)))
 
-    def set_input(self, d: Dims, h_dev, kd_dev, dt: float, n: int = 1):
-        """The input channels from device arrays h (n images stacked) and kd."""
+    def set_input(self, d: Dims, h_dev, kd_dev, base_dev, dt: float, n: int = 1):
+        """The input channels from device arrays h (n images stacked), kd and the base guess."""
         self._alloc(n, d.h, d.w)
         self._chk(self.L.dtd_nn_input(self.ws._ctx, n * d.w * d.h, self._p(h_dev),
-                                      self._p(kd_dev), dt, d.dx, SCALE,
+                                      self._p(kd_dev), self._p(base_dev), dt, d.dx, SCALE,
                                       self._p(self.B["x0"])))
 
     def forward(self):
@@ -226,13 +229,14 @@ class UNet:
         self._back("enc0a", 0, B["x0"], B["e0a"], g0a, g0b)
 
     # -- training and the guess ----------------------------------------------------------------
-    def loss(self, t_dev) -> tuple[float, DeviceArray]:
-        """Mean squared error against t (device, f64, metres) and the output gradient."""
+    def loss(self, t_dev, base_dev) -> tuple[float, DeviceArray]:
+        """Mean squared error of the predicted correction against (t - base) / SCALE (device, f64,
+        metres) and the output gradient."""
         n, h, w = self.shape
         out = ctypes.c_double()
         dy = self.B["g0b"]  # 16-channel view of a level-0 gradient buffer
-        self._chk(self.L.dtd_nn_loss(self.ws._ctx, n * h * w, self._p(self.B["x0"]),
-                                     self._p(self.B["out32"]), OUTC, self._p(t_dev), SCALE,
+        self._chk(self.L.dtd_nn_loss(self.ws._ctx, n * h * w, self._p(self.B["out32"]), OUTC,
+                                     self._p(t_dev), self._p(base_dev), SCALE,
                                      self._p(self.B["parts"]), self._p(dy), ctypes.byref(out)))
         return out.value, dy
 
@@ -243,10 +247,12 @@ class UNet:
                                      self.step_count, self._p(self.wb)))
 
     def guess(self, d: Dims, h_dev, kd_dev, dt: float, g_dev) -> None:
-        """g = the network's h1 for (h, kd, dt) on the device (one image)."""
-        self.set_input(d, h_dev, kd_dev, dt)
+        """g = the network's h1 for (h, kd, dt) on the device (one image): the splitting into g,
+        then g += SCALE x the predicted correction."""
+        lod_device(self.ws, d, h_dev, kd_dev, dt, g_dev)
+        self.set_input(d, h_dev, kd_dev, g_dev, dt)
         self.forward()
-        self._chk(self.L.dtd_nn_guess(self.ws._ctx, d.w * d.h, self._p(h_dev),
+        self._chk(self.L.dtd_nn_guess(self.ws._ctx, d.w * d.h, self._p(g_dev),
                                       self._p(self.B["out32"]), OUTC, SCALE, self._p(g_dev)))
 
     def flat_grad(self) -> np.ndarray:
@@ -255,15 +261,31 @@ class UNet:
         return self.g.numpy()
 
 
+def _bias_mask() -> np.ndarray:
+    offs, n = offsets()
+    m = np.zeros(n, dtype=bool)
+    for name, ci, co, _ in LAYERS:
+        m[offs[name][1]:offs[name][1] + co] = True
+    return m
+
+
 def save(path: str, params: np.ndarray, **meta) -> None:
+    """The weights as the network computes with them (bf16, stored as their upper 16 bits) and
+    the biases in f32."""
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
-    np.savez_compressed(path, params=params.astype(np.float32),
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    u = p.view(np.uint32).astype(np.uint64)
+    bits = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF).astype(np.uint16)
+    mask = _bias_mask()
+    np.savez_compressed(path, weights_bf16=bits, biases=p[mask],
                         **{k: np.asarray(v) for k, v in meta.items()})
 
 
 def load_params(path: str = WEIGHTS) -> np.ndarray:
     with np.load(path, allow_pickle=False) as z:
-        return z["params"].astype(np.float32)
+        flat = (z["weights_bf16"].astype(np.uint32) << 16).view(np.float32).copy()
+        flat[_bias_mask()] = z["biases"]
+    return flat
 
 
 def solve_with_guess(ws: Workspace, net: UNet, d: Dims, h_dev, kd_dev, dt: float,
@@ -296,7 +318,7 @@ class Samples:
         self.kd = [np.ones(w * h)] + [F.lognormal_k(w, h, seed * 1000 + 500 + i)
                                       for i in range(pool - 1)]
         self.rng = np.random.default_rng(seed)
-        self.h_dev, self.k_dev, self.t_dev = (DeviceArray(w * h) for _ in range(3))
+        self.h_dev, self.k_dev, self.t_dev, self.b_dev = (DeviceArray(w * h) for _ in range(4))
 
     def draw(self):
         self.ws.check(load().dtd_sync(self.ws._ctx))  # the last step's kernels are done with
@@ -316,6 +338,7 @@ class Samples:
         self.k_dev.upload(kk.reshape(-1))
         self.t_dev.copy_from(self.h_dev)
         st = solve_rel_device(self.ws, self.d, self.t_dev, self.k_dev, ratio, 1e-9)
+        lod_device(self.ws, self.d, self.h_dev, self.k_dev, ratio, self.b_dev)
         return ratio, st
 
 
@@ -328,9 +351,9 @@ def train(steps: int, out: str, lr: float = 1e-3, w: int = 1024, h: int = 512, p
     run = 0.0
     for k in range(1, steps + 1):
         ratio, _ = S.draw()
-        net.set_input(S.d, S.h_dev, S.k_dev, ratio)
+        net.set_input(S.d, S.h_dev, S.k_dev, S.b_dev, ratio)
         net.forward()
-        loss, dy = net.loss(S.t_dev)
+        loss, dy = net.loss(S.t_dev, S.b_dev)
         net.backward(dy)
         # Cosine decay from lr to lr / 20.
         net.adam(lr * (0.05 + 0.95 * 0.5 * (1.0 + np.cos(np.pi * k / steps))))
@@ -343,6 +366,32 @@ def train(steps: int, out: str, lr: float = 1e-3, w: int = 1024, h: int = 512, p
     return params
 
 
+def evaluate(params: np.ndarray, count: int = 8, w: int = 1024, h: int = 512, seed: int = 99,
+             rel_tol: float = 1e-10) -> list[dict]:
+    """Held-out samples (fields from seeds outside the training pools): the CG iterations from the
+    splitting and from the network's guess, each solve to the same true relative residual."""
+    ws = Workspace()
+    S = Samples(ws, w, h, pool=max(2, count // 2), seed=seed)
+    net = UNet(ws, params)
+    rows = []
+    g = DeviceArray(w * h)
+    for k in range(count):
+        ratio, _ = S.draw()
+        x = DeviceArray.from_numpy(S.h_dev.numpy())
+        plain = solve_rel_device(ws, S.d, x, S.k_dev, ratio, rel_tol)
+        x.upload(S.h_dev.numpy())
+        st, ms = solve_with_guess(ws, net, S.d, x, S.k_dev, ratio, rel_tol, g)
+        synchronize()
+        t, gg, b = S.t_dev.numpy(), g.numpy(), S.b_dev.numpy()
+        rows.append({"ratio": ratio, "iterations_lod": plain.iterations,
+                     "iterations_net": st.iterations, "rel_lod": plain.rel_residual,
+                     "rel_net": st.rel_residual,
+                     "err_lod_m": float(np.sqrt(np.mean((b - t) ** 2))),
+                     "err_net_m": float(np.sqrt(np.mean((gg - t) ** 2))), "guess_ms": ms})
+    ws.close()
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -353,9 +402,16 @@ def main():
     t.add_argument("--w", type=int, default=1024)
     t.add_argument("--h", type=int, default=512)
     t.add_argument("--pool", type=int, default=32)
+    e = sub.add_parser("eval")
+    e.add_argument("--weights", default=WEIGHTS)
+    e.add_argument("--count", type=int, default=8)
     a = ap.parse_args()
     if a.cmd == "train":
         train(a.steps, a.out, a.lr, a.w, a.h, a.pool)
+    else:
+        for r in evaluate(load_params(a.weights), a.count):
+            print(" ".join(f"{k} {v:.4g}" if isinstance(v, float) else f"{k} {v}"
+                           for k, v in r.items()), flush=True)
 
 
 if __name__ == "__main__":

@@ -310,20 +310,24 @@ int dtd_nn_up_cat(dtd_ctx *ctx, int n, int h, int w, int cu, int cs, const uint1
 int dtd_nn_up_cat_bwd(dtd_ctx *ctx, int n, int h, int w, int cu, int cs, const uint16_t *dcat,
                       const uint16_t *dpool, uint16_t *du, uint16_t *dskip);
 
-/* The network's input x (16 channels) from f64 h and kd: h / scale, ln kd, ln(dt kd / dx^2) / 4,
- * zeros. */
-int dtd_nn_input(dtd_ctx *ctx, size_t np, const double *h, const double *kd, double dt,
-                 double dx, double scale, uint16_t *x);
+/* The network's input x (16 channels) from f64 h, kd and the base guess (optional): h / scale,
+ * ln kd, ln(dt kd / dx^2) / 4, base / scale (0 when base is NULL), zeros. */
+int dtd_nn_input(dtd_ctx *ctx, size_t np, const double *h, const double *kd, const double *base,
+                 double dt, double dx, double scale, uint16_t *x);
 
-/* Mean squared error of the prediction (x's channel 0 plus out[p outc]) against t / scale into
+/* Mean squared error of the network's out[p outc] against (t - base) / scale (formed in f64) into
  * *loss, and its gradient dy (16 channels, channel 0 only); parts holds np / 256 + 1 floats. */
-int dtd_nn_loss(dtd_ctx *ctx, size_t np, const uint16_t *x, const float *out, int outc,
-                const double *t, double scale, float *parts, uint16_t *dy, double *loss);
+int dtd_nn_loss(dtd_ctx *ctx, size_t np, const float *out, int outc, const double *t,
+                const double *base, double scale, float *parts, uint16_t *dy, double *loss);
 
-/* The guess g = scale (bf16(h / scale) + out[p outc]) in f64: the prediction dtd_nn_loss
- * trains. */
-int dtd_nn_guess(dtd_ctx *ctx, size_t np, const double *h, const float *out, int outc,
+/* The guess g = base + scale out[p outc] in f64. */
+int dtd_nn_guess(dtd_ctx *ctx, size_t np, const double *base, const float *out, int outc,
                  double scale, double *g);
+
+/* The splitting's first guess (as dtd_lod) of the device array h_dev into out_dev (device),
+ * asynchronous on the context's stream. */
+int dtd_lod_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, const double *h_dev,
+                   const double *kd_dev, double dt, double *out_dev);
 
 /* One Adam step (Kingma & Ba 2015, step counted from 1) on f32 masters w with moments m, v and
  * gradient g; wb (optional) gets bf16(w). */

@@ -103,7 +103,7 @@ LAYERS = [
     ("dec2a", 384, 128, 1), ("dec2b", 128, 128, 1), ("dec1a", 192, 64, 1), ("dec1b", 64, 64, 1),
     ("dec0a", 96, 32, 1), ("dec0b", 32, 32, 1), ("out", 32, 16, 0),
 ]
-OUTC = 2  # output channels kept in f32: the change to h, and a spare (trained to 0)
+OUTC = 2  # output channels kept in f32: the correction to the base guess, and a spare
 
 
 def offsets():
@@ -153,13 +153,13 @@ def forward(params, x0, bf=True):
     return z[..., :OUTC], A
 
 
-def loss(out32, x0, t, bf=True):
-    """Mean of (x0[..., 0] + out32[..., 0] - t)^2 and its gradient dy (n, h, w, 16)."""
+def loss(out32, target, bf=True):
+    """Mean of (out32[..., 0] - target)^2 (target = (h1 - base) / scale) and its gradient dy
+    (n, h, w, 16)."""
     r = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
-    pred = r(x0)[..., 0].astype(np.float64) + out32[..., 0]
-    d = pred - t
+    d = out32[..., 0] - target
     npx = d.size
-    dy = np.zeros(x0.shape)
+    dy = np.zeros(out32.shape[:3] + (16,))
     dy[..., 0] = 2.0 * d / npx
     return float(np.mean(d * d)), r(dy)
 
@@ -210,11 +210,14 @@ def flat_grad(G):
     return g
 
 
-def input_channels(h, kd, dt, dx, scale):
-    """The network's 16-channel input: h / scale, ln kd, ln(dt kd / dx^2) / 4, zeros."""
+def input_channels(h, kd, dt, dx, scale, base=None):
+    """The network's 16-channel input: h / scale, ln kd, ln(dt kd / dx^2) / 4, base / scale (0
+    without a base), zeros."""
     k = np.maximum(kd, 1e-30)
     x = np.zeros(h.shape + (16,))
     x[..., 0] = h / scale
     x[..., 1] = np.log(k)
     x[..., 2] = 0.25 * np.log(dt * k / (dx * dx))
+    if base is not None:
+        x[..., 3] = base / scale
     return x

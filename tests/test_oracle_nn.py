@@ -72,7 +72,7 @@ def test_network_gradient_matches_finite_differences():
     x0[..., 3:] = 0.0
     t = rng.standard_normal((1, 8, 16))
     out, A = N.forward(N.unpack(flat), x0, bf=False)
-    L0, dy = N.loss(out, x0, t, bf=False)
+    L0, dy = N.loss(out, t, bf=False)
     g = N.flat_grad(N.backward(N.unpack(flat), A, dy, bf=False))
     assert np.abs(g).max() > 0
     # Every layer's weights and bias along a random direction; steps of 1e-7 stay clear of the
@@ -83,8 +83,8 @@ def test_network_gradient_matches_finite_differences():
         a, b = offs[name]
         v = np.zeros(npar)
         v[a:b + co] = rng.standard_normal(b + co - a)
-        lp, _ = N.loss(N.forward(N.unpack(flat + eps * v), x0, bf=False)[0], x0, t, bf=False)
-        lm, _ = N.loss(N.forward(N.unpack(flat - eps * v), x0, bf=False)[0], x0, t, bf=False)
+        lp, _ = N.loss(N.forward(N.unpack(flat + eps * v), x0, bf=False)[0], t, bf=False)
+        lm, _ = N.loss(N.forward(N.unpack(flat - eps * v), x0, bf=False)[0], t, bf=False)
         fd = (lp - lm) / (2 * eps)
         assert abs(fd - g @ v) <= 1e-6 * abs(fd), (name, fd, g @ v)
 
@@ -97,3 +97,5 @@ def test_input_channels():
     assert np.allclose(x[..., 1], [[0.0, 1.0]])
     assert np.allclose(x[..., 2], 0.25 * np.log(10.0 * kd))
     assert not x[..., 3:].any()
+    x = N.input_channels(h, kd, 10.0, 1.0, 1000.0, base=np.array([[250.0, 0.0]]))
+    assert np.allclose(x[..., 3], [[0.25, 0.0]]) and not x[..., 4:].any()
