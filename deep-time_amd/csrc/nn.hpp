@@ -9,7 +9,7 @@
 // inputs and cout outputs: wt[co][tap][ci] (tap = 3 (dy + 1) + dx + 1), bf16, with f32 masters.
 //
 // The convolution is an implicit GEMM on the matrix cores: pixels x (9 cin) times (9 cin) x cout,
-// v_mfma_f32_16x16x16_bf16 with f32 accumulation, one wave a tile of 16 pixels and up to 64
+// v_mfma_f32_16x16x16_bf16 with f32 accumulation, one wave a tile of up to 64 pixels and 64
 // outputs.  Its backward pass for the input is the same kernel with the taps mirrored and the
 // weights' channel roles swapped (the adjoint of "read x at p + d, wrapping in x, zero past the
 // rows" is "read at p - d" under the same rules).  The weights' gradient is a GEMM over pixels:
@@ -44,49 +44,65 @@ __device__ __forceinline__ long pixels(const Shape &s) { return (long)s.n * s.h 
 
 // y[p][co] = act(bias[co] + sum over taps and ci of x[p + d(tap)][ci] wt[co][tap][ci]); act is
 // max(0, .) when relu.  y32 (optional) takes the first y32c outputs in f32 as well.  One wave a
-// tile of 16 pixels and NT * 16 outputs (blockIdx.y picks the outputs' group).  CIN > 0 fixes the
-// input channels at compile time, so that the tap and channel loops unroll and their loads issue
-// ahead of the matrix instructions; CIN = 0 reads cin.
-template <int NT, int CIN>
+// tile of MT x 16 consecutive pixels of one row and NT x 16 outputs (blockIdx.y picks the
+// outputs' group): each weight fragment it loads feeds MT matrix instructions and each input
+// fragment NT.  CIN > 0 fixes the input channels at compile time, so that the tap and channel
+// loops unroll and their loads issue ahead of the matrix instructions; CIN = 0 reads cin.
+template <int NT, int MT, int CIN>
 __global__ __launch_bounds__(256) void k_conv3x3(Shape s, int cin_, int cout,
                                                  const uint16_t *__restrict__ x,
                                                  const uint16_t *__restrict__ wt,
                                                  const float *__restrict__ bias, int relu,
                                                  uint16_t *__restrict__ y, float *__restrict__ y32,
                                                  int y32c) {
+    constexpr int TP = 16 * MT;  // pixels a wave
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long tile = (long)blockIdx.x * 4 + wave;
-    if (tile * 16 >= pixels(s)) return;
+    if (tile * TP >= pixels(s)) return;
     const int co0 = blockIdx.y * 16 * NT;
-    const long p0 = tile * 16;
+    const long p0 = tile * TP;
     const int col0 = (int)(p0 % s.w);
     const long rowi = p0 / s.w;  // image row index over all images
     const int row = (int)(rowi % s.h);
     const long img = rowi / s.h;
     const int m = lane & 15, kq = (lane >> 4) * 4;
     const int cin = CIN > 0 ? CIN : cin_;
-    constexpr int U = CIN > 0 && CIN <= 64 ? (CIN > 0 ? CIN / 16 : 1) : 4;
-    f4_t acc[NT];
+    constexpr int U = CIN > 0 && CIN <= 64 ? (CIN > 0 ? CIN / 16 : 1) : 2;
+    f4_t acc[MT][NT];
 #pragma unroll
-    for (int t = 0; t < NT; t++) acc[t] = f4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[i][t] = f4_t{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int tap = 0; tap < 9; tap++) {
         const int dy = tap / 3 - 1, dx = tap % 3 - 1;
         const int yy = row + dy;
         const bool inside = yy >= 0 && yy < s.h;
-        int xx = col0 + m + dx;
-        xx = xx < 0 ? xx + s.w : (xx >= s.w ? xx - s.w : xx);
-        const uint16_t *xp = inside ? x + ((img * s.h + yy) * s.w + xx) * (long)cin : nullptr;
+        const uint16_t *xp[MT];
+#pragma unroll
+        for (int i = 0; i < MT; i++) {
+            int xx = col0 + 16 * i + m + dx;
+            xx = xx < 0 ? xx + s.w : (xx >= s.w ? xx - s.w : xx);
+            xp[i] = inside ? x + ((img * s.h + yy) * s.w + xx) * (long)cin : nullptr;
+        }
         const uint16_t *wp = wt + ((size_t)(co0 + m) * 9 + tap) * cin + kq;
 #pragma unroll U
         for (int c0 = 0; c0 < cin; c0 += 16) {
-            s4_t a = s4_t{0, 0, 0, 0};
-            if (inside) a = *reinterpret_cast<const s4_t *>(xp + c0 + kq);
+            s4_t a[MT], b[NT];
 #pragma unroll
-            for (int t = 0; t < NT; t++) {
-                const s4_t b = *reinterpret_cast<const s4_t *>(wp + (size_t)16 * t * 9 * cin + c0);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+            for (int i = 0; i < MT; i++) {
+                a[i] = s4_t{0, 0, 0, 0};
+                if (inside) a[i] = *reinterpret_cast<const s4_t *>(xp[i] + c0 + kq);
             }
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+                b[t] = *reinterpret_cast<const s4_t *>(wp + (size_t)16 * t * 9 * cin + c0);
+#pragma unroll
+            for (int i = 0; i < MT; i++)
+#pragma unroll
+                for (int t = 0; t < NT; t++)
+                    acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[t], acc[i][t],
+                                                                          0, 0, 0);
         }
     }
 #pragma unroll
@@ -94,13 +110,15 @@ __global__ __launch_bounds__(256) void k_conv3x3(Shape s, int cin_, int cout,
         const int co = co0 + 16 * t + m;
         const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const long p = p0 + kq + q;
-            float v = acc[t][q] + bv;
-            if (relu) v = fmaxf(v, 0.0f);
-            y[p * cout + co] = f2bf(v);
-            if (y32 && co < y32c) y32[p * y32c + co] = v;
-        }
+        for (int i = 0; i < MT; i++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const long p = p0 + 16 * i + kq + q;
+                float v = acc[i][t][q] + bv;
+                if (relu) v = fmaxf(v, 0.0f);
+                y[p * cout + co] = f2bf(v);
+                if (y32 && co < y32c) y32[p * y32c + co] = v;
+            }
     }
 }
 
@@ -395,9 +413,14 @@ template <int NT, int CIN>
 static void conv_launch_c(hipStream_t s, Shape sh, int cin, int cout, const uint16_t *x,
                           const uint16_t *wt, const float *bias, int relu, uint16_t *y,
                           float *y32, int y32c) {
-    const long tiles = (long)sh.n * sh.h * sh.w / 16;
-    dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)(cout / (16 * NT)));
-    k_conv3x3<NT, CIN><<<grid, 256, 0, s>>>(sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+    const long np = (long)sh.n * sh.h * sh.w;
+    if (sh.w % 64 == 0) {  // four 16-pixel tiles a wave, in one row
+        dim3 grid((unsigned)((np / 64 + 3) / 4), (unsigned)(cout / (16 * NT)));
+        k_conv3x3<NT, 4, CIN><<<grid, 256, 0, s>>>(sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+    } else {
+        dim3 grid((unsigned)((np / 16 + 3) / 4), (unsigned)(cout / (16 * NT)));
+        k_conv3x3<NT, 1, CIN><<<grid, 256, 0, s>>>(sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+    }
     LAUNCH_CHECK();
 }
 

@@ -62,11 +62,13 @@ def close_bf16(got, want, frac=0.98):
     assert np.mean(got == want_b) >= frac
 
 
-@pytest.mark.parametrize("ci,co,relu", [(16, 16, 0), (16, 32, 1), (48, 64, 1), (32, 80, 0),
-                                        (96, 32, 1)])
-def test_conv_forward(ws, ci, co, relu):
-    rng = np.random.default_rng(ci * 7 + co)
-    n, h, w = 2, 6, 32
+@pytest.mark.parametrize("ci,co,relu,w", [(16, 16, 0, 32), (16, 32, 1, 32), (48, 64, 1, 32),
+                                          (32, 80, 0, 32), (96, 32, 1, 32), (32, 64, 1, 128),
+                                          (16, 16, 0, 64), (384, 128, 1, 192)])
+def test_conv_forward(ws, ci, co, relu, w):
+    """w a multiple of 64 takes the four-tile kernel, others the one-tile kernel."""
+    rng = np.random.default_rng(ci * 7 + co + w)
+    n, h = 2, 6
     x = N.bf16(rng.standard_normal((n, h, w, ci)))
     wt = N.bf16(rng.standard_normal((co, 9, ci)) / np.sqrt(9 * ci))
     b = rng.standard_normal(co).astype(np.float32)
@@ -82,10 +84,10 @@ def test_conv_forward(ws, ci, co, relu):
     assert np.allclose(get_f32(y32, (n, h, w, 2)), want[..., :2], rtol=1e-5, atol=1e-5)
 
 
-@pytest.mark.parametrize("ci,co", [(16, 32), (64, 48), (32, 16)])
-def test_conv_backward(ws, ci, co):
-    rng = np.random.default_rng(ci + 3 * co)
-    n, h, w = 2, 8, 48
+@pytest.mark.parametrize("ci,co,w", [(16, 32, 48), (64, 48, 48), (32, 16, 48), (64, 128, 128)])
+def test_conv_backward(ws, ci, co, w):
+    rng = np.random.default_rng(ci + 3 * co + w)
+    n, h = 2, 8
     x = N.bf16(rng.standard_normal((n, h, w, ci)))
     wt = N.bf16(rng.standard_normal((co, 9, ci)) / np.sqrt(9 * ci))
     dz = N.bf16(rng.standard_normal((n, h, w, co)))
