@@ -1205,6 +1205,7 @@ int dtd_nn_conv3x3(dtd_ctx *ctx, int n, int h, int w, int cin, int cout, const u
         nn::check_shape(n, h, w), nn::check_channels(cin), nn::check_channels(cout);
         if (!x || !wt || !y) throw Error{DTD_EINVAL, "null array"};
         if (y32 && (y32c < 1 || y32c > cout)) throw Error{DTD_EINVAL, "y32c out of range"};
+        nn::check_aligned(x, 8), nn::check_aligned(wt, 8);
         nn::conv(ctx->stream, nn::Shape{n, h, w}, cin, cout, x, wt, bias, relu, y, y32, y32c);
     });
 }
@@ -1216,6 +1217,7 @@ int dtd_nn_conv3x3_bwd_input(dtd_ctx *ctx, int n, int h, int w, int cin, int cou
         begin(ctx);
         nn::check_shape(n, h, w), nn::check_channels(cin), nn::check_channels(cout);
         if (!dz || !wt || !wt_scratch || !dx) throw Error{DTD_EINVAL, "null array"};
+        nn::check_aligned(dz, 8), nn::check_aligned(wt_scratch, 8);
         const long nw = (long)cin * 9 * cout;
         nn::k_conv_flip<<<nn::blocks(nw), 256, 0, ctx->stream>>>(cin, cout, wt, wt_scratch);
         LAUNCH_CHECK();
@@ -1239,6 +1241,7 @@ int dtd_nn_conv3x3_bwd_weight(dtd_ctx *ctx, int n, int h, int w, int cin, int co
         begin(ctx);
         nn::check_shape(n, h, w), nn::check_channels(cin), nn::check_channels(cout);
         if (!x || !dz || !planes || !part || !dw) throw Error{DTD_EINVAL, "null array"};
+        nn::check_aligned(planes, 8);
         size_t need_part = 0;
         const size_t need_planes = dtd_nn_bwd_weight_scratch(n, h, w, cin, cout, &need_part);
         if (planes_elems < need_planes || part_elems < need_part)
@@ -1271,8 +1274,10 @@ int dtd_nn_relu_bwd(dtd_ctx *ctx, size_t n, const uint16_t *y, const uint16_t *d
     return guard(ctx, [&] {
         begin(ctx);
         if (!y || !dy || !dz) throw Error{DTD_EINVAL, "null array"};
+        nn::check_aligned(y, 16), nn::check_aligned(dy, 16), nn::check_aligned(dz, 16);
         if (n == 0) return;
-        nn::k_relu_bwd<<<nn::blocks((long)n), 256, 0, ctx->stream>>>((long)n, y, dy, dz);
+        nn::k_relu_bwd<<<nn::blocks(((long)n + 7) / 8), 256, 0, ctx->stream>>>((long)n, y, dy,
+                                                                                dz);
         LAUNCH_CHECK();
     });
 }
@@ -1283,7 +1288,8 @@ int dtd_nn_pool(dtd_ctx *ctx, int n, int h, int w, int c, const uint16_t *x, uin
         nn::check_shape(n, h, w), nn::check_channels(c);
         if (h % 2 || w % 2) throw Error{DTD_EINVAL, "pooling needs even sizes"};
         if (!x || !y) throw Error{DTD_EINVAL, "null array"};
-        nn::k_pool<<<nn::blocks((long)n * (h / 2) * (w / 2) * c), 256, 0, ctx->stream>>>(
+        nn::check_aligned(x, 16), nn::check_aligned(y, 16);
+        nn::k_pool<<<nn::blocks((long)n * (h / 2) * (w / 2) * (c / 8)), 256, 0, ctx->stream>>>(
             nn::Shape{n, h, w}, c, x, y);
         LAUNCH_CHECK();
     });
@@ -1296,7 +1302,8 @@ int dtd_nn_up_cat(dtd_ctx *ctx, int n, int h, int w, int cu, int cs, const uint1
         nn::check_shape(n, h, w), nn::check_channels(cu), nn::check_channels(cs);
         if (h % 2 || w % 2) throw Error{DTD_EINVAL, "upsampling needs even sizes"};
         if (!u || !skip || !cat) throw Error{DTD_EINVAL, "null array"};
-        nn::k_up_cat<<<nn::blocks((long)n * h * w * (cu + cs)), 256, 0, ctx->stream>>>(
+        nn::check_aligned(u, 16), nn::check_aligned(skip, 16), nn::check_aligned(cat, 16);
+        nn::k_up_cat<<<nn::blocks((long)n * h * w * ((cu + cs) / 8)), 256, 0, ctx->stream>>>(
             nn::Shape{n, h, w}, cu, cs, u, skip, cat);
         LAUNCH_CHECK();
     });
@@ -1309,14 +1316,16 @@ int dtd_nn_up_cat_bwd(dtd_ctx *ctx, int n, int h, int w, int cu, int cs, const u
         nn::check_shape(n, h, w), nn::check_channels(cu), nn::check_channels(cs);
         if (h % 2 || w % 2) throw Error{DTD_EINVAL, "upsampling needs even sizes"};
         if (!dcat) throw Error{DTD_EINVAL, "null array"};
+        nn::check_aligned(dcat, 16), nn::check_aligned(dpool, 16);
+        nn::check_aligned(du, 16), nn::check_aligned(dskip, 16);
         const nn::Shape sh{n, h, w};
         if (du) {
-            nn::k_up_cat_bwd_u<<<nn::blocks((long)n * (h / 2) * (w / 2) * cu), 256, 0,
+            nn::k_up_cat_bwd_u<<<nn::blocks((long)n * (h / 2) * (w / 2) * (cu / 8)), 256, 0,
                                  ctx->stream>>>(sh, cu, cs, dcat, du);
             LAUNCH_CHECK();
         }
         if (dskip) {
-            nn::k_skip_bwd<<<nn::blocks((long)n * h * w * cs), 256, 0, ctx->stream>>>(
+            nn::k_skip_bwd<<<nn::blocks((long)n * h * w * (cs / 8)), 256, 0, ctx->stream>>>(
                 sh, cu, cs, dcat, dpool, dskip);
             LAUNCH_CHECK();
         }

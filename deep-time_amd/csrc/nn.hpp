@@ -131,12 +131,66 @@ __global__ __launch_bounds__(256) void k_conv_flip(int cin, int cout, const uint
     wt_t[i] = wt[((size_t)co * 9 + (8 - tap)) * cin + ci];
 }
 
-// dz = dy where y > 0, else 0 (the ReLU's backward pass on its output).
+// Eight bf16 values in 16 bytes (element 2k in the low half of word k), for the elementwise
+// kernels: one thread a group of eight channels.
+typedef uint32_t u4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void unpack8(u4_t v, float *f) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        f[2 * k] = __uint_as_float(v[k] << 16);
+        f[2 * k + 1] = __uint_as_float(v[k] & 0xFFFF0000u);
+    }
+}
+
+__device__ __forceinline__ u4_t pack8(const float *f) {
+    u4_t v;
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = (uint32_t)f2bf(f[2 * k]) | ((uint32_t)f2bf(f[2 * k + 1]) << 16);
+    return v;
+}
+
+__device__ __forceinline__ u4_t ld8(const uint16_t *p) { return *reinterpret_cast<const u4_t *>(p); }
+__device__ __forceinline__ void st8(uint16_t *p, u4_t v) { *reinterpret_cast<u4_t *>(p) = v; }
+
+// The parent (h/2 x w/2 grid) of full-size pixel p.
+__device__ __forceinline__ long parent_of(const Shape &s, long p) {
+    const int col = (int)(p % s.w);
+    const long r = p / s.w;
+    const int row = (int)(r % s.h);
+    const long img = r / s.h;
+    return (img * (s.h / 2) + row / 2) * (s.w / 2) + col / 2;
+}
+
+// The first of the four children (full-size pixel) of half-size pixel pc.
+__device__ __forceinline__ long child_of(const Shape &s, long pc) {
+    const int hh = s.h / 2, ww = s.w / 2;
+    const int xc = (int)(pc % ww);
+    const long r = pc / ww;
+    const int yc = (int)(r % hh);
+    const long img = r / hh;
+    return (img * s.h + 2 * yc) * s.w + 2 * xc;
+}
+
+// dz = dy where y > 0, else 0 (the ReLU's backward pass on its output); eight a thread.
 __global__ __launch_bounds__(256) void k_relu_bwd(long n, const uint16_t *__restrict__ y,
                                                   const uint16_t *__restrict__ dy,
                                                   uint16_t *__restrict__ dz) {
-    const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) dz[i] = bf2f(y[i]) > 0.0f ? dy[i] : (uint16_t)0;
+    const long i = ((long)blockIdx.x * 256 + threadIdx.x) * 8;
+    if (i >= n) return;
+    if (i + 8 <= n) {
+        const u4_t yv = ld8(y + i), dv = ld8(dy + i);
+        u4_t o;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t lo = __uint_as_float(yv[k] << 16) > 0.0f ? 0x0000FFFFu : 0u;
+            const uint32_t hi = __uint_as_float(yv[k] & 0xFFFF0000u) > 0.0f ? 0xFFFF0000u : 0u;
+            o[k] = dv[k] & (lo | hi);
+        }
+        st8(dz + i, o);
+    } else {
+        for (long j = i; j < n; j++) dz[j] = bf2f(y[j]) > 0.0f ? dy[j] : (uint16_t)0;
+    }
 }
 
 // xt[ch][p] = x[p][ch] for all pixels p (c channels, tiles of 64 pixels x 16 channels).
@@ -233,86 +287,75 @@ __global__ __launch_bounds__(256) void k_db(long np, const uint16_t *__restrict_
     if (threadIdx.x == 0) db[blockIdx.x] = t[0];
 }
 
-// 2 x 2 mean: y (h/2 x w/2) from x (h x w), c channels.
+// 2 x 2 mean: y (h/2 x w/2) from x (h x w), c channels; eight a thread.
 __global__ __launch_bounds__(256) void k_pool(Shape s, int c, const uint16_t *__restrict__ x,
                                               uint16_t *__restrict__ y) {
-    const int hh = s.h / 2, ww = s.w / 2;
+    const int g = c / 8;
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= (long)s.n * hh * ww * c) return;
-    const int ch = (int)(i % c);
-    const long pc = i / c;
-    const int xc = (int)(pc % ww);
-    const long r = pc / ww;
-    const int yc = (int)(r % hh);
-    const long img = r / hh;
-    const long base = ((img * s.h + 2 * yc) * s.w + 2 * xc) * c + ch;
-    const float v = (bf2f(x[base]) + bf2f(x[base + c])) +
-                    (bf2f(x[base + (long)s.w * c]) + bf2f(x[base + (long)s.w * c + c]));
-    y[i] = f2bf(0.25f * v);
+    if (i >= (long)s.n * (s.h / 2) * (s.w / 2) * g) return;
+    const int ch = (int)(i % g) * 8;
+    const long pc = i / g;
+    const long base = child_of(s, pc) * c + ch, down = (long)s.w * c;
+    float a[8], b[8], d[8], e[8], o[8];
+    unpack8(ld8(x + base), a), unpack8(ld8(x + base + c), b);
+    unpack8(ld8(x + base + down), d), unpack8(ld8(x + base + down + c), e);
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = 0.25f * ((a[k] + b[k]) + (d[k] + e[k]));
+    st8(y + pc * c + ch, pack8(o));
 }
 
 // cat[p][0 .. cu) = u[parent(p)] (nearest, from the half-size grid), cat[p][cu .. cu + cs) =
-// skip[p]; s is the full-size shape.
+// skip[p]; s is the full-size shape; eight channels a thread.
 __global__ __launch_bounds__(256) void k_up_cat(Shape s, int cu, int cs, const uint16_t *__restrict__ u,
                                                 const uint16_t *__restrict__ skip,
                                                 uint16_t *__restrict__ cat) {
-    const int c = cu + cs;
+    const int c = cu + cs, g = c / 8;
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= pixels(s) * c) return;
-    const int ch = (int)(i % c);
-    const long p = i / c;
-    if (ch >= cu) {
-        cat[i] = skip[p * cs + (ch - cu)];
-        return;
-    }
-    const int col = (int)(p % s.w);
-    const long r = p / s.w;
-    const int row = (int)(r % s.h);
-    const long img = r / s.h;
-    const long pc = (img * (s.h / 2) + row / 2) * (s.w / 2) + col / 2;
-    cat[i] = u[pc * cu + ch];
+    if (i >= pixels(s) * g) return;
+    const int ch = (int)(i % g) * 8;
+    const long p = i / g;
+    st8(cat + p * c + ch, ch >= cu ? ld8(skip + p * cs + (ch - cu))
+                                   : ld8(u + parent_of(s, p) * cu + ch));
 }
 
 // The concat's backward pass: du[q][ch] = sum of dcat[p][ch] over q's four children (ch < cu),
 // and dskip[p][ch] = dcat[p][cu + ch] + 0.25 dpool[parent(p)][ch] (the skip's tensor also fed the
-// next level's pooling; dpool null: none).  s is the full-size shape.
+// next level's pooling; dpool null: none).  s is the full-size shape; eight channels a thread.
 __global__ __launch_bounds__(256) void k_up_cat_bwd_u(Shape s, int cu, int cs,
                                                       const uint16_t *__restrict__ dcat,
                                                       uint16_t *__restrict__ du) {
-    const int c = cu + cs, hh = s.h / 2, ww = s.w / 2;
+    const int c = cu + cs, g = cu / 8;
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= (long)s.n * hh * ww * cu) return;
-    const int ch = (int)(i % cu);
-    const long pc = i / cu;
-    const int xc = (int)(pc % ww);
-    const long r = pc / ww;
-    const int yc = (int)(r % hh);
-    const long img = r / hh;
-    const long base = ((img * s.h + 2 * yc) * s.w + 2 * xc) * c + ch;
-    const float v = (bf2f(dcat[base]) + bf2f(dcat[base + c])) +
-                    (bf2f(dcat[base + (long)s.w * c]) + bf2f(dcat[base + (long)s.w * c + c]));
-    du[i] = f2bf(v);
+    if (i >= (long)s.n * (s.h / 2) * (s.w / 2) * g) return;
+    const int ch = (int)(i % g) * 8;
+    const long pc = i / g;
+    const long base = child_of(s, pc) * c + ch, down = (long)s.w * c;
+    float a[8], b[8], d[8], e[8], o[8];
+    unpack8(ld8(dcat + base), a), unpack8(ld8(dcat + base + c), b);
+    unpack8(ld8(dcat + base + down), d), unpack8(ld8(dcat + base + down + c), e);
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = (a[k] + b[k]) + (d[k] + e[k]);
+    st8(du + pc * cu + ch, pack8(o));
 }
 
 __global__ __launch_bounds__(256) void k_skip_bwd(Shape s, int cu, int cs,
                                                   const uint16_t *__restrict__ dcat,
                                                   const uint16_t *__restrict__ dpool,
                                                   uint16_t *__restrict__ dskip) {
-    const int c = cu + cs;
+    const int c = cu + cs, g = cs / 8;
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= pixels(s) * cs) return;
-    const int ch = (int)(i % cs);
-    const long p = i / cs;
-    float v = bf2f(dcat[p * c + cu + ch]);
+    if (i >= pixels(s) * g) return;
+    const int ch = (int)(i % g) * 8;
+    const long p = i / g;
+    float v[8];
+    unpack8(ld8(dcat + p * c + cu + ch), v);
     if (dpool) {
-        const int col = (int)(p % s.w);
-        const long r = p / s.w;
-        const int row = (int)(r % s.h);
-        const long img = r / s.h;
-        const long pc = (img * (s.h / 2) + row / 2) * (s.w / 2) + col / 2;
-        v += 0.25f * bf2f(dpool[pc * cs + ch]);
+        float q[8];
+        unpack8(ld8(dpool + parent_of(s, p) * cs + ch), q);
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] += 0.25f * q[k];
     }
-    dskip[i] = f2bf(v);
+    st8(dskip + p * cs + ch, pack8(v));
 }
 
 // The network's input (16 channels, four used): h / scale, ln kd, ln(dt kd / dx^2) / 4 and the
@@ -408,6 +451,13 @@ static void check_channels(int c) {
 }
 
 static unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
+
+// The kernels read bf16 tensors eight or four values at a time.
+static void check_aligned(const void *p, size_t a) {
+    if (p && (reinterpret_cast<uintptr_t>(p) % a))
+        throw Error{DTD_EINVAL, a == 16 ? "network tensors must be 16-byte aligned"
+                                        : "network tensors must be 8-byte aligned"};
+}
 
 template <int NT, int CIN>
 static void conv_launch_c(hipStream_t s, Shape sh, int cin, int cout, const uint16_t *x,
