@@ -24,6 +24,7 @@ namespace dtd {
 namespace nn {
 
 typedef short s4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
@@ -47,8 +48,11 @@ __device__ __forceinline__ long pixels(const Shape &s) { return (long)s.n * s.h 
 // tile of MT x 16 consecutive pixels of one row and NT x 16 outputs (blockIdx.y picks the
 // outputs' group): each weight fragment it loads feeds MT matrix instructions and each input
 // fragment NT.  CIN > 0 fixes the input channels at compile time, so that the tap and channel
-// loops unroll and their loads issue ahead of the matrix instructions; CIN = 0 reads cin.
-template <int NT, int MT, int CIN>
+// loops unroll and their loads issue ahead of the matrix instructions; CIN = 0 reads cin.  KW = 32
+// (cin a multiple of 32) takes 32 channels a step with v_mfma_f32_16x16x32_bf16 (16-byte
+// fragments, lane l holding channels 8 (l / 16) .. + 7 of its row), KW = 16 takes 16 with
+// v_mfma_f32_16x16x16_bf16 (8-byte fragments, channels 4 (l / 16) .. + 3).
+template <int NT, int MT, int CIN, int KW>
 __global__ __launch_bounds__(256) void k_conv3x3(Shape s, int cin_, int cout,
                                                  const uint16_t *__restrict__ x,
                                                  const uint16_t *__restrict__ wt,
@@ -67,7 +71,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(Shape s, int cin_, int cout,
     const long img = rowi / s.h;
     const int m = lane & 15, kq = (lane >> 4) * 4;
     const int cin = CIN > 0 ? CIN : cin_;
-    constexpr int U = CIN > 0 && CIN <= 64 ? (CIN > 0 ? CIN / 16 : 1) : 2;
+    constexpr int U = CIN > 0 && CIN <= 128 ? (CIN / KW > 0 ? CIN / KW : 1) : 4;
     f4_t acc[MT][NT];
 #pragma unroll
     for (int i = 0; i < MT; i++)
@@ -85,24 +89,43 @@ __global__ __launch_bounds__(256) void k_conv3x3(Shape s, int cin_, int cout,
             xx = xx < 0 ? xx + s.w : (xx >= s.w ? xx - s.w : xx);
             xp[i] = inside ? x + ((img * s.h + yy) * s.w + xx) * (long)cin : nullptr;
         }
-        const uint16_t *wp = wt + ((size_t)(co0 + m) * 9 + tap) * cin + kq;
+        const int ko = KW == 32 ? 2 * kq : kq;  // this lane's first channel in a step
+        const uint16_t *wp = wt + ((size_t)(co0 + m) * 9 + tap) * cin + ko;
 #pragma unroll U
-        for (int c0 = 0; c0 < cin; c0 += 16) {
-            s4_t a[MT], b[NT];
+        for (int c0 = 0; c0 < cin; c0 += KW) {
+            if constexpr (KW == 32) {
+                b8_t a[MT], b[NT];
 #pragma unroll
-            for (int i = 0; i < MT; i++) {
-                a[i] = s4_t{0, 0, 0, 0};
-                if (inside) a[i] = *reinterpret_cast<const s4_t *>(xp[i] + c0 + kq);
-            }
-#pragma unroll
-            for (int t = 0; t < NT; t++)
-                b[t] = *reinterpret_cast<const s4_t *>(wp + (size_t)16 * t * 9 * cin + c0);
-#pragma unroll
-            for (int i = 0; i < MT; i++)
+                for (int i = 0; i < MT; i++) {
+                    a[i] = b8_t{};
+                    if (inside) a[i] = *reinterpret_cast<const b8_t *>(xp[i] + c0 + ko);
+                }
 #pragma unroll
                 for (int t = 0; t < NT; t++)
-                    acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[t], acc[i][t],
-                                                                          0, 0, 0);
+                    b[t] = *reinterpret_cast<const b8_t *>(wp + (size_t)16 * t * 9 * cin + c0);
+#pragma unroll
+                for (int i = 0; i < MT; i++)
+#pragma unroll
+                    for (int t = 0; t < NT; t++)
+                        acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[t], acc[i][t],
+                                                                            0, 0, 0);
+            } else {
+                s4_t a[MT], b[NT];
+#pragma unroll
+                for (int i = 0; i < MT; i++) {
+                    a[i] = s4_t{0, 0, 0, 0};
+                    if (inside) a[i] = *reinterpret_cast<const s4_t *>(xp[i] + c0 + ko);
+                }
+#pragma unroll
+                for (int t = 0; t < NT; t++)
+                    b[t] = *reinterpret_cast<const s4_t *>(wp + (size_t)16 * t * 9 * cin + c0);
+#pragma unroll
+                for (int i = 0; i < MT; i++)
+#pragma unroll
+                    for (int t = 0; t < NT; t++)
+                        acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[t],
+                                                                              acc[i][t], 0, 0, 0);
+            }
         }
     }
 #pragma unroll
@@ -464,13 +487,22 @@ static void conv_launch_c(hipStream_t s, Shape sh, int cin, int cout, const uint
                           const uint16_t *wt, const float *bias, int relu, uint16_t *y,
                           float *y32, int y32c) {
     const long np = (long)sh.n * sh.h * sh.w;
-    if (sh.w % 64 == 0) {  // four 16-pixel tiles a wave, in one row
-        dim3 grid((unsigned)((np / 64 + 3) / 4), (unsigned)(cout / (16 * NT)));
-        k_conv3x3<NT, 4, CIN><<<grid, 256, 0, s>>>(sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
-    } else {
-        dim3 grid((unsigned)((np / 16 + 3) / 4), (unsigned)(cout / (16 * NT)));
-        k_conv3x3<NT, 1, CIN><<<grid, 256, 0, s>>>(sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
-    }
+    const bool k32 = cin % 32 == 0 && reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(wt) % 16 == 0;
+    const bool m4 = sh.w % 64 == 0;  // four 16-pixel tiles a wave, in one row
+    dim3 grid((unsigned)((np / (m4 ? 64 : 16) + 3) / 4), (unsigned)(cout / (16 * NT)));
+    if (m4 && k32)
+        k_conv3x3<NT, 4, CIN, 32><<<grid, 256, 0, s>>>(sh, cin, cout, x, wt, bias, relu, y, y32,
+                                                       y32c);
+    else if (m4)
+        k_conv3x3<NT, 4, CIN, 16><<<grid, 256, 0, s>>>(sh, cin, cout, x, wt, bias, relu, y, y32,
+                                                       y32c);
+    else if (k32)
+        k_conv3x3<NT, 1, CIN, 32><<<grid, 256, 0, s>>>(sh, cin, cout, x, wt, bias, relu, y, y32,
+                                                       y32c);
+    else
+        k_conv3x3<NT, 1, CIN, 16><<<grid, 256, 0, s>>>(sh, cin, cout, x, wt, bias, relu, y, y32,
+                                                       y32c);
     LAUNCH_CHECK();
 }
 
