@@ -9,8 +9,8 @@
 // inputs and cout outputs: wt[co][tap][ci] (tap = 3 (dy + 1) + dx + 1), bf16, with f32 masters.
 //
 // The convolution is an implicit GEMM on the matrix cores: pixels x (9 cin) times (9 cin) x cout,
-// v_mfma_f32_16x16x16_bf16 with f32 accumulation, one wave a tile of up to 64 pixels and 64
-// outputs.  Its backward pass for the input is the same kernel with the taps mirrored and the
+// v_mfma_f32_16x16x32_bf16 (16x16x16 for 16-channel inputs) with f32 accumulation, one wave a
+// tile of up to 64 pixels and 64 outputs.  Its backward pass for the input is the same kernel with the taps mirrored and the
 // weights' channel roles swapped (the adjoint of "read x at p + d, wrapping in x, zero past the
 // rows" is "read at p - d" under the same rules).  The weights' gradient is a GEMM over pixels:
 // per chunk of pixels one wave a (16 outputs, 16 inputs, tap) tile on channel-major copies of the
