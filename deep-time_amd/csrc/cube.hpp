@@ -112,20 +112,40 @@ __global__ __launch_bounds__(kBlock) void kc_opinit(int l, int n6, const uint32_
 }
 
 // `Op::row`, CD:79-91.
-__device__ __forceinline__ double cube_row(int i, int l, const double *__restrict__ diag,
+// Cell i's eight slot neighbours.  With side > 0 the table is known to be the structured one (each
+// face row-major, slot q of an inner cell at i + DI[q] + DJ[q] side, checked by cube_new), and an
+// inner cell forms them; other cells read the slot-major table (kNoCell where there is none).
+__device__ __forceinline__ void cube_nbrs(int i, int l, int side,
+                                          const uint32_t *__restrict__ cols, uint32_t *j) {
+    bool inner = false;
+    if (side > 0) {
+        const int x = i % side, y = (i / side) % side;
+        inner = x > 0 && x < side - 1 && y > 0 && y < side - 1;
+    }
+    if (inner) {
+        const uint32_t c = (uint32_t)i, n = (uint32_t)side;
+        j[0] = c + 1, j[1] = c + 1 - n, j[2] = c - n, j[3] = c - 1 - n;
+        j[4] = c - 1, j[5] = c - 1 + n, j[6] = c + n, j[7] = c + 1 + n;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; q++) j[q] = cols[(size_t)q * l + i];
+    }
+}
+
+__device__ __forceinline__ double cube_row(int i, int l, int side, const double *__restrict__ diag,
                                            const double *__restrict__ k,
                                            const uint32_t *__restrict__ cols,
                                            const double *__restrict__ p) {
     double acc = diag[i] * p[i];
+    uint32_t j[8];
+    cube_nbrs(i, l, side, cols, j);
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const uint32_t j = cols[(size_t)q * l + i];
-        if (j != kNoCell) acc -= k[(size_t)q * l + i] * p[j];
-    }
+    for (int q = 0; q < 8; q++)
+        if (j[q] != kNoCell) acc -= k[(size_t)q * l + i] * p[j[q]];
     return acc;
 }
 
-__global__ __launch_bounds__(kBlock) void kc_apply(int l, const double *__restrict__ diag,
+__global__ __launch_bounds__(kBlock) void kc_apply(int l, int side, const double *__restrict__ diag,
                                                   const double *__restrict__ k,
                                                   const uint32_t *__restrict__ cols,
                                                   const double *__restrict__ p,
@@ -133,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void kc_apply(int l, const double *__restri
                                                   const double *__restrict__ sub) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= l) return;
-    const double a = cube_row(i, l, diag, k, cols, p);
+    const double a = cube_row(i, l, side, diag, k, cols, p);
     out[i] = sub ? sub[i] - a : a;  // sub: r = b - A x (CD:386-390)
 }
 
@@ -483,7 +503,8 @@ __global__ __launch_bounds__(kRowBlock) void kc_dot(int l, const double *__restr
 // The direction pass (CD:411-440), one thread a cell: p = z + beta p_old (z = r / diag for the
 // diagonal), A p, and the term p A p into t (added per chunk by kc_chunks).  Every neighbour's p
 // is formed again by the same expression.
-__global__ __launch_bounds__(kBlock) void kc_dir(int l, int mg, const double *__restrict__ r,
+__global__ __launch_bounds__(kBlock) void kc_dir(int l, int mg, int side,
+                                                const double *__restrict__ r,
                                                 const float *__restrict__ z32,
                                                 const double *__restrict__ diag,
                                                 const double *__restrict__ k,
@@ -503,11 +524,11 @@ __global__ __launch_bounds__(kBlock) void kc_dir(int l, int mg, const double *__
     };
     const double pi = dir(i);
     double acc = diag[i] * pi;
+    uint32_t j[8];
+    cube_nbrs(i, l, side, cols, j);
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const uint32_t j = cols[(size_t)q * l + i];
-        if (j != kNoCell) acc -= k[(size_t)q * l + i] * dir(j);
-    }
+    for (int q = 0; q < 8; q++)
+        if (j[q] != kNoCell) acc -= k[(size_t)q * l + i] * dir(j[q]);
     pnew[i] = pi;
     ap[i] = acc;
     t[i] = pi * acc;
@@ -685,6 +706,7 @@ __global__ void kc_init(CubeState *st, double tol2, int max_iter) {
 // loads of one entry are contiguous.
 struct CLvl {
     int cells, ne, w;                    // cells, CSR entries, ELL width
+    int side;                            // > 0: the finest level of a structured table
     const uint32_t *col;                 // ELL neighbours
     const uint32_t *ell_of;              // per CSR entry, its ELL position (level >= 1)
     const uint32_t *src_ptr, *src;       // per CSR entry, the finer ELL positions (level >= 1)
@@ -749,10 +771,9 @@ __global__ __launch_bounds__(kBlock) void kc_mg_r32(int n, const double *__restr
 template <int W>
 __device__ __forceinline__ float cube_az(const CLvl &L, const float *z, int i) {
     float az = L.diag[i] * z[i];
-    if constexpr (W > 0) {
+    if constexpr (W == 8) {
         uint32_t j[W];
-#pragma unroll
-        for (int q = 0; q < W; q++) j[q] = L.col[(size_t)q * L.cells + i];
+        cube_nbrs(i, L.cells, L.side, L.col, j);  // an inner cell's ELL row is its 8 slots
 #pragma unroll
         for (int q = 0; q < W; q++)
             if (j[q] != kNoCell) az -= L.val[(size_t)q * L.cells + i] * z[j[q]];
@@ -890,6 +911,7 @@ struct dtd_cube {
     int n = 0, n6 = 0;
     double mean_area = 0.0;
     std::vector<uint32_t> slots_h;  // host copy (the subset's neighbour positions)
+    bool structured = false;        // each face row-major with the D8 slots of the reference
     std::vector<void *> owned;
     uint32_t *slots = nullptr;
     double *lap = nullptr, *area = nullptr;
@@ -1003,6 +1025,24 @@ static dtd_cube *cube_new(dtd_ctx *ctx, int n, const uint32_t *slots, const doub
         g->n = n, g->n6 = 6 * n * n, g->mean_area = mean_area;
         const size_t n6 = (size_t)g->n6;
         g->slots_h.assign(slots, slots + 8 * n6);
+        // Is the table the structured one (each face row-major, slot q of an inner cell at
+        // c + DI[q] + DJ[q] n, D8 order as surface/grid.rs:21-31)?  Then inner cells form their
+        // neighbours instead of reading them.
+        {
+            static const int DI[8] = {1, 1, 0, -1, -1, -1, 0, 1}, DJ[8] = {0, -1, -1, -1, 0, 1, 1, 1};
+            bool st = n >= 3;
+            for (size_t f = 0; st && f < 6; f++)
+                for (int j = 1; st && j + 1 < n; j++)
+                    for (int i = 1; st && i + 1 < n; i++) {
+                        const size_t c = f * (size_t)n * n + (size_t)j * n + i;
+                        for (int q = 0; q < 8; q++)
+                            if (slots[8 * c + q] != (uint32_t)((long)c + DI[q] + (long)DJ[q] * n)) {
+                                st = false;
+                                break;
+                            }
+                    }
+            g->structured = st;
+        }
         auto own = [&](size_t bytes) {
             void *p = nullptr;
             HIPCHK(hipMalloc(&p, bytes ? bytes : 1));
@@ -1052,6 +1092,7 @@ static dtd_cube *cube_new(dtd_ctx *ctx, int n, const uint32_t *slots, const doub
             const CPat &p = pats[l];
             CLvl L{};
             L.cells = (int)p.cells, L.ne = (int)p.col.size(), L.w = width[l];
+            L.side = l == 0 && g->structured ? n : 0;
             L.col = (const uint32_t *)put(ecol[l].data(), ecol[l].size() * 4);
             if (!p.parent.empty()) {
                 std::vector<uint32_t> src(p.src.size());
@@ -1210,6 +1251,7 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
     }
     const uint32_t *cells = all ? nullptr : g->cells;
     const uint32_t *cols = all ? g->slots : g->cols;
+    const int side = all && g->structured ? n : 0;  // inner cells form their neighbours
     double *k = all ? g->kfull : g->kpos;
     const int nb = nblocks(l), nc = (l + kCubeChunk - 1) / kCubeChunk;
     // The chunk-sum kernels' grids: four chains a wave (one or two chains a chunk).
@@ -1252,7 +1294,7 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
     const double tol = mg ? tol_m : kCgTolM;
     kc_init<<<1, 64, 0, s>>>(g->st, tol * tol * (double)n6, (int)ctx->opts.max_iter);
     LAUNCH_CHECK();
-    kc_apply<<<nb, kBlock, 0, s>>>(l, g->diag, k, cols, g->x, g->r, g->rhs);
+    kc_apply<<<nb, kBlock, 0, s>>>(l, side, g->diag, k, cols, g->x, g->r, g->rhs);
     LAUNCH_CHECK();
     kc_dot<kDotF64><<<bc, kRowBlock, 0, s>>>(l, g->r, g->r, nullptr, nullptr, g->parts,
                                              nullptr);
@@ -1277,8 +1319,8 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
     double *po = g->p0, *pn = g->p1;
     for (int it = 0; it < max_iter; it++) {
         const int par = it & 1;
-        kc_dir<<<nb, kBlock, 0, s>>>(l, mg, g->r, z32, g->diag, k, cols, po, pn, g->ap, g->tmp,
-                                     g->st);
+        kc_dir<<<nb, kBlock, 0, s>>>(l, mg, side, g->r, z32, g->diag, k, cols, po, pn, g->ap,
+                                     g->tmp, g->st);
         LAUNCH_CHECK();
         kc_chunks<<<bc, kRowBlock, 0, s>>>(l, g->tmp, nullptr, 0.0, g->parts, g->st);
         LAUNCH_CHECK();
@@ -1308,7 +1350,7 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
         }
     }
     // The put-back (CD:286-301) and the answer into h.
-    kc_apply<<<nb, kBlock, 0, s>>>(l, g->diag, k, cols, g->x, g->ap, nullptr);
+    kc_apply<<<nb, kBlock, 0, s>>>(l, side, g->diag, k, cols, g->x, g->ap, nullptr);
     LAUNCH_CHECK();
     kc_putback_sums<<<bc2, kRowBlock, 0, s>>>(l, g->rhs, g->ap, g->s, g->parts);
     LAUNCH_CHECK();

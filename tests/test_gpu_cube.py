@@ -116,3 +116,36 @@ def test_cube_arguments(ws):
             cube.solve(x, kd, 1.0, 7, 1e-6)
     finally:
         cube.close()
+
+
+@pytest.mark.parametrize("solver", [dtdiff.DiffusionSolver.MULTIGRID,
+                                    dtdiff.DiffusionSolver.JACOBI])
+def test_cube_solve_on_a_relabelled_table(ws, solver):
+    """Two inner cells of different faces swap labels: the table is no longer the structured one,
+    so every cell reads its neighbours from the table, and the solve still gives the oracle's
+    bits on the same (relabelled) grid."""
+    import dataclasses
+
+    g0 = C.cube_grid(8, R)
+    a, b = 0 * 64 + 3 * 8 + 4, 3 * 64 + 5 * 8 + 2
+    perm = np.arange(g0.cells)
+    perm[a], perm[b] = b, a
+    slots = np.full_like(g0.slots, C.NO_CELL)
+    has = g0.slots != C.NO_CELL
+    slots[perm] = np.where(has, perm[np.where(has, g0.slots, 0)], C.NO_CELL)
+    lap, area = np.empty_like(g0.lap), np.empty_like(g0.area)
+    lap[perm], area[perm] = g0.lap, g0.area
+    g = dataclasses.replace(g0, slots=slots.astype(np.uint32), lap=lap, area=area)
+    cube = dtdiff.Cube(ws, g)
+    try:
+        h0, kd0 = fields(g0, 3)
+        h, kd = np.empty_like(h0), np.empty_like(kd0)
+        h[perm], kd[perm] = h0, kd0
+        dt = 20.0 * g.mean_area / 1.0e3
+        xo, its_o = O.cube_solve(g, h, kd, dt, int(solver), 1e-6)
+        x = h.copy()
+        its = cube.solve(x, kd, dt, solver, 1e-6)
+        assert its == its_o and its > 0
+        assert same_bits(x, xo)
+    finally:
+        cube.close()

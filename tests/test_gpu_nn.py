@@ -264,3 +264,37 @@ def test_trained_guess_beats_the_splitting(ws):
         assert r["err_net_m"] < 0.5 * r["err_lod_m"], r
         assert r["iterations_net"] <= r["iterations_lod"], r
         assert r["rel_net"] <= 1e-10 and r["rel_lod"] <= 1e-10, r
+
+
+def test_network_arguments_are_checked(ws):
+    """Shapes, channel counts, alignment and nulls are refused with a message, before any
+    launch."""
+    L = dtdiff.load()
+    a = DeviceArray(1 << 16, np.uint16)
+    b = DeviceArray(1 << 16, np.uint16)
+
+    def err(rc):
+        assert rc != 0
+        return L.dtd_last_error(ws._ctx).decode()
+
+    # w not a multiple of 16, channels not multiples of 16.
+    assert "multiple of 16" in err(L.dtd_nn_conv3x3(ws._ctx, 1, 4, 24, 16, 16, p(a), p(b), None, 0,
+                                                    p(a), None, 0))
+    assert "multiples of 16" in err(L.dtd_nn_conv3x3(ws._ctx, 1, 4, 32, 24, 16, p(a), p(b), None,
+                                                     0, p(a), None, 0))
+    # A tensor off its 16-byte alignment (2 bytes in).
+    off = ctypes.c_void_p(a.data_ptr() + 2)
+    assert "aligned" in err(L.dtd_nn_pool(ws._ctx, 1, 4, 32, 16, off, p(b)))
+    assert "aligned" in err(L.dtd_nn_relu_bwd(ws._ctx, 64, off, p(b), p(b)))
+    # Odd sizes for pooling, nulls, a zero Adam step.
+    assert "even" in err(L.dtd_nn_pool(ws._ctx, 1, 5, 32, 16, p(a), p(b)))
+    assert "null" in err(L.dtd_nn_up_cat(ws._ctx, 1, 4, 32, 16, 16, p(a), None, p(b)))
+    f = DeviceArray(64, np.float32)
+    assert "step" in err(L.dtd_nn_adam(ws._ctx, 64, p(f), p(f), p(f), p(f), 1e-3, 0.9, 0.999,
+                                       1e-8, 0, None))
+    # The guessed solve needs its guess.
+    d = dtdiff.Dims(64, 32, 1.0, True)
+    h = DeviceArray.from_numpy(np.zeros(d.w * d.h))
+    k = DeviceArray.from_numpy(np.ones(d.w * d.h))
+    with pytest.raises(dtdiff.DtdError):
+        dtdiff.solve_rel_guess_device(ws, d, h, k, 10.0, 1e-10, 0)
