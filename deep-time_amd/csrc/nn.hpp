@@ -26,6 +26,7 @@ namespace nn {
 typedef short s4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
@@ -145,6 +146,93 @@ __global__ __launch_bounds__(256) void k_conv3x3(Shape s, int cin_, int cout,
     }
 }
 
+// The same convolution with its operands staged in LDS, for w a multiple of 128, cin of 32 and
+// cout of 16 NTB: one block a segment of 128 pixels of a row and 16 NTB outputs.  For each row
+// offset dy and chunk of 32 input channels the block stages the segment's input row (130 pixels
+// with the wrapped halo) and the three taps' weights of that dy; then the three column offsets dx
+// read shifted windows of the staged row.  Wave v takes pixels 32 v .. 32 v + 31 of the segment
+// and all the block's outputs.  Rows past the poles contribute nothing and are skipped.
+template <int NTB>
+__global__ __launch_bounds__(256) void k_conv3x3_lds(Shape s, int cin, int cout,
+                                                     const uint16_t *__restrict__ x,
+                                                     const uint16_t *__restrict__ wt,
+                                                     const float *__restrict__ bias, int relu,
+                                                     uint16_t *__restrict__ y,
+                                                     float *__restrict__ y32, int y32c) {
+    constexpr int BP = 128, KC = 32, AS = KC + 8, BCO = 16 * NTB;
+    __shared__ __align__(16) uint16_t la[(BP + 2) * AS];
+    __shared__ __align__(16) uint16_t lb[3 * BCO * AS];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const long p0 = (long)blockIdx.x * BP;
+    const int co0 = blockIdx.y * BCO;
+    const int col0 = (int)(p0 % s.w);
+    const long rowi = p0 / s.w;
+    const int row = (int)(rowi % s.h);
+    const long img = rowi / s.h;
+    const int m = lane & 15, k8 = (lane >> 4) * 8;
+    f4_t acc[2][NTB];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int n = 0; n < NTB; n++) acc[i][n] = f4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int dy = -1; dy <= 1; dy++) {
+        const int yy = row + dy;
+        if (yy < 0 || yy >= s.h) continue;  // the same for the whole block
+        const uint16_t *xrow = x + (img * s.h + yy) * (long)s.w * cin;
+        for (int c0 = 0; c0 < cin; c0 += KC) {
+            __syncthreads();  // the last stage's reads are done
+            for (int e = tid; e < (BP + 2) * 4; e += 256) {
+                const int pp = e >> 2, ch = (e & 3) * 8;
+                int xx = col0 - 1 + pp;
+                xx = xx < 0 ? xx + s.w : (xx >= s.w ? xx - s.w : xx);
+                *reinterpret_cast<u4_t *>(&la[pp * AS + ch]) =
+                    *reinterpret_cast<const u4_t *>(xrow + (long)xx * cin + c0 + ch);
+            }
+            for (int e = tid; e < 3 * BCO * 4; e += 256) {
+                const int t = e / (BCO * 4), r = e % (BCO * 4), co = r >> 2, ch = (r & 3) * 8;
+                const int tap = (dy + 1) * 3 + t;
+                *reinterpret_cast<u4_t *>(&lb[(t * BCO + co) * AS + ch]) =
+                    *reinterpret_cast<const u4_t *>(wt + ((size_t)(co0 + co) * 9 + tap) * cin +
+                                                    c0 + ch);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < 3; t++) {  // dx = t - 1: pixel q reads staged pixel q + t
+                b8_t a[2], b[NTB];
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+                    a[i] = *reinterpret_cast<const b8_t *>(&la[(wave * 32 + 16 * i + m + t) * AS +
+                                                              k8]);
+#pragma unroll
+                for (int n = 0; n < NTB; n++)
+                    b[n] = *reinterpret_cast<const b8_t *>(&lb[(t * BCO + 16 * n + m) * AS + k8]);
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int n = 0; n < NTB; n++)
+                        acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[n], acc[i][n],
+                                                                            0, 0, 0);
+            }
+        }
+    }
+    const int kq = (lane >> 4) * 4;
+#pragma unroll
+    for (int n = 0; n < NTB; n++) {
+        const int co = co0 + 16 * n + m;
+        const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const long p = p0 + wave * 32 + 16 * i + kq + q;
+                float v = acc[i][n][q] + bv;
+                if (relu) v = fmaxf(v, 0.0f);
+                y[p * cout + co] = f2bf(v);
+                if (y32 && co < y32c) y32[p * y32c + co] = v;
+            }
+    }
+}
+
 // wt_t[ci][tap][co] = wt[co][8 - tap][ci]: the weights of the input-gradient convolution.
 __global__ __launch_bounds__(256) void k_conv_flip(int cin, int cout, const uint16_t *__restrict__ wt,
                                                    uint16_t *__restrict__ wt_t) {
@@ -156,7 +244,6 @@ __global__ __launch_bounds__(256) void k_conv_flip(int cin, int cout, const uint
 
 // Eight bf16 values in 16 bytes (element 2k in the low half of word k), for the elementwise
 // kernels: one thread a group of eight channels.
-typedef uint32_t u4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void unpack8(u4_t v, float *f) {
 #pragma unroll
@@ -489,6 +576,17 @@ static void conv_launch_c(hipStream_t s, Shape sh, int cin, int cout, const uint
     const long np = (long)sh.n * sh.h * sh.w;
     const bool k32 = cin % 32 == 0 && reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
                      reinterpret_cast<uintptr_t>(wt) % 16 == 0;
+    if (k32 && sh.w % 128 == 0 && cout % 32 == 0) {  // operands staged in LDS
+        if (cout % 64 == 0) {
+            k_conv3x3_lds<4><<<dim3((unsigned)(np / 128), (unsigned)(cout / 64)), 256, 0, s>>>(
+                sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+        } else {
+            k_conv3x3_lds<2><<<dim3((unsigned)(np / 128), (unsigned)(cout / 32)), 256, 0, s>>>(
+                sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+        }
+        LAUNCH_CHECK();
+        return;
+    }
     const bool m4 = sh.w % 64 == 0;  // four 16-pixel tiles a wave, in one row
     dim3 grid((unsigned)((np / (m4 ? 64 : 16) + 3) / 4), (unsigned)(cout / (16 * NT)));
     if (m4 && k32)
