@@ -146,20 +146,21 @@ __global__ __launch_bounds__(256) void k_conv3x3(Shape s, int cin_, int cout,
     }
 }
 
-// The same convolution with its operands staged in LDS, for w a multiple of 128, cin of 32 and
-// cout of 16 NTB: one block a segment of 128 pixels of a row and 16 NTB outputs.  For each row
-// offset dy and chunk of 32 input channels the block stages the segment's input row (130 pixels
-// with the wrapped halo) and the three taps' weights of that dy; then the three column offsets dx
-// read shifted windows of the staged row.  Wave v takes pixels 32 v .. 32 v + 31 of the segment
-// and all the block's outputs.  Rows past the poles contribute nothing and are skipped.
-template <int NTB>
+// The same convolution with its operands staged in LDS, for w a multiple of 64 MT, cin of 32 and
+// cout of 16 NTB: one block a segment of 64 MT pixels of a row and 16 NTB outputs.  For each row
+// offset dy and chunk of 32 input channels the block stages the segment's input row (with the
+// wrapped halo pixel at each end) and the three taps' weights of that dy; then the three column offsets dx
+// read shifted windows of the staged row.  Wave v takes pixels 16 MT v .. 16 MT (v + 1) - 1 of
+// the segment and all the block's outputs.  Rows past the poles contribute nothing and are
+// skipped.
+template <int NTB, int MT>
 __global__ __launch_bounds__(256) void k_conv3x3_lds(Shape s, int cin, int cout,
                                                      const uint16_t *__restrict__ x,
                                                      const uint16_t *__restrict__ wt,
                                                      const float *__restrict__ bias, int relu,
                                                      uint16_t *__restrict__ y,
                                                      float *__restrict__ y32, int y32c) {
-    constexpr int BP = 128, KC = 32, AS = KC + 8, BCO = 16 * NTB;
+    constexpr int BP = 64 * MT, KC = 32, AS = KC + 8, BCO = 16 * NTB;
     __shared__ __align__(16) uint16_t la[(BP + 2) * AS];
     __shared__ __align__(16) uint16_t lb[3 * BCO * AS];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -170,49 +171,78 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(Shape s, int cin, int cout,
     const int row = (int)(rowi % s.h);
     const long img = rowi / s.h;
     const int m = lane & 15, k8 = (lane >> 4) * 8;
-    f4_t acc[2][NTB];
+    f4_t acc[MT][NTB];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < MT; i++)
 #pragma unroll
         for (int n = 0; n < NTB; n++) acc[i][n] = f4_t{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int dy = -1; dy <= 1; dy++) {
-        const int yy = row + dy;
-        if (yy < 0 || yy >= s.h) continue;  // the same for the whole block
-        const uint16_t *xrow = x + (img * s.h + yy) * (long)s.w * cin;
-        for (int c0 = 0; c0 < cin; c0 += KC) {
-            __syncthreads();  // the last stage's reads are done
-            for (int e = tid; e < (BP + 2) * 4; e += 256) {
+    // The stages: the row offsets inside the grid (the same for the whole block) times the
+    // 32-channel chunks.  Each thread holds its share of the next stage's operands in registers
+    // while the block computes the current one.
+    const int nch = cin / KC;
+    const int dy0 = row > 0 ? -1 : 0, dy1 = row + 1 < s.h ? 1 : 0;  // the rows inside the grid
+    const int nst = (dy1 - dy0 + 1) * nch;
+    constexpr int NA = (BP + 2) * 4, NB = 3 * BCO * 4;  // 16-byte chunks of a stage
+    constexpr int UA = (NA + 255) / 256, UB = (NB + 255) / 256;
+    u4_t ra[UA], rb[UB];
+    auto fetch = [&](int k) {
+        const int dy = dy0 + k / nch, c0 = (k % nch) * KC;
+        const uint16_t *xrow = x + (img * s.h + row + dy) * (long)s.w * cin;
+#pragma unroll
+        for (int u = 0; u < UA; u++) {
+            const int e = tid + 256 * u;
+            if (e < NA) {
                 const int pp = e >> 2, ch = (e & 3) * 8;
                 int xx = col0 - 1 + pp;
                 xx = xx < 0 ? xx + s.w : (xx >= s.w ? xx - s.w : xx);
-                *reinterpret_cast<u4_t *>(&la[pp * AS + ch]) =
-                    *reinterpret_cast<const u4_t *>(xrow + (long)xx * cin + c0 + ch);
+                ra[u] = *reinterpret_cast<const u4_t *>(xrow + (long)xx * cin + c0 + ch);
             }
-            for (int e = tid; e < 3 * BCO * 4; e += 256) {
+        }
+#pragma unroll
+        for (int u = 0; u < UB; u++) {
+            const int e = tid + 256 * u;
+            if (e < NB) {
                 const int t = e / (BCO * 4), r = e % (BCO * 4), co = r >> 2, ch = (r & 3) * 8;
                 const int tap = (dy + 1) * 3 + t;
-                *reinterpret_cast<u4_t *>(&lb[(t * BCO + co) * AS + ch]) =
-                    *reinterpret_cast<const u4_t *>(wt + ((size_t)(co0 + co) * 9 + tap) * cin +
-                                                    c0 + ch);
+                rb[u] = *reinterpret_cast<const u4_t *>(wt + ((size_t)(co0 + co) * 9 + tap) * cin +
+                                                        c0 + ch);
             }
-            __syncthreads();
+        }
+    };
+    if (nst > 0) fetch(0);
+    for (int k = 0; k < nst; k++) {
+        __syncthreads();  // the last stage's reads are done
 #pragma unroll
-            for (int t = 0; t < 3; t++) {  // dx = t - 1: pixel q reads staged pixel q + t
-                b8_t a[2], b[NTB];
+        for (int u = 0; u < UA; u++) {
+            const int e = tid + 256 * u;
+            if (e < NA) *reinterpret_cast<u4_t *>(&la[(e >> 2) * AS + (e & 3) * 8]) = ra[u];
+        }
 #pragma unroll
-                for (int i = 0; i < 2; i++)
-                    a[i] = *reinterpret_cast<const b8_t *>(&la[(wave * 32 + 16 * i + m + t) * AS +
-                                                              k8]);
+        for (int u = 0; u < UB; u++) {
+            const int e = tid + 256 * u;
+            if (e < NB) {
+                const int t = e / (BCO * 4), r = e % (BCO * 4);
+                *reinterpret_cast<u4_t *>(&lb[(t * BCO + (r >> 2)) * AS + (r & 3) * 8]) = rb[u];
+            }
+        }
+        __syncthreads();
+        if (k + 1 < nst) fetch(k + 1);
+#pragma unroll
+        for (int t = 0; t < 3; t++) {  // dx = t - 1: pixel q reads staged pixel q + t
+            b8_t a[MT], b[NTB];
+#pragma unroll
+            for (int i = 0; i < MT; i++)
+                a[i] = *reinterpret_cast<const b8_t *>(
+                    &la[(wave * 16 * MT + 16 * i + m + t) * AS + k8]);
+#pragma unroll
+            for (int n = 0; n < NTB; n++)
+                b[n] = *reinterpret_cast<const b8_t *>(&lb[(t * BCO + 16 * n + m) * AS + k8]);
+#pragma unroll
+            for (int i = 0; i < MT; i++)
 #pragma unroll
                 for (int n = 0; n < NTB; n++)
-                    b[n] = *reinterpret_cast<const b8_t *>(&lb[(t * BCO + 16 * n + m) * AS + k8]);
-#pragma unroll
-                for (int i = 0; i < 2; i++)
-#pragma unroll
-                    for (int n = 0; n < NTB; n++)
-                        acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[n], acc[i][n],
-                                                                            0, 0, 0);
-            }
+                    acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[n], acc[i][n], 0,
+                                                                        0, 0);
         }
     }
     const int kq = (lane >> 4) * 4;
@@ -221,10 +251,10 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(Shape s, int cin, int cout,
         const int co = co0 + 16 * n + m;
         const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int i = 0; i < MT; i++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const long p = p0 + wave * 32 + 16 * i + kq + q;
+                const long p = p0 + wave * 16 * MT + 16 * i + kq + q;
                 float v = acc[i][n][q] + bv;
                 if (relu) v = fmaxf(v, 0.0f);
                 y[p * cout + co] = f2bf(v);
@@ -577,13 +607,20 @@ static void conv_launch_c(hipStream_t s, Shape sh, int cin, int cout, const uint
     const bool k32 = cin % 32 == 0 && reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
                      reinterpret_cast<uintptr_t>(wt) % 16 == 0;
     if (k32 && sh.w % 128 == 0 && cout % 32 == 0) {  // operands staged in LDS
-        if (cout % 64 == 0) {
-            k_conv3x3_lds<4><<<dim3((unsigned)(np / 128), (unsigned)(cout / 64)), 256, 0, s>>>(
+        const bool wide = sh.w % 256 == 0;           // 256-pixel segments, four tiles a wave
+        const unsigned bx = (unsigned)(np / (wide ? 256 : 128));
+        if (cout % 64 == 0 && wide)
+            k_conv3x3_lds<4, 4><<<dim3(bx, (unsigned)(cout / 64)), 256, 0, s>>>(
                 sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
-        } else {
-            k_conv3x3_lds<2><<<dim3((unsigned)(np / 128), (unsigned)(cout / 32)), 256, 0, s>>>(
+        else if (cout % 64 == 0)
+            k_conv3x3_lds<4, 2><<<dim3(bx, (unsigned)(cout / 64)), 256, 0, s>>>(
                 sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
-        }
+        else if (wide)
+            k_conv3x3_lds<2, 4><<<dim3(bx, (unsigned)(cout / 32)), 256, 0, s>>>(
+                sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+        else
+            k_conv3x3_lds<2, 2><<<dim3(bx, (unsigned)(cout / 32)), 256, 0, s>>>(
+                sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
         LAUNCH_CHECK();
         return;
     }
