@@ -606,10 +606,16 @@ static void conv_launch_c(hipStream_t s, Shape sh, int cin, int cout, const uint
     const long np = (long)sh.n * sh.h * sh.w;
     const bool k32 = cin % 32 == 0 && reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
                      reinterpret_cast<uintptr_t>(wt) % 16 == 0;
-    if (k32 && sh.w % 128 == 0 && cout % 32 == 0) {  // operands staged in LDS
-        const bool wide = sh.w % 256 == 0;           // 256-pixel segments, four tiles a wave
+    if (k32 && sh.w % 128 == 0) {           // operands staged in LDS
+        const bool wide = sh.w % 256 == 0;  // 256-pixel segments, four tiles a wave
         const unsigned bx = (unsigned)(np / (wide ? 256 : 128));
-        if (cout % 64 == 0 && wide)
+        if (cout % 32 != 0 && wide)
+            k_conv3x3_lds<1, 4><<<dim3(bx, (unsigned)(cout / 16)), 256, 0, s>>>(
+                sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+        else if (cout % 32 != 0)
+            k_conv3x3_lds<1, 2><<<dim3(bx, (unsigned)(cout / 16)), 256, 0, s>>>(
+                sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+        else if (cout % 64 == 0 && wide)
             k_conv3x3_lds<4, 4><<<dim3(bx, (unsigned)(cout / 64)), 256, 0, s>>>(
                 sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
         else if (cout % 64 == 0)

@@ -66,7 +66,7 @@ def close_bf16(got, want, frac=0.98):
                                           (32, 80, 0, 32), (96, 32, 1, 32), (32, 64, 1, 128),
                                           (16, 16, 0, 64), (384, 128, 1, 192),
                                           (96, 32, 1, 256), (64, 64, 0, 128),
-                                          (64, 128, 1, 256)])
+                                          (64, 128, 1, 256), (32, 16, 0, 256), (64, 48, 0, 128)])
 def test_conv_forward(ws, ci, co, relu, w):
     """w a multiple of 128 with cin a multiple of 32 takes the LDS-staged kernel, other multiples
     of 64 the four-tile kernel, the rest the one-tile kernel."""
