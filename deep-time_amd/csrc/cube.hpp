@@ -571,7 +571,7 @@ __global__ __launch_bounds__(kBlock) void kc_update(int l, const double *__restr
     const double rv = r[i] - alpha * ap[i];
     r[i] = rv;
     t0[i] = rv * rv;
-    t1[i] = rv * rv / diag[i];
+    if (t1) t1[i] = rv * rv / diag[i];  // the diagonal's rz (Jacobi only)
 }
 
 // The scalar steps, one thread: the parts added in order from -0.0 (`iter().sum()`).
@@ -588,7 +588,8 @@ __global__ __launch_bounds__(kBlock) void kc_step(const double *__restrict__ par
         if (flag && threadIdx.x == 0) *reinterpret_cast<volatile int *>(flag) = st->done;
         return;
     }
-    const int stride = (STEP == kCubeRR || STEP == kCubePutback) ? 2 : 1;
+    // r.r with the diagonal's rz beside it (Jacobi), or alone (multigrid).
+    const int stride = ((STEP == kCubeRR && !mg) || STEP == kCubePutback) ? 2 : 1;
     double a = -0.0, b = -0.0;
     for (int q0 = 0; q0 < np; q0 += kStepTile) {
         const int cnt = min(kStepTile, np - q0);
@@ -1327,11 +1328,13 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
         std::swap(po, pn);
         kc_step<kCubePAP><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
         LAUNCH_CHECK();
-        kc_update<<<nb, kBlock, 0, s>>>(l, po, g->ap, g->diag, g->x, g->r, g->tmp, g->b, g->st);
-        LAUNCH_CHECK();
-        kc_chunks<<<bc2, kRowBlock, 0, s>>>(l, g->tmp, g->b, 0.0, g->parts, g->st);
+        kc_update<<<nb, kBlock, 0, s>>>(l, po, g->ap, g->diag, g->x, g->r, g->tmp,
+                                        mg ? nullptr : g->b, g->st);
         LAUNCH_CHECK();
         if (mg) {
+            // r.r alone (the multigrid's rz comes from the cycle's z).
+            kc_chunks<<<bc, kRowBlock, 0, s>>>(l, g->tmp, nullptr, 0.0, g->parts, g->st);
+            LAUNCH_CHECK();
             kc_step<kCubeRR><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, nullptr);
             LAUNCH_CHECK();
             z32 = cube_precondition(ctx, g, g->r);
@@ -1340,6 +1343,8 @@ static size_t cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h, const double *kd,
             LAUNCH_CHECK();
             kc_step<kCubeRZ><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, ctx->dev_flag + par);
         } else {
+            kc_chunks<<<bc2, kRowBlock, 0, s>>>(l, g->tmp, g->b, 0.0, g->parts, g->st);
+            LAUNCH_CHECK();
             kc_step<kCubeRR><<<1, kBlock, 0, s>>>(g->parts, nc, g->st, mg, ctx->dev_flag + par);
         }
         LAUNCH_CHECK();
