@@ -1230,7 +1230,7 @@ size_t dtd_nn_bwd_weight_scratch(int n, int h, int w, int cin, int cout, size_t 
     const long np = (long)n * h * w;
     const long chunks = (np + nn::kChunk - 1) / nn::kChunk;
     if (f32_elems) *f32_elems = (size_t)chunks * cout * 9 * cin;
-    return (size_t)(cin + cout) * np;
+    return (size_t)(3 * cin + cout) * np;  // x with its columns moved by -1, 0, 1, and dz
 }
 
 int dtd_nn_conv3x3_bwd_weight(dtd_ctx *ctx, int n, int h, int w, int cin, int cout,
@@ -1241,24 +1241,30 @@ int dtd_nn_conv3x3_bwd_weight(dtd_ctx *ctx, int n, int h, int w, int cin, int co
         begin(ctx);
         nn::check_shape(n, h, w), nn::check_channels(cin), nn::check_channels(cout);
         if (!x || !dz || !planes || !part || !dw) throw Error{DTD_EINVAL, "null array"};
-        nn::check_aligned(planes, 8);
+        nn::check_aligned(planes, 16);
         size_t need_part = 0;
         const size_t need_planes = dtd_nn_bwd_weight_scratch(n, h, w, cin, cout, &need_part);
         if (planes_elems < need_planes || part_elems < need_part)
             throw Error{DTD_EINVAL, "scratch too small"};
         hipStream_t s = ctx->stream;
         const long np = (long)n * h * w;
-        uint16_t *xt = planes, *dzt = planes + (size_t)cin * np;
-        nn::k_to_planes<<<dim3((unsigned)((np + 63) / 64), (unsigned)(cin / 16)), 256, 0, s>>>(
-            np, cin, x, xt);
-        LAUNCH_CHECK();
+        const nn::Shape sh{n, h, w};
+        uint16_t *xs = planes, *dzt = planes + (size_t)3 * cin * np;
+        const dim3 tg((unsigned)((np + 63) / 64), (unsigned)(cin / 16));
+        for (int dx = -1; dx <= 1; dx++) {
+            nn::k_to_planes<<<tg, 256, 0, s>>>(sh, cin, dx, x, xs + (size_t)(dx + 1) * cin * np);
+            LAUNCH_CHECK();
+        }
         nn::k_to_planes<<<dim3((unsigned)((np + 63) / 64), (unsigned)(cout / 16)), 256, 0, s>>>(
-            np, cout, dz, dzt);
+            sh, cout, 0, dz, dzt);
         LAUNCH_CHECK();
         const int chunks = (int)((np + nn::kChunk - 1) / nn::kChunk);
         const int combos = (cout / 16) * (cin / 16) * 9;
-        nn::k_conv_dw<<<dim3((unsigned)chunks, (unsigned)((combos + 3) / 4)), 256, 0, s>>>(
-            nn::Shape{n, h, w}, cin, cout, xt, dzt, nn::kChunk, part);
+        const dim3 grid((unsigned)chunks, (unsigned)((combos + 3) / 4));
+        if (np % 32 == 0)
+            nn::k_conv_dw<32><<<grid, 256, 0, s>>>(sh, cin, cout, xs, dzt, nn::kChunk, part);
+        else
+            nn::k_conv_dw<16><<<grid, 256, 0, s>>>(sh, cin, cout, xs, dzt, nn::kChunk, part);
         LAUNCH_CHECK();
         const long nw = (long)cout * 9 * cin;
         nn::k_dw_reduce<<<nn::blocks(nw), 256, 0, s>>>(nw, chunks, part, dw);

@@ -333,15 +333,27 @@ __global__ __launch_bounds__(256) void k_relu_bwd(long n, const uint16_t *__rest
     }
 }
 
-// xt[ch][p] = x[p][ch] for all pixels p (c channels, tiles of 64 pixels x 16 channels).
-__global__ __launch_bounds__(256) void k_to_planes(long np, int c, const uint16_t *__restrict__ x,
+// xt[ch][p] = x[p'][ch] for all pixels p, where p' is p moved dx columns along its row (wrapping):
+// channel-major copies, the column shift applied once here so that the weights' gradient reads
+// every shifted operand as one aligned run.  Tiles of 64 pixels x 16 channels.
+__global__ __launch_bounds__(256) void k_to_planes(Shape s, int c, int dx,
+                                                   const uint16_t *__restrict__ x,
                                                    uint16_t *__restrict__ xt) {
     __shared__ uint16_t t[16][65];
+    const long np = pixels(s);
     const long p0 = (long)blockIdx.x * 64;
     const int c0 = blockIdx.y * 16;
     for (int e = threadIdx.x; e < 64 * 16; e += 256) {
         const int pp = e / 16, cc = e % 16;
-        t[cc][pp] = p0 + pp < np ? x[(p0 + pp) * c + c0 + cc] : (uint16_t)0;
+        const long p = p0 + pp;
+        uint16_t v = 0;
+        if (p < np) {
+            const int col = (int)(p % s.w);
+            int xx = col + dx;
+            xx = xx < 0 ? xx + s.w : (xx >= s.w ? xx - s.w : xx);
+            v = x[(p - col + xx) * c + c0 + cc];
+        }
+        t[cc][pp] = v;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < 64 * 16; e += 256) {
@@ -351,10 +363,13 @@ __global__ __launch_bounds__(256) void k_to_planes(long np, int c, const uint16_
 }
 
 // Per chunk of `chunk` pixels, part[k][co][tap][ci] = sum over the chunk's pixels p of
-// dz[p][co] x[p + d(tap)][ci], from channel-major copies dzt[co][p], xt[ci][p].  One wave a
-// (16 outputs, 16 inputs, tap) tile; blockIdx.x the chunk.
+// dz[p][co] x[p + d(tap)][ci], from channel-major copies dzt[co][p] and xs[dx + 1][ci][p] (x with
+// its columns moved by dx).  One wave a (16 outputs, 16 inputs, tap) tile; blockIdx.x the chunk.
+// KW = 32: 32 pixels a step (v_mfma_f32_16x16x32_bf16, each lane 8 consecutive pixels of one row,
+// np a multiple of 32); KW = 16: 16 pixels a step (16x16x16, 4 pixels a lane).
+template <int KW>
 __global__ __launch_bounds__(256) void k_conv_dw(Shape s, int cin, int cout,
-                                                 const uint16_t *__restrict__ xt,
+                                                 const uint16_t *__restrict__ xs,
                                                  const uint16_t *__restrict__ dzt, int chunk,
                                                  float *__restrict__ part) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -365,33 +380,28 @@ __global__ __launch_bounds__(256) void k_conv_dw(Shape s, int cin, int cout,
     const int dy = tap / 3 - 1, dx = tap % 3 - 1;
     const long np = pixels(s);
     const long pa = (long)blockIdx.x * chunk, pb = min(np, pa + chunk);
-    const int m = lane & 15, kq = (lane >> 4) * 4;
+    const int m = lane & 15, kq = (lane >> 4) * 4, ko = (lane >> 4) * (KW / 4);
     const uint16_t *dzp = dzt + (long)(cot * 16 + m) * np;
-    const uint16_t *xp = xt + (long)(cit * 16 + m) * np;
+    const uint16_t *xp = xs + ((long)(dx + 1) * cin + cit * 16 + m) * np;
+    const long shift = (long)dy * s.w;  // the row above or below, read only inside the grid
     f4_t acc = f4_t{0.0f, 0.0f, 0.0f, 0.0f};
     // Unrolled so that several steps' loads are in flight before their matrix instructions.
-#pragma unroll 8
-    for (long p = pa; p < pb; p += 16) {
-        const long q0 = p + kq;  // this lane's 4 pixels: q0 .. q0 + 3, in one row
-        const s4_t a = *reinterpret_cast<const s4_t *>(dzp + q0);
-        const int col = (int)(q0 % s.w);
-        const long rowi = q0 / s.w;
-        const int row = (int)(rowi % s.h), yy = row + dy;
-        s4_t b = s4_t{0, 0, 0, 0};
-        if (yy >= 0 && yy < s.h) {
-            const uint16_t *r = xp + (rowi + dy) * s.w;
-            if (dx == 0) {
-                b = *reinterpret_cast<const s4_t *>(r + col);
-            } else {
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    int xx = col + q + dx;
-                    xx = xx < 0 ? xx + s.w : (xx >= s.w ? xx - s.w : xx);
-                    b[q] = (short)r[xx];
-                }
-            }
+#pragma unroll 4
+    for (long p = pa; p < pb; p += KW) {
+        const long q0 = p + ko;  // this lane's pixels q0 .. q0 + KW / 4 - 1, in one row
+        const int row = (int)((q0 / s.w) % s.h), yy = row + dy;
+        const bool in = yy >= 0 && yy < s.h;
+        if constexpr (KW == 32) {
+            const b8_t a = *reinterpret_cast<const b8_t *>(dzp + q0);
+            b8_t b = b8_t{};
+            if (in) b = *reinterpret_cast<const b8_t *>(xp + (q0 + shift));
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+        } else {
+            const s4_t a = *reinterpret_cast<const s4_t *>(dzp + q0);
+            s4_t b = s4_t{0, 0, 0, 0};
+            if (in) b = *reinterpret_cast<const s4_t *>(xp + (q0 + shift));
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
     }
     // D[co][ci]: lane holds rows kq + q (outputs), column m (inputs).
     float *out = part + (size_t)blockIdx.x * cout * 9 * cin;

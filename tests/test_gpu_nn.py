@@ -87,10 +87,13 @@ def test_conv_forward(ws, ci, co, relu, w):
     assert np.allclose(get_f32(y32, (n, h, w, 2)), want[..., :2], rtol=1e-5, atol=1e-5)
 
 
-@pytest.mark.parametrize("ci,co,w", [(16, 32, 48), (64, 48, 48), (32, 16, 48), (64, 128, 128)])
-def test_conv_backward(ws, ci, co, w):
+@pytest.mark.parametrize("ci,co,w,h", [(16, 32, 48, 8), (64, 48, 48, 8), (32, 16, 48, 8),
+                                        (64, 128, 128, 8), (32, 32, 48, 3)])
+def test_conv_backward(ws, ci, co, w, h):
+    """Pixel counts that are multiples of 32 take the 32-pixel weight-gradient steps, others
+    (1 x 3 x 48) the 16-pixel steps."""
     rng = np.random.default_rng(ci + 3 * co + w)
-    n, h = 2, 8
+    n = 2 if h == 8 else 1
     x = N.bf16(rng.standard_normal((n, h, w, ci)))
     wt = N.bf16(rng.standard_normal((co, 9, ci)) / np.sqrt(9 * ci))
     dz = N.bf16(rng.standard_normal((n, h, w, co)))
