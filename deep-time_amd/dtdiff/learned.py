@@ -343,9 +343,9 @@ class Samples:
 
 
 def train(steps: int, out: str, lr: float = 1e-3, w: int = 1024, h: int = 512, pool: int = 32,
-          log_every: int = 100, params: np.ndarray | None = None) -> np.ndarray:
+          log_every: int = 100, params: np.ndarray | None = None, seed: int = 7) -> np.ndarray:
     ws = Workspace()
-    S = Samples(ws, w, h, pool)
+    S = Samples(ws, w, h, pool, seed)
     net = UNet(ws, params)
     t0 = time.perf_counter()
     run = 0.0
@@ -402,12 +402,15 @@ def main():
     t.add_argument("--w", type=int, default=1024)
     t.add_argument("--h", type=int, default=512)
     t.add_argument("--pool", type=int, default=32)
+    t.add_argument("--init", default=None, help="start from these weights (fine-tuning)")
+    t.add_argument("--seed", type=int, default=7, help="the training fields' pool seed")
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=WEIGHTS)
     e.add_argument("--count", type=int, default=8)
     a = ap.parse_args()
     if a.cmd == "train":
-        train(a.steps, a.out, a.lr, a.w, a.h, a.pool)
+        train(a.steps, a.out, a.lr, a.w, a.h, a.pool,
+              params=load_params(a.init) if a.init else None, seed=a.seed)
     else:
         for r in evaluate(load_params(a.weights), a.count):
             print(" ".join(f"{k} {v:.4g}" if isinstance(v, float) else f"{k} {v}"
