@@ -1210,6 +1210,23 @@ int dtd_nn_conv3x3(dtd_ctx *ctx, int n, int h, int w, int cin, int cout, const u
     });
 }
 
+int dtd_nn_conv3x3_cat(dtd_ctx *ctx, int n, int h, int w, int cu, int cs, int cout,
+                       const uint16_t *u, const uint16_t *skip, const uint16_t *wt,
+                       const float *bias, int relu, uint16_t *y, float *y32, int y32c) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        nn::check_shape(n, h, w), nn::check_channels(cu), nn::check_channels(cs);
+        nn::check_channels(cout);
+        if (!u || !skip || !wt || !y) throw Error{DTD_EINVAL, "null array"};
+        if (y32 && (y32c < 1 || y32c > cout)) throw Error{DTD_EINVAL, "y32c out of range"};
+        if (!nn::conv_cat_ok(nn::Shape{n, h, w}, cu, cs, u, skip, wt))
+            throw Error{DTD_EINVAL, "the in-place concatenation needs w a multiple of 128, h even, "
+                                    "cu and cs multiples of 32 and 16-byte aligned tensors"};
+        nn::conv_cat(ctx->stream, nn::Shape{n, h, w}, cu, cs, cout, u, skip, wt, bias, relu, y,
+                     y32, y32c);
+    });
+}
+
 int dtd_nn_conv3x3_bwd_input(dtd_ctx *ctx, int n, int h, int w, int cin, int cout,
                              const uint16_t *dz, const uint16_t *wt, uint16_t *wt_scratch,
                              uint16_t *dx) {

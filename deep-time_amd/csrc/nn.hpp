@@ -156,6 +156,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(Shape s, int cin_, int cout,
 template <int NTB, int MT>
 __global__ __launch_bounds__(256) void k_conv3x3_lds(Shape s, int cin, int cout,
                                                      const uint16_t *__restrict__ x,
+                                                     const uint16_t *__restrict__ xu, int cu,
                                                      const uint16_t *__restrict__ wt,
                                                      const float *__restrict__ bias, int relu,
                                                      uint16_t *__restrict__ y,
@@ -185,9 +186,14 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(Shape s, int cin, int cout,
     constexpr int NA = (BP + 2) * 4, NB = 3 * BCO * 4;  // 16-byte chunks of a stage
     constexpr int UA = (NA + 255) / 256, UB = (NB + 255) / 256;
     u4_t ra[UA], rb[UB];
+    // With xu, input channels [0, cu) are the nearest upsampling of xu (h/2 x w/2, cu channels)
+    // and [cu, cin) are x (cin - cu channels): the concatenation read in place.
     auto fetch = [&](int k) {
         const int dy = dy0 + k / nch, c0 = (k % nch) * KC;
-        const uint16_t *xrow = x + (img * s.h + row + dy) * (long)s.w * cin;
+        const bool up = xu && c0 < cu;
+        const int cx = xu ? cin - cu : cin, cc = xu && !up ? c0 - cu : c0;
+        const uint16_t *xrow = up ? xu + (img * (s.h / 2) + (row + dy) / 2) * (long)(s.w / 2) * cu
+                                  : x + (img * s.h + row + dy) * (long)s.w * cx;
 #pragma unroll
         for (int u = 0; u < UA; u++) {
             const int e = tid + 256 * u;
@@ -195,7 +201,8 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(Shape s, int cin, int cout,
                 const int pp = e >> 2, ch = (e & 3) * 8;
                 int xx = col0 - 1 + pp;
                 xx = xx < 0 ? xx + s.w : (xx >= s.w ? xx - s.w : xx);
-                ra[u] = *reinterpret_cast<const u4_t *>(xrow + (long)xx * cin + c0 + ch);
+                ra[u] = up ? *reinterpret_cast<const u4_t *>(xrow + (long)(xx / 2) * cu + c0 + ch)
+                           : *reinterpret_cast<const u4_t *>(xrow + (long)xx * cx + cc + ch);
             }
         }
 #pragma unroll
@@ -621,22 +628,22 @@ static void conv_launch_c(hipStream_t s, Shape sh, int cin, int cout, const uint
         const unsigned bx = (unsigned)(np / (wide ? 256 : 128));
         if (cout % 32 != 0 && wide)
             k_conv3x3_lds<1, 4><<<dim3(bx, (unsigned)(cout / 16)), 256, 0, s>>>(
-                sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+                sh, cin, cout, x, nullptr, 0, wt, bias, relu, y, y32, y32c);
         else if (cout % 32 != 0)
             k_conv3x3_lds<1, 2><<<dim3(bx, (unsigned)(cout / 16)), 256, 0, s>>>(
-                sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+                sh, cin, cout, x, nullptr, 0, wt, bias, relu, y, y32, y32c);
         else if (cout % 64 == 0 && wide)
             k_conv3x3_lds<4, 4><<<dim3(bx, (unsigned)(cout / 64)), 256, 0, s>>>(
-                sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+                sh, cin, cout, x, nullptr, 0, wt, bias, relu, y, y32, y32c);
         else if (cout % 64 == 0)
             k_conv3x3_lds<4, 2><<<dim3(bx, (unsigned)(cout / 64)), 256, 0, s>>>(
-                sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+                sh, cin, cout, x, nullptr, 0, wt, bias, relu, y, y32, y32c);
         else if (wide)
             k_conv3x3_lds<2, 4><<<dim3(bx, (unsigned)(cout / 32)), 256, 0, s>>>(
-                sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+                sh, cin, cout, x, nullptr, 0, wt, bias, relu, y, y32, y32c);
         else
             k_conv3x3_lds<2, 2><<<dim3(bx, (unsigned)(cout / 32)), 256, 0, s>>>(
-                sh, cin, cout, x, wt, bias, relu, y, y32, y32c);
+                sh, cin, cout, x, nullptr, 0, wt, bias, relu, y, y32, y32c);
         LAUNCH_CHECK();
         return;
     }
@@ -654,6 +661,42 @@ static void conv_launch_c(hipStream_t s, Shape sh, int cin, int cout, const uint
     else
         k_conv3x3<NT, 1, CIN, 16><<<grid, 256, 0, s>>>(sh, cin, cout, x, wt, bias, relu, y, y32,
                                                        y32c);
+    LAUNCH_CHECK();
+}
+
+// The convolution of [nearest upsampling of u (h/2 x w/2, cu channels), skip (cs channels)] without
+// forming the concatenation: the LDS-staged kernel reads each channel chunk from its source.
+static bool conv_cat_ok(Shape sh, int cu, int cs, const void *u, const void *skip, const void *wt) {
+    return sh.w % 128 == 0 && sh.h % 2 == 0 && cu % 32 == 0 && cs % 32 == 0 &&
+           reinterpret_cast<uintptr_t>(u) % 16 == 0 && reinterpret_cast<uintptr_t>(skip) % 16 == 0 &&
+           reinterpret_cast<uintptr_t>(wt) % 16 == 0;
+}
+
+static void conv_cat(hipStream_t s, Shape sh, int cu, int cs, int cout, const uint16_t *u,
+                     const uint16_t *skip, const uint16_t *wt, const float *bias, int relu,
+                     uint16_t *y, float *y32, int y32c) {
+    const long np = (long)sh.n * sh.h * sh.w;
+    const int cin = cu + cs;
+    const bool wide = sh.w % 256 == 0;
+    const unsigned bx = (unsigned)(np / (wide ? 256 : 128));
+    if (cout % 32 != 0 && wide)
+        k_conv3x3_lds<1, 4><<<dim3(bx, (unsigned)(cout / 16)), 256, 0, s>>>(
+            sh, cin, cout, skip, u, cu, wt, bias, relu, y, y32, y32c);
+    else if (cout % 32 != 0)
+        k_conv3x3_lds<1, 2><<<dim3(bx, (unsigned)(cout / 16)), 256, 0, s>>>(
+            sh, cin, cout, skip, u, cu, wt, bias, relu, y, y32, y32c);
+    else if (cout % 64 == 0 && wide)
+        k_conv3x3_lds<4, 4><<<dim3(bx, (unsigned)(cout / 64)), 256, 0, s>>>(
+            sh, cin, cout, skip, u, cu, wt, bias, relu, y, y32, y32c);
+    else if (cout % 64 == 0)
+        k_conv3x3_lds<4, 2><<<dim3(bx, (unsigned)(cout / 64)), 256, 0, s>>>(
+            sh, cin, cout, skip, u, cu, wt, bias, relu, y, y32, y32c);
+    else if (wide)
+        k_conv3x3_lds<2, 4><<<dim3(bx, (unsigned)(cout / 32)), 256, 0, s>>>(
+            sh, cin, cout, skip, u, cu, wt, bias, relu, y, y32, y32c);
+    else
+        k_conv3x3_lds<2, 2><<<dim3(bx, (unsigned)(cout / 32)), 256, 0, s>>>(
+            sh, cin, cout, skip, u, cu, wt, bias, relu, y, y32, y32c);
     LAUNCH_CHECK();
 }
 

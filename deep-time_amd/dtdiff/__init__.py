@@ -42,7 +42,8 @@ SYMBOLS = (
     "dtd_cycle_new", "dtd_cycle_levels", "dtd_cycle_level", "dtd_cycle_precondition",
     "dtd_time_kernel", "dtd_solve_batch", "dtd_step_device", "dtd_fill_kd",
     "dtd_slope_failure_device", "dtd_slope_failure", "dtd_cube_new", "dtd_cube_free",
-    "dtd_cube_solve", "dtd_cube_solve_device", "dtd_nn_conv3x3", "dtd_nn_conv3x3_bwd_input",
+    "dtd_cube_solve", "dtd_cube_solve_device", "dtd_nn_conv3x3", "dtd_nn_conv3x3_cat",
+    "dtd_nn_conv3x3_bwd_input",
     "dtd_nn_bwd_weight_scratch", "dtd_nn_conv3x3_bwd_weight", "dtd_nn_relu_bwd", "dtd_nn_pool",
     "dtd_nn_up_cat", "dtd_nn_up_cat_bwd", "dtd_nn_input", "dtd_nn_loss", "dtd_nn_guess",
     "dtd_nn_adam", "dtd_nn_to_bf16", "dtd_sync", "dtd_solve_rel_guess_device", "dtd_lod_device",
@@ -187,6 +188,8 @@ def load(path: str = LIB_PATH):
                                     ctypes.POINTER(ctypes.c_uint)]
     _i, _d, _f = ctypes.c_int, ctypes.c_double, ctypes.c_float
     L.dtd_nn_conv3x3.argtypes = [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _i]
+    L.dtd_nn_conv3x3_cat.argtypes = [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp,
+                                     _vp, _i]
     L.dtd_nn_conv3x3_bwd_input.argtypes = [_vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]
     L.dtd_nn_bwd_weight_scratch.argtypes = [_i, _i, _i, _i, _i, ctypes.POINTER(_sz)]
     L.dtd_nn_bwd_weight_scratch.restype = _sz

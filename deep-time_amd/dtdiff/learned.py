@@ -154,8 +154,23 @@ class UNet:
                                       self._p(kd_dev), self._p(base_dev), dt, d.dx, SCALE,
                                       self._p(self.B["x0"])))
 
-    def forward(self):
+    def _conv_cat(self, name, lvl, cu, u, skip, y):
+        """Layer `name` on [upsampled u, skip] read in place (no concatenation formed)."""
+        ci, co, relu = self.spec[name]
+        a, b = self.offs[name]
+        n, h, w = self._level_shape(lvl)
+        self._chk(self.L.dtd_nn_conv3x3_cat(self.ws._ctx, n, h, w, cu, ci - cu, co, self._p(u),
+                                            self._p(skip), self._p(self.wb, a),
+                                            self._p(self.w, b), relu, self._p(y), None, 0))
+
+    def forward(self, train: bool = True):
+        """The network on the input set by set_input.  train keeps every tensor the backward pass
+        reads; without it the decoder reads its concatenations in place (w a multiple of 512 at
+        level 0, so that the rows of levels 0-2 are multiples of 128)."""
         B = self.B
+        n, h, w = self.shape
+        if not train and w % 512 == 0:
+            return self._forward_infer()
         self._conv("enc0a", 0, B["x0"], B["e0a"]); self._conv("enc0b", 0, B["e0a"], B["e0"])
         self._pool(0, 32, B["e0"], B["p1"])
         self._conv("enc1a", 1, B["p1"], B["e1a"]); self._conv("enc1b", 1, B["e1a"], B["e1"])
@@ -169,6 +184,23 @@ class UNet:
         self._conv("dec1a", 1, B["c1"], B["d1a"]); self._conv("dec1b", 1, B["d1a"], B["d1"])
         self._up_cat(0, 64, 32, B["d1"], B["e0"], B["c0"])
         self._conv("dec0a", 0, B["c0"], B["d0a"]); self._conv("dec0b", 0, B["d0a"], B["d0"])
+        self._conv("out", 0, B["d0"], B["y"], out32=B["out32"])
+
+    def _forward_infer(self):
+        B = self.B
+        self._conv("enc0a", 0, B["x0"], B["e0a"]); self._conv("enc0b", 0, B["e0a"], B["e0"])
+        self._pool(0, 32, B["e0"], B["p1"])
+        self._conv("enc1a", 1, B["p1"], B["e1a"]); self._conv("enc1b", 1, B["e1a"], B["e1"])
+        self._pool(1, 64, B["e1"], B["p2"])
+        self._conv("enc2a", 2, B["p2"], B["e2a"]); self._conv("enc2b", 2, B["e2a"], B["e2"])
+        self._pool(2, 128, B["e2"], B["p3"])
+        self._conv("enc3a", 3, B["p3"], B["e3a"]); self._conv("enc3b", 3, B["e3a"], B["e3"])
+        self._conv_cat("dec2a", 2, 256, B["e3"], B["e2"], B["d2a"])
+        self._conv("dec2b", 2, B["d2a"], B["d2"])
+        self._conv_cat("dec1a", 1, 128, B["d2"], B["e1"], B["d1a"])
+        self._conv("dec1b", 1, B["d1a"], B["d1"])
+        self._conv_cat("dec0a", 0, 64, B["d1"], B["e0"], B["d0a"])
+        self._conv("dec0b", 0, B["d0a"], B["d0"])
         self._conv("out", 0, B["d0"], B["y"], out32=B["out32"])
 
     # -- backward ----------------------------------------------------------------------------
@@ -251,7 +283,7 @@ class UNet:
         then g += SCALE x the predicted correction."""
         lod_device(self.ws, d, h_dev, kd_dev, dt, g_dev)
         self.set_input(d, h_dev, kd_dev, g_dev, dt)
-        self.forward()
+        self.forward(train=False)
         self._chk(self.L.dtd_nn_guess(self.ws._ctx, d.w * d.h, self._p(g_dev),
                                       self._p(self.B["out32"]), OUTC, SCALE, self._p(g_dev)))
 

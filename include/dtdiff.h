@@ -277,6 +277,14 @@ int dtd_nn_conv3x3(dtd_ctx *ctx, int n, int h, int w, int cin, int cout, const u
                    const uint16_t *wt, const float *bias, int relu, uint16_t *y, float *y32,
                    int y32c);
 
+/* dtd_nn_conv3x3 of the concatenation [nearest upsampling of u (h/2 x w/2, cu channels), skip
+ * (h x w, cs channels)] without forming it (the weights as for cin = cu + cs, in that order); the
+ * same outputs as dtd_nn_up_cat then dtd_nn_conv3x3.  Needs w a multiple of 128, h even, cu and cs
+ * multiples of 32 and 16-byte aligned tensors. */
+int dtd_nn_conv3x3_cat(dtd_ctx *ctx, int n, int h, int w, int cu, int cs, int cout,
+                       const uint16_t *u, const uint16_t *skip, const uint16_t *wt,
+                       const float *bias, int relu, uint16_t *y, float *y32, int y32c);
+
 /* The input's gradient dx (cin channels) of a convolution from its output's gradient dz (cout
  * channels); wt_scratch holds cin * 9 * cout bf16. */
 int dtd_nn_conv3x3_bwd_input(dtd_ctx *ctx, int n, int h, int w, int cin, int cout,

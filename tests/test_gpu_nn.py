@@ -152,6 +152,54 @@ def test_pool_up_cat_and_relu(ws):
     assert np.array_equal(get_bf16(d_dz, (4096,)), np.where(yv > 0, dyv, 0).astype(np.float32))
 
 
+@pytest.mark.parametrize("cu,cs,co,w,h", [(256, 128, 128, 128, 4), (128, 64, 64, 256, 6),
+                                           (64, 32, 32, 256, 4), (64, 32, 32, 128, 2)])
+def test_conv_on_the_concatenation_in_place(ws, cu, cs, co, w, h):
+    """dtd_nn_conv3x3_cat equals dtd_nn_up_cat then dtd_nn_conv3x3 bit for bit (the same sums in
+    the same order), and both match the oracle."""
+    rng = np.random.default_rng(cu + cs + co + w + h)
+    n, ci = 2, cu + cs
+    u = N.bf16(rng.standard_normal((n, h // 2, w // 2, cu)))
+    skip = N.bf16(rng.standard_normal((n, h, w, cs)))
+    wt = N.bf16(rng.standard_normal((co, 9, ci)) / np.sqrt(9 * ci))
+    b = rng.standard_normal(co).astype(np.float32)
+    du_, ds_, dw, db = dev_bf16(u), dev_bf16(skip), dev_bf16(wt), DeviceArray.from_numpy(b)
+    L = dtdiff.load()
+    cat = DeviceArray(n * h * w * ci, np.uint16)
+    y0, y1 = (DeviceArray(n * h * w * co, np.uint16) for _ in range(2))
+    ws.check(L.dtd_nn_up_cat(ws._ctx, n, h, w, cu, cs, p(du_), p(ds_), p(cat)))
+    ws.check(L.dtd_nn_conv3x3(ws._ctx, n, h, w, ci, co, p(cat), p(dw), p(db), 1, p(y0), None, 0))
+    ws.check(L.dtd_nn_conv3x3_cat(ws._ctx, n, h, w, cu, cs, co, p(du_), p(ds_), p(dw), p(db), 1,
+                                  p(y1), None, 0))
+    synchronize()
+    assert np.array_equal(y0.numpy(), y1.numpy())
+    want = np.maximum(N.conv3x3(np.concatenate([N.up(u), skip], axis=3), wt, b), 0)
+    close_bf16(get_bf16(y1, (n, h, w, co)), want)
+    # Rows that are not a multiple of 128 are refused.
+    assert L.dtd_nn_conv3x3_cat(ws._ctx, n, h, 64, cu, cs, co, p(du_), p(ds_), p(dw), p(db), 1,
+                                p(y1), None, 0) != 0
+
+
+def test_inference_forward_equals_the_training_forward(ws):
+    """The guess's forward (concatenations read in place) gives the training forward's output."""
+    rng = np.random.default_rng(8)
+    params = LN.init_params(3)
+    offs, _ = LN.offsets()
+    a, b = offs["out"]
+    params[a:b] = (rng.standard_normal(b - a) * 0.05).astype(np.float32)  # a live output layer
+    d = dtdiff.Dims(512, 16, 1.0e4, True)
+    hh, kd, base = (DeviceArray.from_numpy(v) for v in (
+        rng.standard_normal(d.w * d.h) * 100.0, np.exp(rng.standard_normal(d.w * d.h)),
+        rng.standard_normal(d.w * d.h) * 100.0))
+    net = LN.UNet(ws, params)
+    net.set_input(d, hh, kd, base, 1.0e3)
+    net.forward(train=True)
+    want = get_f32(net.B["out32"], (-1,)).copy()
+    assert np.abs(want).max() > 0
+    net.forward(train=False)
+    assert np.array_equal(get_f32(net.B["out32"], (-1,)), want)
+
+
 def test_adam_step(ws):
     rng = np.random.default_rng(6)
     n = 1000
