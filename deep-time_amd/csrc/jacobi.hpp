@@ -25,7 +25,7 @@
 
 namespace dtd {
 
-constexpr int kRowTile = 16;    // rows per block of the row sums
+constexpr int kRowTile = 8;     // rows per block of the row sums (16 and 4 measured slower)
 constexpr int kColTile = 256;   // columns per tile (one per thread of the block)
 constexpr double kCgTolM = 1.0e-6;  // the Jacobi solve's RMS tolerance, metres (D:495)
 
@@ -58,23 +58,30 @@ __global__ __launch_bounds__(kBlock) void kj_resid(Grid g, const double2 *__rest
 // Per row, in ascending column order from -0.0 (`row_sums`/`row_dots`, D:666-745):
 //   kRowsRZ:  s0[j] = sum r r,  s1[j] = sum r (r / g)    (a = r, b = diag)
 //   kRowsPAP: s0[j] = sum p Ap                            (a = p, b = A p)
+//   kRowsRZU: r -= alpha A p (pass_update, D:883-905) written back, then the sums of kRowsRZ
+//             on the new r                  (a = r, b = diag, c = A p; rout may be a)
 // The block takes kRowTile rows; for each tile of kColTile columns its threads form the products
-// (one column each) into LDS, then thread k adds row k's in column order.
-enum { kRowsRZ = 0, kRowsPAP = 1 };
+// (one column each) into LDS, then thread k adds row k's in column order.  A row's sum is one
+// dependent chain of w adds, so the rows are spread thin (kRowTile rows a block, several blocks
+// a CU): one block's chains then run while another's loads are in flight.
+enum { kRowsRZ = 0, kRowsPAP = 1, kRowsRZU = 2 };
 
 template <int MODE>
-__global__ __launch_bounds__(kColTile) void kj_rows(int w, int h, const double *__restrict__ a,
+__global__ __launch_bounds__(kColTile) void kj_rows(int w, int h, const double *a,
                                                     const double *__restrict__ b,
+                                                    const double *__restrict__ c, double *rout,
                                                     double *__restrict__ s0,
                                                     double *__restrict__ s1,
                                                     const JState *__restrict__ st) {
+    constexpr bool two = MODE != kRowsPAP;
     __shared__ double t0[kRowTile][kColTile + 1];
-    __shared__ double t1[MODE == kRowsRZ ? kRowTile : 1][kColTile + 1];
+    __shared__ double t1[two ? kRowTile : 1][kColTile + 1];
     if (st->done) return;
+    const double alpha = MODE == kRowsRZU ? st->alpha : 0.0;
     const int j0 = blockIdx.x * kRowTile;
     const int tid = threadIdx.x;
     // The next tile's values are loaded into registers while the current tile is being added.
-    double qa[kRowTile], qb[kRowTile];
+    double qa[kRowTile], qb[kRowTile], qc[MODE == kRowsRZU ? kRowTile : 1];
 #define DTD_FETCH_ROWS(i0_)                                                                    \
     do {                                                                                       \
         const int i_ = (i0_) + tid;                                                            \
@@ -83,6 +90,7 @@ __global__ __launch_bounds__(kColTile) void kj_rows(int w, int h, const double *
             if (j_ < h && i_ < w) {                                                            \
                 qa[k] = a[(size_t)j_ * w + i_];                                                \
                 qb[k] = b[(size_t)j_ * w + i_];                                                \
+                if (MODE == kRowsRZU) qc[k] = c[(size_t)j_ * w + i_];                          \
             }                                                                                  \
         }                                                                                      \
     } while (0)
@@ -91,7 +99,13 @@ __global__ __launch_bounds__(kColTile) void kj_rows(int w, int h, const double *
     for (int i0 = 0; i0 < w; i0 += kColTile) {
 #pragma unroll
         for (int k = 0; k < kRowTile; k++) {
-            if (MODE == kRowsRZ) {
+            if (MODE == kRowsRZU) {
+                const int j_ = j0 + k, i_ = i0 + tid;
+                const double rn = qa[k] - alpha * qc[k];
+                if (j_ < h && i_ < w) rout[(size_t)j_ * w + i_] = rn;
+                t0[k][tid] = rn * rn;
+                t1[k][tid] = rn * (rn / qb[k]);
+            } else if (MODE == kRowsRZ) {
                 t0[k][tid] = qa[k] * qa[k];
                 t1[k][tid] = qa[k] * (qa[k] / qb[k]);
             } else {
@@ -105,7 +119,7 @@ __global__ __launch_bounds__(kColTile) void kj_rows(int w, int h, const double *
 #pragma unroll 8
             for (int q = 0; q < cols; q++) {
                 acc0 += t0[tid][q];
-                if (MODE == kRowsRZ) acc1 += t1[tid][q];
+                if (two) acc1 += t1[tid][q];
             }
         }
         __syncthreads();
@@ -113,7 +127,7 @@ __global__ __launch_bounds__(kColTile) void kj_rows(int w, int h, const double *
 #undef DTD_FETCH_ROWS
     if (tid < kRowTile && j0 + tid < h) {
         s0[j0 + tid] = acc0;
-        if (MODE == kRowsRZ) s1[j0 + tid] = acc1;
+        if (two) s1[j0 + tid] = acc1;
     }
 }
 
@@ -214,15 +228,6 @@ __global__ __launch_bounds__(kBlock) void kj_apply(Grid g, const double2 *__rest
     ap[c] = apply_es(g, es, c, i, j, [&](int k) { return p[k]; });
 }
 
-// r -= alpha A p (pass_update, D:883-905).
-__global__ __launch_bounds__(kBlock) void kj_update(int n, const double *__restrict__ ap,
-                                                    double *__restrict__ r,
-                                                    const JState *__restrict__ st) {
-    if (st->done) return;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c < n) r[c] -= st->alpha * ap[c];
-}
-
 // After the loop: the pending x += alpha p (pass_x, D:907-919) with p the last direction
 // (iteration k wrote p0 when k is odd, p1 when even).
 __global__ __launch_bounds__(kBlock) void kj_last(int n, const double *__restrict__ p0,
@@ -278,12 +283,9 @@ __global__ void kj_init(JState *st, double tol2, int max_iter) {
 // The first guess with the reference's recurrences (bit for bit), the rows on the transposed
 // grid (a wave's rows side by side in memory): kd^T in kdt, b^T in xt (solved in place), cbuf
 // and zbuf scratch, all w x h.
-__global__ __launch_bounds__(kBlock) void kj_lod_rows_t(Grid g, int wrap_x,
-                                                        const double *__restrict__ kdt,
-                                                        double *__restrict__ xt,
-                                                        double *__restrict__ cbuf,
-                                                        double *__restrict__ zbuf, double dt,
-                                                        double s2) {
+__global__ __launch_bounds__(kBlock) void kj_lod_rows_t(Grid g, int wrap_x, const double *kdt,
+                                                        double *xt, double *cbuf, double *zbuf,
+                                                        double dt, double s2) {
     const int j = blockIdx.x * kBlock + threadIdx.x;
     const int h = g.h;
     if (j < h)
@@ -317,13 +319,13 @@ static void jacobi_iteration(dtd_ctx *ctx, int parity) {
     LAUNCH_CHECK();
     kj_apply<<<nb, kBlock, 0, s>>>(g, ws.es, pnew, ws.ap, st);
     LAUNCH_CHECK();
-    kj_rows<kRowsPAP><<<nr, kColTile, 0, s>>>(ws.w, ws.h, pnew, ws.ap, s0, nullptr, st);
+    kj_rows<kRowsPAP><<<nr, kColTile, 0, s>>>(ws.w, ws.h, pnew, ws.ap, nullptr, nullptr, s0,
+                                              nullptr, st);
     LAUNCH_CHECK();
     kj_step<kJPap><<<1, kStepBlock, 0, s>>>(s0, s1, ws.h, st);
     LAUNCH_CHECK();
-    kj_update<<<nb, kBlock, 0, s>>>((int)ws.n, ws.ap, ws.r, st);
-    LAUNCH_CHECK();
-    kj_rows<kRowsRZ><<<nr, kColTile, 0, s>>>(ws.w, ws.h, ws.r, ws.jdiag, s0, s1, st);
+    kj_rows<kRowsRZU><<<nr, kColTile, 0, s>>>(ws.w, ws.h, ws.r, ws.jdiag, ws.ap, ws.r, s0, s1,
+                                              st);
     LAUNCH_CHECK();
     kj_step<kJUpdate><<<1, kStepBlock, 0, s>>>(s0, s1, ws.h, st, ctx->dev_flag + parity);
     LAUNCH_CHECK();
@@ -367,7 +369,8 @@ static size_t jacobi_solve(dtd_ctx *ctx, const double *b, const double *kd, doub
     LAUNCH_CHECK();
     kj_resid<<<nb, kBlock, 0, s>>>(g, ws.es, b, ws.x, ws.r);
     LAUNCH_CHECK();
-    kj_rows<kRowsRZ><<<nr, kColTile, 0, s>>>(ws.w, ws.h, ws.r, ws.jdiag, s0, s1, st);
+    kj_rows<kRowsRZ><<<nr, kColTile, 0, s>>>(ws.w, ws.h, ws.r, ws.jdiag, nullptr, nullptr, s0,
+                                             s1, st);
     LAUNCH_CHECK();
     kj_step<kJInit><<<1, kStepBlock, 0, s>>>(s0, s1, ws.h, st);
     LAUNCH_CHECK();

@@ -1741,6 +1741,111 @@ struct Line {
     __device__ T &operator[](int i) const { return p[(size_t)i * s]; }
 };
 
+// One thread runs a line's recurrences, so each step would wait on memory: the loads run
+// kLodAhead positions ahead, in registers (chunk c + 1 is loaded while chunk c is being used).
+// The arithmetic is the recurrences', operation for operation.
+constexpr int kLodAhead = 16;
+
+
+// v[q] = L[i0 + D q] (D = 1 or -1) for the positions inside [0, n); 0 elsewhere.
+template <int D, typename T>
+__device__ __forceinline__ void line_chunk(Line<T> L, int i0, int n, double (&v)[kLodAhead]) {
+    const int lo = D > 0 ? i0 : i0 - (kLodAhead - 1), hi = D > 0 ? i0 + kLodAhead - 1 : i0;
+    if (lo >= 0 && hi < n) {
+        const T *p = L.p + (ptrdiff_t)i0 * L.s;
+#pragma unroll
+        for (int q = 0; q < kLodAhead; q++) v[q] = p[(ptrdiff_t)(D * q) * L.s];
+    } else {
+#pragma unroll
+        for (int q = 0; q < kLodAhead; q++) {
+            const int i = i0 + D * q;
+            v[q] = (i >= 0 && i < n) ? (double)L[i] : 0.0;
+        }
+    }
+}
+
+// Runs step(i, q, slot) for i = i0 + D q over [first, end) (D = 1: i from first up to end - 1;
+// D = -1: from first down to end + 1), a chunk of kLodAhead at a time, the chunks alternating
+// between two register slots: fetch(i0, slot) loads the chunk at i0 into `slot` while the chunk
+// before it is used from the other (no copies between the slots, which would wait on the loads).
+// The chunk at `first` must be in slot 0 on entry.  Pairs of chunks whose every position lies
+// before `whole` (D = 1) or after it (D = -1) run without bounds tests.
+template <int D, typename Fetch, typename Step>
+__device__ __forceinline__ void line_walk(int first, int end, int whole, Fetch fetch, Step step) {
+    constexpr int A = kLodAhead;
+    int i0 = first;
+    for (; D > 0 ? i0 + 2 * A - 1 < whole : i0 - (2 * A - 1) > whole; i0 += 2 * D * A) {
+        fetch(i0 + D * A, 1);
+#pragma unroll
+        for (int q = 0; q < A; q++) step(i0 + D * q, q, 0);
+        fetch(i0 + 2 * D * A, 0);
+#pragma unroll
+        for (int q = 0; q < A; q++) step(i0 + D * (A + q), q, 1);
+    }
+    for (; D > 0 ? i0 < end : i0 > end; i0 += 2 * D * A) {
+        fetch(i0 + D * A, 1);
+#pragma unroll
+        for (int q = 0; q < A; q++)
+            if (D > 0 ? i0 + q < end : i0 - q > end) step(i0 + D * q, q, 0);
+        fetch(i0 + 2 * D * A, 0);
+#pragma unroll
+        for (int q = 0; q < A; q++)
+            if (D > 0 ? i0 + A + q < end : i0 - A - q > end) step(i0 + D * (A + q), q, 1);
+    }
+}
+
+// `thomas` (D:1029-1046) on a line of n cells that does not wrap: faces from kd between
+// neighbours, none past the ends (row q: lo = -face(q - 1), up = -face(q), di = 1 + face(q - 1)
+// + face(q)).  b is solved in place; cc is scratch.
+__device__ __forceinline__ void lod_thomas(Line<const double> k, Line<double> b, Line<double> cc,
+                                           int n, double dt, double s2) {
+    const Line<const double> k1{k.p + k.s, k.s};  // k1[i] = k[i + 1]
+    double kc = k[0];
+    double fprev = n > 1 ? face(kc, k1[0], dt, s2) : 0.0;  // face(i - 1) at step i
+    double bt = 1.0 + 0.0 + fprev;
+    cc[0] = 0.0;
+    double prev = b[0] / bt;
+    b[0] = prev;
+    if (n > 1) kc = k1[0];
+    double kn[2][kLodAhead], bv[2][kLodAhead];
+    // Step i (1 .. n - 1) reads k[i + 1] (the faces past the end are 0) and b[i].
+    line_chunk<1>(k1, 1, n - 1, kn[0]);
+    line_chunk<1>(b, 1, n, bv[0]);
+    line_walk<1>(
+        1, n, n - 1,
+        [&](int i0, int t) {
+            line_chunk<1>(k1, i0, n - 1, kn[t]);
+            line_chunk<1>(b, i0, n, bv[t]);
+        },
+        [&](int i, int q, int t) {
+            const double fi = i + 1 < n ? face(kc, kn[t][q], dt, s2) : 0.0;
+            kc = kn[t][q];
+            const double up_prev = -fprev, lo = -fprev;
+            const double di = 1.0 + fprev + fi;
+            const double ci = up_prev / bt;
+            cc[i] = ci;
+            bt = di - lo * ci;
+            prev = (bv[t][q] - lo * prev) / bt;
+            b[i] = prev;
+            fprev = fi;
+        });
+    // Back substitution: b[i] -= cc[i + 1] b[i + 1], with b[i + 1] carried.
+    const Line<double> c1{cc.p + cc.s, cc.s};
+    double cv[2][kLodAhead];
+    line_chunk<-1>(c1, n - 2, n - 1, cv[0]);
+    line_chunk<-1>(b, n - 2, n, bv[0]);
+    line_walk<-1>(
+        n - 2, -1, -1,
+        [&](int i0, int t) {
+            line_chunk<-1>(c1, i0, n - 1, cv[t]);
+            line_chunk<-1>(b, i0, n, bv[t]);
+        },
+        [&](int i, int q, int t) {
+            prev = bv[t][q] - cv[t][q] * prev;
+            b[i] = prev;
+        });
+}
+
 // One row of w cells: kd in k, the right-hand side in b (solved in place), cc and zz scratch.
 __device__ __forceinline__ void lod_line_x(Line<const double> k, Line<double> b, Line<double> cc,
                                            Line<double> zz, int w, int wrap_x, double dt,
@@ -1761,56 +1866,88 @@ __device__ __forceinline__ void lod_line_x(Line<const double> k, Line<double> b,
         // bb[0] = di[0] - gamma, bb[n-1] = di[n-1] - alpha beta / gamma; l[0] = 0, u[n-1] = 0.
         double bt = di0 - gamma;
         cc[0] = 0.0;
-        b[0] /= bt;
+        const double x0 = b[0] / bt;
+        b[0] = x0;
         double zprev = gamma / bt;
         zz[0] = zprev;
-        double xprev = b[0];
+        double xprev = x0;
         double fprev = f0;  // face(i - 1)
-        for (int i = 1; i < w; i++) {
-            const double fi = (i + 1 < w) ? fc(i) : flast;
-            const double lo = -fprev, up_prev = -fprev;  // up[i-1] = -face(i-1)
-            double di = 1.0 + fprev + fi;
-            if (i == w - 1) di = di - alpha * beta / gamma;
-            const double ci = up_prev / bt;
-            cc[i] = ci;
-            bt = di - lo * ci;
-            const double xi = (b[i] - lo * xprev) / bt;
-            b[i] = xi;
-            xprev = xi;
-            const double z0 = (i == w - 1) ? alpha : 0.0;
-            const double zi = (z0 - lo * zprev) / bt;
-            zz[i] = zi;
-            zprev = zi;
-            fprev = fi;
-        }
-        for (int i = w - 2; i >= 0; i--) {
-            b[i] -= cc[i + 1] * b[i + 1];
-            zz[i] -= cc[i + 1] * zz[i + 1];
-        }
-        const double fact = (b[0] + beta * b[w - 1] / gamma) / (1.0 + zz[0] + beta * zz[w - 1] / gamma);
-        for (int i = 0; i < w; i++) b[i] = b[i] - fact * zz[i];
+        double kc = k[1];   // k[i] at step i
+        const Line<const double> k1{k.p + k.s, k.s};  // k1[i] = k[i + 1]
+        double kn[2][kLodAhead], bv[2][kLodAhead];
+        // Step i (1 .. w - 1) reads k[i + 1] (below w) and b[i].
+        line_chunk<1>(k1, 1, w - 1, kn[0]);
+        line_chunk<1>(b, 1, w, bv[0]);
+        line_walk<1>(
+            1, w, w - 1,
+            [&](int i0, int t) {
+                line_chunk<1>(k1, i0, w - 1, kn[t]);
+                line_chunk<1>(b, i0, w, bv[t]);
+            },
+            [&](int i, int q, int t) {
+                const bool last = i == w - 1;
+                const double fi = last ? flast : face(kc, kn[t][q], dt, s2);
+                kc = kn[t][q];
+                const double lo = -fprev, up_prev = -fprev;  // up[i-1] = -face(i-1)
+                double di = 1.0 + fprev + fi;
+                if (last) di = di - alpha * beta / gamma;
+                const double ci = up_prev / bt;
+                cc[i] = ci;
+                bt = di - lo * ci;
+                const double xi = (bv[t][q] - lo * xprev) / bt;
+                b[i] = xi;
+                xprev = xi;
+                const double z0 = last ? alpha : 0.0;
+                const double zi = (z0 - lo * zprev) / bt;
+                zz[i] = zi;
+                zprev = zi;
+                fprev = fi;
+            });
+        // Back substitution of both solves: b[i] -= cc[i + 1] b[i + 1], zz likewise.
+        const double blast = xprev, zlast = zprev;  // b[w - 1], zz[w - 1]
+        double bn = xprev, zn = zprev;
+        const Line<double> c1{cc.p + cc.s, cc.s};
+        double cv[2][kLodAhead], zv[2][kLodAhead];
+        line_chunk<-1>(c1, w - 2, w - 1, cv[0]);
+        line_chunk<-1>(b, w - 2, w, bv[0]);
+        line_chunk<-1>(zz, w - 2, w, zv[0]);
+        line_walk<-1>(
+            w - 2, -1, -1,
+            [&](int i0, int t) {
+                line_chunk<-1>(c1, i0, w - 1, cv[t]);
+                line_chunk<-1>(b, i0, w, bv[t]);
+                line_chunk<-1>(zz, i0, w, zv[t]);
+            },
+            [&](int i, int q, int t) {
+                bn = bv[t][q] - cv[t][q] * bn;
+                b[i] = bn;
+                zn = zv[t][q] - cv[t][q] * zn;
+                zz[i] = zn;
+            });
+        // bn, zn are b[0], zz[0] now.
+        const double fact = (bn + beta * blast / gamma) / (1.0 + zn + beta * zlast / gamma);
+        line_chunk<1>(b, 0, w, bv[0]);
+        line_chunk<1>(zz, 0, w, zv[0]);
+        line_walk<1>(
+            0, w, w,
+            [&](int i0, int t) {
+                line_chunk<1>(b, i0, w, bv[t]);
+                line_chunk<1>(zz, i0, w, zv[t]);
+            },
+            [&](int i, int q, int t) { b[i] = bv[t][q] - fact * zv[t][q]; });
         return;
     }
-    // Plain Thomas: lo[0] = 0, up[w-1] = 0; two wrapping cells fold their two faces into one;
-    // one cell has diagonal 1 (D:400-419).
+    if (!wrap_x) {
+        lod_thomas(k, b, cc, w, dt, s2);
+        return;
+    }
+    // Plain Thomas on one or two wrapping cells: two cells fold their two faces into one; one
+    // cell has diagonal 1 (D:400-419).
     double f2 = 0.0;
-    if (w == 2 && wrap_x) f2 = fc(0) + fc(1);
-    auto west = [&](int i) -> double { return i == 0 ? fc(w - 1) : fc(i - 1); };
-    auto lo_at = [&](int i) -> double {
-        if (i == 0) return 0.0;
-        if (w == 2 && wrap_x) return -f2;
-        return -west(i);
-    };
-    auto up_at = [&](int i) -> double {
-        if (i == w - 1) return 0.0;
-        if (w == 2 && wrap_x) return -f2;
-        return -fc(i);
-    };
-    auto di_at = [&](int i) -> double {
-        if (w == 1) return 1.0;
-        if (w == 2 && wrap_x) return 1.0 + f2;
-        return 1.0 + west(i) + fc(i);
-    };
+    if (w == 2) f2 = fc(0) + fc(1);
+    auto lo_at = [&](int i) -> double { return i == 0 ? 0.0 : -f2; };
+    auto up_at = [&](int i) -> double { return i == w - 1 ? 0.0 : -f2; };
+    auto di_at = [&](int i) -> double { return w == 1 ? 1.0 : 1.0 + f2; };
     double bt = di_at(0);
     cc[0] = 0.0;
     b[0] /= bt;
@@ -1824,58 +1961,30 @@ __device__ __forceinline__ void lod_line_x(Line<const double> k, Line<double> b,
     for (int i = w - 2; i >= 0; i--) b[i] -= cc[i + 1] * b[i + 1];
 }
 
-__device__ __forceinline__ void lod_row(const Grid &g, int wrap_x, const double *__restrict__ kd,
-                                        double dt, double s2, double *__restrict__ x,
-                                        double *__restrict__ cbuf, double *__restrict__ zbuf,
-                                        int j) {
+__device__ __forceinline__ void lod_row(const Grid &g, int wrap_x, const double *kd, double dt,
+                                        double s2, double *x, double *cbuf, double *zbuf, int j) {
     const size_t o = (size_t)j * g.w;
     lod_line_x(Line<const double>{kd + o, 1}, Line<double>{x + o, 1}, Line<double>{cbuf + o, 1},
                Line<double>{zbuf + o, 1}, g.w, wrap_x, dt, s2);
 }
 
-__global__ void k_lod_rows(Grid g, int wrap_x, const double *__restrict__ kd, double dt, double s2,
-                           double *__restrict__ x, double *__restrict__ cbuf,
-                           double *__restrict__ zbuf) {
+__global__ __launch_bounds__(kBlock) void k_lod_rows(Grid g, int wrap_x, const double *kd, double dt,
+                                                     double s2, double *x, double *cbuf,
+                                                     double *zbuf) {
     const int j = blockIdx.x * kBlock + threadIdx.x;
     if (j < g.h) lod_row(g, wrap_x, kd, dt, s2, x, cbuf, zbuf, j);
 }
 
 // The z sweep, one thread per column (coalesced across the wave): `thomas` down the column with
 // north and south faces (D:436-488).
-__device__ __forceinline__ void lod_col(const Grid &g, const double *__restrict__ kd, double dt,
-                                        double s2, double *__restrict__ x,
-                                        double *__restrict__ cbuf, int i) {
-    const int w = g.w, h = g.h;
-    auto fz = [&](int j) -> double {
-        return j + 1 < h ? face(kd[(size_t)j * w + i], kd[(size_t)(j + 1) * w + i], dt, s2) : 0.0;
-    };
-    double fn = 0.0;  // face(j - 1), the north face
-    double fs = fz(0);
-    double bt = 1.0 + fn + fs;
-    cbuf[i] = 0.0;
-    double prev = x[i] / bt;
-    x[i] = prev;
-    for (int j = 1; j < h; j++) {
-        const double up_prev = -fs;  // up[j-1] = -south(j-1)
-        fn = fs;
-        fs = fz(j);
-        const double lo = -fn;
-        const double di = 1.0 + fn + fs;
-        const double cj = up_prev / bt;
-        const size_t cidx = (size_t)j * w + i;
-        cbuf[cidx] = cj;
-        bt = di - lo * cj;
-        prev = (x[cidx] - lo * prev) / bt;
-        x[cidx] = prev;
-    }
-    for (int j = h - 2; j >= 0; j--) {
-        const size_t cidx = (size_t)j * w + i;
-        x[cidx] -= cbuf[cidx + w] * x[cidx + w];
-    }
+__device__ __forceinline__ void lod_col(const Grid &g, const double *kd, double dt, double s2,
+                                        double *x, double *cbuf, int i) {
+    lod_thomas(Line<const double>{kd + i, g.w}, Line<double>{x + i, g.w},
+               Line<double>{cbuf + i, g.w}, g.h, dt, s2);
 }
 
-__global__ void k_lod_cols(Grid g, const double *__restrict__ kd, double dt, double s2,
-                           double *__restrict__ x, double *__restrict__ cbuf) {
+__global__ __launch_bounds__(kBlock) void k_lod_cols(Grid g, const double *kd, double dt, double s2,
+                                                     double *x, double *cbuf) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < g.w) lod_col(g, kd, dt, s2, x, cbuf, i);
 }
