@@ -517,13 +517,14 @@ def bench_learned(args, gen, kw):
     net.guess(d, h0d, kdd, dt, gd)
     ws.check(L.dtd_sync(ws._ctx))
     err_net = float(np.sqrt(np.mean((gd.numpy() - x_ref) ** 2)))
-    # The network's forward alone (the convolutions on the matrix cores), and its flops.
+    # The network's forward alone as the guess runs it (the convolutions on the matrix cores, the
+    # decoder reading its skip concatenations in place), and its flops.
     net.set_input(d, h0d, kdd, gd, dt)
     ws.check(L.dtd_sync(ws._ctx))
     t0 = time.perf_counter()
     reps = 3
     for _ in range(reps):
-        net.forward()
+        net.forward(train=False)
     ws.check(L.dtd_sync(ws._ctx))
     fwd_ms = (time.perf_counter() - t0) * 1e3 / reps
     flops = 0.0
