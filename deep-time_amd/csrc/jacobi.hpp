@@ -57,14 +57,13 @@ __global__ __launch_bounds__(kBlock) void kj_resid(Grid g, const double2 *__rest
 
 // Per row, in ascending column order from -0.0 (`row_sums`/`row_dots`, D:666-745):
 //   kRowsRZ:  s0[j] = sum r r,  s1[j] = sum r (r / g)    (a = r, b = diag)
-//   kRowsPAP: s0[j] = sum p Ap                            (a = p, b = A p)
 //   kRowsRZU: r -= alpha A p (pass_update, D:883-905) written back, then the sums of kRowsRZ
 //             on the new r                  (a = r, b = diag, c = A p; rout may be a)
 // The block takes kRowTile rows; for each tile of kColTile columns its threads form the products
 // (one column each) into LDS, then thread k adds row k's in column order.  A row's sum is one
 // dependent chain of w adds, so the rows are spread thin (kRowTile rows a block, several blocks
 // a CU): one block's chains then run while another's loads are in flight.
-enum { kRowsRZ = 0, kRowsPAP = 1, kRowsRZU = 2 };
+enum { kRowsRZ = 0, kRowsRZU = 1 };
 
 template <int MODE>
 __global__ __launch_bounds__(kColTile) void kj_rows(int w, int h, const double *a,
@@ -73,9 +72,8 @@ __global__ __launch_bounds__(kColTile) void kj_rows(int w, int h, const double *
                                                     double *__restrict__ s0,
                                                     double *__restrict__ s1,
                                                     const JState *__restrict__ st) {
-    constexpr bool two = MODE != kRowsPAP;
     __shared__ double t0[kRowTile][kColTile + 1];
-    __shared__ double t1[two ? kRowTile : 1][kColTile + 1];
+    __shared__ double t1[kRowTile][kColTile + 1];
     if (st->done) return;
     const double alpha = MODE == kRowsRZU ? st->alpha : 0.0;
     const int j0 = blockIdx.x * kRowTile;
@@ -105,11 +103,9 @@ __global__ __launch_bounds__(kColTile) void kj_rows(int w, int h, const double *
                 if (j_ < h && i_ < w) rout[(size_t)j_ * w + i_] = rn;
                 t0[k][tid] = rn * rn;
                 t1[k][tid] = rn * (rn / qb[k]);
-            } else if (MODE == kRowsRZ) {
+            } else {
                 t0[k][tid] = qa[k] * qa[k];
                 t1[k][tid] = qa[k] * (qa[k] / qb[k]);
-            } else {
-                t0[k][tid] = qa[k] * qb[k];
             }
         }
         __syncthreads();
@@ -119,7 +115,7 @@ __global__ __launch_bounds__(kColTile) void kj_rows(int w, int h, const double *
 #pragma unroll 8
             for (int q = 0; q < cols; q++) {
                 acc0 += t0[tid][q];
-                if (two) acc1 += t1[tid][q];
+                acc1 += t1[tid][q];
             }
         }
         __syncthreads();
@@ -127,8 +123,95 @@ __global__ __launch_bounds__(kColTile) void kj_rows(int w, int h, const double *
 #undef DTD_FETCH_ROWS
     if (tid < kRowTile && j0 + tid < h) {
         s0[j0 + tid] = acc0;
-        if (two) s1[j0 + tid] = acc1;
+        s1[j0 + tid] = acc1;
     }
+}
+
+// A p (`Operator::apply_row`, D:609-649) into ap and, per row in ascending column order from -0.0,
+// s0[j] = sum p Ap (`row_dots`, D:666-745), in one pass: the block takes kApplyRows rows; for each
+// tile of kColTile columns its threads form A p for one column of each row, store it and put
+// p Ap into LDS; thread k then adds row k's in column order.  The same values and sums as
+// kj_apply followed by a row-sum pass over p and A p, without reading them back.
+constexpr int kApplyRows = 4;  // rows per block of kj_apply_rows (8 measured slower, 2 the same)
+
+// One cell's operands of A p (`apply_es`): its faces and the five values of p, with flags for
+// the neighbours that exist; gathered a tile ahead, then combined in apply_es's order.
+struct ApplyIn {
+    double e, s, wv, nv, pc, pe, pw, ps, pn;
+    bool he, hw;
+};
+
+__device__ __forceinline__ ApplyIn apply_gather(const Grid &g, const double2 *__restrict__ es,
+                                                const double *__restrict__ p, int c, int i,
+                                                int j) {
+    ApplyIn q;
+    const int c0 = j * g.w;
+    const double2 me = es[c];
+    q.e = me.x, q.s = me.y;
+    q.wv = i > 0 ? es[c - 1].x : (g.wrap ? es[c0 + g.w - 1].x : 0.0);
+    q.nv = j > 0 ? es[c - g.w].y : 0.0;
+    q.pc = p[c];
+    q.he = i + 1 < g.w || g.wrap;
+    q.pe = i + 1 < g.w ? p[c + 1] : (g.wrap ? p[c0] : 0.0);
+    q.hw = i > 0 || g.wrap;
+    q.pw = i > 0 ? p[c - 1] : (g.wrap ? p[c0 + g.w - 1] : 0.0);
+    q.ps = j + 1 < g.h ? p[c + g.w] : 0.0;
+    q.pn = j > 0 ? p[c - g.w] : 0.0;
+    return q;
+}
+
+__device__ __forceinline__ double apply_combine(const ApplyIn &q, int j, int h) {
+    const double d = 1.0 + q.e + q.wv + q.s + q.nv;
+    double o = d * q.pc;
+    if (q.he) o -= q.e * q.pe;
+    if (q.hw) o -= q.wv * q.pw;
+    if (j + 1 < h) o -= q.s * q.ps;
+    if (j > 0) o -= q.nv * q.pn;
+    return o;
+}
+
+template <int RT>
+__global__ __launch_bounds__(kColTile) void kj_apply_rows(Grid g, const double2 *__restrict__ es,
+                                                          const double *__restrict__ p,
+                                                          double *__restrict__ ap,
+                                                          double *__restrict__ s0,
+                                                          const JState *__restrict__ st) {
+    __shared__ double t0[RT][kColTile + 1];
+    if (st->done) return;
+    const int j0 = blockIdx.x * RT, tid = threadIdx.x;
+    // The next tile's operands are gathered into registers while the current tile is being added.
+    ApplyIn q[RT];
+    auto gather = [&](int i0) {
+        const int i = i0 + tid;
+#pragma unroll
+        for (int k = 0; k < RT; k++) {
+            const int j = j0 + k;
+            if (j < g.h && i < g.w) q[k] = apply_gather(g, es, p, j * g.w + i, i, j);
+        }
+    };
+    double acc = -0.0;
+    gather(0);
+    for (int i0 = 0; i0 < g.w; i0 += kColTile) {
+        const int i = i0 + tid;
+#pragma unroll
+        for (int k = 0; k < RT; k++) {
+            const int j = j0 + k;
+            if (j < g.h && i < g.w) {
+                const double o = apply_combine(q[k], j, g.h);
+                ap[j * g.w + i] = o;
+                t0[k][tid] = q[k].pc * o;
+            }
+        }
+        __syncthreads();
+        if (i0 + kColTile < g.w) gather(i0 + kColTile);
+        const int cols = min(kColTile, g.w - i0);
+        if (tid < RT && j0 + tid < g.h) {
+#pragma unroll 8
+            for (int c = 0; c < cols; c++) acc += t0[tid][c];
+        }
+        __syncthreads();
+    }
+    if (tid < RT && j0 + tid < g.h) s0[j0 + tid] = acc;
 }
 
 // The scalar steps (D:940-998), one thread, the row sums added in row order from -0.0.
@@ -317,10 +400,8 @@ static void jacobi_iteration(dtd_ctx *ctx, int parity) {
     double *s0 = ws.jrows, *s1 = ws.jrows + ws.h;
     kj_dir<<<nb, kBlock, 0, s>>>((int)ws.n, ws.r, ws.jdiag, pold, pnew, ws.x, st);
     LAUNCH_CHECK();
-    kj_apply<<<nb, kBlock, 0, s>>>(g, ws.es, pnew, ws.ap, st);
-    LAUNCH_CHECK();
-    kj_rows<kRowsPAP><<<nr, kColTile, 0, s>>>(ws.w, ws.h, pnew, ws.ap, nullptr, nullptr, s0,
-                                              nullptr, st);
+    kj_apply_rows<kApplyRows><<<(ws.h + kApplyRows - 1) / kApplyRows, kColTile, 0, s>>>(
+        g, ws.es, pnew, ws.ap, s0, st);
     LAUNCH_CHECK();
     kj_step<kJPap><<<1, kStepBlock, 0, s>>>(s0, s1, ws.h, st);
     LAUNCH_CHECK();
