@@ -12,7 +12,7 @@ with HIP events on the library's stream; compulsory bytes per launch / duration 
 the CPU oracle (the reference algorithm restated in C, OpenMP over strips) timed on the host on
 the same inputs, with the relative L2 difference of the two answers.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3]
 """
 import argparse
 import json
@@ -48,8 +48,10 @@ def traffic_of(kernel, w, h):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    # The first solve captures the iteration graphs and the second is their first replay; the
+    # timed steps start from the third.
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="lognormal-8192x4096",
                     help="constant-512x256 | lognormal-8192x4096 | constant-16384x8192 | "
                          "regional-512 (many grids, one batched solve)")
