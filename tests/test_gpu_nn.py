@@ -66,7 +66,8 @@ def close_bf16(got, want, frac=0.98):
                                           (32, 80, 0, 32), (96, 32, 1, 32), (32, 64, 1, 128),
                                           (16, 16, 0, 64), (384, 128, 1, 192),
                                           (96, 32, 1, 256), (64, 64, 0, 128),
-                                          (64, 128, 1, 256), (32, 16, 0, 256), (64, 48, 0, 128)])
+                                          (64, 128, 1, 256), (32, 16, 0, 256), (64, 48, 0, 128),
+                                          (32, 32, 1, 512), (32, 16, 0, 512), (96, 32, 1, 512)])
 def test_conv_forward(ws, ci, co, relu, w):
     """w a multiple of 128 with cin a multiple of 32 takes the LDS-staged kernel, other multiples
     of 64 the four-tile kernel, the rest the one-tile kernel."""
@@ -153,7 +154,8 @@ def test_pool_up_cat_and_relu(ws):
 
 
 @pytest.mark.parametrize("cu,cs,co,w,h", [(256, 128, 128, 128, 4), (128, 64, 64, 256, 6),
-                                           (64, 32, 32, 256, 4), (64, 32, 32, 128, 2)])
+                                           (64, 32, 32, 256, 4), (64, 32, 32, 128, 2),
+                                           (64, 32, 32, 512, 2)])
 def test_conv_on_the_concatenation_in_place(ws, cu, cs, co, w, h):
     """dtd_nn_conv3x3_cat equals dtd_nn_up_cat then dtd_nn_conv3x3 bit for bit (the same sums in
     the same order), and both match the oracle."""
