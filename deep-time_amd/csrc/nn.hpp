@@ -146,14 +146,15 @@ __global__ __launch_bounds__(256) void k_conv3x3(Shape s, int cin_, int cout,
     }
 }
 
-// The same convolution with its operands staged in LDS, for w a multiple of 64 MT, cin of 32 and
+// The same convolution with its operands staged in LDS, for w a multiple of 64 MT, cin of KC,
 // cout of 16 NTB: one block a segment of 64 MT pixels of a row and 16 NTB outputs.  For each row
-// offset dy and chunk of 32 input channels the block stages the segment's input row (with the
-// wrapped halo pixel at each end) and the three taps' weights of that dy; then the three column offsets dx
-// read shifted windows of the staged row.  Wave v takes pixels 16 MT v .. 16 MT (v + 1) - 1 of
-// the segment and all the block's outputs.  Rows past the poles contribute nothing and are
-// skipped.
-template <int NTB, int MT>
+// offset dy and chunk of KC input channels the block stages the segment's input row (with the
+// wrapped halo pixel at each end) and the three taps' weights of that dy; then the three column
+// offsets dx read shifted windows of the staged row.  Wave v takes pixels 16 MT v .. 16 MT
+// (v + 1) - 1 of the segment and all the block's outputs.  Rows past the poles contribute nothing
+// and are skipped.  KC = 32 steps with v_mfma_f32_16x16x32_bf16, KC = 16 (16-channel inputs) with
+// v_mfma_f32_16x16x16_bf16.
+template <int NTB, int MT, int KC = 32>
 __global__ __launch_bounds__(256) void k_conv3x3_lds(Shape s, int cin, int cout,
                                                      const uint16_t *__restrict__ x,
                                                      const uint16_t *__restrict__ xu, int cu,
@@ -161,7 +162,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(Shape s, int cin, int cout,
                                                      const float *__restrict__ bias, int relu,
                                                      uint16_t *__restrict__ y,
                                                      float *__restrict__ y32, int y32c) {
-    constexpr int BP = 64 * MT, KC = 32, AS = KC + 8, BCO = 16 * NTB;
+    constexpr int BP = 64 * MT, AS = KC + 8, BCO = 16 * NTB, CPP = KC / 8;  // 16-byte chunks a row
     __shared__ __align__(16) uint16_t la[(BP + 2) * AS];
     __shared__ __align__(16) uint16_t lb[3 * BCO * AS];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -171,7 +172,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(Shape s, int cin, int cout,
     const long rowi = p0 / s.w;
     const int row = (int)(rowi % s.h);
     const long img = rowi / s.h;
-    const int m = lane & 15, k8 = (lane >> 4) * 8;
+    const int m = lane & 15, k8 = (lane >> 4) * (KC / 4);
     f4_t acc[MT][NTB];
 #pragma unroll
     for (int i = 0; i < MT; i++)
@@ -183,7 +184,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(Shape s, int cin, int cout,
     const int nch = cin / KC;
     const int dy0 = row > 0 ? -1 : 0, dy1 = row + 1 < s.h ? 1 : 0;  // the rows inside the grid
     const int nst = (dy1 - dy0 + 1) * nch;
-    constexpr int NA = (BP + 2) * 4, NB = 3 * BCO * 4;  // 16-byte chunks of a stage
+    constexpr int NA = (BP + 2) * CPP, NB = 3 * BCO * CPP;  // 16-byte chunks of a stage
     constexpr int UA = (NA + 255) / 256, UB = (NB + 255) / 256;
     u4_t ra[UA], rb[UB];
     // With xu, input channels [0, cu) are the nearest upsampling of xu (h/2 x w/2, cu channels)
@@ -198,7 +199,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(Shape s, int cin, int cout,
         for (int u = 0; u < UA; u++) {
             const int e = tid + 256 * u;
             if (e < NA) {
-                const int pp = e >> 2, ch = (e & 3) * 8;
+                const int pp = e / CPP, ch = (e % CPP) * 8;
                 int xx = col0 - 1 + pp;
                 xx = xx < 0 ? xx + s.w : (xx >= s.w ? xx - s.w : xx);
                 ra[u] = up ? *reinterpret_cast<const u4_t *>(xrow + (long)(xx / 2) * cu + c0 + ch)
@@ -209,7 +210,8 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(Shape s, int cin, int cout,
         for (int u = 0; u < UB; u++) {
             const int e = tid + 256 * u;
             if (e < NB) {
-                const int t = e / (BCO * 4), r = e % (BCO * 4), co = r >> 2, ch = (r & 3) * 8;
+                const int t = e / (BCO * CPP), r = e % (BCO * CPP), co = r / CPP,
+                          ch = (r % CPP) * 8;
                 const int tap = (dy + 1) * 3 + t;
                 rb[u] = *reinterpret_cast<const u4_t *>(wt + ((size_t)(co0 + co) * 9 + tap) * cin +
                                                         c0 + ch);
@@ -222,34 +224,51 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(Shape s, int cin, int cout,
 #pragma unroll
         for (int u = 0; u < UA; u++) {
             const int e = tid + 256 * u;
-            if (e < NA) *reinterpret_cast<u4_t *>(&la[(e >> 2) * AS + (e & 3) * 8]) = ra[u];
+            if (e < NA) *reinterpret_cast<u4_t *>(&la[(e / CPP) * AS + (e % CPP) * 8]) = ra[u];
         }
 #pragma unroll
         for (int u = 0; u < UB; u++) {
             const int e = tid + 256 * u;
             if (e < NB) {
-                const int t = e / (BCO * 4), r = e % (BCO * 4);
-                *reinterpret_cast<u4_t *>(&lb[(t * BCO + (r >> 2)) * AS + (r & 3) * 8]) = rb[u];
+                const int t = e / (BCO * CPP), r = e % (BCO * CPP);
+                *reinterpret_cast<u4_t *>(&lb[(t * BCO + r / CPP) * AS + (r % CPP) * 8]) = rb[u];
             }
         }
         __syncthreads();
         if (k + 1 < nst) fetch(k + 1);
 #pragma unroll
         for (int t = 0; t < 3; t++) {  // dx = t - 1: pixel q reads staged pixel q + t
-            b8_t a[MT], b[NTB];
+            if constexpr (KC == 32) {
+                b8_t a[MT], b[NTB];
 #pragma unroll
-            for (int i = 0; i < MT; i++)
-                a[i] = *reinterpret_cast<const b8_t *>(
-                    &la[(wave * 16 * MT + 16 * i + m + t) * AS + k8]);
-#pragma unroll
-            for (int n = 0; n < NTB; n++)
-                b[n] = *reinterpret_cast<const b8_t *>(&lb[(t * BCO + 16 * n + m) * AS + k8]);
-#pragma unroll
-            for (int i = 0; i < MT; i++)
+                for (int i = 0; i < MT; i++)
+                    a[i] = *reinterpret_cast<const b8_t *>(
+                        &la[(wave * 16 * MT + 16 * i + m + t) * AS + k8]);
 #pragma unroll
                 for (int n = 0; n < NTB; n++)
-                    acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[n], acc[i][n], 0,
-                                                                        0, 0);
+                    b[n] = *reinterpret_cast<const b8_t *>(&lb[(t * BCO + 16 * n + m) * AS + k8]);
+#pragma unroll
+                for (int i = 0; i < MT; i++)
+#pragma unroll
+                    for (int n = 0; n < NTB; n++)
+                        acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[n], acc[i][n],
+                                                                            0, 0, 0);
+            } else {
+                s4_t a[MT], b[NTB];
+#pragma unroll
+                for (int i = 0; i < MT; i++)
+                    a[i] = *reinterpret_cast<const s4_t *>(
+                        &la[(wave * 16 * MT + 16 * i + m + t) * AS + k8]);
+#pragma unroll
+                for (int n = 0; n < NTB; n++)
+                    b[n] = *reinterpret_cast<const s4_t *>(&lb[(t * BCO + 16 * n + m) * AS + k8]);
+#pragma unroll
+                for (int i = 0; i < MT; i++)
+#pragma unroll
+                    for (int n = 0; n < NTB; n++)
+                        acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[n],
+                                                                              acc[i][n], 0, 0, 0);
+            }
         }
     }
     const int kq = (lane >> 4) * 4;
@@ -644,6 +663,24 @@ static void conv_launch_c(hipStream_t s, Shape sh, int cin, int cout, const uint
         else
             k_conv3x3_lds<2, 2><<<dim3(bx, (unsigned)(cout / 32)), 256, 0, s>>>(
                 sh, cin, cout, x, nullptr, 0, wt, bias, relu, y, y32, y32c);
+        LAUNCH_CHECK();
+        return;
+    }
+    const bool k16 = cin % 16 == 0 && reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(wt) % 16 == 0;
+    if (k16 && sh.w % 128 == 0) {  // 16-channel stages in LDS
+        const bool wide = sh.w % 256 == 0;
+        const dim3 grid((unsigned)(np / (wide ? 256 : 128)), 1);
+        auto go = [&](auto kern, int ntb) {
+            kern<<<dim3(grid.x, (unsigned)(cout / (16 * ntb))), 256, 0, s>>>(
+                sh, cin, cout, x, nullptr, 0, wt, bias, relu, y, y32, y32c);
+        };
+        if (cout % 32 != 0)
+            wide ? go(k_conv3x3_lds<1, 4, 16>, 1) : go(k_conv3x3_lds<1, 2, 16>, 1);
+        else if (cout % 64 == 0)
+            wide ? go(k_conv3x3_lds<4, 4, 16>, 4) : go(k_conv3x3_lds<4, 2, 16>, 4);
+        else
+            wide ? go(k_conv3x3_lds<2, 4, 16>, 2) : go(k_conv3x3_lds<2, 2, 16>, 2);
         LAUNCH_CHECK();
         return;
     }

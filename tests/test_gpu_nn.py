@@ -67,10 +67,13 @@ def close_bf16(got, want, frac=0.98):
                                           (16, 16, 0, 64), (384, 128, 1, 192),
                                           (96, 32, 1, 256), (64, 64, 0, 128),
                                           (64, 128, 1, 256), (32, 16, 0, 256), (64, 48, 0, 128),
-                                          (32, 32, 1, 512), (32, 16, 0, 512), (96, 32, 1, 512)])
+                                          (32, 32, 1, 512), (32, 16, 0, 512), (96, 32, 1, 512),
+                                          (16, 32, 1, 128), (16, 32, 1, 256), (16, 16, 0, 128),
+                                          (16, 64, 0, 256), (48, 32, 1, 256)])
 def test_conv_forward(ws, ci, co, relu, w):
-    """w a multiple of 128 with cin a multiple of 32 takes the LDS-staged kernel, other multiples
-    of 64 the four-tile kernel, the rest the one-tile kernel."""
+    """w a multiple of 128 takes the LDS-staged kernel (32-channel stages for cin a multiple of 32,
+    else 16-channel stages), other multiples of 64 the four-tile kernel, the rest the one-tile
+    kernel."""
     rng = np.random.default_rng(ci * 7 + co + w)
     n, h = 2, 6
     x = N.bf16(rng.standard_normal((n, h, w, ci)))
