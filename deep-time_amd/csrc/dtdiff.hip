@@ -1203,7 +1203,7 @@ int dtd_nn_conv3x3(dtd_ctx *ctx, int n, int h, int w, int cin, int cout, const u
     return guard(ctx, [&] {
         begin(ctx);
         nn::check_shape(n, h, w), nn::check_channels(cin), nn::check_channels(cout);
-        if (!x || !wt || !y) throw Error{DTD_EINVAL, "null array"};
+        if (!x || !wt || (!y && !y32)) throw Error{DTD_EINVAL, "null array"};
         if (y32 && (y32c < 1 || y32c > cout)) throw Error{DTD_EINVAL, "y32c out of range"};
         nn::check_aligned(x, 8), nn::check_aligned(wt, 8);
         nn::conv(ctx->stream, nn::Shape{n, h, w}, cin, cout, x, wt, bias, relu, y, y32, y32c);

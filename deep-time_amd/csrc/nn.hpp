@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(Shape s, int cin_, int cout,
                 const long p = p0 + 16 * i + kq + q;
                 float v = acc[i][t][q] + bv;
                 if (relu) v = fmaxf(v, 0.0f);
-                y[p * cout + co] = f2bf(v);
+                if (y) y[p * cout + co] = f2bf(v);
                 if (y32 && co < y32c) y32[p * y32c + co] = v;
             }
     }
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(Shape s, int cin, int cout,
                 const long p = p0 + wave * 16 * MT + 16 * i + kq + q;
                 float v = acc[i][n][q] + bv;
                 if (relu) v = fmaxf(v, 0.0f);
-                y[p * cout + co] = f2bf(v);
+                if (y) y[p * cout + co] = f2bf(v);
                 if (y32 && co < y32c) y32[p * y32c + co] = v;
             }
     }

@@ -135,7 +135,8 @@ class UNet:
         n, h, w = self._level_shape(lvl)
         self._chk(self.L.dtd_nn_conv3x3(self.ws._ctx, n, h, w, ci, co, self._p(x),
                                         self._p(self.wb, a), self._p(self.w, b), relu,
-                                        self._p(y), self._p(out32) if out32 else None,
+                                        self._p(y) if y is not None else None,
+                                        self._p(out32) if out32 else None,
                                         OUTC if out32 else 0))
 
     def _pool(self, lvl, c, x, y):
@@ -201,7 +202,7 @@ class UNet:
         self._conv("dec1b", 1, B["d1a"], B["d1"])
         self._conv_cat("dec0a", 0, 64, B["d1"], B["e0"], B["d0a"])
         self._conv("dec0b", 0, B["d0a"], B["d0"])
-        self._conv("out", 0, B["d0"], B["y"], out32=B["out32"])
+        self._conv("out", 0, B["d0"], None, out32=B["out32"])  # the guess reads only out32
 
     # -- backward ----------------------------------------------------------------------------
     def _back(self, name, lvl, x_in, y_out, dy, dz, dx=None):

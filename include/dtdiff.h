@@ -272,7 +272,8 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
  * dtd_nn_loss (which returns the loss) and dtd_sync. */
 
 /* y = relu?(bias + conv3x3(x, wt)) (bias may be NULL); y32 (optional) also gets the first y32c
- * output channels in f32, y32[p y32c + ch]. */
+ * output channels in f32, y32[p y32c + ch].  y may be NULL when y32 is given (only the f32
+ * channels are wanted). */
 int dtd_nn_conv3x3(dtd_ctx *ctx, int n, int h, int w, int cin, int cout, const uint16_t *x,
                    const uint16_t *wt, const float *bias, int relu, uint16_t *y, float *y32,
                    int y32c);

@@ -88,7 +88,13 @@ def test_conv_forward(ws, ci, co, relu, w):
     z = N.conv3x3(x, wt, b)
     want = np.maximum(z, 0) if relu else z
     close_bf16(get_bf16(y, (n, h, w, co)), want)
-    assert np.allclose(get_f32(y32, (n, h, w, 2)), want[..., :2], rtol=1e-5, atol=1e-5)
+    got32 = get_f32(y32, (n, h, w, 2)).copy()
+    assert np.allclose(got32, want[..., :2], rtol=1e-5, atol=1e-5)
+    # Only the f32 channels: the same values, no bf16 output.
+    y32b = DeviceArray(n * h * w * 2, np.float32)
+    ws.check(L.dtd_nn_conv3x3(ws._ctx, n, h, w, ci, co, p(dx), p(dw), p(db), relu, None,
+                              p(y32b), 2))
+    assert np.array_equal(get_f32(y32b, (n, h, w, 2)), got32)
 
 
 @pytest.mark.parametrize("ci,co,w,h", [(16, 32, 48, 8), (64, 48, 48, 8), (32, 16, 48, 8),
@@ -341,6 +347,8 @@ def test_network_arguments_are_checked(ws):
                                                     p(a), None, 0))
     assert "multiples of 16" in err(L.dtd_nn_conv3x3(ws._ctx, 1, 4, 32, 24, 16, p(a), p(b), None,
                                                      0, p(a), None, 0))
+    assert "null" in err(L.dtd_nn_conv3x3(ws._ctx, 1, 4, 32, 16, 16, p(a), p(b), None, 0, None,
+                                          None, 0))
     # A tensor off its 16-byte alignment (2 bytes in).
     off = ctypes.c_void_p(a.data_ptr() + 2)
     assert "aligned" in err(L.dtd_nn_pool(ws._ctx, 1, 4, 32, 16, off, p(b)))
