@@ -1362,6 +1362,7 @@ int dtd_nn_input(dtd_ctx *ctx, size_t np, const double *h, const double *kd, con
         if (!h || !kd || !x) throw Error{DTD_EINVAL, "null array"};
         if (!(scale > 0.0) || !(dt > 0.0) || !(dx > 0.0))
             throw Error{DTD_EINVAL, "scale, dt and dx must be positive"};
+        nn::check_aligned(x, 16);
         if (np == 0) return;
         nn::k_nn_input<<<nn::blocks((long)np), 256, 0, ctx->stream>>>((long)np, h, kd, base, dt,
                                                                       dx * dx, scale, x);

@@ -544,13 +544,14 @@ __global__ __launch_bounds__(256) void k_nn_input(long np, const double *__restr
     const long p = (long)blockIdx.x * 256 + threadIdx.x;
     if (p >= np) return;
     const double k = fmax(kd[p], 1e-30);
-    uint16_t *o = x + p * 16;
-    o[0] = f2bf((float)(h[p] / scale));
-    o[1] = f2bf((float)log(k));
-    o[2] = f2bf((float)(0.25 * log(dt * k / dx2)));
-    o[3] = base ? f2bf((float)(base[p] / scale)) : (uint16_t)0;
-#pragma unroll
-    for (int c = 4; c < 16; c++) o[c] = 0;
+    const unsigned c0 = f2bf((float)(h[p] / scale));
+    const unsigned c1 = f2bf((float)log(k));
+    const unsigned c2 = f2bf((float)(0.25 * log(dt * k / dx2)));
+    const unsigned c3 = base ? f2bf((float)(base[p] / scale)) : 0u;
+    // The pixel's 16 channels (32 bytes, 16-byte aligned) in two stores.
+    u4_t *o = reinterpret_cast<u4_t *>(x + p * 16);
+    o[0] = u4_t{c0 | (c1 << 16), c2 | (c3 << 16), 0u, 0u};
+    o[1] = u4_t{0u, 0u, 0u, 0u};
 }
 
 // The guess: g = base + scale out[p][0] (the network predicts the change from the base guess).

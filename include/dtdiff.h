@@ -319,8 +319,8 @@ int dtd_nn_up_cat(dtd_ctx *ctx, int n, int h, int w, int cu, int cs, const uint1
 int dtd_nn_up_cat_bwd(dtd_ctx *ctx, int n, int h, int w, int cu, int cs, const uint16_t *dcat,
                       const uint16_t *dpool, uint16_t *du, uint16_t *dskip);
 
-/* The network's input x (16 channels) from f64 h, kd and the base guess (optional): h / scale,
- * ln kd, ln(dt kd / dx^2) / 4, base / scale (0 when base is NULL), zeros. */
+/* The network's input x (16 channels, 16-byte aligned) from f64 h, kd and the base guess
+ * (optional): h / scale, ln kd, ln(dt kd / dx^2) / 4, base / scale (0 when base is NULL), zeros. */
 int dtd_nn_input(dtd_ctx *ctx, size_t np, const double *h, const double *kd, const double *base,
                  double dt, double dx, double scale, uint16_t *x);
 

@@ -211,6 +211,26 @@ def test_inference_forward_equals_the_training_forward(ws):
     assert np.array_equal(get_f32(net.B["out32"], (-1,)), want)
 
 
+def test_input_channels_match_the_oracle(ws):
+    """dtd_nn_input's 16 channels (two 16-byte stores a pixel) against the oracle's, rounded to
+    bf16; channels 4-15 are zero."""
+    rng = np.random.default_rng(9)
+    n = 3000
+    h = rng.standard_normal(n) * 800.0
+    kd = np.exp(rng.standard_normal(n))
+    kd[:5] = 0.0  # clamped to 1e-30 before the logarithms
+    base = h + rng.standard_normal(n) * 30.0
+    L = dtdiff.load()
+    x = DeviceArray(n * 16, np.uint16)
+    hd, kdd, bd = (DeviceArray.from_numpy(v) for v in (h, kd, base))
+    ws.check(L.dtd_nn_input(ws._ctx, n, p(hd), p(kdd), p(bd), 20.0, 3.0, LN.SCALE, p(x)))
+    got = get_bf16(x, (n, 16))
+    want = N.input_channels(h.reshape(1, 1, n), kd.reshape(1, 1, n), 20.0, 3.0, LN.SCALE,
+                            base=base.reshape(1, 1, n)).reshape(n, 16)
+    close_bf16(got, want, frac=0.999)
+    assert not got[:, 4:].any()
+
+
 def test_adam_step(ws):
     rng = np.random.default_rng(6)
     n = 1000
