@@ -489,9 +489,10 @@ static void lod(dtd_ctx *ctx, const double *b, double *x, const double *kd, doub
     }
     if (x != b)
         HIPCHK(hipMemcpyAsync(x, b, ws.n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    k_lod_rows<<<nblocks(ws.h), kBlock, 0, s>>>(g, wrap_x, kd, dt, s2, x, ws.p0, ws.p1);
+    k_lod_rows<<<(ws.h + kLineBlock - 1) / kLineBlock, kLineBlock, 0, s>>>(g, wrap_x, kd, dt, s2, x,
+                                                                           ws.p0, ws.p1);
     LAUNCH_CHECK();
-    k_lod_cols<<<nblocks(ws.w), kBlock, 0, s>>>(g, kd, dt, s2, x, ws.p0);
+    k_lod_cols<<<(ws.w + kLineBlock - 1) / kLineBlock, kLineBlock, 0, s>>>(g, kd, dt, s2, x, ws.p0);
     LAUNCH_CHECK();
 }
 

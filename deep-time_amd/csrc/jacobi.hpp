@@ -366,10 +366,10 @@ __global__ void kj_init(JState *st, double tol2, int max_iter) {
 // The first guess with the reference's recurrences (bit for bit), the rows on the transposed
 // grid (a wave's rows side by side in memory): kd^T in kdt, b^T in xt (solved in place), cbuf
 // and zbuf scratch, all w x h.
-__global__ __launch_bounds__(kBlock) void kj_lod_rows_t(Grid g, int wrap_x, const double *kdt,
+__global__ __launch_bounds__(kLineBlock) void kj_lod_rows_t(Grid g, int wrap_x, const double *kdt,
                                                         double *xt, double *cbuf, double *zbuf,
                                                         double dt, double s2) {
-    const int j = blockIdx.x * kBlock + threadIdx.x;
+    const int j = blockIdx.x * kLineBlock + threadIdx.x;
     const int h = g.h;
     if (j < h)
         lod_line_x(Line<const double>{kdt + j, h}, Line<double>{xt + j, h},
@@ -437,12 +437,13 @@ static size_t jacobi_solve(dtd_ctx *ctx, const double *b, const double *kd, doub
         LAUNCH_CHECK();
         k_transpose<<<tg, 256, 0, s>>>(b, ws.ap, ws.h, ws.w);
         LAUNCH_CHECK();
-        kj_lod_rows_t<<<nblocks(ws.h), kBlock, 0, s>>>(g, wrap_x, ws.r, ws.ap, ws.p0, ws.p1, dt,
+        kj_lod_rows_t<<<(ws.h + kLineBlock - 1) / kLineBlock, kLineBlock, 0, s>>>(g, wrap_x, ws.r, ws.ap, ws.p0, ws.p1, dt,
                                                         s2);
         LAUNCH_CHECK();
         k_transpose<<<tgt, 256, 0, s>>>(ws.ap, ws.x, ws.w, ws.h);
         LAUNCH_CHECK();
-        k_lod_cols<<<nblocks(ws.w), kBlock, 0, s>>>(g, kd, dt, s2, ws.x, ws.p0);
+        k_lod_cols<<<(ws.w + kLineBlock - 1) / kLineBlock, kLineBlock, 0, s>>>(g, kd, dt, s2, ws.x,
+                                                                              ws.p0);
         LAUNCH_CHECK();
     }
     HIPCHK(hipEventRecord(ctx->ev[2], s));

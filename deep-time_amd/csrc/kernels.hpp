@@ -1745,6 +1745,9 @@ struct Line {
 // kLodAhead positions ahead, in registers (chunk c + 1 is loaded while chunk c is being used).
 // The arithmetic is the recurrences', operation for operation.
 constexpr int kLodAhead = 16;
+// One wave a block for the one-thread-a-line kernels: their few waves then spread over as many
+// CUs (each with its own L1 and memory pipeline) instead of sharing a few.
+constexpr int kLineBlock = 64;
 
 
 // v[q] = L[i0 + D q] (D = 1 or -1) for the positions inside [0, n); 0 elsewhere.
@@ -1968,10 +1971,10 @@ __device__ __forceinline__ void lod_row(const Grid &g, int wrap_x, const double 
                Line<double>{zbuf + o, 1}, g.w, wrap_x, dt, s2);
 }
 
-__global__ __launch_bounds__(kBlock) void k_lod_rows(Grid g, int wrap_x, const double *kd, double dt,
+__global__ __launch_bounds__(kLineBlock) void k_lod_rows(Grid g, int wrap_x, const double *kd, double dt,
                                                      double s2, double *x, double *cbuf,
                                                      double *zbuf) {
-    const int j = blockIdx.x * kBlock + threadIdx.x;
+    const int j = blockIdx.x * kLineBlock + threadIdx.x;
     if (j < g.h) lod_row(g, wrap_x, kd, dt, s2, x, cbuf, zbuf, j);
 }
 
@@ -1983,9 +1986,9 @@ __device__ __forceinline__ void lod_col(const Grid &g, const double *kd, double 
                Line<double>{cbuf + i, g.w}, g.h, dt, s2);
 }
 
-__global__ __launch_bounds__(kBlock) void k_lod_cols(Grid g, const double *kd, double dt, double s2,
+__global__ __launch_bounds__(kLineBlock) void k_lod_cols(Grid g, const double *kd, double dt, double s2,
                                                      double *x, double *cbuf) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = blockIdx.x * kLineBlock + threadIdx.x;
     if (i < g.w) lod_col(g, kd, dt, s2, x, cbuf, i);
 }
 
