@@ -96,22 +96,22 @@ __global__ __launch_bounds__(kBlock) void kb_transpose(const BGrid *__restrict__
     const int ntc = (cols + 31) / 32;
     transpose_tile(src, dst, rows, cols, (m.y / ntc) * 32, (m.y % ntc) * 32, tile);
 }
-__global__ __launch_bounds__(kBlock) void kb_lod_rows(const BGrid *__restrict__ t,
-                                                      const int2 *__restrict__ blk) {
+__global__ __launch_bounds__(kLineBlock) void kb_lod_rows(const BGrid *__restrict__ t,
+                                                          const int2 *__restrict__ blk) {
     const int2 m = blk[blockIdx.x];
     const BGrid &q = t[m.x];
-    const int j = m.y * kBlock + threadIdx.x;
+    const int j = m.y * kLineBlock + threadIdx.x;
     const int h = q.g.h;
     if (j < h)
         lod_line_x(Line<const double>{q.r + j, h}, Line<double>{q.ap + j, h},
                    Line<double>{q.p0 + j, h}, Line<double>{q.p1 + j, h}, q.g.w, q.wrap_x, q.dt,
                    q.s2);
 }
-__global__ __launch_bounds__(kBlock) void kb_lod_cols(const BGrid *__restrict__ t,
-                                                      const int2 *__restrict__ blk) {
+__global__ __launch_bounds__(kLineBlock) void kb_lod_cols(const BGrid *__restrict__ t,
+                                                          const int2 *__restrict__ blk) {
     const int2 m = blk[blockIdx.x];
     const BGrid &q = t[m.x];
-    const int i = m.y * kBlock + threadIdx.x;
+    const int i = m.y * kLineBlock + threadIdx.x;
     if (i < q.g.w) lod_col(q.g, q.kd, q.dt, q.s2, q.x, q.p0, i);
 }
 
@@ -462,9 +462,9 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
     for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].w * gs[g].h, kCgCells);
     std::vector<int2> rng_update, rng_band, rng_rz(n);
     add_launch(bw->update_l, cnt, &rng_update);
-    for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].h);
+    for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].h, kLineBlock);
     add_launch(bw->rows_l, cnt, nullptr);
-    for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].w);
+    for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].w, kLineBlock);
     add_launch(bw->cols_l, cnt, nullptr);
     for (size_t g = 0; g < n; g++) cnt[g] = (int)(((gs[g].w + 31) / 32) * ((gs[g].h + 31) / 32));
     add_launch(bw->tiles_l, cnt, nullptr);
@@ -721,11 +721,11 @@ static void solve_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs_in, doubl
             kb_transpose<<<bw.tiles_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.tiles_l.blk, mode);
             LAUNCH_CHECK();
         }
-        kb_lod_rows<<<bw.rows_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.rows_l.blk);
+        kb_lod_rows<<<bw.rows_l.nblk, kLineBlock, 0, s>>>(bw.d_grids, bw.rows_l.blk);
         LAUNCH_CHECK();
         kb_transpose<<<bw.tiles_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.tiles_l.blk, 2);
         LAUNCH_CHECK();
-        kb_lod_cols<<<bw.cols_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.cols_l.blk);
+        kb_lod_cols<<<bw.cols_l.nblk, kLineBlock, 0, s>>>(bw.d_grids, bw.cols_l.blk);
         LAUNCH_CHECK();
     }
     HIPCHK(hipEventRecord(ctx->ev[2], s));
