@@ -19,14 +19,36 @@ namespace dtd {
 struct Error {
     int code;
     std::string msg;
+    bool hip = false;  // raised by a HIP call (the context's device state is then dropped)
 };
+
+// Failure injection (a test hook, dtd_inject_fault): counts the checked HIP calls made for a
+// context and, once armed, skips the call `after` further calls on and reports it as a device
+// error, so the error paths run without any real device fault.
+struct FaultHook {
+    long after = -1;  // < 0: disarmed
+    long calls = 0;
+};
+inline thread_local FaultHook *t_fault = nullptr;  // the hook of the entry point running here
+inline bool fault_now() {
+    FaultHook *f = t_fault;
+    if (!f) return false;
+    f->calls++;
+    if (f->after < 0) return false;
+    if (f->after-- > 0) return false;
+    f->after = -1;
+    return true;
+}
 
 #define HIPCHK(expr)                                                                     \
     do {                                                                                 \
+        if (::dtd::fault_now())                                                          \
+            throw ::dtd::Error{DTD_EDEVICE, std::string(#expr) + ": injected failure",   \
+                               true};                                                    \
         hipError_t e_ = (expr);                                                          \
         if (e_ != hipSuccess)                                                            \
             throw ::dtd::Error{e_ == hipErrorOutOfMemory ? DTD_ETOOBIG : DTD_EDEVICE,    \
-                               std::string(#expr) + ": " + hipGetErrorString(e_)};       \
+                               std::string(#expr) + ": " + hipGetErrorString(e_), true}; \
     } while (0)
 
 #define LAUNCH_CHECK() HIPCHK(hipGetLastError())
@@ -64,6 +86,7 @@ struct Workspace {
     double *b = nullptr, *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr;
     double *r2 = nullptr;  // the residual's second buffer (the f32 cycle's fused update)
     double *ap = nullptr, *kd = nullptr, *parts = nullptr, *pair = nullptr;
+    long nparts = 0;  // blocks whose partial sums parts holds (two doubles a block)
     double2 *es = nullptr;  // the f64 operator: (east, south) of each cell
     CgState *st = nullptr;
     std::vector<Lvl<float>> lf;
@@ -96,6 +119,9 @@ struct dtd_ctx {
     std::string err;
     size_t cycle_levels = 0;  // levels built by dtd_cycle_new
     bool use_graph = true;
+    dtd::FaultHook fault;         // dtd_inject_fault
+    double *stage = nullptr;      // pinned host staging of dtd_solve's answer
+    size_t stage_n = 0;
     dtd::BatchWs *batch = nullptr;  // the batched solve's buffers (batch.hpp)
     dtd::FailWs *fail = nullptr;    // the slope failure's buffers (failure.hpp)
 };
@@ -136,6 +162,20 @@ int stream_grid(int w, int h, int th) {
     return (stream_waves<V, H>(w, h, th) + kBlock / 64 - 1) / (kBlock / 64);
 }
 
+// The most blocks a launch over a w x h grid can have when each block leaves a partial sum: the
+// cell-parallel passes take 256 cells a block; the streamed passes at their narrowest bands
+// (60 owned columns, bands of at least 4 rows, 4 waves a block) take more on narrow grids.
+inline long parts_cap(int w, int h) {
+    const long waves = (long)((w + 59) / 60) * ((h + 3) / 4);
+    return std::max<long>(nblocks((long)w * h), (waves + kBlock / 64 - 1) / (kBlock / 64));
+}
+
+// A launch of nb blocks that each write a partial sum (or a pair) must fit the buffer.
+inline void need_parts(long nb, long cap) {
+    if (nb > cap)
+        throw Error{DTD_ESTATE, "a launch would write more partial sums than its buffer holds"};
+}
+
 inline float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, a, b));
@@ -164,26 +204,47 @@ inline void download(dtd_ctx *ctx, void *dst, const void *src, size_t bytes) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
 }
 
+// After a device error: end a stream capture the error interrupted, let the work already
+// queued finish, and drop every buffer, level and graph, so the next call builds them afresh.
+inline void drop_device_state(dtd_ctx *ctx) {
+    if (!ctx->stream) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(ctx->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(ctx->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipGetLastError();
+    free_ws(ctx->ws);
+    free_batch(ctx);
+    free_fail(ctx);
+    ctx->cycle_levels = 0;
+}
+
+// Runs one entry point's body: errors become a code and the context's message; an error a HIP
+// call raised (a device error, or running out of device memory) or a host exception drops the
+// context's device state, which the next call builds afresh.
 template <class F>
 int guard(dtd_ctx *ctx, F f) {
+    FaultHook *outer = t_fault;
+    t_fault = ctx ? &ctx->fault : nullptr;
+    int code = DTD_OK;
+    bool drop = false;
     try {
         f();
-        return DTD_OK;
     } catch (const Error &e) {
         if (ctx) ctx->err = e.msg;
-        if ((e.code == DTD_EDEVICE || e.code == DTD_ETOOBIG) && ctx) {
-            // A failed shape is rebuilt on the next call.
-            (void)hipStreamSynchronize(ctx->stream);
-            free_ws(ctx->ws);
-            free_batch(ctx);
-            free_fail(ctx);
-            ctx->cycle_levels = 0;
-        }
-        return e.code;
+        code = e.code;
+        drop = e.hip;
     } catch (const std::exception &e) {
         if (ctx) ctx->err = e.what();
-        return DTD_EDEVICE;
+        code = DTD_EDEVICE;
+        drop = true;
     }
+    t_fault = outer;
+    if (drop && ctx) drop_device_state(ctx);
+    return code;
 }
 
 }  // namespace dtd

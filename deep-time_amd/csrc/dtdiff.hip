@@ -54,6 +54,7 @@ static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
     const int prec = ctx->opts.cycle_precision;
     if (ws.w == w && ws.h == h && ws.wrap == wrap && ws.prec == prec) return;
     free_ws(ws);
+    ctx->cycle_levels = 0;  // a cycle built by dtd_cycle_new belonged to the old shape
     const long n = (long)w * h;
     ws.w = w;
     ws.h = h;
@@ -68,7 +69,8 @@ static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
     ws.p1 = dalloc<double>(n);
     ws.ap = dalloc<double>(n);
     ws.kd = dalloc<double>(n);
-    ws.parts = dalloc<double>(2 * (size_t)nblocks(n) + 2);
+    ws.nparts = parts_cap(w, h);
+    ws.parts = dalloc<double>(2 * (size_t)ws.nparts + 2);
     ws.pair = dalloc<double>(2);
     ws.st = dalloc<CgState>(1);
     ws.es = dalloc<double2>(n);
@@ -180,6 +182,7 @@ static void pack_bf16(Workspace &ws, hipStream_t s) {
 static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt) {
     Workspace &ws = ctx->ws;
     hipStream_t s = ctx->stream;
+    ctx->cycle_levels = 0;  // the levels now hold this solve's operator, not dtd_cycle_new's
     const bool f64 = ws.prec == DTD_CYCLE_F64;
     // The f64 passes form the faces from kd themselves; only the cycle's finest records are kept.
     k_operator<<<nblocks(ws.n), kBlock, 0, s>>>(grid_of(ws), kd, dt, dx * dx, nullptr,
@@ -197,36 +200,40 @@ static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt)
 // with bf16 records is smoothed from them (CB = 1).
 template <typename T, int V, int RM, int CB>
 static int launch_pre_cb(Lvl<T> &L, Lvl<T> &C, T *zout, const CgState *st, hipStream_t s,
-                         R64 rio, double *parts) {
+                         R64 rio, double *parts, long cap) {
     const int th = band_height<V>((const void *)k_spre<T, V, RM, CB>, L.w, L.h);
     const int nb = stream_grid<V, LaneGeom<V>::halo>(L.w, L.h, th);
+    if (RM == kRUpdate) need_parts(nb, cap);
     k_spre<T, V, RM, CB><<<nb, kBlock, 0, s>>>(L, zout, C, th, st, rio, parts);
     LAUNCH_CHECK();
     return nb;
 }
 template <typename T, int V, int RM = kRLevel>
 static int launch_pre(Lvl<T> &L, Lvl<T> &C, T *zout, const CgState *st, hipStream_t s,
-                      R64 rio = R64{}, double *parts = nullptr) {
+                      R64 rio = R64{}, double *parts = nullptr, long cap = 0) {
     if constexpr (std::is_same<T, float>::value)
-        if (L.coefh) return launch_pre_cb<T, V, RM, 1>(L, C, zout, st, s, rio, parts);
-    return launch_pre_cb<T, V, RM, 0>(L, C, zout, st, s, rio, parts);
+        if (L.coefh) return launch_pre_cb<T, V, RM, 1>(L, C, zout, st, s, rio, parts, cap);
+    return launch_pre_cb<T, V, RM, 0>(L, C, zout, st, s, rio, parts, cap);
 }
 
 template <typename T, int V, int RM, int CB>
 static int launch_post_cb(Lvl<T> &L, const T *zin, const T *zc, int wc, T *zout,
-                          const double *dot, double *parts, const CgState *st, hipStream_t s) {
+                          const double *dot, double *parts, long cap, const CgState *st,
+                          hipStream_t s) {
     const int th = band_height<V>((const void *)k_spost<T, V, RM, CB>, L.w, L.h);
     const int nb = stream_grid<V, LaneGeom<V>::halo>(L.w, L.h, th);
+    if (dot) need_parts(nb, cap);
     k_spost<T, V, RM, CB><<<nb, kBlock, 0, s>>>(L, zin, zc, wc, zout, th, dot, parts, st);
     LAUNCH_CHECK();
     return nb;
 }
 template <typename T, int V, int RM = kRLevel>
 static int launch_post(Lvl<T> &L, const T *zin, const T *zc, int wc, T *zout, const double *dot,
-                       double *parts, const CgState *st, hipStream_t s) {
+                       double *parts, long cap, const CgState *st, hipStream_t s) {
     if constexpr (std::is_same<T, float>::value)
-        if (L.coefh) return launch_post_cb<T, V, RM, 1>(L, zin, zc, wc, zout, dot, parts, st, s);
-    return launch_post_cb<T, V, RM, 0>(L, zin, zc, wc, zout, dot, parts, st, s);
+        if (L.coefh)
+            return launch_post_cb<T, V, RM, 1>(L, zin, zc, wc, zout, dot, parts, cap, st, s);
+    return launch_post_cb<T, V, RM, 0>(L, zin, zc, wc, zout, dot, parts, cap, st, s);
 }
 
 // The streamed direction pass; returns its block count (the p.Ap partial sums).
@@ -235,6 +242,7 @@ static int launch_dir(Workspace &ws, const TZ *z, const double *pold, double *pn
                       hipStream_t s) {
     const int th = band_height<1, 1>((const void *)k_sdir<TZ>, ws.w, ws.h);
     const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
+    need_parts(nb, ws.nparts);
     k_sdir<TZ><<<nb, kBlock, 0, s>>>(grid_of(ws), z, pold, pnew, ws.ap, ws.x, th, ws.parts, ws.st);
     LAUNCH_CHECK();
     return nb;
@@ -248,6 +256,7 @@ static int launch_resid(Workspace &ws, const double *kd, double dt, double s2, c
     if (ws.prec == DTD_CYCLE_F64) {
         const int th = band_height<1, 1>((const void *)k_sresid<double>, ws.w, ws.h);
         const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
+        need_parts(nb, ws.nparts);
         k_sresid<double><<<nb, kBlock, 0, s>>>(g, kd, dt, s2, b, xin, ws.x, ws.r, (double *)nullptr,
                                                th, ws.parts);
         LAUNCH_CHECK();
@@ -255,6 +264,7 @@ static int launch_resid(Workspace &ws, const double *kd, double dt, double s2, c
     }
     const int th = band_height<1, 1>((const void *)k_sresid<float>, ws.w, ws.h);
     const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
+    need_parts(nb, ws.nparts);
     k_sresid<float><<<nb, kBlock, 0, s>>>(g, kd, dt, s2, b, xin, ws.x, ws.r, r32 ? ws.lf[0].r : nullptr,
                                           th, ws.parts);
     LAUNCH_CHECK();
@@ -288,9 +298,10 @@ static bool fused_enabled() {
 // rm (f32 levels, fused smoothing, finest level only): where the finest level's r comes from
 // (kRLevel: its own r; kR64: the f64 residual dot; kRUpdate: rio.in - alpha A p, written to
 // rio.out = dot, with the r.r step of the conjugate gradients launched after the pre-smoothing).
+// parts holds the partial sums of at most cap blocks.
 template <typename T>
 static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, double *parts,
-                  CgState *st, hipStream_t s, int rm = kRLevel, R64 rio = R64{},
+                  long cap, CgState *st, hipStream_t s, int rm = kRLevel, R64 rio = R64{},
                   const Lvl<T> *dlv = nullptr) {
     Lvl<T> &L = lv[l];
     size_t lds = 0;
@@ -309,6 +320,7 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
         return 1;
     }
     const int nb = nblocks(L.n);
+    if (dot) need_parts(nb, cap);
     Lvl<T> &C0 = lv[l + 1];
     if (nu == 2 && fused_enabled()) {
         // Fused smoothing (kernels.hpp): pre-smoothing and restriction in one launch, the coarse
@@ -320,8 +332,9 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
             if (rm != kRLevel) {
                 if (l != 0) throw Error{DTD_ESTATE, "the f64 residual feeds the finest level only"};
                 if (rm == kRUpdate) {
-                    const int nbp = wide ? launch_pre<T, VW, kRUpdate>(L, C0, L.z[0], st, s, rio, parts)
-                                         : launch_pre<T, 1, kRUpdate>(L, C0, L.z[0], st, s, rio, parts);
+                    const int nbp =
+                        wide ? launch_pre<T, VW, kRUpdate>(L, C0, L.z[0], st, s, rio, parts, cap)
+                             : launch_pre<T, 1, kRUpdate>(L, C0, L.z[0], st, s, rio, parts, cap);
                     k_cg_step<kStepRR><<<1, kReduceBlock, 0, s>>>(parts, nbp, st);
                     LAUNCH_CHECK();
                 } else if (wide) {
@@ -329,20 +342,20 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
                 } else {
                     launch_pre<T, 1, kR64>(L, C0, L.z[0], st, s, rio);
                 }
-                vcycle<T>(lv, l + 1, nu, nullptr, parts, st, s, kRLevel, R64{}, dlv);
+                vcycle<T>(lv, l + 1, nu, nullptr, parts, cap, st, s, kRLevel, R64{}, dlv);
                 return wide ? launch_post<T, VW, kR64>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts,
-                                                        st, s)
+                                                        cap, st, s)
                             : launch_post<T, 1, kR64>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts,
-                                                       st, s);
+                                                       cap, st, s);
             }
         }
         if (wide)
             launch_pre<T, VW>(L, C0, L.z[0], st, s);
         else
             launch_pre<T, 1>(L, C0, L.z[0], st, s);
-        vcycle<T>(lv, l + 1, nu, nullptr, parts, st, s, kRLevel, R64{}, dlv);
-        return wide ? launch_post<T, VW>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts, st, s)
-                    : launch_post<T, 1>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts, st, s);
+        vcycle<T>(lv, l + 1, nu, nullptr, parts, cap, st, s, kRLevel, R64{}, dlv);
+        return wide ? launch_post<T, VW>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts, cap, st, s)
+                    : launch_post<T, 1>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts, cap, st, s);
     }
     int cur = 0;
     k_first<T><<<nb, kBlock, 0, s>>>(L, L.z[0], st);
@@ -355,7 +368,7 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
     }
     k_restrict<T><<<nblocks(C0.n), kBlock, 0, s>>>(L, L.z[cur], C0, st);
     LAUNCH_CHECK();
-    vcycle<T>(lv, l + 1, nu, nullptr, parts, st, s, kRLevel, R64{}, dlv);
+    vcycle<T>(lv, l + 1, nu, nullptr, parts, cap, st, s, kRLevel, R64{}, dlv);
     {
         const double *d = nu == 1 ? dot : nullptr;
         k_sweep<T><<<nb, kBlock, 0, s>>>(L, L.z[cur], L.z[cur ^ 1], C0.z[1], C0.w, d, parts, st);
@@ -377,9 +390,10 @@ static int run_vcycle(dtd_ctx *ctx, const double *dot, CgState *st, int rm = kRL
     Workspace &ws = ctx->ws;
     const int nu = (int)ctx->opts.nu;
     if (ws.prec == DTD_CYCLE_F64)
-        return vcycle<double>(ws.ld, 0, nu, dot, ws.parts, st, ctx->stream, kRLevel, R64{},
-                              ws.d_ld);
-    return vcycle<float>(ws.lf, 0, nu, dot, ws.parts, st, ctx->stream, rm, rio, ws.d_lf);
+        return vcycle<double>(ws.ld, 0, nu, dot, ws.parts, ws.nparts, st, ctx->stream, kRLevel,
+                              R64{}, ws.d_ld);
+    return vcycle<float>(ws.lf, 0, nu, dot, ws.parts, ws.nparts, st, ctx->stream, rm, rio,
+                         ws.d_lf);
 }
 
 // Whether the f32 cycle's finest level takes the f64 residual straight from the conjugate
@@ -498,6 +512,15 @@ static void lod(dtd_ctx *ctx, const double *b, double *x, const double *kd, doub
 
 enum Guess { kGuessLod, kGuessRhs, kGuessKeep };
 
+// The device entry points' last step: once the solve has finished without error, its answer
+// (ws.x) is copied into the caller's h_dev.  A failure before this point leaves h_dev as it was.
+static void copy_answer(dtd_ctx *ctx, double *h_dev) {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpyAsync(h_dev, ctx->ws.x, ctx->ws.n * sizeof(double), hipMemcpyDeviceToDevice,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+}
+
 // One solve on the workspace (`multigrid_in` + `cg_on`, D:101-116, MG:562-763).  b is the
 // right-hand side on the device (the caller's array or ws.b; only read); kd likewise.  The first
 // guess is the splitting of b (or b itself when the context says so), or (kGuessKeep) the x
@@ -551,6 +574,7 @@ static size_t solve_core(dtd_ctx *ctx, const double *b, const double *kd, double
     {
         const int th = band_height<1, 1>((const void *)k_finish1, ws.w, ws.h);
         const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
+        need_parts(nb, ws.nparts);
         k_finish1<<<nb, kBlock, 0, s>>>(g, kd, dt, dx * dx, b, ws.x, ws.p0, ws.p1, ws.ap, th, ws.parts,
                                         ws.st);
         LAUNCH_CHECK();
@@ -558,6 +582,7 @@ static size_t solve_core(dtd_ctx *ctx, const double *b, const double *kd, double
         LAUNCH_CHECK();
         const int th2 = band_height<1, 1>((const void *)k_finish2, ws.w, ws.h);
         const int nb2 = stream_grid<1, 1>(ws.w, ws.h, th2);
+        need_parts(nb2, ws.nparts);
         k_finish2<<<nb2, kBlock, 0, s>>>(g, kd, dt, dx * dx, b, ws.ap, ws.x, th2, ws.parts, ws.st);
         LAUNCH_CHECK();
         k_pair_sum<<<1, kReduceBlock, 0, s>>>(ws.parts, nb2, ws.pair);
@@ -693,6 +718,7 @@ void dtd_destroy(dtd_ctx *ctx) {
     if (ctx->host_st) (void)hipHostFree(ctx->host_st);
     if (ctx->host_flag) (void)hipHostFree(ctx->host_flag);
     if (ctx->host_pair) (void)hipHostFree(ctx->host_pair);
+    if (ctx->stage) (void)hipHostFree(ctx->stage);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -722,7 +748,16 @@ int dtd_solve(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h
                 ? jacobi_solve(ctx, ws.b, ws.kd, dx, dt, wrap_x, nullptr)
                 : solve_core(ctx, ws.b, ws.kd, dx, dt, wrap_x, tol_m, 0.0, true, kGuessLod,
                              nullptr, nullptr, nullptr);
-        download(ctx, h_inout, ws.x, n * sizeof(double));
+        // The answer comes back through pinned staging and reaches h_inout only once every
+        // device call has succeeded, so h_inout keeps h0 whatever fails.
+        if (ctx->stage_n < n) {
+            if (ctx->stage) (void)hipHostFree(ctx->stage);
+            ctx->stage = nullptr, ctx->stage_n = 0;
+            HIPCHK(hipHostMalloc((void **)&ctx->stage, n * sizeof(double), hipHostMallocDefault));
+            ctx->stage_n = n;
+        }
+        download(ctx, ctx->stage, ws.x, n * sizeof(double));
+        std::memcpy(h_inout, ctx->stage, n * sizeof(double));
         if (iterations) *iterations = its;
     });
 }
@@ -739,14 +774,11 @@ int dtd_solve_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, do
         if (n == 0) return;
         if (!h_dev || !kd_dev) throw Error{DTD_EINVAL, "null array"};
         fit(ctx, (int)w, (int)h, wrap_x && w > 1);
-        Workspace &ws = ctx->ws;
         double rel = 0.0;
         solve_core(ctx, h_dev, kd_dev, dx, dt, wrap_x, tol_m, 0.0, true, kGuessLod, stats, &rel,
                    nullptr);
         if (stats) stats->rel_residual = rel;
-        HIPCHK(hipMemcpyAsync(h_dev, ws.x, n * sizeof(double), hipMemcpyDeviceToDevice,
-                              ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
+        copy_answer(ctx, h_dev);
     });
 }
 
@@ -764,7 +796,6 @@ int dtd_solve_rel_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x
         if (n == 0) return;
         if (!h_dev || !kd_dev) throw Error{DTD_EINVAL, "null array"};
         fit(ctx, (int)w, (int)h, wrap_x && w > 1);
-        Workspace &ws = ctx->ws;
         // The first solve stops at r.r <= rel^2 b.b (tol_m = rel ||b|| / sqrt(n)), b.b formed on
         // the device by the first residual's pass.
         double rel = 0.0, bnorm = 0.0;
@@ -780,9 +811,7 @@ int dtd_solve_rel_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x
                        &rel, &bnorm);
         }
         st->rel_residual = rel;
-        HIPCHK(hipMemcpyAsync(h_dev, ws.x, n * sizeof(double), hipMemcpyDeviceToDevice,
-                              ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
+        copy_answer(ctx, h_dev);
     });
 }
 
@@ -865,8 +894,8 @@ int dtd_cube_solve_device(dtd_ctx *ctx, dtd_cube *g, double *h_dev, const double
         if (!h_dev || !kd_dev) throw Error{DTD_EINVAL, "null array"};
         if (solver != DTD_SOLVER_MULTIGRID && solver != DTD_SOLVER_JACOBI)
             throw Error{DTD_EINVAL, "unknown solver"};
-        if (solver == DTD_SOLVER_MULTIGRID && !(tol_m > 0.0))
-            throw Error{DTD_EINVAL, "tol_m must be > 0"};
+        if (solver == DTD_SOLVER_MULTIGRID && !(std::isfinite(tol_m) && tol_m > 0.0))
+            throw Error{DTD_EINVAL, "tol_m must be finite and positive"};
         const size_t its = cube_solve(ctx, g, h_dev, kd_dev, dt, solver, tol_m);
         if (iterations) *iterations = its;
     });
@@ -881,8 +910,8 @@ int dtd_cube_solve(dtd_ctx *ctx, dtd_cube *g, double *h_inout, const double *kd,
         if (!h_inout || !kd) throw Error{DTD_EINVAL, "null array"};
         if (solver != DTD_SOLVER_MULTIGRID && solver != DTD_SOLVER_JACOBI)
             throw Error{DTD_EINVAL, "unknown solver"};
-        if (solver == DTD_SOLVER_MULTIGRID && !(tol_m > 0.0))
-            throw Error{DTD_EINVAL, "tol_m must be > 0"};
+        if (solver == DTD_SOLVER_MULTIGRID && !(std::isfinite(tol_m) && tol_m > 0.0))
+            throw Error{DTD_EINVAL, "tol_m must be finite and positive"};
         const size_t bytes = (size_t)g->n6 * sizeof(double);
         if (!g->hd) {
             HIPCHK(hipMalloc(&g->hd, bytes));
@@ -931,11 +960,8 @@ int dtd_step_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, dou
         if (n == 0) return;
         if (!h_dev || !kd_dev) throw Error{DTD_EINVAL, "null array"};
         fit(ctx, (int)w, (int)h, wrap_x && w > 1);
-        Workspace &ws = ctx->ws;
         jacobi_solve(ctx, h_dev, kd_dev, dx, dt, wrap_x, stats);
-        HIPCHK(hipMemcpyAsync(h_dev, ws.x, n * sizeof(double), hipMemcpyDeviceToDevice,
-                              ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
+        copy_answer(ctx, h_dev);
     });
 }
 
@@ -1051,13 +1077,13 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
             case DTD_KERNEL_PRE:  // r64, A p, record in; r64, z and the coarse r out
                 if (L.w % 2 || !ws.r2) throw Error{DTD_ESTATE, "the finest width is odd"};
                 launch_pre<float, 2, kRUpdate>(L, C, L.z[0], ws.st, s,
-                                               R64{ws.r, ws.ap, ws.r2, 0.0}, ws.parts);
+                                               R64{ws.r, ws.ap, ws.r2, 0.0}, ws.parts, ws.nparts);
                 b = (L.coefh ? 36.0 : 40.0) * n + 4.0 * C.n;  // bf16 records: 8 B, not 12
                 break;
             case DTD_KERNEL_POST:  // z, r64, record, coarse z in; z out
                 if (L.w % 2) throw Error{DTD_ESTATE, "the finest width is odd"};
-                launch_post<float, 2, kR64>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts, ws.st,
-                                            s);
+                launch_post<float, 2, kR64>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts,
+                                            ws.nparts, ws.st, s);
                 b = (L.coefh ? 24.0 : 28.0) * n + 4.0 * C.n;
                 break;
             case DTD_KERNEL_BANDCOPY: {  // r, record in; z out, the pre-smoother's walk
@@ -1175,12 +1201,12 @@ int dtd_cycle_precondition(dtd_ctx *ctx, const float *r32, const double *r64, fl
         int nbz;
         if (ws.prec == DTD_CYCLE_F64) {
             // An f64 cycle reads the f64 residual; r32 is its value rounded, kept for the ABI.
-            nbz = vcycle<double>(ws.ld, 0, (int)ctx->opts.nu, ws.r, ws.parts, nullptr, s, kRLevel,
-                                 R64{}, ws.d_ld);
+            nbz = vcycle<double>(ws.ld, 0, (int)ctx->opts.nu, ws.r, ws.parts, ws.nparts, nullptr,
+                                 s, kRLevel, R64{}, ws.d_ld);
         } else {
             upload(ctx, ws.lf[0].r, r32, n * sizeof(float));
-            nbz = vcycle<float>(ws.lf, 0, (int)ctx->opts.nu, ws.r, ws.parts, nullptr, s, kRLevel,
-                                R64{}, ws.d_lf);
+            nbz = vcycle<float>(ws.lf, 0, (int)ctx->opts.nu, ws.r, ws.parts, ws.nparts, nullptr,
+                                s, kRLevel, R64{}, ws.d_lf);
         }
         k_sum1<<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.pair);
         LAUNCH_CHECK();
@@ -1394,6 +1420,7 @@ int dtd_nn_guess(dtd_ctx *ctx, size_t np, const double *base, const float *out, 
     return guard(ctx, [&] {
         begin(ctx);
         if (!base || !out || !g) throw Error{DTD_EINVAL, "null array"};
+        if (outc < 1) throw Error{DTD_EINVAL, "outc must be at least 1"};
         if (np == 0) return;
         nn::k_nn_guess<<<nn::blocks((long)np), 256, 0, ctx->stream>>>((long)np, base, out, outc,
                                                                       scale, g);
@@ -1475,10 +1502,15 @@ int dtd_solve_rel_guess_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int 
                        &rel, &bnorm);
         }
         st->rel_residual = rel;
-        HIPCHK(hipMemcpyAsync(h_dev, ws.x, n * sizeof(double), hipMemcpyDeviceToDevice,
-                              ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
+        copy_answer(ctx, h_dev);
     });
+}
+
+int dtd_inject_fault(dtd_ctx *ctx, long after, long *calls) {
+    if (!ctx) return DTD_EINVAL;
+    ctx->fault.after = after < 0 ? -1 : after;
+    if (calls) *calls = ctx->fault.calls;
+    return DTD_OK;
 }
 
 }  // extern "C"

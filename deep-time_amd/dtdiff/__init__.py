@@ -47,6 +47,7 @@ SYMBOLS = (
     "dtd_nn_bwd_weight_scratch", "dtd_nn_conv3x3_bwd_weight", "dtd_nn_relu_bwd", "dtd_nn_pool",
     "dtd_nn_up_cat", "dtd_nn_up_cat_bwd", "dtd_nn_input", "dtd_nn_loss", "dtd_nn_guess",
     "dtd_nn_adam", "dtd_nn_to_bf16", "dtd_sync", "dtd_solve_rel_guess_device", "dtd_lod_device",
+    "dtd_inject_fault",
 )
 # The stage's slope failure constants (crates/deeptime-core/src/surface/marine.rs:231-232).
 FAILURE_RATE = 0.05
@@ -210,6 +211,8 @@ def load(path: str = LIB_PATH):
     L.dtd_solve_rel_guess_device.argtypes = [_vp, _sz, _sz, ctypes.c_double, ctypes.c_int, _vp,
                                              _vp, ctypes.c_double, ctypes.c_double, _vp,
                                              ctypes.POINTER(Stats)]
+    L.dtd_inject_fault.argtypes = [_vp, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
+    L.dtd_inject_fault.restype = ctypes.c_int
     for name in ("dtd_solve", "dtd_solve_device", "dtd_solve_rel_device", "dtd_operator",
                  "dtd_apply", "dtd_lod", "dtd_cycle_new", "dtd_cycle_level",
                  "dtd_cycle_precondition", "dtd_time_kernel"):
@@ -490,6 +493,14 @@ class Cycle:
         self.ws.check(load().dtd_cycle_precondition(self.ws._ctx, r32p, r64p,
                                                     z.ctypes.data_as(_fp), ctypes.byref(rz)))
         return z, rz.value
+
+
+def inject_fault(ws: Workspace, after: int) -> int:
+    """Arms the simulated device error of dtd_inject_fault (after < 0 disarms it); returns the
+    number of checked HIP calls the context has made so far."""
+    calls = ctypes.c_long()
+    ws.check(load().dtd_inject_fault(ws._ctx, after, ctypes.byref(calls)))
+    return calls.value
 
 
 def time_kernel(ws: Workspace, kernel: int, reps: int = 20):

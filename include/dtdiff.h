@@ -258,6 +258,17 @@ int dtd_solve_batch(dtd_ctx *ctx, size_t n, const dtd_grid *grids, double rel_to
  * overwritten: solve again before reading results. */
 int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *bytes);
 
+/* Test hook for the failure seam of the reference's device cycle (`Pre`, multigrid.rs:385-472,
+ * whose recovery drops the device state and builds it again, multigrid.rs:443-455).  Arms a
+ * simulated device error: of the checked HIP calls the context makes from now on, the one after
+ * `after` others is not made, and the entry point that reaches it returns DTD_EDEVICE with a
+ * message naming that call ("... injected failure"), as if HIP had failed there.  Nothing runs on
+ * the device that would not run otherwise.  As after any device error, the caller's output arrays
+ * keep their values, the context's buffers, levels and graphs are dropped, and the next call
+ * builds them again.  after < 0 disarms the hook.  *calls (optional) gets the number of checked
+ * HIP calls the context has made so far, so a test can step the failure through every call. */
+int dtd_inject_fault(dtd_ctx *ctx, long after, long *calls);
+
 /* ---- The learned first guess ---------------------------------------------------------------
  * No reference counterpart: the reference starts conjugate gradients from its splitting
  * (diffusion.rs:110-114).  These entry points run a small convolutional network that predicts h1

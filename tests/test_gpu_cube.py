@@ -114,6 +114,11 @@ def test_cube_arguments(ws):
         assert same_bits(x, h)
         with pytest.raises(dtdiff.DtdError):
             cube.solve(x, kd, 1.0, 7, 1e-6)
+        # The multigrid tolerance must be finite and positive, as for the planar solve.
+        for bad_tol in (0.0, -1e-6, float("inf"), float("nan")):
+            with pytest.raises(dtdiff.DtdError, match="tol_m"):
+                cube.solve(x, kd, 1.0, dtdiff.DiffusionSolver.MULTIGRID, bad_tol)
+        assert same_bits(x, h)
     finally:
         cube.close()
 

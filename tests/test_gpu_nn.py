@@ -385,3 +385,7 @@ def test_network_arguments_are_checked(ws):
     k = DeviceArray.from_numpy(np.ones(d.w * d.h))
     with pytest.raises(dtdiff.DtdError):
         dtdiff.solve_rel_guess_device(ws, d, h, k, 10.0, 1e-10, 0)
+    # The guess's output needs at least one channel.
+    g = DeviceArray(64, np.float64)
+    assert "outc" in err(L.dtd_nn_guess(ws._ctx, 64, p(g), p(f), 0, 1.0, p(g)))
+    assert "outc" in err(L.dtd_nn_guess(ws._ctx, 64, p(g), p(f), -3, 1.0, p(g)))

@@ -249,3 +249,59 @@ def test_unfused_cycle_with_three_sweeps_bit_for_bit(w, h, wrap):
     z, rz = cy.precondition(r.astype(np.float32), r)
     zo, rzo = hy.precondition(r.astype(np.float32), r)
     assert same_bits(z, zo) and abs(rz - rzo) <= 1e-12 * abs(rzo)
+
+
+@pytest.mark.parametrize("w,h", [(1, 1024), (4, 2048), (8, 4096), (1, 65536), (13, 3000)])
+@pytest.mark.parametrize("wrap", [True, False])
+def test_tall_narrow_grids_match_the_oracle(ws, w, h, wrap):
+    """A grid a few cells wide launches many more band blocks (60 columns by 4 rows a wave) than
+    256-cell blocks; each block leaves a partial sum, and the buffer is sized for the larger
+    count.  The solve still meets its tolerance and lies within the bound of the oracle's."""
+    print(f"narrow {w}x{h} wrap={wrap}", flush=True)
+    b, kd = case(w, h, 9)
+    d = dtdiff.Dims(w, h, 2400.0, wrap)
+    for dt, tol in [(2.0e6, 1.0e-6), (2.0e4, 1.0e-6)]:
+        x = b.copy()
+        its = dtdiff.solve(ws, d, x, kd, dt, dtdiff.DiffusionSolver.MULTIGRID, tol)
+        xo, its_o = O.solve(w, h, 2400.0, wrap, b, kd, dt, tol)
+        lim = O.bound(w, h, 2400.0, wrap, kd, dt, tol)
+        assert its < 400 and its <= its_o + 2, (dt, its, its_o)
+        assert rms(x, xo) <= 2.0 * lim
+        assert abs(b.sum() - x.sum()) <= 1e-12 * max(np.abs(b).sum(), 1.0)
+    # From h0 itself as the first guess (the splitting is exact on one-column grids), so the
+    # conjugate gradients iterate through every streamed pass.
+    wsr = dtdiff.Workspace(guess=dtdiff.GUESS_RHS)
+    try:
+        x = b.copy()
+        its = dtdiff.solve(wsr, d, x, kd, 2.0e6, dtdiff.DiffusionSolver.MULTIGRID, 1e-9)
+        exact = O.reference_cg(w, h, 2400.0, wrap, b, kd, 2.0e6)
+        assert 1 <= its < 400
+        assert rms(x, exact) <= O.bound(w, h, 2400.0, wrap, kd, 2.0e6, 1e-9)
+    finally:
+        wsr.close()
+
+
+def test_a_cycle_is_refused_once_the_workspace_moved_on(ws):
+    """dtd_cycle_precondition works on the levels dtd_cycle_new built.  A solve of another shape
+    (new buffers) or of the same shape (the levels then hold the solve's operator) ends that
+    cycle: precondition is refused with DTD_ESTATE instead of reading or writing arrays of the
+    wrong size or using the wrong operator."""
+    for other in [(200, 96), (64, 40)]:
+        w, h = 64, 40
+        b, kd = case(w, h, 2)
+        e, s, g = O.operator(w, h, 2400.0, True, kd, 2e6)
+        cy = dtdiff.Cycle(ws, dtdiff.Dims(w, h, 2400.0, True), e, s, g)
+        r = F.splitmix_f64(3, w * h) - 0.5
+        z, _ = cy.precondition(r.astype(np.float32), r)
+        bo, kdo = case(*other, 4)
+        x = bo.copy()
+        dtdiff.solve(ws, dtdiff.Dims(*other, 2400.0, True), x, kdo, 2e6,
+                     dtdiff.DiffusionSolver.MULTIGRID, 1e-6)
+        assert cy.levels() == 0
+        with pytest.raises(dtdiff.DtdError) as err:
+            cy.precondition(r.astype(np.float32), r)
+        assert err.value.code == dtdiff.DTD_ESTATE
+        # Building the cycle again restores it, with the same bits as before.
+        cy = dtdiff.Cycle(ws, dtdiff.Dims(w, h, 2400.0, True), e, s, g)
+        z2, _ = cy.precondition(r.astype(np.float32), r)
+        assert same_bits(z, z2)
