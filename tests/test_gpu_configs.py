@@ -1,0 +1,144 @@
+"""Every benchmark configuration at its full size on the GPU, through the C ABI.
+
+- constant 512x256 (dt K / dx^2 = 10);
+- log-normal 8192x4096 (sigma 8 cells, dt mean(K) / dx^2 = 50) with the f32 cycle, the plain f64
+  cycle and the learned first guess;
+- constant 16384x8192 with the f32 cycle and the cycle whose two finest levels keep bf16 records;
+- the 512 regional grids (64-1024 x 64-512 cells, log-normal K, ratios 5-200) in one batched solve.
+
+Each solve runs to a true FP64 relative residual ||h0 - A h1|| / ||h0|| <= 1e-10, recomputed
+here on the host from the answer, and is compared with the oracle's solve of the same inputs to
+the same target.  The f32-cycle solves follow the oracle's algorithm and must match its answer to
+a relative L2 of 1e-10.  The other cycles precondition differently; both answers then lie within
+||r|| <= 1e-10 ||h0|| of the exact solution (the operator's eigenvalues are at least 1), so they
+must agree to 2e-10 ||h0||, and the relative difference is printed.
+"""
+import numpy as np
+import pytest
+
+import dtdiff
+import oracle as O
+from dtdiff import fields as F
+from dtdiff.hipmem import DeviceArray, synchronize
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+def oracle_solve(d, h0, kd, dt):
+    tol = REL * float(np.linalg.norm(h0)) / np.sqrt(h0.size)
+    return O.solve(d.w, d.h, d.dx, d.wrap_x, h0, kd, dt, tol)
+
+
+def gpu_solve(d, h0, kd, dt, **opts):
+    ws = dtdiff.Workspace(**opts)
+    try:
+        hd, kdd = DeviceArray.from_numpy(h0), DeviceArray.from_numpy(kd)
+        st = dtdiff.solve_rel_device(ws, d, hd, kdd, dt, REL)
+        return hd.numpy(), st
+    finally:
+        ws.close()
+
+
+def check(name, d, h0, kd, dt, x, xo, same_algorithm):
+    true_rel = O.rel_residual(d.w, d.h, d.dx, d.wrap_x, kd, dt, h0, x)
+    diff = float(np.linalg.norm(x - xo))
+    rel_l2 = diff / float(np.linalg.norm(xo))
+    print(f"{name}: true rel residual {true_rel:.3e}, rel L2 vs oracle {rel_l2:.3e}", flush=True)
+    assert true_rel <= REL, name
+    assert diff <= 2 * REL * float(np.linalg.norm(h0)), name
+    if same_algorithm:
+        assert rel_l2 <= 1e-10, name
+    assert abs(x.sum() - h0.sum()) <= 1e-9 * np.abs(h0).sum(), name  # the put-back's mass
+    return true_rel, rel_l2
+
+
+@pytest.fixture(scope="module")
+def lognormal():
+    d, h0, kd, dt = F.case_lognormal()  # 8192 x 4096
+    xo, its_o = oracle_solve(d, h0, kd, dt)
+    return d, h0, kd, dt, xo, its_o
+
+
+def test_constant_512x256():
+    d, h0, kd, dt = F.case_constant()
+    xo, its_o = oracle_solve(d, h0, kd, dt)
+    x, st = gpu_solve(d, h0, kd, dt)
+    check("constant 512x256", d, h0, kd, dt, x, xo, True)
+    assert st.iterations <= its_o + 2
+
+
+def test_lognormal_8192x4096_f32_cycle(lognormal):
+    d, h0, kd, dt, xo, its_o = lognormal
+    x, st = gpu_solve(d, h0, kd, dt)
+    check("lognormal 8192x4096 f32 cycle", d, h0, kd, dt, x, xo, True)
+    assert st.iterations <= its_o + 2 and st.restarts == 0
+
+
+def test_lognormal_8192x4096_plain_f64_cycle(lognormal):
+    d, h0, kd, dt, xo, its_o = lognormal
+    x, st = gpu_solve(d, h0, kd, dt, cycle_precision=dtdiff.CYCLE_F64)
+    check("lognormal 8192x4096 f64 cycle", d, h0, kd, dt, x, xo, False)
+    assert st.iterations <= its_o + 2
+
+
+def test_lognormal_8192x4096_learned_guess(lognormal):
+    """The committed network's guess at 8192x4096 (trained at 1024x512): the same true-residual
+    stop, in no more iterations than the solve from the splitting."""
+    from dtdiff import learned as LN
+
+    d, h0, kd, dt, xo, its_o = lognormal
+    ws = dtdiff.Workspace()
+    try:
+        hd, kdd, gd = (DeviceArray.from_numpy(h0), DeviceArray.from_numpy(kd),
+                       DeviceArray(d.cells()))
+        plain = dtdiff.solve_rel_device(ws, d, hd, kdd, dt, REL)
+        hd.copy_from(DeviceArray.from_numpy(h0))
+        net = LN.UNet(ws, LN.load_params())
+        st, _ = LN.solve_with_guess(ws, net, d, hd, kdd, dt, REL, gd)
+        synchronize()
+        x = hd.numpy()
+    finally:
+        ws.close()
+    check("lognormal 8192x4096 learned guess", d, h0, kd, dt, x, xo, False)
+    print(f"learned guess: {st.iterations} iterations, splitting {plain.iterations}", flush=True)
+    assert st.iterations <= plain.iterations
+
+
+def test_constant_16384x8192_f32_and_bf16_cycles():
+    d, h0, kd, dt = F.WORKLOADS["constant-16384x8192"][0](**F.WORKLOADS["constant-16384x8192"][1])
+    assert (d.w, d.h) == (16384, 8192)
+    xo, its_o = oracle_solve(d, h0, kd, dt)
+    x, st = gpu_solve(d, h0, kd, dt)
+    check("constant 16384x8192 f32 cycle", d, h0, kd, dt, x, xo, True)
+    assert st.iterations <= its_o + 2
+    del x
+    x, st = gpu_solve(d, h0, kd, dt, cycle_precision=dtdiff.CYCLE_BF16)
+    check("constant 16384x8192 bf16 records", d, h0, kd, dt, x, xo, False)
+    assert st.iterations <= its_o + 3
+
+
+def test_regional_512_grids_in_one_batch():
+    """Every grid of the batch to the true target, each against the oracle's solve of it."""
+    grids = F.WORKLOADS["regional-512"][0](**F.WORKLOADS["regional-512"][1])
+    assert len(grids) == 512
+    ws = dtdiff.Workspace()
+    try:
+        dev = [(DeviceArray.from_numpy(b), DeviceArray.from_numpy(kd)) for (_, b, kd, _) in grids]
+        items = [(d, hd, kdd, dt, 0.0) for (d, _, _, dt), (hd, kdd) in zip(grids, dev)]
+        its, rels, st = dtdiff.solve_batch(ws, items, REL)
+        synchronize()
+        xs = [hd.numpy() for hd, _ in dev]
+    finally:
+        ws.close()
+    print(f"regional-512: {st.cells} cells, iterations <= {st.iterations_max}, "
+          f"restarts {st.restarts}, worst reported rel residual {st.rel_residual_max:.3e}",
+          flush=True)
+    worst_rel, worst_l2 = 0.0, 0.0
+    for (d, b, kd, dt), x, k in zip(grids, xs, its):
+        xo, its_o = oracle_solve(d, b, kd, dt)
+        r, l2 = check(f"grid {d.w}x{d.h}", d, b, kd, dt, x, xo, False)
+        worst_rel, worst_l2 = max(worst_rel, r), max(worst_l2, l2)
+    print(f"regional-512: worst true rel residual {worst_rel:.3e}, worst rel L2 vs oracle "
+          f"{worst_l2:.3e}", flush=True)
