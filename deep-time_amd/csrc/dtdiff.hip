@@ -21,6 +21,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -38,55 +40,47 @@ void free_ws(Workspace &ws) {
     if (ws.jdiag) (void)hipFree(ws.jdiag);
     if (ws.jrows) (void)hipFree(ws.jrows);
     if (ws.jst) (void)hipFree(ws.jst);
-    double *bufs[] = {ws.b, ws.x, ws.r, ws.r2, ws.p0, ws.p1, ws.ap, ws.kd, ws.parts, ws.pair};
-    for (double *p : bufs)
-        if (p) (void)hipFree(p);
-    if (ws.es) (void)hipFree(ws.es);
-    if (ws.st) (void)hipFree(ws.st);
     for (void *p : ws.owned) (void)hipFree(p);
     ws = Workspace();
 }
 
-// Sizes the workspace for a w x h grid and the cycle's precision; the levels' shapes are
-// allocated here and filled by build_operator.
-static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
-    Workspace &ws = ctx->ws;
-    const int prec = ctx->opts.cycle_precision;
-    if (ws.w == w && ws.h == h && ws.wrap == wrap && ws.prec == prec) return;
-    free_ws(ws);
-    ctx->cycle_levels = 0;  // a cycle built by dtd_cycle_new belonged to the old shape
+// Lays out the workspace of a w x h grid for the cycle's precision: the f64 vectors, the partial
+// sums and the state, and each level of the cycle (filled by build_operator).  take(bytes) hands
+// out device memory (hipMalloc in fit; a carve of the fixed scratch array in the
+// allocation-free entry point).  lean: without b, kd and the f64 operator record, which only the
+// host-array entry points, the Jacobi solve and the single-piece entry points use.
+template <class Take>
+static void lay_out(Workspace &ws, const dtd_opts &o, int w, int h, int wrap, bool lean,
+                    Take take) {
+    const int prec = o.cycle_precision;
     const long n = (long)w * h;
     ws.w = w;
     ws.h = h;
     ws.wrap = wrap;
     ws.prec = prec;
     ws.n = n;
-    ws.b = dalloc<double>(n);
-    ws.x = dalloc<double>(n);
-    ws.r = dalloc<double>(n);
-    if (prec != DTD_CYCLE_F64) ws.r2 = dalloc<double>(n);
-    ws.p0 = dalloc<double>(n);
-    ws.p1 = dalloc<double>(n);
-    ws.ap = dalloc<double>(n);
-    ws.kd = dalloc<double>(n);
+    auto f64 = [&](size_t cnt) { return (double *)take((cnt ? cnt : 1) * sizeof(double)); };
+    if (!lean) ws.b = f64(n);
+    ws.x = f64(n);
+    ws.r = f64(n);
+    if (prec != DTD_CYCLE_F64) ws.r2 = f64(n);
+    ws.p0 = f64(n);
+    ws.p1 = f64(n);
+    ws.ap = f64(n);
+    if (!lean) ws.kd = f64(n);
     ws.nparts = parts_cap(w, h);
-    ws.parts = dalloc<double>(2 * (size_t)ws.nparts + 2);
-    ws.pair = dalloc<double>(2);
-    ws.st = dalloc<CgState>(1);
-    ws.es = dalloc<double2>(n);
+    ws.parts = f64(2 * (size_t)ws.nparts + 2);
+    ws.pair = f64(2);
+    ws.st = (CgState *)take(sizeof(CgState));
+    if (!lean) ws.es = (double2 *)take((n ? n : 1) * sizeof(double2));
     auto sh = shapes(w, h, wrap);
-    auto own = [&](size_t cnt, size_t elt) {
-        void *p = nullptr;
-        HIPCHK(hipMalloc(&p, (cnt ? cnt : 1) * elt));
-        ws.owned.push_back(p);
-        return p;
-    };
+    auto own = [&](size_t cnt, size_t elt) { return take((cnt ? cnt : 1) * elt); };
     for (size_t l = 0; l < sh.size(); l++) {
         const long nl = (long)sh[l].w * sh[l].h;
         if (prec == DTD_CYCLE_F64) {
             Lvl<double> L{};
             L.w = sh[l].w, L.h = sh[l].h, L.wrap = sh[l].wrap, L.n = (int)nl;
-            L.omega = (double)ctx->opts.omega;
+            L.omega = (double)o.omega;
             L.coef = (double *)own(kRec * nl, 8);
             L.mass = l == 0 ? nullptr : (double *)own(nl, 8);
             L.r = l == 0 ? ws.r : (double *)own(nl, 8);  // the f64 cycle reads the CG residual
@@ -96,7 +90,7 @@ static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
         } else {
             Lvl<float> L{};
             L.w = sh[l].w, L.h = sh[l].h, L.wrap = sh[l].wrap, L.n = (int)nl;
-            L.omega = ctx->opts.omega;
+            L.omega = o.omega;
             L.coef = (float *)own(kRec * nl, 4);
             L.mass = l == 0 ? nullptr : (float *)own(nl, 4);
             L.r = (float *)own(nl, 4);
@@ -108,19 +102,100 @@ static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
             ws.lf.push_back(L);
         }
     }
-    if (!ws.lf.empty()) {
-        ws.d_lf = (Lvl<float> *)own(ws.lf.size() * sizeof(Lvl<float>), 1);
-        HIPCHK(hipMemcpy(ws.d_lf, ws.lf.data(), ws.lf.size() * sizeof(Lvl<float>),
-                         hipMemcpyHostToDevice));
-    }
-    if (!ws.ld.empty()) {
-        ws.d_ld = (Lvl<double> *)own(ws.ld.size() * sizeof(Lvl<double>), 1);
-        HIPCHK(hipMemcpy(ws.d_ld, ws.ld.data(), ws.ld.size() * sizeof(Lvl<double>),
-                         hipMemcpyHostToDevice));
-    }
+    if (!ws.lf.empty()) ws.d_lf = (Lvl<float> *)own(ws.lf.size() * sizeof(Lvl<float>), 1);
+    if (!ws.ld.empty()) ws.d_ld = (Lvl<double> *)own(ws.ld.size() * sizeof(Lvl<double>), 1);
+}
+
+// The level table the one-block bottom of the cycle reads, copied to the device on stream s (the
+// host copy lives in ws, as long as the device one).
+static void upload_levels(Workspace &ws, hipStream_t s) {
+    if (ws.d_lf)
+        HIPCHK(hipMemcpyAsync(ws.d_lf, ws.lf.data(), ws.lf.size() * sizeof(Lvl<float>),
+                              hipMemcpyHostToDevice, s));
+    if (ws.d_ld)
+        HIPCHK(hipMemcpyAsync(ws.d_ld, ws.ld.data(), ws.ld.size() * sizeof(Lvl<double>),
+                              hipMemcpyHostToDevice, s));
+}
+
+// Sizes the context's workspace for a w x h grid and the cycle's precision (kept while the shape
+// stays).
+static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
+    Workspace &ws = ctx->ws;
+    if (ws.w == w && ws.h == h && ws.wrap == wrap && ws.prec == ctx->opts.cycle_precision) return;
+    free_ws(ws);
+    ctx->cycle_levels = 0;  // a cycle built by dtd_cycle_new belonged to the old shape
+    auto take = [&](size_t bytes) {
+        void *p = nullptr;
+        HIPCHK(hipMalloc(&p, bytes));
+        ws.owned.push_back(p);
+        return p;
+    };
+    lay_out(ws, ctx->opts, w, h, wrap, false, take);
+    upload_levels(ws, ctx->stream);
 }
 
 static Grid grid_of(const Workspace &ws) { return Grid{ws.w, ws.h, ws.wrap, (int)ws.n}; }
+
+// ---- device memory for callers that cannot allocate (no reference counterpart) ---------------
+//
+// dtd_solve_device_noalloc lays its workspace out in one fixed array per GPU instead of
+// hipMalloc'ed buffers.  The array exists on every GPU the library is loaded on, from then until
+// the process ends.  In host code its name stands for no device address: the address on each
+// device comes from hipGetSymbolAddress, looked up once per device by dtd_create (with that
+// device current) and kept in scratch_base.  Every call on one GPU shares that GPU's array.
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex scratch_mu;
+static void *scratch_base[kMaxDevices];
+
+// Looks up the array's address on the current device `dev` (once).
+static void scratch_init(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw Error{DTD_EINVAL, "device ordinal out of range"};
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (scratch_base[dev]) return;
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error{DTD_EDEVICE, "the scratch array has no device address", true};
+    scratch_base[dev] = p;
+}
+
+static void *scratch_of(int dev) {
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    return dev >= 0 && dev < kMaxDevices ? scratch_base[dev] : nullptr;
+}
+
+// Carves consecutive 256-byte aligned pieces from [base, base + cap); with base null it only
+// counts the bytes (the size check before any device work).
+struct Carver {
+    unsigned char *base;
+    size_t cap, used = 0;
+    void *operator()(size_t bytes) {
+        const size_t off = (used + 255) & ~(size_t)255;
+        used = off + bytes;
+        if (base && used > cap)
+            throw Error{DTD_ETOOBIG, "the grid's workspace exceeds the fixed scratch array"};
+        return base ? base + off : nullptr;
+    }
+};
+
+// Puts a scratch-backed workspace in the context's place for one call (direct launches, no
+// graphs) and gives the context its own workspace back afterwards, also when the call throws.
+struct ScratchSwap {
+    dtd_ctx *ctx;
+    Workspace saved;
+    bool graph;
+    ScratchSwap(dtd_ctx *c, Workspace &&sw)
+        : ctx(c), saved(std::move(c->ws)), graph(c->use_graph) {
+        c->ws = std::move(sw);
+        c->use_graph = false;
+    }
+    ~ScratchSwap() {
+        ctx->ws = std::move(saved);
+        ctx->use_graph = graph;
+    }
+};
 
 __global__ void k_sum1(const double *__restrict__ parts, int nparts, double *out) {
     __shared__ double lds[kReduceBlock / 64];
@@ -687,6 +762,7 @@ int dtd_create(const dtd_opts *opts, dtd_ctx **out) {
     ctx->use_graph = !(ng && *ng && *ng != '0');
     int rc = guard(ctx, [&] {
         HIPCHK(hipSetDevice(ctx->opts.device));
+        scratch_init(ctx->opts.device);
         HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         for (auto &e : ctx->ev) HIPCHK(hipEventCreate(&e));
         for (auto &e : ctx->flag_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -774,6 +850,42 @@ int dtd_solve_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, do
         if (n == 0) return;
         if (!h_dev || !kd_dev) throw Error{DTD_EINVAL, "null array"};
         fit(ctx, (int)w, (int)h, wrap_x && w > 1);
+        double rel = 0.0;
+        solve_core(ctx, h_dev, kd_dev, dx, dt, wrap_x, tol_m, 0.0, true, kGuessLod, stats, &rel,
+                   nullptr);
+        if (stats) stats->rel_residual = rel;
+        copy_answer(ctx, h_dev);
+    });
+}
+
+int dtd_solve_device_noalloc(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x,
+                             double *h_dev, const double *kd_dev, double dt, double tol_m,
+                             dtd_stats *stats) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (stats) *stats = dtd_stats{};
+        if (!(std::isfinite(tol_m) && tol_m > 0.0))
+            throw Error{DTD_EINVAL, "tol_m must be finite and positive"};
+        check_grid(w, h);
+        const size_t n = w * h;
+        if (n == 0) return;
+        if (!h_dev || !kd_dev) throw Error{DTD_EINVAL, "null array"};
+        const int wrap = wrap_x && w > 1;
+        // The size check first, by a carve that only counts.
+        Workspace probe;
+        Carver count{nullptr, kScratchBytes};
+        lay_out(probe, ctx->opts, (int)w, (int)h, wrap, true, std::ref(count));
+        if (count.used > kScratchBytes)
+            throw Error{DTD_ETOOBIG, "the grid's workspace (" + std::to_string(count.used) +
+                                         " bytes) exceeds the fixed scratch array (" +
+                                         std::to_string(kScratchBytes) + " bytes)"};
+        unsigned char *base = (unsigned char *)scratch_of(ctx->opts.device);
+        if (!base) throw Error{DTD_ESTATE, "the scratch array was not set up for this device"};
+        Workspace sw;
+        Carver carve{base, kScratchBytes};
+        lay_out(sw, ctx->opts, (int)w, (int)h, wrap, true, std::ref(carve));
+        ScratchSwap swap(ctx, std::move(sw));
+        upload_levels(ctx->ws, ctx->stream);
         double rel = 0.0;
         solve_core(ctx, h_dev, kd_dev, dx, dt, wrap_x, tol_m, 0.0, true, kGuessLod, stats, &rel,
                    nullptr);

@@ -47,7 +47,7 @@ SYMBOLS = (
     "dtd_nn_bwd_weight_scratch", "dtd_nn_conv3x3_bwd_weight", "dtd_nn_relu_bwd", "dtd_nn_pool",
     "dtd_nn_up_cat", "dtd_nn_up_cat_bwd", "dtd_nn_input", "dtd_nn_loss", "dtd_nn_guess",
     "dtd_nn_adam", "dtd_nn_to_bf16", "dtd_sync", "dtd_solve_rel_guess_device", "dtd_lod_device",
-    "dtd_inject_fault",
+    "dtd_inject_fault", "dtd_solve_device_noalloc",
 )
 # The stage's slope failure constants (crates/deeptime-core/src/surface/marine.rs:231-232).
 FAILURE_RATE = 0.05
@@ -211,6 +211,10 @@ def load(path: str = LIB_PATH):
     L.dtd_solve_rel_guess_device.argtypes = [_vp, _sz, _sz, ctypes.c_double, ctypes.c_int, _vp,
                                              _vp, ctypes.c_double, ctypes.c_double, _vp,
                                              ctypes.POINTER(Stats)]
+    L.dtd_solve_device_noalloc.argtypes = [_vp, _sz, _sz, ctypes.c_double, ctypes.c_int, _vp,
+                                           _vp, ctypes.c_double, ctypes.c_double,
+                                           ctypes.POINTER(Stats)]
+    L.dtd_solve_device_noalloc.restype = ctypes.c_int
     L.dtd_inject_fault.argtypes = [_vp, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
     L.dtd_inject_fault.restype = ctypes.c_int
     for name in ("dtd_solve", "dtd_solve_device", "dtd_solve_rel_device", "dtd_operator",
@@ -295,6 +299,14 @@ def solve_device(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float, tol_m: float)
     st = Stats()
     ws.check(load().dtd_solve_device(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x), _ptr(h_dev),
                                      _ptr(kd_dev), dt, tol_m, ctypes.byref(st)))
+    return st
+
+
+def solve_device_noalloc(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float, tol_m: float) -> Stats:
+    """solve_device with the workspace in the library's fixed scratch array (no allocation)."""
+    st = Stats()
+    ws.check(load().dtd_solve_device_noalloc(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x), _ptr(h_dev),
+                                             _ptr(kd_dev), dt, tol_m, ctypes.byref(st)))
     return st
 
 

@@ -115,6 +115,20 @@ int dtd_step_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, dou
 int dtd_solve_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h_dev,
                      const double *kd_dev, double dt, double tol_m, dtd_stats *stats);
 
+/* dtd_solve_device for callers that cannot allocate device memory (code that must not call
+ * hipMalloc: tight serving loops, harnesses that forbid it).  No counterpart in the reference.
+ * Same arguments, kernels and results (bit for bit); the caller still passes h_dev and kd_dev.
+ * The workspace (f64 vectors, partial sums, the cycle's levels) is laid out in one fixed 64 MiB
+ * device array per GPU, which the library holds from its load until the process ends, instead of
+ * the context's buffers; nothing is allocated or freed, and the iterations are launched directly
+ * (no graphs).  A grid whose workspace exceeds 64 MiB (about 800 000 cells with the f32 cycle,
+ * e.g. 1024 x 768) is refused with DTD_ETOOBIG and a message giving both sizes.  Every call on one
+ * GPU, from any context or thread, shares that GPU's array and overwrites what the last call left
+ * there: calls that overlap in time on one GPU race and must not be made. */
+int dtd_solve_device_noalloc(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x,
+                             double *h_dev, const double *kd_dev, double dt, double tol_m,
+                             dtd_stats *stats);
+
 /* The solve to a true relative residual: ||h0 - A h1||_2 / ||h0||_2 <= rel_tol in f64, measured
  * after the mass put-back.  Runs the solve at tol_m = rel_tol * ||h0|| / sqrt(n) and, while the
  * measured residual misses the target, solves again from the last answer at a tighter tol_m
