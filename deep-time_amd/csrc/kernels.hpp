@@ -1052,6 +1052,15 @@ __device__ inline void copy(T (&d)[V], const T (&s)[V]) {
 // alternates between two buffers, so a band's halo rows are read before any update.
 enum { kRLevel = 0, kR64 = 1, kRUpdate = 2 };
 
+// Rows of loads the fused smoothers keep in flight (a divisor of 4: bands are multiples of 4 rows
+// and a walk takes the band's rows plus 4).  Measured at 8192x4096: 4 rows at 3 waves/SIMD
+// (144-157 VGPRs) beat 2 rows at 4 waves (116-125 VGPRs; 308/220 vs 288/201 us) and 1 row.
+#ifndef DTD_SMOOTH_AHEAD
+#define DTD_SMOOTH_AHEAD 4
+#endif
+constexpr int kSmoothAhead = DTD_SMOOTH_AHEAD;
+static_assert(4 % kSmoothAhead == 0, "rows in flight must divide 4");
+
 struct R64 {
     const double *in;  // f64 residual (kR64, kRUpdate)
     const double *ap;  // A p (kRUpdate)
@@ -1066,7 +1075,7 @@ struct R64 {
 template <typename T, int V, int RM = kRLevel, int CB = 0>
 __device__ __forceinline__ double spre_band(const Lvl<T> &L, T *__restrict__ zout,
                                             const Lvl<T> &C, int th, int wave, R64 rio = R64{}) {
-    constexpr int A = 4;  // rows in flight
+    constexpr int A = kSmoothAhead;  // rows in flight
     const int w = L.w, h = L.h, wrap = L.wrap;
     const Band bd = band_at<V>(w, h, wrap, th, wave);
     const int gi0 = bd.gi0;
@@ -1188,7 +1197,7 @@ __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restric
                                              const T *__restrict__ zc, int wc,
                                              T *__restrict__ zout, int th,
                                              const double *__restrict__ dot, int wave) {
-    constexpr int A = 4;
+    constexpr int A = kSmoothAhead;
     const int w = L.w, h = L.h, wrap = L.wrap;
     const Band bd = band_at<V>(w, h, wrap, th, wave);
     const int gi0 = bd.gi0;
