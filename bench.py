@@ -73,6 +73,8 @@ def main():
                     help="one solve, then only the per-kernel timings (for counter runs)")
     ap.add_argument("--cpu-reps", type=int, default=5,
                     help="timed oracle solves after one untimed one (the best is reported)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the built-in band heights (no dtd_tune before the warm-up)")
     ap.add_argument("--no-extras", action="store_true",
                     help="the default workload only: skip the other configurations' block")
     args = ap.parse_args()
@@ -121,6 +123,15 @@ def main():
     def step():
         hd.copy_from(h0d)  # synchronous device-to-device copy
         return dtdiff.solve_rel_device(ws, d, hd, kdd, dt, args.rel)
+
+    tuned = None
+    if not args.no_tune and not args.kernels_only:
+        # The launch plan for this shape, searched once (kept in the workspace, like the graphs).
+        t = time.time()
+        ms_rule, ms_tuned, plan = dtdiff.tune(ws, d, h0d, kdd, dt, reps=5)
+        tuned = {"tune_s": time.time() - t, "ms_rule": ms_rule, "ms_tuned": ms_tuned, "plan": plan}
+        log(f"tuned in {tuned['tune_s']:.1f} s: whole solve {ms_rule:.3f} ms with the built-in "
+            f"band heights, {ms_tuned:.3f} ms with the plan kept")
 
     if args.kernels_only:
         st = step()
@@ -219,6 +230,7 @@ def main():
                          "device_total": last.ms_total},
         "upload_s_pcie": upload_s,
         "kernels": kernels,
+        "tuning": tuned,
     }
     default_run = (args.workload == "lognormal-8192x4096" and not args.size
                    and args.cycle == "f32" and args.guess == "lod" and args.rel == 1e-10)

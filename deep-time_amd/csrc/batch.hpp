@@ -50,8 +50,9 @@ struct BInit {  // a solve's start for one grid
     int active, pad;
 };
 
-// Band height of a streamed kernel on an h-row level of a batch: a function of the shape alone.
-inline int batch_band(int h) { return h >= 64 ? 16 : (h >= 16 ? 8 : 4); }
+// Band height of a streamed kernel on an h-row level of a batch: a function of the shape alone,
+// by the policy the batch was planned with (the built-in one: 16, 8 or 4 rows).
+inline int batch_band(int h, const BatchPolicy &p = BatchPolicy{}) { return p.level(h); }
 
 // ---- kernels: a table maps each block to (grid, block of the grid) ---------------------------
 
@@ -281,6 +282,7 @@ struct BLaunch {  // one launch: its block table (device) and block count
 
 struct BatchWs {
     std::vector<long> key;  // (w, h, wrap) of each grid, omega and scale bits
+    BatchPolicy policy;     // the band heights the launch tables were built with
     int ngrids = 0;
     int nlevels = 0;  // the deepest grid's level count
     size_t cells = 0;
@@ -341,11 +343,12 @@ static inline size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; 
 // Builds the device buffers and launch tables for grids gs (all non-empty).
 static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
     auto key = batch_key(ctx, n, gs);
-    if (ctx->batch && ctx->batch->key == key) return;
+    if (ctx->batch && ctx->batch->key == key && ctx->batch->policy == ctx->batch_policy) return;
     free_batch(ctx);
     BatchWs *bw = new BatchWs();
     ctx->batch = bw;
     bw->key = key;
+    bw->policy = ctx->batch_policy;
     bw->ngrids = (int)n;
     auto own = [&](size_t bytes) {
         void *p = nullptr;
@@ -412,7 +415,7 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
         q = BGrid{};
         const int w = (int)gs[g].w, h = (int)gs[g].h;
         q.g = Grid{w, h, (int)(gs[g].wrap_x && w > 1), w * h};
-        q.th = batch_band(h);
+        q.th = bw->policy.pass(h);
         const size_t o = off[g];
         q.x = x + o, q.r = r + o, q.p0 = p0 + o, q.p1 = p1 + o, q.ap = ap + o;
         const Lvl<float> L0 = level(g, 0);
@@ -429,7 +432,7 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
             BLevel t{};
             t.L = level(g, l);
             t.C = level(g, l + 1);
-            t.th = batch_band(t.L.h);
+            t.th = bw->policy.level(t.L.h);
             t.dot = l == 0 ? bw->grids[g].r : nullptr;
             t.st = bw->st + g;
             lv[l].push_back(t);
@@ -685,6 +688,10 @@ static void solve_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs_in, doubl
     const size_t n = gs.size();
     if (n == 0) return;
     if (n > (size_t)INT32_MAX / 2) throw Error{DTD_ETOOBIG, "too many grids"};
+    if (!ctx->batch_tuning) {
+        auto it = ctx->batch_plans.find(batch_key(ctx, n, gs.data()));
+        ctx->batch_policy = it != ctx->batch_plans.end() ? it->second : BatchPolicy{};
+    }
     plan_batch(ctx, n, gs.data());
     BatchWs &bw = *ctx->batch;
     hipStream_t s = ctx->stream;
@@ -787,6 +794,77 @@ static void solve_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs_in, doubl
     out.ms_cg = elapsed(ctx->ev[2], ctx->ev[3]);
     out.ms_total = elapsed(ctx->ev[0], ctx->ev[4]);
     if (stats) *stats = out;
+}
+
+// dtd_tune_batch: the band-height policy of one shape list, by timing whole batched solves (the
+// built-in policy first, then one class at a time over its candidates, keeping the best so far).
+static void tune_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs, double rel_tol, int reps,
+                       double *ms_rule, double *ms_tuned) {
+    std::vector<dtd_grid> live;
+    for (size_t k = 0; k < count; k++)
+        if (gs[k].w * gs[k].h) live.push_back(gs[k]);
+    if (live.empty()) return;
+    const auto key = batch_key(ctx, live.size(), live.data());
+    // Each timed solve starts from the grids' h0 (the solve writes its answers over them), and the
+    // arrays get h0 back at the end.
+    size_t total = 0;
+    for (const dtd_grid &g : live) total += g.w * g.h;
+    double *save = nullptr;
+    HIPCHK(hipMalloc(&save, total * sizeof(double)));
+    auto copy_all = [&](bool out) {
+        size_t o = 0;
+        for (const dtd_grid &g : live) {
+            const size_t b = g.w * g.h * sizeof(double);
+            HIPCHK(out ? hipMemcpyAsync(save + o, g.h_dev, b, hipMemcpyDeviceToDevice, ctx->stream)
+                       : hipMemcpyAsync(g.h_dev, save + o, b, hipMemcpyDeviceToDevice, ctx->stream));
+            o += g.w * g.h;
+        }
+    };
+    ctx->batch_tuning = true;
+    auto time_policy = [&](const BatchPolicy &p) {
+        ctx->batch_policy = p;
+        std::vector<double> t;
+        for (int k = 0; k <= reps; k++) {
+            copy_all(false);
+            dtd_batch_stats st{};
+            solve_batch(ctx, count, gs, rel_tol, nullptr, nullptr, &st);
+            if (k) t.push_back(st.ms_total);
+        }
+        std::sort(t.begin(), t.end());
+        return t[t.size() / 2];
+    };
+    try {
+        copy_all(true);
+        BatchPolicy best{};
+        const double t_rule = time_policy(best);
+        double t_best = t_rule;
+        const int tall[] = {8, 12, 24, 32}, mid[] = {4, 12, 16}, small[] = {8}, f64[] = {8, 12, 24, 32};
+        auto sweep = [&](int BatchPolicy::*field, const int *cand, int nc) {
+            for (int c = 0; c < nc; c++) {
+                BatchPolicy p = best;
+                p.*field = cand[c];
+                const double t = time_policy(p);
+                if (t < t_best) t_best = t, best = p;
+            }
+        };
+        sweep(&BatchPolicy::tall, tall, 4);
+        sweep(&BatchPolicy::f64, f64, 4);
+        sweep(&BatchPolicy::mid, mid, 3);
+        sweep(&BatchPolicy::small, small, 1);
+        ctx->batch_plans[key] = best;
+        ctx->batch_policy = best;
+        copy_all(false);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (ms_rule) *ms_rule = t_rule;
+        if (ms_tuned) *ms_tuned = t_best;
+    } catch (...) {
+        ctx->batch_tuning = false;
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(save);
+        throw;
+    }
+    ctx->batch_tuning = false;
+    (void)hipFree(save);
 }
 
 }  // namespace dtd

@@ -7,8 +7,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -95,10 +97,34 @@ struct Workspace {
     Lvl<double> *d_ld = nullptr;
     std::vector<void *> owned;  // every allocation of the levels
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // Band heights of the finest level's f64 passes (direction, first residual, the two finish
+    // passes) chosen by the tuner, and the level size from which the one-block bottom of the
+    // cycle takes over; 0 and -1: the built-in rules.
+    int th_dir = 0, th_resid = 0, th_fin1 = 0, th_fin2 = 0, bottom = -1;
     // The Jacobi solve's extra buffers (diag, two row sums a row, its scalars) and graphs.
     double *jdiag = nullptr, *jrows = nullptr;
     void *jst = nullptr;
     hipGraphExec_t jgraph[2] = {nullptr, nullptr};
+};
+
+// A tuned launch plan for one grid shape (dtd_tune): per level the fused smoothers' band
+// heights, the finest level's f64 passes' band heights and the bottom's size.
+struct Plan {
+    std::vector<std::array<int, 2>> lvl;  // (th_pre, th_post) per level
+    int th_dir = 0, th_resid = 0, th_fin1 = 0, th_fin2 = 0, bottom = -1;
+    double ms_rule = 0.0, ms_tuned = 0.0;  // whole solves timed with the built-in rules and the plan
+};
+using PlanKey = std::array<int, 4>;  // w, h, wrap, cycle precision
+
+// The batched solve's band heights by class of level (dtd_tune_batch): levels of at least 64
+// rows, of 16 to 63 rows, of fewer; and the f64 passes of grids of at least 64 rows.
+struct BatchPolicy {
+    int tall = 16, mid = 8, small = 4, f64 = 16;
+    bool operator==(const BatchPolicy &o) const {
+        return tall == o.tall && mid == o.mid && small == o.small && f64 == o.f64;
+    }
+    int level(int h) const { return h >= 64 ? tall : (h >= 16 ? mid : small); }
+    int pass(int h) const { return h >= 64 ? f64 : (h >= 16 ? mid : small); }
 };
 
 struct BatchWs;
@@ -120,6 +146,12 @@ struct dtd_ctx {
     size_t cycle_levels = 0;  // levels built by dtd_cycle_new
     bool use_graph = true;
     dtd::FaultHook fault;         // dtd_inject_fault
+    std::map<dtd::PlanKey, dtd::Plan> plans;  // tuned plans by grid shape (dtd_tune)
+    // Tuned band-height policies of the batched solve by shape list (dtd_tune_batch), and the one
+    // a batch plan is built with.
+    std::map<std::vector<long>, dtd::BatchPolicy> batch_plans;
+    dtd::BatchPolicy batch_policy;
+    bool batch_tuning = false;  // dtd_tune_batch sets batch_policy itself
     double *stage = nullptr;      // pinned host staging of dtd_solve's answer
     size_t stage_n = 0;
     dtd::BatchWs *batch = nullptr;  // the batched solve's buffers (batch.hpp)

@@ -117,6 +117,21 @@ static void upload_levels(Workspace &ws, hipStream_t s) {
                               hipMemcpyHostToDevice, s));
 }
 
+// Puts a tuned plan's band heights and bottom size into the workspace (0 / -1 where it keeps the
+// built-in rules).
+static void apply_plan(Workspace &ws, const Plan &p) {
+    for (size_t l = 0; l < ws.lf.size(); l++) {
+        ws.lf[l].th_pre = l < p.lvl.size() ? p.lvl[l][0] : 0;
+        ws.lf[l].th_post = l < p.lvl.size() ? p.lvl[l][1] : 0;
+    }
+    for (size_t l = 0; l < ws.ld.size(); l++) {
+        ws.ld[l].th_pre = l < p.lvl.size() ? p.lvl[l][0] : 0;
+        ws.ld[l].th_post = l < p.lvl.size() ? p.lvl[l][1] : 0;
+    }
+    ws.th_dir = p.th_dir, ws.th_resid = p.th_resid, ws.th_fin1 = p.th_fin1;
+    ws.th_fin2 = p.th_fin2, ws.bottom = p.bottom;
+}
+
 // Sizes the context's workspace for a w x h grid and the cycle's precision (kept while the shape
 // stays).
 static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
@@ -131,6 +146,8 @@ static void fit(dtd_ctx *ctx, int w, int h, int wrap) {
         return p;
     };
     lay_out(ws, ctx->opts, w, h, wrap, false, take);
+    auto it = ctx->plans.find(PlanKey{w, h, wrap, ctx->opts.cycle_precision});
+    if (it != ctx->plans.end()) apply_plan(ws, it->second);
     upload_levels(ws, ctx->stream);
 }
 
@@ -276,7 +293,7 @@ static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt)
 template <typename T, int V, int RM, int CB>
 static int launch_pre_cb(Lvl<T> &L, Lvl<T> &C, T *zout, const CgState *st, hipStream_t s,
                          R64 rio, double *parts, long cap) {
-    const int th = band_height<V>((const void *)k_spre<T, V, RM, CB>, L.w, L.h);
+    const int th = L.th_pre > 0 ? L.th_pre : band_height<V>((const void *)k_spre<T, V, RM, CB>, L.w, L.h);
     const int nb = stream_grid<V, LaneGeom<V>::halo>(L.w, L.h, th);
     if (RM == kRUpdate) need_parts(nb, cap);
     k_spre<T, V, RM, CB><<<nb, kBlock, 0, s>>>(L, zout, C, th, st, rio, parts);
@@ -295,7 +312,8 @@ template <typename T, int V, int RM, int CB>
 static int launch_post_cb(Lvl<T> &L, const T *zin, const T *zc, int wc, T *zout,
                           const double *dot, double *parts, long cap, const CgState *st,
                           hipStream_t s) {
-    const int th = band_height<V>((const void *)k_spost<T, V, RM, CB>, L.w, L.h);
+    const int th =
+        L.th_post > 0 ? L.th_post : band_height<V>((const void *)k_spost<T, V, RM, CB>, L.w, L.h);
     const int nb = stream_grid<V, LaneGeom<V>::halo>(L.w, L.h, th);
     if (dot) need_parts(nb, cap);
     k_spost<T, V, RM, CB><<<nb, kBlock, 0, s>>>(L, zin, zc, wc, zout, th, dot, parts, st);
@@ -315,7 +333,7 @@ static int launch_post(Lvl<T> &L, const T *zin, const T *zc, int wc, T *zout, co
 template <typename TZ>
 static int launch_dir(Workspace &ws, const TZ *z, const double *pold, double *pnew,
                       hipStream_t s) {
-    const int th = band_height<1, 1>((const void *)k_sdir<TZ>, ws.w, ws.h);
+    const int th = ws.th_dir > 0 ? ws.th_dir : band_height<1, 1>((const void *)k_sdir<TZ>, ws.w, ws.h);
     const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
     need_parts(nb, ws.nparts);
     k_sdir<TZ><<<nb, kBlock, 0, s>>>(grid_of(ws), z, pold, pnew, ws.ap, ws.x, th, ws.parts, ws.st);
@@ -329,7 +347,8 @@ static int launch_resid(Workspace &ws, const double *kd, double dt, double s2, c
                         const double *xin, hipStream_t s, bool r32 = true) {
     const Grid g = grid_of(ws);
     if (ws.prec == DTD_CYCLE_F64) {
-        const int th = band_height<1, 1>((const void *)k_sresid<double>, ws.w, ws.h);
+        const int th = ws.th_resid > 0 ? ws.th_resid
+                                       : band_height<1, 1>((const void *)k_sresid<double>, ws.w, ws.h);
         const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
         need_parts(nb, ws.nparts);
         k_sresid<double><<<nb, kBlock, 0, s>>>(g, kd, dt, s2, b, xin, ws.x, ws.r, (double *)nullptr,
@@ -337,7 +356,8 @@ static int launch_resid(Workspace &ws, const double *kd, double dt, double s2, c
         LAUNCH_CHECK();
         return nb;
     }
-    const int th = band_height<1, 1>((const void *)k_sresid<float>, ws.w, ws.h);
+    const int th = ws.th_resid > 0 ? ws.th_resid
+                                   : band_height<1, 1>((const void *)k_sresid<float>, ws.w, ws.h);
     const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
     need_parts(nb, ws.nparts);
     k_sresid<float><<<nb, kBlock, 0, s>>>(g, kd, dt, s2, b, xin, ws.x, ws.r, r32 ? ws.lf[0].r : nullptr,
@@ -373,15 +393,17 @@ static bool fused_enabled() {
 // rm (f32 levels, fused smoothing, finest level only): where the finest level's r comes from
 // (kRLevel: its own r; kR64: the f64 residual dot; kRUpdate: rio.in - alpha A p, written to
 // rio.out = dot, with the r.r step of the conjugate gradients launched after the pre-smoothing).
-// parts holds the partial sums of at most cap blocks.
+// parts holds the partial sums of at most cap blocks.  Levels of at most `bottom` cells (< 0:
+// the built-in size) go to the one-block bottom.
 template <typename T>
 static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, double *parts,
                   long cap, CgState *st, hipStream_t s, int rm = kRLevel, R64 rio = R64{},
-                  const Lvl<T> *dlv = nullptr) {
+                  const Lvl<T> *dlv = nullptr, int bottom = -1) {
     Lvl<T> &L = lv[l];
     size_t lds = 0;
     for (size_t k = l; k < lv.size(); k++) lds += (kRec + 3) * (size_t)lv[k].n * sizeof(T);
-    if (dlv && l >= 1 && nu == 2 && fused_enabled() && L.n <= bottom_cells() && !L.coefh &&
+    if (bottom < 0) bottom = bottom_cells();
+    if (dlv && l >= 1 && nu == 2 && fused_enabled() && L.n <= bottom && !L.coefh &&
         lds <= kBottomLds && lv.size() - l <= (size_t)kBottomLevels) {
         // This level and every coarser one in one block, in LDS.
         k_bottom<T><<<1, kBottomBlock, lds, s>>>(dlv, (int)l, (int)lv.size(), st);
@@ -417,7 +439,7 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
                 } else {
                     launch_pre<T, 1, kR64>(L, C0, L.z[0], st, s, rio);
                 }
-                vcycle<T>(lv, l + 1, nu, nullptr, parts, cap, st, s, kRLevel, R64{}, dlv);
+                vcycle<T>(lv, l + 1, nu, nullptr, parts, cap, st, s, kRLevel, R64{}, dlv, bottom);
                 return wide ? launch_post<T, VW, kR64>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts,
                                                         cap, st, s)
                             : launch_post<T, 1, kR64>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts,
@@ -428,7 +450,7 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
             launch_pre<T, VW>(L, C0, L.z[0], st, s);
         else
             launch_pre<T, 1>(L, C0, L.z[0], st, s);
-        vcycle<T>(lv, l + 1, nu, nullptr, parts, cap, st, s, kRLevel, R64{}, dlv);
+        vcycle<T>(lv, l + 1, nu, nullptr, parts, cap, st, s, kRLevel, R64{}, dlv, bottom);
         return wide ? launch_post<T, VW>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts, cap, st, s)
                     : launch_post<T, 1>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts, cap, st, s);
     }
@@ -443,7 +465,7 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
     }
     k_restrict<T><<<nblocks(C0.n), kBlock, 0, s>>>(L, L.z[cur], C0, st);
     LAUNCH_CHECK();
-    vcycle<T>(lv, l + 1, nu, nullptr, parts, cap, st, s, kRLevel, R64{}, dlv);
+    vcycle<T>(lv, l + 1, nu, nullptr, parts, cap, st, s, kRLevel, R64{}, dlv, bottom);
     {
         const double *d = nu == 1 ? dot : nullptr;
         k_sweep<T><<<nb, kBlock, 0, s>>>(L, L.z[cur], L.z[cur ^ 1], C0.z[1], C0.w, d, parts, st);
@@ -466,9 +488,9 @@ static int run_vcycle(dtd_ctx *ctx, const double *dot, CgState *st, int rm = kRL
     const int nu = (int)ctx->opts.nu;
     if (ws.prec == DTD_CYCLE_F64)
         return vcycle<double>(ws.ld, 0, nu, dot, ws.parts, ws.nparts, st, ctx->stream, kRLevel,
-                              R64{}, ws.d_ld);
+                              R64{}, ws.d_ld, ws.bottom);
     return vcycle<float>(ws.lf, 0, nu, dot, ws.parts, ws.nparts, st, ctx->stream, rm, rio,
-                         ws.d_lf);
+                         ws.d_lf, ws.bottom);
 }
 
 // Whether the f32 cycle's finest level takes the f64 residual straight from the conjugate
@@ -647,7 +669,8 @@ static size_t solve_core(dtd_ctx *ctx, const double *b, const double *kd, double
     // The last update and the put-back's sums (x' into ap), the put-back and the true residual
     // (x'' into x).
     {
-        const int th = band_height<1, 1>((const void *)k_finish1, ws.w, ws.h);
+        const int th = ws.th_fin1 > 0 ? ws.th_fin1
+                                      : band_height<1, 1>((const void *)k_finish1, ws.w, ws.h);
         const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
         need_parts(nb, ws.nparts);
         k_finish1<<<nb, kBlock, 0, s>>>(g, kd, dt, dx * dx, b, ws.x, ws.p0, ws.p1, ws.ap, th, ws.parts,
@@ -655,7 +678,8 @@ static size_t solve_core(dtd_ctx *ctx, const double *b, const double *kd, double
         LAUNCH_CHECK();
         k_putback_step<<<1, kReduceBlock, 0, s>>>(ws.parts, nb, ws.st);
         LAUNCH_CHECK();
-        const int th2 = band_height<1, 1>((const void *)k_finish2, ws.w, ws.h);
+        const int th2 = ws.th_fin2 > 0 ? ws.th_fin2
+                                       : band_height<1, 1>((const void *)k_finish2, ws.w, ws.h);
         const int nb2 = stream_grid<1, 1>(ws.w, ws.h, th2);
         need_parts(nb2, ws.nparts);
         k_finish2<<<nb2, kBlock, 0, s>>>(g, kd, dt, dx * dx, b, ws.ap, ws.x, th2, ws.parts, ws.st);
@@ -678,6 +702,173 @@ static size_t solve_core(dtd_ctx *ctx, const double *b, const double *kd, double
         stats->ms_total += elapsed(ctx->ev[0], ctx->ev[3]);
     }
     return (size_t)ctx->host_st->its;
+}
+
+// ---- the tuner (dtd_tune; no reference counterpart) -----------------------------------------
+//
+// The reference tunes its cycle by a hand-run sweep of settings (`tune_the_cycle`,
+// multigrid/tests.rs:303-360).  Here the cycle's settings stay the reference's; what is tuned is
+// how the work is cut into launches: the band height of every streamed launch (the fused pre- and
+// post-smoothing of each level, the finest level's f64 passes) and the level size from which the
+// one-block bottom of the cycle takes over.  Band heights are searched kernel by kernel, each
+// candidate launched back to back on the resident data of a solve and timed with HIP events; the
+// bottom size and the whole plan are then judged by timing whole solves, and the built-in rules
+// stay unless the plan is faster.  A band height changes which cells a block's partial sums
+// cover, so a tuned solve's sums round differently; the cycle's correction is unchanged.
+
+static constexpr int kTuneTh[] = {4, 8, 12, 16, 20, 24, 32, 48, 64};
+static constexpr int kTuneBottom[] = {256, 512, 1024, 2048};
+
+// Average duration (ms) of `reps` back-to-back launches enqueued by f, after one untimed.
+template <class F>
+static double time_launches(dtd_ctx *ctx, int reps, F f) {
+    f();
+    HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
+    for (int k = 0; k < reps; k++) f();
+    HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
+    HIPCHK(hipEventSynchronize(ctx->ev[5]));
+    return elapsed(ctx->ev[4], ctx->ev[5]) / reps;
+}
+
+// Median device time (ms) of `reps` whole solves (after one untimed), setup and first guess
+// included, to the relative residual 1e-10; drops the captured graphs first (their launch shapes
+// follow the plan).
+static double time_solves(dtd_ctx *ctx, const double *b, const double *kd, double dx, double dt,
+                          int wrap_x, int reps) {
+    Workspace &ws = ctx->ws;
+    for (auto &g : ws.graph)
+        if (g) (void)hipGraphExecDestroy(g), g = nullptr;
+    std::vector<double> t;
+    for (int k = 0; k <= reps; k++) {
+        dtd_stats st{};
+        solve_core(ctx, b, kd, dx, dt, wrap_x, 0.0, 1e-20, true, kGuessLod, &st, nullptr, nullptr);
+        if (k) t.push_back(st.ms_total);
+    }
+    std::sort(t.begin(), t.end());
+    return t[t.size() / 2];
+}
+
+// The smallest-time candidate band height for one launch kind; set(th) applies it.
+template <class Set, class Launch>
+static int best_th(dtd_ctx *ctx, int rows, int reps, Set set, Launch launch) {
+    int best = 0;
+    double tbest = 0.0;
+    for (int th : kTuneTh) {
+        if (th > ((rows + 3) / 4) * 4 && th != kTuneTh[0]) break;
+        set(th);
+        const double t = time_launches(ctx, reps, launch);
+        if (!best || t < tbest) best = th, tbest = t;
+    }
+    return best;
+}
+
+template <typename T>
+static void tune_levels(dtd_ctx *ctx, std::vector<Lvl<T>> &lv, int reps, int min_cells) {
+    Workspace &ws = ctx->ws;
+    hipStream_t s = ctx->stream;
+    constexpr int VW = 8 / sizeof(T);
+    for (size_t l = 0; l + 1 < lv.size(); l++) {
+        Lvl<T> &L = lv[l], &C = lv[l + 1];
+        if (l > 0 && L.n <= min_cells) break;  // from here the bottom block may take the levels
+        const bool wide = L.w % VW == 0;
+        const bool top = l == 0 && ws.prec != DTD_CYCLE_F64;
+        auto pre = [&] {
+            if constexpr (std::is_same<T, float>::value) {
+                if (top) {
+                    const R64 rio{ws.r, ws.ap, ws.r2, 0.0};
+                    wide ? launch_pre<T, VW, kRUpdate>(L, C, L.z[0], ws.st, s, rio, ws.parts, ws.nparts)
+                         : launch_pre<T, 1, kRUpdate>(L, C, L.z[0], ws.st, s, rio, ws.parts, ws.nparts);
+                    return;
+                }
+            }
+            wide ? launch_pre<T, VW>(L, C, L.z[0], ws.st, s) : launch_pre<T, 1>(L, C, L.z[0], ws.st, s);
+        };
+        auto post = [&] {
+            if constexpr (std::is_same<T, float>::value) {
+                if (top) {
+                    wide ? launch_post<T, VW, kR64>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts,
+                                                    ws.nparts, ws.st, s)
+                         : launch_post<T, 1, kR64>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts,
+                                                   ws.nparts, ws.st, s);
+                    return;
+                }
+            }
+            wide ? launch_post<T, VW>(L, L.z[0], C.z[1], C.w, L.z[1], nullptr, nullptr, 0, ws.st, s)
+                 : launch_post<T, 1>(L, L.z[0], C.z[1], C.w, L.z[1], nullptr, nullptr, 0, ws.st, s);
+        };
+        L.th_pre = best_th(ctx, L.h, reps, [&](int th) { L.th_pre = th; }, pre);
+        L.th_post = best_th(ctx, L.h, reps, [&](int th) { L.th_post = th; }, post);
+    }
+}
+
+// Tunes the plan of the workspace's shape for the solve of (b, kd, dt) and keeps it in the context.
+static void tune(dtd_ctx *ctx, const double *b, const double *kd, double dx, double dt, int wrap_x,
+                 int reps, Plan &plan) {
+    Workspace &ws = ctx->ws;
+    hipStream_t s = ctx->stream;
+    apply_plan(ws, Plan{});
+    const double ms_rule = time_solves(ctx, b, kd, dx, dt, wrap_x, reps);
+    const bool fused = ctx->opts.nu == 2 && fused_enabled() && ws.lf.size() + ws.ld.size() > 1;
+    // Kernel by kernel on the resident data, with a state that never stops.
+    k_time_state<<<1, 64, 0, s>>>(ws.st, kd);
+    LAUNCH_CHECK();
+    const int kreps = std::max(3, reps);
+    if (fused) {
+        if (ws.prec == DTD_CYCLE_F64)
+            tune_levels<double>(ctx, ws.ld, kreps, kTuneBottom[0]);
+        else
+            tune_levels<float>(ctx, ws.lf, kreps, kTuneBottom[0]);
+    }
+    const Grid g = grid_of(ws);
+    const double s2 = dx * dx;
+    const bool f64 = ws.prec == DTD_CYCLE_F64;
+    ws.th_dir = best_th(ctx, ws.h, kreps, [&](int th) { ws.th_dir = th; }, [&] {
+        f64 ? launch_dir<double>(ws, ws.ld[0].z[1], ws.p1, ws.p0, s)
+            : launch_dir<float>(ws, ws.lf[0].z[1], ws.p1, ws.p0, s);
+    });
+    ws.th_resid = best_th(ctx, ws.h, kreps, [&](int th) { ws.th_resid = th; },
+                          [&] { launch_resid(ws, kd, dt, s2, b, ws.x, s, !fused_update(ws, ctx->opts)); });
+    auto fin = [&](int which, int th) {
+        const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
+        need_parts(nb, ws.nparts);
+        if (which == 1)
+            k_finish1<<<nb, kBlock, 0, s>>>(g, kd, dt, s2, b, ws.x, ws.p0, ws.p1, ws.ap, th, ws.parts,
+                                            ws.st);
+        else
+            k_finish2<<<nb, kBlock, 0, s>>>(g, kd, dt, s2, b, ws.ap, ws.x, th, ws.parts, ws.st);
+        LAUNCH_CHECK();
+    };
+    ws.th_fin1 = best_th(ctx, ws.h, kreps, [&](int th) { ws.th_fin1 = th; }, [&] { fin(1, ws.th_fin1); });
+    ws.th_fin2 = best_th(ctx, ws.h, kreps, [&](int th) { ws.th_fin2 = th; }, [&] { fin(2, ws.th_fin2); });
+    // The bottom's size, by whole solves with the band heights found.
+    double t_best = 0.0;
+    int b_best = -1;
+    if (fused) {
+        for (int bc : kTuneBottom) {
+            ws.bottom = bc;
+            const double t = time_solves(ctx, b, kd, dx, dt, wrap_x, reps);
+            if (b_best < 0 || t < t_best) b_best = bc, t_best = t;
+        }
+        ws.bottom = b_best;
+    } else {
+        t_best = time_solves(ctx, b, kd, dx, dt, wrap_x, reps);
+    }
+    // Keep the plan only if the whole solve is faster than with the built-in rules.
+    Plan p;
+    for (size_t l = 0; l < std::max(ws.lf.size(), ws.ld.size()); l++)
+        p.lvl.push_back(ws.prec == DTD_CYCLE_F64 ? std::array<int, 2>{ws.ld[l].th_pre, ws.ld[l].th_post}
+                                                 : std::array<int, 2>{ws.lf[l].th_pre, ws.lf[l].th_post});
+    p.th_dir = ws.th_dir, p.th_resid = ws.th_resid, p.th_fin1 = ws.th_fin1, p.th_fin2 = ws.th_fin2;
+    p.bottom = ws.bottom;
+    const double ms_plan = time_solves(ctx, b, kd, dx, dt, wrap_x, reps);
+    plan = ms_plan < ms_rule ? p : Plan{};
+    plan.ms_rule = ms_rule;
+    plan.ms_tuned = std::min(ms_plan, ms_rule);
+    apply_plan(ws, plan);
+    for (auto &gr : ws.graph)
+        if (gr) (void)hipGraphExecDestroy(gr), gr = nullptr;
+    upload_levels(ws, s);
+    HIPCHK(hipStreamSynchronize(s));
 }
 
 // Calibration kernels of dtd_time_kernel: a float4 copy, and five float4 input streams combined
@@ -884,6 +1075,8 @@ int dtd_solve_device_noalloc(dtd_ctx *ctx, size_t w, size_t h, double dx, int wr
         Workspace sw;
         Carver carve{base, kScratchBytes};
         lay_out(sw, ctx->opts, (int)w, (int)h, wrap, true, std::ref(carve));
+        auto it = ctx->plans.find(PlanKey{(int)w, (int)h, wrap, ctx->opts.cycle_precision});
+        if (it != ctx->plans.end()) apply_plan(sw, it->second);  // launch shapes as dtd_solve_device's
         ScratchSwap swap(ctx, std::move(sw));
         upload_levels(ctx->ws, ctx->stream);
         double rel = 0.0;
@@ -1085,6 +1278,19 @@ int dtd_solve_batch(dtd_ctx *ctx, size_t n, const dtd_grid *grids, double rel_to
         if (n == 0) return;
         if (!grids) throw Error{DTD_EINVAL, "null grid array"};
         solve_batch(ctx, n, grids, rel_tol, iterations, rel_residuals, stats);
+    });
+}
+
+int dtd_tune_batch(dtd_ctx *ctx, size_t n, const dtd_grid *grids, double rel_tol, int reps,
+                   double *ms_rule, double *ms_tuned) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (ms_rule) *ms_rule = 0.0;
+        if (ms_tuned) *ms_tuned = 0.0;
+        if (n == 0) return;
+        if (!grids) throw Error{DTD_EINVAL, "null grid array"};
+        if (reps < 1) throw Error{DTD_EINVAL, "reps must be positive"};
+        tune_batch(ctx, n, grids, rel_tol, std::min(reps, 20), ms_rule, ms_tuned);
     });
 }
 
@@ -1314,11 +1520,11 @@ int dtd_cycle_precondition(dtd_ctx *ctx, const float *r32, const double *r64, fl
         if (ws.prec == DTD_CYCLE_F64) {
             // An f64 cycle reads the f64 residual; r32 is its value rounded, kept for the ABI.
             nbz = vcycle<double>(ws.ld, 0, (int)ctx->opts.nu, ws.r, ws.parts, ws.nparts, nullptr,
-                                 s, kRLevel, R64{}, ws.d_ld);
+                                 s, kRLevel, R64{}, ws.d_ld, ws.bottom);
         } else {
             upload(ctx, ws.lf[0].r, r32, n * sizeof(float));
             nbz = vcycle<float>(ws.lf, 0, (int)ctx->opts.nu, ws.r, ws.parts, ws.nparts, nullptr,
-                                s, kRLevel, R64{}, ws.d_lf);
+                                s, kRLevel, R64{}, ws.d_lf, ws.bottom);
         }
         k_sum1<<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.pair);
         LAUNCH_CHECK();
@@ -1616,6 +1822,64 @@ int dtd_solve_rel_guess_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int 
         st->rel_residual = rel;
         copy_answer(ctx, h_dev);
     });
+}
+
+int dtd_tune(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, const double *h_dev,
+             const double *kd_dev, double dt, int reps, double *ms_rule, double *ms_tuned) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        if (ms_rule) *ms_rule = 0.0;
+        if (ms_tuned) *ms_tuned = 0.0;
+        check_grid(w, h);
+        const size_t n = w * h;
+        const int wrap = wrap_x && w > 1;
+        const PlanKey key{(int)w, (int)h, wrap, ctx->opts.cycle_precision};
+        if (reps < 1) {
+            // Forget the shape's plan: the built-in rules from the next fit on.
+            ctx->plans.erase(key);
+            if (ctx->ws.w == (int)w && ctx->ws.h == (int)h && ctx->ws.wrap == wrap) {
+                apply_plan(ctx->ws, Plan{});
+                for (auto &g : ctx->ws.graph)
+                    if (g) (void)hipGraphExecDestroy(g), g = nullptr;
+                upload_levels(ctx->ws, ctx->stream);
+                HIPCHK(hipStreamSynchronize(ctx->stream));
+            }
+            return;
+        }
+        if (n == 0) return;
+        if (!h_dev || !kd_dev) throw Error{DTD_EINVAL, "null array"};
+        if (!(std::isfinite(dx) && dx > 0.0 && std::isfinite(dt) && dt > 0.0))
+            throw Error{DTD_EINVAL, "dx and dt must be finite and positive"};
+        fit(ctx, (int)w, (int)h, wrap);
+        Plan plan;
+        tune(ctx, h_dev, kd_dev, dx, dt, wrap_x, std::min(reps, 50), plan);
+        ctx->plans[key] = plan;
+        if (ms_rule) *ms_rule = plan.ms_rule;
+        if (ms_tuned) *ms_tuned = plan.ms_tuned;
+    });
+}
+
+int dtd_tune_plan(const dtd_ctx *ctx, size_t w, size_t h, int wrap_x, char *buf, size_t cap) {
+    if (!ctx || !buf || cap == 0) return DTD_EINVAL;
+    const int wrap = wrap_x && w > 1;
+    auto it = ctx->plans.find(PlanKey{(int)w, (int)h, wrap, ctx->opts.cycle_precision});
+    std::string t;
+    if (it == ctx->plans.end()) {
+        t = "{}";
+    } else {
+        const Plan &p = it->second;
+        t = "{\"levels\": [";
+        for (size_t l = 0; l < p.lvl.size(); l++)
+            t += (l ? ", [" : "[") + std::to_string(p.lvl[l][0]) + ", " + std::to_string(p.lvl[l][1]) + "]";
+        t += "], \"direction\": " + std::to_string(p.th_dir) + ", \"residual\": " +
+             std::to_string(p.th_resid) + ", \"finish1\": " + std::to_string(p.th_fin1) +
+             ", \"finish2\": " + std::to_string(p.th_fin2) + ", \"bottom\": " +
+             std::to_string(p.bottom) + ", \"ms_rule\": " + std::to_string(p.ms_rule) +
+             ", \"ms_tuned\": " + std::to_string(p.ms_tuned) + "}";
+    }
+    if (t.size() + 1 > cap) return DTD_EINVAL;
+    std::memcpy(buf, t.c_str(), t.size() + 1);
+    return DTD_OK;
 }
 
 int dtd_inject_fault(dtd_ctx *ctx, long after, long *calls) {

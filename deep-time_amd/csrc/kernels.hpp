@@ -380,6 +380,9 @@ struct Lvl {
     // The same records rounded to bf16 (8 bytes a cell), which the fused smoothing kernels read
     // instead when given (the bf16-storage variant of the cycle); nullptr otherwise.
     unsigned short *coefh;
+    // Band heights of the level's fused pre- and post-smoothing launches chosen by the tuner
+    // (dtd_tune); 0: the occupancy rule.
+    int th_pre, th_post;
 };
 
 // f32 -> bf16 rounded to nearest, ties to even (a NaN stays a quiet NaN); bf16 -> f32 is exact.

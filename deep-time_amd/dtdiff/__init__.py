@@ -47,7 +47,8 @@ SYMBOLS = (
     "dtd_nn_bwd_weight_scratch", "dtd_nn_conv3x3_bwd_weight", "dtd_nn_relu_bwd", "dtd_nn_pool",
     "dtd_nn_up_cat", "dtd_nn_up_cat_bwd", "dtd_nn_input", "dtd_nn_loss", "dtd_nn_guess",
     "dtd_nn_adam", "dtd_nn_to_bf16", "dtd_sync", "dtd_solve_rel_guess_device", "dtd_lod_device",
-    "dtd_inject_fault", "dtd_solve_device_noalloc",
+    "dtd_inject_fault", "dtd_solve_device_noalloc", "dtd_tune", "dtd_tune_plan",
+    "dtd_tune_batch",
 )
 # The stage's slope failure constants (crates/deeptime-core/src/surface/marine.rs:231-232).
 FAILURE_RATE = 0.05
@@ -215,6 +216,14 @@ def load(path: str = LIB_PATH):
                                            _vp, ctypes.c_double, ctypes.c_double,
                                            ctypes.POINTER(Stats)]
     L.dtd_solve_device_noalloc.restype = ctypes.c_int
+    L.dtd_tune.argtypes = [_vp, _sz, _sz, ctypes.c_double, ctypes.c_int, _vp, _vp,
+                           ctypes.c_double, ctypes.c_int, _dp, _dp]
+    L.dtd_tune.restype = ctypes.c_int
+    L.dtd_tune_plan.argtypes = [_vp, _sz, _sz, ctypes.c_int, ctypes.c_char_p, _sz]
+    L.dtd_tune_plan.restype = ctypes.c_int
+    L.dtd_tune_batch.argtypes = [_vp, _sz, ctypes.POINTER(Grid), ctypes.c_double, ctypes.c_int,
+                                 _dp, _dp]
+    L.dtd_tune_batch.restype = ctypes.c_int
     L.dtd_inject_fault.argtypes = [_vp, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
     L.dtd_inject_fault.restype = ctypes.c_int
     for name in ("dtd_solve", "dtd_solve_device", "dtd_solve_rel_device", "dtd_operator",
@@ -505,6 +514,47 @@ class Cycle:
         self.ws.check(load().dtd_cycle_precondition(self.ws._ctx, r32p, r64p,
                                                     z.ctypes.data_as(_fp), ctypes.byref(rz)))
         return z, rz.value
+
+
+def tune(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float, reps: int = 5):
+    """dtd_tune: searches the band heights and the bottom size for d's shape on the solve of
+    (h_dev, kd_dev, dt) and keeps the plan in the workspace; returns (ms with the built-in rules,
+    ms with the plan kept, the plan as a dict)."""
+    import json
+
+    a, b = ctypes.c_double(), ctypes.c_double()
+    ws.check(load().dtd_tune(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x), _ptr(h_dev), _ptr(kd_dev),
+                             dt, reps, ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value, tune_plan(ws, d)
+
+
+def tune_batch(ws: Workspace, items, rel_tol: float = 1e-10, reps: int = 3):
+    """dtd_tune_batch over the grids of `items` (as for solve_batch); their h arrays keep h0.
+    Returns (ms with the built-in band heights, ms with the kept ones)."""
+    n = len(items)
+    arr = (Grid * max(n, 1))()
+    for k, (d, h_dev, kd_dev, dt, tol_m) in enumerate(items):
+        arr[k] = Grid(d.w, d.h, d.dx, int(d.wrap_x), dt, tol_m, _ptr(h_dev).value,
+                      _ptr(kd_dev).value)
+    a, b = ctypes.c_double(), ctypes.c_double()
+    ws.check(load().dtd_tune_batch(ws._ctx, n, arr, rel_tol, reps, ctypes.byref(a),
+                                   ctypes.byref(b)))
+    return a.value, b.value
+
+
+def tune_plan(ws: Workspace, d: Dims) -> dict:
+    """The plan dtd_tune keeps for d's shape ({} when none)."""
+    import json
+
+    buf = ctypes.create_string_buffer(4096)
+    ws.check(load().dtd_tune_plan(ws._ctx, d.w, d.h, int(d.wrap_x), buf, 4096))
+    return json.loads(buf.value.decode())
+
+
+def forget_plan(ws: Workspace, d: Dims) -> None:
+    """Drops the tuned plan of d's shape (the built-in rules from then on)."""
+    ws.check(load().dtd_tune(ws._ctx, d.w, d.h, d.dx, int(d.wrap_x), None, None, 1.0, 0, None,
+                             None))
 
 
 def inject_fault(ws: Workspace, after: int) -> int:

@@ -272,6 +272,37 @@ int dtd_solve_batch(dtd_ctx *ctx, size_t n, const dtd_grid *grids, double rel_to
  * overwritten: solve again before reading results. */
 int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *bytes);
 
+/* The launch tuner (the project's own; the reference tunes its cycle's settings by a hand-run
+ * sweep, `tune_the_cycle`, multigrid/tests.rs:303-360, whose settings stay the reference's here).
+ * For the w x h grid of (h_dev, kd_dev, dt) it searches how the solve's work is cut into launches:
+ * the band height (rows a wave walks) of every streamed launch, the fused pre- and
+ * post-smoothing of each level and the finest level's f64 passes, each candidate launched `reps`
+ * times on the resident data and timed with HIP events; then the level size from which the
+ * one-block bottom of the cycle takes over, by timing whole solves.  The plan is kept only if
+ * whole solves with it are faster than with the built-in rules; either way it is kept in the
+ * context for that shape (and cycle precision) and applied whenever the workspace takes the shape
+ * again.  h_dev and kd_dev are only read.  *ms_rule and *ms_tuned (optional) get the median
+ * device time of a whole solve with the built-in rules and with the plan kept.  reps < 1 forgets
+ * the shape's plan.  A tuned solve's partial sums cover other cells, so its sums round differently
+ * (the cycle's correction is unchanged); it meets the same stopping test. */
+int dtd_tune(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, const double *h_dev,
+             const double *kd_dev, double dt, int reps, double *ms_rule, double *ms_tuned);
+
+/* dtd_tune for the batched solve: the band heights of its launches by class of level (levels of
+ * at least 64 rows, of 16-63 rows, of fewer, and the f64 passes of grids of at least 64 rows),
+ * chosen by timing whole dtd_solve_batch calls on these grids (median of reps after one untimed
+ * each), one class at a time, and kept in the context for this list of shapes; later batches of
+ * the same shapes use it.  The grids' h arrays get their h0 back before every timed solve and at
+ * the end.  *ms_rule and *ms_tuned (optional) get the batch's device time with the built-in
+ * heights and with the kept ones. */
+int dtd_tune_batch(dtd_ctx *ctx, size_t n, const dtd_grid *grids, double rel_tol, int reps,
+                   double *ms_rule, double *ms_tuned);
+
+/* The plan dtd_tune keeps for a shape, as JSON text in buf (cap bytes): per level the pre- and
+ * post-smoothing band heights (0: the built-in rule), the f64 passes' band heights, the bottom's
+ * size (-1: built-in) and the two timings; "{}" when the shape has no plan. */
+int dtd_tune_plan(const dtd_ctx *ctx, size_t w, size_t h, int wrap_x, char *buf, size_t cap);
+
 /* Test hook for the failure seam of the reference's device cycle (`Pre`, multigrid.rs:385-472,
  * whose recovery drops the device state and builds it again, multigrid.rs:443-455).  Arms a
  * simulated device error: of the checked HIP calls the context makes from now on, the one after
