@@ -287,7 +287,8 @@ def extra_configs(args, d2, h02, kd2, dt2, h0d2, kdd2, ms_f32, xo2):
     relative residual 1e-10 on resident inputs (2 untimed solves, then the mean of 5), with its
     iterations, the true residual recomputed on the host, and the relative L2 difference from the
     oracle's answer where the oracle's solve takes seconds: the 512x256 constant case, the plain
-    FP64 cycle and the bf16-record cycle at 8192x4096, the 16384x8192 constant case with the f32,
+    FP64 cycle, the bf16-record cycle and the f32 cycle with bilinear transfers or the exact f64
+    coarsest solve at 8192x4096, the 16384x8192 constant case with the f32,
     bf16-record and plain FP64 cycles, the 512 regional grids in one batched solve, and the learned
     first guess at 8192x4096.  vs_plain_f64 is the plain FP64 cycle's time at the same size over
     this line's."""
@@ -313,8 +314,8 @@ def extra_configs(args, d2, h02, kd2, dt2, h0d2, kdd2, ms_f32, xo2):
         synchronize()
         return (time.perf_counter() - t0) * 1e3 / steps, st
 
-    def single(d, h0, kd, dt, h0d, kdd, cycle, xo=None):
-        ws = dtdiff.Workspace(device=0, cycle_precision=cycles[cycle])
+    def single(d, h0, kd, dt, h0d, kdd, cycle, xo=None, **opts):
+        ws = dtdiff.Workspace(device=0, cycle_precision=cycles[cycle], **opts)
         hd = DeviceArray(d.cells())
         try:
             def step():
@@ -325,7 +326,8 @@ def extra_configs(args, d2, h02, kd2, dt2, h0d2, kdd2, ms_f32, xo2):
         finally:
             ws.close()
             hd.free()
-        r = {"grid": [d.w, d.h], "cycle": cycle, "ms": ms, "unknowns_per_s": d.cells() / ms * 1e3,
+        r = {"grid": [d.w, d.h], "cycle": cycle, **opts, "ms": ms,
+             "unknowns_per_s": d.cells() / ms * 1e3,
              "iterations": int(st.iterations), "restarts": int(st.restarts),
              "rel_residual_true": O.rel_residual(d.w, d.h, d.dx, d.wrap_x, kd, dt, h0, x)}
         if xo is not None:
@@ -352,6 +354,13 @@ def extra_configs(args, d2, h02, kd2, dt2, h0d2, kdd2, ms_f32, xo2):
     bf = single(d2, h02, kd2, dt2, h0d2, kdd2, "bf16", xo2)
     bf["vs_plain_f64"] = f64["ms"] / bf["ms"]
     out["lognormal-8192x4096-bf16"] = bf
+    # The cycle's options (no reference counterpart): bilinear transfers, the exact f64 coarsest
+    # solve; the f32 cycle otherwise as the headline's.
+    for name, opts in (("bilinear", {"transfer": dtdiff.TRANSFER_BILINEAR}),
+                       ("exact-coarsest", {"coarsest": dtdiff.COARSEST_EXACT})):
+        r = single(d2, h02, kd2, dt2, h0d2, kdd2, "f32", xo2, **opts)
+        r["vs_plain_f64"] = f64["ms"] / r["ms"]
+        out[f"lognormal-8192x4096-{name}"] = r
     # The learned first guess on the headline inputs.
     out["lognormal-8192x4096-learned"] = learned_line(d2, h02, kd2, dt2, h0d2, kdd2, f64["ms"],
                                                       timed, xo2)

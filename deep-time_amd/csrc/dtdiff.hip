@@ -86,6 +86,9 @@ static void lay_out(Workspace &ws, const dtd_opts &o, int w, int h, int wrap, bo
             L.r = l == 0 ? ws.r : (double *)own(nl, 8);  // the f64 cycle reads the CG residual
             L.z[0] = (double *)own(nl, 8);
             L.z[1] = (double *)own(nl, 8);
+            L.bil = o.transfer == DTD_TRANSFER_BILINEAR && l + 1 < sh.size();
+            if (o.coarsest == DTD_COARSEST_EXACT && l + 1 == sh.size())
+                L.chol = (double *)own((size_t)nl * nl, 8);
             ws.ld.push_back(L);
         } else {
             Lvl<float> L{};
@@ -99,6 +102,9 @@ static void lay_out(Workspace &ws, const dtd_opts &o, int w, int h, int wrap, bo
             // The bf16 variant: the two finest levels (not a coarsest) keep bf16 records too.
             if (prec == DTD_CYCLE_BF16 && l < 2 && l + 1 < sh.size())
                 L.coefh = (unsigned short *)own(4 * nl, 2);
+            L.bil = o.transfer == DTD_TRANSFER_BILINEAR && l + 1 < sh.size();
+            if (o.coarsest == DTD_COARSEST_EXACT && l + 1 == sh.size())
+                L.chol = (double *)own((size_t)nl * nl, 8);
             ws.lf.push_back(L);
         }
     }
@@ -256,6 +262,13 @@ template <typename T>
 static void build_coarse(std::vector<Lvl<T>> &lv, T scale, hipStream_t s) {
     for (size_t l = 1; l < lv.size(); l++) {
         k_coarsen<T><<<nblocks(lv[l].n), kBlock, 0, s>>>(lv[l - 1], lv[l], scale);
+        LAUNCH_CHECK();
+    }
+    // The exact coarsest solve's factor (an option), from the coarsest level's records.
+    const Lvl<T> &C = lv.back();
+    if (C.chol) {
+        if (C.n > 64) throw Error{DTD_ESTATE, "the coarsest level exceeds one wave"};
+        k_chol<T><<<1, 64, 0, s>>>(C);
         LAUNCH_CHECK();
     }
 }
@@ -419,7 +432,7 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
     const int nb = nblocks(L.n);
     if (dot) need_parts(nb, cap);
     Lvl<T> &C0 = lv[l + 1];
-    if (nu == 2 && fused_enabled()) {
+    if (nu == 2 && fused_enabled() && !L.bil) {
         // Fused smoothing (kernels.hpp): pre-smoothing and restriction in one launch, the coarse
         // cycle, then both post-smoothing sweeps in one launch; 8-byte lanes when the width
         // allows.
@@ -468,7 +481,15 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
     vcycle<T>(lv, l + 1, nu, nullptr, parts, cap, st, s, kRLevel, R64{}, dlv, bottom);
     {
         const double *d = nu == 1 ? dot : nullptr;
-        k_sweep<T><<<nb, kBlock, 0, s>>>(L, L.z[cur], L.z[cur ^ 1], C0.z[1], C0.w, d, parts, st);
+        if (L.bil) {
+            // z' = z + P zc in place, then the sweep from z'.
+            k_add_prolong<T><<<nb, kBlock, 0, s>>>(L, C0, L.z[cur], st);
+            LAUNCH_CHECK();
+            k_sweep<T><<<nb, kBlock, 0, s>>>(L, L.z[cur], L.z[cur ^ 1], nullptr, 0, d, parts, st);
+        } else {
+            k_sweep<T><<<nb, kBlock, 0, s>>>(L, L.z[cur], L.z[cur ^ 1], C0.z[1], C0.w, d, parts,
+                                             st);
+        }
         LAUNCH_CHECK();
         cur ^= 1;
     }
@@ -496,7 +517,8 @@ static int run_vcycle(dtd_ctx *ctx, const double *dot, CgState *st, int rm = kRL
 // Whether the f32 cycle's finest level takes the f64 residual straight from the conjugate
 // gradients (fused smoothing; the residual's update rides on the pre-smoothing pass).
 static bool fused_update(const Workspace &ws, const dtd_opts &o) {
-    return ws.prec != DTD_CYCLE_F64 && o.nu == 2 && fused_enabled() && ws.lf.size() > 1;
+    return ws.prec != DTD_CYCLE_F64 && o.nu == 2 && fused_enabled() && ws.lf.size() > 1 &&
+           o.transfer == DTD_TRANSFER_AGGREGATE;
 }
 
 // One conjugate-gradient iteration (MG:630-733): parity 0 reads the direction from p1 and writes
@@ -808,7 +830,8 @@ static void tune(dtd_ctx *ctx, const double *b, const double *kd, double dx, dou
     hipStream_t s = ctx->stream;
     apply_plan(ws, Plan{});
     const double ms_rule = time_solves(ctx, b, kd, dx, dt, wrap_x, reps);
-    const bool fused = ctx->opts.nu == 2 && fused_enabled() && ws.lf.size() + ws.ld.size() > 1;
+    const bool fused = ctx->opts.nu == 2 && fused_enabled() && ws.lf.size() + ws.ld.size() > 1 &&
+                       ctx->opts.transfer == DTD_TRANSFER_AGGREGATE;
     // Kernel by kernel on the resident data, with a state that never stops.
     k_time_state<<<1, 64, 0, s>>>(ws.st, kd);
     LAUNCH_CHECK();
@@ -932,6 +955,8 @@ void dtd_default_opts(dtd_opts *o) {
     o->max_iter = 400;
     o->cycle_precision = DTD_CYCLE_F32;
     o->guess = DTD_GUESS_LOD;
+    o->transfer = DTD_TRANSFER_AGGREGATE;
+    o->coarsest = DTD_COARSEST_SWEEPS;
 }
 
 int dtd_create(const dtd_opts *opts, dtd_ctx **out) {
@@ -945,7 +970,11 @@ int dtd_create(const dtd_opts *opts, dtd_ctx **out) {
     if (ctx->opts.nu < 1 || ctx->opts.max_iter < 1 ||
         (ctx->opts.cycle_precision != DTD_CYCLE_F32 && ctx->opts.cycle_precision != DTD_CYCLE_F64 &&
          ctx->opts.cycle_precision != DTD_CYCLE_BF16) ||
-        (ctx->opts.guess != DTD_GUESS_LOD && ctx->opts.guess != DTD_GUESS_RHS)) {
+        (ctx->opts.guess != DTD_GUESS_LOD && ctx->opts.guess != DTD_GUESS_RHS) ||
+        (ctx->opts.transfer != DTD_TRANSFER_AGGREGATE &&
+         ctx->opts.transfer != DTD_TRANSFER_BILINEAR) ||
+        (ctx->opts.coarsest != DTD_COARSEST_SWEEPS && ctx->opts.coarsest != DTD_COARSEST_EXACT) ||
+        (ctx->opts.transfer == DTD_TRANSFER_BILINEAR && ctx->opts.cycle_precision == DTD_CYCLE_BF16)) {
         delete ctx;
         return DTD_EINVAL;
     }

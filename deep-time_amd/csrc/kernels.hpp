@@ -383,6 +383,12 @@ struct Lvl {
     // Band heights of the level's fused pre- and post-smoothing launches chosen by the tuner
     // (dtd_tune); 0: the occupancy rule.
     int th_pre, th_post;
+    // Options of the cycle with no reference counterpart.  bil: this level restricts to and is
+    // corrected from the next by the bilinear transfers instead of 2 x 2 aggregation.  chol (the
+    // coarsest level only): its Cholesky factor in f64 (n x n, lower triangle), for the exact
+    // solve that replaces the coarsest level's sweeps.
+    int bil;
+    double *chol;
 };
 
 // f32 -> bf16 rounded to nearest, ties to even (a NaN stays a quiet NaN); bf16 -> f32 is exact.
@@ -526,11 +532,154 @@ __device__ inline T sweep_cell(const Lvl<T> &L, const T *__restrict__ zin,
     return zp(c) + c_wd(L, c) * (L.r[c] - ax);
 }
 
+// ---- the bilinear transfers (an option of the cycle; no reference counterpart) ---------------
+//
+// Along one axis, fine index i lies in coarse cell i / 2 and nearest to coarse cell i / 2 + 1 (i
+// odd) or i / 2 - 1 (i even); that neighbour wraps on a periodic axis of nc cells and otherwise
+// falls back to the cell itself.  P takes 3/4 of the first and 1/4 of the second along each axis;
+// the restriction is P's transpose.
+__device__ inline void bil_axis(int i, int nc, int wrap, int &c, int &nb) {
+    c = i >> 1;
+    int m = (i & 1) ? c + 1 : c - 1;
+    if (wrap)
+        m = (m + nc) % nc;
+    else if (m < 0 || m >= nc)
+        m = c;
+    nb = m;
+}
+template <typename T>
+__device__ inline T bil_weight(int i, int C, int nc, int wrap) {
+    int c, nb;
+    bil_axis(i, nc, wrap, c, nb);
+    return (c == C ? (T)0.75 : (T)0) + (nb == C ? (T)0.25 : (T)0);
+}
+// (P zc) at fine cell (i, j) of a level whose next coarser level is C:
+// 3/4 (3/4 zc[cy][cx] + 1/4 zc[cy][nx]) + 1/4 (3/4 zc[ny][cx] + 1/4 zc[ny][nx]).
+template <typename T>
+__device__ inline T bil_at(const T *__restrict__ zc, int wc, int hc, int cwrap, int i, int j) {
+    int cx, nx, cy, ny;
+    bil_axis(i, wc, cwrap, cx, nx);
+    bil_axis(j, hc, 0, cy, ny);
+    const T a = (T)0.75 * zc[cy * wc + cx] + (T)0.25 * zc[cy * wc + nx];
+    const T b = (T)0.75 * zc[ny * wc + cx] + (T)0.25 * zc[ny * wc + nx];
+    return (T)0.75 * a + (T)0.25 * b;
+}
+// (P^T (r - A z)) at coarse cell k: over the fine rows 2J-1 .. 2J+2 on the level and the fine
+// columns 2I-1 .. 2I+2 (wrapped on a periodic level, each column once), the terms
+// (weight_y * weight_x) * (r - A z) with a non-zero weight, added from +0, rows outer.
+template <typename T>
+__device__ inline T bil_restrict_cell(const Lvl<T> &f, const T *__restrict__ z, const Lvl<T> &c,
+                                      int k) {
+    const int J = k / c.w, I = k - J * c.w;
+    T acc = (T)0;
+    for (int t = -1; t <= 2; t++) {
+        const int j = 2 * J + t;
+        if (j < 0 || j >= f.h) continue;
+        const T wy = bil_weight<T>(j, J, c.h, 0);
+        if (wy == (T)0) continue;
+        int seen[4], ns = 0;
+        for (int u = -1; u <= 2; u++) {
+            int i = 2 * I + u;
+            if (f.wrap)
+                i = ((i % f.w) + f.w) % f.w;
+            else if (i < 0 || i >= f.w)
+                continue;
+            bool dup = false;
+            for (int q = 0; q < ns; q++) dup |= seen[q] == i;
+            if (dup) continue;
+            seen[ns++] = i;
+            const T wx = bil_weight<T>(i, I, c.w, c.wrap);
+            if (wx == (T)0) continue;
+            const int cf = j * f.w + i;
+            acc += (wy * wx) * (f.r[cf] - lvl_apply(f, cf, i, j, [&](int q) { return z[q]; }));
+        }
+    }
+    return acc;
+}
+
+// z += P zc on level L from its next coarser level C, in place (the corrected sweep then starts
+// from z).
+template <typename T>
+__global__ void k_add_prolong(Lvl<T> L, Lvl<T> C, T *__restrict__ z, const CgState *__restrict__ st) {
+    if (st && st->done) return;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= L.n) return;
+    const int j = c / L.w, i = c - j * L.w;
+    z[c] = z[c] + bil_at(C.z[1], C.w, C.h, C.wrap, i, j);
+}
+
+// ---- the exact coarsest solve (an option of the cycle; no reference counterpart) -------------
+//
+// The coarsest level's matrix in f64, row c holding the terms of lvl_apply in its order from 0,
+// factorised A = L L^T in LDS by one wave (row c by lane c: per column k the pivot's square root,
+// the column scaled by it, the trailing rows updated), the lower triangle written to chol.
+template <typename T>
+__global__ __launch_bounds__(64) void k_chol(Lvl<T> L) {
+    __shared__ double A[64][64];
+    const int c = threadIdx.x, n = L.n;
+    if (c < n) {
+        for (int q = 0; q < n; q++) A[c][q] = 0.0;
+        const int j = c / L.w, i = c - j * L.w, c0 = j * L.w;
+        A[c][c] += (double)c_diag(L, c);
+        if (i + 1 < L.w) A[c][c + 1] -= (double)c_east(L, c);
+        if (i > 0) A[c][c - 1] -= (double)c_east(L, c - 1);
+        if (L.wrap) {
+            if (i == L.w - 1) A[c][c0] -= (double)c_east(L, c0 + L.w - 1);
+            if (i == 0) A[c][c0 + L.w - 1] -= (double)c_east(L, c0 + L.w - 1);
+        }
+        if (j + 1 < L.h) A[c][c + L.w] -= (double)c_south(L, c);
+        if (j > 0) A[c][c - L.w] -= (double)c_south(L, c - L.w);
+    }
+    __syncthreads();
+    for (int k = 0; k < n; k++) {
+        if (c == k) A[k][k] = sqrt(A[k][k]);
+        __syncthreads();
+        if (c > k && c < n) A[c][k] = A[c][k] / A[k][k];
+        __syncthreads();
+        if (c > k && c < n)
+            for (int q = k + 1; q <= c; q++) A[c][q] -= A[c][k] * A[q][k];
+        __syncthreads();
+    }
+    if (c < n)
+        for (int q = 0; q <= c; q++) L.chol[c * n + q] = A[c][q];
+}
+
+// z = (L L^T)^-1 r by the block's first wave alone (lanes = cells; the caller's other waves do
+// not take part): r in f64, forward then backward substitution by columns, rounded to T.  ys: 64
+// doubles of LDS.  One wave's LDS accesses take effect in program order; the fences keep the
+// compiler from moving them across a step.
+template <typename T>
+__device__ inline void coarsest_exact_wave(const Lvl<T> &L, const T *r, T *zout, double *ys) {
+    const int c = threadIdx.x, n = L.n;
+    const double *ch = L.chol;
+    auto step = [] {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    };
+    if (c < n) ys[c] = (double)r[c];
+    step();
+    for (int k = 0; k < n; k++) {
+        if (c == k) ys[k] = ys[k] / ch[k * n + k];
+        step();
+        if (c > k && c < n) ys[c] -= ch[c * n + k] * ys[k];
+        step();
+    }
+    for (int k = n - 1; k >= 0; k--) {
+        if (c == k) ys[k] = ys[k] / ch[k * n + k];
+        step();
+        if (c < k) ys[c] -= ch[k * n + c] * ys[k];
+        step();
+    }
+    if (c < n) zout[c] = (T)ys[c];
+}
+
 // One coarse cell of `Level::restrict_into` (MG:332-356): the sum over its 2 x 2 fine block of
-// (r - A z), fine row j0 then j0 + 1, columns ascending, from +0.
+// (r - A z), fine row j0 then j0 + 1, columns ascending, from +0 (the bilinear transpose when the
+// level takes that option).
 template <typename T>
 __device__ inline T restrict_cell(const Lvl<T> &f, const T *__restrict__ z, const Lvl<T> &c,
                                   int k) {
+    if (f.bil) return bil_restrict_cell(f, z, c, k);
     const int jc = k / c.w, ic = k - jc * c.w;
     const int r1 = min(2 * jc + 2, f.h), c1 = min(2 * ic + 2, f.w);
     T acc = (T)0;
@@ -578,6 +727,10 @@ __global__ void k_restrict(Lvl<T> f, const T *__restrict__ z, Lvl<T> c,
     if (st && st->done) return;
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= c.n) return;
+    if (f.bil) {
+        c.r[k] = bil_restrict_cell(f, z, c, k);
+        return;
+    }
     const int jc = k / c.w, ic = k - jc * c.w;
     const int r1 = min(2 * jc + 2, f.h), c1 = min(2 * ic + 2, f.w);
     T acc = (T)0;
@@ -634,8 +787,24 @@ __global__ void k_coarsest(Lvl<T> L, T *__restrict__ zout, int sweeps,
                            const double *__restrict__ dot, double *__restrict__ parts,
                            const CgState *__restrict__ st) {
     __shared__ T zs[2][64];
+    __shared__ double ys[64];
     if (st && st->done) return;
-    coarsest_wave<T>(L, zout, sweeps, dot, parts, zs);
+    if (!L.chol) {
+        coarsest_wave<T>(L, zout, sweeps, dot, parts, zs);
+        return;
+    }
+    coarsest_exact_wave<T>(L, L.r, zout, ys);
+    if (dot && threadIdx.x == 0) {
+        // sum dot * z per strip of 8 rows, the strips in order (as the sweeps' dot).
+        double total = -0.0;
+        for (int s0 = 0; s0 < L.n; s0 += 8 * L.w) {
+            double a = -0.0;
+            const int s1 = min(s0 + 8 * L.w, L.n);
+            for (int q = s0; q < s1; q++) a += dot[q] * (double)(T)ys[q];
+            total += a;
+        }
+        *parts = total;
+    }
 }
 
 // The bottom of the V-cycle (nu = 2) in one block: from level b0 down, each level's first sweep,
@@ -711,7 +880,15 @@ __global__ __launch_bounds__(kBottomBlock) void k_bottom(const Lvl<T> *__restric
         for (int k = t; k < C.n; k += kBottomBlock) C.r[k] = restrict_cell(L, L.z[1], C, k);
         __syncthreads();
     }
-    {
+    if (lv[nl - 1].chol) {
+        // The coarsest level's exact solve (an option), by the first wave.
+        const Lvl<T> L = loc[nb - 1];
+        __shared__ double ys[64];
+        Lvl<T> G = L;
+        G.chol = lv[nl - 1].chol;
+        if (t < 64) coarsest_exact_wave<T>(G, L.r, nb == 1 ? lv[b0].z[1] : L.z[0], ys);
+        __syncthreads();
+    } else {
         // The coarsest level (at most 64 cells): z = wd r, then 15 sweeps, by the first wave with
         // its stencil's records in registers.
         const Lvl<T> L = loc[nb - 1];
@@ -770,7 +947,8 @@ __global__ __launch_bounds__(kBottomBlock) void k_bottom(const Lvl<T> *__restric
         // every point of its stencil); then the corrected sweep and the last.
         for (int c = t; c < L.n; c += kBottomBlock) {
             const int j = c / L.w, i = c - j * L.w;
-            L.z[0][c] = L.z[1][c] + C.z[0][(j >> 1) * C.w + (i >> 1)];
+            L.z[0][c] = L.z[1][c] + (L.bil ? bil_at(C.z[0], C.w, C.h, C.wrap, i, j)
+                                           : C.z[0][(j >> 1) * C.w + (i >> 1)]);
         }
         __syncthreads();
         for (int c = t; c < L.n; c += kBottomBlock) {

@@ -34,6 +34,10 @@ CYCLE_F64 = 1
 CYCLE_BF16 = 2  # the f32 cycle with its two finest levels' records stored in bf16
 GUESS_LOD = 0
 GUESS_RHS = 1
+TRANSFER_AGGREGATE = 0  # the reference's 2x2 aggregation
+TRANSFER_BILINEAR = 1   # bilinear prolongation and its transpose (an option)
+COARSEST_SWEEPS = 0     # the reference's 16 sweeps on the coarsest level
+COARSEST_EXACT = 1      # the coarsest level solved exactly in f64 (an option)
 
 # Every symbol include/dtdiff.h declares (the CPU tests check the library exports them all).
 SYMBOLS = (
@@ -79,7 +83,8 @@ class Dims:
 class Opts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("omega", ctypes.c_float), ("nu", ctypes.c_uint),
                 ("scale", ctypes.c_float), ("max_iter", ctypes.c_uint),
-                ("cycle_precision", ctypes.c_int), ("guess", ctypes.c_int)]
+                ("cycle_precision", ctypes.c_int), ("guess", ctypes.c_int),
+                ("transfer", ctypes.c_int), ("coarsest", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):

@@ -57,6 +57,18 @@ extern "C" {
 #define DTD_CYCLE_BF16 2 /* the f32 cycle, its two finest levels' smoothers reading their
                             operator records stored in bf16 (a reduced-storage variant) */
 
+/* Transfers between the cycle's levels. */
+#define DTD_TRANSFER_AGGREGATE 0 /* the reference's: 2x2 aggregation (restriction sums the block,
+                                    prolongation copies the coarse value; multigrid.rs:117-356) */
+#define DTD_TRANSFER_BILINEAR 1  /* cell-centred bilinear prolongation (weights 3/4, 1/4 along each
+                                    axis from the two nearest coarse centres) and its transpose as
+                                    the restriction (full weighting); no reference counterpart */
+
+/* Solve of the cycle's coarsest level (at most 64 cells). */
+#define DTD_COARSEST_SWEEPS 0 /* the reference's: 16 damped-Jacobi sweeps in the cycle's precision */
+#define DTD_COARSEST_EXACT 1  /* exact: Cholesky in f64 (factorised once a solve); no reference
+                                 counterpart */
+
 /* First guess of a solve. */
 #define DTD_GUESS_LOD 0  /* the splitting, as `multigrid_in` (diffusion.rs:110-114) */
 #define DTD_GUESS_RHS 1  /* h0 itself */
@@ -73,6 +85,9 @@ typedef struct dtd_opts {
     unsigned max_iter;    /* conjugate-gradient iteration cap, 400 */
     int cycle_precision;  /* DTD_CYCLE_F32 */
     int guess;            /* DTD_GUESS_LOD */
+    int transfer;         /* DTD_TRANSFER_AGGREGATE (DTD_TRANSFER_BILINEAR: not with DTD_CYCLE_BF16,
+                             nor in the batched solve) */
+    int coarsest;         /* DTD_COARSEST_SWEEPS (DTD_COARSEST_EXACT: not in the batched solve) */
 } dtd_opts;
 
 /* What a solve did (all times in milliseconds, from HIP events on the context's stream). */

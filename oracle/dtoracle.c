@@ -406,6 +406,8 @@ struct or_hier {
     float omega;
     size_t nu;
     float scale;
+    int bil;       /* bilinear prolongation and its transpose as restriction (an option) */
+    double *chol;  /* the coarsest level's Cholesky factor, for its exact f64 solve (an option) */
 };
 
 static float *fzalloc(size_t n) { return (float *)calloc(n ? n : 1, sizeof(float)); }
@@ -566,12 +568,163 @@ static void restrict_into(const lvl *F, lvl *C) {
     }
 }
 
+/* ---- options of the cycle with no counterpart in the reference ------------------------------
+ * Bilinear transfers: the cell-centred bilinear prolongation P and its transpose as the
+ * restriction (full weighting), in place of 2x2 aggregation; the coarse levels' operators stay
+ * the aggregation ones.  Exact coarsest solve: the coarsest level (at most 64 cells) solved by a
+ * Cholesky factorisation in f64 in place of COARSE_SWEEPS damped-Jacobi sweeps.  The device
+ * kernels do the same operations in the same order. */
+
+/* Along one axis, fine index i lies in coarse cell *c = i / 2 and nearest to coarse cell
+ * *nb = c + 1 (i odd) or c - 1 (i even); *nb wraps on a periodic axis of nc cells and otherwise
+ * falls back to c. */
+static inline void bil_axis(long i, long nc, int wrap, long *c, long *nb) {
+    long cc = i / 2, m = (i & 1) ? cc + 1 : cc - 1;
+    if (wrap)
+        m = (m + nc) % nc;
+    else if (m < 0 || m >= nc)
+        m = cc;
+    *c = cc;
+    *nb = m;
+}
+
+/* P's weight of coarse index C for fine index i along one axis: 3/4 for the cell i lies in, 1/4
+ * for its nearest neighbour, summed when they coincide. */
+static inline float bil_weight(long i, long C, long nc, int wrap) {
+    long c, nb;
+    bil_axis(i, nc, wrap, &c, &nb);
+    return (c == C ? 0.75f : 0.0f) + (nb == C ? 0.25f : 0.0f);
+}
+
+/* (P zc) at fine cell (i, j): 3/4 (3/4 zc[cy][cx] + 1/4 zc[cy][nx]) + 1/4 (3/4 zc[ny][cx] +
+ * 1/4 zc[ny][nx]); cwrap: the coarse level wraps in x. */
+static inline float bil_at(const float *zc, size_t wc, size_t hc, int cwrap, size_t i, size_t j) {
+    long cx, nx, cy, ny;
+    bil_axis((long)i, (long)wc, cwrap, &cx, &nx);
+    bil_axis((long)j, (long)hc, 0, &cy, &ny);
+    float a = 0.75f * zc[cy * (long)wc + cx] + 0.25f * zc[cy * (long)wc + nx];
+    float b = 0.75f * zc[ny * (long)wc + cx] + 0.25f * zc[ny * (long)wc + nx];
+    return 0.75f * a + 0.25f * b;
+}
+
+/* (P^T res) at coarse cell (I, J): over the fine rows 2J-1 .. 2J+2 on the level and the fine
+ * columns 2I-1 .. 2I+2 (wrapped when the fine level wraps, each column once), the terms
+ * (weight_y * weight_x) * res with a non-zero weight, added from +0, rows outer. */
+static inline float bil_restrict_at(const float *res, size_t w, size_t h, int wrapf, size_t wc,
+                                    size_t hc, int cwrap, size_t I, size_t J) {
+    float acc = 0.0f;
+    for (long t = -1; t <= 2; t++) {
+        long j = 2 * (long)J + t;
+        if (j < 0 || j >= (long)h) continue;
+        float wy = bil_weight(j, (long)J, (long)hc, 0);
+        if (wy == 0.0f) continue;
+        long seen[4];
+        int ns = 0;
+        for (long u = -1; u <= 2; u++) {
+            long i = 2 * (long)I + u;
+            if (wrapf)
+                i = ((i % (long)w) + (long)w) % (long)w;
+            else if (i < 0 || i >= (long)w)
+                continue;
+            int dup = 0;
+            for (int q = 0; q < ns; q++) dup |= seen[q] == i;
+            if (dup) continue;
+            seen[ns++] = i;
+            float wx = bil_weight(i, (long)I, (long)wc, cwrap);
+            if (wx == 0.0f) continue;
+            acc += (wy * wx) * res[j * (long)w + i];
+        }
+    }
+    return acc;
+}
+
+void or_bil_prolong(size_t wf, size_t hf, size_t wc, size_t hc, int cwrap, const float *zc,
+                    float *out) {
+    for (size_t j = 0; j < hf; j++)
+        for (size_t i = 0; i < wf; i++) out[j * wf + i] = bil_at(zc, wc, hc, cwrap, i, j);
+}
+
+void or_bil_restrict(size_t wf, size_t hf, int wrapf, size_t wc, size_t hc, int cwrap,
+                     const float *res, float *rc) {
+    for (size_t J = 0; J < hc; J++)
+        for (size_t I = 0; I < wc; I++)
+            rc[J * wc + I] = bil_restrict_at(res, wf, hf, wrapf, wc, hc, cwrap, I, J);
+}
+
+/* The coarsest level's matrix in f64 (row c: the terms of lv_cell in its order, accumulated from
+ * 0), factorised in place: A = L L^T with L in the lower triangle (n x n, row-major). */
+static void coarse_factor(size_t w, size_t h, int wrap, const float *east, const float *south,
+                          const float *diag, double *A) {
+    size_t n = w * h;
+    memset(A, 0, n * n * sizeof(double));
+    for (size_t j = 0; j < h; j++)
+        for (size_t i = 0; i < w; i++) {
+            size_t c = j * w + i, c0 = j * w;
+            double *row = A + c * n;
+            row[c] += (double)diag[c];
+            if (i + 1 < w) row[c + 1] -= (double)east[c];
+            if (i > 0) row[c - 1] -= (double)east[c - 1];
+            if (wrap) {
+                if (i == w - 1) row[c0] -= (double)east[c0 + w - 1];
+                if (i == 0) row[c0 + w - 1] -= (double)east[c0 + w - 1];
+            }
+            if (j + 1 < h) row[c + w] -= (double)south[c];
+            if (j > 0) row[c - w] -= (double)south[c - w];
+        }
+    for (size_t k = 0; k < n; k++) {
+        A[k * n + k] = sqrt(A[k * n + k]);
+        for (size_t i = k + 1; i < n; i++) A[i * n + k] = A[i * n + k] / A[k * n + k];
+        for (size_t i = k + 1; i < n; i++)
+            for (size_t j = k + 1; j <= i; j++) A[i * n + j] -= A[i * n + k] * A[j * n + k];
+    }
+}
+
+/* z = (L L^T)^-1 r in f64 (forward, then backward substitution by columns), rounded to f32. */
+static void coarse_solve(const double *L, size_t n, const float *r, float *z) {
+    double y[64];
+    for (size_t c = 0; c < n; c++) y[c] = (double)r[c];
+    for (size_t k = 0; k < n; k++) {
+        y[k] = y[k] / L[k * n + k];
+        for (size_t i = k + 1; i < n; i++) y[i] -= L[i * n + k] * y[k];
+    }
+    for (size_t kk = n; kk-- > 0;) {
+        y[kk] = y[kk] / L[kk * n + kk];
+        for (size_t i = 0; i < kk; i++) y[i] -= L[kk * n + i] * y[kk];
+    }
+    for (size_t c = 0; c < n; c++) z[c] = (float)y[c];
+}
+
+void or_coarse_exact(size_t w, size_t h, int wrap, const float *east, const float *south,
+                     const float *diag, const float *r, float *z) {
+    size_t n = w * h;
+    double *A = (double *)malloc((n > 0 ? n * n : 1) * sizeof(double));
+    coarse_factor(w, h, wrap, east, south, diag, A);
+    coarse_solve(A, n, r, z);
+    free(A);
+}
+
+/* sum dot * z per strip of STRIP_ROWS rows from -0.0, the strips added in order. */
+static double strip_dot(size_t w, size_t h, const double *dot, const float *z) {
+    double total = sum_zero();
+    for (size_t j0 = 0; j0 < h; j0 += STRIP_ROWS) {
+        double a = sum_zero();
+        size_t j1 = j0 + STRIP_ROWS < h ? j0 + STRIP_ROWS : h;
+        for (size_t c = j0 * w; c < j1 * w; c++) a += dot[c] * (double)z[c];
+        total += a;
+    }
+    return total;
+}
+
 /* `vcycle`, MG:500-523: from z = 0; on the coarsest level a first sweep and COARSE_SWEEPS - 1
  * sweeps; otherwise nu sweeps, restriction, the coarse cycle, the corrected sweep and nu - 1
  * sweeps.  With `dot`, returns sum dot * z of the last sweep. */
 static double vcycle(or_hier *hy, size_t l, const double *dot) {
     lvl *L = &hy->lv[l];
     if (l + 1 == hy->n) {
+        if (hy->chol) {
+            coarse_solve(hy->chol, L->w * L->h, L->r, L->z);
+            return dot ? strip_dot(L->w, L->h, dot, L->z) : 0.0;
+        }
         first_sweep(L);
         double acc = 0.0;
         size_t ks = COARSE_SWEEPS > 2 ? COARSE_SWEEPS : 2;
@@ -580,18 +733,44 @@ static double vcycle(or_hier *hy, size_t l, const double *dot) {
     }
     first_sweep(L);
     for (size_t k = 1; k < hy->nu; k++) sweep(L, NULL, 0, NULL);
-    restrict_into(L, &hy->lv[l + 1]);
+    lvl *C = &hy->lv[l + 1];
+    if (hy->bil) {
+        size_t w = L->w, h = L->h;
+        for (size_t j = 0; j < h; j++)
+            for (size_t i = 0; i < w; i++) L->t[j * w + i] = L->r[j * w + i] - lv_cell(L, L->z, j, i);
+        or_bil_restrict(w, h, L->wrap, C->w, C->h, C->wrap, L->t, C->r);
+    } else {
+        restrict_into(L, C);
+    }
     vcycle(hy, l + 1, NULL);
-    size_t wc = hy->lv[l + 1].w;
-    double acc = sweep(L, hy->lv[l + 1].z, wc, hy->nu == 1 ? dot : NULL);
+    size_t wc = C->w;
+    if (hy->bil) {
+        /* z' = z + P zc, then the sweep from z' (as the aggregation path adds zc per cell). */
+        for (size_t j = 0; j < L->h; j++)
+            for (size_t i = 0; i < L->w; i++)
+                L->z[j * L->w + i] = L->z[j * L->w + i] + bil_at(C->z, C->w, C->h, C->wrap, i, j);
+    }
+    double acc = sweep(L, hy->bil ? NULL : C->z, wc, hy->nu == 1 ? dot : NULL);
     for (size_t k = 1; k < hy->nu; k++) acc = sweep(L, NULL, 0, k + 1 == hy->nu ? dot : NULL);
     return acc;
 }
 
-/* `Hierarchy::new`, MG:481-487. */
+/* `Hierarchy::new`, MG:481-487, with the cycle's options (transfer 1: bilinear; coarsest 1: the
+ * exact f64 solve). */
+or_hier *or_hier_new_opts(size_t w, size_t h, int wrap_x, const double *east, const double *south,
+                          const double *diag, float omega, size_t nu, float scale, int transfer,
+                          int coarsest);
+
 or_hier *or_hier_new(size_t w, size_t h, int wrap_x, const double *east, const double *south,
                      const double *diag, float omega, size_t nu, float scale) {
+    return or_hier_new_opts(w, h, wrap_x, east, south, diag, omega, nu, scale, 0, 0);
+}
+
+or_hier *or_hier_new_opts(size_t w, size_t h, int wrap_x, const double *east, const double *south,
+                          const double *diag, float omega, size_t nu, float scale, int transfer,
+                          int coarsest) {
     or_hier *hy = (or_hier *)calloc(1, sizeof(or_hier));
+    hy->bil = transfer == 1;
     size_t cap = 4;
     hy->lv = (lvl *)calloc(cap, sizeof(lvl));
     hy->omega = omega;
@@ -608,6 +787,12 @@ or_hier *or_hier_new(size_t w, size_t h, int wrap_x, const double *east, const d
         memset(&next, 0, sizeof next);
         if (!level_coarsen(&hy->lv[hy->n - 1], &next, omega, scale)) break;
         hy->lv[hy->n++] = next;
+    }
+    if (coarsest == 1) {
+        const lvl *L = &hy->lv[hy->n - 1];
+        size_t n = L->w * L->h;
+        hy->chol = (double *)malloc((n > 0 ? n * n : 1) * sizeof(double));
+        coarse_factor(L->w, L->h, L->wrap, L->east, L->south, L->diag, hy->chol);
     }
     return hy;
 }
@@ -627,6 +812,7 @@ void or_hier_free(or_hier *hy) {
         free(L->zp);
     }
     free(hy->lv);
+    free(hy->chol);
     free(hy);
 }
 

@@ -63,6 +63,21 @@ void or_hier_level(const or_hier *hy, size_t l, size_t *w, size_t *h, int *wrap,
                    float *south, float *diag, float *wd, float *mass);
 /* z = M^-1 r for r32 on the finest level; returns sum r64 * z (reference order). */
 double or_hier_precondition(or_hier *hy, const float *r32, const double *r64, float *z);
+/* The cycle's options (no counterpart in the reference): transfer 1 replaces 2x2 aggregation by
+ * the cell-centred bilinear prolongation P and its transpose as the restriction; coarsest 1
+ * solves the coarsest level exactly (Cholesky in f64) instead of COARSE_SWEEPS sweeps. */
+or_hier *or_hier_new_opts(size_t w, size_t h, int wrap, const double *east, const double *south,
+                          const double *diag, float omega, size_t nu, float scale, int transfer,
+                          int coarsest);
+/* The bilinear transfers alone: out = P zc (fine wf x hf, coarse wc x hc, cwrap: the coarse
+ * level wraps) and rc = P^T res (wrapf: the fine level wraps). */
+void or_bil_prolong(size_t wf, size_t hf, size_t wc, size_t hc, int cwrap, const float *zc,
+                    float *out);
+void or_bil_restrict(size_t wf, size_t hf, int wrapf, size_t wc, size_t hc, int cwrap,
+                     const float *res, float *rc);
+/* The exact coarsest solve alone: z = A^-1 r for a level's f32 operator, in f64. */
+void or_coarse_exact(size_t w, size_t h, int wrap, const float *east, const float *south,
+                     const float *diag, const float *r, float *z);
 
 /* ---- solves -------------------------------------------------------------------------------- */
 size_t or_cg_on(size_t w, size_t h, double dx, int wrap_x, double *x, const double *b,

@@ -54,6 +54,12 @@ def lib():
     L.or_hier_new.argtypes = [_sz, _sz, ctypes.c_int, _f64p, _f64p, _f64p, ctypes.c_float, _sz,
                               ctypes.c_float]
     L.or_hier_new.restype = ctypes.c_void_p
+    L.or_hier_new_opts.argtypes = [_sz, _sz, ctypes.c_int, _f64p, _f64p, _f64p, ctypes.c_float,
+                                   _sz, ctypes.c_float, ctypes.c_int, ctypes.c_int]
+    L.or_hier_new_opts.restype = ctypes.c_void_p
+    L.or_bil_prolong.argtypes = [_sz, _sz, _sz, _sz, ctypes.c_int, _f32p, _f32p]
+    L.or_bil_restrict.argtypes = [_sz, _sz, ctypes.c_int, _sz, _sz, ctypes.c_int, _f32p, _f32p]
+    L.or_coarse_exact.argtypes = [_sz, _sz, ctypes.c_int, _f32p, _f32p, _f32p, _f32p, _f32p]
     L.or_hier_free.argtypes = [ctypes.c_void_p]
     L.or_hier_levels.argtypes = [ctypes.c_void_p]
     L.or_hier_levels.restype = _sz
@@ -200,10 +206,14 @@ def lod(w, h, dx, wrap, hgt, kd, dt):
 class Hierarchy:
     """`Hierarchy`, multigrid.rs:475-497 (f32 levels of the V-cycle)."""
 
-    def __init__(self, w, h, wrap, east, south, diag, omega=0.8, nu=2, scale=0.5):
+    def __init__(self, w, h, wrap, east, south, diag, omega=0.8, nu=2, scale=0.5, transfer=0,
+                 coarsest=0):
+        """transfer 1: bilinear prolongation and its transpose as the restriction; coarsest 1:
+        the exact f64 solve of the coarsest level (options with no reference counterpart)."""
         self.w, self.h = w, h
         self._keep = (_c(east), _c(south), _c(diag))
-        self.ptr = lib().or_hier_new(w, h, int(wrap), *self._keep, omega, nu, scale)
+        self.ptr = lib().or_hier_new_opts(w, h, int(wrap), *self._keep, omega, nu, scale,
+                                          int(transfer), int(coarsest))
 
     def __del__(self):
         if getattr(self, "ptr", None):
@@ -231,6 +241,28 @@ class Hierarchy:
         z = np.zeros(self.w * self.h, np.float32)
         rz = lib().or_hier_precondition(self.ptr, _c(r32, np.float32), _c(r64), z)
         return z, rz
+
+
+def bil_prolong(wf, hf, wc, hc, cwrap, zc):
+    """The cell-centred bilinear prolongation P zc onto a wf x hf level (an option of the cycle)."""
+    out = np.zeros(wf * hf, np.float32)
+    lib().or_bil_prolong(wf, hf, wc, hc, int(cwrap), _c(zc, np.float32), out)
+    return out
+
+
+def bil_restrict(wf, hf, wrapf, wc, hc, cwrap, res):
+    """P^T res onto the wc x hc level."""
+    out = np.zeros(wc * hc, np.float32)
+    lib().or_bil_restrict(wf, hf, int(wrapf), wc, hc, int(cwrap), _c(res, np.float32), out)
+    return out
+
+
+def coarse_exact(w, h, wrap, east, south, diag, r):
+    """The exact f64 solve of a level's f32 operator (the cycle's exact-coarsest option)."""
+    z = np.zeros(w * h, np.float32)
+    lib().or_coarse_exact(w, h, int(wrap), _c(east, np.float32), _c(south, np.float32),
+                          _c(diag, np.float32), _c(r, np.float32), z)
+    return z
 
 
 # ---- solves --------------------------------------------------------------------------------
