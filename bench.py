@@ -364,6 +364,8 @@ def extra_configs(args, d2, h02, kd2, dt2, h0d2, kdd2, ms_f32, xo2):
     # The learned first guess on the headline inputs.
     out["lognormal-8192x4096-learned"] = learned_line(d2, h02, kd2, dt2, h0d2, kdd2, f64["ms"],
                                                       timed, xo2)
+    out["lognormal-8192x4096-learned-coarse"] = learned_line(d2, h02, kd2, dt2, h0d2, kdd2,
+                                                             f64["ms"], timed, xo2, coarse=True)
     # 16384 x 8192 constant K.
     gen, kw = F.WORKLOADS["constant-16384x8192"]
     d, h0, kd, dt = gen(**kw)
@@ -382,8 +384,9 @@ def extra_configs(args, d2, h02, kd2, dt2, h0d2, kdd2, ms_f32, xo2):
     return out
 
 
-def learned_line(d, h0, kd, dt, h0d, kdd, ms_f64, timed, xo):
-    """The committed network's guess (splitting + U-Net correction) and the solve from it."""
+def learned_line(d, h0, kd, dt, h0d, kdd, ms_f64, timed, xo, coarse=False):
+    """The committed network's guess (splitting + U-Net correction) and the solve from it; coarse:
+    the learned coarse correction (the network on 8 x 8 block means, interpolated back)."""
     import numpy as np
 
     import dtdiff
@@ -397,7 +400,8 @@ def learned_line(d, h0, kd, dt, h0d, kdd, ms_f64, timed, xo):
     hd, gd = DeviceArray(d.cells()), DeviceArray(d.cells())
     L = dtdiff.load()
     try:
-        net = LN.UNet(ws, LN.load_params())
+        net = (LN.CoarseGuess(ws, LN.load_params(LN.WEIGHTS_COARSE)) if coarse
+               else LN.UNet(ws, LN.load_params()))
 
         def step():
             hd.copy_from(h0d)
@@ -411,13 +415,15 @@ def learned_line(d, h0, kd, dt, h0d, kdd, ms_f64, timed, xo):
         ws.close()
         hd.free()
         gd.free()
-    r = {"grid": [d.w, d.h], "cycle": "f32", "guess": "splitting + U-Net", "ms": ms,
+    r = {"grid": [d.w, d.h], "cycle": "f32",
+         "guess": ("splitting + U-Net coarse correction (1/8 resolution)" if coarse
+                   else "splitting + U-Net"), "ms": ms,
          "unknowns_per_s": d.cells() / ms * 1e3, "iterations": int(st.iterations),
          "rel_residual_true": O.rel_residual(d.w, d.h, d.dx, d.wrap_x, kd, dt, h0, x),
          "vs_plain_f64": ms_f64 / ms}
     if xo is not None:
         r["parity_rel_l2_vs_oracle"] = float(np.linalg.norm(x - xo) / np.linalg.norm(xo))
-    log(f"extra learned: {ms:.2f} ms, {st.iterations} its")
+    log(f"extra learned{' coarse' if coarse else ''}: {ms:.2f} ms, {st.iterations} its")
     return r
 
 
@@ -725,6 +731,7 @@ def bench_learned(args, gen, kw):
                         DeviceArray(n))
     ws = dtdiff.Workspace(device=0)
     net = LN.UNet(ws, LN.load_params())
+    cnet = LN.CoarseGuess(ws, LN.load_params(LN.WEIGHTS_COARSE))
     L = dtdiff.load()
 
     def plain():
@@ -744,16 +751,30 @@ def bench_learned(args, gen, kw):
         st = dtdiff.solve_rel_guess_device(ws, d, hd, kdd, dt, args.rel, gd)
         return st, time.perf_counter() - t0, t1 - t0
 
+    def coarse():
+        hd.copy_from(h0d)
+        synchronize()
+        t0 = time.perf_counter()
+        cnet.guess(d, hd, kdd, dt, gd)
+        ws.check(L.dtd_sync(ws._ctx))
+        t1 = time.perf_counter()
+        st = dtdiff.solve_rel_guess_device(ws, d, hd, kdd, dt, args.rel, gd)
+        return st, time.perf_counter() - t0, t1 - t0
+
     for _ in range(args.warmup):
         plain()
         learned()
-    tp, tl, tg = 0.0, 0.0, 0.0
+        coarse()
+    tp, tl, tg, tc, tcg = 0.0, 0.0, 0.0, 0.0, 0.0
     for _ in range(args.steps):
         sp, e = plain()
         tp += e
         sl, e, g = learned()
         tl += e
         tg += g
+        sc, e, g = coarse()
+        tc += e
+        tcg += g
     x_ref = None
     # Accuracy of the two guesses against the converged answer.
     plain()
@@ -765,6 +786,9 @@ def bench_learned(args, gen, kw):
     net.guess(d, h0d, kdd, dt, gd)
     ws.check(L.dtd_sync(ws._ctx))
     err_net = float(np.sqrt(np.mean((gd.numpy() - x_ref) ** 2)))
+    cnet.guess(d, h0d, kdd, dt, gd)
+    ws.check(L.dtd_sync(ws._ctx))
+    err_coarse = float(np.sqrt(np.mean((gd.numpy() - x_ref) ** 2)))
     # The network's forward alone as the guess runs it (the convolutions on the matrix cores, the
     # decoder reading its skip concatenations in place), and its flops.
     net.set_input(d, h0d, kdd, gd, dt)
@@ -782,8 +806,10 @@ def bench_learned(args, gen, kw):
     tflops = flops / (fwd_ms * 1e-3) / 1e12
     log(f"plain {tp * 1e3 / args.steps:.2f} ms ({sp.iterations} its), learned "
         f"{tl * 1e3 / args.steps:.2f} ms ({sl.iterations} its, guess {tg * 1e3 / args.steps:.2f} "
-        f"ms), network forward {fwd_ms:.1f} ms = {tflops:.0f} TFLOP/s; guess RMS error: "
-        f"splitting {err_lod:.3f} m, network {err_net:.3f} m")
+        f"ms), coarse correction {tc * 1e3 / args.steps:.2f} ms ({sc.iterations} its, guess "
+        f"{tcg * 1e3 / args.steps:.2f} ms), network forward {fwd_ms:.1f} ms = {tflops:.0f} "
+        f"TFLOP/s; guess RMS error: splitting {err_lod:.3f} m, network {err_net:.3f} m, coarse "
+        f"correction {err_coarse:.3f} m")
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -822,8 +848,13 @@ def bench_learned(args, gen, kw):
         "iterations_learned": int(sl.iterations),
         "guess_ms": tg * 1e3 / args.steps,
         "network_forward_ms": fwd_ms,
-        "guess_rms_error_m": {"splitting": err_lod, "network": err_net},
-        "rel_residual": {"plain": sp.rel_residual, "learned": sl.rel_residual},
+        "guess_rms_error_m": {"splitting": err_lod, "network": err_net,
+                              "coarse_correction": err_coarse},
+        "rel_residual": {"plain": sp.rel_residual, "learned": sl.rel_residual,
+                         "coarse_correction": sc.rel_residual},
+        "coarse_correction": {"ms": tc * 1e3 / args.steps, "guess_ms": tcg * 1e3 / args.steps,
+                              "iterations": int(sc.iterations),
+                              "unknowns_per_s": n * args.steps / tc},
     }
     print(json.dumps(out), flush=True)
 

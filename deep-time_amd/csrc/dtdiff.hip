@@ -1775,6 +1775,36 @@ int dtd_nn_guess(dtd_ctx *ctx, size_t np, const double *base, const float *out, 
     });
 }
 
+int dtd_nn_block_mean(dtd_ctx *ctx, size_t w, size_t h, int f, int log_mean, const double *in,
+                      double *out) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        check_grid(w, h);
+        if (f < 1 || w % f || h % f) throw Error{DTD_EINVAL, "w and h must be multiples of f"};
+        if (!in || !out) throw Error{DTD_EINVAL, "null array"};
+        const long nc = (long)(w / f) * (long)(h / f);
+        if (nc == 0) return;
+        nn::k_block_mean<<<nn::blocks(nc), 256, 0, ctx->stream>>>((int)w, (int)h, f, log_mean, in,
+                                                                   out);
+        LAUNCH_CHECK();
+    });
+}
+
+int dtd_nn_guess_coarse(dtd_ctx *ctx, size_t w, size_t h, int f, const double *base,
+                        const float *out, int outc, double scale, double *g) {
+    return guard(ctx, [&] {
+        begin(ctx);
+        check_grid(w, h);
+        if (f < 1 || w % f || h % f) throw Error{DTD_EINVAL, "w and h must be multiples of f"};
+        if (!base || !out || !g) throw Error{DTD_EINVAL, "null array"};
+        if (outc < 1) throw Error{DTD_EINVAL, "outc must be at least 1"};
+        if (w * h == 0) return;
+        nn::k_nn_guess_coarse<<<nn::blocks((long)(w * h)), 256, 0, ctx->stream>>>(
+            (int)w, (int)h, f, base, out, outc, scale, g);
+        LAUNCH_CHECK();
+    });
+}
+
 int dtd_lod_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, const double *h_dev,
                    const double *kd_dev, double dt, double *out_dev) {
     return guard(ctx, [&] {

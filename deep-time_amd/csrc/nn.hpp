@@ -562,6 +562,49 @@ __global__ __launch_bounds__(256) void k_nn_guess(long np, const double *__restr
     if (p < np) g[p] = base[p] + scale * (double)out[p * outc];
 }
 
+// ---- the learned coarse correction: the network at 1/f of the grid's resolution -------------
+
+// Block means: out[J (w/f) + I] = the mean over the f x f fine cells of block (I, J) of in, the
+// cells added in row-major order from 0 and the sum divided by f^2; with log_mean, exp of the mean
+// of log(max(in, 1e-30)) (for kd: the block's geometric mean).  One thread a coarse cell.
+__global__ __launch_bounds__(256) void k_block_mean(int w, int h, int f, int log_mean,
+                                                    const double *__restrict__ in,
+                                                    double *__restrict__ out) {
+    const int wc = w / f, hc = h / f;
+    const long k = (long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= (long)wc * hc) return;
+    const int J = (int)(k / wc), I = (int)(k - (long)J * wc);
+    double acc = 0.0;
+    for (int j = J * f; j < J * f + f; j++) {
+        const double *row = in + (size_t)j * w + (size_t)I * f;
+        for (int i = 0; i < f; i++) acc += log_mean ? log(fmax(row[i], 1e-30)) : row[i];
+    }
+    const double m = acc / (double)(f * f);
+    out[k] = log_mean ? exp(m) : m;
+}
+
+// g = base + scale (P out[:, 0]): P interpolates bilinearly between the coarse cells' centres
+// (coarse cell I centred on fine position f I + (f - 1) / 2), periodic in x, constant past the
+// first and last rows' centres.
+__global__ __launch_bounds__(256) void k_nn_guess_coarse(int w, int h, int f,
+                                                         const double *__restrict__ base,
+                                                         const float *__restrict__ out, int outc,
+                                                         double scale, double *__restrict__ g) {
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= (long)w * h) return;
+    const int wc = w / f, hc = h / f;
+    const int j = (int)(p / w), i = (int)(p - (long)j * w);
+    const float X = ((float)i + 0.5f) / (float)f - 0.5f, Y = ((float)j + 0.5f) / (float)f - 0.5f;
+    const float fx = floorf(X), fy = floorf(Y);
+    const float tx = X - fx, ty = Y - fy;
+    const int i0 = (((int)fx % wc) + wc) % wc, i1 = (i0 + 1) % wc;
+    const int j0 = min(max((int)fy, 0), hc - 1), j1 = min(max((int)fy + 1, 0), hc - 1);
+    auto o = [&](int jj, int ii) { return out[((long)jj * wc + ii) * outc]; };
+    const float a = (1.0f - tx) * o(j0, i0) + tx * o(j0, i1);
+    const float b = (1.0f - tx) * o(j1, i0) + tx * o(j1, i1);
+    g[p] = base[p] + scale * (double)((1.0f - ty) * a + ty * b);
+}
+
 // Loss against the target t: the network's out[p][0] against (t - base) / scale (formed in f64);
 // parts[block] = sum of the squared errors over the block (a fixed tree), dy[p][0] =
 // 2 (out - target) / np, the other 15 channels 0.

@@ -52,7 +52,7 @@ SYMBOLS = (
     "dtd_nn_up_cat", "dtd_nn_up_cat_bwd", "dtd_nn_input", "dtd_nn_loss", "dtd_nn_guess",
     "dtd_nn_adam", "dtd_nn_to_bf16", "dtd_sync", "dtd_solve_rel_guess_device", "dtd_lod_device",
     "dtd_inject_fault", "dtd_solve_device_noalloc", "dtd_tune", "dtd_tune_plan",
-    "dtd_tune_batch",
+    "dtd_tune_batch", "dtd_nn_block_mean", "dtd_nn_guess_coarse",
 )
 # The stage's slope failure constants (crates/deeptime-core/src/surface/marine.rs:231-232).
 FAILURE_RATE = 0.05
@@ -211,6 +211,8 @@ def load(path: str = LIB_PATH):
                               ctypes.POINTER(ctypes.c_double)]
     L.dtd_nn_guess.argtypes = [_vp, _sz, _vp, _vp, _i, _d, _vp]
     L.dtd_lod_device.argtypes = [_vp, _sz, _sz, _d, _i, _vp, _vp, _d, _vp]
+    L.dtd_nn_block_mean.argtypes = [_vp, _sz, _sz, _i, _i, _vp, _vp]
+    L.dtd_nn_guess_coarse.argtypes = [_vp, _sz, _sz, _i, _vp, _vp, _i, _d, _vp]
     L.dtd_nn_adam.argtypes = [_vp, _sz, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _i, _vp]
     L.dtd_nn_to_bf16.argtypes = [_vp, _sz, _vp, _vp]
     L.dtd_sync.argtypes = [_vp]

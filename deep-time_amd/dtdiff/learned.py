@@ -36,6 +36,10 @@ LAYERS = [
 ]
 OUTC = 2
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "unet_guess.npz")
+# The learned coarse correction: the same network on f x f block means (f = COARSE).
+COARSE = 8
+WEIGHTS_COARSE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                              "unet_coarse.npz")
 
 
 def offsets():
@@ -294,6 +298,58 @@ class UNet:
         return self.g.numpy()
 
 
+class CoarseGuess:
+    """The learned coarse correction of the first guess: the network at 1/f of the grid's
+    resolution.  Its inputs are the f x f block means of h0, kd (geometric) and the splitting (the
+    diffusion number's channel still in fine cells, so the network sees the same physics at any
+    grid size); it predicts the block means of (h1 - splitting) / SCALE, which are interpolated
+    bilinearly between the blocks' centres and added to the splitting in f64.  At 8192 x 4096 the
+    network runs on 1024 x 512 pixels (1/64 of the full-resolution guess's work).  Needs w a
+    multiple of 128 f and h of 8 f."""
+
+    def __init__(self, ws: Workspace, params: np.ndarray | None = None, f: int = COARSE):
+        self.ws, self.f, self.L = ws, f, load()
+        self.net = UNet(ws, params)
+        self.shape = None
+
+    def _alloc(self, d: Dims):
+        if self.shape == (d.w, d.h):
+            return
+        if d.w % (128 * self.f) or d.h % (8 * self.f):
+            raise ValueError(f"the coarse correction needs w a multiple of {128 * self.f} and h "
+                             f"of {8 * self.f}")
+        self.shape = (d.w, d.h)
+        nc = (d.w // self.f) * (d.h // self.f)
+        self.dc = Dims(d.w // self.f, d.h // self.f, d.dx, d.wrap_x)
+        self.hc, self.kc, self.bc, self.tc = (DeviceArray(nc) for _ in range(4))
+
+    def _mean(self, d: Dims, src, dst, log_mean: int = 0):
+        self.ws.check(self.L.dtd_nn_block_mean(self.ws._ctx, d.w, d.h, self.f, log_mean,
+                                               UNet._p(src), UNet._p(dst)))
+
+    def set_input(self, d: Dims, h_dev, kd_dev, base_dev, dt: float):
+        self._alloc(d)
+        self._mean(d, h_dev, self.hc)
+        self._mean(d, kd_dev, self.kc, 1)
+        self._mean(d, base_dev, self.bc)
+        self.net.set_input(Dims(self.dc.w, self.dc.h, d.dx, d.wrap_x), self.hc, self.kc,
+                           self.bc, dt)
+
+    def loss(self, d: Dims, t_dev):
+        """Against the block means of the solve t (the splitting's already in self.bc)."""
+        self._mean(d, t_dev, self.tc)
+        return self.net.loss(self.tc, self.bc)
+
+    def guess(self, d: Dims, h_dev, kd_dev, dt: float, g_dev) -> None:
+        """g = the splitting + the interpolated coarse correction, on the device."""
+        lod_device(self.ws, d, h_dev, kd_dev, dt, g_dev)
+        self.set_input(d, h_dev, kd_dev, g_dev, dt)
+        self.net.forward(train=False)
+        self.ws.check(self.L.dtd_nn_guess_coarse(self.ws._ctx, d.w, d.h, self.f,
+                                                 UNet._p(g_dev), UNet._p(self.net.B["out32"]),
+                                                 OUTC, SCALE, UNet._p(g_dev)))
+
+
 def _bias_mask() -> np.ndarray:
     offs, n = offsets()
     m = np.zeros(n, dtype=bool)
@@ -399,6 +455,63 @@ def train(steps: int, out: str, lr: float = 1e-3, w: int = 1024, h: int = 512, p
     return params
 
 
+def train_coarse(steps: int, out: str, lr: float = 1e-3, w: int = 2048, h: int = 1024,
+                 pool: int = 32, log_every: int = 100, params: np.ndarray | None = None,
+                 seed: int = 11) -> np.ndarray:
+    """The coarse correction's network on the multigrid's own solves at w x h (the network at
+    w / 8 x h / 8), loss on the block means of the splitting's error."""
+    ws = Workspace()
+    S = Samples(ws, w, h, pool, seed)
+    cg = CoarseGuess(ws, params)
+    t0 = time.perf_counter()
+    run = 0.0
+    for k in range(1, steps + 1):
+        ratio, _ = S.draw()
+        cg.set_input(S.d, S.h_dev, S.k_dev, S.b_dev, ratio)
+        cg.net.forward()
+        loss, dy = cg.loss(S.d, S.t_dev)
+        cg.net.backward(dy)
+        cg.net.adam(lr * (0.05 + 0.95 * 0.5 * (1.0 + np.cos(np.pi * k / steps))))
+        run = loss if k == 1 else 0.98 * run + 0.02 * loss
+        if k % log_every == 0 or k == 1:
+            print(f"step {k:6d}  loss {loss:.3e}  (avg {run:.3e})  "
+                  f"{time.perf_counter() - t0:7.1f} s", flush=True)
+    params = cg.net.params()
+    save(out, params, steps=steps, lr=lr, w=w, h=h, f=COARSE, final_loss=run)
+    return params
+
+
+def evaluate_coarse(params: np.ndarray, count: int = 8, w: int = 2048, h: int = 1024,
+                    seed: int = 99, rel_tol: float = 1e-10) -> list[dict]:
+    """Held-out samples: iterations and guess errors from the splitting and from the splitting
+    with the learned coarse correction, each solve to the same true relative residual."""
+    ws = Workspace()
+    S = Samples(ws, w, h, pool=max(2, count // 2), seed=seed)
+    cg = CoarseGuess(ws, params)
+    rows = []
+    g = DeviceArray(w * h)
+    for k in range(count):
+        ratio, _ = S.draw()
+        x = DeviceArray.from_numpy(S.h_dev.numpy())
+        plain = solve_rel_device(ws, S.d, x, S.k_dev, ratio, rel_tol)
+        x.upload(S.h_dev.numpy())
+        ws.check(load().dtd_sync(ws._ctx))
+        t0 = time.perf_counter()
+        cg.guess(S.d, x, S.k_dev, ratio, g)
+        ws.check(load().dtd_sync(ws._ctx))
+        ms = 1e3 * (time.perf_counter() - t0)
+        st = solve_rel_guess_device(ws, S.d, x, S.k_dev, ratio, rel_tol, g)
+        synchronize()
+        t, gg, b = S.t_dev.numpy(), g.numpy(), S.b_dev.numpy()
+        rows.append({"ratio": ratio, "iterations_lod": plain.iterations,
+                     "iterations_coarse": st.iterations, "rel_lod": plain.rel_residual,
+                     "rel_coarse": st.rel_residual,
+                     "err_lod_m": float(np.sqrt(np.mean((b - t) ** 2))),
+                     "err_coarse_m": float(np.sqrt(np.mean((gg - t) ** 2))), "guess_ms": ms})
+    ws.close()
+    return rows
+
+
 def evaluate(params: np.ndarray, count: int = 8, w: int = 1024, h: int = 512, seed: int = 99,
              rel_tol: float = 1e-10) -> list[dict]:
     """Held-out samples (fields from seeds outside the training pools): the CG iterations from the
@@ -437,14 +550,27 @@ def main():
     t.add_argument("--pool", type=int, default=32)
     t.add_argument("--init", default=None, help="start from these weights (fine-tuning)")
     t.add_argument("--seed", type=int, default=7, help="the training fields' pool seed")
+    t.add_argument("--coarse", action="store_true",
+                   help="the coarse correction's network (default size 2048 x 1024)")
     e = sub.add_parser("eval")
-    e.add_argument("--weights", default=WEIGHTS)
+    e.add_argument("--weights", default=None)
     e.add_argument("--count", type=int, default=8)
+    e.add_argument("--coarse", action="store_true")
     a = ap.parse_args()
     if a.cmd == "train":
-        train(a.steps, a.out, a.lr, a.w, a.h, a.pool,
-              params=load_params(a.init) if a.init else None, seed=a.seed)
+        init = load_params(a.init) if a.init else None
+        if a.coarse:
+            w, h = (a.w, a.h) if (a.w, a.h) != (1024, 512) else (2048, 1024)
+            train_coarse(a.steps, a.out if a.out != WEIGHTS else WEIGHTS_COARSE, a.lr, w, h,
+                         a.pool, params=init, seed=a.seed)
+        else:
+            train(a.steps, a.out, a.lr, a.w, a.h, a.pool, params=init, seed=a.seed)
+    elif a.coarse:
+        for r in evaluate_coarse(load_params(a.weights or WEIGHTS_COARSE), a.count):
+            print(" ".join(f"{k} {v:.4g}" if isinstance(v, float) else f"{k} {v}"
+                           for k, v in r.items()), flush=True)
     else:
+        a.weights = a.weights or WEIGHTS
         for r in evaluate(load_params(a.weights), a.count):
             print(" ".join(f"{k} {v:.4g}" if isinstance(v, float) else f"{k} {v}"
                            for k, v in r.items()), flush=True)

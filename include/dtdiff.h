@@ -420,6 +420,18 @@ int dtd_nn_to_bf16(dtd_ctx *ctx, size_t n, const float *a, uint16_t *b);
 /* Waits for the context's stream. */
 int dtd_sync(dtd_ctx *ctx);
 
+/* The learned coarse correction: the network runs at 1/f of the grid's resolution (f x f blocks)
+ * and predicts the block means of the splitting's error; these two calls move fields between the
+ * resolutions.  dtd_nn_block_mean: out (w/f x h/f) = each block's mean of in, its cells added in
+ * row-major order; log_mean: exp of the mean of log(max(in, 1e-30)) (kd's geometric mean).
+ * dtd_nn_guess_coarse: g = base + scale * (P out[:, 0]), P the bilinear interpolation between the
+ * coarse cells' centres (periodic in x, constant past the outer rows' centres), out holding outc
+ * f32 channels a coarse pixel.  w and h must be multiples of f; asynchronous on the stream. */
+int dtd_nn_block_mean(dtd_ctx *ctx, size_t w, size_t h, int f, int log_mean, const double *in,
+                      double *out);
+int dtd_nn_guess_coarse(dtd_ctx *ctx, size_t w, size_t h, int f, const double *base,
+                        const float *out, int outc, double scale, double *g);
+
 /* dtd_solve_rel_device started from the guess x0_dev (device, w h f64) instead of the
  * splitting. */
 int dtd_solve_rel_guess_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x,

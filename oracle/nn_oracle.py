@@ -221,3 +221,32 @@ def input_channels(h, kd, dt, dx, scale, base=None):
     if base is not None:
         x[..., 3] = base / scale
     return x
+
+
+# ---- the learned coarse correction's transfers ------------------------------------------------
+
+def block_mean(a, w, h, f, log_mean=False):
+    """The f x f blocks' means of a (w x h, row-major); log_mean: exp of the mean of
+    log(max(a, 1e-30)), the geometric mean.  f64."""
+    x = np.asarray(a, dtype=np.float64).reshape(h // f, f, w // f, f)
+    if log_mean:
+        return np.exp(np.log(np.maximum(x, 1e-30)).mean(axis=(1, 3))).reshape(-1)
+    return x.mean(axis=(1, 3)).reshape(-1)
+
+
+def interp_centres(c, w, h, f):
+    """Bilinear interpolation onto the fine w x h cells of the coarse values c (w/f x h/f) placed
+    at the blocks' centres: periodic in x, constant past the first and last rows' centres."""
+    wc, hc = w // f, h // f
+    c = np.asarray(c, dtype=np.float64).reshape(hc, wc)
+    X = (np.arange(w) + 0.5) / f - 0.5
+    Y = (np.arange(h) + 0.5) / f - 0.5
+    x0 = np.floor(X).astype(int)
+    tx = X - x0
+    i0, i1 = x0 % wc, (x0 + 1) % wc
+    y0 = np.floor(Y).astype(int)
+    ty = (Y - y0)[:, None]
+    j0, j1 = np.clip(y0, 0, hc - 1), np.clip(y0 + 1, 0, hc - 1)
+    a = (1 - tx) * c[j0][:, i0] + tx * c[j0][:, i1]
+    b = (1 - tx) * c[j1][:, i0] + tx * c[j1][:, i1]
+    return ((1 - ty) * a + ty * b).reshape(-1)

@@ -106,6 +106,38 @@ def test_lognormal_8192x4096_learned_guess(lognormal):
     assert st.iterations <= plain.iterations
 
 
+def test_lognormal_8192x4096_learned_coarse_correction(lognormal):
+    """The committed coarse-correction network (on the 1024x512 block means): the same stop, no
+    more iterations than from the splitting, a closer first guess."""
+    from dtdiff import learned as LN
+
+    d, h0, kd, dt, xo, its_o = lognormal
+    ws = dtdiff.Workspace()
+    try:
+        hd, kdd, gd = (DeviceArray.from_numpy(h0), DeviceArray.from_numpy(kd),
+                       DeviceArray(d.cells()))
+        plain = dtdiff.solve_rel_device(ws, d, hd, kdd, dt, REL)
+        hd.copy_from(DeviceArray.from_numpy(h0))
+        cg = LN.CoarseGuess(ws, LN.load_params(LN.WEIGHTS_COARSE))
+        cg.guess(d, hd, kdd, dt, gd)  # reads hd (h0), writes the guess into gd
+        ws.check(dtdiff.load().dtd_sync(ws._ctx))
+        g = gd.numpy()
+        st = dtdiff.solve_rel_guess_device(ws, d, hd, kdd, dt, REL, gd)
+        synchronize()
+        x = hd.numpy()
+        split = DeviceArray(d.cells())
+        dtdiff.lod_device(ws, d, DeviceArray.from_numpy(h0), kdd, dt, split)
+        synchronize()
+        s = split.numpy()
+    finally:
+        ws.close()
+    check("lognormal 8192x4096 coarse correction", d, h0, kd, dt, x, xo, False)
+    err_s, err_g = float(np.sqrt(np.mean((s - xo) ** 2))), float(np.sqrt(np.mean((g - xo) ** 2)))
+    print(f"coarse correction: {st.iterations} iterations, splitting {plain.iterations}; guess "
+          f"error {err_g:.2f} m vs {err_s:.2f} m", flush=True)
+    assert st.iterations <= plain.iterations and err_g < err_s
+
+
 def test_constant_16384x8192_f32_and_bf16_cycles():
     d, h0, kd, dt = F.WORKLOADS["constant-16384x8192"][0](**F.WORKLOADS["constant-16384x8192"][1])
     assert (d.w, d.h) == (16384, 8192)

@@ -389,3 +389,44 @@ def test_network_arguments_are_checked(ws):
     g = DeviceArray(64, np.float64)
     assert "outc" in err(L.dtd_nn_guess(ws._ctx, 64, p(g), p(f), 0, 1.0, p(g)))
     assert "outc" in err(L.dtd_nn_guess(ws._ctx, 64, p(g), p(f), -3, 1.0, p(g)))
+
+
+@pytest.mark.parametrize("w,h,f", [(2048, 1024, 8), (256, 64, 8), (96, 48, 4), (32, 16, 1)])
+def test_coarse_correction_transfers_match_the_oracle(ws, w, h, f):
+    """The learned coarse correction's block means (plain and geometric) and its bilinear
+    interpolation back onto the grid, against the oracle."""
+    rng = np.random.default_rng(w + h + f)
+    a = rng.standard_normal(w * h) * 100.0
+    k = np.exp(rng.standard_normal(w * h))
+    L = dtdiff.load()
+    nc = (w // f) * (h // f)
+    ad, kd_, out = DeviceArray.from_numpy(a), DeviceArray.from_numpy(k), DeviceArray(nc)
+    # The calls are asynchronous on the context's stream: wait before reading.
+    ws.check(L.dtd_nn_block_mean(ws._ctx, w, h, f, 0, p(ad), p(out)))
+    assert np.allclose(get_f32(out, (nc,)), N.block_mean(a, w, h, f), rtol=0, atol=1e-12 * 100)
+    ws.check(L.dtd_nn_block_mean(ws._ctx, w, h, f, 1, p(kd_), p(out)))
+    assert np.allclose(get_f32(out, (nc,)), N.block_mean(k, w, h, f, True), rtol=1e-13, atol=0)
+    c = rng.standard_normal(nc).astype(np.float32)
+    c2 = np.stack([c, np.zeros_like(c)], axis=1).reshape(-1)  # two channels, the first read
+    base = rng.standard_normal(w * h)
+    cd, bd, g = DeviceArray.from_numpy(c2), DeviceArray.from_numpy(base), DeviceArray(w * h)
+    ws.check(L.dtd_nn_guess_coarse(ws._ctx, w, h, f, p(bd), p(cd), 2, 1000.0, p(g)))
+    want = base + 1000.0 * N.interp_centres(c, w, h, f)
+    assert np.allclose(get_f32(g, (w * h,)), want, rtol=0, atol=1000.0 * 2e-6 * np.abs(c).max())
+    if f > 1:  # a grid that is not whole blocks is refused
+        assert L.dtd_nn_block_mean(ws._ctx, w + 1, h, f, 0, p(ad), p(out)) != 0
+        assert "multiples of f" in L.dtd_last_error(ws._ctx).decode()
+
+
+def test_trained_coarse_correction_gains_over_the_splitting(ws):
+    """The committed coarse-correction network on held-out fields at 2048x1024 (the network on
+    256x128 block means): a closer first guess than the splitting on every sample, no more CG
+    iterations, the same true-residual stop."""
+    from dtdiff import learned as LN
+
+    rows = LN.evaluate_coarse(LN.load_params(LN.WEIGHTS_COARSE), count=4)
+    for r in rows:
+        print(r, flush=True)
+        assert r["err_coarse_m"] < 0.85 * r["err_lod_m"]
+        assert r["iterations_coarse"] <= r["iterations_lod"]
+        assert r["rel_coarse"] <= 1e-10 and r["rel_lod"] <= 1e-10

@@ -99,3 +99,31 @@ def test_input_channels():
     assert not x[..., 3:].any()
     x = N.input_channels(h, kd, 10.0, 1.0, 1000.0, base=np.array([[250.0, 0.0]]))
     assert np.allclose(x[..., 3], [[0.25, 0.0]]) and not x[..., 4:].any()
+
+
+def test_coarse_transfers_reproduce_constants_and_linear_fields():
+    """The learned coarse correction's block means and its interpolation back: a linear field's
+    block mean is its value at the block's centre; interpolating the centres' values of a linear
+    field gives the field itself between the first and last centres (in x away from the wrap, in
+    y inside the outer centres), a constant everywhere, and f = 1 is the identity."""
+    w, h, f = 64, 32, 8
+    j, i = np.meshgrid(np.arange(h) + 0.5, np.arange(w) + 0.5, indexing="ij")
+    lin = (0.3 * i - 0.7 * j + 2.0).reshape(-1)
+    bm = N.block_mean(lin, w, h, f)
+    J, I = np.meshgrid(np.arange(h // f) * f + f / 2, np.arange(w // f) * f + f / 2, indexing="ij")
+    assert np.allclose(bm, (0.3 * I - 0.7 * J + 2.0).reshape(-1), rtol=0, atol=1e-12)
+    assert np.allclose(N.block_mean(np.exp(lin / 10), w, h, f, True),
+                       np.exp((0.3 * I - 0.7 * J + 2.0).reshape(-1) / 10), rtol=1e-12)
+    back = N.interp_centres(bm, w, h, f).reshape(h, w)
+    inner = (slice(f // 2, h - f // 2), slice(f // 2, w - f // 2))
+    assert np.allclose(back[inner], lin.reshape(h, w)[inner], rtol=0, atol=1e-12)
+    assert np.allclose(N.interp_centres(np.full((h // f) * (w // f), 4.0), w, h, f), 4.0)
+    x = np.random.default_rng(3).standard_normal(w * h)
+    assert np.array_equal(N.interp_centres(x, w, h, 1), x)
+    # Periodic in x: the columns past the last centre blend the last and the first centres.
+    c = np.zeros((h // f, w // f))
+    c[:, 0] = 1.0
+    row = N.interp_centres(c.reshape(-1), w, h, f).reshape(h, w)[h // 2]
+    # Column w-1 lies 1/2 - 1/(2f) of the way from the last centre to the first (wrapped), column
+    # 0 as far short of the first centre.
+    assert abs(row[w - 1] - (0.5 - 0.5 / f)) < 1e-12 and abs(row[0] - (0.5 + 0.5 / f)) < 1e-12
