@@ -178,7 +178,7 @@ def main():
             kernels[name] = {"ms": ms, "bytes": b, "GB/s": b / (ms * 1e-3) / 1e9}
             log(f"kernel {name}: {ms * 1e3:.1f} us, {b / 1e6:.1f} MB, "
                 f"{kernels[name]['GB/s']:.0f} GB/s")
-        roof = smoother_roofline(kernels, args.cycle == "bf16", w, h)
+        roof = smoother_roofline(kernels, args.cycle, w, h)
 
     cpu = None
     parity = None
@@ -243,23 +243,38 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def smoother_roofline(kernels, bf16, w, h):
+def kd_records(cycle):
+    """Whether the f32 cycle's finest smoothers form their records from kd (the library's
+    default; DTD_NO_KD=1 at workspace creation reads stored records instead).  The bf16-record
+    cycle always reads its stored records."""
+    v = os.environ.get("DTD_NO_KD", "")
+    return cycle == "f32" and not (v and v != "0")
+
+
+def smoother_roofline(kernels, cycle, w, h):
     """The finest level's fused smoothing, both kernels as the solve runs them: the
     pre-smoothing (the residual's update r -= alpha A p with r.r, two damped-Jacobi sweeps, the
     residual restricted 2x2) and the post-smoothing (the corrected sweep, the last sweep, r.z).
     Compulsory bytes of the two launches over their summed HIP-event durations."""
     pre, post = kernels["pre_smooth_f32"], kernels["post_smooth_f32"]
-    cb = int(bf16)
-    kp, kq = f"k_spre<float, 2, 2, {cb}>", f"k_spost<float, 2, 1, {cb}>"
+    if kd_records(cycle):
+        # kRUpdateK = 4, kR64K = 3: the records (and the pre-smoothing's A p) formed from kd.
+        kp, kq = "k_spre<float, 2, 4, 0>", "k_spost<float, 2, 3, 0>"
+        what = ("pre: r64, p and kd in (A p and the faces formed from them), r64, z and the "
+                "coarse r out, 37 B/cell; post: r64, z, kd and the coarse z in, z out, 25 B/cell")
+    else:
+        cb = int(cycle == "bf16")
+        rec = 8 if cb else 12
+        kp, kq = f"k_spre<float, 2, 2, {cb}>", f"k_spost<float, 2, 1, {cb}>"
+        what = (f"pre: r64, A p and the {rec}-byte record in, r64, z and the coarse r out, "
+                f"{28 + rec + 1} B/cell; post: r64, z, the record and the coarse z in, z out, "
+                f"{17 + rec} B/cell")
     achieved = (pre["bytes"] + post["bytes"]) / ((pre["ms"] + post["ms"]) * 1e-3) / 1e9
     tp, tq = traffic_of(kp, w, h), traffic_of(kq, w, h)
-    rec = 8 if cb else 12
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": tp + tq if tp is not None and tq is not None else None,
-            "kernel": f"{kp} + {kq}, the finest level's fused smoothing (pre: r64, A p and the "
-                      f"{rec}-byte record in, r64, z and the coarse r out, {28 + rec + 1} B/cell; "
-                      f"post: r64, z, the record and the coarse z in, z out, {17 + rec} B/cell), "
+            "kernel": f"{kp} + {kq}, the finest level's fused smoothing ({what}), "
                       f"{pre['ms'] * 1e3:.1f} + {post['ms'] * 1e3:.1f} us",
             "per_kernel": {kp: {"us": pre["ms"] * 1e3, "GB/s": pre["GB/s"],
                                 "frac": pre["GB/s"] / HBM_PEAK_GBS},

@@ -145,6 +145,9 @@ struct dtd_ctx {
     std::string err;
     size_t cycle_levels = 0;  // levels built by dtd_cycle_new
     bool use_graph = true;
+    // The finest level's fused smoothers form their records (and the pre-smoothing A p) from kd
+    // and p instead of reading them (DTD_NO_KD=1 reads them, for comparison).
+    bool use_kd = true;
     dtd::FaultHook fault;         // dtd_inject_fault
     std::map<dtd::PlanKey, dtd::Plan> plans;  // tuned plans by grid shape (dtd_tune)
     // Tuned band-height policies of the batched solve by shape list (dtd_tune_batch), and the one

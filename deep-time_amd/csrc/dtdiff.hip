@@ -243,7 +243,12 @@ __global__ void k_init_state(CgState *st, double tol2, double rel2, int max_iter
 
 // A state that never stops and takes every branch of the passes (beta and the pending update on,
 // both zero), for timing the loop's kernels on resident data without changing it (the faces from
-// the workspace's own kd buffer, whatever it holds).
+// the workspace's own kd buffer).
+__global__ void k_fill64(double *a, long n, double v) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
 __global__ void k_time_state(CgState *st, const double *kd) {
     if (threadIdx.x != 0) return;
     CgState s{};
@@ -308,7 +313,7 @@ static int launch_pre_cb(Lvl<T> &L, Lvl<T> &C, T *zout, const CgState *st, hipSt
                          R64 rio, double *parts, long cap) {
     const int th = L.th_pre > 0 ? L.th_pre : band_height<V>((const void *)k_spre<T, V, RM, CB>, L.w, L.h);
     const int nb = stream_grid<V, LaneGeom<V>::halo>(L.w, L.h, th);
-    if (RM == kRUpdate) need_parts(nb, cap);
+    if (rm_update(RM)) need_parts(nb, cap);
     k_spre<T, V, RM, CB><<<nb, kBlock, 0, s>>>(L, zout, C, th, st, rio, parts);
     LAUNCH_CHECK();
     return nb;
@@ -345,11 +350,11 @@ static int launch_post(Lvl<T> &L, const T *zin, const T *zc, int wc, T *zout, co
 // The streamed direction pass; returns its block count (the p.Ap partial sums).
 template <typename TZ>
 static int launch_dir(Workspace &ws, const TZ *z, const double *pold, double *pnew,
-                      hipStream_t s) {
+                      hipStream_t s, double *ap) {
     const int th = ws.th_dir > 0 ? ws.th_dir : band_height<1, 1>((const void *)k_sdir<TZ>, ws.w, ws.h);
     const int nb = stream_grid<1, 1>(ws.w, ws.h, th);
     need_parts(nb, ws.nparts);
-    k_sdir<TZ><<<nb, kBlock, 0, s>>>(grid_of(ws), z, pold, pnew, ws.ap, ws.x, th, ws.parts, ws.st);
+    k_sdir<TZ><<<nb, kBlock, 0, s>>>(grid_of(ws), z, pold, pnew, ap, ws.x, th, ws.parts, ws.st);
     LAUNCH_CHECK();
     return nb;
 }
@@ -441,18 +446,32 @@ static int vcycle(std::vector<Lvl<T>> &lv, size_t l, int nu, const double *dot, 
         if constexpr (std::is_same<T, float>::value) {
             if (rm != kRLevel) {
                 if (l != 0) throw Error{DTD_ESTATE, "the f64 residual feeds the finest level only"};
-                if (rm == kRUpdate) {
-                    const int nbp =
-                        wide ? launch_pre<T, VW, kRUpdate>(L, C0, L.z[0], st, s, rio, parts, cap)
-                             : launch_pre<T, 1, kRUpdate>(L, C0, L.z[0], st, s, rio, parts, cap);
+                if (rm_kd(rm) && L.coefh)
+                    throw Error{DTD_ESTATE, "kd-formed records with bf16 records"};
+                if (rm_update(rm)) {
+                    int nbp;
+                    if (rm == kRUpdateK)
+                        nbp = wide ? launch_pre<T, VW, kRUpdateK>(L, C0, L.z[0], st, s, rio, parts, cap)
+                                   : launch_pre<T, 1, kRUpdateK>(L, C0, L.z[0], st, s, rio, parts, cap);
+                    else
+                        nbp = wide ? launch_pre<T, VW, kRUpdate>(L, C0, L.z[0], st, s, rio, parts, cap)
+                                   : launch_pre<T, 1, kRUpdate>(L, C0, L.z[0], st, s, rio, parts, cap);
                     k_cg_step<kStepRR><<<1, kReduceBlock, 0, s>>>(parts, nbp, st);
                     LAUNCH_CHECK();
+                } else if (rm == kR64K) {
+                    wide ? launch_pre<T, VW, kR64K>(L, C0, L.z[0], st, s, rio)
+                         : launch_pre<T, 1, kR64K>(L, C0, L.z[0], st, s, rio);
                 } else if (wide) {
                     launch_pre<T, VW, kR64>(L, C0, L.z[0], st, s, rio);
                 } else {
                     launch_pre<T, 1, kR64>(L, C0, L.z[0], st, s, rio);
                 }
                 vcycle<T>(lv, l + 1, nu, nullptr, parts, cap, st, s, kRLevel, R64{}, dlv, bottom);
+                if (rm_kd(rm))
+                    return wide ? launch_post<T, VW, kR64K>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot,
+                                                             parts, cap, st, s)
+                                : launch_post<T, 1, kR64K>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot,
+                                                            parts, cap, st, s);
                 return wide ? launch_post<T, VW, kR64>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts,
                                                         cap, st, s)
                             : launch_post<T, 1, kR64>(L, L.z[0], C0.z[1], C0.w, L.z[1], dot, parts,
@@ -521,6 +540,12 @@ static bool fused_update(const Workspace &ws, const dtd_opts &o) {
            o.transfer == DTD_TRANSFER_AGGREGATE;
 }
 
+// Whether the fused finest level forms its records (and the pre-smoothing A p) from kd and p:
+// the f32 cycle without bf16 records on the finest level.
+static bool kd_records(const dtd_ctx *ctx) {
+    return ctx->use_kd && fused_update(ctx->ws, ctx->opts) && !ctx->ws.lf[0].coefh;
+}
+
 // One conjugate-gradient iteration (MG:630-733): parity 0 reads the direction from p1 and writes
 // p0 (odd iterations), parity 1 the other way.
 static void enqueue_iteration(dtd_ctx *ctx, int parity) {
@@ -528,15 +553,20 @@ static void enqueue_iteration(dtd_ctx *ctx, int parity) {
     hipStream_t s = ctx->stream;
     double *pold = parity == 0 ? ws.p1 : ws.p0;
     double *pnew = parity == 0 ? ws.p0 : ws.p1;
-    const int nbd = ws.prec == DTD_CYCLE_F64 ? launch_dir<double>(ws, ws.ld[0].z[1], pold, pnew, s)
-                                             : launch_dir<float>(ws, ws.lf[0].z[1], pold, pnew, s);
+    const bool kd = kd_records(ctx);
+    const int nbd = ws.prec == DTD_CYCLE_F64
+                        ? launch_dir<double>(ws, ws.ld[0].z[1], pold, pnew, s, ws.ap)
+                        : launch_dir<float>(ws, ws.lf[0].z[1], pold, pnew, s, kd ? nullptr : ws.ap);
     k_cg_step<kStepPAP><<<1, kReduceBlock, 0, s>>>(ws.parts, nbd, ws.st);
     LAUNCH_CHECK();
     if (fused_update(ws, ctx->opts)) {
         // r_new = r_old - alpha A p inside the finest pre-smoothing (r.r and the stop test after
         // it); the residual's buffers alternate with the parity: parity 0 reads r, writes r2.
+        // With kd-formed records the pre-smoothing forms A p from p itself.
         double *rin = parity == 0 ? ws.r : ws.r2, *rout = parity == 0 ? ws.r2 : ws.r;
-        const int nbz = run_vcycle(ctx, rout, ws.st, kRUpdate, R64{rin, ws.ap, rout, 0.0});
+        const int nbz = kd ? run_vcycle(ctx, rout, ws.st, kRUpdateK,
+                                        R64{rin, nullptr, rout, 0.0, pnew, nullptr, 0.0, 0.0})
+                           : run_vcycle(ctx, rout, ws.st, kRUpdate, R64{rin, ws.ap, rout, 0.0});
         k_cg_step<kStepRZ><<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.st,
                                                        ctx->dev_flag + parity);
         LAUNCH_CHECK();
@@ -668,7 +698,8 @@ static size_t solve_core(dtd_ctx *ctx, const double *b, const double *kd, double
     const int nbr = launch_resid(ws, kd, dt, dx * dx, b, xin, s, !fused);
     k_cg_step<kStepRR0><<<1, kReduceBlock, 0, s>>>(ws.parts, nbr, ws.st);
     LAUNCH_CHECK();
-    const int nbz = fused ? run_vcycle(ctx, ws.r, ws.st, kR64, R64{ws.r, nullptr, nullptr, 0.0})
+    const int nbz = fused ? run_vcycle(ctx, ws.r, ws.st, kd_records(ctx) ? kR64K : kR64,
+                                       R64{ws.r, nullptr, nullptr, 0.0})
                           : run_vcycle(ctx, ws.r, ws.st);
     k_cg_step<kStepRZ0><<<1, kReduceBlock, 0, s>>>(ws.parts, nbz, ws.st);
     LAUNCH_CHECK();
@@ -794,8 +825,15 @@ static void tune_levels(dtd_ctx *ctx, std::vector<Lvl<T>> &lv, int reps, int min
         if (l > 0 && L.n <= min_cells) break;  // from here the bottom block may take the levels
         const bool wide = L.w % VW == 0;
         const bool top = l == 0 && ws.prec != DTD_CYCLE_F64;
+        const bool kd = top && kd_records(ctx);
         auto pre = [&] {
             if constexpr (std::is_same<T, float>::value) {
+                if (kd) {
+                    const R64 rio{ws.r, nullptr, ws.r2, 0.0, ws.p0, nullptr, 0.0, 0.0};
+                    wide ? launch_pre<T, VW, kRUpdateK>(L, C, L.z[0], ws.st, s, rio, ws.parts, ws.nparts)
+                         : launch_pre<T, 1, kRUpdateK>(L, C, L.z[0], ws.st, s, rio, ws.parts, ws.nparts);
+                    return;
+                }
                 if (top) {
                     const R64 rio{ws.r, ws.ap, ws.r2, 0.0};
                     wide ? launch_pre<T, VW, kRUpdate>(L, C, L.z[0], ws.st, s, rio, ws.parts, ws.nparts)
@@ -807,6 +845,13 @@ static void tune_levels(dtd_ctx *ctx, std::vector<Lvl<T>> &lv, int reps, int min
         };
         auto post = [&] {
             if constexpr (std::is_same<T, float>::value) {
+                if (kd) {
+                    wide ? launch_post<T, VW, kR64K>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts,
+                                                     ws.nparts, ws.st, s)
+                         : launch_post<T, 1, kR64K>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts,
+                                                    ws.nparts, ws.st, s);
+                    return;
+                }
                 if (top) {
                     wide ? launch_post<T, VW, kR64>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts,
                                                     ws.nparts, ws.st, s)
@@ -846,8 +891,9 @@ static void tune(dtd_ctx *ctx, const double *b, const double *kd, double dx, dou
     const double s2 = dx * dx;
     const bool f64 = ws.prec == DTD_CYCLE_F64;
     ws.th_dir = best_th(ctx, ws.h, kreps, [&](int th) { ws.th_dir = th; }, [&] {
-        f64 ? launch_dir<double>(ws, ws.ld[0].z[1], ws.p1, ws.p0, s)
-            : launch_dir<float>(ws, ws.lf[0].z[1], ws.p1, ws.p0, s);
+        f64 ? launch_dir<double>(ws, ws.ld[0].z[1], ws.p1, ws.p0, s, ws.ap)
+            : launch_dir<float>(ws, ws.lf[0].z[1], ws.p1, ws.p0, s,
+                                kd_records(ctx) ? nullptr : ws.ap);
     });
     ws.th_resid = best_th(ctx, ws.h, kreps, [&](int th) { ws.th_resid = th; },
                           [&] { launch_resid(ws, kd, dt, s2, b, ws.x, s, !fused_update(ws, ctx->opts)); });
@@ -980,6 +1026,8 @@ int dtd_create(const dtd_opts *opts, dtd_ctx **out) {
     }
     const char *ng = getenv("DTD_NO_GRAPH");
     ctx->use_graph = !(ng && *ng && *ng != '0');
+    const char *nk = getenv("DTD_NO_KD");
+    ctx->use_kd = !(nk && *nk && *nk != '0');
     int rc = guard(ctx, [&] {
         HIPCHK(hipSetDevice(ctx->opts.device));
         scratch_init(ctx->opts.device);
@@ -1399,8 +1447,18 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
         const int nbc = nblocks(ws.n, kCgCells);
         const double n = (double)ws.n;
         double b = 0.0;
+        // The workspace's kd buffer holds the last host solve's kd, and nothing defined after
+        // solves on device arrays: the passes are timed on K = 1 (every face dt / dx^2 = 1; the
+        // values do not change the work, while undefined ones could send the kd-formed faces
+        // down their rare exact-division branch).
+        if (!ws.kd) throw Error{DTD_ESTATE, "this workspace keeps no kd buffer"};
+        k_fill64<<<nblocks(ws.n), kBlock, 0, s>>>(ws.kd, ws.n, 1.0);
+        LAUNCH_CHECK();
         k_time_state<<<1, 64, 0, s>>>(ws.st, ws.kd);
         LAUNCH_CHECK();
+        // The finest level's passes as the solve runs them (kd-formed records: no stored A p, no
+        // records read by the fused smoothers).
+        const bool kd = kd_records(ctx);
         auto one = [&] {
             switch (kernel) {
             case DTD_KERNEL_SWEEP:  // r, record, z in; z out (f32)
@@ -1408,9 +1466,9 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
                                                      nullptr, ws.st);
                 b = 24.0 * n;
                 break;
-            case DTD_KERNEL_DIRECTION:  // z (f32), p_old, x, kd in; p, A p, x out
-                launch_dir<float>(ws, L.z[1], ws.p1, ws.p0, s);
-                b = (4.0 + 8.0 * 3 + 8.0 * 3) * n;
+            case DTD_KERNEL_DIRECTION:  // z (f32), p_old, x, kd in; p, A p (unless kd), x out
+                launch_dir<float>(ws, L.z[1], ws.p1, ws.p0, s, kd ? nullptr : ws.ap);
+                b = (4.0 + 8.0 * 3 + 8.0 * (kd ? 2 : 3)) * n;
                 break;
             case DTD_KERNEL_UPDATE:  // A p, r in; r, r32 out
                 k_update<float><<<nbc, kBlock, 0, s>>>((int)ws.n, ws.ap, ws.r, L.r, ws.parts,
@@ -1421,17 +1479,30 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
                 k_restrict<float><<<nblocks(C.n), kBlock, 0, s>>>(L, L.z[1], C, ws.st);
                 b = 20.0 * n + 4.0 * C.n;
                 break;
-            case DTD_KERNEL_PRE:  // r64, A p, record in; r64, z and the coarse r out
+            case DTD_KERNEL_PRE:  // r64, A p (or p and kd), record in; r64, z and the coarse r out
                 if (L.w % 2 || !ws.r2) throw Error{DTD_ESTATE, "the finest width is odd"};
-                launch_pre<float, 2, kRUpdate>(L, C, L.z[0], ws.st, s,
-                                               R64{ws.r, ws.ap, ws.r2, 0.0}, ws.parts, ws.nparts);
-                b = (L.coefh ? 36.0 : 40.0) * n + 4.0 * C.n;  // bf16 records: 8 B, not 12
+                if (kd) {
+                    launch_pre<float, 2, kRUpdateK>(
+                        L, C, L.z[0], ws.st, s, R64{ws.r, nullptr, ws.r2, 0.0, ws.p0, nullptr, 0.0, 0.0},
+                        ws.parts, ws.nparts);
+                    b = 36.0 * n + 4.0 * C.n;  // r64, p, kd in; r64, z out
+                } else {
+                    launch_pre<float, 2, kRUpdate>(L, C, L.z[0], ws.st, s,
+                                                   R64{ws.r, ws.ap, ws.r2, 0.0}, ws.parts, ws.nparts);
+                    b = (L.coefh ? 36.0 : 40.0) * n + 4.0 * C.n;  // bf16 records: 8 B, not 12
+                }
                 break;
-            case DTD_KERNEL_POST:  // z, r64, record, coarse z in; z out
+            case DTD_KERNEL_POST:  // z, r64, record (or kd), coarse z in; z out
                 if (L.w % 2) throw Error{DTD_ESTATE, "the finest width is odd"};
-                launch_post<float, 2, kR64>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts,
-                                            ws.nparts, ws.st, s);
-                b = (L.coefh ? 24.0 : 28.0) * n + 4.0 * C.n;
+                if (kd) {
+                    launch_post<float, 2, kR64K>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts,
+                                                 ws.nparts, ws.st, s);
+                    b = 24.0 * n + 4.0 * C.n;
+                } else {
+                    launch_post<float, 2, kR64>(L, L.z[0], C.z[1], C.w, L.z[1], ws.r, ws.parts,
+                                                ws.nparts, ws.st, s);
+                    b = (L.coefh ? 24.0 : 28.0) * n + 4.0 * C.n;
+                }
                 break;
             case DTD_KERNEL_BANDCOPY: {  // r, record in; z out, the pre-smoother's walk
                 const int th = band_height<2>((const void *)k_bandcopy<2>, L.w, L.h);

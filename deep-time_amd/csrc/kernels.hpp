@@ -104,6 +104,60 @@ __device__ inline double face(double ka, double kb, double dt, double s2) {
     return 0.5 * (ka + kb) * dt / s2;
 }
 
+// q / b correctly rounded from z = 1 / b (itself correctly rounded), without a division: y0 = q z
+// is within 1.5 ulp of q / b, one correction y1 = y0 + (q - b y0) z makes it faithful, and a second
+// gives the rounded quotient exactly (Markstein's theorem: Muller et al., Handbook of
+// Floating-Point Arithmetic, 2010, section 5.3); each remainder q - b y is one fma.  The theorem
+// asks for no underflow: a quotient below 2^-960 (or q = 0) is taken from the division itself.
+__device__ __forceinline__ double div_by(double q, double b, double z) {
+    double y = q * z;
+    double r = fma(-b, y, q);
+    y = fma(r, z, y);
+    r = fma(-b, y, q);
+    y = fma(r, z, y);
+    if (__builtin_expect(fabs(y) < 0x1p-960, 0)) y = q / b;
+    return y;
+}
+
+// `face` with the division by dx^2 as div_by (rs2 = 1 / dx^2): the same bits.
+__device__ __forceinline__ double face_rz(double ka, double kb, double dt, double s2, double rs2) {
+    return div_by(0.5 * (ka + kb) * dt, s2, rs2);
+}
+
+// A x at one cell of an f64 lane in the order of `Operator::apply_row` (D:609-649), from its own
+// and its neighbours' values; ew is the west neighbour's east face, sn the north neighbour's
+// south face.  `inner`: every term present, no wrap (most cells).
+__device__ inline double lane_apply64(double xc, double xe, double xw, double xs, double xn,
+                                      double e, double ew, double s, double sn, int gi, int gj,
+                                      int w, int h, int wrap, bool inner) {
+    if (inner) {
+        const double d = 1.0 + e + ew + s + sn;
+        double o = d * xc;
+        o -= e * xe;
+        o -= ew * xw;
+        o -= s * xs;
+        o -= sn * xn;
+        return o;
+    }
+    const double wv = (gi > 0 || wrap) ? ew : 0.0;
+    const double nv = gj > 0 ? sn : 0.0;
+    const double d = 1.0 + e + wv + s + nv;
+    double o = d * xc;
+    if (gi + 1 < w || wrap) o -= e * xe;
+    if (gi > 0 || wrap) o -= wv * xw;
+    if (gj + 1 < h) o -= s * xs;
+    if (gj > 0) o -= nv * xn;
+    return o;
+}
+
+// The diagonal of one cell of an f64 lane (the sum Operator::new takes).
+__device__ inline double lane_diag64(double e, double ew, double s, double sn, int gi, int gj,
+                                     int wrap) {
+    const double wv = (gi > 0 || wrap) ? ew : 0.0;
+    const double nv = gj > 0 ? sn : 0.0;
+    return 1.0 + e + wv + s + nv;
+}
+
 // A cycle level's operator record: (east, south, diag) of a cell side by side.  omega / diag, the
 // smoother's weight, is one correctly rounded division of diag, formed where it is used: the same
 // bits as a stored copy, without its 4 bytes a cell of traffic.
@@ -1231,7 +1285,14 @@ __device__ inline void copy(T (&d)[V], const T (&s)[V]) {
 // from r_in and A p as the rows are read (the reference's update pass, MG:703-723, with the same
 // expression), written to r_out for the owned cells, and summed as r.r.  The f64 residual
 // alternates between two buffers, so a band's halo rows are read before any update.
-enum { kRLevel = 0, kR64 = 1, kRUpdate = 2 };
+enum { kRLevel = 0, kR64 = 1, kRUpdate = 2, kR64K = 3, kRUpdateK = 4 };
+// kR64K and kRUpdateK are kR64 and kRUpdate on the finest level of the f32 cycle with its records
+// (east, south, diag) formed from kd as the rows are read, with Operator::new's expressions and
+// rounded to f32 (the bits k_operator stores), instead of read; kRUpdateK also forms A p from the
+// direction p (the bits the direction pass forms) instead of reading it.
+__host__ __device__ constexpr bool rm_r64(int rm) { return rm == kR64 || rm == kR64K; }
+__host__ __device__ constexpr bool rm_update(int rm) { return rm == kRUpdate || rm == kRUpdateK; }
+__host__ __device__ constexpr bool rm_kd(int rm) { return rm == kR64K || rm == kRUpdateK; }
 
 // Rows of loads the fused smoothers keep in flight (a divisor of 4: bands are multiples of 4 rows
 // and a walk takes the band's rows plus 4).  Measured at 8192x4096: 4 rows at 3 waves/SIMD
@@ -1247,21 +1308,77 @@ struct R64 {
     const double *ap;  // A p (kRUpdate)
     double *out;       // updated residual (kRUpdate)
     double alpha;
+    const double *p;   // the direction (kRUpdateK)
+    const double *kd;  // the faces' diffusivity, dt and dx^2 (kR64K, kRUpdateK)
+    double dt, s2;
+    double rs2;        // 1 / dx^2
 };
+
+// The f64 faces and diagonal of a lane's V cells of row j, formed from kd as Operator::new forms
+// them (D:556-603): K the row's kd, Ks the row below's (the south faces; none on the last row), kx
+// kd of the column beyond the wave's end lane (lane 0: west of its first cell; lane 63: east of
+// its last), sn the row above's south faces.  Every lane runs it (its shuffles).
+template <int V>
+struct KRow {
+    double e[V], s[V], d[V];
+    double ew;  // the west face of the lane's first cell
+};
+template <int V>
+__device__ __forceinline__ void krow(const double (&K)[V], const double (&Ks)[V], double kx,
+                                     const double (&sn)[V], int gi0, int j, int w, int h,
+                                     int wrap, double dt, double s2, double rs2, KRow<V> &o) {
+    const int lane = threadIdx.x & 63;
+    const double kel = from_east(K[0]);
+    const double ke = lane == 63 ? kx : kel;
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        const bool has_e = gi0 + v + 1 < w || wrap;
+        o.e[v] = has_e ? face_rz(K[v], v + 1 < V ? K[v + 1] : ke, dt, s2, rs2) : 0.0;
+        o.s[v] = j + 1 < h ? face_rz(K[v], Ks[v], dt, s2, rs2) : 0.0;
+    }
+    const double ewl = from_west(o.e[V - 1]);
+    o.ew = lane == 0 ? face_rz(kx, K[0], dt, s2, rs2) : ewl;
+#pragma unroll
+    for (int v = 0; v < V; v++)
+        o.d[v] = lane_diag64(o.e[v], v > 0 ? o.e[v - 1] : o.ew, o.s[v], sn[v], gi0 + v, j, wrap);
+}
+
+// The column beyond the wave's end lane (lane 0: west of its first cell; lane 63: east of its
+// last), wrapped on a periodic level; *ok: it is on the level (the other lanes load nothing).
+template <int V>
+__device__ __forceinline__ int end_column(const Band &bd, int w, int wrap, bool &ok) {
+    const int lane = threadIdx.x & 63;
+    int gx = lane == 0 ? bd.gi0 - 1 : bd.gi0 + V;
+    ok = false;
+    if (bd.active && (lane == 0 || lane == 63)) {
+        if (wrap)
+            gx = ((gx % w) + w) % w, ok = true;
+        else
+            ok = gx >= 0 && gx < w;
+    }
+    return gx;
+}
 
 // Pre-smoothing of level L (nu = 2): z1 = wd r, z2 = z1 + wd (r - A z1) -> zout, and the coarse
 // residual C.r = sum over each 2x2 block of (r - A z2): fine row j0 then j0 + 1, columns
 // ascending, from +0 (MG:332-356).  Rows are loaded kAhead steps before they are consumed.
-// Returns the lane's share of r.r (kRUpdate; else 0).
+// Returns the lane's share of r.r (kRUpdate, kRUpdateK; else 0).  With kR64K and kRUpdateK the
+// records are formed from kd: kd (and for kRUpdateK the direction p) come one row ahead of r, the
+// south faces and A p of a row needing the row below; kRUpdateK's A p is the direction pass's.
 template <typename T, int V, int RM = kRLevel, int CB = 0>
 __device__ __forceinline__ double spre_band(const Lvl<T> &L, T *__restrict__ zout,
                                             const Lvl<T> &C, int th, int wave, R64 rio = R64{}) {
     constexpr int A = kSmoothAhead;  // rows in flight
+    constexpr bool KD = rm_kd(RM), UPD = rm_update(RM), UPK = RM == kRUpdateK;
     const int w = L.w, h = L.h, wrap = L.wrap;
     const Band bd = band_at<V>(w, h, wrap, th, wave);
     const int gi0 = bd.gi0;
-    T qr[A][V], qwd[A][V], qdg[A][V], qe[A][V], qs[A][V];
+    constexpr int QR = KD ? 1 : A;  // the record queues, unused when the records come from kd
+    T qr[A][V], qwd[QR][V], qdg[QR][V], qe[QR][V], qs[QR][V];
     double qr64[RM == kRLevel ? 1 : A][V], qap[RM == kRUpdate ? A : 1][V];
+    double qk[KD ? A : 1][V], qkx[KD ? A : 1], qp[UPK ? A : 1][V], qpx[UPK ? A : 1];
+    bool x_ok = false;
+    const int gx = KD ? end_column<V>(bd, w, wrap, x_ok) : 0;
     double acc_rr = 0.0;
 #define DTD_FETCH_PRE(j, u)                                                                   \
     do {                                                                                      \
@@ -1271,11 +1388,29 @@ __device__ __forceinline__ double spre_band(const Lvl<T> &L, T *__restrict__ zou
             if (RM == kRLevel) loadv<T, V>(L.r + g_, qr[u]);                                  \
             if (RM != kRLevel) loadv<double, V>(rio.in + g_, qr64[RM == kRLevel ? 0 : u]);    \
             if (RM == kRUpdate) loadv<double, V>(rio.ap + g_, qap[RM == kRUpdate ? u : 0]);   \
-            load_rec<T, V, CB>(L, g_, qe[u], qs[u], qdg[u], qwd[u]);                          \
+            if (!KD) load_rec<T, V, CB>(L, g_, qe[KD ? 0 : u], qs[KD ? 0 : u],                \
+                                        qdg[KD ? 0 : u], qwd[KD ? 0 : u]);                    \
         } else {                                                                              \
-            zero(qr[u]), zero(qwd[u]), zero(qdg[u]), zero(qe[u]), zero(qs[u]);                \
+            zero(qr[u]);                                                                      \
+            if (!KD) zero(qwd[KD ? 0 : u]), zero(qdg[KD ? 0 : u]), zero(qe[KD ? 0 : u]),      \
+                     zero(qs[KD ? 0 : u]);                                                    \
             if (RM != kRLevel) zero(qr64[RM == kRLevel ? 0 : u]);                             \
             if (RM == kRUpdate) zero(qap[RM == kRUpdate ? u : 0]);                            \
+        }                                                                                     \
+        if (KD) {                                                                             \
+            const size_t gx_ = (size_t)j_ * w + gx;                                           \
+            const bool rx_ = j_ >= 0 && j_ < h && x_ok;                                       \
+            qkx[KD ? u : 0] = rx_ ? rio.kd[gx_] : 0.0;                                        \
+            if (UPK) qpx[UPK ? u : 0] = rx_ ? rio.p[gx_] : 0.0;                               \
+            const int j1_ = j_ + 1;                                                           \
+            if (j1_ >= 0 && j1_ < h && bd.col_ok) {                                           \
+                const size_t g1_ = (size_t)j1_ * w + gi0;                                     \
+                loadv<double, V>(rio.kd + g1_, qk[KD ? u : 0]);                               \
+                if (UPK) loadv<double, V>(rio.p + g1_, qp[UPK ? u : 0]);                      \
+            } else {                                                                          \
+                zero(qk[KD ? u : 0]);                                                         \
+                if (UPK) zero(qp[UPK ? u : 0]);                                               \
+            }                                                                                 \
         }                                                                                     \
     } while (0)
     // Rows by age: a = newest (ja), b = ja - 1, c = ja - 2, d = ja - 3.
@@ -1286,7 +1421,30 @@ __device__ __forceinline__ double spre_band(const Lvl<T> &L, T *__restrict__ zou
     zero(dgb), zero(ea), zero(eb), zero(sa), zero(sb), zero(sc);
     T acc[V];
     zero(acc);
+    // kd-formed records: row a's kd, the row above's south faces (f64); kRUpdateK: p of rows
+    // a - 1 and a.
+    double K1[KD ? V : 1], S0[KD ? V : 1], P0[UPK ? V : 1], P1[UPK ? V : 1];
+    zero(K1), zero(S0), zero(P0), zero(P1);
     const int steps = bd.active ? bd.rows + 4 : 0;  // a multiple of A
+    if constexpr (KD) {
+        // Row row0 - 2's kd and p, and the row above it (its south faces, its p for A p).
+        if (bd.active) {
+            const int j2 = bd.row0 - 2, j3 = bd.row0 - 3;
+            double K3[V];
+            zero(K3);
+            if (j3 >= 0 && j3 < h && bd.col_ok) {
+                loadv<double, V>(rio.kd + (size_t)j3 * w + gi0, K3);
+                if constexpr (UPK) loadv<double, V>(rio.p + (size_t)j3 * w + gi0, P0);
+            }
+            if (j2 >= 0 && j2 < h && bd.col_ok) {
+                loadv<double, V>(rio.kd + (size_t)j2 * w + gi0, K1);
+                if constexpr (UPK) loadv<double, V>(rio.p + (size_t)j2 * w + gi0, P1);
+            }
+#pragma unroll
+            for (int v = 0; v < V; v++)
+                S0[v] = j3 >= 0 && j3 + 1 < h ? face_rz(K3[v], K1[v], rio.dt, rio.s2, rio.rs2) : 0.0;
+        }
+    }
 #pragma unroll
     for (int u = 0; u < A; u++) DTD_FETCH_PRE(bd.row0 - 2 + u, u);
     for (int k0 = 0; k0 < steps; k0 += A) {
@@ -1296,27 +1454,76 @@ __device__ __forceinline__ double spre_band(const Lvl<T> &L, T *__restrict__ zou
             copy(z1c, z1b), copy(z1b, z1a), copy(z2d, z2c), copy(z2c, z2b);
             copy(rc, rb), copy(rb, ra), copy(wdb, wda), copy(dgc, dgb), copy(dgb, dga);
             copy(ec, eb), copy(eb, ea), copy(sd, sc), copy(sc, sb), copy(sb, sa);
-            weights<T, V, CB>(L, qdg[u], qwd[u], wda), copy(dga, qdg[u]), copy(ea, qe[u]);
-            copy(sa, qs[u]);
-            if (RM == kRLevel) {
-                copy(ra, qr[u]);
-            } else {
-                // The f64 residual of row a (updated when asked), rounded for the cycle.
-                const bool own_a = bd.own && ja >= bd.row0 && ja < bd.row0 + bd.rows && ja < h;
+            if constexpr (KD) {
+                // Row a's records from kd (its south faces from row a + 1's, one slot ahead).
+                double K2[V], r0[V];
+                copy(K2, qk[KD ? u : 0]);
+                copy(r0, qr64[RM == kRLevel ? 0 : u]);
+                const double kx = qkx[KD ? u : 0];
+                double P2[UPK ? V : 1], px = 0.0;
+                if constexpr (UPK) copy(P2, qp[UPK ? u : 0]), px = qpx[UPK ? u : 0];
+                if (k0 + u + A < steps) DTD_FETCH_PRE(ja + A, u);
+                KRow<V> kr;
+                krow<V>(K1, K2, kx, S0, gi0, ja, w, h, wrap, rio.dt, rio.s2, rio.rs2, kr);
                 double rv[V];
+                if constexpr (UPK) {
+                    // A p of row a, the direction pass's expression; r -= alpha A p.
+                    const int lane = threadIdx.x & 63;
+                    const double pel = from_east(P1[0]), pwl = from_west(P1[V - 1]);
+                    const double pe = lane == 63 ? px : pel, pw = lane == 0 ? px : pwl;
+                    const bool inner = bd.inner && ja > 0 && ja + 1 < h;
+#pragma unroll
+                    for (int v = 0; v < V; v++) {
+                        const double o = lane_apply64(P1[v], v + 1 < V ? P1[v + 1] : pe,
+                                                      v > 0 ? P1[v - 1] : pw, P2[v], P0[v],
+                                                      kr.e[v], v > 0 ? kr.e[v - 1] : kr.ew,
+                                                      kr.s[v], S0[v], gi0 + v, ja, w, h, wrap,
+                                                      inner);
+                        rv[v] = r0[v] - rio.alpha * o;
+                    }
+                    copy(P0, P1), copy(P1, P2);
+                } else {
+                    copy(rv, r0);
+                }
 #pragma unroll
                 for (int v = 0; v < V; v++) {
-                    const double r0 = qr64[RM == kRLevel ? 0 : u][v];
-                    rv[v] = RM == kRUpdate ? r0 - rio.alpha * qap[RM == kRUpdate ? u : 0][v] : r0;
+                    ea[v] = (T)kr.e[v], sa[v] = (T)kr.s[v], dga[v] = (T)kr.d[v];
                     ra[v] = (T)rv[v];
                 }
-                if (RM == kRUpdate && own_a) {
-                    storev<double, V>(rio.out + (size_t)ja * w + gi0, rv);
+                weights<T, V, 0>(L, dga, dga, wda);
+                copy(S0, kr.s), copy(K1, K2);
+                if (UPD) {
+                    const bool own_a = bd.own && ja >= bd.row0 && ja < bd.row0 + bd.rows && ja < h;
+                    if (own_a) {
+                        storev<double, V>(rio.out + (size_t)ja * w + gi0, rv);
 #pragma unroll
-                    for (int v = 0; v < V; v++) acc_rr += rv[v] * rv[v];
+                        for (int v = 0; v < V; v++) acc_rr += rv[v] * rv[v];
+                    }
                 }
+            } else {
+                weights<T, V, CB>(L, qdg[KD ? 0 : u], qwd[KD ? 0 : u], wda);
+                copy(dga, qdg[KD ? 0 : u]), copy(ea, qe[KD ? 0 : u]);
+                copy(sa, qs[KD ? 0 : u]);
+                if (RM == kRLevel) {
+                    copy(ra, qr[u]);
+                } else {
+                    // The f64 residual of row a (updated when asked), rounded for the cycle.
+                    const bool own_a = bd.own && ja >= bd.row0 && ja < bd.row0 + bd.rows && ja < h;
+                    double rv[V];
+#pragma unroll
+                    for (int v = 0; v < V; v++) {
+                        const double r0 = qr64[RM == kRLevel ? 0 : u][v];
+                        rv[v] = RM == kRUpdate ? r0 - rio.alpha * qap[RM == kRUpdate ? u : 0][v] : r0;
+                        ra[v] = (T)rv[v];
+                    }
+                    if (RM == kRUpdate && own_a) {
+                        storev<double, V>(rio.out + (size_t)ja * w + gi0, rv);
+#pragma unroll
+                        for (int v = 0; v < V; v++) acc_rr += rv[v] * rv[v];
+                    }
+                }
+                if (k0 + u + A < steps) DTD_FETCH_PRE(ja + A, u);
             }
-            if (k0 + u + A < steps) DTD_FETCH_PRE(ja + A, u);
             const bool oka = ja >= 0 && ja < h && bd.col_ok;
 #pragma unroll
             for (int v = 0; v < V; v++) z1a[v] = oka ? wda[v] * ra[v] : (T)0;
@@ -1353,17 +1560,19 @@ __device__ __forceinline__ double spre_band(const Lvl<T> &L, T *__restrict__ zou
     return acc_rr;
 }
 
-// With RM = kRUpdate the residual's update of this iteration rides on the pass (alpha from st)
-// and parts[block] gets the block's r.r.
+// With RM = kRUpdate or kRUpdateK the residual's update of this iteration rides on the pass
+// (alpha from st) and parts[block] gets the block's r.r; kR64K and kRUpdateK take kd, dt and dx^2
+// for the records from st.
 template <typename T, int V, int RM = kRLevel, int CB = 0>
 __global__ __launch_bounds__(kBlock) void k_spre(Lvl<T> L, T *__restrict__ zout, Lvl<T> C, int th,
                                                  const CgState *__restrict__ st, R64 rio,
                                                  double *__restrict__ parts) {
     __shared__ double lds[kBlock / 64];
     if (st && st->done) return;
-    if (RM == kRUpdate) rio.alpha = st->alpha;
+    if (rm_update(RM)) rio.alpha = st->alpha;
+    if (rm_kd(RM)) rio.kd = st->kd, rio.dt = st->dt, rio.s2 = st->s2, rio.rs2 = 1.0 / st->s2;
     const double acc = spre_band<T, V, RM, CB>(L, zout, C, th, launch_wave(), rio);
-    if (RM == kRUpdate) {
+    if (rm_update(RM)) {
         const double sum = block_sum<kBlock>(acc, lds);
         if (threadIdx.x == 0) parts[blockIdx.x] = sum;
     }
@@ -1371,38 +1580,53 @@ __global__ __launch_bounds__(kBlock) void k_spre(Lvl<T> L, T *__restrict__ zout,
 
 // Post-smoothing of level L (nu = 2): z' = zin + zc[j/2][i/2], z3 = z' + wd (r - A z'),
 // z4 = z3 + wd (r - A z3) -> zout; with dot, per block sum dot * z4 into parts[block].
-// Returns the lane's share of sum dot * z4 (0 without dot).  With RM = kR64, r is the f64
-// residual dot itself, rounded to T as it is read.
+// Returns the lane's share of sum dot * z4 (0 without dot).  With RM = kR64 (kR64K), r is the f64
+// residual dot itself, rounded to T as it is read; kR64K forms the records from kd (rio.kd, dt,
+// s2): row b's when row a's kd arrives (its south faces).
 template <typename T, int V, int RM = kRLevel, int CB = 0>
 __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restrict__ zin,
                                              const T *__restrict__ zc, int wc,
                                              T *__restrict__ zout, int th,
-                                             const double *__restrict__ dot, int wave) {
-    constexpr int A = kSmoothAhead;
+                                             const double *__restrict__ dot, int wave,
+                                             R64 rio = R64{}) {
+    // Two rows in flight when the records come from kd: the f64 faces' registers would otherwise
+    // cost a wave per SIMD.
+    constexpr bool KD = rm_kd(RM), R64M = rm_r64(RM);
+    constexpr int A = KD ? 2 : kSmoothAhead;
     const int w = L.w, h = L.h, wrap = L.wrap;
     const Band bd = band_at<V>(w, h, wrap, th, wave);
     const int gi0 = bd.gi0;
-    T qr[A][V], qwd[A][V], qdg[A][V], qe[A][V], qs[A][V], qz[A][V], qc[A][V];
-    double qr64[RM == kR64 ? A : 1][V];
+    constexpr int QR = KD ? 1 : A;
+    T qr[A][V], qwd[QR][V], qdg[QR][V], qe[QR][V], qs[QR][V], qz[A][V], qc[A][V];
+    double qr64[R64M ? A : 1][V], qk[KD ? A : 1][V], qkx[KD ? A : 1];
+    bool x_ok = false;
+    const int gx = KD ? end_column<V>(bd, w, wrap, x_ok) : 0;
     // z' = z + zc is formed when the row is consumed.
 #define DTD_FETCH_POST(j, u)                                                                  \
     do {                                                                                      \
         const int j_ = (j);                                                                   \
         if (j_ >= 0 && j_ < h && bd.col_ok) {                                                 \
             const size_t g_ = (size_t)j_ * w + gi0;                                           \
-            if (RM == kR64)                                                                   \
-                loadv<double, V>(dot + g_, qr64[RM == kR64 ? u : 0]);                         \
+            if (R64M)                                                                         \
+                loadv<double, V>(dot + g_, qr64[R64M ? u : 0]);                               \
             else                                                                              \
                 loadv<T, V>(L.r + g_, qr[u]);                                                 \
             loadv<T, V>(zin + g_, qz[u]);                                                     \
-            load_rec<T, V, CB>(L, g_, qe[u], qs[u], qdg[u], qwd[u]);                              \
+            if (KD)                                                                           \
+                loadv<double, V>(rio.kd + g_, qk[KD ? u : 0]);                                \
+            else                                                                              \
+                load_rec<T, V, CB>(L, g_, qe[KD ? 0 : u], qs[KD ? 0 : u], qdg[KD ? 0 : u],    \
+                                   qwd[KD ? 0 : u]);                                          \
             const T *zcr_ = zc + (size_t)(j_ >> 1) * wc;                                      \
             _Pragma("unroll") for (int v = 0; v < V; v++) qc[u][v] = zcr_[(gi0 + v) >> 1];    \
         } else {                                                                              \
-            zero(qr[u]), zero(qwd[u]), zero(qdg[u]), zero(qe[u]), zero(qs[u]), zero(qz[u]);   \
-            zero(qc[u]);                                                                      \
-            if (RM == kR64) zero(qr64[RM == kR64 ? u : 0]);                                   \
+            zero(qr[u]), zero(qz[u]), zero(qc[u]);                                            \
+            if (!KD) zero(qwd[KD ? 0 : u]), zero(qdg[KD ? 0 : u]), zero(qe[KD ? 0 : u]),      \
+                     zero(qs[KD ? 0 : u]);                                                    \
+            if (KD) zero(qk[KD ? u : 0]);                                                     \
+            if (R64M) zero(qr64[R64M ? u : 0]);                                               \
         }                                                                                     \
+        if (KD) qkx[KD ? u : 0] = j_ >= 0 && j_ < h && x_ok ? rio.kd[(size_t)j_ * w + gx] : 0.0; \
     } while (0)
     T zpa[V], zpb[V], zpc[V], z3b[V], z3c[V], z3d[V];
     T ra[V], rb[V], rc[V], wda[V], wdb[V], wdc[V], dga[V], dgb[V], dgc[V];
@@ -1410,8 +1634,11 @@ __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restric
     zero(zpa), zero(zpb), zero(z3b), zero(z3c), zero(ra), zero(rb), zero(wda), zero(wdb);
     zero(dga), zero(dgb), zero(ea), zero(eb), zero(sa), zero(sb), zero(sc);
     // With RM = kR64 the f64 residual of rows a, b, c stays in registers for the dot of row c.
-    double r64a[RM == kR64 ? V : 1], r64b[RM == kR64 ? V : 1], r64c[RM == kR64 ? V : 1];
+    double r64a[R64M ? V : 1], r64b[R64M ? V : 1], r64c[R64M ? V : 1];
     zero(r64a), zero(r64b), zero(r64c);
+    // kd-formed records: row b's kd and the end column's, row c's south faces (f64).
+    double KB[KD ? V : 1], S0[KD ? V : 1], kxb = 0.0;
+    zero(KB), zero(S0);
     double acc = 0.0;
     const int steps = bd.active ? bd.rows + 4 : 0;
 #pragma unroll
@@ -1421,15 +1648,31 @@ __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restric
         for (int u = 0; u < A; u++) {
             const int ja = bd.row0 - 2 + k0 + u, jb = ja - 1, jc = ja - 2;
             copy(zpc, zpb), copy(zpb, zpa), copy(z3d, z3c), copy(z3c, z3b);
-            copy(rc, rb), copy(rb, ra), copy(wdc, wdb), copy(wdb, wda), copy(dgc, dgb);
-            copy(dgb, dga), copy(ec, eb), copy(eb, ea), copy(sd, sc), copy(sc, sb), copy(sb, sa);
-            weights<T, V, CB>(L, qdg[u], qwd[u], wda), copy(dga, qdg[u]), copy(ea, qe[u]);
-            copy(sa, qs[u]);
-            if (RM == kR64) {
+            copy(rc, rb), copy(rb, ra), copy(wdc, wdb), copy(dgc, dgb), copy(ec, eb);
+            copy(sd, sc), copy(sc, sb);
+            if constexpr (KD) {
+                // Row b's records, its south faces from row a's kd (just arrived).
+                double KA[V];
+                copy(KA, qk[KD ? u : 0]);
+                KRow<V> kr;
+                krow<V>(KB, KA, kxb, S0, gi0, jb, w, h, wrap, rio.dt, rio.s2, rio.rs2, kr);
+#pragma unroll
+                for (int v = 0; v < V; v++)
+                    eb[v] = (T)kr.e[v], sb[v] = (T)kr.s[v], dgb[v] = (T)kr.d[v];
+                weights<T, V, 0>(L, dgb, dgb, wdb);
+                copy(S0, kr.s), copy(KB, KA);
+                kxb = qkx[KD ? u : 0];
+            } else {
+                copy(wdb, wda), copy(dgb, dga), copy(eb, ea), copy(sb, sa);
+                weights<T, V, CB>(L, qdg[KD ? 0 : u], qwd[KD ? 0 : u], wda);
+                copy(dga, qdg[KD ? 0 : u]), copy(ea, qe[KD ? 0 : u]);
+                copy(sa, qs[KD ? 0 : u]);
+            }
+            if (R64M) {
                 copy(r64c, r64b), copy(r64b, r64a);
 #pragma unroll
                 for (int v = 0; v < V; v++) {
-                    r64a[v] = qr64[RM == kR64 ? u : 0][v];
+                    r64a[v] = qr64[R64M ? u : 0][v];
                     ra[v] = (T)r64a[v];
                 }
             } else {
@@ -1449,7 +1692,7 @@ __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restric
             if (bd.own && okc && jc >= bd.row0) {
                 const size_t g = (size_t)jc * w + gi0;
                 storev<T, V>(zout + g, z4);
-                if (RM == kR64) {
+                if (R64M) {
 #pragma unroll
                     for (int v = 0; v < V; v++) acc += r64c[v] * (double)z4[v];
                 } else if (dot) {
@@ -1472,8 +1715,10 @@ __global__ __launch_bounds__(kBlock) void k_spost(Lvl<T> L, const T *__restrict_
                                                   const CgState *__restrict__ st) {
     __shared__ double lds[kBlock / 64];
     if (st && st->done) return;
+    R64 rio{};
+    if (rm_kd(RM)) rio.kd = st->kd, rio.dt = st->dt, rio.s2 = st->s2, rio.rs2 = 1.0 / st->s2;
     const double acc =
-        spost_band<T, V, RM, CB>(L, zin, zc, wc, zout, th, dot, launch_wave());
+        spost_band<T, V, RM, CB>(L, zin, zc, wc, zout, th, dot, launch_wave(), rio);
     if (dot) {
         const double sum = block_sum<kBlock>(acc, lds);
         if (threadIdx.x == 0) parts[blockIdx.x] = sum;
@@ -1607,7 +1852,7 @@ __device__ __forceinline__ double sdir_band(const Grid &g, const double *__restr
                     if (jb + 1 < h) o -= sb * pa;
                     if (jb > 0) o -= nv * pc;
                 }
-                ap[(size_t)jb * w + gi] = o;
+                if (ap) ap[(size_t)jb * w + gi] = o;  // null: the pre-smoothing forms A p itself
                 acc += pb * o;
             }
         }
@@ -1632,40 +1877,6 @@ __global__ __launch_bounds__(kBlock) void k_sdir(Grid g, const TZ *__restrict__ 
 }
 
 // ---- the other f64 passes, streamed the same way -------------------------------------------
-
-// A x at one cell of an f64 lane in the order of `Operator::apply_row` (D:609-649), from its own
-// and its neighbours' values; ew is the west neighbour's east face, sn the north neighbour's
-// south face.  `inner`: every term present, no wrap (most cells).
-__device__ inline double lane_apply64(double xc, double xe, double xw, double xs, double xn,
-                                      double e, double ew, double s, double sn, int gi, int gj,
-                                      int w, int h, int wrap, bool inner) {
-    if (inner) {
-        const double d = 1.0 + e + ew + s + sn;
-        double o = d * xc;
-        o -= e * xe;
-        o -= ew * xw;
-        o -= s * xs;
-        o -= sn * xn;
-        return o;
-    }
-    const double wv = (gi > 0 || wrap) ? ew : 0.0;
-    const double nv = gj > 0 ? sn : 0.0;
-    const double d = 1.0 + e + wv + s + nv;
-    double o = d * xc;
-    if (gi + 1 < w || wrap) o -= e * xe;
-    if (gi > 0 || wrap) o -= wv * xw;
-    if (gj + 1 < h) o -= s * xs;
-    if (gj > 0) o -= nv * xn;
-    return o;
-}
-
-// The diagonal of one cell of an f64 lane (the sum Operator::new takes).
-__device__ inline double lane_diag64(double e, double ew, double s, double sn, int gi, int gj,
-                                     int wrap) {
-    const double wv = (gi > 0 || wrap) ? ew : 0.0;
-    const double nv = gj > 0 ? sn : 0.0;
-    return 1.0 + e + wv + s + nv;
-}
 
 // The first residual (MG:596-619): r = b - A x, its copy in the cycle's precision, per block
 // (r.r, b.b).  x is read from xin; when xout differs (the first guess is b itself) the owned
