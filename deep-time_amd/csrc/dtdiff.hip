@@ -264,8 +264,8 @@ __global__ void k_time_state(CgState *st, const double *kd) {
 // The coarse levels from the finest (`Hierarchy::new`, MG:481-487; device `COARSEN`/`CDIAG`,
 // MGGPU:62-140): one launch a level.
 template <typename T>
-static void build_coarse(std::vector<Lvl<T>> &lv, T scale, hipStream_t s) {
-    for (size_t l = 1; l < lv.size(); l++) {
+static void build_coarse(std::vector<Lvl<T>> &lv, T scale, hipStream_t s, size_t from = 1) {
+    for (size_t l = from; l < lv.size(); l++) {
         k_coarsen<T><<<nblocks(lv[l].n), kBlock, 0, s>>>(lv[l - 1], lv[l], scale);
         LAUNCH_CHECK();
     }
@@ -288,12 +288,24 @@ static void pack_bf16(Workspace &ws, hipStream_t s) {
         }
 }
 
+static bool kd_records(const dtd_ctx *ctx);
+
 // `Operator::new` on the device from kd (already on the device), with the cycle's levels.
 static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt) {
     Workspace &ws = ctx->ws;
     hipStream_t s = ctx->stream;
     ctx->cycle_levels = 0;  // the levels now hold this solve's operator, not dtd_cycle_new's
     const bool f64 = ws.prec == DTD_CYCLE_F64;
+    if (!f64 && kd_records(ctx)) {
+        // The fused finest smoothers form the finest records from kd, and nothing else of the
+        // solve reads them: level 1 straight from kd, no finest records stored.
+        k_coarsen_kd<<<nblocks(ws.lf[1].n), kBlock, 0, s>>>(ws.lf[0], ws.lf[1], ctx->opts.scale,
+                                                            kd, dt, dx * dx);
+        LAUNCH_CHECK();
+        build_coarse<float>(ws.lf, ctx->opts.scale, s, 2);
+        pack_bf16(ws, s);
+        return;
+    }
     // The f64 passes form the faces from kd themselves; only the cycle's finest records are kept.
     k_operator<<<nblocks(ws.n), kBlock, 0, s>>>(grid_of(ws), kd, dt, dx * dx, nullptr,
                                                 f64 ? nullptr : ws.lf[0].coef,
@@ -1453,6 +1465,10 @@ int dtd_time_kernel(dtd_ctx *ctx, int kernel, int reps, double *ms_avg, double *
         // down their rare exact-division branch).
         if (!ws.kd) throw Error{DTD_ESTATE, "this workspace keeps no kd buffer"};
         k_fill64<<<nblocks(ws.n), kBlock, 0, s>>>(ws.kd, ws.n, 1.0);
+        LAUNCH_CHECK();
+        // The finest records on the same K (a solve with kd-formed records stores none).
+        k_operator<<<nblocks(ws.n), kBlock, 0, s>>>(grid_of(ws), ws.kd, 1.0, 1.0, nullptr, L.coef,
+                                                    nullptr);
         LAUNCH_CHECK();
         k_time_state<<<1, 64, 0, s>>>(ws.st, ws.kd);
         LAUNCH_CHECK();

@@ -503,45 +503,73 @@ __device__ inline T coarse_mass(const Lvl<T> &f, int ic, int jc) {
         for (int i = 2 * ic; i < c1; i++) m += f.mass ? f.mass[j * f.w + i] : (T)1;
     return m;
 }
-template <typename T>
-__device__ inline T coarse_east(const Lvl<T> &f, int wc, int cwrap, T scale, int ic, int jc) {
+// east(c), south(c): the finer level's east and south faces of cell c.
+template <typename T, class E>
+__device__ inline T coarse_east(const Lvl<T> &f, int wc, int cwrap, T scale, int ic, int jc,
+                                E east) {
     if (!(ic + 1 < wc || cwrap)) return (T)0;
     const int r1 = min(2 * jc + 2, f.h), last_col = min(2 * ic + 2, f.w) - 1;
     T e = (T)0;
-    for (int j = 2 * jc; j < r1; j++) e += c_east(f, j * f.w + last_col);
+    for (int j = 2 * jc; j < r1; j++) e += east(j * f.w + last_col);
     return scale * e;
 }
-template <typename T>
-__device__ inline T coarse_south(const Lvl<T> &f, int hc, T scale, int ic, int jc) {
+template <typename T, class S>
+__device__ inline T coarse_south(const Lvl<T> &f, int hc, T scale, int ic, int jc, S south) {
     if (!(jc + 1 < hc)) return (T)0;
     const int last_row = min(2 * jc + 2, f.h) - 1, c1 = min(2 * ic + 2, f.w);
     T s = (T)0;
-    for (int i = 2 * ic; i < c1; i++) s += c_south(f, last_row * f.w + i);
+    for (int i = 2 * ic; i < c1; i++) s += south(last_row * f.w + i);
     return scale * s;
 }
 
 // One coarse level from the finer one: mass, faces, and the diagonal mass + e + w + s + n
 // (MG:189-198); the west and north faces are formed by the same sums as the
 // neighbours' own, so every value equals the reference's.
-template <typename T>
-__device__ __forceinline__ void coarsen_cell(const Lvl<T> &f, const Lvl<T> &c, T scale, int k) {
+template <typename T, class E, class S>
+__device__ __forceinline__ void coarsen_cell(const Lvl<T> &f, const Lvl<T> &c, T scale, int k,
+                                             E east, S south) {
     const int jc = k / c.w, ic = k - jc * c.w;
     const T m = coarse_mass(f, ic, jc);
-    const T e = coarse_east(f, c.w, c.wrap, scale, ic, jc);
-    const T s = coarse_south(f, c.h, scale, ic, jc);
-    const T west = ic > 0 ? coarse_east(f, c.w, c.wrap, scale, ic - 1, jc)
-                          : (c.wrap ? coarse_east(f, c.w, c.wrap, scale, c.w - 1, jc) : (T)0);
-    const T north = jc > 0 ? coarse_south(f, c.h, scale, ic, jc - 1) : (T)0;
+    const T e = coarse_east(f, c.w, c.wrap, scale, ic, jc, east);
+    const T s = coarse_south(f, c.h, scale, ic, jc, south);
+    const T west = ic > 0 ? coarse_east(f, c.w, c.wrap, scale, ic - 1, jc, east)
+                          : (c.wrap ? coarse_east(f, c.w, c.wrap, scale, c.w - 1, jc, east) : (T)0);
+    const T north = jc > 0 ? coarse_south(f, c.h, scale, ic, jc - 1, south) : (T)0;
     const T d = m + e + west + s + north;
     c.mass[k] = m;
     T *q = c.coef + kRec * (size_t)k;
     q[0] = e, q[1] = s, q[2] = d;
+}
+template <typename T>
+__device__ __forceinline__ void coarsen_cell(const Lvl<T> &f, const Lvl<T> &c, T scale, int k) {
+    coarsen_cell(f, c, scale, k, [&](int q) { return c_east(f, q); },
+                 [&](int q) { return c_south(f, q); });
 }
 
 template <typename T>
 __global__ void k_coarsen(Lvl<T> f, Lvl<T> c, T scale) {
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k < c.n) coarsen_cell(f, c, scale, k);
+}
+
+// Level 1 of the f32 cycle straight from kd, with no finest records in between (the fused finest
+// smoothers form theirs from kd): each fine face is the f32 rounding of the f64 face
+// Operator::new forms (D:556-572), exactly the value `Level::finest` would store (MG:96-114),
+// summed as from stored records.  f is the finest level (its sizes; mass 1).
+__global__ void k_coarsen_kd(Lvl<float> f, Lvl<float> c, float scale,
+                             const double *__restrict__ kd, double dt, double s2) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= c.n) return;
+    const int w = f.w, h = f.h;
+    auto east = [&](int q) {
+        const int j = q / w, i = q - j * w;
+        if (i + 1 < w) return (float)face(kd[q], kd[q + 1], dt, s2);
+        return f.wrap ? (float)face(kd[q], kd[j * w], dt, s2) : 0.0f;
+    };
+    auto south = [&](int q) {
+        return q + w < w * h ? (float)face(kd[q], kd[q + w], dt, s2) : 0.0f;
+    };
+    coarsen_cell(f, c, scale, k, east, south);
 }
 
 // One cell of A x on a level in the order of `Level::apply_row` (MG:203-240): diagonal, east
