@@ -75,10 +75,13 @@ def test_first_guess_bit_for_bit(ws, w, h, wrap):
 
 @pytest.mark.parametrize("w,h,wrap,gen", [(64, 64, True, "shelf"), (128, 64, False, "shelf"),
                                           (1024, 512, True, "shelf"), (256, 2048, False, "shelf"),
-                                          (2048, 1024, True, "lognormal"), (8192, 64, True, "lognormal")])
+                                          (2048, 1024, True, "lognormal"), (8192, 64, True, "lognormal"),
+                                          (64, 4096, False, "lognormal"), (512, 4096, True, "lognormal"),
+                                          (128, 8192, True, "lognormal")])
 def test_parallel_first_guess_matches_the_recurrences(ws, w, h, wrap, gen):
     """The partition + cyclic-reduction line solver solves the same systems: equal to the
-    oracle's recurrences to 1e-12 of the field's largest value."""
+    oracle's recurrences to 1e-12 of the field's largest value (lines of 64 to 8192 cells, rows
+    and the transposed grid's columns)."""
     print(f"parallel lod {w}x{h} wrap={wrap} {gen}", flush=True)
     assert parallel_lines(w, h)
     if gen == "shelf":
