@@ -1066,10 +1066,29 @@ template <int V, int H = (V >= 2 ? 1 : 2)> struct LaneGeom {
     static constexpr int own = (64 - 2 * halo) * V;  // owned columns per wave
 };
 
+// A value one lane over, by a DPP wave shift (wave_shr:1 = 0x138: lane i gets lane i - 1's;
+// wave_shl:1 = 0x130: lane i gets lane i + 1's) folded into a VALU move, instead of a ds_bpermute
+// through the LDS crossbar.  The lane with no source (0, resp. 63) keeps its own value, as
+// __shfl_up / __shfl_down give it; 8-byte values move as two 4-byte halves.
+template <int CTRL>
+__device__ inline int dpp_shift(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL, typename T>
+__device__ inline T dpp_move(T v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte lanes");
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, dpp_shift<CTRL>(__builtin_bit_cast(int, v)));
+    } else {
+        const long long b = __builtin_bit_cast(long long, v);
+        const int lo = dpp_shift<CTRL>((int)(b & 0xffffffff)), hi = dpp_shift<CTRL>((int)(b >> 32));
+        return __builtin_bit_cast(T, ((long long)hi << 32) | (unsigned)lo);
+    }
+}
 template <typename T>
-__device__ inline T from_west(T v) { return __shfl_up(v, 1, 64); }    // lane - 1's value
+__device__ inline T from_west(T v) { return dpp_move<0x138>(v); }  // lane - 1's value
 template <typename T>
-__device__ inline T from_east(T v) { return __shfl_down(v, 1, 64); }  // lane + 1's value
+__device__ inline T from_east(T v) { return dpp_move<0x130>(v); }  // lane + 1's value
 
 template <typename T, int V>
 __device__ inline void loadv(const T *p, T (&d)[V]) {
