@@ -132,6 +132,17 @@ struct FailWs;
 
 }  // namespace dtd
 
+namespace dtd {
+// The opt-in audit of the multigrid solve (`diffusion::audit`, D:118-199): over the audited
+// solves, for the pairs [multigrid - Jacobi, Jacobi - tight, multigrid - tight], the largest
+// |difference| of any cell, the largest RMS of one solve and the sum of squares.
+struct AuditSums {
+    unsigned long long solves = 0, its_new = 0, its_old = 0;
+    double max[3] = {0.0, 0.0, 0.0}, max_rms[3] = {0.0, 0.0, 0.0}, sq[3] = {0.0, 0.0, 0.0};
+    double cells = 0.0;
+};
+}  // namespace dtd
+
 struct dtd_ctx {
     dtd_opts opts;
     hipStream_t stream = nullptr;
@@ -159,6 +170,12 @@ struct dtd_ctx {
     size_t stage_n = 0;
     dtd::BatchWs *batch = nullptr;  // the batched solve's buffers (batch.hpp)
     dtd::FailWs *fail = nullptr;    // the slope failure's buffers (failure.hpp)
+    // The audit (DEEPTIME_DIFFUSION_AUDIT at creation, or dtd_audit): its sums, and its device
+    // buffer (the audited answer, the Jacobi answer and the per-block partials).
+    bool audit_on = false;
+    dtd::AuditSums audit;
+    double *audit_buf = nullptr;
+    size_t audit_n = 0;
 };
 
 namespace dtd {
@@ -254,6 +271,8 @@ inline void drop_device_state(dtd_ctx *ctx) {
     free_ws(ctx->ws);
     free_batch(ctx);
     free_fail(ctx);
+    if (ctx->audit_buf) (void)hipFree(ctx->audit_buf);
+    ctx->audit_buf = nullptr, ctx->audit_n = 0;
     ctx->cycle_levels = 0;
 }
 

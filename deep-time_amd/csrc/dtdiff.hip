@@ -1040,6 +1040,8 @@ int dtd_create(const dtd_opts *opts, dtd_ctx **out) {
     ctx->use_graph = !(ng && *ng && *ng != '0');
     const char *nk = getenv("DTD_NO_KD");
     ctx->use_kd = !(nk && *nk && *nk != '0');
+    // The reference's switch for its audit (D:127-130): set at all, it is on.
+    ctx->audit_on = getenv("DEEPTIME_DIFFUSION_AUDIT") != nullptr;
     int rc = guard(ctx, [&] {
         HIPCHK(hipSetDevice(ctx->opts.device));
         scratch_init(ctx->opts.device);
@@ -1067,6 +1069,7 @@ void dtd_destroy(dtd_ctx *ctx) {
     free_ws(ctx->ws);
     free_batch(ctx);
     free_fail(ctx);
+    if (ctx->audit_buf) (void)hipFree(ctx->audit_buf);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->flag_ev)
@@ -1080,6 +1083,119 @@ void dtd_destroy(dtd_ctx *ctx) {
 }
 
 const char *dtd_last_error(const dtd_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+// ---- the opt-in audit (`diffusion::audit`, D:118-199) ------------------------------------------
+
+// Per block, over its cells (grid-stride), for the pairs (a, b), (b, c), (a, c): the largest
+// |difference| and the sum of squared differences, into parts[6 * block + 2 * pair + {0, 1}].
+__global__ __launch_bounds__(kBlock) void k_audit_diff(long n, const double *__restrict__ a,
+                                                       const double *__restrict__ b,
+                                                       const double *__restrict__ c,
+                                                       double *__restrict__ parts) {
+    __shared__ double lds[kBlock / 64];
+    double mx[3] = {0.0, 0.0, 0.0}, sq[3] = {0.0, 0.0, 0.0};
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock) {
+        const double d[3] = {fabs(a[i] - b[i]), fabs(b[i] - c[i]), fabs(a[i] - c[i])};
+#pragma unroll
+        for (int k = 0; k < 3; k++) mx[k] = fmax(mx[k], d[k]), sq[k] += d[k] * d[k];
+    }
+    for (int k = 0; k < 3; k++) {
+        double m = mx[k];
+        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        if (lane == 0) lds[wid] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int q = 1; q < kBlock / 64; q++) m = fmax(m, lds[q]);
+            parts[6 * blockIdx.x + 2 * k] = m;
+        }
+        __syncthreads();
+        const double t = block_sum<kBlock>(sq[k], lds);
+        if (threadIdx.x == 0) parts[6 * blockIdx.x + 2 * k + 1] = t;
+    }
+}
+
+constexpr double kAuditTightM = 1.0e-9;  // `audit::TIGHT_M` (D:124): the tight solve's RMS residual
+constexpr int kAuditBlocks = 1024;
+
+// After a multigrid solve of (b, kd) whose answer is in ws.x: the Jacobi solve and a tight
+// multigrid solve of the same step (on the same workspace), the three differences, added to the
+// context's sums (`audit::record`, D:148-176).  Leaves the audited answer in ws.x again.
+static void audit_record(dtd_ctx *ctx, const double *b, const double *kd, double dx, double dt,
+                         int wrap_x, size_t its) {
+    Workspace &ws = ctx->ws;
+    hipStream_t s = ctx->stream;
+    const size_t n = (size_t)ws.n;
+    if (ctx->audit_n < n) {
+        if (ctx->audit_buf) HIPCHK(hipFree(ctx->audit_buf));
+        ctx->audit_buf = nullptr, ctx->audit_n = 0;
+        HIPCHK(hipMalloc((void **)&ctx->audit_buf, (2 * n + 6 * kAuditBlocks) * sizeof(double)));
+        ctx->audit_n = n;
+    }
+    double *nw = ctx->audit_buf, *old = nw + ctx->audit_n, *parts = old + ctx->audit_n;
+    HIPCHK(hipMemcpyAsync(nw, ws.x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    const size_t its_old = jacobi_solve(ctx, b, kd, dx, dt, wrap_x, nullptr);
+    HIPCHK(hipMemcpyAsync(old, ws.x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    solve_core(ctx, b, kd, dx, dt, wrap_x, kAuditTightM, 0.0, true, kGuessLod, nullptr, nullptr,
+               nullptr);
+    const int nb = (int)std::min<size_t>(kAuditBlocks, (n + kBlock - 1) / kBlock);
+    k_audit_diff<<<nb, kBlock, 0, s>>>((long)n, nw, old, ws.x, parts);
+    LAUNCH_CHECK();
+    std::vector<double> hp(6 * (size_t)nb);
+    HIPCHK(hipMemcpyAsync(hp.data(), parts, hp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(ws.x, nw, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    AuditSums &a = ctx->audit;
+    a.solves += 1, a.its_new += its, a.its_old += its_old, a.cells += (double)n;
+    for (int k = 0; k < 3; k++) {
+        double mx = 0.0, sq = 0.0;
+        for (int q = 0; q < nb; q++) mx = std::max(mx, hp[6 * q + 2 * k]), sq += hp[6 * q + 2 * k + 1];
+        a.max[k] = std::max(a.max[k], mx);
+        a.max_rms[k] = std::max(a.max_rms[k], std::sqrt(sq / (double)n));
+        a.sq[k] += sq;
+    }
+}
+
+// A number as Rust's `{:.3e}` writes it: three decimals, the exponent without sign padding.
+static std::string rust_e3(double v) {
+    char b[64];
+    std::snprintf(b, sizeof b, "%.3e", v);
+    std::string t(b);
+    const size_t e = t.find('e');
+    if (e == std::string::npos) return t;
+    const int ex = std::atoi(t.c_str() + e + 1);
+    return t.substr(0, e) + "e" + std::to_string(ex);
+}
+
+int dtd_audit(dtd_ctx *ctx, int on, int reset) {
+    if (!ctx) return DTD_EINVAL;
+    ctx->audit_on = on != 0;
+    if (reset) ctx->audit = AuditSums{};
+    return DTD_OK;
+}
+
+int dtd_audit_line(const dtd_ctx *ctx, char *buf, size_t cap) {
+    if (!ctx || !buf || cap == 0) return DTD_EINVAL;
+    const AuditSums &s = ctx->audit;
+    buf[0] = 0;
+    if (s.solves == 0) return DTD_ESTATE;
+    const double cells = std::max(s.cells, 1.0);
+    auto rms = [&](int k) { return std::sqrt(s.sq[k] / cells); };
+    auto per = [&](unsigned long long v) { return (double)v / (double)s.solves; };
+    char its[64];
+    std::snprintf(its, sizeof its, "its_multigrid=%.1f its_jacobi=%.1f", per(s.its_new),
+                  per(s.its_old));
+    std::string line = "diffusion_audit solves=" + std::to_string(s.solves) + " " + its;
+    const char *names[3] = {"new_vs_old", "old_err", "new_err"};
+    for (int k = 0; k < 3; k++) {
+        line += std::string(" ") + names[k] + "_max_m=" + rust_e3(s.max[k]);
+        line += std::string(" ") + names[k] + "_rms_m=" + rust_e3(rms(k));
+        line += std::string(" ") + names[k] + "_worst_rms_m=" + rust_e3(s.max_rms[k]);
+    }
+    if (line.size() + 1 > cap) return DTD_EINVAL;
+    std::memcpy(buf, line.c_str(), line.size() + 1);
+    return DTD_OK;
+}
 
 int dtd_solve(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h_inout,
               const double *kd, double dt, int solver, double tol_m, size_t *iterations) {
@@ -1104,6 +1220,8 @@ int dtd_solve(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, double *h
                 ? jacobi_solve(ctx, ws.b, ws.kd, dx, dt, wrap_x, nullptr)
                 : solve_core(ctx, ws.b, ws.kd, dx, dt, wrap_x, tol_m, 0.0, true, kGuessLod,
                              nullptr, nullptr, nullptr);
+        if (solver == DTD_SOLVER_MULTIGRID && ctx->audit_on)
+            audit_record(ctx, ws.b, ws.kd, dx, dt, wrap_x, its);
         // The answer comes back through pinned staging and reaches h_inout only once every
         // device call has succeeded, so h_inout keeps h0 whatever fails.
         if (ctx->stage_n < n) {
@@ -1131,9 +1249,10 @@ int dtd_solve_device(dtd_ctx *ctx, size_t w, size_t h, double dx, int wrap_x, do
         if (!h_dev || !kd_dev) throw Error{DTD_EINVAL, "null array"};
         fit(ctx, (int)w, (int)h, wrap_x && w > 1);
         double rel = 0.0;
-        solve_core(ctx, h_dev, kd_dev, dx, dt, wrap_x, tol_m, 0.0, true, kGuessLod, stats, &rel,
-                   nullptr);
+        const size_t its = solve_core(ctx, h_dev, kd_dev, dx, dt, wrap_x, tol_m, 0.0, true,
+                                      kGuessLod, stats, &rel, nullptr);
         if (stats) stats->rel_residual = rel;
+        if (ctx->audit_on) audit_record(ctx, h_dev, kd_dev, dx, dt, wrap_x, its);
         copy_answer(ctx, h_dev);
     });
 }

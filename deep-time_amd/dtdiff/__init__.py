@@ -52,7 +52,8 @@ SYMBOLS = (
     "dtd_nn_up_cat", "dtd_nn_up_cat_bwd", "dtd_nn_input", "dtd_nn_loss", "dtd_nn_guess",
     "dtd_nn_adam", "dtd_nn_to_bf16", "dtd_sync", "dtd_solve_rel_guess_device", "dtd_lod_device",
     "dtd_inject_fault", "dtd_solve_device_noalloc", "dtd_tune", "dtd_tune_plan",
-    "dtd_tune_batch", "dtd_nn_block_mean", "dtd_nn_guess_coarse",
+    "dtd_tune_batch", "dtd_nn_block_mean", "dtd_nn_guess_coarse", "dtd_audit",
+    "dtd_audit_line",
 )
 # The stage's slope failure constants (crates/deeptime-core/src/surface/marine.rs:231-232).
 FAILURE_RATE = 0.05
@@ -233,6 +234,10 @@ def load(path: str = LIB_PATH):
     L.dtd_tune_batch.restype = ctypes.c_int
     L.dtd_inject_fault.argtypes = [_vp, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
     L.dtd_inject_fault.restype = ctypes.c_int
+    L.dtd_audit.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+    L.dtd_audit.restype = ctypes.c_int
+    L.dtd_audit_line.argtypes = [_vp, ctypes.c_char_p, ctypes.c_size_t]
+    L.dtd_audit_line.restype = ctypes.c_int
     for name in ("dtd_solve", "dtd_solve_device", "dtd_solve_rel_device", "dtd_operator",
                  "dtd_apply", "dtd_lod", "dtd_cycle_new", "dtd_cycle_level",
                  "dtd_cycle_precondition", "dtd_time_kernel"):
@@ -570,6 +575,24 @@ def inject_fault(ws: Workspace, after: int) -> int:
     calls = ctypes.c_long()
     ws.check(load().dtd_inject_fault(ws._ctx, after, ctypes.byref(calls)))
     return calls.value
+
+
+def audit(ws: Workspace, on: bool = True, reset: bool = False) -> None:
+    """Switches the opt-in audit of the multigrid solve (`diffusion::audit`, diffusion.rs:118-199;
+    on from the start when DEEPTIME_DIFFUSION_AUDIT is set): each solve's answer is compared with
+    the Jacobi solve's and with a solve to 1e-9 m RMS.  reset clears the sums."""
+    ws.check(load().dtd_audit(ws._ctx, int(on), int(reset)))
+
+
+def audit_line(ws: Workspace) -> str | None:
+    """`audit::audit_line` (diffusion.rs:179-199): the sums so far, or None before any audited
+    solve."""
+    buf = ctypes.create_string_buffer(1024)
+    rc = load().dtd_audit_line(ws._ctx, buf, len(buf))
+    if rc == DTD_ESTATE:
+        return None
+    ws.check(rc)
+    return buf.value.decode()
 
 
 def time_kernel(ws: Workspace, kernel: int, reps: int = 20):

@@ -280,6 +280,23 @@ int dtd_solve_batch(dtd_ctx *ctx, size_t n, const dtd_grid *grids, double rel_to
 #define DTD_KERNEL_STREAM5 7    /* calibration: five finest-level arrays combined into one */
 #define DTD_KERNEL_BANDCOPY 8   /* calibration: the pre-smoother's band walk without its stencil */
 
+/* The opt-in audit of the multigrid solve, `diffusion::audit`
+ * (crates/deeptime-core/src/surface/diffusion.rs:118-199; switched on there by the environment
+ * variable DEEPTIME_DIFFUSION_AUDIT, D:127-130, which dtd_create reads too).  While it is on, each
+ * multigrid solve of dtd_solve and dtd_solve_device is followed by the Jacobi solve and by a
+ * multigrid solve to an RMS residual of 1e-9 m (`audit::TIGHT_M`) of the same step, on the same
+ * context, and the context adds, for [multigrid - Jacobi, Jacobi - tight, multigrid - tight], the
+ * largest |difference| of any cell, the largest RMS of one solve and the sum of squares
+ * (`audit::record`, D:148-176).  The caller's answer is the multigrid solve's, as without the
+ * audit.  dtd_audit switches it (reset != 0 also clears the sums). */
+int dtd_audit(dtd_ctx *ctx, int on, int reset);
+
+/* `audit::audit_line` (D:179-199): the sums so far as one text line in buf (cap bytes),
+ * "diffusion_audit solves=... its_multigrid=... its_jacobi=... new_vs_old_max_m=..." with the
+ * reference's fields and number formats; DTD_ESTATE (and "") when no solve has been audited,
+ * where the reference returns None. */
+int dtd_audit_line(const dtd_ctx *ctx, char *buf, size_t cap);
+
 /* Measurement hook, no reference counterpart: launches `kernel` reps times back to back on the
  * context's stream over the resident data of the last solve and returns the average duration
  * between two HIP events (ms) and the kernel's compulsory bytes per launch (every array it
