@@ -13,7 +13,11 @@
 // table of (grid, block of that grid) pairs, so its blocks never straddle two grids, and writes one
 // partial sum per block; a step kernel with one block per grid adds each grid's partials in block
 // order into that grid's CgState.  Band heights depend on the grid's shape alone, so a grid's
-// answer is the same bits whatever else is in the batch.  A grid whose loop has stopped skips
+// answer is the same bits whatever else is in the batch.  An iteration is the single grid's: the
+// direction pass stores no A p, the finest level's fused pre-smoothing updates the residual
+// (r -= alpha A p, A p and the level's records formed from p and kd) and leaves r.r's partials,
+// and the residual and direction buffers alternate with the parity; a grid of one level (its
+// finest level is the coarsest) keeps the stored A p and the separate update.  A grid whose loop has stopped skips
 // every later pass (its state says done) while the others go on; the host loop ends when all
 // have stopped.
 #pragma once
@@ -33,16 +37,21 @@ struct BGrid {  // one grid's f64 side
     const double *kd, *b;  // the caller's arrays (b: h on entry, read until the end)
     double *h;             // the caller's array, written last
     double *x, *r, *p0, *p1, *ap;
-    float *r32, *z32, *coef32;  // the finest f32 level
+    double *ap_dir;             // where the direction pass stores A p (null: not stored)
+    float *r32, *z32, *coef32;  // the finest f32 level (r32 null when the cycle reads r itself)
     CgState *st;
 };
 
 struct BLevel {      // one grid's level l of the f32 cycle
     Lvl<float> L, C;  // C: level l + 1 (unused on the grid's coarsest level)
-    int th;          // band height of the fused smoothing kernels on this level
-    int pad;
+    int th, th_post;  // band heights of the fused pre- and post-smoothing on this level
     const double *dot;  // the f64 residual on the finest level (the r.z sum), else null
     const CgState *st;
+    // The finest level of a grid with more levels: the two residual buffers (r, r2) and the two
+    // direction buffers (p0, p1); iteration parity q reads r64[q], writes r64[q ^ 1], and forms
+    // A p from pd[q].
+    double *r64[2];
+    const double *pd[2];
 };
 
 struct BInit {  // a solve's start for one grid
@@ -117,7 +126,8 @@ __global__ __launch_bounds__(kLineBlock) void kb_lod_cols(const BGrid *__restric
 }
 
 // A fresh state for each grid (inactive grids are marked done and skipped by every pass).
-__global__ void kb_init(CgState *st, const BInit *__restrict__ in, int n, int max_iter) {
+__global__ void kb_init(CgState *st, const BInit *__restrict__ in, const BGrid *__restrict__ t,
+                        int n, int max_iter) {
     const int g = blockIdx.x * kBlock + threadIdx.x;
     if (g >= n) return;
     CgState s{};
@@ -126,6 +136,7 @@ __global__ void kb_init(CgState *st, const BInit *__restrict__ in, int n, int ma
     s.max_iter = max_iter;
     s.skip = !in[g].active;
     s.done = s.skip ? 5 : 0;
+    s.kd = t[g].kd, s.dt = t[g].dt, s.s2 = t[g].s2;
     st[g] = s;
 }
 
@@ -163,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void kb_sdir(const BGrid *__restrict__ t,
     const double *pold = parity == 0 ? q.p1 : q.p0;
     double *pnew = parity == 0 ? q.p0 : q.p1;
     const double acc =
-        sdir_band<float>(q.g, q.kd, q.dt, q.s2, q.z32, pold, pnew, q.ap, q.x, q.th, q.st,
+        sdir_band<float>(q.g, q.kd, q.dt, q.s2, q.z32, pold, pnew, q.ap_dir, q.x, q.th, q.st,
                          block_wave(m.y));
     const double sum = block_sum<kBlock>(acc, lds);
     if (threadIdx.x == 0) parts[blockIdx.x] = sum;
@@ -176,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void kb_update(const BGrid *__restrict__ t,
     const int2 m = blk[blockIdx.x];
     const BGrid &q = t[m.x];
     if (q.st->done) return;
-    const double acc = update_block<float>(q.g.n, q.ap, q.r, q.r32, q.st->alpha, m.y);
+    const double acc = update_block<float>(q.g.n, q.ap_dir, q.r, q.r32, q.st->alpha, m.y);
     const double sum = block_sum<kBlock>(acc, lds);
     if (threadIdx.x == 0) parts[blockIdx.x] = sum;
 }
@@ -198,12 +209,56 @@ __global__ __launch_bounds__(kBlock) void kb_spost(const BLevel *__restrict__ t,
     const int2 m = blk[blockIdx.x];
     const BLevel &q = t[m.x];
     if (q.st->done) return;
-    const double acc = spost_band<float, V>(q.L, q.L.z[0], q.C.z[1], q.C.w, q.L.z[1], q.th,
+    const double acc = spost_band<float, V>(q.L, q.L.z[0], q.C.z[1], q.C.w, q.L.z[1], q.th_post,
                                             q.dot, block_wave(m.y));
     if (q.dot) {
         const double sum = block_sum<kBlock>(acc, lds);
         if (threadIdx.x == 0) parts[blockIdx.x] = sum;
     }
+}
+
+// The finest level of the grids with more levels, as the single grid's fused smoothing runs it
+// (records formed from kd; RM = kR64K: the first cycle, r the f64 residual r64[0]; RM =
+// kRUpdateK: iteration `parity`, r64[parity] - alpha A p into r64[parity ^ 1] with r.r's
+// partials, A p from pd[parity]).
+template <int V, int RM>
+__global__ __launch_bounds__(kBlock) void kb_spre0(const BLevel *__restrict__ t,
+                                                   const int2 *__restrict__ blk, int parity,
+                                                   double *__restrict__ parts) {
+    __shared__ double lds[kBlock / 64];
+    const int2 m = blk[blockIdx.x];
+    const BLevel &q = t[m.x];
+    const CgState *st = q.st;
+    if (st->done) return;
+    R64 rio{};
+    rio.in = q.r64[RM == kRUpdateK ? parity : 0];
+    rio.out = q.r64[RM == kRUpdateK ? parity ^ 1 : 0];
+    rio.p = q.pd[parity];
+    if (rm_update(RM)) rio.alpha = st->alpha;
+    rio.kd = st->kd, rio.dt = st->dt, rio.s2 = st->s2, rio.rs2 = 1.0 / st->s2;
+    const double acc = spre_band<float, V, RM, 0>(q.L, q.L.z[0], q.C, q.th, block_wave(m.y), rio);
+    if (rm_update(RM)) {
+        const double sum = block_sum<kBlock>(acc, lds);
+        if (threadIdx.x == 0) parts[blockIdx.x] = sum;
+    }
+}
+
+// Its post-smoothing: the records from kd, r.z against the current residual r64[cur].
+template <int V>
+__global__ __launch_bounds__(kBlock) void kb_spost0(const BLevel *__restrict__ t,
+                                                    const int2 *__restrict__ blk, int cur,
+                                                    double *__restrict__ parts) {
+    __shared__ double lds[kBlock / 64];
+    const int2 m = blk[blockIdx.x];
+    const BLevel &q = t[m.x];
+    const CgState *st = q.st;
+    if (st->done) return;
+    R64 rio{};
+    rio.kd = st->kd, rio.dt = st->dt, rio.s2 = st->s2, rio.rs2 = 1.0 / st->s2;
+    const double acc = spost_band<float, V, kR64K, 0>(q.L, q.L.z[0], q.C.z[1], q.C.w, q.L.z[1],
+                                                      q.th_post, q.r64[cur], block_wave(m.y), rio);
+    const double sum = block_sum<kBlock>(acc, lds);
+    if (threadIdx.x == 0) parts[blockIdx.x] = sum;
 }
 
 // The coarsest level of each grid whose cycle ends here: one wave a grid.
@@ -297,7 +352,8 @@ struct BatchWs {
     int2 *tables = nullptr;  // every block and range table, one allocation
     // f64 side
     BLaunch cells_l, update_l, rows_l, cols_l, tiles_l, band_l;
-    const int2 *rng_band = nullptr, *rng_update = nullptr, *rng_rz = nullptr;
+    const int2 *rng_band = nullptr, *rng_rr = nullptr, *rng_rz = nullptr;
+    int rr_base_pre1 = 0, rr_base_update = 0;  // r.r partials: pre2[0]'s, pre1[0]'s, the update's
     // cycle, per level
     std::vector<BLaunch> coarsen_l, pre2, pre1, post2, post1, coarsest;
     int rz_base_post1 = 0, rz_base_coarsest = 0;
@@ -378,6 +434,7 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
                 ltotal[l] += round_up((size_t)sh[g][l].w * sh[g][l].h, 64);
             }
     double *x = (double *)own(total * 8), *r = (double *)own(total * 8);
+    double *rnext = (double *)own(total * 8);  // the residual's second buffer
     double *p0 = (double *)own(total * 8), *p1 = (double *)own(total * 8);
     double *ap = (double *)own(total * 8);
     bw->st = (CgState *)own(n * sizeof(CgState));
@@ -419,7 +476,11 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
         const size_t o = off[g];
         q.x = x + o, q.r = r + o, q.p0 = p0 + o, q.p1 = p1 + o, q.ap = ap + o;
         const Lvl<float> L0 = level(g, 0);
-        q.r32 = L0.r, q.z32 = L0.z[1], q.coef32 = L0.coef;
+        // A grid of one level keeps the stored A p, the separate update and the f32 residual its
+        // coarsest solve reads; the others' finest level reads the f64 residual itself.
+        const bool fused = bw->nlev[g] > 1;
+        q.ap_dir = fused ? nullptr : q.ap;
+        q.r32 = fused ? nullptr : L0.r, q.z32 = L0.z[1], q.coef32 = L0.coef;
         q.st = bw->st + g;
     }
     bw->d_grids = (BGrid *)own(n * sizeof(BGrid));
@@ -432,9 +493,15 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
             BLevel t{};
             t.L = level(g, l);
             t.C = level(g, l + 1);
-            t.th = bw->policy.level(t.L.h);
+            t.th = l == 0 ? bw->policy.finest(true, t.L.h) : bw->policy.level(t.L.h);
+            t.th_post = l == 0 ? bw->policy.finest(false, t.L.h) : t.th;
             t.dot = l == 0 ? bw->grids[g].r : nullptr;
             t.st = bw->st + g;
+            if (l == 0) {
+                const size_t o = off[g];
+                t.r64[0] = r + o, t.r64[1] = rnext + o;
+                t.pd[0] = p0 + o, t.pd[1] = p1 + o;
+            }
             lv[l].push_back(t);
             lv_grid[l].push_back((int)g);
         }
@@ -462,8 +529,9 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
     std::vector<int> cnt(n);
     for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].w * gs[g].h);
     add_launch(bw->cells_l, cnt, nullptr);
-    for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].w * gs[g].h, kCgCells);
-    std::vector<int2> rng_update, rng_band, rng_rz(n);
+    for (size_t g = 0; g < n; g++)
+        cnt[g] = bw->nlev[g] > 1 ? 0 : nblocks((long)gs[g].w * gs[g].h, kCgCells);
+    std::vector<int2> rng_update, rng_band, rng_rz(n), rng_rr(n);
     add_launch(bw->update_l, cnt, &rng_update);
     for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].h, kLineBlock);
     add_launch(bw->rows_l, cnt, nullptr);
@@ -481,30 +549,46 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
     bw->coarsest.resize(L);
     for (int l = 0; l < L; l++) {
         const size_t nt = lv[l].size();
-        std::vector<int> cz(nt, 0), c2(nt, 0), c1(nt, 0), cc(nt, 0);
+        std::vector<int> cz(nt, 0), c2(nt, 0), c1(nt, 0), cc(nt, 0), q2(nt, 0), q1(nt, 0);
         for (size_t t = 0; t < nt; t++) {
             const BLevel &q = lv[l][t];
             const int g = lv_grid[l][t];
             const bool last = l + 1 == bw->nlev[g];
             if (!last) {
                 cz[t] = nblocks(q.C.n);
+                // c: the pre-smoothing's blocks, q: the post-smoothing's.
                 if (q.L.w % 2 == 0)
-                    c2[t] = stream_grid<2, 1>(q.L.w, q.L.h, q.th);
+                    c2[t] = stream_grid<2, 1>(q.L.w, q.L.h, q.th),
+                    q2[t] = stream_grid<2, 1>(q.L.w, q.L.h, q.th_post);
                 else
-                    c1[t] = stream_grid<1, 2>(q.L.w, q.L.h, q.th);
+                    c1[t] = stream_grid<1, 2>(q.L.w, q.L.h, q.th),
+                    q1[t] = stream_grid<1, 2>(q.L.w, q.L.h, q.th_post);
             } else {
                 if (q.L.n > 64) throw Error{DTD_ESTATE, "the coarsest level exceeds one wave"};
                 cc[t] = 1;
             }
         }
         add_launch(bw->coarsen_l[l], cz, nullptr);
-        std::vector<int2> r2, r1, rc;
-        add_launch(bw->pre2[l], c2, nullptr);
-        add_launch(bw->pre1[l], c1, nullptr);
-        add_launch(bw->post2[l], c2, l == 0 ? &r2 : nullptr);
-        add_launch(bw->post1[l], c1, l == 0 ? &r1 : nullptr);
+        std::vector<int2> r2, r1, rc, s2, s1;
+        add_launch(bw->pre2[l], c2, l == 0 ? &s2 : nullptr);
+        add_launch(bw->pre1[l], c1, l == 0 ? &s1 : nullptr);
+        add_launch(bw->post2[l], q2, l == 0 ? &r2 : nullptr);
+        add_launch(bw->post1[l], q1, l == 0 ? &r1 : nullptr);
         add_launch(bw->coarsest[l], cc, l == 0 ? &rc : nullptr);
         if (l == 0) {
+            // The r.r partials of an iteration: pre2's blocks, then pre1's, then the separate
+            // update's (grids of one level).
+            bw->rr_base_pre1 = bw->pre2[0].nblk;
+            bw->rr_base_update = bw->pre2[0].nblk + bw->pre1[0].nblk;
+            for (size_t t = 0; t < nt; t++) {
+                const int g = lv_grid[0][t];
+                if (c2[t])
+                    rng_rr[g] = s2[t];
+                else if (c1[t])
+                    rng_rr[g] = make_int2(bw->rr_base_pre1 + s1[t].x, s1[t].y);
+                else
+                    rng_rr[g] = make_int2(bw->rr_base_update + rng_update[g].x, rng_update[g].y);
+            }
             // The r.z partials: post2's blocks, then post1's, then the coarsest launch's.
             bw->rz_base_post1 = bw->post2[0].nblk;
             bw->rz_base_coarsest = bw->post2[0].nblk + bw->post1[0].nblk;
@@ -521,16 +605,16 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
     }
     const size_t at_rng = tab.size();
     tab.insert(tab.end(), rng_band.begin(), rng_band.end());
-    tab.insert(tab.end(), rng_update.begin(), rng_update.end());
+    tab.insert(tab.end(), rng_rr.begin(), rng_rr.end());
     tab.insert(tab.end(), rng_rz.begin(), rng_rz.end());
     bw->tables = (int2 *)own(tab.size() * sizeof(int2));
     HIPCHK(hipMemcpy(bw->tables, tab.data(), tab.size() * sizeof(int2), hipMemcpyHostToDevice));
     for (auto &p : pend) p.l->blk = bw->tables + p.at;
     bw->rng_band = bw->tables + at_rng;
-    bw->rng_update = bw->rng_band + n;
-    bw->rng_rz = bw->rng_update + n;
+    bw->rng_rr = bw->rng_band + n;
+    bw->rng_rz = bw->rng_rr + n;
     // Partial sums: pairs for the widest launch (and the r.z launches side by side).
-    size_t most = (size_t)std::max({bw->band_l.nblk, bw->update_l.nblk,
+    size_t most = (size_t)std::max({bw->band_l.nblk, bw->rr_base_update + bw->update_l.nblk,
                                     bw->post2[0].nblk + bw->post1[0].nblk + bw->coarsest[0].nblk});
     bw->parts = (double *)own(2 * most * sizeof(double) + 16);
     HIPCHK(hipHostMalloc((void **)&bw->h_st, n * sizeof(CgState), hipHostMallocDefault));
@@ -542,17 +626,45 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
 
 // The V-cycle of every grid (`vcycle`, MG:500-523): down the levels (pre-smoothing and
 // restriction, or the coarsest solve of the grids that end there), then up (post-smoothing; the
-// finest level's leaves the r.z partials).
-static void batch_vcycle(BatchWs &bw, hipStream_t s) {
+// finest level's leaves the r.z partials).  parity < 0: the first cycle (the finest level reads
+// the first residual); else iteration `parity`, whose finest pre-smoothing updates the residual
+// first, after which the r.r step runs (before the finest post-smoothing reuses the partials).
+static void batch_vcycle(BatchWs &bw, hipStream_t s, int parity) {
     const int L = bw.nlevels;
+    const int par = parity < 0 ? 0 : parity;
     for (int l = 0; l < L; l++) {
-        if (bw.pre2[l].nblk) {
-            kb_spre<2><<<bw.pre2[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.pre2[l].blk);
-            LAUNCH_CHECK();
-        }
-        if (bw.pre1[l].nblk) {
-            kb_spre<1><<<bw.pre1[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.pre1[l].blk);
-            LAUNCH_CHECK();
+        if (l == 0) {
+            if (bw.pre2[0].nblk) {
+                if (parity < 0)
+                    kb_spre0<2, kR64K><<<bw.pre2[0].nblk, kBlock, 0, s>>>(bw.d_lv[0], bw.pre2[0].blk,
+                                                                       par, bw.parts);
+                else
+                    kb_spre0<2, kRUpdateK><<<bw.pre2[0].nblk, kBlock, 0, s>>>(
+                        bw.d_lv[0], bw.pre2[0].blk, par, bw.parts);
+                LAUNCH_CHECK();
+            }
+            if (bw.pre1[0].nblk) {
+                if (parity < 0)
+                    kb_spre0<1, kR64K><<<bw.pre1[0].nblk, kBlock, 0, s>>>(
+                        bw.d_lv[0], bw.pre1[0].blk, par, bw.parts + bw.rr_base_pre1);
+                else
+                    kb_spre0<1, kRUpdateK><<<bw.pre1[0].nblk, kBlock, 0, s>>>(
+                        bw.d_lv[0], bw.pre1[0].blk, par, bw.parts + bw.rr_base_pre1);
+                LAUNCH_CHECK();
+            }
+            if (parity >= 0) {
+                kb_step<kStepRR><<<bw.ngrids, kReduceBlock, 0, s>>>(bw.rng_rr, bw.st, bw.parts);
+                LAUNCH_CHECK();
+            }
+        } else {
+            if (bw.pre2[l].nblk) {
+                kb_spre<2><<<bw.pre2[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.pre2[l].blk);
+                LAUNCH_CHECK();
+            }
+            if (bw.pre1[l].nblk) {
+                kb_spre<1><<<bw.pre1[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.pre1[l].blk);
+                LAUNCH_CHECK();
+            }
         }
         if (bw.coarsest[l].nblk) {
             kb_coarsest<<<bw.coarsest[l].nblk, 64, 0, s>>>(
@@ -560,31 +672,44 @@ static void batch_vcycle(BatchWs &bw, hipStream_t s) {
             LAUNCH_CHECK();
         }
     }
+    // The current residual after iteration `parity`: r64[parity ^ 1]; in the first cycle r64[0].
+    const int cur = parity < 0 ? 0 : parity ^ 1;
     for (int l = L - 2; l >= 0; l--) {
         if (bw.post2[l].nblk) {
-            kb_spost<2><<<bw.post2[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.post2[l].blk, bw.parts);
+            if (l == 0)
+                kb_spost0<2><<<bw.post2[0].nblk, kBlock, 0, s>>>(bw.d_lv[0], bw.post2[0].blk, cur,
+                                                                 bw.parts);
+            else
+                kb_spost<2><<<bw.post2[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.post2[l].blk,
+                                                                bw.parts);
             LAUNCH_CHECK();
         }
         if (bw.post1[l].nblk) {
-            kb_spost<1><<<bw.post1[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.post1[l].blk,
-                                                            bw.parts + bw.rz_base_post1);
+            if (l == 0)
+                kb_spost0<1><<<bw.post1[0].nblk, kBlock, 0, s>>>(bw.d_lv[0], bw.post1[0].blk, cur,
+                                                                 bw.parts + bw.rz_base_post1);
+            else
+                kb_spost<1><<<bw.post1[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.post1[l].blk,
+                                                                bw.parts + bw.rz_base_post1);
             LAUNCH_CHECK();
         }
     }
 }
 
-// One iteration of every grid still running (MG:630-733), then the all-stopped flag.
+// One iteration of every grid still running (MG:630-733), then the all-stopped flag.  The grids
+// of one level take the separate update (their r.r partials after the finest pre-smoothings').
 static void batch_iteration(BatchWs &bw, int parity, hipStream_t s, int *host_flag) {
     const int n = bw.ngrids;
     kb_sdir<<<bw.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.band_l.blk, parity, bw.parts);
     LAUNCH_CHECK();
     kb_step<kStepPAP><<<n, kReduceBlock, 0, s>>>(bw.rng_band, bw.st, bw.parts);
     LAUNCH_CHECK();
-    kb_update<<<bw.update_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.update_l.blk, bw.parts);
-    LAUNCH_CHECK();
-    kb_step<kStepRR><<<n, kReduceBlock, 0, s>>>(bw.rng_update, bw.st, bw.parts);
-    LAUNCH_CHECK();
-    batch_vcycle(bw, s);
+    if (bw.update_l.nblk) {
+        kb_update<<<bw.update_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.update_l.blk,
+                                                      bw.parts + bw.rr_base_update);
+        LAUNCH_CHECK();
+    }
+    batch_vcycle(bw, s, parity);
     kb_step<kStepRZ><<<n, kReduceBlock, 0, s>>>(bw.rng_rz, bw.st, bw.parts);
     LAUNCH_CHECK();
     kb_all_done<<<1, kBlock, 0, s>>>(bw.st, n, bw.flag, host_flag);
@@ -617,13 +742,13 @@ static void batch_pass(dtd_ctx *ctx, BatchWs &bw, int from_b) {
     hipStream_t s = ctx->stream;
     const int n = bw.ngrids;
     HIPCHK(hipMemcpyAsync(bw.init, bw.h_init, n * sizeof(BInit), hipMemcpyHostToDevice, s));
-    kb_init<<<nblocks(n), kBlock, 0, s>>>(bw.st, bw.init, n, (int)ctx->opts.max_iter);
+    kb_init<<<nblocks(n), kBlock, 0, s>>>(bw.st, bw.init, bw.d_grids, n, (int)ctx->opts.max_iter);
     LAUNCH_CHECK();
     kb_sresid<<<bw.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.band_l.blk, from_b, bw.parts);
     LAUNCH_CHECK();
     kb_step<kStepRR0><<<n, kReduceBlock, 0, s>>>(bw.rng_band, bw.st, bw.parts);
     LAUNCH_CHECK();
-    batch_vcycle(bw, s);
+    batch_vcycle(bw, s, -1);
     kb_step<kStepRZ0><<<n, kReduceBlock, 0, s>>>(bw.rng_rz, bw.st, bw.parts);
     LAUNCH_CHECK();
     kb_all_done<<<1, kBlock, 0, s>>>(bw.st, n, bw.flag);
@@ -840,6 +965,7 @@ static void tune_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs, double re
         const double t_rule = time_policy(best);
         double t_best = t_rule;
         const int tall[] = {8, 12, 24, 32}, mid[] = {4, 12, 16}, small[] = {8}, f64[] = {8, 12, 24, 32};
+        const int fpre[] = {8, 24, 32, 48}, fpost[] = {8, 24, 32};
         auto sweep = [&](int BatchPolicy::*field, const int *cand, int nc) {
             for (int c = 0; c < nc; c++) {
                 BatchPolicy p = best;
@@ -848,6 +974,8 @@ static void tune_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs, double re
                 if (t < t_best) t_best = t, best = p;
             }
         };
+        sweep(&BatchPolicy::fine_pre, fpre, 4);
+        sweep(&BatchPolicy::fine_post, fpost, 3);
         sweep(&BatchPolicy::tall, tall, 4);
         sweep(&BatchPolicy::f64, f64, 4);
         sweep(&BatchPolicy::mid, mid, 3);

@@ -120,11 +120,16 @@ using PlanKey = std::array<int, 4>;  // w, h, wrap, cycle precision
 // rows, of 16 to 63 rows, of fewer; and the f64 passes of grids of at least 64 rows.
 struct BatchPolicy {
     int tall = 16, mid = 8, small = 4, f64 = 16;
+    // The finest level's fused pre- and post-smoothing (records formed from kd) of grids of at
+    // least 64 rows: their own heights, the pre-smoothing holding more registers.
+    int fine_pre = 32, fine_post = 24;
     bool operator==(const BatchPolicy &o) const {
-        return tall == o.tall && mid == o.mid && small == o.small && f64 == o.f64;
+        return tall == o.tall && mid == o.mid && small == o.small && f64 == o.f64 &&
+               fine_pre == o.fine_pre && fine_post == o.fine_post;
     }
     int level(int h) const { return h >= 64 ? tall : (h >= 16 ? mid : small); }
     int pass(int h) const { return h >= 64 ? f64 : (h >= 16 ? mid : small); }
+    int finest(bool pre, int h) const { return h >= 64 ? (pre ? fine_pre : fine_post) : level(h); }
 };
 
 struct BatchWs;
