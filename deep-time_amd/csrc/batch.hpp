@@ -236,7 +236,7 @@ __global__ __launch_bounds__(kBlock) void kb_spre0(const BLevel *__restrict__ t,
     rio.p = q.pd[parity];
     if (rm_update(RM)) rio.alpha = st->alpha;
     rio.kd = st->kd, rio.dt = st->dt, rio.s2 = st->s2, rio.rs2 = 1.0 / st->s2;
-    const double acc = spre_band<float, V, RM, 0>(q.L, q.L.z[0], q.C, q.th, block_wave(m.y), rio);
+    const double acc = spre_any<float, V, RM, 0>(q.L, q.L.z[0], q.C, q.th, block_wave(m.y), rio);
     if (rm_update(RM)) {
         const double sum = block_sum<kBlock>(acc, lds);
         if (threadIdx.x == 0) parts[blockIdx.x] = sum;
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void kb_spost0(const BLevel *__restrict__ t
     if (st->done) return;
     R64 rio{};
     rio.kd = st->kd, rio.dt = st->dt, rio.s2 = st->s2, rio.rs2 = 1.0 / st->s2;
-    const double acc = spost_band<float, V, kR64K, 0>(q.L, q.L.z[0], q.C.z[1], q.C.w, q.L.z[1],
+    const double acc = spost_any<float, V, kR64K, 0>(q.L, q.L.z[0], q.C.z[1], q.C.w, q.L.z[1],
                                                       q.th_post, q.r64[cur], block_wave(m.y), rio);
     const double sum = block_sum<kBlock>(acc, lds);
     if (threadIdx.x == 0) parts[blockIdx.x] = sum;

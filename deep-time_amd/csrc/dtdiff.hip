@@ -781,7 +781,9 @@ static size_t solve_core(dtd_ctx *ctx, const double *b, const double *kd, double
 // stay unless the plan is faster.  A band height changes which cells a block's partial sums
 // cover, so a tuned solve's sums round differently; the cycle's correction is unchanged.
 
-static constexpr int kTuneTh[] = {4, 8, 12, 16, 20, 24, 32, 48, 64};
+// Band heights tried: short bands in many rounds of waves, up to tall ones that cover a finest
+// level in about one round (the halo rows re-read then cost least).
+static constexpr int kTuneTh[] = {4, 8, 12, 16, 20, 24, 28, 32, 40, 48, 56, 64, 72, 96, 128};
 static constexpr int kTuneBottom[] = {256, 512, 1024, 2048};
 
 // Average duration (ms) of `reps` back-to-back launches enqueued by f, after one untimed.

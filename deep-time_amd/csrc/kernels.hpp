@@ -119,9 +119,22 @@ __device__ __forceinline__ double div_by(double q, double b, double z) {
     return y;
 }
 
-// `face` with the division by dx^2 as div_by (rs2 = 1 / dx^2): the same bits.
+// `face` with the division by dx^2 as div_by (rs2 = 1 / dx^2): the same bits.  P2: dx^2 is a
+// power of two, so rs2 is exact and q rs2 is the correctly rounded q / dx^2 (one rounding of the
+// same real number), with no correction steps.
+template <int P2 = 0>
 __device__ __forceinline__ double face_rz(double ka, double kb, double dt, double s2, double rs2) {
-    return div_by(0.5 * (ka + kb) * dt, s2, rs2);
+    if constexpr (P2)
+        return 0.5 * (ka + kb) * dt * rs2;
+    else
+        return div_by(0.5 * (ka + kb) * dt, s2, rs2);
+}
+
+// dx^2 is a normal power of two (its reciprocal is then exact).
+__host__ __device__ inline bool pow2_exact(double s2) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, s2);
+    const unsigned e = (unsigned)(b >> 52) & 0x7ffu;
+    return (b >> 63) == 0 && (b & ((1ull << 52) - 1)) == 0 && e >= 1 && e <= 2046;
 }
 
 // A x at one cell of an f64 lane in the order of `Operator::apply_row` (D:609-649), from its own
@@ -1090,22 +1103,40 @@ __device__ inline T from_west(T v) { return dpp_move<0x138>(v); }  // lane - 1's
 template <typename T>
 __device__ inline T from_east(T v) { return dpp_move<0x130>(v); }  // lane + 1's value
 
+// A load or store through the global address space: the streamed kernels' arrays are all device
+// memory, and a pointer that the compiler cannot place (one read from the state, say) would
+// otherwise take a flat access.
+template <typename T>
+__device__ __forceinline__ T gld(const T *p) {
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ __forceinline__ void gst(T *p, T v) {
+    *(__attribute__((address_space(1))) T *)p = v;
+}
+
+// Plain vector types for the wide loads and stores (HIP's float2 & co. are structs, whose copies
+// go through a generic pointer).
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
 template <typename T, int V>
 __device__ inline void loadv(const T *p, T (&d)[V]) {
     if constexpr (V * sizeof(T) == 8 && V == 2) {
-        const float2 q = *reinterpret_cast<const float2 *>(p);
+        const f32x2 q = gld(reinterpret_cast<const f32x2 *>(p));
         d[0] = q.x, d[1] = q.y;
     } else if constexpr (V * sizeof(T) == 16) {
         if constexpr (sizeof(T) == 4) {
-            const float4 q = *reinterpret_cast<const float4 *>(p);
+            const f32x4 q = gld(reinterpret_cast<const f32x4 *>(p));
             d[0] = q.x, d[1] = q.y, d[2] = q.z, d[3] = q.w;
         } else {
-            const double2 q = *reinterpret_cast<const double2 *>(p);
+            const f64x2 q = gld(reinterpret_cast<const f64x2 *>(p));
             d[0] = q.x, d[1] = q.y;
         }
     } else {
 #pragma unroll
-        for (int v = 0; v < V; v++) d[v] = p[v];
+        for (int v = 0; v < V; v++) d[v] = gld(p + v);
     }
 }
 // The operator records of V consecutive cells (kRec values each): e, s, diag.
@@ -1173,15 +1204,15 @@ __device__ inline void weights(const Lvl<T> &L, const T (&d)[V], const T (&wdq)[
 template <typename T, int V>
 __device__ inline void storev(T *p, const T (&d)[V]) {
     if constexpr (V * sizeof(T) == 8 && V == 2) {
-        *reinterpret_cast<float2 *>(p) = make_float2(d[0], d[1]);
+        gst(reinterpret_cast<f32x2 *>(p), f32x2{d[0], d[1]});
     } else if constexpr (V * sizeof(T) == 16) {
         if constexpr (sizeof(T) == 4)
-            *reinterpret_cast<float4 *>(p) = make_float4(d[0], d[1], d[2], d[3]);
+            gst(reinterpret_cast<f32x4 *>(p), f32x4{d[0], d[1], d[2], d[3]});
         else
-            *reinterpret_cast<double2 *>(p) = make_double2(d[0], d[1]);
+            gst(reinterpret_cast<f64x2 *>(p), f64x2{d[0], d[1]});
     } else {
 #pragma unroll
-        for (int v = 0; v < V; v++) p[v] = d[v];
+        for (int v = 0; v < V; v++) gst(p + v, d[v]);
     }
 }
 
@@ -1298,7 +1329,11 @@ __device__ inline Band band_at(int w, int h, int wrap, int th, int wave) {
 }
 
 // The wave number of this thread's wave when a grid's waves start at block `block`.
-__device__ inline int block_wave(int block) { return block * (kBlock / 64) + (threadIdx.x >> 6); }
+// The same in every lane of the wave, and marked so (readfirstlane): band rows and row offsets
+// derived from it stay in scalar registers, and tests on them are scalar branches.
+__device__ inline int block_wave(int block) {
+    return __builtin_amdgcn_readfirstlane(block * (kBlock / 64) + (int)(threadIdx.x >> 6));
+}
 
 // The launch's blocks go to the 8 XCDs in turn (block b to XCD b mod 8).  Numbering the bands so
 // that each XCD takes one contiguous run of them puts a band and the band below it (which reads
@@ -1370,7 +1405,7 @@ struct KRow {
     double e[V], s[V], d[V];
     double ew;  // the west face of the lane's first cell
 };
-template <int V>
+template <int V, int P2 = 0>
 __device__ __forceinline__ void krow(const double (&K)[V], const double (&Ks)[V], double kx,
                                      const double (&sn)[V], int gi0, int j, int w, int h,
                                      int wrap, double dt, double s2, double rs2, KRow<V> &o) {
@@ -1380,11 +1415,11 @@ __device__ __forceinline__ void krow(const double (&K)[V], const double (&Ks)[V]
 #pragma unroll
     for (int v = 0; v < V; v++) {
         const bool has_e = gi0 + v + 1 < w || wrap;
-        o.e[v] = has_e ? face_rz(K[v], v + 1 < V ? K[v + 1] : ke, dt, s2, rs2) : 0.0;
-        o.s[v] = j + 1 < h ? face_rz(K[v], Ks[v], dt, s2, rs2) : 0.0;
+        o.e[v] = has_e ? face_rz<P2>(K[v], v + 1 < V ? K[v + 1] : ke, dt, s2, rs2) : 0.0;
+        o.s[v] = j + 1 < h ? face_rz<P2>(K[v], Ks[v], dt, s2, rs2) : 0.0;
     }
     const double ewl = from_west(o.e[V - 1]);
-    o.ew = lane == 0 ? face_rz(kx, K[0], dt, s2, rs2) : ewl;
+    o.ew = lane == 0 ? face_rz<P2>(kx, K[0], dt, s2, rs2) : ewl;
 #pragma unroll
     for (int v = 0; v < V; v++)
         o.d[v] = lane_diag64(o.e[v], v > 0 ? o.e[v - 1] : o.ew, o.s[v], sn[v], gi0 + v, j, wrap);
@@ -1412,10 +1447,11 @@ __device__ __forceinline__ int end_column(const Band &bd, int w, int wrap, bool 
 // Returns the lane's share of r.r (kRUpdate, kRUpdateK; else 0).  With kR64K and kRUpdateK the
 // records are formed from kd: kd (and for kRUpdateK the direction p) come one row ahead of r, the
 // south faces and A p of a row needing the row below; kRUpdateK's A p is the direction pass's.
-template <typename T, int V, int RM = kRLevel, int CB = 0>
+template <typename T, int V, int RM = kRLevel, int CB = 0, int PW2 = 0>
 __device__ __forceinline__ double spre_band(const Lvl<T> &L, T *__restrict__ zout,
                                             const Lvl<T> &C, int th, int wave, R64 rio = R64{}) {
-    constexpr int A = kSmoothAhead;  // rows in flight
+    // Rows in flight; with the records from kd (edge bands only) two, as the post-smoothing.
+    constexpr int A = rm_kd(RM) ? 2 : kSmoothAhead;
     constexpr bool KD = rm_kd(RM), UPD = rm_update(RM), UPK = RM == kRUpdateK;
     const int w = L.w, h = L.h, wrap = L.wrap;
     const Band bd = band_at<V>(w, h, wrap, th, wave);
@@ -1489,7 +1525,7 @@ __device__ __forceinline__ double spre_band(const Lvl<T> &L, T *__restrict__ zou
             }
 #pragma unroll
             for (int v = 0; v < V; v++)
-                S0[v] = j3 >= 0 && j3 + 1 < h ? face_rz(K3[v], K1[v], rio.dt, rio.s2, rio.rs2) : 0.0;
+                S0[v] = j3 >= 0 && j3 + 1 < h ? face_rz<PW2>(K3[v], K1[v], rio.dt, rio.s2, rio.rs2) : 0.0;
         }
     }
 #pragma unroll
@@ -1511,7 +1547,7 @@ __device__ __forceinline__ double spre_band(const Lvl<T> &L, T *__restrict__ zou
                 if constexpr (UPK) copy(P2, qp[UPK ? u : 0]), px = qpx[UPK ? u : 0];
                 if (k0 + u + A < steps) DTD_FETCH_PRE(ja + A, u);
                 KRow<V> kr;
-                krow<V>(K1, K2, kx, S0, gi0, ja, w, h, wrap, rio.dt, rio.s2, rio.rs2, kr);
+                krow<V, PW2>(K1, K2, kx, S0, gi0, ja, w, h, wrap, rio.dt, rio.s2, rio.rs2, kr);
                 double rv[V];
                 if constexpr (UPK) {
                     // A p of row a, the direction pass's expression; r -= alpha A p.
@@ -1607,6 +1643,244 @@ __device__ __forceinline__ double spre_band(const Lvl<T> &L, T *__restrict__ zou
     return acc_rr;
 }
 
+// ---- interior bands of the finest level with kd-formed records -------------------------------
+//
+// Most waves of the finest level's fused smoothing own a band that touches no edge of the grid:
+// every lane's cells have unwrapped east and west neighbours, the end lanes' outer columns lie on
+// the grid, and the rows the walk reads (three beyond the band on either side) do too.  Such a
+// band takes the code below: the same cells, operations and order as the general band code takes
+// on its inner path, with every boundary test, zeroing and select folded away, the loads
+// unconditional, and the end lanes' outer neighbours brought in by the lane moves themselves.
+
+// v one lane over by a DPP wave shift; the lane with no source gets 0 (bound_ctrl) where the
+// value is not used, or `old`.
+template <int CTRL, typename T>
+__device__ inline T dpp_zero(T v) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(
+            T, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+    } else {
+        const long long b = __builtin_bit_cast(long long, v);
+        const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffff), CTRL, 0xF, 0xF, true);
+        const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+        return __builtin_bit_cast(T, ((long long)hi << 32) | (unsigned)lo);
+    }
+}
+__device__ inline double dpp_or(double v, double old, int east) {
+    const long long b = __builtin_bit_cast(long long, v), o = __builtin_bit_cast(long long, old);
+    int lo, hi;
+    if (east) {
+        lo = __builtin_amdgcn_update_dpp((int)(o & 0xffffffff), (int)(b & 0xffffffff), 0x130, 0xF, 0xF, false);
+        hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), 0x130, 0xF, 0xF, false);
+    } else {
+        lo = __builtin_amdgcn_update_dpp((int)(o & 0xffffffff), (int)(b & 0xffffffff), 0x138, 0xF, 0xF, false);
+        hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), 0x138, 0xF, 0xF, false);
+    }
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+// The wave's band needs no boundary rule (the same answer in every lane, so the branch on it is
+// uniform): `margin` rows above and below the band are on the level.
+template <int V>
+__device__ inline bool interior_band(const Band &bd, int h, int margin) {
+    const bool ok = bd.active && bd.inner && bd.row0 - margin >= 0 &&
+                    bd.row0 + bd.rows + margin <= h;
+    return __all(ok);
+}
+
+// A x for an interior row of two-cell lanes, `apply_row`'s five terms in order; the outer lanes'
+// outer values are not used.
+__device__ __forceinline__ void in_ax(const float (&x)[2], const float (&xs)[2],
+                                      const float (&xn)[2], const float (&d)[2],
+                                      const float (&e)[2], const float (&s)[2],
+                                      const float (&sn)[2], float (&ax)[2]) {
+    const float xe0 = dpp_zero<0x130>(x[0]), xwl = dpp_zero<0x138>(x[1]);
+    const float ewl = dpp_zero<0x138>(e[1]);
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+        const float xe = v == 0 ? x[1] : xe0;
+        const float xw = v == 1 ? x[0] : xwl;
+        const float ew = v == 1 ? e[0] : ewl;
+        float o = d[v] * x[v];
+        o -= e[v] * xe;
+        o -= ew * xw;
+        o -= s[v] * xs[v];
+        o -= sn[v] * xn[v];
+        ax[v] = o;
+    }
+}
+__device__ __forceinline__ void in_sweep(const float (&x)[2], const float (&xs)[2],
+                                         const float (&xn)[2], const float (&r)[2],
+                                         const float (&wd)[2], const float (&d)[2],
+                                         const float (&e)[2], const float (&s)[2],
+                                         const float (&sn)[2], float (&out)[2]) {
+    float ax[2];
+    in_ax(x, xs, xn, d, e, s, sn, ax);
+#pragma unroll
+    for (int v = 0; v < 2; v++) out[v] = x[v] + wd[v] * (r[v] - ax[v]);
+}
+
+// An interior row's f64 faces and diagonal from kd (krow with every face present): K the row's
+// kd, Ks the row below's, kx the end lane's outer column's kd (lane 0: west of its first cell;
+// lane 63: east of its last), sn the row above's south faces.
+template <int PW2>
+__device__ __forceinline__ void in_krow(const double (&K)[2], const double (&Ks)[2], double kx,
+                                        const double (&sn)[2], double dt, double s2, double rs2,
+                                        KRow<2> &o) {
+    const double ke = dpp_or(K[0], kx, 1);
+    o.e[0] = face_rz<PW2>(K[0], K[1], dt, s2, rs2);
+    o.e[1] = face_rz<PW2>(K[1], ke, dt, s2, rs2);
+    o.s[0] = face_rz<PW2>(K[0], Ks[0], dt, s2, rs2);
+    o.s[1] = face_rz<PW2>(K[1], Ks[1], dt, s2, rs2);
+    // Lane 0's west face from its outer column; the others' from the lane to the west.
+    o.ew = dpp_or(o.e[1], face_rz<PW2>(kx, K[0], dt, s2, rs2), 0);
+    o.d[0] = 1.0 + o.e[0] + o.ew + o.s[0] + sn[0];
+    o.d[1] = 1.0 + o.e[1] + o.e[0] + o.s[1] + sn[1];
+}
+
+// spre_band for an interior band of the finest level of the f32 cycle, two cells a lane, records
+// formed from kd (RM = kR64K, kRUpdateK).
+template <int RM, int PW2>
+__device__ __forceinline__ double spre_band_in(const Lvl<float> &L, float *__restrict__ zout,
+                                               const Lvl<float> &C, const Band &bd, R64 rio) {
+    constexpr int V = 2, Q = 4;  // queue slots: a slot holds a row, refilled in place
+    constexpr bool UPK = RM == kRUpdateK;
+    const int w = L.w;
+    const int lane = threadIdx.x & 63;
+    const int gi0 = bd.gi0;
+    // The end lanes' outer columns; the other lanes load the nearer end lane's (the same line),
+    // unused.
+    const int gx = lane < 32 ? gi0 - V * lane - 1 : gi0 + V * (63 - lane) + V;
+    const bool own_l = bd.own;
+    const float omega = L.omega;
+    // Rows by slot, at the step of row ja (slot u): r, kd and the end columns' kd and p of rows
+    // ja .. ja + 3 (row ja + k in slot (u + k) mod 4), p of rows ja - 1 .. ja + 2 (row ja - 1 + k
+    // in slot (u + k) mod 4).  After the step, slot u takes rows ja + 4 (and p row ja + 3): no
+    // loaded value outlives its slot, so every load lands in the register its reader uses.
+    double qr64[Q][V], qk[Q][V], qkx[Q], qp[UPK ? Q : 1][V], qpx[UPK ? Q : 1];
+    // The last rows the walk reads: r (and the end columns) to row0 + rows + 1, kd and p one
+    // further.  The queue's loads past them take those rows again (cache hits, values unused):
+    // unconditional loads keep the loop's counter waits exact.
+    const int jr = bd.row0 + bd.rows + 1, jk = jr + 1;
+    auto fetch = [&](int j, int u) {
+        const size_t g = (size_t)min(j, jr) * w, gk = (size_t)min(j, jk) * w;
+        loadv<double, V>(rio.in + g + gi0, qr64[u]);
+        loadv<double, V>(rio.kd + gk + gi0, qk[u]);
+        qkx[u] = gld(rio.kd + g + gx);
+        if constexpr (UPK) qpx[UPK ? u : 0] = gld(rio.p + g + gx);
+    };
+    auto fetch_p = [&](int j, int u) {
+        if constexpr (UPK) loadv<double, V>(rio.p + (size_t)min(j, jk) * w + gi0, qp[UPK ? u : 0]);
+    };
+    float z1a[V], z1b[V], z1c[V], z2b[V], z2c[V], z2d[V];
+    float ra[V], rb[V], rc[V], wda[V], wdb[V], dga[V], dgb[V], dgc[V];
+    float ea[V], eb[V], ec[V], sa[V], sb[V], sc[V], sd[V];
+    zero(z1a), zero(z1b), zero(z2b), zero(z2c), zero(ra), zero(rb), zero(wda), zero(dga);
+    zero(dgb), zero(ea), zero(eb), zero(sa), zero(sb), zero(sc);
+    float acc[V];
+    zero(acc);
+    double acc_rr = 0.0;
+    double S0[V];
+    const int base = bd.row0 - 2;  // the walk's first row a
+    {
+        // The south faces of row base - 1.
+        double K3[V], K2[V];
+        loadv<double, V>(rio.kd + (size_t)(base - 1) * w + gi0, K3);
+        loadv<double, V>(rio.kd + (size_t)base * w + gi0, K2);
+#pragma unroll
+        for (int v = 0; v < V; v++) S0[v] = face_rz<PW2>(K3[v], K2[v], rio.dt, rio.s2, rio.rs2);
+    }
+#pragma unroll
+    for (int u = 0; u < Q; u++) fetch(base + u, u), fetch_p(base - 1 + u, u);
+    // The queue's first rows in: from here the compiler's counter waits in the loop are exact
+    // (vmcnt(0) expcnt(7) lgkmcnt(15)).
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    const int steps = bd.rows + 4;  // a multiple of Q
+    for (int k0 = 0; k0 < steps; k0 += Q) {
+#pragma unroll
+        for (int u = 0; u < Q; u++) {
+            const int ja = base + k0 + u, jb = ja - 1, jc = ja - 2;
+            const int u1 = (u + 1) % Q, u2 = (u + 2) % Q;
+            copy(z1c, z1b), copy(z1b, z1a), copy(z2d, z2c), copy(z2c, z2b);
+            copy(rc, rb), copy(rb, ra), copy(wdb, wda), copy(dgc, dgb), copy(dgb, dga);
+            copy(ec, eb), copy(eb, ea), copy(sd, sc), copy(sc, sb), copy(sb, sa);
+            // Row a's records from kd (rows a and a + 1), then A p of row a (p rows a - 1 .. a + 1).
+            KRow<V> kr;
+            in_krow<PW2>(qk[u], qk[u1], qkx[u], S0, rio.dt, rio.s2, rio.rs2, kr);
+            double rv[V];
+            if constexpr (UPK) {
+                // The direction pass's expression; r -= alpha A p.
+                const double(&P0)[V] = qp[UPK ? u : 0];
+                const double(&P1)[V] = qp[UPK ? u1 : 0];
+                const double(&P2)[V] = qp[UPK ? u2 : 0];
+                const double px = qpx[UPK ? u : 0];
+                const double pe = dpp_or(P1[0], px, 1);
+                const double pw = dpp_or(P1[1], px, 0);
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    const double ew = v > 0 ? kr.e[v - 1] : kr.ew;
+                    double o = kr.d[v] * P1[v];
+                    o -= kr.e[v] * (v + 1 < V ? P1[v + 1] : pe);
+                    o -= ew * (v > 0 ? P1[v - 1] : pw);
+                    o -= kr.s[v] * P2[v];
+                    o -= S0[v] * P0[v];
+                    rv[v] = qr64[u][v] - rio.alpha * o;
+                }
+            } else {
+                copy(rv, qr64[u]);
+            }
+            copy(S0, kr.s);
+            // Slot u is spent: rows a + 4 (p: a + 3) go into it.
+            fetch(ja + Q, u), fetch_p(ja + Q - 1, u);
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                ea[v] = (float)kr.e[v], sa[v] = (float)kr.s[v], dga[v] = (float)kr.d[v];
+                ra[v] = (float)rv[v];
+                wda[v] = dga[v] != 0.0f ? omega / dga[v] : 0.0f;
+            }
+            if (UPK && own_l && ja >= bd.row0 && ja < bd.row0 + bd.rows) {
+                storev<double, V>(rio.out + (size_t)ja * w + gi0, rv);
+#pragma unroll
+                for (int v = 0; v < V; v++) acc_rr += rv[v] * rv[v];
+            }
+#pragma unroll
+            for (int v = 0; v < V; v++) z1a[v] = wda[v] * ra[v];
+            // Second sweep of row b.
+            in_sweep(z1b, z1a, z1c, rb, wdb, dgb, eb, sb, sc, z2b);
+            if (own_l && jb >= bd.row0 && jb < bd.row0 + bd.rows)
+                storev<float, V>(zout + (size_t)jb * w + gi0, z2b);
+            // Residual of row c, restricted in 2 x 2 blocks.
+            float res[V];
+            in_ax(z2c, z2b, z2d, dgc, ec, sc, sd, res);
+#pragma unroll
+            for (int v = 0; v < V; v++) res[v] = rc[v] - res[v];
+            if (own_l && jc >= bd.row0) {
+                if ((jc & 1) == 0) acc[0] = 0.0f;
+                acc[0] += res[0];
+                acc[0] += res[1];
+                if (jc & 1) gst(C.r + (size_t)(jc >> 1) * C.w + (gi0 >> 1), acc[0]);
+            }
+        }
+    }
+    return acc_rr;
+}
+
+// One band of a level's fused pre-smoothing: an interior band of the finest kd-formed level
+// takes spre_band_in (with the exact multiply when dx^2 is a power of two), any other the
+// general band code.
+template <typename T, int V, int RM, int CB>
+__device__ __forceinline__ double spre_any(const Lvl<T> &L, T *__restrict__ zout, const Lvl<T> &C,
+                                           int th, int wave, R64 rio) {
+    if constexpr (rm_kd(RM) && V == 2 && CB == 0 && std::is_same<T, float>::value) {
+        const Band bd = band_at<V>(L.w, L.h, L.wrap, th, wave);
+        if (interior_band<V>(bd, L.h, 3)) {
+            if (pow2_exact(rio.s2)) return spre_band_in<RM, 1>(L, zout, C, bd, rio);
+            return spre_band_in<RM, 0>(L, zout, C, bd, rio);
+        }
+    }
+    return spre_band<T, V, RM, CB, 0>(L, zout, C, th, wave, rio);
+}
+
 // With RM = kRUpdate or kRUpdateK the residual's update of this iteration rides on the pass
 // (alpha from st) and parts[block] gets the block's r.r; kR64K and kRUpdateK take kd, dt and dx^2
 // for the records from st.
@@ -1618,7 +1892,7 @@ __global__ __launch_bounds__(kBlock) void k_spre(Lvl<T> L, T *__restrict__ zout,
     if (st && st->done) return;
     if (rm_update(RM)) rio.alpha = st->alpha;
     if (rm_kd(RM)) rio.kd = st->kd, rio.dt = st->dt, rio.s2 = st->s2, rio.rs2 = 1.0 / st->s2;
-    const double acc = spre_band<T, V, RM, CB>(L, zout, C, th, launch_wave(), rio);
+    const double acc = spre_any<T, V, RM, CB>(L, zout, C, th, launch_wave(), rio);
     if (rm_update(RM)) {
         const double sum = block_sum<kBlock>(acc, lds);
         if (threadIdx.x == 0) parts[blockIdx.x] = sum;
@@ -1630,7 +1904,7 @@ __global__ __launch_bounds__(kBlock) void k_spre(Lvl<T> L, T *__restrict__ zout,
 // Returns the lane's share of sum dot * z4 (0 without dot).  With RM = kR64 (kR64K), r is the f64
 // residual dot itself, rounded to T as it is read; kR64K forms the records from kd (rio.kd, dt,
 // s2): row b's when row a's kd arrives (its south faces).
-template <typename T, int V, int RM = kRLevel, int CB = 0>
+template <typename T, int V, int RM = kRLevel, int CB = 0, int PW2 = 0>
 __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restrict__ zin,
                                              const T *__restrict__ zc, int wc,
                                              T *__restrict__ zout, int th,
@@ -1702,7 +1976,7 @@ __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restric
                 double KA[V];
                 copy(KA, qk[KD ? u : 0]);
                 KRow<V> kr;
-                krow<V>(KB, KA, kxb, S0, gi0, jb, w, h, wrap, rio.dt, rio.s2, rio.rs2, kr);
+                krow<V, PW2>(KB, KA, kxb, S0, gi0, jb, w, h, wrap, rio.dt, rio.s2, rio.rs2, kr);
 #pragma unroll
                 for (int v = 0; v < V; v++)
                     eb[v] = (T)kr.e[v], sb[v] = (T)kr.s[v], dgb[v] = (T)kr.d[v];
@@ -1753,6 +2027,99 @@ __device__ __forceinline__ double spost_band(const Lvl<T> &L, const T *__restric
     return acc;
 }
 
+// spost_band for an interior band of the finest level of the f32 cycle, two cells a lane, records
+// formed from kd (RM = kR64K): r is the f64 residual `dot`, rounded as it is read.  Two rows of
+// loads in flight, four rows a trip.
+template <int PW2, int A = 2>
+__device__ __forceinline__ double spost_band_in(const Lvl<float> &L, const float *__restrict__ zin,
+                                                const float *__restrict__ zc, int wc,
+                                                float *__restrict__ zout,
+                                                const double *__restrict__ dot, const Band &bd,
+                                                R64 rio) {
+    constexpr int V = 2;
+    const int w = L.w;
+    const int lane = threadIdx.x & 63;
+    const int gi0 = bd.gi0;
+    const int gx = lane < 32 ? gi0 - V * lane - 1 : gi0 + V * (63 - lane) + V;
+    const bool own_l = bd.own;
+    const float omega = L.omega;
+    double qr64[A][V], qk[A][V], qkx[A];
+    float qz[A][V], qc[A];
+    auto fetch = [&](int j, int u) {
+        const size_t g = (size_t)j * w;
+        loadv<double, V>(dot + g + gi0, qr64[u]);
+        loadv<float, V>(zin + g + gi0, qz[u]);
+        loadv<double, V>(rio.kd + g + gi0, qk[u]);
+        qkx[u] = gld(rio.kd + g + gx);
+        qc[u] = gld(zc + (size_t)(j >> 1) * wc + (gi0 >> 1));
+    };
+    float zpa[V], zpb[V], zpc[V], z3b[V], z3c[V], z3d[V];
+    float ra[V], rb[V], rc[V], wdb[V], wdc[V], dgb[V], dgc[V];
+    float eb[V], ec[V], sb[V], sc[V], sd[V];
+    zero(zpa), zero(zpb), zero(z3b), zero(z3c), zero(ra), zero(rb), zero(wdb);
+    zero(dgb), zero(eb), zero(sb), zero(sc);
+    double r64a[V], r64b[V], r64c[V];
+    zero(r64a), zero(r64b), zero(r64c);
+    double KB[V], S0[V], kxb = 0.0;
+    zero(KB), zero(S0);
+    double acc = 0.0;
+    const int steps = bd.rows + 4;  // a multiple of 4
+#pragma unroll
+    for (int u = 0; u < A; u++) fetch(bd.row0 - 2 + u, u);
+    for (int k0 = 0; k0 < steps; k0 += 4) {
+#pragma unroll
+        for (int uu = 0; uu < 4; uu++) {
+            const int u = uu % A;
+            const int ja = bd.row0 - 2 + k0 + uu, jc = ja - 2;
+            copy(zpc, zpb), copy(zpb, zpa), copy(z3d, z3c), copy(z3c, z3b);
+            copy(rc, rb), copy(rb, ra), copy(wdc, wdb), copy(dgc, dgb), copy(ec, eb);
+            copy(sd, sc), copy(sc, sb);
+            // Row b's records, its south faces from row a's kd (just arrived).
+            KRow<V> kr;
+            in_krow<PW2>(KB, qk[u], kxb, S0, rio.dt, rio.s2, rio.rs2, kr);
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                eb[v] = (float)kr.e[v], sb[v] = (float)kr.s[v], dgb[v] = (float)kr.d[v];
+                wdb[v] = dgb[v] != 0.0f ? omega / dgb[v] : 0.0f;
+            }
+            copy(S0, kr.s), copy(KB, qk[u]);
+            kxb = qkx[u];
+            copy(r64c, r64b), copy(r64b, r64a), copy(r64a, qr64[u]);
+#pragma unroll
+            for (int v = 0; v < V; v++) ra[v] = (float)r64a[v];
+#pragma unroll
+            for (int v = 0; v < V; v++) zpa[v] = qz[u][v] + qc[u];
+            fetch(min(ja + A, bd.row0 + bd.rows + 1), u);  // past the walk: its last row again
+            in_sweep(zpb, zpa, zpc, rb, wdb, dgb, eb, sb, sc, z3b);
+            float z4[V];
+            in_sweep(z3c, z3b, z3d, rc, wdc, dgc, ec, sc, sd, z4);
+            if (own_l && jc >= bd.row0) {
+                storev<float, V>(zout + (size_t)jc * w + gi0, z4);
+#pragma unroll
+                for (int v = 0; v < V; v++) acc += r64c[v] * (double)z4[v];
+            }
+        }
+    }
+    return acc;
+}
+
+// One band of a level's fused post-smoothing (interior bands of the finest kd-formed level take
+// spost_band_in).
+template <typename T, int V, int RM, int CB>
+__device__ __forceinline__ double spost_any(const Lvl<T> &L, const T *__restrict__ zin,
+                                            const T *__restrict__ zc, int wc, T *__restrict__ zout,
+                                            int th, const double *__restrict__ dot, int wave,
+                                            R64 rio) {
+    if constexpr (rm_kd(RM) && V == 2 && CB == 0 && std::is_same<T, float>::value) {
+        const Band bd = band_at<V>(L.w, L.h, L.wrap, th, wave);
+        if (interior_band<V>(bd, L.h, 3)) {
+            if (pow2_exact(rio.s2)) return spost_band_in<1>(L, zin, zc, wc, zout, dot, bd, rio);
+            return spost_band_in<0>(L, zin, zc, wc, zout, dot, bd, rio);
+        }
+    }
+    return spost_band<T, V, RM, CB, 0>(L, zin, zc, wc, zout, th, dot, wave, rio);
+}
+
 template <typename T, int V, int RM = kRLevel, int CB = 0>
 __global__ __launch_bounds__(kBlock) void k_spost(Lvl<T> L, const T *__restrict__ zin,
                                                   const T *__restrict__ zc, int wc,
@@ -1764,8 +2131,7 @@ __global__ __launch_bounds__(kBlock) void k_spost(Lvl<T> L, const T *__restrict_
     if (st && st->done) return;
     R64 rio{};
     if (rm_kd(RM)) rio.kd = st->kd, rio.dt = st->dt, rio.s2 = st->s2, rio.rs2 = 1.0 / st->s2;
-    const double acc =
-        spost_band<T, V, RM, CB>(L, zin, zc, wc, zout, th, dot, launch_wave(), rio);
+    const double acc = spost_any<T, V, RM, CB>(L, zin, zc, wc, zout, th, dot, launch_wave(), rio);
     if (dot) {
         const double sum = block_sum<kBlock>(acc, lds);
         if (threadIdx.x == 0) parts[blockIdx.x] = sum;

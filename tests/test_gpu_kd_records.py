@@ -6,7 +6,7 @@ With kd-formed records the pre-smoothing forms each cell's faces 0.5 (K_a + K_b)
 A p from kd and the direction p itself, and the post-smoothing forms the faces again, instead of
 reading (east, south, diag) records and the direction pass's stored A p.  The division by dx^2 is
 a reciprocal with two fma corrections there (correctly rounded); a dx whose square is not a power
-of two exercises it.  Shapes: periodic and closed, widths that are not a multiple of a band (124
+of two exercises it, and a power of two (1, 4, 1/8) the exact multiply by 1 / dx^2 taken instead.  Shapes: periodic and closed, widths that are not a multiple of a band (124
 columns), odd widths (one-column lanes), heights that are not a multiple of a band's rows.
 """
 import numpy as np
@@ -20,7 +20,8 @@ from dtdiff.hipmem import DeviceArray
 pytestmark = pytest.mark.gpu
 
 CASES = [(1024, 512, True, 2400.0), (1000, 300, False, 3.7), (317, 129, True, 2400.0),
-         (250, 77, False, 1.0), (2048, 1030, True, 0.3)]
+         (250, 77, False, 1.0), (2048, 1030, True, 0.3), (512, 256, True, 4.0),
+         (300, 100, False, 0.125)]
 
 
 def solve(monkeypatch, d, h0, kd, dt, kd_records):
