@@ -789,8 +789,10 @@ static void batch_pass(dtd_ctx *ctx, BatchWs &bw, int from_b) {
 static void solve_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs_in, double rel_tol,
                         size_t *iterations, double *rels, dtd_batch_stats *stats) {
     if (ctx->opts.cycle_precision != DTD_CYCLE_F32 || ctx->opts.nu != 2 ||
-        ctx->opts.transfer != DTD_TRANSFER_AGGREGATE || ctx->opts.coarsest != DTD_COARSEST_SWEEPS)
-        throw Error{DTD_EINVAL, "the batched solve runs the reference's f32 cycle with nu = 2"};
+        ctx->opts.transfer != DTD_TRANSFER_AGGREGATE || ctx->opts.coarsest != DTD_COARSEST_SWEEPS ||
+        ctx->opts.sums != DTD_SUMS_BLOCKS)
+        throw Error{DTD_EINVAL, "the batched solve runs the reference's f32 cycle with nu = 2, "
+                                "its sums per block"};
     const bool relative = rel_tol > 0.0;
     if (!std::isfinite(rel_tol)) throw Error{DTD_EINVAL, "rel_tol must be finite"};
     // The non-empty grids take part; the others get 0 iterations and residual 0.

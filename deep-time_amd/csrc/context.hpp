@@ -90,6 +90,7 @@ struct Workspace {
     double *ap = nullptr, *kd = nullptr, *parts = nullptr, *pair = nullptr;
     long nparts = 0;  // blocks whose partial sums parts holds (two doubles a block)
     double2 *es = nullptr;  // the f64 operator: (east, south) of each cell
+    double *sums = nullptr;  // DTD_SUMS_STRIPS: strip sums (two sets) and the put-back's row pairs
     CgState *st = nullptr;
     std::vector<Lvl<float>> lf;
     std::vector<Lvl<double>> ld;

@@ -73,6 +73,8 @@ static void lay_out(Workspace &ws, const dtd_opts &o, int w, int h, int wrap, bo
     ws.pair = f64(2);
     ws.st = (CgState *)take(sizeof(CgState));
     if (!lean) ws.es = (double2 *)take((n ? n : 1) * sizeof(double2));
+    if (!lean && o.sums == DTD_SUMS_STRIPS)
+        ws.sums = f64(2 * (size_t)((h + kStripRows - 1) / kStripRows) + 2 * (size_t)h);
     auto sh = shapes(w, h, wrap);
     auto own = [&](size_t cnt, size_t elt) { return take((cnt ? cnt : 1) * elt); };
     for (size_t l = 0; l < sh.size(); l++) {
@@ -306,8 +308,10 @@ static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt)
         pack_bf16(ws, s);
         return;
     }
-    // The f64 passes form the faces from kd themselves; only the cycle's finest records are kept.
-    k_operator<<<nblocks(ws.n), kBlock, 0, s>>>(grid_of(ws), kd, dt, dx * dx, nullptr,
+    // The f64 passes form the faces from kd themselves; only the cycle's finest records are kept
+    // (and, for the reference's summation order, the f64 faces its unfused passes read).
+    k_operator<<<nblocks(ws.n), kBlock, 0, s>>>(grid_of(ws), kd, dt, dx * dx,
+                                                ctx->opts.sums == DTD_SUMS_STRIPS ? ws.es : nullptr,
                                                 f64 ? nullptr : ws.lf[0].coef,
                                                 f64 ? ws.ld[0].coef : nullptr);
     LAUNCH_CHECK();
@@ -555,7 +559,8 @@ static bool fused_update(const Workspace &ws, const dtd_opts &o) {
 // Whether the fused finest level forms its records (and the pre-smoothing A p) from kd and p:
 // the f32 cycle without bf16 records on the finest level.
 static bool kd_records(const dtd_ctx *ctx) {
-    return ctx->use_kd && fused_update(ctx->ws, ctx->opts) && !ctx->ws.lf[0].coefh;
+    return ctx->use_kd && ctx->opts.sums != DTD_SUMS_STRIPS && fused_update(ctx->ws, ctx->opts) &&
+           !ctx->ws.lf[0].coefh;
 }
 
 // One conjugate-gradient iteration (MG:630-733): parity 0 reads the direction from p1 and writes
@@ -640,7 +645,8 @@ static void lod(dtd_ctx *ctx, const double *b, double *x, const double *kd, doub
     const Grid g = grid_of(ws);
     const double s2 = dx * dx;
     hipStream_t s = ctx->stream;
-    if (parallel_lod_enabled() && parallel_line(ws.w) && parallel_line(ws.h)) {
+    if (parallel_lod_enabled() && ctx->opts.sums != DTD_SUMS_STRIPS && parallel_line(ws.w) &&
+        parallel_line(ws.h)) {
         // Rows from b into x; then the columns as rows of the transposed grid (kd^T in p0, x^T
         // in p1), transposed back into x.
         auto lines = [&](int nlines, int n, int ring, const double *k, const double *src,
@@ -682,6 +688,123 @@ static void copy_answer(dtd_ctx *ctx, double *h_dev) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
 }
 
+// The last update, the put-back and the true residual, as solve_core ends (k_finish2 from x' in
+// ws.ap with the state's lost and weight); the pair of sums (r.r, b.b) into ws.pair.
+static void finish_put_back(dtd_ctx *ctx, const double *b, const double *kd, double dx, double dt) {
+    Workspace &ws = ctx->ws;
+    hipStream_t s = ctx->stream;
+    const int th2 = ws.th_fin2 > 0 ? ws.th_fin2
+                                   : band_height<1, 1>((const void *)k_finish2, ws.w, ws.h);
+    const int nb2 = stream_grid<1, 1>(ws.w, ws.h, th2);
+    need_parts(nb2, ws.nparts);
+    k_finish2<<<nb2, kBlock, 0, s>>>(grid_of(ws), kd, dt, dx * dx, b, ws.ap, ws.x, th2, ws.parts,
+                                     ws.st);
+    LAUNCH_CHECK();
+    k_pair_sum<<<1, kReduceBlock, 0, s>>>(ws.parts, nb2, ws.pair);
+    LAUNCH_CHECK();
+}
+
+// solve_core with the reference's summation order (DTD_SUMS_STRIPS): the same solve with every
+// sum of the loop taken per strip of 8 rows by one lane, the strips in order (k_strip_sums,
+// k_strips_step), and the put-back's per row (k_putback_rows, k_putback_fold), so the iterates,
+// the iteration count and h1 are the reference's bit for bit.  The passes run unfused on the
+// stored f64 faces (the cycle on the stored f32 records), the first guess is the splitting's own
+// recurrences, and the host reads the stop flag after each iteration (no graphs).
+static size_t solve_core_strips(dtd_ctx *ctx, const double *b, const double *kd, double dx,
+                                double dt, int wrap_x, double tol_m, double rel2, bool setup,
+                                Guess guess, dtd_stats *stats, double *rel, double *bnorm) {
+    Workspace &ws = ctx->ws;
+    hipStream_t s = ctx->stream;
+    const Grid g = grid_of(ws);
+    if (!ws.sums || ws.lf.empty()) throw Error{DTD_ESTATE, "the workspace has no strip sums"};
+    const int n = (int)ws.n, nb = nblocks(ws.n);
+    const int ns = (ws.h + kStripRows - 1) / kStripRows, sb = (ns + 63) / 64;
+    const long strip_cells = (long)kStripRows * ws.w;
+    double *sums = ws.sums, *bbs = ws.sums + ns, *rows = ws.sums + 2 * ns;
+    float *r32 = ws.lf[0].r, *z32 = ws.lf[0].z[1];
+    HIPCHK(hipEventRecord(ctx->ev[0], s));
+    if (setup) build_operator(ctx, kd, dx, dt);
+    HIPCHK(hipEventRecord(ctx->ev[1], s));
+    if (guess == kGuessLod && ctx->opts.guess == DTD_GUESS_LOD)
+        lod(ctx, b, ws.x, kd, dx, dt, wrap_x);
+    else if (guess != kGuessKeep)
+        HIPCHK(hipMemcpyAsync(ws.x, b, ws.n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipEventRecord(ctx->ev[2], s));
+    k_init_state<<<1, 64, 0, s>>>(ws.st, tol_m * tol_m * (double)ws.n, rel2,
+                                  (int)ctx->opts.max_iter, kd, dt, dx * dx);
+    LAUNCH_CHECK();
+    // r = b - A x, r.r (MG:596-626); the cycle's z and r.z (MG:627).
+    k_resid_es<<<nb, kBlock, 0, s>>>(g, ws.es, b, ws.x, ws.r, r32);
+    LAUNCH_CHECK();
+    k_strip_sums<double><<<sb, 64, 0, s>>>(ws.r, ws.r, ws.n, strip_cells, ns, sums, nullptr);
+    LAUNCH_CHECK();
+    if (rel2 > 0.0) {
+        k_strip_sums<double><<<sb, 64, 0, s>>>(b, b, ws.n, strip_cells, ns, bbs, nullptr);
+        LAUNCH_CHECK();
+    }
+    k_strips_step<kStepRR0><<<1, 64, 0, s>>>(sums, ns, rel2 > 0.0 ? bbs : nullptr, ws.st, nullptr);
+    LAUNCH_CHECK();
+    run_vcycle(ctx, nullptr, ws.st);
+    k_strip_sums<float><<<sb, 64, 0, s>>>(ws.r, z32, ws.n, strip_cells, ns, sums, ws.st);
+    LAUNCH_CHECK();
+    k_strips_step<kStepRZ0><<<1, 64, 0, s>>>(sums, ns, nullptr, ws.st, ctx->dev_flag);
+    LAUNCH_CHECK();
+    HIPCHK(hipStreamSynchronize(s));
+    // The loop (MG:630-733): iteration k (1-based) writes its direction to p0 when k is odd.
+    const int max_iter = (int)ctx->opts.max_iter;
+    for (int k = 0; k < max_iter && !ctx->host_flag[0]; k++) {
+        const double *pold = (k & 1) ? ws.p0 : ws.p1;
+        double *pnew = (k & 1) ? ws.p1 : ws.p0;
+        k_dir_cells<<<nb, kBlock, 0, s>>>(n, z32, pold, pnew, ws.x, ws.st);
+        LAUNCH_CHECK();
+        k_apply<<<nb, kBlock, 0, s>>>(g, ws.es, pnew, ws.ap);
+        LAUNCH_CHECK();
+        k_strip_sums<double><<<sb, 64, 0, s>>>(pnew, ws.ap, ws.n, strip_cells, ns, sums, ws.st);
+        LAUNCH_CHECK();
+        k_strips_step<kStepPAP><<<1, 64, 0, s>>>(sums, ns, nullptr, ws.st, nullptr);
+        LAUNCH_CHECK();
+        k_update<float><<<nblocks(ws.n, kCgCells), kBlock, 0, s>>>(n, ws.ap, ws.r, r32, ws.parts,
+                                                                   ws.st);
+        LAUNCH_CHECK();
+        k_strip_sums<double><<<sb, 64, 0, s>>>(ws.r, ws.r, ws.n, strip_cells, ns, sums, ws.st);
+        LAUNCH_CHECK();
+        k_strips_step<kStepRR><<<1, 64, 0, s>>>(sums, ns, nullptr, ws.st, nullptr);
+        LAUNCH_CHECK();
+        run_vcycle(ctx, nullptr, ws.st);
+        k_strip_sums<float><<<sb, 64, 0, s>>>(ws.r, z32, ws.n, strip_cells, ns, sums, ws.st);
+        LAUNCH_CHECK();
+        k_strips_step<kStepRZ><<<1, 64, 0, s>>>(sums, ns, nullptr, ws.st, ctx->dev_flag);
+        LAUNCH_CHECK();
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    // x' = x + alpha p (the last direction) into ap; A x' into r; the put-back's sums per row,
+    // the rows in order; then the put-back and the true residual.
+    k_last_update<<<nb, kBlock, 0, s>>>(n, ws.x, ws.p0, ws.p1, ws.ap, ws.st);
+    LAUNCH_CHECK();
+    k_apply<<<nb, kBlock, 0, s>>>(g, ws.es, ws.ap, ws.r);
+    LAUNCH_CHECK();
+    k_putback_rows<<<(ws.h + 63) / 64, 64, 0, s>>>(g, ws.es, b, ws.r, rows);
+    LAUNCH_CHECK();
+    k_putback_fold<<<1, 64, 0, s>>>(rows, ws.h, ws.st);
+    LAUNCH_CHECK();
+    finish_put_back(ctx, b, kd, dx, dt);
+    HIPCHK(hipEventRecord(ctx->ev[3], s));
+    HIPCHK(hipMemcpyAsync(ctx->host_st, ws.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(ctx->host_pair, ws.pair, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const double rr = ctx->host_pair[0], bb = ctx->host_pair[1];
+    if (rel) *rel = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    if (bnorm) *bnorm = std::sqrt(bb);
+    if (stats) {
+        stats->iterations += (size_t)ctx->host_st->its;
+        stats->ms_setup += elapsed(ctx->ev[0], ctx->ev[1]);
+        stats->ms_guess += elapsed(ctx->ev[1], ctx->ev[2]);
+        stats->ms_cg += elapsed(ctx->ev[2], ctx->ev[3]);
+        stats->ms_total += elapsed(ctx->ev[0], ctx->ev[3]);
+    }
+    return (size_t)ctx->host_st->its;
+}
+
 // One solve on the workspace (`multigrid_in` + `cg_on`, D:101-116, MG:562-763).  b is the
 // right-hand side on the device (the caller's array or ws.b; only read); kd likewise.  The first
 // guess is the splitting of b (or b itself when the context says so), or (kGuessKeep) the x
@@ -690,6 +813,9 @@ static void copy_answer(dtd_ctx *ctx, double *h_dev) {
 static size_t solve_core(dtd_ctx *ctx, const double *b, const double *kd, double dx, double dt,
                          int wrap_x, double tol_m, double rel2, bool setup, Guess guess,
                          dtd_stats *stats, double *rel, double *bnorm) {
+    if (ctx->opts.sums == DTD_SUMS_STRIPS)
+        return solve_core_strips(ctx, b, kd, dx, dt, wrap_x, tol_m, rel2, setup, guess, stats, rel,
+                                 bnorm);
     Workspace &ws = ctx->ws;
     hipStream_t s = ctx->stream;
     const Grid g = grid_of(ws);
@@ -1017,6 +1143,7 @@ void dtd_default_opts(dtd_opts *o) {
     o->guess = DTD_GUESS_LOD;
     o->transfer = DTD_TRANSFER_AGGREGATE;
     o->coarsest = DTD_COARSEST_SWEEPS;
+    o->sums = DTD_SUMS_BLOCKS;
 }
 
 int dtd_create(const dtd_opts *opts, dtd_ctx **out) {
@@ -1034,7 +1161,12 @@ int dtd_create(const dtd_opts *opts, dtd_ctx **out) {
         (ctx->opts.transfer != DTD_TRANSFER_AGGREGATE &&
          ctx->opts.transfer != DTD_TRANSFER_BILINEAR) ||
         (ctx->opts.coarsest != DTD_COARSEST_SWEEPS && ctx->opts.coarsest != DTD_COARSEST_EXACT) ||
-        (ctx->opts.transfer == DTD_TRANSFER_BILINEAR && ctx->opts.cycle_precision == DTD_CYCLE_BF16)) {
+        (ctx->opts.transfer == DTD_TRANSFER_BILINEAR && ctx->opts.cycle_precision == DTD_CYCLE_BF16) ||
+        (ctx->opts.sums != DTD_SUMS_BLOCKS && ctx->opts.sums != DTD_SUMS_STRIPS) ||
+        (ctx->opts.sums == DTD_SUMS_STRIPS &&
+         (ctx->opts.cycle_precision != DTD_CYCLE_F32 ||
+          ctx->opts.transfer != DTD_TRANSFER_AGGREGATE ||
+          ctx->opts.coarsest != DTD_COARSEST_SWEEPS))) {
         delete ctx;
         return DTD_EINVAL;
     }
@@ -1271,6 +1403,8 @@ int dtd_solve_device_noalloc(dtd_ctx *ctx, size_t w, size_t h, double dx, int wr
         const size_t n = w * h;
         if (n == 0) return;
         if (!h_dev || !kd_dev) throw Error{DTD_EINVAL, "null array"};
+        if (ctx->opts.sums == DTD_SUMS_STRIPS)
+            throw Error{DTD_EINVAL, "the reference's summation order needs the allocating entry points"};
         const int wrap = wrap_x && w > 1;
         // The size check first, by a carve that only counts.
         Workspace probe;

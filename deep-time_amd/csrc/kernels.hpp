@@ -2977,4 +2977,121 @@ __global__ __launch_bounds__(MAXT, 2048 / MAXT) void k_lod_lines(int n, int ring
         *reinterpret_cast<double2 *>(xl + a + q) = make_double2(d[q], d[q + 1]);
 }
 
+// ---- sums in the reference's order (DTD_SUMS_STRIPS) ----------------------------------------
+//
+// The reference adds each pass's products per strip of STRIP_ROWS rows, in row-major order from
+// -0.0 (`sum_zero`, D:507-510), then the strip sums in strip order (MG:596-619, 652-694, 703-723;
+// `Level::sweep`'s r.z, MG:314-318), and the put-back's two sums per row, then the rows in order
+// (MG:740-755).  These kernels keep that order exactly: a strip's (a row's) sum is one lane's chain
+// of adds.  They serve the mode that reproduces the reference's iterates bit for bit; the fast
+// path sums per block instead.
+
+constexpr int kStripRows = 8;  // STRIP_ROWS (D:502)
+
+// out[s] = the strip's sum of x[c] * y[c] (y widened to f64), c over the strip's cells in order.
+template <typename TY>
+__global__ void k_strip_sums(const double *__restrict__ x, const TY *__restrict__ y, long n,
+                             long strip_cells, int nstrips, double *__restrict__ out,
+                             const CgState *__restrict__ st) {
+    if (st && st->done) return;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nstrips) return;
+    const long c0 = (long)s * strip_cells, c1 = min(c0 + strip_cells, n);
+    constexpr int U = 8;
+    double a = -0.0;
+    long c = c0;
+    for (; c + U <= c1; c += U) {
+        double xv[U], yv[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) xv[k] = x[c + k], yv[k] = (double)y[c + k];
+#pragma unroll
+        for (int k = 0; k < U; k++) a += xv[k] * yv[k];
+    }
+    for (; c < c1; c++) a += x[c] * (double)y[c];
+    out[s] = a;
+}
+
+// The strip sums added in order from -0.0 (and b.b's for the first residual), then the step;
+// the stop flag into host memory when given.
+template <int STEP>
+__global__ void k_strips_step(const double *__restrict__ sums, int nstrips,
+                              const double *__restrict__ bb_sums, CgState *st, int *flag) {
+    if (threadIdx.x != 0) return;
+    if (STEP == kStepRR0 || !st->done) {
+        double t = -0.0;
+        for (int k = 0; k < nstrips; k++) t += sums[k];
+        double bb = 0.0;
+        if (STEP == kStepRR0 && bb_sums) {
+            bb = -0.0;
+            for (int k = 0; k < nstrips; k++) bb += bb_sums[k];
+        }
+        cg_step_apply<STEP>(t, bb, st);
+    }
+    if (flag) *reinterpret_cast<volatile int *>(flag) = st->done;
+}
+
+// r = b - A x in f64 (the stored faces, `apply_row`'s order) and its f32 copy (MG:596-619, less
+// the sum); x takes xin first when they differ.
+__global__ void k_resid_es(Grid g, const double2 *__restrict__ es, const double *__restrict__ b,
+                           const double *__restrict__ x, double *__restrict__ r,
+                           float *__restrict__ r32) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= g.n) return;
+    const int j = c / g.w, i = c - j * g.w;
+    const double rv = b[c] - apply_es(g, es, c, i, j, [&](int k) { return x[k]; });
+    r[c] = rv;
+    r32[c] = (float)rv;
+}
+
+// The direction pass less A p (MG:633-660): the pending x += alpha p_old, p = z + beta p_old (z
+// on the first iteration).
+__global__ void k_dir_cells(int n, const float *__restrict__ z, const double *__restrict__ pold,
+                            double *__restrict__ pnew, double *__restrict__ x,
+                            const CgState *__restrict__ st) {
+    if (st->done) return;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    if (st->have_pending) x[c] += st->alpha * pold[c];
+    pnew[c] = st->have_beta ? (double)z[c] + st->beta * pold[c] : (double)z[c];
+}
+
+// After the loop: x' = x + alpha p when an update is pending (MG:734-736), p the direction of the
+// last iteration (p0 after an odd count), into xout.
+__global__ void k_last_update(int n, const double *__restrict__ x, const double *__restrict__ p0,
+                              const double *__restrict__ p1, double *__restrict__ xout,
+                              const CgState *__restrict__ st) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const double *p = (st->its & 1) ? p0 : p1;
+    xout[c] = st->have_pending ? x[c] + st->alpha * p[c] : x[c];
+}
+
+// The put-back's row sums (MG:740-751): per row, sum (b - A x) and sum (diag - 1) in column
+// order from -0.0; one lane a row.
+__global__ void k_putback_rows(Grid g, const double2 *__restrict__ es, const double *__restrict__ b,
+                               const double *__restrict__ ax, double *__restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= g.h) return;
+    double l = -0.0, d1 = -0.0;
+    for (int i = 0; i < g.w; i++) {
+        const int c = j * g.w + i;
+        l += b[c] - ax[c];
+        d1 += diag_es(g, es, c, i, j) - 1.0;
+    }
+    out[2 * (size_t)j] = l;
+    out[2 * (size_t)j + 1] = d1;
+}
+
+// The rows folded in order from (-0.0, -0.0) (MG:752-755): the put-back's lost and weight.
+__global__ void k_putback_fold(const double *__restrict__ rows, int h, CgState *st) {
+    if (threadIdx.x != 0) return;
+    double l = -0.0, d1 = -0.0;
+    for (int j = 0; j < h; j++) {
+        l = l + rows[2 * (size_t)j];
+        d1 = d1 + rows[2 * (size_t)j + 1];
+    }
+    st->lost = l;
+    st->weight = d1;
+}
+
 }  // namespace dtd

@@ -38,6 +38,8 @@ TRANSFER_AGGREGATE = 0  # the reference's 2x2 aggregation
 TRANSFER_BILINEAR = 1   # bilinear prolongation and its transpose (an option)
 COARSEST_SWEEPS = 0     # the reference's 16 sweeps on the coarsest level
 COARSEST_EXACT = 1      # the coarsest level solved exactly in f64 (an option)
+SUMS_BLOCKS = 0         # the multigrid solve's sums per block (fast)
+SUMS_STRIPS = 1         # the reference's order: per 8-row strip, per row for the put-back
 
 # Every symbol include/dtdiff.h declares (the CPU tests check the library exports them all).
 SYMBOLS = (
@@ -85,7 +87,7 @@ class Opts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("omega", ctypes.c_float), ("nu", ctypes.c_uint),
                 ("scale", ctypes.c_float), ("max_iter", ctypes.c_uint),
                 ("cycle_precision", ctypes.c_int), ("guess", ctypes.c_int),
-                ("transfer", ctypes.c_int), ("coarsest", ctypes.c_int)]
+                ("transfer", ctypes.c_int), ("coarsest", ctypes.c_int), ("sums", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):

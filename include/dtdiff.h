@@ -73,6 +73,18 @@ extern "C" {
 #define DTD_GUESS_LOD 0  /* the splitting, as `multigrid_in` (diffusion.rs:110-114) */
 #define DTD_GUESS_RHS 1  /* h0 itself */
 
+/* Order of the multigrid solve's sums (r.r, p.A p, r.z, the put-back's two). */
+#define DTD_SUMS_BLOCKS 0 /* per block of the fused passes, the blocks in order: run to run the
+                             same, fast, but not the reference's rounding (its iterates agree to
+                             ~1e-15 and its iteration count can differ by one or two) */
+#define DTD_SUMS_STRIPS 1 /* the reference's: per strip of 8 rows in row-major order, then the
+                             strips in order (multigrid.rs:596-723, 314-318), the put-back per row
+                             then the rows (multigrid.rs:740-755), with the splitting's own
+                             recurrences as the first guess: the reference's iterates and
+                             iteration count bit for bit.  With the reference's cycle only
+                             (DTD_CYCLE_F32, aggregation, 16 coarsest sweeps); several times
+                             slower. */
+
 typedef struct dtd_ctx dtd_ctx;
 
 /* Settings of a context; dtd_default_opts gives the reference's (multigrid.rs:45-56,
@@ -88,6 +100,7 @@ typedef struct dtd_opts {
     int transfer;         /* DTD_TRANSFER_AGGREGATE (DTD_TRANSFER_BILINEAR: not with DTD_CYCLE_BF16,
                              nor in the batched solve) */
     int coarsest;         /* DTD_COARSEST_SWEEPS (DTD_COARSEST_EXACT: not in the batched solve) */
+    int sums;             /* DTD_SUMS_BLOCKS (DTD_SUMS_STRIPS: single-grid multigrid solves only) */
 } dtd_opts;
 
 /* What a solve did (all times in milliseconds, from HIP events on the context's stream). */
