@@ -376,6 +376,10 @@ def extra_configs(args, d2, h02, kd2, dt2, h0d2, kdd2, ms_f32, xo2):
         r = single(d2, h02, kd2, dt2, h0d2, kdd2, "f32", xo2, **opts)
         r["vs_plain_f64"] = f64["ms"] / r["ms"]
         out[f"lognormal-8192x4096-{name}"] = r
+    # The reference's summation order on the headline inputs, the reference's call (an RMS
+    # tolerance): the answer and the iteration count against the oracle's, bit for bit.
+    out["lognormal-8192x4096-strips"] = strips_line(d2, h02, kd2, dt2, h0d2, kdd2, f64["ms"],
+                                                    timed, xo2)
     # The learned first guess on the headline inputs.
     out["lognormal-8192x4096-learned"] = learned_line(d2, h02, kd2, dt2, h0d2, kdd2, f64["ms"],
                                                       timed, xo2)
@@ -397,6 +401,46 @@ def extra_configs(args, d2, h02, kd2, dt2, h0d2, kdd2, ms_f32, xo2):
     # The 512 regional grids, one batched solve a step.
     out["regional-512"] = batch_line(timed, xo2 is not None)
     return out
+
+
+def strips_line(d, h0, kd, dt, h0d, kdd, ms_f64, timed, xo):
+    """The solve with the reference's summation order (DTD_SUMS_STRIPS) to the RMS tolerance
+    tol_m = 1e-10 ||h0|| / sqrt(n), the oracle's call on the CPU leg: whether h1 and the
+    iteration count equal the oracle's exactly."""
+    import numpy as np
+
+    import dtdiff
+    from dtdiff.hipmem import DeviceArray
+
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+
+    tol_m = 1e-10 * float(np.linalg.norm(h0)) / np.sqrt(d.cells())
+    ws = dtdiff.Workspace(device=0, sums=dtdiff.SUMS_STRIPS)
+    hd = DeviceArray(d.cells())
+    try:
+        def step():
+            hd.copy_from(h0d)
+            return dtdiff.solve_device(ws, d, hd, kdd, dt, tol_m)
+
+        ms, st = timed(step)
+        x = hd.numpy()
+    finally:
+        ws.close()
+        hd.free()
+    r = {"grid": [d.w, d.h], "cycle": "f32", "sums": "strips (the reference's order)",
+         "tol_m": tol_m, "ms": ms, "unknowns_per_s": d.cells() / ms * 1e3,
+         "iterations": int(st.iterations),
+         "rel_residual_true": O.rel_residual(d.w, d.h, d.dx, d.wrap_x, kd, dt, h0, x),
+         "vs_plain_f64": ms_f64 / ms}
+    if xo is not None:
+        its_o = O.solve(d.w, d.h, d.dx, d.wrap_x, h0, kd, dt, tol_m)[1]
+        r["iterations_oracle"] = int(its_o)
+        r["bits_equal_oracle"] = bool(np.array_equal(x.view(np.uint64), xo.view(np.uint64)))
+        r["parity_rel_l2_vs_oracle"] = float(np.linalg.norm(x - xo) / np.linalg.norm(xo))
+    log(f"extra strips: {ms:.2f} ms, {st.iterations} its, bits equal the oracle's: "
+        f"{r.get('bits_equal_oracle')}")
+    return r
 
 
 def learned_line(d, h0, kd, dt, h0d, kdd, ms_f64, timed, xo, coarse=False):
