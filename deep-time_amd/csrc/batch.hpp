@@ -125,6 +125,26 @@ __global__ __launch_bounds__(kLineBlock) void kb_lod_cols(const BGrid *__restric
     if (i < q.g.w) lod_col(q.g, q.kd, q.dt, q.s2, q.x, q.p0, i);
 }
 
+// The row pass of the first guess by the partition method, one wave a row in place of the rows'
+// recurrences and their transposes: b into x (widths a multiple of 16 from 32 to 1024); a
+// block takes 4 rows of one grid.
+__global__ __launch_bounds__(kBlock) void kb_lod_rows_wave(const BGrid *__restrict__ t,
+                                                           const int2 *__restrict__ blk) {
+    __shared__ double sm[kBlock / 64][6 * 128];
+    const int2 m = blk[blockIdx.x];
+    const BGrid &q = t[m.x];
+    const int wv = threadIdx.x >> 6, w = q.g.w;
+    const int j = m.y * (kBlock / 64) + wv;
+    const bool on = j < q.g.h;
+    const size_t o = (size_t)(on ? j : 0) * w;
+    lod_line_wave(w, q.wrap_x && w > 2, on, q.kd + o, q.b + o, q.x + o, q.dt / q.s2, sm[wv]);
+}
+
+// Whether a grid's rows go by the one-wave partition method (the others' by the recurrences on
+// the transposed grid): a function of the width alone, so a grid's answer does not depend on the
+// batch it is in.
+inline bool rows_by_wave(size_t w) { return w % kSeg == 0 && w >= 32 && w <= 64 * kSeg; }
+
 // A fresh state for each grid (inactive grids are marked done and skipped by every pass).
 __global__ void kb_init(CgState *st, const BInit *__restrict__ in, const BGrid *__restrict__ t,
                         int n, int max_iter) {
@@ -261,6 +281,17 @@ __global__ __launch_bounds__(kBlock) void kb_spost0(const BLevel *__restrict__ t
     if (threadIdx.x == 0) parts[blockIdx.x] = sum;
 }
 
+// The bottom of each grid's cycle: its levels from the first of at most the bottom's size down,
+// one block a grid in LDS (bottom_cycle; tab: the grid's first level in lvs, its level count, the
+// grid).
+__global__ __launch_bounds__(kBottomBlock) void kb_bottom(const Lvl<float> *__restrict__ lvs,
+                                                          const int4 *__restrict__ tab,
+                                                          const CgState *__restrict__ st) {
+    const int4 m = tab[blockIdx.x];
+    if (st[m.z].done) return;
+    bottom_cycle<float>(lvs + m.x, 0, m.y);
+}
+
 // The coarsest level of each grid whose cycle ends here: one wave a grid.
 __global__ __launch_bounds__(64) void kb_coarsest(const BLevel *__restrict__ t,
                                                   const int2 *__restrict__ blk,
@@ -351,11 +382,16 @@ struct BatchWs {
     int *flag = nullptr;
     int2 *tables = nullptr;  // every block and range table, one allocation
     // f64 side
-    BLaunch cells_l, update_l, rows_l, cols_l, tiles_l, band_l;
+    BLaunch cells_l, update_l, rows_l, rows4_l, cols_l, tiles_l, band_l;
     const int2 *rng_band = nullptr, *rng_rr = nullptr, *rng_rz = nullptr;
     int rr_base_pre1 = 0, rr_base_update = 0;  // r.r partials: pre2[0]'s, pre1[0]'s, the update's
     // cycle, per level
     std::vector<BLaunch> coarsen_l, pre2, pre1, post2, post1, coarsest;
+    // the bottoms: every grid's levels from its bottom on, the launch table, the LDS bytes
+    const Lvl<float> *bottom_lv = nullptr;
+    const int4 *bottom_tab = nullptr;
+    int nbottom = 0;
+    size_t bottom_lds = 0;
     int rz_base_post1 = 0, rz_base_coarsest = 0;
     std::vector<void *> owned;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -509,6 +545,40 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
         HIPCHK(hipMemcpy(d, lv[l].data(), lv[l].size() * sizeof(BLevel), hipMemcpyHostToDevice));
         bw->d_lv.push_back(d);
     }
+    // Each grid's bottom: the first level from 1 on that one block cycles in LDS, as the single
+    // grid's cycle chooses it (at most bottom_cells() cells; that level and the coarser ones'
+    // records, r and z within kBottomLds; at most kBottomLevels levels).  A grid of one level
+    // has none (b0 = its level count).
+    std::vector<int> b0(n);
+    {
+        std::vector<Lvl<float>> blv;
+        std::vector<int4> btab;
+        for (size_t g = 0; g < n; g++) {
+            const int nl = bw->nlev[g];
+            b0[g] = nl;
+            if (nl < 2 || !fused_enabled()) continue;
+            for (int l = 1; l < nl; l++) {
+                size_t lds = 0;
+                for (int k = l; k < nl; k++) lds += (kRec + 3) * (size_t)sh[g][k].w * sh[g][k].h * 4;
+                if ((long)sh[g][l].w * sh[g][l].h <= bottom_cells() && lds <= kBottomLds &&
+                    nl - l <= kBottomLevels) {
+                    b0[g] = l;
+                    btab.push_back(make_int4((int)blv.size(), nl - l, (int)g, 0));
+                    for (int k = l; k < nl; k++) blv.push_back(level(g, k));
+                    bw->bottom_lds = std::max(bw->bottom_lds, lds);
+                    break;
+                }
+            }
+        }
+        bw->nbottom = (int)btab.size();
+        if (bw->nbottom) {
+            Lvl<float> *dl = (Lvl<float> *)own(blv.size() * sizeof(Lvl<float>));
+            HIPCHK(hipMemcpy(dl, blv.data(), blv.size() * sizeof(Lvl<float>), hipMemcpyHostToDevice));
+            int4 *dt = (int4 *)own(btab.size() * sizeof(int4));
+            HIPCHK(hipMemcpy(dt, btab.data(), btab.size() * sizeof(int4), hipMemcpyHostToDevice));
+            bw->bottom_lv = dl, bw->bottom_tab = dt;
+        }
+    }
     // Block and range tables, in one host vector; launches keep their offsets until upload.
     std::vector<int2> tab;
     struct Pending {
@@ -533,11 +603,19 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
         cnt[g] = bw->nlev[g] > 1 ? 0 : nblocks((long)gs[g].w * gs[g].h, kCgCells);
     std::vector<int2> rng_update, rng_band, rng_rz(n), rng_rr(n);
     add_launch(bw->update_l, cnt, &rng_update);
-    for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].h, kLineBlock);
+    // The first guess's rows: by the one-wave partition method (rows4_l, four rows a block), or
+    // by the recurrences on the transposed grid (rows_l, tiles_l); then every grid's columns.
+    for (size_t g = 0; g < n; g++)
+        cnt[g] = rows_by_wave(gs[g].w) ? 0 : nblocks((long)gs[g].h, kLineBlock);
     add_launch(bw->rows_l, cnt, nullptr);
+    for (size_t g = 0; g < n; g++)
+        cnt[g] = rows_by_wave(gs[g].w) ? nblocks((long)gs[g].h, kBlock / 64) : 0;
+    add_launch(bw->rows4_l, cnt, nullptr);
     for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].w, kLineBlock);
     add_launch(bw->cols_l, cnt, nullptr);
-    for (size_t g = 0; g < n; g++) cnt[g] = (int)(((gs[g].w + 31) / 32) * ((gs[g].h + 31) / 32));
+    for (size_t g = 0; g < n; g++)
+        cnt[g] = rows_by_wave(gs[g].w) ? 0
+                                       : (int)(((gs[g].w + 31) / 32) * ((gs[g].h + 31) / 32));
     add_launch(bw->tiles_l, cnt, nullptr);
     for (size_t g = 0; g < n; g++)
         cnt[g] = stream_grid<1, 1>((int)gs[g].w, (int)gs[g].h, bw->grids[g].th);
@@ -556,6 +634,7 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
             const bool last = l + 1 == bw->nlev[g];
             if (!last) {
                 cz[t] = nblocks(q.C.n);
+                if (l >= b0[g]) continue;  // the grid's bottom cycles this level
                 // c: the pre-smoothing's blocks, q: the post-smoothing's.
                 if (q.L.w % 2 == 0)
                     c2[t] = stream_grid<2, 1>(q.L.w, q.L.h, q.th),
@@ -563,7 +642,7 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
                 else
                     c1[t] = stream_grid<1, 2>(q.L.w, q.L.h, q.th),
                     q1[t] = stream_grid<1, 2>(q.L.w, q.L.h, q.th_post);
-            } else {
+            } else if (l < b0[g]) {
                 if (q.L.n > 64) throw Error{DTD_ESTATE, "the coarsest level exceeds one wave"};
                 cc[t] = 1;
             }
@@ -671,6 +750,12 @@ static void batch_vcycle(BatchWs &bw, hipStream_t s, int parity) {
                 bw.d_lv[l], bw.coarsest[l].blk, bw.parts + (l == 0 ? bw.rz_base_coarsest : 0));
             LAUNCH_CHECK();
         }
+    }
+    // Every grid's bottom, once its levels above have restricted into it.
+    if (bw.nbottom) {
+        kb_bottom<<<bw.nbottom, kBottomBlock, bw.bottom_lds, s>>>(bw.bottom_lv, bw.bottom_tab,
+                                                                  bw.st);
+        LAUNCH_CHECK();
     }
     // The current residual after iteration `parity`: r64[parity ^ 1]; in the first cycle r64[0].
     const int cur = parity < 0 ? 0 : parity ^ 1;
@@ -852,14 +937,20 @@ static void solve_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs_in, doubl
     }
     const bool lod_guess = ctx->opts.guess == DTD_GUESS_LOD;
     if (lod_guess) {
-        for (int mode = 0; mode < 2; mode++) {
-            kb_transpose<<<bw.tiles_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.tiles_l.blk, mode);
+        if (bw.rows4_l.nblk) {
+            kb_lod_rows_wave<<<bw.rows4_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.rows4_l.blk);
             LAUNCH_CHECK();
         }
-        kb_lod_rows<<<bw.rows_l.nblk, kLineBlock, 0, s>>>(bw.d_grids, bw.rows_l.blk);
-        LAUNCH_CHECK();
-        kb_transpose<<<bw.tiles_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.tiles_l.blk, 2);
-        LAUNCH_CHECK();
+        if (bw.rows_l.nblk) {
+            for (int mode = 0; mode < 2; mode++) {
+                kb_transpose<<<bw.tiles_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.tiles_l.blk, mode);
+                LAUNCH_CHECK();
+            }
+            kb_lod_rows<<<bw.rows_l.nblk, kLineBlock, 0, s>>>(bw.d_grids, bw.rows_l.blk);
+            LAUNCH_CHECK();
+            kb_transpose<<<bw.tiles_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.tiles_l.blk, 2);
+            LAUNCH_CHECK();
+        }
         kb_lod_cols<<<bw.cols_l.nblk, kLineBlock, 0, s>>>(bw.d_grids, bw.cols_l.blk);
         LAUNCH_CHECK();
     }

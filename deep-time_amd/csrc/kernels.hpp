@@ -922,14 +922,15 @@ __device__ inline T bottom_sweep(const Lvl<T> &L, const T *z, int c, int i, int 
     return z[c] + c_wd(L, c) * (L.r[c] - ax);
 }
 
+// Levels b0 .. nl - 1 of a cycle (every one of at most kBottomLevels, their records, r and z
+// at most kBottomLds bytes) by one block of kBottomBlock threads in LDS: the top level's
+// residual in, its correction out to lv[b0].z[1].
 template <typename T>
-__global__ __launch_bounds__(kBottomBlock) void k_bottom(const Lvl<T> *__restrict__ lv, int b0,
-                                                         int nl, const CgState *__restrict__ st) {
+__device__ __forceinline__ void bottom_cycle(const Lvl<T> *__restrict__ lv, int b0, int nl) {
     extern __shared__ double bottom_mem[];
     __shared__ Lvl<T> loc[kBottomLevels];
     __shared__ int cells[kBottomLevels];
     __shared__ T zs[2][64];
-    if (st && st->done) return;
     const int t = threadIdx.x, nb = nl - b0;
     // The level table: one thread a level reads it, then each level's LDS place follows the
     // sizes of the levels before it.
@@ -1058,6 +1059,13 @@ __global__ __launch_bounds__(kBottomBlock) void k_bottom(const Lvl<T> *__restric
         }
         __syncthreads();
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBottomBlock) void k_bottom(const Lvl<T> *__restrict__ lv, int b0,
+                                                         int nl, const CgState *__restrict__ st) {
+    if (st && st->done) return;
+    bottom_cycle<T>(lv, b0, nl);
 }
 
 // ---- fused, register-streaming smoothing of a level (nu = 2) -------------------------------
@@ -2844,6 +2852,77 @@ __global__ void k_transpose(const double *__restrict__ src, double *__restrict__
     transpose_tile(src, dst, rows, cols, blockIdx.y * 32, blockIdx.x * 32, t);
 }
 
+// The two reduced equations of a 16-cell segment: F (the segment's first unknown) couples the last
+// unknown of the segment before (a[0]) and its own last (c[0]); L couples its own first (a[1])
+// and the first of the segment after (c[1]).
+struct SegEq {
+    double a[2], b[2], c[2], r[2];
+};
+
+// One segment of the partition method: kk[0 .. 17] the diffusivity of cells a - 1 .. a + 16
+// (has_w / has_e: the faces beyond the segment exist), dts2 = dt / dx^2 (divided once: this
+// solver is a first guess and need not repeat the reference's rounding, only its system), xs the
+// segment's right-hand side.  Row q of the segment: lo = -f[q], up = -f[q + 1], di = 1 + f[q] +
+// f[q + 1].  The interior rows 1 .. 14 are eliminated against x_a and x_b (forward, then
+// backward; x_q = de[q] - al[q] x_a - ga[q] x_b), leaving eq.
+__device__ __forceinline__ void seg_reduce(const double (&kk)[kSeg + 2], bool has_w, bool has_e,
+                                           double dts2, const double *__restrict__ xs, SegEq &eq,
+                                           double (&al)[kSeg], double (&ga)[kSeg],
+                                           double (&de)[kSeg], double (&d)[kSeg]) {
+    double f[kSeg + 1];
+#pragma unroll
+    for (int q = 0; q <= kSeg; q++) f[q] = 0.5 * (kk[q] + kk[q + 1]) * dts2;
+    if (!has_w) f[0] = 0.0;
+    if (!has_e) f[kSeg] = 0.0;
+#pragma unroll
+    for (int q = 0; q < kSeg; q += 2) {
+        const double2 v = *reinterpret_cast<const double2 *>(xs + q);
+        d[q] = v.x, d[q + 1] = v.y;
+    }
+    {
+        const double lo = -f[1], up = -f[2], ri = 1.0 / (1.0 + f[1] + f[2]);
+        al[1] = lo * ri, ga[1] = up * ri, de[1] = d[1] * ri;
+    }
+#pragma unroll
+    for (int q = 2; q <= kSeg - 2; q++) {
+        const double lo = -f[q], up = -f[q + 1], di = 1.0 + f[q] + f[q + 1];
+        const double ri = 1.0 / (di - lo * ga[q - 1]);
+        al[q] = -lo * al[q - 1] * ri;
+        ga[q] = up * ri;
+        de[q] = (d[q] - lo * de[q - 1]) * ri;
+    }
+#pragma unroll
+    for (int q = kSeg - 3; q >= 1; q--) {
+        de[q] = de[q] - ga[q] * de[q + 1];
+        al[q] = al[q] - ga[q] * al[q + 1];
+        ga[q] = -ga[q] * ga[q + 1];
+    }
+    const double lo0 = -f[0], up0 = -f[1], di0 = 1.0 + f[0] + f[1];
+    eq.a[0] = lo0;
+    eq.b[0] = di0 - up0 * al[1];
+    eq.c[0] = -up0 * ga[1];
+    eq.r[0] = d[0] - up0 * de[1];
+    const double lo15 = -f[kSeg - 1], up15 = -f[kSeg], di15 = 1.0 + f[kSeg - 1] + f[kSeg];
+    eq.a[1] = -lo15 * al[kSeg - 2];
+    eq.b[1] = di15 - lo15 * ga[kSeg - 2];
+    eq.c[1] = up15;
+    eq.r[1] = d[kSeg - 1] - lo15 * de[kSeg - 2];
+}
+
+// The segment's 16 values from its end unknowns, into out.
+__device__ __forceinline__ void seg_back(double xf, double xl, const double (&al)[kSeg],
+                                         const double (&ga)[kSeg], const double (&de)[kSeg],
+                                         double *__restrict__ out) {
+    double d[kSeg];
+    d[0] = xf;
+    d[kSeg - 1] = xl;
+#pragma unroll
+    for (int q = 1; q <= kSeg - 2; q++) d[q] = de[q] - al[q] * xf - ga[q] * xl;
+#pragma unroll
+    for (int q = 0; q < kSeg; q += 2)
+        *reinterpret_cast<double2 *>(out + q) = make_double2(d[q], d[q + 1]);
+}
+
 // One line per block (n / 16 threads, a power of two), from src to dst (may be the same array);
 // faces from kd along the line, the last one wrapping when `ring`.
 template <int MAXT>
@@ -2869,58 +2948,13 @@ __global__ __launch_bounds__(MAXT, 2048 / MAXT) void k_lod_lines(int n, int ring
     const bool has_w = a > 0 || ring, has_e = b + 1 < n || ring;
     kk[0] = has_w ? k[a > 0 ? a - 1 : n - 1] : 0.0;
     kk[kSeg + 1] = has_e ? k[b + 1 < n ? b + 1 : 0] : 0.0;
-    // face(i) for i = a - 1 .. b: f[q] = face(a - 1 + q), with dt / dx^2 divided once (this
-    // solver is a first guess: it need not repeat the reference's rounding, only its system).
-    const double dts2 = dt / s2;
-    double f[kSeg + 1];
-#pragma unroll
-    for (int q = 0; q <= kSeg; q++) f[q] = 0.5 * (kk[q] + kk[q + 1]) * dts2;
-    if (!has_w) f[0] = 0.0;
-    if (!has_e) f[kSeg] = 0.0;
-    double d[kSeg];
-#pragma unroll
-    for (int q = 0; q < kSeg; q += 2) {
-        const double2 v = *reinterpret_cast<const double2 *>(xs + a + q);
-        d[q] = v.x, d[q + 1] = v.y;
-    }
-    // Row q of the segment: lo = -f[q], up = -f[q + 1], di = 1 + f[q] + f[q + 1].
-    // Forward over the interior rows 1 .. 14, against x_a.
-    double al[kSeg], ga[kSeg], de[kSeg];
-    {
-        const double lo = -f[1], up = -f[2], ri = 1.0 / (1.0 + f[1] + f[2]);
-        al[1] = lo * ri, ga[1] = up * ri, de[1] = d[1] * ri;
-    }
-#pragma unroll
-    for (int q = 2; q <= kSeg - 2; q++) {
-        const double lo = -f[q], up = -f[q + 1], di = 1.0 + f[q] + f[q + 1];
-        const double ri = 1.0 / (di - lo * ga[q - 1]);
-        al[q] = -lo * al[q - 1] * ri;
-        ga[q] = up * ri;
-        de[q] = (d[q] - lo * de[q - 1]) * ri;
-    }
-    // Backward: x_q = de[q] - al[q] x_a - ga[q] x_b.
-#pragma unroll
-    for (int q = kSeg - 3; q >= 1; q--) {
-        de[q] = de[q] - ga[q] * de[q + 1];
-        al[q] = al[q] - ga[q] * al[q + 1];
-        ga[q] = -ga[q] * ga[q + 1];
-    }
-    // The segment's two reduced equations: F (row 0) couples L of the segment before, L (row
-    // 15) couples F of the segment after.
-    {
-        const double lo0 = -f[0], up0 = -f[1], di0 = 1.0 + f[0] + f[1];
-        A[2 * t] = lo0;
-        B[2 * t] = di0 - up0 * al[1];
-        Cc[2 * t] = -up0 * ga[1];
-        R[2 * t] = d[0] - up0 * de[1];
-        const double lo15 = -f[kSeg - 1], up15 = -f[kSeg], di15 = 1.0 + f[kSeg - 1] + f[kSeg];
-        A[2 * t + 1] = -lo15 * al[kSeg - 2];
-        B[2 * t + 1] = di15 - lo15 * ga[kSeg - 2];
-        Cc[2 * t + 1] = up15;
-        R[2 * t + 1] = d[kSeg - 1] - lo15 * de[kSeg - 2];
-        Bi[2 * t] = 1.0 / B[2 * t];
-        Bi[2 * t + 1] = 1.0 / B[2 * t + 1];
-    }
+    double al[kSeg], ga[kSeg], de[kSeg], d[kSeg];
+    SegEq eq;
+    seg_reduce(kk, has_w, has_e, dt / s2, xs + a, eq, al, ga, de, d);
+    A[2 * t] = eq.a[0], B[2 * t] = eq.b[0], Cc[2 * t] = eq.c[0], R[2 * t] = eq.r[0];
+    A[2 * t + 1] = eq.a[1], B[2 * t + 1] = eq.b[1], Cc[2 * t + 1] = eq.c[1], R[2 * t + 1] = eq.r[1];
+    Bi[2 * t] = 1.0 / B[2 * t];
+    Bi[2 * t + 1] = 1.0 / B[2 * t + 1];
     __syncthreads();
     // Parallel cyclic reduction: after the step of stride s every equation couples the unknowns
     // 2s away.  A ring stops at stride N / 2, where both couplings reach the same unknown.
@@ -2968,13 +3002,80 @@ __global__ __launch_bounds__(MAXT, 2048 / MAXT) void k_lod_lines(int n, int ring
         }
         xf = sol[0], xl_ = sol[1];
     }
-    d[0] = xf;
-    d[kSeg - 1] = xl_;
+    seg_back(xf, xl_, al, ga, de, xl + a);
+}
+
+// One line of n cells (a multiple of 16, 32 <= n <= 1024) by the partition method in one wave,
+// for lines of any such length: lane t takes the segment [16 t, 16 t + 16) and reduces it
+// (seg_reduce); the 2 n / 16 reduced equations go to LDS and lane 0 solves them by Thomas's
+// recurrences, a ring (periodic row) with the Sherman-Morrison correction (Press et al.,
+// Numerical Recipes, 2nd ed., section 2.7: two right-hand sides, one elimination); the lanes then
+// recover their interiors.  The same system as the reference's row recurrences, solved to
+// rounding.  sm: 6 x 128 doubles of LDS for this wave; every lane of the wave calls it (`on`:
+// the wave has a line).
+__device__ __forceinline__ void lod_line_wave(int n, int ring, bool on, const double *__restrict__ k,
+                                              const double *__restrict__ src,
+                                              double *__restrict__ dst, double dts2,
+                                              double *sm) {
+    const int lane = threadIdx.x & 63, P = n / kSeg, N = 2 * P;
+    double *A = sm, *B = sm + 128, *C = sm + 256, *R = sm + 384, *U = sm + 512, *X = sm + 640;
+    const bool mine = on && lane < P;
+    const int a = lane * kSeg, b = a + kSeg - 1;
+    double al[kSeg], ga[kSeg], de[kSeg], d[kSeg];
+    if (mine) {
+        double kk[kSeg + 2];
 #pragma unroll
-    for (int q = 1; q <= kSeg - 2; q++) d[q] = de[q] - al[q] * xf - ga[q] * xl_;
+        for (int q = 0; q < kSeg; q += 2) {
+            const double2 v = *reinterpret_cast<const double2 *>(k + a + q);
+            kk[1 + q] = v.x, kk[2 + q] = v.y;
+        }
+        const bool has_w = a > 0 || ring, has_e = b + 1 < n || ring;
+        kk[0] = has_w ? k[a > 0 ? a - 1 : n - 1] : 0.0;
+        kk[kSeg + 1] = has_e ? k[b + 1 < n ? b + 1 : 0] : 0.0;
+        SegEq eq;
+        seg_reduce(kk, has_w, has_e, dts2, src + a, eq, al, ga, de, d);
 #pragma unroll
-    for (int q = 0; q < kSeg; q += 2)
-        *reinterpret_cast<double2 *>(xl + a + q) = make_double2(d[q], d[q + 1]);
+        for (int e = 0; e < 2; e++) {
+            A[2 * lane + e] = eq.a[e], B[2 * lane + e] = eq.b[e], C[2 * lane + e] = eq.c[e];
+            R[2 * lane + e] = eq.r[e];
+        }
+    }
+    __syncthreads();
+    if (on && lane == 0) {
+        // Row 0 couples x[N - 1] by A[0], row N - 1 couples x[0] by C[N - 1] (a ring; zero for
+        // a closed line).  Thomas on the tridiagonal part with b[0] and b[N - 1] changed for the
+        // ring, right-hand sides R and u = (gamma, 0, .., 0, C[N - 1]).
+        const double beta = ring ? A[0] : 0.0, alpha = ring ? C[N - 1] : 0.0;
+        const double gamma = -B[0];
+        double bb0 = ring ? B[0] - gamma : B[0];
+        double cp = C[0] / bb0;
+        R[0] = R[0] / bb0;
+        U[0] = ring ? gamma / bb0 : 0.0;
+        C[0] = cp;
+        for (int i = 1; i < N; i++) {
+            double bi = B[i];
+            if (ring && i == N - 1) bi -= alpha * beta / gamma;
+            const double m = bi - A[i] * C[i - 1];
+            const double ui = ring && i == N - 1 ? alpha : 0.0;
+            C[i] = C[i] / m;
+            R[i] = (R[i] - A[i] * R[i - 1]) / m;
+            U[i] = (ui - A[i] * U[i - 1]) / m;
+        }
+        double xn = R[N - 1], zn = U[N - 1];
+        X[N - 1] = xn, U[N - 1] = zn;
+        for (int i = N - 2; i >= 0; i--) {
+            xn = R[i] - C[i] * xn;
+            zn = U[i] - C[i] * zn;
+            X[i] = xn, U[i] = zn;
+        }
+        if (ring) {
+            const double fact = (X[0] + beta * X[N - 1] / gamma) /
+                                (1.0 + U[0] + beta * U[N - 1] / gamma);
+            for (int i = 0; i < N; i++) X[i] -= fact * U[i];
+        }
+    }
+    __syncthreads();
+    if (mine) seg_back(X[2 * lane], X[2 * lane + 1], al, ga, de, dst + a);
 }
 
 // ---- sums in the reference's order (DTD_SUMS_STRIPS) ----------------------------------------

@@ -116,6 +116,39 @@ def test_batch_first_guess_is_the_splitting(ws):
         assert np.abs(x - xo).max() <= 1e-13 * np.abs(xo).max(), d
 
 
+# Widths that are multiples of 16 from 32 to 1024 (the regional workload's kind): the first
+# guess's rows go by the one-wave partition method instead of the recurrences.
+SEGMENTED = [
+    ("lognormal", 32, 64, True), ("lognormal", 96, 32, False), ("constant", 160, 96, True),
+    ("lognormal", 512, 64, True), ("lognormal", 1024, 40, True), ("shelf", 48, 33, False),
+    ("lognormal", 272, 128, True),
+]
+
+
+def test_batch_rows_by_partition_give_the_splitting(ws):
+    """The rows solved by the partition method in one wave (a ring's reduced system by
+    Sherman-Morrison) give the splitting's guess to rounding: with a tolerance the first guess
+    meets, each grid's answer is the oracle's (the recurrences) to 1e-12."""
+    print("batch rows by partition", flush=True)
+    grids = make(SEGMENTED, seed=71)
+    xs, its, _, _ = run(ws, grids, 0.0, [1e30] * len(grids))
+    assert its == [0] * len(grids)
+    for (d, b, kd, dt), x in zip(grids, xs):
+        xo, its_o = O.solve(d.w, d.h, d.dx, d.wrap_x, b, kd, dt, 1e30)
+        assert np.abs(x - xo).max() <= 1e-12 * np.abs(xo).max(), d
+
+
+def test_batch_rows_by_partition_solve_to_the_target(ws):
+    print("batch rows by partition, solved", flush=True)
+    grids = make(SEGMENTED, seed=73)
+    xs, its, rels, _ = run(ws, grids, 1e-10)
+    for (d, b, kd, dt), x, k, rel in zip(grids, xs, its, rels):
+        assert O.rel_residual(d.w, d.h, d.dx, d.wrap_x, kd, dt, b, x) <= 1e-10, d
+        tol = 1e-10 * np.linalg.norm(b) / np.sqrt(b.size)
+        xo, its_o = O.solve(d.w, d.h, d.dx, d.wrap_x, b, kd, dt, tol)
+        assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo), d
+
+
 def test_a_grid_answer_does_not_depend_on_the_batch(ws):
     print("batch independence", flush=True)
     grids = make(MIXED, seed=21)
