@@ -3007,18 +3007,20 @@ __global__ __launch_bounds__(MAXT, 2048 / MAXT) void k_lod_lines(int n, int ring
 
 // One line of n cells (a multiple of 16, 32 <= n <= 1024) by the partition method in one wave,
 // for lines of any such length: lane t takes the segment [16 t, 16 t + 16) and reduces it
-// (seg_reduce); the 2 n / 16 reduced equations go to LDS and lane 0 solves them by Thomas's
-// recurrences, a ring (periodic row) with the Sherman-Morrison correction (Press et al.,
-// Numerical Recipes, 2nd ed., section 2.7: two right-hand sides, one elimination); the lanes then
-// recover their interiors.  The same system as the reference's row recurrences, solved to
-// rounding.  sm: 6 x 128 doubles of LDS for this wave; every lane of the wave calls it (`on`:
-// the wave has a line).
+// (seg_reduce); the N = 2 n / 16 reduced equations go to LDS and are solved by parallel cyclic
+// reduction (two equations a lane, ceil(log2 N) steps, coefficients past either end taken as the
+// identity), a ring (periodic row) with the Sherman-Morrison correction (Press et al., Numerical
+// Recipes, 2nd ed., section 2.7: the corners folded into the first and last diagonals, a second
+// right-hand side u = (gamma, 0, .., 0, alpha) reduced alongside); the lanes then recover their
+// interiors.  The same system as the reference's row recurrences, solved to rounding.  sm: 6 x 128
+// doubles of LDS for this wave; every lane of the block calls it (`on`: the wave has a line), as
+// it synchronises the block.
 __device__ __forceinline__ void lod_line_wave(int n, int ring, bool on, const double *__restrict__ k,
                                               const double *__restrict__ src,
                                               double *__restrict__ dst, double dts2,
                                               double *sm) {
     const int lane = threadIdx.x & 63, P = n / kSeg, N = 2 * P;
-    double *A = sm, *B = sm + 128, *C = sm + 256, *R = sm + 384, *U = sm + 512, *X = sm + 640;
+    double *A = sm, *B = sm + 128, *C = sm + 256, *R = sm + 384, *U = sm + 512, *Bi = sm + 640;
     const bool mine = on && lane < P;
     const int a = lane * kSeg, b = a + kSeg - 1;
     double al[kSeg], ga[kSeg], de[kSeg], d[kSeg];
@@ -3041,41 +3043,80 @@ __device__ __forceinline__ void lod_line_wave(int n, int ring, bool on, const do
         }
     }
     __syncthreads();
-    if (on && lane == 0) {
-        // Row 0 couples x[N - 1] by A[0], row N - 1 couples x[0] by C[N - 1] (a ring; zero for
-        // a closed line).  Thomas on the tridiagonal part with b[0] and b[N - 1] changed for the
-        // ring, right-hand sides R and u = (gamma, 0, .., 0, C[N - 1]).
-        const double beta = ring ? A[0] : 0.0, alpha = ring ? C[N - 1] : 0.0;
-        const double gamma = -B[0];
-        double bb0 = ring ? B[0] - gamma : B[0];
-        double cp = C[0] / bb0;
-        R[0] = R[0] / bb0;
-        U[0] = ring ? gamma / bb0 : 0.0;
-        C[0] = cp;
-        for (int i = 1; i < N; i++) {
-            double bi = B[i];
-            if (ring && i == N - 1) bi -= alpha * beta / gamma;
-            const double m = bi - A[i] * C[i - 1];
-            const double ui = ring && i == N - 1 ? alpha : 0.0;
-            C[i] = C[i] / m;
-            R[i] = (R[i] - A[i] * R[i - 1]) / m;
-            U[i] = (ui - A[i] * U[i - 1]) / m;
-        }
-        double xn = R[N - 1], zn = U[N - 1];
-        X[N - 1] = xn, U[N - 1] = zn;
-        for (int i = N - 2; i >= 0; i--) {
-            xn = R[i] - C[i] * xn;
-            zn = U[i] - C[i] * zn;
-            X[i] = xn, U[i] = zn;
-        }
-        if (ring) {
-            const double fact = (X[0] + beta * X[N - 1] / gamma) /
-                                (1.0 + U[0] + beta * U[N - 1] / gamma);
-            for (int i = 0; i < N; i++) X[i] -= fact * U[i];
+    // Row 0 couples x[N - 1] by beta = A[0], row N - 1 couples x[0] by alpha = C[N - 1] (a ring;
+    // dropped for a closed line).
+    double beta = 0.0, alpha = 0.0, gamma = 1.0;
+    if (on && ring) beta = A[0], alpha = C[N - 1], gamma = -B[0];
+    __syncthreads();
+    if (mine) {
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const int i = 2 * lane + e;
+            double ui = 0.0;
+            if (i == 0) {
+                if (ring) B[0] -= gamma, ui = gamma;
+                A[0] = 0.0;
+            }
+            if (i == N - 1) {
+                if (ring) B[N - 1] -= alpha * beta / gamma, ui = alpha;
+                C[N - 1] = 0.0;
+            }
+            U[i] = ui;
+            Bi[i] = 1.0 / B[i];
         }
     }
     __syncthreads();
-    if (mine) seg_back(X[2 * lane], X[2 * lane + 1], al, ga, de, dst + a);
+    // After the step of stride s every equation couples the unknowns 2 s away; past the ends
+    // nothing.
+    for (int s = 1; s < N; s <<= 1) {
+        double na[2], nb[2], nc[2], nr[2], nu[2];
+        if (mine) {
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const int i = 2 * lane + e, im = i - s, ip = i + s;
+                double a_m = 0.0, bi_m = 1.0, c_m = 0.0, r_m = 0.0, u_m = 0.0;
+                double a_p = 0.0, bi_p = 1.0, c_p = 0.0, r_p = 0.0, u_p = 0.0;
+                if (im >= 0) a_m = A[im], bi_m = Bi[im], c_m = C[im], r_m = R[im], u_m = U[im];
+                if (ip < N) a_p = A[ip], bi_p = Bi[ip], c_p = C[ip], r_p = R[ip], u_p = U[ip];
+                const double k1 = A[i] * bi_m, k2 = C[i] * bi_p;
+                na[e] = -a_m * k1;
+                nc[e] = -c_p * k2;
+                nb[e] = B[i] - c_m * k1 - a_p * k2;
+                nr[e] = R[i] - r_m * k1 - r_p * k2;
+                nu[e] = U[i] - u_m * k1 - u_p * k2;
+            }
+        }
+        __syncthreads();
+        if (mine) {
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const int i = 2 * lane + e;
+                A[i] = na[e], B[i] = nb[e], C[i] = nc[e], R[i] = nr[e], U[i] = nu[e];
+                Bi[i] = 1.0 / nb[e];
+            }
+        }
+        __syncthreads();
+    }
+    // Every equation stands alone: x = R / B, z = U / B; the ring's correction x - fact z.
+    double x[2] = {0.0, 0.0};
+    if (mine) {
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const int i = 2 * lane + e;
+            x[e] = R[i] * Bi[i];
+            if (ring) A[i] = x[e], C[i] = U[i] * Bi[i];
+        }
+    }
+    if (ring) {
+        __syncthreads();
+        if (mine) {
+            const double fact = (A[0] + beta * A[N - 1] / gamma) /
+                                (1.0 + C[0] + beta * C[N - 1] / gamma);
+#pragma unroll
+            for (int e = 0; e < 2; e++) x[e] -= fact * C[2 * lane + e];
+        }
+    }
+    if (mine) seg_back(x[0], x[1], al, ga, de, dst + a);
 }
 
 // ---- sums in the reference's order (DTD_SUMS_STRIPS) ----------------------------------------
