@@ -348,14 +348,27 @@ __global__ void kb_pair(const int2 *__restrict__ rng, const CgState *__restrict_
 
 // flag = 1 when every grid's loop has stopped.
 // flag[0] = every grid has stopped; also into host (host memory the device can write) if given.
+// With mask given (host memory the device can write), also mask[g] = grid g still runs and
+// mask[n] = how many do.
 __global__ __launch_bounds__(kBlock) void kb_all_done(const CgState *__restrict__ st, int n,
-                                                      int *flag, int *host = nullptr) {
+                                                      int *flag, int *host = nullptr,
+                                                      int *mask = nullptr) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
     int open = 0;
-    for (int g = threadIdx.x; g < n; g += kBlock) open |= st[g].done == 0;
-    open = __syncthreads_or(open);
+    for (int g = threadIdx.x; g < n; g += kBlock) {
+        const int o = st[g].done == 0;
+        open += o;
+        if (mask) reinterpret_cast<volatile int *>(mask)[g] = o;
+    }
+    if (open) atomicAdd(&cnt, open);
+    __syncthreads();
     if (threadIdx.x == 0) {
-        flag[0] = !open;
-        if (host) *reinterpret_cast<volatile int *>(host) = !open;
+        flag[0] = cnt == 0;
+        if (mask) reinterpret_cast<volatile int *>(mask)[n] = cnt;
+        __threadfence_system();
+        if (host) *reinterpret_cast<volatile int *>(host) = cnt == 0;
     }
 }
 
@@ -364,6 +377,20 @@ __global__ __launch_bounds__(kBlock) void kb_all_done(const CgState *__restrict_
 struct BLaunch {  // one launch: its block table (device) and block count
     const int2 *blk = nullptr;
     int nblk = 0;
+};
+
+// The launch tables of the iterations (and of a pass's first residual, first cycle and finish)
+// over one set of grids: every grid of the batch, or the ones still running (the others' blocks
+// left out, so that a launch dispatches only blocks with work).
+struct BTables {
+    BLaunch update_l, band_l;
+    const int2 *rng_band = nullptr, *rng_rr = nullptr, *rng_rz = nullptr;
+    int rr_base_pre1 = 0, rr_base_update = 0;  // r.r partials: pre2[0]'s, pre1[0]'s, the update's
+    int rz_base_post1 = 0, rz_base_coarsest = 0;
+    std::vector<BLaunch> pre2, pre1, post2, post1, coarsest;  // per level
+    const int4 *bottom_tab = nullptr;
+    int nbottom = 0;
+    int ngrids = 0;  // grids with blocks in the tables
 };
 
 struct BatchWs {
@@ -380,25 +407,33 @@ struct BatchWs {
     BInit *init = nullptr;
     double *parts = nullptr, *pairs = nullptr;
     int *flag = nullptr;
-    int2 *tables = nullptr;  // every block and range table, one allocation
-    // f64 side
-    BLaunch cells_l, update_l, rows_l, rows4_l, cols_l, tiles_l, band_l;
-    const int2 *rng_band = nullptr, *rng_rr = nullptr, *rng_rz = nullptr;
-    int rr_base_pre1 = 0, rr_base_update = 0;  // r.r partials: pre2[0]'s, pre1[0]'s, the update's
-    // cycle, per level
-    std::vector<BLaunch> coarsen_l, pre2, pre1, post2, post1, coarsest;
-    // the bottoms: every grid's levels from its bottom on, the launch table, the LDS bytes
+    int2 *tables = nullptr;  // the set-up's block tables, one allocation
+    // the set-up's launches: the operator, the first guess, the coarsening per level
+    BLaunch cells_l, rows_l, rows4_l, cols_l, tiles_l;
+    std::vector<BLaunch> coarsen_l;
+    // the iterations' launches over every grid, and over the grids still running
+    BTables all, part;
+    int2 *part_tab = nullptr;  // room for part's tables (as many entries as all's)
+    int4 *part_bottom = nullptr;
+    size_t tab_entries = 0;
+    // what the iterations' tables are built from: each level's tasks (grid and record), each
+    // grid's first level its bottom cycles (its level count: none), the bottoms' table
+    std::vector<std::vector<int>> lv_grid;
+    std::vector<std::vector<BLevel>> lv;
+    std::vector<int> b0;
+    std::vector<int4> btab;
+    // the bottoms: every grid's levels from its bottom on, the LDS bytes
     const Lvl<float> *bottom_lv = nullptr;
-    const int4 *bottom_tab = nullptr;
-    int nbottom = 0;
     size_t bottom_lds = 0;
-    int rz_base_post1 = 0, rz_base_coarsest = 0;
     std::vector<void *> owned;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     // pinned host
     CgState *h_st = nullptr;
     double *h_pairs = nullptr;
     BInit *h_init = nullptr;
+    int *h_mask = nullptr, *d_mask = nullptr;  // two slots of n + 1 (kb_all_done's mask)
+    std::vector<int2> h_part_tab;               // part's tables as last uploaded
+    std::vector<int4> h_part_bot;
 };
 
 void free_batch(dtd_ctx *ctx) {
@@ -410,6 +445,7 @@ void free_batch(dtd_ctx *ctx) {
     if (bw->h_st) (void)hipHostFree(bw->h_st);
     if (bw->h_pairs) (void)hipHostFree(bw->h_pairs);
     if (bw->h_init) (void)hipHostFree(bw->h_init);
+    if (bw->h_mask) (void)hipHostFree(bw->h_mask);
     delete bw;
     ctx->batch = nullptr;
 }
@@ -431,6 +467,113 @@ static std::vector<long> batch_key(const dtd_ctx *ctx, size_t n, const dtd_grid 
 }
 
 static inline size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+// The iterations' launch tables of the grids with active[g] (every grid when active is null):
+// block tables and each grid's ranges of partials into tab, whose device copy will sit at dtab,
+// and the bottoms' table into bot (at dbot).  A grid left out has no blocks and empty ranges.
+static void iteration_tables(const BatchWs &bw, const int *active, BTables &T,
+                             std::vector<int2> &tab, int2 *dtab, std::vector<int4> &bot,
+                             int4 *dbot) {
+    const int n = bw.ngrids, L = bw.nlevels;
+    tab.clear();
+    bot.clear();
+    auto on = [&](int g) { return !active || active[g] != 0; };
+    auto add_launch = [&](BLaunch &bl, const std::vector<int> &blocks_of,
+                          std::vector<int2> *rng) {
+        // blocks_of[task] blocks for each task, in task order.
+        const size_t at = tab.size();
+        for (size_t t = 0; t < blocks_of.size(); t++) {
+            if (rng) rng->push_back(make_int2((int)(tab.size() - at), blocks_of[t]));
+            for (int b = 0; b < blocks_of[t]; b++) tab.push_back(make_int2((int)t, b));
+        }
+        bl.nblk = (int)(tab.size() - at);
+        bl.blk = dtab ? dtab + at : nullptr;
+    };
+    T.ngrids = 0;
+    for (int g = 0; g < n; g++) T.ngrids += on(g);
+    std::vector<int> cnt(n);
+    for (int g = 0; g < n; g++)
+        cnt[g] = on(g) && bw.nlev[g] <= 1 ? nblocks((long)bw.grids[g].g.n, kCgCells) : 0;
+    std::vector<int2> rng_update, rng_band, rng_rz(n, make_int2(0, 0)), rng_rr(n, make_int2(0, 0));
+    add_launch(T.update_l, cnt, &rng_update);
+    for (int g = 0; g < n; g++)
+        cnt[g] = on(g) ? stream_grid<1, 1>(bw.grids[g].g.w, bw.grids[g].g.h, bw.grids[g].th) : 0;
+    add_launch(T.band_l, cnt, &rng_band);
+    // The smoothing launches of the grids for which level l is neither the coarsest nor cycled
+    // by the grid's bottom, split by lane width; the coarsest launches.
+    T.pre2.assign(L, BLaunch{}), T.pre1.assign(L, BLaunch{});
+    T.post2.assign(L, BLaunch{}), T.post1.assign(L, BLaunch{});
+    T.coarsest.assign(L, BLaunch{});
+    for (int l = 0; l < L; l++) {
+        const size_t nt = bw.lv[l].size();
+        std::vector<int> c2(nt, 0), c1(nt, 0), cc(nt, 0), q2(nt, 0), q1(nt, 0);
+        for (size_t t = 0; t < nt; t++) {
+            const BLevel &q = bw.lv[l][t];
+            const int g = bw.lv_grid[l][t];
+            if (!on(g)) continue;
+            const bool last = l + 1 == bw.nlev[g];
+            if (!last) {
+                if (l >= bw.b0[g]) continue;  // the grid's bottom cycles this level
+                // c: the pre-smoothing's blocks, q: the post-smoothing's.
+                if (q.L.w % 2 == 0)
+                    c2[t] = stream_grid<2, 1>(q.L.w, q.L.h, q.th),
+                    q2[t] = stream_grid<2, 1>(q.L.w, q.L.h, q.th_post);
+                else
+                    c1[t] = stream_grid<1, 2>(q.L.w, q.L.h, q.th),
+                    q1[t] = stream_grid<1, 2>(q.L.w, q.L.h, q.th_post);
+            } else if (l < bw.b0[g]) {
+                if (q.L.n > 64) throw Error{DTD_ESTATE, "the coarsest level exceeds one wave"};
+                cc[t] = 1;
+            }
+        }
+        std::vector<int2> r2, r1, rc, s2, s1;
+        add_launch(T.pre2[l], c2, l == 0 ? &s2 : nullptr);
+        add_launch(T.pre1[l], c1, l == 0 ? &s1 : nullptr);
+        add_launch(T.post2[l], q2, l == 0 ? &r2 : nullptr);
+        add_launch(T.post1[l], q1, l == 0 ? &r1 : nullptr);
+        add_launch(T.coarsest[l], cc, l == 0 ? &rc : nullptr);
+        if (l == 0) {
+            // The r.r partials of an iteration: pre2's blocks, then pre1's, then the separate
+            // update's (grids of one level).
+            T.rr_base_pre1 = T.pre2[0].nblk;
+            T.rr_base_update = T.pre2[0].nblk + T.pre1[0].nblk;
+            // The r.z partials: post2's blocks, then post1's, then the coarsest launch's.
+            T.rz_base_post1 = T.post2[0].nblk;
+            T.rz_base_coarsest = T.post2[0].nblk + T.post1[0].nblk;
+            for (size_t t = 0; t < nt; t++) {
+                const int g = bw.lv_grid[0][t];
+                if (c2[t])
+                    rng_rr[g] = s2[t], rng_rz[g] = r2[t];
+                else if (c1[t])
+                    rng_rr[g] = make_int2(T.rr_base_pre1 + s1[t].x, s1[t].y),
+                    rng_rz[g] = make_int2(T.rz_base_post1 + r1[t].x, r1[t].y);
+                else
+                    rng_rr[g] = make_int2(T.rr_base_update + rng_update[g].x, rng_update[g].y),
+                    rng_rz[g] = make_int2(T.rz_base_coarsest + rc[t].x, cc[t]);
+            }
+        }
+    }
+    const size_t at_rng = tab.size();
+    tab.insert(tab.end(), rng_band.begin(), rng_band.end());
+    tab.insert(tab.end(), rng_rr.begin(), rng_rr.end());
+    tab.insert(tab.end(), rng_rz.begin(), rng_rz.end());
+    T.rng_band = dtab ? dtab + at_rng : nullptr;
+    T.rng_rr = dtab ? T.rng_band + n : nullptr;
+    T.rng_rz = dtab ? T.rng_rr + n : nullptr;
+    for (const int4 &b : bw.btab)
+        if (on(b.z)) bot.push_back(b);
+    T.nbottom = (int)bot.size();
+    T.bottom_tab = dbot;
+}
+
+// The entries of the iterations' tables over every grid.
+static size_t iteration_entries(const BatchWs &bw) {
+    BTables T;
+    std::vector<int2> tab;
+    std::vector<int4> bot;
+    iteration_tables(bw, nullptr, T, tab, nullptr, bot, nullptr);
+    return tab.size();
+}
 
 // Builds the device buffers and launch tables for grids gs (all non-empty).
 static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
@@ -549,156 +692,104 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
     // grid's cycle chooses it (at most bottom_cells() cells; that level and the coarser ones'
     // records, r and z within kBottomLds; at most kBottomLevels levels).  A grid of one level
     // has none (b0 = its level count).
-    std::vector<int> b0(n);
+    bw->b0.assign(n, 0);
     {
         std::vector<Lvl<float>> blv;
-        std::vector<int4> btab;
         for (size_t g = 0; g < n; g++) {
             const int nl = bw->nlev[g];
-            b0[g] = nl;
+            bw->b0[g] = nl;
             if (nl < 2 || !fused_enabled()) continue;
             for (int l = 1; l < nl; l++) {
                 size_t lds = 0;
                 for (int k = l; k < nl; k++) lds += (kRec + 3) * (size_t)sh[g][k].w * sh[g][k].h * 4;
                 if ((long)sh[g][l].w * sh[g][l].h <= bottom_cells() && lds <= kBottomLds &&
                     nl - l <= kBottomLevels) {
-                    b0[g] = l;
-                    btab.push_back(make_int4((int)blv.size(), nl - l, (int)g, 0));
+                    bw->b0[g] = l;
+                    bw->btab.push_back(make_int4((int)blv.size(), nl - l, (int)g, 0));
                     for (int k = l; k < nl; k++) blv.push_back(level(g, k));
                     bw->bottom_lds = std::max(bw->bottom_lds, lds);
                     break;
                 }
             }
         }
-        bw->nbottom = (int)btab.size();
-        if (bw->nbottom) {
+        if (!blv.empty()) {
             Lvl<float> *dl = (Lvl<float> *)own(blv.size() * sizeof(Lvl<float>));
             HIPCHK(hipMemcpy(dl, blv.data(), blv.size() * sizeof(Lvl<float>), hipMemcpyHostToDevice));
-            int4 *dt = (int4 *)own(btab.size() * sizeof(int4));
-            HIPCHK(hipMemcpy(dt, btab.data(), btab.size() * sizeof(int4), hipMemcpyHostToDevice));
-            bw->bottom_lv = dl, bw->bottom_tab = dt;
+            bw->bottom_lv = dl;
         }
     }
-    // Block and range tables, in one host vector; launches keep their offsets until upload.
+    // The set-up's block tables.
     std::vector<int2> tab;
     struct Pending {
         BLaunch *l;
         size_t at;
     };
     std::vector<Pending> pend;
-    auto add_launch = [&](BLaunch &bl, const std::vector<int> &blocks_of, std::vector<int2> *rng) {
+    auto add_launch = [&](BLaunch &bl, const std::vector<int> &blocks_of) {
         // blocks_of[task] blocks for each task, in task order.
         const size_t at = tab.size();
-        for (size_t t = 0; t < blocks_of.size(); t++) {
-            if (rng) rng->push_back(make_int2((int)(tab.size() - at), blocks_of[t]));
+        for (size_t t = 0; t < blocks_of.size(); t++)
             for (int b = 0; b < blocks_of[t]; b++) tab.push_back(make_int2((int)t, b));
-        }
         bl.nblk = (int)(tab.size() - at);
         pend.push_back({&bl, at});
     };
     std::vector<int> cnt(n);
     for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].w * gs[g].h);
-    add_launch(bw->cells_l, cnt, nullptr);
-    for (size_t g = 0; g < n; g++)
-        cnt[g] = bw->nlev[g] > 1 ? 0 : nblocks((long)gs[g].w * gs[g].h, kCgCells);
-    std::vector<int2> rng_update, rng_band, rng_rz(n), rng_rr(n);
-    add_launch(bw->update_l, cnt, &rng_update);
+    add_launch(bw->cells_l, cnt);
     // The first guess's rows: by the one-wave partition method (rows4_l, four rows a block), or
     // by the recurrences on the transposed grid (rows_l, tiles_l); then every grid's columns.
     for (size_t g = 0; g < n; g++)
         cnt[g] = rows_by_wave(gs[g].w) ? 0 : nblocks((long)gs[g].h, kLineBlock);
-    add_launch(bw->rows_l, cnt, nullptr);
+    add_launch(bw->rows_l, cnt);
     for (size_t g = 0; g < n; g++)
         cnt[g] = rows_by_wave(gs[g].w) ? nblocks((long)gs[g].h, kBlock / 64) : 0;
-    add_launch(bw->rows4_l, cnt, nullptr);
+    add_launch(bw->rows4_l, cnt);
     for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].w, kLineBlock);
-    add_launch(bw->cols_l, cnt, nullptr);
+    add_launch(bw->cols_l, cnt);
     for (size_t g = 0; g < n; g++)
         cnt[g] = rows_by_wave(gs[g].w) ? 0
                                        : (int)(((gs[g].w + 31) / 32) * ((gs[g].h + 31) / 32));
-    add_launch(bw->tiles_l, cnt, nullptr);
-    for (size_t g = 0; g < n; g++)
-        cnt[g] = stream_grid<1, 1>((int)gs[g].w, (int)gs[g].h, bw->grids[g].th);
-    add_launch(bw->band_l, cnt, &rng_band);
-    // The cycle: coarsening of level l + 1 from level l's tasks; the smoothing launches of the
-    // grids for which level l is not the coarsest, split by lane width; the coarsest launches.
+    add_launch(bw->tiles_l, cnt);
+    // Coarsening of level l + 1 from level l's tasks.
     bw->coarsen_l.resize(L);
-    bw->pre2.resize(L), bw->pre1.resize(L), bw->post2.resize(L), bw->post1.resize(L);
-    bw->coarsest.resize(L);
     for (int l = 0; l < L; l++) {
-        const size_t nt = lv[l].size();
-        std::vector<int> cz(nt, 0), c2(nt, 0), c1(nt, 0), cc(nt, 0), q2(nt, 0), q1(nt, 0);
-        for (size_t t = 0; t < nt; t++) {
-            const BLevel &q = lv[l][t];
-            const int g = lv_grid[l][t];
-            const bool last = l + 1 == bw->nlev[g];
-            if (!last) {
-                cz[t] = nblocks(q.C.n);
-                if (l >= b0[g]) continue;  // the grid's bottom cycles this level
-                // c: the pre-smoothing's blocks, q: the post-smoothing's.
-                if (q.L.w % 2 == 0)
-                    c2[t] = stream_grid<2, 1>(q.L.w, q.L.h, q.th),
-                    q2[t] = stream_grid<2, 1>(q.L.w, q.L.h, q.th_post);
-                else
-                    c1[t] = stream_grid<1, 2>(q.L.w, q.L.h, q.th),
-                    q1[t] = stream_grid<1, 2>(q.L.w, q.L.h, q.th_post);
-            } else if (l < b0[g]) {
-                if (q.L.n > 64) throw Error{DTD_ESTATE, "the coarsest level exceeds one wave"};
-                cc[t] = 1;
-            }
-        }
-        add_launch(bw->coarsen_l[l], cz, nullptr);
-        std::vector<int2> r2, r1, rc, s2, s1;
-        add_launch(bw->pre2[l], c2, l == 0 ? &s2 : nullptr);
-        add_launch(bw->pre1[l], c1, l == 0 ? &s1 : nullptr);
-        add_launch(bw->post2[l], q2, l == 0 ? &r2 : nullptr);
-        add_launch(bw->post1[l], q1, l == 0 ? &r1 : nullptr);
-        add_launch(bw->coarsest[l], cc, l == 0 ? &rc : nullptr);
-        if (l == 0) {
-            // The r.r partials of an iteration: pre2's blocks, then pre1's, then the separate
-            // update's (grids of one level).
-            bw->rr_base_pre1 = bw->pre2[0].nblk;
-            bw->rr_base_update = bw->pre2[0].nblk + bw->pre1[0].nblk;
-            for (size_t t = 0; t < nt; t++) {
-                const int g = lv_grid[0][t];
-                if (c2[t])
-                    rng_rr[g] = s2[t];
-                else if (c1[t])
-                    rng_rr[g] = make_int2(bw->rr_base_pre1 + s1[t].x, s1[t].y);
-                else
-                    rng_rr[g] = make_int2(bw->rr_base_update + rng_update[g].x, rng_update[g].y);
-            }
-            // The r.z partials: post2's blocks, then post1's, then the coarsest launch's.
-            bw->rz_base_post1 = bw->post2[0].nblk;
-            bw->rz_base_coarsest = bw->post2[0].nblk + bw->post1[0].nblk;
-            for (size_t t = 0; t < nt; t++) {
-                const int g = lv_grid[0][t];
-                if (c2[t])
-                    rng_rz[g] = r2[t];
-                else if (c1[t])
-                    rng_rz[g] = make_int2(bw->rz_base_post1 + r1[t].x, r1[t].y);
-                else
-                    rng_rz[g] = make_int2(bw->rz_base_coarsest + rc[t].x, 1);
-            }
-        }
+        std::vector<int> cz(lv[l].size(), 0);
+        for (size_t t = 0; t < lv[l].size(); t++)
+            if (l + 1 < bw->nlev[lv_grid[l][t]]) cz[t] = nblocks(lv[l][t].C.n);
+        add_launch(bw->coarsen_l[l], cz);
     }
-    const size_t at_rng = tab.size();
-    tab.insert(tab.end(), rng_band.begin(), rng_band.end());
-    tab.insert(tab.end(), rng_rr.begin(), rng_rr.end());
-    tab.insert(tab.end(), rng_rz.begin(), rng_rz.end());
     bw->tables = (int2 *)own(tab.size() * sizeof(int2));
     HIPCHK(hipMemcpy(bw->tables, tab.data(), tab.size() * sizeof(int2), hipMemcpyHostToDevice));
-    for (auto &p : pend) p.l->blk = bw->tables + p.at;
-    bw->rng_band = bw->tables + at_rng;
-    bw->rng_rr = bw->rng_band + n;
-    bw->rng_rz = bw->rng_rr + n;
+    for (auto &q : pend) q.l->blk = bw->tables + q.at;
+    bw->lv = std::move(lv);
+    bw->lv_grid = std::move(lv_grid);
+    // The iterations' tables over every grid, and room for those over the running grids.
+    {
+        std::vector<int2> itab;
+        std::vector<int4> ibot;
+        const size_t ne = iteration_entries(*bw);
+        int2 *d_all = (int2 *)own(ne * sizeof(int2));
+        bw->part_tab = (int2 *)own(ne * sizeof(int2));
+        bw->tab_entries = ne;
+        int4 *d_bot = (int4 *)own((bw->btab.size() + 1) * sizeof(int4));
+        bw->part_bottom = (int4 *)own((bw->btab.size() + 1) * sizeof(int4));
+        iteration_tables(*bw, nullptr, bw->all, itab, d_all, ibot, d_bot);
+        HIPCHK(hipMemcpy(d_all, itab.data(), itab.size() * sizeof(int2), hipMemcpyHostToDevice));
+        if (!ibot.empty())
+            HIPCHK(hipMemcpy(d_bot, ibot.data(), ibot.size() * sizeof(int4),
+                             hipMemcpyHostToDevice));
+    }
     // Partial sums: pairs for the widest launch (and the r.z launches side by side).
-    size_t most = (size_t)std::max({bw->band_l.nblk, bw->rr_base_update + bw->update_l.nblk,
-                                    bw->post2[0].nblk + bw->post1[0].nblk + bw->coarsest[0].nblk});
+    const BTables &T = bw->all;
+    size_t most = (size_t)std::max({T.band_l.nblk, T.rr_base_update + T.update_l.nblk,
+                                    T.post2[0].nblk + T.post1[0].nblk + T.coarsest[0].nblk});
     bw->parts = (double *)own(2 * most * sizeof(double) + 16);
     HIPCHK(hipHostMalloc((void **)&bw->h_st, n * sizeof(CgState), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&bw->h_pairs, 2 * n * sizeof(double), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&bw->h_init, n * sizeof(BInit), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&bw->h_mask, 2 * (n + 1) * sizeof(int),
+                         hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer((void **)&bw->d_mask, bw->h_mask, 0));
 }
 
 // ---- enqueueing ------------------------------------------------------------------------------
@@ -708,74 +799,74 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
 // finest level's leaves the r.z partials).  parity < 0: the first cycle (the finest level reads
 // the first residual); else iteration `parity`, whose finest pre-smoothing updates the residual
 // first, after which the r.r step runs (before the finest post-smoothing reuses the partials).
-static void batch_vcycle(BatchWs &bw, hipStream_t s, int parity) {
+static void batch_vcycle(const BatchWs &bw, const BTables &T, hipStream_t s, int parity) {
     const int L = bw.nlevels;
     const int par = parity < 0 ? 0 : parity;
     for (int l = 0; l < L; l++) {
         if (l == 0) {
-            if (bw.pre2[0].nblk) {
+            if (T.pre2[0].nblk) {
                 if (parity < 0)
-                    kb_spre0<2, kR64K><<<bw.pre2[0].nblk, kBlock, 0, s>>>(bw.d_lv[0], bw.pre2[0].blk,
+                    kb_spre0<2, kR64K><<<T.pre2[0].nblk, kBlock, 0, s>>>(bw.d_lv[0], T.pre2[0].blk,
                                                                        par, bw.parts);
                 else
-                    kb_spre0<2, kRUpdateK><<<bw.pre2[0].nblk, kBlock, 0, s>>>(
-                        bw.d_lv[0], bw.pre2[0].blk, par, bw.parts);
+                    kb_spre0<2, kRUpdateK><<<T.pre2[0].nblk, kBlock, 0, s>>>(
+                        bw.d_lv[0], T.pre2[0].blk, par, bw.parts);
                 LAUNCH_CHECK();
             }
-            if (bw.pre1[0].nblk) {
+            if (T.pre1[0].nblk) {
                 if (parity < 0)
-                    kb_spre0<1, kR64K><<<bw.pre1[0].nblk, kBlock, 0, s>>>(
-                        bw.d_lv[0], bw.pre1[0].blk, par, bw.parts + bw.rr_base_pre1);
+                    kb_spre0<1, kR64K><<<T.pre1[0].nblk, kBlock, 0, s>>>(
+                        bw.d_lv[0], T.pre1[0].blk, par, bw.parts + T.rr_base_pre1);
                 else
-                    kb_spre0<1, kRUpdateK><<<bw.pre1[0].nblk, kBlock, 0, s>>>(
-                        bw.d_lv[0], bw.pre1[0].blk, par, bw.parts + bw.rr_base_pre1);
+                    kb_spre0<1, kRUpdateK><<<T.pre1[0].nblk, kBlock, 0, s>>>(
+                        bw.d_lv[0], T.pre1[0].blk, par, bw.parts + T.rr_base_pre1);
                 LAUNCH_CHECK();
             }
             if (parity >= 0) {
-                kb_step<kStepRR><<<bw.ngrids, kReduceBlock, 0, s>>>(bw.rng_rr, bw.st, bw.parts);
+                kb_step<kStepRR><<<bw.ngrids, kReduceBlock, 0, s>>>(T.rng_rr, bw.st, bw.parts);
                 LAUNCH_CHECK();
             }
         } else {
-            if (bw.pre2[l].nblk) {
-                kb_spre<2><<<bw.pre2[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.pre2[l].blk);
+            if (T.pre2[l].nblk) {
+                kb_spre<2><<<T.pre2[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], T.pre2[l].blk);
                 LAUNCH_CHECK();
             }
-            if (bw.pre1[l].nblk) {
-                kb_spre<1><<<bw.pre1[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.pre1[l].blk);
+            if (T.pre1[l].nblk) {
+                kb_spre<1><<<T.pre1[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], T.pre1[l].blk);
                 LAUNCH_CHECK();
             }
         }
-        if (bw.coarsest[l].nblk) {
-            kb_coarsest<<<bw.coarsest[l].nblk, 64, 0, s>>>(
-                bw.d_lv[l], bw.coarsest[l].blk, bw.parts + (l == 0 ? bw.rz_base_coarsest : 0));
+        if (T.coarsest[l].nblk) {
+            kb_coarsest<<<T.coarsest[l].nblk, 64, 0, s>>>(
+                bw.d_lv[l], T.coarsest[l].blk, bw.parts + (l == 0 ? T.rz_base_coarsest : 0));
             LAUNCH_CHECK();
         }
     }
     // Every grid's bottom, once its levels above have restricted into it.
-    if (bw.nbottom) {
-        kb_bottom<<<bw.nbottom, kBottomBlock, bw.bottom_lds, s>>>(bw.bottom_lv, bw.bottom_tab,
+    if (T.nbottom) {
+        kb_bottom<<<T.nbottom, kBottomBlock, bw.bottom_lds, s>>>(bw.bottom_lv, T.bottom_tab,
                                                                   bw.st);
         LAUNCH_CHECK();
     }
     // The current residual after iteration `parity`: r64[parity ^ 1]; in the first cycle r64[0].
     const int cur = parity < 0 ? 0 : parity ^ 1;
     for (int l = L - 2; l >= 0; l--) {
-        if (bw.post2[l].nblk) {
+        if (T.post2[l].nblk) {
             if (l == 0)
-                kb_spost0<2><<<bw.post2[0].nblk, kBlock, 0, s>>>(bw.d_lv[0], bw.post2[0].blk, cur,
+                kb_spost0<2><<<T.post2[0].nblk, kBlock, 0, s>>>(bw.d_lv[0], T.post2[0].blk, cur,
                                                                  bw.parts);
             else
-                kb_spost<2><<<bw.post2[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.post2[l].blk,
+                kb_spost<2><<<T.post2[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], T.post2[l].blk,
                                                                 bw.parts);
             LAUNCH_CHECK();
         }
-        if (bw.post1[l].nblk) {
+        if (T.post1[l].nblk) {
             if (l == 0)
-                kb_spost0<1><<<bw.post1[0].nblk, kBlock, 0, s>>>(bw.d_lv[0], bw.post1[0].blk, cur,
-                                                                 bw.parts + bw.rz_base_post1);
+                kb_spost0<1><<<T.post1[0].nblk, kBlock, 0, s>>>(bw.d_lv[0], T.post1[0].blk, cur,
+                                                                 bw.parts + T.rz_base_post1);
             else
-                kb_spost<1><<<bw.post1[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.post1[l].blk,
-                                                                bw.parts + bw.rz_base_post1);
+                kb_spost<1><<<T.post1[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], T.post1[l].blk,
+                                                                bw.parts + T.rz_base_post1);
             LAUNCH_CHECK();
         }
     }
@@ -783,21 +874,22 @@ static void batch_vcycle(BatchWs &bw, hipStream_t s, int parity) {
 
 // One iteration of every grid still running (MG:630-733), then the all-stopped flag.  The grids
 // of one level take the separate update (their r.r partials after the finest pre-smoothings').
-static void batch_iteration(BatchWs &bw, int parity, hipStream_t s, int *host_flag) {
+static void batch_iteration(const BatchWs &bw, const BTables &T, int parity, hipStream_t s,
+                            int *host_flag, int *mask) {
     const int n = bw.ngrids;
-    kb_sdir<<<bw.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.band_l.blk, parity, bw.parts);
+    kb_sdir<<<T.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, T.band_l.blk, parity, bw.parts);
     LAUNCH_CHECK();
-    kb_step<kStepPAP><<<n, kReduceBlock, 0, s>>>(bw.rng_band, bw.st, bw.parts);
+    kb_step<kStepPAP><<<n, kReduceBlock, 0, s>>>(T.rng_band, bw.st, bw.parts);
     LAUNCH_CHECK();
-    if (bw.update_l.nblk) {
-        kb_update<<<bw.update_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.update_l.blk,
-                                                      bw.parts + bw.rr_base_update);
+    if (T.update_l.nblk) {
+        kb_update<<<T.update_l.nblk, kBlock, 0, s>>>(bw.d_grids, T.update_l.blk,
+                                                      bw.parts + T.rr_base_update);
         LAUNCH_CHECK();
     }
-    batch_vcycle(bw, s, parity);
-    kb_step<kStepRZ><<<n, kReduceBlock, 0, s>>>(bw.rng_rz, bw.st, bw.parts);
+    batch_vcycle(bw, T, s, parity);
+    kb_step<kStepRZ><<<n, kReduceBlock, 0, s>>>(T.rng_rz, bw.st, bw.parts);
     LAUNCH_CHECK();
-    kb_all_done<<<1, kBlock, 0, s>>>(bw.st, n, bw.flag, host_flag);
+    kb_all_done<<<1, kBlock, 0, s>>>(bw.st, n, bw.flag, host_flag, mask);
     LAUNCH_CHECK();
 }
 
@@ -807,7 +899,8 @@ static void batch_graphs(dtd_ctx *ctx, BatchWs &bw) {
         hipGraph_t gr = nullptr;
         HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         try {
-            batch_iteration(bw, k, ctx->stream, ctx->dev_flag + k);
+            batch_iteration(bw, bw.all, k, ctx->stream, ctx->dev_flag + k,
+                            bw.d_mask + k * (bw.ngrids + 1));
         } catch (...) {
             (void)hipStreamEndCapture(ctx->stream, &gr);
             if (gr) (void)hipGraphDestroy(gr);
@@ -820,49 +913,79 @@ static void batch_graphs(dtd_ctx *ctx, BatchWs &bw) {
     }
 }
 
+// The iterations' tables over the grids with active[g] into bw.part, its device copy queued on
+// the stream.  The part tables must not be in use by queued work (they are rebuilt at most once
+// a pass, before the pass launches anything with them).
+static void part_tables(dtd_ctx *ctx, BatchWs &bw, const int *active) {
+    iteration_tables(bw, active, bw.part, bw.h_part_tab, bw.part_tab, bw.h_part_bot,
+                     bw.part_bottom);
+    if (bw.h_part_tab.size() > bw.tab_entries)
+        throw Error{DTD_ESTATE, "the running grids' tables exceed the batch's"};
+    HIPCHK(hipMemcpyAsync(bw.part_tab, bw.h_part_tab.data(), bw.h_part_tab.size() * sizeof(int2),
+                          hipMemcpyHostToDevice, ctx->stream));
+    if (!bw.h_part_bot.empty())
+        HIPCHK(hipMemcpyAsync(bw.part_bottom, bw.h_part_bot.data(),
+                              bw.h_part_bot.size() * sizeof(int4), hipMemcpyHostToDevice,
+                              ctx->stream));
+}
+
 // One pass of the loop over the active grids (h_init says which, and their tolerances): the
 // first residual (from b when from_b, else from x), the cycle, the iterations, the last update,
-// the put-back and the true residual.  Leaves h_st and h_pairs filled.
-static void batch_pass(dtd_ctx *ctx, BatchWs &bw, int from_b) {
+// the put-back and the true residual.  Leaves h_st and h_pairs filled.  part: the pass runs on
+// bw.part (built for its active grids) with the iterations launched one by one; otherwise on
+// every grid's tables and the captured iterations, until at most a quarter of the grids still
+// run: then the remaining iterations launch only their blocks.
+static void batch_pass(dtd_ctx *ctx, BatchWs &bw, int from_b, bool part) {
     hipStream_t s = ctx->stream;
     const int n = bw.ngrids;
+    const BTables *T = part ? &bw.part : &bw.all;
     HIPCHK(hipMemcpyAsync(bw.init, bw.h_init, n * sizeof(BInit), hipMemcpyHostToDevice, s));
     kb_init<<<nblocks(n), kBlock, 0, s>>>(bw.st, bw.init, bw.d_grids, n, (int)ctx->opts.max_iter);
     LAUNCH_CHECK();
-    kb_sresid<<<bw.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.band_l.blk, from_b, bw.parts);
+    kb_sresid<<<T->band_l.nblk, kBlock, 0, s>>>(bw.d_grids, T->band_l.blk, from_b, bw.parts);
     LAUNCH_CHECK();
-    kb_step<kStepRR0><<<n, kReduceBlock, 0, s>>>(bw.rng_band, bw.st, bw.parts);
+    kb_step<kStepRR0><<<n, kReduceBlock, 0, s>>>(T->rng_band, bw.st, bw.parts);
     LAUNCH_CHECK();
-    batch_vcycle(bw, s, -1);
-    kb_step<kStepRZ0><<<n, kReduceBlock, 0, s>>>(bw.rng_rz, bw.st, bw.parts);
+    batch_vcycle(bw, *T, s, -1);
+    kb_step<kStepRZ0><<<n, kReduceBlock, 0, s>>>(T->rng_rz, bw.st, bw.parts);
     LAUNCH_CHECK();
     kb_all_done<<<1, kBlock, 0, s>>>(bw.st, n, bw.flag);
     LAUNCH_CHECK();
-    if (ctx->use_graph) batch_graphs(ctx, bw);
+    bool eager = part || !ctx->use_graph;
+    if (!eager) batch_graphs(ctx, bw);
     // The loop: a pass ends when every grid has stopped.  Iteration k + 1 is launched before the
     // flag after iteration k is read (the flag before the loop counts as iteration 0's), so the
     // device runs at most one iteration that every grid skips; the state's cap stops every grid by
-    // max_iter.
+    // max_iter.  The running grids after iteration k - 1 include every grid that iteration k + 1
+    // finds running.
     HIPCHK(hipMemcpyAsync(&ctx->host_flag[1], bw.flag, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(ctx->flag_ev[1], s));
     const int max_iter = (int)ctx->opts.max_iter;
     for (int k = 0; k <= max_iter + 1; k++) {
         const int par = k & 1;
-        if (ctx->use_graph)
+        if (!eager)
             HIPCHK(hipGraphLaunch(bw.graph[par], s));
         else
-            batch_iteration(bw, par, s, ctx->dev_flag + par);
+            batch_iteration(bw, *T, par, s, ctx->dev_flag + par, bw.d_mask + par * (n + 1));
         HIPCHK(hipEventRecord(ctx->flag_ev[par], s));
         HIPCHK(hipEventSynchronize(ctx->flag_ev[par ^ 1]));
         if (ctx->host_flag[par ^ 1]) break;
+        const int *mask = bw.h_mask + (par ^ 1) * (n + 1);
+        if (T == &bw.all && k >= 1 && 4 * mask[n] <= n) {
+            part_tables(ctx, bw, mask);
+            T = &bw.part;
+            eager = true;
+        }
     }
-    kb_finish1<<<bw.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.band_l.blk, bw.parts);
+    // The last update, the put-back and the true residual of every grid the pass ran.
+    const BTables &F = part ? bw.part : bw.all;
+    kb_finish1<<<F.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, F.band_l.blk, bw.parts);
     LAUNCH_CHECK();
-    kb_putback<<<n, kReduceBlock, 0, s>>>(bw.rng_band, bw.st, bw.parts);
+    kb_putback<<<n, kReduceBlock, 0, s>>>(F.rng_band, bw.st, bw.parts);
     LAUNCH_CHECK();
-    kb_finish2<<<bw.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.band_l.blk, bw.parts);
+    kb_finish2<<<F.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, F.band_l.blk, bw.parts);
     LAUNCH_CHECK();
-    kb_pair<<<n, kReduceBlock, 0, s>>>(bw.rng_band, bw.st, bw.parts, bw.pairs);
+    kb_pair<<<n, kReduceBlock, 0, s>>>(F.rng_band, bw.st, bw.parts, bw.pairs);
     LAUNCH_CHECK();
     HIPCHK(hipMemcpyAsync(bw.h_st, bw.st, n * sizeof(CgState), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(bw.h_pairs, bw.pairs, 2 * n * sizeof(double), hipMemcpyDeviceToHost,
@@ -955,7 +1078,7 @@ static void solve_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs_in, doubl
         LAUNCH_CHECK();
     }
     HIPCHK(hipEventRecord(ctx->ev[2], s));
-    batch_pass(ctx, bw, lod_guess ? 0 : 1);
+    batch_pass(ctx, bw, lod_guess ? 0 : 1, false);
     // Per grid: iterations, the true relative residual; then the grids that missed the target
     // solve again from their last answer, aiming below it by the factor they missed it by (at most
     // four times), the others left alone.
@@ -974,6 +1097,7 @@ static void solve_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs_in, doubl
         }
     };
     collect(true);
+    std::vector<int> again_of(n, 0);
     for (int round = 0; relative && round < 4; round++) {
         bool any = false;
         for (size_t g = 0; g < n; g++) {
@@ -986,9 +1110,12 @@ static void solve_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs_in, doubl
             }
             const double nn = (double)(gs[g].w * gs[g].h);
             bw.h_init[g] = BInit{tolm[g] * tolm[g] * nn, 0.0, again ? 1 : 0, 0};
+            again_of[g] = again;
         }
         if (!any) break;
-        batch_pass(ctx, bw, 0);
+        // Only the grids solved again have blocks in the pass's launches.
+        part_tables(ctx, bw, again_of.data());
+        batch_pass(ctx, bw, 0, true);
         collect(false);
     }
     HIPCHK(hipEventRecord(ctx->ev[3], s));
