@@ -82,6 +82,31 @@ __global__ __launch_bounds__(kBlock) void kb_coarsen(const BLevel *__restrict__ 
     if (k < q.C.n) coarsen_cell(q.L, q.C, scale, k);
 }
 
+// Level 1 of a grid's cycle straight from kd (k_coarsen_kd's faces: the f32 roundings of the
+// operator's f64 faces, summed as from stored records); the level-0 task of grid g is task g.
+__global__ __launch_bounds__(kBlock) void kb_coarsen_kd(const BGrid *__restrict__ t,
+                                                        const BLevel *__restrict__ lv0,
+                                                        const int2 *__restrict__ blk,
+                                                        float scale) {
+    const int2 m = blk[blockIdx.x];
+    const BLevel &q = lv0[m.x];
+    const BGrid &G = t[m.x];
+    const int k = m.y * kBlock + threadIdx.x;
+    if (k >= q.C.n) return;
+    const int w = q.L.w, h = q.L.h;
+    const double *kd = G.kd;
+    const double dt = G.dt, s2 = G.s2;
+    auto east = [&](int c) {
+        const int j = c / w, i = c - j * w;
+        if (i + 1 < w) return (float)face(kd[c], kd[c + 1], dt, s2);
+        return q.L.wrap ? (float)face(kd[c], kd[j * w], dt, s2) : 0.0f;
+    };
+    auto south = [&](int c) {
+        return c + w < w * h ? (float)face(kd[c], kd[c + w], dt, s2) : 0.0f;
+    };
+    coarsen_cell(q.L, q.C, scale, k, east, south);
+}
+
 // The caller's h = x.
 __global__ __launch_bounds__(kBlock) void kb_copy_out(const BGrid *__restrict__ t,
                                                       const int2 *__restrict__ blk) {
@@ -408,8 +433,9 @@ struct BatchWs {
     double *parts = nullptr, *pairs = nullptr;
     int *flag = nullptr;
     int2 *tables = nullptr;  // the set-up's block tables, one allocation
-    // the set-up's launches: the operator, the first guess, the coarsening per level
-    BLaunch cells_l, rows_l, rows4_l, cols_l, tiles_l;
+    // the set-up's launches: every cell (the answer's copy), the finest records of the grids of
+    // one level (the others' level 1 comes from kd), the first guess, the coarsening per level
+    BLaunch cells_l, oper_l, rows_l, rows4_l, cols_l, tiles_l;
     std::vector<BLaunch> coarsen_l;
     // the iterations' launches over every grid, and over the grids still running
     BTables all, part;
@@ -736,6 +762,8 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
     std::vector<int> cnt(n);
     for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].w * gs[g].h);
     add_launch(bw->cells_l, cnt);
+    for (size_t g = 0; g < n; g++) cnt[g] = bw->nlev[g] > 1 ? 0 : nblocks((long)gs[g].w * gs[g].h);
+    add_launch(bw->oper_l, cnt);
     // The first guess's rows: by the one-wave partition method (rows4_l, four rows a block), or
     // by the recurrences on the transposed grid (rows_l, tiles_l); then every grid's columns.
     for (size_t g = 0; g < n; g++)
@@ -1043,12 +1071,20 @@ static void solve_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs_in, doubl
     HIPCHK(hipMemcpyAsync(bw.d_grids, bw.grids.data(), n * sizeof(BGrid), hipMemcpyHostToDevice,
                           s));
     HIPCHK(hipEventRecord(ctx->ev[0], s));
-    kb_operator<<<bw.cells_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.cells_l.blk);
-    LAUNCH_CHECK();
+    // The finest records only for the grids of one level (their coarsest solve reads them; the
+    // others' finest smoothers form theirs from kd): the others' level 1 straight from kd.
+    if (bw.oper_l.nblk) {
+        kb_operator<<<bw.oper_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.oper_l.blk);
+        LAUNCH_CHECK();
+    }
     for (int l = 0; l + 1 < bw.nlevels; l++) {
         if (!bw.coarsen_l[l].nblk) continue;
-        kb_coarsen<<<bw.coarsen_l[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.coarsen_l[l].blk,
-                                                           ctx->opts.scale);
+        if (l == 0)
+            kb_coarsen_kd<<<bw.coarsen_l[0].nblk, kBlock, 0, s>>>(
+                bw.d_grids, bw.d_lv[0], bw.coarsen_l[0].blk, ctx->opts.scale);
+        else
+            kb_coarsen<<<bw.coarsen_l[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], bw.coarsen_l[l].blk,
+                                                               ctx->opts.scale);
         LAUNCH_CHECK();
     }
     HIPCHK(hipEventRecord(ctx->ev[1], s));
