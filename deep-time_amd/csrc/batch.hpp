@@ -142,12 +142,37 @@ __global__ __launch_bounds__(kLineBlock) void kb_lod_rows(const BGrid *__restric
                    Line<double>{q.p0 + j, h}, Line<double>{q.p1 + j, h}, q.g.w, q.wrap_x, q.dt,
                    q.s2);
 }
+// Whether a grid's columns go by the partition method in place (the others' by one thread's
+// recurrences a column): a function of the height alone.
+__host__ __device__ inline bool cols_by_segments(size_t h) { return h % kSeg == 0 && h >= 2 * kSeg; }
+
+// One thread a column: the recurrences down the column, or (cols_by_segments) the column's reduced
+// system, whose equations kb_col_reduce left in p0.
 __global__ __launch_bounds__(kLineBlock) void kb_lod_cols(const BGrid *__restrict__ t,
                                                           const int2 *__restrict__ blk) {
     const int2 m = blk[blockIdx.x];
     const BGrid &q = t[m.x];
     const int i = m.y * kLineBlock + threadIdx.x;
-    if (i < q.g.w) lod_col(q.g, q.kd, q.dt, q.s2, q.x, q.p0, i);
+    if (i >= q.g.w) return;
+    if (cols_by_segments(q.g.h))
+        col_thomas(q.g.w, q.g.h, q.p0, i);
+    else
+        lod_col(q.g, q.kd, q.dt, q.s2, q.x, q.p0, i);
+}
+
+// The column segments of the grids whose columns go by the partition method: a block takes 64
+// columns x 4 segments (mode 0: the reduced equations into p0; 1: the segments' values into x).
+__global__ __launch_bounds__(kBlock) void kb_col_seg(const BGrid *__restrict__ t,
+                                                     const int2 *__restrict__ blk, int mode) {
+    const int2 m = blk[blockIdx.x];
+    const BGrid &q = t[m.x];
+    const int w = q.g.w, h = q.g.h, ncb = (w + 63) / 64;
+    const int i = (m.y % ncb) * 64 + (threadIdx.x & 63), s = (m.y / ncb) * 4 + (threadIdx.x >> 6);
+    if (i >= w || s >= h / kSeg) return;
+    if (mode == 0)
+        col_reduce(w, h, q.kd, q.x, q.dt / q.s2, q.p0, i, s);
+    else
+        col_back(w, h, q.kd, q.x, q.dt / q.s2, q.p0, i, s);
 }
 
 // The row pass of the first guess by the partition method, one wave a row in place of the rows'
@@ -435,7 +460,7 @@ struct BatchWs {
     int2 *tables = nullptr;  // the set-up's block tables, one allocation
     // the set-up's launches: every cell (the answer's copy), the finest records of the grids of
     // one level (the others' level 1 comes from kd), the first guess, the coarsening per level
-    BLaunch cells_l, oper_l, rows_l, rows4_l, cols_l, tiles_l;
+    BLaunch cells_l, oper_l, rows_l, rows4_l, cols_l, colseg_l, tiles_l;
     std::vector<BLaunch> coarsen_l;
     // the iterations' launches over every grid, and over the grids still running
     BTables all, part;
@@ -775,6 +800,11 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
     for (size_t g = 0; g < n; g++) cnt[g] = nblocks((long)gs[g].w, kLineBlock);
     add_launch(bw->cols_l, cnt);
     for (size_t g = 0; g < n; g++)
+        cnt[g] = cols_by_segments(gs[g].h)
+                     ? (int)(((gs[g].w + 63) / 64) * ((gs[g].h / kSeg + 3) / 4))
+                     : 0;
+    add_launch(bw->colseg_l, cnt);
+    for (size_t g = 0; g < n; g++)
         cnt[g] = rows_by_wave(gs[g].w) ? 0
                                        : (int)(((gs[g].w + 31) / 32) * ((gs[g].h + 31) / 32));
     add_launch(bw->tiles_l, cnt);
@@ -1110,8 +1140,16 @@ static void solve_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs_in, doubl
             kb_transpose<<<bw.tiles_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.tiles_l.blk, 2);
             LAUNCH_CHECK();
         }
+        if (bw.colseg_l.nblk) {
+            kb_col_seg<<<bw.colseg_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.colseg_l.blk, 0);
+            LAUNCH_CHECK();
+        }
         kb_lod_cols<<<bw.cols_l.nblk, kLineBlock, 0, s>>>(bw.d_grids, bw.cols_l.blk);
         LAUNCH_CHECK();
+        if (bw.colseg_l.nblk) {
+            kb_col_seg<<<bw.colseg_l.nblk, kBlock, 0, s>>>(bw.d_grids, bw.colseg_l.blk, 1);
+            LAUNCH_CHECK();
+        }
     }
     HIPCHK(hipEventRecord(ctx->ev[2], s));
     batch_pass(ctx, bw, lod_guess ? 0 : 1, false);

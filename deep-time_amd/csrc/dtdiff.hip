@@ -647,8 +647,8 @@ static void lod(dtd_ctx *ctx, const double *b, double *x, const double *kd, doub
     hipStream_t s = ctx->stream;
     if (parallel_lod_enabled() && ctx->opts.sums != DTD_SUMS_STRIPS && parallel_line(ws.w) &&
         parallel_line(ws.h)) {
-        // Rows from b into x; then the columns as rows of the transposed grid (kd^T in p0, x^T
-        // in p1), transposed back into x.
+        // Rows from b into x; then the columns in place by the segments' reduced equations (in
+        // p0, n / 2 values).
         auto lines = [&](int nlines, int n, int ring, const double *k, const double *src,
                          double *dst) {
             if (n / kSeg <= 512)
@@ -658,13 +658,26 @@ static void lod(dtd_ctx *ctx, const double *b, double *x, const double *kd, doub
             LAUNCH_CHECK();
         };
         lines(ws.h, ws.w, wrap_x && ws.w > 2, kd, b, x);
-        const dim3 tg((ws.w + 31) / 32, (ws.h + 31) / 32), tgt((ws.h + 31) / 32, (ws.w + 31) / 32);
-        k_transpose<<<tg, 256, 0, s>>>(kd, ws.p0, ws.h, ws.w);
+        const dim3 cg((ws.w + 63) / 64, (ws.h / kSeg + 3) / 4);
+        k_col_reduce<<<cg, kBlock, 0, s>>>(ws.w, ws.h, kd, x, dt / s2, ws.p0);
         LAUNCH_CHECK();
-        k_transpose<<<tg, 256, 0, s>>>(x, ws.p1, ws.h, ws.w);
-        LAUNCH_CHECK();
-        lines(ws.w, ws.h, 0, ws.p0, ws.p1, ws.p1);
-        k_transpose<<<tgt, 256, 0, s>>>(ws.p1, x, ws.w, ws.h);
+        const int N = 2 * (ws.h / kSeg);
+        if (N % kSeg == 0 && N >= 2 * kSeg) {
+            // Long columns: the reduced systems by a second partition (its equations after red,
+            // at p0 + n / 2).
+            double *red2 = ws.p0 + ws.n / 2;
+            const dim3 gg((ws.w + 63) / 64, (N / kSeg + 3) / 4);
+            k_tri_group<<<gg, kBlock, 0, s>>>(ws.w, N, ws.p0, red2, 0);
+            LAUNCH_CHECK();
+            k_tri_thomas<<<(ws.w + 63) / 64, 64, 0, s>>>(ws.w, 2 * (N / kSeg), red2);
+            LAUNCH_CHECK();
+            k_tri_group<<<gg, kBlock, 0, s>>>(ws.w, N, ws.p0, red2, 1);
+            LAUNCH_CHECK();
+        } else {
+            k_col_thomas<<<(ws.w + 63) / 64, 64, 0, s>>>(ws.w, ws.h, ws.p0);
+            LAUNCH_CHECK();
+        }
+        k_col_back<<<cg, kBlock, 0, s>>>(ws.w, ws.h, kd, x, dt / s2, ws.p0);
         LAUNCH_CHECK();
         return;
     }

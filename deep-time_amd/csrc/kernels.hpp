@@ -2865,20 +2865,16 @@ struct SegEq {
 // segment's right-hand side.  Row q of the segment: lo = -f[q], up = -f[q + 1], di = 1 + f[q] +
 // f[q + 1].  The interior rows 1 .. 14 are eliminated against x_a and x_b (forward, then
 // backward; x_q = de[q] - al[q] x_a - ga[q] x_b), leaving eq.
-__device__ __forceinline__ void seg_reduce(const double (&kk)[kSeg + 2], bool has_w, bool has_e,
-                                           double dts2, const double *__restrict__ xs, SegEq &eq,
-                                           double (&al)[kSeg], double (&ga)[kSeg],
-                                           double (&de)[kSeg], double (&d)[kSeg]) {
+// seg_reduce with the right-hand side already in d.
+__device__ __forceinline__ void seg_reduce_d(const double (&kk)[kSeg + 2], bool has_w,
+                                             bool has_e, double dts2, SegEq &eq,
+                                             double (&al)[kSeg], double (&ga)[kSeg],
+                                             double (&de)[kSeg], const double (&d)[kSeg]) {
     double f[kSeg + 1];
 #pragma unroll
     for (int q = 0; q <= kSeg; q++) f[q] = 0.5 * (kk[q] + kk[q + 1]) * dts2;
     if (!has_w) f[0] = 0.0;
     if (!has_e) f[kSeg] = 0.0;
-#pragma unroll
-    for (int q = 0; q < kSeg; q += 2) {
-        const double2 v = *reinterpret_cast<const double2 *>(xs + q);
-        d[q] = v.x, d[q + 1] = v.y;
-    }
     {
         const double lo = -f[1], up = -f[2], ri = 1.0 / (1.0 + f[1] + f[2]);
         al[1] = lo * ri, ga[1] = up * ri, de[1] = d[1] * ri;
@@ -2907,6 +2903,18 @@ __device__ __forceinline__ void seg_reduce(const double (&kk)[kSeg + 2], bool ha
     eq.b[1] = di15 - lo15 * ga[kSeg - 2];
     eq.c[1] = up15;
     eq.r[1] = d[kSeg - 1] - lo15 * de[kSeg - 2];
+}
+
+__device__ __forceinline__ void seg_reduce(const double (&kk)[kSeg + 2], bool has_w, bool has_e,
+                                           double dts2, const double *__restrict__ xs, SegEq &eq,
+                                           double (&al)[kSeg], double (&ga)[kSeg],
+                                           double (&de)[kSeg], double (&d)[kSeg]) {
+#pragma unroll
+    for (int q = 0; q < kSeg; q += 2) {
+        const double2 v = *reinterpret_cast<const double2 *>(xs + q);
+        d[q] = v.x, d[q + 1] = v.y;
+    }
+    seg_reduce_d(kk, has_w, has_e, dts2, eq, al, ga, de, d);
 }
 
 // The segment's 16 values from its end unknowns, into out.
@@ -3117,6 +3125,232 @@ __device__ __forceinline__ void lod_line_wave(int n, int ring, bool on, const do
         }
     }
     if (mine) seg_back(x[0], x[1], al, ga, de, dst + a);
+}
+
+// ---- the first guess's column sweep by the partition method, in place --------------------
+//
+// The columns' systems (D:436-488) solved like the rows' (segments of 16 rows, each reduced to two
+// equations coupling its first and last unknowns), with no transposes: a wave takes 64 adjacent
+// columns of one segment, so every load and store of a row is one coalesced access.  Three passes:
+// col_reduce writes each segment's two equations (k = 2 s, 2 s + 1 of column i) to red, as four
+// N x w arrays a, b, c, r (N = 2 h / 16; entry k w + i); col_thomas solves each column's N
+// equations (a closed line: Thomas's recurrences, one thread a column), leaving the unknowns in
+// r; col_back repeats the segment's elimination (the same arithmetic, so the same coefficients)
+// and writes the segment's 16 values.  h is a multiple of 16 of at least 32.
+
+// Column i's segment s: kd of rows 16 s - 1 .. 16 s + 16 (zero beyond the ends) and x of its rows.
+__device__ __forceinline__ void col_seg_load(int w, int h, const double *__restrict__ kd,
+                                             const double *__restrict__ x, int i, int s,
+                                             double (&kk)[kSeg + 2], double (&d)[kSeg]) {
+    const size_t c0 = (size_t)s * kSeg * w + i;
+#pragma unroll
+    for (int q = 0; q < kSeg; q++) kk[1 + q] = gld(kd + c0 + (size_t)q * w);
+#pragma unroll
+    for (int q = 0; q < kSeg; q++) d[q] = gld(x + c0 + (size_t)q * w);
+    kk[0] = s > 0 ? gld(kd + c0 - w) : 0.0;
+    kk[kSeg + 1] = (s + 1) * kSeg < h ? gld(kd + c0 + (size_t)kSeg * w) : 0.0;
+}
+
+__device__ __forceinline__ void col_reduce(int w, int h, const double *__restrict__ kd,
+                                           const double *__restrict__ x, double dts2,
+                                           double *__restrict__ red, int i, int s) {
+    const int N = 2 * (h / kSeg);
+    const size_t Nw = (size_t)N * w;
+    double kk[kSeg + 2], d[kSeg], al[kSeg], ga[kSeg], de[kSeg];
+    col_seg_load(w, h, kd, x, i, s, kk, d);
+    SegEq eq;
+    seg_reduce_d(kk, s > 0, (s + 1) * kSeg < h, dts2, eq, al, ga, de, d);
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const size_t o = (size_t)(2 * s + e) * w + i;
+        gst(red + o, eq.a[e]);
+        gst(red + Nw + o, eq.b[e]);
+        gst(red + 2 * Nw + o, eq.c[e]);
+        gst(red + 3 * Nw + o, eq.r[e]);
+    }
+}
+
+// Thomas's recurrences on column i of N tridiagonal equations stored as four N x w arrays a, b,
+// c, r (entry k w + i; a closed line), the unknowns left in r.  The loads of 16 steps go out
+// together ahead of the recurrence's chain (a column's equations lie w apart: each load of a
+// step is a separate access).
+__device__ __forceinline__ void tri_thomas(int w, int N, double *__restrict__ red, int i) {
+    constexpr int U = 16;
+    const size_t Nw = (size_t)N * w;
+    const double *A = red, *B = red + Nw;
+    double *C = red + 2 * Nw, *R = red + 3 * Nw;
+    double cp = 0.0, rp = 0.0;
+    for (int k0 = 0; k0 < N; k0 += U) {
+        double a[U], b[U], c[U], r[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = k0 + u;
+            const size_t o = (size_t)(k < N ? k : N - 1) * w + i;
+            a[u] = k > 0 ? gld(A + o) : 0.0, b[u] = gld(B + o), c[u] = gld(C + o),
+            r[u] = gld(R + o);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = k0 + u;
+            if (k >= N) break;
+            const double inv = 1.0 / (b[u] - a[u] * cp);
+            cp = c[u] * inv;
+            rp = (r[u] - a[u] * rp) * inv;
+            const size_t o = (size_t)k * w + i;
+            gst(C + o, cp), gst(R + o, rp);
+        }
+    }
+    double xn = rp;
+    for (int k1 = N - 2; k1 >= 0; k1 -= U) {
+        double c[U], r[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = k1 - u;
+            const size_t o = (size_t)(k >= 0 ? k : 0) * w + i;
+            c[u] = gld(C + o), r[u] = gld(R + o);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = k1 - u;
+            if (k < 0) break;
+            xn = r[u] - c[u] * xn;
+            gst(R + (size_t)k * w + i, xn);
+        }
+    }
+}
+
+__device__ __forceinline__ void col_thomas(int w, int h, double *__restrict__ red, int i) {
+    tri_thomas(w, 2 * (h / kSeg), red, i);
+}
+
+// A second partition for long columns (N = 2 h / 16 a multiple of 16 of at least 32): the
+// reduced system's equations in groups of 16, each group's interior eliminated against its first
+// and last unknowns (the same recurrences as seg_reduce with general coefficients: row q reads
+// a[q] x[q - 1] + b[q] x[q] + c[q] x[q + 1] = r[q]), leaving 2 N / 16 equations (red2, the same
+// four-array layout), solved by tri_thomas; the groups' interiors follow.
+__device__ __forceinline__ void tri_group_load(int w, int N, const double *__restrict__ red, int i,
+                                               int grp, double (&a)[kSeg], double (&b)[kSeg],
+                                               double (&c)[kSeg], double (&r)[kSeg]) {
+    const size_t Nw = (size_t)N * w, o0 = (size_t)grp * kSeg * w + i;
+#pragma unroll
+    for (int q = 0; q < kSeg; q++) {
+        const size_t o = o0 + (size_t)q * w;
+        a[q] = gld(red + o), b[q] = gld(red + Nw + o), c[q] = gld(red + 2 * Nw + o),
+        r[q] = gld(red + 3 * Nw + o);
+    }
+}
+__device__ __forceinline__ void tri_group_reduce(const double (&a)[kSeg], const double (&b)[kSeg],
+                                                 const double (&c)[kSeg], const double (&r)[kSeg],
+                                                 SegEq &eq, double (&al)[kSeg],
+                                                 double (&ga)[kSeg], double (&de)[kSeg]) {
+    {
+        const double ri = 1.0 / b[1];
+        al[1] = a[1] * ri, ga[1] = c[1] * ri, de[1] = r[1] * ri;
+    }
+#pragma unroll
+    for (int q = 2; q <= kSeg - 2; q++) {
+        const double ri = 1.0 / (b[q] - a[q] * ga[q - 1]);
+        al[q] = -a[q] * al[q - 1] * ri;
+        ga[q] = c[q] * ri;
+        de[q] = (r[q] - a[q] * de[q - 1]) * ri;
+    }
+#pragma unroll
+    for (int q = kSeg - 3; q >= 1; q--) {
+        de[q] = de[q] - ga[q] * de[q + 1];
+        al[q] = al[q] - ga[q] * al[q + 1];
+        ga[q] = -ga[q] * ga[q + 1];
+    }
+    eq.a[0] = a[0];
+    eq.b[0] = b[0] - c[0] * al[1];
+    eq.c[0] = -c[0] * ga[1];
+    eq.r[0] = r[0] - c[0] * de[1];
+    eq.a[1] = -a[kSeg - 1] * al[kSeg - 2];
+    eq.b[1] = b[kSeg - 1] - a[kSeg - 1] * ga[kSeg - 2];
+    eq.c[1] = c[kSeg - 1];
+    eq.r[1] = r[kSeg - 1] - a[kSeg - 1] * de[kSeg - 2];
+}
+__device__ __forceinline__ void tri_group_fwd(int w, int N, const double *__restrict__ red,
+                                              double *__restrict__ red2, int i, int grp) {
+    double a[kSeg], b[kSeg], c[kSeg], r[kSeg], al[kSeg], ga[kSeg], de[kSeg];
+    tri_group_load(w, N, red, i, grp, a, b, c, r);
+    SegEq eq;
+    tri_group_reduce(a, b, c, r, eq, al, ga, de);
+    const size_t N2w = (size_t)(2 * (N / kSeg)) * w;
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+        const size_t o = (size_t)(2 * grp + e) * w + i;
+        gst(red2 + o, eq.a[e]);
+        gst(red2 + N2w + o, eq.b[e]);
+        gst(red2 + 2 * N2w + o, eq.c[e]);
+        gst(red2 + 3 * N2w + o, eq.r[e]);
+    }
+}
+__device__ __forceinline__ void tri_group_back(int w, int N, double *__restrict__ red,
+                                               const double *__restrict__ red2, int i, int grp) {
+    double a[kSeg], b[kSeg], c[kSeg], r[kSeg], al[kSeg], ga[kSeg], de[kSeg];
+    tri_group_load(w, N, red, i, grp, a, b, c, r);
+    SegEq eq;
+    tri_group_reduce(a, b, c, r, eq, al, ga, de);
+    const size_t N2w = (size_t)(2 * (N / kSeg)) * w;
+    const double *R2 = red2 + 3 * N2w;
+    const double xf = R2[(size_t)(2 * grp) * w + i], xl = R2[(size_t)(2 * grp + 1) * w + i];
+    double *R = red + 3 * (size_t)N * w;
+    const size_t o0 = (size_t)grp * kSeg * w + i;
+    gst(R + o0, xf);
+#pragma unroll
+    for (int q = 1; q <= kSeg - 2; q++) gst(R + o0 + (size_t)q * w, de[q] - al[q] * xf - ga[q] * xl);
+    gst(R + o0 + (size_t)(kSeg - 1) * w, xl);
+}
+
+__device__ __forceinline__ void col_back(int w, int h, const double *__restrict__ kd,
+                                         double *__restrict__ x, double dts2,
+                                         const double *__restrict__ red, int i, int s) {
+    const int N = 2 * (h / kSeg);
+    const double *R = red + 3 * (size_t)N * w;
+    double kk[kSeg + 2], d[kSeg], al[kSeg], ga[kSeg], de[kSeg];
+    col_seg_load(w, h, kd, x, i, s, kk, d);
+    SegEq eq;
+    seg_reduce_d(kk, s > 0, (s + 1) * kSeg < h, dts2, eq, al, ga, de, d);
+    const double xf = R[(size_t)(2 * s) * w + i], xl = R[(size_t)(2 * s + 1) * w + i];
+    const size_t c0 = (size_t)s * kSeg * w + i;
+    gst(x + c0, xf);
+#pragma unroll
+    for (int q = 1; q <= kSeg - 2; q++) gst(x + c0 + (size_t)q * w, de[q] - al[q] * xf - ga[q] * xl);
+    gst(x + c0 + (size_t)(kSeg - 1) * w, xl);
+}
+
+// The single grid's passes: blocks of 64 columns x 4 segments (grid ceil(w / 64) x ceil(P / 4)),
+// and one thread a column.
+__global__ __launch_bounds__(kBlock) void k_col_reduce(int w, int h, const double *__restrict__ kd,
+                                                       const double *__restrict__ x, double dts2,
+                                                       double *__restrict__ red) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), s = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i < w && s < h / kSeg) col_reduce(w, h, kd, x, dts2, red, i, s);
+}
+__global__ __launch_bounds__(64) void k_col_thomas(int w, int h, double *__restrict__ red) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i < w) col_thomas(w, h, red, i);
+}
+// The second partition's passes (mode 0: the groups' equations into red2; 1: the groups'
+// values into red's r), blocks of 64 columns x 4 groups; and Thomas on red2, one thread a column.
+__global__ __launch_bounds__(kBlock) void k_tri_group(int w, int N, double *__restrict__ red,
+                                                      double *__restrict__ red2, int mode) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), grp = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= w || grp >= N / kSeg) return;
+    if (mode == 0)
+        tri_group_fwd(w, N, red, red2, i, grp);
+    else
+        tri_group_back(w, N, red, red2, i, grp);
+}
+__global__ __launch_bounds__(64) void k_tri_thomas(int w, int N, double *__restrict__ red) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i < w) tri_thomas(w, N, red, i);
+}
+__global__ __launch_bounds__(kBlock) void k_col_back(int w, int h, const double *__restrict__ kd,
+                                                     double *__restrict__ x, double dts2,
+                                                     const double *__restrict__ red) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), s = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i < w && s < h / kSeg) col_back(w, h, kd, x, dts2, red, i, s);
 }
 
 // ---- sums in the reference's order (DTD_SUMS_STRIPS) ----------------------------------------

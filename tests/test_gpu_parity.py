@@ -80,8 +80,8 @@ def test_first_guess_bit_for_bit(ws, w, h, wrap):
                                           (128, 8192, True, "lognormal")])
 def test_parallel_first_guess_matches_the_recurrences(ws, w, h, wrap, gen):
     """The partition + cyclic-reduction line solver solves the same systems: equal to the
-    oracle's recurrences to 1e-12 of the field's largest value (lines of 64 to 8192 cells, rows
-    and the transposed grid's columns)."""
+    oracle's recurrences to 1e-12 of the field's largest value (lines of 64 to 8192 cells: rows
+    by cyclic reduction in LDS, columns in place by their segments' reduced systems)."""
     print(f"parallel lod {w}x{h} wrap={wrap} {gen}", flush=True)
     assert parallel_lines(w, h)
     if gen == "shelf":
