@@ -505,8 +505,8 @@ def batch_line(timed, with_cpu):
     h0_all = DeviceArray.from_numpy(np.concatenate([g[1] for g in grids]))
     kd_all = DeviceArray.from_numpy(np.concatenate([g[2] for g in grids]))
     h_all = DeviceArray(total)
-    items = [(d, h_all.data_ptr() + 8 * int(o), kd_all.data_ptr() + 8 * int(o), dt, 0.0)
-             for (d, _, _, dt), o in zip(grids, offs[:-1])]
+    items = dtdiff.BatchGrids([(d, h_all.data_ptr() + 8 * int(o), kd_all.data_ptr() + 8 * int(o),
+                                dt, 0.0) for (d, _, _, dt), o in zip(grids, offs[:-1])])
     ws = dtdiff.Workspace(device=0)
     try:
         def step():
@@ -936,8 +936,8 @@ def bench_batch(args, gen, kw):
     kd_all = DeviceArray.from_numpy(np.concatenate([g[2] for g in grids]))
     h_all = DeviceArray(total)
     synchronize()
-    items = [(d, h_all.data_ptr() + 8 * int(o), kd_all.data_ptr() + 8 * int(o), dt, 0.0)
-             for (d, _, _, dt), o in zip(grids, offs[:-1])]
+    items = dtdiff.BatchGrids([(d, h_all.data_ptr() + 8 * int(o), kd_all.data_ptr() + 8 * int(o),
+                                dt, 0.0) for (d, _, _, dt), o in zip(grids, offs[:-1])])
     ws = dtdiff.Workspace(device=0)
 
     def step():

@@ -447,15 +447,26 @@ def step_device(ws: Workspace, d: Dims, h_dev, kd_dev, dt: float) -> Stats:
     return st
 
 
+class BatchGrids:
+    """A batch's dtd_grid records built once, for callers that solve the same grids (the same
+    device arrays) again and again: filling 512 records from Python costs about a millisecond,
+    which a C caller's loop would not."""
+
+    def __init__(self, items):
+        self.n = len(items)
+        self.arr = (Grid * max(self.n, 1))()
+        for k, (d, h_dev, kd_dev, dt, tol_m) in enumerate(items):
+            self.arr[k] = Grid(d.w, d.h, d.dx, int(d.wrap_x), dt, tol_m, _ptr(h_dev).value,
+                               _ptr(kd_dev).value)
+
+
 def solve_batch(ws: Workspace, items, rel_tol: float = 0.0):
     """Many grids at once (dtd_solve_batch).  items: (Dims, h_dev, kd_dev, dt, tol_m) per grid,
-    device arrays solved in place.  rel_tol > 0 takes every grid to that true relative residual
-    (tol_m ignored).  Returns (iterations per grid, relative residual per grid, BatchStats)."""
-    n = len(items)
-    arr = (Grid * max(n, 1))()
-    for k, (d, h_dev, kd_dev, dt, tol_m) in enumerate(items):
-        arr[k] = Grid(d.w, d.h, d.dx, int(d.wrap_x), dt, tol_m, _ptr(h_dev).value,
-                      _ptr(kd_dev).value)
+    device arrays solved in place, or a BatchGrids built from such a list.  rel_tol > 0 takes
+    every grid to that true relative residual (tol_m ignored).  Returns (iterations per grid,
+    relative residual per grid, BatchStats)."""
+    grids = items if isinstance(items, BatchGrids) else BatchGrids(items)
+    n, arr = grids.n, grids.arr
     its = (ctypes.c_size_t * max(n, 1))()
     rels = (ctypes.c_double * max(n, 1))()
     st = BatchStats()
