@@ -429,6 +429,9 @@ struct BLaunch {  // one launch: its block table (device) and block count
     int nblk = 0;
 };
 
+// The band height of every streamed launch in a pass that solves a few grids again.
+constexpr int kRestartTh = 4;
+
 // The launch tables of the iterations (and of a pass's first residual, first cycle and finish)
 // over one set of grids: every grid of the batch, or the ones still running (the others' blocks
 // left out, so that a launch dispatches only blocks with work).
@@ -441,6 +444,9 @@ struct BTables {
     const int4 *bottom_tab = nullptr;
     int nbottom = 0;
     int ngrids = 0;  // grids with blocks in the tables
+    // the grid and level tables the launches read (their band heights made the block tables)
+    const BGrid *grids = nullptr;
+    std::vector<const BLevel *> lv;
 };
 
 struct BatchWs {
@@ -476,6 +482,12 @@ struct BatchWs {
     // the bottoms: every grid's levels from its bottom on, the LDS bytes
     const Lvl<float> *bottom_lv = nullptr;
     size_t bottom_lds = 0;
+    // The same grid and level tables with every band height kRestartTh, for the pass that solves
+    // a few grids again: their launches then have kRestartTh-row bands, many more blocks a grid,
+    // which shortens each launch's longest walk (such a pass is latency-bound).
+    BGrid *d_grids_r = nullptr;
+    std::vector<BLevel *> d_lv_r;
+    std::vector<BGrid> grids_r;  // host copy, refreshed before each such pass
     std::vector<void *> owned;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     // pinned host
@@ -524,8 +536,12 @@ static inline size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; 
 // and the bottoms' table into bot (at dbot).  A grid left out has no blocks and empty ranges.
 static void iteration_tables(const BatchWs &bw, const int *active, BTables &T,
                              std::vector<int2> &tab, int2 *dtab, std::vector<int4> &bot,
-                             int4 *dbot) {
+                             int4 *dbot, bool restart_th = false) {
     const int n = bw.ngrids, L = bw.nlevels;
+    T.grids = restart_th ? bw.d_grids_r : bw.d_grids;
+    T.lv.assign(bw.d_lv.begin(), bw.d_lv.end());
+    if (restart_th) T.lv.assign(bw.d_lv_r.begin(), bw.d_lv_r.end());
+    auto gth = [&](int g) { return restart_th ? kRestartTh : bw.grids[g].th; };
     tab.clear();
     bot.clear();
     auto on = [&](int g) { return !active || active[g] != 0; };
@@ -548,7 +564,7 @@ static void iteration_tables(const BatchWs &bw, const int *active, BTables &T,
     std::vector<int2> rng_update, rng_band, rng_rz(n, make_int2(0, 0)), rng_rr(n, make_int2(0, 0));
     add_launch(T.update_l, cnt, &rng_update);
     for (int g = 0; g < n; g++)
-        cnt[g] = on(g) ? stream_grid<1, 1>(bw.grids[g].g.w, bw.grids[g].g.h, bw.grids[g].th) : 0;
+        cnt[g] = on(g) ? stream_grid<1, 1>(bw.grids[g].g.w, bw.grids[g].g.h, gth(g)) : 0;
     add_launch(T.band_l, cnt, &rng_band);
     // The smoothing launches of the grids for which level l is neither the coarsest nor cycled
     // by the grid's bottom, split by lane width; the coarsest launches.
@@ -562,16 +578,18 @@ static void iteration_tables(const BatchWs &bw, const int *active, BTables &T,
             const BLevel &q = bw.lv[l][t];
             const int g = bw.lv_grid[l][t];
             if (!on(g)) continue;
+            const int th = restart_th ? kRestartTh : q.th;
+            const int th_post = restart_th ? kRestartTh : q.th_post;
             const bool last = l + 1 == bw.nlev[g];
             if (!last) {
                 if (l >= bw.b0[g]) continue;  // the grid's bottom cycles this level
                 // c: the pre-smoothing's blocks, q: the post-smoothing's.
                 if (q.L.w % 2 == 0)
-                    c2[t] = stream_grid<2, 1>(q.L.w, q.L.h, q.th),
-                    q2[t] = stream_grid<2, 1>(q.L.w, q.L.h, q.th_post);
+                    c2[t] = stream_grid<2, 1>(q.L.w, q.L.h, th),
+                    q2[t] = stream_grid<2, 1>(q.L.w, q.L.h, th_post);
                 else
-                    c1[t] = stream_grid<1, 2>(q.L.w, q.L.h, q.th),
-                    q1[t] = stream_grid<1, 2>(q.L.w, q.L.h, q.th_post);
+                    c1[t] = stream_grid<1, 2>(q.L.w, q.L.h, th),
+                    q1[t] = stream_grid<1, 2>(q.L.w, q.L.h, th_post);
             } else if (l < bw.b0[g]) {
                 if (q.L.n > 64) throw Error{DTD_ESTATE, "the coarsest level exceeds one wave"};
                 cc[t] = 1;
@@ -714,6 +732,7 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
         q.st = bw->st + g;
     }
     bw->d_grids = (BGrid *)own(n * sizeof(BGrid));
+    bw->d_grids_r = (BGrid *)own(n * sizeof(BGrid));
     // Level tables: every grid that has level l, in grid order (task index = position).
     std::vector<std::vector<int>> lv_grid(L);  // grid of each task of level l
     std::vector<std::vector<BLevel>> lv(L);
@@ -738,6 +757,11 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
         BLevel *d = (BLevel *)own(lv[l].size() * sizeof(BLevel));
         HIPCHK(hipMemcpy(d, lv[l].data(), lv[l].size() * sizeof(BLevel), hipMemcpyHostToDevice));
         bw->d_lv.push_back(d);
+        std::vector<BLevel> lr = lv[l];
+        for (BLevel &q : lr) q.th = q.th_post = kRestartTh;
+        BLevel *dr = (BLevel *)own(lr.size() * sizeof(BLevel));
+        HIPCHK(hipMemcpy(dr, lr.data(), lr.size() * sizeof(BLevel), hipMemcpyHostToDevice));
+        bw->d_lv_r.push_back(dr);
     }
     // Each grid's bottom: the first level from 1 on that one block cycles in LDS, as the single
     // grid's cycle chooses it (at most bottom_cells() cells; that level and the coarser ones'
@@ -864,20 +888,20 @@ static void batch_vcycle(const BatchWs &bw, const BTables &T, hipStream_t s, int
         if (l == 0) {
             if (T.pre2[0].nblk) {
                 if (parity < 0)
-                    kb_spre0<2, kR64K><<<T.pre2[0].nblk, kBlock, 0, s>>>(bw.d_lv[0], T.pre2[0].blk,
+                    kb_spre0<2, kR64K><<<T.pre2[0].nblk, kBlock, 0, s>>>(T.lv[0], T.pre2[0].blk,
                                                                        par, bw.parts);
                 else
                     kb_spre0<2, kRUpdateK><<<T.pre2[0].nblk, kBlock, 0, s>>>(
-                        bw.d_lv[0], T.pre2[0].blk, par, bw.parts);
+                        T.lv[0], T.pre2[0].blk, par, bw.parts);
                 LAUNCH_CHECK();
             }
             if (T.pre1[0].nblk) {
                 if (parity < 0)
                     kb_spre0<1, kR64K><<<T.pre1[0].nblk, kBlock, 0, s>>>(
-                        bw.d_lv[0], T.pre1[0].blk, par, bw.parts + T.rr_base_pre1);
+                        T.lv[0], T.pre1[0].blk, par, bw.parts + T.rr_base_pre1);
                 else
                     kb_spre0<1, kRUpdateK><<<T.pre1[0].nblk, kBlock, 0, s>>>(
-                        bw.d_lv[0], T.pre1[0].blk, par, bw.parts + T.rr_base_pre1);
+                        T.lv[0], T.pre1[0].blk, par, bw.parts + T.rr_base_pre1);
                 LAUNCH_CHECK();
             }
             if (parity >= 0) {
@@ -886,17 +910,17 @@ static void batch_vcycle(const BatchWs &bw, const BTables &T, hipStream_t s, int
             }
         } else {
             if (T.pre2[l].nblk) {
-                kb_spre<2><<<T.pre2[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], T.pre2[l].blk);
+                kb_spre<2><<<T.pre2[l].nblk, kBlock, 0, s>>>(T.lv[l], T.pre2[l].blk);
                 LAUNCH_CHECK();
             }
             if (T.pre1[l].nblk) {
-                kb_spre<1><<<T.pre1[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], T.pre1[l].blk);
+                kb_spre<1><<<T.pre1[l].nblk, kBlock, 0, s>>>(T.lv[l], T.pre1[l].blk);
                 LAUNCH_CHECK();
             }
         }
         if (T.coarsest[l].nblk) {
             kb_coarsest<<<T.coarsest[l].nblk, 64, 0, s>>>(
-                bw.d_lv[l], T.coarsest[l].blk, bw.parts + (l == 0 ? T.rz_base_coarsest : 0));
+                T.lv[l], T.coarsest[l].blk, bw.parts + (l == 0 ? T.rz_base_coarsest : 0));
             LAUNCH_CHECK();
         }
     }
@@ -911,19 +935,19 @@ static void batch_vcycle(const BatchWs &bw, const BTables &T, hipStream_t s, int
     for (int l = L - 2; l >= 0; l--) {
         if (T.post2[l].nblk) {
             if (l == 0)
-                kb_spost0<2><<<T.post2[0].nblk, kBlock, 0, s>>>(bw.d_lv[0], T.post2[0].blk, cur,
+                kb_spost0<2><<<T.post2[0].nblk, kBlock, 0, s>>>(T.lv[0], T.post2[0].blk, cur,
                                                                  bw.parts);
             else
-                kb_spost<2><<<T.post2[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], T.post2[l].blk,
+                kb_spost<2><<<T.post2[l].nblk, kBlock, 0, s>>>(T.lv[l], T.post2[l].blk,
                                                                 bw.parts);
             LAUNCH_CHECK();
         }
         if (T.post1[l].nblk) {
             if (l == 0)
-                kb_spost0<1><<<T.post1[0].nblk, kBlock, 0, s>>>(bw.d_lv[0], T.post1[0].blk, cur,
+                kb_spost0<1><<<T.post1[0].nblk, kBlock, 0, s>>>(T.lv[0], T.post1[0].blk, cur,
                                                                  bw.parts + T.rz_base_post1);
             else
-                kb_spost<1><<<T.post1[l].nblk, kBlock, 0, s>>>(bw.d_lv[l], T.post1[l].blk,
+                kb_spost<1><<<T.post1[l].nblk, kBlock, 0, s>>>(T.lv[l], T.post1[l].blk,
                                                                 bw.parts + T.rz_base_post1);
             LAUNCH_CHECK();
         }
@@ -935,12 +959,12 @@ static void batch_vcycle(const BatchWs &bw, const BTables &T, hipStream_t s, int
 static void batch_iteration(const BatchWs &bw, const BTables &T, int parity, hipStream_t s,
                             int *host_flag, int *mask) {
     const int n = bw.ngrids;
-    kb_sdir<<<T.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, T.band_l.blk, parity, bw.parts);
+    kb_sdir<<<T.band_l.nblk, kBlock, 0, s>>>(T.grids, T.band_l.blk, parity, bw.parts);
     LAUNCH_CHECK();
     kb_step<kStepPAP><<<n, kReduceBlock, 0, s>>>(T.rng_band, bw.st, bw.parts);
     LAUNCH_CHECK();
     if (T.update_l.nblk) {
-        kb_update<<<T.update_l.nblk, kBlock, 0, s>>>(bw.d_grids, T.update_l.blk,
+        kb_update<<<T.update_l.nblk, kBlock, 0, s>>>(T.grids, T.update_l.blk,
                                                       bw.parts + T.rr_base_update);
         LAUNCH_CHECK();
     }
@@ -974,9 +998,14 @@ static void batch_graphs(dtd_ctx *ctx, BatchWs &bw) {
 // The iterations' tables over the grids with active[g] into bw.part, its device copy queued on
 // the stream.  The part tables must not be in use by queued work (they are rebuilt at most once
 // a pass, before the pass launches anything with them).
-static void part_tables(dtd_ctx *ctx, BatchWs &bw, const int *active) {
+// restart_th: with every band kRestartTh rows (the grid and level tables of that height), unless
+// the tables would not fit the room; then with the batch's own heights.
+static void part_tables(dtd_ctx *ctx, BatchWs &bw, const int *active, bool restart_th = false) {
     iteration_tables(bw, active, bw.part, bw.h_part_tab, bw.part_tab, bw.h_part_bot,
-                     bw.part_bottom);
+                     bw.part_bottom, restart_th);
+    if (restart_th && bw.h_part_tab.size() > bw.tab_entries)
+        iteration_tables(bw, active, bw.part, bw.h_part_tab, bw.part_tab, bw.h_part_bot,
+                         bw.part_bottom, false);
     if (bw.h_part_tab.size() > bw.tab_entries)
         throw Error{DTD_ESTATE, "the running grids' tables exceed the batch's"};
     HIPCHK(hipMemcpyAsync(bw.part_tab, bw.h_part_tab.data(), bw.h_part_tab.size() * sizeof(int2),
@@ -1000,7 +1029,7 @@ static void batch_pass(dtd_ctx *ctx, BatchWs &bw, int from_b, bool part) {
     HIPCHK(hipMemcpyAsync(bw.init, bw.h_init, n * sizeof(BInit), hipMemcpyHostToDevice, s));
     kb_init<<<nblocks(n), kBlock, 0, s>>>(bw.st, bw.init, bw.d_grids, n, (int)ctx->opts.max_iter);
     LAUNCH_CHECK();
-    kb_sresid<<<T->band_l.nblk, kBlock, 0, s>>>(bw.d_grids, T->band_l.blk, from_b, bw.parts);
+    kb_sresid<<<T->band_l.nblk, kBlock, 0, s>>>(T->grids, T->band_l.blk, from_b, bw.parts);
     LAUNCH_CHECK();
     kb_step<kStepRR0><<<n, kReduceBlock, 0, s>>>(T->rng_band, bw.st, bw.parts);
     LAUNCH_CHECK();
@@ -1037,11 +1066,11 @@ static void batch_pass(dtd_ctx *ctx, BatchWs &bw, int from_b, bool part) {
     }
     // The last update, the put-back and the true residual of every grid the pass ran.
     const BTables &F = part ? bw.part : bw.all;
-    kb_finish1<<<F.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, F.band_l.blk, bw.parts);
+    kb_finish1<<<F.band_l.nblk, kBlock, 0, s>>>(F.grids, F.band_l.blk, bw.parts);
     LAUNCH_CHECK();
     kb_putback<<<n, kReduceBlock, 0, s>>>(F.rng_band, bw.st, bw.parts);
     LAUNCH_CHECK();
-    kb_finish2<<<F.band_l.nblk, kBlock, 0, s>>>(bw.d_grids, F.band_l.blk, bw.parts);
+    kb_finish2<<<F.band_l.nblk, kBlock, 0, s>>>(F.grids, F.band_l.blk, bw.parts);
     LAUNCH_CHECK();
     kb_pair<<<n, kReduceBlock, 0, s>>>(F.rng_band, bw.st, bw.parts, bw.pairs);
     LAUNCH_CHECK();
@@ -1187,8 +1216,12 @@ static void solve_batch(dtd_ctx *ctx, size_t count, const dtd_grid *gs_in, doubl
             again_of[g] = again;
         }
         if (!any) break;
-        // Only the grids solved again have blocks in the pass's launches.
-        part_tables(ctx, bw, again_of.data());
+        // Only the grids solved again have blocks in the pass's launches, with short bands.
+        bw.grids_r = bw.grids;
+        for (BGrid &q : bw.grids_r) q.th = kRestartTh;
+        HIPCHK(hipMemcpyAsync(bw.d_grids_r, bw.grids_r.data(), n * sizeof(BGrid),
+                              hipMemcpyHostToDevice, s));
+        part_tables(ctx, bw, again_of.data(), true);
         batch_pass(ctx, bw, 0, true);
         collect(false);
     }
