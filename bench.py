@@ -54,6 +54,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="lognormal-8192x4096",
                     help="constant-512x256 | lognormal-8192x4096 | constant-16384x8192 | "
+                         "belt-8192x4096 (the Jacobi step's mostly empty grid) | "
                          "regional-512 (many grids, one batched solve)")
     ap.add_argument("--size", default=None, help="WxH: the same workload at another size")
     ap.add_argument("--solver", choices=["multigrid", "jacobi"], default="multigrid",
@@ -574,7 +575,8 @@ def bench_jacobi(args, gen, kw):
     synchronize()
     elapsed = time.perf_counter() - t0
     log(f"timed {args.steps} steps: {elapsed * 1e3 / args.steps:.3f} ms/step, {st.iterations} "
-        f"its; setup {st.ms_setup:.2f} guess {st.ms_guess:.2f} cg {st.ms_cg:.2f} ms")
+        f"its, {st.rows_visited} of {h} rows visited; setup {st.ms_setup:.2f} guess "
+        f"{st.ms_guess:.2f} cg {st.ms_cg:.2f} ms")
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -606,6 +608,7 @@ def bench_jacobi(args, gen, kw):
         "roofline": None,
         "cpu_baseline": cpu,
         "iterations": int(st.iterations),
+        "rows_visited": int(st.rows_visited),
         "ms_breakdown": {"setup": st.ms_setup, "guess": st.ms_guess, "cg": st.ms_cg,
                          "device_total": st.ms_total},
     }

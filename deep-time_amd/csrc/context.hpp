@@ -105,6 +105,11 @@ struct Workspace {
     // The Jacobi solve's extra buffers (diag, two row sums a row, its scalars) and graphs.
     double *jdiag = nullptr, *jrows = nullptr;
     void *jst = nullptr;
+    // The rows' live strips of the Jacobi loop (one byte a strip of 8 rows, and one a row for the
+    // zero-row test) and its flags (any zero row, a zero row's elevation not normal, the visited
+    // strips, a visited value not finite).
+    unsigned char *jzero = nullptr, *jlive = nullptr;
+    int *jflags = nullptr, *jlist = nullptr;  // jlist: the visited strips in order
     hipGraphExec_t jgraph[2] = {nullptr, nullptr};
 };
 
@@ -165,6 +170,7 @@ struct dtd_ctx {
     // The finest level's fused smoothers form their records (and the pre-smoothing A p) from kd
     // and p instead of reading them (DTD_NO_KD=1 reads them, for comparison).
     bool use_kd = true;
+    bool use_live = true;  // the Jacobi loop skips the rows `Live` skips (DTD_NO_LIVE=1: it visits all)
     dtd::FaultHook fault;         // dtd_inject_fault
     std::map<dtd::PlanKey, dtd::Plan> plans;  // tuned plans by grid shape (dtd_tune)
     // Tuned band-height policies of the batched solve by shape list (dtd_tune_batch), and the one

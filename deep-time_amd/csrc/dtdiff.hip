@@ -40,6 +40,10 @@ void free_ws(Workspace &ws) {
     if (ws.jdiag) (void)hipFree(ws.jdiag);
     if (ws.jrows) (void)hipFree(ws.jrows);
     if (ws.jst) (void)hipFree(ws.jst);
+    if (ws.jzero) (void)hipFree(ws.jzero);
+    if (ws.jlive) (void)hipFree(ws.jlive);
+    if (ws.jflags) (void)hipFree(ws.jflags);
+    if (ws.jlist) (void)hipFree(ws.jlist);
     for (void *p : ws.owned) (void)hipFree(p);
     ws = Workspace();
 }
@@ -1187,6 +1191,8 @@ int dtd_create(const dtd_opts *opts, dtd_ctx **out) {
     ctx->use_graph = !(ng && *ng && *ng != '0');
     const char *nk = getenv("DTD_NO_KD");
     ctx->use_kd = !(nk && *nk && *nk != '0');
+    const char *nl = getenv("DTD_NO_LIVE");
+    ctx->use_live = !(nl && *nl && *nl != '0');
     // The reference's switch for its audit (D:127-130): set at all, it is on.
     ctx->audit_on = getenv("DEEPTIME_DIFFUSION_AUDIT") != nullptr;
     int rc = guard(ctx, [&] {

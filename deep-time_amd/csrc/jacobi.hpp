@@ -17,7 +17,11 @@
 // Rows with nothing to diffuse (`Live`, D:285-367) are skipped by the reference only when doing so
 // gives the bits of visiting them (their faces are +0 and their elevations normal, so the passes
 // carry +0 through them and leave them as they are), and when a visited value ends up not finite
-// it solves again on every row.  The device visits every row, which is that same result.
+// it solves again on every row.  The device's loop skips the same strips of 8 rows (the first
+// guess, the first residual and the passes after the loop visit every row, which leaves the
+// skipped rows as they are): a skipped strip's row sums are the +0 that visiting it gives, its
+// directions stay +0, its x and r are not touched.  A visited value that is not finite makes it
+// solve again on every row, as the reference does.
 #pragma once
 
 #include "context.hpp"
@@ -26,6 +30,7 @@
 namespace dtd {
 
 constexpr int kRowTile = 8;     // rows per block of the row sums (16 and 4 measured slower)
+static_assert(kRowTile == 8, "a block of the row sums is one strip of `Live` (STRIP_ROWS, D:502)");
 constexpr int kColTile = 256;   // columns per tile (one per thread of the block)
 constexpr double kCgTolM = 1.0e-6;  // the Jacobi solve's RMS tolerance, metres (D:495)
 
@@ -33,6 +38,14 @@ constexpr double kCgTolM = 1.0e-6;  // the Jacobi solve's RMS tolerance, metres 
 struct JState {
     double rz, alpha, beta, tol2, lost, weight;
     int have_beta, pending, done, its, max_iter, pad;
+};
+
+// The strips of kRowTile rows the loop visits (`Live`, D:285-367): their indices in ascending
+// order (list) and how many (*count), of ns strips of an h-row grid.
+struct JLive {
+    const int *list;
+    const int *count;
+    int ns, h;
 };
 
 // diag (D:585-600), stored once a solve: every pass divides by it.
@@ -71,12 +84,14 @@ __global__ __launch_bounds__(kColTile) void kj_rows(int w, int h, const double *
                                                     const double *__restrict__ c, double *rout,
                                                     double *__restrict__ s0,
                                                     double *__restrict__ s1,
-                                                    const JState *__restrict__ st) {
+                                                    const JState *__restrict__ st,
+                                                    JLive lv) {
     __shared__ double t0[kRowTile][kColTile + 1];
     __shared__ double t1[kRowTile][kColTile + 1];
-    if (st->done) return;
+    if (st->done || (int)blockIdx.x >= *lv.count) return;
     const double alpha = MODE == kRowsRZU ? st->alpha : 0.0;
-    const int j0 = blockIdx.x * kRowTile;
+    // Block b takes the b-th visited strip (a skipped strip's r stays +0).
+    const int j0 = lv.list[blockIdx.x] * kRowTile;
     const int tid = threadIdx.x;
     // The next tile's values are loaded into registers while the current tile is being added.
     double qa[kRowTile], qb[kRowTile], qc[MODE == kRowsRZU ? kRowTile : 1];
@@ -175,10 +190,15 @@ __global__ __launch_bounds__(kColTile) void kj_apply_rows(Grid g, const double2 
                                                           const double *__restrict__ p,
                                                           double *__restrict__ ap,
                                                           double *__restrict__ s0,
-                                                          const JState *__restrict__ st) {
+                                                          const JState *__restrict__ st,
+                                                          JLive lv) {
+    static_assert(kRowTile % RT == 0, "a block's rows lie in one strip");
+    constexpr int per = kRowTile / RT;
     __shared__ double t0[RT][kColTile + 1];
-    if (st->done) return;
-    const int j0 = blockIdx.x * RT, tid = threadIdx.x;
+    if (st->done || (int)blockIdx.x >= *lv.count * per) return;
+    // Blocks per + k take the b-th visited strip's rows (a skipped strip's p and A p are +0).
+    const int j0 = lv.list[blockIdx.x / per] * kRowTile + (blockIdx.x % per) * RT,
+              tid = threadIdx.x;
     // The next tile's operands are gathered into registers while the current tile is being added.
     ApplyIn q[RT];
     auto gather = [&](int i0) {
@@ -221,16 +241,22 @@ enum { kJInit = 0, kJPap = 1, kJUpdate = 2 };
 // from -0.0.
 constexpr int kStepBlock = 1024, kStepChunk = 4096;
 
+// Only the visited strips' rows are added: a skipped row's sum is +0, and adding +0 to a running
+// sum changes nothing but a -0 (all terms so far -0) into +0, so the rows in order give the
+// visited rows' total in order, except that a -0 total becomes +0 when a row was skipped.  A row
+// past h in the last strip reads -0, which adds nothing.
 template <int STEP>
 __device__ inline void kj_step_body(const double *__restrict__ s0, const double *__restrict__ s1,
-                                    int h, JState *st, double *l0, double *l1) {
+                                    int h, JState *st, double *l0, double *l1, JLive lv) {
     if (st->done) return;
     double v0 = -0.0, v1 = -0.0;
-    for (int c0 = 0; c0 < h; c0 += kStepChunk) {
-        const int cnt = min(kStepChunk, h - c0);
+    const int nl = *lv.count, rows = nl * kRowTile;
+    for (int c0 = 0; c0 < rows; c0 += kStepChunk) {
+        const int cnt = min(kStepChunk, rows - c0);
         for (int k = threadIdx.x; k < cnt; k += kStepBlock) {
-            l0[k] = s0[c0 + k];
-            if (STEP != kJPap) l1[k] = s1[c0 + k];
+            const int q = c0 + k, j = lv.list[q / kRowTile] * kRowTile + q % kRowTile;
+            l0[k] = j < h ? s0[j] : -0.0;
+            if (STEP != kJPap) l1[k] = j < h ? s1[j] : -0.0;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -243,6 +269,7 @@ __device__ inline void kj_step_body(const double *__restrict__ s0, const double 
         __syncthreads();
     }
     if (threadIdx.x != 0) return;
+    if (nl < lv.ns) v0 += 0.0, v1 += 0.0;  // the skipped rows' +0
     if (STEP == kJInit) {
         // rr0 > tol2 starts the loop with rz = sum r z (D:947-953).
         if (v0 > st->tol2)
@@ -277,9 +304,9 @@ __device__ inline void kj_step_body(const double *__restrict__ s0, const double 
 template <int STEP>
 __global__ __launch_bounds__(kStepBlock) void kj_step(const double *__restrict__ s0,
                                                       const double *__restrict__ s1, int h,
-                                                      JState *st, int *flag = nullptr) {
+                                                      JState *st, JLive lv, int *flag = nullptr) {
     __shared__ double l0[kStepChunk], l1[STEP == kJPap ? 1 : kStepChunk];
-    kj_step_body<STEP>(s0, s1, h, st, l0, l1);
+    kj_step_body<STEP>(s0, s1, h, st, l0, l1, lv);
     if (flag && threadIdx.x == 0) *reinterpret_cast<volatile int *>(flag) = st->done;
 }
 
@@ -290,13 +317,22 @@ __global__ __launch_bounds__(kBlock) void kj_dir(int n, const double *__restrict
                                                  const double *__restrict__ pold,
                                                  double *__restrict__ pnew,
                                                  double *__restrict__ x,
-                                                 const JState *__restrict__ st) {
+                                                 const JState *__restrict__ st, JLive lv,
+                                                 int strip_cells) {
     if (st->done) return;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= n) return;
-    const double po = (st->have_beta || st->pending) ? pold[c] : 0.0;
-    if (st->pending) x[c] += st->alpha * po;
-    pnew[c] = st->have_beta ? r[c] / diag[c] + st->beta * po : r[c] / diag[c];
+    // Chunks of kBlock cells of the visited strips, in turn over the grid's blocks.
+    const int cps = (strip_cells + kBlock - 1) / kBlock;
+    const int chunks = *lv.count * cps;
+    const bool have_beta = st->have_beta, pending = st->pending;
+    const double alpha = st->alpha, beta = st->beta;
+    for (int q = blockIdx.x; q < chunks; q += gridDim.x) {
+        const int li = q / cps, off = (q - li * cps) * kBlock + (int)threadIdx.x;
+        const int c = lv.list[li] * strip_cells + off;
+        if (off >= strip_cells || c >= n) continue;
+        const double po = (have_beta || pending) ? pold[c] : 0.0;
+        if (pending) x[c] += alpha * po;
+        pnew[c] = have_beta ? r[c] / diag[c] + beta * po : r[c] / diag[c];
+    }
 }
 
 // A p (`Operator::apply_row`, D:609-649).
@@ -380,12 +416,109 @@ __global__ __launch_bounds__(kLineBlock) void kj_lod_rows_t(Grid g, int wrap_x, 
 
 namespace dtd {
 
+// ---- `Live` (D:285-367): the strips of 8 rows the loop visits -------------------------------
+
+__device__ inline bool is_normal64(double v) {
+    const double a = fabs(v);
+    return a >= 2.2250738585072014e-308 && a <= 1.7976931348623157e308;
+}
+
+// One block a row: zero[j] = row j's diffusivity is +0 throughout (every bit clear; not -0 or
+// NaN); flags[0] |= some row is; flags[1] |= a zero row has an elevation whose half is not normal
+// (D:306-310, 326-334).
+__global__ __launch_bounds__(kBlock) void kj_live_rows(int w, const double *__restrict__ kd,
+                                                       const double *__restrict__ b,
+                                                       unsigned char *__restrict__ zero,
+                                                       int *flags) {
+    const size_t c0 = (size_t)blockIdx.x * w;
+    int z = 1;
+    for (int i = threadIdx.x; i < w; i += kBlock) z &= __double_as_longlong(kd[c0 + i]) == 0;
+    z = __syncthreads_and(z);
+    int ok = 1;
+    if (z) {
+        for (int i = threadIdx.x; i < w; i += kBlock) ok &= is_normal64(0.5 * b[c0 + i]);
+        ok = __syncthreads_and(ok);
+    }
+    if (threadIdx.x == 0) {
+        zero[blockIdx.x] = (unsigned char)z;
+        if (z) atomicOr(flags, 1);
+        if (z && !ok) atomicOr(flags + 1, 1);
+    }
+}
+
+// One block: live[s] = strip s holds a row that is not skipped (a row is skipped when it and its
+// neighbours, where it has them, are zero rows; D:312-316); every strip when no row is a zero
+// row, a zero row's elevations fail the normal test, or dt and dx^2 are not finite and positive
+// (all_rows).  flags[2] = the visited strips' rows; list = the visited strips in order,
+// flags[4] = how many.
+__global__ __launch_bounds__(1024) void kj_live_strips(int h, const unsigned char *__restrict__ zero,
+                                                       int *flags, int all_rows,
+                                                       unsigned char *__restrict__ live,
+                                                       int *__restrict__ list) {
+    __shared__ int rows;
+    if (threadIdx.x == 0) rows = 0;
+    __syncthreads();
+    const bool all = all_rows || !flags[0] || flags[1];
+    const int ns = (h + kRowTile - 1) / kRowTile;
+    int mine = 0;
+    for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+        const int a = s * kRowTile, e = min(a + kRowTile, h);
+        bool v = all;
+        for (int j = a; j < e && !v; j++) {
+            const bool skip = zero[j] && (j == 0 || zero[j - 1]) && (j + 1 == h || zero[j + 1]);
+            v = !skip;
+        }
+        live[s] = (unsigned char)v;
+        if (v) mine += e - a;
+    }
+    atomicAdd(&rows, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        flags[2] = rows;
+        int k = 0;
+        for (int q = 0; q < ns; q++)
+            if (live[q]) list[k++] = q;
+        flags[4] = k;
+    }
+}
+
+// The directions' +0 in the skipped strips (the loop never writes them there; the visited rows
+// beside them read them across their +0 faces).
+__global__ __launch_bounds__(kBlock) void kj_zero_dead(int n, int strip_cells,
+                                                       const unsigned char *__restrict__ live,
+                                                       double *__restrict__ p0,
+                                                       double *__restrict__ p1) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n || live[c / strip_cells]) return;
+    p0[c] = 0.0, p1[c] = 0.0;
+}
+
+// flags[3] |= a visited cell of x is not finite (D:272-279).
+__global__ __launch_bounds__(kBlock) void kj_live_finite(int n, int strip_cells,
+                                                         const unsigned char *__restrict__ live,
+                                                         const double *__restrict__ x, int *flags) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    const int bad = c < n && live[c / strip_cells] && !isfinite(x[c]);
+    if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(flags + 3, 1);
+}
+
 // Device buffers of the Jacobi solve beside the workspace's vectors (kept with the shape).
 static void jacobi_fit(Workspace &ws) {
     if (ws.jdiag) return;
     ws.jdiag = dalloc<double>(ws.n);
     ws.jrows = dalloc<double>(2 * (size_t)ws.h);
     ws.jst = dalloc<JState>(1);
+    ws.jzero = dalloc<unsigned char>((size_t)ws.h);
+    ws.jlive = dalloc<unsigned char>((size_t)(ws.h + kRowTile - 1) / kRowTile);
+    ws.jflags = dalloc<int>(5);
+    ws.jlist = dalloc<int>((size_t)(ws.h + kRowTile - 1) / kRowTile);
+}
+
+// The direction pass's grid: its blocks take the visited strips' chunks of cells in turn.
+constexpr int kDirBlocks = 16384;
+
+static JLive jlive_of(const Workspace &ws) {
+    return JLive{ws.jlist, ws.jflags + 4, (ws.h + kRowTile - 1) / kRowTile, ws.h};
 }
 
 // One iteration (D:955-992): parity 0 reads the direction from p1 and writes p0, parity 1 the
@@ -398,24 +531,26 @@ static void jacobi_iteration(dtd_ctx *ctx, int parity) {
     const int nb = nblocks(ws.n), nr = (ws.h + kRowTile - 1) / kRowTile;
     double *pold = parity == 0 ? ws.p1 : ws.p0, *pnew = parity == 0 ? ws.p0 : ws.p1;
     double *s0 = ws.jrows, *s1 = ws.jrows + ws.h;
-    kj_dir<<<nb, kBlock, 0, s>>>((int)ws.n, ws.r, ws.jdiag, pold, pnew, ws.x, st);
+    const JLive lv = jlive_of(ws);
+    kj_dir<<<std::min(nb, kDirBlocks), kBlock, 0, s>>>((int)ws.n, ws.r, ws.jdiag, pold, pnew, ws.x,
+                                                       st, lv, kRowTile * ws.w);
     LAUNCH_CHECK();
-    kj_apply_rows<kApplyRows><<<(ws.h + kApplyRows - 1) / kApplyRows, kColTile, 0, s>>>(
-        g, ws.es, pnew, ws.ap, s0, st);
+    kj_apply_rows<kApplyRows><<<nr * (kRowTile / kApplyRows), kColTile, 0, s>>>(
+        g, ws.es, pnew, ws.ap, s0, st, lv);
     LAUNCH_CHECK();
-    kj_step<kJPap><<<1, kStepBlock, 0, s>>>(s0, s1, ws.h, st);
+    kj_step<kJPap><<<1, kStepBlock, 0, s>>>(s0, s1, ws.h, st, lv);
     LAUNCH_CHECK();
     kj_rows<kRowsRZU><<<nr, kColTile, 0, s>>>(ws.w, ws.h, ws.r, ws.jdiag, ws.ap, ws.r, s0, s1,
-                                              st);
+                                              st, lv);
     LAUNCH_CHECK();
-    kj_step<kJUpdate><<<1, kStepBlock, 0, s>>>(s0, s1, ws.h, st, ctx->dev_flag + parity);
+    kj_step<kJUpdate><<<1, kStepBlock, 0, s>>>(s0, s1, ws.h, st, lv, ctx->dev_flag + parity);
     LAUNCH_CHECK();
 }
 
 // `step_in` (D:262-283) of b (device; only read) into ws.x: the operator, the splitting's first
 // guess by the reference's recurrences, the loop to CG_TOL_M, the last update and the put-back.
 static size_t jacobi_solve(dtd_ctx *ctx, const double *b, const double *kd, double dx, double dt,
-                           int wrap_x, dtd_stats *stats) {
+                           int wrap_x, dtd_stats *stats, bool all_rows = false) {
     Workspace &ws = ctx->ws;
     hipStream_t s = ctx->stream;
     jacobi_fit(ws);
@@ -447,14 +582,25 @@ static size_t jacobi_solve(dtd_ctx *ctx, const double *b, const double *kd, doub
         LAUNCH_CHECK();
     }
     HIPCHK(hipEventRecord(ctx->ev[2], s));
+    // The strips the loop visits (`Live::of`, D:298-357).
+    HIPCHK(hipMemsetAsync(ws.jflags, 0, 5 * sizeof(int), s));
+    const bool every = all_rows || !ctx->use_live ||
+                       !(std::isfinite(dt) && dt > 0.0 && std::isfinite(s2) && s2 > 0.0);
+    kj_live_rows<<<ws.h, kBlock, 0, s>>>(ws.w, kd, b, ws.jzero, ws.jflags);
+    LAUNCH_CHECK();
+    kj_live_strips<<<1, 1024, 0, s>>>(ws.h, ws.jzero, ws.jflags, every ? 1 : 0, ws.jlive,
+                                      ws.jlist);
+    LAUNCH_CHECK();
+    kj_zero_dead<<<nb, kBlock, 0, s>>>((int)ws.n, kRowTile * ws.w, ws.jlive, ws.p0, ws.p1);
+    LAUNCH_CHECK();
     kj_init<<<1, 64, 0, s>>>(st, kCgTolM * kCgTolM * (double)ws.n, (int)ctx->opts.max_iter);
     LAUNCH_CHECK();
     kj_resid<<<nb, kBlock, 0, s>>>(g, ws.es, b, ws.x, ws.r);
     LAUNCH_CHECK();
     kj_rows<kRowsRZ><<<nr, kColTile, 0, s>>>(ws.w, ws.h, ws.r, ws.jdiag, nullptr, nullptr, s0,
-                                             s1, st);
+                                             s1, st, jlive_of(ws));
     LAUNCH_CHECK();
-    kj_step<kJInit><<<1, kStepBlock, 0, s>>>(s0, s1, ws.h, st);
+    kj_step<kJInit><<<1, kStepBlock, 0, s>>>(s0, s1, ws.h, st, jlive_of(ws));
     LAUNCH_CHECK();
     if (ctx->use_graph)
         for (int k = 0; k < 2; k++) {
@@ -498,17 +644,27 @@ static size_t jacobi_solve(dtd_ctx *ctx, const double *b, const double *kd, doub
     LAUNCH_CHECK();
     kj_putback<<<nb, kBlock, 0, s>>>((int)ws.n, ws.jdiag, ws.x, st);
     LAUNCH_CHECK();
+    kj_live_finite<<<nb, kBlock, 0, s>>>((int)ws.n, kRowTile * ws.w, ws.jlive, ws.x, ws.jflags);
+    LAUNCH_CHECK();
     HIPCHK(hipEventRecord(ctx->ev[3], s));
     JState hs{};
+    int hflags[5] = {0, 0, 0, 0, 0};
     HIPCHK(hipMemcpyAsync(ctx->host_st, st, sizeof(JState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hflags, ws.jflags, sizeof(hflags), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     std::memcpy(&hs, ctx->host_st, sizeof(JState));
+    if (hflags[2] < ws.h && hflags[3]) {
+        // A value that is not finite would reach the skipped rows in the full loop (0 * inf):
+        // solve on every row instead (D:272-279); its count is the step's, as there.
+        return jacobi_solve(ctx, b, kd, dx, dt, wrap_x, stats, true);
+    }
     if (stats) {
         stats->iterations += (size_t)hs.its;
         stats->ms_setup += elapsed(ctx->ev[0], ctx->ev[1]);
         stats->ms_guess += elapsed(ctx->ev[1], ctx->ev[2]);
         stats->ms_cg += elapsed(ctx->ev[2], ctx->ev[3]);
         stats->ms_total += elapsed(ctx->ev[0], ctx->ev[3]);
+        stats->rows_visited = (size_t)hflags[2];
     }
     return (size_t)hs.its;
 }

@@ -94,7 +94,7 @@ class Stats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_size_t), ("restarts", ctypes.c_size_t),
                 ("rel_residual", ctypes.c_double), ("ms_setup", ctypes.c_double),
                 ("ms_guess", ctypes.c_double), ("ms_cg", ctypes.c_double),
-                ("ms_total", ctypes.c_double)]
+                ("ms_total", ctypes.c_double), ("rows_visited", ctypes.c_size_t)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -93,6 +93,20 @@ def case_lognormal(seed: int = 2, w: int = 8192, h: int = 4096, ratio: float = 5
     return d, spectral_noise(w, h, seed), kd, ratio / float(kd.mean())
 
 
+def case_belt(seed: int = 5, w: int = 8192, h: int = 4096, ratio: float = 50.0):
+    """A mostly empty grid: log-normal K (sigma 8 cells) in the belt of rows [0.4 h, 0.6 h), K = +0
+    on every other row (a range with nothing to diffuse around it: the rows the Jacobi step's
+    `Live` skips), dt mean(K in the belt) / dx^2 = 50, k^-2 h0 offset by 2000 m (so that every
+    elevation is normal)."""
+    d = Dims(w, h, 1.0, True)
+    kd = lognormal_k(w, h, seed).reshape(h, w)
+    a, b = (4 * h) // 10, (6 * h) // 10
+    kd[:a] = 0.0
+    kd[b:] = 0.0
+    mean = float(kd[a:b].mean()) if b > a else 1.0
+    return d, spectral_noise(w, h, seed) + 2000.0, kd.reshape(-1), ratio / mean
+
+
 def case_regional(seed: int = 4, count: int = 512):
     """Independent regional grids: w uniform in {64, 96, ..., 1024}, h in {64, 96, ..., 512}, each
     periodic in x with its own log-normal K (sigma 8 cells) and k^-2 h0, dt mean(K) / dx^2 uniform
@@ -113,4 +127,5 @@ WORKLOADS = {
     "lognormal-8192x4096": (case_lognormal, dict(seed=2, w=8192, h=4096)),
     "constant-16384x8192": (case_constant, dict(seed=3, w=16384, h=8192)),
     "regional-512": (case_regional, dict(seed=4, count=512)),
+    "belt-8192x4096": (case_belt, dict(seed=5, w=8192, h=4096)),
 }

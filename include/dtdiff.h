@@ -112,6 +112,8 @@ typedef struct dtd_stats {
     double ms_guess;      /* first guess */
     double ms_cg;         /* conjugate gradients */
     double ms_total;      /* the whole solve on the device */
+    size_t rows_visited;  /* the Jacobi step: rows its loop visits (h unless `Live` skips some,
+                             diffusion.rs:285-367); 0 for the multigrid solves */
 } dtd_stats;
 
 void dtd_default_opts(dtd_opts *opts);

@@ -129,3 +129,52 @@ def test_jacobi_large_grid(ws):
     x, its = gpu_step(ws, w, h, True, hgt, kd, 2e6)
     xo, its_o = O.step_in(w, h, DX, True, hgt, kd, 2e6)
     assert its == its_o and close(x, xo)
+
+
+def visited_rows(w, h, hgt, kd):
+    """The rows of the strips the oracle's `Live::of` visits (every row when it skips none)."""
+    lv = O.live_of(w, h, DX, hgt, kd, DT)
+    if lv is None:
+        return h
+    return sum(min(8, h - 8 * s) for s in range(len(lv)) if lv[s])
+
+
+def test_jacobi_loop_skips_the_rows_live_skips(monkeypatch):
+    """The device loop visits the strips `Live::of` visits (diffusion.rs:298-357) and gives the
+    bits of the loop that visits every row (DTD_NO_LIVE=1), iterations included; a NaN in a
+    visited row sends it back over every row, as the reference does."""
+    print("jacobi live strips", flush=True)
+    cases = [("ranges", 160, 96, 1, True, 2.0), ("ranges", 96, 61, 4, False, 0.5),
+             ("ranges", 64, 41, 5, True, 2.0)]
+    cases += [("blob", *BLOBS[k][:2], k, BLOBS[k][2], 0.0) for k in range(len(BLOBS))]
+    cases += [("nan", 64, 48, 0, True, 0.0)]
+    live = dtdiff.Workspace()
+    monkeypatch.setenv("DTD_NO_LIVE", "1")
+    every = dtdiff.Workspace()
+    skipped = 0
+    try:
+        for gen, w, h, seed, wrap, pct in cases:
+            if gen == "ranges":
+                hgt, kd = ranges(w, h, seed, pct)
+            else:
+                hgt, kd = blobs(w, h, BLOBS[seed][3])
+                if gen == "nan":
+                    hgt[20 * 64 + 11] = np.nan
+            d = dtdiff.Dims(w, h, DX, wrap)
+            out = []
+            for wsx in (live, every):
+                hd, kdd = DeviceArray.from_numpy(hgt), DeviceArray.from_numpy(kd)
+                st = dtdiff.step_device(wsx, d, hd, kdd, DT)
+                synchronize()
+                out.append((hd.numpy(), st))
+            (xl, sl), (xe, se) = out
+            assert sl.iterations == se.iterations, (gen, w, h, seed)
+            assert same_bits(xl, xe), (gen, w, h, seed)
+            assert se.rows_visited == h
+            want = h if gen == "nan" else visited_rows(w, h, hgt, kd)
+            assert sl.rows_visited == want, (gen, w, h, seed, sl.rows_visited, want)
+            skipped += h - sl.rows_visited
+    finally:
+        live.close()
+        every.close()
+    assert skipped > 0
