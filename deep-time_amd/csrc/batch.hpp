@@ -82,8 +82,8 @@ __global__ __launch_bounds__(kBlock) void kb_coarsen(const BLevel *__restrict__ 
     if (k < q.C.n) coarsen_cell(q.L, q.C, scale, k);
 }
 
-// Level 1 of a grid's cycle straight from kd (k_coarsen_kd's faces: the f32 roundings of the
-// operator's f64 faces, summed as from stored records); the level-0 task of grid g is task g.
+// Level 1 of a grid's cycle straight from kd (coarsen_kd_walk, as k_coarsen_kd); the level-0 task
+// of grid g is task g; its blocks as k_coarsen_kd's grid, row-major.
 __global__ __launch_bounds__(kBlock) void kb_coarsen_kd(const BGrid *__restrict__ t,
                                                         const BLevel *__restrict__ lv0,
                                                         const int2 *__restrict__ blk,
@@ -91,20 +91,11 @@ __global__ __launch_bounds__(kBlock) void kb_coarsen_kd(const BGrid *__restrict_
     const int2 m = blk[blockIdx.x];
     const BLevel &q = lv0[m.x];
     const BGrid &G = t[m.x];
-    const int k = m.y * kBlock + threadIdx.x;
-    if (k >= q.C.n) return;
-    const int w = q.L.w, h = q.L.h;
-    const double *kd = G.kd;
-    const double dt = G.dt, s2 = G.s2;
-    auto east = [&](int c) {
-        const int j = c / w, i = c - j * w;
-        if (i + 1 < w) return (float)face(kd[c], kd[c + 1], dt, s2);
-        return q.L.wrap ? (float)face(kd[c], kd[j * w], dt, s2) : 0.0f;
-    };
-    auto south = [&](int c) {
-        return c + w < w * h ? (float)face(kd[c], kd[c + w], dt, s2) : 0.0f;
-    };
-    coarsen_cell(q.L, q.C, scale, k, east, south);
+    const int ncb = (q.C.w + 63) / 64;
+    const int ic = (m.y % ncb) * 64 + (threadIdx.x & 63);
+    const int jc0 = ((m.y / ncb) * (kBlock / 64) + (int)(threadIdx.x >> 6)) * kCoarsenRows;
+    if (jc0 >= q.C.h) return;  // the whole wave
+    coarsen_kd_walk(q.L, q.C, scale, G.kd, G.dt, G.s2, ic, jc0);
 }
 
 // The caller's h = x.
@@ -836,8 +827,14 @@ static void plan_batch(dtd_ctx *ctx, size_t n, const dtd_grid *gs) {
     bw->coarsen_l.resize(L);
     for (int l = 0; l < L; l++) {
         std::vector<int> cz(lv[l].size(), 0);
-        for (size_t t = 0; t < lv[l].size(); t++)
-            if (l + 1 < bw->nlev[lv_grid[l][t]]) cz[t] = nblocks(lv[l][t].C.n);
+        for (size_t t = 0; t < lv[l].size(); t++) {
+            if (!(l + 1 < bw->nlev[lv_grid[l][t]])) continue;
+            const Lvl<float> &C = lv[l][t].C;
+            // Level 1 from kd by kb_coarsen_kd's blocks; the others a thread a coarse cell.
+            cz[t] = l == 0 ? ((C.w + 63) / 64) * ((C.h + (kBlock / 64) * kCoarsenRows - 1) /
+                                                  ((kBlock / 64) * kCoarsenRows))
+                           : nblocks(C.n);
+        }
         add_launch(bw->coarsen_l[l], cz);
     }
     bw->tables = (int2 *)own(tab.size() * sizeof(int2));

@@ -305,7 +305,9 @@ static void build_operator(dtd_ctx *ctx, const double *kd, double dx, double dt)
     if (!f64 && kd_records(ctx)) {
         // The fused finest smoothers form the finest records from kd, and nothing else of the
         // solve reads them: level 1 straight from kd, no finest records stored.
-        k_coarsen_kd<<<nblocks(ws.lf[1].n), kBlock, 0, s>>>(ws.lf[0], ws.lf[1], ctx->opts.scale,
+        const dim3 cg((ws.lf[1].w + 63) / 64,
+                      (ws.lf[1].h + (kBlock / 64) * kCoarsenRows - 1) / ((kBlock / 64) * kCoarsenRows));
+        k_coarsen_kd<<<cg, kBlock, 0, s>>>(ws.lf[0], ws.lf[1], ctx->opts.scale,
                                                             kd, dt, dx * dx);
         LAUNCH_CHECK();
         build_coarse<float>(ws.lf, ctx->opts.scale, s, 2);

@@ -568,21 +568,72 @@ __global__ void k_coarsen(Lvl<T> f, Lvl<T> c, T scale) {
 // Level 1 of the f32 cycle straight from kd, with no finest records in between (the fused finest
 // smoothers form theirs from kd): each fine face is the f32 rounding of the f64 face
 // Operator::new forms (D:556-572), exactly the value `Level::finest` would store (MG:96-114),
-// summed as from stored records.  f is the finest level (its sizes; mass 1).
-__global__ void k_coarsen_kd(Lvl<float> f, Lvl<float> c, float scale,
-                             const double *__restrict__ kd, double dt, double s2) {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= c.n) return;
-    const int w = f.w, h = f.h;
-    auto east = [&](int q) {
-        const int j = q / w, i = q - j * w;
+// summed as from stored records (coarse_east / coarse_south's sums).  f is the finest level (its
+// sizes; mass 1).  A wave takes 64 adjacent coarse columns and walks kCoarsenRows coarse rows:
+// each coarse cell forms its own east and south faces; its west face is the lane before's east
+// face (the wave's first lane and the first coarse column form theirs), its north face the south
+// face of the row before (the walk's first row forms it).  The same sums, so the same bits, with
+// half the faces formed.
+constexpr int kCoarsenRows = 8;
+
+template <typename T>
+__device__ inline T from_west(T v);  // lane - 1's value (a DPP move, defined below)
+
+__device__ __forceinline__ void coarsen_kd_walk(const Lvl<float> &f, const Lvl<float> &c,
+                                                float scale, const double *__restrict__ kd,
+                                                double dt, double s2, int ic, int jc0) {
+    const int w = f.w, h = f.h, lane = threadIdx.x & 63;
+    const bool ok = ic < c.w;
+    const int icc = ok ? ic : c.w - 1;  // past the last column: in bounds, nothing stored
+    auto east_f = [&](int j, int i) {
+        const size_t q = (size_t)j * w + i;
         if (i + 1 < w) return (float)face(kd[q], kd[q + 1], dt, s2);
-        return f.wrap ? (float)face(kd[q], kd[j * w], dt, s2) : 0.0f;
+        return f.wrap ? (float)face(kd[q], kd[(size_t)j * w], dt, s2) : 0.0f;
     };
-    auto south = [&](int q) {
-        return q + w < w * h ? (float)face(kd[q], kd[q + w], dt, s2) : 0.0f;
+    auto south_f = [&](int j, int i) {
+        const size_t q = (size_t)j * w + i;
+        return j + 1 < h ? (float)face(kd[q], kd[q + w], dt, s2) : 0.0f;
     };
-    coarsen_cell(f, c, scale, k, east, south);
+    auto ce = [&](int x, int jc) {
+        if (!(x + 1 < c.w || c.wrap)) return 0.0f;
+        const int r1 = min(2 * jc + 2, h), last_col = min(2 * x + 2, w) - 1;
+        float e = 0.0f;
+        for (int j = 2 * jc; j < r1; j++) e += east_f(j, last_col);
+        return scale * e;
+    };
+    auto cs = [&](int x, int jc) {
+        if (!(jc + 1 < c.h)) return 0.0f;
+        const int last_row = min(2 * jc + 2, h) - 1, c1 = min(2 * x + 2, w);
+        float v = 0.0f;
+        for (int i = 2 * x; i < c1; i++) v += south_f(last_row, i);
+        return scale * v;
+    };
+    const int jc1 = min(jc0 + kCoarsenRows, c.h);
+    float north = jc0 > 0 ? cs(icc, jc0 - 1) : 0.0f;
+    for (int jc = jc0; jc < jc1; jc++) {
+        const float e = ce(icc, jc), s = cs(icc, jc);
+        float west = from_west(e);
+        if (lane == 0 || icc == 0) west = icc > 0 ? ce(icc - 1, jc) : (c.wrap ? ce(c.w - 1, jc) : 0.0f);
+        const float m = coarse_mass(f, icc, jc);
+        const float d = m + e + west + s + north;
+        if (ok) {
+            const size_t k = (size_t)jc * c.w + ic;
+            c.mass[k] = m;
+            float *q = c.coef + kRec * k;
+            q[0] = e, q[1] = s, q[2] = d;
+        }
+        north = s;
+    }
+}
+
+// Grid: ceil(c.w / 64) x ceil(c.h / (4 kCoarsenRows)) blocks of kBlock threads.
+__global__ __launch_bounds__(kBlock) void k_coarsen_kd(Lvl<float> f, Lvl<float> c, float scale,
+                                                       const double *__restrict__ kd, double dt,
+                                                       double s2) {
+    const int ic = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int jc0 = (blockIdx.y * (kBlock / 64) + (int)(threadIdx.x >> 6)) * kCoarsenRows;
+    if (jc0 >= c.h) return;  // the whole wave
+    coarsen_kd_walk(f, c, scale, kd, dt, s2, ic, jc0);
 }
 
 // One cell of A x on a level in the order of `Level::apply_row` (MG:203-240): diagonal, east
