@@ -976,12 +976,53 @@ __device__ inline T bottom_sweep(const Lvl<T> &L, const T *z, int c, int i, int 
 // Levels b0 .. nl - 1 of a cycle (every one of at most kBottomLevels, their records, r and z
 // at most kBottomLds bytes) by one block of kBottomBlock threads in LDS: the top level's
 // residual in, its correction out to lv[b0].z[1].
+// row and column of cell c of a level w wide (c < 2^22, w < 2^22: the float quotient is within
+// one of the integer one, and the corrections fix it).
+__device__ inline int bottom_row(int c, int w, float rw) {
+    int j = (int)((float)c * rw);
+    if (j * w > c) j--;
+    if ((j + 1) * w <= c) j++;
+    return j;
+}
+
+// n values from global g into LDS l by the block, four loads in flight a thread (16-byte words
+// when both ends allow).
+template <typename T>
+__device__ inline void bottom_copy(const T *__restrict__ g, T *l, int n) {
+    const int t = threadIdx.x;
+    constexpr int per = 16 / sizeof(T);
+    if (n % per == 0 && ((reinterpret_cast<size_t>(g) | reinterpret_cast<size_t>(l)) & 15) == 0) {
+        using W = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+        const W *gw = reinterpret_cast<const W *>(g);
+        W *lw = reinterpret_cast<W *>(l);
+        const int nw = n / per;
+        int k = t;
+        for (; k + kBottomBlock < nw; k += 2 * kBottomBlock) {
+            const W a = gw[k], b = gw[k + kBottomBlock];
+            lw[k] = a, lw[k + kBottomBlock] = b;
+        }
+        for (; k < nw; k += kBottomBlock) lw[k] = gw[k];
+        return;
+    }
+    int k = t;
+    for (; k + 3 * kBottomBlock < n; k += 4 * kBottomBlock) {
+        const T a = g[k], b = g[k + kBottomBlock], c = g[k + 2 * kBottomBlock],
+                d = g[k + 3 * kBottomBlock];
+        l[k] = a, l[k + kBottomBlock] = b, l[k + 2 * kBottomBlock] = c, l[k + 3 * kBottomBlock] = d;
+    }
+    for (; k < n; k += kBottomBlock) l[k] = g[k];
+}
+
+// Levels b0 .. nl - 1 of a cycle (every one of at most kBottomLevels, their records, r and z
+// at most kBottomLds bytes) by one block of kBottomBlock threads in LDS: the top level's
+// residual in, its correction out to lv[b0].z[1].  The restriction first forms each fine cell's
+// r - A z (into z[0], free by then) by its own thread, then sums each coarse block's four in
+// restrict_cell's order; the coarsest level's neighbours come by lane shuffles.
 template <typename T>
 __device__ __forceinline__ void bottom_cycle(const Lvl<T> *__restrict__ lv, int b0, int nl) {
     extern __shared__ double bottom_mem[];
     __shared__ Lvl<T> loc[kBottomLevels];
     __shared__ int cells[kBottomLevels];
-    __shared__ T zs[2][64];
     const int t = threadIdx.x, nb = nl - b0;
     // The level table: one thread a level reads it, then each level's LDS place follows the
     // sizes of the levels before it.
@@ -1004,27 +1045,36 @@ __device__ __forceinline__ void bottom_cycle(const Lvl<T> *__restrict__ lv, int 
         loc[t] = L;
     }
     __syncthreads();
-    for (int q = 0; q < nb; q++) {
-        const T *gc = lv[b0 + q].coef;
-        T *lc = loc[q].coef;
-        for (int k = t; k < kRec * cells[q]; k += kBottomBlock) lc[k] = gc[k];
-    }
-    {
-        const T *gr = lv[b0].r;
-        T *lr = loc[0].r;
-        for (int c = t; c < cells[0]; c += kBottomBlock) lr[c] = gr[c];
-    }
+    for (int q = 0; q < nb; q++) bottom_copy<T>(lv[b0 + q].coef, loc[q].coef, kRec * cells[q]);
+    bottom_copy<T>(lv[b0].r, loc[0].r, cells[0]);
     __syncthreads();
     for (int q = 0; q + 1 < nb; q++) {
         const Lvl<T> L = loc[q], C = loc[q + 1];
+        const float rw = 1.0f / (float)L.w;
         for (int c = t; c < L.n; c += kBottomBlock) L.z[0][c] = c_wd(L, c) * L.r[c];
         __syncthreads();
         for (int c = t; c < L.n; c += kBottomBlock) {
-            const int j = c / L.w;
+            const int j = bottom_row(c, L.w, rw);
             L.z[1][c] = bottom_sweep(L, L.z[0], c, c - j * L.w, j);
         }
         __syncthreads();
-        for (int k = t; k < C.n; k += kBottomBlock) C.r[k] = restrict_cell(L, L.z[1], C, k);
+        if (L.bil) {
+            for (int k = t; k < C.n; k += kBottomBlock) C.r[k] = restrict_cell(L, L.z[1], C, k);
+        } else {
+            for (int c = t; c < L.n; c += kBottomBlock) {
+                const int j = bottom_row(c, L.w, rw), i = c - j * L.w;
+                L.z[0][c] = L.r[c] - lvl_apply(L, c, i, j, [&](int k) { return L.z[1][k]; });
+            }
+            __syncthreads();
+            for (int k = t; k < C.n; k += kBottomBlock) {
+                const int jc = bottom_row(k, C.w, 1.0f / (float)C.w), ic = k - jc * C.w;
+                const int r1 = min(2 * jc + 2, L.h), c1 = min(2 * ic + 2, L.w);
+                T acc = (T)0;
+                for (int j = 2 * jc; j < r1; j++)
+                    for (int i = 2 * ic; i < c1; i++) acc += L.z[0][j * L.w + i];
+                C.r[k] = acc;
+            }
+        }
         __syncthreads();
     }
     if (lv[nl - 1].chol) {
@@ -1037,7 +1087,7 @@ __device__ __forceinline__ void bottom_cycle(const Lvl<T> *__restrict__ lv, int 
         __syncthreads();
     } else {
         // The coarsest level (at most 64 cells): z = wd r, then 15 sweeps, by the first wave with
-        // its stencil's records in registers.
+        // its stencil's records in registers and its neighbours' z by lane shuffles.
         const Lvl<T> L = loc[nb - 1];
         if (t < 64) {
             const int c = t;
@@ -1058,54 +1108,46 @@ __device__ __forceinline__ void bottom_cycle(const Lvl<T> *__restrict__ lv, int 
                 if (hww) cw = c_east(L, c0 + L.w - 1), kw = c0 + L.w - 1;
                 if (hs) cs = c_south(L, c), ks = c + L.w;
                 if (hn) cn = c_south(L, c - L.w), kn = c - L.w;
-                zs[0][c] = wd * rr;
             }
-            // One wave: its LDS accesses take effect in program order; the fence keeps the
-            // compiler from moving them across the sweep boundary.
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            int cur = 0;
+            T z = live ? wd * rr : (T)0;
             for (int k = 1; k < 16; k++) {
-                if (live) {
-                    const T *zz = zs[cur];
-                    // lvl_apply's terms in its order: diag, east, west, the wrapped east / west,
-                    // south, north.
-                    T o = d * zz[c];
-                    if (he) o -= ce * zz[ke];
-                    if (hw) o -= cw * zz[kw];
-                    if (hew) o -= ce * zz[ke];
-                    if (hww) o -= cw * zz[kw];
-                    if (hs) o -= cs * zz[ks];
-                    if (hn) o -= cn * zz[kn];
-                    zs[cur ^ 1][c] = zz[c] + wd * (rr - o);
-                }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                cur ^= 1;
+                const T zE = __shfl(z, ke), zW = __shfl(z, kw), zS = __shfl(z, ks),
+                        zN = __shfl(z, kn);
+                // lvl_apply's terms in its order: diag, east, west, the wrapped east / west,
+                // south, north.
+                T o = d * z;
+                if (he) o -= ce * zE;
+                if (hw) o -= cw * zW;
+                if (hew) o -= ce * zE;
+                if (hww) o -= cw * zW;
+                if (hs) o -= cs * zS;
+                if (hn) o -= cn * zN;
+                if (live) z = z + wd * (rr - o);
             }
-            if (live) (nb == 1 ? lv[b0].z[1] : L.z[0])[c] = zs[cur][c];
+            if (live) (nb == 1 ? lv[b0].z[1] : L.z[0])[c] = z;
         }
         __syncthreads();
     }
     // Back up: each level's result in its z[0] (the coarsest's too).
     for (int q = nb - 2; q >= 0; q--) {
         const Lvl<T> L = loc[q], C = loc[q + 1];
+        const float rw = 1.0f / (float)L.w;
         // z' = z + the coarse correction, once a cell (the sum the corrected sweep takes for
         // every point of its stencil); then the corrected sweep and the last.
         for (int c = t; c < L.n; c += kBottomBlock) {
-            const int j = c / L.w, i = c - j * L.w;
+            const int j = bottom_row(c, L.w, rw), i = c - j * L.w;
             L.z[0][c] = L.z[1][c] + (L.bil ? bil_at(C.z[0], C.w, C.h, C.wrap, i, j)
                                            : C.z[0][(j >> 1) * C.w + (i >> 1)]);
         }
         __syncthreads();
         for (int c = t; c < L.n; c += kBottomBlock) {
-            const int j = c / L.w;
+            const int j = bottom_row(c, L.w, rw);
             L.z[1][c] = bottom_sweep(L, L.z[0], c, c - j * L.w, j);
         }
         __syncthreads();
         T *out = q > 0 ? L.z[0] : lv[b0].z[1];
         for (int c = t; c < L.n; c += kBottomBlock) {
-            const int j = c / L.w;
+            const int j = bottom_row(c, L.w, rw);
             out[c] = bottom_sweep(L, L.z[1], c, c - j * L.w, j);
         }
         __syncthreads();
