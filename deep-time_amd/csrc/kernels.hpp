@@ -3106,6 +3106,13 @@ __global__ __launch_bounds__(MAXT, 2048 / MAXT) void k_lod_lines(int n, int ring
     seg_back(xf, xl_, al, ga, de, xl + a);
 }
 
+// The wave's LDS writes before this point are seen by its reads after it: one wave's LDS
+// accesses take effect in program order; the fence keeps the compiler from moving them across.
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // One line of n cells (a multiple of 16, 32 <= n <= 1024) by the partition method in one wave,
 // for lines of any such length: lane t takes the segment [16 t, 16 t + 16) and reduces it
 // (seg_reduce); the N = 2 n / 16 reduced equations go to LDS and are solved by parallel cyclic
@@ -3114,8 +3121,8 @@ __global__ __launch_bounds__(MAXT, 2048 / MAXT) void k_lod_lines(int n, int ring
 // Recipes, 2nd ed., section 2.7: the corners folded into the first and last diagonals, a second
 // right-hand side u = (gamma, 0, .., 0, alpha) reduced alongside); the lanes then recover their
 // interiors.  The same system as the reference's row recurrences, solved to rounding.  sm: 6 x 128
-// doubles of LDS for this wave; every lane of the block calls it (`on`: the wave has a line), as
-// it synchronises the block.
+// doubles of LDS for this wave alone; every lane of the wave calls it (`on`: the wave has a line),
+// as it synchronises the wave.
 __device__ __forceinline__ void lod_line_wave(int n, int ring, bool on, const double *__restrict__ k,
                                               const double *__restrict__ src,
                                               double *__restrict__ dst, double dts2,
@@ -3143,12 +3150,12 @@ __device__ __forceinline__ void lod_line_wave(int n, int ring, bool on, const do
             R[2 * lane + e] = eq.r[e];
         }
     }
-    __syncthreads();
+    wave_lds_sync();
     // Row 0 couples x[N - 1] by beta = A[0], row N - 1 couples x[0] by alpha = C[N - 1] (a ring;
     // dropped for a closed line).
     double beta = 0.0, alpha = 0.0, gamma = 1.0;
     if (on && ring) beta = A[0], alpha = C[N - 1], gamma = -B[0];
-    __syncthreads();
+    wave_lds_sync();
     if (mine) {
 #pragma unroll
         for (int e = 0; e < 2; e++) {
@@ -3166,7 +3173,7 @@ __device__ __forceinline__ void lod_line_wave(int n, int ring, bool on, const do
             Bi[i] = 1.0 / B[i];
         }
     }
-    __syncthreads();
+    wave_lds_sync();
     // After the step of stride s every equation couples the unknowns 2 s away; past the ends
     // nothing.
     for (int s = 1; s < N; s <<= 1) {
@@ -3187,7 +3194,7 @@ __device__ __forceinline__ void lod_line_wave(int n, int ring, bool on, const do
                 nu[e] = U[i] - u_m * k1 - u_p * k2;
             }
         }
-        __syncthreads();
+        wave_lds_sync();
         if (mine) {
 #pragma unroll
             for (int e = 0; e < 2; e++) {
@@ -3196,7 +3203,7 @@ __device__ __forceinline__ void lod_line_wave(int n, int ring, bool on, const do
                 Bi[i] = 1.0 / nb[e];
             }
         }
-        __syncthreads();
+        wave_lds_sync();
     }
     // Every equation stands alone: x = R / B, z = U / B; the ring's correction x - fact z.
     double x[2] = {0.0, 0.0};
@@ -3209,7 +3216,7 @@ __device__ __forceinline__ void lod_line_wave(int n, int ring, bool on, const do
         }
     }
     if (ring) {
-        __syncthreads();
+        wave_lds_sync();
         if (mine) {
             const double fact = (A[0] + beta * A[N - 1] / gamma) /
                                 (1.0 + C[0] + beta * C[N - 1] / gamma);
